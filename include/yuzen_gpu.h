@@ -1,0 +1,171 @@
+/*
+ * yuzen_gpu.h -- C-ABI of the MI355X implementation of Yuzen's implicit
+ * film-height step (well-mixed model, 1-D "xh" and 2-D "xyh").
+ *
+ * Two seams are exported, both plain C: pointers, sizes, return codes.
+ *
+ *  (1) The linear-solve seam, one call per solve, with the contract of
+ *      ILinearSolverBackend::Solve (reference yuzen/src/numerics/
+ *      LinearSolverBackend.h:56-103): x is the initial guess and the
+ *      output; the matrix and the right-hand side are left as given;
+ *      return 0 converged, 1 iteration cap, 2 breakdown or NaN; resid is
+ *      the relative residual ||b - A x|| / ||b|| of the (row-scaled)
+ *      system. Nothing throws across the seam.
+ *
+ *  (2) The step seam, the device-resident equivalent of
+ *      C_NewtonKantorovich::TimeAdvance (reference yuzen/src/numerics/
+ *      NewtonKantorovich.cpp:424-486) for WellMixed_HeightEquationBase
+ *      (reference yuzen/src/model/well_mixed/HeightEquationBase.h:48-1601):
+ *      fields are uploaded once, every Newton pass (assembly of A and of
+ *      the Jacobian, residual, Krylov solve, update) runs on the GPU, and
+ *      only the Newton control scalars cross back to the host.
+ *
+ * Errors: every call returns an int status (0 success) or NULL; the text of
+ * the last error of the calling thread is yz_last_error(). A context is
+ * bound to one HIP stream and is not thread-safe.
+ *
+ * Field layout (both seams, and the CPU oracle): the reference's row map,
+ * row = nxp * iy + ix with nxp = nx + 4 and two ghost layers per side
+ * (reference docs/numerics.md:54-75, GridTraits.h WM2D_Grid::Index);
+ * 1-D: row = ix, n = nx + 4.
+ */
+#ifndef YUZEN_GPU_H
+#define YUZEN_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Boundary types (reference yuzen/src/model/BoundaryCondition.h:40-47).
+ * Symmetry and Periodic are implemented; the others are refused. */
+enum { YZ_BC_SYMMETRY = 0, YZ_BC_PERIODIC = 1, YZ_BC_PINNING = 2,
+       YZ_BC_FRESHINLET = 3, YZ_BC_OUTLET = 4 };
+
+/* Preconditioner of the height equation's Krylov solve.
+ * YZ_PC_AMG is what PreconditionType = "ILU" selects in the reference's
+ * AMGCL backend (smoothed aggregation + SPAI0, LinearSolverBackend.cpp:
+ * 662-679); YZ_PC_JACOBI is "None"/"Jacobi" (damped Jacobi);
+ * YZ_PC_LEARNED is the learned 13-point approximate inverse (no reference
+ * counterpart), with AMG as the fallback when the stall test trips. */
+enum { YZ_PC_JACOBI = 0, YZ_PC_AMG = 1, YZ_PC_LEARNED = 2 };
+
+/* Everything one well-mixed film-height case needs, already made
+ * dimensionless (reference NonDimensionalNumbers::Build,
+ * yuzen/src/model/NonDimensionalNumbers.cpp:43-141). */
+typedef struct yz_case {
+    int dim;                    /* 1: "xh", 2: "xyh" */
+    int nx, ny;                 /* interior cells (ny = 1 in 1-D) */
+    double xmin, xmax, ymin, ymax;
+    int bc[4];                  /* Xmin, Xmax, Ymin, Ymax: YZ_BC_* */
+
+    /* physics switches (Physics.*.Enable) */
+    int capillary, vertical_gravity, inplane_gravity, centrifugal,
+        marangoni, disjoining;
+    /* groups */
+    double Bo, Pgx, Pgy, Sp, As, Ma, Dp;
+    double hstar, dis_n, dis_m;  /* Pi = (hstar/h)^n - (hstar/h)^m */
+    double slip;                 /* Lambda = slip length / h0; 0: no slip */
+    double rot_x, rot_y;         /* centre of rotation */
+    double visc_c0, visc_exp;    /* mu(c) = exp(visc_exp (c - visc_c0)) */
+    double sig_c0;               /* sigma(c) = 1 + Ma (c - sig_c0) */
+
+    /* time integration and Newton (Numerics.HeightEquation) */
+    double theta;                /* 0.5: Crank-Nicolson */
+    int nk_max, nk_min, jac_interval;
+    double nk_eps, nk_eps_rel;
+    int ew, extrap;              /* YUZEN_OPT ew / extrap (both default on) */
+
+    /* linear solve (Numerics.HeightEquation.LinearSolver) */
+    int lin_maxit;
+    double lin_eps;
+    int row_scaling;
+    int precond;                 /* YZ_PC_* */
+    int krylov_fp32;             /* 0: FP64 Krylov; 1: FP32 inner solve */
+
+    /* step control (Numerics.TimeStepControl) */
+    int step_control;
+    double tsc_change, tsc_criterion, tsc_floor;
+    int tsc_keep;
+    double dt_min, dt_max;
+} yz_case;
+
+/* Fill c with the reference's defaults (Input.cpp getin defaults):
+ * theta 0.5, NewtonMaxIterations 30, NewtonMinIterations 1,
+ * NewtonEpsilon 1e-10, NewtonEpsilonRel 1e-6, JacobianUpdateInterval 1,
+ * MaxIterations 5000, Epsilon 1e-13, RowScaling on, AMG, EW and
+ * extrapolation on, step control off. */
+void yz_case_defaults(yz_case *c);
+
+const char *yz_last_error(void);
+/* sizeof(yz_case) and sizeof(yz_lin_settings) as compiled, so that a
+ * binding can check its mirror of the two structures */
+int yz_sizeof_case(void);
+int yz_sizeof_lin_settings(void);
+
+/* ---- step seam --------------------------------------------------------
+ * yz_step_create: h, s, c are padded arrays of n = (nx+4)(ny+4) rows
+ * (1-D: nx+4); their ghost values are ignored and formed again from the
+ * boundary conditions. dt0 is the first step (Numerics.InitialDt). */
+typedef struct yz_step yz_step;
+yz_step *yz_step_create(const yz_case *c, const double *h, const double *s,
+                        const double *conc, double dt0);
+/* One implicit step, TimeAdvance's contract: returns 0 on an accepted
+ * step, 1 when the Newton iteration failed at a fixed step (the reference
+ * stops the run: FailedFixedStep), 2 on a device error. nk_iter is the
+ * number of Newton passes of the accepted attempt, lin_iter the Krylov
+ * iterations summed over them, resid the last Newton correction. */
+int yz_step_advance(yz_step *st, int *nk_iter, int *lin_iter, double *resid);
+/* n doubles, ghosts included */
+int yz_step_get_h(yz_step *st, double *h);
+int yz_step_set_h(yz_step *st, const double *h);
+/* clock: time reached, step now in force, step before it */
+int yz_step_clock(yz_step *st, double *time, double *dt_now, double *dt_old);
+/* per-phase timing of the last yz_step_advance, milliseconds (HIP events):
+ * [0] assembly, [1] residual, [2] AMG setup, [3] Krylov, [4] update,
+ * [5] whole step */
+int yz_step_timing(yz_step *st, double *ms6);
+void yz_step_destroy(yz_step *st);
+
+/* Assembly-level access (parity tests): at film h (n doubles, padded),
+ * with the step's dt and the explicit half Ax_old taken at h_old:
+ *   ia[13*N] (2-D) / ia[5*N] (1-D): rows of I + theta dt J over the N
+ *   interior cells, slot order oM mm om pm Mo mo oo po Po mp op pp oP
+ *   (1-D: M m o p P), unscaled;
+ *   ax[n]: A(h) h, face by face for the capillary and vertical-gravity
+ *   terms (HeightEquationBase::ApplyA), 0 on ghost rows;
+ *   rhs[n]: the Newton right-hand side (NewtonKantorovich::MakeRHS).
+ * Any output pointer may be NULL. */
+int yz_step_assemble(yz_step *st, const double *h_old, const double *h,
+                     double dt, double *ia, double *ax, double *rhs);
+
+/* ---- linear-solve seam -------------------------------------------------
+ * Solve A x = b, A in CRS (rowptr[n+1], col[nnz], val[nnz]; any column
+ * order), with BiCGSTAB under settings s. When grid_nx > 0 the rows are
+ * the (grid_nx+4)(grid_ny+4) rows of a film-height matrix on that grid
+ * (ghost rows included) and the structured AMG is available; otherwise
+ * precond AMG falls back to Jacobi on the device. */
+typedef struct yz_lin_settings {
+    int precond;       /* YZ_PC_JACOBI or YZ_PC_AMG */
+    int scaling;       /* row scaling by the diagonal */
+    int max_iter;
+    double epsilon;
+    int grid_dim, grid_nx, grid_ny;   /* optional structure, 0: none */
+    int bc[4];
+} yz_lin_settings;
+
+typedef struct yz_linctx yz_linctx;
+yz_linctx *yz_linsolve_create(void);
+void yz_linsolve_destroy(yz_linctx *ctx);
+int yz_linsolve(yz_linctx *ctx, int n, int nnz, const int *rowptr,
+                const int *col, const double *val, const double *rhs,
+                double *x_inout, const yz_lin_settings *s, int *iter,
+                double *resid);
+/* solve-to-solve state for restart (ILinearSolverBackend::Memory /
+ * SetMemory): the device backend carries none, so Memory is empty */
+int yz_memory_get(yz_linctx *ctx, double *buf, int cap);
+int yz_memory_set(yz_linctx *ctx, const double *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YUZEN_GPU_H */

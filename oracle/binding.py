@@ -1,0 +1,122 @@
+"""ctypes binding of oracle/liboracle.so (the CPU FP64 oracle).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and
+the CPU-baseline leg of bench.py.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from yuzen_amd.capi import YzCase
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    P = C.POINTER
+    pd, pi_ = P(C.c_double), P(C.c_int)
+    lib.yzo_create.restype = C.c_void_p
+    lib.yzo_create.argtypes = [P(YzCase), pd, pd, pd, C.c_double]
+    lib.yzo_destroy.argtypes = [C.c_void_p]
+    lib.yzo_n.argtypes = [C.c_void_p]
+    lib.yzo_nnz.argtypes = [C.c_void_p]
+    lib.yzo_advance.argtypes = [C.c_void_p, pi_, pi_, pd]
+    lib.yzo_get_h.argtypes = [C.c_void_p, pd]
+    lib.yzo_set_h.argtypes = [C.c_void_p, pd]
+    lib.yzo_clock.argtypes = [C.c_void_p, pd, pd, pd]
+    lib.yzo_pattern.argtypes = [C.c_void_p, pi_, pi_, C.c_char_p]
+    lib.yzo_assemble.argtypes = [C.c_void_p, pd, pd, C.c_double, pd, pd, pd, pd, pd]
+    lib.yzo_linsolve.argtypes = [C.c_int, C.c_int, pi_, pi_, pd, pd, pd, C.c_int,
+                                 C.c_int, C.c_int, C.c_double, pi_, pd]
+    _lib = lib
+    return lib
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _pi(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class OracleStep:
+    """The oracle's time stepper over one case."""
+
+    def __init__(self, case, h, s, c, dt0):
+        self.lib = load()
+        self.case = case.copy()
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self.p = self.lib.yzo_create(C.byref(self.case), _pd(h), _pd(s), _pd(c), dt0)
+        self.n = self.lib.yzo_n(self.p)
+        self.nnz = self.lib.yzo_nnz(self.p)
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.lib.yzo_destroy(self.p)
+            self.p = None
+
+    def advance(self):
+        nk, lin, r = C.c_int(), C.c_int(), C.c_double()
+        rc = self.lib.yzo_advance(self.p, C.byref(nk), C.byref(lin), C.byref(r))
+        return rc, nk.value, lin.value, r.value
+
+    def h(self):
+        out = np.empty(self.n)
+        self.lib.yzo_get_h(self.p, _pd(out))
+        return out
+
+    def set_h(self, h):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        self.lib.yzo_set_h(self.p, _pd(h))
+
+    def clock(self):
+        t, a, b = C.c_double(), C.c_double(), C.c_double()
+        self.lib.yzo_clock(self.p, C.byref(t), C.byref(a), C.byref(b))
+        return t.value, a.value, b.value
+
+    def pattern(self):
+        rp = np.empty(self.n + 1, dtype=np.int32)
+        col = np.empty(self.nnz, dtype=np.int32)
+        bc = C.create_string_buffer(self.n)
+        self.lib.yzo_pattern(self.p, _pi(rp), _pi(col), bc)
+        return rp, col, np.frombuffer(bc.raw, dtype=np.int8).astype(bool)
+
+    def assemble(self, h_old, h, dt):
+        n, nnz = self.n, self.nnz
+        A, J, IA = np.empty(nnz), np.empty(nnz), np.empty(nnz)
+        ax, rhs = np.empty(n), np.empty(n)
+        h_old = np.ascontiguousarray(h_old, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        self.lib.yzo_assemble(self.p, _pd(h_old), _pd(h), dt, _pd(A), _pd(J), _pd(IA), _pd(ax), _pd(rhs))
+        return A, J, IA, ax, rhs
+
+
+def linsolve(rowptr, col, val, rhs, x0, precond=1, scaling=1, maxit=5000, eps=1e-13):
+    lib = load()
+    n = len(rowptr) - 1
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    it, r = C.c_int(), C.c_double()
+    rc = lib.yzo_linsolve(n, len(col), _pi(rowptr), _pi(col), _pd(val), _pd(rhs), _pd(x),
+                          precond, scaling, maxit, eps, C.byref(it), C.byref(r))
+    return rc, x, it.value, r.value

@@ -1,0 +1,1534 @@
+/*
+ * yz_oracle.cpp -- CPU FP64 restatement of Yuzen's well-mixed implicit
+ * film-height step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker: the parity tests, the
+ * smoke check and the CPU-baseline leg of the benchmark load it; the
+ * product (yuzen_amd/csrc, libyuzen_hip.so) never includes, links or calls
+ * it.
+ *
+ * It restates, in plain loops over the reference's own row layout (two
+ * ghost layers, row = nxp*iy + ix, ghost rows kept in the system as
+ * boundary-condition rows), the following reference functions. Each
+ * function below cites the file:line it follows; paths are relative to the
+ * reference root.
+ *
+ *   grid, coordinates      yuzen/src/model/Grids.cpp:88-110,
+ *                          yuzen/src/grid/RectGrid.h:375-376,462-464
+ *   ghost rows (Symmetry,  yuzen/src/model/BoundaryCondition.cpp:
+ *   Periodic)              ApplyEdge, ApplyCorners, MirrorXY
+ *   mobility               yuzen/src/model/Mobility.h:49-191
+ *   limiter, face values   yuzen/src/numerics/FiniteDifferences.cpp:22-228
+ *   capillary              LaplacePressureKernel.h:354-579,
+ *                          WM2D/LaplacePressure_2D.cpp:39-128,
+ *                          WM1D/LaplacePressure_1D.cpp:35-77
+ *   vertical gravity       VerticalGravityKernel.h:150-433
+ *   disjoining pressure    DisjoiningPressureKernel.h:150-435
+ *   in-plane gravity,      BodyForceKernel.h (limited: LimitedFace,
+ *   centrifugal            SetMatrixA_Limited, SetJacobiMatrix_Limited),
+ *                          ParallelGravityKernel.h, CentrifugalKernel.h
+ *   Marangoni              SolutoCapillarityKernel.h (limited forms)
+ *   A, J, ApplyA           well_mixed/HeightEquationBase.h:669-827,1061-1091
+ *   IA, Rhs, Newton loop   yuzen/src/numerics/NewtonKantorovich.cpp:74-523
+ *   step control           yuzen/src/numerics/ImplicitSolver.cpp:144-307
+ *   SatisfyBCRows          yuzen/src/numerics/CRS_Matrix.cpp:198-302
+ *   ScaleRows              yuzen/src/numerics/LinearSolverBackend.cpp:227-252
+ *   BiCGSTAB, SA-AMG,      AMGCL 1.4.4 (un-vendored third-party library,
+ *   SPAI0, damped Jacobi   called at LinearSolverBackend.cpp:596-679):
+ *                          restated from the published methods (van der
+ *                          Vorst's BiCGSTAB with right preconditioning;
+ *                          smoothed aggregation of Vanek, Mandel & Brezina
+ *                          with constant near-null space, strength threshold
+ *                          0.08 halved per level, prolongator smoothed with
+ *                          the filtered matrix and omega = 4/3 / rho;
+ *                          SPAI0 m_i = a_ii / sum_j a_ij^2; direct solve on
+ *                          the coarsest level below 500 rows). The
+ *                          aggregation order of AMGCL is not reproduced: the
+ *                          hierarchy and the iteration counts are not pinned,
+ *                          only the solution (to the solve's tolerance).
+ *
+ * Only the limited Koren/van Leer height schemes (the default,
+ * Numerics.Convection.Height = "Koren") and the Symmetry/Periodic edges are
+ * restated; no concentration equation (Physics.ConcField.Model = "None": the
+ * concentration field is frozen), no evaporation, no source.
+ */
+#include "../include/yuzen_gpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace yzo {
+
+using std::vector;
+
+/* ------------------------------------------------------------------ */
+/* Grid: Grids.cpp:88-110 (two ghost layers, spacing recomputed over the
+ * padded extent), RectGrid.h:375-376 (X of a half-shifted site).       */
+/* ------------------------------------------------------------------ */
+struct Grid {
+    int dim = 2, nx = 0, ny = 1, nxp = 0, nyp = 1, n = 0;
+    double dx = 1, dy = 1, xlo = 0, ylo = 0;
+    int bc[4] = {0, 0, 0, 0};
+
+    void build(const yz_case &c) {
+        dim = c.dim; nx = c.nx; ny = (dim == 2) ? c.ny : 1;
+        for (int k = 0; k < 4; ++k) bc[k] = c.bc[k];
+        const double dx0 = (c.xmax - c.xmin) / (double)nx;
+        xlo = c.xmin - 2 * dx0;
+        const double xhi = c.xmax + 2 * dx0;
+        nxp = nx + 4;
+        dx = (xhi - xlo) / nxp;
+        if (dim == 2) {
+            const double dy0 = (c.ymax - c.ymin) / (double)ny;
+            ylo = c.ymin - 2 * dy0;
+            const double yhi = c.ymax + 2 * dy0;
+            nyp = ny + 4;
+            dy = (yhi - ylo) / nyp;
+        } else {
+            nyp = 1; dy = 1; ylo = 0;
+        }
+        n = nxp * nyp;
+    }
+    int row(int ix, int iy) const { return nxp * iy + ix; }
+    bool inside(int ix, int iy) const {
+        if (ix < 2 || ix > nxp - 3) return false;
+        if (dim == 2 && (iy < 2 || iy > nyp - 3)) return false;
+        return true;
+    }
+    /* cell centre (Zface / Xedge: the x bit set) */
+    double X(int ix) const { return xlo + ix * dx + dx * 0.5; }
+    double Y(int iy) const { return ylo + iy * dy + dy * 0.5; }
+    /* face coordinates (HalfXm keeps the index, HalfXp raises it) */
+    double Xface(int ix) const { return xlo + ix * dx; }
+    double Yface(int iy) const { return ylo + iy * dy; }
+
+    /* the interior point a ghost index maps to along one axis
+     * (BoundaryCondition.cpp ApplyEdge: Symmetry EqualVal(L0,L3),
+     * EqualVal(L1,L2); Periodic EqualVal(L0,opp3), EqualVal(L1,opp2)) */
+    static int fold(int i, int np, int lo_bc, int hi_bc) {
+        const int m = np - 4;
+        if (i < 2) return lo_bc == YZ_BC_PERIODIC ? i + m : 3 - i;
+        if (i > np - 3) return hi_bc == YZ_BC_PERIODIC ? i - m : 2 * (np - 3) + 1 - i;
+        return i;
+    }
+    /* the row a ghost row copies (corners fold along both axes, as
+     * MirrorXY does) */
+    int src(int ix, int iy) const {
+        const int fx = fold(ix, nxp, bc[0], bc[1]);
+        const int fy = (dim == 2) ? fold(iy, nyp, bc[2], bc[3]) : iy;
+        return row(fx, fy);
+    }
+};
+
+/* stencil slots: 2-D  oM mm om pm Mo mo oo po Po mp op pp oP  (FullIdx
+ * order, GridTraits.h WM2D_Grid::FullIdx); 1-D  M m o p P            */
+struct Stencil {
+    int ns = 13, center = 6;
+    int ox[13], oy[13];
+    int adj[4];          /* AdjIdx(d): mo po om op */
+    int face[4][4];      /* FaceStencil(d, j): LL L R RR */
+    int axis[2][5];      /* AxisStencil(a, j) */
+    int ndir = 4, naxis = 2;
+    void build(int dim) {
+        if (dim == 2) {
+            static const int x[13] = {0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0};
+            static const int y[13] = {-2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2};
+            ns = 13; center = 6; ndir = 4; naxis = 2;
+            for (int k = 0; k < 13; ++k) { ox[k] = x[k]; oy[k] = y[k]; }
+            enum { oM, mm, om, pm, Mo, mo, oo, po, Po, mp, op, pp, oP };
+            adj[0] = mo; adj[1] = po; adj[2] = om; adj[3] = op;
+            const int f[4][4] = {{Mo, mo, oo, po}, {mo, oo, po, Po},
+                                 {oM, om, oo, op}, {om, oo, op, oP}};
+            std::memcpy(face, f, sizeof(f));
+            const int a[2][5] = {{Mo, mo, oo, po, Po}, {oM, om, oo, op, oP}};
+            std::memcpy(axis, a, sizeof(a));
+        } else {
+            ns = 5; center = 2; ndir = 2; naxis = 1;
+            for (int k = 0; k < 5; ++k) { ox[k] = k - 2; oy[k] = 0; }
+            adj[0] = 1; adj[1] = 3;
+            const int f[2][4] = {{0, 1, 2, 3}, {1, 2, 3, 4}};
+            for (int d = 0; d < 2; ++d) for (int j = 0; j < 4; ++j) face[d][j] = f[d][j];
+            for (int j = 0; j < 5; ++j) axis[0][j] = j;
+        }
+    }
+    /* StencilDir(k): Ym, Xm, Xp, Yp in 2-D (GridTraits.h WM2D_Grid) */
+    int sdir(int k) const { return ndir == 4 ? (k == 0 ? 2 : k == 1 ? 0 : k == 2 ? 1 : 3) : k; }
+    int dir_of_adj(int slot) const { for (int d = 0; d < ndir; ++d) if (adj[d] == slot) return d; return -1; }
+    bool in_face(int d, int slot) const { for (int j = 0; j < 4; ++j) if (face[d][j] == slot) return true; return false; }
+};
+
+/* CRS with the reference's storage rule: the diagonal first
+ * (CRS_Matrix.h:26-29, 59-68). Interior rows hold the stencil slots in
+ * slot order after the diagonal; a ghost row holds (diag, source). */
+struct Csr {
+    int n = 0;
+    vector<int> ptr, col;
+    vector<double> val;
+    vector<char> bc;
+    vector<int> slotpos;   /* interior row: position of slot k, -1 ghost */
+    int ns = 0;
+    double &at(int row, int slot) { return val[slotpos[(size_t)row * ns + slot]]; }
+    void clear_nonbc() {
+        for (int i = 0; i < n; ++i)
+            if (!bc[i]) for (int k = ptr[i]; k < ptr[i + 1]; ++k) val[k] = 0;
+    }
+    void matvec(const vector<double> &x, vector<double> &y) const {
+        y.assign(n, 0.0);
+        for (int i = 0; i < n; ++i) {
+            double s = 0;
+            for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[col[k]];
+            y[i] = s;
+        }
+    }
+};
+
+/* ------------------------------------------------------------------ */
+/* Mobility (Mobility.h:49-191)                                         */
+/* ------------------------------------------------------------------ */
+struct Mobility {
+    double lambda = 0, a = 0;
+    bool slip = false;
+    static constexpr double kSeriesRel = 1e-3;
+    void set(double l) { lambda = l > 0 ? l : 0.0; a = 3.0 * lambda; slip = lambda > 0; }
+    double D(double h) const { const double u = std::fabs(h); return h * u * (u + a); }
+    double dD(double h) const { const double u = std::fabs(h); return u * (3.0 * u + 2.0 * a); }
+    double DperH(double h) const { const double u = std::fabs(h); return u * (u + a); }
+    double dDperH(double h) const { return h < 0 ? -(2.0 * (-h) + a) : 2.0 * h + a; }
+    double Face(double h1, double h2) const {
+        if (slip) return FaceDK(h1, h2);
+        if (!(h1 > 0) || !(h2 > 0)) return 0;
+        return 2.0 * (h1 * h1) * (h2 * h2) / (h1 + h2);
+    }
+    double Face_d1(double h1, double h2) const {
+        if (slip) return FaceDK_d1(h1, h2);
+        if (!(h1 > 0) || !(h2 > 0)) return 0;
+        return 2.0 * (h1 * h2 * h2) * (h1 + h2 + h2) / ((h1 + h2) * (h1 + h2));
+    }
+    double Upwind(double hf) const { return DperH(hf); }
+    double dUpwind(double hf) const { return dDperH(hf); }
+    double Shear(double hf) const { return hf + 2.0 * lambda; }
+    double G(double h) const {
+        const double x = a / h;
+        double f;
+        if (x < 0.1) {
+            double t = x * x, s = 0;
+            for (int k = 2; k <= 18; ++k) { s += ((k % 2 == 0) ? t : -t) / k; t *= x; }
+            f = s;
+        } else f = x - std::log1p(x);
+        return -f / (a * a);
+    }
+    double K(double h) const { const double u = dD(h) / D(h); return 2.0 * u * u - (6.0 * h + 2.0 * a) / D(h); }
+    double dK(double h) const {
+        const double d = D(h), d1 = dD(h), d2 = 6.0 * h + 2.0 * a, d3 = 6.0;
+        const double u = d1 / d, du = d2 / d - u * u;
+        return 4.0 * u * du - (d3 * d - d2 * d1) / (d * d);
+    }
+    double FaceDK(double h1, double h2) const {
+        if (!(h1 > 0) || !(h2 > 0)) return 0;
+        const double hm = 0.5 * (h1 + h2), dl = h2 - h1;
+        if (std::fabs(dl) < kSeriesRel * hm) return D(hm) / (1.0 + dl * dl * K(hm) / 24.0);
+        return dl / (G(h2) - G(h1));
+    }
+    double FaceDK_d1(double h1, double h2) const {
+        if (!(h1 > 0) || !(h2 > 0)) return 0;
+        const double hm = 0.5 * (h1 + h2), dl = h2 - h1;
+        if (std::fabs(dl) < kSeriesRel * hm) {
+            const double k = K(hm), q = 1.0 + dl * dl * k / 24.0;
+            const double Fhm = (dD(hm) * q - D(hm) * dl * dl * dK(hm) / 24.0) / (q * q);
+            const double Fdl = -D(hm) * (dl * k / 12.0) / (q * q);
+            return 0.5 * Fhm - Fdl;
+        }
+        const double dg = G(h2) - G(h1);
+        return (-dg + dl / D(h1)) / (dg * dg);
+    }
+};
+
+/* ------------------------------------------------------------------ */
+/* Limited face interpolation (FiniteDifferences.cpp, Koren / VanLeer)  */
+/* ------------------------------------------------------------------ */
+struct Limiter {
+    int vanleer = 0;  /* 0 Koren, 1 van Leer */
+    double phi(double a, double b, double *dA = nullptr, double *dB = nullptr) const {
+        double p = 0, pa = 0, pb = 0;
+        const bool same = (a > 0 && b > 0) || (a < 0 && b < 0);
+        if (same) {
+            if (!vanleer) {
+                const double s = (b > 0) ? 1.0 : -1.0;
+                const double aa = std::fabs(a), ab = std::fabs(b);
+                const double t1 = 2.0 * aa, t2 = (2.0 * ab + aa) / 3.0, t3 = 2.0 * ab;
+                if (t1 <= t2 && t1 <= t3) { p = s * t1; pa = 2.0; pb = 0.0; }
+                else if (t2 <= t3) { p = s * t2; pa = 1.0 / 3.0; pb = 2.0 / 3.0; }
+                else { p = s * t3; pa = 0.0; pb = 2.0; }
+            } else {
+                const double sum = a + b, ra = a / sum, rb = b / sum;
+                p = 2.0 * a * rb; pa = 2.0 * rb * rb; pb = 2.0 * ra * ra;
+            }
+        }
+        if (dA) *dA = pa;
+        if (dB) *dB = pb;
+        return p;
+    }
+    static void pos(double sign, int &uu, int &u, int &d) {
+        if (sign >= 0) { uu = 0; u = 1; d = 2; } else { uu = 3; u = 2; d = 1; }
+    }
+    double value(const double f[4], double sign) const {
+        int uu, u, d; pos(sign, uu, u, d);
+        return f[u] + 0.5 * phi(f[u] - f[uu], f[d] - f[u]);
+    }
+    /* downwind form (rowPos = -1 in FaceWeights) */
+    void weights(const double f[4], double sign, double w[4]) const {
+        int uu, u, d; pos(sign, uu, u, d);
+        const double a = f[u] - f[uu], b = f[d] - f[u];
+        const double p = phi(a, b);
+        for (int j = 0; j < 4; ++j) w[j] = 0;
+        const double psi = (b != 0) ? p / b : 0.0;
+        w[u] = 1.0 - 0.5 * psi;
+        w[d] = 0.5 * psi;
+    }
+    double derivative(const double f[4], double sign, double g[4]) const {
+        int uu, u, d; pos(sign, uu, u, d);
+        double pa, pb;
+        const double p = phi(f[u] - f[uu], f[d] - f[u], &pa, &pb);
+        for (int j = 0; j < 4; ++j) g[j] = 0;
+        g[uu] = -0.5 * pa;
+        g[u] = 1.0 + 0.5 * (pa - pb);
+        g[d] = 0.5 * pb;
+        return f[u] + 0.5 * p;
+    }
+};
+
+static inline double Sign(double v) { return v >= 0.0 ? 1.0 : -1.0; }
+
+/* ------------------------------------------------------------------ */
+/* The film-height equation                                             */
+/* ------------------------------------------------------------------ */
+struct Equation {
+    yz_case cs;
+    Grid g;
+    Stencil st;
+    Mobility mob;
+    Limiter lim;
+    vector<double> h, s, c;          /* the working field (guess) */
+    vector<double> Ccap[4];          /* capillary C(d)[slot] */
+    vector<double> dFace[4];         /* capillary D of the last SetMatrixA */
+    Csr A, Arest, J, IA;
+
+    double visc(double cc) const { return std::exp(cs.visc_exp * (cc - cs.visc_c0)); }
+    double sigma(double cc) const { return 1.0 + (cs.marangoni ? cs.Ma : 0.0) * (cc - cs.sig_c0); }
+    double spacing(int axis) const { return axis == 0 ? g.dx : g.dy; }
+    int nb(int i, int slot) const {
+        const int ix = i % g.nxp, iy = i / g.nxp;
+        return g.row(ix + st.ox[slot], iy + st.oy[slot]);
+    }
+    bool ghost_row(int i) const { return !g.inside(i % g.nxp, i / g.nxp); }
+
+    void build_pattern(Csr &m) {
+        m.n = g.n; m.ns = st.ns;
+        m.ptr.assign(g.n + 1, 0); m.col.clear(); m.val.clear();
+        m.bc.assign(g.n, 0); m.slotpos.assign((size_t)g.n * st.ns, -1);
+        for (int i = 0; i < g.n; ++i) {
+            const int ix = i % g.nxp, iy = i / g.nxp;
+            if (!g.inside(ix, iy)) {
+                m.bc[i] = 1;
+                m.col.push_back(i); m.val.push_back(1.0);
+                const int sr = g.src(ix, iy);
+                m.col.push_back(sr); m.val.push_back(-1.0);
+            } else {
+                m.slotpos[(size_t)i * st.ns + st.center] = (int)m.col.size();
+                m.col.push_back(i); m.val.push_back(0.0);
+                for (int k = 0; k < st.ns; ++k) {
+                    if (k == st.center) continue;
+                    m.slotpos[(size_t)i * st.ns + k] = (int)m.col.size();
+                    m.col.push_back(nb(i, k)); m.val.push_back(0.0);
+                }
+            }
+            m.ptr[i + 1] = (int)m.col.size();
+        }
+    }
+
+    void build(const yz_case &c_) {
+        cs = c_;
+        g.build(cs);
+        st.build(cs.dim);
+        mob.set(cs.slip);
+        lim.vanleer = 0;
+        build_pattern(A); Arest = A; J = A; IA = A;
+        /* capillary C(d): LaplacePressure_2D.cpp:39-94 / _1D.cpp:35-77 */
+        for (int d = 0; d < st.ndir; ++d) Ccap[d].assign(st.ns, 0.0);
+        if (cs.dim == 2) {
+            enum { oM, mm, om, pm, Mo, mo, oo, po, Po, mp, op, pp, oP };
+            const double dx = g.dx, dy = g.dy, dx2 = dx * dx, dy2 = dy * dy;
+            const double dx4 = dx2 * dx2, dy4 = dy2 * dy2, dxy = dx2 * dy2;
+            vector<double> *C = Ccap;
+            C[0][mm] = +1.0 / dxy; C[0][om] = -1.0 / dxy; C[0][Mo] = +1.0 / dx4;
+            C[0][mo] = 1.0 * (-3.0 / dx4 - 2.0 / dxy); C[0][oo] = 1.0 * (+3.0 / dx4 + 2.0 / dxy);
+            C[0][po] = -1.0 / dx4; C[0][mp] = +1.0 / dxy; C[0][op] = -1.0 / dxy;
+            C[1][om] = -1.0 / dxy; C[1][pm] = +1.0 / dxy; C[1][mo] = -1.0 / dx4;
+            C[1][oo] = 1.0 * (+3.0 / dx4 + 2.0 / dxy); C[1][po] = 1.0 * (-3.0 / dx4 - 2.0 / dxy);
+            C[1][Po] = +1.0 / dx4; C[1][op] = -1.0 / dxy; C[1][pp] = +1.0 / dxy;
+            C[2][oM] = +1.0 / dy4; C[2][mm] = +1.0 / dxy; C[2][om] = 1.0 * (-3.0 / dy4 - 2.0 / dxy);
+            C[2][pm] = +1.0 / dxy; C[2][mo] = -1.0 / dxy; C[2][oo] = 1.0 * (+3.0 / dy4 + 2.0 / dxy);
+            C[2][po] = -1.0 / dxy; C[2][op] = -1.0 / dy4;
+            C[3][om] = -1.0 / dy4; C[3][mo] = -1.0 / dxy; C[3][oo] = 1.0 * (+3.0 / dy4 + 2.0 / dxy);
+            C[3][po] = -1.0 / dxy; C[3][mp] = +1.0 / dxy; C[3][op] = 1.0 * (-3.0 / dy4 - 2.0 / dxy);
+            C[3][pp] = +1.0 / dxy; C[3][oP] = +1.0 / dy4;
+        } else {
+            const double dx = g.dx, dx2 = dx * dx, dx4 = dx2 * dx2;
+            Ccap[0][0] = +1.0 / dx4; Ccap[0][1] = 1.0 * (-3.0 / dx4); Ccap[0][2] = 1.0 * (+3.0 / dx4); Ccap[0][3] = -1.0 / dx4;
+            Ccap[1][1] = -1.0 / dx4; Ccap[1][2] = 1.0 * (+3.0 / dx4); Ccap[1][3] = 1.0 * (-3.0 / dx4); Ccap[1][4] = +1.0 / dx4;
+        }
+        for (int d = 0; d < st.ndir; ++d) dFace[d].assign(g.n, 0.0);
+    }
+
+    void fill_ghosts(vector<double> &v) const {
+        for (int i = 0; i < g.n; ++i) {
+            const int ix = i % g.nxp, iy = i / g.nxp;
+            if (!g.inside(ix, iy)) v[i] = v[g.src(ix, iy)];
+        }
+    }
+
+    /* FaceGradient: LaplacePressure_2D.cpp:105-128, _1D.cpp */
+    double face_gradient(const double *v, int d) const {
+        if (cs.dim == 2) {
+            enum { oM, mm, om, pm, Mo, mo, oo, po, Po, mp, op, pp, oP };
+            const double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy;
+            const double dn4 = (d < 2) ? dx2 * dx2 : dy2 * dy2, dxy = dx2 * dy2;
+            double a1, a, b, b1, am, ap, bm, bp;
+            switch (d) {
+            case 0: a1 = v[Mo]; a = v[mo]; b = v[oo]; b1 = v[po]; am = v[mm]; ap = v[mp]; bm = v[om]; bp = v[op]; break;
+            case 1: a1 = v[mo]; a = v[oo]; b = v[po]; b1 = v[Po]; am = v[om]; ap = v[op]; bm = v[pm]; bp = v[pp]; break;
+            case 2: a1 = v[oM]; a = v[om]; b = v[oo]; b1 = v[op]; am = v[mm]; ap = v[pm]; bm = v[mo]; bp = v[po]; break;
+            default: a1 = v[om]; a = v[oo]; b = v[op]; b1 = v[oP]; am = v[mo]; ap = v[po]; bm = v[mp]; bp = v[pp]; break;
+            }
+            const double normal = (((b1 - b) - 2.0 * (b - a)) + (a - a1)) / dn4;
+            const double cross = (((bp - b) - (b - bm)) - ((ap - a) - (a - am))) / dxy;
+            return normal + cross;
+        }
+        const double dx2 = g.dx * g.dx, dx4 = dx2 * dx2;
+        double a1, a, b, b1;
+        if (d == 0) { a1 = v[0]; a = v[1]; b = v[2]; b1 = v[3]; }
+        else { a1 = v[1]; a = v[2]; b = v[3]; b1 = v[4]; }
+        return (((b1 - b) - 2.0 * (b - a)) + (a - a1)) / dx4;
+    }
+
+    /* capillary face mobility, NonlinearTerm (LaplacePressureKernel.h:242-270) */
+    double cap_D(int i, int d) const {
+        const int a = nb(i, st.adj[d]);
+        const double Hoo = h[i], Haj = h[a];
+        const double cf = 0.5 * (c[i] + c[a]);
+        const double val = mob.slip ? mob.FaceDK(Hoo, Haj) : 2.0 * (Hoo * Hoo) * (Haj * Haj) / (Hoo + Haj);
+        return val * sigma(cf) / visc(cf);
+    }
+    /* NonlinearJacobi (:279-322): derivative at the centre (cen) or the
+     * neighbour of face d */
+    double cap_dD(int i, int d, bool cen) const {
+        const int a = nb(i, st.adj[d]);
+        const double H1 = cen ? h[i] : h[a], H2 = cen ? h[a] : h[i];
+        const double cf = 0.5 * (c[i] + c[a]);
+        const double den = std::pow(H1 + H2, 2.0);
+        const double val = mob.slip ? mob.FaceDK_d1(H1, H2) : 2.0 * (H1 * H2 * H2) * (H1 + H2 + H2) / den;
+        return val * sigma(cf) / visc(cf);
+    }
+
+    /* ---- SetMatrixA of each term ---- */
+    void capillary_A(Csr &M) {
+        const double Coef = 1.0 / 3.0;
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double D[4], Tf[4];
+            for (int d = 0; d < st.ndir; ++d) { D[d] = cap_D(i, d); Tf[d] = D[d] / h[i]; dFace[d][i] = D[d]; }
+            double topo = 0;
+            for (int k = 0; k < st.ns; ++k) {
+                double cc = 0;
+                for (int d = 0; d < st.ndir; ++d) cc += Ccap[d][k] * Tf[d];
+                topo += cc * s[nb(i, k)];
+            }
+            for (int k = 0; k < st.ns; ++k) {
+                double v = 0;
+                for (int d = 0; d < st.ndir; ++d) v += Ccap[d][k] * D[d];
+                if (k == st.center) { v += topo; M.at(i, k) += Coef * v; }
+                else M.at(i, k) += Coef * v;
+            }
+        }
+    }
+    /* narrow C(d) of the second-difference terms */
+    double narrowC(int d, int slot) const {
+        const double hh = spacing(d / 2), h2 = hh * hh;
+        if (slot == st.adj[d]) return 1.0 / h2;
+        if (slot == st.center) return -1.0 / h2;
+        return 0.0;
+    }
+    double vg_G(int i, int d) const {
+        const int a = nb(i, st.adj[d]);
+        return mob.Face(h[i], h[a]) / visc(0.5 * (c[i] + c[a]));
+    }
+    double vg_T(int i, int d) const {
+        const int a = nb(i, st.adj[d]);
+        const double ho = h[i], ha = h[a];
+        const double gg = mob.Face(ho, ha);
+        const double mu = visc(0.5 * (c[i] + c[a]));
+        return (gg == 0) ? 0.0 : gg / (ho + ha) / mu;
+    }
+    double vg_dG(int i, int d, bool cen) const {
+        const int a = nb(i, st.adj[d]);
+        const double ho = h[i], ha = h[a];
+        const double mu = visc(0.5 * (c[i] + c[a]));
+        return (cen ? mob.Face_d1(ho, ha) : mob.Face_d1(ha, ho)) / mu;
+    }
+    void vgrav_A(Csr &M) {
+        const double Coef = -1.0 * cs.Bo / 3.0;
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double G[4], Tface[4], Tsum[4];
+            for (int d = 0; d < st.ndir; ++d) { G[d] = vg_G(i, d); Tface[d] = vg_T(i, d); }
+            for (int d = 0; d < st.ndir; ++d) {
+                Tsum[d] = 0;
+                for (int k = 0; k < st.ns; ++k) Tsum[d] += narrowC(d, k) * s[nb(i, k)];
+            }
+            double Too = 0;
+            for (int k = 0; k < st.ndir; ++k) { const int dd = st.sdir(k); Too += Tsum[dd] * Tface[dd]; }
+            for (int d = 0; d < st.ndir; ++d) Tsum[d] *= Tface[d];
+            /* NarrowIdx order: m o p (1-D) / om mo oo po op (2-D) */
+            for (int k = 0; k < st.ns; ++k) {
+                const bool isnarrow = (k == st.center) || st.dir_of_adj(k) >= 0;
+                if (!isnarrow) continue;
+                double v = 0;
+                for (int d = 0; d < st.ndir; ++d) v += narrowC(d, k) * G[d];
+                if (k == st.center) v += Too; else v += Tsum[st.dir_of_adj(k)];
+                M.at(i, k) += Coef * v;
+            }
+        }
+    }
+    double Pi(double hh) const {
+        if (hh <= 0) return 0;
+        const double r = cs.hstar / hh;
+        return std::pow(r, cs.dis_n) - std::pow(r, cs.dis_m);
+    }
+    double dPi(double hh) const {
+        if (hh <= 0) return 0;
+        const double r = cs.hstar / hh;
+        return -(cs.dis_n * std::pow(r, cs.dis_n) - cs.dis_m * std::pow(r, cs.dis_m)) / hh;
+    }
+    double PiPerH(double hh) const { if (hh <= 0) return 0; return Pi(hh) / hh; }
+    void disjoining_A(Csr &M) {
+        const double Coef = cs.Dp / 3.0;
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double G[4];
+            for (int d = 0; d < st.ndir; ++d) G[d] = vg_G(i, d);
+            for (int k = 0; k < st.ns; ++k) {
+                const bool isnarrow = (k == st.center) || st.dir_of_adj(k) >= 0;
+                if (!isnarrow) continue;
+                const double php = PiPerH(h[nb(i, k)]);
+                double v = 0;
+                for (int d = 0; d < st.ndir; ++d) v += narrowC(d, k) * G[d];
+                v *= php;
+                M.at(i, k) += Coef * v;
+            }
+        }
+    }
+
+    /* wall faces of the upwinded terms (GridTraits.h WallFace) */
+    bool wall_face(int i, int d) const {
+        const int ix = i % g.nxp, iy = i / g.nxp;
+        const int Nx = g.nxp, Ny = g.nyp;
+        auto isw = [](int t) { return t == YZ_BC_SYMMETRY || t == YZ_BC_PINNING; };
+        if ((d == 0 && ix == 2) || (d == 1 && ix == 1)) return isw(g.bc[0]);
+        if ((d == 1 && ix == Nx - 3) || (d == 0 && ix == Nx - 2)) return isw(g.bc[1]);
+        if (cs.dim == 2) {
+            if ((d == 2 && iy == 2) || (d == 3 && iy == 1)) return isw(g.bc[2]);
+            if ((d == 3 && iy == Ny - 3) || (d == 2 && iy == Ny - 2)) return isw(g.bc[3]);
+        }
+        return false;
+    }
+    void face_heights(int i, int d, double f[4]) const {
+        for (int j = 0; j < 4; ++j) f[j] = h[nb(i, st.face[d][j])];
+    }
+
+    /* body forces: mode 0 in-plane gravity, 1 centrifugal */
+    double bf_drive_face(int mode, int i, int d) const {
+        if (mode == 0) return Sign(d / 2 == 0 ? cs.Pgx : cs.Pgy);
+        const int ix = i % g.nxp, iy = i / g.nxp;
+        switch (d) {
+        case 0: return g.Xface(ix) - cs.rot_x;
+        case 1: return g.Xface(ix + 1) - cs.rot_x;
+        case 2: return g.Yface(iy) - cs.rot_y;
+        default: return g.Yface(iy + 1) - cs.rot_y;
+        }
+    }
+    double bf_weight(int mode, int i, int d) const { return mode == 0 ? 1.0 : bf_drive_face(mode, i, d); }
+    double bf_mag(int mode, int axis) const {
+        if (mode == 0) return (axis == 0 ? cs.Pgx : cs.Pgy) / 3.0;
+        return cs.Sp / 3.0;
+    }
+    /* LimitedFace (BodyForceKernel.h) */
+    void bf_face(int mode, int i, int j, double &hf, double &D, double wt[4], double *gd) const {
+        const double dv = bf_drive_face(mode, i, j);
+        const double hs = spacing(j / 2);
+        double f[4];
+        face_heights(i, j, f);
+        if (gd) { hf = lim.derivative(f, dv, gd); for (int q = 0; q < 4; ++q) gd[q] /= hs; }
+        else hf = lim.value(f, dv);
+        lim.weights(f, dv, wt);
+        for (int q = 0; q < 4; ++q) wt[q] /= hs;
+        const int a = nb(i, st.adj[j]);
+        const double cc = 0.5 * (c[i] + c[a]);
+        D = (mob.slip ? mob.Upwind(hf) : hf * hf) / visc(cc);
+        if (wall_face(i, j)) D = 0;
+    }
+    void bodyforce_A(int mode, Csr &M) {
+        const bool side = (mode == 1 && cs.dim == 1);
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double W[4][13] = {{0}}, D[4], w[4];
+            for (int j = 0; j < st.ndir; ++j) {
+                double hf, wt[4];
+                bf_face(mode, i, j, hf, D[j], wt, nullptr);
+                w[j] = bf_weight(mode, i, j);
+                for (int q = 0; q < 4; ++q) W[j][st.face[j][q]] = wt[q];
+            }
+            for (int a = 0; a < st.naxis; ++a) {
+                const int dm = 2 * a, dp = 2 * a + 1;
+                const double Coef = bf_mag(mode, a);
+                for (int q = 0; q < 5; ++q) {
+                    const int idx = st.axis[a][q];
+                    const double v = -w[dm] * W[dm][idx] * D[dm] + w[dp] * W[dp][idx] * D[dp];
+                    M.at(i, idx) += Coef * v;
+                }
+            }
+            if (side) {
+                const double hoo = h[i];
+                M.at(i, st.center) += mob.slip ? bf_mag(mode, 0) * (mob.DperH(hoo) / visc(c[i]))
+                                               : bf_mag(mode, 0) * (hoo * hoo / visc(c[i]));
+            }
+        }
+    }
+    /* Marangoni (SolutoCapillarityKernel.h, limited) */
+    double ma_coef() const { return cs.As * cs.Ma / 2.0; }
+    double ma_up(double dc) const { return Sign(ma_coef()) * Sign(dc); }
+    void face_conc(int i, int d, double &dc, double &cc) const {
+        const int lo = nb(i, st.face[d][1]), hi = nb(i, st.face[d][2]);
+        const double hh = spacing(d / 2);
+        dc = (c[hi] - c[lo]) / hh;
+        cc = (c[hi] + c[lo]) / 2.0;
+    }
+    void marangoni_A(Csr &M) {
+        const double Coef = ma_coef();
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double Val[13] = {0};
+            for (int d = 0; d < st.ndir; ++d) {
+                double dc, cc, f[4], wt[4];
+                face_conc(i, d, dc, cc);
+                face_heights(i, d, f);
+                const double up = ma_up(dc);
+                const double hf = lim.value(f, up);
+                lim.weights(f, up, wt);
+                const double S = wall_face(i, d) ? 0.0
+                               : mob.slip ? dc / visc(cc) * mob.Shear(hf) : dc / visc(cc) * hf;
+                const double hh = spacing(d / 2);
+                const double sg = (d % 2 == 0) ? -1.0 : +1.0;
+                for (int q = 0; q < 4; ++q) Val[st.face[d][q]] += sg * wt[q] * S / hh;
+            }
+            for (int k = 0; k < st.ns; ++k) {
+                bool wide = (k == st.center);
+                for (int d = 0; d < st.ndir; ++d) wide = wide || st.in_face(d, k);
+                if (wide) M.at(i, k) += Coef * Val[k];
+            }
+        }
+    }
+
+    /* ---- SetJacobiMatrix of each term ---- */
+    void capillary_J(Csr &M) {
+        const double Coef = 1.0 / 3.0;
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            const double Hoo = h[i];
+            double a[4], b[4], gA[4], gC[4], sA[4], sC[4];
+            for (int d = 0; d < st.ndir; ++d) {
+                a[d] = b[d] = 0;
+                for (int k = 0; k < st.ns; ++k) { a[d] += Ccap[d][k] * h[nb(i, k)]; b[d] += Ccap[d][k] * s[nb(i, k)]; }
+                gA[d] = cap_dD(i, d, false);
+                gC[d] = cap_dD(i, d, true);
+                sA[d] = gA[d] / Hoo;
+                sC[d] = gC[d] / Hoo - cap_D(i, d) / (Hoo * Hoo);
+            }
+            double ga = 0, sb = 0;
+            for (int k = 0; k < st.ndir; ++k) { const int dd = st.sdir(k); ga += gC[dd] * a[dd]; }
+            for (int d = 0; d < st.ndir; ++d) sb += sC[d] * b[d];
+            M.at(i, st.center) += Coef * (ga + sb * Hoo);
+            for (int d = 0; d < st.ndir; ++d)
+                M.at(i, st.adj[d]) += Coef * (gA[d] * a[d] + sA[d] * b[d] * Hoo);
+        }
+    }
+    void vgrav_J(Csr &M) {
+        const double Coef = -1.0 * cs.Bo / 3.0;
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double a[4], b[4], gA[4], gC[4], kF[4];
+            for (int d = 0; d < st.ndir; ++d) {
+                a[d] = b[d] = 0;
+                for (int k = 0; k < st.ns; ++k) { a[d] += narrowC(d, k) * h[nb(i, k)]; b[d] += narrowC(d, k) * s[nb(i, k)]; }
+                gA[d] = vg_dG(i, d, false);
+                gC[d] = vg_dG(i, d, true);
+                kF[d] = vg_T(i, d);
+            }
+            double ga = 0;
+            for (int k = 0; k < st.ndir; ++k) { const int dd = st.sdir(k); ga += gC[dd] * (a[dd] + b[dd]) - kF[dd] * b[dd]; }
+            M.at(i, st.center) += Coef * ga;
+            for (int d = 0; d < st.ndir; ++d)
+                M.at(i, st.adj[d]) += Coef * (gA[d] * (a[d] + b[d]) - kF[d] * b[d]);
+        }
+    }
+    void disjoining_J(Csr &M) {
+        const double Coef = cs.Dp / 3.0;
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double Ad[4], G[4], gA[4], gC[4];
+            for (int d = 0; d < st.ndir; ++d) {
+                Ad[d] = 0;
+                for (int k = 0; k < st.ns; ++k) Ad[d] += narrowC(d, k) * Pi(h[nb(i, k)]);
+                G[d] = vg_G(i, d);
+                gA[d] = vg_dG(i, d, false);
+                gC[d] = vg_dG(i, d, true);
+            }
+            double ga = 0, gc = 0;
+            for (int k = 0; k < st.ndir; ++k) {
+                const int dd = st.sdir(k);
+                ga += gC[dd] * Ad[dd];
+                gc += G[dd] * narrowC(dd, st.center);
+            }
+            {
+                const double hj = h[i];
+                const double dpi = dPi(hj) - PiPerH(hj);
+                M.at(i, st.center) += Coef * (ga + gc * dpi);
+            }
+            for (int d = 0; d < st.ndir; ++d) {
+                const int idx = st.adj[d];
+                const double hj = h[nb(i, idx)];
+                const double dpi = dPi(hj) - PiPerH(hj);
+                M.at(i, idx) += Coef * (gA[d] * Ad[d] + G[d] * narrowC(d, idx) * dpi);
+            }
+        }
+    }
+    void bodyforce_J(int mode, Csr &M) {
+        const bool side = (mode == 1 && cs.dim == 1);
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double X[4][13] = {{0}};
+            for (int j = 0; j < st.ndir; ++j) {
+                double hf, D, wt[4], gd[4];
+                bf_face(mode, i, j, hf, D, wt, gd);
+                const double r3 = !mob.slip ? 3.0 : (mob.Upwind(hf) != 0.0 ? mob.dD(hf) / mob.Upwind(hf) : 0.0);
+                const double sg = (j % 2 == 0) ? -1.0 : +1.0;
+                const double w = bf_weight(mode, i, j);
+                for (int q = 0; q < 4; ++q)
+                    X[j][st.face[j][q]] = mob.slip ? sg * w * (r3 * gd[q] - wt[q]) * D : sg * w * (3.0 * gd[q] - wt[q]) * D;
+            }
+            for (int a = 0; a < st.naxis; ++a) {
+                const int dm = 2 * a, dp = 2 * a + 1;
+                const double Coef = bf_mag(mode, a);
+                for (int q = 0; q < 5; ++q) {
+                    const int idx = st.axis[a][q];
+                    M.at(i, idx) += Coef * (X[dm][idx] + X[dp][idx]);
+                }
+            }
+            if (side) {
+                const double hoo = h[i];
+                M.at(i, st.center) += mob.slip ? bf_mag(mode, 0) * ((mob.dD(hoo) - mob.DperH(hoo)) / visc(c[i]))
+                                               : bf_mag(mode, 0) * (2.0 * hoo * hoo / visc(c[i]));
+            }
+        }
+    }
+    void marangoni_J(Csr &M) {
+        const double Coef = ma_coef();
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double a[4], gg[4][13] = {{0}};
+            for (int d = 0; d < st.ndir; ++d) {
+                double dc, cc, f[4], wt[4], gd[4];
+                face_conc(i, d, dc, cc);
+                face_heights(i, d, f);
+                const double up = ma_up(dc);
+                const double hf = lim.derivative(f, up, gd);
+                lim.weights(f, up, wt);
+                const double r = wall_face(i, d) ? 0.0 : dc / visc(cc);
+                const double hh = spacing(d / 2);
+                const double sg = (d % 2 == 0) ? -1.0 : +1.0;
+                if (mob.slip) {
+                    const double L2 = 2.0 * mob.lambda;
+                    a[d] = sg / hh;
+                    for (int q = 0; q < 4; ++q) gg[d][st.face[d][q]] = r * ((2.0 * hf + L2) * gd[q] - (hf + L2) * wt[q]);
+                    continue;
+                }
+                a[d] = sg * hf / hh;
+                for (int q = 0; q < 4; ++q) gg[d][st.face[d][q]] = r * (2.0 * gd[q] - wt[q]);
+            }
+            /* ScatterJacobiWide (UpwindKernelBase.h) */
+            for (int k = 0; k < st.ns; ++k) {
+                bool wide = (k == st.center);
+                for (int d = 0; d < st.ndir; ++d) wide = wide || st.in_face(d, k);
+                if (!wide) continue;
+                double v = 0;
+                if (k == st.center) {
+                    for (int j = 0; j < st.ndir; ++j) { const int d = st.sdir(j); v += gg[d][k] * a[d]; }
+                } else {
+                    const int first = st.dir_of_adj(k);
+                    if (first >= 0 && st.in_face(first, k)) v += gg[first][k] * a[first];
+                    for (int d = 0; d < st.ndir; ++d) {
+                        if (d == first || !st.in_face(d, k)) continue;
+                        v += gg[d][k] * a[d];
+                    }
+                }
+                M.at(i, k) += Coef * v;
+            }
+        }
+    }
+
+    /* ---- SetMatrixA (HeightEquationBase.h:669-751), order of
+     * RegisterPhysics: capillary, vertical gravity, in-plane gravity,
+     * Marangoni, centrifugal, disjoining ---- */
+    void set_matrix_A() {
+        A.clear_nonbc(); Arest.clear_nonbc();
+        if (cs.capillary) capillary_A(A);
+        if (cs.vertical_gravity) vgrav_A(A);
+        if (cs.inplane_gravity) bodyforce_A(0, Arest);
+        if (cs.marangoni) marangoni_A(Arest);
+        if (cs.centrifugal) bodyforce_A(1, Arest);
+        if (cs.disjoining) disjoining_A(Arest);
+        for (int i = 0; i < g.n; ++i) {
+            if (A.bc[i]) continue;
+            for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) A.val[k] += Arest.val[k];
+        }
+    }
+    /* SetMatrixJacobi (:1061-1091) */
+    void set_matrix_J() {
+        for (int i = 0; i < g.n; ++i) {
+            if (A.bc[i]) continue;
+            for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) J.val[k] = A.val[k];
+        }
+        if (cs.capillary) capillary_J(J);
+        if (cs.vertical_gravity) vgrav_J(J);
+        if (cs.inplane_gravity) bodyforce_J(0, J);
+        if (cs.marangoni) marangoni_J(J);
+        if (cs.centrifugal) bodyforce_J(1, J);
+        if (cs.disjoining) disjoining_J(J);
+    }
+    /* ApplyA (:803-827) with AddConservative of the capillary
+     * (LaplacePressureKernel.h:439-477) and vertical gravity
+     * (VerticalGravityKernel.h:233-269) */
+    void apply_A(const vector<double> &x, vector<double> &y) const {
+        const bool any = cs.capillary || cs.vertical_gravity;
+        if (!any) { A.matvec(x, y); return; }
+        Arest.matvec(x, y);
+        for (int i = 0; i < g.n; ++i) {
+            if (!A.bc[i]) continue;
+            double v = 0;
+            for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) v += A.val[k] * x[A.col[k]];
+            y[i] = v;
+        }
+        if (cs.capillary) {
+            const double Coef = 1.0 / 3.0;
+            double vh[13], vs[13];
+            for (int i = 0; i < g.n; ++i) {
+                if (A.bc[i]) continue;
+                for (int k = 0; k < st.ns; ++k) { vh[k] = x[nb(i, k)]; vs[k] = s[nb(i, k)]; }
+                double r = 0;
+                for (int d = 0; d < st.ndir; ++d) {
+                    const double f = Coef * (dFace[d][i] * (face_gradient(vh, d) + face_gradient(vs, d)));
+                    r += (d % 2 == 0) ? -f : f;
+                }
+                y[i] += r;
+            }
+        }
+        if (cs.vertical_gravity) {
+            const double Coef = -1.0 * cs.Bo / 3.0;
+            for (int i = 0; i < g.n; ++i) {
+                if (A.bc[i]) continue;
+                double r = 0;
+                for (int d = 0; d < st.ndir; ++d) {
+                    const int a = nb(i, st.adj[d]);
+                    const int lo = (d % 2 == 0) ? a : i, hi = (d % 2 == 0) ? i : a;
+                    const double dn = spacing(d / 2);
+                    const double hl = h[lo], hh = h[hi];
+                    const double mu = visc(0.5 * (c[lo] + c[hi]));
+                    double gf = mob.Face(hl, hh);
+                    const double tk = (gf == 0) ? 0.0 : gf / (hl + hh) / mu;
+                    gf /= mu;
+                    const double xl = x[lo], xh = x[hi];
+                    const double f = Coef * ((gf * (xh - xl) + (s[hi] - s[lo]) * (tk * (xl + xh))) / (dn * dn));
+                    r += (d % 2 == 0) ? -f : f;
+                }
+                y[i] += r;
+            }
+        }
+    }
+};
+
+/* ------------------------------------------------------------------ */
+/* Linear algebra: ScaleRows, BiCGSTAB, SA-AMG, SPAI0, damped Jacobi    */
+/* ------------------------------------------------------------------ */
+struct SpM {  /* plain CSR, sorted columns not required */
+    int n = 0;
+    vector<int> ptr, col;
+    vector<double> val;
+    void mv(const double *x, double *y) const {
+        for (int i = 0; i < n; ++i) {
+            double s = 0;
+            for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[col[k]];
+            y[i] = s;
+        }
+    }
+    double diag(int i) const {
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) if (col[k] == i) return val[k];
+        return 0;
+    }
+};
+
+static double dot(const vector<double> &a, const vector<double> &b) {
+    double s = 0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+static SpM transpose(const SpM &A, int ncols) {
+    SpM T; T.n = ncols;
+    T.ptr.assign(ncols + 1, 0);
+    for (int k = 0; k < A.ptr[A.n]; ++k) T.ptr[A.col[k] + 1]++;
+    for (int i = 0; i < ncols; ++i) T.ptr[i + 1] += T.ptr[i];
+    T.col.resize(A.ptr[A.n]); T.val.resize(A.ptr[A.n]);
+    vector<int> pos(T.ptr.begin(), T.ptr.end() - 1);
+    for (int i = 0; i < A.n; ++i)
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+            const int p = pos[A.col[k]]++;
+            T.col[p] = i; T.val[p] = A.val[k];
+        }
+    return T;
+}
+/* C = A * B (A: n x m, B: m x p) */
+static SpM spgemm(const SpM &A, const SpM &B, int p) {
+    SpM C; C.n = A.n; C.ptr.assign(A.n + 1, 0);
+    vector<int> marker(p, -1);
+    vector<double> acc(p, 0.0);
+    vector<int> cols;
+    for (int i = 0; i < A.n; ++i) {
+        cols.clear();
+        for (int ka = A.ptr[i]; ka < A.ptr[i + 1]; ++ka) {
+            const int j = A.col[ka]; const double va = A.val[ka];
+            for (int kb = B.ptr[j]; kb < B.ptr[j + 1]; ++kb) {
+                const int c = B.col[kb];
+                if (marker[c] != i) { marker[c] = i; acc[c] = 0; cols.push_back(c); }
+                acc[c] += va * B.val[kb];
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        for (int c : cols) { C.col.push_back(c); C.val.push_back(acc[c]); }
+        C.ptr[i + 1] = (int)C.col.size();
+    }
+    return C;
+}
+
+struct AmgLevel {
+    SpM A, P, R;
+    vector<double> spai;      /* SPAI0 diagonal */
+    vector<double> x, b, r;
+    int nc = 0;
+};
+
+struct Amg {
+    vector<AmgLevel> lv;
+    vector<double> lu;        /* dense LU of the coarsest level */
+    vector<int> piv;
+    int nco = 0;
+    int coarse_enough = 500;
+
+    void build(const SpM &A0) {
+        lv.clear();
+        lv.emplace_back();
+        lv[0].A = A0;
+        double eps_strong = 0.08;
+        while (lv.back().A.n > coarse_enough) {
+            AmgLevel &L = lv.back();
+            const SpM &A = L.A;
+            const int n = A.n;
+            vector<double> dia(n);
+            for (int i = 0; i < n; ++i) dia[i] = A.diag(i);
+            /* strength: a_ij^2 > eps^2 |a_ii a_jj| */
+            const double e2 = eps_strong * eps_strong;
+            vector<char> strong(A.ptr[n], 0);
+            for (int i = 0; i < n; ++i)
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+                    const int j = A.col[k];
+                    if (j == i) continue;
+                    const double v = A.val[k];
+                    strong[k] = (v * v > e2 * std::fabs(dia[i] * dia[j])) ? 1 : 0;
+                }
+            /* aggregation: pass 1 seeds whose strong neighbourhood is free;
+             * pass 2 attaches the rest to a neighbouring aggregate; pass 3
+             * makes aggregates of what is left */
+            const int UND = -1, ISO = -2;
+            vector<int> id(n, UND);
+            for (int i = 0; i < n; ++i) {
+                bool any = false;
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) if (strong[k]) { any = true; break; }
+                if (!any) id[i] = ISO;
+            }
+            int na = 0;
+            for (int i = 0; i < n; ++i) {
+                if (id[i] != UND) continue;
+                bool free = true;
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+                    if (strong[k] && id[A.col[k]] != UND) { free = false; break; }
+                if (!free) continue;
+                id[i] = na;
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) if (strong[k]) id[A.col[k]] = na;
+                ++na;
+            }
+            vector<int> id2 = id;
+            for (int i = 0; i < n; ++i) {
+                if (id[i] != UND) continue;
+                double best = -1; int ba = UND;
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+                    if (strong[k] && id[A.col[k]] >= 0 && std::fabs(A.val[k]) > best) { best = std::fabs(A.val[k]); ba = id[A.col[k]]; }
+                if (ba != UND) id2[i] = ba;
+            }
+            id = id2;
+            for (int i = 0; i < n; ++i) {
+                if (id[i] != UND) continue;
+                id[i] = na;
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) if (strong[k] && id[A.col[k]] == UND) id[A.col[k]] = na;
+                ++na;
+            }
+            if (na == 0 || na >= n) break;
+            /* tentative prolongator, columns of unit norm */
+            vector<int> cnt(na, 0);
+            for (int i = 0; i < n; ++i) if (id[i] >= 0) cnt[id[i]]++;
+            SpM Pt; Pt.n = n; Pt.ptr.assign(n + 1, 0);
+            for (int i = 0; i < n; ++i) {
+                if (id[i] >= 0) { Pt.col.push_back(id[i]); Pt.val.push_back(1.0 / std::sqrt((double)cnt[id[i]])); }
+                Pt.ptr[i + 1] = (int)Pt.col.size();
+            }
+            /* filtered matrix: weak connections lumped into the diagonal */
+            SpM Af; Af.n = n; Af.ptr.assign(n + 1, 0);
+            vector<double> df(n);
+            for (int i = 0; i < n; ++i) {
+                double dd = 0;
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+                    const int j = A.col[k];
+                    if (j == i) dd += A.val[k];
+                    else if (!strong[k]) dd += A.val[k];
+                }
+                df[i] = dd;
+                Af.col.push_back(i); Af.val.push_back(dd);
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+                    if (A.col[k] != i && strong[k]) { Af.col.push_back(A.col[k]); Af.val.push_back(A.val[k]); }
+                Af.ptr[i + 1] = (int)Af.col.size();
+            }
+            /* omega = 4/3 / rho(D^-1 Af), rho by Gershgorin */
+            double rho = 0;
+            for (int i = 0; i < n; ++i) {
+                double sr = 0;
+                for (int k = Af.ptr[i]; k < Af.ptr[i + 1]; ++k) sr += std::fabs(Af.val[k]);
+                sr /= std::fabs(df[i]);
+                rho = std::max(rho, sr);
+            }
+            const double omega = (4.0 / 3.0) / rho;
+            SpM S = Af;
+            for (int i = 0; i < n; ++i)
+                for (int k = S.ptr[i]; k < S.ptr[i + 1]; ++k)
+                    S.val[k] = (S.col[k] == i ? 1.0 : 0.0) - omega * Af.val[k] / df[i];
+            L.P = spgemm(S, Pt, na);
+            L.R = transpose(L.P, na);
+            L.nc = na;
+            SpM AP = spgemm(A, L.P, na);
+            AmgLevel next;
+            next.A = spgemm(L.R, AP, na);
+            lv.push_back(std::move(next));
+            eps_strong *= 0.5;
+        }
+        for (auto &L : lv) {
+            const SpM &A = L.A;
+            L.spai.assign(A.n, 0.0);
+            for (int i = 0; i < A.n; ++i) {
+                double ss = 0, dd = 0;
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) { ss += A.val[k] * A.val[k]; if (A.col[k] == i) dd = A.val[k]; }
+                L.spai[i] = (ss > 0) ? dd / ss : 0.0;
+            }
+            L.x.assign(A.n, 0); L.b.assign(A.n, 0); L.r.assign(A.n, 0);
+        }
+        /* coarsest: dense LU with partial pivoting */
+        const SpM &C = lv.back().A;
+        nco = C.n;
+        lu.assign((size_t)nco * nco, 0.0);
+        for (int i = 0; i < nco; ++i)
+            for (int k = C.ptr[i]; k < C.ptr[i + 1]; ++k) lu[(size_t)i * nco + C.col[k]] += C.val[k];
+        piv.assign(nco, 0);
+        for (int k = 0; k < nco; ++k) {
+            int p = k;
+            for (int i = k + 1; i < nco; ++i) if (std::fabs(lu[(size_t)i * nco + k]) > std::fabs(lu[(size_t)p * nco + k])) p = i;
+            piv[k] = p;
+            if (p != k) for (int j = 0; j < nco; ++j) std::swap(lu[(size_t)k * nco + j], lu[(size_t)p * nco + j]);
+            const double d = lu[(size_t)k * nco + k];
+            if (d == 0) continue;
+            for (int i = k + 1; i < nco; ++i) {
+                const double m = lu[(size_t)i * nco + k] / d;
+                lu[(size_t)i * nco + k] = m;
+                for (int j = k + 1; j < nco; ++j) lu[(size_t)i * nco + j] -= m * lu[(size_t)k * nco + j];
+            }
+        }
+    }
+    void coarse_solve(const vector<double> &b, vector<double> &x) const {
+        x = b;
+        for (int k = 0; k < nco; ++k) std::swap(x[k], x[piv[k]]);
+        for (int i = 0; i < nco; ++i)
+            for (int j = 0; j < i; ++j) x[i] -= lu[(size_t)i * nco + j] * x[j];
+        for (int i = nco - 1; i >= 0; --i) {
+            for (int j = i + 1; j < nco; ++j) x[i] -= lu[(size_t)i * nco + j] * x[j];
+            const double d = lu[(size_t)i * nco + i];
+            x[i] = (d != 0) ? x[i] / d : 0.0;
+        }
+    }
+    void cycle(size_t l) {
+        AmgLevel &L = lv[l];
+        if (l + 1 == lv.size()) { coarse_solve(L.b, L.x); return; }
+        const int n = L.A.n;
+        /* pre-smoothing from x = 0: x = M b */
+        for (int i = 0; i < n; ++i) L.x[i] = L.spai[i] * L.b[i];
+        L.A.mv(L.x.data(), L.r.data());
+        for (int i = 0; i < n; ++i) L.r[i] = L.b[i] - L.r[i];
+        AmgLevel &Nx = lv[l + 1];
+        L.R.mv(L.r.data(), Nx.b.data());
+        cycle(l + 1);
+        vector<double> corr(n);
+        L.P.mv(Nx.x.data(), corr.data());
+        for (int i = 0; i < n; ++i) L.x[i] += corr[i];
+        L.A.mv(L.x.data(), L.r.data());
+        for (int i = 0; i < n; ++i) L.x[i] += L.spai[i] * (L.b[i] - L.r[i]);
+    }
+    void apply(const vector<double> &rhs, vector<double> &x) {
+        lv[0].b = rhs;
+        cycle(0);
+        x = lv[0].x;
+    }
+};
+
+/* BiCGSTAB with right preconditioning; returns 0 / 1 / 2 as the backend */
+template <class Pc>
+static int bicgstab(const SpM &A, const vector<double> &b, vector<double> &x, Pc &M,
+                    int maxit, double tol, int &iters, double &resid) {
+    const int n = A.n;
+    vector<double> r(n), rh, p(n, 0), v(n, 0), ph(n), s(n), sh(n), t(n);
+    A.mv(x.data(), r.data());
+    for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    rh = r;
+    const double nb = std::sqrt(dot(b, b));
+    iters = 0;
+    if (nb == 0) { std::fill(x.begin(), x.end(), 0.0); resid = 0; return 0; }
+    const double eps = tol * nb;
+    double res = std::sqrt(dot(r, r));
+    if (res <= eps) { resid = res / nb; return 0; }
+    double rho1 = 0, rho2 = 1, alpha = 1, omega = 1;
+    int it = 0;
+    for (; it < maxit; ++it) {
+        rho1 = dot(r, rh);
+        if (std::fabs(rho1) < 1e-300) { iters = it; resid = res / nb; return 2; }
+        if (it == 0) p = r;
+        else {
+            const double beta = (rho1 * alpha) / (rho2 * omega);
+            for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        }
+        M.apply(p, ph);
+        A.mv(ph.data(), v.data());
+        const double rv = dot(rh, v);
+        if (rv == 0) { iters = it; resid = res / nb; return 2; }
+        alpha = rho1 / rv;
+        for (int i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+        res = std::sqrt(dot(s, s));
+        if (res < eps) {
+            for (int i = 0; i < n; ++i) x[i] += alpha * ph[i];
+            ++it;
+            break;
+        }
+        M.apply(s, sh);
+        A.mv(sh.data(), t.data());
+        const double tt = dot(t, t);
+        omega = (tt != 0) ? dot(t, s) / tt : 0.0;
+        if (omega == 0) { iters = it; resid = res / nb; return 2; }
+        for (int i = 0; i < n; ++i) x[i] += alpha * ph[i] + omega * sh[i];
+        for (int i = 0; i < n; ++i) r[i] = s[i] - omega * t[i];
+        res = std::sqrt(dot(r, r));
+        rho2 = rho1;
+        if (res < eps) { ++it; break; }
+        if (!(res == res)) { iters = it; resid = res; return 2; }
+    }
+    iters = it;
+    resid = res / nb;
+    if (!(resid == resid)) return 2;
+    return resid <= tol ? 0 : 1;
+}
+
+struct Jacobi {
+    vector<double> dinv;
+    void build(const SpM &A) {
+        dinv.resize(A.n);
+        for (int i = 0; i < A.n; ++i) { const double d = A.diag(i); dinv[i] = d != 0 ? 0.72 / d : 0.0; }
+    }
+    void apply(const vector<double> &b, vector<double> &x) const {
+        x.resize(b.size());
+        for (size_t i = 0; i < b.size(); ++i) x[i] = dinv[i] * b[i];
+    }
+};
+
+/* one solve through the backend contract (LinearSolverBackend.cpp:539-728) */
+static int linsolve(const Csr &M, const vector<double> &rhs, vector<double> &x,
+                    int precond, int scaling, int maxit, double eps, int &iter, double &resid) {
+    SpM A; A.n = M.n; A.ptr = M.ptr; A.col = M.col; A.val = M.val;
+    vector<double> b = rhs;
+    if (scaling) {
+        for (int i = 0; i < A.n; ++i) {
+            const double d = M.val[M.ptr[i]];      /* the diagonal is stored first */
+            if (std::fabs(d) < 1.0e-30) continue;
+            const double inv = 1.0 / d;
+            for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) A.val[k] *= inv;
+            b[i] *= inv;
+        }
+    }
+    if (precond == YZ_PC_AMG) {
+        Amg amg; amg.build(A);
+        return bicgstab(A, b, x, amg, maxit, eps, iter, resid);
+    }
+    Jacobi jac; jac.build(A);
+    return bicgstab(A, b, x, jac, maxit, eps, iter, resid);
+}
+
+/* ------------------------------------------------------------------ */
+/* Newton-Kantorovich step (NewtonKantorovich.cpp, ImplicitSolver.cpp)  */
+/* ------------------------------------------------------------------ */
+struct Stepper {
+    Equation eq;
+    vector<double> phi_old2, phi_old, phi_new, phi_guess, Ax_old, Ax_guess, Rhs, corr;
+    double time = 0, dt_now = 0, dt_old = 0;
+    int tsc_count = 0;
+    bool extrap_blocked = false;
+    double extrap_move = 0;
+    double ew_prev = 0, ew_eta = 0;
+    int nk_pass = 0;
+    long long lin_total = 0;
+    bool first_step = true;
+
+    void init(const yz_case &c, const double *h, const double *s, const double *conc, double dt0) {
+        eq.build(c);
+        const int n = eq.g.n;
+        eq.h.assign(h, h + n); eq.s.assign(s, s + n); eq.c.assign(conc, conc + n);
+        eq.fill_ghosts(eq.h); eq.fill_ghosts(eq.s); eq.fill_ghosts(eq.c);
+        phi_new = eq.h; phi_old = eq.h; phi_old2 = eq.h;
+        Ax_old.assign(n, 0); Ax_guess.assign(n, 0); Rhs.assign(n, 0); corr.assign(n, 0);
+        dt_now = dt0; dt_old = 0;
+    }
+    static double absmax(const vector<double> &v) {
+        double m = 0;
+        for (double x : v) { const double a = std::fabs(x); if (a > m) m = a; }
+        return m;
+    }
+    void make_IA() {
+        const double th = eq.cs.theta, dt = dt_now;
+        Csr &J = eq.J, &IA = eq.IA;
+        for (int i = 0; i < J.n; ++i) {
+            const int p0 = J.ptr[i], p1 = J.ptr[i + 1];
+            if (J.bc[i]) { for (int k = p0; k < p1; ++k) IA.val[k] = J.val[k]; continue; }
+            IA.val[p0] = 1.0 + th * dt * J.val[p0];
+            for (int k = p0 + 1; k < p1; ++k) IA.val[k] = th * dt * J.val[k];
+        }
+    }
+    void make_rhs() {
+        const double th = eq.cs.theta, dt = dt_now;
+        for (int i = 0; i < eq.g.n; ++i) {
+            if (eq.A.bc[i]) { Rhs[i] = 0; continue; }
+            Rhs[i] = phi_old[i] - phi_guess[i] - (dt * th * Ax_guess[i]) - (dt * (1.0 - th) * Ax_old[i]) - dt * 0.0;
+        }
+    }
+    /* SatisfyBCRows for single-row sets */
+    void satisfy_bc(vector<double> &x) const {
+        const Csr &M = eq.IA;
+        for (int i = 0; i < M.n; ++i) {
+            if (!M.bc[i]) continue;
+            double sum = 0;
+            for (int k = M.ptr[i] + 1; k < M.ptr[i + 1]; ++k) sum += M.val[k] * x[M.col[k]];
+            x[i] = (Rhs[i] - sum) / M.val[M.ptr[i]];
+        }
+    }
+    /* Solve (NewtonKantorovich.cpp:74-154) */
+    int solve(int &iter, double &resid) {
+        eq.h = phi_guess;
+        eq.set_matrix_A();
+        if (nk_pass % eq.cs.jac_interval == 0) { eq.set_matrix_J(); make_IA(); }
+        ++nk_pass;
+        eq.apply_A(phi_guess, Ax_guess);
+        make_rhs();
+        double eps = eq.cs.lin_eps;
+        if (eq.cs.ew) {
+            const double kEtaMax = 1.0e-4;
+            const double rn = absmax(Rhs);
+            double eta = kEtaMax;
+            if (nk_pass > 1 && ew_prev > 0) {
+                const double ratio = rn / ew_prev;
+                eta = 0.9 * ratio * ratio;
+                const double safe = 0.9 * ew_eta * ew_eta;
+                if (safe > 0.1 && safe > eta) eta = safe;
+            }
+            if (eta > kEtaMax) eta = kEtaMax;
+            if (eta < eq.cs.lin_eps) eta = eq.cs.lin_eps;
+            ew_eta = eta; ew_prev = rn;
+            eps = eta;
+        }
+        const int ch = linsolve(eq.IA, Rhs, corr, eq.cs.precond == YZ_PC_JACOBI ? YZ_PC_JACOBI : YZ_PC_AMG,
+                                eq.cs.row_scaling, eq.cs.lin_maxit, eps, iter, resid);
+        if (ch == 0) satisfy_bc(corr);
+        if (ch != 0) std::fill(corr.begin(), corr.end(), 0.0);
+        return ch;
+    }
+    /* NKLoop (:157-422) */
+    bool nk_loop(int &iter, double &resid_eq) {
+        nk_pass = 0;
+        int lin_step = 0;
+        phi_guess = phi_old;
+        extrap_move = 0;
+        if (eq.cs.extrap && !extrap_blocked && !first_step) {
+            const double r = (dt_old > 0) ? dt_now / dt_old : 0.0;
+            bool ok = (r > 0);
+            double move = 0;
+            for (size_t i = 0; i < phi_old.size() && ok; ++i) {
+                const double gg = phi_old[i] + r * (phi_old[i] - phi_old2[i]);
+                if (!eq.A.bc[i] && phi_old[i] > 0 && !(gg > 0)) ok = false;
+                const double m = std::fabs(gg - phi_old[i]);
+                if (m > move) move = m;
+            }
+            if (ok && move > 0) {
+                for (size_t i = 0; i < phi_old.size(); ++i) phi_guess[i] = phi_old[i] + r * (phi_old[i] - phi_old2[i]);
+                extrap_move = move;
+            }
+        }
+        /* BeforeNKLoop: the working field from the state of the step */
+        eq.h = phi_old;
+        eq.set_matrix_A();
+        eq.apply_A(phi_old, Ax_old);
+
+        double resid_eq_old = 1e40, resid_first = 0, resid_best = 1e40;
+        int stall = 0;
+        const int stall_limit = 5;
+        bool done = false;
+        int outer;
+        resid_eq = 0;
+        for (outer = 0; outer < eq.cs.nk_max; ++outer) {
+            int it = 0; double lr = 0;
+            const int ch = solve(it, lr);
+            lin_step += it;
+            resid_eq = absmax(corr);
+            if (std::getenv("YZO_DEBUG"))
+                std::fprintf(stderr, "[oracle] pass %d lin %d resid_lin %.3e corr %.3e rc %d\n",
+                             outer, it, lr, resid_eq, ch);
+            if (ch != 0) { done = false; break; }
+            if (outer == 0) resid_first = resid_eq;
+            if (resid_eq < resid_best * 0.9) { resid_best = resid_eq; stall = 0; }
+            else stall++;
+            if (stall > stall_limit) {
+                const double floorPhi = 1000.0 * std::numeric_limits<double>::epsilon() * absmax(phi_guess);
+                done = (resid_eq < eq.cs.nk_eps) || (resid_first > 0 && resid_eq < resid_first * 1.0e-4)
+                    || (resid_eq <= floorPhi);
+                break;
+            }
+            const double alpha = (resid_eq > resid_eq_old) ? 0.5 : 1.0;
+            for (size_t i = 0; i < phi_guess.size(); ++i) phi_guess[i] += corr[i] * alpha;
+            if (outer >= eq.cs.nk_min - 1) {
+                if (resid_eq < eq.cs.nk_eps) { done = true; break; }
+                if (eq.cs.nk_eps_rel > 0 && resid_first > 0 && resid_eq < resid_first * eq.cs.nk_eps_rel) { done = true; break; }
+            }
+            resid_eq_old = resid_eq;
+        }
+        iter = outer + 1;
+        lin_total = lin_step;
+        return done;
+    }
+    /* LocalTruncationCriterion (ImplicitSolver.cpp:271-307) */
+    double truncation_criterion() const {
+        double th = eq.cs.tsc_criterion;
+        if (th == 0) th = 1.0e-2;
+        if (dt_old == 0) return 1.0;
+        const double wfloor = eq.cs.tsc_floor * absmax(phi_old);
+        double ek = 0;
+        for (size_t i = 0; i < phi_guess.size(); ++i) {
+            const double hm = phi_old2[i], ho = phi_old[i], hp = phi_guess[i];
+            double w = std::fabs(ho);
+            if (w < wfloor) w = wfloor;
+            const double den = (dt_old + dt_now) * w;
+            const double e = (den == 0) ? 0.0 : 2.0 * (dt_now / dt_old) * (dt_old * hp + dt_now * hm - (dt_old + dt_now) * ho) / den;
+            ek = std::max(ek, std::fabs(e));
+        }
+        return 1.0 - ek / th;
+    }
+    /* TimeAdvance (:424-486) with TimeStepAdjustment (ImplicitSolver.cpp:144-190) */
+    int advance(int &iter, int &lin, double &resid) {
+        phi_old2 = phi_old;
+        phi_old = phi_new;
+        bool step_done = false;
+        bool ok = false;
+        double dt_request = dt_now;
+        long long lin_sum = 0;
+        do {
+            ok = nk_loop(iter, resid);
+            lin_sum += lin_total;
+            if (!ok && extrap_move > 0) {
+                extrap_blocked = true;
+                ok = nk_loop(iter, resid);
+                lin_sum += lin_total;
+                extrap_blocked = false;
+            }
+            if (eq.cs.step_control) {
+                const double crit = ok ? truncation_criterion() : -1.0;
+                if (crit > 0.0) {
+                    step_done = true;
+                    if (crit > 0.90 && dt_now < eq.cs.dt_max) {
+                        if (++tsc_count >= eq.cs.tsc_keep) {
+                            tsc_count = 0;
+                            dt_request = std::min(dt_now / eq.cs.tsc_change, eq.cs.dt_max);
+                        }
+                    }
+                } else {
+                    dt_now *= eq.cs.tsc_change;
+                    dt_request = dt_now;
+                    if (dt_now < eq.cs.dt_min) { lin = (int)lin_sum; return 1; }
+                }
+            } else {
+                if (!ok) { lin = (int)lin_sum; return 1; }
+                step_done = true;
+            }
+        } while (!step_done);
+        phi_new = phi_guess;
+        eq.h = phi_new;
+        time += dt_now;
+        dt_old = dt_now;
+        dt_now = dt_request;
+        first_step = false;
+        lin = (int)lin_sum;
+        return 0;
+    }
+};
+
+static thread_local std::string g_err;
+
+}  // namespace yzo
+
+/* ------------------------------------------------------------------ */
+/* C API of the oracle (ctypes)                                         */
+/* ------------------------------------------------------------------ */
+extern "C" {
+
+void yzo_case_defaults(yz_case *c) {
+    std::memset(c, 0, sizeof(*c));
+    c->dim = 2; c->nx = c->ny = 16;
+    c->xmin = 0; c->xmax = 1; c->ymin = 0; c->ymax = 1;
+    c->capillary = 1;
+    c->dis_n = 3; c->dis_m = 2; c->hstar = 0.01;
+    c->visc_c0 = 0.5; c->sig_c0 = 0.5;
+    c->As = 1;
+    c->theta = 0.5;
+    c->nk_max = 30; c->nk_min = 1; c->jac_interval = 1;
+    c->nk_eps = 1e-10; c->nk_eps_rel = 1e-6;
+    c->ew = 1; c->extrap = 1;
+    c->lin_maxit = 5000; c->lin_eps = 1e-13; c->row_scaling = 1; c->precond = YZ_PC_AMG;
+    c->tsc_change = 0.5; c->tsc_criterion = 0.01; c->tsc_floor = 0; c->tsc_keep = 50;
+    c->dt_min = 1e-15; c->dt_max = 1e30;
+}
+
+void *yzo_create(const yz_case *c, const double *h, const double *s, const double *conc, double dt0) {
+    auto *st = new yzo::Stepper();
+    st->init(*c, h, s, conc, dt0);
+    return st;
+}
+void yzo_destroy(void *p) { delete static_cast<yzo::Stepper *>(p); }
+int yzo_n(void *p) { return static_cast<yzo::Stepper *>(p)->eq.g.n; }
+int yzo_advance(void *p, int *nk, int *lin, double *resid) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    int it = 0, li = 0; double r = 0;
+    const int rc = st->advance(it, li, r);
+    if (nk) *nk = it;
+    if (lin) *lin = li;
+    if (resid) *resid = r;
+    return rc;
+}
+void yzo_get_h(void *p, double *h) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    std::copy(st->phi_new.begin(), st->phi_new.end(), h);
+}
+void yzo_set_h(void *p, const double *h) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    st->phi_new.assign(h, h + st->eq.g.n);
+    st->eq.fill_ghosts(st->phi_new);
+}
+void yzo_clock(void *p, double *t, double *dt_now, double *dt_old) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    if (t) *t = st->time;
+    if (dt_now) *dt_now = st->dt_now;
+    if (dt_old) *dt_old = st->dt_old;
+}
+int yzo_nnz(void *p) { return (int)static_cast<yzo::Stepper *>(p)->eq.A.col.size(); }
+/* CRS structure (rowptr n+1, col nnz) and the slot position table */
+void yzo_pattern(void *p, int *rowptr, int *col, char *bc) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    const yzo::Csr &A = st->eq.A;
+    std::copy(A.ptr.begin(), A.ptr.end(), rowptr);
+    std::copy(A.col.begin(), A.col.end(), col);
+    if (bc) std::copy(A.bc.begin(), A.bc.end(), bc);
+}
+/* Assemble at film h with the explicit half at h_old: A, J, IA values
+ * (nnz each), ApplyA(h) (n), Rhs (n). Any pointer may be NULL. */
+void yzo_assemble(void *p, const double *h_old, const double *h, double dt,
+                  double *A, double *J, double *IA, double *ax, double *rhs) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    const int n = st->eq.g.n;
+    std::vector<double> ho(h_old, h_old + n), hg(h, h + n);
+    st->eq.fill_ghosts(ho); st->eq.fill_ghosts(hg);
+    const double save_dt = st->dt_now;
+    st->dt_now = dt;
+    st->phi_old = ho;
+    st->eq.h = ho; st->eq.set_matrix_A(); st->eq.apply_A(ho, st->Ax_old);
+    st->phi_guess = hg;
+    st->eq.h = hg; st->eq.set_matrix_A(); st->eq.set_matrix_J(); st->make_IA();
+    st->eq.apply_A(hg, st->Ax_guess);
+    st->make_rhs();
+    const int nnz = (int)st->eq.A.val.size();
+    if (A) std::copy(st->eq.A.val.begin(), st->eq.A.val.begin() + nnz, A);
+    if (J) std::copy(st->eq.J.val.begin(), st->eq.J.val.begin() + nnz, J);
+    if (IA) std::copy(st->eq.IA.val.begin(), st->eq.IA.val.begin() + nnz, IA);
+    if (ax) std::copy(st->Ax_guess.begin(), st->Ax_guess.end(), ax);
+    if (rhs) std::copy(st->Rhs.begin(), st->Rhs.end(), rhs);
+    st->dt_now = save_dt;
+}
+/* the backend contract on a caller's CRS */
+int yzo_linsolve(int n, int nnz, const int *rowptr, const int *col, const double *val,
+                 const double *rhs, double *x, int precond, int scaling, int maxit, double eps,
+                 int *iter, double *resid) {
+    yzo::Csr M;
+    M.n = n; M.ptr.assign(rowptr, rowptr + n + 1); M.col.assign(col, col + nnz);
+    M.val.assign(val, val + nnz); M.bc.assign(n, 0);
+    /* the diagonal first, as CRS_Matrix stores it */
+    for (int i = 0; i < n; ++i) {
+        for (int k = M.ptr[i]; k < M.ptr[i + 1]; ++k)
+            if (M.col[k] == i && k != M.ptr[i]) { std::swap(M.col[k], M.col[M.ptr[i]]); std::swap(M.val[k], M.val[M.ptr[i]]); }
+    }
+    std::vector<double> b(rhs, rhs + n), xx(x, x + n);
+    int it = 0; double r = 0;
+    const int rc = yzo::linsolve(M, b, xx, precond, scaling, maxit, eps, it, r);
+    std::copy(xx.begin(), xx.end(), x);
+    if (iter) *iter = it;
+    if (resid) *resid = r;
+    return rc;
+}
+
+}  /* extern "C" */

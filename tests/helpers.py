@@ -1,0 +1,79 @@
+"""Shared builders for the tests: smooth non-flat fields on small grids (the
+shapes of the reference's test_jacobian.cpp: 1-D 28 cells, 2-D 18 x 14,
+film, substrate and concentration all varying), and the map from the
+oracle's CRS (reference layout, ghost rows kept) to stencil slots."""
+import numpy as np
+
+from yuzen_amd import capi, cases
+
+TERMS = ["capillary", "vertical_gravity", "inplane_gravity", "centrifugal", "marangoni", "disjoining"]
+
+
+def smooth_case(dim, terms, slip=0.0, periodic=False, nx=None, ny=None):
+    c = capi.case_defaults()
+    c.dim = dim
+    c.nx = nx or (18 if dim == 2 else 28)
+    c.ny = (ny or 14) if dim == 2 else 1
+    c.xmin, c.xmax, c.ymin, c.ymax = -0.5, 0.5, -0.4, 0.4
+    for k in range(4):
+        c.bc[k] = capi.BC_PERIODIC if periodic else capi.BC_SYMMETRY
+    c.capillary = c.vertical_gravity = c.inplane_gravity = 0
+    c.centrifugal = c.marangoni = c.disjoining = 0
+    for t in ([terms] if isinstance(terms, str) else terms):
+        setattr(c, t, 1)
+    c.Bo, c.Pgx, c.Pgy, c.Sp = 0.7, 0.4, -0.2, 1.3
+    c.As, c.Ma, c.Dp, c.hstar = 1.0, 0.3, 2.0, 0.1
+    c.rot_x, c.rot_y = 0.13, -0.07
+    c.visc_exp = 3.0
+    c.slip = slip
+    X, Y = cases.cell_centres(c)
+    Xh, Yh = 0.5, 0.4
+    if periodic:
+        h = 0.5 + 0.3 * np.sin(2 * np.pi * X) * (1.1 + 0.4 * np.cos(2 * np.pi * Y / 0.8))
+        s = 0.1 * np.cos(2 * np.pi * X) * np.sin(2 * np.pi * Y / 0.8)
+        cc = 0.5 + 0.15 * np.sin(2 * np.pi * X) + 0.1 * np.cos(2 * np.pi * Y / 0.8)
+    else:
+        h = 0.2 + 0.6 * (1 + np.sin(np.pi * X / Xh + 0.3)) * (1.1 + 0.4 * np.cos(np.pi * Y / Yh - 0.2))
+        s = 0.3 * np.cos(0.7 * np.pi * X / Xh) * np.sin(0.5 * np.pi * Y / Yh + 0.4)
+        cc = 0.5 + 0.15 * np.sin(0.8 * np.pi * X / Xh - 0.5) + 0.1 * np.cos(0.6 * np.pi * Y / Yh)
+    return c, h.ravel(), s.ravel(), cc.ravel()
+
+
+def slot_offsets(dim):
+    if dim == 2:
+        ox = [0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0]
+        oy = [-2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2]
+    else:
+        ox = [-2, -1, 0, 1, 2]
+        oy = [0] * 5
+    return ox, oy
+
+
+def csr_to_planes(case, rowptr, col, val):
+    """Interior rows of the oracle's CRS as stencil planes [ns, N]: the
+    entry of an unfolded neighbour (ghost or interior) goes to its slot."""
+    dim = case.dim
+    nx, ny = case.nx, (case.ny if dim == 2 else 1)
+    nxp = nx + 4
+    ox, oy = slot_offsets(dim)
+    ns = len(ox)
+    slot_of = {(ox[k], oy[k]): k for k in range(ns)}
+    out = np.zeros((ns, nx * ny))
+    for iy in range(ny):
+        for ix in range(nx):
+            i = iy * nx + ix
+            r = ((iy + 2) if dim == 2 else 0) * nxp + ix + 2
+            for k in range(rowptr[r], rowptr[r + 1]):
+                cc = col[k]
+                dxo = cc % nxp - (ix + 2)
+                dyo = cc // nxp - ((iy + 2) if dim == 2 else 0)
+                out[slot_of[(dxo, dyo)], i] += val[k]
+    return out
+
+
+def interior(case, v):
+    dim = case.dim
+    nx, ny = case.nx, (case.ny if dim == 2 else 1)
+    nxp = nx + 4
+    a = np.asarray(v).reshape((ny + 4) if dim == 2 else 1, nxp)
+    return a[2:-2, 2:-2].ravel() if dim == 2 else a[0, 2:-2].copy()

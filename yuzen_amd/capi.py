@@ -1,0 +1,124 @@
+"""ctypes mirror of include/yuzen_gpu.h and the loader of libyuzen_hip.so.
+
+The structure layout below must match ``struct yz_case`` and
+``struct yz_lin_settings`` field for field; ``tests/test_capi.py`` checks the
+sizes against the compiled library.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libyuzen_hip.so")
+
+BC_SYMMETRY, BC_PERIODIC, BC_PINNING, BC_FRESHINLET, BC_OUTLET = range(5)
+PC_JACOBI, PC_AMG, PC_LEARNED = range(3)
+
+
+class YzCase(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
+        ("xmin", C.c_double), ("xmax", C.c_double),
+        ("ymin", C.c_double), ("ymax", C.c_double),
+        ("bc", C.c_int * 4),
+        ("capillary", C.c_int), ("vertical_gravity", C.c_int),
+        ("inplane_gravity", C.c_int), ("centrifugal", C.c_int),
+        ("marangoni", C.c_int), ("disjoining", C.c_int),
+        ("Bo", C.c_double), ("Pgx", C.c_double), ("Pgy", C.c_double),
+        ("Sp", C.c_double), ("As", C.c_double), ("Ma", C.c_double),
+        ("Dp", C.c_double),
+        ("hstar", C.c_double), ("dis_n", C.c_double), ("dis_m", C.c_double),
+        ("slip", C.c_double), ("rot_x", C.c_double), ("rot_y", C.c_double),
+        ("visc_c0", C.c_double), ("visc_exp", C.c_double),
+        ("sig_c0", C.c_double),
+        ("theta", C.c_double),
+        ("nk_max", C.c_int), ("nk_min", C.c_int), ("jac_interval", C.c_int),
+        ("nk_eps", C.c_double), ("nk_eps_rel", C.c_double),
+        ("ew", C.c_int), ("extrap", C.c_int),
+        ("lin_maxit", C.c_int), ("lin_eps", C.c_double),
+        ("row_scaling", C.c_int), ("precond", C.c_int),
+        ("krylov_fp32", C.c_int),
+        ("step_control", C.c_int),
+        ("tsc_change", C.c_double), ("tsc_criterion", C.c_double),
+        ("tsc_floor", C.c_double), ("tsc_keep", C.c_int),
+        ("dt_min", C.c_double), ("dt_max", C.c_double),
+    ]
+
+    def copy(self):
+        c = YzCase()
+        C.pointer(c)[0] = self
+        return c
+
+
+class YzLinSettings(C.Structure):
+    _fields_ = [
+        ("precond", C.c_int), ("scaling", C.c_int), ("max_iter", C.c_int),
+        ("epsilon", C.c_double),
+        ("grid_dim", C.c_int), ("grid_nx", C.c_int), ("grid_ny", C.c_int),
+        ("bc", C.c_int * 4),
+    ]
+
+
+def case_defaults():
+    """The reference's defaults (Input.cpp getin defaults), as
+    yz_case_defaults sets them."""
+    c = YzCase()
+    c.dim, c.nx, c.ny = 2, 16, 16
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, 1.0, 0.0, 1.0
+    c.capillary = 1
+    c.dis_n, c.dis_m, c.hstar = 3.0, 2.0, 0.01
+    c.visc_c0, c.sig_c0 = 0.5, 0.5
+    c.As = 1.0
+    c.theta = 0.5
+    c.nk_max, c.nk_min, c.jac_interval = 30, 1, 1
+    c.nk_eps, c.nk_eps_rel = 1e-10, 1e-6
+    c.ew, c.extrap = 1, 1
+    c.lin_maxit, c.lin_eps, c.row_scaling, c.precond = 5000, 1e-13, 1, PC_AMG
+    c.tsc_change, c.tsc_criterion, c.tsc_floor, c.tsc_keep = 0.5, 0.01, 0.0, 50
+    c.dt_min, c.dt_max = 1e-15, 1e30
+    return c
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library. There is no fallback: a missing library is an
+    error, so nothing can silently run anywhere but on the device."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libyuzen_hip.so is not built (run __graft_entry__.build() or "
+            "python -m yuzen_amd.build)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    d, i = C.c_double, C.c_int
+    pd, pi_ = P(C.c_double), P(C.c_int)
+    lib.yz_last_error.restype = C.c_char_p
+    lib.yz_case_defaults.argtypes = [P(YzCase)]
+    lib.yz_step_create.restype = C.c_void_p
+    lib.yz_step_create.argtypes = [P(YzCase), pd, pd, pd, d]
+    lib.yz_step_advance.argtypes = [C.c_void_p, pi_, pi_, pd]
+    lib.yz_step_get_h.argtypes = [C.c_void_p, pd]
+    lib.yz_step_set_h.argtypes = [C.c_void_p, pd]
+    lib.yz_step_clock.argtypes = [C.c_void_p, pd, pd, pd]
+    lib.yz_step_timing.argtypes = [C.c_void_p, pd]
+    lib.yz_step_destroy.argtypes = [C.c_void_p]
+    lib.yz_step_assemble.argtypes = [C.c_void_p, pd, pd, d, pd, pd, pd]
+    lib.yz_linsolve_create.restype = C.c_void_p
+    lib.yz_linsolve_destroy.argtypes = [C.c_void_p]
+    lib.yz_linsolve.argtypes = [C.c_void_p, i, i, pi_, pi_, pd, pd, pd,
+                                P(YzLinSettings), pi_, pd]
+    lib.yz_memory_get.argtypes = [C.c_void_p, pd, i]
+    lib.yz_memory_set.argtypes = [C.c_void_p, pd, i]
+    lib.yz_sizeof_case.restype = C.c_int
+    lib.yz_sizeof_lin_settings.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def last_error():
+    lib = load()
+    e = lib.yz_last_error()
+    return e.decode() if e else ""

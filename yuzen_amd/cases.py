@@ -1,0 +1,159 @@
+"""The benchmark and parity cases as concrete, seeded inputs.
+
+Each builder returns ``(case, h, s, c, dt0, steps)``: a filled
+:class:`yuzen_amd.capi.YzCase`, the padded film, substrate and
+concentration arrays (n = (nx+4)(ny+4) rows, ghosts left 0: the step forms
+them from the boundary conditions), the first step and the step count.
+
+Noise comes from a counter-based generator keyed by (seed, ix, iy)
+(splitmix64 of the key), so every consumer -- the device path and the CPU
+oracle -- sees the same initial film bit for bit, whatever the order it is
+generated in.
+"""
+import math
+
+import numpy as np
+
+from .capi import (YzCase, case_defaults, BC_SYMMETRY, BC_PERIODIC,
+                   PC_AMG)
+
+_M64 = (1 << 64) - 1
+
+
+def _splitmix64(z):
+    z = (z + 0x9E3779B97F4A7C15) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def uniform_pm1(seed, ix, iy):
+    """U(-1, 1) for integer arrays ix, iy (interior cell numbers from 0),
+    keyed by (seed, ix, iy)."""
+    ix = np.asarray(ix, dtype=np.uint64)
+    iy = np.asarray(iy, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) * np.uint64(0x100000001B3)) ^ (iy << np.uint64(32)) ^ ix
+        z = key + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return 2.0 * u - 1.0
+
+
+def padded_shape(case):
+    nxp = case.nx + 4
+    nyp = case.ny + 4 if case.dim == 2 else 1
+    return nyp, nxp
+
+
+def cell_centres(case):
+    """Cell-centre coordinates over the padded grid, as the grid builds
+    them (two ghosts, spacing over the padded extent)."""
+    nyp, nxp = padded_shape(case)
+    dx0 = (case.xmax - case.xmin) / case.nx
+    xlo = case.xmin - 2 * dx0
+    dx = ((case.xmax + 2 * dx0) - xlo) / nxp
+    x = xlo + np.arange(nxp) * dx + dx * 0.5
+    if case.dim == 2:
+        dy0 = (case.ymax - case.ymin) / case.ny
+        ylo = case.ymin - 2 * dy0
+        dy = ((case.ymax + 2 * dy0) - ylo) / nyp
+        y = ylo + np.arange(nyp) * dy + dy * 0.5
+    else:
+        y = np.zeros(1)
+    return np.meshgrid(x, y)
+
+
+def interior_index(case):
+    nyp, nxp = padded_shape(case)
+    ix = np.arange(nxp) - 2
+    iy = (np.arange(nyp) - 2) if case.dim == 2 else np.zeros(1, dtype=int)
+    IX, IY = np.meshgrid(ix, iy)
+    return IX, IY
+
+
+def disjoining_dp(theta_deg, hstar, n=3.0, m=2.0, As=1.0):
+    """Dp = As (n-1)(m-1) / (2 H* (n-m)) theta_e^2
+    (reference NonDimensionalNumbers.cpp:103-118)."""
+    th = theta_deg * math.pi / 180.0
+    return As * (n - 1.0) * (m - 1.0) / (2.0 * hstar * (n - m)) * th * th
+
+
+def leveling_1d(nx=512, steps=200):
+    """1-D capillary leveling over a substrate step: N cells on x in
+    [0, 10], s = 0.5 (1 + tanh((x - 5)/0.2)), film h = 1 on it, capillary
+    pressure only, Symmetry (no-flux) ends, dt = 1e-3 fixed, Newton
+    tolerance 1e-10, BiCGSTAB + AMG."""
+    c = case_defaults()
+    c.dim, c.nx, c.ny = 1, nx, 1
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, 10.0, -1.0, 1.0
+    for k in range(4):
+        c.bc[k] = BC_SYMMETRY
+    c.capillary = 1
+    c.nk_eps = 1e-10
+    c.precond = PC_AMG
+    X, _ = cell_centres(c)
+    s = 0.5 * (1.0 + np.tanh((X - 5.0) / 0.2))
+    h = np.ones_like(X)
+    conc = np.full_like(X, 0.5)
+    return c, h.ravel(), s.ravel(), conc.ravel(), 1e-3, steps
+
+
+def spinodal_2d(n=224, steps=2000, seed=2024):
+    """2-D spinodal dewetting: n x n periodic cells, side 0.64, h = 1 +
+    0.01 U(-1, 1) per cell, capillary + disjoining Pi = (h*/h)^3 -
+    (h*/h)^2 with h* = 0.01 and theta_e = 10 deg, dt adaptive from 1e-5."""
+    c = case_defaults()
+    c.dim, c.nx, c.ny = 2, n, n
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, 0.64, 0.0, 0.64
+    for k in range(4):
+        c.bc[k] = BC_PERIODIC
+    c.capillary = 1
+    c.disjoining = 1
+    c.hstar, c.dis_n, c.dis_m = 0.01, 3.0, 2.0
+    c.Dp = disjoining_dp(10.0, 0.01)
+    c.step_control = 1
+    c.tsc_change, c.tsc_criterion, c.tsc_keep = 0.5, 0.01, 50
+    c.dt_min, c.dt_max = 1e-15, 1e-5 * 1e4
+    IX, IY = interior_index(c)
+    h = 1.0 + 0.01 * uniform_pm1(seed, np.clip(IX, 0, n - 1), np.clip(IY, 0, n - 1))
+    s = np.zeros_like(h)
+    conc = np.full_like(h, 0.5)
+    return c, h.ravel(), s.ravel(), conc.ravel(), 1e-5, steps
+
+
+def allphys_2d(n=1024, steps=20, seed=7, L=10.0, dt=1e-4, nk_eps=1e-8):
+    """2-D all-physics film, the throughput case: n x n cells on a square
+    of side L = 10 with Symmetry walls; capillary, disjoining (h* = 0.01,
+    theta_e = 15 deg), vertical gravity Bo = 0.5, in-plane gravity
+    Pgx = 0.1, Pgy = 0.05, spin Sp = 0.2 about the centre of the square,
+    Marangoni As Ma = 0.1 (As = 1, Ma = 0.1) on a frozen concentration
+    c = 0.5 + 0.1 sin(2 pi x / L) cos(2 pi y / L), viscosity
+    mu = exp(2 (c - 0.5)), slip Lambda = 0.05; film
+    h = 0.55 + 0.45 tanh((x cos30 + y sin30 - L/2)/0.05) + 1e-3 U(-1,1),
+    flat substrate; fixed dt = 1e-4, Newton tolerance 1e-8."""
+    c = case_defaults()
+    c.dim, c.nx, c.ny = 2, n, n
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, L, 0.0, L
+    for k in range(4):
+        c.bc[k] = BC_SYMMETRY
+    c.capillary = c.disjoining = c.vertical_gravity = 1
+    c.inplane_gravity = c.centrifugal = c.marangoni = 1
+    c.hstar, c.dis_n, c.dis_m = 0.01, 3.0, 2.0
+    c.Dp = disjoining_dp(15.0, 0.01)
+    c.Bo, c.Pgx, c.Pgy, c.Sp = 0.5, 0.1, 0.05, 0.2
+    c.As, c.Ma = 1.0, 0.1
+    c.slip = 0.05
+    c.rot_x, c.rot_y = 0.5 * L, 0.5 * L
+    c.visc_c0, c.visc_exp, c.sig_c0 = 0.5, 2.0, 0.5
+    c.nk_eps = nk_eps
+    X, Y = cell_centres(c)
+    IX, IY = interior_index(c)
+    a = math.radians(30.0)
+    h = 0.55 + 0.45 * np.tanh((X * math.cos(a) + Y * math.sin(a) - 0.5 * L) / 0.05)
+    h = h + 1e-3 * uniform_pm1(seed, np.clip(IX, 0, n - 1), np.clip(IY, 0, n - 1))
+    s = np.zeros_like(h)
+    conc = 0.5 + 0.1 * np.sin(2 * math.pi * X / L) * np.cos(2 * math.pi * Y / L)
+    return c, h.ravel(), s.ravel(), conc.ravel(), dt, steps

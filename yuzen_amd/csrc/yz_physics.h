@@ -1,0 +1,168 @@
+// yz_physics.h -- per-cell physics of the well-mixed film-height equation,
+// device side (gfx950). Everything a cell's row needs is computed here
+// from the 13-point (2-D) / 5-point (1-D) stencil of the film h, the
+// substrate s and the frozen concentration c, held in registers:
+//
+//   * the row of A(h) (all six terms) and of the Jacobian J = d(A(h)h)/dh,
+//   * A(h)h in the form the reference's ApplyA forms it: the capillary and
+//     vertical-gravity terms face by face (differences of differences), the
+//     upwinded and disjoining terms as the product of their entries with h.
+//
+// Formulas follow the reference kernels (paths relative to the reference
+// root, yuzen/src/model/well_mixed/):
+//   capillary          LaplacePressureKernel.h:242-579,
+//                      WM2D/LaplacePressure_2D.cpp:39-128, WM1D/...:35-77
+//   vertical gravity   VerticalGravityKernel.h:150-433
+//   disjoining         DisjoiningPressureKernel.h:150-435
+//   body forces        BodyForceKernel.h (LimitedFace, *_Limited),
+//                      ParallelGravityKernel.h, CentrifugalKernel.h
+//   Marangoni          SolutoCapillarityKernel.h (limited forms)
+//   mobility           ../Mobility.h:49-191
+//   limiter            ../../numerics/FiniteDifferences.cpp:94-228
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+#include "../../include/yuzen_gpu.h"
+
+namespace yz {
+
+// Stencil slots. 2-D: oM mm om pm Mo mo oo po Po mp op pp oP (the
+// reference's FullIdx order); 1-D: M m o p P.
+enum { S_oM, S_mm, S_om, S_pm, S_Mo, S_mo, S_oo, S_po, S_Po, S_mp, S_op, S_pp, S_oP };
+
+struct Geo {
+    int dim, nx, ny;        // interior cells
+    int nxp, nyp;           // padded extents (nyp = 1 in 1-D)
+    int N;                  // interior unknowns nx*ny
+    int ns, center, ndir;   // stencil size, centre slot, faces
+    double dx, dy, xlo, ylo;
+    int bc[4];
+};
+
+__host__ __device__ inline int slot_ox(int dim, int k) {
+    if (dim == 1) return k - 2;
+    const int t[13] = {0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0};
+    return t[k];
+}
+__host__ __device__ inline int slot_oy(int dim, int k) {
+    if (dim == 1) return 0;
+    const int t[13] = {-2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2};
+    return t[k];
+}
+// AdjIdx(d): the neighbour across face d (Xm, Xp, Ym, Yp)
+__host__ __device__ inline int adj_slot(int dim, int d) {
+    if (dim == 1) return d == 0 ? 1 : 3;
+    const int t[4] = {S_mo, S_po, S_om, S_op};
+    return t[d];
+}
+// FaceStencil(d, j): LL L R RR of face d
+__host__ __device__ inline int face_slot(int dim, int d, int j) {
+    if (dim == 1) return d == 0 ? j : j + 1;
+    const int t[4][4] = {{S_Mo, S_mo, S_oo, S_po}, {S_mo, S_oo, S_po, S_Po},
+                         {S_oM, S_om, S_oo, S_op}, {S_om, S_oo, S_op, S_oP}};
+    return t[d][j];
+}
+// AxisStencil(a, q)
+__host__ __device__ inline int axis_slot(int dim, int a, int q) {
+    if (dim == 1) return q;
+    const int t[2][5] = {{S_Mo, S_mo, S_oo, S_po, S_Po}, {S_oM, S_om, S_oo, S_op, S_oP}};
+    return t[a][q];
+}
+// StencilDir(k): Ym Xm Xp Yp in 2-D
+__host__ __device__ inline int sdir(int ndir, int k) {
+    if (ndir == 2) return k;
+    const int t[4] = {2, 0, 1, 3};
+    return t[k];
+}
+
+__device__ inline double sgn(double v) { return v >= 0.0 ? 1.0 : -1.0; }
+
+// ---- mobility (Mobility.h) -------------------------------------------
+struct Mob {
+    double lambda, a;
+    bool slip;
+    __device__ double D(double h) const { const double u = fabs(h); return h * u * (u + a); }
+    __device__ double dD(double h) const { const double u = fabs(h); return u * (3.0 * u + 2.0 * a); }
+    __device__ double DperH(double h) const { const double u = fabs(h); return u * (u + a); }
+    __device__ double Gint(double h) const {
+        const double x = a / h;
+        double f;
+        if (x < 0.1) {
+            double t = x * x, s = 0;
+            for (int k = 2; k <= 18; ++k) { s += ((k % 2 == 0) ? t : -t) / k; t *= x; }
+            f = s;
+        } else f = x - log1p(x);
+        return -f / (a * a);
+    }
+    __device__ double K(double h) const { const double u = dD(h) / D(h); return 2.0 * u * u - (6.0 * h + 2.0 * a) / D(h); }
+    __device__ double dK(double h) const {
+        const double d = D(h), d1 = dD(h), d2 = 6.0 * h + 2.0 * a, d3 = 6.0;
+        const double u = d1 / d, du = d2 / d - u * u;
+        return 4.0 * u * du - (d3 * d - d2 * d1) / (d * d);
+    }
+    __device__ double FaceDK(double h1, double h2) const {
+        if (!(h1 > 0) || !(h2 > 0)) return 0;
+        const double hm = 0.5 * (h1 + h2), dl = h2 - h1;
+        if (fabs(dl) < 1e-3 * hm) return D(hm) / (1.0 + dl * dl * K(hm) / 24.0);
+        return dl / (Gint(h2) - Gint(h1));
+    }
+    __device__ double FaceDK_d1(double h1, double h2) const {
+        if (!(h1 > 0) || !(h2 > 0)) return 0;
+        const double hm = 0.5 * (h1 + h2), dl = h2 - h1;
+        if (fabs(dl) < 1e-3 * hm) {
+            const double k = K(hm), q = 1.0 + dl * dl * k / 24.0;
+            const double Fhm = (dD(hm) * q - D(hm) * dl * dl * dK(hm) / 24.0) / (q * q);
+            const double Fdl = -D(hm) * (dl * k / 12.0) / (q * q);
+            return 0.5 * Fhm - Fdl;
+        }
+        const double dg = Gint(h2) - Gint(h1);
+        return (-dg + dl / D(h1)) / (dg * dg);
+    }
+    // Mobility::Face / Face_d1 (0 when a side is dry)
+    __device__ double Face(double h1, double h2) const {
+        if (slip) return FaceDK(h1, h2);
+        if (!(h1 > 0) || !(h2 > 0)) return 0;
+        return 2.0 * (h1 * h1) * (h2 * h2) / (h1 + h2);
+    }
+    __device__ double Face_d1(double h1, double h2) const {
+        if (slip) return FaceDK_d1(h1, h2);
+        if (!(h1 > 0) || !(h2 > 0)) return 0;
+        return 2.0 * (h1 * h2 * h2) * (h1 + h2 + h2) / ((h1 + h2) * (h1 + h2));
+    }
+};
+
+// ---- Koren limiter (FiniteDifferences.cpp Limiter/FaceValue/...) -------
+__device__ inline double koren(double a, double b, double *pa, double *pb) {
+    double p = 0, qa = 0, qb = 0;
+    if ((a > 0 && b > 0) || (a < 0 && b < 0)) {
+        const double s = (b > 0) ? 1.0 : -1.0;
+        const double aa = fabs(a), ab = fabs(b);
+        const double t1 = 2.0 * aa, t2 = (2.0 * ab + aa) / 3.0, t3 = 2.0 * ab;
+        if (t1 <= t2 && t1 <= t3) { p = s * t1; qa = 2.0; qb = 0.0; }
+        else if (t2 <= t3) { p = s * t2; qa = 1.0 / 3.0; qb = 2.0 / 3.0; }
+        else { p = s * t3; qa = 0.0; qb = 2.0; }
+    }
+    *pa = qa; *pb = qb;
+    return p;
+}
+// The limited face value hf, its downwind-form weights w (sum w f = hf)
+// and its derivative g = d hf / d f.
+__device__ inline double limited_face(const double f[4], double sign, double w[4], double g[4]) {
+    int uu, u, d;
+    if (sign >= 0) { uu = 0; u = 1; d = 2; } else { uu = 3; u = 2; d = 1; }
+    const double a = f[u] - f[uu], b = f[d] - f[u];
+    double pa, pb;
+    const double p = koren(a, b, &pa, &pb);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { w[j] = 0; g[j] = 0; }
+    const double psi = (b != 0) ? p / b : 0.0;
+    w[u] = 1.0 - 0.5 * psi;
+    w[d] = 0.5 * psi;
+    g[uu] = -0.5 * pa;
+    g[u] = 1.0 + 0.5 * (pa - pb);
+    g[d] = 0.5 * pb;
+    return f[u] + 0.5 * p;
+}
+
+}  // namespace yz

@@ -1,0 +1,400 @@
+// yz_rows.h -- the fused row kernel of a Newton pass.
+//
+// One thread per interior cell. From the stencil of the film h, the
+// substrate s and the frozen concentration c it forms, in registers,
+//   * A(h)h as the reference's ApplyA forms it (HeightEquationBase.h:
+//     803-827): the capillary and vertical-gravity terms face by face
+//     (LaplacePressureKernel.h:439-477, VerticalGravityKernel.h:233-269),
+//     the other terms as the product of their entries of A with h;
+//   * the row of the Jacobian J = A + sum of the terms' SetJacobiMatrix
+//     (HeightEquationBase.h:1061-1091) and of IA = I + theta dt J
+//     (NewtonKantorovich.cpp:488-510);
+//   * the Newton right-hand side Rhs = h_old - h - dt (theta A(h)h +
+//     (1-theta) A(h_old)h_old) (NewtonKantorovich.cpp:512-523);
+// and writes the row-scaled IA planes and right-hand side
+// (LinearSolverBackend.cpp:227-252, ScaleRows: divided by the diagonal).
+// A and J are never stored: the pass reads h, s, c once and writes the
+// 13 planes of IA and one vector.
+#pragma once
+#include "yz_physics.h"
+
+namespace yz {
+
+struct RowArgs {
+    Geo g;
+    int cap, vg, pg, cf, ma, dj;
+    double Bo, Pgx, Pgy, Sp, AsMa2, Dp, hstar, pn, pm, rotx, roty;
+    double visc_c0, visc_exp, sig_c0, Ma_eff;
+    Mob mob;
+    double Ccap[4][13];
+    double theta, dt;
+    int row_scaling;
+    const double *h, *s, *c;   // padded
+    const double *hold;        // padded, h at the start of the step
+    const double *axold;       // N: A(h_old) h_old
+    double *ax;                // mode 0: A(h)h (N)
+    double *ia;                // mode 1: ns planes x N, row-scaled
+    double *b;                 // mode 1: row-scaled Rhs (N)
+    double *ia_raw;            // optional: unscaled IA planes
+    double *rhs_raw;           // optional: unscaled Rhs (N)
+    double *ax_raw;            // optional: A(h)h (N) in mode 1
+    unsigned long long *rmax;  // mode 1: max |Rhs| (bits of a non-negative double)
+};
+
+__device__ inline double visc_of(const RowArgs &a, double cc) { return exp(a.visc_exp * (cc - a.visc_c0)); }
+__device__ inline double sig_of(const RowArgs &a, double cc) { return 1.0 + a.Ma_eff * (cc - a.sig_c0); }
+
+// the third difference of v across face d (LaplacePressure_2D.cpp:105-128)
+template <int DIM>
+__device__ inline double face_grad(const RowArgs &A, const double *v, int d) {
+    if (DIM == 2) {
+        const double dx2 = A.g.dx * A.g.dx, dy2 = A.g.dy * A.g.dy;
+        const double dn4 = (d < 2) ? dx2 * dx2 : dy2 * dy2, dxy = dx2 * dy2;
+        double a1, a, b, b1, am, ap, bm, bp;
+        switch (d) {
+        case 0: a1 = v[S_Mo]; a = v[S_mo]; b = v[S_oo]; b1 = v[S_po]; am = v[S_mm]; ap = v[S_mp]; bm = v[S_om]; bp = v[S_op]; break;
+        case 1: a1 = v[S_mo]; a = v[S_oo]; b = v[S_po]; b1 = v[S_Po]; am = v[S_om]; ap = v[S_op]; bm = v[S_pm]; bp = v[S_pp]; break;
+        case 2: a1 = v[S_oM]; a = v[S_om]; b = v[S_oo]; b1 = v[S_op]; am = v[S_mm]; ap = v[S_pm]; bm = v[S_mo]; bp = v[S_po]; break;
+        default: a1 = v[S_om]; a = v[S_oo]; b = v[S_op]; b1 = v[S_oP]; am = v[S_mo]; ap = v[S_po]; bm = v[S_mp]; bp = v[S_pp]; break;
+        }
+        const double normal = (((b1 - b) - 2.0 * (b - a)) + (a - a1)) / dn4;
+        const double cross = (((bp - b) - (b - bm)) - ((ap - a) - (a - am))) / dxy;
+        return normal + cross;
+    }
+    const double dx2 = A.g.dx * A.g.dx, dx4 = dx2 * dx2;
+    double a1, a, b, b1;
+    if (d == 0) { a1 = v[0]; a = v[1]; b = v[2]; b1 = v[3]; }
+    else { a1 = v[1]; a = v[2]; b = v[3]; b1 = v[4]; }
+    return (((b1 - b) - 2.0 * (b - a)) + (a - a1)) / dx4;
+}
+
+__device__ inline bool is_wall(int t) { return t == YZ_BC_SYMMETRY || t == YZ_BC_PINNING; }
+
+// mode 0: A(h)h only (the explicit half); mode 1: the whole pass
+template <int DIM, int MODE>
+__global__ void __launch_bounds__(256) k_rows(RowArgs A) {
+    constexpr int NS = DIM == 2 ? 13 : 5;
+    constexpr int ND = DIM == 2 ? 4 : 2;
+    constexpr int C0 = DIM == 2 ? 6 : 2;
+    const Geo &g = A.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < g.N;
+    double rabs = 0;
+    if (active) {
+        const int ix = i % g.nx, iy = i / g.nx;
+        const int ixp = ix + 2, iyp = DIM == 2 ? iy + 2 : 0;
+        const int p = iyp * g.nxp + ixp;
+        double H[NS], S[NS], Cc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int q = p + slot_ox(DIM, k) + slot_oy(DIM, k) * g.nxp;
+            H[k] = A.h[q]; S[k] = A.s[q]; Cc[k] = A.c[q];
+        }
+        double Ar[NS], Jc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) { Ar[k] = 0; Jc[k] = 0; }
+        double ax = 0;
+        const Mob &mob = A.mob;
+        auto spacing = [&](int axis) { return axis == 0 ? g.dx : g.dy; };
+        auto wall = [&](int d) {
+            if (d == 0) return ixp == 2 && is_wall(g.bc[0]);
+            if (d == 1) return ixp == g.nxp - 3 && is_wall(g.bc[1]);
+            if (d == 2) return iyp == 2 && is_wall(g.bc[2]);
+            return iyp == g.nyp - 3 && is_wall(g.bc[3]);
+        };
+
+        // ---- capillary pressure (conservative; coefficient 1/3) ----
+        if (A.cap) {
+            const double Coef = 1.0 / 3.0;
+            const double Hoo = H[C0];
+            double D[ND], gA[ND], gC[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const int a = adj_slot(DIM, d);
+                const double Haj = H[a];
+                const double cf = 0.5 * (Cc[C0] + Cc[a]);
+                const double so = sig_of(A, cf) / visc_of(A, cf);
+                const double val = mob.slip ? mob.FaceDK(Hoo, Haj) : 2.0 * (Hoo * Hoo) * (Haj * Haj) / (Hoo + Haj);
+                D[d] = val * so;
+                if (MODE == 1) {
+                    const double den = (Hoo + Haj) * (Hoo + Haj);
+                    const double vc = mob.slip ? mob.FaceDK_d1(Hoo, Haj) : 2.0 * (Hoo * Haj * Haj) * (Hoo + Haj + Haj) / den;
+                    const double va = mob.slip ? mob.FaceDK_d1(Haj, Hoo) : 2.0 * (Haj * Hoo * Hoo) * (Haj + Hoo + Hoo) / den;
+                    gC[d] = vc * so;
+                    gA[d] = va * so;
+                }
+            }
+            // residual, face by face
+            double r = 0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const double f = Coef * (D[d] * (face_grad<DIM>(A, H, d) + face_grad<DIM>(A, S, d)));
+                r += (d % 2 == 0) ? -f : f;
+            }
+            ax += r;
+            if (MODE == 1) {
+                double topo = 0;
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    double cc = 0;
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) cc += A.Ccap[d][k] * (D[d] / Hoo);
+                    topo += cc * S[k];
+                }
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    double v = 0;
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) v += A.Ccap[d][k] * D[d];
+                    if (k == C0) v += topo;
+                    Jc[k] += Coef * v;
+                }
+                double av[ND], bv[ND];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    av[d] = bv[d] = 0;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) { av[d] += A.Ccap[d][k] * H[k]; bv[d] += A.Ccap[d][k] * S[k]; }
+                }
+                double ga = 0, sb = 0;
+#pragma unroll
+                for (int k = 0; k < ND; ++k) { const int dd = sdir(ND, k); ga += gC[dd] * av[dd]; }
+#pragma unroll
+                for (int d = 0; d < ND; ++d) sb += (gC[d] / Hoo - D[d] / (Hoo * Hoo)) * bv[d];
+                Jc[C0] += Coef * (ga + sb * Hoo);
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                    Jc[adj_slot(DIM, d)] += Coef * (gA[d] * av[d] + (gA[d] / Hoo) * bv[d] * Hoo);
+            }
+        }
+
+        // face mobility of the potential terms (vertical gravity, disjoining)
+        double Gf[ND], Tk[ND], gAf[ND], gCf[ND], inv2[ND];
+        if (A.vg || A.dj) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const int a = adj_slot(DIM, d);
+                const double ho = H[C0], ha = H[a];
+                const double mu = visc_of(A, 0.5 * (Cc[C0] + Cc[a]));
+                const double gf = mob.Face(ho, ha);
+                Gf[d] = gf / mu;
+                Tk[d] = (gf == 0) ? 0.0 : gf / (ho + ha) / mu;
+                const double hs = spacing(d / 2);
+                inv2[d] = 1.0 / (hs * hs);
+                if (MODE == 1) {
+                    gCf[d] = mob.Face_d1(ho, ha) / mu;
+                    gAf[d] = mob.Face_d1(ha, ho) / mu;
+                }
+            }
+        }
+
+        // ---- vertical gravity (conservative; -Bo/3) ----
+        if (A.vg) {
+            const double Coef = -1.0 * A.Bo / 3.0;
+            double r = 0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const int a = adj_slot(DIM, d);
+                const int lo = (d % 2 == 0) ? a : C0, hi = (d % 2 == 0) ? C0 : a;
+                const double dn = spacing(d / 2);
+                const double f = Coef * ((Gf[d] * (H[hi] - H[lo]) + (S[hi] - S[lo]) * (Tk[d] * (H[lo] + H[hi]))) / (dn * dn));
+                r += (d % 2 == 0) ? -f : f;
+            }
+            ax += r;
+            if (MODE == 1) {
+                double Tsum[ND];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) Tsum[d] = inv2[d] * S[adj_slot(DIM, d)] + (-inv2[d]) * S[C0];
+                double Too = 0, cen = 0;
+#pragma unroll
+                for (int k = 0; k < ND; ++k) { const int dd = sdir(ND, k); Too += Tsum[dd] * Tk[dd]; }
+#pragma unroll
+                for (int d = 0; d < ND; ++d) cen += (-inv2[d]) * Gf[d];
+                Jc[C0] += Coef * (cen + Too);
+#pragma unroll
+                for (int d = 0; d < ND; ++d) Jc[adj_slot(DIM, d)] += Coef * (inv2[d] * Gf[d] + Tsum[d] * Tk[d]);
+                double gs = 0;
+#pragma unroll
+                for (int k = 0; k < ND; ++k) {
+                    const int dd = sdir(ND, k);
+                    const double ad = inv2[dd] * H[adj_slot(DIM, dd)] + (-inv2[dd]) * H[C0];
+                    gs += gCf[dd] * (ad + Tsum[dd]) - Tk[dd] * Tsum[dd];
+                }
+                Jc[C0] += Coef * gs;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const double ad = inv2[d] * H[adj_slot(DIM, d)] + (-inv2[d]) * H[C0];
+                    Jc[adj_slot(DIM, d)] += Coef * (gAf[d] * (ad + Tsum[d]) - Tk[d] * Tsum[d]);
+                }
+            }
+        }
+
+        // ---- in-plane gravity (Pg/3) and centrifugal (Sp/3): limited upwind ----
+#pragma unroll
+        for (int mode = 0; mode < 2; ++mode) {
+            if (mode == 0 && !A.pg) continue;
+            if (mode == 1 && !A.cf) continue;
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                double dv;
+                if (mode == 0) dv = sgn(j < 2 ? A.Pgx : A.Pgy);
+                else if (j == 0) dv = (g.xlo + ixp * g.dx) - A.rotx;
+                else if (j == 1) dv = (g.xlo + (ixp + 1) * g.dx) - A.rotx;
+                else if (j == 2) dv = (g.ylo + iyp * g.dy) - A.roty;
+                else dv = (g.ylo + (iyp + 1) * g.dy) - A.roty;
+                const double w = (mode == 0) ? 1.0 : dv;
+                const double Coef = (mode == 0) ? ((j < 2 ? A.Pgx : A.Pgy) / 3.0) : A.Sp / 3.0;
+                const double hs = spacing(j / 2);
+                double f[4], wt[4], gd[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) f[q] = H[face_slot(DIM, j, q)];
+                const double hf = limited_face(f, dv, wt, gd);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { wt[q] /= hs; gd[q] /= hs; }
+                const double cc = 0.5 * (Cc[C0] + Cc[adj_slot(DIM, j)]);
+                double D = (mob.slip ? mob.DperH(hf) : hf * hf) / visc_of(A, cc);
+                if (wall(j)) D = 0;
+                const double sg = (j % 2 == 0) ? -1.0 : 1.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Ar[face_slot(DIM, j, q)] += Coef * (sg * w * wt[q] * D);
+                if (MODE == 1) {
+                    const double r3 = !mob.slip ? 3.0 : (mob.DperH(hf) != 0.0 ? mob.dD(hf) / mob.DperH(hf) : 0.0);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) Jc[face_slot(DIM, j, q)] += Coef * (sg * w * (r3 * gd[q] - wt[q]) * D);
+                }
+            }
+            if (mode == 1 && DIM == 1) {
+                // sideways outflow of the 1-D centrifugal term (BodyForceKernel.h SideOutflow)
+                const double hoo = H[C0], mu = visc_of(A, Cc[C0]), Mg = A.Sp / 3.0;
+                Ar[C0] += mob.slip ? Mg * (mob.DperH(hoo) / mu) : Mg * (hoo * hoo / mu);
+                if (MODE == 1)
+                    Jc[C0] += mob.slip ? Mg * ((mob.dD(hoo) - mob.DperH(hoo)) / mu) : Mg * (2.0 * hoo * hoo / mu);
+            }
+        }
+
+        // ---- Marangoni (As Ma / 2): limited upwind by the sign of the stress ----
+        if (A.ma) {
+            const double Coef = A.AsMa2;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const int lo = face_slot(DIM, d, 1), hi = face_slot(DIM, d, 2);
+                const double hh = spacing(d / 2);
+                const double dc = (Cc[hi] - Cc[lo]) / hh;
+                const double cc = (Cc[hi] + Cc[lo]) / 2.0;
+                const double up = sgn(Coef) * sgn(dc);
+                double f[4], wt[4], gd[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) f[q] = H[face_slot(DIM, d, q)];
+                const double hf = limited_face(f, up, wt, gd);
+                const bool wl = wall(d);
+                const double mu = visc_of(A, cc);
+                const double Sf = wl ? 0.0 : (mob.slip ? dc / mu * (hf + 2.0 * mob.lambda) : dc / mu * hf);
+                const double sg = (d % 2 == 0) ? -1.0 : 1.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Ar[face_slot(DIM, d, q)] += Coef * (sg * wt[q] * Sf / hh);
+                if (MODE == 1) {
+                    const double r = wl ? 0.0 : dc / mu;
+                    if (mob.slip) {
+                        const double L2 = 2.0 * mob.lambda, ad = sg / hh;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            Jc[face_slot(DIM, d, q)] += Coef * (r * ((2.0 * hf + L2) * gd[q] - (hf + L2) * wt[q]) * ad);
+                    } else {
+                        const double ad = sg * hf / hh;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) Jc[face_slot(DIM, d, q)] += Coef * (r * (2.0 * gd[q] - wt[q]) * ad);
+                    }
+                }
+            }
+        }
+
+        // ---- disjoining pressure (Dp/3), Pi = (h*/h)^n - (h*/h)^m ----
+        if (A.dj) {
+            const double Coef = A.Dp / 3.0;
+            auto Pi = [&](double hh) {
+                if (hh <= 0) return 0.0;
+                const double r = A.hstar / hh;
+                return pow(r, A.pn) - pow(r, A.pm);
+            };
+            auto dPi = [&](double hh) {
+                if (hh <= 0) return 0.0;
+                const double r = A.hstar / hh;
+                return -(A.pn * pow(r, A.pn) - A.pm * pow(r, A.pm)) / hh;
+            };
+            double PiS[NS];
+            PiS[C0] = Pi(H[C0]);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) PiS[adj_slot(DIM, d)] = Pi(H[adj_slot(DIM, d)]);
+            auto PiPerH = [&](int k) { return H[k] <= 0 ? 0.0 : PiS[k] / H[k]; };
+            double cen = 0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) cen += (-inv2[d]) * Gf[d];
+            Ar[C0] += Coef * (cen * PiPerH(C0));
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const int a = adj_slot(DIM, d);
+                Ar[a] += Coef * ((inv2[d] * Gf[d]) * PiPerH(a));
+            }
+            if (MODE == 1) {
+                double Ad[ND];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) Ad[d] = inv2[d] * PiS[adj_slot(DIM, d)] + (-inv2[d]) * PiS[C0];
+                double ga = 0, gc = 0;
+#pragma unroll
+                for (int k = 0; k < ND; ++k) {
+                    const int dd = sdir(ND, k);
+                    ga += gCf[dd] * Ad[dd];
+                    gc += Gf[dd] * (-inv2[dd]);
+                }
+                const double dpc = dPi(H[C0]) - PiPerH(C0);
+                Jc[C0] += Coef * (ga + gc * dpc);
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const int a = adj_slot(DIM, d);
+                    const double dpa = dPi(H[a]) - PiPerH(a);
+                    Jc[a] += Coef * (gAf[d] * Ad[d] + Gf[d] * inv2[d] * dpa);
+                }
+            }
+        }
+
+        // A(h)h of the terms held as entries
+#pragma unroll
+        for (int k = 0; k < NS; ++k) ax += Ar[k] * H[k];
+
+        if (MODE == 0) {
+            A.ax[i] = ax;
+        } else {
+            const double R = A.hold[p] - H[C0] - A.dt * A.theta * ax - A.dt * (1.0 - A.theta) * A.axold[i];
+            rabs = fabs(R);
+            if (A.rhs_raw) A.rhs_raw[i] = R;
+            if (A.ax_raw) A.ax_raw[i] = ax;
+            const double td = A.theta * A.dt;
+            double IA[NS];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) IA[k] = td * (Jc[k] + Ar[k]);
+            IA[C0] = 1.0 + IA[C0];
+            if (A.ia_raw) {
+#pragma unroll
+                for (int k = 0; k < NS; ++k) A.ia_raw[(size_t)k * g.N + i] = IA[k];
+            }
+            const double dg = IA[C0];
+            const double inv = (A.row_scaling && fabs(dg) >= 1.0e-30) ? 1.0 / dg : 1.0;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) A.ia[(size_t)k * g.N + i] = A.row_scaling ? IA[k] * inv : IA[k];
+            A.b[i] = A.row_scaling ? R * inv : R;
+        }
+    }
+    if (MODE == 1) {
+        __shared__ double sh[4];
+        for (int off = 32; off > 0; off >>= 1) rabs = fmax(rabs, __shfl_down(rabs, off, 64));
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = rabs;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, sh[w]);
+            atomicMax(A.rmax, (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
+}  // namespace yz

@@ -1,0 +1,897 @@
+// yz_solver.hip -- BiCGSTAB and structured smoothed-aggregation multigrid
+// over stencil operators (yz_stencil.h), FP64, for gfx950.
+//
+// BiCGSTAB follows the method of van der Vorst (1992) with right
+// preconditioning, as the reference's backends run it (Eigen's BiCGSTAB,
+// AMGCL's bicgstab: reference yuzen/src/numerics/LinearSolverBackend.cpp:
+// 87-111, 662-679): convergence on ||r|| <= tol ||b|| of the row-scaled
+// system; breakdown (rho, (rh,v) or omega zero, NaN) returns 2.
+//
+// The multigrid is smoothed aggregation (Vanek, Mandel & Brezina 1996), the
+// method of the reference's AMGCL preconditioner (smoothed_aggregation +
+// spai0), laid out for the structured grid: aggregates are the 3x3 blocks
+// of cells (3 in 1-D), the tentative prolongator the normalised constant
+// on each, smoothed once with damped Jacobi, P = (I - omega D^-1 A) P_t,
+// omega = 4/3 / rho(D^-1 A) with rho bounded by Gershgorin; the coarse
+// operator is the Galerkin product P^T A P, a 5x5 box stencil on the grid
+// of aggregates; SPAI0 (m_i = a_ii / sum_j a_ij^2) smooths before and after
+// the coarse correction; the coarsest level (<= 256 cells) is solved with
+// its dense inverse. Every level is a set of value planes: no column
+// indices, no hashing, no atomics in the set-up.
+//
+// Reductions are deterministic: fixed grid, per-block partial sums in LDS
+// (wave64 shuffles), one finishing block summing the partials in a fixed
+// order.
+#include "yz_solver.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/yuzen_gpu.h"
+
+namespace yz {
+
+#define YZ_CHECK(x)                                                                 \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) {                                                     \
+            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_),  \
+                         __FILE__, __LINE__);                                       \
+            return -1;                                                              \
+        }                                                                           \
+    } while (0)
+
+static constexpr int BLK = 256;
+static constexpr int RED_BLOCKS = 1024;   // partial sums per reduction
+static constexpr int COARSE_MAX = 256;
+
+static inline int nblk(int n) { return (n + BLK - 1) / BLK; }
+
+// ---------------------------------------------------------------------
+// block reduction of m values (wave64)
+// ---------------------------------------------------------------------
+template <int M>
+__device__ inline void block_sum_store(double (&v)[M], double *part, int stride) {
+    __shared__ double sh[M][BLK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        double x = v[m];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) sh[m][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            double s = 0;
+            for (int w = 0; w < BLK / 64; ++w) s += sh[m][w];
+            part[m * stride + blockIdx.x] = s;
+        }
+    }
+}
+
+// sum the partials of quantity m into out[m] (one block, fixed order)
+template <int M>
+__global__ void k_finish(const double *part, double *out) {
+    __shared__ double sh[M][BLK];
+    for (int m = 0; m < M; ++m) {
+        double s = 0;
+        for (int b = threadIdx.x; b < RED_BLOCKS; b += BLK) s += part[m * RED_BLOCKS + b];
+        sh[m][threadIdx.x] = s;
+    }
+    __syncthreads();
+    for (int w = BLK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int m = 0; m < M; ++m) sh[m][threadIdx.x] += sh[m][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int m = 0; m < M; ++m) out[m] = sh[m][0];
+}
+
+// ---------------------------------------------------------------------
+// stencil products
+// ---------------------------------------------------------------------
+template <int NS>
+__device__ inline double row_dot(const StencilOp &op, const double *__restrict__ val,
+                                 const double *__restrict__ x, int i, int ix, int iy) {
+    double s = 0;
+    const int N = op.N;
+    const int rx = op.dim == 2 ? op.nx : 0;
+    bool inside = true;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) inside = inside && stencil_inside(op, ix, iy, k);
+    if (inside) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) s += val[(size_t)k * N + i] * x[i + op.ox[k] + op.oy[k] * rx];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) s += val[(size_t)k * N + i] * x[stencil_col(op, ix, iy, k)];
+    }
+    return s;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_spmv(StencilOp op, const double *__restrict__ val,
+                                              const double *__restrict__ x, double *__restrict__ y) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    y[i] = row_dot<NS>(op, val, x, i, ix, iy);
+}
+
+template <class F>
+static void dispatch_ns(int ns, F f) {
+    switch (ns) {
+    case 5: f(std::integral_constant<int, 5>()); break;
+    case 13: f(std::integral_constant<int, 13>()); break;
+    default: f(std::integral_constant<int, 25>()); break;
+    }
+}
+
+void launch_spmv(const StencilOp &op, const double *val, const double *x, double *y, hipStream_t st) {
+    dispatch_ns(op.ns, [&](auto c) {
+        constexpr int NS = decltype(c)::value;
+        hipLaunchKernelGGL(k_spmv<NS>, dim3(nblk(op.N)), dim3(BLK), 0, st, op, val, x, y);
+    });
+}
+
+// ---------------------------------------------------------------------
+// BiCGSTAB kernels. Device scalars sc[]:
+// 0 rho1, 1 rho2, 2 alpha, 3 omega, 4 beta, 5 (rh,v), 6 (s,s), 7 (t,t),
+// 8 (t,s), 9 (r,r), 10 (b,b), 11 (r,rh) next
+// ---------------------------------------------------------------------
+enum { SC_RHO1, SC_RHO2, SC_ALPHA, SC_OMEGA, SC_BETA, SC_RV, SC_SS, SC_TT, SC_TS, SC_RR, SC_BB,
+       SC_RRH, SC_N };
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_init_residual(StencilOp op, const double *__restrict__ val,
+                                                       const double *__restrict__ x, const double *__restrict__ b,
+                                                       double *__restrict__ r, double *__restrict__ rh,
+                                                       double *__restrict__ part) {
+    double acc[2] = {0, 0};
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < op.N; i += RED_BLOCKS * BLK) {
+        const int ix = i % op.nx, iy = i / op.nx;
+        const double ri = b[i] - row_dot<NS>(op, val, x, i, ix, iy);
+        r[i] = ri; rh[i] = ri;
+        acc[0] += b[i] * b[i];
+        acc[1] += ri * ri;
+    }
+    block_sum_store<2>(acc, part, RED_BLOCKS);
+}
+
+__global__ void k_p_update(int N, int first, const double *__restrict__ r, double *__restrict__ p,
+                           const double *__restrict__ v, const double *__restrict__ sc) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= N) return;
+    if (first) { p[i] = r[i]; return; }
+    const double beta = sc[SC_BETA], omega = sc[SC_OMEGA];
+    p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_spmv_dot(StencilOp op, const double *__restrict__ val,
+                                                  const double *__restrict__ x, double *__restrict__ y,
+                                                  const double *__restrict__ w, double *__restrict__ part) {
+    double acc[1] = {0};
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < op.N; i += RED_BLOCKS * BLK) {
+        const int ix = i % op.nx, iy = i / op.nx;
+        const double yi = row_dot<NS>(op, val, x, i, ix, iy);
+        y[i] = yi;
+        acc[0] += w[i] * yi;
+    }
+    block_sum_store<1>(acc, part, RED_BLOCKS);
+}
+
+__global__ void k_s_update(int N, const double *__restrict__ r, const double *__restrict__ v,
+                           double *__restrict__ s, const double *__restrict__ sc, double *__restrict__ part) {
+    double acc[1] = {0};
+    const double alpha = sc[SC_ALPHA];
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < N; i += RED_BLOCKS * BLK) {
+        const double si = r[i] - alpha * v[i];
+        s[i] = si;
+        acc[0] += si * si;
+    }
+    block_sum_store<1>(acc, part, RED_BLOCKS);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_spmv_dot2(StencilOp op, const double *__restrict__ val,
+                                                   const double *__restrict__ x, double *__restrict__ t,
+                                                   const double *__restrict__ s, double *__restrict__ part) {
+    double acc[2] = {0, 0};
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < op.N; i += RED_BLOCKS * BLK) {
+        const int ix = i % op.nx, iy = i / op.nx;
+        const double ti = row_dot<NS>(op, val, x, i, ix, iy);
+        t[i] = ti;
+        acc[0] += ti * ti;
+        acc[1] += ti * s[i];
+    }
+    block_sum_store<2>(acc, part, RED_BLOCKS);
+}
+
+__global__ void k_xr_update(int N, double *__restrict__ x, const double *__restrict__ ph,
+                            const double *__restrict__ sh, double *__restrict__ r,
+                            const double *__restrict__ s, const double *__restrict__ t,
+                            const double *__restrict__ rh, const double *__restrict__ sc,
+                            double *__restrict__ part) {
+    double acc[2] = {0, 0};
+    const double alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < N; i += RED_BLOCKS * BLK) {
+        x[i] += alpha * ph[i] + omega * sh[i];
+        const double ri = s[i] - omega * t[i];
+        r[i] = ri;
+        acc[0] += ri * ri;
+        acc[1] += ri * rh[i];
+    }
+    block_sum_store<2>(acc, part, RED_BLOCKS);
+}
+
+__global__ void k_x_alpha(int N, double *__restrict__ x, const double *__restrict__ ph,
+                          const double *__restrict__ sc) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= N) return;
+    x[i] += sc[SC_ALPHA] * ph[i];
+}
+
+// scalar steps (one thread)
+__global__ void k_sc_init(double *sc, const double *sums) {
+    sc[SC_BB] = sums[0];
+    sc[SC_RR] = sums[1];
+    sc[SC_RHO1] = sums[1];
+    sc[SC_RHO2] = 1; sc[SC_ALPHA] = 1; sc[SC_OMEGA] = 1; sc[SC_BETA] = 0;
+}
+__global__ void k_sc_alpha(double *sc, const double *sums) {
+    sc[SC_RV] = sums[0];
+    sc[SC_ALPHA] = (sums[0] != 0) ? sc[SC_RHO1] / sums[0] : 0.0;
+}
+__global__ void k_sc_ss(double *sc, const double *sums) { sc[SC_SS] = sums[0]; }
+__global__ void k_sc_omega(double *sc, const double *sums) {
+    sc[SC_TT] = sums[0]; sc[SC_TS] = sums[1];
+    sc[SC_OMEGA] = (sums[0] != 0) ? sums[1] / sums[0] : 0.0;
+}
+__global__ void k_sc_next(double *sc, const double *sums) {
+    sc[SC_RR] = sums[0];
+    sc[SC_RHO2] = sc[SC_RHO1];
+    sc[SC_RHO1] = sums[1];
+    sc[SC_BETA] = (sc[SC_RHO1] * sc[SC_ALPHA]) / (sc[SC_RHO2] * sc[SC_OMEGA]);
+}
+
+// ---------------------------------------------------------------------
+// Jacobi (the reference's damped_jacobi preconditioner: 0.72 D^-1)
+// ---------------------------------------------------------------------
+template <int NS>
+__global__ void k_diag(StencilOp op, const double *__restrict__ val, double *__restrict__ dg) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    double d = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        if (stencil_col(op, ix, iy, k) == i) d += val[(size_t)k * op.N + i];
+    dg[i] = d;
+}
+__global__ void k_jacobi_inv(int N, double *dg) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= N) return;
+    dg[i] = dg[i] != 0 ? 0.72 / dg[i] : 0.0;
+}
+__global__ void k_scale(int N, const double *__restrict__ d, const double *__restrict__ b, double *__restrict__ x) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= N) return;
+    x[i] = d[i] * b[i];
+}
+
+// ---------------------------------------------------------------------
+// multigrid set-up
+// ---------------------------------------------------------------------
+// merge the slots of row i that fold onto the same column; returns count
+template <int NS>
+__device__ inline int merged_row(const StencilOp &op, const double *val, int i, int ix, int iy,
+                                 int *cols, double *vals) {
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const double v = val[(size_t)k * op.N + i];
+        const int c = stencil_col(op, ix, iy, k);
+        int q = 0;
+        for (; q < m; ++q) if (cols[q] == c) break;
+        if (q == m) { cols[m] = c; vals[m] = v; ++m; }
+        else vals[q] += v;
+    }
+    return m;
+}
+
+// SPAI0 diagonal, true diagonal and the Gershgorin bound of D^-1 A
+template <int NS>
+__global__ void k_level_diag(StencilOp op, const double *__restrict__ val, double *__restrict__ minv,
+                             double *__restrict__ dg, unsigned long long *__restrict__ rho_bits) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    double g = 0;
+    if (i < op.N) {
+        const int ix = i % op.nx, iy = i / op.nx;
+        int cols[NS];
+        double vals[NS];
+        const int m = merged_row<NS>(op, val, i, ix, iy, cols, vals);
+        double d = 0, ss = 0, off = 0;
+        for (int q = 0; q < m; ++q) {
+            ss += vals[q] * vals[q];
+            if (cols[q] == i) d = vals[q]; else off += fabs(vals[q]);
+        }
+        minv[i] = (ss > 0) ? d / ss : 0.0;
+        dg[i] = d;
+        g = (d != 0) ? (fabs(d) + off) / fabs(d) : 0.0;
+    }
+    // block max, then one atomic per block (non-negative doubles order as
+    // their bit patterns)
+    __shared__ double sh[BLK / 64];
+    for (int off = 32; off > 0; off >>= 1) g = fmax(g, __shfl_down(g, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = g;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0;
+        for (int w = 0; w < BLK / 64; ++w) mx = fmax(mx, sh[w]);
+        atomicMax(rho_bits, (unsigned long long)__double_as_longlong(mx));
+    }
+}
+
+__device__ inline int agg_size(int I, int n) {
+    const int lo = 3 * I, hi = min(3 * I + 3, n);
+    return hi - lo;
+}
+// coarse offset between aggregates along an axis, wrapped when periodic
+__device__ inline int coarse_diff(int J, int I, int nc, int periodic) {
+    int d = J - I;
+    if (periodic) {
+        if (d > nc / 2) d -= nc;
+        if (d < -(nc / 2)) d += nc;
+    }
+    return d;
+}
+
+// smoothed prolongator, P(i, agg(i) + e), e in [-1,1]^dim
+template <int NS>
+__global__ void k_build_P(StencilOp op, const double *__restrict__ val, const double *__restrict__ dg,
+                          const unsigned long long *__restrict__ rho_bits, int ncx, int ncy,
+                          double *__restrict__ P) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    const double rho = __longlong_as_double((long long)*rho_bits);
+    const double omega = (rho > 0) ? (4.0 / 3.0) / rho : 0.0;
+    const int Ix = ix / 3, Iy = iy / 3;
+    const int NP = op.dim == 2 ? 9 : 3;
+    double p[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) p[e] = 0;
+    const double tI = 1.0 / sqrt((double)(agg_size(Ix, op.nx) * (op.dim == 2 ? agg_size(Iy, op.ny) : 1)));
+    p[op.dim == 2 ? 4 : 1] = tI;
+    const double w = (dg[i] != 0) ? omega / dg[i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int c = stencil_col(op, ix, iy, k);
+        const int jx = c % op.nx, jy = c / op.nx;
+        const int Jx = jx / 3, Jy = jy / 3;
+        const int ex = coarse_diff(Jx, Ix, ncx, op.perx);
+        const int ey = op.dim == 2 ? coarse_diff(Jy, Iy, ncy, op.pery) : 0;
+        const double tJ = 1.0 / sqrt((double)(agg_size(Jx, op.nx) * (op.dim == 2 ? agg_size(Jy, op.ny) : 1)));
+        const int e = op.dim == 2 ? (ey + 1) * 3 + (ex + 1) : ex + 1;
+        p[e] -= w * val[(size_t)k * op.N + i] * tJ;
+    }
+    for (int e = 0; e < NP; ++e) P[(size_t)e * op.N + i] = p[e];
+}
+
+// A P (i, agg(i) + f), f in [-2,2]^dim
+template <int NS>
+__global__ void k_build_AP(StencilOp op, const double *__restrict__ val, const double *__restrict__ P,
+                           int ncx, int ncy, double *__restrict__ AP) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    const int Ix = ix / 3, Iy = iy / 3;
+    const bool two = op.dim == 2;
+    double ap[25];
+#pragma unroll
+    for (int f = 0; f < 25; ++f) ap[f] = 0;
+    for (int k = 0; k < NS; ++k) {
+        const double a = val[(size_t)k * op.N + i];
+        if (a == 0) continue;
+        const int c = stencil_col(op, ix, iy, k);
+        const int jx = c % op.nx, jy = c / op.nx;
+        const int ex0 = coarse_diff(jx / 3, Ix, ncx, op.perx);
+        const int ey0 = two ? coarse_diff(jy / 3, Iy, ncy, op.pery) : 0;
+        if (two) {
+            for (int ey = -1; ey <= 1; ++ey)
+                for (int ex = -1; ex <= 1; ++ex) {
+                    const double pv = P[(size_t)((ey + 1) * 3 + (ex + 1)) * op.N + c];
+                    const int fx = ex0 + ex, fy = ey0 + ey;
+                    ap[(fy + 2) * 5 + (fx + 2)] += a * pv;
+                }
+        } else {
+            for (int ex = -1; ex <= 1; ++ex) {
+                const double pv = P[(size_t)(ex + 1) * op.N + c];
+                ap[ex0 + ex + 2] += a * pv;
+            }
+        }
+    }
+    const int NF = two ? 25 : 5;
+    for (int f = 0; f < NF; ++f) AP[(size_t)f * op.N + i] = ap[f];
+}
+
+// Galerkin product on the coarse grid: Ac(I, I + q), q in [-2,2]^dim
+__global__ void k_build_RAP(StencilOp fop, const double *__restrict__ P, const double *__restrict__ AP,
+                            StencilOp cop, double *__restrict__ Ac) {
+    const int I = blockIdx.x * BLK + threadIdx.x;
+    if (I >= cop.N) return;
+    const bool two = fop.dim == 2;
+    const int Ix = I % cop.nx, Iy = I / cop.nx;
+    double ac[25];
+#pragma unroll
+    for (int q = 0; q < 25; ++q) ac[q] = 0;
+    const int ylo = two ? -1 : 0, yhi = two ? 1 : 0;
+    for (int ey = ylo; ey <= yhi; ++ey) {
+        int Ky = Iy + ey;
+        if (two) {
+            if (cop.pery) Ky = (Ky + cop.ny) % cop.ny;
+            else if (Ky < 0 || Ky >= cop.ny) continue;
+        }
+        for (int ex = -1; ex <= 1; ++ex) {
+            int Kx = Ix + ex;
+            if (cop.perx) Kx = (Kx + cop.nx) % cop.nx;
+            else if (Kx < 0 || Kx >= cop.nx) continue;
+            // fine cells of aggregate K; P(i, I) is the slot I - agg(i) = -e
+            const int pe = two ? (1 - ey) * 3 + (1 - ex) : (1 - ex);
+            const int x0 = 3 * Kx, x1 = min(3 * Kx + 3, fop.nx);
+            const int y0 = two ? 3 * Ky : 0, y1 = two ? min(3 * Ky + 3, fop.ny) : 1;
+            for (int fy = y0; fy < y1; ++fy)
+                for (int fx = x0; fx < x1; ++fx) {
+                    const int i = fy * fop.nx + fx;
+                    const double pv = P[(size_t)pe * fop.N + i];
+                    if (pv == 0) continue;
+                    // AP(i, K + f) -> Ac(I, I + e + f)
+                    if (two) {
+                        for (int gy = -2; gy <= 2; ++gy) {
+                            const int qy = ey + gy;
+                            if (qy < -2 || qy > 2) continue;
+                            for (int gx = -2; gx <= 2; ++gx) {
+                                const int qx = ex + gx;
+                                if (qx < -2 || qx > 2) continue;
+                                ac[(qy + 2) * 5 + (qx + 2)] += pv * AP[(size_t)((gy + 2) * 5 + (gx + 2)) * fop.N + i];
+                            }
+                        }
+                    } else {
+                        for (int gx = -2; gx <= 2; ++gx) {
+                            const int qx = ex + gx;
+                            if (qx < -2 || qx > 2) continue;
+                            ac[qx + 2] += pv * AP[(size_t)(gx + 2) * fop.N + i];
+                        }
+                    }
+                }
+        }
+    }
+    const int NQ = two ? 25 : 5;
+    for (int q = 0; q < NQ; ++q) Ac[(size_t)q * cop.N + I] = ac[q];
+}
+
+// dense coarsest matrix
+template <int NS>
+__global__ void k_dense(StencilOp op, const double *__restrict__ val, double *__restrict__ Dn) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    for (int j = 0; j < op.N; ++j) Dn[(size_t)i * op.N + j] = 0;
+    for (int k = 0; k < NS; ++k) Dn[(size_t)i * op.N + stencil_col(op, ix, iy, k)] += val[(size_t)k * op.N + i];
+}
+
+// Gauss-Jordan inverse with partial pivoting, one block; A (n x n) is
+// destroyed, inv receives A^-1
+__global__ void __launch_bounds__(1024) k_gauss_jordan(int n, double *A, double *inv) {
+    __shared__ double pv[1024];
+    __shared__ int pi[1024];
+    __shared__ int piv_row;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) inv[t] = ((t / n) == (t % n)) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        double best = -1; int br = k;
+        for (int r = k + threadIdx.x; r < n; r += blockDim.x) {
+            const double a = fabs(A[(size_t)r * n + k]);
+            if (a > best) { best = a; br = r; }
+        }
+        pv[threadIdx.x] = best; pi[threadIdx.x] = br;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = -1; int r0 = k;
+            for (int t = 0; t < (int)blockDim.x; ++t) if (pv[t] > b) { b = pv[t]; r0 = pi[t]; }
+            piv_row = r0;
+        }
+        __syncthreads();
+        const int pr = piv_row;
+        if (pr != k)
+            for (int j = threadIdx.x; j < n; j += blockDim.x) {
+                double a = A[(size_t)k * n + j]; A[(size_t)k * n + j] = A[(size_t)pr * n + j]; A[(size_t)pr * n + j] = a;
+                double b = inv[(size_t)k * n + j]; inv[(size_t)k * n + j] = inv[(size_t)pr * n + j]; inv[(size_t)pr * n + j] = b;
+            }
+        __syncthreads();
+        const double d = A[(size_t)k * n + k];
+        const double di = (d != 0) ? 1.0 / d : 0.0;
+        __syncthreads();
+        for (int j = threadIdx.x; j < n; j += blockDim.x) { A[(size_t)k * n + j] *= di; inv[(size_t)k * n + j] *= di; }
+        __syncthreads();
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            const int r = t / n, j = t % n;
+            if (r == k) continue;
+            const double f = A[(size_t)r * n + k];
+            if (f == 0) continue;
+            // column k of row r is eliminated last, after every other column
+            if (j != k) {
+                A[(size_t)r * n + j] -= f * A[(size_t)k * n + j];
+            }
+            inv[(size_t)r * n + j] -= f * inv[(size_t)k * n + j];
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < n; r += blockDim.x) if (r != k) A[(size_t)r * n + k] = 0;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------
+// V-cycle kernels
+// ---------------------------------------------------------------------
+// pre-smoothing from zero and the residual: x = M b, r = b - A x
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_pre(StencilOp op, const double *__restrict__ val,
+                                             const double *__restrict__ m, const double *__restrict__ b,
+                                             double *__restrict__ x, double *__restrict__ r) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int c = stencil_col(op, ix, iy, k);
+        s += val[(size_t)k * op.N + i] * (m[c] * b[c]);
+    }
+    x[i] = m[i] * b[i];
+    r[i] = b[i] - s;
+}
+
+// restriction b_c = P^T r
+__global__ void k_restrict(StencilOp fop, const double *__restrict__ P, const double *__restrict__ r,
+                           StencilOp cop, double *__restrict__ bc) {
+    const int I = blockIdx.x * BLK + threadIdx.x;
+    if (I >= cop.N) return;
+    const bool two = fop.dim == 2;
+    const int Ix = I % cop.nx, Iy = I / cop.nx;
+    double s = 0;
+    const int ylo = two ? -1 : 0, yhi = two ? 1 : 0;
+    for (int ey = ylo; ey <= yhi; ++ey) {
+        int Ky = Iy + ey;
+        if (two) {
+            if (cop.pery) Ky = (Ky + cop.ny) % cop.ny;
+            else if (Ky < 0 || Ky >= cop.ny) continue;
+        }
+        for (int ex = -1; ex <= 1; ++ex) {
+            int Kx = Ix + ex;
+            if (cop.perx) Kx = (Kx + cop.nx) % cop.nx;
+            else if (Kx < 0 || Kx >= cop.nx) continue;
+            const int pe = two ? (1 - ey) * 3 + (1 - ex) : (1 - ex);
+            const int x0 = 3 * Kx, x1 = min(3 * Kx + 3, fop.nx);
+            const int y0 = two ? 3 * Ky : 0, y1 = two ? min(3 * Ky + 3, fop.ny) : 1;
+            for (int fy = y0; fy < y1; ++fy)
+                for (int fx = x0; fx < x1; ++fx) {
+                    const int i = fy * fop.nx + fx;
+                    s += P[(size_t)pe * fop.N + i] * r[i];
+                }
+        }
+    }
+    bc[I] = s;
+}
+
+// prolongation x += P x_c
+__global__ void k_prolong(StencilOp fop, const double *__restrict__ P, StencilOp cop,
+                          const double *__restrict__ xc, double *__restrict__ x) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= fop.N) return;
+    const bool two = fop.dim == 2;
+    const int ix = i % fop.nx, iy = i / fop.nx;
+    const int Ix = ix / 3, Iy = iy / 3;
+    double s = 0;
+    const int ylo = two ? -1 : 0, yhi = two ? 1 : 0;
+    for (int ey = ylo; ey <= yhi; ++ey) {
+        int Jy = Iy + ey;
+        if (two) {
+            if (cop.pery) Jy = (Jy + cop.ny) % cop.ny;
+            else if (Jy < 0 || Jy >= cop.ny) continue;
+        }
+        for (int ex = -1; ex <= 1; ++ex) {
+            int Jx = Ix + ex;
+            if (cop.perx) Jx = (Jx + cop.nx) % cop.nx;
+            else if (Jx < 0 || Jx >= cop.nx) continue;
+            const int e = two ? (ey + 1) * 3 + (ex + 1) : ex + 1;
+            s += P[(size_t)e * fop.N + i] * xc[Jy * cop.nx + Jx];
+        }
+    }
+    x[i] += s;
+}
+
+// post-smoothing, out of place: y = x + M (b - A x)
+template <int NS>
+__global__ void __launch_bounds__(BLK) k_post(StencilOp op, const double *__restrict__ val,
+                                              const double *__restrict__ m, const double *__restrict__ b,
+                                              const double *__restrict__ x, double *__restrict__ y) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    const double ax = row_dot<NS>(op, val, x, i, ix, iy);
+    y[i] = x[i] + m[i] * (b[i] - ax);
+}
+
+__global__ void k_dense_mv(int n, const double *__restrict__ Ainv, const double *__restrict__ b,
+                           double *__restrict__ x) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    double s = 0;
+    for (int j = 0; j < n; ++j) s += Ainv[(size_t)i * n + j] * b[j];
+    x[i] = s;
+}
+
+// ---------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------
+static int dalloc(double **p, size_t n) {
+    if (*p) return 0;
+    YZ_CHECK(hipMalloc((void **)p, n * sizeof(double)));
+    return 0;
+}
+
+static StencilOp box_op(int dim, int nx, int ny, int perx, int pery) {
+    StencilOp op;
+    std::memset(&op, 0, sizeof(op));
+    op.dim = dim; op.nx = nx; op.ny = dim == 2 ? ny : 1; op.N = op.nx * op.ny;
+    op.perx = perx; op.pery = pery;
+    if (dim == 2) {
+        op.ns = 25;
+        for (int q = 0; q < 25; ++q) { op.ox[q] = q % 5 - 2; op.oy[q] = q / 5 - 2; }
+    } else {
+        op.ns = 5;
+        for (int q = 0; q < 5; ++q) { op.ox[q] = q - 2; op.oy[q] = 0; }
+    }
+    return op;
+}
+
+int solver_init(Solver &S, const StencilOp &op0, hipStream_t stream) {
+    if (S.kr.N == op0.N && S.op0.ns == op0.ns) { S.op0 = op0; S.stream = stream; return 0; }
+    solver_free(S);
+    S.op0 = op0;
+    S.stream = stream;
+    const int N = op0.N;
+    if (dalloc(&S.A, (size_t)op0.ns * N)) return -1;
+    Krylov &k = S.kr;
+    k.N = N;
+    double **vs[] = {&k.r, &k.rh, &k.p, &k.v, &k.s, &k.t, &k.ph, &k.sh};
+    for (double **v : vs) if (dalloc(v, N)) return -1;
+    if (dalloc(&k.part, 4 * RED_BLOCKS)) return -1;
+    if (dalloc(&k.sc, 64)) return -1;
+    YZ_CHECK(hipMemset(k.sc, 0, 64 * sizeof(double)));
+    YZ_CHECK(hipHostMalloc((void **)&k.host_sc, 64 * sizeof(double), hipHostMallocDefault));
+    // multigrid hierarchy shapes (fixed for the grid)
+    Amg &g = S.amg;
+    g.lv.clear();
+    AmgLevel L0;
+    L0.op = op0;
+    g.lv.push_back(L0);
+    while ((int)g.lv.size() < S.max_levels) {
+        AmgLevel &L = g.lv.back();
+        if (L.op.N <= COARSE_MAX) break;
+        const int ncx = (L.op.nx + 2) / 3;
+        const int ncy = L.op.dim == 2 ? (L.op.ny + 2) / 3 : 1;
+        if (L.op.perx && ncx < 5) break;
+        if (L.op.dim == 2 && L.op.pery && ncy < 5) break;
+        if (ncx * ncy >= L.op.N) break;
+        L.ncx = ncx; L.ncy = ncy;
+        AmgLevel C;
+        C.op = box_op(L.op.dim, ncx, ncy, L.op.perx, L.op.pery);
+        g.lv.push_back(C);
+    }
+    for (size_t l = 0; l < g.lv.size(); ++l) {
+        AmgLevel &L = g.lv[l];
+        const int n = L.op.N;
+        if (l > 0 && dalloc(&L.val, (size_t)L.op.ns * n)) return -1;
+        if (dalloc(&L.minv, n) || dalloc(&L.x, n) || dalloc(&L.b, n) || dalloc(&L.r, n)) return -1;
+        if (l + 1 < g.lv.size()) {
+            const int np = L.op.dim == 2 ? 9 : 3, nf = L.op.dim == 2 ? 25 : 5;
+            if (dalloc(&L.P, (size_t)np * n) || dalloc(&L.AP, (size_t)nf * n)) return -1;
+        }
+    }
+    g.lv[0].val = S.A;
+    g.nco = g.lv.back().op.N;
+    if (dalloc(&g.coarse_inv, (size_t)g.nco * g.nco) || dalloc(&g.coarse_dense, (size_t)g.nco * g.nco)) return -1;
+    if (dalloc(&g.scal, 2 * S.max_levels)) return -1;
+    if (dalloc(&S.dinv_jac, N)) return -1;
+    return 0;
+}
+
+void solver_free(Solver &S) {
+    auto f = [](double *&p) { if (p) { (void)hipFree(p); p = nullptr; } };
+    for (size_t l = 0; l < S.amg.lv.size(); ++l) {
+        AmgLevel &L = S.amg.lv[l];
+        if (l > 0) f(L.val);
+        f(L.minv); f(L.P); f(L.AP); f(L.x); f(L.b); f(L.r);
+    }
+    S.amg.lv.clear();
+    f(S.amg.coarse_inv); f(S.amg.coarse_dense); f(S.amg.scal);
+    f(S.A); f(S.dinv_jac);
+    Krylov &k = S.kr;
+    f(k.r); f(k.rh); f(k.p); f(k.v); f(k.s); f(k.t); f(k.ph); f(k.sh); f(k.part); f(k.sc);
+    if (k.host_sc) { (void)hipHostFree(k.host_sc); k.host_sc = nullptr; }
+    k.N = 0;
+}
+
+int amg_setup(Solver &S) {
+    Amg &g = S.amg;
+    hipStream_t st = S.stream;
+    g.lv[0].val = S.A;
+    YZ_CHECK(hipMemsetAsync(g.scal, 0, 2 * S.max_levels * sizeof(double), st));
+    for (size_t l = 0; l < g.lv.size(); ++l) {
+        AmgLevel &L = g.lv[l];
+        unsigned long long *rho = reinterpret_cast<unsigned long long *>(g.scal) + l;
+        dispatch_ns(L.op.ns, [&](auto c) {
+            constexpr int NS = decltype(c)::value;
+            // L.r holds the true diagonal during the set-up
+            hipLaunchKernelGGL(k_level_diag<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv,
+                               L.r, rho);
+        });
+        if (l + 1 == g.lv.size()) break;
+        AmgLevel &C = g.lv[l + 1];
+        dispatch_ns(L.op.ns, [&](auto c) {
+            constexpr int NS = decltype(c)::value;
+            hipLaunchKernelGGL(k_build_P<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.r, rho,
+                               L.ncx, L.ncy, L.P);
+            hipLaunchKernelGGL(k_build_AP<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.P, L.ncx,
+                               L.ncy, L.AP);
+        });
+        hipLaunchKernelGGL(k_build_RAP, dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.AP, C.op, C.val);
+    }
+    AmgLevel &Lc = g.lv.back();
+    dispatch_ns(Lc.op.ns, [&](auto c) {
+        constexpr int NS = decltype(c)::value;
+        hipLaunchKernelGGL(k_dense<NS>, dim3(nblk(Lc.op.N)), dim3(BLK), 0, st, Lc.op, Lc.val, g.coarse_dense);
+    });
+    hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), 0, st, g.nco, g.coarse_dense, g.coarse_inv);
+    YZ_CHECK(hipGetLastError());
+    g.built = 1;
+    return 0;
+}
+
+int jacobi_setup(Solver &S) {
+    dispatch_ns(S.op0.ns, [&](auto c) {
+        constexpr int NS = decltype(c)::value;
+        hipLaunchKernelGGL(k_diag<NS>, dim3(nblk(S.op0.N)), dim3(BLK), 0, S.stream, S.op0, S.A, S.dinv_jac);
+    });
+    hipLaunchKernelGGL(k_jacobi_inv, dim3(nblk(S.op0.N)), dim3(BLK), 0, S.stream, S.op0.N, S.dinv_jac);
+    YZ_CHECK(hipGetLastError());
+    return 0;
+}
+
+// one V-cycle: out = M^-1 in
+static void vcycle(Solver &S, const double *in, double *out) {
+    Amg &g = S.amg;
+    hipStream_t st = S.stream;
+    const int nl = (int)g.lv.size();
+    if (nl == 1) {
+        hipLaunchKernelGGL(k_dense_mv, dim3(nblk(g.nco)), dim3(BLK), 0, st, g.nco, g.coarse_inv, in, out);
+        return;
+    }
+    // downward
+    const double *b = in;
+    for (int l = 0; l + 1 < nl; ++l) {
+        AmgLevel &L = g.lv[l];
+        AmgLevel &C = g.lv[l + 1];
+        dispatch_ns(L.op.ns, [&](auto c) {
+            constexpr int NS = decltype(c)::value;
+            hipLaunchKernelGGL(k_pre<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv, b, L.x, L.r);
+        });
+        hipLaunchKernelGGL(k_restrict, dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.r, C.op, C.b);
+        b = C.b;
+    }
+    AmgLevel &Lc = g.lv[nl - 1];
+    hipLaunchKernelGGL(k_dense_mv, dim3(nblk(g.nco)), dim3(BLK), 0, st, g.nco, g.coarse_inv, Lc.b, Lc.x);
+    // upward
+    for (int l = nl - 2; l >= 0; --l) {
+        AmgLevel &L = g.lv[l];
+        AmgLevel &C = g.lv[l + 1];
+        const double *bl = (l == 0) ? in : L.b;
+        hipLaunchKernelGGL(k_prolong, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.P, C.op, C.x, L.x);
+        double *dst = (l == 0) ? out : L.r;   // r is free here
+        dispatch_ns(L.op.ns, [&](auto c) {
+            constexpr int NS = decltype(c)::value;
+            hipLaunchKernelGGL(k_post<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv, bl, L.x, dst);
+        });
+        if (l > 0) std::swap(L.x, L.r);
+    }
+}
+
+static void precond_apply(Solver &S, int precond, const double *in, double *out) {
+    if (precond == YZ_PC_AMG) vcycle(S, in, out);
+    else hipLaunchKernelGGL(k_scale, dim3(nblk(S.op0.N)), dim3(BLK), 0, S.stream, S.op0.N, S.dinv_jac, in, out);
+}
+
+static int read_scalars(Solver &S) {
+    YZ_CHECK(hipMemcpyAsync(S.kr.host_sc, S.kr.sc, SC_N * sizeof(double), hipMemcpyDeviceToHost, S.stream));
+    YZ_CHECK(hipStreamSynchronize(S.stream));
+    return 0;
+}
+
+int bicgstab(Solver &S, const double *b, double *x, int precond, int maxit, double tol, int *iters,
+             double *resid) {
+    Krylov &k = S.kr;
+    const StencilOp &op = S.op0;
+    const int N = op.N;
+    hipStream_t st = S.stream;
+    double *sums = k.sc + 32;
+    *iters = 0;
+    *resid = 0;
+    dispatch_ns(op.ns, [&](auto c) {
+        constexpr int NS = decltype(c)::value;
+        hipLaunchKernelGGL(k_init_residual<NS>, dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, x, b, k.r, k.rh, k.part);
+    });
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, sums);
+    hipLaunchKernelGGL(k_sc_init, dim3(1), dim3(1), 0, st, k.sc, sums);
+    if (read_scalars(S)) return 2;
+    const double *h = k.host_sc;
+    const double nb = std::sqrt(h[SC_BB]);
+    if (nb == 0) {
+        YZ_CHECK(hipMemsetAsync(x, 0, N * sizeof(double), st));
+        return 0;
+    }
+    const double eps = tol * nb;
+    double res = std::sqrt(h[SC_RR]);
+    if (res <= eps) { *resid = res / nb; return 0; }
+    int it = 0;
+    for (; it < maxit; ++it) {
+        if (it > 0 && std::fabs(h[SC_RHO1]) < 1e-300) { *iters = it; *resid = res / nb; return 2; }
+        hipLaunchKernelGGL(k_p_update, dim3(nblk(N)), dim3(BLK), 0, st, N, it == 0 ? 1 : 0, k.r, k.p, k.v, k.sc);
+        precond_apply(S, precond, k.p, k.ph);
+        dispatch_ns(op.ns, [&](auto c) {
+            constexpr int NS = decltype(c)::value;
+            hipLaunchKernelGGL(k_spmv_dot<NS>, dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, k.ph, k.v, k.rh, k.part);
+        });
+        hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLK), 0, st, k.part, sums);
+        hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(1), 0, st, k.sc, sums);
+        hipLaunchKernelGGL(k_s_update, dim3(RED_BLOCKS), dim3(BLK), 0, st, N, k.r, k.v, k.s, k.sc, k.part);
+        hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLK), 0, st, k.part, sums);
+        hipLaunchKernelGGL(k_sc_ss, dim3(1), dim3(1), 0, st, k.sc, sums);
+        if (read_scalars(S)) return 2;
+        if (h[SC_RV] == 0) { *iters = it; *resid = res / nb; return 2; }
+        res = std::sqrt(h[SC_SS]);
+        if (res < eps) {
+            hipLaunchKernelGGL(k_x_alpha, dim3(nblk(N)), dim3(BLK), 0, st, N, x, k.ph, k.sc);
+            ++it;
+            break;
+        }
+        precond_apply(S, precond, k.s, k.sh);
+        dispatch_ns(op.ns, [&](auto c) {
+            constexpr int NS = decltype(c)::value;
+            hipLaunchKernelGGL(k_spmv_dot2<NS>, dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, k.sh, k.t, k.s, k.part);
+        });
+        hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, sums);
+        hipLaunchKernelGGL(k_sc_omega, dim3(1), dim3(1), 0, st, k.sc, sums);
+        hipLaunchKernelGGL(k_xr_update, dim3(RED_BLOCKS), dim3(BLK), 0, st, N, x, k.ph, k.sh, k.r, k.s, k.t, k.rh,
+                           k.sc, k.part);
+        hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, sums);
+        hipLaunchKernelGGL(k_sc_next, dim3(1), dim3(1), 0, st, k.sc, sums);
+        if (read_scalars(S)) return 2;
+        if (h[SC_OMEGA] == 0) { *iters = it; *resid = res / nb; return 2; }
+        res = std::sqrt(h[SC_RR]);
+        if (res < eps) { ++it; break; }
+        if (!(res == res)) { *iters = it; *resid = res; return 2; }
+    }
+    *iters = it;
+    *resid = res / nb;
+    if (!(*resid == *resid)) return 2;
+    return (*resid <= tol) ? 0 : 1;
+}
+
+}  // namespace yz

@@ -1,0 +1,640 @@
+// yz_step.hip -- the device-resident implicit film-height step and the
+// C-ABI of libyuzen_hip.so (include/yuzen_gpu.h).
+//
+// Host control mirrors the reference's Newton-Kantorovich driver
+// (yuzen/src/numerics/NewtonKantorovich.cpp, paths relative to the
+// reference root):
+//   TimeAdvance :424-486  -- retry from h^n when the extrapolated start
+//                            fails; step control or FailedFixedStep
+//   NKLoop      :157-422  -- extrapolated start, the exits (absolute,
+//                            relative to the first correction, stagnation
+//                            with its three acceptance tests), alpha 1/2
+//                            when the correction grew
+//   Solve       :74-154   -- assembly, Eisenstat-Walker tolerance,
+//                            linear solve, correction zeroed on failure
+// and the step control of ImplicitSolver.cpp:144-307. Every vector stays
+// on the device; per Newton pass the host reads back three scalars
+// (max |Rhs|, the solve's outcome, max |correction|).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/yuzen_gpu.h"
+#include "yz_rows.h"
+#include "yz_solver.h"
+
+namespace yz {
+
+static thread_local std::string g_err;
+static void set_err(const std::string &s) { g_err = s; }
+
+#define YZ_TRY(x)                                                                           \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) {                                                             \
+            set_err(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x);       \
+            return 2;                                                                       \
+        }                                                                                   \
+    } while (0)
+
+static constexpr int TB = 256;
+static inline int nb(int n) { return (n + TB - 1) / TB; }
+
+// ghost cells: copies of the cell they mirror or wrap to
+// (BoundaryCondition.cpp ApplyEdge / ApplyCorners for Symmetry, Periodic)
+__global__ void k_ghosts(Geo g, double *v) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.nxp * g.nyp) return;
+    const int ix = p % g.nxp, iy = p / g.nxp;
+    const bool inx = ix >= 2 && ix <= g.nxp - 3;
+    const bool iny = g.dim == 1 || (iy >= 2 && iy <= g.nyp - 3);
+    if (inx && iny) return;
+    const int sx = fold_axis(ix - 2, g.nx, g.bc[0] == YZ_BC_PERIODIC ? 1 : 0) + 2;
+    const int sy = g.dim == 2 ? fold_axis(iy - 2, g.ny, g.bc[2] == YZ_BC_PERIODIC ? 1 : 0) + 2 : 0;
+    v[p] = v[sy * g.nxp + sx];
+}
+
+// h += alpha * corr on the interior cells
+__global__ void k_update(Geo g, double *h, const double *corr, double alpha) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.N) return;
+    const int ix = i % g.nx, iy = i / g.nx;
+    const int p = (g.dim == 2 ? (iy + 2) : 0) * g.nxp + ix + 2;
+    h[p] += corr[i] * alpha;
+}
+
+__global__ void k_absmax(int n, const double *v, unsigned long long *out) {
+    double m = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = fmax(m, fabs(v[i]));
+    __shared__ double sh[TB / 64];
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0;
+        for (int w = 0; w < TB / 64; ++w) mx = fmax(mx, sh[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+    }
+}
+
+// extrapolated start g = h_n + r (h_n - h_n-1) over every row; flags a real
+// cell that had liquid and would have none (NewtonKantorovich.cpp:178-198)
+__global__ void k_extrap(Geo g, const double *hn, const double *hn1, double r, double *out,
+                         unsigned long long *move_bits, int *bad) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0;
+    if (p < g.nxp * g.nyp) {
+        const double v = hn[p] + r * (hn[p] - hn1[p]);
+        out[p] = v;
+        const int ix = p % g.nxp, iy = p / g.nxp;
+        const bool inside = ix >= 2 && ix <= g.nxp - 3 && (g.dim == 1 || (iy >= 2 && iy <= g.nyp - 3));
+        if (inside && hn[p] > 0 && !(v > 0)) atomicOr(bad, 1);
+        m = fabs(v - hn[p]);
+    }
+    __shared__ double sh[TB / 64];
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0;
+        for (int w = 0; w < TB / 64; ++w) mx = fmax(mx, sh[w]);
+        atomicMax(move_bits, (unsigned long long)__double_as_longlong(mx));
+    }
+}
+
+// interior cells of a padded array -> packed N
+__global__ void k_pack(Geo g, const double *pad, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.N) return;
+    const int ix = i % g.nx, iy = i / g.nx;
+    out[i] = pad[(g.dim == 2 ? iy + 2 : 0) * g.nxp + ix + 2];
+}
+
+__global__ void k_truncation(Geo g, const double *hm, const double *ho, const double *hp, double dt0, double dt,
+                             double wfloor, unsigned long long *out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (p < g.nxp * g.nyp) {
+        double w = fabs(ho[p]);
+        if (w < wfloor) w = wfloor;
+        const double den = (dt0 + dt) * w;
+        e = (den == 0) ? 0.0 : fabs(2.0 * (dt / dt0) * (dt0 * hp[p] + dt * hm[p] - (dt0 + dt) * ho[p]) / den);
+    }
+    __shared__ double sh[TB / 64];
+    for (int off = 32; off > 0; off >>= 1) e = fmax(e, __shfl_down(e, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0;
+        for (int w = 0; w < TB / 64; ++w) mx = fmax(mx, sh[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+    }
+}
+
+static double bits_to_double(unsigned long long b) {
+    double d;
+    std::memcpy(&d, &b, sizeof(d));
+    return d;
+}
+
+}  // namespace yz
+
+using namespace yz;
+
+struct yz_step {
+    yz_case cs;
+    Geo g;
+    RowArgs ra;
+    hipStream_t st = nullptr;
+    double *h = nullptr, *hold = nullptr, *hold2 = nullptr, *hnew = nullptr, *s = nullptr, *c = nullptr;
+    double *axold = nullptr, *b = nullptr, *corr = nullptr, *scratch = nullptr;
+    unsigned long long *dmax = nullptr;   // [0] rhs, [1] corr, [2] move, [3] trunc
+    int *dflag = nullptr;
+    unsigned long long *hmax = nullptr;   // pinned mirror
+    int *hflag = nullptr;
+    Solver S;
+    double time = 0, dt_now = 0, dt_old = 0;
+    int tsc_count = 0;
+    bool extrap_blocked = false, first_step = true;
+    double extrap_move = 0, ew_prev = 0, ew_eta = 0;
+    int nk_pass = 0;
+    long long lin_step = 0;
+    hipEvent_t ev[12];
+    double tms[6];
+};
+
+static int check_case(const yz_case &c) {
+    if (c.dim != 1 && c.dim != 2) { set_err("dim must be 1 or 2"); return 1; }
+    if (c.nx < 3 || (c.dim == 2 && c.ny < 3)) { set_err("grid too small"); return 1; }
+    const int nb_ = c.dim == 2 ? 4 : 2;
+    for (int k = 0; k < nb_; ++k)
+        if (c.bc[k] != YZ_BC_SYMMETRY && c.bc[k] != YZ_BC_PERIODIC) {
+            set_err("only Symmetry and Periodic edges are implemented"); return 1;
+        }
+    if ((c.bc[0] == YZ_BC_PERIODIC) != (c.bc[1] == YZ_BC_PERIODIC) ||
+        (c.dim == 2 && (c.bc[2] == YZ_BC_PERIODIC) != (c.bc[3] == YZ_BC_PERIODIC))) {
+        set_err("a Periodic edge needs a Periodic opposite edge"); return 1;
+    }
+    if (c.jac_interval != 1) { set_err("JacobianUpdateInterval other than 1 is not implemented"); return 1; }
+    if (c.precond != YZ_PC_AMG && c.precond != YZ_PC_JACOBI) { set_err("unknown preconditioner"); return 1; }
+    if (c.krylov_fp32) { set_err("the FP32 Krylov path is not built in this library"); return 1; }
+    return 0;
+}
+
+static void build_geo(const yz_case &c, Geo &g) {
+    std::memset(&g, 0, sizeof(g));
+    g.dim = c.dim; g.nx = c.nx; g.ny = c.dim == 2 ? c.ny : 1;
+    g.nxp = g.nx + 4; g.nyp = c.dim == 2 ? g.ny + 4 : 1;
+    g.N = g.nx * g.ny;
+    g.ns = c.dim == 2 ? 13 : 5; g.center = c.dim == 2 ? 6 : 2; g.ndir = c.dim == 2 ? 4 : 2;
+    const double dx0 = (c.xmax - c.xmin) / (double)c.nx;
+    g.xlo = c.xmin - 2 * dx0;
+    g.dx = ((c.xmax + 2 * dx0) - g.xlo) / g.nxp;
+    if (c.dim == 2) {
+        const double dy0 = (c.ymax - c.ymin) / (double)c.ny;
+        g.ylo = c.ymin - 2 * dy0;
+        g.dy = ((c.ymax + 2 * dy0) - g.ylo) / g.nyp;
+    } else { g.ylo = 0; g.dy = 1; }
+    for (int k = 0; k < 4; ++k) g.bc[k] = c.bc[k];
+}
+
+// capillary C(d) (LaplacePressure_2D.cpp:39-94 / LaplacePressure_1D.cpp:35-77)
+static void build_ccap(const Geo &g, double C[4][13]) {
+    std::memset(C, 0, sizeof(double) * 52);
+    if (g.dim == 2) {
+        const double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy, dx4 = dx2 * dx2, dy4 = dy2 * dy2, dxy = dx2 * dy2;
+        C[0][S_mm] = +1.0 / dxy; C[0][S_om] = -1.0 / dxy; C[0][S_Mo] = +1.0 / dx4;
+        C[0][S_mo] = 1.0 * (-3.0 / dx4 - 2.0 / dxy); C[0][S_oo] = 1.0 * (+3.0 / dx4 + 2.0 / dxy);
+        C[0][S_po] = -1.0 / dx4; C[0][S_mp] = +1.0 / dxy; C[0][S_op] = -1.0 / dxy;
+        C[1][S_om] = -1.0 / dxy; C[1][S_pm] = +1.0 / dxy; C[1][S_mo] = -1.0 / dx4;
+        C[1][S_oo] = 1.0 * (+3.0 / dx4 + 2.0 / dxy); C[1][S_po] = 1.0 * (-3.0 / dx4 - 2.0 / dxy);
+        C[1][S_Po] = +1.0 / dx4; C[1][S_op] = -1.0 / dxy; C[1][S_pp] = +1.0 / dxy;
+        C[2][S_oM] = +1.0 / dy4; C[2][S_mm] = +1.0 / dxy; C[2][S_om] = 1.0 * (-3.0 / dy4 - 2.0 / dxy);
+        C[2][S_pm] = +1.0 / dxy; C[2][S_mo] = -1.0 / dxy; C[2][S_oo] = 1.0 * (+3.0 / dy4 + 2.0 / dxy);
+        C[2][S_po] = -1.0 / dxy; C[2][S_op] = -1.0 / dy4;
+        C[3][S_om] = -1.0 / dy4; C[3][S_mo] = -1.0 / dxy; C[3][S_oo] = 1.0 * (+3.0 / dy4 + 2.0 / dxy);
+        C[3][S_po] = -1.0 / dxy; C[3][S_mp] = +1.0 / dxy; C[3][S_op] = 1.0 * (-3.0 / dy4 - 2.0 / dxy);
+        C[3][S_pp] = +1.0 / dxy; C[3][S_oP] = +1.0 / dy4;
+    } else {
+        const double dx2 = g.dx * g.dx, dx4 = dx2 * dx2;
+        C[0][0] = +1.0 / dx4; C[0][1] = 1.0 * (-3.0 / dx4); C[0][2] = 1.0 * (+3.0 / dx4); C[0][3] = -1.0 / dx4;
+        C[1][1] = -1.0 / dx4; C[1][2] = 1.0 * (+3.0 / dx4); C[1][3] = 1.0 * (-3.0 / dx4); C[1][4] = +1.0 / dx4;
+    }
+}
+
+static StencilOp fine_op(const Geo &g) {
+    StencilOp op;
+    std::memset(&op, 0, sizeof(op));
+    op.dim = g.dim; op.nx = g.nx; op.ny = g.ny; op.N = g.N; op.ns = g.ns;
+    op.perx = g.bc[0] == YZ_BC_PERIODIC; op.pery = g.dim == 2 && g.bc[2] == YZ_BC_PERIODIC;
+    for (int k = 0; k < g.ns; ++k) { op.ox[k] = slot_ox(g.dim, k); op.oy[k] = slot_oy(g.dim, k); }
+    return op;
+}
+
+static void rows_launch(yz_step *S, int mode, const double *h) {
+    RowArgs a = S->ra;
+    a.h = h;
+    const dim3 grid(nb(S->g.N)), blk(TB);
+    if (S->g.dim == 2) {
+        if (mode == 0) hipLaunchKernelGGL((k_rows<2, 0>), grid, blk, 0, S->st, a);
+        else hipLaunchKernelGGL((k_rows<2, 1>), grid, blk, 0, S->st, a);
+    } else {
+        if (mode == 0) hipLaunchKernelGGL((k_rows<1, 0>), grid, blk, 0, S->st, a);
+        else hipLaunchKernelGGL((k_rows<1, 1>), grid, blk, 0, S->st, a);
+    }
+}
+
+static int sync_read(yz_step *S) {
+    YZ_TRY(hipMemcpyAsync(S->hmax, S->dmax, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(S->hflag, S->dflag, sizeof(int), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+static double ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
+
+// Solve (NewtonKantorovich.cpp:74-154): assemble, EW tolerance, solve
+static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
+    *dev_err = 0;
+    const yz_case &c = S->cs;
+    S->ra.dt = S->dt_now;
+    S->ra.axold = S->axold;
+    S->ra.hold = S->hold;
+    YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+    YZ_TRY(hipEventRecord(S->ev[0], S->st));
+    rows_launch(S, 1, S->h);
+    YZ_TRY(hipGetLastError());
+    YZ_TRY(hipEventRecord(S->ev[1], S->st));
+    ++S->nk_pass;
+    if (sync_read(S)) { *dev_err = 1; return 2; }
+    S->tms[0] += ev_ms(S->ev[0], S->ev[1]);
+    double eps = c.lin_eps;
+    if (c.ew) {
+        const double kEtaMax = 1.0e-4;
+        const double rn = bits_to_double(S->hmax[0]);
+        double eta = kEtaMax;
+        if (S->nk_pass > 1 && S->ew_prev > 0) {
+            const double ratio = rn / S->ew_prev;
+            eta = 0.9 * ratio * ratio;
+            const double safe = 0.9 * S->ew_eta * S->ew_eta;
+            if (safe > 0.1 && safe > eta) eta = safe;
+        }
+        if (eta > kEtaMax) eta = kEtaMax;
+        if (eta < c.lin_eps) eta = c.lin_eps;
+        S->ew_eta = eta; S->ew_prev = rn;
+        eps = eta;
+    }
+    YZ_TRY(hipEventRecord(S->ev[2], S->st));
+    int rc;
+    if (c.precond == YZ_PC_AMG) rc = amg_setup(S->S); else rc = jacobi_setup(S->S);
+    if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
+    YZ_TRY(hipEventRecord(S->ev[3], S->st));
+    const int ch = bicgstab(S->S, S->b, S->corr, c.precond, c.lin_maxit, eps, iter, resid);
+    YZ_TRY(hipEventRecord(S->ev[4], S->st));
+    YZ_TRY(hipEventSynchronize(S->ev[4]));
+    S->tms[2] += ev_ms(S->ev[2], S->ev[3]);
+    S->tms[3] += ev_ms(S->ev[3], S->ev[4]);
+    if (ch != 0) YZ_TRY(hipMemsetAsync(S->corr, 0, S->g.N * sizeof(double), S->st));
+    return ch;
+}
+
+// NKLoop (NewtonKantorovich.cpp:157-422)
+static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out) {
+    const yz_case &c = S->cs;
+    const Geo &g = S->g;
+    const int np = g.nxp * g.nyp;
+    S->nk_pass = 0;
+    S->lin_step = 0;
+    S->extrap_move = 0;
+    bool guessed = false;
+    if (c.extrap && !S->extrap_blocked && !S->first_step) {
+        const double r = (S->dt_old > 0) ? S->dt_now / S->dt_old : 0.0;
+        if (r > 0) {
+            YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+            YZ_TRY(hipMemsetAsync(S->dflag, 0, sizeof(int), S->st));
+            hipLaunchKernelGGL(k_extrap, dim3(nb(np)), dim3(TB), 0, S->st, g, S->hold, S->hold2, r, S->h,
+                               S->dmax + 2, S->dflag);
+            if (sync_read(S)) return 2;
+            const double move = bits_to_double(S->hmax[2]);
+            if (!S->hflag[0] && move > 0) { S->extrap_move = move; guessed = true; }
+        }
+    }
+    if (!guessed) YZ_TRY(hipMemcpyAsync(S->h, S->hold, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    // BeforeNKLoop + the explicit half A(h^n) h^n
+    {
+        RowArgs a = S->ra;
+        a.ax = S->axold;
+        RowArgs keep = S->ra;
+        S->ra = a;
+        rows_launch(S, 0, S->hold);
+        S->ra = keep;
+        YZ_TRY(hipGetLastError());
+    }
+    double resid_eq_old = 1e40, resid_first = 0, resid_best = 1e40, resid_eq = 0;
+    int stall = 0;
+    const int stall_limit = 5;
+    bool done = false;
+    int outer;
+    for (outer = 0; outer < c.nk_max; ++outer) {
+        int it = 0, dev_err = 0;
+        double lr = 0;
+        const int ch = nk_solve(S, &it, &lr, &dev_err);
+        if (dev_err) return 2;
+        S->lin_step += it;
+        YZ_TRY(hipMemsetAsync(S->dmax + 1, 0, sizeof(unsigned long long), S->st));
+        hipLaunchKernelGGL(k_absmax, dim3(std::min(nb(g.N), 1024)), dim3(TB), 0, S->st, g.N, S->corr, S->dmax + 1);
+        if (sync_read(S)) return 2;
+        resid_eq = bits_to_double(S->hmax[1]);
+        if (std::getenv("YZ_DEBUG"))
+            std::fprintf(stderr, "[hip] pass %d lin %d resid_lin %.3e corr %.3e rc %d\n", outer, it, lr, resid_eq, ch);
+        if (ch != 0) { done = false; break; }
+        if (outer == 0) resid_first = resid_eq;
+        if (resid_eq < resid_best * 0.9) { resid_best = resid_eq; stall = 0; }
+        else stall++;
+        if (stall > stall_limit) {
+            // floor 1000 eps |h|: the guess's max over every row
+            YZ_TRY(hipMemsetAsync(S->dmax + 3, 0, sizeof(unsigned long long), S->st));
+            hipLaunchKernelGGL(k_absmax, dim3(std::min(nb(np), 1024)), dim3(TB), 0, S->st, np, S->h, S->dmax + 3);
+            if (sync_read(S)) return 2;
+            const double floorPhi = 1000.0 * std::numeric_limits<double>::epsilon() * bits_to_double(S->hmax[3]);
+            done = (resid_eq < c.nk_eps) || (resid_first > 0 && resid_eq < resid_first * 1.0e-4) || (resid_eq <= floorPhi);
+            break;
+        }
+        const double alpha = (resid_eq > resid_eq_old) ? 0.5 : 1.0;
+        YZ_TRY(hipEventRecord(S->ev[5], S->st));
+        hipLaunchKernelGGL(k_update, dim3(nb(g.N)), dim3(TB), 0, S->st, g, S->h, S->corr, alpha);
+        hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, S->h);
+        YZ_TRY(hipEventRecord(S->ev[6], S->st));
+        if (outer >= c.nk_min - 1) {
+            if (resid_eq < c.nk_eps) { done = true; break; }
+            if (c.nk_eps_rel > 0 && resid_first > 0 && resid_eq < resid_first * c.nk_eps_rel) { done = true; break; }
+        }
+        resid_eq_old = resid_eq;
+    }
+    *iter_out = outer + 1;
+    *resid_out = resid_eq;
+    *done_out = done;
+    return 0;
+}
+
+extern "C" {
+
+void yz_case_defaults(yz_case *c) {
+    std::memset(c, 0, sizeof(*c));
+    c->dim = 2; c->nx = c->ny = 16;
+    c->xmin = 0; c->xmax = 1; c->ymin = 0; c->ymax = 1;
+    c->capillary = 1;
+    c->dis_n = 3; c->dis_m = 2; c->hstar = 0.01;
+    c->visc_c0 = 0.5; c->sig_c0 = 0.5;
+    c->As = 1;
+    c->theta = 0.5;
+    c->nk_max = 30; c->nk_min = 1; c->jac_interval = 1;
+    c->nk_eps = 1e-10; c->nk_eps_rel = 1e-6;
+    c->ew = 1; c->extrap = 1;
+    c->lin_maxit = 5000; c->lin_eps = 1e-13; c->row_scaling = 1; c->precond = YZ_PC_AMG;
+    c->tsc_change = 0.5; c->tsc_criterion = 0.01; c->tsc_floor = 0; c->tsc_keep = 50;
+    c->dt_min = 1e-15; c->dt_max = 1e30;
+}
+
+const char *yz_last_error(void) { return g_err.c_str(); }
+int yz_sizeof_case(void) { return (int)sizeof(yz_case); }
+int yz_sizeof_lin_settings(void) { return (int)sizeof(yz_lin_settings); }
+
+yz_step *yz_step_create(const yz_case *c, const double *h, const double *s, const double *conc, double dt0) {
+    if (!c || !h || !s || !conc) { set_err("null argument"); return nullptr; }
+    if (check_case(*c)) return nullptr;
+    yz_step *S = new yz_step();
+    S->cs = *c;
+    build_geo(*c, S->g);
+    const Geo &g = S->g;
+    const int np = g.nxp * g.nyp;
+    auto fail = [&](const char *what) { set_err(what); yz_step_destroy(S); return (yz_step *)nullptr; };
+    if (hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+    double **pads[] = {&S->h, &S->hold, &S->hold2, &S->hnew, &S->s, &S->c};
+    for (double **p : pads) if (hipMalloc((void **)p, np * sizeof(double)) != hipSuccess) return fail("hipMalloc");
+    double **vecs[] = {&S->axold, &S->b, &S->corr, &S->scratch};
+    for (double **p : vecs) if (hipMalloc((void **)p, g.N * sizeof(double)) != hipSuccess) return fail("hipMalloc");
+    if (hipMalloc((void **)&S->dmax, 4 * sizeof(unsigned long long)) != hipSuccess) return fail("hipMalloc");
+    if (hipMalloc((void **)&S->dflag, sizeof(int)) != hipSuccess) return fail("hipMalloc");
+    if (hipHostMalloc((void **)&S->hmax, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) return fail("hipHostMalloc");
+    if (hipHostMalloc((void **)&S->hflag, sizeof(int), hipHostMallocDefault) != hipSuccess) return fail("hipHostMalloc");
+    for (auto &e : S->ev) if (hipEventCreate(&e) != hipSuccess) return fail("event");
+    if (hipMemcpy(S->hnew, h, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return fail("copy");
+    if (hipMemcpy(S->s, s, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return fail("copy");
+    if (hipMemcpy(S->c, conc, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return fail("copy");
+    for (double *p : {S->hnew, S->s, S->c}) hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, p);
+    hipMemcpyAsync(S->hold, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st);
+    hipMemcpyAsync(S->hold2, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st);
+    hipMemcpyAsync(S->h, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st);
+    hipMemsetAsync(S->corr, 0, g.N * sizeof(double), S->st);
+    if (hipStreamSynchronize(S->st) != hipSuccess) return fail("init");
+    if (solver_init(S->S, fine_op(g), S->st)) return fail("solver allocation");
+    // row kernel arguments
+    RowArgs &a = S->ra;
+    std::memset(&a, 0, sizeof(a));
+    a.g = g;
+    a.cap = c->capillary; a.vg = c->vertical_gravity; a.pg = c->inplane_gravity; a.cf = c->centrifugal;
+    a.ma = c->marangoni; a.dj = c->disjoining;
+    a.Bo = c->Bo; a.Pgx = c->Pgx; a.Pgy = c->Pgy; a.Sp = c->Sp; a.AsMa2 = c->As * c->Ma / 2.0; a.Dp = c->Dp;
+    a.hstar = c->hstar; a.pn = c->dis_n; a.pm = c->dis_m; a.rotx = c->rot_x; a.roty = c->rot_y;
+    a.visc_c0 = c->visc_c0; a.visc_exp = c->visc_exp; a.sig_c0 = c->sig_c0;
+    a.Ma_eff = c->marangoni ? c->Ma : 0.0;
+    a.mob.lambda = c->slip > 0 ? c->slip : 0.0; a.mob.a = 3.0 * a.mob.lambda; a.mob.slip = c->slip > 0;
+    build_ccap(g, a.Ccap);
+    a.theta = c->theta;
+    a.row_scaling = c->row_scaling;
+    a.s = S->s; a.c = S->c;
+    a.ia = S->S.A; a.b = S->b; a.rmax = S->dmax;
+    a.ax = S->axold;
+    S->dt_now = dt0; S->dt_old = 0;
+    return S;
+}
+
+void yz_step_destroy(yz_step *S) {
+    if (!S) return;
+    solver_free(S->S);
+    for (double *p : {S->h, S->hold, S->hold2, S->hnew, S->s, S->c, S->axold, S->b, S->corr, S->scratch})
+        if (p) (void)hipFree(p);
+    if (S->dmax) (void)hipFree(S->dmax);
+    if (S->dflag) (void)hipFree(S->dflag);
+    if (S->hmax) (void)hipHostFree(S->hmax);
+    if (S->hflag) (void)hipHostFree(S->hflag);
+    for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
+    if (S->st) (void)hipStreamDestroy(S->st);
+    delete S;
+}
+
+// TimeAdvance (NewtonKantorovich.cpp:424-486) + TimeStepAdjustment
+// (ImplicitSolver.cpp:144-190)
+int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
+    if (!S) { set_err("null step"); return 2; }
+    const yz_case &c = S->cs;
+    const Geo &g = S->g;
+    const int np = g.nxp * g.nyp;
+    for (double &t : S->tms) t = 0;
+    YZ_TRY(hipEventRecord(S->ev[10], S->st));
+    YZ_TRY(hipMemcpyAsync(S->hold2, S->hold, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    YZ_TRY(hipMemcpyAsync(S->hold, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    bool step_done = false, ok = false;
+    double dt_request = S->dt_now;
+    long long lin_sum = 0;
+    int it = 0;
+    double r = 0;
+    do {
+        if (nk_loop(S, &it, &r, &ok)) return 2;
+        lin_sum += S->lin_step;
+        if (!ok && S->extrap_move > 0) {
+            S->extrap_blocked = true;
+            const int e = nk_loop(S, &it, &r, &ok);
+            S->extrap_blocked = false;
+            if (e) return 2;
+            lin_sum += S->lin_step;
+        }
+        if (c.step_control) {
+            double crit = -1.0;
+            if (ok) {
+                if (S->dt_old == 0) crit = 1.0;
+                else {
+                    double th = c.tsc_criterion;
+                    if (th == 0) th = 1.0e-2;
+                    YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+                    hipLaunchKernelGGL(k_absmax, dim3(std::min(nb(np), 1024)), dim3(TB), 0, S->st, np, S->hold, S->dmax + 2);
+                    if (sync_read(S)) return 2;
+                    const double wfloor = c.tsc_floor * bits_to_double(S->hmax[2]);
+                    hipLaunchKernelGGL(k_truncation, dim3(nb(np)), dim3(TB), 0, S->st, g, S->hold2, S->hold, S->h,
+                                       S->dt_old, S->dt_now, wfloor, S->dmax + 3);
+                    if (sync_read(S)) return 2;
+                    crit = 1.0 - bits_to_double(S->hmax[3]) / th;
+                }
+            }
+            if (crit > 0.0) {
+                step_done = true;
+                if (crit > 0.90 && S->dt_now < c.dt_max) {
+                    if (++S->tsc_count >= c.tsc_keep) {
+                        S->tsc_count = 0;
+                        dt_request = std::min(S->dt_now / c.tsc_change, c.dt_max);
+                    }
+                }
+            } else {
+                S->dt_now *= c.tsc_change;
+                dt_request = S->dt_now;
+                if (S->dt_now < c.dt_min) { set_err("step fell below MinDt"); return 1; }
+            }
+        } else {
+            if (!ok) {
+                set_err("the Newton iteration did not converge at the fixed step");
+                if (nk_iter) *nk_iter = it;
+                if (lin_iter) *lin_iter = (int)lin_sum;
+                if (resid) *resid = r;
+                return 1;
+            }
+            step_done = true;
+        }
+    } while (!step_done);
+    YZ_TRY(hipMemcpyAsync(S->hnew, S->h, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    YZ_TRY(hipEventRecord(S->ev[11], S->st));
+    YZ_TRY(hipEventSynchronize(S->ev[11]));
+    S->tms[5] = ev_ms(S->ev[10], S->ev[11]);
+    S->time += S->dt_now;
+    S->dt_old = S->dt_now;
+    S->dt_now = dt_request;
+    S->first_step = false;
+    if (nk_iter) *nk_iter = it;
+    if (lin_iter) *lin_iter = (int)lin_sum;
+    if (resid) *resid = r;
+    return 0;
+}
+
+int yz_step_get_h(yz_step *S, double *h) {
+    if (!S || !h) { set_err("null argument"); return 2; }
+    const int np = S->g.nxp * S->g.nyp;
+    YZ_TRY(hipMemcpyAsync(h, S->hnew, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+int yz_step_set_h(yz_step *S, const double *h) {
+    if (!S || !h) { set_err("null argument"); return 2; }
+    const Geo &g = S->g;
+    const int np = g.nxp * g.nyp;
+    YZ_TRY(hipMemcpyAsync(S->hnew, h, np * sizeof(double), hipMemcpyHostToDevice, S->st));
+    hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, S->hnew);
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+int yz_step_clock(yz_step *S, double *t, double *dt_now, double *dt_old) {
+    if (!S) { set_err("null step"); return 2; }
+    if (t) *t = S->time;
+    if (dt_now) *dt_now = S->dt_now;
+    if (dt_old) *dt_old = S->dt_old;
+    return 0;
+}
+
+int yz_step_timing(yz_step *S, double *ms6) {
+    if (!S || !ms6) { set_err("null argument"); return 2; }
+    for (int k = 0; k < 6; ++k) ms6[k] = S->tms[k];
+    return 0;
+}
+
+int yz_step_assemble(yz_step *S, const double *h_old, const double *h, double dt, double *ia, double *ax,
+                     double *rhs) {
+    if (!S || !h_old || !h) { set_err("null argument"); return 2; }
+    const Geo &g = S->g;
+    const int np = g.nxp * g.nyp;
+    double *dia = nullptr, *dax = nullptr, *drhs = nullptr;
+    YZ_TRY(hipMalloc((void **)&dia, (size_t)g.ns * g.N * sizeof(double)));
+    YZ_TRY(hipMalloc((void **)&dax, g.N * sizeof(double)));
+    YZ_TRY(hipMalloc((void **)&drhs, g.N * sizeof(double)));
+    YZ_TRY(hipMemcpyAsync(S->hold, h_old, np * sizeof(double), hipMemcpyHostToDevice, S->st));
+    YZ_TRY(hipMemcpyAsync(S->h, h, np * sizeof(double), hipMemcpyHostToDevice, S->st));
+    hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, S->hold);
+    hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, S->h);
+    RowArgs keep = S->ra;
+    S->ra.ax = S->axold;
+    rows_launch(S, 0, S->hold);
+    S->ra = keep;
+    S->ra.dt = dt; S->ra.axold = S->axold; S->ra.hold = S->hold;
+    S->ra.ia_raw = dia; S->ra.ax_raw = dax; S->ra.rhs_raw = drhs;
+    YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+    rows_launch(S, 1, S->h);
+    S->ra = keep;
+    YZ_TRY(hipGetLastError());
+    std::vector<double> tax(g.N), trhs(g.N);
+    if (ia) YZ_TRY(hipMemcpyAsync(ia, dia, (size_t)g.ns * g.N * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(tax.data(), dax, g.N * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(trhs.data(), drhs, g.N * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    (void)hipFree(dia); (void)hipFree(dax); (void)hipFree(drhs);
+    for (int i = 0; i < g.N; ++i) {
+        const int ix = i % g.nx, iy = i / g.nx;
+        const int p = (g.dim == 2 ? iy + 2 : 0) * g.nxp + ix + 2;
+        if (ax) ax[p] = tax[i];
+        if (rhs) rhs[p] = trhs[i];
+    }
+    if (ax || rhs)
+        for (int p = 0; p < np; ++p) {
+            const int ix = p % g.nxp, iy = p / g.nxp;
+            const bool inside = ix >= 2 && ix <= g.nxp - 3 && (g.dim == 1 || (iy >= 2 && iy <= g.nyp - 3));
+            if (inside) continue;
+            if (ax) ax[p] = 0;
+            if (rhs) rhs[p] = 0;
+        }
+    // the step's own state is left as it was before the call
+    YZ_TRY(hipMemcpyAsync(S->hold, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    YZ_TRY(hipMemcpyAsync(S->h, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+"""Python handle on the device step (ctypes over libyuzen_hip.so).
+
+``HipStep`` is the film-height equation of one case resident on the GPU: it
+is created from a :class:`yuzen_amd.capi.YzCase` and the padded fields,
+advances one implicit step per ``advance()`` (the reference's
+``C_NewtonKantorovich::TimeAdvance`` contract: 0 accepted, 1 failed at a
+fixed step, 2 device error) and hands back the film. ``linsolve`` is the
+one-system seam with the contract of ``ILinearSolverBackend::Solve``.
+Nothing here computes: every call goes to the HIP library, and a missing
+library raises.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _pi(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class HipStep:
+    def __init__(self, case, h, s, c, dt0):
+        self.lib = capi.load()
+        self.case = case.copy()
+        self.dim = case.dim
+        nyp = case.ny + 4 if case.dim == 2 else 1
+        self.n = (case.nx + 4) * nyp
+        self.N = case.nx * (case.ny if case.dim == 2 else 1)
+        self.ns = 13 if case.dim == 2 else 5
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        assert h.size == self.n and s.size == self.n and c.size == self.n
+        self.p = self.lib.yz_step_create(C.byref(self.case), _pd(h), _pd(s), _pd(c), dt0)
+        if not self.p:
+            raise RuntimeError("yz_step_create: " + capi.last_error())
+
+    def close(self):
+        if getattr(self, "p", None):
+            self.lib.yz_step_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
+
+    def advance(self):
+        nk, lin, r = C.c_int(), C.c_int(), C.c_double()
+        rc = self.lib.yz_step_advance(self.p, C.byref(nk), C.byref(lin), C.byref(r))
+        if rc == 2:
+            raise RuntimeError("yz_step_advance: " + capi.last_error())
+        return rc, nk.value, lin.value, r.value
+
+    def h(self):
+        out = np.empty(self.n)
+        if self.lib.yz_step_get_h(self.p, _pd(out)):
+            raise RuntimeError(capi.last_error())
+        return out
+
+    def set_h(self, h):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        if self.lib.yz_step_set_h(self.p, _pd(h)):
+            raise RuntimeError(capi.last_error())
+
+    def clock(self):
+        t, a, b = C.c_double(), C.c_double(), C.c_double()
+        self.lib.yz_step_clock(self.p, C.byref(t), C.byref(a), C.byref(b))
+        return t.value, a.value, b.value
+
+    def timing(self):
+        out = np.zeros(6)
+        self.lib.yz_step_timing(self.p, _pd(out))
+        return out
+
+    def assemble(self, h_old, h, dt):
+        ia = np.empty(self.ns * self.N)
+        ax = np.empty(self.n)
+        rhs = np.empty(self.n)
+        h_old = np.ascontiguousarray(h_old, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        if self.lib.yz_step_assemble(self.p, _pd(h_old), _pd(h), dt, _pd(ia), _pd(ax), _pd(rhs)):
+            raise RuntimeError(capi.last_error())
+        return ia.reshape(self.ns, self.N), ax, rhs
+
+
+class LinSolver:
+    """The linear-solve seam (one CRS system per call)."""
+
+    def __init__(self):
+        self.lib = capi.load()
+        self.p = self.lib.yz_linsolve_create()
+        if not self.p:
+            raise RuntimeError("yz_linsolve_create failed")
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.lib.yz_linsolve_destroy(self.p)
+            self.p = None
+
+    def solve(self, rowptr, col, val, rhs, x0, precond=capi.PC_AMG, scaling=1, max_iter=5000,
+              epsilon=1e-13, grid=None):
+        s = capi.YzLinSettings()
+        s.precond, s.scaling, s.max_iter, s.epsilon = precond, scaling, max_iter, epsilon
+        if grid is not None:
+            s.grid_dim, s.grid_nx, s.grid_ny = grid[0], grid[1], grid[2]
+            for k in range(4):
+                s.bc[k] = grid[3][k]
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.array(x0, dtype=np.float64)
+        it, r = C.c_int(), C.c_double()
+        rc = self.lib.yz_linsolve(self.p, len(rowptr) - 1, len(col), _pi(rowptr), _pi(col), _pd(val),
+                                  _pd(rhs), _pd(x), C.byref(s), C.byref(it), C.byref(r))
+        return rc, x, it.value, r.value
