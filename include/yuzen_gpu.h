@@ -124,6 +124,12 @@ int yz_step_clock(yz_step *st, double *time, double *dt_now, double *dt_old);
  * [0] assembly, [1] residual, [2] AMG setup, [3] Krylov, [4] update,
  * [5] whole step */
 int yz_step_timing(yz_step *st, double *ms6);
+/* launch timing of the fine-level stencil product of the Krylov iteration
+ * (the SpMV v = A p with its fused dot product), HIP events on the step's
+ * stream: returns the launches timed and their total milliseconds since
+ * the last reset; enable 1 resets and starts, 0 resets and stops, -1 only
+ * reads */
+int yz_step_kernel_time(yz_step *st, int enable, long long *count, double *total_ms);
 void yz_step_destroy(yz_step *st);
 
 /* Assembly-level access (parity tests): at film h (n doubles, padded),

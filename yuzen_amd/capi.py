@@ -105,6 +105,7 @@ def load():
     lib.yz_step_clock.argtypes = [C.c_void_p, pd, pd, pd]
     lib.yz_step_timing.argtypes = [C.c_void_p, pd]
     lib.yz_step_destroy.argtypes = [C.c_void_p]
+    lib.yz_step_kernel_time.argtypes = [C.c_void_p, i, P(C.c_longlong), pd]
     lib.yz_step_assemble.argtypes = [C.c_void_p, pd, pd, d, pd, pd, pd]
     lib.yz_linsolve_create.restype = C.c_void_p
     lib.yz_linsolve_destroy.argtypes = [C.c_void_p]

@@ -119,6 +119,31 @@ struct Mob {
         const double dg = Gint(h2) - Gint(h1);
         return (-dg + dl / D(h1)) / (dg * dg);
     }
+    // FaceDK(h1, h2) with both derivatives, dF/dh1 and dF/dh2, from one
+    // evaluation of g at each height (the same expressions as FaceDK,
+    // FaceDK_d1(h1, h2) and FaceDK_d1(h2, h1))
+    __device__ void FaceDK3(double h1, double h2, double &F, double &F1, double &F2) const {
+        if (!(h1 > 0) || !(h2 > 0)) { F = F1 = F2 = 0; return; }
+        const double hm = 0.5 * (h1 + h2), dl = h2 - h1;
+        if (fabs(dl) < 1e-3 * hm) {
+            const double Dm = D(hm), dDm = dD(hm), k = K(hm), dk = dK(hm);
+            const double q = 1.0 + dl * dl * k / 24.0;
+            F = Dm / q;
+            const double Fhm = (dDm * q - Dm * dl * dl * dk / 24.0) / (q * q);
+            const double Fdl = -Dm * (dl * k / 12.0) / (q * q);
+            F1 = 0.5 * Fhm - Fdl;
+            // the same at (h2, h1): hm unchanged, dl -> -dl
+            const double Fdl2 = -Dm * ((-dl) * k / 12.0) / (q * q);
+            F2 = 0.5 * Fhm - Fdl2;
+            return;
+        }
+        const double g1 = Gint(h1), g2 = Gint(h2);
+        const double dg = g2 - g1;
+        F = dl / dg;
+        F1 = (-dg + dl / D(h1)) / (dg * dg);
+        const double dg2 = g1 - g2, dl2 = h1 - h2;
+        F2 = (-dg2 + dl2 / D(h2)) / (dg2 * dg2);
+    }
     // Mobility::Face / Face_d1 (0 when a side is dry)
     __device__ double Face(double h1, double h2) const {
         if (slip) return FaceDK(h1, h2);

@@ -103,6 +103,21 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
             return iyp == g.nyp - 3 && is_wall(g.bc[3]);
         };
 
+        // per face: viscosity and surface tension at the mean concentration
+        // of the two cells (every term takes them there), and the slip
+        // mobility with both derivatives, evaluated once and shared by the
+        // capillary, vertical-gravity and disjoining terms
+        double muF[ND], sgF[ND], Fm[ND], Fm1[ND], Fm2[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const int a = adj_slot(DIM, d);
+            const double cf = 0.5 * (Cc[C0] + Cc[a]);
+            muF[d] = visc_of(A, cf);
+            sgF[d] = sig_of(A, cf);
+            if (mob.slip) mob.FaceDK3(H[C0], H[a], Fm[d], Fm1[d], Fm2[d]);
+            else Fm[d] = Fm1[d] = Fm2[d] = 0;
+        }
+
         // ---- capillary pressure (conservative; coefficient 1/3) ----
         if (A.cap) {
             const double Coef = 1.0 / 3.0;
@@ -112,14 +127,13 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
             for (int d = 0; d < ND; ++d) {
                 const int a = adj_slot(DIM, d);
                 const double Haj = H[a];
-                const double cf = 0.5 * (Cc[C0] + Cc[a]);
-                const double so = sig_of(A, cf) / visc_of(A, cf);
-                const double val = mob.slip ? mob.FaceDK(Hoo, Haj) : 2.0 * (Hoo * Hoo) * (Haj * Haj) / (Hoo + Haj);
+                const double so = sgF[d] / muF[d];
+                const double val = mob.slip ? Fm[d] : 2.0 * (Hoo * Hoo) * (Haj * Haj) / (Hoo + Haj);
                 D[d] = val * so;
                 if (MODE == 1) {
                     const double den = (Hoo + Haj) * (Hoo + Haj);
-                    const double vc = mob.slip ? mob.FaceDK_d1(Hoo, Haj) : 2.0 * (Hoo * Haj * Haj) * (Hoo + Haj + Haj) / den;
-                    const double va = mob.slip ? mob.FaceDK_d1(Haj, Hoo) : 2.0 * (Haj * Hoo * Hoo) * (Haj + Hoo + Hoo) / den;
+                    const double vc = mob.slip ? Fm1[d] : 2.0 * (Hoo * Haj * Haj) * (Hoo + Haj + Haj) / den;
+                    const double va = mob.slip ? Fm2[d] : 2.0 * (Haj * Hoo * Hoo) * (Haj + Hoo + Hoo) / den;
                     gC[d] = vc * so;
                     gA[d] = va * so;
                 }
@@ -175,15 +189,19 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
             for (int d = 0; d < ND; ++d) {
                 const int a = adj_slot(DIM, d);
                 const double ho = H[C0], ha = H[a];
-                const double mu = visc_of(A, 0.5 * (Cc[C0] + Cc[a]));
-                const double gf = mob.Face(ho, ha);
+                const double mu = muF[d];
+                const double gf = mob.slip ? Fm[d] : ((ho > 0 && ha > 0) ? 2.0 * (ho * ho) * (ha * ha) / (ho + ha) : 0.0);
                 Gf[d] = gf / mu;
                 Tk[d] = (gf == 0) ? 0.0 : gf / (ho + ha) / mu;
                 const double hs = spacing(d / 2);
                 inv2[d] = 1.0 / (hs * hs);
                 if (MODE == 1) {
-                    gCf[d] = mob.Face_d1(ho, ha) / mu;
-                    gAf[d] = mob.Face_d1(ha, ho) / mu;
+                    if (mob.slip) { gCf[d] = Fm1[d] / mu; gAf[d] = Fm2[d] / mu; }
+                    else if (ho > 0 && ha > 0) {
+                        const double den = (ho + ha) * (ho + ha);
+                        gCf[d] = 2.0 * (ho * ha * ha) * (ho + ha + ha) / den / mu;
+                        gAf[d] = 2.0 * (ha * ho * ho) * (ha + ho + ho) / den / mu;
+                    } else { gCf[d] = gAf[d] = 0; }
                 }
             }
         }
@@ -251,8 +269,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
                 const double hf = limited_face(f, dv, wt, gd);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) { wt[q] /= hs; gd[q] /= hs; }
-                const double cc = 0.5 * (Cc[C0] + Cc[adj_slot(DIM, j)]);
-                double D = (mob.slip ? mob.DperH(hf) : hf * hf) / visc_of(A, cc);
+                double D = (mob.slip ? mob.DperH(hf) : hf * hf) / muF[j];
                 if (wall(j)) D = 0;
                 const double sg = (j % 2 == 0) ? -1.0 : 1.0;
 #pragma unroll
@@ -280,14 +297,13 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
                 const int lo = face_slot(DIM, d, 1), hi = face_slot(DIM, d, 2);
                 const double hh = spacing(d / 2);
                 const double dc = (Cc[hi] - Cc[lo]) / hh;
-                const double cc = (Cc[hi] + Cc[lo]) / 2.0;
                 const double up = sgn(Coef) * sgn(dc);
                 double f[4], wt[4], gd[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) f[q] = H[face_slot(DIM, d, q)];
                 const double hf = limited_face(f, up, wt, gd);
                 const bool wl = wall(d);
-                const double mu = visc_of(A, cc);
+                const double mu = muF[d];
                 const double Sf = wl ? 0.0 : (mob.slip ? dc / mu * (hf + 2.0 * mob.lambda) : dc / mu * hf);
                 const double sg = (d % 2 == 0) ? -1.0 : 1.0;
 #pragma unroll
@@ -311,20 +327,21 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
         // ---- disjoining pressure (Dp/3), Pi = (h*/h)^n - (h*/h)^m ----
         if (A.dj) {
             const double Coef = A.Dp / 3.0;
-            auto Pi = [&](double hh) {
-                if (hh <= 0) return 0.0;
-                const double r = A.hstar / hh;
-                return pow(r, A.pn) - pow(r, A.pm);
-            };
-            auto dPi = [&](double hh) {
-                if (hh <= 0) return 0.0;
-                const double r = A.hstar / hh;
-                return -(A.pn * pow(r, A.pn) - A.pm * pow(r, A.pm)) / hh;
-            };
-            double PiS[NS];
-            PiS[C0] = Pi(H[C0]);
+            // (h*/h)^n and (h*/h)^m once per point; Pi and dPi from them
+            double PiS[NS], dPiS[NS];
 #pragma unroll
-            for (int d = 0; d < ND; ++d) PiS[adj_slot(DIM, d)] = Pi(H[adj_slot(DIM, d)]);
+            for (int k = 0; k < NS; ++k) { PiS[k] = 0; dPiS[k] = 0; }
+#pragma unroll
+            for (int d = -1; d < ND; ++d) {
+                const int k = d < 0 ? C0 : adj_slot(DIM, d);
+                const double hh = H[k];
+                if (hh > 0) {
+                    const double r = A.hstar / hh;
+                    const double rn = pow(r, A.pn), rm = pow(r, A.pm);
+                    PiS[k] = rn - rm;
+                    dPiS[k] = -(A.pn * rn - A.pm * rm) / hh;
+                }
+            }
             auto PiPerH = [&](int k) { return H[k] <= 0 ? 0.0 : PiS[k] / H[k]; };
             double cen = 0;
 #pragma unroll
@@ -346,12 +363,12 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
                     ga += gCf[dd] * Ad[dd];
                     gc += Gf[dd] * (-inv2[dd]);
                 }
-                const double dpc = dPi(H[C0]) - PiPerH(C0);
+                const double dpc = dPiS[C0] - PiPerH(C0);
                 Jc[C0] += Coef * (ga + gc * dpc);
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                     const int a = adj_slot(DIM, d);
-                    const double dpa = dPi(H[a]) - PiPerH(a);
+                    const double dpa = dPiS[a] - PiPerH(a);
                     Jc[a] += Coef * (gAf[d] * Ad[d] + Gf[d] * inv2[d] * dpa);
                 }
             }

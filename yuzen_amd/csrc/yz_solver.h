@@ -52,6 +52,12 @@ struct Solver {
     Krylov kr;
     hipStream_t stream = nullptr;
     int max_levels = 12;
+    // launch timing of the fine-level stencil product of the iteration
+    // (k_spmv_dot), with HIP events on the solver's stream
+    int ktime = 0;
+    hipEvent_t kev[2] = {nullptr, nullptr};
+    long long kcount = 0;
+    double kms = 0;
 };
 
 // allocate (idempotent for the same grid)

@@ -15,7 +15,7 @@
 // omega = 4/3 / rho(D^-1 A) with rho bounded by Gershgorin; the coarse
 // operator is the Galerkin product P^T A P, a 5x5 box stencil on the grid
 // of aggregates; SPAI0 (m_i = a_ii / sum_j a_ij^2) smooths before and after
-// the coarse correction; the coarsest level (<= 256 cells) is solved with
+// the coarse correction; the coarsest level (<= 128 cells) is solved with
 // its dense inverse. Every level is a set of value planes: no column
 // indices, no hashing, no atomics in the set-up.
 //
@@ -44,7 +44,7 @@ namespace yz {
 
 static constexpr int BLK = 256;
 static constexpr int RED_BLOCKS = 1024;   // partial sums per reduction
-static constexpr int COARSE_MAX = 256;
+static constexpr int COARSE_MAX = 128;   // coarsest dense matrix fits in LDS
 
 static inline int nblk(int n) { return (n + BLK - 1) / BLK; }
 
@@ -72,9 +72,13 @@ __device__ inline void block_sum_store(double (&v)[M], double *part, int stride)
     }
 }
 
-// sum the partials of quantity m into out[m] (one block, fixed order)
+enum { FIN_INIT, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_NEXT };
+__device__ inline void apply_scalars(int op, double *sc, const double *sums);
+
+// sum the partials of quantity m (one block, fixed order) and apply the
+// BiCGSTAB scalar step op to them
 template <int M>
-__global__ void k_finish(const double *part, double *out) {
+__global__ void k_finish(const double *part, double *sc, int op) {
     __shared__ double sh[M][BLK];
     for (int m = 0; m < M; ++m) {
         double s = 0;
@@ -87,8 +91,11 @@ __global__ void k_finish(const double *part, double *out) {
             for (int m = 0; m < M; ++m) sh[m][threadIdx.x] += sh[m][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int m = 0; m < M; ++m) out[m] = sh[m][0];
+    if (threadIdx.x == 0) {
+        double sums[M];
+        for (int m = 0; m < M; ++m) sums[m] = sh[m][0];
+        apply_scalars(op, sc, sums);
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -236,27 +243,31 @@ __global__ void k_x_alpha(int N, double *__restrict__ x, const double *__restric
     x[i] += sc[SC_ALPHA] * ph[i];
 }
 
-// scalar steps (one thread)
-__global__ void k_sc_init(double *sc, const double *sums) {
-    sc[SC_BB] = sums[0];
-    sc[SC_RR] = sums[1];
-    sc[SC_RHO1] = sums[1];
-    sc[SC_RHO2] = 1; sc[SC_ALPHA] = 1; sc[SC_OMEGA] = 1; sc[SC_BETA] = 0;
-}
-__global__ void k_sc_alpha(double *sc, const double *sums) {
-    sc[SC_RV] = sums[0];
-    sc[SC_ALPHA] = (sums[0] != 0) ? sc[SC_RHO1] / sums[0] : 0.0;
-}
-__global__ void k_sc_ss(double *sc, const double *sums) { sc[SC_SS] = sums[0]; }
-__global__ void k_sc_omega(double *sc, const double *sums) {
-    sc[SC_TT] = sums[0]; sc[SC_TS] = sums[1];
-    sc[SC_OMEGA] = (sums[0] != 0) ? sums[1] / sums[0] : 0.0;
-}
-__global__ void k_sc_next(double *sc, const double *sums) {
-    sc[SC_RR] = sums[0];
-    sc[SC_RHO2] = sc[SC_RHO1];
-    sc[SC_RHO1] = sums[1];
-    sc[SC_BETA] = (sc[SC_RHO1] * sc[SC_ALPHA]) / (sc[SC_RHO2] * sc[SC_OMEGA]);
+// scalar steps of the iteration (thread 0 of the finishing block)
+__device__ inline void apply_scalars(int op, double *sc, const double *sums) {
+    switch (op) {
+    case FIN_INIT:
+        sc[SC_BB] = sums[0]; sc[SC_RR] = sums[1]; sc[SC_RHO1] = sums[1];
+        sc[SC_RHO2] = 1; sc[SC_ALPHA] = 1; sc[SC_OMEGA] = 1; sc[SC_BETA] = 0;
+        break;
+    case FIN_ALPHA:
+        sc[SC_RV] = sums[0];
+        sc[SC_ALPHA] = (sums[0] != 0) ? sc[SC_RHO1] / sums[0] : 0.0;
+        break;
+    case FIN_SS:
+        sc[SC_SS] = sums[0];
+        break;
+    case FIN_OMEGA:
+        sc[SC_TT] = sums[0]; sc[SC_TS] = sums[1];
+        sc[SC_OMEGA] = (sums[0] != 0) ? sums[1] / sums[0] : 0.0;
+        break;
+    default:
+        sc[SC_RR] = sums[0];
+        sc[SC_RHO2] = sc[SC_RHO1];
+        sc[SC_RHO1] = sums[1];
+        sc[SC_BETA] = (sc[SC_RHO1] * sc[SC_ALPHA]) / (sc[SC_RHO2] * sc[SC_OMEGA]);
+        break;
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -485,54 +496,74 @@ __global__ void k_dense(StencilOp op, const double *__restrict__ val, double *__
     for (int k = 0; k < NS; ++k) Dn[(size_t)i * op.N + stencil_col(op, ix, iy, k)] += val[(size_t)k * op.N + i];
 }
 
-// Gauss-Jordan inverse with partial pivoting, one block; A (n x n) is
-// destroyed, inv receives A^-1
-__global__ void __launch_bounds__(1024) k_gauss_jordan(int n, double *A, double *inv) {
-    __shared__ double pv[1024];
-    __shared__ int pi[1024];
-    __shared__ int piv_row;
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) inv[t] = ((t / n) == (t % n)) ? 1.0 : 0.0;
-    __syncthreads();
+// In-place Gauss-Jordan inversion with row pivoting, one block of 1024
+// threads. M (n x n, row-major) is the working matrix: in LDS when
+// n <= 128 (128 KB), else the global scratch it is given. The row swaps of
+// the elimination are undone as column swaps at the end. The inverse is
+// written transposed (invT[j n + i] = inv(i, j)) so that the coarse
+// product reads it coalesced.
+template <bool SHARED>
+__device__ void gj_body(int n, double *M, int *perm, double *red_v, int *red_i, double *invT) {
+    const int tid = threadIdx.x, nt = blockDim.x;
     for (int k = 0; k < n; ++k) {
         double best = -1; int br = k;
-        for (int r = k + threadIdx.x; r < n; r += blockDim.x) {
-            const double a = fabs(A[(size_t)r * n + k]);
+        for (int r = k + tid; r < n; r += nt) {
+            const double a = fabs(M[r * n + k]);
             if (a > best) { best = a; br = r; }
         }
-        pv[threadIdx.x] = best; pi[threadIdx.x] = br;
+        red_v[tid] = best; red_i[tid] = br;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double b = -1; int r0 = k;
-            for (int t = 0; t < (int)blockDim.x; ++t) if (pv[t] > b) { b = pv[t]; r0 = pi[t]; }
-            piv_row = r0;
+        for (int w = nt / 2; w > 0; w >>= 1) {
+            if (tid < w && (red_v[tid + w] > red_v[tid] || (red_v[tid + w] == red_v[tid] && red_i[tid + w] < red_i[tid]))) {
+                red_v[tid] = red_v[tid + w]; red_i[tid] = red_i[tid + w];
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        const int pr = piv_row;
+        const int pr = red_i[0];
+        if (tid == 0) perm[k] = pr;
         if (pr != k)
-            for (int j = threadIdx.x; j < n; j += blockDim.x) {
-                double a = A[(size_t)k * n + j]; A[(size_t)k * n + j] = A[(size_t)pr * n + j]; A[(size_t)pr * n + j] = a;
-                double b = inv[(size_t)k * n + j]; inv[(size_t)k * n + j] = inv[(size_t)pr * n + j]; inv[(size_t)pr * n + j] = b;
-            }
+            for (int j = tid; j < n; j += nt) { const double t = M[k * n + j]; M[k * n + j] = M[pr * n + j]; M[pr * n + j] = t; }
         __syncthreads();
-        const double d = A[(size_t)k * n + k];
-        const double di = (d != 0) ? 1.0 / d : 0.0;
+        const double piv = M[k * n + k];
+        const double pinv = (piv != 0) ? 1.0 / piv : 0.0;
         __syncthreads();
-        for (int j = threadIdx.x; j < n; j += blockDim.x) { A[(size_t)k * n + j] *= di; inv[(size_t)k * n + j] *= di; }
+        for (int j = tid; j < n; j += nt) M[k * n + j] = (j == k) ? pinv : M[k * n + j] * pinv;
         __syncthreads();
-        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        // eliminate column k from the other rows; f is read before any write
+        for (int t = tid; t < n * n; t += nt) {
             const int r = t / n, j = t % n;
-            if (r == k) continue;
-            const double f = A[(size_t)r * n + k];
-            if (f == 0) continue;
-            // column k of row r is eliminated last, after every other column
-            if (j != k) {
-                A[(size_t)r * n + j] -= f * A[(size_t)k * n + j];
-            }
-            inv[(size_t)r * n + j] -= f * inv[(size_t)k * n + j];
+            if (r == k || j == k) continue;
+            M[r * n + j] -= M[r * n + k] * M[k * n + j];
         }
         __syncthreads();
-        for (int r = threadIdx.x; r < n; r += blockDim.x) if (r != k) A[(size_t)r * n + k] = 0;
+        for (int r = tid; r < n; r += nt)
+            if (r != k) M[r * n + k] = -M[r * n + k] * M[k * n + k];
         __syncthreads();
+    }
+    // undo the pivoting as column swaps, last first
+    for (int k = n - 1; k >= 0; --k) {
+        const int pr = perm[k];
+        if (pr != k)
+            for (int r = tid; r < n; r += nt) { const double t = M[r * n + k]; M[r * n + k] = M[r * n + pr]; M[r * n + pr] = t; }
+        __syncthreads();
+    }
+    for (int t = tid; t < n * n; t += nt) {
+        const int i = t / n, j = t % n;
+        invT[j * n + i] = M[i * n + j];
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_gauss_jordan(int n, double *Ag, double *invT) {
+    __shared__ double red_v[1024];
+    __shared__ int red_i[1024];
+    __shared__ int perm[1024];
+    if (n <= 128) {
+        __shared__ double Ms[128 * 128];
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) Ms[t] = Ag[t];
+        __syncthreads();
+        gj_body<true>(n, Ms, perm, red_v, red_i, invT);
+    } else {
+        gj_body<false>(n, Ag, perm, red_v, red_i, invT);
     }
 }
 
@@ -628,12 +659,13 @@ __global__ void __launch_bounds__(BLK) k_post(StencilOp op, const double *__rest
     y[i] = x[i] + m[i] * (b[i] - ax);
 }
 
-__global__ void k_dense_mv(int n, const double *__restrict__ Ainv, const double *__restrict__ b,
+// x = A^-1 b with the inverse stored transposed (coalesced over i)
+__global__ void k_dense_mv(int n, const double *__restrict__ AinvT, const double *__restrict__ b,
                            double *__restrict__ x) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     double s = 0;
-    for (int j = 0; j < n; ++j) s += Ainv[(size_t)i * n + j] * b[j];
+    for (int j = 0; j < n; ++j) s += AinvT[(size_t)j * n + i] * b[j];
     x[i] = s;
 }
 
@@ -725,6 +757,7 @@ void solver_free(Solver &S) {
     f(S.A); f(S.dinv_jac);
     Krylov &k = S.kr;
     f(k.r); f(k.rh); f(k.p); f(k.v); f(k.s); f(k.t); f(k.ph); f(k.sh); f(k.part); f(k.sc);
+    for (auto &e : S.kev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (k.host_sc) { (void)hipHostFree(k.host_sc); k.host_sc = nullptr; }
     k.N = 0;
 }
@@ -830,15 +863,13 @@ int bicgstab(Solver &S, const double *b, double *x, int precond, int maxit, doub
     const StencilOp &op = S.op0;
     const int N = op.N;
     hipStream_t st = S.stream;
-    double *sums = k.sc + 32;
     *iters = 0;
     *resid = 0;
     dispatch_ns(op.ns, [&](auto c) {
         constexpr int NS = decltype(c)::value;
         hipLaunchKernelGGL(k_init_residual<NS>, dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, x, b, k.r, k.rh, k.part);
     });
-    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, sums);
-    hipLaunchKernelGGL(k_sc_init, dim3(1), dim3(1), 0, st, k.sc, sums);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_INIT);
     if (read_scalars(S)) return 2;
     const double *h = k.host_sc;
     const double nb = std::sqrt(h[SC_BB]);
@@ -854,16 +885,23 @@ int bicgstab(Solver &S, const double *b, double *x, int precond, int maxit, doub
         if (it > 0 && std::fabs(h[SC_RHO1]) < 1e-300) { *iters = it; *resid = res / nb; return 2; }
         hipLaunchKernelGGL(k_p_update, dim3(nblk(N)), dim3(BLK), 0, st, N, it == 0 ? 1 : 0, k.r, k.p, k.v, k.sc);
         precond_apply(S, precond, k.p, k.ph);
+        if (S.ktime) {
+            if (!S.kev[0]) { (void)hipEventCreate(&S.kev[0]); (void)hipEventCreate(&S.kev[1]); }
+            (void)hipEventRecord(S.kev[0], st);
+        }
         dispatch_ns(op.ns, [&](auto c) {
             constexpr int NS = decltype(c)::value;
             hipLaunchKernelGGL(k_spmv_dot<NS>, dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, k.ph, k.v, k.rh, k.part);
         });
-        hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLK), 0, st, k.part, sums);
-        hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(1), 0, st, k.sc, sums);
+        if (S.ktime) (void)hipEventRecord(S.kev[1], st);
+        hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_ALPHA);
         hipLaunchKernelGGL(k_s_update, dim3(RED_BLOCKS), dim3(BLK), 0, st, N, k.r, k.v, k.s, k.sc, k.part);
-        hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLK), 0, st, k.part, sums);
-        hipLaunchKernelGGL(k_sc_ss, dim3(1), dim3(1), 0, st, k.sc, sums);
+        hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_SS);
         if (read_scalars(S)) return 2;
+        if (S.ktime) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, S.kev[0], S.kev[1]) == hipSuccess) { S.kms += ms; S.kcount++; }
+        }
         if (h[SC_RV] == 0) { *iters = it; *resid = res / nb; return 2; }
         res = std::sqrt(h[SC_SS]);
         if (res < eps) {
@@ -876,12 +914,10 @@ int bicgstab(Solver &S, const double *b, double *x, int precond, int maxit, doub
             constexpr int NS = decltype(c)::value;
             hipLaunchKernelGGL(k_spmv_dot2<NS>, dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, k.sh, k.t, k.s, k.part);
         });
-        hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, sums);
-        hipLaunchKernelGGL(k_sc_omega, dim3(1), dim3(1), 0, st, k.sc, sums);
+        hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_OMEGA);
         hipLaunchKernelGGL(k_xr_update, dim3(RED_BLOCKS), dim3(BLK), 0, st, N, x, k.ph, k.sh, k.r, k.s, k.t, k.rh,
                            k.sc, k.part);
-        hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, sums);
-        hipLaunchKernelGGL(k_sc_next, dim3(1), dim3(1), 0, st, k.sc, sums);
+        hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_NEXT);
         if (read_scalars(S)) return 2;
         if (h[SC_OMEGA] == 0) { *iters = it; *resid = res / nb; return 2; }
         res = std::sqrt(h[SC_RR]);

@@ -581,6 +581,14 @@ int yz_step_clock(yz_step *S, double *t, double *dt_now, double *dt_old) {
     return 0;
 }
 
+int yz_step_kernel_time(yz_step *S, int enable, long long *count, double *total_ms) {
+    if (!S) { set_err("null step"); return 2; }
+    if (count) *count = S->S.kcount;
+    if (total_ms) *total_ms = S->S.kms;
+    if (enable >= 0) { S->S.ktime = enable; S->S.kcount = 0; S->S.kms = 0; }
+    return 0;
+}
+
 int yz_step_timing(yz_step *S, double *ms6) {
     if (!S || !ms6) { set_err("null argument"); return 2; }
     for (int k = 0; k < 6; ++k) ms6[k] = S->tms[k];

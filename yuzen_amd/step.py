@@ -77,6 +77,13 @@ class HipStep:
         self.lib.yz_step_timing(self.p, _pd(out))
         return out
 
+    def kernel_time(self, enable=-1):
+        """(launches, total ms) of the fine-level Krylov SpMV since the last
+        reset; enable=1 resets and starts timing, 0 resets and stops."""
+        n, ms = C.c_longlong(), C.c_double()
+        self.lib.yz_step_kernel_time(self.p, enable, C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
     def assemble(self, h_old, h, dt):
         ia = np.empty(self.ns * self.N)
         ax = np.empty(self.n)
