@@ -21,6 +21,7 @@ struct AmgLevel {
     double *val = nullptr;     // ns planes x N
     double *minv = nullptr;    // SPAI0 diagonal
     double *P = nullptr;       // 9 planes x N (1-D: 3): prolongator to the next level
+    double *R = nullptr;       // 81 planes x Nc (1-D: 9): P^T by coarse cell, for the restriction
     double *AP = nullptr;      // 25 planes x N (1-D: 5): A P, set-up scratch
     double *x = nullptr, *b = nullptr, *r = nullptr;
     int ncx = 0, ncy = 0;      // next level's grid

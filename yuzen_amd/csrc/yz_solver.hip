@@ -317,19 +317,32 @@ __device__ inline int merged_row(const StencilOp &op, const double *val, int i, 
 
 // SPAI0 diagonal, true diagonal and the Gershgorin bound of D^-1 A
 template <int NS>
-__global__ void k_level_diag(StencilOp op, const double *__restrict__ val, double *__restrict__ minv,
+__global__ void __launch_bounds__(BLK) k_level_diag(StencilOp op, const double *__restrict__ val, double *__restrict__ minv,
                              double *__restrict__ dg, unsigned long long *__restrict__ rho_bits) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     double g = 0;
     if (i < op.N) {
         const int ix = i % op.nx, iy = i / op.nx;
-        int cols[NS];
-        double vals[NS];
-        const int m = merged_row<NS>(op, val, i, ix, iy, cols, vals);
         double d = 0, ss = 0, off = 0;
-        for (int q = 0; q < m; ++q) {
-            ss += vals[q] * vals[q];
-            if (cols[q] == i) d = vals[q]; else off += fabs(vals[q]);
+        bool inside = true;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) inside = inside && stencil_inside(op, ix, iy, k);
+        if (inside) {
+            // every slot its own column (the offsets are distinct)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const double v = val[(size_t)k * op.N + i];
+                ss += v * v;
+                if (op.ox[k] == 0 && op.oy[k] == 0) d = v; else off += fabs(v);
+            }
+        } else {
+            int cols[NS];
+            double vals[NS];
+            const int m = merged_row<NS>(op, val, i, ix, iy, cols, vals);
+            for (int q = 0; q < m; ++q) {
+                ss += vals[q] * vals[q];
+                if (cols[q] == i) d = vals[q]; else off += fabs(vals[q]);
+            }
         }
         minv[i] = (ss > 0) ? d / ss : 0.0;
         dg[i] = d;
@@ -362,127 +375,153 @@ __device__ inline int coarse_diff(int J, int I, int nc, int periodic) {
     return d;
 }
 
-// smoothed prolongator, P(i, agg(i) + e), e in [-1,1]^dim
-template <int NS>
-__global__ void k_build_P(StencilOp op, const double *__restrict__ val, const double *__restrict__ dg,
+// the offset u = i - 3J of fine cell i from the first cell of aggregate J
+// along one axis, wrapped into [-3, 5] when the axis is periodic
+__device__ inline int agg_offset(int i, int J, int n, int periodic) {
+    int u = i - 3 * J;
+    if (periodic) {
+        if (u > 5) u -= n;
+        if (u < -3) u += n;
+    }
+    return u;
+}
+
+// smoothed prolongator, P(i, agg(i) + e), e in [-1,1]^DIM, written twice:
+// per fine cell (P[e][i], for the prolongation) and per coarse cell
+// (R[u][J] = P(i, J), u the fine offset in [-3,5]^DIM from aggregate J,
+// for the restriction)
+template <int NS, int DIM>
+__global__ void __launch_bounds__(BLK) k_build_P(StencilOp op, const double *__restrict__ val, const double *__restrict__ dg,
                           const unsigned long long *__restrict__ rho_bits, int ncx, int ncy,
-                          double *__restrict__ P) {
+                          double *__restrict__ P, double *__restrict__ R) {
+    constexpr int NP = DIM == 2 ? 9 : 3;
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= op.N) return;
     const int ix = i % op.nx, iy = i / op.nx;
     const double rho = __longlong_as_double((long long)*rho_bits);
     const double omega = (rho > 0) ? (4.0 / 3.0) / rho : 0.0;
     const int Ix = ix / 3, Iy = iy / 3;
-    const int NP = op.dim == 2 ? 9 : 3;
-    double p[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) p[e] = 0;
-    const double tI = 1.0 / sqrt((double)(agg_size(Ix, op.nx) * (op.dim == 2 ? agg_size(Iy, op.ny) : 1)));
-    p[op.dim == 2 ? 4 : 1] = tI;
     const double w = (dg[i] != 0) ? omega / dg[i] : 0.0;
+    int ek[NS];
+    double vk[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const int c = stencil_col(op, ix, iy, k);
         const int jx = c % op.nx, jy = c / op.nx;
         const int Jx = jx / 3, Jy = jy / 3;
         const int ex = coarse_diff(Jx, Ix, ncx, op.perx);
-        const int ey = op.dim == 2 ? coarse_diff(Jy, Iy, ncy, op.pery) : 0;
-        const double tJ = 1.0 / sqrt((double)(agg_size(Jx, op.nx) * (op.dim == 2 ? agg_size(Jy, op.ny) : 1)));
-        const int e = op.dim == 2 ? (ey + 1) * 3 + (ex + 1) : ex + 1;
-        p[e] -= w * val[(size_t)k * op.N + i] * tJ;
+        const int ey = DIM == 2 ? coarse_diff(Jy, Iy, ncy, op.pery) : 0;
+        const double tJ = 1.0 / sqrt((double)(agg_size(Jx, op.nx) * (DIM == 2 ? agg_size(Jy, op.ny) : 1)));
+        ek[k] = DIM == 2 ? (ey + 1) * 3 + (ex + 1) : ex + 1;
+        vk[k] = w * val[(size_t)k * op.N + i] * tJ;
     }
-    for (int e = 0; e < NP; ++e) P[(size_t)e * op.N + i] = p[e];
+    const double tI = 1.0 / sqrt((double)(agg_size(Ix, op.nx) * (DIM == 2 ? agg_size(Iy, op.ny) : 1)));
+    const int ncN = ncx * (DIM == 2 ? ncy : 1);
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        double pv = (e == NP / 2) ? tI : 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) if (ek[k] == e) pv -= vk[k];
+        P[(size_t)e * op.N + i] = pv;
+        // the coarse cell J = agg(i) + e, and i's offset from it
+        const int ex = DIM == 2 ? e % 3 - 1 : e - 1;
+        const int ey = DIM == 2 ? e / 3 - 1 : 0;
+        int Jx = Ix + ex, Jy = Iy + ey;
+        if (op.perx) Jx = (Jx + ncx) % ncx;
+        else if (Jx < 0 || Jx >= ncx) continue;
+        if (DIM == 2) {
+            if (op.pery) Jy = (Jy + ncy) % ncy;
+            else if (Jy < 0 || Jy >= ncy) continue;
+        }
+        const int u = agg_offset(ix, Jx, op.nx, op.perx) + 3;
+        const int v = DIM == 2 ? agg_offset(iy, Jy, op.ny, op.pery) + 3 : 0;
+        R[(size_t)(v * 9 + u) * ncN + Jy * ncx + Jx] = pv;
+    }
 }
 
-// A P (i, agg(i) + f), f in [-2,2]^dim
-template <int NS>
-__global__ void k_build_AP(StencilOp op, const double *__restrict__ val, const double *__restrict__ P,
+// A P (i, agg(i) + f), f in [-2,2]^DIM
+template <int NS, int DIM>
+__global__ void __launch_bounds__(BLK) k_build_AP(StencilOp op, const double *__restrict__ val, const double *__restrict__ P,
                            int ncx, int ncy, double *__restrict__ AP) {
+    constexpr int NF = DIM == 2 ? 25 : 5;
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= op.N) return;
     const int ix = i % op.nx, iy = i / op.nx;
     const int Ix = ix / 3, Iy = iy / 3;
-    const bool two = op.dim == 2;
-    double ap[25];
+    int dxk[NS], dyk[NS], ck[NS];
+    double ak[NS];
 #pragma unroll
-    for (int f = 0; f < 25; ++f) ap[f] = 0;
     for (int k = 0; k < NS; ++k) {
-        const double a = val[(size_t)k * op.N + i];
-        if (a == 0) continue;
         const int c = stencil_col(op, ix, iy, k);
-        const int jx = c % op.nx, jy = c / op.nx;
-        const int ex0 = coarse_diff(jx / 3, Ix, ncx, op.perx);
-        const int ey0 = two ? coarse_diff(jy / 3, Iy, ncy, op.pery) : 0;
-        if (two) {
-            for (int ey = -1; ey <= 1; ++ey)
-                for (int ex = -1; ex <= 1; ++ex) {
-                    const double pv = P[(size_t)((ey + 1) * 3 + (ex + 1)) * op.N + c];
-                    const int fx = ex0 + ex, fy = ey0 + ey;
-                    ap[(fy + 2) * 5 + (fx + 2)] += a * pv;
-                }
-        } else {
-            for (int ex = -1; ex <= 1; ++ex) {
-                const double pv = P[(size_t)(ex + 1) * op.N + c];
-                ap[ex0 + ex + 2] += a * pv;
-            }
-        }
+        ck[k] = c;
+        ak[k] = val[(size_t)k * op.N + i];
+        dxk[k] = coarse_diff((c % op.nx) / 3, Ix, ncx, op.perx);
+        dyk[k] = DIM == 2 ? coarse_diff((c / op.nx) / 3, Iy, ncy, op.pery) : 0;
     }
-    const int NF = two ? 25 : 5;
-    for (int f = 0; f < NF; ++f) AP[(size_t)f * op.N + i] = ap[f];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const int fx = DIM == 2 ? f % 5 - 2 : f - 2;
+        const int fy = DIM == 2 ? f / 5 - 2 : 0;
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int ex = fx - dxk[k], ey = fy - dyk[k];
+            if (ex < -1 || ex > 1 || ey < -1 || ey > 1) continue;
+            const int e = DIM == 2 ? (ey + 1) * 3 + (ex + 1) : ex + 1;
+            acc += ak[k] * P[(size_t)e * op.N + ck[k]];
+        }
+        AP[(size_t)f * op.N + i] = acc;
+    }
 }
 
-// Galerkin product on the coarse grid: Ac(I, I + q), q in [-2,2]^dim
-__global__ void k_build_RAP(StencilOp fop, const double *__restrict__ P, const double *__restrict__ AP,
+// Galerkin product on the coarse grid: Ac(I, I + q), q in [-2,2]^DIM
+template <int DIM>
+__global__ void __launch_bounds__(BLK) k_build_RAP(StencilOp fop, const double *__restrict__ P, const double *__restrict__ AP,
                             StencilOp cop, double *__restrict__ Ac) {
+    constexpr int NQ = DIM == 2 ? 25 : 5;
     const int I = blockIdx.x * BLK + threadIdx.x;
     if (I >= cop.N) return;
-    const bool two = fop.dim == 2;
     const int Ix = I % cop.nx, Iy = I / cop.nx;
-    double ac[25];
+    double ac[NQ];
 #pragma unroll
-    for (int q = 0; q < 25; ++q) ac[q] = 0;
-    const int ylo = two ? -1 : 0, yhi = two ? 1 : 0;
-    for (int ey = ylo; ey <= yhi; ++ey) {
-        int Ky = Iy + ey;
-        if (two) {
-            if (cop.pery) Ky = (Ky + cop.ny) % cop.ny;
-            else if (Ky < 0 || Ky >= cop.ny) continue;
-        }
+    for (int q = 0; q < NQ; ++q) ac[q] = 0;
+#pragma unroll
+    for (int ey = (DIM == 2 ? -1 : 0); ey <= (DIM == 2 ? 1 : 0); ++ey) {
+#pragma unroll
         for (int ex = -1; ex <= 1; ++ex) {
-            int Kx = Ix + ex;
+            int Kx = Ix + ex, Ky = Iy + ey;
+            bool ok = true;
             if (cop.perx) Kx = (Kx + cop.nx) % cop.nx;
-            else if (Kx < 0 || Kx >= cop.nx) continue;
-            // fine cells of aggregate K; P(i, I) is the slot I - agg(i) = -e
-            const int pe = two ? (1 - ey) * 3 + (1 - ex) : (1 - ex);
+            else ok = ok && Kx >= 0 && Kx < cop.nx;
+            if (DIM == 2) {
+                if (cop.pery) Ky = (Ky + cop.ny) % cop.ny;
+                else ok = ok && Ky >= 0 && Ky < cop.ny;
+            }
+            if (!ok) continue;
+            // fine cells i of aggregate K; P(i, I) is the slot I - agg(i) = -e
+            const int pe = DIM == 2 ? (1 - ey) * 3 + (1 - ex) : (1 - ex);
             const int x0 = 3 * Kx, x1 = min(3 * Kx + 3, fop.nx);
-            const int y0 = two ? 3 * Ky : 0, y1 = two ? min(3 * Ky + 3, fop.ny) : 1;
+            const int y0 = DIM == 2 ? 3 * Ky : 0, y1 = DIM == 2 ? min(3 * Ky + 3, fop.ny) : 1;
             for (int fy = y0; fy < y1; ++fy)
                 for (int fx = x0; fx < x1; ++fx) {
                     const int i = fy * fop.nx + fx;
                     const double pv = P[(size_t)pe * fop.N + i];
-                    if (pv == 0) continue;
-                    // AP(i, K + f) -> Ac(I, I + e + f)
-                    if (two) {
-                        for (int gy = -2; gy <= 2; ++gy) {
-                            const int qy = ey + gy;
-                            if (qy < -2 || qy > 2) continue;
-                            for (int gx = -2; gx <= 2; ++gx) {
-                                const int qx = ex + gx;
-                                if (qx < -2 || qx > 2) continue;
-                                ac[(qy + 2) * 5 + (qx + 2)] += pv * AP[(size_t)((gy + 2) * 5 + (gx + 2)) * fop.N + i];
-                            }
-                        }
-                    } else {
+                    // AP(i, K + g) -> Ac(I, I + e + g)
+#pragma unroll
+                    for (int gy = (DIM == 2 ? -2 : 0); gy <= (DIM == 2 ? 2 : 0); ++gy) {
+#pragma unroll
                         for (int gx = -2; gx <= 2; ++gx) {
-                            const int qx = ex + gx;
-                            if (qx < -2 || qx > 2) continue;
-                            ac[qx + 2] += pv * AP[(size_t)(gx + 2) * fop.N + i];
+                            const int qx = ex + gx, qy = ey + gy;
+                            if (qx < -2 || qx > 2 || qy < -2 || qy > 2) continue;
+                            const int g = DIM == 2 ? (gy + 2) * 5 + (gx + 2) : gx + 2;
+                            const int q = DIM == 2 ? (qy + 2) * 5 + (qx + 2) : qx + 2;
+                            ac[q] += pv * AP[(size_t)g * fop.N + i];
                         }
                     }
                 }
         }
     }
-    const int NQ = two ? 25 : 5;
+#pragma unroll
     for (int q = 0; q < NQ; ++q) Ac[(size_t)q * cop.N + I] = ac[q];
 }
 
@@ -588,34 +627,46 @@ __global__ void __launch_bounds__(BLK) k_pre(StencilOp op, const double *__restr
     r[i] = b[i] - s;
 }
 
-// restriction b_c = P^T r
-__global__ void k_restrict(StencilOp fop, const double *__restrict__ P, const double *__restrict__ r,
-                           StencilOp cop, double *__restrict__ bc) {
-    const int I = blockIdx.x * BLK + threadIdx.x;
-    if (I >= cop.N) return;
-    const bool two = fop.dim == 2;
-    const int Ix = I % cop.nx, Iy = I / cop.nx;
+// restriction b_c = P^T r from the per-coarse-cell weights R: a block of
+// 16 x 16 coarse cells (256 in 1-D) stages the fine residual it needs
+// (a 56 x 56 tile; 776 cells in 1-D) in LDS, coalesced; each thread then
+// sums its 81 (9) weights, read coalesced, against the tile
+template <int DIM>
+__global__ void __launch_bounds__(256) k_restrict(StencilOp fop, const double *__restrict__ R,
+                                                  const double *__restrict__ r, StencilOp cop,
+                                                  double *__restrict__ bc) {
+    constexpr int TX = DIM == 2 ? 16 : 256, TY = DIM == 2 ? 16 : 1;
+    constexpr int FX = 3 * TX + 8, FY = DIM == 2 ? 3 * TY + 8 : 1;
+    __shared__ double tile[FY * FX];
+    const int X0 = blockIdx.x * TX, Y0 = DIM == 2 ? blockIdx.y * TY : 0;
+    const int fx0 = 3 * X0 - 3, fy0 = DIM == 2 ? 3 * Y0 - 3 : 0;
+    for (int t = threadIdx.x; t < FX * FY; t += 256) {
+        const int lx = t % FX, ly = t / FX;
+        int gx = fx0 + lx, gy = fy0 + ly;
+        bool ok = true;
+        if (fop.perx) gx = ((gx % fop.nx) + fop.nx) % fop.nx;
+        else ok = ok && gx >= 0 && gx < fop.nx;
+        if (DIM == 2) {
+            if (fop.pery) gy = ((gy % fop.ny) + fop.ny) % fop.ny;
+            else ok = ok && gy >= 0 && gy < fop.ny;
+        }
+        tile[t] = ok ? r[gy * fop.nx + gx] : 0.0;
+    }
+    __syncthreads();
+    const int tx = DIM == 2 ? threadIdx.x % TX : threadIdx.x, ty = DIM == 2 ? threadIdx.x / TX : 0;
+    const int Ix = X0 + tx, Iy = Y0 + ty;
+    if (Ix >= cop.nx || (DIM == 2 && Iy >= cop.ny)) return;
+    const int I = Iy * cop.nx + Ix;
     double s = 0;
-    const int ylo = two ? -1 : 0, yhi = two ? 1 : 0;
-    for (int ey = ylo; ey <= yhi; ++ey) {
-        int Ky = Iy + ey;
-        if (two) {
-            if (cop.pery) Ky = (Ky + cop.ny) % cop.ny;
-            else if (Ky < 0 || Ky >= cop.ny) continue;
-        }
-        for (int ex = -1; ex <= 1; ++ex) {
-            int Kx = Ix + ex;
-            if (cop.perx) Kx = (Kx + cop.nx) % cop.nx;
-            else if (Kx < 0 || Kx >= cop.nx) continue;
-            const int pe = two ? (1 - ey) * 3 + (1 - ex) : (1 - ex);
-            const int x0 = 3 * Kx, x1 = min(3 * Kx + 3, fop.nx);
-            const int y0 = two ? 3 * Ky : 0, y1 = two ? min(3 * Ky + 3, fop.ny) : 1;
-            for (int fy = y0; fy < y1; ++fy)
-                for (int fx = x0; fx < x1; ++fx) {
-                    const int i = fy * fop.nx + fx;
-                    s += P[(size_t)pe * fop.N + i] * r[i];
-                }
-        }
+    if (DIM == 2) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v)
+#pragma unroll
+            for (int u = 0; u < 9; ++u)
+                s += R[(size_t)(v * 9 + u) * cop.N + I] * tile[(3 * ty + v) * FX + 3 * tx + u];
+    } else {
+#pragma unroll
+        for (int u = 0; u < 9; ++u) s += R[(size_t)u * cop.N + I] * tile[3 * tx + u];
     }
     bc[I] = s;
 }
@@ -734,7 +785,12 @@ int solver_init(Solver &S, const StencilOp &op0, hipStream_t stream) {
         if (dalloc(&L.minv, n) || dalloc(&L.x, n) || dalloc(&L.b, n) || dalloc(&L.r, n)) return -1;
         if (l + 1 < g.lv.size()) {
             const int np = L.op.dim == 2 ? 9 : 3, nf = L.op.dim == 2 ? 25 : 5;
+            const int nr = L.op.dim == 2 ? 81 : 9;
             if (dalloc(&L.P, (size_t)np * n) || dalloc(&L.AP, (size_t)nf * n)) return -1;
+            const size_t rn = (size_t)nr * g.lv[l + 1].op.N;
+            if (dalloc(&L.R, rn)) return -1;
+            // weights of fine cells outside the grid are never written
+            YZ_CHECK(hipMemset(L.R, 0, rn * sizeof(double)));
         }
     }
     g.lv[0].val = S.A;
@@ -750,7 +806,7 @@ void solver_free(Solver &S) {
     for (size_t l = 0; l < S.amg.lv.size(); ++l) {
         AmgLevel &L = S.amg.lv[l];
         if (l > 0) f(L.val);
-        f(L.minv); f(L.P); f(L.AP); f(L.x); f(L.b); f(L.r);
+        f(L.minv); f(L.P); f(L.R); f(L.AP); f(L.x); f(L.b); f(L.r);
     }
     S.amg.lv.clear();
     f(S.amg.coarse_inv); f(S.amg.coarse_dense); f(S.amg.scal);
@@ -780,12 +836,22 @@ int amg_setup(Solver &S) {
         AmgLevel &C = g.lv[l + 1];
         dispatch_ns(L.op.ns, [&](auto c) {
             constexpr int NS = decltype(c)::value;
-            hipLaunchKernelGGL(k_build_P<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.r, rho,
-                               L.ncx, L.ncy, L.P);
-            hipLaunchKernelGGL(k_build_AP<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.P, L.ncx,
-                               L.ncy, L.AP);
+            if (L.op.dim == 2) {
+                hipLaunchKernelGGL((k_build_P<NS, 2>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.r, rho,
+                                   L.ncx, L.ncy, L.P, L.R);
+                hipLaunchKernelGGL((k_build_AP<NS, 2>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.P,
+                                   L.ncx, L.ncy, L.AP);
+            } else {
+                hipLaunchKernelGGL((k_build_P<NS, 1>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.r, rho,
+                                   L.ncx, L.ncy, L.P, L.R);
+                hipLaunchKernelGGL((k_build_AP<NS, 1>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.P,
+                                   L.ncx, L.ncy, L.AP);
+            }
         });
-        hipLaunchKernelGGL(k_build_RAP, dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.AP, C.op, C.val);
+        if (L.op.dim == 2)
+            hipLaunchKernelGGL(k_build_RAP<2>, dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.AP, C.op, C.val);
+        else
+            hipLaunchKernelGGL(k_build_RAP<1>, dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.AP, C.op, C.val);
     }
     AmgLevel &Lc = g.lv.back();
     dispatch_ns(Lc.op.ns, [&](auto c) {
@@ -826,7 +892,12 @@ static void vcycle(Solver &S, const double *in, double *out) {
             constexpr int NS = decltype(c)::value;
             hipLaunchKernelGGL(k_pre<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv, b, L.x, L.r);
         });
-        hipLaunchKernelGGL(k_restrict, dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.r, C.op, C.b);
+        if (L.op.dim == 2)
+            hipLaunchKernelGGL(k_restrict<2>, dim3((C.op.nx + 15) / 16, (C.op.ny + 15) / 16), dim3(256), 0, st, L.op,
+                               L.R, L.r, C.op, C.b);
+        else
+            hipLaunchKernelGGL(k_restrict<1>, dim3((C.op.nx + 255) / 256), dim3(256), 0, st, L.op, L.R, L.r, C.op,
+                               C.b);
         b = C.b;
     }
     AmgLevel &Lc = g.lv[nl - 1];
