@@ -42,8 +42,23 @@ def load():
     lib.yzo_assemble.argtypes = [C.c_void_p, pd, pd, C.c_double, pd, pd, pd, pd, pd]
     lib.yzo_linsolve.argtypes = [C.c_int, C.c_int, pi_, pi_, pd, pd, pd, C.c_int,
                                  C.c_int, C.c_int, C.c_double, pi_, pd]
+    lib.yzo_mobility.argtypes = [C.c_double, C.c_double, C.c_double, pd]
+    lib.yzo_limiter.argtypes = [pd, C.c_double, pd]
     _lib = lib
     return lib
+
+
+def mobility(lam, h1, h2):
+    out = np.zeros(6)
+    load().yzo_mobility(lam, h1, h2, _pd(out))
+    return out
+
+
+def limiter(f, sign):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(10)
+    load().yzo_limiter(_pd(f), sign, _pd(out))
+    return out
 
 
 def _pd(a):

@@ -1,0 +1,391 @@
+// ref_golden.cpp -- golden-vector generator built against the reference's
+// own term kernels (test infrastructure).
+//
+// Compiled by oracle/build_ref.sh together with the reference's
+// Eigen-free translation units (everything under yuzen/src except
+// numerics/LinearSolverBackend.cpp, which needs Eigen/AMGCL, and
+// mainProg.cpp). It does what the reference's own
+// yuzen/tests/test_jacobian.cpp harness does (MakeInput :82-138, a
+// Field_WM1D/2D over a MakeGrid grid :140-149, one term's SetMatrixA and
+// SetJacobiMatrix :189-269), on smooth non-flat fields, and writes what the
+// reference computes:
+//   * per case (dimension x term x slip x boundary kind): the CRS of A and
+//     of J = A + the term's Jacobian part, row by row in the reference's
+//     storage order, and A(h)h (the conservative form for the capillary and
+//     vertical-gravity terms, AddConservative; the product otherwise);
+//   * Mobility::Face/FaceDK and derivatives over a grid of heights and slip
+//     lengths; the Koren limiter's face value, weights and derivative;
+//     NonDimensionalNumbers::Dp over contact angles.
+// Output: binary files (little-endian) plus a JSON manifest, into the
+// directory given as argv[1].
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "NonDimensionalNumbers.h"
+#include "LubFields.h"
+#include "BoundaryCondition.h"
+#include "CRS_Matrix.h"
+#include "WellMixedTraits.h"
+#include "FiniteDifferences.h"
+#include "Mobility.h"
+
+using namespace yuzen;
+using namespace yuzen::grid;
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+
+enum Term { tLaplace, tVertical, tParallel, tCentrifugal, tSoluto, tDisjoining };
+const char *kName[] = {"capillary", "vertical_gravity", "inplane_gravity", "centrifugal", "marangoni",
+                       "disjoining"};
+
+struct Params {
+    double Bo = 0.7, Pgx = 0.4, Pgy = -0.2, omega = 1.140175425099138, Ma = 0.3, theta_deg = 10.0,
+           hstar = 0.1, visc_exp = 3.0, slip = 0.0, rotx = 0.13, roty = -0.07;
+};
+
+// Everything dimensionless from the start: L0 = h0 = sigma0 = mu0 =
+// rho = 1, so that As = 1, Bo = g_z, Pg = g_xy, Sp = omega^2, Ma = the
+// surface-tension coefficient (NonDimensionalNumbers.cpp:43-141).
+Input MakeInput(Term t, const Params &p) {
+    Input inp;
+    inp.Physics.ConcField.EnableConvection = inp.Physics.ConcField.EnableDiffusion = false;
+    inp.Physics.Source.Enable = false;
+    inp.Output.Overwrite = inp.Output.Dimensioned = false;
+    inp.Output.FieldData.NeutralFile = inp.Output.FieldData.VTKFile = false;
+    inp.Numerics.TimeStepControl.Enable = false;
+    inp.Numerics.HeightEquation.LinearSolver.RowScaling = false;
+    inp.Numerics.ConcEquation.LinearSolver.RowScaling = false;
+    inp.Numerics.ManufacturedSolution.Enable = false;
+    inp.Restart.Enable = inp.Restart.ASCII = false;
+    inp.LengthScaleType = "custom";
+    inp.LengthScale = 1.0;
+    inp.Fluid.Density.Value = 1.0;
+    inp.Fluid.SurfaceTension.BaseValue = 1.0;
+    inp.Fluid.SurfaceTension.BaseConc = 0.5;
+    inp.Fluid.SurfaceTension.Coef = p.Ma;
+    inp.Fluid.Viscosity.BaseValue = 1.0;
+    inp.Fluid.Viscosity.BaseConc = 0.5;
+    inp.Fluid.Viscosity.Exponent = p.visc_exp;
+    inp.Fluid.Diffusivity.BaseValue = 1.0;
+    inp.Fluid.Diffusivity.BaseConc = 0.5;
+    inp.Fluid.Diffusivity.Exponent = 0.0;
+    inp.Fluid.MolarMass.Solvent = 100.0;
+    inp.Fluid.MolarMass.Resin = 2000.0;
+    inp.Initial.SurfaceShape.TotalHeight = 1.0;
+    inp.Initial.SurfaceShape.PrecursorRatio = p.hstar;
+    inp.Physics.ConcField.Model = "None";
+    inp.Physics.LaplacePressure.Enable = (t == tLaplace);
+    inp.Physics.SolutoCapillarity.Enable = (t == tSoluto);
+    inp.Physics.VerticalGravity.Enable = (t == tVertical);
+    inp.Physics.VerticalGravity.Acceleration = p.Bo;
+    inp.Physics.HorizontalGravity.Enable = (t == tParallel);
+    inp.Physics.HorizontalGravity.Acceleration.assign(2, 0.0);
+    inp.Physics.HorizontalGravity.Acceleration[0] = p.Pgx;
+    inp.Physics.HorizontalGravity.Acceleration[1] = p.Pgy;
+    inp.Physics.Centrifugal.Enable = (t == tCentrifugal);
+    inp.Physics.Centrifugal.RotationRate = p.omega;
+    inp.Physics.Centrifugal.RotationAxis.assign(2, 0.0);
+    inp.Physics.Centrifugal.RotationAxis[0] = p.rotx;
+    inp.Physics.Centrifugal.RotationAxis[1] = p.roty;
+    inp.Physics.Evaporation.Enable = false;
+    inp.Physics.Evaporation.Coef = 0.0;
+    inp.Physics.DisjoiningPressure.Enable = (t == tDisjoining);
+    inp.Physics.DisjoiningPressure.ContactAngle = p.theta_deg;
+    inp.Physics.DisjoiningPressure.ExponentN = 3.0;
+    inp.Physics.DisjoiningPressure.ExponentM = 2.0;
+    inp.Physics.Slip.Enable = p.slip > 0;
+    inp.Physics.Slip.Length = p.slip;
+    return inp;
+}
+
+Input::I_Grid MakeGrid(bool twoD, int nx, int ny) {
+    Input::I_Grid g;
+    g.Dimension = twoD ? "xyh" : "xh";
+    g.Xmin = -0.5; g.Xmax = 0.5;
+    g.Ymin = -0.4; g.Ymax = 0.4;
+    if (!twoD) { g.Ymin = -1.0; g.Ymax = 1.0; }
+    g.Nx = nx; g.Ny = twoD ? ny : 1; g.Nz = 1;
+    return g;
+}
+
+Input::I_Substrate Flat() { Input::I_Substrate s; s.TotalHeight = 0.0; return s; }
+
+// smooth, non-flat fields over the interior; the ghosts take the value of
+// the cell the boundary condition copies (mirror or wrap), as the field
+// would hold them in a run
+struct Shapes {
+    bool periodic;
+    double H(double x, double y) const {
+        if (periodic) return 0.5 + 0.3 * std::sin(2 * PI * x) * (1.1 + 0.4 * std::cos(2 * PI * y / 0.8));
+        return 0.2 + 0.6 * (1 + std::sin(PI * x / 0.5 + 0.3)) * (1.1 + 0.4 * std::cos(PI * y / 0.4 - 0.2));
+    }
+    double S(double x, double y) const {
+        if (periodic) return 0.1 * std::cos(2 * PI * x) * std::sin(2 * PI * y / 0.8);
+        return 0.3 * std::cos(0.7 * PI * x / 0.5) * std::sin(0.5 * PI * y / 0.4 + 0.4);
+    }
+    double C(double x, double y) const {
+        if (periodic) return 0.5 + 0.15 * std::sin(2 * PI * x) + 0.1 * std::cos(2 * PI * y / 0.8);
+        return 0.5 + 0.15 * std::sin(0.8 * PI * x / 0.5 - 0.5) + 0.1 * std::cos(0.6 * PI * y / 0.4);
+    }
+};
+
+int fold(int i, int np, bool periodic) {
+    const int m = np - 4;
+    if (i < 2) return periodic ? i + m : 3 - i;
+    if (i > np - 3) return periodic ? i - m : 2 * (np - 3) + 1 - i;
+    return i;
+}
+
+void write_vec(std::ofstream &f, const std::vector<double> &v) {
+    f.write(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(double));
+}
+void write_ivec(std::ofstream &f, const std::vector<int> &v) {
+    f.write(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(int));
+}
+
+template <class Tr, class FieldT, class TermT>
+void dump_term(TermT &term, FieldT &field, const Shapes &sh, const std::string &path, bool conservative) {
+    typedef typename Tr::Grid Grid;
+    typedef typename Tr::Site Site;
+    Grid *G = field.GetGridPtr();
+    int n = 0;
+    for (Site s = G->template First<Site>(); s != G->template Last<Site>(); ++s)
+        n = std::max(n, Tr::Index(G, s) + 1);
+    int nxp = G->nx();
+    int nyp = n / nxp;
+    const bool twoD = nyp > 1;
+    std::vector<double> hv(n), sv(n), cv(n), xs(n), ys(n);
+    std::vector<char> inside(n, 0);
+    for (Site s = G->template First<Site>(); s != G->template Last<Site>(); ++s) {
+        double x, y;
+        Tr::SiteXY(G, s, x, y);
+        const int i = Tr::Index(G, s);
+        xs[i] = x; ys[i] = y;
+        inside[i] = (G->CheckGhost(s) == Grid::Inside);
+    }
+    for (int i = 0; i < n; ++i) {
+        const int ix = i % nxp, iy = i / nxp;
+        const int fx = fold(ix, nxp, sh.periodic), fy = twoD ? fold(iy, nyp, sh.periodic) : iy;
+        const int src = fy * nxp + fx;
+        hv[i] = sh.H(xs[src], ys[src]);
+        sv[i] = sh.S(xs[src], ys[src]);
+        cv[i] = sh.C(xs[src], ys[src]);
+    }
+    for (Site s = G->template First<Site>(); s != G->template Last<Site>(); ++s) {
+        const int i = Tr::Index(G, s);
+        field.HH(s) = hv[i]; field.SS(s) = sv[i]; field.CC(s) = cv[i];
+    }
+    CRS_Matrix A;
+    A.Build(n);
+    for (int i = 0; i < n; ++i) { A(i, i) = 0.0; if (!inside[i]) A.SetAsBC(i); }
+    term.SetMatrixA(A, field);
+    Vector<double> hx(n), ax(n);
+    for (int i = 0; i < n; ++i) hx[i] = hv[i];
+    if (conservative) {
+        ax = 0;
+        term.AddConservative(ax, hx, A, field);
+    } else {
+        A.MatVec(ax, hx);
+    }
+    std::vector<int> rp(n + 1, 0), col;
+    std::vector<double> av, jv;
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < A.NonZero(i); ++k) { col.push_back(A.Jcol(i, k)); av.push_back(A.Element(i, k)); }
+        rp[i + 1] = (int)col.size();
+    }
+    CRS_Matrix J;
+    J.Build(n);
+    for (int i = 0; i < n; ++i) { J(i, i) = 0.0; if (!inside[i]) J.SetAsBC(i); }
+    term.SetMatrixA(J, field);
+    term.SetJacobiMatrix(J, field);
+    // J in A's column order (J's insertion order may add entries)
+    std::vector<int> rpj(n + 1, 0), colj;
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < J.NonZero(i); ++k) { colj.push_back(J.Jcol(i, k)); jv.push_back(J.Element(i, k)); }
+        rpj[i + 1] = (int)colj.size();
+    }
+    std::vector<double> axv(n);
+    for (int i = 0; i < n; ++i) axv[i] = inside[i] ? ax[i] : 0.0;
+    std::ofstream f(path, std::ios::binary);
+    const int hdr[4] = {n, (int)col.size(), (int)colj.size(), nxp};
+    f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    write_ivec(f, rp); write_ivec(f, col); write_vec(f, av);
+    write_ivec(f, rpj); write_ivec(f, colj); write_vec(f, jv);
+    write_vec(f, hv); write_vec(f, sv); write_vec(f, cv); write_vec(f, axv);
+}
+
+template <class K>
+void build_term(K &k, Input &inp, typename K::Grid *G, NonDimensionalNumbers *nd, typename K::Boundary *bc) {
+    k.Build(inp.Physics, G, nd, bc);
+}
+
+std::string run_case(const std::string &dir, bool twoD, Term t, double slip, bool periodic, std::ostream &man,
+                     bool &first) {
+    Params p;
+    p.slip = slip;
+    if (t == tDisjoining) p.hstar = 0.1;
+    Input inp = MakeInput(t, p);
+    NonDimensionalNumbers nd;
+    nd.Build(inp, inp.Initial.SurfaceShape.TotalHeight);
+    const int nx = twoD ? 18 : 28, ny = 14;
+    Input::I_Grid g = MakeGrid(twoD, nx, ny);
+    Shapes sh{periodic};
+    const auto bk = periodic ? BoundaryCondition::Periodic : BoundaryCondition::Symmetry;
+    std::ostringstream name;
+    name << (twoD ? "2d" : "1d") << "_" << kName[t] << "_slip" << (slip > 0 ? 1 : 0) << (periodic ? "_per" : "_sym");
+    const std::string path = dir + "/" + name.str() + ".bin";
+    const bool cons = (t == tLaplace || t == tVertical);
+    const C_Interpolation_base4::t_Scheme koren = C_Interpolation_base4::Koren;
+    if (twoD) {
+        Field_WM2D field;
+        field.Build(g, Flat(), nd.ExtentScale());
+        BoundaryCondition2D bc(field.GetGridPtr()->nx(), field.GetGridPtr()->ny(), bk, bk, bk, bk);
+        C_Grid2D *G = field.GetGridPtr();
+        switch (t) {
+        case tLaplace: { WM2D_Traits::LaplacePressure k; build_term(k, inp, G, &nd, &bc); dump_term<WM2D_Grid>(k, field, sh, path, cons); break; }
+        case tVertical: { WM2D_Traits::VerticalGravity k; build_term(k, inp, G, &nd, &bc); dump_term<WM2D_Grid>(k, field, sh, path, cons); break; }
+        case tParallel: { WM2D_Traits::ParallelGravity k; build_term(k, inp, G, &nd, &bc); k.SetConvectionScheme(koren); dump_term<WM2D_Grid>(k, field, sh, path, cons); break; }
+        case tCentrifugal: { WM2D_Traits::Centrifugal k; build_term(k, inp, G, &nd, &bc); k.SetConvectionScheme(koren); dump_term<WM2D_Grid>(k, field, sh, path, cons); break; }
+        case tSoluto: { WM2D_Traits::SolutoCapillarity k; build_term(k, inp, G, &nd, &bc); k.SetConvectionScheme(koren); dump_term<WM2D_Grid>(k, field, sh, path, cons); break; }
+        default: { WM2D_Traits::DisjoiningPressure k; build_term(k, inp, G, &nd, &bc); dump_term<WM2D_Grid>(k, field, sh, path, cons); break; }
+        }
+    } else {
+        Field_WM1D field;
+        field.Build(g, Flat(), nd.ExtentScale());
+        BoundaryCondition1D bc(field.GetGridPtr()->nx(), bk, bk);
+        C_Grid1D *G = field.GetGridPtr();
+        switch (t) {
+        case tLaplace: { WM1D_Traits::LaplacePressure k; build_term(k, inp, G, &nd, &bc); dump_term<WM1D_Grid>(k, field, sh, path, cons); break; }
+        case tVertical: { WM1D_Traits::VerticalGravity k; build_term(k, inp, G, &nd, &bc); dump_term<WM1D_Grid>(k, field, sh, path, cons); break; }
+        case tParallel: { WM1D_Traits::ParallelGravity k; build_term(k, inp, G, &nd, &bc); k.SetConvectionScheme(koren); dump_term<WM1D_Grid>(k, field, sh, path, cons); break; }
+        case tCentrifugal: { WM1D_Traits::Centrifugal k; build_term(k, inp, G, &nd, &bc); k.SetConvectionScheme(koren); dump_term<WM1D_Grid>(k, field, sh, path, cons); break; }
+        case tSoluto: { WM1D_Traits::SolutoCapillarity k; build_term(k, inp, G, &nd, &bc); k.SetConvectionScheme(koren); dump_term<WM1D_Grid>(k, field, sh, path, cons); break; }
+        default: { WM1D_Traits::DisjoiningPressure k; build_term(k, inp, G, &nd, &bc); dump_term<WM1D_Grid>(k, field, sh, path, cons); break; }
+        }
+    }
+    char buf[1024];
+    std::snprintf(buf, sizeof(buf),
+                  "%s\n  {\"name\": \"%s\", \"dim\": %d, \"nx\": %d, \"ny\": %d, \"term\": \"%s\", \"periodic\": %d, "
+                  "\"xmin\": -0.5, \"xmax\": 0.5, \"ymin\": %.17g, \"ymax\": %.17g, "
+                  "\"As\": %.17g, \"Ma\": %.17g, \"Bo\": %.17g, \"Pgx\": %.17g, \"Pgy\": %.17g, \"Sp\": %.17g, "
+                  "\"Dp\": %.17g, \"hstar\": %.17g, \"slip\": %.17g, \"rot_x\": %.17g, \"rot_y\": %.17g, "
+                  "\"visc_c0\": 0.5, \"visc_exp\": %.17g, \"sig_c0\": 0.5}",
+                  first ? "" : ",", name.str().c_str(), twoD ? 2 : 1, nx, twoD ? ny : 1, kName[t], periodic ? 1 : 0,
+                  twoD ? -0.4 : -1.0, twoD ? 0.4 : 1.0, nd.As(), nd.Ma(), nd.Bo(), nd.Pgx(), nd.Pgy(), nd.Sp(),
+                  nd.Dp(), nd.HStar(), slip, p.rotx, p.roty, p.visc_exp);
+    man << buf;
+    first = false;
+    return path;
+}
+
+void dump_mobility(const std::string &dir) {
+    // Mobility (Mobility.h) over heights, separations and slip lengths,
+    // both branches of FaceDK (series and closed form) included
+    std::vector<double> out;
+    const double lambdas[] = {0.0, 1e-6, 1e-3, 0.05, 0.5, 3.0};
+    const double hs[] = {1e-3, 0.02, 0.3, 1.0, 7.0};
+    const double rel[] = {0.0, 1e-5, 9e-4, 1.1e-3, 0.05, 0.7, 3.0, 100.0};
+    for (double l : lambdas) {
+        Mobility m;
+        m.Set(l);
+        for (double h1 : hs)
+            for (double r : rel) {
+                const double h2 = h1 * (1.0 + r);
+                out.push_back(l); out.push_back(h1); out.push_back(h2);
+                out.push_back(m.Face(h1, h2)); out.push_back(m.Face_d1(h1, h2)); out.push_back(m.Face_d1(h2, h1));
+                out.push_back(m.D(h1)); out.push_back(m.dD(h1)); out.push_back(m.DperH(h1));
+            }
+    }
+    std::ofstream f(dir + "/mobility.bin", std::ios::binary);
+    const int hdr[2] = {(int)out.size() / 9, 9};
+    f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    write_vec(f, out);
+}
+
+void dump_limiter(const std::string &dir) {
+    // Koren's limited face value, downwind weights and derivative
+    // (FiniteDifferences.cpp) over four-point stencils through every branch
+    C_Interpolation_base4 k;
+    k.Build(C_Interpolation_base4::Koren);
+    std::vector<double> out;
+    unsigned long long z = 12345;
+    auto rnd = [&]() {
+        z = z * 6364136223846793005ULL + 1442695040888963407ULL;
+        return ((z >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0;
+    };
+    for (int t = 0; t < 400; ++t) {
+        double f[4];
+        for (int j = 0; j < 4; ++j) f[j] = rnd();
+        if (t % 7 == 0) f[2] = f[1];           // a zero difference
+        if (t % 11 == 0) f[1] = 0.5 * (f[0] + f[2]);
+        const double sign = (t % 2 == 0) ? 1.0 : -1.0;
+        double w[4], g[4];
+        const double v = k.FaceValue(f, sign);
+        k.FaceWeights(f, sign, -1, w);
+        const double vd = k.FaceDerivative(f, sign, g);
+        for (int j = 0; j < 4; ++j) out.push_back(f[j]);
+        out.push_back(sign); out.push_back(v);
+        for (int j = 0; j < 4; ++j) out.push_back(w[j]);
+        out.push_back(vd);
+        for (int j = 0; j < 4; ++j) out.push_back(g[j]);
+    }
+    std::ofstream fo(dir + "/limiter.bin", std::ios::binary);
+    const int hdr[2] = {(int)out.size() / 15, 15};
+    fo.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    write_vec(fo, out);
+}
+
+void dump_dp(const std::string &dir) {
+    // Dp of NonDimensionalNumbers::Build over contact angle, H*, n, m
+    std::vector<double> out;
+    const double th[] = {5, 10, 15, 20, 40};
+    const double hst[] = {0.01, 0.05, 0.1};
+    const double nm[][2] = {{3, 2}, {4, 3}, {9, 3}};
+    for (double a : th)
+        for (double hstar : hst)
+            for (auto &e : nm) {
+                Params p;
+                p.theta_deg = a; p.hstar = hstar;
+                Input inp = MakeInput(tDisjoining, p);
+                inp.Physics.DisjoiningPressure.ExponentN = e[0];
+                inp.Physics.DisjoiningPressure.ExponentM = e[1];
+                NonDimensionalNumbers nd;
+                nd.Build(inp, 1.0);
+                out.push_back(a); out.push_back(hstar); out.push_back(e[0]); out.push_back(e[1]);
+                out.push_back(nd.Dp());
+            }
+    std::ofstream f(dir + "/dp.bin", std::ios::binary);
+    const int hdr[2] = {(int)out.size() / 5, 5};
+    f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    write_vec(f, out);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc < 2) { std::fprintf(stderr, "usage: ref_golden OUTDIR\n"); return 1; }
+    const std::string dir = argv[1];
+    std::ofstream man(dir + "/manifest.json");
+    man << "{\"cases\": [";
+    bool first = true;
+    const Term all[] = {tLaplace, tVertical, tParallel, tCentrifugal, tSoluto, tDisjoining};
+    for (int d2 = 0; d2 < 2; ++d2)
+        for (Term t : all)
+            for (double slip : {0.0, 0.05})
+                run_case(dir, d2 == 1, t, slip, false, man, first);
+    for (int d2 = 0; d2 < 2; ++d2)
+        for (Term t : all) run_case(dir, d2 == 1, t, 0.05, true, man, first);
+    man << "\n]}\n";
+    dump_mobility(dir);
+    dump_limiter(dir);
+    dump_dp(dir);
+    std::printf("golden vectors written to %s\n", dir.c_str());
+    return 0;
+}

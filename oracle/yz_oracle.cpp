@@ -1510,6 +1510,21 @@ void yzo_assemble(void *p, const double *h_old, const double *h, double dt,
     if (rhs) std::copy(st->Rhs.begin(), st->Rhs.end(), rhs);
     st->dt_now = save_dt;
 }
+/* Mobility (for the tests against the reference's tables): out =
+ * Face(h1,h2), Face_d1(h1,h2), Face_d1(h2,h1), D(h1), dD(h1), DperH(h1) */
+void yzo_mobility(double lambda, double h1, double h2, double *out) {
+    yzo::Mobility m;
+    m.set(lambda);
+    out[0] = m.Face(h1, h2); out[1] = m.Face_d1(h1, h2); out[2] = m.Face_d1(h2, h1);
+    out[3] = m.D(h1); out[4] = m.dD(h1); out[5] = m.DperH(h1);
+}
+/* Koren limiter: out = value, weights[4], derivative value, g[4] */
+void yzo_limiter(const double *f, double sign, double *out) {
+    yzo::Limiter k;
+    out[0] = k.value(f, sign);
+    k.weights(f, sign, out + 1);
+    out[5] = k.derivative(f, sign, out + 6);
+}
 /* the backend contract on a caller's CRS */
 int yzo_linsolve(int n, int nnz, const int *rowptr, const int *col, const double *val,
                  const double *rhs, double *x, int precond, int scaling, int maxit, double eps,

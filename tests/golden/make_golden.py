@@ -1,0 +1,69 @@
+"""Writes the golden fixtures of tests/golden from the reference's own code.
+
+Runs oracle/_ref/ref_golden (built by oracle/build_ref.sh from the reference
+sources: its term kernels, Mobility, the Koren limiter and
+NonDimensionalNumbers, driven the way the reference's test_jacobian.cpp
+drives them) into a scratch directory and packs what it wrote:
+
+  ref_terms.npz   per case: CRS of A and J (rowptr/col/val), the fields
+                  h, s, c on every row, and A(h)h
+  ref_scalars.npz mobility table, limiter table, Dp table
+  manifest.json   the dimensionless parameters of every case
+
+    python tests/golden/make_golden.py
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+EXE = os.path.join(ROOT, "oracle", "_ref", "ref_golden")
+
+
+def read_case(path):
+    with open(path, "rb") as f:
+        n, nnz, nnzj, nxp = np.fromfile(f, dtype=np.int32, count=4)
+        rp = np.fromfile(f, dtype=np.int32, count=n + 1)
+        col = np.fromfile(f, dtype=np.int32, count=nnz)
+        av = np.fromfile(f, dtype=np.float64, count=nnz)
+        rpj = np.fromfile(f, dtype=np.int32, count=n + 1)
+        colj = np.fromfile(f, dtype=np.int32, count=nnzj)
+        jv = np.fromfile(f, dtype=np.float64, count=nnzj)
+        h, s, c, ax = (np.fromfile(f, dtype=np.float64, count=n) for _ in range(4))
+    return dict(rp=rp, col=col, av=av, rpj=rpj, colj=colj, jv=jv, h=h, s=s, c=c, ax=ax)
+
+
+def read_table(path):
+    with open(path, "rb") as f:
+        rows, cols = np.fromfile(f, dtype=np.int32, count=2)
+        return np.fromfile(f, dtype=np.float64, count=rows * cols).reshape(rows, cols)
+
+
+def main():
+    if not os.path.exists(EXE):
+        subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")])
+    with tempfile.TemporaryDirectory() as tmp:
+        subprocess.check_call([EXE, tmp], stdout=subprocess.DEVNULL)
+        man = json.load(open(os.path.join(tmp, "manifest.json")))
+        arrays = {}
+        for case in man["cases"]:
+            d = read_case(os.path.join(tmp, case["name"] + ".bin"))
+            for k, v in d.items():
+                arrays[f"{case['name']}__{k}"] = v
+        np.savez_compressed(os.path.join(HERE, "ref_terms.npz"), **arrays)
+        np.savez_compressed(os.path.join(HERE, "ref_scalars.npz"),
+                            mobility=read_table(os.path.join(tmp, "mobility.bin")),
+                            limiter=read_table(os.path.join(tmp, "limiter.bin")),
+                            dp=read_table(os.path.join(tmp, "dp.bin")))
+        with open(os.path.join(HERE, "manifest.json"), "w") as f:
+            json.dump(man, f, indent=1)
+    print("wrote", os.path.join(HERE, "ref_terms.npz"))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
