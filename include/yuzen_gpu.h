@@ -44,10 +44,12 @@ enum { YZ_BC_SYMMETRY = 0, YZ_BC_PERIODIC = 1, YZ_BC_PINNING = 2,
 /* Preconditioner of the height equation's Krylov solve.
  * YZ_PC_AMG is what PreconditionType = "ILU" selects in the reference's
  * AMGCL backend (smoothed aggregation + SPAI0, LinearSolverBackend.cpp:
- * 662-679); YZ_PC_JACOBI is "None"/"Jacobi" (damped Jacobi);
- * YZ_PC_LEARNED is the learned 13-point approximate inverse (no reference
- * counterpart), with AMG as the fallback when the stall test trips. */
-enum { YZ_PC_JACOBI = 0, YZ_PC_AMG = 1, YZ_PC_LEARNED = 2 };
+ * 662-679); YZ_PC_JACOBI is "None"/"Jacobi" (damped Jacobi).
+ * No reference counterpart: YZ_PC_LEARNED is the learned 13-point
+ * approximate inverse M alone; YZ_PC_LEARNED_AMG the multigrid with M as
+ * its finest smoother. Both run the FP32 inner solve, with the FP64 AMG
+ * solve as the fallback when the stall test trips. */
+enum { YZ_PC_JACOBI = 0, YZ_PC_AMG = 1, YZ_PC_LEARNED = 2, YZ_PC_LEARNED_AMG = 3 };
 
 /* Everything one well-mixed film-height case needs, already made
  * dimensionless (reference NonDimensionalNumbers::Build,
@@ -87,13 +89,22 @@ typedef struct yz_case {
     double tsc_change, tsc_criterion, tsc_floor;
     int tsc_keep;
     double dt_min, dt_max;
+
+    /* learned preconditioner (no reference counterpart): the network is
+     * trained in place on the first learn_passes Newton passes of the
+     * context, learn_iters Adam steps of rate learn_lr on learn_probes
+     * probe vectors each, and frozen afterwards (0 passes: use the weights
+     * as loaded by yz_learn_set_params or the initial ones) */
+    int learn_passes, learn_iters, learn_probes;
+    double learn_lr;
 } yz_case;
 
 /* Fill c with the reference's defaults (Input.cpp getin defaults):
  * theta 0.5, NewtonMaxIterations 30, NewtonMinIterations 1,
  * NewtonEpsilon 1e-10, NewtonEpsilonRel 1e-6, JacobianUpdateInterval 1,
  * MaxIterations 5000, Epsilon 1e-13, RowScaling on, AMG, EW and
- * extrapolation on, step control off. */
+ * extrapolation on, step control off; learned preconditioner trained on
+ * 2 passes of 100 Adam steps, rate 2e-3, 4 probes. */
 void yz_case_defaults(yz_case *c);
 
 const char *yz_last_error(void);
@@ -143,6 +154,45 @@ void yz_step_destroy(yz_step *st);
  * Any output pointer may be NULL. */
 int yz_step_assemble(yz_step *st, const double *h_old, const double *h,
                      double dt, double *ia, double *ax, double *rhs);
+
+/* ---- learned preconditioner (no reference counterpart) -----------------
+ * A per-cell network, 8 -> 64 -> 64 -> 13 with ReLU, gives the 13 weights
+ * of row i of an approximate inverse M of the row-scaled IA, times the row
+ * scale 1 / (1 + sum_{k != centre} |IA_ik|). Training minimises the probe
+ * loss L = (1 / (P N)) sum_p ||v_p - IA M v_p||^2 over Rademacher probes
+ * v_p; the network runs on the MFMA units in BF16 with FP32 accumulation.
+ * The weights are YZ_LEARN_NPARAM floats, packed W1[64][16] (columns 8..15
+ * zero), b1[64], W2[64][64], b2[64], W3[16][64] (rows 13..15 zero),
+ * b3[16] (entries 13..15 zero); row o of W holds the weights of output o.
+ * Features of cell i, with off = sum_{k != centre} |IA_ik| of the
+ * row-scaled row: 0.25 log10(off), the four near neighbours / off (x-,
+ * x+, y-, y+), the sum of the four far axial entries / off, h_i and
+ * c_i - visc_c0. Valid on a context whose precond is YZ_PC_LEARNED or
+ * YZ_PC_LEARNED_AMG. */
+enum { YZ_LEARN_NPARAM = 6288, YZ_LEARN_NFEAT = 8, YZ_LEARN_NOUT = 13 };
+int yz_learn_get_params(yz_step *st, float *params);
+int yz_learn_set_params(yz_step *st, const float *params);
+/* Assemble the row-scaled FP32 IA at the context's film h and step, with
+ * the explicit half at h (a Newton pass's system at its starting point),
+ * and keep it as the training matrix. Any output may be NULL: ia[13 N]
+ * (planes, row-scaled), feat[8 N], mscale[N]. */
+int yz_learn_sample(yz_step *st, float *ia, float *feat, float *mscale);
+/* Probe loss and its gradient (YZ_LEARN_NPARAM floats, padding entries 0)
+ * at the current weights on the sampled matrix; probes p = 0..nprobe-1 of
+ * the given seed. Entry i of probe p is +1 or -1 by bit 16 of
+ * mix(seed * 0x9E3779B1 ^ p * 0x85EBCA77 ^ i * 0xC2B2AE3D), mix(h) =
+ * (h ^= h >> 15, h *= 0x2C1B3C6D, h ^= h >> 12, h *= 0x297A2D39,
+ * h ^= h >> 15) in 32-bit arithmetic (1: +1). */
+int yz_learn_loss_grad(yz_step *st, int nprobe, unsigned seed, double *loss, float *grad);
+/* iters Adam steps (beta 0.9 / 0.999, eps 1e-8) of rate lr on the sampled
+ * matrix; loss before the first and before the last step */
+int yz_learn_train(yz_step *st, int iters, double lr, int nprobe, double *loss_first, double *loss_last);
+/* M (13 N planes) for the sampled matrix at the current weights */
+int yz_learn_apply(yz_step *st, float *M);
+/* counters: [0] Newton passes whose FP32 solve failed and were solved
+ * again in FP64, [1] Adam steps taken, [2] training milliseconds,
+ * [3] probe loss of the first training step, [4] of the last */
+int yz_learn_stats(yz_step *st, double *out5);
 
 /* ---- linear-solve seam -------------------------------------------------
  * Solve A x = b, A in CRS (rowptr[n+1], col[nnz], val[nnz]; any column

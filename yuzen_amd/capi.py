@@ -11,7 +11,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libyuzen_hip.so")
 
 BC_SYMMETRY, BC_PERIODIC, BC_PINNING, BC_FRESHINLET, BC_OUTLET = range(5)
-PC_JACOBI, PC_AMG, PC_LEARNED = range(3)
+PC_JACOBI, PC_AMG, PC_LEARNED, PC_LEARNED_AMG = range(4)
+LEARN_NPARAM, LEARN_NFEAT, LEARN_NOUT = 6288, 8, 13
 
 
 class YzCase(C.Structure):
@@ -41,6 +42,8 @@ class YzCase(C.Structure):
         ("tsc_change", C.c_double), ("tsc_criterion", C.c_double),
         ("tsc_floor", C.c_double), ("tsc_keep", C.c_int),
         ("dt_min", C.c_double), ("dt_max", C.c_double),
+        ("learn_passes", C.c_int), ("learn_iters", C.c_int),
+        ("learn_probes", C.c_int), ("learn_lr", C.c_double),
     ]
 
     def copy(self):
@@ -75,6 +78,7 @@ def case_defaults():
     c.lin_maxit, c.lin_eps, c.row_scaling, c.precond = 5000, 1e-13, 1, PC_AMG
     c.tsc_change, c.tsc_criterion, c.tsc_floor, c.tsc_keep = 0.5, 0.01, 0.0, 50
     c.dt_min, c.dt_max = 1e-15, 1e30
+    c.learn_passes, c.learn_iters, c.learn_probes, c.learn_lr = 2, 100, 4, 2e-3
     return c
 
 
@@ -113,6 +117,14 @@ def load():
                                 P(YzLinSettings), pi_, pd]
     lib.yz_memory_get.argtypes = [C.c_void_p, pd, i]
     lib.yz_memory_set.argtypes = [C.c_void_p, pd, i]
+    pf = P(C.c_float)
+    lib.yz_learn_get_params.argtypes = [C.c_void_p, pf]
+    lib.yz_learn_set_params.argtypes = [C.c_void_p, pf]
+    lib.yz_learn_sample.argtypes = [C.c_void_p, pf, pf, pf]
+    lib.yz_learn_loss_grad.argtypes = [C.c_void_p, i, C.c_uint, pd, pf]
+    lib.yz_learn_train.argtypes = [C.c_void_p, i, d, i, pd, pd]
+    lib.yz_learn_apply.argtypes = [C.c_void_p, pf]
+    lib.yz_learn_stats.argtypes = [C.c_void_p, pd]
     lib.yz_sizeof_case.restype = C.c_int
     lib.yz_sizeof_lin_settings.restype = C.c_int
     _lib = lib

@@ -38,6 +38,10 @@ struct RowArgs {
     double *ia_raw;            // optional: unscaled IA planes
     double *rhs_raw;           // optional: unscaled Rhs (N)
     double *ax_raw;            // optional: A(h)h (N) in mode 1
+    float *ia32;               // mode 1, FP32 inner solve: row-scaled IA planes (replaces ia)
+    float *b32;                // mode 1, FP32 inner solve: row-scaled Rhs (replaces b)
+    float *feat;               // optional: 8 x N features of the learned preconditioner
+    float *mscale;             // optional: 1 / (1 + sum_k!=c |IA_k|) of the scaled row
     unsigned long long *rmax;  // mode 1: max |Rhs| (bits of a non-negative double)
 };
 
@@ -396,9 +400,36 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
             }
             const double dg = IA[C0];
             const double inv = (A.row_scaling && fabs(dg) >= 1.0e-30) ? 1.0 / dg : 1.0;
+            if (A.ia32) {
 #pragma unroll
-            for (int k = 0; k < NS; ++k) A.ia[(size_t)k * g.N + i] = A.row_scaling ? IA[k] * inv : IA[k];
-            A.b[i] = A.row_scaling ? R * inv : R;
+                for (int k = 0; k < NS; ++k) A.ia32[(size_t)k * g.N + i] = (float)(A.row_scaling ? IA[k] * inv : IA[k]);
+                A.b32[i] = (float)(A.row_scaling ? R * inv : R);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NS; ++k) A.ia[(size_t)k * g.N + i] = A.row_scaling ? IA[k] * inv : IA[k];
+                A.b[i] = A.row_scaling ? R * inv : R;
+            }
+            if (A.feat) {
+                // the learned preconditioner's inputs: the stiffness and the
+                // shape of the scaled row (the face mobilities and spacings
+                // it carries), the film height and the concentration
+                double off = 0, far = 0;
+#pragma unroll
+                for (int k = 0; k < NS; ++k) if (k != C0) off += fabs(IA[k] * inv);
+#pragma unroll
+                for (int d = 0; d < ND; ++d) far += IA[face_slot(DIM, d, d % 2 == 0 ? 0 : 3)] * inv;
+                const double so = off > 1e-30 ? off : 1e-30;
+                const size_t N = g.N;
+                A.feat[0 * N + i] = (float)(0.25 * log10(so));
+                A.feat[1 * N + i] = (float)(IA[adj_slot(DIM, 0)] * inv / so);
+                A.feat[2 * N + i] = (float)(IA[adj_slot(DIM, 1)] * inv / so);
+                A.feat[3 * N + i] = (float)(DIM == 2 ? IA[adj_slot(DIM, 2)] * inv / so : 0.0);
+                A.feat[4 * N + i] = (float)(DIM == 2 ? IA[adj_slot(DIM, 3)] * inv / so : 0.0);
+                A.feat[5 * N + i] = (float)(far / so);
+                A.feat[6 * N + i] = (float)H[C0];
+                A.feat[7 * N + i] = (float)(Cc[C0] - A.visc_c0);
+                A.mscale[i] = (float)(1.0 / (1.0 + off));
+            }
         }
     }
     if (MODE == 1) {

@@ -1,7 +1,11 @@
 // yz_solver.h -- the device linear solver of the film-height step:
 // BiCGSTAB with right preconditioning over a structured stencil operator,
-// preconditioned by a structured smoothed-aggregation multigrid (SPAI0
-// smoothing, dense inverse on the coarsest level) or damped Jacobi.
+// preconditioned by a structured smoothed-aggregation multigrid, damped
+// Jacobi, or an explicit 13-point stencil M (the learned approximate
+// inverse), alone or as the multigrid's finest smoother. Templated on the
+// value type: FP64 for the plain method, FP32 for the inexact-Newton inner
+// solve (dot products and the Krylov scalars stay FP64, and so does the
+// coarsest level's dense solve).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <vector>
@@ -10,47 +14,46 @@
 
 namespace yz {
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
-// one multigrid level
+template <typename T>
 struct AmgLevel {
     StencilOp op;
-    double *val = nullptr;     // ns planes x N
-    double *minv = nullptr;    // SPAI0 diagonal
-    double *P = nullptr;       // 9 planes x N (1-D: 3): prolongator to the next level
-    double *R = nullptr;       // 81 planes x Nc (1-D: 9): P^T by coarse cell, for the restriction
-    double *AP = nullptr;      // 25 planes x N (1-D: 5): A P, set-up scratch
-    double *x = nullptr, *b = nullptr, *r = nullptr;
-    int ncx = 0, ncy = 0;      // next level's grid
+    T *val = nullptr;     // ns planes x N
+    T *minv = nullptr;    // SPAI0 diagonal
+    T *P = nullptr;       // 9 planes x N (1-D: 3): prolongator to the next level
+    T *R = nullptr;       // 81 planes x Nc (1-D: 9): P^T by coarse cell, for the restriction
+    T *AP = nullptr;      // 25 planes x N (1-D: 5): A P, set-up scratch
+    T *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr;
+    int ncx = 0, ncy = 0; // next level's grid
 };
 
+template <typename T>
 struct Amg {
-    std::vector<AmgLevel> lv;  // lv[0].val aliases the fine operator
-    double *coarse_inv = nullptr;   // dense inverse of the coarsest level
-    double *coarse_dense = nullptr; // dense matrix scratch
+    std::vector<AmgLevel<T>> lv;   // lv[0].val aliases the fine operator
+    double *coarse_inv = nullptr;  // dense inverse of the coarsest level (transposed), FP64
+    double *coarse_dense = nullptr;
     int nco = 0;
-    double *scal = nullptr;    // device scalars: [0] rho bound (as bits)
+    double *scal = nullptr;        // per level: Gershgorin bound of D^-1 A (bits)
     int built = 0;
 };
 
+template <typename T>
 struct Krylov {
     int N = 0;
-    double *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *s = nullptr,
-           *t = nullptr, *ph = nullptr, *sh = nullptr;
+    T *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr,
+      *ph = nullptr, *sh = nullptr;
     double *part = nullptr;    // reduction partials
     double *sc = nullptr;      // device scalars
     double *host_sc = nullptr; // pinned mirror
 };
 
-struct Solver {
+template <typename T>
+struct SolverT {
     StencilOp op0;             // the fine operator
-    double *A = nullptr;       // fine planes (row-scaled)
-    double *dinv_jac = nullptr;
-    Amg amg;
-    Krylov kr;
+    T *A = nullptr;            // fine planes (row-scaled)
+    T *dinv_jac = nullptr;
+    const T *M = nullptr;      // explicit 13-point preconditioner / finest smoother (planes), or null
+    Amg<T> amg;
+    Krylov<T> kr;
     hipStream_t stream = nullptr;
     int max_levels = 12;
     // launch timing of the fine-level stencil product of the iteration
@@ -59,23 +62,29 @@ struct Solver {
     hipEvent_t kev[2] = {nullptr, nullptr};
     long long kcount = 0;
     double kms = 0;
+    // the stall test of the learned preconditioner: give up when the
+    // residual has not fallen by stall_factor over stall_window iterations
+    int stall_window = 0;
+    double stall_factor = 0.5;
 };
 
-// allocate (idempotent for the same grid)
-int solver_init(Solver &S, const StencilOp &op0, hipStream_t stream);
-void solver_free(Solver &S);
-// multigrid set-up on the current fine planes S.A
-int amg_setup(Solver &S);
-// Jacobi set-up
-int jacobi_setup(Solver &S);
-// BiCGSTAB: x (device, N) is the initial guess and the result; b (device);
-// precond YZ_PC_AMG / YZ_PC_JACOBI. Returns 0 / 1 / 2 as the backend
-// contract; iters and the relative residual out.
-int bicgstab(Solver &S, const double *b, double *x, int precond, int maxit,
-             double tol, int *iters, double *resid);
+typedef SolverT<double> Solver;
 
-// device helpers used by the step
-void launch_spmv(const StencilOp &op, const double *val, const double *x, double *y,
-                 hipStream_t st);
+// allocate (idempotent for the same grid)
+template <typename T> int solver_init(SolverT<T> &S, const StencilOp &op0, hipStream_t stream);
+template <typename T> void solver_free(SolverT<T> &S);
+// multigrid set-up on the current fine planes S.A
+template <typename T> int amg_setup(SolverT<T> &S);
+// Jacobi set-up
+template <typename T> int jacobi_setup(SolverT<T> &S);
+// BiCGSTAB: x (device, N) is the initial guess and the result; b (device);
+// precond YZ_PC_*. Returns 0 / 1 / 2 as the backend contract, 3 when the
+// stall test stopped it; iters and the relative residual out.
+template <typename T>
+int bicgstab(SolverT<T> &S, const T *b, T *x, int precond, int maxit, double tol, int *iters,
+             double *resid);
+
+template <typename T>
+void launch_spmv(const StencilOp &op, const T *val, const T *x, T *y, hipStream_t st);
 
 }  // namespace yz

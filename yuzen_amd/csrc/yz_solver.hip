@@ -101,9 +101,9 @@ __global__ void k_finish(const double *part, double *sc, int op) {
 // ---------------------------------------------------------------------
 // stencil products
 // ---------------------------------------------------------------------
-template <int NS>
-__device__ inline double row_dot(const StencilOp &op, const double *__restrict__ val,
-                                 const double *__restrict__ x, int i, int ix, int iy) {
+template <int NS, typename T>
+__device__ inline double row_dot(const StencilOp &op, const T *__restrict__ val,
+                                 const T *__restrict__ x, int i, int ix, int iy) {
     double s = 0;
     const int N = op.N;
     const int rx = op.dim == 2 ? op.nx : 0;
@@ -120,9 +120,9 @@ __device__ inline double row_dot(const StencilOp &op, const double *__restrict__
     return s;
 }
 
-template <int NS>
-__global__ void __launch_bounds__(BLK) k_spmv(StencilOp op, const double *__restrict__ val,
-                                              const double *__restrict__ x, double *__restrict__ y) {
+template <int NS, typename T>
+__global__ void __launch_bounds__(BLK) k_spmv(StencilOp op, const T *__restrict__ val,
+                                              const T *__restrict__ x, T *__restrict__ y) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= op.N) return;
     const int ix = i % op.nx, iy = i / op.nx;
@@ -138,12 +138,6 @@ static void dispatch_ns(int ns, F f) {
     }
 }
 
-void launch_spmv(const StencilOp &op, const double *val, const double *x, double *y, hipStream_t st) {
-    dispatch_ns(op.ns, [&](auto c) {
-        constexpr int NS = decltype(c)::value;
-        hipLaunchKernelGGL(k_spmv<NS>, dim3(nblk(op.N)), dim3(BLK), 0, st, op, val, x, y);
-    });
-}
 
 // ---------------------------------------------------------------------
 // BiCGSTAB kernels. Device scalars sc[]:
@@ -153,10 +147,10 @@ void launch_spmv(const StencilOp &op, const double *val, const double *x, double
 enum { SC_RHO1, SC_RHO2, SC_ALPHA, SC_OMEGA, SC_BETA, SC_RV, SC_SS, SC_TT, SC_TS, SC_RR, SC_BB,
        SC_RRH, SC_N };
 
-template <int NS>
-__global__ void __launch_bounds__(BLK) k_init_residual(StencilOp op, const double *__restrict__ val,
-                                                       const double *__restrict__ x, const double *__restrict__ b,
-                                                       double *__restrict__ r, double *__restrict__ rh,
+template <int NS, typename T>
+__global__ void __launch_bounds__(BLK) k_init_residual(StencilOp op, const T *__restrict__ val,
+                                                       const T *__restrict__ x, const T *__restrict__ b,
+                                                       T *__restrict__ r, T *__restrict__ rh,
                                                        double *__restrict__ part) {
     double acc[2] = {0, 0};
     for (int i = blockIdx.x * BLK + threadIdx.x; i < op.N; i += RED_BLOCKS * BLK) {
@@ -169,8 +163,9 @@ __global__ void __launch_bounds__(BLK) k_init_residual(StencilOp op, const doubl
     block_sum_store<2>(acc, part, RED_BLOCKS);
 }
 
-__global__ void k_p_update(int N, int first, const double *__restrict__ r, double *__restrict__ p,
-                           const double *__restrict__ v, const double *__restrict__ sc) {
+template <typename T>
+__global__ void k_p_update(int N, int first, const T *__restrict__ r, T *__restrict__ p,
+                           const T *__restrict__ v, const double *__restrict__ sc) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= N) return;
     if (first) { p[i] = r[i]; return; }
@@ -178,10 +173,10 @@ __global__ void k_p_update(int N, int first, const double *__restrict__ r, doubl
     p[i] = r[i] + beta * (p[i] - omega * v[i]);
 }
 
-template <int NS>
-__global__ void __launch_bounds__(BLK) k_spmv_dot(StencilOp op, const double *__restrict__ val,
-                                                  const double *__restrict__ x, double *__restrict__ y,
-                                                  const double *__restrict__ w, double *__restrict__ part) {
+template <int NS, typename T>
+__global__ void __launch_bounds__(BLK) k_spmv_dot(StencilOp op, const T *__restrict__ val,
+                                                  const T *__restrict__ x, T *__restrict__ y,
+                                                  const T *__restrict__ w, double *__restrict__ part) {
     double acc[1] = {0};
     for (int i = blockIdx.x * BLK + threadIdx.x; i < op.N; i += RED_BLOCKS * BLK) {
         const int ix = i % op.nx, iy = i / op.nx;
@@ -192,8 +187,9 @@ __global__ void __launch_bounds__(BLK) k_spmv_dot(StencilOp op, const double *__
     block_sum_store<1>(acc, part, RED_BLOCKS);
 }
 
-__global__ void k_s_update(int N, const double *__restrict__ r, const double *__restrict__ v,
-                           double *__restrict__ s, const double *__restrict__ sc, double *__restrict__ part) {
+template <typename T>
+__global__ void k_s_update(int N, const T *__restrict__ r, const T *__restrict__ v,
+                           T *__restrict__ s, const double *__restrict__ sc, double *__restrict__ part) {
     double acc[1] = {0};
     const double alpha = sc[SC_ALPHA];
     for (int i = blockIdx.x * BLK + threadIdx.x; i < N; i += RED_BLOCKS * BLK) {
@@ -204,10 +200,10 @@ __global__ void k_s_update(int N, const double *__restrict__ r, const double *__
     block_sum_store<1>(acc, part, RED_BLOCKS);
 }
 
-template <int NS>
-__global__ void __launch_bounds__(BLK) k_spmv_dot2(StencilOp op, const double *__restrict__ val,
-                                                   const double *__restrict__ x, double *__restrict__ t,
-                                                   const double *__restrict__ s, double *__restrict__ part) {
+template <int NS, typename T>
+__global__ void __launch_bounds__(BLK) k_spmv_dot2(StencilOp op, const T *__restrict__ val,
+                                                   const T *__restrict__ x, T *__restrict__ t,
+                                                   const T *__restrict__ s, double *__restrict__ part) {
     double acc[2] = {0, 0};
     for (int i = blockIdx.x * BLK + threadIdx.x; i < op.N; i += RED_BLOCKS * BLK) {
         const int ix = i % op.nx, iy = i / op.nx;
@@ -219,10 +215,11 @@ __global__ void __launch_bounds__(BLK) k_spmv_dot2(StencilOp op, const double *_
     block_sum_store<2>(acc, part, RED_BLOCKS);
 }
 
-__global__ void k_xr_update(int N, double *__restrict__ x, const double *__restrict__ ph,
-                            const double *__restrict__ sh, double *__restrict__ r,
-                            const double *__restrict__ s, const double *__restrict__ t,
-                            const double *__restrict__ rh, const double *__restrict__ sc,
+template <typename T>
+__global__ void k_xr_update(int N, T *__restrict__ x, const T *__restrict__ ph,
+                            const T *__restrict__ sh, T *__restrict__ r,
+                            const T *__restrict__ s, const T *__restrict__ t,
+                            const T *__restrict__ rh, const double *__restrict__ sc,
                             double *__restrict__ part) {
     double acc[2] = {0, 0};
     const double alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
@@ -236,7 +233,8 @@ __global__ void k_xr_update(int N, double *__restrict__ x, const double *__restr
     block_sum_store<2>(acc, part, RED_BLOCKS);
 }
 
-__global__ void k_x_alpha(int N, double *__restrict__ x, const double *__restrict__ ph,
+template <typename T>
+__global__ void k_x_alpha(int N, T *__restrict__ x, const T *__restrict__ ph,
                           const double *__restrict__ sc) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= N) return;
@@ -273,8 +271,8 @@ __device__ inline void apply_scalars(int op, double *sc, const double *sums) {
 // ---------------------------------------------------------------------
 // Jacobi (the reference's damped_jacobi preconditioner: 0.72 D^-1)
 // ---------------------------------------------------------------------
-template <int NS>
-__global__ void k_diag(StencilOp op, const double *__restrict__ val, double *__restrict__ dg) {
+template <int NS, typename T>
+__global__ void k_diag(StencilOp op, const T *__restrict__ val, T *__restrict__ dg) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= op.N) return;
     const int ix = i % op.nx, iy = i / op.nx;
@@ -284,12 +282,14 @@ __global__ void k_diag(StencilOp op, const double *__restrict__ val, double *__r
         if (stencil_col(op, ix, iy, k) == i) d += val[(size_t)k * op.N + i];
     dg[i] = d;
 }
-__global__ void k_jacobi_inv(int N, double *dg) {
+template <typename T>
+__global__ void k_jacobi_inv(int N, T *dg) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= N) return;
     dg[i] = dg[i] != 0 ? 0.72 / dg[i] : 0.0;
 }
-__global__ void k_scale(int N, const double *__restrict__ d, const double *__restrict__ b, double *__restrict__ x) {
+template <typename T>
+__global__ void k_scale(int N, const T *__restrict__ d, const T *__restrict__ b, T *__restrict__ x) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= N) return;
     x[i] = d[i] * b[i];
@@ -299,8 +299,8 @@ __global__ void k_scale(int N, const double *__restrict__ d, const double *__res
 // multigrid set-up
 // ---------------------------------------------------------------------
 // merge the slots of row i that fold onto the same column; returns count
-template <int NS>
-__device__ inline int merged_row(const StencilOp &op, const double *val, int i, int ix, int iy,
+template <int NS, typename T>
+__device__ inline int merged_row(const StencilOp &op, const T *val, int i, int ix, int iy,
                                  int *cols, double *vals) {
     int m = 0;
 #pragma unroll
@@ -316,9 +316,9 @@ __device__ inline int merged_row(const StencilOp &op, const double *val, int i, 
 }
 
 // SPAI0 diagonal, true diagonal and the Gershgorin bound of D^-1 A
-template <int NS>
-__global__ void __launch_bounds__(BLK) k_level_diag(StencilOp op, const double *__restrict__ val, double *__restrict__ minv,
-                             double *__restrict__ dg, unsigned long long *__restrict__ rho_bits) {
+template <int NS, typename T>
+__global__ void __launch_bounds__(BLK) k_level_diag(StencilOp op, const T *__restrict__ val, T *__restrict__ minv,
+                             T *__restrict__ dg, unsigned long long *__restrict__ rho_bits) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     double g = 0;
     if (i < op.N) {
@@ -390,10 +390,10 @@ __device__ inline int agg_offset(int i, int J, int n, int periodic) {
 // per fine cell (P[e][i], for the prolongation) and per coarse cell
 // (R[u][J] = P(i, J), u the fine offset in [-3,5]^DIM from aggregate J,
 // for the restriction)
-template <int NS, int DIM>
-__global__ void __launch_bounds__(BLK) k_build_P(StencilOp op, const double *__restrict__ val, const double *__restrict__ dg,
+template <int NS, int DIM, typename T>
+__global__ void __launch_bounds__(BLK) k_build_P(StencilOp op, const T *__restrict__ val, const T *__restrict__ dg,
                           const unsigned long long *__restrict__ rho_bits, int ncx, int ncy,
-                          double *__restrict__ P, double *__restrict__ R) {
+                          T *__restrict__ P, T *__restrict__ R) {
     constexpr int NP = DIM == 2 ? 9 : 3;
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= op.N) return;
@@ -440,9 +440,9 @@ __global__ void __launch_bounds__(BLK) k_build_P(StencilOp op, const double *__r
 }
 
 // A P (i, agg(i) + f), f in [-2,2]^DIM
-template <int NS, int DIM>
-__global__ void __launch_bounds__(BLK) k_build_AP(StencilOp op, const double *__restrict__ val, const double *__restrict__ P,
-                           int ncx, int ncy, double *__restrict__ AP) {
+template <int NS, int DIM, typename T>
+__global__ void __launch_bounds__(BLK) k_build_AP(StencilOp op, const T *__restrict__ val, const T *__restrict__ P,
+                           int ncx, int ncy, T *__restrict__ AP) {
     constexpr int NF = DIM == 2 ? 25 : 5;
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= op.N) return;
@@ -475,9 +475,9 @@ __global__ void __launch_bounds__(BLK) k_build_AP(StencilOp op, const double *__
 }
 
 // Galerkin product on the coarse grid: Ac(I, I + q), q in [-2,2]^DIM
-template <int DIM>
-__global__ void __launch_bounds__(BLK) k_build_RAP(StencilOp fop, const double *__restrict__ P, const double *__restrict__ AP,
-                            StencilOp cop, double *__restrict__ Ac) {
+template <int DIM, typename T>
+__global__ void __launch_bounds__(BLK) k_build_RAP(StencilOp fop, const T *__restrict__ P, const T *__restrict__ AP,
+                            StencilOp cop, T *__restrict__ Ac) {
     constexpr int NQ = DIM == 2 ? 25 : 5;
     const int I = blockIdx.x * BLK + threadIdx.x;
     if (I >= cop.N) return;
@@ -526,8 +526,8 @@ __global__ void __launch_bounds__(BLK) k_build_RAP(StencilOp fop, const double *
 }
 
 // dense coarsest matrix
-template <int NS>
-__global__ void k_dense(StencilOp op, const double *__restrict__ val, double *__restrict__ Dn) {
+template <int NS, typename T>
+__global__ void k_dense(StencilOp op, const T *__restrict__ val, double *__restrict__ Dn) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= op.N) return;
     const int ix = i % op.nx, iy = i / op.nx;
@@ -610,10 +610,10 @@ __global__ void __launch_bounds__(1024) k_gauss_jordan(int n, double *Ag, double
 // V-cycle kernels
 // ---------------------------------------------------------------------
 // pre-smoothing from zero and the residual: x = M b, r = b - A x
-template <int NS>
-__global__ void __launch_bounds__(BLK) k_pre(StencilOp op, const double *__restrict__ val,
-                                             const double *__restrict__ m, const double *__restrict__ b,
-                                             double *__restrict__ x, double *__restrict__ r) {
+template <int NS, typename T>
+__global__ void __launch_bounds__(BLK) k_pre(StencilOp op, const T *__restrict__ val,
+                                             const T *__restrict__ m, const T *__restrict__ b,
+                                             T *__restrict__ x, T *__restrict__ r) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= op.N) return;
     const int ix = i % op.nx, iy = i / op.nx;
@@ -631,10 +631,10 @@ __global__ void __launch_bounds__(BLK) k_pre(StencilOp op, const double *__restr
 // 16 x 16 coarse cells (256 in 1-D) stages the fine residual it needs
 // (a 56 x 56 tile; 776 cells in 1-D) in LDS, coalesced; each thread then
 // sums its 81 (9) weights, read coalesced, against the tile
-template <int DIM>
-__global__ void __launch_bounds__(256) k_restrict(StencilOp fop, const double *__restrict__ R,
-                                                  const double *__restrict__ r, StencilOp cop,
-                                                  double *__restrict__ bc) {
+template <int DIM, typename T>
+__global__ void __launch_bounds__(256) k_restrict(StencilOp fop, const T *__restrict__ R,
+                                                  const T *__restrict__ r, StencilOp cop,
+                                                  T *__restrict__ bc) {
     constexpr int TX = DIM == 2 ? 16 : 256, TY = DIM == 2 ? 16 : 1;
     constexpr int FX = 3 * TX + 8, FY = DIM == 2 ? 3 * TY + 8 : 1;
     __shared__ double tile[FY * FX];
@@ -672,8 +672,9 @@ __global__ void __launch_bounds__(256) k_restrict(StencilOp fop, const double *_
 }
 
 // prolongation x += P x_c
-__global__ void k_prolong(StencilOp fop, const double *__restrict__ P, StencilOp cop,
-                          const double *__restrict__ xc, double *__restrict__ x) {
+template <typename T>
+__global__ void k_prolong(StencilOp fop, const T *__restrict__ P, StencilOp cop,
+                          const T *__restrict__ xc, T *__restrict__ x) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= fop.N) return;
     const bool two = fop.dim == 2;
@@ -699,10 +700,10 @@ __global__ void k_prolong(StencilOp fop, const double *__restrict__ P, StencilOp
 }
 
 // post-smoothing, out of place: y = x + M (b - A x)
-template <int NS>
-__global__ void __launch_bounds__(BLK) k_post(StencilOp op, const double *__restrict__ val,
-                                              const double *__restrict__ m, const double *__restrict__ b,
-                                              const double *__restrict__ x, double *__restrict__ y) {
+template <int NS, typename T>
+__global__ void __launch_bounds__(BLK) k_post(StencilOp op, const T *__restrict__ val,
+                                              const T *__restrict__ m, const T *__restrict__ b,
+                                              const T *__restrict__ x, T *__restrict__ y) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= op.N) return;
     const int ix = i % op.nx, iy = i / op.nx;
@@ -711,8 +712,9 @@ __global__ void __launch_bounds__(BLK) k_post(StencilOp op, const double *__rest
 }
 
 // x = A^-1 b with the inverse stored transposed (coalesced over i)
-__global__ void k_dense_mv(int n, const double *__restrict__ AinvT, const double *__restrict__ b,
-                           double *__restrict__ x) {
+template <typename T>
+__global__ void k_dense_mv(int n, const double *__restrict__ AinvT, const T *__restrict__ b,
+                           T *__restrict__ x) {
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     double s = 0;
@@ -720,12 +722,32 @@ __global__ void k_dense_mv(int n, const double *__restrict__ AinvT, const double
     x[i] = s;
 }
 
+// residual r = b - A x, and the explicit-stencil smoothing step
+// out = xbase + M t (the learned approximate inverse as the finest smoother)
+template <int NS, typename T>
+__global__ void __launch_bounds__(BLK) k_resid(StencilOp op, const T *__restrict__ val, const T *__restrict__ x,
+                                               const T *__restrict__ b, T *__restrict__ r) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    r[i] = b[i] - row_dot<NS>(op, val, x, i, ix, iy);
+}
+template <int NS, typename T>
+__global__ void __launch_bounds__(BLK) k_spmv_add(StencilOp op, const T *__restrict__ val, const T *__restrict__ t,
+                                                  const T *__restrict__ xbase, T *__restrict__ out) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    out[i] = xbase[i] + row_dot<NS>(op, val, t, i, ix, iy);
+}
+
 // ---------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------
-static int dalloc(double **p, size_t n) {
+template <typename T>
+static int dalloc(T **p, size_t n) {
     if (*p) return 0;
-    YZ_CHECK(hipMalloc((void **)p, n * sizeof(double)));
+    YZ_CHECK(hipMalloc((void **)p, n * sizeof(T)));
     return 0;
 }
 
@@ -744,29 +766,38 @@ static StencilOp box_op(int dim, int nx, int ny, int perx, int pery) {
     return op;
 }
 
-int solver_init(Solver &S, const StencilOp &op0, hipStream_t stream) {
+template <typename T>
+void launch_spmv(const StencilOp &op, const T *val, const T *x, T *y, hipStream_t st) {
+    dispatch_ns(op.ns, [&](auto c) {
+        constexpr int NS = decltype(c)::value;
+        hipLaunchKernelGGL((k_spmv<NS, T>), dim3(nblk(op.N)), dim3(BLK), 0, st, op, val, x, y);
+    });
+}
+
+template <typename T>
+int solver_init(SolverT<T> &S, const StencilOp &op0, hipStream_t stream) {
     if (S.kr.N == op0.N && S.op0.ns == op0.ns) { S.op0 = op0; S.stream = stream; return 0; }
     solver_free(S);
     S.op0 = op0;
     S.stream = stream;
     const int N = op0.N;
     if (dalloc(&S.A, (size_t)op0.ns * N)) return -1;
-    Krylov &k = S.kr;
+    Krylov<T> &k = S.kr;
     k.N = N;
-    double **vs[] = {&k.r, &k.rh, &k.p, &k.v, &k.s, &k.t, &k.ph, &k.sh};
-    for (double **v : vs) if (dalloc(v, N)) return -1;
+    T **vs[] = {&k.r, &k.rh, &k.p, &k.v, &k.s, &k.t, &k.ph, &k.sh};
+    for (T **v : vs) if (dalloc(v, N)) return -1;
     if (dalloc(&k.part, 4 * RED_BLOCKS)) return -1;
     if (dalloc(&k.sc, 64)) return -1;
     YZ_CHECK(hipMemset(k.sc, 0, 64 * sizeof(double)));
     YZ_CHECK(hipHostMalloc((void **)&k.host_sc, 64 * sizeof(double), hipHostMallocDefault));
     // multigrid hierarchy shapes (fixed for the grid)
-    Amg &g = S.amg;
+    Amg<T> &g = S.amg;
     g.lv.clear();
-    AmgLevel L0;
+    AmgLevel<T> L0;
     L0.op = op0;
     g.lv.push_back(L0);
     while ((int)g.lv.size() < S.max_levels) {
-        AmgLevel &L = g.lv.back();
+        AmgLevel<T> &L = g.lv.back();
         if (L.op.N <= COARSE_MAX) break;
         const int ncx = (L.op.nx + 2) / 3;
         const int ncy = L.op.dim == 2 ? (L.op.ny + 2) / 3 : 1;
@@ -774,15 +805,15 @@ int solver_init(Solver &S, const StencilOp &op0, hipStream_t stream) {
         if (L.op.dim == 2 && L.op.pery && ncy < 5) break;
         if (ncx * ncy >= L.op.N) break;
         L.ncx = ncx; L.ncy = ncy;
-        AmgLevel C;
+        AmgLevel<T> C;
         C.op = box_op(L.op.dim, ncx, ncy, L.op.perx, L.op.pery);
         g.lv.push_back(C);
     }
     for (size_t l = 0; l < g.lv.size(); ++l) {
-        AmgLevel &L = g.lv[l];
+        AmgLevel<T> &L = g.lv[l];
         const int n = L.op.N;
         if (l > 0 && dalloc(&L.val, (size_t)L.op.ns * n)) return -1;
-        if (dalloc(&L.minv, n) || dalloc(&L.x, n) || dalloc(&L.b, n) || dalloc(&L.r, n)) return -1;
+        if (dalloc(&L.minv, n) || dalloc(&L.x, n) || dalloc(&L.b, n) || dalloc(&L.r, n) || dalloc(&L.t, n)) return -1;
         if (l + 1 < g.lv.size()) {
             const int np = L.op.dim == 2 ? 9 : 3, nf = L.op.dim == 2 ? 25 : 5;
             const int nr = L.op.dim == 2 ? 81 : 9;
@@ -790,7 +821,7 @@ int solver_init(Solver &S, const StencilOp &op0, hipStream_t stream) {
             const size_t rn = (size_t)nr * g.lv[l + 1].op.N;
             if (dalloc(&L.R, rn)) return -1;
             // weights of fine cells outside the grid are never written
-            YZ_CHECK(hipMemset(L.R, 0, rn * sizeof(double)));
+            YZ_CHECK(hipMemset(L.R, 0, rn * sizeof(T)));
         }
     }
     g.lv[0].val = S.A;
@@ -801,62 +832,66 @@ int solver_init(Solver &S, const StencilOp &op0, hipStream_t stream) {
     return 0;
 }
 
-void solver_free(Solver &S) {
-    auto f = [](double *&p) { if (p) { (void)hipFree(p); p = nullptr; } };
+template <typename T>
+void solver_free(SolverT<T> &S) {
+    auto f = [](auto *&p) { if (p) { (void)hipFree((void *)p); p = nullptr; } };
     for (size_t l = 0; l < S.amg.lv.size(); ++l) {
-        AmgLevel &L = S.amg.lv[l];
+        AmgLevel<T> &L = S.amg.lv[l];
         if (l > 0) f(L.val);
-        f(L.minv); f(L.P); f(L.R); f(L.AP); f(L.x); f(L.b); f(L.r);
+        f(L.minv); f(L.P); f(L.R); f(L.AP); f(L.x); f(L.b); f(L.r); f(L.t);
     }
     S.amg.lv.clear();
     f(S.amg.coarse_inv); f(S.amg.coarse_dense); f(S.amg.scal);
     f(S.A); f(S.dinv_jac);
-    Krylov &k = S.kr;
+    Krylov<T> &k = S.kr;
     f(k.r); f(k.rh); f(k.p); f(k.v); f(k.s); f(k.t); f(k.ph); f(k.sh); f(k.part); f(k.sc);
     for (auto &e : S.kev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (k.host_sc) { (void)hipHostFree(k.host_sc); k.host_sc = nullptr; }
     k.N = 0;
 }
 
-int amg_setup(Solver &S) {
-    Amg &g = S.amg;
+template <typename T>
+int amg_setup(SolverT<T> &S) {
+    Amg<T> &g = S.amg;
     hipStream_t st = S.stream;
     g.lv[0].val = S.A;
     YZ_CHECK(hipMemsetAsync(g.scal, 0, 2 * S.max_levels * sizeof(double), st));
     for (size_t l = 0; l < g.lv.size(); ++l) {
-        AmgLevel &L = g.lv[l];
+        AmgLevel<T> &L = g.lv[l];
         unsigned long long *rho = reinterpret_cast<unsigned long long *>(g.scal) + l;
         dispatch_ns(L.op.ns, [&](auto c) {
             constexpr int NS = decltype(c)::value;
             // L.r holds the true diagonal during the set-up
-            hipLaunchKernelGGL(k_level_diag<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv,
+            hipLaunchKernelGGL((k_level_diag<NS, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv,
                                L.r, rho);
         });
         if (l + 1 == g.lv.size()) break;
-        AmgLevel &C = g.lv[l + 1];
+        AmgLevel<T> &C = g.lv[l + 1];
         dispatch_ns(L.op.ns, [&](auto c) {
             constexpr int NS = decltype(c)::value;
             if (L.op.dim == 2) {
-                hipLaunchKernelGGL((k_build_P<NS, 2>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.r, rho,
-                                   L.ncx, L.ncy, L.P, L.R);
-                hipLaunchKernelGGL((k_build_AP<NS, 2>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.P,
+                hipLaunchKernelGGL((k_build_P<NS, 2, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.r,
+                                   rho, L.ncx, L.ncy, L.P, L.R);
+                hipLaunchKernelGGL((k_build_AP<NS, 2, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.P,
                                    L.ncx, L.ncy, L.AP);
             } else {
-                hipLaunchKernelGGL((k_build_P<NS, 1>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.r, rho,
-                                   L.ncx, L.ncy, L.P, L.R);
-                hipLaunchKernelGGL((k_build_AP<NS, 1>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.P,
+                hipLaunchKernelGGL((k_build_P<NS, 1, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.r,
+                                   rho, L.ncx, L.ncy, L.P, L.R);
+                hipLaunchKernelGGL((k_build_AP<NS, 1, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.P,
                                    L.ncx, L.ncy, L.AP);
             }
         });
         if (L.op.dim == 2)
-            hipLaunchKernelGGL(k_build_RAP<2>, dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.AP, C.op, C.val);
+            hipLaunchKernelGGL((k_build_RAP<2, T>), dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.AP, C.op,
+                               C.val);
         else
-            hipLaunchKernelGGL(k_build_RAP<1>, dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.AP, C.op, C.val);
+            hipLaunchKernelGGL((k_build_RAP<1, T>), dim3(nblk(C.op.N)), dim3(BLK), 0, st, L.op, L.P, L.AP, C.op,
+                               C.val);
     }
-    AmgLevel &Lc = g.lv.back();
+    AmgLevel<T> &Lc = g.lv.back();
     dispatch_ns(Lc.op.ns, [&](auto c) {
         constexpr int NS = decltype(c)::value;
-        hipLaunchKernelGGL(k_dense<NS>, dim3(nblk(Lc.op.N)), dim3(BLK), 0, st, Lc.op, Lc.val, g.coarse_dense);
+        hipLaunchKernelGGL((k_dense<NS, T>), dim3(nblk(Lc.op.N)), dim3(BLK), 0, st, Lc.op, Lc.val, g.coarse_dense);
     });
     hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), 0, st, g.nco, g.coarse_dense, g.coarse_inv);
     YZ_CHECK(hipGetLastError());
@@ -864,73 +899,103 @@ int amg_setup(Solver &S) {
     return 0;
 }
 
-int jacobi_setup(Solver &S) {
+template <typename T>
+int jacobi_setup(SolverT<T> &S) {
     dispatch_ns(S.op0.ns, [&](auto c) {
         constexpr int NS = decltype(c)::value;
-        hipLaunchKernelGGL(k_diag<NS>, dim3(nblk(S.op0.N)), dim3(BLK), 0, S.stream, S.op0, S.A, S.dinv_jac);
+        hipLaunchKernelGGL((k_diag<NS, T>), dim3(nblk(S.op0.N)), dim3(BLK), 0, S.stream, S.op0, S.A, S.dinv_jac);
     });
-    hipLaunchKernelGGL(k_jacobi_inv, dim3(nblk(S.op0.N)), dim3(BLK), 0, S.stream, S.op0.N, S.dinv_jac);
+    hipLaunchKernelGGL((k_jacobi_inv<T>), dim3(nblk(S.op0.N)), dim3(BLK), 0, S.stream, S.op0.N, S.dinv_jac);
     YZ_CHECK(hipGetLastError());
     return 0;
 }
 
-// one V-cycle: out = M^-1 in
-static void vcycle(Solver &S, const double *in, double *out) {
-    Amg &g = S.amg;
+// one V-cycle: out = M^-1 in; with smoother != null the finest level is
+// smoothed with that explicit stencil instead of SPAI0
+template <typename T>
+static void vcycle(SolverT<T> &S, const T *in, T *out, const T *smoother) {
+    Amg<T> &g = S.amg;
     hipStream_t st = S.stream;
     const int nl = (int)g.lv.size();
     if (nl == 1) {
-        hipLaunchKernelGGL(k_dense_mv, dim3(nblk(g.nco)), dim3(BLK), 0, st, g.nco, g.coarse_inv, in, out);
+        hipLaunchKernelGGL((k_dense_mv<T>), dim3(nblk(g.nco)), dim3(BLK), 0, st, g.nco, g.coarse_inv, in, out);
         return;
     }
     // downward
-    const double *b = in;
+    const T *b = in;
     for (int l = 0; l + 1 < nl; ++l) {
-        AmgLevel &L = g.lv[l];
-        AmgLevel &C = g.lv[l + 1];
-        dispatch_ns(L.op.ns, [&](auto c) {
-            constexpr int NS = decltype(c)::value;
-            hipLaunchKernelGGL(k_pre<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv, b, L.x, L.r);
-        });
+        AmgLevel<T> &L = g.lv[l];
+        AmgLevel<T> &C = g.lv[l + 1];
+        if (l == 0 && smoother) {
+            launch_spmv(L.op, smoother, b, L.x, st);
+            dispatch_ns(L.op.ns, [&](auto c) {
+                constexpr int NS = decltype(c)::value;
+                hipLaunchKernelGGL((k_resid<NS, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.x, b, L.r);
+            });
+        } else {
+            dispatch_ns(L.op.ns, [&](auto c) {
+                constexpr int NS = decltype(c)::value;
+                hipLaunchKernelGGL((k_pre<NS, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv, b, L.x,
+                                   L.r);
+            });
+        }
         if (L.op.dim == 2)
-            hipLaunchKernelGGL(k_restrict<2>, dim3((C.op.nx + 15) / 16, (C.op.ny + 15) / 16), dim3(256), 0, st, L.op,
-                               L.R, L.r, C.op, C.b);
+            hipLaunchKernelGGL((k_restrict<2, T>), dim3((C.op.nx + 15) / 16, (C.op.ny + 15) / 16), dim3(256), 0, st,
+                               L.op, L.R, L.r, C.op, C.b);
         else
-            hipLaunchKernelGGL(k_restrict<1>, dim3((C.op.nx + 255) / 256), dim3(256), 0, st, L.op, L.R, L.r, C.op,
-                               C.b);
+            hipLaunchKernelGGL((k_restrict<1, T>), dim3((C.op.nx + 255) / 256), dim3(256), 0, st, L.op, L.R, L.r,
+                               C.op, C.b);
         b = C.b;
     }
-    AmgLevel &Lc = g.lv[nl - 1];
-    hipLaunchKernelGGL(k_dense_mv, dim3(nblk(g.nco)), dim3(BLK), 0, st, g.nco, g.coarse_inv, Lc.b, Lc.x);
+    AmgLevel<T> &Lc = g.lv[nl - 1];
+    hipLaunchKernelGGL((k_dense_mv<T>), dim3(nblk(g.nco)), dim3(BLK), 0, st, g.nco, g.coarse_inv, Lc.b, Lc.x);
     // upward
     for (int l = nl - 2; l >= 0; --l) {
-        AmgLevel &L = g.lv[l];
-        AmgLevel &C = g.lv[l + 1];
-        const double *bl = (l == 0) ? in : L.b;
-        hipLaunchKernelGGL(k_prolong, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.P, C.op, C.x, L.x);
-        double *dst = (l == 0) ? out : L.r;   // r is free here
-        dispatch_ns(L.op.ns, [&](auto c) {
-            constexpr int NS = decltype(c)::value;
-            hipLaunchKernelGGL(k_post<NS>, dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv, bl, L.x, dst);
-        });
+        AmgLevel<T> &L = g.lv[l];
+        AmgLevel<T> &C = g.lv[l + 1];
+        const T *bl = (l == 0) ? in : L.b;
+        hipLaunchKernelGGL((k_prolong<T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.P, C.op, C.x, L.x);
+        T *dst = (l == 0) ? out : L.r;   // r is free here
+        if (l == 0 && smoother) {
+            dispatch_ns(L.op.ns, [&](auto c) {
+                constexpr int NS = decltype(c)::value;
+                hipLaunchKernelGGL((k_resid<NS, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.x, bl, L.t);
+                hipLaunchKernelGGL((k_spmv_add<NS, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, smoother, L.t,
+                                   L.x, dst);
+            });
+        } else {
+            dispatch_ns(L.op.ns, [&](auto c) {
+                constexpr int NS = decltype(c)::value;
+                hipLaunchKernelGGL((k_post<NS, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv, bl,
+                                   L.x, dst);
+            });
+        }
         if (l > 0) std::swap(L.x, L.r);
     }
 }
 
-static void precond_apply(Solver &S, int precond, const double *in, double *out) {
-    if (precond == YZ_PC_AMG) vcycle(S, in, out);
-    else hipLaunchKernelGGL(k_scale, dim3(nblk(S.op0.N)), dim3(BLK), 0, S.stream, S.op0.N, S.dinv_jac, in, out);
+template <typename T>
+static void precond_apply(SolverT<T> &S, int precond, const T *in, T *out) {
+    switch (precond) {
+    case YZ_PC_AMG: vcycle(S, in, out, (const T *)nullptr); break;
+    case YZ_PC_LEARNED_AMG: vcycle(S, in, out, S.M); break;
+    case YZ_PC_LEARNED: launch_spmv(S.op0, S.M, in, out, S.stream); break;
+    default:
+        hipLaunchKernelGGL((k_scale<T>), dim3(nblk(S.op0.N)), dim3(BLK), 0, S.stream, S.op0.N, S.dinv_jac, in, out);
+        break;
+    }
 }
 
-static int read_scalars(Solver &S) {
+template <typename T>
+static int read_scalars(SolverT<T> &S) {
     YZ_CHECK(hipMemcpyAsync(S.kr.host_sc, S.kr.sc, SC_N * sizeof(double), hipMemcpyDeviceToHost, S.stream));
     YZ_CHECK(hipStreamSynchronize(S.stream));
     return 0;
 }
 
-int bicgstab(Solver &S, const double *b, double *x, int precond, int maxit, double tol, int *iters,
-             double *resid) {
-    Krylov &k = S.kr;
+template <typename T>
+int bicgstab(SolverT<T> &S, const T *b, T *x, int precond, int maxit, double tol, int *iters, double *resid) {
+    Krylov<T> &k = S.kr;
     const StencilOp &op = S.op0;
     const int N = op.N;
     hipStream_t st = S.stream;
@@ -938,35 +1003,38 @@ int bicgstab(Solver &S, const double *b, double *x, int precond, int maxit, doub
     *resid = 0;
     dispatch_ns(op.ns, [&](auto c) {
         constexpr int NS = decltype(c)::value;
-        hipLaunchKernelGGL(k_init_residual<NS>, dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, x, b, k.r, k.rh, k.part);
+        hipLaunchKernelGGL((k_init_residual<NS, T>), dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, x, b, k.r, k.rh,
+                           k.part);
     });
     hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_INIT);
     if (read_scalars(S)) return 2;
     const double *h = k.host_sc;
     const double nb = std::sqrt(h[SC_BB]);
     if (nb == 0) {
-        YZ_CHECK(hipMemsetAsync(x, 0, N * sizeof(double), st));
+        YZ_CHECK(hipMemsetAsync(x, 0, N * sizeof(T), st));
         return 0;
     }
     const double eps = tol * nb;
     double res = std::sqrt(h[SC_RR]);
     if (res <= eps) { *resid = res / nb; return 0; }
+    double res_mark = res;
     int it = 0;
     for (; it < maxit; ++it) {
         if (it > 0 && std::fabs(h[SC_RHO1]) < 1e-300) { *iters = it; *resid = res / nb; return 2; }
-        hipLaunchKernelGGL(k_p_update, dim3(nblk(N)), dim3(BLK), 0, st, N, it == 0 ? 1 : 0, k.r, k.p, k.v, k.sc);
-        precond_apply(S, precond, k.p, k.ph);
+        hipLaunchKernelGGL((k_p_update<T>), dim3(nblk(N)), dim3(BLK), 0, st, N, it == 0 ? 1 : 0, k.r, k.p, k.v, k.sc);
+        precond_apply(S, precond, (const T *)k.p, k.ph);
         if (S.ktime) {
             if (!S.kev[0]) { (void)hipEventCreate(&S.kev[0]); (void)hipEventCreate(&S.kev[1]); }
             (void)hipEventRecord(S.kev[0], st);
         }
         dispatch_ns(op.ns, [&](auto c) {
             constexpr int NS = decltype(c)::value;
-            hipLaunchKernelGGL(k_spmv_dot<NS>, dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, k.ph, k.v, k.rh, k.part);
+            hipLaunchKernelGGL((k_spmv_dot<NS, T>), dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, k.ph, k.v, k.rh,
+                               k.part);
         });
         if (S.ktime) (void)hipEventRecord(S.kev[1], st);
         hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_ALPHA);
-        hipLaunchKernelGGL(k_s_update, dim3(RED_BLOCKS), dim3(BLK), 0, st, N, k.r, k.v, k.s, k.sc, k.part);
+        hipLaunchKernelGGL((k_s_update<T>), dim3(RED_BLOCKS), dim3(BLK), 0, st, N, k.r, k.v, k.s, k.sc, k.part);
         hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_SS);
         if (read_scalars(S)) return 2;
         if (S.ktime) {
@@ -976,29 +1044,49 @@ int bicgstab(Solver &S, const double *b, double *x, int precond, int maxit, doub
         if (h[SC_RV] == 0) { *iters = it; *resid = res / nb; return 2; }
         res = std::sqrt(h[SC_SS]);
         if (res < eps) {
-            hipLaunchKernelGGL(k_x_alpha, dim3(nblk(N)), dim3(BLK), 0, st, N, x, k.ph, k.sc);
+            hipLaunchKernelGGL((k_x_alpha<T>), dim3(nblk(N)), dim3(BLK), 0, st, N, x, k.ph, k.sc);
             ++it;
             break;
         }
-        precond_apply(S, precond, k.s, k.sh);
+        precond_apply(S, precond, (const T *)k.s, k.sh);
         dispatch_ns(op.ns, [&](auto c) {
             constexpr int NS = decltype(c)::value;
-            hipLaunchKernelGGL(k_spmv_dot2<NS>, dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, k.sh, k.t, k.s, k.part);
+            hipLaunchKernelGGL((k_spmv_dot2<NS, T>), dim3(RED_BLOCKS), dim3(BLK), 0, st, op, S.A, k.sh, k.t, k.s,
+                               k.part);
         });
         hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_OMEGA);
-        hipLaunchKernelGGL(k_xr_update, dim3(RED_BLOCKS), dim3(BLK), 0, st, N, x, k.ph, k.sh, k.r, k.s, k.t, k.rh,
-                           k.sc, k.part);
+        hipLaunchKernelGGL((k_xr_update<T>), dim3(RED_BLOCKS), dim3(BLK), 0, st, N, x, k.ph, k.sh, k.r, k.s, k.t,
+                           k.rh, k.sc, k.part);
         hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, st, k.part, k.sc, (int)FIN_NEXT);
         if (read_scalars(S)) return 2;
         if (h[SC_OMEGA] == 0) { *iters = it; *resid = res / nb; return 2; }
         res = std::sqrt(h[SC_RR]);
         if (res < eps) { ++it; break; }
         if (!(res == res)) { *iters = it; *resid = res; return 2; }
+        // stall test: the residual must fall by stall_factor every window
+        if (S.stall_window > 0 && (it + 1) % S.stall_window == 0) {
+            if (res > S.stall_factor * res_mark) { *iters = it + 1; *resid = res / nb; return 3; }
+            res_mark = res;
+        }
     }
     *iters = it;
     *resid = res / nb;
     if (!(*resid == *resid)) return 2;
     return (*resid <= tol) ? 0 : 1;
 }
+
+// the two value types
+template int solver_init<double>(SolverT<double> &, const StencilOp &, hipStream_t);
+template int solver_init<float>(SolverT<float> &, const StencilOp &, hipStream_t);
+template void solver_free<double>(SolverT<double> &);
+template void solver_free<float>(SolverT<float> &);
+template int amg_setup<double>(SolverT<double> &);
+template int amg_setup<float>(SolverT<float> &);
+template int jacobi_setup<double>(SolverT<double> &);
+template int jacobi_setup<float>(SolverT<float> &);
+template int bicgstab<double>(SolverT<double> &, const double *, double *, int, int, double, int *, double *);
+template int bicgstab<float>(SolverT<float> &, const float *, float *, int, int, double, int *, double *);
+template void launch_spmv<double>(const StencilOp &, const double *, const double *, double *, hipStream_t);
+template void launch_spmv<float>(const StencilOp &, const float *, const float *, float *, hipStream_t);
 
 }  // namespace yz

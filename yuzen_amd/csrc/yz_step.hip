@@ -28,6 +28,7 @@
 #include "../../include/yuzen_gpu.h"
 #include "yz_rows.h"
 #include "yz_solver.h"
+#include "yz_mlp.h"
 
 namespace yz {
 
@@ -137,6 +138,11 @@ __global__ void k_truncation(Geo g, const double *hm, const double *ho, const do
     }
 }
 
+__global__ void k_f2d(int n, const float *a, double *b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
 static double bits_to_double(unsigned long long b) {
     double d;
     std::memcpy(&d, &b, sizeof(d));
@@ -159,6 +165,11 @@ struct yz_step {
     unsigned long long *hmax = nullptr;   // pinned mirror
     int *hflag = nullptr;
     Solver S;
+    // the FP32 inner solve and the learned preconditioner
+    SolverT<float> S32;
+    float *b32 = nullptr, *corr32 = nullptr, *feat = nullptr, *mscale = nullptr, *M32 = nullptr;
+    Mlp mlp;
+    long long fallbacks = 0;
     double time = 0, dt_now = 0, dt_old = 0;
     int tsc_count = 0;
     bool extrap_blocked = false, first_step = true;
@@ -182,8 +193,13 @@ static int check_case(const yz_case &c) {
         set_err("a Periodic edge needs a Periodic opposite edge"); return 1;
     }
     if (c.jac_interval != 1) { set_err("JacobianUpdateInterval other than 1 is not implemented"); return 1; }
-    if (c.precond != YZ_PC_AMG && c.precond != YZ_PC_JACOBI) { set_err("unknown preconditioner"); return 1; }
-    if (c.krylov_fp32) { set_err("the FP32 Krylov path is not built in this library"); return 1; }
+    if (c.precond < YZ_PC_JACOBI || c.precond > YZ_PC_LEARNED_AMG) { set_err("unknown preconditioner"); return 1; }
+    if (c.dim != 2 && (c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG)) {
+        set_err("the learned preconditioner is built for the 2-D film"); return 1;
+    }
+    if (c.learn_passes > 0 && (c.learn_iters < 0 || c.learn_probes < 1 || !(c.learn_lr >= 0))) {
+        set_err("learn_iters >= 0, learn_probes >= 1 and learn_lr >= 0 are needed to train"); return 1;
+    }
     return 0;
 }
 
@@ -263,16 +279,35 @@ static double ev_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Solve (NewtonKantorovich.cpp:74-154): assemble, EW tolerance, solve
+static bool fp32_path(const yz_case &c) {
+    return c.krylov_fp32 || c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG;
+}
+
+// the FP32 inner solve cannot reach below this relative residual
+static constexpr double kEpsFP32 = 1.0e-6;
+
+// Solve (NewtonKantorovich.cpp:74-154): assemble, EW tolerance, solve.
+// With the FP32 inner solve the pass is an inexact-Newton step: the row
+// kernel writes FP32 planes, the Krylov vectors are FP32, the tolerance is
+// floored at kEpsFP32; the Newton residual, the assembly and the update stay
+// FP64. When the FP32 solve fails or its stall test trips, the pass is
+// assembled again in FP64 and solved with the FP64 AMG-BiCGSTAB.
 static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     *dev_err = 0;
     const yz_case &c = S->cs;
+    const bool f32 = fp32_path(c);
     S->ra.dt = S->dt_now;
     S->ra.axold = S->axold;
     S->ra.hold = S->hold;
+    RowArgs keep = S->ra;
+    if (f32) {
+        S->ra.ia32 = S->S32.A; S->ra.b32 = S->b32;
+        if (c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG) { S->ra.feat = S->feat; S->ra.mscale = S->mscale; }
+    }
     YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
     YZ_TRY(hipEventRecord(S->ev[0], S->st));
     rows_launch(S, 1, S->h);
+    S->ra = keep;
     YZ_TRY(hipGetLastError());
     YZ_TRY(hipEventRecord(S->ev[1], S->st));
     ++S->nk_pass;
@@ -295,15 +330,71 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
         eps = eta;
     }
     YZ_TRY(hipEventRecord(S->ev[2], S->st));
-    int rc;
-    if (c.precond == YZ_PC_AMG) rc = amg_setup(S->S); else rc = jacobi_setup(S->S);
-    if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
-    YZ_TRY(hipEventRecord(S->ev[3], S->st));
-    const int ch = bicgstab(S->S, S->b, S->corr, c.precond, c.lin_maxit, eps, iter, resid);
-    YZ_TRY(hipEventRecord(S->ev[4], S->st));
-    YZ_TRY(hipEventSynchronize(S->ev[4]));
-    S->tms[2] += ev_ms(S->ev[2], S->ev[3]);
-    S->tms[3] += ev_ms(S->ev[3], S->ev[4]);
+    int ch = -1;
+    if (f32) {
+        int rc = 0;
+        if (c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG) {
+            // in-place training on the first passes' own matrices, then frozen
+            if (S->mlp.trained_passes < c.learn_passes && c.learn_iters > 0) {
+                YZ_TRY(hipEventRecord(S->ev[8], S->st));
+                double l0 = 0, l1 = 0;
+                rc = mlp_train(S->mlp, S->S32, S->feat, S->mscale, c.learn_iters, c.learn_lr, c.learn_probes,
+                               0x1234u + 104729u * (unsigned)S->mlp.trained_passes, &l0, &l1, S->st);
+                YZ_TRY(hipEventRecord(S->ev[9], S->st));
+                YZ_TRY(hipEventSynchronize(S->ev[9]));
+                S->mlp.train_ms += ev_ms(S->ev[8], S->ev[9]);
+                if (S->mlp.trained_passes == 0) S->mlp.loss_first = l0;
+                S->mlp.loss_last = l1;
+                ++S->mlp.trained_passes;
+            }
+            if (!rc) rc = mlp_infer(S->mlp, S->g.N, S->feat, S->mscale, S->M32, S->st);
+            S->S32.M = S->M32;
+            S->S32.stall_window = 40;
+        }
+        if (!rc && (c.precond == YZ_PC_AMG || c.precond == YZ_PC_LEARNED_AMG)) rc = amg_setup(S->S32);
+        if (!rc && c.precond == YZ_PC_JACOBI) rc = jacobi_setup(S->S32);
+        if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
+        YZ_TRY(hipEventRecord(S->ev[3], S->st));
+        const double eps32 = eps > kEpsFP32 ? eps : kEpsFP32;
+        ch = bicgstab(S->S32, (const float *)S->b32, S->corr32, c.precond, c.lin_maxit, eps32, iter, resid);
+        YZ_TRY(hipEventRecord(S->ev[4], S->st));
+        YZ_TRY(hipEventSynchronize(S->ev[4]));
+        S->tms[2] += ev_ms(S->ev[2], S->ev[3]);
+        S->tms[3] += ev_ms(S->ev[3], S->ev[4]);
+        if (ch == 0) {
+            hipLaunchKernelGGL(k_f2d, dim3(nb(S->g.N)), dim3(TB), 0, S->st, S->g.N, (const float *)S->corr32, S->corr);
+            return 0;
+        }
+        // fallback: this pass again in FP64 with the AMG
+        ++S->fallbacks;
+        YZ_TRY(hipMemsetAsync(S->corr32, 0, S->g.N * sizeof(float), S->st));
+        YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+        rows_launch(S, 1, S->h);
+        YZ_TRY(hipGetLastError());
+        YZ_TRY(hipMemsetAsync(S->corr, 0, S->g.N * sizeof(double), S->st));
+        int it2 = 0;
+        double r2 = 0;
+        YZ_TRY(hipEventRecord(S->ev[2], S->st));
+        if (amg_setup(S->S)) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
+        YZ_TRY(hipEventRecord(S->ev[3], S->st));
+        ch = bicgstab(S->S, S->b, S->corr, YZ_PC_AMG, c.lin_maxit, eps, &it2, &r2);
+        YZ_TRY(hipEventRecord(S->ev[4], S->st));
+        YZ_TRY(hipEventSynchronize(S->ev[4]));
+        S->tms[2] += ev_ms(S->ev[2], S->ev[3]);
+        S->tms[3] += ev_ms(S->ev[3], S->ev[4]);
+        *iter += it2;
+        *resid = r2;
+    } else {
+        int rc;
+        if (c.precond == YZ_PC_AMG) rc = amg_setup(S->S); else rc = jacobi_setup(S->S);
+        if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
+        YZ_TRY(hipEventRecord(S->ev[3], S->st));
+        ch = bicgstab(S->S, S->b, S->corr, c.precond, c.lin_maxit, eps, iter, resid);
+        YZ_TRY(hipEventRecord(S->ev[4], S->st));
+        YZ_TRY(hipEventSynchronize(S->ev[4]));
+        S->tms[2] += ev_ms(S->ev[2], S->ev[3]);
+        S->tms[3] += ev_ms(S->ev[3], S->ev[4]);
+    }
     if (ch != 0) YZ_TRY(hipMemsetAsync(S->corr, 0, S->g.N * sizeof(double), S->st));
     return ch;
 }
@@ -404,6 +495,7 @@ void yz_case_defaults(yz_case *c) {
     c->lin_maxit = 5000; c->lin_eps = 1e-13; c->row_scaling = 1; c->precond = YZ_PC_AMG;
     c->tsc_change = 0.5; c->tsc_criterion = 0.01; c->tsc_floor = 0; c->tsc_keep = 50;
     c->dt_min = 1e-15; c->dt_max = 1e30;
+    c->learn_passes = 2; c->learn_iters = 100; c->learn_probes = 4; c->learn_lr = 2e-3;
 }
 
 const char *yz_last_error(void) { return g_err.c_str(); }
@@ -439,6 +531,17 @@ yz_step *yz_step_create(const yz_case *c, const double *h, const double *s, cons
     hipMemsetAsync(S->corr, 0, g.N * sizeof(double), S->st);
     if (hipStreamSynchronize(S->st) != hipSuccess) return fail("init");
     if (solver_init(S->S, fine_op(g), S->st)) return fail("solver allocation");
+    if (fp32_path(*c)) {
+        if (solver_init(S->S32, fine_op(g), S->st)) return fail("solver allocation");
+        for (float **p : {&S->b32, &S->corr32, &S->mscale})
+            if (hipMalloc((void **)p, g.N * sizeof(float)) != hipSuccess) return fail("hipMalloc");
+        if (hipMemset(S->corr32, 0, g.N * sizeof(float)) != hipSuccess) return fail("memset");
+        if (c->precond == YZ_PC_LEARNED || c->precond == YZ_PC_LEARNED_AMG) {
+            if (hipMalloc((void **)&S->feat, (size_t)8 * g.N * sizeof(float)) != hipSuccess) return fail("hipMalloc");
+            if (hipMalloc((void **)&S->M32, (size_t)g.ns * g.N * sizeof(float)) != hipSuccess) return fail("hipMalloc");
+            if (mlp_init(S->mlp, g.N, S->st)) return fail("learned preconditioner allocation");
+        }
+    }
     // row kernel arguments
     RowArgs &a = S->ra;
     std::memset(&a, 0, sizeof(a));
@@ -463,6 +566,10 @@ yz_step *yz_step_create(const yz_case *c, const double *h, const double *s, cons
 void yz_step_destroy(yz_step *S) {
     if (!S) return;
     solver_free(S->S);
+    solver_free(S->S32);
+    mlp_free(S->mlp);
+    for (float *p : {S->b32, S->corr32, S->feat, S->mscale, S->M32})
+        if (p) (void)hipFree(p);
     for (double *p : {S->h, S->hold, S->hold2, S->hnew, S->s, S->c, S->axold, S->b, S->corr, S->scratch})
         if (p) (void)hipFree(p);
     if (S->dmax) (void)hipFree(S->dmax);
@@ -642,6 +749,113 @@ int yz_step_assemble(yz_step *S, const double *h_old, const double *h, double dt
     YZ_TRY(hipMemcpyAsync(S->hold, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     YZ_TRY(hipMemcpyAsync(S->h, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+// ---- learned preconditioner ------------------------------------------
+
+static int learn_ctx(yz_step *S) {
+    if (!S) { set_err("null step"); return 2; }
+    if (S->cs.precond != YZ_PC_LEARNED && S->cs.precond != YZ_PC_LEARNED_AMG) {
+        set_err("the context was not created with a learned preconditioner"); return 2;
+    }
+    return 0;
+}
+
+int yz_learn_get_params(yz_step *S, float *params) {
+    if (int e = learn_ctx(S)) return e;
+    if (!params) { set_err("null argument"); return 2; }
+    YZ_TRY(hipMemcpyAsync(params, S->mlp.params, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+int yz_learn_set_params(yz_step *S, const float *params) {
+    if (int e = learn_ctx(S)) return e;
+    if (!params) { set_err("null argument"); return 2; }
+    std::vector<float> p(params, params + MLP_NP);
+    for (int o = 0; o < 64; ++o)
+        for (int i = MLP_IN; i < 16; ++i) p[MLP_W1 + o * 16 + i] = 0.0f;
+    for (int o = MLP_OUT; o < 16; ++o) {
+        for (int i = 0; i < 64; ++i) p[MLP_W3 + o * 64 + i] = 0.0f;
+        p[MLP_B3 + o] = 0.0f;
+    }
+    for (float v : p)
+        if (!std::isfinite(v)) { set_err("non-finite weight"); return 2; }
+    YZ_TRY(hipMemcpyAsync(S->mlp.params, p.data(), MLP_NP * sizeof(float), hipMemcpyHostToDevice, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+int yz_learn_sample(yz_step *S, float *ia, float *feat, float *mscale) {
+    if (int e = learn_ctx(S)) return e;
+    const Geo &g = S->g;
+    const int np = g.nxp * g.nyp;
+    // the Newton system at h = hnew, the step's current film: the explicit
+    // half also at h (into scratch), so the state of the step is untouched
+    // but for the pass-local iterate h
+    YZ_TRY(hipMemcpyAsync(S->h, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    RowArgs keep = S->ra;
+    S->ra.ax = S->scratch;
+    rows_launch(S, 0, S->h);
+    S->ra = keep;
+    S->ra.dt = S->dt_now; S->ra.axold = S->scratch; S->ra.hold = S->h;
+    S->ra.ia32 = S->S32.A; S->ra.b32 = S->b32; S->ra.feat = S->feat; S->ra.mscale = S->mscale;
+    YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+    rows_launch(S, 1, S->h);
+    S->ra = keep;
+    YZ_TRY(hipGetLastError());
+    const size_t N = g.N;
+    if (ia) YZ_TRY(hipMemcpyAsync(ia, S->S32.A, g.ns * N * sizeof(float), hipMemcpyDeviceToHost, S->st));
+    if (feat) YZ_TRY(hipMemcpyAsync(feat, S->feat, MLP_IN * N * sizeof(float), hipMemcpyDeviceToHost, S->st));
+    if (mscale) YZ_TRY(hipMemcpyAsync(mscale, S->mscale, N * sizeof(float), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+int yz_learn_loss_grad(yz_step *S, int nprobe, unsigned seed, double *loss, float *grad) {
+    if (int e = learn_ctx(S)) return e;
+    if (nprobe < 1) { set_err("nprobe must be positive"); return 2; }
+    double l = 0;
+    if (mlp_loss_grad(S->mlp, S->S32, S->feat, S->mscale, nprobe, seed, &l, S->st)) {
+        set_err("probe loss failed"); return 2;
+    }
+    if (grad) YZ_TRY(hipMemcpyAsync(grad, S->mlp.grads, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    if (loss) *loss = l;
+    return 0;
+}
+
+int yz_learn_train(yz_step *S, int iters, double lr, int nprobe, double *loss_first, double *loss_last) {
+    if (int e = learn_ctx(S)) return e;
+    if (iters < 1 || nprobe < 1 || !(lr >= 0)) { set_err("iters, nprobe >= 1 and lr >= 0 are needed"); return 2; }
+    YZ_TRY(hipEventRecord(S->ev[8], S->st));
+    if (mlp_train(S->mlp, S->S32, S->feat, S->mscale, iters, lr, nprobe, 0x51u + (unsigned)S->mlp.adam_t,
+                  loss_first, loss_last, S->st)) {
+        set_err("training failed"); return 2;
+    }
+    YZ_TRY(hipEventRecord(S->ev[9], S->st));
+    YZ_TRY(hipEventSynchronize(S->ev[9]));
+    S->mlp.train_ms += ev_ms(S->ev[8], S->ev[9]);
+    return 0;
+}
+
+int yz_learn_apply(yz_step *S, float *M) {
+    if (int e = learn_ctx(S)) return e;
+    if (!M) { set_err("null argument"); return 2; }
+    if (mlp_infer(S->mlp, S->g.N, S->feat, S->mscale, S->M32, S->st)) { set_err("inference failed"); return 2; }
+    YZ_TRY(hipMemcpyAsync(M, S->M32, (size_t)S->g.ns * S->g.N * sizeof(float), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+int yz_learn_stats(yz_step *S, double *out5) {
+    if (!S || !out5) { set_err("null argument"); return 2; }
+    out5[0] = (double)S->fallbacks;
+    out5[1] = (double)S->mlp.adam_t;
+    out5[2] = S->mlp.train_ms;
+    out5[3] = S->mlp.loss_first;
+    out5[4] = S->mlp.loss_last;
     return 0;
 }
 

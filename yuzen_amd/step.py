@@ -20,6 +20,10 @@ def _pd(a):
     return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
 
 
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
 def _pi(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
@@ -93,6 +97,54 @@ class HipStep:
         if self.lib.yz_step_assemble(self.p, _pd(h_old), _pd(h), dt, _pd(ia), _pd(ax), _pd(rhs)):
             raise RuntimeError(capi.last_error())
         return ia.reshape(self.ns, self.N), ax, rhs
+
+    # ---- learned preconditioner (contexts with PC_LEARNED / PC_LEARNED_AMG)
+    def _check(self, rc, what):
+        if rc:
+            raise RuntimeError(what + ": " + capi.last_error())
+
+    def learn_params(self):
+        out = np.empty(capi.LEARN_NPARAM, dtype=np.float32)
+        self._check(self.lib.yz_learn_get_params(self.p, _pf(out)), "yz_learn_get_params")
+        return out
+
+    def learn_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == capi.LEARN_NPARAM
+        self._check(self.lib.yz_learn_set_params(self.p, _pf(p)), "yz_learn_set_params")
+
+    def learn_sample(self):
+        """The row-scaled FP32 IA planes (ns, N), features (8, N) and row
+        scales (N) of the Newton system at the current film."""
+        ia = np.empty((self.ns, self.N), dtype=np.float32)
+        feat = np.empty((capi.LEARN_NFEAT, self.N), dtype=np.float32)
+        ms = np.empty(self.N, dtype=np.float32)
+        self._check(self.lib.yz_learn_sample(self.p, _pf(ia), _pf(feat), _pf(ms)), "yz_learn_sample")
+        return ia, feat, ms
+
+    def learn_loss_grad(self, nprobe, seed):
+        g = np.empty(capi.LEARN_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.yz_learn_loss_grad(self.p, nprobe, seed, C.byref(loss), _pf(g)),
+                    "yz_learn_loss_grad")
+        return loss.value, g
+
+    def learn_train(self, iters, lr, nprobe):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.lib.yz_learn_train(self.p, iters, lr, nprobe, C.byref(a), C.byref(b)),
+                    "yz_learn_train")
+        return a.value, b.value
+
+    def learn_apply(self):
+        M = np.empty((self.ns, self.N), dtype=np.float32)
+        self._check(self.lib.yz_learn_apply(self.p, _pf(M)), "yz_learn_apply")
+        return M
+
+    def learn_stats(self):
+        """fallbacks, Adam steps, training ms, first and last probe loss"""
+        out = np.zeros(5)
+        self._check(self.lib.yz_learn_stats(self.p, _pd(out)), "yz_learn_stats")
+        return out
 
 
 class LinSolver:
