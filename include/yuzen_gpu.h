@@ -97,6 +97,10 @@ typedef struct yz_case {
      * as loaded by yz_learn_set_params or the initial ones) */
     int learn_passes, learn_iters, learn_probes;
     double learn_lr;
+    /* the FP32 inner solve's stall test: it gives up, and the pass is
+     * solved again in FP64 with the AMG, when its residual has not halved
+     * over stall_window iterations (0: no test) */
+    int stall_window;
 } yz_case;
 
 /* Fill c with the reference's defaults (Input.cpp getin defaults):
@@ -104,7 +108,7 @@ typedef struct yz_case {
  * NewtonEpsilon 1e-10, NewtonEpsilonRel 1e-6, JacobianUpdateInterval 1,
  * MaxIterations 5000, Epsilon 1e-13, RowScaling on, AMG, EW and
  * extrapolation on, step control off; learned preconditioner trained on
- * 2 passes of 100 Adam steps, rate 2e-3, 4 probes. */
+ * 2 passes of 100 Adam steps, rate 2e-3, 4 probes; stall window 40. */
 void yz_case_defaults(yz_case *c);
 
 const char *yz_last_error(void);
@@ -136,7 +140,8 @@ int yz_step_clock(yz_step *st, double *time, double *dt_now, double *dt_old);
  * [5] whole step */
 int yz_step_timing(yz_step *st, double *ms6);
 /* launch timing of the fine-level stencil product of the Krylov iteration
- * (the SpMV v = A p with its fused dot product), HIP events on the step's
+ * (the SpMV v = A p with its fused dot product) of the case's own solve
+ * (the FP32 inner solve when the case runs it), HIP events on the step's
  * stream: returns the launches timed and their total milliseconds since
  * the last reset; enable 1 resets and starts, 0 resets and stops, -1 only
  * reads */

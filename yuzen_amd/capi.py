@@ -44,6 +44,7 @@ class YzCase(C.Structure):
         ("dt_min", C.c_double), ("dt_max", C.c_double),
         ("learn_passes", C.c_int), ("learn_iters", C.c_int),
         ("learn_probes", C.c_int), ("learn_lr", C.c_double),
+        ("stall_window", C.c_int),
     ]
 
     def copy(self):
@@ -79,6 +80,7 @@ def case_defaults():
     c.tsc_change, c.tsc_criterion, c.tsc_floor, c.tsc_keep = 0.5, 0.01, 0.0, 50
     c.dt_min, c.dt_max = 1e-15, 1e30
     c.learn_passes, c.learn_iters, c.learn_probes, c.learn_lr = 2, 100, 4, 2e-3
+    c.stall_window = 40
     return c
 
 

@@ -361,14 +361,40 @@ __global__ void k_err(StencilOp op, const float *__restrict__ A, const float *__
         part[blockIdx.x] = s;
     }
 }
-// w = A^T e (each row scatters its entries into the columns it reads)
+// the cells i along one axis whose slot offset o folds onto cell j: the
+// direct one j - o, and at a Symmetry edge the mirrored ones -1 - j - o and
+// 2n - 1 - j - o (three distinct candidates, each kept when it lies in the
+// grid and really folds onto j); at a Periodic edge the one wrap
+__device__ inline int fold_sources(int j, int o, int n, int periodic, int *src) {
+    if (periodic) { int i = (j - o) % n; if (i < 0) i += n; src[0] = i; return 1; }
+    int m = 0;
+    const int cand[3] = {j - o, -1 - j - o, 2 * n - 1 - j - o};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int i = cand[c];
+        if (i >= 0 && i < n && fold_axis(i + o, n, 0) == j) src[m++] = i;
+    }
+    return m;
+}
+// w = A^T e by gathering, in a fixed order, the entries of every row that
+// reads column j (deterministic, no atomics)
 __global__ void k_transpose_apply(StencilOp op, const float *__restrict__ A, const float *__restrict__ e,
                                   float *__restrict__ w) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= op.N) return;
-    const int ix = j % op.nx, iy = j / op.nx;
-    const float ej = e[j];
-    for (int k = 0; k < op.ns; ++k) atomicAdd(&w[stencil_col(op, ix, iy, k)], A[(size_t)k * op.N + j] * ej);
+    const int jx = j % op.nx, jy = j / op.nx;
+    float s = 0;
+    for (int k = 0; k < op.ns; ++k) {
+        int sx[3], sy[3];
+        const int mx = fold_sources(jx, op.ox[k], op.nx, op.perx, sx);
+        const int my = op.dim == 2 ? fold_sources(jy, op.oy[k], op.ny, op.pery, sy) : (sy[0] = 0, 1);
+        for (int a = 0; a < my; ++a)
+            for (int b = 0; b < mx; ++b) {
+                const int i = sy[a] * op.nx + sx[b];
+                s += A[(size_t)k * op.N + i] * e[i];
+            }
+    }
+    w[j] = s;
 }
 // G[k][i] (+)= coef w_i v_col(i,k)
 __global__ void k_gacc(StencilOp op, const float *__restrict__ w, const float *__restrict__ v, float coef,
@@ -478,7 +504,6 @@ int mlp_loss_grad(Mlp &m, const SolverT<float> &S, const float *feat, const floa
                            (const float *)m.v, m.e, m.loss_part);
         hipLaunchKernelGGL(k_sum_part, dim3(1), dim3(MBLK), 0, st, LOSS_BLOCKS, (const double *)m.loss_part,
                            m.loss_dev, p > 0 ? 1 : 0);
-        MLP_CHECK(hipMemsetAsync(m.w, 0, N * sizeof(float), st));
         hipLaunchKernelGGL(k_transpose_apply, dim3(nbk), dim3(MBLK), 0, st, op, S.A, (const float *)m.e, m.w);
         hipLaunchKernelGGL(k_gacc, dim3(nbk), dim3(MBLK), 0, st, op, (const float *)m.w, (const float *)m.v, coef,
                            p == 0 ? 1 : 0, m.G);

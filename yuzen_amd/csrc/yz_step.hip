@@ -197,6 +197,7 @@ static int check_case(const yz_case &c) {
     if (c.dim != 2 && (c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG)) {
         set_err("the learned preconditioner is built for the 2-D film"); return 1;
     }
+    if (c.stall_window < 0) { set_err("stall_window must be >= 0"); return 1; }
     if (c.learn_passes > 0 && (c.learn_iters < 0 || c.learn_probes < 1 || !(c.learn_lr >= 0))) {
         set_err("learn_iters >= 0, learn_probes >= 1 and learn_lr >= 0 are needed to train"); return 1;
     }
@@ -349,8 +350,8 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
             }
             if (!rc) rc = mlp_infer(S->mlp, S->g.N, S->feat, S->mscale, S->M32, S->st);
             S->S32.M = S->M32;
-            S->S32.stall_window = 40;
         }
+        S->S32.stall_window = c.stall_window;
         if (!rc && (c.precond == YZ_PC_AMG || c.precond == YZ_PC_LEARNED_AMG)) rc = amg_setup(S->S32);
         if (!rc && c.precond == YZ_PC_JACOBI) rc = jacobi_setup(S->S32);
         if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
@@ -496,6 +497,7 @@ void yz_case_defaults(yz_case *c) {
     c->tsc_change = 0.5; c->tsc_criterion = 0.01; c->tsc_floor = 0; c->tsc_keep = 50;
     c->dt_min = 1e-15; c->dt_max = 1e30;
     c->learn_passes = 2; c->learn_iters = 100; c->learn_probes = 4; c->learn_lr = 2e-3;
+    c->stall_window = 40;
 }
 
 const char *yz_last_error(void) { return g_err.c_str(); }
@@ -690,9 +692,14 @@ int yz_step_clock(yz_step *S, double *t, double *dt_now, double *dt_old) {
 
 int yz_step_kernel_time(yz_step *S, int enable, long long *count, double *total_ms) {
     if (!S) { set_err("null step"); return 2; }
-    if (count) *count = S->S.kcount;
-    if (total_ms) *total_ms = S->S.kms;
-    if (enable >= 0) { S->S.ktime = enable; S->S.kcount = 0; S->S.kms = 0; }
+    // the solver in use: the FP32 inner solve when the case runs it
+    const bool f32 = fp32_path(S->cs);
+    int &on = f32 ? S->S32.ktime : S->S.ktime;
+    long long &n = f32 ? S->S32.kcount : S->S.kcount;
+    double &ms = f32 ? S->S32.kms : S->S.kms;
+    if (count) *count = n;
+    if (total_ms) *total_ms = ms;
+    if (enable >= 0) { on = enable; n = 0; ms = 0; }
     return 0;
 }
 
