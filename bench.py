@@ -1,24 +1,30 @@
 #!/usr/bin/env python3
-"""Throughput of the implicit film-height step on one MI355X.
+"""Throughput of the implicit film-height step on one MI355X: the learned-
+preconditioned FP32 inexact-Newton step against the plain FP64 AMG step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--no-cpu]
 
 Workload: the 2-D well-mixed all-physics film on 1024 x 1024 cells
 (``yuzen_amd.cases.allphys_2d``: capillary, disjoining, vertical and
 in-plane gravity, spin, Marangoni on a frozen concentration, slip; Symmetry
-walls; fixed dt; Newton tolerance 1e-8). A step is one implicit time step
-(``C_NewtonKantorovich::TimeAdvance``): every Newton pass assembles the
-Jacobian on the device, solves with BiCGSTAB preconditioned by the
-structured smoothed-aggregation multigrid, and updates the film. The film,
-substrate and concentration are resident in HBM before the timed region;
-W untimed steps run first, then exactly K steps are timed between two
-device synchronisations.
+walls; fixed dt 2e-6; Newton tolerance 1e-8). A step is one implicit time
+step (``C_NewtonKantorovich::TimeAdvance``): every Newton pass assembles the
+residual and Jacobian in FP64 on the device, solves for the correction and
+updates the film.
 
-Printed: one JSON line with the steps/s, the roofline of the dominant
-kernel (the fine-level stencil product of the Krylov iteration, timed with
-HIP events on its own stream over the timed region) and a CPU baseline:
-the FP64 CPU oracle on one host core over a bounded sample of the same
-workload.
+Two runs of the same case, each on a fresh context with the fields resident
+in HBM: W untimed steps, then exactly K steps timed between two device
+synchronisations.
+  * learned (the value): FP32 BiCGSTAB preconditioned by the learned
+    13-point approximate inverse; the network is trained in place on the
+    first Newton passes' own matrices (inside the warm-up), then frozen.
+  * plain: FP64 BiCGSTAB preconditioned by the structured SA-AMG.
+The line reports both, their ratio and how far apart their films end.
+
+Roofline: the dominant kernel of the learned run, the fine-level stencil
+product of the Krylov iteration (k_spmv_dot<13, float>), timed with HIP
+events on the step's own stream over the timed region. CPU baseline: the
+FP64 CPU oracle on one host core over a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -36,6 +42,7 @@ if ROOT not in sys.path:
 METRIC = "Newton steps/s, 1024² 2D film: learned-precond FP32 vs plain FP64 AMG @1e-8"
 HBM_PEAK_GBS = 8000.0
 DT = 2e-6
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmv_dot13_f32_allphys1024_traffic.json")
 
 
 def cpu_model():
@@ -55,7 +62,8 @@ def cpu_baseline(n, passes_per_step):
     of the first step of the same case (the explicit half A(h^n)h^n, the
     assembly of A and J, the multigrid set-up and the BiCGSTAB solve),
     scaled to steps/s with the Newton passes per step of the device run
-    (the two take the same passes: the parity tests pin the counts)."""
+    (the plain method takes the same passes on both: the parity tests pin
+    the counts)."""
     from yuzen_amd import cases
     from oracle import binding as ob
     c, h, s, cc, dt, _ = cases.allphys_2d(n=n, dt=DT)
@@ -76,93 +84,179 @@ def cpu_baseline(n, passes_per_step):
     }
 
 
+COPY_SRC = """
+import torch
+n = 1 << 28
+a = torch.empty(n, dtype=torch.float32, device="cuda")
+b = torch.empty_like(a)
+a.fill_(1.0)
+for _ in range(3):
+    b.copy_(a)
+e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+torch.cuda.synchronize()
+e0.record()
+for _ in range(20):
+    b.copy_(a)
+e1.record()
+torch.cuda.synchronize()
+print(2 * 4 * n / (e0.elapsed_time(e1) / 20 / 1e3) / 1e9)
+"""
+
+
+def copy_bandwidth():
+    """A plain device copy of 1 GiB (read + write), GB/s: what this box's
+    HBM gives a simple streaming kernel. Run in a child process (torch's
+    copy kernel, a known-size reference for the traffic counters too)."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", COPY_SRC], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr.strip()[-200:])
+    return round(float(r.stdout.strip().splitlines()[-1]), 1)
+
+
+def run(c, h, s, cc, steps, warmup):
+    from yuzen_amd.step import HipStep
+    g = HipStep(c, h, s, cc, DT)
+    for _ in range(warmup):
+        rc, nk, lin, r = g.advance()
+        if rc != 0:
+            raise SystemExit(f"warm-up step failed (rc={rc})")
+    g.kernel_time(1)
+    passes = lins = 0
+    phases = np.zeros(6)
+    g.h()  # synchronise
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rc, nk, lin, r = g.advance()
+        if rc != 0:
+            raise SystemExit(f"timed step failed (rc={rc})")
+        passes += nk
+        lins += lin
+        phases += g.timing()
+    hfinal = g.h()  # synchronise
+    el = time.perf_counter() - t0
+    kcount, kms = g.kernel_time(0)
+    learn = g.learn_stats()
+    g.close()
+    return {
+        "el": el, "passes": passes, "lins": lins, "phases": phases, "kcount": kcount, "kms": kms,
+        "h": hfinal, "learn": learn,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the step is single-GPU and no multi-GPU figure is produced", file=sys.stderr)
         sys.exit(2)
 
-    from yuzen_amd import cases
-    from yuzen_amd.step import HipStep
+    from yuzen_amd import capi, cases
 
     c, h, s, cc, dt, _ = cases.allphys_2d(n=args.n, dt=DT)
-    step = HipStep(c, h, s, cc, dt)
-    for _ in range(args.warmup):
-        rc, nk, lin, r = step.advance()
-        if rc != 0:
-            raise SystemExit(f"warm-up step failed (rc={rc})")
-    step.kernel_time(1)
-    passes = lins = 0
-    phases = np.zeros(6)
-    step.h()  # synchronise
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rc, nk, lin, r = step.advance()
-        if rc != 0:
-            raise SystemExit(f"timed step failed (rc={rc})")
-        passes += nk
-        lins += lin
-        phases += step.timing()
-    hfinal = step.h()  # synchronise
-    el = time.perf_counter() - t0
-    kcount, kms = step.kernel_time(0)
-    steps_per_s = args.steps / el
+    cl = c.copy()
+    cl.precond, cl.krylov_fp32 = capi.PC_LEARNED, 1
+    cl.learn_passes, cl.learn_iters, cl.learn_probes, cl.learn_lr = 3, 150, 4, 2e-3
+    L = run(cl, h, s, cc, args.steps, args.warmup)
+    P = None
+    if not args.no_plain:
+        cp = c.copy()
+        cp.precond, cp.krylov_fp32 = capi.PC_AMG, 0
+        P = run(cp, h, s, cc, args.steps, args.warmup)
 
     N = c.nx * c.ny
-    # algorithmic bytes of one launch of the fine-level product
-    # v = A p, (rh, v): 13 planes of FP64 values, p once, rh once, v written
-    bytes_per_launch = N * (13 * 8 + 8 + 8 + 8)
-    avg_s = (kms / 1e3) / max(kcount, 1)
-    achieved = bytes_per_launch / avg_s / 1e9 if kcount else 0.0
+    # algorithmic bytes of one launch of the fine-level product v = A p,
+    # (rh, v): 13 FP32 planes, p once, rh once, v written
+    bytes_per_launch = N * (13 * 4 + 4 + 4 + 4)
+    avg_s = (L["kms"] / 1e3) / max(L["kcount"], 1)
+    achieved = bytes_per_launch / avg_s / 1e9 if L["kcount"] else 0.0
+    traffic, traffic_src = None, None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get("n") == args.n:
+            traffic = t.get("bytes_per_launch")
+            traffic_src = t.get("source")
     roof = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "kernel": "yz::k_spmv_dot<13> (fine-level stencil SpMV + fused dot)",
-        "launches": kcount,
+        "traffic": traffic,
+        "kernel": "yz::k_spmv_dot<13, float> (fine-level stencil SpMV + fused dot, FP32)",
+        "launches": L["kcount"],
         "avg_us": round(avg_s * 1e6, 2),
         "bytes_per_launch": bytes_per_launch,
+        "traffic_source": traffic_src,
     }
+    try:
+        roof["copy_GBs"] = copy_bandwidth()
+    except Exception as e:  # torch is plumbing only; the figure is optional
+        roof["copy_GBs"] = None
+        roof["copy_error"] = str(e)[:120]
+
+    def phases(R):
+        return {k: round(v / args.steps, 3) for k, v in
+                zip(["assembly", "residual", "setup", "krylov", "update", "step"], R["phases"])}
+
+    v_l = args.steps / L["el"]
     out = {
         "metric": METRIC,
-        "value": round(steps_per_s, 4),
+        "value": round(v_l, 4),
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * el / args.steps, 3),
+        "ms_per_step": round(1e3 * L["el"] / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"2-D WM all-physics film {args.n}x{args.n}, plain FP64 AMG-BiCGSTAB "
-                        "(the learned FP32 path is not built yet)",
+            "workload": f"2-D WM all-physics film {args.n}x{args.n} (capillary, disjoining h*=0.01 15deg, "
+                        "Bo 0.5, Pg (0.1, 0.05), spin 0.2, Marangoni 0.1, slip 0.05, frozen c), "
+                        "fixed dt 2e-6, Newton tol 1e-8: learned-preconditioned FP32 BiCGSTAB inner solve "
+                        "(FP64 Newton residual and Jacobian), vs plain FP64 SA-AMG BiCGSTAB",
             "cells": N, "dt": DT, "newton_tol": c.nk_eps, "linear_eps": c.lin_eps,
-            "precond": "structured SA-AMG + SPAI0", "domain": "[0,10]^2, Symmetry walls",
+            "precond": "learned 13-point approximate inverse (MLP 8-64-64-13, BF16 MFMA), trained in place",
+            "domain": "[0,10]^2, Symmetry walls",
         },
-        "newton_passes_per_step": round(passes / args.steps, 3),
-        "krylov_iters_per_pass": round(lins / max(passes, 1), 2),
-        "phases_ms_per_step": {k: round(v / args.steps, 3) for k, v in
-                               zip(["assembly", "residual", "amg_setup", "krylov", "update", "step"], phases)},
+        "newton_passes_per_step": round(L["passes"] / args.steps, 3),
+        "krylov_iters_per_pass": round(L["lins"] / max(L["passes"], 1), 2),
+        "phases_ms_per_step": phases(L),
+        "learned": {
+            "fallbacks": int(L["learn"][0]), "adam_steps": int(L["learn"][1]),
+            "training_ms": round(float(L["learn"][2]), 1),
+            "probe_loss_first": round(float(L["learn"][3]), 5), "probe_loss_last": round(float(L["learn"][4]), 5),
+        },
         "roofline": roof,
         "cpu_baseline": None,
-        "h_checksum": float(np.sum(hfinal)),
+        "h_checksum": float(np.sum(L["h"])),
     }
-    step.close()
+    if P is not None:
+        v_p = args.steps / P["el"]
+        rel = float(np.abs(L["h"] - P["h"]).max() / np.abs(P["h"]).max())
+        out["plain_fp64_amg"] = {
+            "value": round(v_p, 4), "ms_per_step": round(1e3 * P["el"] / args.steps, 3),
+            "newton_passes_per_step": round(P["passes"] / args.steps, 3),
+            "krylov_iters_per_pass": round(P["lins"] / max(P["passes"], 1), 2),
+            "phases_ms_per_step": phases(P),
+            "h_checksum": float(np.sum(P["h"])),
+        }
+        out["speedup_vs_plain_fp64"] = round(v_l / v_p, 3)
+        out["h_rel_diff_vs_plain_fp64"] = rel
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.n, passes / args.steps)
+        pp = (P["passes"] if P is not None else L["passes"]) / args.steps
+        out["cpu_baseline"] = cpu_baseline(args.n, pp)
     print(json.dumps(out), flush=True)
 
 
