@@ -44,6 +44,11 @@ struct Krylov {
     double *part = nullptr;    // reduction partials
     double *sc = nullptr;      // device scalars
     double *host_sc = nullptr; // pinned mirror
+    // the fused iteration of the explicit-stencil preconditioner
+    // (bicgstab_stencil): second search direction, per-block partials
+    T *p2 = nullptr;
+    double *fpart = nullptr;
+    int ftiles = 0;
 };
 
 template <typename T>
@@ -83,6 +88,13 @@ template <typename T> int jacobi_setup(SolverT<T> &S);
 template <typename T>
 int bicgstab(SolverT<T> &S, const T *b, T *x, int precond, int maxit, double tol, int *iters,
              double *resid);
+
+// BiCGSTAB right-preconditioned by the explicit 13-point stencil S.M, with
+// the iteration fused into five kernels per step and no host round trip
+// inside a chunk of iterations (2-D, 13-point operators; others go to
+// bicgstab). FP32. Same contract as bicgstab.
+int bicgstab_stencil(SolverT<float> &S, const float *b, float *x, int maxit, double tol, int *iters,
+                     double *resid);
 
 template <typename T>
 void launch_spmv(const StencilOp &op, const T *val, const T *x, T *y, hipStream_t st);

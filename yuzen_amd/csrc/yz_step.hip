@@ -357,7 +357,10 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
         if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
         YZ_TRY(hipEventRecord(S->ev[3], S->st));
         const double eps32 = eps > kEpsFP32 ? eps : kEpsFP32;
-        ch = bicgstab(S->S32, (const float *)S->b32, S->corr32, c.precond, c.lin_maxit, eps32, iter, resid);
+        if (c.precond == YZ_PC_LEARNED)
+            ch = bicgstab_stencil(S->S32, (const float *)S->b32, S->corr32, c.lin_maxit, eps32, iter, resid);
+        else
+            ch = bicgstab(S->S32, (const float *)S->b32, S->corr32, c.precond, c.lin_maxit, eps32, iter, resid);
         YZ_TRY(hipEventRecord(S->ev[4], S->st));
         YZ_TRY(hipEventSynchronize(S->ev[4]));
         S->tms[2] += ev_ms(S->ev[2], S->ev[3]);
