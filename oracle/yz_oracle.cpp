@@ -51,6 +51,21 @@
  * Numerics.Convection.Height = "Koren") and the Symmetry/Periodic edges are
  * restated; no concentration equation (Physics.ConcField.Model = "None": the
  * concentration field is frozen), no evaporation, no source.
+ *
+ * How each part is pinned (tests/test_oracle_golden.py,
+ * tests/test_oracle_physics.py):
+ *   mobility, limiter, Dp, every term's A and J, A(h)h: against the
+ *     reference's own term code compiled from its sources
+ *     (oracle/build_ref.sh, ref_golden.cpp), to 1e-12;
+ *   IA, Rhs, the Newton loop and its exits, step control: against the
+ *     exact discrete Crank-Nicolson amplification of linear leveling, the
+ *     second-order space convergence and volume conservation; the
+ *     reference's Newton loop cannot run here (its linear solve needs
+ *     Eigen/AMGCL), so Newton trajectories are parity unpinned against the
+ *     reference itself;
+ *   BiCGSTAB / SA-AMG / Jacobi: the 1-D Poisson problem of the reference's
+ *     backend test and the residual definition; AMG hierarchy and
+ *     iteration counts: parity unpinned (only the solution to tolerance).
  */
 #include "../include/yuzen_gpu.h"
 
