@@ -5,29 +5,39 @@
 // The method is the one of bicgstab() in yz_solver.hip (van der Vorst 1992,
 // right preconditioning, convergence on ||r|| <= tol ||b||, the half-step
 // exit on ||s||, breakdown on (rh, v) = 0, omega = 0, rho = 0 or NaN, the
-// stall test), rearranged for the device:
+// stall test), rearranged for the device into four kernels per iteration k:
 //
-//   K1  p_new = r + beta (p - omega v);  ph = M p_new    (p_new recomputed
-//       at the 13 columns from r, p, v: no separate update pass; p is
-//       double-buffered so no cell reads a neighbour's new value)
-//   K2  v = A ph;  (rh, v)                 -> alpha
-//   K3  s = r - alpha v;  sh = M s;  (s, s) -> half-step exit
-//   K4  t = A sh;  (t, t), (t, s)          -> omega
-//   K5  x += alpha ph + omega sh;  r = s - omega t;  (r, r), (r, rh)
-//                                          -> beta, exits, stall test
+//   K1  x += alpha ph + omega sh (iteration k-1);  r = s - omega t;
+//       p = r + beta (p_old - omega v);  ph = M p;  partial (r, r)
+//       (r and p are formed again at each of the 13 columns from s, t,
+//       p_old, v, so no separate update pass runs; p is double-buffered)
+//   K2  the exits on ||r|| (converged, breakdown, stall, cap);
+//       v = A ph;  partial (rh, v)
+//   K3  alpha = rho / (rh, v);  s = r - alpha v;  sh = M s;
+//       partials (s, s), (s, rh)
+//   K4  the half-step exit on ||s||;  t = A sh;
+//       partials (t, t), (t, s), (t, rh)
+// and K1 of the next iteration takes omega = (t, s) / (t, t) and
+// rho_new = (r_new, rh) = (s, rh) - omega (t, rh), beta = (rho_new / rho)
+// (alpha / omega) from those partials.
+//
+// M is read as FP16 raw rows and an FP32 row scale (M = diag(m) Y): the
+// preconditioner is any fixed linear operator, so its rounding changes the
+// iteration count at most, never the answer.
 //
 // Every dot product is reduced in FP64 in a fixed order with no atomics and
 // no finishing kernel: each of the G blocks writes its partial, and every
 // block of the next kernel sums the G partials in the same order, so all
-// blocks derive the same scalars and the same exit decision (converged,
-// half-step exit, breakdown, stall, iteration cap). Block 0 records the
-// scalars for the kernel after it; scalars and partials are double-buffered
-// by the parity of the kernel's sequence number, so no block reads what
-// another block of the same kernel writes. Only stream order makes the
-// values visible (a last-arriving-block finish needs device-scope atomics,
-// which serialise, or release fences, which write the L2 back, on this
-// many-XCD chip). After an exit every kernel returns at once; the host
-// enqueues a chunk of iterations and reads the state once per chunk.
+// blocks derive the same scalars and the same exit decision. Block 0
+// records the scalars for the kernel after it; scalars and partials are
+// double-buffered by the parity of the kernel's sequence number, so no
+// block reads what another block of the same kernel writes. Only stream
+// order makes the values visible (a last-arriving-block finish needs
+// device-scope atomics, which serialise at the memory side, or release
+// fences, which write the L2 back, on this many-XCD chip). After an exit
+// every kernel returns at once; the host enqueues a chunk of iterations
+// and reads the state once per chunk.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,8 +61,10 @@ namespace yz {
 
 namespace {
 
-constexpr int TX = 64, TY = 4, TB = TX * TY;
-constexpr int G = 512;     // blocks of every kernel (grid-stride over 64 x 4 tiles): partials per reduction
+constexpr int TX = 64, TY = 8, TB = 256;   // 64 x 8 tiles, 256 threads, two rows per thread
+constexpr int RPT = TY / (TB / TX);         // rows per thread
+constexpr int LW = TX + 4, LH = TY + 4;     // LDS tile with the 2-cell halo
+constexpr int G = 512;     // blocks of every kernel (grid-stride over the tiles): partials per reduction
 constexpr int CHUNK = 8;   // iterations enqueued between two reads of the state
 
 // slots of the 2-D 13-point stencil: oM mm om pm Mo mo oo po Po mp op pp oP
@@ -61,12 +73,13 @@ constexpr int OY[13] = {-2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2};
 
 // the scalars of the iteration; two copies, by the parity of the kernel's
 // sequence number q: kernel q reads copy (q-1)&1 and block 0 writes copy q&1
-enum { S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_RES, S_EPS, S_NB, S_MARK, S_STATE, S_IT, S_RR, S_N = 16 };
+enum { S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_RES, S_EPS, S_NB, S_MARK, S_STATE, S_IT, S_SRH, S_N = 16 };
 enum { ST_RUN = 0, ST_DONE = 1, ST_HALF = 2, ST_BREAK = 3, ST_STALL = 4, ST_CAP = 5, ST_ZERO = 7 };
+constexpr int NQ = 3;      // quantities per partial set
 
 struct Ctl {
     double *sc;      // 2 x S_N
-    double *part;    // 2 (parity) x 2 (quantities) x G
+    double *part;    // 2 (parity) x NQ x G
     int ntx, nty;    // tiles
     int maxit, window;
     double factor, tol;
@@ -82,8 +95,8 @@ __device__ inline Seq seq(const Ctl &c, int q) {
     Seq s;
     s.in = c.sc + ((q - 1) & 1) * S_N;
     s.out = c.sc + (q & 1) * S_N;
-    s.pin = c.part + ((q - 1) & 1) * 2 * G;
-    s.pout = c.part + (q & 1) * 2 * G;
+    s.pin = c.part + ((q - 1) & 1) * NQ * G;
+    s.pout = c.part + (q & 1) * NQ * G;
     return s;
 }
 
@@ -140,139 +153,189 @@ __device__ inline void commit_sc(const double (&c)[S_N], double *__restrict__ ou
     if (blockIdx.x == 0 && threadIdx.x < S_N) out[threadIdx.x] = c[threadIdx.x];
 }
 
-// the columns of the 13 slots of cell (ix, iy): direct offsets inside,
-// folded at the edges
-__device__ inline void columns(const StencilOp &op, int ix, int iy, int (&j)[13]) {
-    const int i = iy * op.nx + ix;
-    if (ix >= 2 && ix < op.nx - 2 && iy >= 2 && iy < op.ny - 2) {
-#pragma unroll
-        for (int k = 0; k < 13; ++k) j[k] = i + OX[k] + OY[k] * op.nx;
-    } else {
-#pragma unroll
-        for (int k = 0; k < 13; ++k) j[k] = stencil_col(op, ix, iy, k);
+// stage f(j) over tile (x0, y0) and its 2-cell halo into L (folded at the
+// edges like the stencil's columns): coalesced row loads instead of 13
+// scattered loads per cell and vector
+template <class F>
+__device__ inline void stage(const StencilOp &op, int x0, int y0, float (&L)[LH][LW], F f) {
+    const bool inside = x0 >= 2 && x0 + TX + 2 <= op.nx && y0 >= 2 && y0 + TY + 2 <= op.ny;
+    for (int q = threadIdx.x; q < LH * LW; q += TB) {
+        const int ly = q / LW, lx = q - ly * LW;
+        int gx = x0 + lx - 2, gy = y0 + ly - 2;
+        if (!inside) { gx = fold_axis(gx, op.nx, op.perx); gy = fold_axis(gy, op.ny, op.pery); }
+        L[ly][lx] = f(gy * op.nx + gx);
     }
 }
 
-// cell of tile t of this thread
-__device__ inline bool tile_cell(const StencilOp &op, const Ctl &c, int t, int &ix, int &iy) {
-    ix = (t % c.ntx) * TX + (threadIdx.x & (TX - 1));
-    iy = (t / c.ntx) * TY + (threadIdx.x / TX);
-    return ix < op.nx && iy < op.ny;
+// the 13-point product of planes W (row i) with the staged tile at local
+// cell (lx, ly) (tile coordinates, without the halo)
+template <class T, class Cv>
+__device__ inline float apply13(const T *__restrict__ W, int N, int i, const float (&L)[LH][LW], int lx, int ly, Cv cv) {
+    float a = 0;
+#pragma unroll
+    for (int k = 0; k < 13; ++k) a = fmaf(cv(W[k * N + i]), L[ly + 2 + OY[k]][lx + 2 + OX[k]], a);
+    return a;
+}
+__device__ inline float idf(float v) { return v; }
+__device__ inline float h2f(__half v) { return __half2float(v); }
+
+struct Vecs {
+    float *x, *r, *rh, *p0, *p1, *ph, *v, *s, *sh, *t;
+};
+
+// the cells of tile (x0, y0) this thread owns: rows ty and ty + 4, column tx
+template <class F>
+__device__ inline void for_own(const StencilOp &op, int x0, int y0, F f) {
+    const int lx = threadIdx.x & (TX - 1);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int ly = threadIdx.x / TX + r * (TB / TX);
+        if (x0 + lx < op.nx && y0 + ly < op.ny) f(lx, ly, (y0 + ly) * op.nx + x0 + lx);
+    }
 }
 
-// K0 (q = 0): r = b - A x, rh = r, p = 0, v = 0; partials (b, b), (r, r)
+// K0 (q = 0): r = b - A x, rh = r; partials (b, b), (r, r)
 __global__ void __launch_bounds__(TB) k_f_init(StencilOp op, const float *__restrict__ A,
-                                               const float *__restrict__ b, const float *__restrict__ x,
-                                               float *__restrict__ r, float *__restrict__ rh,
-                                               float *__restrict__ p, float *__restrict__ v, Ctl c) {
+                                               const float *__restrict__ b, Vecs V, Ctl c) {
+    __shared__ float L[LH][LW];
     const Seq sq = seq(c, 0);
     double acc[2] = {0, 0};
+    const float *__restrict__ x = V.x;
     for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
-        int ix, iy;
-        if (!tile_cell(op, c, t, ix, iy)) continue;
-        const int i = iy * op.nx + ix;
-        int j[13];
-        columns(op, ix, iy, j);
-        float ax = 0;
-#pragma unroll
-        for (int k = 0; k < 13; ++k) ax = fmaf(A[k * op.N + i], x[j[k]], ax);
-        const float bi = b[i];
-        const float ri = bi - ax;
-        r[i] = ri; rh[i] = ri; p[i] = 0.0f; v[i] = 0.0f;
-        acc[0] += (double)bi * bi;
-        acc[1] += (double)ri * ri;
+        const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
+        __syncthreads();
+        stage(op, x0, y0, L, [&](int j) { return x[j]; });
+        __syncthreads();
+        for_own(op, x0, y0, [&](int lx, int ly, int i) {
+            const float bi = b[i];
+            const float ri = bi - apply13(A, op.N, i, L, lx, ly, idf);
+            V.r[i] = ri; V.rh[i] = ri;
+            acc[0] += (double)bi * bi;
+            acc[1] += (double)ri * ri;
+        });
     }
     store_partials<2>(acc, sq.pout);
 }
 
-// K1 (q = 5k + 1): the scalar step closing iteration k - 1 (or opening the
-// solve, k = 0) from K5's (K0's) partials, then p_new = r + beta (p_old -
-// omega v), ph = M p_new. With work = 0 only the scalar step runs (one
-// block, at the end of a chunk, so the host can read the state; the K1
-// that follows recomputes the same scalars from the same inputs).
-__global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const float *__restrict__ M,
-                                             const float *__restrict__ r, const float *__restrict__ pold,
-                                             const float *__restrict__ v, float *__restrict__ pnew,
-                                             float *__restrict__ ph, Ctl c, int q, int k, int work) {
+// K1 (q = 4k + 1). k = 0: opens the solve from K0's partials, p = r.
+// k >= 1: omega, rho, beta from K4's partials; the x update of iteration
+// k - 1 (or, after the half-step exit, x += alpha ph and the end);
+// r = s - omega t; p = r + beta (p_old - omega v); ph = M p; partial (r, r).
+__global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restrict__ Y,
+                                             const float *__restrict__ ms, Vecs V, Ctl c, int q, int k) {
+    __shared__ float L[LH][LW];
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
-    if (k == 0 || s[S_STATE] == ST_RUN) {
+    const bool half = k > 0 && s[S_STATE] == ST_HALF;
+    const double alpha_prev = s[S_ALPHA];
+    if (k == 0) {
         double tot[2];
         sum_partials<2>(sq.pin, tot);
-        if (k == 0) {
-            const double nb = sqrt(tot[0]), res = sqrt(tot[1]);
-            s[S_NB] = nb; s[S_EPS] = c.tol * nb; s[S_RR] = tot[1]; s[S_RES] = res; s[S_MARK] = res;
-            s[S_RHO] = tot[1]; s[S_ALPHA] = 1; s[S_OMEGA] = 1; s[S_BETA] = 0; s[S_IT] = 0;
-            s[S_STATE] = (nb == 0) ? ST_ZERO : (res <= s[S_EPS] ? ST_DONE : ST_RUN);
+        const double nb = sqrt(tot[0]), res = sqrt(tot[1]);
+        s[S_NB] = nb; s[S_EPS] = c.tol * nb; s[S_RES] = res; s[S_MARK] = res;
+        s[S_RHO] = tot[1]; s[S_ALPHA] = 1; s[S_OMEGA] = 1; s[S_BETA] = 0; s[S_IT] = 0; s[S_SRH] = 0;
+        s[S_STATE] = (nb == 0) ? ST_ZERO : (res <= s[S_EPS] ? ST_DONE : ST_RUN);
+    } else if (half) {
+        s[S_IT] += 1;
+        s[S_STATE] = ST_DONE;
+    } else if (s[S_STATE] == ST_RUN) {
+        double tot[3];
+        sum_partials<3>(sq.pin, tot);   // (t, t), (t, s), (t, rh)
+        const double omega = tot[0] != 0 ? tot[1] / tot[0] : 0.0;
+        if (omega == 0 || !(omega == omega)) {
+            s[S_STATE] = ST_BREAK;
         } else {
-            const double it = s[S_IT] + 1;
-            const double res = sqrt(tot[0]);
-            s[S_IT] = it; s[S_RR] = tot[0]; s[S_RES] = res;
-            s[S_BETA] = (tot[1] / s[S_RHO]) * (s[S_ALPHA] / s[S_OMEGA]);
-            s[S_RHO] = tot[1];
-            int st = ST_RUN;
-            if (res < s[S_EPS]) st = ST_DONE;
-            else if (!(res == res)) st = ST_BREAK;
-            else if (fabs(s[S_RHO]) < 1e-300) st = ST_BREAK;
-            else if (c.window > 0 && ((long long)it % c.window) == 0) {
-                if (res > c.factor * s[S_MARK]) st = ST_STALL;
-                else s[S_MARK] = res;
-            }
-            if (st == ST_RUN && it >= c.maxit) st = ST_CAP;
-            s[S_STATE] = st;
+            const double rho = s[S_SRH] - omega * tot[2];
+            s[S_BETA] = (rho / s[S_RHO]) * (s[S_ALPHA] / omega);
+            s[S_RHO] = rho;
+            s[S_OMEGA] = omega;
         }
     }
     commit_sc(s, sq.out);
-    if (!work || s[S_STATE] != ST_RUN) return;
+    if (!half && s[S_STATE] != ST_RUN) return;
+    if (half) {
+        const float a = (float)alpha_prev;
+        for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
+            const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
+            for_own(op, x0, y0, [&](int, int, int i) { V.x[i] += a * V.ph[i]; });
+        }
+        return;
+    }
+    const float *__restrict__ pold = (k & 1) ? V.p0 : V.p1;
+    float *__restrict__ pnew = (k & 1) ? V.p1 : V.p0;
+    const float alpha = (float)s[S_ALPHA], omega = (float)s[S_OMEGA];
     const float beta = (float)s[S_BETA], bw = (float)(s[S_BETA] * s[S_OMEGA]);
-    for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
-        int ix, iy;
-        if (!tile_cell(op, c, t, ix, iy)) continue;
-        const int i = iy * op.nx + ix;
-        int j[13];
-        columns(op, ix, iy, j);
-        float a = 0;
-#pragma unroll
-        for (int kk = 0; kk < 13; ++kk) {
-            const int qq = j[kk];
-            a = fmaf(M[kk * op.N + i], r[qq] + beta * pold[qq] - bw * v[qq], a);
-        }
-        pnew[i] = r[i] + beta * pold[i] - bw * v[i];
-        ph[i] = a;
-    }
-}
-
-// K2 (q = 5k + 2): v = A ph; partial (rh, v)
-__global__ void __launch_bounds__(TB) k_f_av(StencilOp op, const float *__restrict__ A,
-                                             const float *__restrict__ ph, const float *__restrict__ rh,
-                                             float *__restrict__ v, Ctl c, int q) {
-    const Seq sq = seq(c, q);
-    double s[S_N];
-    load_sc(sq.in, s);
-    commit_sc(s, sq.out);
-    if (s[S_STATE] != ST_RUN) return;
+    const float *__restrict__ sv = V.s, *__restrict__ tv = V.t, *__restrict__ vv = V.v, *__restrict__ rv = V.r;
     double acc[1] = {0};
     for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
-        int ix, iy;
-        if (!tile_cell(op, c, t, ix, iy)) continue;
-        const int i = iy * op.nx + ix;
-        int j[13];
-        columns(op, ix, iy, j);
-        float a = 0;
-#pragma unroll
-        for (int k = 0; k < 13; ++k) a = fmaf(A[k * op.N + i], ph[j[k]], a);
-        v[i] = a;
-        acc[0] += (double)rh[i] * a;
+        const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
+        __syncthreads();
+        if (k == 0) stage(op, x0, y0, L, [&](int j) { return rv[j]; });
+        else stage(op, x0, y0, L, [&](int j) { return (sv[j] - omega * tv[j]) + beta * pold[j] - bw * vv[j]; });
+        __syncthreads();
+        for_own(op, x0, y0, [&](int lx, int ly, int i) {
+            const float pi = L[ly + 2][lx + 2];
+            const float ri = (k == 0) ? pi : sv[i] - omega * tv[i];
+            if (k > 0) {
+                V.x[i] += alpha * V.ph[i] + omega * V.sh[i];
+                V.r[i] = ri;
+            }
+            pnew[i] = pi;
+            V.ph[i] = ms[i] * apply13(Y, op.N, i, L, lx, ly, h2f);
+            acc[0] += (double)ri * ri;
+        });
     }
     store_partials<1>(acc, sq.pout);
 }
 
-// K3 (q = 5k + 3): alpha = rho / (rh, v); s = r - alpha v; sh = M s;
-// partial (s, s)
-__global__ void __launch_bounds__(TB) k_f_sm(StencilOp op, const float *__restrict__ M,
-                                             const float *__restrict__ r, const float *__restrict__ v,
-                                             float *__restrict__ sv, float *__restrict__ sh, Ctl c, int q) {
+// K2 (q = 4k + 2): for k >= 1 the exits on ||r_k||; v = A ph; partial (rh, v)
+__global__ void __launch_bounds__(TB) k_f_av(StencilOp op, const float *__restrict__ A, Vecs V, Ctl c, int q,
+                                             int k, int work) {
+    __shared__ float L[LH][LW];
+    const Seq sq = seq(c, q);
+    double s[S_N];
+    load_sc(sq.in, s);
+    if (k > 0 && s[S_STATE] == ST_RUN) {
+        double tot[1];
+        sum_partials<1>(sq.pin, tot);
+        const double it = (double)k;
+        const double res = sqrt(tot[0]);
+        s[S_IT] = it; s[S_RES] = res;
+        int st = ST_RUN;
+        if (res < s[S_EPS]) st = ST_DONE;
+        else if (!(res == res)) st = ST_BREAK;
+        else if (fabs(s[S_RHO]) < 1e-300) st = ST_BREAK;
+        else if (c.window > 0 && ((long long)it % c.window) == 0) {
+            if (res > c.factor * s[S_MARK]) st = ST_STALL;
+            else s[S_MARK] = res;
+        }
+        if (st == ST_RUN && it >= c.maxit) st = ST_CAP;
+        s[S_STATE] = st;
+    }
+    commit_sc(s, sq.out);
+    if (!work || s[S_STATE] != ST_RUN) return;
+    const float *__restrict__ ph = V.ph;
+    double acc[1] = {0};
+    for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
+        const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
+        __syncthreads();
+        stage(op, x0, y0, L, [&](int j) { return ph[j]; });
+        __syncthreads();
+        for_own(op, x0, y0, [&](int lx, int ly, int i) {
+            const float a = apply13(A, op.N, i, L, lx, ly, idf);
+            V.v[i] = a;
+            acc[0] += (double)V.rh[i] * a;
+        });
+    }
+    store_partials<1>(acc, sq.pout);
+}
+
+// K3 (q = 4k + 3): alpha = rho / (rh, v); s = r - alpha v; sh = M s;
+// partials (s, s), (s, rh)
+__global__ void __launch_bounds__(TB) k_f_sm(StencilOp op, const __half *__restrict__ Y,
+                                             const float *__restrict__ ms, Vecs V, Ctl c, int q) {
+    __shared__ float L[LH][LW];
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
@@ -285,102 +348,57 @@ __global__ void __launch_bounds__(TB) k_f_sm(StencilOp op, const float *__restri
     commit_sc(s, sq.out);
     if (s[S_STATE] != ST_RUN) return;
     const float alpha = (float)s[S_ALPHA];
-    double acc[1] = {0};
+    const float *__restrict__ rv = V.r, *__restrict__ vv = V.v;
+    double acc[2] = {0, 0};
     for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
-        int ix, iy;
-        if (!tile_cell(op, c, t, ix, iy)) continue;
-        const int i = iy * op.nx + ix;
-        int j[13];
-        columns(op, ix, iy, j);
-        float a = 0;
-#pragma unroll
-        for (int k = 0; k < 13; ++k) {
-            const int qq = j[k];
-            a = fmaf(M[k * op.N + i], r[qq] - alpha * v[qq], a);
-        }
-        const float si = r[i] - alpha * v[i];
-        sv[i] = si;
-        sh[i] = a;
-        acc[0] += (double)si * si;
+        const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
+        __syncthreads();
+        stage(op, x0, y0, L, [&](int j) { return rv[j] - alpha * vv[j]; });
+        __syncthreads();
+        for_own(op, x0, y0, [&](int lx, int ly, int i) {
+            const float si = L[ly + 2][lx + 2];
+            V.s[i] = si;
+            V.sh[i] = ms[i] * apply13(Y, op.N, i, L, lx, ly, h2f);
+            acc[0] += (double)si * si;
+            acc[1] += (double)si * V.rh[i];
+        });
     }
-    store_partials<1>(acc, sq.pout);
+    store_partials<2>(acc, sq.pout);
 }
 
-// K4 (q = 5k + 4): the half-step exit on ||s||; t = A sh; partials (t, t),
-// (t, s)
-__global__ void __launch_bounds__(TB) k_f_at(StencilOp op, const float *__restrict__ A,
-                                             const float *__restrict__ sh, const float *__restrict__ sv,
-                                             float *__restrict__ tv, Ctl c, int q) {
+// K4 (q = 4k + 4): the half-step exit on ||s||; t = A sh; partials (t, t),
+// (t, s), (t, rh)
+__global__ void __launch_bounds__(TB) k_f_at(StencilOp op, const float *__restrict__ A, Vecs V, Ctl c, int q) {
+    __shared__ float L[LH][LW];
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
     if (s[S_STATE] == ST_RUN) {
-        double tot[1];
-        sum_partials<1>(sq.pin, tot);
+        double tot[2];
+        sum_partials<2>(sq.pin, tot);
         const double res = sqrt(tot[0]);
+        s[S_SRH] = tot[1];
         if (!(res == res)) s[S_STATE] = ST_BREAK;
         else if (res < s[S_EPS]) { s[S_RES] = res; s[S_STATE] = ST_HALF; }
     }
     commit_sc(s, sq.out);
     if (s[S_STATE] != ST_RUN) return;
-    double acc[2] = {0, 0};
+    const float *__restrict__ sh = V.sh;
+    double acc[3] = {0, 0, 0};
     for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
-        int ix, iy;
-        if (!tile_cell(op, c, t, ix, iy)) continue;
-        const int i = iy * op.nx + ix;
-        int j[13];
-        columns(op, ix, iy, j);
-        float a = 0;
-#pragma unroll
-        for (int k = 0; k < 13; ++k) a = fmaf(A[k * op.N + i], sh[j[k]], a);
-        tv[i] = a;
-        acc[0] += (double)a * a;
-        acc[1] += (double)a * sv[i];
+        const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
+        __syncthreads();
+        stage(op, x0, y0, L, [&](int j) { return sh[j]; });
+        __syncthreads();
+        for_own(op, x0, y0, [&](int lx, int ly, int i) {
+            const float a = apply13(A, op.N, i, L, lx, ly, idf);
+            V.t[i] = a;
+            acc[0] += (double)a * a;
+            acc[1] += (double)a * V.s[i];
+            acc[2] += (double)a * V.rh[i];
+        });
     }
-    store_partials<2>(acc, sq.pout);
-}
-
-// K5 (q = 5k + 5): omega = (t, s) / (t, t); x += alpha ph + omega sh;
-// r = s - omega t; partials (r, r), (r, rh). After the half-step exit:
-// x += alpha ph, and the solve ends.
-__global__ void __launch_bounds__(TB) k_f_xr(int nx, int ny, float *__restrict__ x, const float *__restrict__ ph,
-                                             const float *__restrict__ sh, const float *__restrict__ sv,
-                                             const float *__restrict__ tv, float *__restrict__ r,
-                                             const float *__restrict__ rh, Ctl c, int q) {
-    const Seq sq = seq(c, q);
-    double s[S_N];
-    load_sc(sq.in, s);
-    const bool half = s[S_STATE] == ST_HALF;
-    if (half) {
-        s[S_IT] += 1;
-        s[S_STATE] = ST_DONE;
-    } else if (s[S_STATE] == ST_RUN) {
-        double tot[2];
-        sum_partials<2>(sq.pin, tot);
-        const double omega = tot[0] != 0 ? tot[1] / tot[0] : 0.0;
-        if (omega == 0 || !(omega == omega)) s[S_STATE] = ST_BREAK;
-        else s[S_OMEGA] = omega;
-    }
-    commit_sc(s, sq.out);
-    if (!half && s[S_STATE] != ST_RUN) return;
-    const float alpha = (float)s[S_ALPHA], omega = (float)s[S_OMEGA];
-    double acc[2] = {0, 0};
-    for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
-        const int ix = (t % c.ntx) * TX + (threadIdx.x & (TX - 1));
-        const int iy = (t / c.ntx) * TY + (threadIdx.x / TX);
-        if (ix >= nx || iy >= ny) continue;
-        const int i = iy * nx + ix;
-        if (half) {
-            x[i] += alpha * ph[i];
-        } else {
-            x[i] += alpha * ph[i] + omega * sh[i];
-            const float ri = sv[i] - omega * tv[i];
-            r[i] = ri;
-            acc[0] += (double)ri * ri;
-            acc[1] += (double)ri * rh[i];
-        }
-    }
-    if (!half) store_partials<2>(acc, sq.pout);
+    store_partials<3>(acc, sq.pout);
 }
 
 }  // namespace
@@ -390,39 +408,41 @@ int bicgstab_stencil(SolverT<float> &S, const float *b, float *x, int maxit, dou
     const StencilOp &op = S.op0;
     *iters = 0;
     *resid = 0;
-    bool ok = op.dim == 2 && op.ns == 13 && S.M;
+    bool ok = op.dim == 2 && op.ns == 13 && S.Mh && S.Mscale;
     for (int k = 0; ok && k < 13; ++k) ok = op.ox[k] == OX[k] && op.oy[k] == OY[k];
-    if (!ok) return bicgstab(S, b, x, YZ_PC_LEARNED, maxit, tol, iters, resid);
+    if (!ok) {
+        if (!S.M) return 2;
+        return bicgstab(S, b, x, YZ_PC_LEARNED, maxit, tol, iters, resid);
+    }
     Krylov<float> &kr = S.kr;
     hipStream_t st = S.stream;
     const int ntx = (op.nx + TX - 1) / TX, nty = (op.ny + TY - 1) / TY;
-    if (kr.ftiles != 1) {
-        if (kr.p2) (void)hipFree(kr.p2);
-        if (kr.fpart) (void)hipFree(kr.fpart);
-        kr.p2 = nullptr; kr.fpart = nullptr; kr.ftiles = 0;
+    if (!kr.ftiles) {
         KR_CHECK(hipMalloc((void **)&kr.p2, (size_t)op.N * sizeof(float)));
-        KR_CHECK(hipMalloc((void **)&kr.fpart, (size_t)2 * 2 * G * sizeof(double)));
+        KR_CHECK(hipMalloc((void **)&kr.fpart, (size_t)2 * NQ * G * sizeof(double)));
         kr.ftiles = 1;
     }
     Ctl c;
     c.sc = kr.sc + 32;   // two copies of S_N scalars (the solver's array holds 64)
     c.part = kr.fpart; c.ntx = ntx; c.nty = nty;
     c.maxit = maxit; c.window = S.stall_window; c.factor = S.stall_factor; c.tol = tol;
+    Vecs V;
+    V.x = x; V.r = kr.r; V.rh = kr.rh; V.p0 = kr.p; V.p1 = kr.p2; V.ph = kr.ph; V.v = kr.v; V.s = kr.s;
+    V.sh = kr.sh; V.t = kr.t;
     const dim3 grid(G), blk(TB);
     const double *h = kr.host_sc;
-    float *P[2] = {kr.p, kr.p2};
-    // the scalars as K1 of iteration k would see them
+    // the scalars as K2 of iteration k would see them (one block, no work;
+    // the K2 that follows derives the same values from the same inputs)
     auto check = [&](int k) -> int {
-        const int q = 5 * k + 1;
-        hipLaunchKernelGGL(k_f_pm, dim3(1), blk, 0, st, op, S.M, (const float *)kr.r, (const float *)P[k & 1],
-                           (const float *)kr.v, P[(k + 1) & 1], kr.ph, c, q, k, 0);
+        const int q = 4 * k + 2;
+        hipLaunchKernelGGL(k_f_av, dim3(1), blk, 0, st, op, (const float *)S.A, V, c, q, k, 0);
         KR_CHECK(hipGetLastError());
         KR_CHECK(hipMemcpyAsync(kr.host_sc, c.sc + (q & 1) * S_N, S_N * sizeof(double), hipMemcpyDeviceToHost, st));
         KR_CHECK(hipStreamSynchronize(st));
         return 0;
     };
-    hipLaunchKernelGGL(k_f_init, grid, blk, 0, st, op, (const float *)S.A, b, (const float *)x, kr.r, kr.rh, P[0],
-                       kr.v, c);
+    hipLaunchKernelGGL(k_f_init, grid, blk, 0, st, op, (const float *)S.A, b, V, c);
+    hipLaunchKernelGGL(k_f_pm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, 1, 0);
     if (check(0)) return 2;
     int state = (int)h[S_STATE];
     if (state == ST_ZERO) {
@@ -433,24 +453,17 @@ int bicgstab_stencil(SolverT<float> &S, const float *b, float *x, int maxit, dou
     while (state == ST_RUN && k < maxit) {
         const int n = std::min(CHUNK, maxit - k);
         for (int u = 0; u < n; ++u, ++k) {
-            const int q = 5 * k;
-            hipLaunchKernelGGL(k_f_pm, grid, blk, 0, st, op, S.M, (const float *)kr.r, (const float *)P[k & 1],
-                               (const float *)kr.v, P[(k + 1) & 1], kr.ph, c, q + 1, k, 1);
+            const int q = 4 * k;
             const bool timed = S.ktime && u == 0;
             if (timed) {
                 if (!S.kev[0]) { (void)hipEventCreate(&S.kev[0]); (void)hipEventCreate(&S.kev[1]); }
                 (void)hipEventRecord(S.kev[0], st);
             }
-            hipLaunchKernelGGL(k_f_av, grid, blk, 0, st, op, (const float *)S.A, (const float *)kr.ph,
-                               (const float *)kr.rh, kr.v, c, q + 2);
+            hipLaunchKernelGGL(k_f_av, grid, blk, 0, st, op, (const float *)S.A, V, c, q + 2, k, 1);
             if (timed) (void)hipEventRecord(S.kev[1], st);
-            hipLaunchKernelGGL(k_f_sm, grid, blk, 0, st, op, S.M, (const float *)kr.r, (const float *)kr.v, kr.s,
-                               kr.sh, c, q + 3);
-            hipLaunchKernelGGL(k_f_at, grid, blk, 0, st, op, (const float *)S.A, (const float *)kr.sh,
-                               (const float *)kr.s, kr.t, c, q + 4);
-            hipLaunchKernelGGL(k_f_xr, grid, blk, 0, st, op.nx, op.ny, x, (const float *)kr.ph,
-                               (const float *)kr.sh, (const float *)kr.s, (const float *)kr.t, kr.r,
-                               (const float *)kr.rh, c, q + 5);
+            hipLaunchKernelGGL(k_f_sm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, q + 3);
+            hipLaunchKernelGGL(k_f_at, grid, blk, 0, st, op, (const float *)S.A, V, c, q + 4);
+            hipLaunchKernelGGL(k_f_pm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, q + 5, k + 1);
         }
         KR_CHECK(hipGetLastError());
         if (check(k)) return 2;

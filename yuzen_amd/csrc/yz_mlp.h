@@ -17,6 +17,7 @@
 // (__builtin_amdgcn_mfma_f32_16x16x16bf16_1k, one wave per 16 cells); the
 // FP32 master weights are updated with Adam.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "yz_solver.h"
@@ -54,6 +55,8 @@ int mlp_init(Mlp &m, int N, hipStream_t st);
 void mlp_free(Mlp &m);
 // M (13 x N planes) from the features and the row scale
 int mlp_infer(Mlp &m, int N, const float *feat, const float *mscale, float *M, hipStream_t st);
+// the raw outputs Y (13 x N planes, FP16): M = diag(mscale) Y
+int mlp_infer_half(Mlp &m, int N, const float *feat, __half *Y, hipStream_t st);
 // loss and gradient of the probe loss on operator A (S.op0, S.A) for the
 // current parameters: fills m.grads, returns the loss
 int mlp_loss_grad(Mlp &m, const SolverT<float> &S, const float *feat, const float *mscale, int nprobe,

@@ -152,10 +152,13 @@ __device__ inline f32x4 tile_forward(const WLds &W, const float *__restrict__ fe
     return y;
 }
 
-// M[o][cell] = Y[cell][o] * mscale[cell], o < 13
+// M[o][cell] = Y[cell][o] * mscale[cell], o < 13 (FP32); or, with HALF, the
+// raw outputs Y[o][cell] in FP16 (the row scale is applied by the consumer)
+template <bool HALF>
 __global__ void __launch_bounds__(MBLK) k_mlp_forward(int N, const float *__restrict__ P,
                                                       const float *__restrict__ feat,
-                                                      const float *__restrict__ mscale, float *__restrict__ M) {
+                                                      const float *__restrict__ mscale, float *__restrict__ M,
+                                                      __half *__restrict__ Yh) {
     __shared__ WLds W;
     __shared__ float tiles[WAVES][2][16 * LDT];
     load_weights(P, W);
@@ -174,7 +177,10 @@ __global__ void __launch_bounds__(MBLK) k_mlp_forward(int N, const float *__rest
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int cell = c0 + 4 * (l >> 4) + j;
-                if (cell < N) M[(size_t)o * N + cell] = y[j] * mscale[cell];
+                if (cell < N) {
+                    if (HALF) Yh[(size_t)o * N + cell] = __float2half(y[j]);
+                    else M[(size_t)o * N + cell] = y[j] * mscale[cell];
+                }
             }
         }
         __syncthreads();
@@ -485,7 +491,17 @@ void mlp_free(Mlp &m) {
 int mlp_infer(Mlp &m, int N, const float *feat, const float *mscale, float *M, hipStream_t st) {
     const int ntiles = (N + 15) / 16;
     const int blocks = std::min((ntiles + WAVES - 1) / WAVES, 2048);
-    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(MBLK), 0, st, N, (const float *)m.params, feat, mscale, M);
+    hipLaunchKernelGGL(k_mlp_forward<false>, dim3(blocks), dim3(MBLK), 0, st, N, (const float *)m.params, feat, mscale,
+                       M, (__half *)nullptr);
+    MLP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int mlp_infer_half(Mlp &m, int N, const float *feat, __half *Y, hipStream_t st) {
+    const int ntiles = (N + 15) / 16;
+    const int blocks = std::min((ntiles + WAVES - 1) / WAVES, 2048);
+    hipLaunchKernelGGL(k_mlp_forward<true>, dim3(blocks), dim3(MBLK), 0, st, N, (const float *)m.params, feat,
+                       (const float *)nullptr, (float *)nullptr, Y);
     MLP_CHECK(hipGetLastError());
     return 0;
 }

@@ -7,6 +7,7 @@
 // solve (dot products and the Krylov scalars stay FP64, and so does the
 // coarsest level's dense solve).
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <vector>
 
@@ -57,6 +58,10 @@ struct SolverT {
     T *A = nullptr;            // fine planes (row-scaled)
     T *dinv_jac = nullptr;
     const T *M = nullptr;      // explicit 13-point preconditioner / finest smoother (planes), or null
+    // the same preconditioner as FP16 raw rows and an FP32 row scale,
+    // M = diag(Mscale) Mh (the fused FP32 iteration reads these)
+    const __half *Mh = nullptr;
+    const float *Mscale = nullptr;
     Amg<T> amg;
     Krylov<T> kr;
     hipStream_t stream = nullptr;

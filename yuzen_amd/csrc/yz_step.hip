@@ -168,6 +168,7 @@ struct yz_step {
     // the FP32 inner solve and the learned preconditioner
     SolverT<float> S32;
     float *b32 = nullptr, *corr32 = nullptr, *feat = nullptr, *mscale = nullptr, *M32 = nullptr;
+    __half *Yh = nullptr;   // the network's raw rows, FP16 (learned preconditioner alone)
     Mlp mlp;
     long long fallbacks = 0;
     double time = 0, dt_now = 0, dt_old = 0;
@@ -348,8 +349,15 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
                 S->mlp.loss_last = l1;
                 ++S->mlp.trained_passes;
             }
-            if (!rc) rc = mlp_infer(S->mlp, S->g.N, S->feat, S->mscale, S->M32, S->st);
-            S->S32.M = S->M32;
+            if (c.precond == YZ_PC_LEARNED) {
+                // the fused iteration reads M as FP16 rows and the FP32 row scale
+                if (!rc) rc = mlp_infer_half(S->mlp, S->g.N, S->feat, S->Yh, S->st);
+                S->S32.Mh = S->Yh;
+                S->S32.Mscale = S->mscale;
+            } else {
+                if (!rc) rc = mlp_infer(S->mlp, S->g.N, S->feat, S->mscale, S->M32, S->st);
+                S->S32.M = S->M32;
+            }
         }
         S->S32.stall_window = c.stall_window;
         if (!rc && (c.precond == YZ_PC_AMG || c.precond == YZ_PC_LEARNED_AMG)) rc = amg_setup(S->S32);
@@ -544,6 +552,7 @@ yz_step *yz_step_create(const yz_case *c, const double *h, const double *s, cons
         if (c->precond == YZ_PC_LEARNED || c->precond == YZ_PC_LEARNED_AMG) {
             if (hipMalloc((void **)&S->feat, (size_t)8 * g.N * sizeof(float)) != hipSuccess) return fail("hipMalloc");
             if (hipMalloc((void **)&S->M32, (size_t)g.ns * g.N * sizeof(float)) != hipSuccess) return fail("hipMalloc");
+            if (hipMalloc((void **)&S->Yh, (size_t)g.ns * g.N * sizeof(__half)) != hipSuccess) return fail("hipMalloc");
             if (mlp_init(S->mlp, g.N, S->st)) return fail("learned preconditioner allocation");
         }
     }
@@ -575,6 +584,7 @@ void yz_step_destroy(yz_step *S) {
     mlp_free(S->mlp);
     for (float *p : {S->b32, S->corr32, S->feat, S->mscale, S->M32})
         if (p) (void)hipFree(p);
+    if (S->Yh) (void)hipFree(S->Yh);
     for (double *p : {S->h, S->hold, S->hold2, S->hnew, S->s, S->c, S->axold, S->b, S->corr, S->scratch})
         if (p) (void)hipFree(p);
     if (S->dmax) (void)hipFree(S->dmax);
