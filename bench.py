@@ -22,8 +22,8 @@ synchronisations.
 The line reports both, their ratio and how far apart their films end.
 
 Roofline: the dominant kernel of the learned run, the fine-level stencil
-product of the Krylov iteration (k_spmv_dot<13, float>), timed with HIP
-events on the step's own stream over the timed region. CPU baseline: the
+product of the fused Krylov iteration (k_f_av: v = A ph with (rh, v)),
+timed with HIP events on the step's own stream over the timed region. CPU baseline: the
 FP64 CPU oracle on one host core over a bounded sample of the same workload.
 """
 import argparse
@@ -42,7 +42,7 @@ if ROOT not in sys.path:
 METRIC = "Newton steps/s, 1024² 2D film: learned-precond FP32 vs plain FP64 AMG @1e-8"
 HBM_PEAK_GBS = 8000.0
 DT = 2e-6
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmv_dot13_f32_allphys1024_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "f_av_fp32_allphys1024_traffic.json")
 
 
 def cpu_model():
@@ -197,7 +197,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "yz::k_spmv_dot<13, float> (fine-level stencil SpMV + fused dot, FP32)",
+        "kernel": "yz::k_f_av (fused FP32 iteration: v = A ph on 64x16 LDS-staged tiles + (rh, v))",
         "launches": L["kcount"],
         "avg_us": round(avg_s * 1e6, 2),
         "bytes_per_launch": bytes_per_launch,

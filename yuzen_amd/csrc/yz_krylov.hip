@@ -61,10 +61,12 @@ namespace yz {
 
 namespace {
 
-constexpr int TX = 64, TY = 8, TB = 256;   // 64 x 8 tiles, 256 threads, two rows per thread
+constexpr int TX = 64, TY = 16, TB = 256;  // 64 x 16 tiles, 256 threads, four rows per thread
 constexpr int RPT = TY / (TB / TX);         // rows per thread
 constexpr int LW = TX + 4, LH = TY + 4;     // LDS tile with the 2-cell halo
-constexpr int G = 512;     // blocks of every kernel (grid-stride over the tiles): partials per reduction
+// blocks of every kernel (grid-stride over the tiles), so partials per
+// reduction: measured at 1024^2 against 256, 512, 1536, 2048 and 4096
+constexpr int G = 1024;
 constexpr int CHUNK = 8;   // iterations enqueued between two reads of the state
 
 // slots of the 2-D 13-point stencil: oM mm om pm Mo mo oo po Po mp op pp oP
@@ -183,7 +185,8 @@ struct Vecs {
     float *x, *r, *rh, *p0, *p1, *ph, *v, *s, *sh, *t;
 };
 
-// the cells of tile (x0, y0) this thread owns: rows ty and ty + 4, column tx
+// the cells of tile (x0, y0) this thread owns: column tx, rows ty, ty + 4,
+// ty + 8, ty + 12
 template <class F>
 __device__ inline void for_own(const StencilOp &op, int x0, int y0, F f) {
     const int lx = threadIdx.x & (TX - 1);
