@@ -101,6 +101,12 @@ typedef struct yz_case {
      * solved again in FP64 with the AMG, when its residual has not halved
      * over stall_window iterations (0: no test) */
     int stall_window;
+    /* 1: the fused FP32 iteration of the learned preconditioner is replayed
+     * from a captured HIP graph (two iterations per graph); 0 (default):
+     * launched kernel by kernel. Same arithmetic, same results to the bit;
+     * at 1024^2 the replay measured 3 % slower than the launches on MI355X,
+     * so it is for callers that capture or batch. */
+    int graphs;
 } yz_case;
 
 /* Fill c with the reference's defaults (Input.cpp getin defaults):
@@ -108,7 +114,8 @@ typedef struct yz_case {
  * NewtonEpsilon 1e-10, NewtonEpsilonRel 1e-6, JacobianUpdateInterval 1,
  * MaxIterations 5000, Epsilon 1e-13, RowScaling on, AMG, EW and
  * extrapolation on, step control off; learned preconditioner trained on
- * 2 passes of 100 Adam steps, rate 2e-3, 4 probes; stall window 40. */
+ * 2 passes of 100 Adam steps, rate 2e-3, 4 probes; stall window 40;
+ * graphs off. */
 void yz_case_defaults(yz_case *c);
 
 const char *yz_last_error(void);

@@ -1461,6 +1461,7 @@ void yzo_case_defaults(yz_case *c) {
     c->dt_min = 1e-15; c->dt_max = 1e30;
     c->learn_passes = 2; c->learn_iters = 100; c->learn_probes = 4; c->learn_lr = 2e-3;
     c->stall_window = 40;
+    c->graphs = 0;
 }
 
 void *yzo_create(const yz_case *c, const double *h, const double *s, const double *conc, double dt0) {

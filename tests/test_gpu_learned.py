@@ -190,3 +190,27 @@ def test_learned_preconditioner_gain():
     assert runs["learned_amg"] < 0.8 * runs["spai0_amg"]
     assert runs["learned"] < 0.5 * runs["jacobi"]
     assert runs["learned"] < 0.8 * runs["untrained"]
+
+
+def test_graph_replay_matches_kernel_by_kernel():
+    # the fused iteration replayed from its captured two-iteration graph
+    # gives the same films and counts, to the bit, as launching it kernel by
+    # kernel, over steps with eager work (assembly, set-up, training)
+    # between the replays
+    c, h, s, cc, dt, _ = cases.allphys_2d(n=96, L=3.0, dt=1e-5)
+    c.precond, c.krylov_fp32 = capi.PC_LEARNED, 1
+    runs = []
+    for graphs in (0, 1):
+        cg = c.copy()
+        cg.graphs = graphs
+        g = HipStep(cg, h, s, cc, dt)
+        counts = []
+        for k in range(4):
+            r = g.advance()
+            assert r[0] == 0
+            counts.append((r[1], r[2]))
+        runs.append((counts, g.h()))
+        g.close()
+    print("kernel by kernel", runs[0][0], "graph", runs[1][0])
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1])

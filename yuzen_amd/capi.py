@@ -44,7 +44,7 @@ class YzCase(C.Structure):
         ("dt_min", C.c_double), ("dt_max", C.c_double),
         ("learn_passes", C.c_int), ("learn_iters", C.c_int),
         ("learn_probes", C.c_int), ("learn_lr", C.c_double),
-        ("stall_window", C.c_int),
+        ("stall_window", C.c_int), ("graphs", C.c_int),
     ]
 
     def copy(self):
@@ -81,6 +81,7 @@ def case_defaults():
     c.dt_min, c.dt_max = 1e-15, 1e30
     c.learn_passes, c.learn_iters, c.learn_probes, c.learn_lr = 2, 100, 4, 2e-3
     c.stall_window = 40
+    c.graphs = 0
     return c
 
 

@@ -67,7 +67,7 @@ constexpr int LW = TX + 4, LH = TY + 4;     // LDS tile with the 2-cell halo
 // blocks of every kernel (grid-stride over the tiles), so partials per
 // reduction: measured at 1024^2 against 256, 512, 1536, 2048 and 4096
 constexpr int G = 1024;
-constexpr int CHUNK = 8;   // iterations enqueued between two reads of the state
+constexpr int CHUNK = 8;   // iterations enqueued between two reads of the state (even)
 
 // slots of the 2-D 13-point stencil: oM mm om pm Mo mo oo po Po mp op pp oP
 constexpr int OX[13] = {0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0};
@@ -75,7 +75,9 @@ constexpr int OY[13] = {-2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2};
 
 // the scalars of the iteration; two copies, by the parity of the kernel's
 // sequence number q: kernel q reads copy (q-1)&1 and block 0 writes copy q&1
-enum { S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_RES, S_EPS, S_NB, S_MARK, S_STATE, S_IT, S_SRH, S_N = 16 };
+// (S_CHK: the partial (r, r) in flight closes an iteration, so K2 applies
+// the exits to it)
+enum { S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_RES, S_EPS, S_NB, S_MARK, S_STATE, S_IT, S_SRH, S_CHK, S_N = 16 };
 enum { ST_RUN = 0, ST_DONE = 1, ST_HALF = 2, ST_BREAK = 3, ST_STALL = 4, ST_CAP = 5, ST_ZERO = 7 };
 constexpr int NQ = 3;      // quantities per partial set
 
@@ -220,24 +222,27 @@ __global__ void __launch_bounds__(TB) k_f_init(StencilOp op, const float *__rest
     store_partials<2>(acc, sq.pout);
 }
 
-// K1 (q = 4k + 1). k = 0: opens the solve from K0's partials, p = r.
-// k >= 1: omega, rho, beta from K4's partials; the x update of iteration
-// k - 1 (or, after the half-step exit, x += alpha ph and the end);
-// r = s - omega t; p = r + beta (p_old - omega v); ph = M p; partial (r, r).
+// K1 (q = 4k + 1; first: k = 0; par: k & 1). k = 0: opens the solve from
+// K0's partials, p = r. k >= 1: omega, rho, beta from K4's partials; the x
+// update of iteration k - 1 (or, after the half-step exit, x += alpha ph
+// and the end); r = s - omega t; p = r + beta (p_old - omega v); ph = M p;
+// partial (r, r).
 __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restrict__ Y,
-                                             const float *__restrict__ ms, Vecs V, Ctl c, int q, int k) {
+                                             const float *__restrict__ ms, Vecs V, Ctl c, int q, int first,
+                                             int par) {
     __shared__ float L[LH][LW];
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
-    const bool half = k > 0 && s[S_STATE] == ST_HALF;
+    const bool half = !first && s[S_STATE] == ST_HALF;
     const double alpha_prev = s[S_ALPHA];
-    if (k == 0) {
+    if (first) {
         double tot[2];
         sum_partials<2>(sq.pin, tot);
         const double nb = sqrt(tot[0]), res = sqrt(tot[1]);
         s[S_NB] = nb; s[S_EPS] = c.tol * nb; s[S_RES] = res; s[S_MARK] = res;
         s[S_RHO] = tot[1]; s[S_ALPHA] = 1; s[S_OMEGA] = 1; s[S_BETA] = 0; s[S_IT] = 0; s[S_SRH] = 0;
+        s[S_CHK] = 0;
         s[S_STATE] = (nb == 0) ? ST_ZERO : (res <= s[S_EPS] ? ST_DONE : ST_RUN);
     } else if (half) {
         s[S_IT] += 1;
@@ -253,6 +258,7 @@ __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restr
             s[S_BETA] = (rho / s[S_RHO]) * (s[S_ALPHA] / omega);
             s[S_RHO] = rho;
             s[S_OMEGA] = omega;
+            s[S_CHK] = 1;
         }
     }
     commit_sc(s, sq.out);
@@ -265,8 +271,8 @@ __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restr
         }
         return;
     }
-    const float *__restrict__ pold = (k & 1) ? V.p0 : V.p1;
-    float *__restrict__ pnew = (k & 1) ? V.p1 : V.p0;
+    const float *__restrict__ pold = par ? V.p0 : V.p1;
+    float *__restrict__ pnew = par ? V.p1 : V.p0;
     const float alpha = (float)s[S_ALPHA], omega = (float)s[S_OMEGA];
     const float beta = (float)s[S_BETA], bw = (float)(s[S_BETA] * s[S_OMEGA]);
     const float *__restrict__ sv = V.s, *__restrict__ tv = V.t, *__restrict__ vv = V.v, *__restrict__ rv = V.r;
@@ -274,13 +280,13 @@ __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restr
     for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
-        if (k == 0) stage(op, x0, y0, L, [&](int j) { return rv[j]; });
+        if (first) stage(op, x0, y0, L, [&](int j) { return rv[j]; });
         else stage(op, x0, y0, L, [&](int j) { return (sv[j] - omega * tv[j]) + beta * pold[j] - bw * vv[j]; });
         __syncthreads();
         for_own(op, x0, y0, [&](int lx, int ly, int i) {
             const float pi = L[ly + 2][lx + 2];
-            const float ri = (k == 0) ? pi : sv[i] - omega * tv[i];
-            if (k > 0) {
+            const float ri = first ? pi : sv[i] - omega * tv[i];
+            if (!first) {
                 V.x[i] += alpha * V.ph[i] + omega * V.sh[i];
                 V.r[i] = ri;
             }
@@ -294,15 +300,16 @@ __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restr
 
 // K2 (q = 4k + 2): for k >= 1 the exits on ||r_k||; v = A ph; partial (rh, v)
 __global__ void __launch_bounds__(TB) k_f_av(StencilOp op, const float *__restrict__ A, Vecs V, Ctl c, int q,
-                                             int k, int work) {
+                                             int work) {
     __shared__ float L[LH][LW];
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
-    if (k > 0 && s[S_STATE] == ST_RUN) {
+    if (s[S_CHK] != 0 && s[S_STATE] == ST_RUN) {
         double tot[1];
         sum_partials<1>(sq.pin, tot);
-        const double it = (double)k;
+        const double it = s[S_IT] + 1;
+        s[S_CHK] = 0;
         const double res = sqrt(tot[0]);
         s[S_IT] = it; s[S_RES] = res;
         int st = ST_RUN;
@@ -434,42 +441,59 @@ int bicgstab_stencil(SolverT<float> &S, const float *b, float *x, int maxit, dou
     V.sh = kr.sh; V.t = kr.t;
     const dim3 grid(G), blk(TB);
     const double *h = kr.host_sc;
-    // the scalars as K2 of iteration k would see them (one block, no work;
-    // the K2 that follows derives the same values from the same inputs)
-    auto check = [&](int k) -> int {
-        const int q = 4 * k + 2;
-        hipLaunchKernelGGL(k_f_av, dim3(1), blk, 0, st, op, (const float *)S.A, V, c, q, k, 0);
+    // the scalars as K2 would see them next (one block, no work; the K2
+    // that follows derives the same values from the same inputs)
+    auto check = [&]() -> int {
+        hipLaunchKernelGGL(k_f_av, dim3(1), blk, 0, st, op, (const float *)S.A, V, c, 2, 0);
         KR_CHECK(hipGetLastError());
-        KR_CHECK(hipMemcpyAsync(kr.host_sc, c.sc + (q & 1) * S_N, S_N * sizeof(double), hipMemcpyDeviceToHost, st));
+        KR_CHECK(hipMemcpyAsync(kr.host_sc, c.sc, S_N * sizeof(double), hipMemcpyDeviceToHost, st));
         KR_CHECK(hipStreamSynchronize(st));
         return 0;
     };
+    // two iterations, k even then odd: every kernel argument is fixed by
+    // its position (sequence parities 2, 3, 4, 5; p buffers by k's parity),
+    // so the pair is captured once as a HIP graph and replayed
+    auto pair = [&](bool timed) {
+        for (int u = 0; u < 2; ++u) {
+            if (timed && u == 0) (void)hipEventRecord(S.kev[0], st);
+            hipLaunchKernelGGL(k_f_av, grid, blk, 0, st, op, (const float *)S.A, V, c, 2, 1);
+            if (timed && u == 0) (void)hipEventRecord(S.kev[1], st);
+            hipLaunchKernelGGL(k_f_sm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, 3);
+            hipLaunchKernelGGL(k_f_at, grid, blk, 0, st, op, (const float *)S.A, V, c, 4);
+            hipLaunchKernelGGL(k_f_pm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, 5, 0, u == 0 ? 1 : 0);
+        }
+    };
+    if (S.use_graph && (!kr.fgexec || kr.fg_x != x || kr.fg_maxit != maxit || kr.fg_window != c.window ||
+                        kr.fg_factor != c.factor)) {
+        if (kr.fgexec) { (void)hipGraphExecDestroy(kr.fgexec); kr.fgexec = nullptr; }
+        hipGraph_t g = nullptr;
+        KR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        pair(false);
+        KR_CHECK(hipStreamEndCapture(st, &g));
+        KR_CHECK(hipGraphInstantiate(&kr.fgexec, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        kr.fg_x = x; kr.fg_maxit = maxit; kr.fg_window = c.window; kr.fg_factor = c.factor;
+    }
+    if (S.ktime && !S.kev[0]) { (void)hipEventCreate(&S.kev[0]); (void)hipEventCreate(&S.kev[1]); }
     hipLaunchKernelGGL(k_f_init, grid, blk, 0, st, op, (const float *)S.A, b, V, c);
-    hipLaunchKernelGGL(k_f_pm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, 1, 0);
-    if (check(0)) return 2;
+    hipLaunchKernelGGL(k_f_pm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, 1, 1, 0);
+    if (check()) return 2;
     int state = (int)h[S_STATE];
     if (state == ST_ZERO) {
         KR_CHECK(hipMemsetAsync(x, 0, op.N * sizeof(float), st));
         return 0;
     }
-    int k = 0;
-    while (state == ST_RUN && k < maxit) {
-        const int n = std::min(CHUNK, maxit - k);
-        for (int u = 0; u < n; ++u, ++k) {
-            const int q = 4 * k;
+    // chunks of CHUNK iterations (an even number: every chunk starts at an
+    // even k); iterations past an exit, or past maxit (K2 sets the cap),
+    // return at once
+    for (int k = 0; state == ST_RUN && k < maxit + CHUNK; k += CHUNK) {
+        for (int u = 0; u < CHUNK / 2; ++u) {
             const bool timed = S.ktime && u == 0;
-            if (timed) {
-                if (!S.kev[0]) { (void)hipEventCreate(&S.kev[0]); (void)hipEventCreate(&S.kev[1]); }
-                (void)hipEventRecord(S.kev[0], st);
-            }
-            hipLaunchKernelGGL(k_f_av, grid, blk, 0, st, op, (const float *)S.A, V, c, q + 2, k, 1);
-            if (timed) (void)hipEventRecord(S.kev[1], st);
-            hipLaunchKernelGGL(k_f_sm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, q + 3);
-            hipLaunchKernelGGL(k_f_at, grid, blk, 0, st, op, (const float *)S.A, V, c, q + 4);
-            hipLaunchKernelGGL(k_f_pm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, q + 5, k + 1);
+            if (S.use_graph && !timed) KR_CHECK(hipGraphLaunch(kr.fgexec, st));
+            else pair(timed);
         }
         KR_CHECK(hipGetLastError());
-        if (check(k)) return 2;
+        if (check()) return 2;
         if (S.ktime) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, S.kev[0], S.kev[1]) == hipSuccess) { S.kms += ms; S.kcount++; }

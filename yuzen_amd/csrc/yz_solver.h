@@ -50,6 +50,12 @@ struct Krylov {
     T *p2 = nullptr;
     double *fpart = nullptr;
     int ftiles = 0;
+    // two iterations of it captured as a HIP graph, and what the capture
+    // baked in
+    hipGraphExec_t fgexec = nullptr;
+    const T *fg_x = nullptr;
+    int fg_maxit = -1, fg_window = -1;
+    double fg_factor = 0;
 };
 
 template <typename T>
@@ -76,6 +82,8 @@ struct SolverT {
     // residual has not fallen by stall_factor over stall_window iterations
     int stall_window = 0;
     double stall_factor = 0.5;
+    // replay the fused iteration from a captured HIP graph
+    int use_graph = 0;
 };
 
 typedef SolverT<double> Solver;

@@ -360,6 +360,7 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
             }
         }
         S->S32.stall_window = c.stall_window;
+        S->S32.use_graph = c.graphs;
         if (!rc && (c.precond == YZ_PC_AMG || c.precond == YZ_PC_LEARNED_AMG)) rc = amg_setup(S->S32);
         if (!rc && c.precond == YZ_PC_JACOBI) rc = jacobi_setup(S->S32);
         if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
@@ -509,6 +510,7 @@ void yz_case_defaults(yz_case *c) {
     c->dt_min = 1e-15; c->dt_max = 1e30;
     c->learn_passes = 2; c->learn_iters = 100; c->learn_probes = 4; c->learn_lr = 2e-3;
     c->stall_window = 40;
+    c->graphs = 0;
 }
 
 const char *yz_last_error(void) { return g_err.c_str(); }
