@@ -206,6 +206,33 @@ int yz_learn_apply(yz_step *st, float *M);
  * [3] probe loss of the first training step, [4] of the last */
 int yz_learn_stats(yz_step *st, double *out5);
 
+/* ---- batched sweep (no reference counterpart) ---------------------------
+ * n independent 2-D films advanced one fixed implicit step per call,
+ * together on one stream: every Newton round assembles all films still
+ * iterating, then solves all their systems in one batched FP32 BiCGSTAB
+ * with the learned preconditioner (one launch per kernel for every film),
+ * then applies each film's Newton decisions. A film's numbers are the ones
+ * it gets in a step context of its own with the same weights (tested).
+ * Each case: dim 2, precond YZ_PC_LEARNED, step_control 0, learn_passes 0
+ * (the films share one network, set with yz_batch_set_params), the same
+ * lin_maxit. h, s, conc:
+ * the films' padded arrays, concatenated in film order; dt0[n]. */
+typedef struct yz_batch yz_batch;
+yz_batch *yz_batch_create(int n, const yz_case *cases, const double *h,
+                          const double *s, const double *conc,
+                          const double *dt0);
+int yz_batch_set_params(yz_batch *b, const float *params);
+/* one step for every film: returns 0 when every film's step was accepted,
+ * 1 when some film's Newton iteration failed (its status is 1), 2 on a
+ * device error; status, nk_iter, lin_iter: n ints each, or NULL */
+int yz_batch_advance(yz_batch *b, int *status, int *nk_iter, int *lin_iter);
+int yz_batch_get_h(yz_batch *b, int film, double *h);
+/* [0] milliseconds of the last yz_batch_advance (HIP events), [1] Newton
+ * passes, [2] Krylov iterations, summed over the films, [3] FP64
+ * fallbacks so far */
+int yz_batch_stats(yz_batch *b, double *out4);
+void yz_batch_destroy(yz_batch *b);
+
 /* ---- linear-solve seam -------------------------------------------------
  * Solve A x = b, A in CRS (rowptr[n+1], col[nnz], val[nnz]; any column
  * order), with BiCGSTAB under settings s. When grid_nx > 0 the rows are

@@ -128,6 +128,13 @@ def load():
     lib.yz_learn_train.argtypes = [C.c_void_p, i, d, i, pd, pd]
     lib.yz_learn_apply.argtypes = [C.c_void_p, pf]
     lib.yz_learn_stats.argtypes = [C.c_void_p, pd]
+    lib.yz_batch_create.restype = C.c_void_p
+    lib.yz_batch_create.argtypes = [i, P(YzCase), pd, pd, pd, pd]
+    lib.yz_batch_set_params.argtypes = [C.c_void_p, pf]
+    lib.yz_batch_advance.argtypes = [C.c_void_p, pi_, pi_, pi_]
+    lib.yz_batch_get_h.argtypes = [C.c_void_p, i, pd]
+    lib.yz_batch_stats.argtypes = [C.c_void_p, pd]
+    lib.yz_batch_destroy.argtypes = [C.c_void_p]
     lib.yz_sizeof_case.restype = C.c_int
     lib.yz_sizeof_lin_settings.restype = C.c_int
     _lib = lib

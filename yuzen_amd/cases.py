@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 from .capi import (YzCase, case_defaults, BC_SYMMETRY, BC_PERIODIC,
-                   PC_AMG)
+                   PC_AMG, PC_LEARNED)
 
 _M64 = (1 << 64) - 1
 
@@ -157,3 +157,48 @@ def allphys_2d(n=1024, steps=20, seed=7, L=10.0, dt=1e-4, nk_eps=1e-8):
     s = np.zeros_like(h)
     conc = 0.5 + 0.1 * np.sin(2 * math.pi * X / L) * np.cos(2 * math.pi * Y / L)
     return c, h.ravel(), s.ravel(), conc.ravel(), dt, steps
+
+
+SWEEP_SIDES = (64, 96, 128, 192, 256, 384, 512)
+
+
+def sweep_film(k, seed=4242, side=None, dx=0.01, dt=2e-6, nk_eps=1e-8):
+    """Film k of the batched parameter sweep: side N drawn uniformly from
+    {64, 96, 128, 192, 256, 384, 512} (or the given side), theta_e ~
+    U(5, 20) degrees (disjoining, h* = 0.01, n = 3, m = 2), spin Sp ~
+    U(0, 0.5) about the square's centre, Bo ~ U(0, 1), capillary on;
+    h = 1 + 0.02 U(-1, 1) per cell; square of side N dx (dx = 0.01),
+    Symmetry walls, frozen uniform concentration; fixed dt 2e-6 (the cell
+    size and step of the 1024^2 all-physics film), Newton
+    tolerance 1e-8; FP32 Krylov with the learned preconditioner (the
+    network shared by the batch: learn_passes 0). The draws for film k are
+    keyed by (seed, k), the film's noise by (seed + 1 + k, ix, iy)."""
+    def draw(j):
+        return 0.5 * (uniform_pm1(seed, k, j) + 1.0)
+    n = side if side is not None else SWEEP_SIDES[min(int(draw(0) * len(SWEEP_SIDES)), len(SWEEP_SIDES) - 1)]
+    theta = 5.0 + 15.0 * draw(1)
+    sp = 0.5 * draw(2)
+    bo = draw(3)
+    L = n * dx
+    c = case_defaults()
+    c.dim, c.nx, c.ny = 2, n, n
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, L, 0.0, L
+    for b in range(4):
+        c.bc[b] = BC_SYMMETRY
+    c.capillary = c.disjoining = c.vertical_gravity = c.centrifugal = 1
+    c.hstar, c.dis_n, c.dis_m = 0.01, 3.0, 2.0
+    c.Dp = disjoining_dp(theta, 0.01)
+    c.Bo, c.Sp = bo, sp
+    c.rot_x, c.rot_y = 0.5 * L, 0.5 * L
+    c.nk_eps = nk_eps
+    c.precond, c.krylov_fp32, c.learn_passes = PC_LEARNED, 1, 0
+    IX, IY = interior_index(c)
+    h = 1.0 + 0.02 * uniform_pm1(seed + 1 + k, np.clip(IX, 0, n - 1), np.clip(IY, 0, n - 1))
+    s = np.zeros_like(h)
+    conc = np.full_like(h, 0.5)
+    return c, h.ravel(), s.ravel(), conc.ravel(), dt
+
+
+def sweep(n_films=256, seed=4242, **kw):
+    """The batched parameter sweep: n_films films of sweep_film."""
+    return [sweep_film(k, seed=seed, **kw) for k in range(n_films)]

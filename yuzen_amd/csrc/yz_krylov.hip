@@ -43,6 +43,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 #include "../../include/yuzen_gpu.h"
 #include "yz_solver.h"
@@ -106,14 +107,17 @@ __device__ inline Seq seq(const Ctl &c, int q) {
 
 // every block sums the previous kernel's G partials of K quantities, in
 // the same fixed order, so all blocks hold the same totals
+// (slots from n on, blocks that had no tile, count as exact zeros: the
+// sums are the same whether a film ran alone on G blocks or on n in a
+// batch)
 template <int K>
-__device__ inline void sum_partials(const double *__restrict__ pin, double (&tot)[K]) {
+__device__ inline void sum_partials(const double *__restrict__ pin, int n, double (&tot)[K]) {
     __shared__ double sh[K][TB / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int m = 0; m < K; ++m) {
         double x = 0;
-        for (int t = threadIdx.x; t < G; t += TB) x += pin[m * G + t];
+        for (int t = threadIdx.x; t < G; t += TB) x += (t < n) ? pin[m * G + t] : 0.0;
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
         if (lane == 0) sh[m][wave] = x;
     }
@@ -129,7 +133,7 @@ __device__ inline void sum_partials(const double *__restrict__ pin, double (&tot
 
 // this block's partials of K quantities
 template <int K>
-__device__ inline void store_partials(double (&v)[K], double *__restrict__ pout) {
+__device__ inline void store_partials(double (&v)[K], double *__restrict__ pout, int lb) {
     __shared__ double sh[K][TB / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -144,7 +148,7 @@ __device__ inline void store_partials(double (&v)[K], double *__restrict__ pout)
         for (int m = 0; m < K; ++m) {
             double s = 0;
             for (int w = 0; w < TB / 64; ++w) s += sh[m][w];
-            pout[m * G + blockIdx.x] = s;
+            pout[m * G + lb] = s;
         }
     }
 }
@@ -153,8 +157,8 @@ __device__ inline void load_sc(const double *__restrict__ in, double (&c)[S_N]) 
 #pragma unroll
     for (int k = 0; k < S_N; ++k) c[k] = in[k];
 }
-__device__ inline void commit_sc(const double (&c)[S_N], double *__restrict__ out) {
-    if (blockIdx.x == 0 && threadIdx.x < S_N) out[threadIdx.x] = c[threadIdx.x];
+__device__ inline void commit_sc(const double (&c)[S_N], double *__restrict__ out, int lb) {
+    if (lb == 0 && threadIdx.x < S_N) out[threadIdx.x] = c[threadIdx.x];
 }
 
 // stage f(j) over tile (x0, y0) and its 2-cell halo into L (folded at the
@@ -187,6 +191,32 @@ struct Vecs {
     float *x, *r, *rh, *p0, *p1, *ph, *v, *s, *sh, *t;
 };
 
+// everything one film's fused iteration reads. A launch covers a table of
+// films: block b works for film blk[b].x as its local block blk[b].y of g
+// (a film runs on as many blocks as it has tiles, up to G, whether alone
+// or in a batch)
+struct Film {
+    StencilOp op;
+    const float *A;
+    const __half *Y;
+    const float *ms;
+    const float *b;
+    Vecs V;
+    Ctl c;
+    int g;
+    double *gather;   // where the state read copies the scalars (S_N)
+};
+// ONE: a single film, its descriptor passed by value (in the kernel
+// arguments, no dependent loads before the work starts); otherwise the
+// tables
+#define FILM_PROLOGUE                                                   \
+    const int2 fb = ONE ? make_int2(0, (int)blockIdx.x) : blk[blockIdx.x]; \
+    const Film &f = ONE ? one : films[fb.x];                            \
+    const int lb = fb.y;                                                \
+    const StencilOp &op = f.op;                                         \
+    const Ctl &c = f.c;                                                 \
+    const Vecs &V = f.V
+
 // the cells of tile (x0, y0) this thread owns: column tx, rows ty, ty + 4,
 // ty + 8, ty + 12
 template <class F>
@@ -200,13 +230,15 @@ __device__ inline void for_own(const StencilOp &op, int x0, int y0, F f) {
 }
 
 // K0 (q = 0): r = b - A x, rh = r; partials (b, b), (r, r)
-__global__ void __launch_bounds__(TB) k_f_init(StencilOp op, const float *__restrict__ A,
-                                               const float *__restrict__ b, Vecs V, Ctl c) {
+template <bool ONE>
+__global__ void __launch_bounds__(TB) k_f_init(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk) {
     __shared__ float L[LH][LW];
+    FILM_PROLOGUE;
+    const float *__restrict__ A = f.A, *__restrict__ b = f.b;
     const Seq sq = seq(c, 0);
     double acc[2] = {0, 0};
     const float *__restrict__ x = V.x;
-    for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
+    for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
         stage(op, x0, y0, L, [&](int j) { return x[j]; });
@@ -219,7 +251,7 @@ __global__ void __launch_bounds__(TB) k_f_init(StencilOp op, const float *__rest
             acc[1] += (double)ri * ri;
         });
     }
-    store_partials<2>(acc, sq.pout);
+    store_partials<2>(acc, sq.pout, lb);
 }
 
 // K1 (q = 4k + 1; first: k = 0; par: k & 1). k = 0: opens the solve from
@@ -227,10 +259,13 @@ __global__ void __launch_bounds__(TB) k_f_init(StencilOp op, const float *__rest
 // update of iteration k - 1 (or, after the half-step exit, x += alpha ph
 // and the end); r = s - omega t; p = r + beta (p_old - omega v); ph = M p;
 // partial (r, r).
-__global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restrict__ Y,
-                                             const float *__restrict__ ms, Vecs V, Ctl c, int q, int first,
-                                             int par) {
+template <bool ONE>
+__global__ void __launch_bounds__(TB) k_f_pm(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk,
+                                             int q, int first, int par) {
     __shared__ float L[LH][LW];
+    FILM_PROLOGUE;
+    const __half *__restrict__ Y = f.Y;
+    const float *__restrict__ ms = f.ms;
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
@@ -238,7 +273,7 @@ __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restr
     const double alpha_prev = s[S_ALPHA];
     if (first) {
         double tot[2];
-        sum_partials<2>(sq.pin, tot);
+        sum_partials<2>(sq.pin, f.g, tot);
         const double nb = sqrt(tot[0]), res = sqrt(tot[1]);
         s[S_NB] = nb; s[S_EPS] = c.tol * nb; s[S_RES] = res; s[S_MARK] = res;
         s[S_RHO] = tot[1]; s[S_ALPHA] = 1; s[S_OMEGA] = 1; s[S_BETA] = 0; s[S_IT] = 0; s[S_SRH] = 0;
@@ -249,7 +284,7 @@ __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restr
         s[S_STATE] = ST_DONE;
     } else if (s[S_STATE] == ST_RUN) {
         double tot[3];
-        sum_partials<3>(sq.pin, tot);   // (t, t), (t, s), (t, rh)
+        sum_partials<3>(sq.pin, f.g, tot);   // (t, t), (t, s), (t, rh)
         const double omega = tot[0] != 0 ? tot[1] / tot[0] : 0.0;
         if (omega == 0 || !(omega == omega)) {
             s[S_STATE] = ST_BREAK;
@@ -261,11 +296,11 @@ __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restr
             s[S_CHK] = 1;
         }
     }
-    commit_sc(s, sq.out);
+    commit_sc(s, sq.out, lb);
     if (!half && s[S_STATE] != ST_RUN) return;
     if (half) {
         const float a = (float)alpha_prev;
-        for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
+        for (int t = lb; t < c.ntx * c.nty; t += f.g) {
             const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
             for_own(op, x0, y0, [&](int, int, int i) { V.x[i] += a * V.ph[i]; });
         }
@@ -277,7 +312,7 @@ __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restr
     const float beta = (float)s[S_BETA], bw = (float)(s[S_BETA] * s[S_OMEGA]);
     const float *__restrict__ sv = V.s, *__restrict__ tv = V.t, *__restrict__ vv = V.v, *__restrict__ rv = V.r;
     double acc[1] = {0};
-    for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
+    for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
         if (first) stage(op, x0, y0, L, [&](int j) { return rv[j]; });
@@ -295,19 +330,22 @@ __global__ void __launch_bounds__(TB) k_f_pm(StencilOp op, const __half *__restr
             acc[0] += (double)ri * ri;
         });
     }
-    store_partials<1>(acc, sq.pout);
+    store_partials<1>(acc, sq.pout, lb);
 }
 
 // K2 (q = 4k + 2): for k >= 1 the exits on ||r_k||; v = A ph; partial (rh, v)
-__global__ void __launch_bounds__(TB) k_f_av(StencilOp op, const float *__restrict__ A, Vecs V, Ctl c, int q,
-                                             int work) {
+template <bool ONE>
+__global__ void __launch_bounds__(TB) k_f_av(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk,
+                                             int q, int work) {
     __shared__ float L[LH][LW];
+    FILM_PROLOGUE;
+    const float *__restrict__ A = f.A;
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
     if (s[S_CHK] != 0 && s[S_STATE] == ST_RUN) {
         double tot[1];
-        sum_partials<1>(sq.pin, tot);
+        sum_partials<1>(sq.pin, f.g, tot);
         const double it = s[S_IT] + 1;
         s[S_CHK] = 0;
         const double res = sqrt(tot[0]);
@@ -323,11 +361,15 @@ __global__ void __launch_bounds__(TB) k_f_av(StencilOp op, const float *__restri
         if (st == ST_RUN && it >= c.maxit) st = ST_CAP;
         s[S_STATE] = st;
     }
-    commit_sc(s, sq.out);
-    if (!work || s[S_STATE] != ST_RUN) return;
+    commit_sc(s, sq.out, lb);
+    if (!work) {
+        if (lb == 0 && threadIdx.x < S_N) f.gather[threadIdx.x] = s[threadIdx.x];
+        return;
+    }
+    if (s[S_STATE] != ST_RUN) return;
     const float *__restrict__ ph = V.ph;
     double acc[1] = {0};
-    for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
+    for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
         stage(op, x0, y0, L, [&](int j) { return ph[j]; });
@@ -338,29 +380,33 @@ __global__ void __launch_bounds__(TB) k_f_av(StencilOp op, const float *__restri
             acc[0] += (double)V.rh[i] * a;
         });
     }
-    store_partials<1>(acc, sq.pout);
+    store_partials<1>(acc, sq.pout, lb);
 }
 
 // K3 (q = 4k + 3): alpha = rho / (rh, v); s = r - alpha v; sh = M s;
 // partials (s, s), (s, rh)
-__global__ void __launch_bounds__(TB) k_f_sm(StencilOp op, const __half *__restrict__ Y,
-                                             const float *__restrict__ ms, Vecs V, Ctl c, int q) {
+template <bool ONE>
+__global__ void __launch_bounds__(TB) k_f_sm(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk,
+                                             int q) {
     __shared__ float L[LH][LW];
+    FILM_PROLOGUE;
+    const __half *__restrict__ Y = f.Y;
+    const float *__restrict__ ms = f.ms;
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
     if (s[S_STATE] == ST_RUN) {
         double tot[1];
-        sum_partials<1>(sq.pin, tot);
+        sum_partials<1>(sq.pin, f.g, tot);
         if (tot[0] == 0 || !(tot[0] == tot[0])) s[S_STATE] = ST_BREAK;
         else s[S_ALPHA] = s[S_RHO] / tot[0];
     }
-    commit_sc(s, sq.out);
+    commit_sc(s, sq.out, lb);
     if (s[S_STATE] != ST_RUN) return;
     const float alpha = (float)s[S_ALPHA];
     const float *__restrict__ rv = V.r, *__restrict__ vv = V.v;
     double acc[2] = {0, 0};
-    for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
+    for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
         stage(op, x0, y0, L, [&](int j) { return rv[j] - alpha * vv[j]; });
@@ -373,29 +419,33 @@ __global__ void __launch_bounds__(TB) k_f_sm(StencilOp op, const __half *__restr
             acc[1] += (double)si * V.rh[i];
         });
     }
-    store_partials<2>(acc, sq.pout);
+    store_partials<2>(acc, sq.pout, lb);
 }
 
 // K4 (q = 4k + 4): the half-step exit on ||s||; t = A sh; partials (t, t),
 // (t, s), (t, rh)
-__global__ void __launch_bounds__(TB) k_f_at(StencilOp op, const float *__restrict__ A, Vecs V, Ctl c, int q) {
+template <bool ONE>
+__global__ void __launch_bounds__(TB) k_f_at(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk,
+                                             int q) {
     __shared__ float L[LH][LW];
+    FILM_PROLOGUE;
+    const float *__restrict__ A = f.A;
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
     if (s[S_STATE] == ST_RUN) {
         double tot[2];
-        sum_partials<2>(sq.pin, tot);
+        sum_partials<2>(sq.pin, f.g, tot);
         const double res = sqrt(tot[0]);
         s[S_SRH] = tot[1];
         if (!(res == res)) s[S_STATE] = ST_BREAK;
         else if (res < s[S_EPS]) { s[S_RES] = res; s[S_STATE] = ST_HALF; }
     }
-    commit_sc(s, sq.out);
+    commit_sc(s, sq.out, lb);
     if (s[S_STATE] != ST_RUN) return;
     const float *__restrict__ sh = V.sh;
     double acc[3] = {0, 0, 0};
-    for (int t = blockIdx.x; t < c.ntx * c.nty; t += G) {
+    for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
         stage(op, x0, y0, L, [&](int j) { return sh[j]; });
@@ -408,10 +458,201 @@ __global__ void __launch_bounds__(TB) k_f_at(StencilOp op, const float *__restri
             acc[2] += (double)a * V.rh[i];
         });
     }
-    store_partials<3>(acc, sq.pout);
+    store_partials<3>(acc, sq.pout, lb);
 }
 
 }  // namespace
+
+int fused_table_reserve(FusedTable &T, int nfilm, int nblk) {
+    if (nfilm > T.cap_films) {
+        if (T.d_films) (void)hipFree(T.d_films);
+        if (T.h_films) (void)hipHostFree(T.h_films);
+        if (T.d_gather) (void)hipFree(T.d_gather);
+        if (T.h_gather) (void)hipHostFree(T.h_gather);
+        T.d_films = T.h_films = nullptr; T.d_gather = T.h_gather = nullptr;
+        KR_CHECK(hipMalloc(&T.d_films, (size_t)nfilm * sizeof(Film)));
+        KR_CHECK(hipHostMalloc(&T.h_films, (size_t)nfilm * sizeof(Film), hipHostMallocDefault));
+        KR_CHECK(hipMalloc((void **)&T.d_gather, (size_t)nfilm * S_N * sizeof(double)));
+        KR_CHECK(hipHostMalloc((void **)&T.h_gather, (size_t)nfilm * S_N * sizeof(double), hipHostMallocDefault));
+        T.cap_films = nfilm;
+    }
+    if (nblk > T.cap_blk) {
+        for (int k = 0; k < 2; ++k) {
+            if (T.d_blk[k]) (void)hipFree(T.d_blk[k]);
+            if (T.h_blk[k]) (void)hipHostFree(T.h_blk[k]);
+            T.d_blk[k] = T.h_blk[k] = nullptr;
+            KR_CHECK(hipMalloc(&T.d_blk[k], (size_t)nblk * sizeof(int2)));
+            KR_CHECK(hipHostMalloc(&T.h_blk[k], (size_t)nblk * sizeof(int2), hipHostMallocDefault));
+        }
+        T.cap_blk = nblk;
+    }
+    return 0;
+}
+
+void fused_table_free(FusedTable &T) {
+    if (T.d_films) (void)hipFree(T.d_films);
+    if (T.h_films) (void)hipHostFree(T.h_films);
+    if (T.d_gather) (void)hipFree(T.d_gather);
+    if (T.h_gather) (void)hipHostFree(T.h_gather);
+    for (int k = 0; k < 2; ++k) {
+        if (T.d_blk[k]) (void)hipFree(T.d_blk[k]);
+        if (T.h_blk[k]) (void)hipHostFree(T.h_blk[k]);
+    }
+    if (T.gexec) (void)hipGraphExecDestroy(T.gexec);
+    T = FusedTable();
+}
+
+int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, const float *const *b,
+                           float *const *x, int maxit, const double *tol, int *iters, double *resid, int *rc,
+                           hipStream_t st) {
+    if (nf <= 0) return 0;
+    int total = 0;
+    for (int f = 0; f < nf; ++f) {
+        const StencilOp &op = S[f]->op0;
+        bool ok = op.dim == 2 && op.ns == 13 && S[f]->Mh && S[f]->Mscale;
+        for (int k = 0; ok && k < 13; ++k) ok = op.ox[k] == OX[k] && op.oy[k] == OY[k];
+        if (!ok) return -1;
+        const int nt = ((op.nx + TX - 1) / TX) * ((op.ny + TY - 1) / TY);
+        total += std::min(nt, G);
+        Krylov<float> &kr = S[f]->kr;
+        if (!kr.ftiles) {
+            KR_CHECK(hipMalloc((void **)&kr.p2, (size_t)op.N * sizeof(float)));
+            KR_CHECK(hipMalloc((void **)&kr.fpart, (size_t)2 * NQ * G * sizeof(double)));
+            kr.ftiles = 1;
+        }
+    }
+    if (fused_table_reserve(T, nf, std::max(total, nf))) return -1;
+    Film *F = (Film *)T.h_films;
+    for (int f = 0; f < nf; ++f) {
+        SolverT<float> &s = *S[f];
+        Krylov<float> &kr = s.kr;
+        Film &d = F[f];
+        d.op = s.op0;
+        d.A = s.A; d.Y = s.Mh; d.ms = s.Mscale; d.b = b[f];
+        d.V.x = x[f]; d.V.r = kr.r; d.V.rh = kr.rh; d.V.p0 = kr.p; d.V.p1 = kr.p2; d.V.ph = kr.ph;
+        d.V.v = kr.v; d.V.s = kr.s; d.V.sh = kr.sh; d.V.t = kr.t;
+        d.c.sc = kr.sc + 32;   // two copies of S_N scalars (the solver's array holds 64)
+        d.c.part = kr.fpart;
+        d.c.ntx = (s.op0.nx + TX - 1) / TX; d.c.nty = (s.op0.ny + TY - 1) / TY;
+        d.c.maxit = maxit; d.c.window = s.stall_window; d.c.factor = s.stall_factor; d.c.tol = tol[f];
+        d.g = std::min(d.c.ntx * d.c.nty, G);
+        d.gather = T.d_gather + (size_t)f * S_N;
+    }
+    KR_CHECK(hipMemcpyAsync(T.d_films, T.h_films, (size_t)nf * sizeof(Film), hipMemcpyHostToDevice, st));
+    // block tables: [0] every block of the given films, [1] one block per film
+    auto fill = [&](int k, const int *which, int n, bool all) -> int {
+        int2 *B = (int2 *)T.h_blk[k];
+        int m = 0;
+        for (int u = 0; u < n; ++u) {
+            const int f = which ? which[u] : u;
+            const int g = all ? F[f].g : 1;
+            for (int l = 0; l < g; ++l) B[m++] = make_int2(f, l);
+        }
+        return m;
+    };
+    std::vector<int> active(nf);
+    for (int f = 0; f < nf; ++f) active[f] = f;
+    int nblk = fill(0, nullptr, nf, true);
+    int nchk = fill(1, nullptr, nf, false);
+    KR_CHECK(hipMemcpyAsync(T.d_blk[0], T.h_blk[0], (size_t)nblk * sizeof(int2), hipMemcpyHostToDevice, st));
+    KR_CHECK(hipMemcpyAsync(T.d_blk[1], T.h_blk[1], (size_t)nchk * sizeof(int2), hipMemcpyHostToDevice, st));
+    const Film *dF = (const Film *)T.d_films;
+    const int2 *dB = (const int2 *)T.d_blk[0], *dC = (const int2 *)T.d_blk[1];
+    const dim3 blk(TB);
+    const bool one = nf == 1;
+    const Film F0 = F[0];
+#define FLAUNCH(K, nb, B, ...)                                                              \
+    do {                                                                                    \
+        if (one) hipLaunchKernelGGL(K<true>, dim3(nb), blk, 0, st, F0, dF, B, ##__VA_ARGS__);  \
+        else hipLaunchKernelGGL(K<false>, dim3(nb), blk, 0, st, F0, dF, B, ##__VA_ARGS__);     \
+    } while (0)
+    const double *h = T.h_gather;
+    // the scalars as K2 would see them next (one block per film, no work;
+    // the K2 that follows derives the same values from the same inputs)
+    auto check = [&]() -> int {
+        FLAUNCH(k_f_av, nchk, dC, 2, 0);
+        KR_CHECK(hipGetLastError());
+        KR_CHECK(hipMemcpyAsync(T.h_gather, T.d_gather, (size_t)nf * S_N * sizeof(double), hipMemcpyDeviceToHost, st));
+        KR_CHECK(hipStreamSynchronize(st));
+        return 0;
+    };
+    SolverT<float> &S0 = *S[0];
+    const bool timing = nf == 1 && S0.ktime;
+    if (timing && !S0.kev[0]) { (void)hipEventCreate(&S0.kev[0]); (void)hipEventCreate(&S0.kev[1]); }
+    // two iterations, k even then odd: every kernel argument is fixed by
+    // its position (sequence parities 2, 3, 4, 5; p buffers by k's parity),
+    // so the pair can be captured once as a HIP graph and replayed
+    auto pair = [&](bool timed) {
+        for (int u = 0; u < 2; ++u) {
+            if (timed && u == 0) (void)hipEventRecord(S0.kev[0], st);
+            FLAUNCH(k_f_av, nblk, dB, 2, 1);
+            if (timed && u == 0) (void)hipEventRecord(S0.kev[1], st);
+            FLAUNCH(k_f_sm, nblk, dB, 3);
+            FLAUNCH(k_f_at, nblk, dB, 4);
+            FLAUNCH(k_f_pm, nblk, dB, 5, 0, u == 0 ? 1 : 0);
+        }
+    };
+    // a graph only for one film: its tables never change between solves
+    const bool graph = nf == 1 && S0.use_graph;
+    if (graph && (!T.gexec || T.g_x != x[0] || T.g_maxit != maxit || T.g_window != S0.stall_window ||
+                  T.g_factor != S0.stall_factor || T.g_nblk != nblk)) {
+        if (T.gexec) { (void)hipGraphExecDestroy(T.gexec); T.gexec = nullptr; }
+        hipGraph_t gr = nullptr;
+        KR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        pair(false);
+        KR_CHECK(hipStreamEndCapture(st, &gr));
+        KR_CHECK(hipGraphInstantiate(&T.gexec, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        T.g_x = x[0]; T.g_maxit = maxit; T.g_window = S0.stall_window; T.g_factor = S0.stall_factor;
+        T.g_nblk = nblk;
+    }
+    FLAUNCH(k_f_init, nblk, dB);
+    FLAUNCH(k_f_pm, nblk, dB, 1, 1, 0);
+    if (check()) return -1;
+    auto running = [&](int f) { return (int)h[(size_t)f * S_N + S_STATE] == ST_RUN; };
+    for (int f = 0; f < nf; ++f)
+        if ((int)h[(size_t)f * S_N + S_STATE] == ST_ZERO) KR_CHECK(hipMemsetAsync(x[f], 0, S[f]->op0.N * sizeof(float), st));
+    // chunks of CHUNK iterations (an even number: every chunk starts at an
+    // even k) over the films still running; iterations past an exit, or
+    // past maxit (K2 sets the cap), return at once
+    int last_na = nf;
+    for (int k = 0; k < maxit + CHUNK; k += CHUNK) {
+        int na = 0;
+        for (int f = 0; f < nf; ++f) if (running(f)) active[na++] = f;
+        if (!na) break;
+        if (na != last_na) {
+            // only the films still running get blocks
+            nblk = fill(0, active.data(), na, true);
+            KR_CHECK(hipMemcpyAsync(T.d_blk[0], T.h_blk[0], (size_t)nblk * sizeof(int2), hipMemcpyHostToDevice, st));
+            last_na = na;
+        }
+        for (int u = 0; u < CHUNK / 2; ++u) {
+            const bool timed = timing && u == 0;
+            if (graph && !timed) KR_CHECK(hipGraphLaunch(T.gexec, st));
+            else pair(timed);
+        }
+        KR_CHECK(hipGetLastError());
+        if (check()) return -1;
+        if (timing) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, S0.kev[0], S0.kev[1]) == hipSuccess) { S0.kms += ms; S0.kcount++; }
+        }
+    }
+    for (int f = 0; f < nf; ++f) {
+        const double *hf = h + (size_t)f * S_N;
+        const int state = (int)hf[S_STATE];
+        iters[f] = (int)hf[S_IT];
+        resid[f] = hf[S_NB] > 0 ? hf[S_RES] / hf[S_NB] : 0.0;
+        switch (state) {
+        case ST_DONE: case ST_ZERO: rc[f] = 0; break;
+        case ST_STALL: rc[f] = 3; break;
+        case ST_CAP: case ST_RUN: rc[f] = (resid[f] <= tol[f]) ? 0 : 1; break;
+        default: rc[f] = 2; break;
+        }
+    }
+    return 0;
+#undef FLAUNCH
+}
 
 int bicgstab_stencil(SolverT<float> &S, const float *b, float *x, int maxit, double tol, int *iters,
                      double *resid) {
@@ -424,92 +665,10 @@ int bicgstab_stencil(SolverT<float> &S, const float *b, float *x, int maxit, dou
         if (!S.M) return 2;
         return bicgstab(S, b, x, YZ_PC_LEARNED, maxit, tol, iters, resid);
     }
-    Krylov<float> &kr = S.kr;
-    hipStream_t st = S.stream;
-    const int ntx = (op.nx + TX - 1) / TX, nty = (op.ny + TY - 1) / TY;
-    if (!kr.ftiles) {
-        KR_CHECK(hipMalloc((void **)&kr.p2, (size_t)op.N * sizeof(float)));
-        KR_CHECK(hipMalloc((void **)&kr.fpart, (size_t)2 * NQ * G * sizeof(double)));
-        kr.ftiles = 1;
-    }
-    Ctl c;
-    c.sc = kr.sc + 32;   // two copies of S_N scalars (the solver's array holds 64)
-    c.part = kr.fpart; c.ntx = ntx; c.nty = nty;
-    c.maxit = maxit; c.window = S.stall_window; c.factor = S.stall_factor; c.tol = tol;
-    Vecs V;
-    V.x = x; V.r = kr.r; V.rh = kr.rh; V.p0 = kr.p; V.p1 = kr.p2; V.ph = kr.ph; V.v = kr.v; V.s = kr.s;
-    V.sh = kr.sh; V.t = kr.t;
-    const dim3 grid(G), blk(TB);
-    const double *h = kr.host_sc;
-    // the scalars as K2 would see them next (one block, no work; the K2
-    // that follows derives the same values from the same inputs)
-    auto check = [&]() -> int {
-        hipLaunchKernelGGL(k_f_av, dim3(1), blk, 0, st, op, (const float *)S.A, V, c, 2, 0);
-        KR_CHECK(hipGetLastError());
-        KR_CHECK(hipMemcpyAsync(kr.host_sc, c.sc, S_N * sizeof(double), hipMemcpyDeviceToHost, st));
-        KR_CHECK(hipStreamSynchronize(st));
-        return 0;
-    };
-    // two iterations, k even then odd: every kernel argument is fixed by
-    // its position (sequence parities 2, 3, 4, 5; p buffers by k's parity),
-    // so the pair is captured once as a HIP graph and replayed
-    auto pair = [&](bool timed) {
-        for (int u = 0; u < 2; ++u) {
-            if (timed && u == 0) (void)hipEventRecord(S.kev[0], st);
-            hipLaunchKernelGGL(k_f_av, grid, blk, 0, st, op, (const float *)S.A, V, c, 2, 1);
-            if (timed && u == 0) (void)hipEventRecord(S.kev[1], st);
-            hipLaunchKernelGGL(k_f_sm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, 3);
-            hipLaunchKernelGGL(k_f_at, grid, blk, 0, st, op, (const float *)S.A, V, c, 4);
-            hipLaunchKernelGGL(k_f_pm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, 5, 0, u == 0 ? 1 : 0);
-        }
-    };
-    if (S.use_graph && (!kr.fgexec || kr.fg_x != x || kr.fg_maxit != maxit || kr.fg_window != c.window ||
-                        kr.fg_factor != c.factor)) {
-        if (kr.fgexec) { (void)hipGraphExecDestroy(kr.fgexec); kr.fgexec = nullptr; }
-        hipGraph_t g = nullptr;
-        KR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        pair(false);
-        KR_CHECK(hipStreamEndCapture(st, &g));
-        KR_CHECK(hipGraphInstantiate(&kr.fgexec, g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
-        kr.fg_x = x; kr.fg_maxit = maxit; kr.fg_window = c.window; kr.fg_factor = c.factor;
-    }
-    if (S.ktime && !S.kev[0]) { (void)hipEventCreate(&S.kev[0]); (void)hipEventCreate(&S.kev[1]); }
-    hipLaunchKernelGGL(k_f_init, grid, blk, 0, st, op, (const float *)S.A, b, V, c);
-    hipLaunchKernelGGL(k_f_pm, grid, blk, 0, st, op, S.Mh, S.Mscale, V, c, 1, 1, 0);
-    if (check()) return 2;
-    int state = (int)h[S_STATE];
-    if (state == ST_ZERO) {
-        KR_CHECK(hipMemsetAsync(x, 0, op.N * sizeof(float), st));
-        return 0;
-    }
-    // chunks of CHUNK iterations (an even number: every chunk starts at an
-    // even k); iterations past an exit, or past maxit (K2 sets the cap),
-    // return at once
-    for (int k = 0; state == ST_RUN && k < maxit + CHUNK; k += CHUNK) {
-        for (int u = 0; u < CHUNK / 2; ++u) {
-            const bool timed = S.ktime && u == 0;
-            if (S.use_graph && !timed) KR_CHECK(hipGraphLaunch(kr.fgexec, st));
-            else pair(timed);
-        }
-        KR_CHECK(hipGetLastError());
-        if (check()) return 2;
-        if (S.ktime) {
-            float ms = 0;
-            if (hipEventElapsedTime(&ms, S.kev[0], S.kev[1]) == hipSuccess) { S.kms += ms; S.kcount++; }
-        }
-        state = (int)h[S_STATE];
-    }
-    *iters = (int)h[S_IT];
-    const double nb = h[S_NB];
-    *resid = nb > 0 ? h[S_RES] / nb : 0.0;
-    switch (state) {
-    case ST_DONE: return 0;
-    case ST_STALL: return 3;
-    case ST_CAP:
-    case ST_RUN: return (*resid <= tol) ? 0 : 1;
-    default: return 2;
-    }
+    SolverT<float> *Sp = &S;
+    int rc = 2;
+    if (bicgstab_stencil_batch(S.kr.table, 1, &Sp, &b, &x, maxit, &tol, iters, resid, &rc, S.stream)) return 2;
+    return rc;
 }
 
 }  // namespace yz

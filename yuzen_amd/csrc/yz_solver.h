@@ -37,6 +37,21 @@ struct Amg {
     int built = 0;
 };
 
+// device and pinned tables of one launch of the fused iteration (the
+// films, the block-to-film map, the state read) and, for a single film,
+// its captured HIP graph with what the capture baked in
+struct FusedTable {
+    void *d_films = nullptr, *h_films = nullptr;
+    void *d_blk[2] = {nullptr, nullptr}, *h_blk[2] = {nullptr, nullptr};
+    double *d_gather = nullptr, *h_gather = nullptr;
+    int cap_films = 0, cap_blk = 0;
+    hipGraphExec_t gexec = nullptr;
+    const float *g_x = nullptr;
+    int g_maxit = -1, g_window = -1, g_nblk = -1;
+    double g_factor = 0;
+};
+void fused_table_free(FusedTable &T);
+
 template <typename T>
 struct Krylov {
     int N = 0;
@@ -50,12 +65,7 @@ struct Krylov {
     T *p2 = nullptr;
     double *fpart = nullptr;
     int ftiles = 0;
-    // two iterations of it captured as a HIP graph, and what the capture
-    // baked in
-    hipGraphExec_t fgexec = nullptr;
-    const T *fg_x = nullptr;
-    int fg_maxit = -1, fg_window = -1;
-    double fg_factor = 0;
+    FusedTable table;   // the launch tables when this solver runs alone
 };
 
 template <typename T>
@@ -108,6 +118,14 @@ int bicgstab(SolverT<T> &S, const T *b, T *x, int precond, int maxit, double tol
 // bicgstab). FP32. Same contract as bicgstab.
 int bicgstab_stencil(SolverT<float> &S, const float *b, float *x, int maxit, double tol, int *iters,
                      double *resid);
+// the same for nf films at once on stream st, each with its own solver,
+// right-hand side and guess/result: one launch per kernel covers every film
+// still running, and a film's arithmetic is the same as alone. rc, iters,
+// resid per film (bicgstab's codes); returns nonzero on a device error or
+// an operator it cannot take.
+int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, const float *const *b,
+                           float *const *x, int maxit, const double *tol, int *iters, double *resid, int *rc,
+                           hipStream_t st);
 
 template <typename T>
 void launch_spmv(const StencilOp &op, const T *val, const T *x, T *y, hipStream_t st);

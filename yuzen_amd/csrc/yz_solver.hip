@@ -847,8 +847,7 @@ void solver_free(SolverT<T> &S) {
     f(k.r); f(k.rh); f(k.p); f(k.v); f(k.s); f(k.t); f(k.ph); f(k.sh); f(k.part); f(k.sc);
     f(k.p2); f(k.fpart);
     k.ftiles = 0;
-    if (k.fgexec) { (void)hipGraphExecDestroy(k.fgexec); k.fgexec = nullptr; }
-    k.fg_x = nullptr; k.fg_maxit = -1; k.fg_window = -1;
+    fused_table_free(k.table);
     for (auto &e : S.kev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (k.host_sc) { (void)hipHostFree(k.host_sc); k.host_sc = nullptr; }
     k.N = 0;

@@ -158,6 +158,7 @@ struct yz_step {
     Geo g;
     RowArgs ra;
     hipStream_t st = nullptr;
+    bool own_stream = true;
     double *h = nullptr, *hold = nullptr, *hold2 = nullptr, *hnew = nullptr, *s = nullptr, *c = nullptr;
     double *axold = nullptr, *b = nullptr, *corr = nullptr, *scratch = nullptr;
     unsigned long long *dmax = nullptr;   // [0] rhs, [1] corr, [2] move, [3] trunc
@@ -288,6 +289,52 @@ static bool fp32_path(const yz_case &c) {
 // the FP32 inner solve cannot reach below this relative residual
 static constexpr double kEpsFP32 = 1.0e-6;
 
+// the linear tolerance of this pass (NewtonKantorovich.cpp:108-131):
+// Eisenstat-Walker from max |Rhs| (read back into hmax[0]) when on
+static double ew_eps(yz_step *S) {
+    const yz_case &c = S->cs;
+    if (!c.ew) return c.lin_eps;
+    const double kEtaMax = 1.0e-4;
+    const double rn = bits_to_double(S->hmax[0]);
+    double eta = kEtaMax;
+    if (S->nk_pass > 1 && S->ew_prev > 0) {
+        const double ratio = rn / S->ew_prev;
+        eta = 0.9 * ratio * ratio;
+        const double safe = 0.9 * S->ew_eta * S->ew_eta;
+        if (safe > 0.1 && safe > eta) eta = safe;
+    }
+    if (eta > kEtaMax) eta = kEtaMax;
+    if (eta < c.lin_eps) eta = c.lin_eps;
+    S->ew_eta = eta; S->ew_prev = rn;
+    return eta;
+}
+
+// the pass again in FP64 with the AMG, after the FP32 solve failed or its
+// stall test tripped: FP64 assembly, AMG set-up, FP64 BiCGSTAB from zero.
+// Returns the solve's code; iter accumulates.
+static int fp64_fallback(yz_step *S, double eps, int *iter, double *resid, int *dev_err) {
+    const yz_case &c = S->cs;
+    ++S->fallbacks;
+    YZ_TRY(hipMemsetAsync(S->corr32, 0, S->g.N * sizeof(float), S->st));
+    YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+    rows_launch(S, 1, S->h);
+    YZ_TRY(hipGetLastError());
+    YZ_TRY(hipMemsetAsync(S->corr, 0, S->g.N * sizeof(double), S->st));
+    int it2 = 0;
+    double r2 = 0;
+    YZ_TRY(hipEventRecord(S->ev[2], S->st));
+    if (amg_setup(S->S)) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
+    YZ_TRY(hipEventRecord(S->ev[3], S->st));
+    const int ch = bicgstab(S->S, S->b, S->corr, YZ_PC_AMG, c.lin_maxit, eps, &it2, &r2);
+    YZ_TRY(hipEventRecord(S->ev[4], S->st));
+    YZ_TRY(hipEventSynchronize(S->ev[4]));
+    S->tms[2] += ev_ms(S->ev[2], S->ev[3]);
+    S->tms[3] += ev_ms(S->ev[3], S->ev[4]);
+    *iter += it2;
+    *resid = r2;
+    return ch;
+}
+
 // Solve (NewtonKantorovich.cpp:74-154): assemble, EW tolerance, solve.
 // With the FP32 inner solve the pass is an inexact-Newton step: the row
 // kernel writes FP32 planes, the Krylov vectors are FP32, the tolerance is
@@ -315,22 +362,7 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     ++S->nk_pass;
     if (sync_read(S)) { *dev_err = 1; return 2; }
     S->tms[0] += ev_ms(S->ev[0], S->ev[1]);
-    double eps = c.lin_eps;
-    if (c.ew) {
-        const double kEtaMax = 1.0e-4;
-        const double rn = bits_to_double(S->hmax[0]);
-        double eta = kEtaMax;
-        if (S->nk_pass > 1 && S->ew_prev > 0) {
-            const double ratio = rn / S->ew_prev;
-            eta = 0.9 * ratio * ratio;
-            const double safe = 0.9 * S->ew_eta * S->ew_eta;
-            if (safe > 0.1 && safe > eta) eta = safe;
-        }
-        if (eta > kEtaMax) eta = kEtaMax;
-        if (eta < c.lin_eps) eta = c.lin_eps;
-        S->ew_eta = eta; S->ew_prev = rn;
-        eps = eta;
-    }
+    const double eps = ew_eps(S);
     YZ_TRY(hipEventRecord(S->ev[2], S->st));
     int ch = -1;
     if (f32) {
@@ -378,25 +410,8 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
             hipLaunchKernelGGL(k_f2d, dim3(nb(S->g.N)), dim3(TB), 0, S->st, S->g.N, (const float *)S->corr32, S->corr);
             return 0;
         }
-        // fallback: this pass again in FP64 with the AMG
-        ++S->fallbacks;
-        YZ_TRY(hipMemsetAsync(S->corr32, 0, S->g.N * sizeof(float), S->st));
-        YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
-        rows_launch(S, 1, S->h);
-        YZ_TRY(hipGetLastError());
-        YZ_TRY(hipMemsetAsync(S->corr, 0, S->g.N * sizeof(double), S->st));
-        int it2 = 0;
-        double r2 = 0;
-        YZ_TRY(hipEventRecord(S->ev[2], S->st));
-        if (amg_setup(S->S)) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
-        YZ_TRY(hipEventRecord(S->ev[3], S->st));
-        ch = bicgstab(S->S, S->b, S->corr, YZ_PC_AMG, c.lin_maxit, eps, &it2, &r2);
-        YZ_TRY(hipEventRecord(S->ev[4], S->st));
-        YZ_TRY(hipEventSynchronize(S->ev[4]));
-        S->tms[2] += ev_ms(S->ev[2], S->ev[3]);
-        S->tms[3] += ev_ms(S->ev[3], S->ev[4]);
-        *iter += it2;
-        *resid = r2;
+        ch = fp64_fallback(S, eps, iter, resid, dev_err);
+        if (*dev_err) return 2;
     } else {
         int rc;
         if (c.precond == YZ_PC_AMG) rc = amg_setup(S->S); else rc = jacobi_setup(S->S);
@@ -491,6 +506,9 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
     return 0;
 }
 
+static yz_step *step_new(const yz_case *c, const double *h, const double *s, const double *conc, double dt0,
+                         hipStream_t shared);
+
 extern "C" {
 
 void yz_case_defaults(yz_case *c) {
@@ -518,6 +536,16 @@ int yz_sizeof_case(void) { return (int)sizeof(yz_case); }
 int yz_sizeof_lin_settings(void) { return (int)sizeof(yz_lin_settings); }
 
 yz_step *yz_step_create(const yz_case *c, const double *h, const double *s, const double *conc, double dt0) {
+    return step_new(c, h, s, conc, dt0, nullptr);
+}
+
+}  // extern "C"
+
+// a step context; with a shared stream it is a member of a batch: it runs
+// on the batch's stream and has no network of its own (the batch holds the
+// shared one)
+static yz_step *step_new(const yz_case *c, const double *h, const double *s, const double *conc, double dt0,
+                         hipStream_t shared) {
     if (!c || !h || !s || !conc) { set_err("null argument"); return nullptr; }
     if (check_case(*c)) return nullptr;
     yz_step *S = new yz_step();
@@ -526,7 +554,8 @@ yz_step *yz_step_create(const yz_case *c, const double *h, const double *s, cons
     const Geo &g = S->g;
     const int np = g.nxp * g.nyp;
     auto fail = [&](const char *what) { set_err(what); yz_step_destroy(S); return (yz_step *)nullptr; };
-    if (hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+    if (shared) { S->st = shared; S->own_stream = false; }
+    else if (hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking) != hipSuccess) return fail("stream");
     double **pads[] = {&S->h, &S->hold, &S->hold2, &S->hnew, &S->s, &S->c};
     for (double **p : pads) if (hipMalloc((void **)p, np * sizeof(double)) != hipSuccess) return fail("hipMalloc");
     double **vecs[] = {&S->axold, &S->b, &S->corr, &S->scratch};
@@ -540,10 +569,11 @@ yz_step *yz_step_create(const yz_case *c, const double *h, const double *s, cons
     if (hipMemcpy(S->s, s, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return fail("copy");
     if (hipMemcpy(S->c, conc, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return fail("copy");
     for (double *p : {S->hnew, S->s, S->c}) hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, p);
-    hipMemcpyAsync(S->hold, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st);
-    hipMemcpyAsync(S->hold2, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st);
-    hipMemcpyAsync(S->h, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st);
-    hipMemsetAsync(S->corr, 0, g.N * sizeof(double), S->st);
+    if (hipMemcpyAsync(S->hold, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st) != hipSuccess ||
+        hipMemcpyAsync(S->hold2, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st) != hipSuccess ||
+        hipMemcpyAsync(S->h, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st) != hipSuccess ||
+        hipMemsetAsync(S->corr, 0, g.N * sizeof(double), S->st) != hipSuccess)
+        return fail("init");
     if (hipStreamSynchronize(S->st) != hipSuccess) return fail("init");
     if (solver_init(S->S, fine_op(g), S->st)) return fail("solver allocation");
     if (fp32_path(*c)) {
@@ -553,9 +583,11 @@ yz_step *yz_step_create(const yz_case *c, const double *h, const double *s, cons
         if (hipMemset(S->corr32, 0, g.N * sizeof(float)) != hipSuccess) return fail("memset");
         if (c->precond == YZ_PC_LEARNED || c->precond == YZ_PC_LEARNED_AMG) {
             if (hipMalloc((void **)&S->feat, (size_t)8 * g.N * sizeof(float)) != hipSuccess) return fail("hipMalloc");
-            if (hipMalloc((void **)&S->M32, (size_t)g.ns * g.N * sizeof(float)) != hipSuccess) return fail("hipMalloc");
             if (hipMalloc((void **)&S->Yh, (size_t)g.ns * g.N * sizeof(__half)) != hipSuccess) return fail("hipMalloc");
-            if (mlp_init(S->mlp, g.N, S->st)) return fail("learned preconditioner allocation");
+            if (!shared) {
+                if (hipMalloc((void **)&S->M32, (size_t)g.ns * g.N * sizeof(float)) != hipSuccess) return fail("hipMalloc");
+                if (mlp_init(S->mlp, g.N, S->st)) return fail("learned preconditioner allocation");
+            }
         }
     }
     // row kernel arguments
@@ -579,6 +611,8 @@ yz_step *yz_step_create(const yz_case *c, const double *h, const double *s, cons
     return S;
 }
 
+extern "C" {
+
 void yz_step_destroy(yz_step *S) {
     if (!S) return;
     solver_free(S->S);
@@ -594,7 +628,7 @@ void yz_step_destroy(yz_step *S) {
     if (S->hmax) (void)hipHostFree(S->hmax);
     if (S->hflag) (void)hipHostFree(S->hflag);
     for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
-    if (S->st) (void)hipStreamDestroy(S->st);
+    if (S->st && S->own_stream) (void)hipStreamDestroy(S->st);
     delete S;
 }
 
@@ -879,6 +913,353 @@ int yz_learn_stats(yz_step *S, double *out5) {
     out5[3] = S->mlp.loss_first;
     out5[4] = S->mlp.loss_last;
     return 0;
+}
+
+}  // extern "C"
+
+// ---- batched sweep ------------------------------------------------------
+//
+// Many independent films advanced one implicit step together. Each film is
+// a step context on the batch's one stream, with its own fields, Newton
+// state and FP32 solver; the network is the batch's, shared. The Newton
+// passes of all films proceed in rounds: every round enqueues the
+// assembly of every film still iterating, reads their max |Rhs| with one
+// synchronisation, solves all their systems in one batched fused BiCGSTAB
+// (one launch per kernel for every film still running), reads their
+// corrections' max with one synchronisation, and applies each film's
+// Newton decisions (NewtonKantorovich.cpp:157-422, as nk_loop; a film's
+// numbers are the ones it gets alone). Rare paths (the stagnation floor,
+// the FP64 fallback) run per film.
+
+namespace {
+
+enum { BF_START, BF_PASS, BF_END, BF_DONE };
+
+struct BatchFilm {
+    yz_step *S = nullptr;
+    int phase = BF_START;
+    bool retried = false, ok = false, guessed = false, extrap_pending = false;
+    int outer = 0, stall = 0, status = 0, nk_out = 0;
+    double resid_eq_old = 0, resid_first = 0, resid_best = 0, resid_eq = 0, eps = 0;
+    long long lin_sum = 0;
+};
+
+}  // namespace
+
+struct yz_batch {
+    hipStream_t st = nullptr;
+    std::vector<BatchFilm> f;
+    Mlp mlp;                  // the shared network
+    FusedTable table;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double ms_last = 0;
+    long long passes_last = 0, lin_last = 0;
+};
+
+// start (or, with extrapolation blocked, restart) a film's Newton loop:
+// the extrapolated guess when the step allows one; its check is read back
+// with the round's synchronisation
+static int bf_loop_start(BatchFilm &F) {
+    yz_step *S = F.S;
+    const yz_case &c = S->cs;
+    const Geo &g = S->g;
+    S->nk_pass = 0;
+    S->lin_step = 0;
+    S->extrap_move = 0;
+    F.guessed = false;
+    F.extrap_pending = false;
+    F.outer = 0; F.stall = 0;
+    F.resid_eq_old = 1e40; F.resid_first = 0; F.resid_best = 1e40; F.resid_eq = 0;
+    if (c.extrap && !S->extrap_blocked && !S->first_step) {
+        const double r = (S->dt_old > 0) ? S->dt_now / S->dt_old : 0.0;
+        if (r > 0) {
+            const int np = g.nxp * g.nyp;
+            YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+            YZ_TRY(hipMemsetAsync(S->dflag, 0, sizeof(int), S->st));
+            hipLaunchKernelGGL(k_extrap, dim3(nb(np)), dim3(TB), 0, S->st, g, S->hold, S->hold2, r, S->h,
+                               S->dmax + 2, S->dflag);
+            YZ_TRY(hipMemcpyAsync(S->hmax, S->dmax, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, S->st));
+            YZ_TRY(hipMemcpyAsync(S->hflag, S->dflag, sizeof(int), hipMemcpyDeviceToHost, S->st));
+            F.extrap_pending = true;
+        }
+    }
+    return 0;
+}
+
+// after the round's synchronisation: take or drop the guess, then the
+// explicit half A(h^n) h^n (BeforeNKLoop)
+static int bf_loop_begin(BatchFilm &F) {
+    yz_step *S = F.S;
+    const int np = S->g.nxp * S->g.nyp;
+    if (F.extrap_pending) {
+        const double move = bits_to_double(S->hmax[2]);
+        if (!S->hflag[0] && move > 0) { S->extrap_move = move; F.guessed = true; }
+    }
+    if (!F.guessed) YZ_TRY(hipMemcpyAsync(S->h, S->hold, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    RowArgs keep = S->ra;
+    S->ra.ax = S->axold;
+    rows_launch(S, 0, S->hold);
+    S->ra = keep;
+    YZ_TRY(hipGetLastError());
+    return 0;
+}
+
+// the assembly of a pass (FP32 planes, features, row scale), max |Rhs|
+// read back with the round's synchronisation
+static int bf_assemble(BatchFilm &F) {
+    yz_step *S = F.S;
+    S->ra.dt = S->dt_now;
+    S->ra.axold = S->axold;
+    S->ra.hold = S->hold;
+    RowArgs keep = S->ra;
+    S->ra.ia32 = S->S32.A; S->ra.b32 = S->b32; S->ra.feat = S->feat; S->ra.mscale = S->mscale;
+    YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+    rows_launch(S, 1, S->h);
+    S->ra = keep;
+    YZ_TRY(hipGetLastError());
+    ++S->nk_pass;
+    YZ_TRY(hipMemcpyAsync(S->hmax, S->dmax, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, S->st));
+    return 0;
+}
+
+// a film's Newton decisions after its pass (nk_loop's body after the
+// solve); corrections' max already in hmax[1]. Sets F.phase.
+static int bf_newton(BatchFilm &F, int ch, int lin_it) {
+    yz_step *S = F.S;
+    const yz_case &c = S->cs;
+    const Geo &g = S->g;
+    const int np = g.nxp * g.nyp;
+    S->lin_step += lin_it;
+    const double resid_eq = bits_to_double(S->hmax[1]);
+    F.resid_eq = resid_eq;
+    bool done = false, end = false;
+    if (ch != 0) {
+        end = true;
+    } else {
+        if (F.outer == 0) F.resid_first = resid_eq;
+        if (resid_eq < F.resid_best * 0.9) { F.resid_best = resid_eq; F.stall = 0; }
+        else F.stall++;
+        if (F.stall > 5) {
+            YZ_TRY(hipMemsetAsync(S->dmax + 3, 0, sizeof(unsigned long long), S->st));
+            hipLaunchKernelGGL(k_absmax, dim3(std::min(nb(np), 1024)), dim3(TB), 0, S->st, np, S->h, S->dmax + 3);
+            if (sync_read(S)) return 2;
+            const double floorPhi = 1000.0 * std::numeric_limits<double>::epsilon() * bits_to_double(S->hmax[3]);
+            done = (resid_eq < c.nk_eps) || (F.resid_first > 0 && resid_eq < F.resid_first * 1.0e-4) ||
+                   (resid_eq <= floorPhi);
+            end = true;
+        } else {
+            const double alpha = (resid_eq > F.resid_eq_old) ? 0.5 : 1.0;
+            hipLaunchKernelGGL(k_update, dim3(nb(g.N)), dim3(TB), 0, S->st, g, S->h, S->corr, alpha);
+            hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, S->h);
+            if (F.outer >= c.nk_min - 1) {
+                if (resid_eq < c.nk_eps) { done = true; end = true; }
+                else if (c.nk_eps_rel > 0 && F.resid_first > 0 && resid_eq < F.resid_first * c.nk_eps_rel) {
+                    done = true; end = true;
+                }
+            }
+            F.resid_eq_old = resid_eq;
+        }
+    }
+    if (!end && F.outer + 1 >= c.nk_max) { end = true; F.outer++; }
+    else if (!end) { F.outer++; return F.phase = BF_PASS, 0; }
+    F.ok = done;
+    F.nk_out = F.outer + 1;
+    F.phase = BF_END;
+    return 0;
+}
+
+// a film's Newton loop ended: retry without the guess, fail, or accept
+// (TimeAdvance with a fixed step, NewtonKantorovich.cpp:424-486)
+static int bf_loop_end(BatchFilm &F) {
+    yz_step *S = F.S;
+    const int np = S->g.nxp * S->g.nyp;
+    F.lin_sum += S->lin_step;
+    if (!F.ok && S->extrap_move > 0 && !F.retried) {
+        F.retried = true;
+        S->extrap_blocked = true;
+        F.phase = BF_START;
+        return 0;
+    }
+    S->extrap_blocked = false;
+    if (!F.ok) {
+        F.status = 1;
+    } else {
+        F.status = 0;
+        YZ_TRY(hipMemcpyAsync(S->hnew, S->h, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+        S->time += S->dt_now;
+        S->dt_old = S->dt_now;
+        S->first_step = false;
+    }
+    F.phase = BF_DONE;
+    return 0;
+}
+
+extern "C" {
+
+yz_batch *yz_batch_create(int n, const yz_case *cases, const double *h, const double *s, const double *conc,
+                          const double *dt0) {
+    if (n < 1 || !cases || !h || !s || !conc || !dt0) { set_err("null argument or empty batch"); return nullptr; }
+    for (int i = 0; i < n; ++i) {
+        const yz_case &c = cases[i];
+        if (c.dim != 2 || c.precond != YZ_PC_LEARNED) {
+            set_err("a batch takes 2-D films with the learned preconditioner"); return nullptr;
+        }
+        if (c.step_control) { set_err("a batch advances fixed steps (step_control 0)"); return nullptr; }
+        if (c.learn_passes != 0) { set_err("a batch uses one shared network: learn_passes 0 (yz_batch_set_params)"); return nullptr; }
+        if (c.lin_maxit != cases[0].lin_maxit) { set_err("the films of a batch share lin_maxit"); return nullptr; }
+    }
+    yz_batch *B = new yz_batch();
+    auto fail = [&](const char *what) { if (what) set_err(what); yz_batch_destroy(B); return (yz_batch *)nullptr; };
+    if (hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+    for (auto &e : B->ev) if (hipEventCreate(&e) != hipSuccess) return fail("event");
+    B->f.resize(n);
+    size_t off = 0;
+    int nmax = 0;
+    for (int i = 0; i < n; ++i) {
+        const yz_case &c = cases[i];
+        const size_t np = (size_t)(c.nx + 4) * (c.ny + 4);
+        B->f[i].S = step_new(&c, h + off, s + off, conc + off, dt0[i], B->st);
+        if (!B->f[i].S) return fail(nullptr);
+        off += np;
+        nmax = std::max(nmax, c.nx * c.ny);
+    }
+    if (mlp_init(B->mlp, nmax, B->st)) return fail("network allocation");
+    return B;
+}
+
+int yz_batch_set_params(yz_batch *B, const float *params) {
+    if (!B || !params) { set_err("null argument"); return 2; }
+    std::vector<float> p(params, params + MLP_NP);
+    for (float v : p)
+        if (!std::isfinite(v)) { set_err("non-finite weight"); return 2; }
+    for (int o = 0; o < 64; ++o)
+        for (int i = MLP_IN; i < 16; ++i) p[MLP_W1 + o * 16 + i] = 0.0f;
+    for (int o = MLP_OUT; o < 16; ++o) {
+        for (int i = 0; i < 64; ++i) p[MLP_W3 + o * 64 + i] = 0.0f;
+        p[MLP_B3 + o] = 0.0f;
+    }
+    YZ_TRY(hipMemcpyAsync(B->mlp.params, p.data(), MLP_NP * sizeof(float), hipMemcpyHostToDevice, B->st));
+    YZ_TRY(hipStreamSynchronize(B->st));
+    return 0;
+}
+
+int yz_batch_advance(yz_batch *B, int *status, int *nk_iter, int *lin_iter) {
+    if (!B) { set_err("null batch"); return 2; }
+    const int n = (int)B->f.size();
+    YZ_TRY(hipEventRecord(B->ev[0], B->st));
+    for (BatchFilm &F : B->f) {
+        yz_step *S = F.S;
+        const int np = S->g.nxp * S->g.nyp;
+        for (double &t : S->tms) t = 0;
+        YZ_TRY(hipMemcpyAsync(S->hold2, S->hold, np * sizeof(double), hipMemcpyDeviceToDevice, B->st));
+        YZ_TRY(hipMemcpyAsync(S->hold, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, B->st));
+        F.phase = BF_START; F.retried = false; F.ok = false; F.lin_sum = 0; F.status = 0; F.nk_out = 0;
+    }
+    long long passes = 0;
+    std::vector<SolverT<float> *> sv;
+    std::vector<const float *> bv;
+    std::vector<float *> xv;
+    std::vector<double> tol, resid;
+    std::vector<int> its, rcs, who;
+    for (;;) {
+        // Newton loops that start (or restart) in this round
+        bool any_start = false;
+        for (BatchFilm &F : B->f)
+            if (F.phase == BF_START) { if (bf_loop_start(F)) return 2; any_start = true; }
+        if (any_start) YZ_TRY(hipStreamSynchronize(B->st));
+        for (BatchFilm &F : B->f)
+            if (F.phase == BF_START) { if (bf_loop_begin(F)) return 2; F.phase = BF_PASS; }
+        // the passes of this round
+        who.clear();
+        for (int i = 0; i < n; ++i) if (B->f[i].phase == BF_PASS) who.push_back(i);
+        if (who.empty()) break;
+        for (int i : who) if (bf_assemble(B->f[i])) return 2;
+        YZ_TRY(hipStreamSynchronize(B->st));
+        const int m = (int)who.size();
+        sv.resize(m); bv.resize(m); xv.resize(m); tol.resize(m); resid.resize(m); its.resize(m); rcs.resize(m);
+        for (int u = 0; u < m; ++u) {
+            BatchFilm &F = B->f[who[u]];
+            yz_step *S = F.S;
+            F.eps = ew_eps(S);
+            if (mlp_infer_half(B->mlp, S->g.N, S->feat, S->Yh, B->st)) { set_err("inference failed"); return 2; }
+            S->S32.Mh = S->Yh;
+            S->S32.Mscale = S->mscale;
+            S->S32.stall_window = S->cs.stall_window;
+            sv[u] = &S->S32; bv[u] = S->b32; xv[u] = S->corr32;
+            tol[u] = F.eps > kEpsFP32 ? F.eps : kEpsFP32;
+        }
+        if (bicgstab_stencil_batch(B->table, m, sv.data(), bv.data(), xv.data(), B->f[who[0]].S->cs.lin_maxit,
+                                   tol.data(), its.data(), resid.data(), rcs.data(), B->st)) {
+            set_err("batched solve failed"); return 2;
+        }
+        passes += m;
+        for (int u = 0; u < m; ++u) {
+            BatchFilm &F = B->f[who[u]];
+            yz_step *S = F.S;
+            if (rcs[u] == 0) {
+                hipLaunchKernelGGL(k_f2d, dim3(nb(S->g.N)), dim3(TB), 0, B->st, S->g.N, (const float *)S->corr32,
+                                   S->corr);
+            } else {
+                int dev_err = 0;
+                rcs[u] = fp64_fallback(S, F.eps, &its[u], &resid[u], &dev_err);
+                if (dev_err) return 2;
+                if (rcs[u] != 0) YZ_TRY(hipMemsetAsync(S->corr, 0, S->g.N * sizeof(double), B->st));
+            }
+            YZ_TRY(hipMemsetAsync(S->dmax + 1, 0, sizeof(unsigned long long), B->st));
+            hipLaunchKernelGGL(k_absmax, dim3(std::min(nb(S->g.N), 1024)), dim3(TB), 0, B->st, S->g.N, S->corr,
+                               S->dmax + 1);
+            YZ_TRY(hipMemcpyAsync(S->hmax, S->dmax, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, B->st));
+        }
+        YZ_TRY(hipStreamSynchronize(B->st));
+        for (int u = 0; u < m; ++u) {
+            BatchFilm &F = B->f[who[u]];
+            if (bf_newton(F, rcs[u], its[u])) return 2;
+            if (F.phase == BF_END && bf_loop_end(F)) return 2;
+        }
+    }
+    YZ_TRY(hipEventRecord(B->ev[1], B->st));
+    YZ_TRY(hipEventSynchronize(B->ev[1]));
+    B->ms_last = ev_ms(B->ev[0], B->ev[1]);
+    B->passes_last = passes;
+    long long lin = 0;
+    int worst = 0;
+    for (int i = 0; i < n; ++i) {
+        const BatchFilm &F = B->f[i];
+        if (status) status[i] = F.status;
+        if (nk_iter) nk_iter[i] = F.nk_out;
+        if (lin_iter) lin_iter[i] = (int)F.lin_sum;
+        lin += F.lin_sum;
+        worst = std::max(worst, F.status);
+    }
+    B->lin_last = lin;
+    if (worst) set_err("the Newton iteration did not converge at the fixed step for some film");
+    return worst;
+}
+
+int yz_batch_get_h(yz_batch *B, int film, double *h) {
+    if (!B || !h || film < 0 || film >= (int)B->f.size()) { set_err("bad argument"); return 2; }
+    return yz_step_get_h(B->f[film].S, h);
+}
+
+int yz_batch_stats(yz_batch *B, double *out4) {
+    if (!B || !out4) { set_err("null argument"); return 2; }
+    long long fb = 0;
+    for (const BatchFilm &F : B->f) fb += F.S->fallbacks;
+    out4[0] = B->ms_last;
+    out4[1] = (double)B->passes_last;
+    out4[2] = (double)B->lin_last;
+    out4[3] = (double)fb;
+    return 0;
+}
+
+void yz_batch_destroy(yz_batch *B) {
+    if (!B) return;
+    for (BatchFilm &F : B->f) if (F.S) yz_step_destroy(F.S);
+    mlp_free(B->mlp);
+    fused_table_free(B->table);
+    for (auto &e : B->ev) if (e) (void)hipEventDestroy(e);
+    if (B->st) (void)hipStreamDestroy(B->st);
+    delete B;
 }
 
 }  // extern "C"

@@ -178,3 +178,73 @@ class LinSolver:
         rc = self.lib.yz_linsolve(self.p, len(rowptr) - 1, len(col), _pi(rowptr), _pi(col), _pd(val),
                                   _pd(rhs), _pd(x), C.byref(s), C.byref(it), C.byref(r))
         return rc, x, it.value, r.value
+
+
+class HipBatch:
+    """Independent 2-D films advanced together (the batched sweep): one
+    fixed implicit step per ``advance()`` for every film, their Krylov
+    solves batched into one launch per kernel. ``films`` is a list of
+    ``(case, h, s, c, dt0)``; the films share one network
+    (``set_params``)."""
+
+    def __init__(self, films):
+        self.lib = capi.load()
+        self.n = len(films)
+        arr = (capi.YzCase * self.n)()
+        hs, ss, cs, dts = [], [], [], []
+        self.sizes = []
+        for k, (case, h, s, c, dt0) in enumerate(films):
+            arr[k] = case
+            n = (case.nx + 4) * (case.ny + 4)
+            self.sizes.append(n)
+            for lst, a in ((hs, h), (ss, s), (cs, c)):
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                assert a.size == n
+                lst.append(a)
+            dts.append(dt0)
+        self._cases = arr
+        h = np.concatenate(hs)
+        s = np.concatenate(ss)
+        c = np.concatenate(cs)
+        dt = np.ascontiguousarray(dts, dtype=np.float64)
+        self.p = self.lib.yz_batch_create(self.n, arr, _pd(h), _pd(s), _pd(c), _pd(dt))
+        if not self.p:
+            raise RuntimeError("yz_batch_create: " + capi.last_error())
+
+    def close(self):
+        if getattr(self, "p", None):
+            self.lib.yz_batch_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == capi.LEARN_NPARAM
+        if self.lib.yz_batch_set_params(self.p, _pf(p)):
+            raise RuntimeError("yz_batch_set_params: " + capi.last_error())
+
+    def advance(self):
+        """(rc, status[n], nk[n], lin[n]); rc 0 all accepted, 1 some film
+        failed (status 1), device errors raise"""
+        st = np.zeros(self.n, dtype=np.int32)
+        nk = np.zeros(self.n, dtype=np.int32)
+        lin = np.zeros(self.n, dtype=np.int32)
+        rc = self.lib.yz_batch_advance(self.p, _pi(st), _pi(nk), _pi(lin))
+        if rc == 2:
+            raise RuntimeError("yz_batch_advance: " + capi.last_error())
+        return rc, st, nk, lin
+
+    def h(self, film):
+        out = np.empty(self.sizes[film])
+        if self.lib.yz_batch_get_h(self.p, film, _pd(out)):
+            raise RuntimeError(capi.last_error())
+        return out
+
+    def stats(self):
+        """last step's milliseconds, Newton passes, Krylov iterations;
+        FP64 fallbacks so far"""
+        out = np.zeros(4)
+        self.lib.yz_batch_stats(self.p, _pd(out))
+        return out
