@@ -19,7 +19,9 @@ synchronisations.
     13-point approximate inverse; the network is trained in place on the
     first Newton passes' own matrices (inside the warm-up), then frozen.
   * plain: FP64 BiCGSTAB preconditioned by the structured SA-AMG.
-The line reports both, their ratio and how far apart their films end.
+The line reports both, their ratio and how far apart their films end, and
+(unless --no-sweep) the batched parameter sweep: 256 films advanced
+together, in cell-steps/s, beside the same films advanced one at a time.
 
 Roofline: the dominant kernel of the learned run, the fine-level stencil
 product of the fused Krylov iteration (k_f_av: v = A ph with (rh, v)),
@@ -120,6 +122,66 @@ def copy_bandwidth():
     return round(2 * n / (ms.value / reps / 1e3) / 1e9, 1)
 
 
+def sweep_leg(n_films=256, steps=3, warmup=1, alone=16):
+    """The batched parameter sweep (256 films of 64^2 .. 512^2,
+    yuzen_amd.cases.sweep): the network trained on one 256^2 film of the
+    sweep, then every film advanced together, one fixed step per call;
+    beside it, the first `alone` films advanced one context at a time."""
+    from yuzen_amd import cases
+    from yuzen_amd.step import HipStep, HipBatch
+    c, h, s, cc, dt = cases.sweep_film(0, side=256)
+    g = HipStep(c, h, s, cc, dt)
+    g.learn_sample()
+    g.learn_train(300, 2e-3, 4)
+    params = g.learn_params()
+    g.close()
+    films = cases.sweep(n_films)
+    cells = sum(f[0].nx * f[0].ny for f in films)
+    B = HipBatch(films)
+    B.set_params(params)
+    for _ in range(warmup):
+        if B.advance()[0] != 0:
+            raise RuntimeError("sweep warm-up step failed")
+    el = passes = lins = 0.0
+    for _ in range(steps):
+        t = time.perf_counter()
+        rc = B.advance()[0]
+        el += time.perf_counter() - t
+        if rc != 0:
+            raise RuntimeError("sweep step failed")
+        st = B.stats()
+        passes += st[1]
+        lins += st[2]
+    fallbacks = int(B.stats()[3])
+    B.close()
+    sub = films[:alone]
+    sub_cells = sum(f[0].nx * f[0].ny for f in sub)
+    el1 = 0.0
+    for (c, h, s, cc, dt) in sub:
+        g = HipStep(c, h, s, cc, dt)
+        g.learn_set_params(params)
+        g.advance()
+        t = time.perf_counter()
+        for _ in range(steps):
+            g.advance()
+        el1 += time.perf_counter() - t
+        g.close()
+    return {
+        "workload": f"{n_films} independent 2-D WM films, side from {{64..512}}, theta_e U(5,20) deg, Sp U(0,0.5), "
+                    f"Bo U(0,1), h = 1 + 0.02 U(-1,1), dx 0.01, fixed dt 2e-6, Newton tol 1e-8, FP32 Krylov + "
+                    f"FP64 Newton + learned preconditioner, one batched launch per kernel",
+        "films": n_films, "cells": cells, "steps": steps, "warmup": warmup,
+        "batch_steps_per_s": round(steps / el, 4),
+        "cell_steps_per_s": round(cells * steps / el, 1),
+        "newton_passes_per_film_step": round(passes / steps / n_films, 3),
+        "krylov_iters_per_pass": round(lins / max(passes, 1), 2),
+        "fp64_fallbacks": fallbacks,
+        "alone_cell_steps_per_s": round(sub_cells * steps / el1, 1),
+        "alone_sample": f"the first {alone} films, one step context each, {steps} steps each",
+        "batch_gain_per_cell": round((cells * steps / el) / (sub_cells * steps / el1), 3),
+    }
+
+
 def run(c, h, s, cc, steps, warmup):
     from yuzen_amd.step import HipStep
     g = HipStep(c, h, s, cc, DT)
@@ -158,6 +220,7 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -260,6 +323,11 @@ def main():
         }
         out["speedup_vs_plain_fp64"] = round(v_l / v_p, 3)
         out["h_rel_diff_vs_plain_fp64"] = rel
+    if not args.no_sweep:
+        try:
+            out["batched_sweep"] = sweep_leg()
+        except Exception as e:  # reported, not fatal to the headline line
+            out["batched_sweep"] = {"error": str(e)[:200]}
     if not args.no_cpu:
         pp = (P["passes"] if P is not None else L["passes"]) / args.steps
         out["cpu_baseline"] = cpu_baseline(args.n, pp)
