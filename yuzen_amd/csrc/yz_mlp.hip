@@ -11,7 +11,8 @@
 // recomputes the forward for its tile, forms dY from dL/dM, and accumulates
 // the weight gradients over all its tiles in MFMA accumulators; each wave
 // writes its partial gradients once, and one kernel sums them in a fixed
-// order.
+// order. Each wave works on LDS tiles of its own, so waves synchronise
+// among their own lanes only (the block barrier is for the weights).
 #include "yz_mlp.h"
 
 #include <cmath>
@@ -37,7 +38,7 @@ static constexpr int MBLK = 256;          // 4 waves
 static constexpr int WAVES = MBLK / 64;
 static constexpr int LDT = 68;            // padded row of a 16 x 64 activation tile (floats)
 static constexpr int LDY = 20;            // padded row of the 16 x 16 output tile
-static constexpr int TRAIN_BLOCKS = 256;  // blocks of the backward pass (1024 waves of partials)
+static constexpr int TRAIN_BLOCKS = 512;  // blocks of the backward pass (2048 waves of partials)
 
 // round to nearest even (finite inputs)
 __device__ inline short f2bf(float f) {
@@ -47,6 +48,14 @@ __device__ inline short f2bf(float f) {
 }
 __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+// the wave's own LDS tile written by some lanes and read by others: order
+// the accesses within the wave (each wave has private tiles, so no block
+// barrier is needed)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 struct WLds {
@@ -132,7 +141,7 @@ __device__ inline f32x4 tile_forward(const WLds &W, const float *__restrict__ fe
 #pragma unroll
         for (int j = 0; j < 4; ++j) H1[(4 * (l >> 4) + j) * LDT + 16 * nb + (l & 15)] = fmaxf(acc[j] + bias, 0.0f);
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
         f32x4 acc = {0, 0, 0, 0};
@@ -142,7 +151,7 @@ __device__ inline f32x4 tile_forward(const WLds &W, const float *__restrict__ fe
 #pragma unroll
         for (int j = 0; j < 4; ++j) H2[(4 * (l >> 4) + j) * LDT + 16 * nb + (l & 15)] = fmaxf(acc[j] + bias, 0.0f);
     }
-    __syncthreads();
+    wave_sync();
     f32x4 y = {0, 0, 0, 0};
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) y = mfma(fragA(H2, LDT, kb, l), fragWf(W.w3, 64, 0, kb, l), y);
@@ -166,14 +175,11 @@ __global__ void __launch_bounds__(MBLK) k_mlp_forward(int N, const float *__rest
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float *H1 = tiles[wv][0], *H2 = tiles[wv][1];
     const int ntiles = (N + 15) / 16;
-    // every wave of the block runs the same number of rounds (the barriers
-    // inside tile_forward need the whole block)
-    for (int base = blockIdx.x * WAVES; base < ntiles; base += gridDim.x * WAVES) {
-        const int tile = base + wv;
+    for (int tile = blockIdx.x * WAVES + wv; tile < ntiles; tile += gridDim.x * WAVES) {
         const int c0 = tile * 16;
-        const f32x4 y = tile_forward(W, feat, N, tile < ntiles ? c0 : N, H1, H2, l);
+        const f32x4 y = tile_forward(W, feat, N, c0, H1, H2, l);
         const int o = l & 15;
-        if (tile < ntiles && o < MLP_OUT) {
+        if (o < MLP_OUT) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int cell = c0 + 4 * (l >> 4) + j;
@@ -183,7 +189,7 @@ __global__ void __launch_bounds__(MBLK) k_mlp_forward(int N, const float *__rest
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
@@ -197,7 +203,7 @@ __global__ void __launch_bounds__(MBLK) k_mlp_backward(int N, const float *__res
     __shared__ float tiles[WAVES][3][16 * LDT];
     __shared__ float ytile[WAVES][16 * LDY];
     load_weights(P, W);
-    __syncthreads();
+    __syncthreads();  // (the only block barrier: the weights)
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float *H1 = tiles[wv][0], *H2 = tiles[wv][1], *DZ = tiles[wv][2], *DY = ytile[wv];
     f32x4 gW3[4], gW2[16], gW1[4];
@@ -207,10 +213,8 @@ __global__ void __launch_bounds__(MBLK) k_mlp_backward(int N, const float *__res
     for (int q = 0; q < 16; ++q) gW2[q] = (f32x4){0, 0, 0, 0};
     float gb1 = 0, gb2 = 0, gb3 = 0;
     const int ntiles = (N + 15) / 16;
-    for (int base = blockIdx.x * WAVES; base < ntiles; base += gridDim.x * WAVES) {
-        const int tile = base + wv;
-        const bool live = tile < ntiles;
-        const int c0 = live ? tile * 16 : N;
+    for (int tile = blockIdx.x * WAVES + wv; tile < ntiles; tile += gridDim.x * WAVES) {
+        const int c0 = tile * 16;
         (void)tile_forward(W, feat, N, c0, H1, H2, l);
         // dY[m][o] = G[o][cell] mscale[cell] (0 past the 13 outputs or the grid)
         for (int t = l; t < 16 * 16; t += 64) {
@@ -218,7 +222,7 @@ __global__ void __launch_bounds__(MBLK) k_mlp_backward(int N, const float *__res
             const int cell = c0 + m;
             DY[m * LDY + o] = (o < MLP_OUT && cell < N) ? G[(size_t)o * N + cell] * mscale[cell] : 0.0f;
         }
-        __syncthreads();
+        wave_sync();
         // dW3 += dY^T H2, db3 += sum dY
 #pragma unroll
         for (int ib = 0; ib < 4; ++ib) gW3[ib] = mfma(fragT(DY, LDY, 0, l), fragT(H2, LDT, ib, l), gW3[ib]);
@@ -235,7 +239,7 @@ __global__ void __launch_bounds__(MBLK) k_mlp_backward(int N, const float *__res
                 DZ[m * LDT + i] = H2[m * LDT + i] > 0.0f ? acc[j] : 0.0f;
             }
         }
-        __syncthreads();
+        wave_sync();
         // dW2 += dZ2^T H1, db2 += sum dZ2
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob)
@@ -252,7 +256,7 @@ __global__ void __launch_bounds__(MBLK) k_mlp_backward(int N, const float *__res
             for (int kb = 0; kb < 4; ++kb) acc = mfma(fragA(DZ, LDT, kb, l), fragWb(W.w2, 64, ib, kb, l), acc);
             dz1[ib] = acc;
         }
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int ib = 0; ib < 4; ++ib)
 #pragma unroll
@@ -260,13 +264,13 @@ __global__ void __launch_bounds__(MBLK) k_mlp_backward(int N, const float *__res
                 const int m = 4 * (l >> 4) + j, i = 16 * ib + (l & 15);
                 H2[m * LDT + i] = H1[m * LDT + i] > 0.0f ? dz1[ib][j] : 0.0f;
             }
-        __syncthreads();
+        wave_sync();
         // dW1 += dZ1^T X, db1 += sum dZ1
         const bf16x4 xb = fragXB(feat, N, c0, l);
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob) gW1[ob] = mfma(fragT(H2, LDT, ob, l), xb, gW1[ob]);
         for (int m = 0; m < 16; ++m) gb1 += H2[m * LDT + l];
-        __syncthreads();
+        wave_sync();
     }
     // this wave's partial gradients
     float *out = partial + (size_t)(blockIdx.x * WAVES + wv) * MLP_NP;
@@ -298,16 +302,37 @@ __global__ void __launch_bounds__(MBLK) k_mlp_backward(int N, const float *__res
     if (l < 16) out[MLP_B3 + l] = gb3;
 }
 
-__global__ void k_mlp_reduce(int nwaves, const float *__restrict__ partial, float *__restrict__ grads) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= MLP_NP) return;
-    float s = 0;
-    for (int w = 0; w < nwaves; ++w) s += partial[(size_t)w * MLP_NP + p];
-    // padding entries (W1 columns 8..15, W3 rows 13..15) stay frozen at 0
-    if (p >= MLP_W1 && p < MLP_B1 && (p - MLP_W1) % 16 >= MLP_IN) s = 0;
-    if (p >= MLP_W3 && p < MLP_B3 && (p - MLP_W3) / 64 >= MLP_OUT) s = 0;
-    if (p >= MLP_B3 && p - MLP_B3 >= MLP_OUT) s = 0;
-    grads[p] = s;
+// the per-wave partial gradients summed in a fixed order: block b takes
+// parameters 64 b .. 64 b + 63 (one per lane), its four waves a quarter of
+// the partial sets each (four interleaved sums per lane), and the waves'
+// sums are added in wave order
+__global__ void __launch_bounds__(256) k_mlp_reduce(int nwaves, const float *__restrict__ partial,
+                                                    float *__restrict__ grads) {
+    __shared__ float sh[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (p < MLP_NP) {
+        const int q = (nwaves + 3) / 4;
+        const int w0 = wv * q, w1 = min(nwaves, w0 + q);
+        int w = w0;
+        for (; w + 3 < w1; w += 4) {
+            s0 += partial[(size_t)w * MLP_NP + p];
+            s1 += partial[(size_t)(w + 1) * MLP_NP + p];
+            s2 += partial[(size_t)(w + 2) * MLP_NP + p];
+            s3 += partial[(size_t)(w + 3) * MLP_NP + p];
+        }
+        for (; w < w1; ++w) s0 += partial[(size_t)w * MLP_NP + p];
+    }
+    sh[wv][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (wv != 0 || p >= MLP_NP) return;
+    float t = sh[0][lane] + sh[1][lane] + sh[2][lane] + sh[3][lane];
+    // padding entries (W1 columns 8..15, W3 rows 13..15, b3 13..15) stay 0
+    if (p >= MLP_W1 && p < MLP_B1 && (p - MLP_W1) % 16 >= MLP_IN) t = 0;
+    if (p >= MLP_W3 && p < MLP_B3 && (p - MLP_W3) / 64 >= MLP_OUT) t = 0;
+    if (p >= MLP_B3 && p - MLP_B3 >= MLP_OUT) t = 0;
+    grads[p] = t;
 }
 
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
@@ -526,7 +551,7 @@ int mlp_loss_grad(Mlp &m, const SolverT<float> &S, const float *feat, const floa
     }
     hipLaunchKernelGGL(k_mlp_backward, dim3(TRAIN_BLOCKS), dim3(MBLK), 0, st, N, (const float *)m.params, feat,
                        mscale, (const float *)m.G, m.partial);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + 255) / 256), dim3(256), 0, st, m.nwaves,
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + 63) / 64), dim3(256), 0, st, m.nwaves,
                        (const float *)m.partial, m.grads);
     MLP_CHECK(hipGetLastError());
     if (loss) {
