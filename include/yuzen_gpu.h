@@ -107,7 +107,28 @@ typedef struct yz_case {
      * at 1024^2 the replay measured 3 % slower than the launches on MI355X,
      * so it is for callers that capture or batch. */
     int graphs;
+
+    /* the well-mixed concentration (Physics.ConcField, reference
+     * Input.cpp:440-477): 0 (default) frozen, as Model = "None" runs it;
+     * 1 solved with the film in every Newton pass and the step completed
+     * with the fluxes of the step, as Model = "WM" runs it
+     * (HeightEquationBase.h:1390-1580, ConcEquationBase.h:261-418) */
+    int conc_model;
+    int conc_convection, conc_diffusion;  /* EnableConvection, EnableDiffusion */
+    int conc_scheme;             /* Numerics.Convection.Concentration: YZ_CONV_* */
+    double Df;                   /* diffusion number D0 T0 / L0^2 */
+    double diff_c0, diff_exp;    /* D(c) = exp(diff_exp (c - diff_c0)) */
+    /* Numerics.ConcEquation.LinearSolver */
+    int conc_lin_maxit;
+    double conc_lin_eps;
+    int conc_row_scaling;
 } yz_case;
+
+/* Face interpolation of the concentration's advection (reference
+ * FiniteDifferences.cpp:22-93, keys :214-228); the unlimited schemes.
+ * UTOPIA is the reference's default (Input.cpp:2302-2303). */
+enum { YZ_CONV_UPWIND1 = 0, YZ_CONV_CENTRAL2 = 1, YZ_CONV_CENTRAL4 = 2, YZ_CONV_UPWIND2 = 3,
+       YZ_CONV_UTOPIA = 4, YZ_CONV_QUICK = 5, YZ_CONV_KK = 6 };
 
 /* Fill c with the reference's defaults (Input.cpp getin defaults):
  * theta 0.5, NewtonMaxIterations 30, NewtonMinIterations 1,
@@ -166,6 +187,27 @@ void yz_step_destroy(yz_step *st);
  * Any output pointer may be NULL. */
 int yz_step_assemble(yz_step *st, const double *h_old, const double *h,
                      double dt, double *ia, double *ax, double *rhs);
+
+/* ---- the well-mixed concentration (conc_model 1) ------------------------
+ * The concentration of the state (n doubles, ghosts included): the field
+ * the next step starts from (HeightEquationBase.h:1570-1577). With
+ * conc_model 0 it is the frozen input. */
+int yz_step_get_c(yz_step *st, double *c);
+/* The face volume fluxes of the film h with concentration c (both n
+ * doubles, padded), HeightEquationBase::FaceVolumeFluxes
+ * (HeightEquationBase.h:852-894): F[a*n + i] is the flux through the face
+ * between row i and its neighbour on the high side along axis a, the sum
+ * over the enabled terms of their FaceFlux in the order the reference
+ * registers them; the low face of a first interior cell is stored at its
+ * ghost row; other rows 0. r[n]: the sideways outflow rate of the 1-D
+ * centrifugal term (BodyForceKernel.h:153-159), 0 elsewhere. Either output
+ * may be NULL. */
+int yz_step_face_fluxes(yz_step *st, const double *h, const double *c, double *F, double *r);
+/* counters of the concentration since the context was made: [0] solves,
+ * [1] their Krylov iterations, [2] solves that needed the multigrid after
+ * the Jacobi try, [3] solute moved by RedistributeExcess above 1,
+ * [4] below 0 (reference _excessMoved, _deficitMoved, last step) */
+int yz_step_conc_stats(yz_step *st, double *out5);
 
 /* ---- learned preconditioner (no reference counterpart) -----------------
  * A per-cell network, 8 -> 64 -> 64 -> 13 with ReLU, gives the 13 weights

@@ -44,6 +44,10 @@ def load():
                                  C.c_int, C.c_int, C.c_double, pi_, pd]
     lib.yzo_mobility.argtypes = [C.c_double, C.c_double, C.c_double, pd]
     lib.yzo_limiter.argtypes = [pd, C.c_double, pd]
+    lib.yzo_get_c.argtypes = [C.c_void_p, pd]
+    lib.yzo_face_fluxes.argtypes = [C.c_void_p, pd, pd, pd, pd]
+    lib.yzo_conc_stats.argtypes = [C.c_void_p, pd]
+    lib.yzo_conc_face.argtypes = [C.c_int, pd, C.c_double, pd]
     _lib = lib
     return lib
 
@@ -58,6 +62,14 @@ def limiter(f, sign):
     f = np.ascontiguousarray(f, dtype=np.float64)
     out = np.zeros(10)
     load().yzo_limiter(_pd(f), sign, _pd(out))
+    return out
+
+
+def conc_face(scheme, f, sign):
+    """FaceValue and FaceWeights of an unlimited scheme: [value, w0..w3]"""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(5)
+    load().yzo_conc_face(scheme, _pd(f), sign, _pd(out))
     return out
 
 
@@ -100,6 +112,23 @@ class OracleStep:
     def set_h(self, h):
         h = np.ascontiguousarray(h, dtype=np.float64)
         self.lib.yzo_set_h(self.p, _pd(h))
+
+    def c(self):
+        out = np.empty(self.n)
+        self.lib.yzo_get_c(self.p, _pd(out))
+        return out
+
+    def face_fluxes(self, h, c):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        F, r = np.zeros(2 * self.n), np.zeros(self.n)
+        self.lib.yzo_face_fluxes(self.p, _pd(h), _pd(c), _pd(F), _pd(r))
+        return F.reshape(2, self.n), r
+
+    def conc_stats(self):
+        out = np.zeros(5)
+        self.lib.yzo_conc_stats(self.p, _pd(out))
+        return out
 
     def clock(self):
         t, a, b = C.c_double(), C.c_double(), C.c_double()

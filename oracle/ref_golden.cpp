@@ -13,9 +13,12 @@
 //     of J = A + the term's Jacobian part, row by row in the reference's
 //     storage order, and A(h)h (the conservative form for the capillary and
 //     vertical-gravity terms, AddConservative; the product otherwise);
+//   * the term's FaceFlux through every face of every real cell and its
+//     SideRate (what FaceVolumeFluxes sums for the concentration);
 //   * Mobility::Face/FaceDK and derivatives over a grid of heights and slip
 //     lengths; the Koren limiter's face value, weights and derivative;
-//     NonDimensionalNumbers::Dp over contact angles.
+//     NonDimensionalNumbers::Dp over contact angles; FaceValue and
+//     FaceWeights of the unlimited face schemes.
 // Output: binary files (little-endian) plus a JSON manifest, into the
 // directory given as argv[1].
 #include <cmath>
@@ -213,12 +216,25 @@ void dump_term(TermT &term, FieldT &field, const Shapes &sh, const std::string &
     }
     std::vector<double> axv(n);
     for (int i = 0; i < n; ++i) axv[i] = inside[i] ? ax[i] : 0.0;
+    // FaceFlux of every face of every real cell (the flux along +axis
+    // through face d, PhysicsTermBase.h:160-188) and SideRate
+    const int ndir = twoD ? 4 : 2;
+    std::vector<double> ffv((size_t)n * ndir, 0.0), side(n, 0.0);
+    std::vector<Site> svec(Tr::nStencil);
+    for (Site s = G->template First<Site>(); s != G->template Last<Site>(); ++s) {
+        const int i = Tr::Index(G, s);
+        if (!inside[i]) continue;
+        G->SetSite(svec, s);
+        for (int d = 0; d < ndir; ++d) ffv[(size_t)i * ndir + d] = term.FaceFlux(s, d, field, svec);
+        side[i] = term.SideRate(s, field);
+    }
     std::ofstream f(path, std::ios::binary);
     const int hdr[4] = {n, (int)col.size(), (int)colj.size(), nxp};
     f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
     write_ivec(f, rp); write_ivec(f, col); write_vec(f, av);
     write_ivec(f, rpj); write_ivec(f, colj); write_vec(f, jv);
     write_vec(f, hv); write_vec(f, sv); write_vec(f, cv); write_vec(f, axv);
+    write_vec(f, ffv); write_vec(f, side);
 }
 
 template <class K>
@@ -342,6 +358,40 @@ void dump_limiter(const std::string &dir) {
     write_vec(fo, out);
 }
 
+void dump_schemes(const std::string &dir) {
+    // the unlimited face schemes of the concentration's advection
+    // (FiniteDifferences.cpp:22-93, 152-170): FaceValue and FaceWeights
+    const C_Interpolation_base4::t_Scheme sc[] = {
+        C_Interpolation_base4::Upwind_1st, C_Interpolation_base4::Central_2nd, C_Interpolation_base4::Central_4th,
+        C_Interpolation_base4::Upwind_2nd, C_Interpolation_base4::Upwind_3rd, C_Interpolation_base4::QUICK,
+        C_Interpolation_base4::KK};
+    std::vector<double> out;
+    unsigned long long z = 777;
+    auto rnd = [&]() {
+        z = z * 6364136223846793005ULL + 1442695040888963407ULL;
+        return ((z >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0;
+    };
+    for (int k = 0; k < 7; ++k) {
+        C_Interpolation_base4 in;
+        in.Build(sc[k]);
+        for (int t = 0; t < 20; ++t) {
+            double f[4], w[4];
+            for (int j = 0; j < 4; ++j) f[j] = rnd();
+            const double sign = (t % 3 == 0) ? 0.0 : (t % 3 == 1 ? 0.7 : -1.3);
+            const double v = in.FaceValue(f, sign);
+            in.FaceWeights(f, sign, -1, w);
+            out.push_back(k); out.push_back(sign);
+            for (int j = 0; j < 4; ++j) out.push_back(f[j]);
+            out.push_back(v);
+            for (int j = 0; j < 4; ++j) out.push_back(w[j]);
+        }
+    }
+    std::ofstream fo(dir + "/schemes.bin", std::ios::binary);
+    const int hdr[2] = {(int)out.size() / 11, 11};
+    fo.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    write_vec(fo, out);
+}
+
 void dump_dp(const std::string &dir) {
     // Dp of NonDimensionalNumbers::Build over contact angle, H*, n, m
     std::vector<double> out;
@@ -386,6 +436,7 @@ int main(int argc, char **argv) {
     dump_mobility(dir);
     dump_limiter(dir);
     dump_dp(dir);
+    dump_schemes(dir);
     std::printf("golden vectors written to %s\n", dir.c_str());
     return 0;
 }

@@ -47,10 +47,31 @@
  *                          hierarchy and the iteration counts are not pinned,
  *                          only the solution (to the solve's tolerance).
  *
+ *   face volume fluxes     HeightEquationBase.h:852-894 and each term's
+ *                          FaceFlux/SideRate (LaplacePressureKernel.h:480-
+ *                          491, VerticalGravityKernel.h:271-287,
+ *                          BodyForceKernel.h:118-159,
+ *                          SolutoCapillarityKernel.h:104-138,
+ *                          DisjoiningPressureKernel.h:160-168)
+ *   well-mixed             HeightEquationBase.h:1314-1444 (BeforeNKLoop,
+ *   concentration          EveryNKLoop), :1447-1580 (AfterNKLoop), :985-1058
+ *                          (ConsistentFluxes), :1128-1273
+ *                          (RedistributeExcess, MoveExcess), :1098-1109
+ *                          (ClampConc), :1292-1305 (ConcCorrection);
+ *                          ConcEquationBase.h:261-418 (Solve), :427-518
+ *                          (SoluteFlux, Reconstruct); unlimited face
+ *                          schemes FiniteDifferences.cpp:22-93, 152-170;
+ *                          the Jacobi-then-multigrid solve of a
+ *                          concentration system LinearSolverBackend.cpp:
+ *                          405-470, 587-626 (Jacobi in place of SPAI0 for
+ *                          the try: iteration counts are not pinned)
+ *
  * Only the limited Koren/van Leer height schemes (the default,
  * Numerics.Convection.Height = "Koren") and the Symmetry/Periodic edges are
- * restated; no concentration equation (Physics.ConcField.Model = "None": the
- * concentration field is frozen), no evaporation, no source.
+ * restated; no evaporation, no source. The concentration is frozen
+ * (Physics.ConcField.Model = "None") with conc_model 0 and solved with the
+ * film (Model = "WM", unlimited face schemes, UTOPIA the default) with
+ * conc_model 1.
  *
  * How each part is pinned (tests/test_oracle_golden.py,
  * tests/test_oracle_physics.py):
@@ -806,6 +827,84 @@ struct Equation {
         }
     }
 
+    /* ---- FaceFlux of each term: the flux through face d of cell i
+     * along +axis, as the row's A h takes it (PhysicsTermBase.h:160-188) */
+    double flux_capillary(int i, int d) const {   /* LaplacePressureKernel.h:480-491 */
+        double vh[13], vs[13];
+        for (int k = 0; k < st.ns; ++k) { vh[k] = h[nb(i, k)]; vs[k] = s[nb(i, k)]; }
+        return (1.0 / 3.0) * (cap_D(i, d) * (face_gradient(vh, d) + face_gradient(vs, d)));
+    }
+    double flux_vgrav(int i, int d) const {       /* VerticalGravityKernel.h:271-287 */
+        const int a = nb(i, st.adj[d]);
+        const int lo = (d % 2 == 0) ? a : i, hi = (d % 2 == 0) ? i : a;
+        const double dn = spacing(d / 2);
+        const double hl = h[lo], hh = h[hi];
+        const double mu = visc(0.5 * (c[lo] + c[hi]));
+        double gf = mob.Face(hl, hh);
+        const double tk = (gf == 0) ? 0.0 : gf / (hl + hh) / mu;
+        gf /= mu;
+        return (-1.0 * cs.Bo / 3.0) * ((gf * (hh - hl) + (s[hi] - s[lo]) * (tk * (hl + hh))) / (dn * dn));
+    }
+    double flux_bodyforce(int mode, int i, int j) const {   /* BodyForceKernel.h:118-151 (limited) */
+        double hf, D, wt[4];
+        bf_face(mode, i, j, hf, D, wt, nullptr);
+        double hs = 0;
+        for (int q = 0; q < 4; ++q) hs += wt[q] * h[nb(i, st.face[j][q])];
+        return bf_mag(mode, j / 2) * (bf_weight(mode, i, j) * hs * D);
+    }
+    double flux_marangoni(int i, int d) const {   /* SolutoCapillarityKernel.h:104-138 (limited) */
+        double dc, cc, f[4], wt[4];
+        face_conc(i, d, dc, cc);
+        const double hh = spacing(d / 2);
+        face_heights(i, d, f);
+        const double up = ma_up(dc);
+        const double hf = lim.value(f, up);
+        lim.weights(f, up, wt);
+        const double S = wall_face(i, d) ? 0.0 : dc / visc(cc) * (mob.slip ? mob.Shear(hf) : hf);
+        double hs = 0;
+        for (int q = 0; q < 4; ++q) hs += wt[q] * f[q];
+        return ma_coef() * (S * hs / hh);
+    }
+    double flux_disjoining(int i, int d) const {  /* DisjoiningPressureKernel.h:160-168 */
+        const int a = nb(i, st.adj[d]);
+        const double v = narrowC(d, st.adj[d]) * Pi(h[a]) + narrowC(d, st.center) * Pi(h[i]);
+        const double f = (cs.Dp / 3.0) * (vg_G(i, d) * v);
+        return (d % 2 == 0) ? -f : f;
+    }
+    double face_flux(int i, int d) const {
+        double F = 0;
+        if (cs.capillary) F += flux_capillary(i, d);
+        if (cs.vertical_gravity) F += flux_vgrav(i, d);
+        if (cs.inplane_gravity) F += flux_bodyforce(0, i, d);
+        if (cs.marangoni) F += flux_marangoni(i, d);
+        if (cs.centrifugal) F += flux_bodyforce(1, i, d);
+        if (cs.disjoining) F += flux_disjoining(i, d);
+        return F;
+    }
+    /* SideRate: the 1-D centrifugal term's sideways outflow
+     * (BodyForceKernel.h:153-159) */
+    double side_rate(int i) const {
+        if (!(cs.centrifugal && cs.dim == 1)) return 0;
+        const double hoo = h[i], coo = c[i];
+        if (mob.slip) return bf_mag(1, 0) * (mob.D(hoo) / visc(coo));
+        return bf_mag(1, 0) * (hoo * hoo / visc(coo)) * hoo;
+    }
+    /* FaceVolumeFluxes (HeightEquationBase.h:852-894) of the working field */
+    void face_volume_fluxes(vector<double> F[2], vector<double> &r) const {
+        for (int a = 0; a < st.naxis; ++a) F[a].assign(g.n, 0.0);
+        r.assign(g.n, 0.0);
+        for (int i = 0; i < g.n; ++i) {
+            if (ghost_row(i)) continue;
+            for (int a = 0; a < st.naxis; ++a) {
+                const int il = nb(i, st.adj[2 * a]);
+                const bool below = ghost_row(il);
+                F[a][i] = face_flux(i, 2 * a + 1);
+                if (below) F[a][il] = face_flux(i, 2 * a);
+            }
+            r[i] = side_rate(i);
+        }
+    }
+
     /* ---- SetMatrixA (HeightEquationBase.h:669-751), order of
      * RegisterPhysics: capillary, vertical gravity, in-plane gravity,
      * Marangoni, centrifugal, disjoining ---- */
@@ -1237,11 +1336,26 @@ struct Stepper {
     long long lin_total = 0;
     bool first_step = true;
 
+    /* the well-mixed concentration (conc_model 1) */
+    bool conc_on = false;
+    vector<double> c_field;            /* the field's c (_field->CC) */
+    vector<double> c0, cx;             /* the concentration equation's _phi_old, _phi_new */
+    vector<double> c_iter, c_prev;     /* _cIterate, _concPrev */
+    vector<double> dguess, dvec, rhsD;
+    vector<double> F0[2], F1[2], G2[2], r0, r1;
+    Csr CM;                            /* the concentration system (wide stencil, ghost rows) */
+    double cw1[4] = {0, 0, 0, 0}, cw2[4] = {0, 0, 0, 0};   /* scheme c1, c2 */
+    double coupled = 0;
+    double cstat[5] = {0, 0, 0, 0, 0};
+
     void init(const yz_case &c, const double *h, const double *s, const double *conc, double dt0) {
         eq.build(c);
         const int n = eq.g.n;
         eq.h.assign(h, h + n); eq.s.assign(s, s + n); eq.c.assign(conc, conc + n);
         eq.fill_ghosts(eq.h); eq.fill_ghosts(eq.s); eq.fill_ghosts(eq.c);
+        conc_on = c.conc_model == 1;
+        c_field = eq.c;
+        if (conc_on) conc_init();
         phi_new = eq.h; phi_old = eq.h; phi_old2 = eq.h;
         Ax_old.assign(n, 0); Ax_guess.assign(n, 0); Rhs.assign(n, 0); corr.assign(n, 0);
         dt_now = dt0; dt_old = 0;
@@ -1278,6 +1392,435 @@ struct Stepper {
             x[i] = (Rhs[i] - sum) / M.val[M.ptr[i]];
         }
     }
+    /* ---------------- the well-mixed concentration ---------------- */
+    /* the scheme's coefficients (FiniteDifferences.cpp:22-93): c1 and c2
+     * of LL L R RR, over the sum of c1 */
+    void conc_scheme() {
+        double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+        switch (eq.cs.conc_scheme) {
+        case YZ_CONV_UPWIND1: a[1] = 1; a[2] = 1; b[1] = 1; b[2] = -1; break;
+        case YZ_CONV_CENTRAL2: a[1] = 1; a[2] = 1; break;
+        case YZ_CONV_CENTRAL4: a[0] = -1; a[1] = 9; a[2] = 9; a[3] = -1; break;
+        case YZ_CONV_UPWIND2: a[0] = -1; a[1] = 3; a[2] = 3; a[3] = -1; b[0] = -1; b[1] = 3; b[2] = -3; b[3] = 1; break;
+        case YZ_CONV_QUICK: a[0] = -1; a[1] = 9; a[2] = 9; a[3] = -1; b[0] = -1; b[1] = 3; b[2] = -3; b[3] = 1; break;
+        case YZ_CONV_KK: a[0] = -1; a[1] = 7; a[2] = 7; a[3] = -1; b[0] = -3; b[1] = 9; b[2] = -9; b[3] = 3; break;
+        default: a[0] = -1; a[1] = 7; a[2] = 7; a[3] = -1; b[0] = -1; b[1] = 3; b[2] = -3; b[3] = 1; break;
+        }
+        double den = 0;
+        for (int j = 0; j < 4; ++j) den += a[j];
+        for (int j = 0; j < 4; ++j) { cw1[j] = a[j] / den; cw2[j] = b[j] / den; }
+    }
+    /* FaceWeights / FaceValue of an unlimited scheme (FiniteDifferences.cpp:152-170) */
+    void conc_weights(double sign, double w[4]) const {
+        for (int j = 0; j < 4; ++j) w[j] = cw1[j] + Sign(sign) * cw2[j];
+    }
+    double conc_face(const double f[4], double sign) const {
+        double v = 0;
+        for (int j = 0; j < 4; ++j) v += (cw1[j] + Sign(sign) * cw2[j]) * f[j];
+        return v;
+    }
+    /* KFace (ConcEquationBase.h:146-151) */
+    double kface(double ha, double hb, double ca, double cb, double dn) const {
+        if (!eq.cs.conc_diffusion) return 0;
+        return eq.cs.Df * std::exp(eq.cs.diff_exp * (0.5 * (ca + cb) - eq.cs.diff_c0)) * (0.5 * (ha + hb)) / (dn * dn);
+    }
+    bool wide_slot(int k) const {
+        if (k == eq.st.center) return true;
+        for (int d = 0; d < eq.st.ndir; ++d) if (eq.st.in_face(d, k)) return true;
+        return false;
+    }
+    void conc_init() {
+        conc_scheme();
+        const int n = eq.g.n, ns = eq.st.ns;
+        /* the pattern: ghost rows (1, -1 to the row they copy) and the
+         * wide stencil of every real cell, the diagonal first
+         * (ConcEquationBase.h:242-258) */
+        Csr &m = CM;
+        m.n = n; m.ns = ns;
+        m.ptr.assign(n + 1, 0); m.col.clear(); m.val.clear();
+        m.bc.assign(n, 0); m.slotpos.assign((size_t)n * ns, -1);
+        for (int i = 0; i < n; ++i) {
+            const int ix = i % eq.g.nxp, iy = i / eq.g.nxp;
+            if (!eq.g.inside(ix, iy)) {
+                m.bc[i] = 1;
+                m.col.push_back(i); m.val.push_back(1.0);
+                m.col.push_back(eq.g.src(ix, iy)); m.val.push_back(-1.0);
+            } else {
+                m.slotpos[(size_t)i * ns + eq.st.center] = (int)m.col.size();
+                m.col.push_back(i); m.val.push_back(0.0);
+                for (int k = 0; k < ns; ++k) {
+                    if (k == eq.st.center || !wide_slot(k)) continue;
+                    m.slotpos[(size_t)i * ns + k] = (int)m.col.size();
+                    m.col.push_back(eq.nb(i, k)); m.val.push_back(0.0);
+                }
+            }
+            m.ptr[i + 1] = (int)m.col.size();
+        }
+        c0 = c_field; cx = c_field; c_iter = c_field; c_prev = c_field;
+        dguess.assign(n, 0.0); dvec.assign(n, 0.0); rhsD.assign(n, 0.0);
+    }
+    bool film_not_positive(const vector<double> &hh) const {
+        for (int i = 0; i < eq.g.n; ++i)
+            if (!eq.ghost_row(i) && !(hh[i] > 0.0)) return true;
+        return false;
+    }
+    /* the ghost rows from the rows they copy (SatisfyBCRows on the
+     * concentration's single-row sets, b = rhs) */
+    void conc_bc(vector<double> &x, const vector<double> *rhs) const {
+        for (int i = 0; i < eq.g.n; ++i) {
+            if (!CM.bc[i]) continue;
+            const int p = CM.ptr[i];
+            const double b = rhs ? (*rhs)[i] : 0.0;
+            x[i] = (b - CM.val[p + 1] * x[CM.col[p + 1]]) / CM.val[p];
+        }
+    }
+    /* SoluteFlux (ConcEquationBase.h:427-457) of face d of real cell i */
+    double solute_flux(int i, int d) const {
+        const double th = 0.5;
+        const int a = d / 2;
+        const bool high = (d % 2 == 1);
+        const int ia = eq.nb(i, eq.st.adj[d]);
+        const int lo = high ? i : ia, hi = high ? ia : i;
+        const double f1v = F1[a][lo], f0v = F0[a][lo];
+        double f1[4], f0[4];
+        for (int q = 0; q < 4; ++q) {
+            const int j = eq.nb(i, eq.st.face[d][q]);
+            f1[q] = cx[j]; f0[q] = c0[j];
+        }
+        double G1 = f1v * conc_face(f1, f1v);
+        double G0 = f0v * conc_face(f0, f0v);
+        if (eq.cs.conc_diffusion) {
+            const double dn = eq.spacing(a);
+            const double K1 = kface(eq.h[lo], eq.h[hi], eq.c[lo], eq.c[hi], dn);
+            const double K0 = kface(phi_old[lo], phi_old[hi], c0[lo], c0[hi], dn);
+            G1 -= K1 * (cx[hi] - cx[lo]);
+            G0 -= K0 * (c0[hi] - c0[lo]);
+        }
+        return th * G1 + (1.0 - th) * G0;
+    }
+    /* Reconstruct (ConcEquationBase.h:459-518): c1 from the balance of
+     * h c with the solute fluxes formed once per face from c1 */
+    void reconstruct() {
+        const double th = 0.5, dt = dt_now;
+        const int n = eq.g.n;
+        for (int a = 0; a < eq.st.naxis; ++a) G2[a].assign(n, 0.0);
+        for (int i = 0; i < n; ++i) {
+            if (CM.bc[i]) continue;
+            for (int a = 0; a < eq.st.naxis; ++a) {
+                G2[a][i] = solute_flux(i, 2 * a + 1);
+                const int il = eq.nb(i, eq.st.adj[2 * a]);
+                if (CM.bc[il]) G2[a][il] = solute_flux(i, 2 * a);
+            }
+        }
+        const vector<double> c1 = cx;
+        for (int i = 0; i < n; ++i) {
+            if (CM.bc[i]) continue;
+            const double h1 = phi_guess[i], h0 = phi_old[i];
+            if (!(h1 > 0.0)) continue;
+            double div = 0;
+            for (int a = 0; a < eq.st.naxis; ++a) div += G2[a][i] - G2[a][eq.nb(i, eq.st.adj[2 * a])];
+            const double cc0 = c0[i];
+            double q = h0 * cc0 - dt * div;
+            if (eq.cs.centrifugal && eq.cs.dim == 1) q -= dt * (th * r1[i] * c1[i] + (1.0 - th) * r0[i] * cc0);
+            cx[i] = q / h1;
+        }
+    }
+    /* one solve of the concentration (ConcEquationBase.h:261-418): the
+     * change d = c1 - c0 with the film of the pass (commit false) or of
+     * the step (commit true); the linear solve tries damped Jacobi for at
+     * most 300 iterations and falls back to the multigrid
+     * (LinearSolverBackend.cpp:405-470, 587-626) */
+    int conc_solve(bool commit, int &iter, double &resid) {
+        const double dt = dt_now, th = 0.5;
+        const int n = eq.g.n;
+        for (int i = 0; i < n; ++i) {
+            if (!CM.bc[i]) continue;
+            double v = 0;
+            for (int k = CM.ptr[i]; k < CM.ptr[i + 1]; ++k) v += CM.val[k] * c0[CM.col[k]];
+            rhsD[i] = 0.0 - v;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (CM.bc[i]) continue;
+            double M[13];
+            for (int k = 0; k < eq.st.ns; ++k) M[k] = 0;
+            const double h1 = phi_guess[i], h0 = phi_old[i];
+            const double rr1 = (eq.cs.centrifugal && eq.cs.dim == 1) ? r1[i] : 0.0;
+            double diag = h1 + th * dt * (rr1 - 0.0);
+            double R = dt * 0.0 * c0[i];
+            for (int d = 0; d < eq.st.ndir; ++d) {
+                const int a = d / 2;
+                const bool high = (d % 2 == 1);
+                const double sg = high ? 1.0 : -1.0;
+                const int adj = eq.st.adj[d];
+                const int ia = eq.nb(i, adj);
+                const int lo = high ? i : ia;
+                const double f1v = F1[a][lo], f0v = F0[a][lo];
+                if (eq.cs.conc_convection) {
+                    double f0[4], w1[4], w0[4];
+                    int col[4];
+                    for (int q = 0; q < 4; ++q) {
+                        col[q] = eq.st.face[d][q];
+                        f0[q] = c0[eq.nb(i, col[q])];
+                    }
+                    conc_weights(f1v, w1);
+                    conc_weights(f0v, w0);
+                    double e1 = 0, e0 = 0;
+                    for (int q = 0; q < 4; ++q) {
+                        M[col[q]] += th * dt * sg * f1v * w1[q];
+                        e1 += w1[q] * (f0[q] - c0[i]);
+                        e0 += w0[q] * (f0[q] - c0[i]);
+                    }
+                    R -= dt * sg * (th * f1v * e1 + (1.0 - th) * f0v * e0);
+                } else {
+                    diag += th * dt * sg * f1v;
+                }
+                if (eq.cs.conc_diffusion) {
+                    const double dn = eq.spacing(a);
+                    const double K1 = kface(eq.h[i], eq.h[ia], eq.c[i], eq.c[ia], dn);
+                    const double K0 = kface(h0, phi_old[ia], c0[i], c0[ia], dn);
+                    diag += th * dt * K1;
+                    M[adj] -= th * dt * K1;
+                    const double dc0 = high ? (c0[ia] - c0[i]) : (c0[i] - c0[ia]);
+                    R += dt * sg * (th * K1 * dc0 + (1.0 - th) * K0 * dc0);
+                }
+            }
+            M[eq.st.center] += diag;
+            for (int k = 0; k < eq.st.ns; ++k)
+                if (wide_slot(k)) CM.at(i, k) = M[k];
+            rhsD[i] = R;
+        }
+        dvec = dguess;
+        const vector<double> x0 = dvec;
+        const int maxit = eq.cs.conc_lin_maxit;
+        int it = 0;
+        double r = 0;
+        int ch = linsolve(CM, rhsD, dvec, YZ_PC_JACOBI, eq.cs.conc_row_scaling, std::min(maxit, 300),
+                          eq.cs.conc_lin_eps, it, r);
+        cstat[0] += 1;
+        if (ch != 0) {
+            const int it_try = it;
+            dvec = x0;
+            ch = linsolve(CM, rhsD, dvec, YZ_PC_AMG, eq.cs.conc_row_scaling, maxit, eq.cs.conc_lin_eps, it, r);
+            it += it_try;
+            cstat[2] += 1;
+        }
+        cstat[1] += it;
+        if (std::getenv("YZO_DEBUG"))
+            std::fprintf(stderr, "[oracle] conc solve commit %d: iterations %d resid %.3e rc %d\n", (int)commit, it, r, ch);
+        if (std::getenv("YZO_DEBUG") && ch != 0) {
+            double dmin = 1e300, worst = 1e300, hmin = 1e300, fmax = 0;
+            int ndd = 0;
+            for (int i = 0; i < n; ++i) {
+                if (CM.bc[i]) continue;
+                const double dg = CM.val[CM.ptr[i]];
+                double off = 0;
+                for (int k = CM.ptr[i] + 1; k < CM.ptr[i + 1]; ++k) off += std::fabs(CM.val[k]);
+                dmin = std::min(dmin, dg);
+                worst = std::min(worst, (std::fabs(dg) - off) / std::fabs(dg));
+                if (std::fabs(dg) < off) ++ndd;
+                hmin = std::min(hmin, phi_guess[i]);
+                for (int a = 0; a < eq.st.naxis; ++a) fmax = std::max(fmax, std::fabs(F1[a][i]));
+            }
+            std::fprintf(stderr, "[oracle]   failed system: min diag %.3e, rows not diagonally dominant %d, worst margin %.3e, "
+                         "min h %.3e, max |F1| %.3e\n", dmin, ndd, worst, hmin, fmax);
+        }
+        iter = it; resid = r;
+        if (ch == 0) {
+            conc_bc(dvec, &rhsD);
+            dguess = dvec;
+        } else {
+            bool finite = true;
+            for (int i = 0; i < n && finite; ++i) finite = std::isfinite(dvec[i]);
+            if (!finite) std::fill(dvec.begin(), dvec.end(), 0.0);
+            std::fill(dguess.begin(), dguess.end(), 0.0);
+        }
+        for (int i = 0; i < n; ++i) cx[i] = c0[i] + dvec[i];
+        if (ch == 0 && commit && eq.cs.conc_convection) reconstruct();
+        conc_bc(cx, nullptr);
+        return ch;
+    }
+    /* BeforeNKLoop's concentration part (HeightEquationBase.h:1314-1388) */
+    void conc_before() {
+        eq.c = c_field;
+        c_iter = c_field;
+        c_prev = c_field;
+        eq.h = phi_old;
+        eq.face_volume_fluxes(F0, r0);
+        std::fill(dguess.begin(), dguess.end(), 0.0);
+        coupled = 0;
+    }
+    void clamp_conc() {
+        for (double &v : eq.c) if (v > 1.0) v = 1.0;
+    }
+    /* EveryNKLoop (HeightEquationBase.h:1391-1444) */
+    bool conc_every() {
+        eq.h = phi_guess;
+        if (film_not_positive(phi_guess)) { coupled = 0; return true; }
+        eq.face_volume_fluxes(F1, r1);
+        int it = 0; double r = 0;
+        if (conc_solve(false, it, r) != 0) return false;
+        eq.c = cx;
+        c_iter = cx;
+        clamp_conc();
+        double dmax = 0;
+        for (int i = 0; i < eq.g.n; ++i) {
+            const double d = std::fabs(eq.c[i] - c_prev[i]);
+            if (d > dmax) dmax = d;
+            c_prev[i] = eq.c[i];
+        }
+        coupled = dmax;
+        return true;
+    }
+    /* ConsistentFluxes (HeightEquationBase.h:985-1058) */
+    void consistent_fluxes(double dt) {
+        const int n = eq.g.n;
+        const double th = eq.cs.theta;
+        vector<double> rho(n, 0.0);
+        double sum = 0;
+        int cnt = 0;
+        for (int i = 0; i < n; ++i) {
+            if (CM.bc[i]) continue;
+            double div1 = 0, div0 = 0;
+            for (int a = 0; a < eq.st.naxis; ++a) {
+                const int il = eq.nb(i, eq.st.adj[2 * a]);
+                div1 += F1[a][i] - F1[a][il];
+                div0 += F0[a][i] - F0[a][il];
+            }
+            const double r = (phi_old[i] - phi_guess[i]) / dt
+                           - (th * (div1 + r1[i]) + (1.0 - th) * (div0 + r0[i]) + (0.0 - 0.0));
+            rho[i] = r;
+            sum += r;
+            ++cnt;
+        }
+        if (cnt == 0) return;
+        const double mean = sum / cnt;
+        const int nx = eq.g.nxp, ny = eq.g.nyp;
+        const double w = 1.0 / th;
+        if (eq.cs.dim == 1) {
+            double acc = 0;
+            for (int ix = 2; ix < nx - 3; ix++) { acc += rho[ix] - mean; F1[0][ix] += w * acc; }
+            return;
+        }
+        const int m = nx - 4;
+        vector<double> rowSum(ny, 0.0);
+        for (int jy = 2; jy <= ny - 3; jy++) {
+            double r = 0;
+            for (int ix = 2; ix <= nx - 3; ix++) r += rho[jy * nx + ix] - mean;
+            rowSum[jy] = r;
+        }
+        double accY = 0;
+        for (int jy = 2; jy < ny - 3; jy++) {
+            accY += rowSum[jy] / m;
+            for (int ix = 2; ix <= nx - 3; ix++) F1[1][jy * nx + ix] += w * accY;
+        }
+        for (int jy = 2; jy <= ny - 3; jy++) {
+            const double share = rowSum[jy] / m;
+            double acc = 0;
+            for (int ix = 2; ix < nx - 3; ix++) {
+                acc += (rho[jy * nx + ix] - mean) - share;
+                F1[0][jy * nx + ix] += w * acc;
+            }
+        }
+    }
+    /* MoveExcess (HeightEquationBase.h:1172-1273) on the film phi_guess */
+    void move_excess(vector<double> &c, bool lower) {
+        const vector<double> &hh = phi_guess;
+        const int n = eq.g.n;
+        const double bound = lower ? 0.0 : 1.0;
+        auto outside = [lower](double v) { return lower ? (v < 0.0) : (v > 1.0); };
+        auto beyond = [lower](double h, double v) { return lower ? -h * v : h * (v - 1.0); };
+        auto roomOf = [lower](double h, double v) { return lower ? h * v : h * (1.0 - v); };
+        double &moved = cstat[lower ? 4 : 3];
+        vector<double> room(n, 0.0), gain(n, 0.0);
+        bool any = true;
+        const int ndir = eq.st.ndir;
+        for (int pass = 0; pass < 4 && any; ++pass) {
+            for (int i = 0; i < n; ++i) {
+                const double h = hh[i];
+                const bool inside = lower ? (c[i] > 0.0) : (c[i] < 1.0);
+                room[i] = (!CM.bc[i] && h > 0.0 && inside) ? roomOf(h, c[i]) : 0.0;
+                gain[i] = 0.0;
+            }
+            any = false;
+            for (int i = 0; i < n; ++i) {
+                if (CM.bc[i] || !outside(c[i])) continue;
+                const double h = hh[i];
+                if (!(h > 0.0)) { c[i] = bound; continue; }
+                const double e = beyond(h, c[i]);
+                if (pass == 0) moved += e;
+                double r = 0.0;
+                for (int d = 0; d < ndir; ++d) r += room[eq.nb(i, eq.st.adj[d])];
+                if (!(r > 0.0)) { any = true; continue; }
+                const double f = (e < r) ? e / r : 1.0;
+                for (int d = 0; d < ndir; ++d) {
+                    const int j = eq.nb(i, eq.st.adj[d]);
+                    gain[j] += f * room[j];
+                }
+                if (f < 1.0) c[i] = bound;
+                else { c[i] = lower ? -(e - r) / h : 1.0 + (e - r) / h; any = true; }
+            }
+            for (int i = 0; i < n; ++i) {
+                if (gain[i] > 0.0) {
+                    const double dc = gain[i] / hh[i];
+                    c[i] += lower ? -dc : dc;
+                    if (outside(c[i])) any = true;
+                }
+            }
+        }
+        if (!any) return;
+        double left = 0.0, total = 0.0;
+        for (int i = 0; i < n; ++i) {
+            if (CM.bc[i]) continue;
+            const double h = hh[i];
+            if (outside(c[i])) { left += beyond(h, c[i]); c[i] = bound; }
+            else if (h > 0.0) total += roomOf(h, c[i]);
+        }
+        const double f = (total > 0.0) ? std::min(1.0, left / total) : 0.0;
+        if (f > 0.0)
+            for (int i = 0; i < n; ++i) {
+                if (CM.bc[i] || !(hh[i] > 0.0)) continue;
+                if (lower) { if (!(c[i] > 0.0)) continue; c[i] -= f * c[i]; if (c[i] < 0.0) c[i] = 0.0; }
+                else { if (!(c[i] < 1.0)) continue; c[i] += f * (1.0 - c[i]); if (c[i] > 1.0) c[i] = 1.0; }
+            }
+    }
+    /* RedistributeExcess (HeightEquationBase.h:1128-1170) */
+    void redistribute(vector<double> &c) {
+        cstat[3] = cstat[4] = 0;
+        bool over = false, under = false;
+        for (int i = 0; i < eq.g.n; ++i) {
+            if (CM.bc[i]) continue;
+            if (c[i] > 1.0) over = true;
+            if (c[i] < 0.0) under = true;
+        }
+        if (over) move_excess(c, false);
+        if (under) move_excess(c, true);
+        if (over || under) conc_bc(c, nullptr);
+        for (double &v : c) {
+            if (v > 1.0) v = 1.0;
+            if (under && v < 0.0) v = 0.0;
+        }
+    }
+    /* AfterNKLoop's concentration part (HeightEquationBase.h:1447-1577) */
+    void conc_after() {
+        eq.h = phi_guess;
+        eq.face_volume_fluxes(F1, r1);
+        consistent_fluxes(dt_now);
+        const bool noFilm = film_not_positive(phi_guess);
+        const vector<double> lastPass = cx;
+        bool solved = true;
+        if (!noFilm) {
+            int it = 0; double r = 0;
+            if (conc_solve(true, it, r) != 0) solved = false;
+        }
+        if (!solved) cx = lastPass;
+        redistribute(cx);
+        c0 = cx;
+        eq.c = cx;
+        c_iter = cx;
+        clamp_conc();
+        c_field = eq.c;
+    }
+
     /* Solve (NewtonKantorovich.cpp:74-154) */
     int solve(int &iter, double &resid) {
         eq.h = phi_guess;
@@ -1330,6 +1873,7 @@ struct Stepper {
             }
         }
         /* BeforeNKLoop: the working field from the state of the step */
+        if (conc_on) conc_before(); else eq.c = c_field;
         eq.h = phi_old;
         eq.set_matrix_A();
         eq.apply_A(phi_old, Ax_old);
@@ -1341,10 +1885,12 @@ struct Stepper {
         int outer;
         resid_eq = 0;
         for (outer = 0; outer < eq.cs.nk_max; ++outer) {
+            if (conc_on && !conc_every()) { done = false; break; }
             int it = 0; double lr = 0;
             const int ch = solve(it, lr);
             lin_step += it;
             resid_eq = absmax(corr);
+            if (conc_on && coupled > resid_eq) resid_eq = coupled;
             if (std::getenv("YZO_DEBUG"))
                 std::fprintf(stderr, "[oracle] pass %d lin %d resid_lin %.3e corr %.3e rc %d\n",
                              outer, it, lr, resid_eq, ch);
@@ -1425,6 +1971,7 @@ struct Stepper {
             }
         } while (!step_done);
         phi_new = phi_guess;
+        if (conc_on) conc_after();
         eq.h = phi_new;
         time += dt_now;
         dt_old = dt_now;
@@ -1462,6 +2009,11 @@ void yzo_case_defaults(yz_case *c) {
     c->learn_passes = 2; c->learn_iters = 100; c->learn_probes = 4; c->learn_lr = 2e-3;
     c->stall_window = 40;
     c->graphs = 0;
+    c->conc_model = 0;
+    c->conc_convection = 1; c->conc_diffusion = 1;
+    c->conc_scheme = YZ_CONV_UTOPIA;
+    c->Df = 1.0; c->diff_c0 = 0.5; c->diff_exp = 0.0;
+    c->conc_lin_maxit = 5000; c->conc_lin_eps = 1e-13; c->conc_row_scaling = 1;
 }
 
 void *yzo_create(const yz_case *c, const double *h, const double *s, const double *conc, double dt0) {
@@ -1488,6 +2040,37 @@ void yzo_set_h(void *p, const double *h) {
     auto *st = static_cast<yzo::Stepper *>(p);
     st->phi_new.assign(h, h + st->eq.g.n);
     st->eq.fill_ghosts(st->phi_new);
+}
+void yzo_get_c(void *p, double *c) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    std::copy(st->c_field.begin(), st->c_field.end(), c);
+}
+/* FaceVolumeFluxes of film h with concentration c: F[2 n] (axis-major), r[n] */
+void yzo_face_fluxes(void *p, const double *h, const double *c, double *F, double *r) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    const int n = st->eq.g.n;
+    const std::vector<double> keep_h = st->eq.h, keep_c = st->eq.c;
+    st->eq.h.assign(h, h + n); st->eq.c.assign(c, c + n);
+    st->eq.fill_ghosts(st->eq.h); st->eq.fill_ghosts(st->eq.c);
+    std::vector<double> FF[2], rr;
+    st->eq.face_volume_fluxes(FF, rr);
+    for (int a = 0; a < st->eq.st.naxis; ++a) std::copy(FF[a].begin(), FF[a].end(), F + (size_t)a * n);
+    if (r) std::copy(rr.begin(), rr.end(), r);
+    st->eq.h = keep_h; st->eq.c = keep_c;
+}
+/* an unlimited face scheme (conc_scheme): out = FaceValue, FaceWeights[4] */
+void yzo_conc_face(int scheme, const double *f, double sign, double *out) {
+    yzo::Stepper st;
+    st.eq.cs.conc_scheme = scheme;
+    st.conc_scheme();
+    out[0] = st.conc_face(f, sign);
+    st.conc_weights(sign, out + 1);
+}
+/* [0] concentration solves, [1] their iterations, [2] solves that needed
+ * the multigrid, [3] solute moved above 1, [4] below 0 (last step) */
+void yzo_conc_stats(void *p, double *out5) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    for (int k = 0; k < 5; ++k) out5[k] = st->cstat[k];
 }
 void yzo_clock(void *p, double *t, double *dt_now, double *dt_old) {
     auto *st = static_cast<yzo::Stepper *>(p);

@@ -6,8 +6,10 @@ NonDimensionalNumbers, driven the way the reference's test_jacobian.cpp
 drives them) into a scratch directory and packs what it wrote:
 
   ref_terms.npz   per case: CRS of A and J (rowptr/col/val), the fields
-                  h, s, c on every row, and A(h)h
-  ref_scalars.npz mobility table, limiter table, Dp table
+                  h, s, c on every row, A(h)h, the FaceFlux of every face
+                  of every real cell and the SideRate
+  ref_scalars.npz mobility table, limiter table, Dp table, the unlimited
+                  face schemes' FaceValue and FaceWeights
   manifest.json   the dimensionless parameters of every case
 
     python tests/golden/make_golden.py
@@ -35,7 +37,10 @@ def read_case(path):
         colj = np.fromfile(f, dtype=np.int32, count=nnzj)
         jv = np.fromfile(f, dtype=np.float64, count=nnzj)
         h, s, c, ax = (np.fromfile(f, dtype=np.float64, count=n) for _ in range(4))
-    return dict(rp=rp, col=col, av=av, rpj=rpj, colj=colj, jv=jv, h=h, s=s, c=c, ax=ax)
+        ndir = 2 if nxp == n else 4
+        ff = np.fromfile(f, dtype=np.float64, count=n * ndir).reshape(n, ndir)
+        side = np.fromfile(f, dtype=np.float64, count=n)
+    return dict(rp=rp, col=col, av=av, rpj=rpj, colj=colj, jv=jv, h=h, s=s, c=c, ax=ax, ff=ff, side=side)
 
 
 def read_table(path):
@@ -59,7 +64,8 @@ def main():
         np.savez_compressed(os.path.join(HERE, "ref_scalars.npz"),
                             mobility=read_table(os.path.join(tmp, "mobility.bin")),
                             limiter=read_table(os.path.join(tmp, "limiter.bin")),
-                            dp=read_table(os.path.join(tmp, "dp.bin")))
+                            dp=read_table(os.path.join(tmp, "dp.bin")),
+                            schemes=read_table(os.path.join(tmp, "schemes.bin")))
         with open(os.path.join(HERE, "manifest.json"), "w") as f:
             json.dump(man, f, indent=1)
     print("wrote", os.path.join(HERE, "ref_terms.npz"))

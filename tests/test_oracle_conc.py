@@ -1,0 +1,164 @@
+"""The oracle's well-mixed concentration (conc_model 1) against what the
+mathematics fixes.
+
+* Conservation: the step is one finite-volume step of h and of h c
+  (reference ConcStep.h:21-48, HeightEquationBase.h:1447-1577): in a closed
+  box the solute sum of h c over the cells changes by rounding only, and
+  a uniform concentration stays uniform. The second holds only when the
+  fluxes the solute is carried with are the ones the film's step took
+  (ConsistentFluxes), so it pins that too.
+* Closed forms on a film at rest or in uniform flow, against the textbook
+  discretisations built here independently:
+  - a flat film with no forces: the step is the Crank-Nicolson step of
+    c_t = Df lap c with the 5-point Laplacian and zero-flux walls;
+  - a flat film carried by in-plane gravity over a periodic line: the
+    Crank-Nicolson step of c_t + u c_x = 0 with the third-order upwind
+    face value (-c_{i-1} + 5 c_i + 2 c_{i+1}) / 6 (UTOPIA).
+* RedistributeExcess: a sharp front near c = 1 carried by the flow makes
+  UTOPIA overshoot; the step keeps 0 <= c <= 1 and the solute sum.
+"""
+import numpy as np
+import pytest
+
+from yuzen_amd import capi, cases
+from oracle import binding as ob
+
+
+def _interior(c, v):
+    nyp, nxp = cases.padded_shape(c)
+    a = np.asarray(v).reshape(nyp, nxp)
+    return a[2:-2, 2:-2] if c.dim == 2 else a[0, 2:-2]
+
+
+def _allphys(n=24, periodic=False):
+    c, h, s, cc, dt, _ = cases.allphys_2d(n=n, L=1.5, dt=2e-5)
+    c.conc_model, c.Df, c.diff_exp = 1, 0.05, 0.7
+    if periodic:
+        # a film that is periodic itself: no front across the seam
+        for k in range(4):
+            c.bc[k] = capi.BC_PERIODIC
+        c.centrifugal = 0
+        X, Y = cases.cell_centres(c)
+        L = c.xmax - c.xmin
+        h = (0.6 + 0.25 * np.sin(2 * np.pi * X / L) * np.cos(2 * np.pi * Y / L)).ravel()
+    return c, h, s, cc, dt
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_solute_conserved(periodic):
+    c, h, s, cc, dt = _allphys(periodic=periodic)
+    o = ob.OracleStep(c, h, s, cc, dt)
+    sol0 = (_interior(c, o.h()) * _interior(c, o.c())).sum()
+    moved = 0.0
+    for _ in range(4):
+        rc, nk, lin, _r = o.advance()
+        assert rc == 0
+        sol = (_interior(c, o.h()) * _interior(c, o.c())).sum()
+        assert abs(sol - sol0) <= 2e-14 * sol0
+        moved = max(moved, np.abs(o.c() - cc).max())
+    assert moved > 1e-4          # the concentration did move
+
+
+def test_uniform_concentration_stays_uniform():
+    c, h, s, cc, dt = _allphys()
+    cu = np.full_like(cc, 0.37)
+    o = ob.OracleStep(c, h, s, cu, dt)
+    for _ in range(3):
+        assert o.advance()[0] == 0
+        ci = _interior(c, o.c())
+        assert np.abs(ci - 0.37).max() <= 1e-15
+
+
+def _flat_case(dim, n, periodic, Df):
+    c = capi.case_defaults()
+    c.dim, c.nx, c.ny = dim, n, (n - 2 if dim == 2 else 1)
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, 1.0, 0.0, 0.8
+    for k in range(4):
+        c.bc[k] = capi.BC_PERIODIC if periodic else capi.BC_SYMMETRY
+    c.capillary = 1
+    c.conc_model, c.Df, c.diff_exp = 1, Df, 0.0
+    c.nk_eps = 1e-12
+    return c
+
+
+def test_diffusion_on_a_film_at_rest_is_crank_nicolson():
+    c = _flat_case(2, 12, False, 0.3)
+    X, Y = cases.cell_centres(c)
+    h = np.ones_like(X)
+    conc = 0.5 + 0.2 * np.cos(np.pi * X) * np.cos(2.5 * Y) + 0.05 * np.sin(7 * X * Y)
+    dt = 4e-3
+    o = ob.OracleStep(c, h.ravel(), np.zeros(h.size), conc.ravel(), dt)
+    assert o.advance()[0] == 0
+    got = _interior(c, o.c())
+    # textbook: c1 - c0 = dt Df (theta L c1 + (1 - theta) L c0), 5-point L,
+    # zero-flux walls (mirror ghosts), spacings of the padded grid
+    nx, ny = c.nx, c.ny
+    dx = (c.xmax - c.xmin) / nx
+    dy = (c.ymax - c.ymin) / ny
+    N = nx * ny
+    L = np.zeros((N, N))
+    for iy in range(ny):
+        for ix in range(nx):
+            i = iy * nx + ix
+            for jx, jy, d2 in ((ix - 1, iy, dx * dx), (ix + 1, iy, dx * dx), (ix, iy - 1, dy * dy), (ix, iy + 1, dy * dy)):
+                if 0 <= jx < nx and 0 <= jy < ny:
+                    L[i, jy * nx + jx] += 1.0 / d2
+                    L[i, i] -= 1.0 / d2
+    c0 = _interior(c, conc).ravel()
+    I = np.eye(N)
+    want = np.linalg.solve(I - 0.5 * dt * c.Df * L, (I + 0.5 * dt * c.Df * L) @ c0)
+    assert np.abs(got.ravel() - want).max() <= 1e-12
+    assert np.abs(got.ravel() - c0).max() > 1e-3
+
+
+def test_advection_in_uniform_flow_is_utopia_crank_nicolson():
+    c = _flat_case(1, 40, True, 0.0)
+    c.conc_diffusion = 0
+    c.inplane_gravity, c.Pgx = 1, 0.6
+    X, _ = cases.cell_centres(c)
+    h = np.ones_like(X)
+    conc = 0.5 + 0.3 * np.sin(2 * np.pi * X) + 0.1 * np.cos(6 * np.pi * X)
+    dt = 2e-3
+    o = ob.OracleStep(c, h.ravel(), np.zeros(h.size), conc.ravel(), dt)
+    assert o.advance()[0] == 0
+    got = _interior(c, o.c())
+    assert np.abs(_interior(c, o.h()) - 1.0).max() == 0.0
+    # the face flux of the flat film: Pgx/3 h_f^2/mu / dx (the body force
+    # carries 1/spacing in its face value), mu = 1 at c... taken from the
+    # oracle's own face fluxes only for its magnitude
+    F, _ = o.face_fluxes(h.ravel(), conc.ravel())
+    u = F[0][2]
+    assert u > 0 and np.allclose(F[0][2:-3], u, rtol=0, atol=1e-12 * u)
+    n = c.nx
+    D = np.zeros((n, n))
+    for i in range(n):
+        for j, w in ((i - 1, -1.0 / 6), (i, 5.0 / 6), (i + 1, 2.0 / 6)):
+            D[i, j % n] += w          # face i+1/2
+            D[i, (j - 1) % n] -= w    # face i-1/2
+    c0 = _interior(c, conc)
+    I = np.eye(n)
+    want = np.linalg.solve(I + 0.5 * dt * u * D, (I - 0.5 * dt * u * D) @ c0)
+    assert np.abs(got - want).max() <= 1e-12
+    assert np.abs(got - c0).max() > 1e-3
+
+
+def test_excess_above_one_is_moved_not_cut():
+    c = _flat_case(1, 48, True, 0.0)
+    c.conc_diffusion = 0
+    c.inplane_gravity, c.Pgx = 1, 2.0
+    X, _ = cases.cell_centres(c)
+    h = np.ones_like(X) + 0.1 * np.sin(2 * np.pi * X)
+    conc = np.where((X > 0.3) & (X < 0.6), 0.999, 0.2)
+    dt = 4e-3
+    o = ob.OracleStep(c, h.ravel(), np.zeros(h.size), conc.ravel(), dt)
+    sol0 = (_interior(c, o.h()) * _interior(c, o.c())).sum()
+    moved = 0.0
+    for _ in range(6):
+        assert o.advance()[0] == 0
+        st = o.conc_stats()
+        moved += st[3]
+        ci = _interior(c, o.c())
+        assert ci.max() <= 1.0 and ci.min() >= 0.0
+        sol = (_interior(c, o.h()) * ci).sum()
+        assert abs(sol - sol0) <= 1e-13 * sol0
+    assert moved > 0.0

@@ -47,6 +47,39 @@ def as_dict(rp, col, val, rows):
 
 
 @pytest.mark.parametrize("m", MAN["cases"], ids=[m["name"] for m in MAN["cases"]])
+def test_face_fluxes_match_reference(m):
+    """FaceVolumeFluxes of one term against that term's own FaceFlux and
+    SideRate: the high face of every real cell, the low face of a first
+    cell (stored at its ghost row), and every other low face against the
+    same face taken from the cell below it (the reference forms a face the
+    same way whichever cell asks)."""
+    g = {k: TERMS[f"{m['name']}__{k}"] for k in ("h", "s", "c", "ff", "side")}
+    c = case_from(m)
+    o = ob.OracleStep(c, g["h"], g["s"], g["c"], 1e-3)
+    _, _, bc = o.pattern()
+    F, r = o.face_fluxes(g["h"], g["c"])
+    ff = g["ff"]
+    scale = max(np.abs(ff).max(), 1e-300)
+    nxp = c.nx + 4
+    checked = 0
+    for i in np.nonzero(~bc)[0]:
+        for a in range(c.dim):
+            lo = i - (1 if a == 0 else nxp)
+            assert abs(F[a][i] - ff[i, 2 * a + 1]) <= 1e-12 * scale, (m["name"], i, a)
+            assert abs(F[a][lo] - ff[i, 2 * a]) <= 1e-12 * scale, (m["name"], i, a, "low")
+            checked += 2
+    assert checked > 0
+    assert np.abs(r - g["side"]).max() <= 1e-12 * max(np.abs(g["side"]).max(), 1e-300)
+
+
+def test_conc_schemes_match_reference():
+    for row in SCAL["schemes"]:
+        k, sign, f, v, w = int(row[0]), row[1], row[2:6], row[6], row[7:11]
+        mine = ob.conc_face(k, f, sign)
+        assert mine[0] == v and np.array_equal(mine[1:], w), (k, sign, mine, row)
+
+
+@pytest.mark.parametrize("m", MAN["cases"], ids=[m["name"] for m in MAN["cases"]])
 def test_terms_match_reference(m):
     g = {k: TERMS[f"{m['name']}__{k}"] for k in ("rp", "col", "av", "rpj", "colj", "jv", "h", "s", "c", "ax")}
     c = case_from(m)

@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(_HERE, "libyuzen_hip.so")
 
 BC_SYMMETRY, BC_PERIODIC, BC_PINNING, BC_FRESHINLET, BC_OUTLET = range(5)
 PC_JACOBI, PC_AMG, PC_LEARNED, PC_LEARNED_AMG = range(4)
+(CONV_UPWIND1, CONV_CENTRAL2, CONV_CENTRAL4, CONV_UPWIND2, CONV_UTOPIA, CONV_QUICK,
+ CONV_KK) = range(7)
 LEARN_NPARAM, LEARN_NFEAT, LEARN_NOUT = 6288, 8, 13
 
 
@@ -45,6 +47,12 @@ class YzCase(C.Structure):
         ("learn_passes", C.c_int), ("learn_iters", C.c_int),
         ("learn_probes", C.c_int), ("learn_lr", C.c_double),
         ("stall_window", C.c_int), ("graphs", C.c_int),
+        ("conc_model", C.c_int),
+        ("conc_convection", C.c_int), ("conc_diffusion", C.c_int),
+        ("conc_scheme", C.c_int),
+        ("Df", C.c_double), ("diff_c0", C.c_double), ("diff_exp", C.c_double),
+        ("conc_lin_maxit", C.c_int), ("conc_lin_eps", C.c_double),
+        ("conc_row_scaling", C.c_int),
     ]
 
     def copy(self):
@@ -82,6 +90,9 @@ def case_defaults():
     c.learn_passes, c.learn_iters, c.learn_probes, c.learn_lr = 2, 100, 4, 2e-3
     c.stall_window = 40
     c.graphs = 0
+    c.conc_model, c.conc_convection, c.conc_diffusion, c.conc_scheme = 0, 1, 1, CONV_UTOPIA
+    c.Df, c.diff_c0, c.diff_exp = 1.0, 0.5, 0.0
+    c.conc_lin_maxit, c.conc_lin_eps, c.conc_row_scaling = 5000, 1e-13, 1
     return c
 
 
@@ -114,6 +125,9 @@ def load():
     lib.yz_step_destroy.argtypes = [C.c_void_p]
     lib.yz_step_kernel_time.argtypes = [C.c_void_p, i, P(C.c_longlong), pd]
     lib.yz_step_assemble.argtypes = [C.c_void_p, pd, pd, d, pd, pd, pd]
+    lib.yz_step_get_c.argtypes = [C.c_void_p, pd]
+    lib.yz_step_face_fluxes.argtypes = [C.c_void_p, pd, pd, pd, pd]
+    lib.yz_step_conc_stats.argtypes = [C.c_void_p, pd]
     lib.yz_linsolve_create.restype = C.c_void_p
     lib.yz_linsolve_destroy.argtypes = [C.c_void_p]
     lib.yz_linsolve.argtypes = [C.c_void_p, i, i, pi_, pi_, pd, pd, pd,

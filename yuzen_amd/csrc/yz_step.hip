@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/yuzen_gpu.h"
+#include "yz_conc.h"
 #include "yz_rows.h"
 #include "yz_solver.h"
 #include "yz_mlp.h"
@@ -180,6 +181,24 @@ struct yz_step {
     long long lin_step = 0;
     hipEvent_t ev[12];
     double tms[6];
+    // the well-mixed concentration (conc_model 1): the field's c, the
+    // concentration equation's c0 (start of step) and cx (its solve), the
+    // last pass's c, the face fluxes at the start (Fc0, rc0) and at the film
+    // of the pass (Fc1, rc1), the solute fluxes, the change d (also the next
+    // solve's guess) and its own FP64 solver
+    bool conc = false;
+    double *cfield = nullptr, *c0 = nullptr, *cx = nullptr, *cprev = nullptr, *clast = nullptr;
+    double *Fc0[2] = {nullptr, nullptr}, *Fc1[2] = {nullptr, nullptr}, *G2[2] = {nullptr, nullptr};
+    double *rc0 = nullptr, *rc1 = nullptr;
+    double *ex[4] = {nullptr, nullptr, nullptr, nullptr};   // room, gain, f, e
+    double *dconc = nullptr, *dsave = nullptr, *bconc = nullptr, *cscratch = nullptr, *csums = nullptr;
+    int *cflags = nullptr, *hcflags = nullptr;
+    unsigned long long *cdmax = nullptr, *hcdmax = nullptr;
+    double *hcsums = nullptr;
+    Solver Sc;
+    ConcArgs ca;
+    double coupled = 0;
+    double cstat[5] = {0, 0, 0, 0, 0};
 };
 
 static int check_case(const yz_case &c) {
@@ -200,6 +219,13 @@ static int check_case(const yz_case &c) {
         set_err("the learned preconditioner is built for the 2-D film"); return 1;
     }
     if (c.stall_window < 0) { set_err("stall_window must be >= 0"); return 1; }
+    if (c.conc_model != 0 && c.conc_model != 1) { set_err("conc_model must be 0 (frozen) or 1 (well-mixed)"); return 1; }
+    if (c.conc_model == 1) {
+        if (c.conc_scheme < YZ_CONV_UPWIND1 || c.conc_scheme > YZ_CONV_KK) {
+            set_err("conc_scheme: only the unlimited face schemes are implemented for the concentration"); return 1;
+        }
+        if (c.conc_lin_maxit < 1 || !(c.conc_lin_eps > 0)) { set_err("conc_lin_maxit >= 1 and conc_lin_eps > 0"); return 1; }
+    }
     if (c.learn_passes > 0 && (c.learn_iters < 0 || c.learn_probes < 1 || !(c.learn_lr >= 0))) {
         set_err("learn_iters >= 0, learn_probes >= 1 and learn_lr >= 0 are needed to train"); return 1;
     }
@@ -427,6 +453,166 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     return ch;
 }
 
+// ---- the well-mixed concentration ---------------------------------------
+
+// FaceVolumeFluxes of film h with the working concentration
+static void conc_fluxes(yz_step *S, const double *h, double *const *F, double *r) {
+    RowArgs a = S->ra;
+    a.h = h;
+    a.c = S->c;
+    launch_face_fluxes(a, F[0], F[1], r, S->st);
+}
+
+// whether a real cell of the film S->h holds no liquid (FilmNotPositive)
+static int film_not_positive(yz_step *S, bool *no_film) {
+    YZ_TRY(hipMemsetAsync(S->cflags, 0, 4 * sizeof(int), S->st));
+    launch_film_check(S->g, S->h, S->cflags, S->st);
+    YZ_TRY(hipMemcpyAsync(S->hcflags, S->cflags, 4 * sizeof(int), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    *no_film = S->hcflags[0] != 0;
+    return 0;
+}
+
+// one solve of the concentration (ConcEquationBase.h:261-418): the system
+// at the film of the pass (or of the step, commit), the Jacobi try of at
+// most 300 iterations and the multigrid after it
+// (LinearSolverBackend.cpp:405-470, 587-626), cx = c0 + d and, at the
+// step's completion, c from the balance of h c. *ch: the solve's code.
+static int conc_solve(yz_step *S, bool commit, int *ch) {
+    const yz_case &c = S->cs;
+    const Geo &g = S->g;
+    ConcArgs a = S->ca;
+    a.dt = S->dt_now;
+    launch_conc_rows(a, S->st);
+    YZ_TRY(hipGetLastError());
+    YZ_TRY(hipMemcpyAsync(S->dsave, S->dconc, g.N * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    if (jacobi_setup(S->Sc)) { set_err("concentration preconditioner set-up failed"); return 2; }
+    int it = 0, it2 = 0;
+    double r = 0;
+    int rc = bicgstab(S->Sc, (const double *)S->bconc, S->dconc, YZ_PC_JACOBI, std::min(c.conc_lin_maxit, 300),
+                      c.conc_lin_eps, &it, &r);
+    S->cstat[0] += 1;
+    if (rc != 0) {
+        YZ_TRY(hipMemcpyAsync(S->dconc, S->dsave, g.N * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+        if (amg_setup(S->Sc)) { set_err("concentration preconditioner set-up failed"); return 2; }
+        rc = bicgstab(S->Sc, (const double *)S->bconc, S->dconc, YZ_PC_AMG, c.conc_lin_maxit, c.conc_lin_eps, &it2, &r);
+        it += it2;
+        S->cstat[2] += 1;
+    }
+    S->cstat[1] += it;
+    if (std::getenv("YZ_DEBUG"))
+        std::fprintf(stderr, "[hip] conc solve commit %d: iterations %d (multigrid %d) resid %.3e rc %d\n", (int)commit,
+                     it, it2, r, rc);
+    // a failed solve starts the next one from zero, and what it left is
+    // not used (the pass fails, or the step keeps its last pass)
+    if (rc != 0) YZ_TRY(hipMemsetAsync(S->dconc, 0, g.N * sizeof(double), S->st));
+    launch_conc_update(a, S->dconc, S->st);
+    if (rc == 0 && commit && c.conc_convection) {
+        launch_reconstruct(a, S->st);
+        launch_ghosts(g, S->cx, S->st);
+    }
+    YZ_TRY(hipGetLastError());
+    *ch = rc;
+    return 0;
+}
+
+// BeforeNKLoop's concentration part (HeightEquationBase.h:1314-1388)
+static int conc_before(yz_step *S) {
+    const Geo &g = S->g;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    YZ_TRY(hipMemcpyAsync(S->c, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    YZ_TRY(hipMemcpyAsync(S->cprev, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    conc_fluxes(S, S->hold, S->Fc0, S->rc0);
+    YZ_TRY(hipMemsetAsync(S->dconc, 0, g.N * sizeof(double), S->st));
+    YZ_TRY(hipGetLastError());
+    S->coupled = 0;
+    return 0;
+}
+
+// EveryNKLoop (HeightEquationBase.h:1391-1444): the concentration with the
+// fluxes of this pass's film, the working c from it (capped at 1), and how
+// far it moved. *ok false: the concentration solve failed (the pass fails).
+static int conc_every(yz_step *S, bool *ok) {
+    *ok = true;
+    bool no_film = false;
+    if (film_not_positive(S, &no_film)) return 2;
+    if (no_film) { S->coupled = 0; return 0; }
+    conc_fluxes(S, S->h, S->Fc1, S->rc1);
+    int ch = 0;
+    if (conc_solve(S, false, &ch)) return 2;
+    if (ch != 0) { *ok = false; return 0; }
+    YZ_TRY(hipMemsetAsync(S->cdmax, 0, sizeof(unsigned long long), S->st));
+    launch_conc_take(S->g, S->cx, S->c, S->cprev, S->cdmax, S->st);
+    YZ_TRY(hipMemcpyAsync(S->hcdmax, S->cdmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    S->coupled = bits_to_double(*S->hcdmax);
+    return 0;
+}
+
+// RedistributeExcess (HeightEquationBase.h:1128-1273) on cx, with the film
+// of the step's end
+static int conc_redistribute(yz_step *S) {
+    const Geo &g = S->g;
+    ExcessArgs e;
+    e.g = g; e.h = S->h; e.c = S->cx;
+    e.room = S->ex[0]; e.gain = S->ex[1]; e.f = S->ex[2]; e.e = S->ex[3];
+    e.part = S->csums; e.sums = S->csums + 256; e.flags = S->cflags; e.lower = 0;
+    S->cstat[3] = S->cstat[4] = 0;
+    YZ_TRY(hipMemsetAsync(S->cflags, 0, 4 * sizeof(int), S->st));
+    launch_excess_flags(e, S->st);
+    YZ_TRY(hipMemcpyAsync(S->hcflags, S->cflags, 4 * sizeof(int), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    const bool over = S->hcflags[0] != 0, under = S->hcflags[1] != 0;
+    if (!over && !under) return 0;
+    for (int lower = 0; lower < 2; ++lower) {
+        if ((lower == 0 && !over) || (lower == 1 && !under)) continue;
+        e.lower = lower;
+        bool any = true;
+        for (int pass = 0; pass < 4 && any; ++pass) {
+            YZ_TRY(hipMemsetAsync(S->cflags + 2, 0, sizeof(int), S->st));
+            launch_excess_pass(e, pass, S->st);
+            YZ_TRY(hipGetLastError());
+            YZ_TRY(hipMemcpyAsync(S->hcflags, S->cflags, 4 * sizeof(int), hipMemcpyDeviceToHost, S->st));
+            YZ_TRY(hipMemcpyAsync(S->hcsums, S->csums + 256, 4 * sizeof(double), hipMemcpyDeviceToHost, S->st));
+            YZ_TRY(hipStreamSynchronize(S->st));
+            if (pass == 0) S->cstat[3 + lower] = S->hcsums[0];
+            any = S->hcflags[2] != 0;
+        }
+        if (any) launch_excess_spread(e, S->st);
+    }
+    launch_excess_caps(e, under ? 1 : 0, S->st);
+    YZ_TRY(hipGetLastError());
+    return 0;
+}
+
+// AfterNKLoop's concentration part (HeightEquationBase.h:1447-1577): the
+// face fluxes at the film the iteration ended on, made those of the step
+// the film took (ConsistentFluxes), the concentration solved once more with
+// them and formed from the balance of h c, the excess moved, the step kept
+static int conc_after(yz_step *S) {
+    const Geo &g = S->g;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    conc_fluxes(S, S->h, S->Fc1, S->rc1);
+    ConcArgs a = S->ca;
+    a.dt = S->dt_now;
+    launch_consistent_fluxes(a, S->Fc1[0], S->Fc1[1], S->cscratch, S->st);
+    YZ_TRY(hipGetLastError());
+    bool no_film = false;
+    if (film_not_positive(S, &no_film)) return 2;
+    YZ_TRY(hipMemcpyAsync(S->clast, S->cx, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    bool solved = true;
+    if (!no_film) {
+        int ch = 0;
+        if (conc_solve(S, true, &ch)) return 2;
+        if (ch != 0) solved = false;
+    }
+    if (!solved) YZ_TRY(hipMemcpyAsync(S->cx, S->clast, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    if (conc_redistribute(S)) return 2;
+    launch_conc_commit(g, S->cx, S->c0, S->c, S->cfield, S->st);
+    YZ_TRY(hipGetLastError());
+    return 0;
+}
+
 // NKLoop (NewtonKantorovich.cpp:157-422)
 static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out) {
     const yz_case &c = S->cs;
@@ -450,6 +636,7 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
     }
     if (!guessed) YZ_TRY(hipMemcpyAsync(S->h, S->hold, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     // BeforeNKLoop + the explicit half A(h^n) h^n
+    if (S->conc && conc_before(S)) return 2;
     {
         RowArgs a = S->ra;
         a.ax = S->axold;
@@ -465,6 +652,11 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
     bool done = false;
     int outer;
     for (outer = 0; outer < c.nk_max; ++outer) {
+        if (S->conc) {
+            bool ok = true;
+            if (conc_every(S, &ok)) return 2;
+            if (!ok) { done = false; break; }
+        }
         int it = 0, dev_err = 0;
         double lr = 0;
         const int ch = nk_solve(S, &it, &lr, &dev_err);
@@ -474,6 +666,7 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
         hipLaunchKernelGGL(k_absmax, dim3(std::min(nb(g.N), 1024)), dim3(TB), 0, S->st, g.N, S->corr, S->dmax + 1);
         if (sync_read(S)) return 2;
         resid_eq = bits_to_double(S->hmax[1]);
+        if (S->conc && S->coupled > resid_eq) resid_eq = S->coupled;
         if (std::getenv("YZ_DEBUG"))
             std::fprintf(stderr, "[hip] pass %d lin %d resid_lin %.3e corr %.3e rc %d\n", outer, it, lr, resid_eq, ch);
         if (ch != 0) { done = false; break; }
@@ -529,6 +722,11 @@ void yz_case_defaults(yz_case *c) {
     c->learn_passes = 2; c->learn_iters = 100; c->learn_probes = 4; c->learn_lr = 2e-3;
     c->stall_window = 40;
     c->graphs = 0;
+    c->conc_model = 0;
+    c->conc_convection = 1; c->conc_diffusion = 1;
+    c->conc_scheme = YZ_CONV_UTOPIA;
+    c->Df = 1.0; c->diff_c0 = 0.5; c->diff_exp = 0.0;
+    c->conc_lin_maxit = 5000; c->conc_lin_eps = 1e-13; c->conc_row_scaling = 1;
 }
 
 const char *yz_last_error(void) { return g_err.c_str(); }
@@ -590,6 +788,62 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
             }
         }
     }
+    if (c->conc_model == 1) {
+        S->conc = true;
+        double **cp[] = {&S->cfield, &S->c0, &S->cx, &S->cprev, &S->clast, &S->Fc0[0], &S->Fc0[1], &S->Fc1[0],
+                         &S->Fc1[1], &S->G2[0], &S->G2[1], &S->rc0, &S->rc1, &S->ex[0], &S->ex[1], &S->ex[2],
+                         &S->ex[3]};
+        for (double **p : cp) {
+            if (hipMalloc((void **)p, np * sizeof(double)) != hipSuccess) return fail("hipMalloc");
+            if (hipMemset(*p, 0, np * sizeof(double)) != hipSuccess) return fail("memset");
+        }
+        for (double **p : {&S->dconc, &S->dsave, &S->bconc})
+            if (hipMalloc((void **)p, g.N * sizeof(double)) != hipSuccess) return fail("hipMalloc");
+        if (hipMemset(S->dconc, 0, g.N * sizeof(double)) != hipSuccess) return fail("memset");
+        if (hipMalloc((void **)&S->cscratch, ((size_t)g.N + 2 * g.ny + 256 + 8) * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&S->csums, (256 + 8) * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&S->cflags, 4 * sizeof(int)) != hipSuccess ||
+            hipMalloc((void **)&S->cdmax, sizeof(unsigned long long)) != hipSuccess)
+            return fail("hipMalloc");
+        if (hipHostMalloc((void **)&S->hcflags, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&S->hcdmax, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&S->hcsums, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+            return fail("hipHostMalloc");
+        for (double *p : {S->cfield, S->c0, S->cx, S->cprev})
+            if (hipMemcpy(p, S->c, np * sizeof(double), hipMemcpyDeviceToDevice) != hipSuccess) return fail("copy");
+        if (solver_init(S->Sc, fine_op(g), S->st)) return fail("solver allocation");
+        ConcArgs &k = S->ca;
+        std::memset(&k, 0, sizeof(k));
+        k.g = g;
+        k.theta_h = c->theta;
+        k.conv = c->conc_convection; k.diff = c->conc_diffusion;
+        k.side = (c->dim == 1 && c->centrifugal) ? 1 : 0;
+        // the scheme's c1 and c2 over the sum of c1 (FiniteDifferences.cpp:22-93)
+        double c1[4] = {0, 0, 0, 0}, c2[4] = {0, 0, 0, 0};
+        switch (c->conc_scheme) {
+        case YZ_CONV_UPWIND1: c1[1] = 1; c1[2] = 1; c2[1] = 1; c2[2] = -1; break;
+        case YZ_CONV_CENTRAL2: c1[1] = 1; c1[2] = 1; break;
+        case YZ_CONV_CENTRAL4: c1[0] = -1; c1[1] = 9; c1[2] = 9; c1[3] = -1; break;
+        case YZ_CONV_UPWIND2: c1[0] = -1; c1[1] = 3; c1[2] = 3; c1[3] = -1; c2[0] = -1; c2[1] = 3; c2[2] = -3; c2[3] = 1; break;
+        case YZ_CONV_QUICK: c1[0] = -1; c1[1] = 9; c1[2] = 9; c1[3] = -1; c2[0] = -1; c2[1] = 3; c2[2] = -3; c2[3] = 1; break;
+        case YZ_CONV_KK: c1[0] = -1; c1[1] = 7; c1[2] = 7; c1[3] = -1; c2[0] = -3; c2[1] = 9; c2[2] = -9; c2[3] = 3; break;
+        default: c1[0] = -1; c1[1] = 7; c1[2] = 7; c1[3] = -1; c2[0] = -1; c2[1] = 3; c2[2] = -3; c2[3] = 1; break;
+        }
+        double den = 0;
+        for (int j = 0; j < 4; ++j) den += c1[j];
+        for (int j = 0; j < 4; ++j) {
+            const double a1 = c1[j] / den, a2 = c2[j] / den;
+            k.wpos[j] = a1 + 1.0 * a2;
+            k.wneg[j] = a1 + (-1.0) * a2;
+        }
+        k.Df = c->Df; k.diff_c0 = c->diff_c0; k.diff_exp = c->diff_exp;
+        k.row_scaling = c->conc_row_scaling;
+        k.h = S->h; k.c = S->c; k.hold = S->hold; k.c0 = S->c0;
+        for (int ax = 0; ax < 2; ++ax) { k.F0[ax] = S->Fc0[ax]; k.F1[ax] = S->Fc1[ax]; k.G2[ax] = S->G2[ax]; }
+        k.r0 = S->rc0; k.r1 = S->rc1;
+        k.cx = S->cx;
+        k.A = S->Sc.A; k.b = S->bconc;
+    }
     // row kernel arguments
     RowArgs &a = S->ra;
     std::memset(&a, 0, sizeof(a));
@@ -617,6 +871,15 @@ void yz_step_destroy(yz_step *S) {
     if (!S) return;
     solver_free(S->S);
     solver_free(S->S32);
+    solver_free(S->Sc);
+    for (double *p : {S->cfield, S->c0, S->cx, S->cprev, S->clast, S->Fc0[0], S->Fc0[1], S->Fc1[0], S->Fc1[1],
+                      S->G2[0], S->G2[1], S->rc0, S->rc1, S->ex[0], S->ex[1], S->ex[2], S->ex[3], S->dconc,
+                      S->dsave, S->bconc, S->cscratch, S->csums})
+        if (p) (void)hipFree(p);
+    if (S->cflags) (void)hipFree(S->cflags);
+    if (S->cdmax) (void)hipFree(S->cdmax);
+    for (void *p : {(void *)S->hcflags, (void *)S->hcdmax, (void *)S->hcsums})
+        if (p) (void)hipHostFree(p);
     mlp_free(S->mlp);
     for (float *p : {S->b32, S->corr32, S->feat, S->mscale, S->M32})
         if (p) (void)hipFree(p);
@@ -699,6 +962,7 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
             step_done = true;
         }
     } while (!step_done);
+    if (S->conc && conc_after(S)) return 2;
     YZ_TRY(hipMemcpyAsync(S->hnew, S->h, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     YZ_TRY(hipEventRecord(S->ev[11], S->st));
     YZ_TRY(hipEventSynchronize(S->ev[11]));
@@ -710,6 +974,51 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
     if (nk_iter) *nk_iter = it;
     if (lin_iter) *lin_iter = (int)lin_sum;
     if (resid) *resid = r;
+    return 0;
+}
+
+int yz_step_get_c(yz_step *S, double *c) {
+    if (!S || !c) { set_err("null argument"); return 2; }
+    const int np = S->g.nxp * S->g.nyp;
+    YZ_TRY(hipMemcpyAsync(c, S->conc ? S->cfield : S->c, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+int yz_step_face_fluxes(yz_step *S, const double *h, const double *c, double *F, double *r) {
+    if (!S || !h || !c) { set_err("null argument"); return 2; }
+    const Geo &g = S->g;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    double *buf = nullptr;
+    YZ_TRY(hipMalloc((void **)&buf, 5 * np * sizeof(double)));
+    double *dh = buf, *dc = buf + np, *F0 = buf + 2 * np, *F1 = buf + 3 * np, *dr = buf + 4 * np;
+    int rc = 0;
+    do {
+        if (hipMemcpyAsync(dh, h, np * sizeof(double), hipMemcpyHostToDevice, S->st) != hipSuccess ||
+            hipMemcpyAsync(dc, c, np * sizeof(double), hipMemcpyHostToDevice, S->st) != hipSuccess ||
+            hipMemsetAsync(F0, 0, 3 * np * sizeof(double), S->st) != hipSuccess) { rc = 2; break; }
+        hipLaunchKernelGGL(k_ghosts, dim3(nb((int)np)), dim3(TB), 0, S->st, g, dh);
+        hipLaunchKernelGGL(k_ghosts, dim3(nb((int)np)), dim3(TB), 0, S->st, g, dc);
+        RowArgs a = S->ra;
+        a.h = dh;
+        a.c = dc;
+        launch_face_fluxes(a, F0, F1, dr, S->st);
+        if (hipGetLastError() != hipSuccess) { rc = 2; break; }
+        if (F && (hipMemcpyAsync(F, F0, np * sizeof(double), hipMemcpyDeviceToHost, S->st) != hipSuccess ||
+                  hipMemcpyAsync(F + np, F1, np * sizeof(double), hipMemcpyDeviceToHost, S->st) != hipSuccess)) {
+            rc = 2; break;
+        }
+        if (r && hipMemcpyAsync(r, dr, np * sizeof(double), hipMemcpyDeviceToHost, S->st) != hipSuccess) { rc = 2; break; }
+        if (hipStreamSynchronize(S->st) != hipSuccess) rc = 2;
+    } while (0);
+    (void)hipFree(buf);
+    if (rc) set_err("face fluxes: device error");
+    return rc;
+}
+
+int yz_step_conc_stats(yz_step *S, double *out5) {
+    if (!S || !out5) { set_err("null argument"); return 2; }
+    for (int k = 0; k < 5; ++k) out5[k] = S->cstat[k];
     return 0;
 }
 
@@ -1107,6 +1416,7 @@ yz_batch *yz_batch_create(int n, const yz_case *cases, const double *h, const do
         if (c.step_control) { set_err("a batch advances fixed steps (step_control 0)"); return nullptr; }
         if (c.learn_passes != 0) { set_err("a batch uses one shared network: learn_passes 0 (yz_batch_set_params)"); return nullptr; }
         if (c.lin_maxit != cases[0].lin_maxit) { set_err("the films of a batch share lin_maxit"); return nullptr; }
+        if (c.conc_model != 0) { set_err("a batch takes films with a frozen concentration (conc_model 0)"); return nullptr; }
     }
     yz_batch *B = new yz_batch();
     auto fail = [&](const char *what) { if (what) set_err(what); yz_batch_destroy(B); return (yz_batch *)nullptr; };

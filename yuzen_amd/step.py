@@ -71,6 +71,29 @@ class HipStep:
         if self.lib.yz_step_set_h(self.p, _pd(h)):
             raise RuntimeError(capi.last_error())
 
+    def c(self):
+        """the concentration of the state (the frozen input with conc_model 0)"""
+        out = np.empty(self.n)
+        if self.lib.yz_step_get_c(self.p, _pd(out)):
+            raise RuntimeError(capi.last_error())
+        return out
+
+    def face_fluxes(self, h, c):
+        """FaceVolumeFluxes of film h with concentration c: (F[2, n], r[n])"""
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        F, r = np.zeros(2 * self.n), np.zeros(self.n)
+        if self.lib.yz_step_face_fluxes(self.p, _pd(h), _pd(c), _pd(F), _pd(r)):
+            raise RuntimeError(capi.last_error())
+        return F.reshape(2, self.n), r
+
+    def conc_stats(self):
+        """[solves, their Krylov iterations, solves that needed the multigrid,
+        solute moved above 1, below 0 (last step)]"""
+        out = np.zeros(5)
+        self.lib.yz_step_conc_stats(self.p, _pd(out))
+        return out
+
     def clock(self):
         t, a, b = C.c_double(), C.c_double(), C.c_double()
         self.lib.yz_step_clock(self.p, C.byref(t), C.byref(a), C.byref(b))
