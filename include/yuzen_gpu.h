@@ -136,7 +136,10 @@ enum { YZ_CONV_UPWIND1 = 0, YZ_CONV_CENTRAL2 = 1, YZ_CONV_CENTRAL4 = 2, YZ_CONV_
  * MaxIterations 5000, Epsilon 1e-13, RowScaling on, AMG, EW and
  * extrapolation on, step control off; learned preconditioner trained on
  * 2 passes of 100 Adam steps, rate 2e-3, 4 probes; stall window 40;
- * graphs off. */
+ * graphs off; concentration frozen (conc_model 0), and for conc_model 1:
+ * convection and diffusion on, UTOPIA, Df 1, D(c) exponent 0 about 0.5,
+ * the concentration solve's MaxIterations 5000, Epsilon 1e-13, RowScaling
+ * on. */
 void yz_case_defaults(yz_case *c);
 
 const char *yz_last_error(void);
@@ -288,6 +291,13 @@ typedef struct yz_lin_settings {
     double epsilon;
     int grid_dim, grid_nx, grid_ny;   /* optional structure, 0: none */
     int bc[4];
+    /* LinearSolverSettings::conc (LinearSolverBackend.h:38-51): a
+     * concentration system. With YZ_PC_AMG the solve first tries damped
+     * Jacobi for at most 300 iterations and, if that does not converge,
+     * starts again from the guess with the multigrid, as the reference's
+     * backend does for a concentration equation (LinearSolverBackend.cpp:
+     * 405-470, 587-626); iter counts both. */
+    int conc;
 } yz_lin_settings;
 
 typedef struct yz_linctx yz_linctx;

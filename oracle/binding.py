@@ -48,6 +48,8 @@ def load():
     lib.yzo_face_fluxes.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_conc_stats.argtypes = [C.c_void_p, pd]
     lib.yzo_conc_face.argtypes = [C.c_int, pd, C.c_double, pd]
+    lib.yzo_conc_nnz.argtypes = [C.c_void_p]
+    lib.yzo_conc_system.argtypes = [C.c_void_p, pi_, pi_, pd, pd]
     _lib = lib
     return lib
 
@@ -124,6 +126,15 @@ class OracleStep:
         F, r = np.zeros(2 * self.n), np.zeros(self.n)
         self.lib.yzo_face_fluxes(self.p, _pd(h), _pd(c), _pd(F), _pd(r))
         return F.reshape(2, self.n), r
+
+    def conc_system(self):
+        """the last concentration system: (rowptr, col, val, rhs), unscaled"""
+        nnz = self.lib.yzo_conc_nnz(self.p)
+        rp = np.empty(self.n + 1, dtype=np.int32)
+        col = np.empty(nnz, dtype=np.int32)
+        val, rhs = np.empty(nnz), np.empty(self.n)
+        self.lib.yzo_conc_system(self.p, _pi(rp), _pi(col), _pd(val), _pd(rhs))
+        return rp, col, val, rhs
 
     def conc_stats(self):
         out = np.zeros(5)

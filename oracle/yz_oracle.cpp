@@ -2058,6 +2058,17 @@ void yzo_face_fluxes(void *p, const double *h, const double *c, double *F, doubl
     if (r) std::copy(rr.begin(), rr.end(), r);
     st->eq.h = keep_h; st->eq.c = keep_c;
 }
+/* the concentration system of the last solve (unscaled CRS, the diagonal
+ * first; ghost rows (1, -1)) and its right-hand side */
+int yzo_conc_nnz(void *p) { return (int)static_cast<yzo::Stepper *>(p)->CM.col.size(); }
+void yzo_conc_system(void *p, int *rowptr, int *col, double *val, double *rhs) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    const yzo::Csr &M = st->CM;
+    std::copy(M.ptr.begin(), M.ptr.end(), rowptr);
+    std::copy(M.col.begin(), M.col.end(), col);
+    std::copy(M.val.begin(), M.val.end(), val);
+    std::copy(st->rhsD.begin(), st->rhsD.end(), rhs);
+}
 /* an unlimited face scheme (conc_scheme): out = FaceValue, FaceWeights[4] */
 void yzo_conc_face(int scheme, const double *f, double sign, double *out) {
     yzo::Stepper st;

@@ -110,3 +110,28 @@ def test_linsolve_seam_matches_oracle():
         print(f"linsolve grid={grid is not None}: rc={rc} it={it} resid={r:.2e} (oracle it={it0})", flush=True)
         assert rc == 0 and rc0 == 0
         assert np.abs(x - x0).max() <= 1e-9 * np.abs(x0).max()
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_linsolve_seam_takes_the_concentration_system(dim):
+    """A concentration system of the reference's layout (the 9-point wide
+    stencil in 2-D, ghost rows) through the seam with the concentration
+    role: the grid read off the pattern, the solution to the tolerance."""
+    if dim == 2:
+        c, h, s, cc, dt, _ = cases.allphys_2d(n=40, L=1.5, dt=2e-5)
+    else:
+        c, h, s, cc, dt, _ = cases.leveling_1d(nx=128, steps=1)
+        c.marangoni, c.As, c.Ma = 1, 1.0, 0.3
+        X, _ = cases.cell_centres(c)
+        cc = (0.5 + 0.2 * np.tanh((X - 4.0) / 0.5)).ravel()
+    c.conc_model, c.Df = 1, 0.05
+    o = ob.OracleStep(c, h, s, cc, dt)
+    assert o.advance()[0] == 0
+    rp, col, val, rhs = o.conc_system()
+    rc0, x0, it0, r0 = ob.linsolve(rp, col, val, rhs, np.zeros(o.n), eps=1e-13)
+    ls = LinSolver()
+    rc, x, it, r = ls.solve(rp, col, val, rhs, np.zeros(o.n), epsilon=1e-13, conc=1)
+    print(f"concentration seam {dim}-D: rc={rc} it={it} resid={r:.2e} (oracle it={it0})", flush=True)
+    assert rc == 0 and rc0 == 0
+    assert np.abs(x - x0).max() <= 1e-10 * max(np.abs(x0).max(), 1e-300)
+    assert np.abs(x0).max() > 0

@@ -67,6 +67,7 @@ class YzLinSettings(C.Structure):
         ("epsilon", C.c_double),
         ("grid_dim", C.c_int), ("grid_nx", C.c_int), ("grid_ny", C.c_int),
         ("bc", C.c_int * 4),
+        ("conc", C.c_int),
     ]
 
 

@@ -6,11 +6,13 @@
 // codes are 0 converged / 1 iteration cap / 2 breakdown, and nothing
 // throws across the boundary.
 //
-// The CRS is the film-height matrix of a structured grid in the reference's
-// row layout: two ghost layers per side whose rows are copies x_g - x_src =
-// r_g (BoundaryCondition::EqualVal: Symmetry and Periodic edges). The grid
-// comes from the settings or is read off the matrix itself (the column
-// offsets of an interior row of the 13-point pattern). The ghost rows are
+// The CRS is a film-height or well-mixed concentration matrix of a
+// structured grid in the reference's row layout: two ghost layers per side
+// whose rows are copies x_g - x_src = r_g (BoundaryCondition::EqualVal:
+// Symmetry and Periodic edges), interior rows within the 13-point stencil.
+// The grid comes from the settings or is read off the matrix itself (the
+// column offsets of an interior row of the 13-point film-height pattern or
+// of the 9-point wide concentration pattern, ConcEquationBase.h:242-258). The ghost rows are
 // eliminated, the interior rows become stencil planes, and the system is
 // solved on the device; the ghost entries of x are formed again from their
 // rows afterwards, as CRS_Matrix::SatisfyBCRows makes them.
@@ -77,10 +79,10 @@ int yz_linsolve(yz_linctx *ctx, int n, int nnz, const int *rowptr, const int *co
         int maxoff = 0, cnt = 0;
         for (int i = 0; i < n && cnt == 0; ++i) {
             const int m = rowptr[i + 1] - rowptr[i];
-            if (m != 13 && m != 5) continue;
+            if (m != 13 && m != 9 && m != 5) continue;
             int mo = 0;
             for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) mo = std::max(mo, col[k] - i);
-            if (m == 13 && mo >= 4 && mo % 2 == 0) { maxoff = mo; cnt = 13; }
+            if ((m == 13 || m == 9) && mo >= 4 && mo % 2 == 0) { maxoff = mo; cnt = 13; }
             else if (m == 5 && mo == 2) { maxoff = mo; cnt = 5; }
         }
         if (cnt == 13) {
@@ -176,10 +178,24 @@ int yz_linsolve(yz_linctx *ctx, int n, int nnz, const int *rowptr, const int *co
     if (hipMemcpyAsync(ctx->b, b.data(), N * sizeof(double), hipMemcpyHostToDevice, ctx->st) != hipSuccess) return 2;
     if (hipMemcpyAsync(ctx->x, x.data(), N * sizeof(double), hipMemcpyHostToDevice, ctx->st) != hipSuccess) return 2;
     const int pc = s->precond == YZ_PC_AMG ? YZ_PC_AMG : YZ_PC_JACOBI;
-    if ((pc == YZ_PC_AMG ? amg_setup(S) : jacobi_setup(S)) != 0) return 2;
-    int it = 0;
+    int it = 0, tried = 0;
     double rr = 0;
-    const int rc = bicgstab(S, ctx->b, ctx->x, pc, s->max_iter, s->epsilon, &it, &rr);
+    int rc = -1;
+    if (s->conc && pc == YZ_PC_AMG) {
+        // a concentration system: damped Jacobi first, at most 300
+        // iterations; from the guess again with the multigrid if it fails
+        if (jacobi_setup(S) != 0) return 2;
+        rc = bicgstab(S, ctx->b, ctx->x, YZ_PC_JACOBI, std::min(s->max_iter, 300), s->epsilon, &it, &rr);
+        if (rc != 0) {
+            tried = it;
+            if (hipMemcpyAsync(ctx->x, x.data(), N * sizeof(double), hipMemcpyHostToDevice, ctx->st) != hipSuccess) return 2;
+        }
+    }
+    if (rc != 0) {
+        if ((pc == YZ_PC_AMG ? amg_setup(S) : jacobi_setup(S)) != 0) return 2;
+        rc = bicgstab(S, ctx->b, ctx->x, pc, s->max_iter, s->epsilon, &it, &rr);
+        it += tried;
+    }
     if (hipMemcpyAsync(x.data(), ctx->x, N * sizeof(double), hipMemcpyDeviceToHost, ctx->st) != hipSuccess) return 2;
     if (hipStreamSynchronize(ctx->st) != hipSuccess) return 2;
     for (int iy = 0; iy < op.ny; ++iy)

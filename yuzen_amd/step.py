@@ -185,9 +185,10 @@ class LinSolver:
             self.p = None
 
     def solve(self, rowptr, col, val, rhs, x0, precond=capi.PC_AMG, scaling=1, max_iter=5000,
-              epsilon=1e-13, grid=None):
+              epsilon=1e-13, grid=None, conc=0):
         s = capi.YzLinSettings()
         s.precond, s.scaling, s.max_iter, s.epsilon = precond, scaling, max_iter, epsilon
+        s.conc = conc
         if grid is not None:
             s.grid_dim, s.grid_nx, s.grid_ny = grid[0], grid[1], grid[2]
             for k in range(4):
