@@ -5,9 +5,10 @@
 //   A (16 x 16, rows m, depth k):  a[j] = A[l % 16][4 (l / 16) + j]
 //   B (16 x 16, depth k, cols n):  b[j] = B[4 (l / 16) + j][l % 16]
 //   C (16 x 16):                   c[j] = C[4 (l / 16) + j][l % 16]
-// One wave takes 16 cells at a time: the 16 x 8 features (padded to 16)
-// times W1^T, then 16 x 64 times W2^T and W3^T, the activations passing
-// through the wave's own LDS tile between layers. The backward pass
+// One wave takes 16 cells at a time. Training: the 16 x 8 features (padded
+// to 16) times W1^T, then 16 x 64 times W2^T and W3^T, the activations
+// passing through the wave's own LDS tile between layers. (Inference runs
+// the product transposed, with no LDS, k_mlp_forward.) The backward pass
 // recomputes the forward for its tile, forms dY from dL/dM, and accumulates
 // the weight gradients over all its tiles in MFMA accumulators; each wave
 // writes its partial gradients once, and one kernel sums them in a fixed
@@ -161,6 +162,35 @@ __device__ inline f32x4 tile_forward(const WLds &W, const float *__restrict__ fe
     return y;
 }
 
+// Inference runs the network transposed, H^T = W X^T: the weights are the
+// A operands, held in registers for the whole kernel, and the activations
+// of 16 cells are the B operands. The C layout of one layer's output,
+// C[4 (l/16) + j][l % 16] (features x cells), is the B layout of the next
+// layer's input, so the activations never leave the registers: no LDS tile
+// and no lane exchange between the layers.
+//
+// A fragment of W (row-major [O][I]) for output block ob, input block kb:
+// a[j] = W[16 ob + l % 16][16 kb + 4 (l / 16) + j]
+__device__ inline bf16x4 fragWa(const short *W, int I, int ob, int kb, int l) {
+    bf16x4 a;
+    const short *r = W + (16 * ob + (l & 15)) * I + 16 * kb + 4 * (l >> 4);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = r[j];
+    return a;
+}
+// the features of cells c0..c0+15 as the B operand of X^T:
+// b[j] = feat[4 (l / 16) + j][c0 + l % 16] (features 8..15 zero)
+__device__ inline bf16x4 fragXT(const float *__restrict__ feat, int N, int c0, int l) {
+    bf16x4 b;
+    const int cell = c0 + (l & 15);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int f = 4 * (l >> 4) + j;
+        b[j] = f2bf((f < MLP_IN && cell < N) ? feat[(size_t)f * N + cell] : 0.0f);
+    }
+    return b;
+}
+
 // M[o][cell] = Y[cell][o] * mscale[cell], o < 13 (FP32); or, with HALF, the
 // raw outputs Y[o][cell] in FP16 (the row scale is applied by the consumer)
 template <bool HALF>
@@ -169,27 +199,63 @@ __global__ void __launch_bounds__(MBLK) k_mlp_forward(int N, const float *__rest
                                                       const float *__restrict__ mscale, float *__restrict__ M,
                                                       __half *__restrict__ Yh) {
     __shared__ WLds W;
-    __shared__ float tiles[WAVES][2][16 * LDT];
     load_weights(P, W);
     __syncthreads();
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    float *H1 = tiles[wv][0], *H2 = tiles[wv][1];
+    const int r0 = 4 * (l >> 4);   // the rows (features, outputs) this lane holds: r0 .. r0 + 3
+    bf16x4 w1[4], w2[16], w3[4];
+    float b1[4][4], b2[4][4], b3[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob) {
+        w1[ob] = fragWa(W.w1, 16, ob, 0, l);
+        w3[ob] = fragWa(W.w3, 64, 0, ob, l);
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) w2[ob * 4 + kb] = fragWa(W.w2, 64, ob, kb, l);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { b1[ob][j] = W.b1[16 * ob + r0 + j]; b2[ob][j] = W.b2[16 * ob + r0 + j]; }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b3[j] = W.b3[r0 + j];
     const int ntiles = (N + 15) / 16;
-    for (int tile = blockIdx.x * WAVES + wv; tile < ntiles; tile += gridDim.x * WAVES) {
+    const int stride = gridDim.x * WAVES;
+    int tile = blockIdx.x * WAVES + wv;
+    bf16x4 xb = fragXT(feat, N, tile * 16, l);
+    for (; tile < ntiles; tile += stride) {
         const int c0 = tile * 16;
-        const f32x4 y = tile_forward(W, feat, N, c0, H1, H2, l);
-        const int o = l & 15;
-        if (o < MLP_OUT) {
+        // the next tile's features, in flight while this tile computes
+        const bf16x4 xn = fragXT(feat, N, (tile + stride) * 16, l);
+        bf16x4 h1[4], h2[4];
+#pragma unroll
+        for (int ob = 0; ob < 4; ++ob) {
+            f32x4 acc = {0, 0, 0, 0};
+            acc = mfma(w1[ob], xb, acc);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h1[ob][j] = f2bf(fmaxf(acc[j] + b1[ob][j], 0.0f));
+        }
+#pragma unroll
+        for (int ob = 0; ob < 4; ++ob) {
+            f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) acc = mfma(w2[ob * 4 + kb], h1[kb], acc);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h2[ob][j] = f2bf(fmaxf(acc[j] + b2[ob][j], 0.0f));
+        }
+        f32x4 y = {0, 0, 0, 0};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) y = mfma(w3[kb], h2[kb], y);
+        const int cell = c0 + (l & 15);
+        if (cell < N) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int cell = c0 + 4 * (l >> 4) + j;
-                if (cell < N) {
-                    if (HALF) Yh[(size_t)o * N + cell] = __float2half(y[j]);
-                    else M[(size_t)o * N + cell] = y[j] * mscale[cell];
+                const int o = r0 + j;
+                if (o < MLP_OUT) {
+                    const float v = y[j] + b3[j];
+                    if (HALF) Yh[(size_t)o * N + cell] = __float2half(v);
+                    else M[(size_t)o * N + cell] = v * mscale[cell];
                 }
             }
         }
-        wave_sync();
+        xb = xn;
     }
 }
 
@@ -513,9 +579,13 @@ void mlp_free(Mlp &m) {
     m = Mlp();
 }
 
+// blocks of the inference: each block stages the weights once, then its
+// waves keep them in registers over many 16-cell tiles
+static constexpr int INFER_BLOCKS = 1024;
+
 int mlp_infer(Mlp &m, int N, const float *feat, const float *mscale, float *M, hipStream_t st) {
     const int ntiles = (N + 15) / 16;
-    const int blocks = std::min((ntiles + WAVES - 1) / WAVES, 2048);
+    const int blocks = std::min((ntiles + WAVES - 1) / WAVES, INFER_BLOCKS);
     hipLaunchKernelGGL(k_mlp_forward<false>, dim3(blocks), dim3(MBLK), 0, st, N, (const float *)m.params, feat, mscale,
                        M, (__half *)nullptr);
     MLP_CHECK(hipGetLastError());
@@ -524,7 +594,7 @@ int mlp_infer(Mlp &m, int N, const float *feat, const float *mscale, float *M, h
 
 int mlp_infer_half(Mlp &m, int N, const float *feat, __half *Y, hipStream_t st) {
     const int ntiles = (N + 15) / 16;
-    const int blocks = std::min((ntiles + WAVES - 1) / WAVES, 2048);
+    const int blocks = std::min((ntiles + WAVES - 1) / WAVES, INFER_BLOCKS);
     hipLaunchKernelGGL(k_mlp_forward<true>, dim3(blocks), dim3(MBLK), 0, st, N, (const float *)m.params, feat,
                        (const float *)nullptr, (float *)nullptr, Y);
     MLP_CHECK(hipGetLastError());
