@@ -122,6 +122,15 @@ typedef struct yz_case {
     int conc_lin_maxit;
     double conc_lin_eps;
     int conc_row_scaling;
+
+    /* the initial guess of each Newton pass's linear solve: 0 the previous
+     * pass's correction, as the reference passes _correctVec
+     * (NewtonKantorovich.cpp:130); 1 zero; -1 (default) the reference's on
+     * the FP64 path and zero on the FP32 inner solve (no reference
+     * counterpart: each inexact solve starts afresh, which at 1024^2 takes
+     * 28 % fewer iterations than starting from a correction up to 10^4
+     * times larger than the one sought) */
+    int zero_guess;
 } yz_case;
 
 /* Face interpolation of the concentration's advection (reference
@@ -139,7 +148,7 @@ enum { YZ_CONV_UPWIND1 = 0, YZ_CONV_CENTRAL2 = 1, YZ_CONV_CENTRAL4 = 2, YZ_CONV_
  * graphs off; concentration frozen (conc_model 0), and for conc_model 1:
  * convection and diffusion on, UTOPIA, Df 1, D(c) exponent 0 about 0.5,
  * the concentration solve's MaxIterations 5000, Epsilon 1e-13, RowScaling
- * on. */
+ * on; zero_guess -1. */
 void yz_case_defaults(yz_case *c);
 
 const char *yz_last_error(void);

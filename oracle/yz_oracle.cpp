@@ -1845,6 +1845,10 @@ struct Stepper {
             ew_eta = eta; ew_prev = rn;
             eps = eta;
         }
+        /* the previous pass's correction as the guess (the reference's), or
+         * zero (zero_guess 1; the oracle is the FP64 path, so -1 is the
+         * reference's) */
+        if (eq.cs.zero_guess == 1) std::fill(corr.begin(), corr.end(), 0.0);
         const int ch = linsolve(eq.IA, Rhs, corr, eq.cs.precond == YZ_PC_JACOBI ? YZ_PC_JACOBI : YZ_PC_AMG,
                                 eq.cs.row_scaling, eq.cs.lin_maxit, eps, iter, resid);
         if (ch == 0) satisfy_bc(corr);
@@ -2014,6 +2018,7 @@ void yzo_case_defaults(yz_case *c) {
     c->conc_scheme = YZ_CONV_UTOPIA;
     c->Df = 1.0; c->diff_c0 = 0.5; c->diff_exp = 0.0;
     c->conc_lin_maxit = 5000; c->conc_lin_eps = 1e-13; c->conc_row_scaling = 1;
+    c->zero_guess = -1;
 }
 
 void *yzo_create(const yz_case *c, const double *h, const double *s, const double *conc, double dt0) {

@@ -91,6 +91,15 @@ def test_step_allphys_2d_small():
     _run_pair(c, h, s, cc, dt, 4, "allphys_2d")
 
 
+def test_step_allphys_2d_zero_guess():
+    """The plain FP64 path with every pass's solve started from zero
+    (zero_guess 1, a departure from the reference's guess) against the
+    oracle run the same way."""
+    c, h, s, cc, dt, _ = cases.allphys_2d(n=48, L=2.0, dt=2e-5)
+    c.zero_guess = 1
+    _run_pair(c, h, s, cc, dt, 4, "allphys_2d zero guess")
+
+
 def test_step_spinodal_2d_small():
     c, h, s, cc, dt, _ = cases.spinodal_2d(n=32)
     _run_pair(c, h, s, cc, dt, 4, "spinodal_2d")

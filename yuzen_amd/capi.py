@@ -53,6 +53,7 @@ class YzCase(C.Structure):
         ("Df", C.c_double), ("diff_c0", C.c_double), ("diff_exp", C.c_double),
         ("conc_lin_maxit", C.c_int), ("conc_lin_eps", C.c_double),
         ("conc_row_scaling", C.c_int),
+        ("zero_guess", C.c_int),
     ]
 
     def copy(self):
@@ -94,6 +95,7 @@ def case_defaults():
     c.conc_model, c.conc_convection, c.conc_diffusion, c.conc_scheme = 0, 1, 1, CONV_UTOPIA
     c.Df, c.diff_c0, c.diff_exp = 1.0, 0.5, 0.0
     c.conc_lin_maxit, c.conc_lin_eps, c.conc_row_scaling = 5000, 1e-13, 1
+    c.zero_guess = -1
     return c
 
 

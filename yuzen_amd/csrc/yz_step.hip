@@ -219,6 +219,7 @@ static int check_case(const yz_case &c) {
         set_err("the learned preconditioner is built for the 2-D film"); return 1;
     }
     if (c.stall_window < 0) { set_err("stall_window must be >= 0"); return 1; }
+    if (c.zero_guess < -1 || c.zero_guess > 1) { set_err("zero_guess must be -1, 0 or 1"); return 1; }
     if (c.conc_model != 0 && c.conc_model != 1) { set_err("conc_model must be 0 (frozen) or 1 (well-mixed)"); return 1; }
     if (c.conc_model == 1) {
         if (c.conc_scheme < YZ_CONV_UPWIND1 || c.conc_scheme > YZ_CONV_KK) {
@@ -314,6 +315,13 @@ static bool fp32_path(const yz_case &c) {
 
 // the FP32 inner solve cannot reach below this relative residual
 static constexpr double kEpsFP32 = 1.0e-6;
+
+// whether each pass's solve starts from zero rather than from the previous
+// pass's correction (zero_guess -1: the reference's guess on the FP64 path,
+// zero on the FP32 inner solve)
+static bool zero_guess(const yz_case &c) {
+    return c.zero_guess == 1 || (c.zero_guess < 0 && fp32_path(c));
+}
 
 // the linear tolerance of this pass (NewtonKantorovich.cpp:108-131):
 // Eisenstat-Walker from max |Rhs| (read back into hmax[0]) when on
@@ -424,6 +432,7 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
         if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
         YZ_TRY(hipEventRecord(S->ev[3], S->st));
         const double eps32 = eps > kEpsFP32 ? eps : kEpsFP32;
+        if (zero_guess(c)) YZ_TRY(hipMemsetAsync(S->corr32, 0, S->g.N * sizeof(float), S->st));
         if (c.precond == YZ_PC_LEARNED)
             ch = bicgstab_stencil(S->S32, (const float *)S->b32, S->corr32, c.lin_maxit, eps32, iter, resid);
         else
@@ -443,6 +452,7 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
         if (c.precond == YZ_PC_AMG) rc = amg_setup(S->S); else rc = jacobi_setup(S->S);
         if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
         YZ_TRY(hipEventRecord(S->ev[3], S->st));
+        if (zero_guess(c)) YZ_TRY(hipMemsetAsync(S->corr, 0, S->g.N * sizeof(double), S->st));
         ch = bicgstab(S->S, S->b, S->corr, c.precond, c.lin_maxit, eps, iter, resid);
         YZ_TRY(hipEventRecord(S->ev[4], S->st));
         YZ_TRY(hipEventSynchronize(S->ev[4]));
@@ -727,6 +737,7 @@ void yz_case_defaults(yz_case *c) {
     c->conc_scheme = YZ_CONV_UTOPIA;
     c->Df = 1.0; c->diff_c0 = 0.5; c->diff_exp = 0.0;
     c->conc_lin_maxit = 5000; c->conc_lin_eps = 1e-13; c->conc_row_scaling = 1;
+    c->zero_guess = -1;
 }
 
 const char *yz_last_error(void) { return g_err.c_str(); }
@@ -1497,6 +1508,7 @@ int yz_batch_advance(yz_batch *B, int *status, int *nk_iter, int *lin_iter) {
             S->S32.stall_window = S->cs.stall_window;
             sv[u] = &S->S32; bv[u] = S->b32; xv[u] = S->corr32;
             tol[u] = F.eps > kEpsFP32 ? F.eps : kEpsFP32;
+            if (zero_guess(S->cs)) YZ_TRY(hipMemsetAsync(S->corr32, 0, S->g.N * sizeof(float), B->st));
         }
         if (bicgstab_stencil_batch(B->table, m, sv.data(), bv.data(), xv.data(), B->f[who[0]].S->cs.lin_maxit,
                                    tol.data(), its.data(), resid.data(), rcs.data(), B->st)) {
