@@ -184,6 +184,18 @@ __device__ inline float apply13(const T *__restrict__ W, int N, int i, const flo
     for (int k = 0; k < 13; ++k) a = fmaf(cv(W[k * N + i]), L[ly + 2 + OY[k]][lx + 2 + OX[k]], a);
     return a;
 }
+// the same for A with a unit diagonal: slot 6 (the centre) contributes its
+// staged value without a load; fmaf(1, x, a) and a + x round the same, so
+// the result is the same to the bit
+__device__ inline float apply13u(const float *__restrict__ W, int N, int i, const float (&L)[LH][LW], int lx, int ly) {
+    float a = 0;
+#pragma unroll
+    for (int k = 0; k < 13; ++k) {
+        if (k == 6) a = a + L[ly + 2][lx + 2];
+        else a = fmaf(W[k * N + i], L[ly + 2 + OY[k]][lx + 2 + OX[k]], a);
+    }
+    return a;
+}
 __device__ inline float idf(float v) { return v; }
 __device__ inline float h2f(__half v) { return __half2float(v); }
 
@@ -197,6 +209,7 @@ struct Vecs {
 // or in a batch)
 struct Film {
     StencilOp op;
+    int unitd;        // A's diagonal is stored as exactly 1 (apply13u)
     const float *A;
     const __half *Y;
     const float *ms;
@@ -245,7 +258,7 @@ __global__ void __launch_bounds__(TB) k_f_init(const Film one, const Film *__res
         __syncthreads();
         for_own(op, x0, y0, [&](int lx, int ly, int i) {
             const float bi = b[i];
-            const float ri = bi - apply13(A, op.N, i, L, lx, ly, idf);
+            const float ri = bi - (f.unitd ? apply13u(A, op.N, i, L, lx, ly) : apply13(A, op.N, i, L, lx, ly, idf));
             V.r[i] = ri; V.rh[i] = ri;
             acc[0] += (double)bi * bi;
             acc[1] += (double)ri * ri;
@@ -375,7 +388,7 @@ __global__ void __launch_bounds__(TB) k_f_av(const Film one, const Film *__restr
         stage(op, x0, y0, L, [&](int j) { return ph[j]; });
         __syncthreads();
         for_own(op, x0, y0, [&](int lx, int ly, int i) {
-            const float a = apply13(A, op.N, i, L, lx, ly, idf);
+            const float a = f.unitd ? apply13u(A, op.N, i, L, lx, ly) : apply13(A, op.N, i, L, lx, ly, idf);
             V.v[i] = a;
             acc[0] += (double)V.rh[i] * a;
         });
@@ -451,7 +464,7 @@ __global__ void __launch_bounds__(TB) k_f_at(const Film one, const Film *__restr
         stage(op, x0, y0, L, [&](int j) { return sh[j]; });
         __syncthreads();
         for_own(op, x0, y0, [&](int lx, int ly, int i) {
-            const float a = apply13(A, op.N, i, L, lx, ly, idf);
+            const float a = f.unitd ? apply13u(A, op.N, i, L, lx, ly) : apply13(A, op.N, i, L, lx, ly, idf);
             V.t[i] = a;
             acc[0] += (double)a * a;
             acc[1] += (double)a * V.s[i];
@@ -528,6 +541,7 @@ int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, cons
         Krylov<float> &kr = s.kr;
         Film &d = F[f];
         d.op = s.op0;
+        d.unitd = s.unit_diag;
         d.A = s.A; d.Y = s.Mh; d.ms = s.Mscale; d.b = b[f];
         d.V.x = x[f]; d.V.r = kr.r; d.V.rh = kr.rh; d.V.p0 = kr.p; d.V.p1 = kr.p2; d.V.ph = kr.ph;
         d.V.v = kr.v; d.V.s = kr.s; d.V.sh = kr.sh; d.V.t = kr.t;

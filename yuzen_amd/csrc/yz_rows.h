@@ -43,6 +43,7 @@ struct RowArgs {
     float *feat;               // optional: 8 x N features of the learned preconditioner
     float *mscale;             // optional: 1 / (1 + sum_k!=c |IA_k|) of the scaled row
     unsigned long long *rmax;  // mode 1: max |Rhs| (bits of a non-negative double)
+    int *nonunit;              // FP32 planes: set when a row's stored diagonal is not exactly 1
 };
 
 __device__ inline double visc_of(const RowArgs &a, double cc) { return exp(a.visc_exp * (cc - a.visc_c0)); }
@@ -403,6 +404,12 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
             if (A.ia32) {
 #pragma unroll
                 for (int k = 0; k < NS; ++k) A.ia32[(size_t)k * g.N + i] = (float)(A.row_scaling ? IA[k] * inv : IA[k]);
+                // a finite diagonal scaled by its own inverse is 1 within an
+                // FP64 ulp, so exactly 1 in FP32: the fused FP32 iteration may
+                // then skip that plane (the flag tells it when it may not)
+                const bool unit = A.row_scaling && fabs(dg) >= 1.0e-30 && isfinite(dg);
+                if (unit) A.ia32[(size_t)C0 * g.N + i] = 1.0f;
+                else if (A.nonunit) atomicOr(A.nonunit, 1);
                 A.b32[i] = (float)(A.row_scaling ? R * inv : R);
             } else {
 #pragma unroll

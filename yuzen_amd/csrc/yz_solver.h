@@ -94,6 +94,9 @@ struct SolverT {
     double stall_factor = 0.5;
     // replay the fused iteration from a captured HIP graph
     int use_graph = 0;
+    // every row's diagonal of A is stored as exactly 1 (row-scaled FP32
+    // planes): the fused iteration does not read that plane
+    int unit_diag = 0;
 };
 
 typedef SolverT<double> Solver;

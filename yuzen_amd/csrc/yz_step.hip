@@ -388,6 +388,8 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
         if (c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG) { S->ra.feat = S->feat; S->ra.mscale = S->mscale; }
     }
     YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+    YZ_TRY(hipMemsetAsync(S->dflag, 0, sizeof(int), S->st));
+    if (f32) S->ra.nonunit = S->dflag;
     YZ_TRY(hipEventRecord(S->ev[0], S->st));
     rows_launch(S, 1, S->h);
     S->ra = keep;
@@ -395,6 +397,7 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     YZ_TRY(hipEventRecord(S->ev[1], S->st));
     ++S->nk_pass;
     if (sync_read(S)) { *dev_err = 1; return 2; }
+    if (f32) S->S32.unit_diag = S->hflag[0] == 0;
     S->tms[0] += ev_ms(S->ev[0], S->ev[1]);
     const double eps = ew_eps(S);
     YZ_TRY(hipEventRecord(S->ev[2], S->st));
@@ -1333,12 +1336,15 @@ static int bf_assemble(BatchFilm &F) {
     S->ra.hold = S->hold;
     RowArgs keep = S->ra;
     S->ra.ia32 = S->S32.A; S->ra.b32 = S->b32; S->ra.feat = S->feat; S->ra.mscale = S->mscale;
+    S->ra.nonunit = S->dflag;
     YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
+    YZ_TRY(hipMemsetAsync(S->dflag, 0, sizeof(int), S->st));
     rows_launch(S, 1, S->h);
     S->ra = keep;
     YZ_TRY(hipGetLastError());
     ++S->nk_pass;
     YZ_TRY(hipMemcpyAsync(S->hmax, S->dmax, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(S->hflag, S->dflag, sizeof(int), hipMemcpyDeviceToHost, S->st));
     return 0;
 }
 
@@ -1502,6 +1508,7 @@ int yz_batch_advance(yz_batch *B, int *status, int *nk_iter, int *lin_iter) {
             BatchFilm &F = B->f[who[u]];
             yz_step *S = F.S;
             F.eps = ew_eps(S);
+            S->S32.unit_diag = S->hflag[0] == 0;
             if (mlp_infer_half(B->mlp, S->g.N, S->feat, S->Yh, B->st)) { set_err("inference failed"); return 2; }
             S->S32.Mh = S->Yh;
             S->S32.Mscale = S->mscale;
