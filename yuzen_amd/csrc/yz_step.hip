@@ -177,6 +177,7 @@ struct yz_step {
     int tsc_count = 0;
     bool extrap_blocked = false, first_step = true;
     double extrap_move = 0, ew_prev = 0, ew_eta = 0;
+    double nk_first = 0, nk_prev = 0;
     int nk_pass = 0;
     long long lin_step = 0;
     hipEvent_t ev[12];
@@ -316,6 +317,25 @@ static bool fp32_path(const yz_case &c) {
 // the FP32 inner solve cannot reach below this relative residual
 static constexpr double kEpsFP32 = 1.0e-6;
 
+// the FP32 inner solve's tolerance (no reference counterpart): EW's,
+// floored at kEpsFP32, and not tighter than the Newton exit can see. Once
+// the previous correction prev is within a few decades of the exit
+// threshold (NewtonEpsilon, or NewtonEpsilonRel x the first correction),
+// the solve only needs an error below 1 % of that threshold relative to the
+// correction sought, so its relative residual may be 0.01 thr / prev (at
+// most EW's cap 1e-4): the last pass of a step no longer runs to 1e-6 for a
+// correction the exit test then finds 10^3 times below its threshold (the
+// oversolving safeguard of inexact Newton methods)
+static double fp32_tol(const yz_case &c, double eps, double first, double prev) {
+    const double kEtaMax = 1.0e-4;
+    double t = eps > kEpsFP32 ? eps : kEpsFP32;
+    if (prev > 0) {
+        const double thr = std::max(c.nk_eps, c.nk_eps_rel > 0 ? c.nk_eps_rel * first : 0.0);
+        t = std::max(t, std::min(kEtaMax, 0.01 * thr / prev));
+    }
+    return t;
+}
+
 // whether each pass's solve starts from zero rather than from the previous
 // pass's correction (zero_guess -1: the reference's guess on the FP64 path,
 // zero on the FP32 inner solve)
@@ -434,7 +454,7 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
         if (!rc && c.precond == YZ_PC_JACOBI) rc = jacobi_setup(S->S32);
         if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
         YZ_TRY(hipEventRecord(S->ev[3], S->st));
-        const double eps32 = eps > kEpsFP32 ? eps : kEpsFP32;
+        const double eps32 = fp32_tol(c, eps, S->nk_first, S->nk_prev);
         if (zero_guess(c)) YZ_TRY(hipMemsetAsync(S->corr32, 0, S->g.N * sizeof(float), S->st));
         if (c.precond == YZ_PC_LEARNED)
             ch = bicgstab_stencil(S->S32, (const float *)S->b32, S->corr32, c.lin_maxit, eps32, iter, resid);
@@ -660,6 +680,7 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
         YZ_TRY(hipGetLastError());
     }
     double resid_eq_old = 1e40, resid_first = 0, resid_best = 1e40, resid_eq = 0;
+    S->nk_first = S->nk_prev = 0;
     int stall = 0;
     const int stall_limit = 5;
     bool done = false;
@@ -683,7 +704,8 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
         if (std::getenv("YZ_DEBUG"))
             std::fprintf(stderr, "[hip] pass %d lin %d resid_lin %.3e corr %.3e rc %d\n", outer, it, lr, resid_eq, ch);
         if (ch != 0) { done = false; break; }
-        if (outer == 0) resid_first = resid_eq;
+        if (outer == 0) resid_first = S->nk_first = resid_eq;
+        S->nk_prev = resid_eq;   // the FP32 inner solve's tolerance reads both
         if (resid_eq < resid_best * 0.9) { resid_best = resid_eq; stall = 0; }
         else stall++;
         if (stall > stall_limit) {
@@ -1514,7 +1536,7 @@ int yz_batch_advance(yz_batch *B, int *status, int *nk_iter, int *lin_iter) {
             S->S32.Mscale = S->mscale;
             S->S32.stall_window = S->cs.stall_window;
             sv[u] = &S->S32; bv[u] = S->b32; xv[u] = S->corr32;
-            tol[u] = F.eps > kEpsFP32 ? F.eps : kEpsFP32;
+            tol[u] = fp32_tol(S->cs, F.eps, F.resid_first, F.outer > 0 ? F.resid_eq : 0.0);
             if (zero_guess(S->cs)) YZ_TRY(hipMemsetAsync(S->corr32, 0, S->g.N * sizeof(float), B->st));
         }
         if (bicgstab_stencil_batch(B->table, m, sv.data(), bv.data(), xv.data(), B->f[who[0]].S->cs.lin_maxit,
