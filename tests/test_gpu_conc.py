@@ -101,7 +101,9 @@ def _pair(c, h, s, cc, dt, steps, label, bar=1e-10, counts=True):
         assert rc <= bar, f"step {k}: concentration {rc:.2e}"
         so, sg = o.conc_stats(), g.conc_stats()
         moved += sg[3] + sg[4]
-        assert abs(so[3] - sg[3]) <= 1e-12 * max(so[3], 1e-300) + 1e-300
+        # the excess moved is a sum of differences of near-equal values: it
+        # carries the film's rounding, so it is held to the film's bar
+        assert abs(so[3] - sg[3]) <= bar * max(so[3], 1e-300) + 1e-300
     st = g.conc_stats()
     print(f"{label}: {steps} steps, worst film {worst_h:.2e}, concentration {worst_c:.2e}; "
           f"device solves {st[0]:.0f}, iterations {st[1]:.0f}, multigrid {st[2]:.0f}, moved {moved:.3e}")

@@ -25,6 +25,10 @@ struct RowArgs {
     int cap, vg, pg, cf, ma, dj;
     double Bo, Pgx, Pgy, Sp, AsMa2, Dp, hstar, pn, pm, rotx, roty;
     double visc_c0, visc_exp, sig_c0, Ma_eff;
+    // 1/dx, 1/dy, 1/dx^2, 1/dy^2 (formed once on the host), and the
+    // disjoining exponents n, m as whole numbers when they are (0: not)
+    double inv_h[2], inv_h2[2];
+    int pn_i, pm_i;
     Mob mob;
     double Ccap[4][13];
     double theta, dt;
@@ -47,6 +51,14 @@ struct RowArgs {
 };
 
 __device__ inline double visc_of(const RowArgs &a, double cc) { return exp(a.visc_exp * (cc - a.visc_c0)); }
+// r^e: repeated products when e is a small whole number k (the usual
+// disjoining exponents 3 and 2), pow otherwise
+__device__ inline double pow_k(double r, double e, int k) {
+    if (k <= 0) return pow(r, e);
+    double p = r;
+    for (int j = 1; j < k; ++j) p *= r;
+    return p;
+}
 __device__ inline double sig_of(const RowArgs &a, double cc) { return 1.0 + a.Ma_eff * (cc - a.sig_c0); }
 
 // the third difference of v across face d (LaplacePressure_2D.cpp:105-128)
@@ -198,8 +210,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
                 const double gf = mob.slip ? Fm[d] : ((ho > 0 && ha > 0) ? 2.0 * (ho * ho) * (ha * ha) / (ho + ha) : 0.0);
                 Gf[d] = gf / mu;
                 Tk[d] = (gf == 0) ? 0.0 : gf / (ho + ha) / mu;
-                const double hs = spacing(d / 2);
-                inv2[d] = 1.0 / (hs * hs);
+                inv2[d] = A.inv_h2[d / 2];
                 if (MODE == 1) {
                     if (mob.slip) { gCf[d] = Fm1[d] / mu; gAf[d] = Fm2[d] / mu; }
                     else if (ho > 0 && ha > 0) {
@@ -267,13 +278,13 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
                 else dv = (g.ylo + (iyp + 1) * g.dy) - A.roty;
                 const double w = (mode == 0) ? 1.0 : dv;
                 const double Coef = (mode == 0) ? ((j < 2 ? A.Pgx : A.Pgy) / 3.0) : A.Sp / 3.0;
-                const double hs = spacing(j / 2);
+                const double ihs = A.inv_h[j / 2];
                 double f[4], wt[4], gd[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) f[q] = H[face_slot(DIM, j, q)];
                 const double hf = limited_face(f, dv, wt, gd);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) { wt[q] /= hs; gd[q] /= hs; }
+                for (int q = 0; q < 4; ++q) { wt[q] *= ihs; gd[q] *= ihs; }
                 double D = (mob.slip ? mob.DperH(hf) : hf * hf) / muF[j];
                 if (wall(j)) D = 0;
                 const double sg = (j % 2 == 0) ? -1.0 : 1.0;
@@ -300,8 +311,8 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 const int lo = face_slot(DIM, d, 1), hi = face_slot(DIM, d, 2);
-                const double hh = spacing(d / 2);
-                const double dc = (Cc[hi] - Cc[lo]) / hh;
+                const double ihh = A.inv_h[d / 2];
+                const double dc = (Cc[hi] - Cc[lo]) * ihh;
                 const double up = sgn(Coef) * sgn(dc);
                 double f[4], wt[4], gd[4];
 #pragma unroll
@@ -311,17 +322,18 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
                 const double mu = muF[d];
                 const double Sf = wl ? 0.0 : (mob.slip ? dc / mu * (hf + 2.0 * mob.lambda) : dc / mu * hf);
                 const double sg = (d % 2 == 0) ? -1.0 : 1.0;
+                const double Sfh = Sf * ihh;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) Ar[face_slot(DIM, d, q)] += Coef * (sg * wt[q] * Sf / hh);
+                for (int q = 0; q < 4; ++q) Ar[face_slot(DIM, d, q)] += Coef * (sg * wt[q] * Sfh);
                 if (MODE == 1) {
                     const double r = wl ? 0.0 : dc / mu;
                     if (mob.slip) {
-                        const double L2 = 2.0 * mob.lambda, ad = sg / hh;
+                        const double L2 = 2.0 * mob.lambda, ad = sg * ihh;
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
                             Jc[face_slot(DIM, d, q)] += Coef * (r * ((2.0 * hf + L2) * gd[q] - (hf + L2) * wt[q]) * ad);
                     } else {
-                        const double ad = sg * hf / hh;
+                        const double ad = sg * hf * ihh;
 #pragma unroll
                         for (int q = 0; q < 4; ++q) Jc[face_slot(DIM, d, q)] += Coef * (r * (2.0 * gd[q] - wt[q]) * ad);
                     }
@@ -333,21 +345,23 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
         if (A.dj) {
             const double Coef = A.Dp / 3.0;
             // (h*/h)^n and (h*/h)^m once per point; Pi and dPi from them
-            double PiS[NS], dPiS[NS];
+            double PiS[NS], dPiS[NS], PiH[NS];
 #pragma unroll
-            for (int k = 0; k < NS; ++k) { PiS[k] = 0; dPiS[k] = 0; }
+            for (int k = 0; k < NS; ++k) { PiS[k] = 0; dPiS[k] = 0; PiH[k] = 0; }
 #pragma unroll
             for (int d = -1; d < ND; ++d) {
                 const int k = d < 0 ? C0 : adj_slot(DIM, d);
                 const double hh = H[k];
                 if (hh > 0) {
-                    const double r = A.hstar / hh;
-                    const double rn = pow(r, A.pn), rm = pow(r, A.pm);
+                    const double ih = 1.0 / hh;
+                    const double r = A.hstar * ih;
+                    const double rn = pow_k(r, A.pn, A.pn_i), rm = pow_k(r, A.pm, A.pm_i);
                     PiS[k] = rn - rm;
-                    dPiS[k] = -(A.pn * rn - A.pm * rm) / hh;
+                    dPiS[k] = -(A.pn * rn - A.pm * rm) * ih;
+                    PiH[k] = PiS[k] * ih;
                 }
             }
-            auto PiPerH = [&](int k) { return H[k] <= 0 ? 0.0 : PiS[k] / H[k]; };
+            auto PiPerH = [&](int k) { return PiH[k]; };
             double cen = 0;
 #pragma unroll
             for (int d = 0; d < ND; ++d) cen += (-inv2[d]) * Gf[d];

@@ -896,6 +896,10 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
     a.visc_c0 = c->visc_c0; a.visc_exp = c->visc_exp; a.sig_c0 = c->sig_c0;
     a.Ma_eff = c->marangoni ? c->Ma : 0.0;
     a.mob.lambda = c->slip > 0 ? c->slip : 0.0; a.mob.a = 3.0 * a.mob.lambda; a.mob.slip = c->slip > 0;
+    a.inv_h[0] = 1.0 / g.dx; a.inv_h[1] = 1.0 / g.dy;
+    a.inv_h2[0] = 1.0 / (g.dx * g.dx); a.inv_h2[1] = 1.0 / (g.dy * g.dy);
+    auto whole = [](double e) { return (e >= 1.0 && e <= 16.0 && e == std::floor(e)) ? (int)e : 0; };
+    a.pn_i = whole(c->dis_n); a.pm_i = whole(c->dis_m);
     build_ccap(g, a.Ccap);
     a.theta = c->theta;
     a.row_scaling = c->row_scaling;
