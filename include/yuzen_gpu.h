@@ -321,6 +321,72 @@ int yz_linsolve(yz_linctx *ctx, int n, int nnz, const int *rowptr,
 int yz_memory_get(yz_linctx *ctx, double *buf, int cap);
 int yz_memory_set(yz_linctx *ctx, const double *buf, int len);
 
+
+/* ------------------------------------------------------------------ */
+/* The ConcZ step: a 2-D film whose concentration is resolved through   */
+/* its thickness on a zeta grid that stretches with the film. One call  */
+/* is one time step: CZ_HeightEquation_2D::TimeAdvance (a predictor and */
+/* a midpoint stage of the semi-implicit film, CZ_HeightEquation_2D.cpp */
+/* :1949-2020) and AfterTimeStepTrial (:207-386: the conservative       */
+/* update, the velocity through the film, the concentration step and    */
+/* the coupling passes, RedistributeExcess and RedistributeDeficit),    */
+/* with CZ_ConcEquation_2D.cpp for the concentration. The film is 2-D   */
+/* (Grid.Dimension "xyh"), the step fixed (no step control), the        */
+/* concentration scheme unlimited; the create call refuses the rest.    */
+/* ------------------------------------------------------------------ */
+enum { YZ_EVAP_RAOULT = 0, YZ_EVAP_POWERN = 1 };
+
+typedef struct yz_cz_case {
+    int nx, ny, nz;                 /* real cells: Grid.Nx, Ny, Nz */
+    double xmin, xmax, ymin, ymax;  /* nondimensional */
+    int bc[4];                      /* YZ_BC_SYMMETRY or YZ_BC_PERIODIC */
+    int capillary, vertical_gravity, inplane_gravity, centrifugal, marangoni, evaporation;
+    double As, Ma, Bo, Pgx, Pgy, Sp, rot_x, rot_y;
+    double sig_c0;                  /* sigma(c) = 1 + Ma (c - sig_c0) */
+    double visc_c0, visc_exp;       /* mu(c) = exp(visc_exp (c - visc_c0)) */
+    double Df, diff_c0, diff_exp;   /* D(c) = exp(diff_exp (c - diff_c0)) */
+    double Ev, evap_coef;           /* E = Ev evap_coef f(c_s) g(h) (Evaporation.cpp:62-103) */
+    int evap_func;                  /* YZ_EVAP_RAOULT (default) or YZ_EVAP_POWERN */
+    double evap_power, evap_concr;  /* PowerN: max(1 - c - concr, 0)^power */
+    double molar_resin, molar_solvent;  /* Raoult: Fluid.MolarMass (2200, 106) */
+    double precursor;               /* Initial.SurfaceShape.PrecursorRatio (0: no ramp) */
+    int conc_scheme;                /* YZ_CONV_*: Numerics.Convection.Concentration (UTOPIA) */
+    int lin_maxit;                  /* LinearSolver.MaxIterations of both (5000) */
+    double h_eps, c_eps;            /* LinearSolver.Epsilon of the film and the concentration (1e-13) */
+} yz_cz_case;
+
+void yz_cz_case_defaults(yz_cz_case *c);
+
+/* h, s: the film and the substrate with their ghost columns, (nx+4)(ny+4)
+ * rows, row = (nx+4) iy + ix; conc: every cell of the 3-D grid with its
+ * ghosts, (nx+4)(ny+4)(nz+4) rows, row = (nx+4)(ny+4) kz + (nx+4) iy + ix
+ * (kz = 2 .. nz+1 the real levels, from the substrate up) */
+typedef struct yz_cz yz_cz;
+yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s,
+                    const double *conc, double dt);
+/* one step. Returns 0; 1 the film's solve did not converge, 2 the
+ * concentration's, 3 the coupling passes did not settle (the reference
+ * stops a run with a fixed step on each: FailedFixedStep), or -1 on a
+ * device error (yz_last_error). The state is that of the step's start
+ * after a failure. lin_iter: linear iterations of the step (film and
+ * concentration), passes: coupling passes taken again, resid: the last
+ * film solve's relative residual. */
+int yz_cz_advance(yz_cz *st, int *lin_iter, int *passes, double *resid);
+void yz_cz_get_h(yz_cz *st, double *h);
+void yz_cz_get_c(yz_cz *st, double *conc);
+/* the volume fluxes of the last concentration solve (BuildVolumeFluxes):
+ * qx through x face ix (between ix-1 and ix) of level kz, qy through y face
+ * iy, w through the zeta face below cell kz; each (nx+4)(ny+4)(nz+4) */
+void yz_cz_volume_fluxes(yz_cz *st, double *qx, double *qy, double *w);
+/* counters since create: [0] film solves, [1] their iterations, [2]
+ * concentration solves, [3] their iterations, [4] coupling passes taken
+ * again, [5] solute moved by RedistributeExcess, [6] moved and [7] made up
+ * by RedistributeDeficit (c times cells, summed) */
+void yz_cz_stats(yz_cz *st, double *out8);
+void yz_cz_destroy(yz_cz *st);
+/* sizeof(yz_cz_case), for bindings to check their mirror */
+int yz_sizeof_cz_case(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from yuzen_amd.capi import YzCase
+from yuzen_amd.capi import YzCase, YzCzCase
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -50,6 +50,18 @@ def load():
     lib.yzo_conc_face.argtypes = [C.c_int, pd, C.c_double, pd]
     lib.yzo_conc_nnz.argtypes = [C.c_void_p]
     lib.yzo_conc_system.argtypes = [C.c_void_p, pi_, pi_, pd, pd]
+    lib.yzo_cz_case_defaults.argtypes = [P(YzCzCase)]
+    lib.yzo_cz_create.restype = C.c_void_p
+    lib.yzo_cz_create.argtypes = [P(YzCzCase), pd, pd, pd, C.c_double]
+    lib.yzo_cz_destroy.argtypes = [C.c_void_p]
+    lib.yzo_cz_advance.argtypes = [C.c_void_p, pi_, pi_, pd]
+    lib.yzo_cz_get_h.argtypes = [C.c_void_p, pd]
+    lib.yzo_cz_get_c.argtypes = [C.c_void_p, pd]
+    lib.yzo_cz_volume_fluxes.argtypes = [C.c_void_p, pd, pd, pd]
+    lib.yzo_cz_stats.argtypes = [C.c_void_p, pd]
+    lib.yzo_cz_velocity.argtypes = [C.c_void_p, pd, pd, pd, pd]
+    lib.yzo_cz_redistribute.restype = C.c_double
+    lib.yzo_cz_redistribute.argtypes = [C.c_void_p, pd, pd, pd, pd]
     _lib = lib
     return lib
 
@@ -175,3 +187,64 @@ def linsolve(rowptr, col, val, rhs, x0, precond=1, scaling=1, maxit=5000, eps=1e
     rc = lib.yzo_linsolve(n, len(col), _pi(rowptr), _pi(col), _pd(val), _pd(rhs), _pd(x),
                           precond, scaling, maxit, eps, C.byref(it), C.byref(r))
     return rc, x, it.value, r.value
+
+
+class OracleCZ:
+    """The oracle's ConcZ stepper over one film (yz_cz_oracle.cpp)."""
+
+    def __init__(self, case, h, s, c, dt):
+        self.lib = load()
+        self.case = case.copy()
+        self.n2 = (case.nx + 4) * (case.ny + 4)
+        self.n3 = self.n2 * (case.nz + 4)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        assert h.size == self.n2 and s.size == self.n2 and c.size == self.n3
+        self.p = self.lib.yzo_cz_create(C.byref(self.case), _pd(h), _pd(s), _pd(c), dt)
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.lib.yzo_cz_destroy(self.p)
+            self.p = None
+
+    def advance(self):
+        lin, ps, r = C.c_int(), C.c_int(), C.c_double()
+        rc = self.lib.yzo_cz_advance(self.p, C.byref(lin), C.byref(ps), C.byref(r))
+        return rc, lin.value, ps.value, r.value
+
+    def h(self):
+        out = np.empty(self.n2)
+        self.lib.yzo_cz_get_h(self.p, _pd(out))
+        return out
+
+    def c(self):
+        out = np.empty(self.n3)
+        self.lib.yzo_cz_get_c(self.p, _pd(out))
+        return out
+
+    def volume_fluxes(self):
+        qx, qy, w = np.empty(self.n3), np.empty(self.n3), np.empty(self.n3)
+        self.lib.yzo_cz_volume_fluxes(self.p, _pd(qx), _pd(qy), _pd(w))
+        return qx, qy, w
+
+    def stats(self):
+        out = np.empty(8)
+        self.lib.yzo_cz_stats(self.p, _pd(out))
+        return out
+
+    def velocity(self, h, c):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        ux, uy = np.empty(self.n3), np.empty(self.n3)
+        self.lib.yzo_cz_velocity(self.p, _pd(h), _pd(c), _pd(ux), _pd(uy))
+        return ux, uy
+
+    def redistribute(self, h, c):
+        """RedistributeExcess then RedistributeDeficit on a copy of c:
+        (c, excess moved, deficit moved, deficit made up)"""
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        c = np.array(c, dtype=np.float64)
+        mv, ad = C.c_double(), C.c_double()
+        ex = self.lib.yzo_cz_redistribute(self.p, _pd(h), _pd(c), C.byref(mv), C.byref(ad))
+        return c, ex, mv.value, ad.value

@@ -2164,3 +2164,7 @@ int yzo_linsolve(int n, int nnz, const int *rowptr, const int *col, const double
 }
 
 }  /* extern "C" */
+
+/* the ConcZ step, restated in its own file on the same CRS, BiCGSTAB and
+ * multigrid */
+#include "yz_cz_oracle.cpp"

@@ -29,6 +29,7 @@ def test_structure_sizes_match():
     lib = capi.load()
     assert lib.yz_sizeof_case() == C.sizeof(capi.YzCase)
     assert lib.yz_sizeof_lin_settings() == C.sizeof(capi.YzLinSettings)
+    assert lib.yz_sizeof_cz_case() == C.sizeof(capi.YzCzCase)
 
 
 def test_defaults_agree():
@@ -37,6 +38,19 @@ def test_defaults_agree():
     lib.yz_case_defaults(C.byref(c))
     d = capi.case_defaults()
     for name, _ in capi.YzCase._fields_:
+        a, b = getattr(c, name), getattr(d, name)
+        if name == "bc":
+            assert list(a) == list(b)
+        else:
+            assert a == b, name
+
+
+def test_cz_defaults_agree():
+    lib = capi.load()
+    c = capi.YzCzCase()
+    lib.yz_cz_case_defaults(C.byref(c))
+    d = capi.cz_case_defaults()
+    for name, _ in capi.YzCzCase._fields_:
         a, b = getattr(c, name), getattr(d, name)
         if name == "bc":
             assert list(a) == list(b)

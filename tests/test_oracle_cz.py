@@ -1,0 +1,237 @@
+"""The ConcZ oracle (oracle/yz_cz_oracle.cpp) pinned without the reference
+build: the reference's ConcZ equations cannot be compiled here (every
+equation object holds a LinearSolver, whose constructor needs the
+Eigen/AMGCL backend unit), and its tests hold no ConcZ fixture. Each check
+below is fixed by the mathematics, so that a dropped term, a wrong sign or
+index, or a transposed axis fails one of them:
+
+* the capillary operator: a small cosine mode of a flat film decays per
+  step by the Crank-Nicolson factor of the discrete symbol of grad^2 grad^2
+  (x, y and the cross term) with the mobility h^3 / 3 mu;
+* the diffusion through the film: a cosine of zeta through a flat film
+  decays by the Crank-Nicolson factor of the cell-centred Neumann
+  Laplacian, Df As / h^2 4 sin^2(pi dzeta / 2) / dzeta^2, which also pins
+  the no-flux ghost rows at the substrate and the free surface;
+* the velocity through the film (Simpson's rule of CalVelocity): the
+  in-plane gravity profile Pgx h^2 (zeta - zeta^2 / 2) / mu and the
+  Marangoni shear As Ma c_x h zeta / mu, exact for a uniform viscosity;
+* the volume fluxes obey the discrete geometric conservation law cell by
+  cell and add up to the film's flux face by face;
+* a uniform concentration stays uniform; the solute of a closed film is
+  kept to rounding, with evaporation, Marangoni flow and the coupling
+  passes; the volume falls by what evaporates; a flat film evaporating
+  at a constant rate thins as h0 - E t with its column solute kept;
+* swapping x and y transposes the result (the y faces are not a copy of
+  the x faces in the code);
+* RedistributeExcess and RedistributeDeficit on small columns with
+  hand-computed answers.
+"""
+import numpy as np
+import pytest
+
+from yuzen_amd import capi
+from oracle import binding as ob
+from tests.cz_cases import _allphys, _case, _grid, _real2, _real3, _solute
+
+
+def test_capillary_mode_decays_by_the_discrete_symbol():
+    n, L, eps, dt = 16, 4.0, 1e-7, 0.05
+    c = _case(n=n, L=L)
+    x = _grid(n, L)
+    X, Y = np.meshgrid(x, x)
+    kx, ky = 2 * np.pi / L, 2 * 2 * np.pi / L
+    mode = np.cos(kx * X) * np.cos(ky * Y)
+    h = (1.0 + eps * mode).ravel()
+    cc = np.full((c.nz + 4) * (n + 4) ** 2, 0.3)
+    o = ob.OracleCZ(c, h, np.zeros_like(h), cc, dt)
+    assert o.advance()[0] == 0
+    dx = L / n
+    ax = (2 - 2 * np.cos(kx * dx)) / dx ** 2
+    ay = (2 - 2 * np.cos(ky * dx)) / dx ** 2
+    lam = (1.0 / 3.0) * 1.0 * (ax + ay) ** 2      # D sigma (ax + ay)^2, D = h^3 / 3 mu
+    g = (1 - 0.5 * dt * lam) / (1 + 0.5 * dt * lam)
+    dh = _real2(c, o.h()) - 1.0
+    m = _real2(c, mode)
+    amp = (dh * m).sum() / (m * m).sum()
+    assert abs(amp / eps - g) < 1e-5, (amp / eps, g)
+    # the rest is the O(eps^2) of the mobility
+    assert np.abs(dh - amp * m).max() < 1e-3 * eps
+
+
+def test_diffusion_through_the_film_decays_by_the_neumann_symbol():
+    n, nz, dt, a = 8, 16, 0.5, 1e-3
+    c = _case(n=n, nz=nz, capillary=0, Df=2.0, As=0.01)
+    dz = 1.0 / nz
+    kz = np.arange(nz + 4)
+    zeta = dz * (kz - 1.5)
+    for m in (1, 3):
+        prof = np.cos(np.pi * m * zeta)
+        cc = (0.3 + a * prof[:, None, None] * np.ones((1, n + 4, n + 4))).ravel()
+        h = np.ones((n + 4) ** 2)
+        o = ob.OracleCZ(c, h, np.zeros_like(h), cc, dt)
+        assert o.advance()[0] == 0
+        lam = c.Df * c.As / 1.0 ** 2 * 4 * np.sin(np.pi * m * dz / 2) ** 2 / dz ** 2
+        g = (1 - 0.5 * dt * lam) / (1 + 0.5 * dt * lam)
+        got = _real3(c, o.c())[:, 3, 5] - 0.3
+        want = a * g * prof[2:-2]
+        assert np.abs(got - want).max() < 1e-13, (m, np.abs(got - want).max())
+
+
+def test_velocity_profiles_are_simpsons_exact_ones():
+    n, L, nz = 10, 5.0, 8
+    c = _case(n=n, L=L, nz=nz, per=False, capillary=0, inplane_gravity=1, Pgx=0.7, Pgy=-0.4)
+    x = _grid(n, L)
+    X, Y = np.meshgrid(x, x)
+    h = 1.0 + 0.2 * np.sin(X) + 0.1 * np.cos(0.7 * Y)
+    cc3 = np.full((nz + 4, n + 4, n + 4), 0.3)
+    o = ob.OracleCZ(c, h.ravel(), np.zeros(h.size), cc3.ravel(), 0.01)
+    ux, uy = o.velocity(h.ravel(), cc3.ravel())
+    ux = ux.reshape(nz + 4, n + 4, n + 4)
+    uy = uy.reshape(nz + 4, n + 4, n + 4)
+    dz = 1.0 / nz
+    zeta = dz * (np.arange(nz + 4) - 1.5)
+    prof = zeta - zeta ** 2 / 2
+    for kz in range(2, nz + 3):
+        for jy in range(2, n + 2):
+            for ix in range(3, n + 2):      # inner x faces (the walls carry none)
+                hf = 0.5 * (h[jy, ix - 1] + h[jy, ix])
+                assert abs(ux[kz, jy, ix] - 0.7 * hf ** 2 * prof[kz]) < 1e-13
+                hf = 0.5 * (h[ix - 1, jy] + h[ix, jy])
+                assert abs(uy[kz, ix, jy] - (-0.4) * hf ** 2 * prof[kz]) < 1e-13
+        assert ux[kz, 4, 2] == 0.0 and ux[kz, 4, n + 2] == 0.0
+    # Marangoni: a surface concentration linear in x
+    c2 = _case(n=n, L=L, nz=nz, per=False, capillary=0, marangoni=1, Ma=2.0, As=0.3)
+    cc3 = np.empty((nz + 4, n + 4, n + 4))
+    cc3[:] = 0.3 + 0.01 * X[None, :, :]
+    o2 = ob.OracleCZ(c2, h.ravel(), np.zeros(h.size), cc3.ravel(), 0.01)
+    ux, _ = o2.velocity(h.ravel(), cc3.ravel())
+    ux = ux.reshape(nz + 4, n + 4, n + 4)
+    for kz in range(2, nz + 3):
+        for ix in range(4, n + 1):          # the 4-point derivative sees no ghost
+            hf = 0.5 * (h[5, ix - 1] + h[5, ix])
+            want = 0.3 * 2.0 * 0.01 * hf * dz * (kz - 1.5)
+            assert abs(ux[kz, 5, ix] - want) < 1e-14
+
+
+@pytest.mark.parametrize("per", [True, False])
+def test_solute_kept_and_volume_falls_by_the_evaporation(per):
+    c, h, s, cc = _allphys(per=per)
+    dt = 0.02
+    o = ob.OracleCZ(c, h, s, cc, dt)
+    sol0 = _solute(c, h, cc)
+    v = _real2(c, h).sum()
+    for k in range(4):
+        rc, lin, passes, _ = o.advance()
+        assert rc == 0
+        H, C3 = o.h(), o.c()
+        assert abs(_solute(c, H, C3) - sol0) <= 1e-13 * sol0
+        _, _, w = o.volume_fluxes()
+        evap = _real2(c, w.reshape(c.nz + 4, c.ny + 4, c.nx + 4)[c.nz + 2].ravel()).sum()
+        vn = _real2(c, H).sum()
+        assert abs((v - vn) - dt * evap) < 1e-13 * v, (v - vn, dt * evap)
+        v = vn
+    assert o.stats()[4] > 0      # the coupling passes ran
+
+
+def test_volume_fluxes_obey_the_geometric_conservation_law():
+    c, h, s, cc = _allphys(per=False)
+    dt = 0.02
+    o = ob.OracleCZ(c, h, s, cc, dt)
+    h0 = o.h()
+    assert o.advance()[0] == 0
+    qx, qy, w = o.volume_fluxes()
+    nx, ny, nz = c.nx + 4, c.ny + 4, c.nz + 4
+    qx, qy, w = (a.reshape(nz, ny, nx) for a in (qx, qy, w))
+    L = c.xmax - c.xmin
+    dx = dy = L / c.nx
+    dzeta = 1.0 / c.nz
+    # the last pass's dh/dt is that of the film the step ended with
+    ht = (o.h() - h0).reshape(ny, nx) / dt
+    worst = 0.0
+    for jy in range(2, ny - 2):
+        for ix in range(2, nx - 2):
+            for kz in range(2, nz - 2):
+                r = (dzeta * ht[jy, ix] + dzeta * (qx[kz, jy, ix + 1] - qx[kz, jy, ix]) / dx
+                     + dzeta * (qy[kz, jy + 1, ix] - qy[kz, jy, ix]) / dy + w[kz + 1, jy, ix] - w[kz, jy, ix])
+                worst = max(worst, abs(r))
+            assert w[2, jy, ix] == 0.0
+    assert worst < 1e-12, worst
+
+
+def test_uniform_concentration_stays_uniform():
+    c, h, s, cc = _allphys(per=False, )
+    c.evaporation = 0
+    cc = np.full_like(cc, 0.37)
+    o = ob.OracleCZ(c, h, s, cc, 0.02)
+    for k in range(3):
+        assert o.advance()[0] == 0
+    assert np.abs(_real3(c, o.c()) - 0.37).max() < 1e-14
+    assert not np.allclose(o.h(), h)     # the film moved
+
+
+def test_flat_film_thins_at_the_evaporation_rate():
+    n, nz, dt = 6, 8, 0.05
+    c = _case(n=n, nz=nz, evaporation=1, Ev=0.2, evap_func=capi.EVAP_POWERN, evap_power=0.0)
+    h = np.full((n + 4) ** 2, 1.0)
+    cc = np.full((nz + 4) * (n + 4) ** 2, 0.3)
+    o = ob.OracleCZ(c, h, np.zeros_like(h), cc, dt)
+    for k in range(1, 5):
+        assert o.advance()[0] == 0
+        hk = 1.0 - 0.2 * dt * k
+        assert np.abs(_real2(c, o.h()) - hk).max() < 1e-14
+        col = _real3(c, o.c()).mean(axis=0)
+        assert np.abs(col * hk - 0.3).max() < 1e-13
+    prof = _real3(c, o.c())[:, 2, 2]
+    assert prof[-1] > prof[0]       # the solute piles up under the surface
+
+
+def test_swapping_x_and_y_transposes_the_step():
+    c, h, s, cc = _allphys(per=False)
+    c.Pgy = c.Pgx
+    n, nz = c.nx, c.nz
+    h2 = h.reshape(n + 4, n + 4)
+    s2 = s.reshape(n + 4, n + 4)
+    c3 = cc.reshape(nz + 4, n + 4, n + 4)
+    o = ob.OracleCZ(c, h, s, cc, 0.02)
+    t = ob.OracleCZ(c, h2.T.copy().ravel(), s2.T.copy().ravel(), c3.transpose(0, 2, 1).copy().ravel(), 0.02)
+    for k in range(2):
+        assert o.advance()[0] == 0 and t.advance()[0] == 0
+    a = o.h().reshape(n + 4, n + 4)
+    b = t.h().reshape(n + 4, n + 4).T
+    assert np.abs(a - b).max() < 1e-12
+    a3 = o.c().reshape(nz + 4, n + 4, n + 4)
+    b3 = t.c().reshape(nz + 4, n + 4, n + 4).transpose(0, 2, 1)
+    assert np.abs(_real3(c, a3) - _real3(c, b3)).max() < 1e-12
+
+
+def test_redistribution_on_small_columns():
+    n, nz = 3, 4
+    c = _case(n=n, nz=nz, per=True)
+    N2, N3 = (n + 4) ** 2, (nz + 4) * (n + 4) ** 2
+    h = np.full(N2, 2.0)
+    o = ob.OracleCZ(c, h, np.zeros(N2), np.full(N3, 0.5), 0.01)
+
+    def col(a, ix, jy):
+        return a.reshape(nz + 4, n + 4, n + 4)[2:-2, jy, ix]
+
+    cc = np.full((nz + 4, n + 4, n + 4), 0.5)
+    # excess: the top cell over 1 goes down, what reaches the substrate
+    # goes back up into the cells with room
+    cc[2:-2, 3, 3] = [0.9, 0.95, 0.7, 1.3]
+    # deficit inside a column: filled from the column's own solute
+    cc[2:-2, 4, 2] = [0.5, -0.1, 0.3, 0.3]
+    out, ex, mv, ad = o.redistribute(h, cc.ravel())
+    assert np.allclose(col(out, 3, 3), [0.9, 0.95, 1.0, 1.0]), col(out, 3, 3)
+    assert abs(ex - 0.3) < 1e-15
+    f = 0.1 / 1.1
+    assert np.allclose(col(out, 2, 4), [0.5 - f * 0.5, 0.0, 0.3 - f * 0.3, 0.3 - f * 0.3])
+    assert abs(mv - 0.1 * 2.0 / 4) < 1e-15 and ad == 0.0
+    # a column with less than no solute takes it from its neighbours
+    cc = np.full((nz + 4, n + 4, n + 4), 0.5)
+    cc[2:-2, 3, 3] = -0.2
+    out, ex, mv, ad = o.redistribute(h, cc.ravel())
+    r = _real3(c, out)
+    assert abs(r[:, 1, 1]).max() == 0.0
+    sol = (2.0 * r.mean(axis=0)).sum()
+    assert abs(sol - (2.0 * (8 * 0.5 - 0.2))) < 1e-13
+    assert ad == 0.0

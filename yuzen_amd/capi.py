@@ -15,6 +15,7 @@ PC_JACOBI, PC_AMG, PC_LEARNED, PC_LEARNED_AMG = range(4)
 (CONV_UPWIND1, CONV_CENTRAL2, CONV_CENTRAL4, CONV_UPWIND2, CONV_UTOPIA, CONV_QUICK,
  CONV_KK) = range(7)
 LEARN_NPARAM, LEARN_NFEAT, LEARN_NOUT = 6288, 8, 13
+EVAP_RAOULT, EVAP_POWERN = 0, 1
 
 
 class YzCase(C.Structure):
@@ -99,6 +100,56 @@ def case_defaults():
     return c
 
 
+class YzCzCase(C.Structure):
+    """struct yz_cz_case: one ConcZ film (CZ_HeightEquation_2D,
+    CZ_ConcEquation_2D)."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("xmin", C.c_double), ("xmax", C.c_double),
+        ("ymin", C.c_double), ("ymax", C.c_double),
+        ("bc", C.c_int * 4),
+        ("capillary", C.c_int), ("vertical_gravity", C.c_int),
+        ("inplane_gravity", C.c_int), ("centrifugal", C.c_int),
+        ("marangoni", C.c_int), ("evaporation", C.c_int),
+        ("As", C.c_double), ("Ma", C.c_double), ("Bo", C.c_double),
+        ("Pgx", C.c_double), ("Pgy", C.c_double), ("Sp", C.c_double),
+        ("rot_x", C.c_double), ("rot_y", C.c_double),
+        ("sig_c0", C.c_double),
+        ("visc_c0", C.c_double), ("visc_exp", C.c_double),
+        ("Df", C.c_double), ("diff_c0", C.c_double), ("diff_exp", C.c_double),
+        ("Ev", C.c_double), ("evap_coef", C.c_double),
+        ("evap_func", C.c_int),
+        ("evap_power", C.c_double), ("evap_concr", C.c_double),
+        ("molar_resin", C.c_double), ("molar_solvent", C.c_double),
+        ("precursor", C.c_double),
+        ("conc_scheme", C.c_int),
+        ("lin_maxit", C.c_int),
+        ("h_eps", C.c_double), ("c_eps", C.c_double),
+    ]
+
+    def copy(self):
+        c = YzCzCase()
+        C.pointer(c)[0] = self
+        return c
+
+
+def cz_case_defaults():
+    """The reference's defaults for a ConcZ film (Input.cpp getin
+    defaults: Fluid.MolarMass 2200 / 106, Raoult evaporation with
+    coefficient 1, UTOPIA, LinearSolver MaxIterations 5000 and Epsilon
+    1e-13), as yz_cz_case_defaults sets them."""
+    c = YzCzCase()
+    c.nx, c.ny, c.nz = 16, 16, 8
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, 1.0, 0.0, 1.0
+    c.capillary = 1
+    c.As, c.sig_c0, c.visc_c0, c.Df, c.diff_c0 = 1.0, 0.5, 0.5, 1.0, 0.5
+    c.Ev, c.evap_coef, c.evap_func, c.evap_power = 1.0, 1.0, EVAP_RAOULT, 0.5
+    c.molar_resin, c.molar_solvent = 2200.0, 106.0
+    c.conc_scheme = CONV_UTOPIA
+    c.lin_maxit, c.h_eps, c.c_eps = 5000, 1e-13, 1e-13
+    return c
+
+
 _lib = None
 
 
@@ -152,6 +203,16 @@ def load():
     lib.yz_batch_get_h.argtypes = [C.c_void_p, i, pd]
     lib.yz_batch_stats.argtypes = [C.c_void_p, pd]
     lib.yz_batch_destroy.argtypes = [C.c_void_p]
+    lib.yz_cz_case_defaults.argtypes = [P(YzCzCase)]
+    lib.yz_cz_create.restype = C.c_void_p
+    lib.yz_cz_create.argtypes = [P(YzCzCase), pd, pd, pd, d]
+    lib.yz_cz_advance.argtypes = [C.c_void_p, pi_, pi_, pd]
+    lib.yz_cz_get_h.argtypes = [C.c_void_p, pd]
+    lib.yz_cz_get_c.argtypes = [C.c_void_p, pd]
+    lib.yz_cz_volume_fluxes.argtypes = [C.c_void_p, pd, pd, pd]
+    lib.yz_cz_stats.argtypes = [C.c_void_p, pd]
+    lib.yz_cz_destroy.argtypes = [C.c_void_p]
+    lib.yz_sizeof_cz_case.restype = C.c_int
     lib.yz_sizeof_case.restype = C.c_int
     lib.yz_sizeof_lin_settings.restype = C.c_int
     _lib = lib

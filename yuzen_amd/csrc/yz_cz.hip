@@ -1,0 +1,1875 @@
+// yz_cz.hip -- the ConcZ step on the device: a 2-D film whose
+// concentration is resolved through its thickness on a zeta grid that
+// stretches with the film.
+//
+// One yz_cz_advance is one time step of the reference's ConcZ model:
+//   CZ_HeightEquation_2D::Solve (CZ_HeightEquation_2D.cpp:1988-2020): the
+//     semi-implicit film solved twice, with the coefficients of the start
+//     of the step and again with those of the midpoint (SolveStage
+//     :1949-1963, SetMatrixA :750-1081, the ghost rows :1083-1142);
+//   AfterTimeStepTrial (:207-386): the conservative update through the
+//     face fluxes (StoreFaceFluxParts / ConservativeUpdate :1161-1301),
+//     the velocity through the film (CalVelocity :1381-1749, the in-plane
+//     components), the concentration step on the moving zeta grid
+//     (CZ_ConcEquation_2D.cpp: BuildVolumeFluxes :144-263, SetMatrixA
+//     :290-570, SetBoundaryCondition :588-732, SolveStage :755-893,
+//     ExplicitPart :895-913), the coupling passes with Aitken relaxation
+//     when the Marangoni term is on, RedistributeExcess and
+//     RedistributeDeficit (:388-570), and the ghosts of the end state
+//     (RefreshGhosts, ApplyBoundaryRows :985-1005).
+//
+// Layout in HBM (FP64 throughout): the film grid is (nx+4) x (ny+4) rows
+// with two ghost layers per side, row = Nx iy + ix; the concentration grid
+// adds (nz+4) levels, row = Nx Ny kz + Nx iy + ix, so every level is a
+// contiguous plane and a column is strided by Nx Ny. The linear systems are
+// held as planes over the real cells only (ghost rows are exact copies in
+// the increments solved for, and are folded into the stencil): 13 planes
+// for the film (the structured multigrid solver of yz_solver.hip), 21 for
+// the concentration, which is solved here by BiCGSTAB preconditioned with
+// the exact solve of each column's zeta coupling (a banded LU per column:
+// the diffusion through the film is the stiff direction).
+//
+// The kernels are HBM-bound stencil sweeps: one thread per cell, or per
+// column where the reference's recursion runs up the column (the Simpson
+// integral of the velocity, the zeta volume fluxes, the column solves and
+// the redistribution), with the level planes read coalesced across the
+// threads of a wave.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/yuzen_gpu.h"
+#include "yz_solver.h"
+
+namespace yz {
+void set_error(const std::string &s);
+}
+
+using namespace yz;
+
+namespace {
+
+constexpr int TBK = 256;
+constexpr int RB = 512;          // reduction blocks
+inline int nblk(long n) { return (int)std::min<long>((n + TBK - 1) / TBK, 65535L * 16); }
+
+#define CZ_CHECK(x)                                                            \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess) {                                                \
+            yz::set_error(std::string("HIP: ") + hipGetErrorString(e_));       \
+            return -1;                                                         \
+        }                                                                      \
+    } while (0)
+
+struct Geo3 {
+    int Nx, Ny, Nz, nx, ny, nz, n2, n3, nint;
+    double dx, dy, dz, xlo, ylo;
+    int wall[4];
+    int perx, pery;
+};
+
+__host__ __device__ inline int I2(const Geo3 &g, int i, int j) { return j * g.Nx + i; }
+__host__ __device__ inline int I3(const Geo3 &g, int i, int j, int k) { return k * g.n2 + j * g.Nx + i; }
+// the real index of a padded cell (ix, jy, kz all real)
+__host__ __device__ inline int IR(const Geo3 &g, int i, int j, int k) {
+    return ((k - 2) * g.ny + (j - 2)) * g.nx + (i - 2);
+}
+// the index a ghost index maps to along one axis (the film's ghost rows,
+// CZ_HeightEquation_2D.cpp:1083-1142; the same rule on the 3-D grid)
+__host__ __device__ inline int fold_pad(int i, int np, int per) {
+    const int m = np - 4;
+    if (i < 2) return per ? i + m : 3 - i;
+    if (i > np - 3) return per ? i - m : 2 * (np - 3) + 1 - i;
+    return i;
+}
+
+// slots of a film row as the reference stores them (the diagonal, then
+// the insertion order of SetMatrixA), and where each lands among the
+// solver's 13 slots (oM mm om pm Mo mo oo po Po mp op pp oP)
+__constant__ int c_hox[13] = {0, -2, -1, 1, 2, 0, 0, 0, 0, -1, -1, 1, 1};
+__constant__ int c_hoy[13] = {0, 0, 0, 0, 0, -2, -1, 1, 2, -1, 1, -1, 1};
+__constant__ int c_hslot[13] = {6, 4, 5, 7, 8, 0, 2, 10, 12, 1, 9, 3, 11};
+// slots of a concentration row as stored (the diagonal, then put()'s order,
+// CZ_ConcEquation_2D.cpp:535-556)
+__constant__ int c_cox[21] = {0, 0, -1, 0, 1, -2, -1, 1, 2, -1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+__constant__ int c_coy[21] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, -1, 1, -1, 1};
+__constant__ int c_coz[21] = {0, -2, -1, -1, -1, 0, 0, 0, 0, 1, 1, 1, 2, 0, 0, 0, 0, -1, -1, 1, 1};
+
+struct Args {
+    Geo3 g;
+    yz_cz_case cs;
+    double dt, theta;
+    double w1[4], w2[4];       // face weights of the scheme: c1, c2
+    const double *S;
+    double *hO, *hN, *fH, *tH, *fC, *tC;
+    double *Dx, *Dy, *sgx, *sgy, *Mx, *My;
+    double *qCapX, *qExpX, *qCapY, *qExpY, *qX, *qY, *colSrc, *dtdh;
+    double *Qx, *Qy, *W, *zr;  // zr: the z ghost right-hand sides, 4 x n2
+    double *hP, *hb, *hx;      // film system (solver planes), interior
+    double *cA, *react;        // concentration A (21 planes) and the surface term, interior
+    double *cP, *cb;           // row-scaled I + theta dt A, its right-hand side, interior
+    int *flags;                // [0] right-hand side not zero, [1] below 0, [2] columns, [3] left
+};
+
+__device__ inline double visc(const yz_cz_case &c, double x) { return 1.0 * exp(c.visc_exp * (x - c.visc_c0)); }
+__device__ inline double diffu(const yz_cz_case &c, double x) { return 1.0 * exp(c.diff_exp * (x - c.diff_c0)); }
+__device__ inline double sigm(const yz_cz_case &c, double x) { return 1.0 + c.Ma * (x - c.sig_c0); }
+__device__ inline double sgn(double v) { return v >= 0.0 ? 1.0 : -1.0; }
+
+// Evaluate (Evaporation.cpp:62-103, 148-220), uniform in space
+__device__ inline double evap_eval(const yz_cz_case &c, double hh, double cc) {
+    double f_conc;
+    if (c.evap_func == YZ_EVAP_POWERN) {
+        double v = 1.0 - cc - c.evap_concr;
+        v = fmax(v, 0.0);
+        f_conc = pow(v, c.evap_power);
+    } else {
+        const double x = (cc < 0.0) ? 0.0 : ((cc > 1.0) ? 1.0 : cc);
+        const double chi = (x / c.molar_resin) / (x / c.molar_resin + (1.0 - x) / c.molar_solvent);
+        f_conc = 1.0 - chi;
+    }
+    double f_mod = 1.0;
+    if (c.precursor != 0.0) {
+        const double v = (hh - c.precursor) / ((20.0 - 1.0) * c.precursor);
+        f_mod = fmax(0.0, fmin(v, 1.0));
+    }
+    return c.evap_coef * f_conc * 1.0 * f_mod;
+}
+__device__ inline double solvent_limit(double evp, double h, double c, double dt) {
+    if (!(evp > 0.0)) return evp;
+    const double cc = (c < 0.0) ? 0.0 : ((c > 1.0) ? 1.0 : c);
+    const double solvent = (h > 0.0) ? h * (1.0 - cc) : 0.0;
+    if (evp * dt > solvent) evp = solvent / dt;
+    return evp;
+}
+__device__ inline double column_mean(const Geo3 &g, const double *C, int ix, int jy) {
+    double s = 0.0;
+    for (int kz = 2; kz < g.Nz - 2; kz++) s += C[I3(g, ix, jy, kz)];
+    return s / (double)(g.Nz - 4);
+}
+__device__ inline double evap_rate(const Args &a, const double *H, const double *C, int ix, int jy) {
+    if (!a.cs.evaporation) return 0.0;
+    const Geo3 &g = a.g;
+    const double cc = 0.5 * (C[I3(g, ix, jy, g.Nz - 3)] + C[I3(g, ix, jy, g.Nz - 2)]);
+    return evap_eval(a.cs, H[I2(g, ix, jy)], cc);
+}
+__device__ inline bool wall_face(const Args &a, int ix, int jy, int dir) {
+    const Geo3 &g = a.g;
+    if (dir == 0) return (ix == 2 && g.wall[0]) || (ix == g.Nx - 2 && g.wall[1]);
+    return (jy == 2 && g.wall[2]) || (jy == g.Ny - 2 && g.wall[3]);
+}
+
+// ---------------------------------------------------------------------
+// film: the face quantities of the stage (FaceMobility with its depth
+// mean of 1/mu, the surface tension of the face, MarangoniFlux) and the
+// face flux parts (StoreFaceFluxParts), one thread per cell for the
+// x face on its low side and the y face below it
+// ---------------------------------------------------------------------
+__device__ void face_values(const Args &a, int ix, int jy, int dir, double &D, double &sg, double &M) {
+    const Geo3 &g = a.g;
+    const int ax = dir == 0, ay = dir == 1;
+    const int lo2 = I2(g, ix - ax, jy - ay), hi2 = I2(g, ix, jy);
+    // both depth means of 1/mu in one sweep (the sums in the same order)
+    double n2 = 0, d2 = 0, n1 = 0, d1 = 0;
+    for (int kz = 2; kz < g.Nz - 2; kz++) {
+        const double c = 0.5 * (a.fC[kz * g.n2 + lo2] + a.fC[kz * g.n2 + hi2]);
+        const double r = 1.0 - g.dz * (-1.5 + (double)kz);
+        const double mu = visc(a.cs, c);
+        n2 += (r * r) / mu; d2 += r * r;
+        n1 += r / mu; d1 += r;
+    }
+    const double hl = a.fH[lo2], hh = a.fH[hi2];
+    const double cl_t = a.fC[(g.Nz - 2) * g.n2 + lo2], cl_m = a.fC[(g.Nz - 3) * g.n2 + lo2];
+    const double ch_t = a.fC[(g.Nz - 2) * g.n2 + hi2], ch_m = a.fC[(g.Nz - 3) * g.n2 + hi2];
+    sg = sigm(a.cs, 0.25 * (cl_t + cl_m + ch_t + ch_m));
+    const bool w = wall_face(a, ix, jy, dir);
+    D = 0.0;
+    M = 0.0;
+    if (!w) {
+        const double denom = hl + hh;
+        if (!(denom <= 0.0)) {
+            const double mob = 2.0 * (hl * hl) * (hh * hh) / denom;
+            D = mob / 3.0 * (n2 / d2);
+        }
+        if (a.cs.marangoni && !(hl + hh <= 0.0)) {
+            const double ds = dir == 0 ? g.dx : g.dy;
+            const double cs1 = 0.5 * (cl_t + cl_m), cs2 = 0.5 * (ch_t + ch_m);
+            const double dcds = (cs2 - cs1) / ds;
+            M = 0.5 * (hl * hh) * a.cs.As * a.cs.Ma * dcds * (n1 / d1);
+        }
+    }
+}
+
+__global__ void k_faces(Args a) {
+    const Geo3 &g = a.g;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.n2) return;
+    const int ix = p % g.Nx, jy = p / g.Nx;
+    const double rdx = 1.0 / g.dx, rdx2 = rdx * rdx, rdy = 1.0 / g.dy, rdy2 = rdy * rdy;
+    auto Sf = [&](int i, int j) { return a.S[I2(g, i, j)]; };
+    auto Hf = [&](int i, int j) { return a.fH[I2(g, i, j)]; };
+    if (ix >= 2 && ix <= g.Nx - 2 && jy >= 2 && jy < g.Ny - 2) {
+        double D, sg, M;
+        face_values(a, ix, jy, 0, D, sg, M);
+        a.Dx[p] = D; a.sgx[p] = sg; a.Mx[p] = M;
+        double cap = 0.0, expl = 0.0;
+        if (a.cs.capillary) {
+            cap = D * sg;
+            double d3s = (Sf(ix + 1, jy) - 3.0 * Sf(ix, jy) + 3.0 * Sf(ix - 1, jy) - Sf(ix - 2, jy)) * rdx * rdx2;
+            d3s += ((Sf(ix, jy + 1) - 2.0 * Sf(ix, jy) + Sf(ix, jy - 1))
+                    - (Sf(ix - 1, jy + 1) - 2.0 * Sf(ix - 1, jy) + Sf(ix - 1, jy - 1))) * rdx * rdy2;
+            expl += cap * d3s;
+        }
+        if (a.cs.vertical_gravity)
+            expl -= D * a.cs.Bo * ((Hf(ix, jy) + Sf(ix, jy)) - (Hf(ix - 1, jy) + Sf(ix - 1, jy))) * rdx;
+        if (a.cs.inplane_gravity) expl += D * a.cs.Pgx;
+        if (a.cs.centrifugal) expl += D * a.cs.Sp * ((g.xlo + ix * g.dx) - a.cs.rot_x);
+        if (a.cs.marangoni) expl += M;
+        a.qCapX[p] = cap;
+        a.qExpX[p] = expl;
+    }
+    if (ix >= 2 && ix < g.Nx - 2 && jy >= 2 && jy <= g.Ny - 2) {
+        double D, sg, M;
+        face_values(a, ix, jy, 1, D, sg, M);
+        a.Dy[p] = D; a.sgy[p] = sg; a.My[p] = M;
+        double cap = 0.0, expl = 0.0;
+        if (a.cs.capillary) {
+            cap = D * sg;
+            double d3s = (Sf(ix, jy + 1) - 3.0 * Sf(ix, jy) + 3.0 * Sf(ix, jy - 1) - Sf(ix, jy - 2)) * rdy * rdy2;
+            d3s += ((Sf(ix - 1, jy) - 2.0 * Sf(ix, jy) + Sf(ix + 1, jy))
+                    - (Sf(ix - 1, jy - 1) - 2.0 * Sf(ix, jy - 1) + Sf(ix + 1, jy - 1))) * rdy * rdx2;
+            expl += cap * d3s;
+        }
+        if (a.cs.vertical_gravity)
+            expl -= D * a.cs.Bo * ((Hf(ix, jy) + Sf(ix, jy)) - (Hf(ix, jy - 1) + Sf(ix, jy - 1))) * rdy;
+        if (a.cs.inplane_gravity) expl += D * a.cs.Pgy;
+        if (a.cs.centrifugal) expl += D * a.cs.Sp * ((g.ylo + jy * g.dy) - a.cs.rot_y);
+        if (a.cs.marangoni) expl += M;
+        a.qCapY[p] = cap;
+        a.qExpY[p] = expl;
+    }
+}
+
+// SetMatrixA of the film for one real cell, then the row of
+// I + theta dt A with its right-hand side h_old - (1 - theta) dt A h_old
+// - dt B (MakeMatrix_IA / MakeRHS, SemiImplicit.cpp:158-190), scaled by
+// its diagonal (ScaleRows), into the solver's planes
+__global__ void k_hrows(Args a) {
+    const Geo3 &g = a.g;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = g.nx * g.ny;
+    if (q >= N) return;
+    const int ix = q % g.nx + 2, jy = q / g.nx + 2;
+    const int p = I2(g, ix, jy);
+    const yz_cz_case &cs = a.cs;
+    const double rdx = 1.0 / g.dx, rdx2 = rdx * rdx, rdx4 = rdx2 * rdx2;
+    const double rdy = 1.0 / g.dy, rdy2 = rdy * rdy, rdy4 = rdy2 * rdy2;
+    const double rdx2dy2 = rdx2 * rdy2;
+    auto SS = [&](int i, int j) { return a.S[I2(g, i, j)]; };
+    auto HH = [&](int i, int j) { return a.fH[I2(g, i, j)]; };
+    double A[13];
+    const double DL = a.Dx[p], DR = a.Dx[p + 1];
+    const double sigL = a.sgx[p], sigR = a.sgx[p + 1];
+    const double aL = cs.capillary ? DL * sigL * rdx4 : 0.0;
+    const double aR = cs.capillary ? DR * sigR * rdx4 : 0.0;
+    A[1] = +aL;
+    A[2] = -3.0 * aL - aR;
+    A[0] = +3.0 * aL + 3.0 * aR;
+    A[3] = -aL - 3.0 * aR;
+    A[4] = +aR;
+    const double DB = a.Dy[p], DT = a.Dy[p + g.Nx];
+    const double sigB = a.sgy[p], sigT = a.sgy[p + g.Nx];
+    const double bB = cs.capillary ? DB * sigB * rdy4 : 0.0;
+    const double bT = cs.capillary ? DT * sigT * rdy4 : 0.0;
+    A[5] = +bB;
+    A[6] = -3.0 * bB - bT;
+    A[0] += +3.0 * bB + 3.0 * bT;
+    A[7] = -bB - 3.0 * bT;
+    A[8] = +bT;
+    const double cL = cs.capillary ? DL * sigL * rdx2dy2 : 0.0;
+    const double cR = cs.capillary ? DR * sigR * rdx2dy2 : 0.0;
+    const double cB = cs.capillary ? DB * sigB * rdx2dy2 : 0.0;
+    const double cT = cs.capillary ? DT * sigT * rdx2dy2 : 0.0;
+    A[9] = +cL;
+    A[2] += -2.0 * cL;
+    A[10] = +cL;
+    A[11] = +cR;
+    A[3] += -2.0 * cR;
+    A[12] = +cR;
+    A[6] += -cL - cR;
+    A[0] += +2.0 * cL + 2.0 * cR;
+    A[7] += -cL - cR;
+    A[9] += +cB;
+    A[6] += -2.0 * cB;
+    A[11] += +cB;
+    A[10] += +cT;
+    A[7] += -2.0 * cT;
+    A[12] += +cT;
+    A[2] += -cB - cT;
+    A[0] += +2.0 * cB + 2.0 * cT;
+    A[3] += -cB - cT;
+
+    const double sPo = SS(ix + 2, jy), spo = SS(ix + 1, jy), smo = SS(ix - 1, jy), sMo = SS(ix - 2, jy);
+    const double soo = SS(ix, jy);
+    double dhdt = 0.0;
+    if (cs.capillary) {
+        const double d3sL = (spo - 3.0 * soo + 3.0 * smo - sMo) * rdx * rdx2;
+        const double d3sR = (sPo - 3.0 * spo + 3.0 * soo - smo) * rdx * rdx2;
+        dhdt -= (DR * sigR * d3sR - DL * sigL * d3sL) * rdx;
+    }
+    if (cs.vertical_gravity) {
+        const double hsP = HH(ix + 1, jy) + spo, hsO = HH(ix, jy) + soo, hsM = HH(ix - 1, jy) + smo;
+        const double gR = DR * cs.Bo * (hsP - hsO) * rdx;
+        const double gL = DL * cs.Bo * (hsO - hsM) * rdx;
+        dhdt += (gR - gL) * rdx;
+    }
+    if (cs.inplane_gravity) {
+        const double fR = DR * cs.Pgx, fL = DL * cs.Pgx;
+        dhdt -= (fR - fL) * rdx;
+    }
+    if (cs.centrifugal) {
+        const double xR = (g.xlo + (ix + 1) * g.dx) - cs.rot_x, xL = (g.xlo + ix * g.dx) - cs.rot_x;
+        dhdt -= (DR * cs.Sp * xR - DL * cs.Sp * xL) * rdx;
+    }
+    if (cs.marangoni) dhdt -= (a.Mx[p + 1] - a.Mx[p]) * rdx;
+    if (cs.capillary) {
+        const double soP = SS(ix, jy + 2), sop = SS(ix, jy + 1), som = SS(ix, jy - 1), soM = SS(ix, jy - 2);
+        const double d3sB = (sop - 3.0 * soo + 3.0 * som - soM) * rdy * rdy2;
+        const double d3sT = (soP - 3.0 * sop + 3.0 * soo - som) * rdy * rdy2;
+        dhdt -= (DT * sigT * d3sT - DB * sigB * d3sB) * rdy;
+        const double smm = SS(ix - 1, jy - 1), smp = SS(ix - 1, jy + 1);
+        const double spm = SS(ix + 1, jy - 1), spp = SS(ix + 1, jy + 1);
+        const double xL_ = ((som - 2.0 * soo + sop) - (smm - 2.0 * smo + smp)) * rdx * rdy2;
+        const double xR_ = ((spm - 2.0 * spo + spp) - (som - 2.0 * soo + sop)) * rdx * rdy2;
+        dhdt -= (DR * sigR * xR_ - DL * sigL * xL_) * rdx;
+        const double yB_ = ((smo - 2.0 * soo + spo) - (smm - 2.0 * som + spm)) * rdy * rdx2;
+        const double yT_ = ((smp - 2.0 * sop + spp) - (smo - 2.0 * soo + spo)) * rdy * rdx2;
+        dhdt -= (DT * sigT * yT_ - DB * sigB * yB_) * rdy;
+    }
+    if (cs.vertical_gravity) {
+        const double sop = SS(ix, jy + 1), som = SS(ix, jy - 1);
+        const double hsT = HH(ix, jy + 1) + sop, hsO = HH(ix, jy) + soo, hsB = HH(ix, jy - 1) + som;
+        const double gT = DT * cs.Bo * (hsT - hsO) * rdy;
+        const double gB = DB * cs.Bo * (hsO - hsB) * rdy;
+        dhdt += (gT - gB) * rdy;
+    }
+    if (cs.inplane_gravity) dhdt -= (DT * cs.Pgy - DB * cs.Pgy) * rdy;
+    if (cs.centrifugal) {
+        const double yT = (g.ylo + (jy + 1) * g.dy) - cs.rot_y, yB = (g.ylo + jy * g.dy) - cs.rot_y;
+        dhdt -= (DT * cs.Sp * yT - DB * cs.Sp * yB) * rdy;
+    }
+    if (cs.marangoni) dhdt -= (a.My[p + g.Nx] - a.My[p]) * rdy;
+    double Evap = cs.Ev * evap_rate(a, a.fH, a.fC, ix, jy);
+    if (Evap > 0.0) {
+        Evap = solvent_limit(Evap, a.fH[p], column_mean(g, a.fC, ix, jy), a.dt);
+        Evap = solvent_limit(Evap, a.tH[p], column_mean(g, a.tC, ix, jy), a.dt);
+    }
+    dhdt -= Evap;
+    a.colSrc[p] = 0.0 - Evap - 0.0;
+    const double B = -dhdt;
+    // A h_old in storage order (CRS_Matrix::MatVec)
+    double Ax = 0.0;
+    Ax += A[0] * a.hO[p];
+    for (int k = 1; k < 13; ++k) Ax += A[k] * a.hO[I2(g, ix + c_hox[k], jy + c_hoy[k])];
+    const double th = a.theta, dt = a.dt;
+    const double rhs = a.hO[p] - dt * (1.0 - th) * Ax - dt * B;
+    const double d = 1.0 + th * dt * A[0];
+    const double inv = (fabs(d) < 1.0e-30) ? 1.0 : 1.0 / d;
+    for (int k = 0; k < 13; ++k) {
+        const double v = (k == 0) ? d : th * dt * A[k];
+        a.hP[(size_t)c_hslot[k] * N + q] = (fabs(d) < 1.0e-30) ? v : v * inv;
+    }
+    a.hb[q] = (fabs(d) < 1.0e-30) ? rhs : rhs * inv;
+    a.hx[q] = a.hN[p];
+}
+
+// the film's ghosts from the real rows they copy (CopyGhosts: a corner
+// reads an edge, so both axes fold)
+__global__ void k_hghosts(Geo3 g, double *h) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.n2) return;
+    const int ix = p % g.Nx, jy = p / g.Nx;
+    const bool gx = ix < 2 || ix > g.Nx - 3, gy = jy < 2 || jy > g.Ny - 3;
+    if (!gx && !gy) return;
+    const int sx = fold_pad(ix, g.Nx, g.perx), sy = fold_pad(jy, g.Ny, g.pery);
+    h[p] = h[I2(g, sx, sy)];
+}
+__global__ void k_hscatter(Geo3 g, const double *x, double *h) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    h[I2(g, q % g.nx + 2, q / g.nx + 2)] = x[q];
+}
+
+// ConservativeUpdate (:1232-1301): the face fluxes with the
+// Crank-Nicolson h of the solve, then the end of the step through them
+__device__ inline double hcv(const Args &a, int i, int j) {
+    const int p = I2(a.g, i, j);
+    const double u = a.hN[p] * a.theta;
+    const double v = a.hO[p] * (1.0 - a.theta);
+    return u + v;
+}
+__global__ void k_hflux(Args a) {
+    const Geo3 &g = a.g;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.n2) return;
+    const int ix = p % g.Nx, jy = p / g.Nx;
+    const double rdx = 1.0 / g.dx, rdx2 = rdx * rdx, rdy = 1.0 / g.dy, rdy2 = rdy * rdy;
+    if (ix >= 2 && ix <= g.Nx - 2 && jy >= 2 && jy < g.Ny - 2) {
+        double d3h = (hcv(a, ix + 1, jy) - 3.0 * hcv(a, ix, jy) + 3.0 * hcv(a, ix - 1, jy) - hcv(a, ix - 2, jy)) * rdx * rdx2;
+        d3h += ((hcv(a, ix, jy + 1) - 2.0 * hcv(a, ix, jy) + hcv(a, ix, jy - 1))
+                - (hcv(a, ix - 1, jy + 1) - 2.0 * hcv(a, ix - 1, jy) + hcv(a, ix - 1, jy - 1))) * rdx * rdy2;
+        a.qX[p] = a.qCapX[p] * d3h + a.qExpX[p];
+    }
+    if (ix >= 2 && ix < g.Nx - 2 && jy >= 2 && jy <= g.Ny - 2) {
+        double d3h = (hcv(a, ix, jy + 1) - 3.0 * hcv(a, ix, jy) + 3.0 * hcv(a, ix, jy - 1) - hcv(a, ix, jy - 2)) * rdy * rdy2;
+        d3h += ((hcv(a, ix - 1, jy) - 2.0 * hcv(a, ix, jy) + hcv(a, ix + 1, jy))
+                - (hcv(a, ix - 1, jy - 1) - 2.0 * hcv(a, ix, jy - 1) + hcv(a, ix + 1, jy - 1))) * rdy * rdx2;
+        a.qY[p] = a.qCapY[p] * d3h + a.qExpY[p];
+    }
+}
+__global__ void k_hupdate(Args a) {
+    const Geo3 &g = a.g;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int ix = q % g.nx + 2, jy = q / g.nx + 2;
+    const int p = I2(g, ix, jy);
+    double div = (a.qX[p + 1] - a.qX[p]) * (1.0 / g.dx);
+    div += (a.qY[p + g.Nx] - a.qY[p]) * (1.0 / g.dy);
+    a.hN[p] = a.hO[p] + a.dt * (a.colSrc[p] - div);
+}
+// the midpoint film and dh/dt of the step (CalDtDh)
+__global__ void k_hmid(Args a, double *mid, int with_rate) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.g.n2) return;
+    mid[p] = (a.hO[p] + a.hN[p]) * 0.5;
+    if (with_rate) {
+        const double d = a.hN[p] - a.hO[p];
+        a.dtdh[p] = d / a.dt;
+    }
+}
+
+// ---------------------------------------------------------------------
+// the velocity through the film at one face (CalVelocity, Simpson's rule
+// up the column) and the layer volume fluxes of that face
+// (BuildVolumeFluxes: the layer means, and the difference from the
+// film's flux spread evenly through the depth). One thread per face.
+// ---------------------------------------------------------------------
+__device__ void face_column(const Args &a, int ix, int jy, int dir) {
+    const Geo3 &g = a.g;
+    const yz_cz_case &cs = a.cs;
+    const int ax = dir == 0, ay = dir == 1;
+    const int bx = ay, by = ax;
+    auto P2 = [&](int k) { return I2(g, ix + k * ax, jy + k * ay); };
+    auto C = [&](int k, int kz) { return a.fC[kz * g.n2 + P2(k)]; };
+    const double ds = dir == 0 ? g.dx : g.dy;
+    const double dn = dir == 0 ? g.dy : g.dx;
+    const double ds3 = pow(ds, 3.0);
+    const double hpo = a.fH[P2(1)], hoo = a.fH[P2(0)], hmo = a.fH[P2(-1)], hMo = a.fH[P2(-2)];
+    const double spo = a.S[P2(1)], soo = a.S[P2(0)], smo = a.S[P2(-1)], sMo = a.S[P2(-2)];
+    const double hL = 0.5 * (hmo + hoo), sL = 0.5 * (smo + soo);
+    const double xL = dir == 0 ? (g.xlo + ix * g.dx) - cs.rot_x : (g.ylo + jy * g.dy) - cs.rot_y;
+    const bool w = wall_face(a, ix, jy, dir);
+    double *Q = dir == 0 ? a.Qx : a.Qy;
+    const double qfilm = dir == 0 ? a.qX[P2(0)] : a.qY[P2(0)];
+    const double hf = 0.5 * (a.fH[P2(-1)] + a.fH[P2(0)]);
+    double Cooo = C(0, g.Nz - 2), Cmoo = C(-1, g.Nz - 2), Coom = C(0, g.Nz - 3), Cmom = C(-1, g.Nz - 3);
+    const double dcdx_suf = 0.5 * ((-C(1, g.Nz - 2) + 27.0 * Cooo - 27.0 * Cmoo + C(-2, g.Nz - 2))
+                                   + (-C(1, g.Nz - 3) + 27.0 * Coom - 27.0 * Cmom + C(-2, g.Nz - 3))) / (24.0 * ds);
+    const double dx3hs = ((hpo + spo) - 3.0 * (hoo + soo) + 3.0 * (hmo + smo) - (hMo + sMo)) / ds3;
+    const double dxhs = (-(hpo + spo) + 27.0 * (hoo + soo) - 27.0 * (hmo + smo) + (hMo + sMo)) / (24.0 * ds);
+    const double sig = sigm(cs, 0.25 * (Cooo + Coom + Cmoo + Cmom));
+    Cooo = C(0, 2); Cmoo = C(-1, 2); Coom = C(0, 1); Cmom = C(-1, 1);
+    const double muB = visc(cs, 0.25 * (Cooo + Coom + Cmoo + Cmom));
+    double fB1 = 1.0 / muB, fB2 = sL / muB;
+    const double dzz = g.dz * hL;
+    double sum1 = 0, sum2 = 0;
+    auto HS = [&](int s, int t) {
+        const int p = I2(g, ix + s * ax + t * bx, jy + s * ay + t * by);
+        return a.fH[p] + a.S[p];
+    };
+    const double dn2 = dn * dn;
+    const double dxyy = ((HS(0, 1) - 2.0 * (hoo + soo) + HS(0, -1)) - (HS(-1, 1) - 2.0 * (hmo + smo) + HS(-1, -1))) / (ds * dn2);
+    const double kLap = cs.capillary ? sig * (dx3hs + dxyy) : 0.0;
+    const double kMa = cs.marangoni ? cs.As * cs.Ma * dcdx_suf : 0.0;
+    const double kBo = cs.vertical_gravity ? -cs.Bo * dxhs : 0.0;
+    const double kPg = cs.inplane_gravity ? (dir == 0 ? cs.Pgx : cs.Pgy) : 0.0;
+    const double kSp = cs.centrifugal ? cs.Sp * xL : 0.0;
+    const double hLsL = hL + sL;
+    // the profile at levels 2 .. Nz-2; the layer mean of level k needs
+    // u[k-1], u[k], u[k+1]
+    double um1 = 0.0, u0 = 0.0;
+    double sum = 0.0;
+    const int base = P2(0);
+    for (int kz = 2; kz <= g.Nz - 2; kz++) {
+        const double zT = sL + dzz * (-1.5 + (double)kz);
+        const double zM = (kz == 2) ? sL + 0.25 * dzz : sL + dzz * (-2.0 + (double)kz);
+        if (kz > 2) {
+            Coom = Cooo; Cmom = Cmoo;
+            Cooo = C(0, kz); Cmoo = C(-1, kz);
+        }
+        const double muM = (kz == 2) ? visc(cs, 0.375 * (Cooo + Cmoo) + 0.125 * (Coom + Cmom))
+                                     : visc(cs, 0.25 * (Cooo + Coom + Cmoo + Cmom));
+        const double muT = visc(cs, 0.50 * (Cooo + Cmoo));
+        const double fT1 = 1.0 / muT, fT2 = zT / muT, fM1 = 1.0 / muM, fM2 = zM / muM;
+        const double dzt = (kz == 2) ? 0.5 * dzz : dzz;
+        sum1 += (fB1 + fT1 + 4.0 * fM1) * dzt / 6.0;
+        sum2 += (fB2 + fT2 + 4.0 * fM2) * dzt / 6.0;
+        fB1 = fT1; fB2 = fT2;
+        const double PP = hLsL * sum1 - sum2;
+        double uu = 0;
+        if (cs.capillary) uu += kLap * PP;
+        if (cs.marangoni) uu += kMa * sum1;
+        if (cs.vertical_gravity) uu += kBo * PP;
+        if (cs.inplane_gravity) uu += kPg * PP;
+        if (cs.centrifugal) uu += kSp * PP;
+        if (w) uu = 0.0;
+        if (kz >= 3) {
+            const int k = kz - 1;
+            double um;
+            if (k == 2) um = u0 + (uu - 3.0 * u0) / 18.0;
+            else um = u0 + (uu - 2.0 * u0 + um1) / 24.0;
+            const double qv = hf * um;
+            Q[k * g.n2 + base] = qv;
+            sum += qv * g.dz;
+        }
+        um1 = u0;
+        u0 = uu;
+    }
+    const double d = qfilm - sum;
+    for (int kz = 2; kz < g.Nz - 2; kz++) Q[kz * g.n2 + base] += d;
+}
+__global__ void k_faces3(Args a) {
+    const Geo3 &g = a.g;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.n2) return;
+    const int ix = p % g.Nx, jy = p / g.Nx;
+    if (ix >= 2 && ix <= g.Nx - 2 && jy >= 2 && jy < g.Ny - 2) face_column(a, ix, jy, 0);
+    if (ix >= 2 && ix < g.Nx - 2 && jy >= 2 && jy <= g.Ny - 2) face_column(a, ix, jy, 1);
+}
+// the zeta volume fluxes of a column from the substrate up (the discrete
+// geometric conservation law), the free surface from the film's sources
+__global__ void k_w(Args a) {
+    const Geo3 &g = a.g;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int ix = q % g.nx + 2, jy = q / g.nx + 2;
+    const int p = I2(g, ix, jy);
+    const double ht = a.dtdh[p];
+    double w = 0.0;
+    a.W[2 * g.n2 + p] = 0.0;
+    for (int kz = 2; kz < g.Nz - 2; kz++) {
+        const int r = kz * g.n2 + p;
+        double div = ht + (a.Qx[r + 1] - a.Qx[r]) / g.dx;
+        div += (a.Qy[r + g.Nx] - a.Qy[r]) / g.dy;
+        w -= g.dz * div;
+        a.W[r + g.n2] = w;
+    }
+    a.W[(g.Nz - 2) * g.n2 + p] = -(a.colSrc[p] + 0.0);
+}
+
+// ---------------------------------------------------------------------
+// the concentration's ghost rows: the right-hand sides of the z ghosts of
+// every real column (SetBoundaryCondition), and the ghosts of a vector
+// from the rows (ApplyBoundaryRows: the z ghosts from the real cells,
+// then the y ghost rows, then the x ghost columns, which is the fixed
+// point the reference's sweeps reach)
+// ---------------------------------------------------------------------
+__global__ void k_cbc(Args a, int atEnd) {
+    const Geo3 &g = a.g;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int ix = q % g.nx + 2, jy = q / g.nx + 2;
+    const int p = I2(g, ix, jy);
+    const yz_cz_case &cs = a.cs;
+    auto C = [&](int i, int j, int k) { return a.fC[I3(g, i, j, k)]; };
+    auto Hh = [&](int i, int j) { return a.fH[I2(g, i, j)]; };
+    auto Ss = [&](int i, int j) { return a.S[I2(g, i, j)]; };
+    const double hh = Hh(ix, jy);
+    const double hx0 = (Hh(ix - 2, jy) - 8.0 * Hh(ix - 1, jy) + 8.0 * Hh(ix + 1, jy) - Hh(ix + 2, jy)) / (12.0 * g.dx);
+    const double sx0 = (Ss(ix - 2, jy) - 8.0 * Ss(ix - 1, jy) + 8.0 * Ss(ix + 1, jy) - Ss(ix + 2, jy)) / (12.0 * g.dx);
+    const double hy0 = (Hh(ix, jy - 2) - 8.0 * Hh(ix, jy - 1) + 8.0 * Hh(ix, jy + 1) - Hh(ix, jy + 2)) / (12.0 * g.dy);
+    const double sy0 = (Ss(ix, jy - 2) - 8.0 * Ss(ix, jy - 1) + 8.0 * Ss(ix, jy + 1) - Ss(ix, jy + 2)) / (12.0 * g.dy);
+    {
+        const int k = 2;
+        const double cb = 0.5 * (C(ix, jy, k - 1) + C(ix, jy, k));
+        const double D = cs.Df * diffu(cs, cb);
+        const double cx = (C(ix + 1, jy, k) - C(ix - 1, jy, k)) / (2.0 * g.dx);
+        const double cy = (C(ix, jy + 1, k) - C(ix, jy - 1, k)) / (2.0 * g.dy);
+        const double hA = sx0, hAy = sy0;
+        const double cz = hh * (0.0 / D + hA * cx + hAy * cy) / (cs.As + hA * hA + hAy * hAy);
+        a.zr[0 * g.n2 + p] = -3.0 * g.dz * cz;
+        a.zr[1 * g.n2 + p] = -g.dz * cz;
+    }
+    {
+        const int k = g.Nz - 3;
+        const double csf = 0.5 * (C(ix, jy, k) + C(ix, jy, k + 1));
+        const double D = cs.Df * diffu(cs, csf);
+        const double cx = (C(ix + 1, jy, k) - C(ix - 1, jy, k)) / (2.0 * g.dx);
+        const double cy = (C(ix, jy + 1, k) - C(ix, jy - 1, k)) / (2.0 * g.dy);
+        const double hA = hx0 + sx0, hAy = hy0 + sy0;
+        double WT = a.W[(g.Nz - 2) * g.n2 + p];
+        if (atEnd) {
+            WT = cs.Ev * evap_rate(a, a.fH, a.fC, ix, jy);
+            double cm = 0.0;
+            for (int kk = 2; kk < g.Nz - 2; kk++) cm += C(ix, jy, kk);
+            cm /= (double)(g.Nz - 4);
+            WT = solvent_limit(WT, hh, cm, a.dt);
+        }
+        const double G = WT * csf;
+        const double cz = hh * (G / D + hA * cx + hAy * cy) / (cs.As + hA * hA + hAy * hAy);
+        const double grad = g.dz * cz;
+        a.zr[2 * g.n2 + p] = grad;
+        a.zr[3 * g.n2 + p] = 3.0 * grad;
+    }
+}
+// rows c(0) = c(3) + r0, c(1) = c(2) + r1, c(Nz-2) = c(Nz-3) + r2,
+// c(Nz-1) = c(Nz-4) + r3 of every real column
+__global__ void k_cghost_z(Geo3 g, const double *zr, double *x) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
+    double s;
+    s = zr[0 * g.n2 + p]; s -= (-1.0) * x[3 * g.n2 + p]; x[0 * g.n2 + p] = s / 1.0;
+    s = zr[1 * g.n2 + p]; s -= (-1.0) * x[2 * g.n2 + p]; x[1 * g.n2 + p] = s / 1.0;
+    s = zr[2 * g.n2 + p]; s -= (-1.0) * x[(g.Nz - 3) * g.n2 + p]; x[(g.Nz - 2) * g.n2 + p] = s / 1.0;
+    s = zr[3 * g.n2 + p]; s -= (-1.0) * x[(g.Nz - 4) * g.n2 + p]; x[(g.Nz - 1) * g.n2 + p] = s / 1.0;
+}
+// the y ghost rows of the real x columns, every level
+__global__ void k_cghost_y(Geo3 g, double *x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = g.nx * 4;
+    if (t >= per * g.Nz) return;
+    const int kz = t / per, r = t % per;
+    const int ix = r % g.nx + 2, gj = r / g.nx;
+    const int jy = gj < 2 ? gj : g.Ny - 4 + gj;
+    const int sy = fold_pad(jy, g.Ny, g.pery);
+    x[I3(g, ix, jy, kz)] = 0.0 + x[I3(g, ix, sy, kz)];
+}
+// the x ghost columns, every row and level
+__global__ void k_cghost_x(Geo3 g, double *x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = 4 * g.Ny;
+    if (t >= per * g.Nz) return;
+    const int kz = t / per, r = t % per;
+    const int jy = r / 4, gi = r % 4;
+    const int ix = gi < 2 ? gi : g.Nx - 4 + gi;
+    const int sx = fold_pad(ix, g.Nx, g.perx);
+    x[I3(g, ix, jy, kz)] = 0.0 + x[I3(g, sx, jy, kz)];
+}
+
+// ---------------------------------------------------------------------
+// SetMatrixA of the concentration for one real cell (:290-570), its row
+// of I + theta dt A and right-hand side -dt (A c_old + B) (SolveStage,
+// with A c_old in the form of ExplicitPart), scaled by the diagonal
+// ---------------------------------------------------------------------
+__global__ void k_crows(Args a, const double *cold) {
+    const Geo3 &g = a.g;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nint) return;
+    const int ix = q % g.nx + 2, jy = (q / g.nx) % g.ny + 2, kz = q / (g.nx * g.ny) + 2;
+    const yz_cz_case &cs = a.cs;
+    const double As = cs.As, Df = cs.Df, dx = g.dx, dy = g.dy, dz = g.dz;
+    auto C = [&](int i, int j, int k) { return a.fC[I3(g, i, j, k)]; };
+    auto Hh = [&](int i, int j) { return a.fH[I2(g, i, j)]; };
+    auto Ss = [&](int i, int j) { return a.S[I2(g, i, j)]; };
+    const bool wallXmin = (ix == 2) && g.wall[0];
+    const bool wallXmax = (ix == g.Nx - 3) && g.wall[1];
+    const bool wallYmin = (jy == 2) && g.wall[2];
+    const bool wallYmax = (jy == g.Ny - 3) && g.wall[3];
+    const double hoo = Hh(ix, jy);
+    const double hpo = Hh(ix + 1, jy), hPo = Hh(ix + 2, jy), hmo = Hh(ix - 1, jy), hMo = Hh(ix - 2, jy);
+    const double spo = Ss(ix + 1, jy), sPo = Ss(ix + 2, jy), smo = Ss(ix - 1, jy), sMo = Ss(ix - 2, jy);
+    const double soo = Ss(ix, jy);
+    const double hL = 0.5 * (hoo + hmo), hR = 0.5 * (hoo + hpo);
+    const double hx0 = (hMo - 8.0 * hmo + 8.0 * hpo - hPo) / (12.0 * dx);
+    const double sx0 = (sMo - 8.0 * smo + 8.0 * spo - sPo) / (12.0 * dx);
+    const double hxL = (hMo - 27.0 * hmo + 27.0 * hoo - hpo) / (24.0 * dx);
+    const double sxL = (sMo - 27.0 * smo + 27.0 * soo - spo) / (24.0 * dx);
+    const double hxR = (hmo - 27.0 * hoo + 27.0 * hpo - hPo) / (24.0 * dx);
+    const double sxR = (smo - 27.0 * soo + 27.0 * spo - sPo) / (24.0 * dx);
+    const double hoP = Hh(ix, jy + 2), hop = Hh(ix, jy + 1), hom = Hh(ix, jy - 1), hoM = Hh(ix, jy - 2);
+    const double soP = Ss(ix, jy + 2), sop = Ss(ix, jy + 1), som = Ss(ix, jy - 1), soM = Ss(ix, jy - 2);
+    const double hS = 0.5 * (hoo + hom), hN = 0.5 * (hoo + hop);
+    const double hy0 = (hoM - 8.0 * hom + 8.0 * hop - hoP) / (12.0 * dy);
+    const double sy0 = (soM - 8.0 * som + 8.0 * sop - soP) / (12.0 * dy);
+    const double hyS = (hoM - 27.0 * hom + 27.0 * hoo - hop) / (24.0 * dy);
+    const double syS = (soM - 27.0 * som + 27.0 * soo - sop) / (24.0 * dy);
+    const double hyN = (hom - 27.0 * hoo + 27.0 * hop - hoP) / (24.0 * dy);
+    const double syN = (som - 27.0 * soo + 27.0 * sop - soP) / (24.0 * dy);
+    const double zeta = dz * (-1.5 + (double)kz);
+    const bool bottom = (kz == 2), top = (kz == g.Nz - 3);
+    double xz[5][5], yz[5][3];
+    for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) xz[i][j] = 0.0;
+    for (int i = 0; i < 5; ++i) for (int j = 0; j < 3; ++j) yz[i][j] = 0.0;
+    double rct = 0.0;
+#define XZ(di, dk, v) (xz[(di) + 2][(dk) + 2] += (v))
+#define YZ(dj, dk, v) do { if ((dj) == 0) xz[2][(dk) + 2] += (v); else yz[(dj) + 2][(dk) + 1] += (v); } while (0)
+    const int r3 = I3(g, ix, jy, kz);
+    {
+        const double QL = a.Qx[r3], QR = a.Qx[r3 + 1];
+        const double sL = sgn(QL), sR = sgn(QR);
+        for (int t = 0; t < 4; t++) {
+            XZ(t - 2, 0, +QL * (a.w1[t] + sL * a.w2[t]) / dx);
+            XZ(t - 1, 0, -QR * (a.w1[t] + sR * a.w2[t]) / dx);
+        }
+    }
+    {
+        const double QS = a.Qy[r3], QN = a.Qy[r3 + g.Nx];
+        const double sS = sgn(QS), sN = sgn(QN);
+        for (int t = 0; t < 4; t++) {
+            YZ(t - 2, 0, +QS * (a.w1[t] + sS * a.w2[t]) / dy);
+            YZ(t - 1, 0, -QN * (a.w1[t] + sN * a.w2[t]) / dy);
+        }
+    }
+    {
+        const double WB = a.W[r3], WT = a.W[r3 + g.n2];
+        if (!bottom) {
+            const double sB = sgn(WB);
+            for (int t = 0; t < 4; t++) XZ(0, t - 2, +WB * (a.w1[t] + sB * a.w2[t]) / dz);
+        }
+        if (!top) {
+            const double sT = sgn(WT);
+            for (int t = 0; t < 4; t++) XZ(0, t - 1, -WT * (a.w1[t] + sT * a.w2[t]) / dz);
+        } else {
+            rct += WT / dz;
+        }
+    }
+    const double Coo = C(ix, jy, kz);
+    {
+        const double DL = Df * diffu(cs, 0.5 * (C(ix - 1, jy, kz) + Coo));
+        const double DR = Df * diffu(cs, 0.5 * (Coo + C(ix + 1, jy, kz)));
+        const double AL = (hxL * zeta + sxL) / hL;
+        const double AR = (hxR * zeta + sxR) / hR;
+        const double aL = wallXmin ? 0.0 : hL * DL / dx, aR = wallXmax ? 0.0 : hR * DR / dx;
+        XZ(-1, 0, +aL / dx);
+        XZ(0, 0, -aL / dx);
+        XZ(-1, 1, +aL * AL / (4.0 * dz)); XZ(0, 1, +aL * AL / (4.0 * dz));
+        XZ(-1, -1, -aL * AL / (4.0 * dz)); XZ(0, -1, -aL * AL / (4.0 * dz));
+        XZ(1, 0, +aR / dx);
+        XZ(0, 0, -aR / dx);
+        XZ(0, 1, -aR * AR / (4.0 * dz)); XZ(1, 1, -aR * AR / (4.0 * dz));
+        XZ(0, -1, +aR * AR / (4.0 * dz)); XZ(1, -1, +aR * AR / (4.0 * dz));
+    }
+    {
+        const double DS = Df * diffu(cs, 0.5 * (C(ix, jy - 1, kz) + Coo));
+        const double DN = Df * diffu(cs, 0.5 * (Coo + C(ix, jy + 1, kz)));
+        const double AS = (hyS * zeta + syS) / hS;
+        const double AN = (hyN * zeta + syN) / hN;
+        const double aS = wallYmin ? 0.0 : hS * DS / dy, aN = wallYmax ? 0.0 : hN * DN / dy;
+        YZ(-1, 0, +aS / dy);
+        YZ(0, 0, -aS / dy);
+        YZ(-1, 1, +aS * AS / (4.0 * dz)); YZ(0, 1, +aS * AS / (4.0 * dz));
+        YZ(-1, -1, -aS * AS / (4.0 * dz)); YZ(0, -1, -aS * AS / (4.0 * dz));
+        YZ(1, 0, +aN / dy);
+        YZ(0, 0, -aN / dy);
+        YZ(0, 1, -aN * AN / (4.0 * dz)); YZ(1, 1, -aN * AN / (4.0 * dz));
+        YZ(0, -1, +aN * AN / (4.0 * dz)); YZ(1, -1, +aN * AN / (4.0 * dz));
+    }
+    for (int side = 0; side < 2; side++) {
+        if (side == 0 && bottom) continue;
+        if (side == 1 && top) continue;
+        const int k0 = (side == 0) ? kz - 1 : kz;
+        const double zf = zeta + (side == 0 ? -0.5 : 0.5) * dz;
+        const double Dz = Df * diffu(cs, 0.5 * (C(ix, jy, k0) + C(ix, jy, k0 + 1)));
+        const double hA = hx0 * zf + sx0;
+        const double hAy = hy0 * zf + sy0;
+        const double gsg = (side == 1) ? +1.0 : -1.0;
+        const double kzz = gsg * Dz * (As / hoo + (hA * hA + hAy * hAy) / hoo) / (dz * dz);
+        XZ(0, k0 + 1 - kz, +kzz);
+        XZ(0, k0 - kz, -kzz);
+        const double kx = -gsg * Dz * hA / (4.0 * dx) / dz;
+        XZ(1, k0 - kz, +kx); XZ(1, k0 + 1 - kz, +kx);
+        XZ(-1, k0 - kz, -kx); XZ(-1, k0 + 1 - kz, -kx);
+        const double ky = -gsg * Dz * hAy / (4.0 * dy) / dz;
+        YZ(1, k0 - kz, +ky); YZ(1, k0 + 1 - kz, +ky);
+        YZ(-1, k0 - kz, -ky); YZ(-1, k0 + 1 - kz, -ky);
+    }
+#undef XZ
+#undef YZ
+    const double rh = 1.0 / hoo;
+    double A[21];
+    double offsum = 0.0;
+    {
+        const double v[20] = {xz[2][0], xz[1][1], xz[2][1], xz[3][1], xz[0][2], xz[1][2], xz[3][2], xz[4][2],
+                              xz[1][3], xz[2][3], xz[3][3], xz[2][4], yz[0][1], yz[1][1], yz[3][1], yz[4][1],
+                              yz[1][0], yz[3][0], yz[1][2], yz[3][2]};
+        for (int k = 0; k < 20; ++k) {
+            const double e = -v[k] * rh;
+            A[k + 1] = e;
+            offsum += e;
+        }
+    }
+    const double react = -rct * rh;
+    A[0] = -offsum + react;
+    const int n = g.nint;
+    for (int k = 0; k < 21; ++k) a.cA[(size_t)k * n + q] = A[k];
+    a.react[q] = react;
+    // ExplicitPart of c_old (the ghosts set from this stage's rows)
+    const double xr = cold[r3];
+    double s = react * xr;
+    for (int k = 1; k < 21; ++k) s += A[k] * (cold[I3(g, ix + c_cox[k], jy + c_coy[k], kz + c_coz[k])] - xr);
+    const double rhs = -a.dt * (s + 0.0);
+    if (rhs != 0.0) atomicOr(&a.flags[0], 1);
+    const double th = a.theta, dt = a.dt;
+    const double d = 1.0 + th * dt * A[0];
+    const bool sc = !(fabs(d) < 1.0e-30);
+    const double inv = sc ? 1.0 / d : 1.0;
+    for (int k = 0; k < 21; ++k) {
+        const double v = (k == 0) ? d : th * dt * A[k];
+        a.cP[(size_t)k * n + q] = sc ? v * inv : v;
+    }
+    a.cb[q] = sc ? rhs * inv : rhs;
+}
+
+// ---------------------------------------------------------------------
+// the concentration system on the real cells: 21 planes, the neighbours
+// folded at the edges (x and y: wrapped or mirrored; zeta: mirrored, the
+// increments of the ghost rows being copies)
+// ---------------------------------------------------------------------
+__device__ inline int fold_in(int j, int n, int per) {
+    if (j >= 0 && j < n) return j;
+    if (per) { j %= n; if (j < 0) j += n; return j; }
+    if (j < 0) return -1 - j;
+    return 2 * n - 1 - j;
+}
+__device__ inline int col3(const Geo3 &g, int i, int j, int k, int s) {
+    const int ii = fold_in(i + c_cox[s], g.nx, g.perx);
+    const int jj = fold_in(j + c_coy[s], g.ny, g.pery);
+    const int kk = fold_in(k + c_coz[s], g.nz, 0);
+    return (kk * g.ny + jj) * g.nx + ii;
+}
+
+// block partial sums of up to 3 products, reduced in a fixed order
+template <int K>
+__device__ inline void block_partials(double (&v)[K], double *part) {
+    __shared__ double red[K][TBK];
+    for (int k = 0; k < K; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = TBK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < K; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; ++k) part[k * RB + blockIdx.x] = red[k][0];
+}
+// y = P x, and partial (w, y)
+__global__ void k3_spmv(Geo3 g, const double *P, const double *x, double *y, const double *w, double *part) {
+    double acc[1] = {0.0};
+    const int n = g.nint;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int i = q % g.nx, j = (q / g.nx) % g.ny, k = q / (g.nx * g.ny);
+        double s = 0.0;
+        for (int t = 0; t < 21; ++t) s += P[(size_t)t * n + q] * x[col3(g, i, j, k, t)];
+        y[q] = s;
+        if (w) acc[0] += w[q] * s;
+    }
+    if (part) block_partials<1>(acc, part);
+}
+// the column preconditioner: the zeta bands of each column, folded, by a
+// banded LU without pivoting (forward elimination streamed up the column)
+__global__ void k3_lu(Geo3 g, const double *P, double *L1, double *L2, double *U0, double *U1, double *U2) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = g.nx * g.ny;
+    if (c >= nc) return;
+    const int n = g.nint, nz = g.nz;
+    // the band row of level l: entries at offsets -2 .. 2
+    auto band = [&](int l, double r[5]) {
+        for (int o = 0; o < 5; ++o) r[o] = 0.0;
+        const int q = l * nc + c;
+        const int sl[5] = {1, 3, 0, 10, 12};
+        const int dzs[5] = {-2, -1, 0, 1, 2};
+        for (int m = 0; m < 5; ++m) {
+            int t = l + dzs[m];
+            if (t < 0) t = -1 - t;
+            if (t >= nz) t = 2 * nz - 1 - t;
+            r[t - l + 2] += P[(size_t)sl[m] * n + q];
+        }
+    };
+    double r0[5], r1[5], r2[5];
+    band(0, r0);
+    if (nz > 1) band(1, r1);
+    for (int k = 0; k < nz; ++k) {
+        if (k + 2 < nz) band(k + 2, r2);
+        const int q = k * nc + c;
+        const double piv = r0[2];
+        const double ip = 1.0 / piv;
+        U0[q] = ip; U1[q] = r0[3]; U2[q] = r0[4];
+        if (k + 1 < nz) {
+            const double f1 = r1[1] * ip;
+            L1[q + nc] = f1;
+            r1[2] -= f1 * r0[3];
+            r1[3] -= f1 * r0[4];
+        }
+        if (k + 2 < nz) {
+            const double f2 = r2[0] * ip;
+            L2[q + 2 * nc] = f2;
+            r2[1] -= f2 * r0[3];
+            r2[2] -= f2 * r0[4];
+        }
+        for (int o = 0; o < 5; ++o) { r0[o] = r1[o]; r1[o] = r2[o]; }
+    }
+}
+__global__ void k3_prec(Geo3 g, const double *L1, const double *L2, const double *U0, const double *U1,
+                        const double *U2, const double *b, double *x) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = g.nx * g.ny;
+    if (c >= nc) return;
+    const int nz = g.nz;
+    double y1 = 0.0, y2 = 0.0;   // y[k-1], y[k-2]
+    for (int k = 0; k < nz; ++k) {
+        const int q = k * nc + c;
+        double y = b[q];
+        if (k >= 1) y -= L1[q] * y1;
+        if (k >= 2) y -= L2[q] * y2;
+        x[q] = y;
+        y2 = y1; y1 = y;
+    }
+    double x1 = 0.0, x2 = 0.0;   // x[k+1], x[k+2]
+    for (int k = nz - 1; k >= 0; --k) {
+        const int q = k * nc + c;
+        double v = x[q];
+        if (k + 1 < nz) v -= U1[q] * x1;
+        if (k + 2 < nz) v -= U2[q] * x2;
+        v *= U0[q];
+        x[q] = v;
+        x2 = x1; x1 = v;
+    }
+}
+// one block sums the partials of K quantities in order
+__global__ void k_final(const double *part, int K, double *out) {
+    __shared__ double red[TBK];
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < RB; b += TBK) s += part[k * RB + b];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int t = TBK / 2; t > 0; t >>= 1) {
+            if ((int)threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = red[0];
+        __syncthreads();
+    }
+}
+// r = b - A x (x = 0 here: r = b), rh = r; partials (b, b), (r, r)
+__global__ void k3_init(int n, const double *b, double *r, double *rh, double *p, double *v, double *part) {
+    double acc[2] = {0.0, 0.0};
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const double v0 = b[q];
+        r[q] = v0; rh[q] = v0; p[q] = 0.0; v[q] = 0.0;
+        acc[0] += v0 * v0;
+        acc[1] += v0 * v0;
+    }
+    block_partials<2>(acc, part);
+}
+__global__ void k3_p(int n, double beta, double omega, const double *r, double *p, const double *v) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        p[q] = r[q] + beta * (p[q] - omega * v[q]);
+}
+// s = r - alpha v; partial (s, s)
+__global__ void k3_s(int n, double alpha, const double *r, const double *v, double *s, double *part) {
+    double acc[1] = {0.0};
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const double t = r[q] - alpha * v[q];
+        s[q] = t;
+        acc[0] += t * t;
+    }
+    block_partials<1>(acc, part);
+}
+// t = A sh; partials (t, t), (t, s)
+__global__ void k3_t(Geo3 g, const double *P, const double *sh, double *t, const double *s, double *part) {
+    double acc[2] = {0.0, 0.0};
+    const int n = g.nint;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int i = q % g.nx, j = (q / g.nx) % g.ny, k = q / (g.nx * g.ny);
+        double sum = 0.0;
+        for (int m = 0; m < 21; ++m) sum += P[(size_t)m * n + q] * sh[col3(g, i, j, k, m)];
+        t[q] = sum;
+        acc[0] += sum * sum;
+        acc[1] += sum * s[q];
+    }
+    block_partials<2>(acc, part);
+}
+// x += alpha ph + omega sh; r = s - omega t; partials (r, r), (r, rh)
+__global__ void k3_xr(int n, double alpha, double omega, double *x, const double *ph, const double *sh, double *r,
+                      const double *s, const double *t, const double *rh, double *part) {
+    double acc[2] = {0.0, 0.0};
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        x[q] += alpha * ph[q] + omega * sh[q];
+        const double v = s[q] - omega * t[q];
+        r[q] = v;
+        acc[0] += v * v;
+        acc[1] += v * rh[q];
+    }
+    block_partials<2>(acc, part);
+}
+__global__ void k3_xa(int n, double alpha, double *x, const double *ph) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) x[q] += alpha * ph[q];
+}
+
+// ---------------------------------------------------------------------
+// after the solve: the increment with its ghosts from the rows, the
+// solute the solver's residual leaves put back evenly, the end state
+// ---------------------------------------------------------------------
+// cn = c_old + d on the real cells (the ghosts follow from the rows)
+__global__ void k_cplus(Geo3 g, const double *cold, const double *dint, double *cn) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nint) return;
+    const int ix = q % g.nx + 2, jy = (q / g.nx) % g.ny + 2, kz = q / (g.nx * g.ny) + 2;
+    const int r = I3(g, ix, jy, kz);
+    cn[r] = cold[r] + dint[q];
+}
+// d = cn - c_old and ct = d theta + c_old, every row
+__global__ void k_cdiff(int n3, double theta, const double *cold, const double *cn, double *d, double *ct) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n3) return;
+    const double v = cn[r] - cold[r];
+    d[r] = v;
+    const double t = v * theta;
+    ct[r] = t + cold[r];
+}
+// partials of sum h (-dt (A ct + B) - d) and sum h (1 + theta dt react)
+__global__ void k_csolute(Args a, const double *ct, const double *d, double *part) {
+    const Geo3 &g = a.g;
+    double acc[2] = {0.0, 0.0};
+    const int n = g.nint;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int ix = q % g.nx + 2, jy = (q / g.nx) % g.ny + 2, kz = q / (g.nx * g.ny) + 2;
+        const int r3 = I3(g, ix, jy, kz);
+        const double xr = ct[r3];
+        double s = a.react[q] * xr;
+        for (int k = 1; k < 21; ++k)
+            s += a.cA[(size_t)k * n + q] * (ct[I3(g, ix + c_cox[k], jy + c_coy[k], kz + c_coz[k])] - xr);
+        const double h = a.fH[I2(g, ix, jy)];
+        acc[0] += h * (-a.dt * (s + 0.0) - d[r3]);
+        acc[1] += h * (1.0 + a.theta * a.dt * a.react[q]);
+    }
+    block_partials<2>(acc, part);
+}
+__global__ void k_cnew(int n3, double alpha, const double *cold, const double *d, double *cnew) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n3) return;
+    cnew[r] = cold[r] + (d[r] + alpha);
+}
+
+// ---------------------------------------------------------------------
+// coupling passes: r = (c0 + c) / 2 - cm with max |r|; Aitken sums
+// ---------------------------------------------------------------------
+__global__ void k_resid(int n3, const double *c0, const double *c, const double *cm, double *r,
+                        unsigned long long *emax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (i < n3) {
+        const double v = (c0[i] + c[i]) * 0.5 - cm[i];
+        r[i] = v;
+        e = fabs(v);
+    }
+    for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o));
+    if ((threadIdx.x & 63) == 0 && e > 0.0) atomicMax(emax, (unsigned long long)__double_as_longlong(e));
+}
+__global__ void k_aitken(int n3, const double *r, const double *rp, double *part) {
+    double acc[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += gridDim.x * blockDim.x) {
+        const double d = r[i] - rp[i];
+        acc[0] += rp[i] * d;
+        acc[1] += d * d;
+    }
+    block_partials<2>(acc, part);
+}
+__global__ void k_relax(int n3, double omega, const double *r, double *cm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) cm[i] += omega * r[i];
+}
+__global__ void k_avg(int n3, const double *x, const double *y, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) out[i] = (x[i] + y[i]) * 0.5;
+}
+__global__ void k_extrap(int n3, double rr, const double *c, const double *prev, double *cm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) { const double v = c[i]; cm[i] = v + rr * (v - prev[i]); }
+}
+
+// ---------------------------------------------------------------------
+// RedistributeExcess (:403-441), one thread per real column; the solute
+// moved in a per-column array
+// ---------------------------------------------------------------------
+__global__ void k_excess(Geo3 g, double *c, double *moved) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
+    const int kb = 2, kt = g.Nz - 3;
+    double mv = 0.0;
+    bool any = false;
+    for (int kz = kb; kz <= kt; kz++) if (c[kz * g.n2 + p] > 1.0) { any = true; break; }
+    if (any) {
+        for (int kz = kt; kz > kb; kz--) {
+            const double ck = c[kz * g.n2 + p];
+            if (ck > 1.0) {
+                const double e = ck - 1.0;
+                c[kz * g.n2 + p] = 1.0;
+                c[(kz - 1) * g.n2 + p] += e;
+                mv += e;
+            }
+        }
+        const double cb = c[kb * g.n2 + p];
+        if (cb > 1.0) {
+            double e = cb - 1.0;
+            c[kb * g.n2 + p] = 1.0;
+            for (int kz = kb + 1; kz <= kt && e > 0.0; kz++) {
+                const double ck = c[kz * g.n2 + p];
+                const double take = fmin(e, 1.0 - ck);
+                if (take > 0.0) { c[kz * g.n2 + p] = ck + take; e -= take; mv += take; }
+            }
+            if (e > 0.0)
+                for (int kz = kb; kz <= kt; kz++) c[kz * g.n2 + p] += e / (double)(kt - kb + 1);
+        }
+    }
+    moved[q] = mv;
+}
+// RedistributeDeficit (:449-570): whether any real cell is below 0
+__global__ void k_below(Geo3 g, const double *c, int *flags) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nint) return;
+    const int ix = t % g.nx + 2, jy = (t / g.nx) % g.ny + 2, kz = t / (g.nx * g.ny) + 2;
+    if (c[I3(g, ix, jy, kz)] < 0.0) atomicOr(&flags[1], 1);
+}
+// within each column: its cells below 0 from its cells above
+__global__ void k_deficit_col(Geo3 g, const double *H, double *c, double *moved, int *flags) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
+    const int kb = 2, kt = g.Nz - 3;
+    const double nLay = (double)(kt - kb + 1);
+    double def = 0.0, pos = 0.0;
+    for (int kz = kb; kz <= kt; kz++) {
+        const double ck = c[kz * g.n2 + p];
+        if (ck < 0.0) def -= ck; else pos += ck;
+    }
+    moved[q] = 0.0;
+    if (def == 0.0) return;
+    moved[q] = def * H[p] / nLay;
+    if (pos >= def) {
+        const double f = def / pos;
+        for (int kz = kb; kz <= kt; kz++) {
+            const double ck = c[kz * g.n2 + p];
+            c[kz * g.n2 + p] = (ck < 0.0) ? 0.0 : ck - f * ck;
+        }
+    } else {
+        const double m = (pos - def) / nLay;
+        for (int kz = kb; kz <= kt; kz++) c[kz * g.n2 + p] = m;
+        atomicOr(&flags[2], 1);
+    }
+}
+__device__ inline double col_mean(const Geo3 &g, const double *c, int p) {
+    double s = 0.0;
+    for (int kz = 2; kz <= g.Nz - 3; kz++) s += c[kz * g.n2 + p];
+    return s / (double)(g.Nz - 4);
+}
+// a pass between columns: room of every column
+__global__ void k_def_room(Geo3 g, const double *H, const double *c, double *room, double *fcol) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
+    const double m = col_mean(g, c, p), h = H[p];
+    room[q] = (m > 0.0 && h > 0.0) ? h * m : 0.0;
+    fcol[q] = 0.0;
+}
+// the columns below 0: what they take from their neighbours (f), their
+// new mean; added: what a column without film makes up
+__global__ void k_def_take(Geo3 g, const double *H, double *c, const double *room, double *fcol, double *added,
+                           int *flags) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int i = q % g.nx, j = q / g.nx;
+    const int p = I2(g, i + 2, j + 2);
+    added[q] = 0.0;
+    const double m = col_mean(g, c, p), h = H[p];
+    if (!(m < 0.0)) return;
+    if (!(h > 0.0)) {
+        added[q] = -(h * m);
+        for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] *= 0.0;
+        return;
+    }
+    const double e = -h * m;
+    const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+    double r = 0.0;
+    for (int t = 0; t < 4; t++) {
+        const int ii = i + di[t], jj = j + dj[t];
+        if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny) r += room[jj * g.nx + ii];
+    }
+    if (!(r > 0.0)) { atomicOr(&flags[3], 1); return; }
+    const double f = (e < r) ? e / r : 1.0;
+    fcol[q] = f;
+    const double mNew = (f < 1.0) ? 0.0 : -(e - r) / h;
+    for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] = mNew;
+    if (f >= 1.0) atomicOr(&flags[3], 1);
+}
+// the neighbours give up what was taken, in the order the reference's
+// column loop adds it (x outer, y inner: west, south, north, east)
+__global__ void k_def_give(Geo3 g, const double *fcol, double *c, int *flags) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int i = q % g.nx, j = q / g.nx;
+    const int p = I2(g, i + 2, j + 2);
+    double t = 0.0;
+    bool got = false;
+    const int di[4] = {-1, 0, 0, 1}, dj[4] = {0, -1, 1, 0};
+    for (int u = 0; u < 4; u++) {
+        const int ii = i + di[u], jj = j + dj[u];
+        if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny) {
+            const double f = fcol[jj * g.nx + ii];
+            if (f > 0.0) { t += f; got = true; }
+        }
+    }
+    if (got && t > 0.0) {
+        for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] *= (1.0 - t);
+        if (col_mean(g, c, p) < 0.0) atomicOr(&flags[3], 1);
+    }
+}
+// what is left: lack and total per column
+__global__ void k_def_left(Geo3 g, const double *H, double *c, double *lack, double *total) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
+    const double m = col_mean(g, c, p), h = H[p];
+    lack[q] = 0.0; total[q] = 0.0;
+    if (m < 0.0) {
+        lack[q] = -(h * m);
+        for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] *= 0.0;
+    } else if (h > 0.0) {
+        total[q] = h * m;
+    }
+}
+__global__ void k_def_scale(Geo3 g, const double *H, double f, double *c) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nx * g.ny) return;
+    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
+    if (H[p] > 0.0 && col_mean(g, c, p) > 0.0)
+        for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] *= (1.0 - f);
+}
+// a sum over the columns in the reference's loop order (x outer, y inner)
+__global__ void k_colsum(Geo3 g, const double *v, double *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j) s += v[j * g.nx + i];
+    *out = s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------
+struct yz_cz {
+    Geo3 g;
+    yz_cz_case cs;
+    double dt = 0;
+    hipStream_t st = nullptr;
+    Args a;
+    Solver Sh;                              // the film's solver
+    // device state
+    double *S = nullptr, *hO = nullptr, *hN = nullptr, *fH = nullptr, *tH = nullptr, *mid = nullptr;
+    double *hO_keep = nullptr, *hN_keep = nullptr;
+    double *fC = nullptr, *tC = nullptr, *cO = nullptr, *cN = nullptr, *c0 = nullptr, *cm = nullptr;
+    double *cPrev = nullptr, *keepO = nullptr, *keepN = nullptr, *prevKeep = nullptr, *r = nullptr, *rp = nullptr;
+    double *cn = nullptr, *d = nullptr, *ct = nullptr;
+    double *f2[16] = {nullptr};
+    double *Qx = nullptr, *Qy = nullptr, *W = nullptr, *zr = nullptr;
+    double *cA = nullptr, *react = nullptr, *cP = nullptr, *cb = nullptr;
+    double *kv[9] = {nullptr};              // x r rh p v s t ph sh (real cells)
+    double *lu[5] = {nullptr};
+    double *part = nullptr, *dsc = nullptr, *hsc = nullptr;
+    double *colv[4] = {nullptr};
+    int *flags = nullptr, *hflags = nullptr;
+    unsigned long long *emax = nullptr;
+    double dtPrev = 1.0;
+    bool prevSet = false;
+    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+template <typename T>
+int dal(T **p, size_t n) {
+    if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+        yz::set_error("ConcZ: device allocation failed");
+        return -1;
+    }
+    return hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)) == hipSuccess ? 0 : -1;
+}
+int cp_d2d(yz_cz *z, double *dst, const double *src, size_t n) {
+    CZ_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, z->st));
+    return 0;
+}
+int sync(yz_cz *z) {
+    CZ_CHECK(hipStreamSynchronize(z->st));
+    CZ_CHECK(hipGetLastError());
+    return 0;
+}
+// read K reduced partials
+int read_sums(yz_cz *z, int K, double *out) {
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(TBK), 0, z->st, z->part, K, z->dsc);
+    CZ_CHECK(hipMemcpyAsync(z->hsc, z->dsc, K * sizeof(double), hipMemcpyDeviceToHost, z->st));
+    if (sync(z)) return -1;
+    for (int k = 0; k < K; ++k) out[k] = z->hsc[k];
+    return 0;
+}
+int read_flags(yz_cz *z) {
+    CZ_CHECK(hipMemcpyAsync(z->hflags, z->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, z->st));
+    return sync(z);
+}
+int zero_flags(yz_cz *z) {
+    CZ_CHECK(hipMemsetAsync(z->flags, 0, 4 * sizeof(int), z->st));
+    return 0;
+}
+double colsum(yz_cz *z, const double *v, int *err) {
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, z->st, z->g, v, z->dsc);
+    if (hipMemcpyAsync(z->hsc, z->dsc, sizeof(double), hipMemcpyDeviceToHost, z->st) != hipSuccess || sync(z)) {
+        *err = 1;
+        return 0;
+    }
+    return z->hsc[0];
+}
+
+int apply_rows(yz_cz *z, double *x) {
+    const Geo3 &g = z->g;
+    const int nc = g.nx * g.ny;
+    hipLaunchKernelGGL(k_cghost_z, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, z->zr, x);
+    hipLaunchKernelGGL(k_cghost_y, dim3(nblk((long)g.nx * 4 * g.Nz)), dim3(TBK), 0, z->st, g, x);
+    hipLaunchKernelGGL(k_cghost_x, dim3(nblk((long)4 * g.Ny * g.Nz)), dim3(TBK), 0, z->st, g, x);
+    return 0;
+}
+
+// SolveStage of the film: assemble at the field, solve into hN
+int h_stage(yz_cz *z, int &iter, double &resid) {
+    const Geo3 &g = z->g;
+    Args &a = z->a;
+    hipLaunchKernelGGL(k_faces, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, a);
+    hipLaunchKernelGGL(k_hrows, dim3(nblk(g.nx * g.ny)), dim3(TBK), 0, z->st, a);
+    if (sync(z)) return -1;
+    if (amg_setup(z->Sh)) { yz::set_error("ConcZ: film preconditioner set-up failed"); return -1; }
+    int it = 0;
+    double rs = 0;
+    const int ch = bicgstab(z->Sh, (const double *)a.hb, a.hx, YZ_PC_AMG, z->cs.lin_maxit, z->cs.h_eps, &it, &rs);
+    iter += it;
+    resid = rs;
+    z->stats[0] += 1;
+    z->stats[1] += it;
+    if (ch != 0) return 1;
+    hipLaunchKernelGGL(k_hscatter, dim3(nblk(g.nx * g.ny)), dim3(TBK), 0, z->st, g, (const double *)a.hx, z->hN);
+    hipLaunchKernelGGL(k_hghosts, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, g, z->hN);
+    return 0;
+}
+void conservative_update(yz_cz *z) {
+    const Geo3 &g = z->g;
+    hipLaunchKernelGGL(k_hghosts, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, g, z->hN);
+    hipLaunchKernelGGL(k_hflux, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, z->a);
+    hipLaunchKernelGGL(k_hupdate, dim3(nblk(g.nx * g.ny)), dim3(TBK), 0, z->st, z->a);
+    hipLaunchKernelGGL(k_hghosts, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, g, z->hN);
+}
+
+// BiCGSTAB on the concentration system (the oracle's and the reference
+// backend's contract: ||r|| <= eps ||b||, 0 / 1 / 2), x = 0 to start
+int c_krylov(yz_cz *z, int &iters, double &resid) {
+    const Geo3 &g = z->g;
+    const int n = g.nint;
+    double *x = z->kv[0], *r = z->kv[1], *rh = z->kv[2], *p = z->kv[3], *v = z->kv[4], *s = z->kv[5], *t = z->kv[6],
+           *ph = z->kv[7], *sh = z->kv[8];
+    const double *P = z->cP;
+    const int nc = g.nx * g.ny;
+    hipLaunchKernelGGL(k3_lu, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, P, z->lu[0], z->lu[1], z->lu[2], z->lu[3], z->lu[4]);
+    CZ_CHECK(hipMemsetAsync(x, 0, n * sizeof(double), z->st));
+    hipLaunchKernelGGL(k3_init, dim3(RB), dim3(TBK), 0, z->st, n, (const double *)z->cb, r, rh, p, v, z->part);
+    double sc[3];
+    if (read_sums(z, 2, sc)) return -1;
+    const double nb = std::sqrt(sc[0]);
+    iters = 0;
+    resid = 0;
+    if (nb == 0) return 0;
+    const double eps = z->cs.c_eps * nb;
+    double res = std::sqrt(sc[1]);
+    if (res <= eps) { resid = res / nb; return 0; }
+    double rho1 = sc[1], rho2 = 1, alpha = 1, omega = 1;   // (r, rh) = (r, r) at the start
+    int it = 0;
+    const int maxit = z->cs.lin_maxit;
+    for (; it < maxit; ++it) {
+        if (std::fabs(rho1) < 1e-300) { iters = it; resid = res / nb; return 2; }
+        const double beta = it == 0 ? 0.0 : (rho1 * alpha) / (rho2 * omega);
+        hipLaunchKernelGGL(k3_p, dim3(RB), dim3(TBK), 0, z->st, n, beta, it == 0 ? 0.0 : omega, (const double *)r, p,
+                           (const double *)v);
+        hipLaunchKernelGGL(k3_prec, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, z->lu[0], z->lu[1], z->lu[2], z->lu[3],
+                           z->lu[4], (const double *)p, ph);
+        hipLaunchKernelGGL(k3_spmv, dim3(RB), dim3(TBK), 0, z->st, g, P, (const double *)ph, v, (const double *)rh,
+                           z->part);
+        if (read_sums(z, 1, sc)) return -1;
+        const double rv = sc[0];
+        if (rv == 0) { iters = it; resid = res / nb; return 2; }
+        alpha = rho1 / rv;
+        hipLaunchKernelGGL(k3_s, dim3(RB), dim3(TBK), 0, z->st, n, alpha, (const double *)r, (const double *)v, s, z->part);
+        if (read_sums(z, 1, sc)) return -1;
+        res = std::sqrt(sc[0]);
+        if (res < eps) {
+            hipLaunchKernelGGL(k3_xa, dim3(RB), dim3(TBK), 0, z->st, n, alpha, x, (const double *)ph);
+            ++it;
+            break;
+        }
+        hipLaunchKernelGGL(k3_prec, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, z->lu[0], z->lu[1], z->lu[2], z->lu[3],
+                           z->lu[4], (const double *)s, sh);
+        hipLaunchKernelGGL(k3_t, dim3(RB), dim3(TBK), 0, z->st, g, P, (const double *)sh, t, (const double *)s, z->part);
+        if (read_sums(z, 2, sc)) return -1;
+        const double tt = sc[0];
+        omega = (tt != 0) ? sc[1] / tt : 0.0;
+        if (omega == 0) { iters = it; resid = res / nb; return 2; }
+        hipLaunchKernelGGL(k3_xr, dim3(RB), dim3(TBK), 0, z->st, n, alpha, omega, x, (const double *)ph,
+                           (const double *)sh, r, (const double *)s, (const double *)t, (const double *)rh, z->part);
+        if (read_sums(z, 2, sc)) return -1;
+        res = std::sqrt(sc[0]);
+        rho2 = rho1;
+        rho1 = sc[1];
+        if (res < eps) { ++it; break; }
+        if (!(res == res)) { iters = it; resid = res; return 2; }
+    }
+    iters = it;
+    resid = res / nb;
+    if (!(resid == resid)) return 2;
+    return resid <= z->cs.c_eps ? 0 : 1;
+}
+
+// SolveStage of the concentration at the field (fC, fH), from cO into cN
+int c_stage(yz_cz *z, int &iter, double &resid) {
+    const Geo3 &g = z->g;
+    Args &a = z->a;
+    hipLaunchKernelGGL(k_cbc, dim3(nblk(g.nx * g.ny)), dim3(TBK), 0, z->st, a, 0);
+    apply_rows(z, z->cO);
+    if (zero_flags(z)) return -1;
+    hipLaunchKernelGGL(k_crows, dim3(nblk(g.nint)), dim3(TBK), 0, z->st, a, (const double *)z->cO);
+    if (read_flags(z)) return -1;
+    const bool any = z->hflags[0] != 0;
+    int ch = 0, it = 0;
+    double rs = 0;
+    if (any) {
+        ch = c_krylov(z, it, rs);
+        if (ch < 0) return -1;
+        z->stats[2] += 1;
+        z->stats[3] += it;
+    } else {
+        CZ_CHECK(hipMemsetAsync(z->kv[0], 0, g.nint * sizeof(double), z->st));
+    }
+    iter += it;
+    resid = rs;
+    if (ch != 0) return 2;
+    // the increment with its ghosts from the rows, then the solute put back
+    CZ_CHECK(hipMemsetAsync(z->cn, 0, g.n3 * sizeof(double), z->st));
+    hipLaunchKernelGGL(k_cplus, dim3(nblk(g.nint)), dim3(TBK), 0, z->st, g, (const double *)z->cO,
+                       (const double *)z->kv[0], z->cn);
+    apply_rows(z, z->cn);
+    hipLaunchKernelGGL(k_cdiff, dim3(nblk(g.n3)), dim3(TBK), 0, z->st, g.n3, z->a.theta, (const double *)z->cO,
+                       (const double *)z->cn, z->d, z->ct);
+    double alpha = 0.0;
+    if (any) {
+        hipLaunchKernelGGL(k_csolute, dim3(RB), dim3(TBK), 0, z->st, a, (const double *)z->ct, (const double *)z->d,
+                           z->part);
+        double sc[2];
+        if (read_sums(z, 2, sc)) return -1;
+        alpha = (sc[1] > 0.0) ? sc[0] / sc[1] : 0.0;
+    }
+    hipLaunchKernelGGL(k_cnew, dim3(nblk(g.n3)), dim3(TBK), 0, z->st, g.n3, alpha, (const double *)z->cO,
+                       (const double *)z->d, z->cN);
+    apply_rows(z, z->cN);
+    return 0;
+}
+
+// the velocity through the film and the volume fluxes of the pass, at the
+// field (fH, fC); BuildVolumeFluxes reads nothing the concentration's
+// stages change, so both stages share them
+void volume_fluxes(yz_cz *z) {
+    const Geo3 &g = z->g;
+    hipLaunchKernelGGL(k_faces3, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, z->a);
+    hipLaunchKernelGGL(k_w, dim3(nblk(g.nx * g.ny)), dim3(TBK), 0, z->st, z->a);
+}
+
+int c_solve_at_mid(yz_cz *z, const double *cmid, int &iter, double &resid) {
+    const size_t n3 = z->g.n3;
+    if (cp_d2d(z, z->fC, cmid, n3)) return -1;
+    const int ch = c_stage(z, iter, resid);
+    if (cp_d2d(z, z->fC, z->cO, n3)) return -1;
+    return ch;
+}
+int c_solve(yz_cz *z, int &iter, double &resid) {
+    const Geo3 &g = z->g;
+    int ch = c_stage(z, iter, resid);
+    if (ch != 0) return ch;
+    // the midpoint of the prediction
+    hipLaunchKernelGGL(k_avg, dim3(nblk(g.n3)), dim3(TBK), 0, z->st, g.n3, (const double *)z->cO,
+                       (const double *)z->cN, z->fC);
+    ch = c_stage(z, iter, resid);
+    if (cp_d2d(z, z->fC, z->cO, g.n3)) return -1;
+    return ch;
+}
+
+int redistribute_deficit(yz_cz *z, double &moved, double &added) {
+    const Geo3 &g = z->g;
+    const int nc = g.nx * g.ny;
+    moved = added = 0;
+    int err = 0;
+    if (zero_flags(z)) return -1;
+    hipLaunchKernelGGL(k_below, dim3(nblk(g.nint)), dim3(TBK), 0, z->st, g, (const double *)z->cN, z->flags);
+    if (read_flags(z)) return -1;
+    if (!z->hflags[1]) return 0;
+    hipLaunchKernelGGL(k_deficit_col, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH, z->cN,
+                       z->colv[0], z->flags);
+    moved = colsum(z, z->colv[0], &err);
+    if (err || read_flags(z)) return -1;
+    if (!z->hflags[2]) return 0;
+    bool left = true;
+    for (int pass = 0; pass < 4 && left; ++pass) {
+        hipLaunchKernelGGL(k_def_room, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH,
+                           (const double *)z->cN, z->colv[0], z->colv[1]);
+        if (zero_flags(z)) return -1;
+        hipLaunchKernelGGL(k_def_take, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH, z->cN,
+                           (const double *)z->colv[0], z->colv[1], z->colv[2], z->flags);
+        added += colsum(z, z->colv[2], &err);
+        hipLaunchKernelGGL(k_def_give, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->colv[1], z->cN,
+                           z->flags);
+        if (err || read_flags(z)) return -1;
+        left = z->hflags[3] != 0;
+    }
+    if (!left) return 0;
+    hipLaunchKernelGGL(k_def_left, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH, z->cN, z->colv[0],
+                       z->colv[1]);
+    const double lack = colsum(z, z->colv[0], &err);
+    const double total = colsum(z, z->colv[1], &err);
+    if (err) return -1;
+    const double f = (total > 0.0) ? std::min(1.0, lack / total) : 0.0;
+    if (f > 0.0) hipLaunchKernelGGL(k_def_scale, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH, f, z->cN);
+    if (total < lack) added += lack - total;
+    return 0;
+}
+
+// one step (TimeAdvance, Solve, AfterTimeStepTrial)
+int cz_advance(yz_cz *z, int &lin, int &passes_out, double &resid_out) {
+    const Geo3 &g = z->g;
+    const size_t n2 = g.n2, n3 = g.n3;
+    lin = 0; passes_out = 0; resid_out = 0;
+    auto restore = [&]() -> int {
+        if (cp_d2d(z, z->hO, z->hO_keep, n2) || cp_d2d(z, z->hN, z->hN_keep, n2) || cp_d2d(z, z->fH, z->tH, n2) ||
+            cp_d2d(z, z->fC, z->tC, n3))
+            return -1;
+        return sync(z);
+    };
+    if (cp_d2d(z, z->hO_keep, z->hO, n2) || cp_d2d(z, z->hN_keep, z->hN, n2)) return -1;
+    if (cp_d2d(z, z->hO, z->hN, n2) || cp_d2d(z, z->fH, z->tH, n2) || cp_d2d(z, z->fC, z->tC, n3)) return -1;
+    int iter = 0;
+    double resid = 0;
+    int ch = h_stage(z, iter, resid);
+    if (ch < 0) return -1;
+    if (ch != 0) { if (restore()) return -1; return 1; }
+    hipLaunchKernelGGL(k_hmid, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, z->a, z->fH, 0);
+    const bool extrap = z->prevSet;
+    if (extrap) {
+        const double rr = 0.5 * z->dt / z->dtPrev;
+        hipLaunchKernelGGL(k_extrap, dim3(nblk(g.n3)), dim3(TBK), 0, z->st, g.n3, rr, (const double *)z->cN,
+                           (const double *)z->cPrev, z->cm);
+        if (cp_d2d(z, z->fC, z->cm, n3)) return -1;
+    }
+    ch = h_stage(z, iter, resid);
+    if (ch < 0) return -1;
+    if (cp_d2d(z, z->fH, z->hO, n2)) return -1;
+    if (extrap && cp_d2d(z, z->fC, z->cN, n3)) return -1;
+    if (ch != 0) { if (restore()) return -1; return 1; }
+
+    conservative_update(z);
+    if (cp_d2d(z, z->fH, z->hN, n2)) return -1;
+    if (cp_d2d(z, z->c0, z->cN, n3)) return -1;
+    if (!extrap && cp_d2d(z, z->cm, z->c0, n3)) return -1;
+    if (cp_d2d(z, z->keepO, z->cO, n3) || cp_d2d(z, z->keepN, z->cN, n3) || cp_d2d(z, z->prevKeep, z->cPrev, n3))
+        return -1;
+    const double dtPrevKeep = z->dtPrev;
+    const bool prevSetKeep = z->prevSet;
+    if (cp_d2d(z, z->cPrev, z->c0, n3)) return -1;
+    z->dtPrev = z->dt;
+    z->prevSet = true;
+    const int maxPass = z->cs.marangoni ? 1 + 8 : 1;
+    const double floor = std::max(1e-10, 10.0 * z->cs.c_eps);
+    double e0 = 0, e = 0, omega = 1.0;
+    int passes = 0, fail = 0, pass;
+    for (pass = 0; pass < maxPass; pass++) {
+        hipLaunchKernelGGL(k_hmid, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, z->a, z->mid, 0);
+        if (pass > 0) {
+            if (cp_d2d(z, z->fH, z->mid, n2) || cp_d2d(z, z->fC, z->cm, n3)) return -1;
+            int it2 = 0;
+            double r2 = 0;
+            ch = h_stage(z, it2, r2);
+            if (ch < 0) return -1;
+            iter += it2;
+            resid = r2;
+            if (cp_d2d(z, z->fC, z->c0, n3)) return -1;
+            if (ch != 0) { fail = 1; break; }
+            conservative_update(z);
+        }
+        hipLaunchKernelGGL(k_hmid, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, z->a, z->mid, 1);
+        if (cp_d2d(z, z->fH, z->mid, n2) || cp_d2d(z, z->fC, z->cm, n3)) return -1;
+        volume_fluxes(z);
+        if (cp_d2d(z, z->fC, z->c0, n3)) return -1;
+        int cit = 0;
+        double crs = 0;
+        int cch;
+        if (pass > 0) cch = c_solve_at_mid(z, z->cm, cit, crs);
+        else if (maxPass > 1) {
+            if (cp_d2d(z, z->cO, z->cN, n3)) return -1;
+            cch = c_solve_at_mid(z, z->cm, cit, crs);
+        } else {
+            if (cp_d2d(z, z->cO, z->cN, n3)) return -1;
+            cch = c_solve(z, cit, crs);
+        }
+        if (cch < 0) return -1;
+        iter += cit;
+        if (cch != 0) { fail = 2; break; }
+        if (maxPass == 1) break;
+        passes = pass;
+        CZ_CHECK(hipMemsetAsync(z->emax, 0, sizeof(unsigned long long), z->st));
+        hipLaunchKernelGGL(k_resid, dim3(nblk(g.n3)), dim3(TBK), 0, z->st, g.n3, (const double *)z->c0,
+                           (const double *)z->cN, (const double *)z->cm, z->r, z->emax);
+        unsigned long long eb = 0;
+        CZ_CHECK(hipMemcpyAsync(&eb, z->emax, sizeof(eb), hipMemcpyDeviceToHost, z->st));
+        if (sync(z)) return -1;
+        std::memcpy(&e, &eb, sizeof(e));
+        if (pass == 0) e0 = e;
+        const bool settled = !(e > 0.1 * e0) || e <= floor;
+        if (pass > 0 && !settled) {
+            hipLaunchKernelGGL(k_aitken, dim3(RB), dim3(TBK), 0, z->st, g.n3, (const double *)z->r,
+                               (const double *)z->rp, z->part);
+            double sc[2];
+            if (read_sums(z, 2, sc)) return -1;
+            if (sc[1] > 0) omega = -omega * sc[0] / sc[1];
+        }
+        if (settled) break;
+        hipLaunchKernelGGL(k_relax, dim3(nblk(g.n3)), dim3(TBK), 0, z->st, g.n3, omega, (const double *)z->r, z->cm);
+        if (cp_d2d(z, z->rp, z->r, n3)) return -1;
+    }
+    if (!fail && maxPass > 1 && pass == maxPass) fail = 3;
+    if (cp_d2d(z, z->fH, z->hN, n2)) return -1;
+    lin = iter;
+    resid_out = resid;
+    if (fail) {
+        if (cp_d2d(z, z->cO, z->keepO, n3) || cp_d2d(z, z->cN, z->keepN, n3) || cp_d2d(z, z->cPrev, z->prevKeep, n3))
+            return -1;
+        z->dtPrev = dtPrevKeep;
+        z->prevSet = prevSetKeep;
+        if (restore()) return -1;
+        return fail;
+    }
+    z->stats[4] += passes;
+    passes_out = passes;
+    // RedistributeExcess, RedistributeDeficit
+    const int nc = g.nx * g.ny;
+    hipLaunchKernelGGL(k_excess, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, z->cN, z->colv[3]);
+    int err = 0;
+    z->stats[5] += colsum(z, z->colv[3], &err);
+    if (err) return -1;
+    double mv = 0, ad = 0;
+    if (redistribute_deficit(z, mv, ad)) return -1;
+    z->stats[6] += mv;
+    z->stats[7] += ad;
+    // the ghosts of the end state from its own conditions (RefreshGhosts)
+    if (cp_d2d(z, z->fC, z->cN, n3)) return -1;
+    hipLaunchKernelGGL(k_cbc, dim3(nblk(nc)), dim3(TBK), 0, z->st, z->a, 1);
+    apply_rows(z, z->cN);
+    if (cp_d2d(z, z->fC, z->cN, n3)) return -1;
+    if (cp_d2d(z, z->tH, z->fH, n2) || cp_d2d(z, z->tC, z->fC, n3)) return -1;
+    return sync(z);
+}
+
+}  // namespace
+
+extern "C" {
+
+void yz_cz_case_defaults(yz_cz_case *c) {
+    std::memset(c, 0, sizeof(*c));
+    c->nx = c->ny = 16; c->nz = 8;
+    c->xmin = 0; c->xmax = 1; c->ymin = 0; c->ymax = 1;
+    c->capillary = 1;
+    c->As = 1; c->sig_c0 = 0.5; c->visc_c0 = 0.5; c->Df = 1; c->diff_c0 = 0.5;
+    c->Ev = 1; c->evap_coef = 1; c->evap_func = YZ_EVAP_RAOULT; c->evap_power = 0.5;
+    c->molar_resin = 2200; c->molar_solvent = 106;
+    c->conc_scheme = YZ_CONV_UTOPIA;
+    c->lin_maxit = 5000; c->h_eps = 1e-13; c->c_eps = 1e-13;
+}
+
+int yz_sizeof_cz_case(void) { return (int)sizeof(yz_cz_case); }
+
+yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const double *conc, double dt) {
+    if (!c || !h || !s || !conc) { yz::set_error("ConcZ: null argument"); return nullptr; }
+    if (c->nx < 4 || c->ny < 4 || c->nz < 2) {
+        yz::set_error("ConcZ: the film is 2-D here (nx, ny >= 4, nz >= 2); a 1-D film (Ny = 1) is not taken");
+        return nullptr;
+    }
+    for (int k = 0; k < 4; ++k)
+        if (c->bc[k] != YZ_BC_SYMMETRY && c->bc[k] != YZ_BC_PERIODIC) {
+            yz::set_error("ConcZ: only Symmetry and Periodic edges are taken");
+            return nullptr;
+        }
+    if ((c->bc[0] == YZ_BC_PERIODIC) != (c->bc[1] == YZ_BC_PERIODIC) ||
+        (c->bc[2] == YZ_BC_PERIODIC) != (c->bc[3] == YZ_BC_PERIODIC)) {
+        yz::set_error("ConcZ: a periodic edge needs its opposite edge periodic");
+        return nullptr;
+    }
+    if (c->conc_scheme < YZ_CONV_UPWIND1 || c->conc_scheme > YZ_CONV_KK) {
+        yz::set_error("ConcZ: unknown concentration scheme (the limited schemes are not taken)");
+        return nullptr;
+    }
+    if (!(dt > 0) || !(c->xmax > c->xmin) || !(c->ymax > c->ymin) || c->lin_maxit < 1) {
+        yz::set_error("ConcZ: bad dt, extent or iteration cap");
+        return nullptr;
+    }
+    yz_cz *z = new yz_cz();
+    z->cs = *c;
+    z->dt = dt;
+    Geo3 &g = z->g;
+    g.nx = c->nx; g.ny = c->ny; g.nz = c->nz;
+    g.Nx = c->nx + 4; g.Ny = c->ny + 4; g.Nz = c->nz + 4;
+    g.n2 = g.Nx * g.Ny; g.n3 = g.n2 * g.Nz; g.nint = g.nx * g.ny * g.nz;
+    // the grid with its ghosts (Grids.cpp:155-184)
+    const double dx0 = (c->xmax - c->xmin) / (double)c->nx, dy0 = (c->ymax - c->ymin) / (double)c->ny;
+    g.xlo = c->xmin - 2 * dx0; g.ylo = c->ymin - 2 * dy0;
+    g.dx = ((c->xmax + 2 * dx0) - g.xlo) / (double)g.Nx;
+    g.dy = ((c->ymax + 2 * dy0) - g.ylo) / (double)g.Ny;
+    const double dz0 = (1.0 - 0.0) / (double)c->nz;
+    g.dz = ((1.0 + 2 * dz0) - (0.0 - 2 * dz0)) / (double)g.Nz;
+    for (int k = 0; k < 4; ++k) g.wall[k] = c->bc[k] != YZ_BC_PERIODIC;
+    g.perx = !g.wall[0]; g.pery = !g.wall[2];
+    bool ok = hipStreamCreateWithFlags(&z->st, hipStreamNonBlocking) == hipSuccess;
+    const size_t n2 = g.n2, n3 = g.n3, ni = g.nint;
+    double **a2[] = {&z->S, &z->hO, &z->hN, &z->fH, &z->tH, &z->mid, &z->hO_keep, &z->hN_keep};
+    for (double **p : a2) ok = ok && !dal(p, n2);
+    for (int k = 0; k < 16; ++k) ok = ok && !dal(&z->f2[k], n2);
+    double **a3[] = {&z->fC, &z->tC, &z->cO, &z->cN, &z->c0, &z->cm, &z->cPrev, &z->keepO, &z->keepN,
+                     &z->prevKeep, &z->r, &z->rp, &z->cn, &z->d, &z->ct, &z->Qx, &z->Qy, &z->W};
+    for (double **p : a3) ok = ok && !dal(p, n3);
+    ok = ok && !dal(&z->zr, 4 * n2);
+    ok = ok && !dal(&z->cA, 21 * ni) && !dal(&z->cP, 21 * ni) && !dal(&z->react, ni) && !dal(&z->cb, ni);
+    for (int k = 0; k < 9; ++k) ok = ok && !dal(&z->kv[k], ni);
+    for (int k = 0; k < 5; ++k) ok = ok && !dal(&z->lu[k], ni);
+    for (int k = 0; k < 4; ++k) ok = ok && !dal(&z->colv[k], (size_t)g.nx * g.ny);
+    ok = ok && !dal(&z->part, 4 * RB) && !dal(&z->dsc, 8) && !dal(&z->flags, 4) && !dal(&z->emax, 1);
+    ok = ok && hipHostMalloc((void **)&z->hsc, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&z->hflags, 4 * sizeof(int), hipHostMallocDefault) == hipSuccess;
+    // the film's solver over the real cells
+    StencilOp op;
+    std::memset(&op, 0, sizeof(op));
+    op.dim = 2; op.nx = g.nx; op.ny = g.ny; op.N = g.nx * g.ny; op.ns = 13;
+    op.perx = g.perx; op.pery = g.pery;
+    {
+        const int ox[13] = {0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0};
+        const int oy[13] = {-2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2};
+        for (int k = 0; k < 13; ++k) { op.ox[k] = ox[k]; op.oy[k] = oy[k]; }
+    }
+    double *hb = nullptr, *hx = nullptr;
+    ok = ok && solver_init(z->Sh, op, z->st) == 0;
+    ok = ok && !dal(&hb, (size_t)op.N) && !dal(&hx, (size_t)op.N);
+    if (!ok) {
+        if (yz_last_error()[0] == 0) yz::set_error("ConcZ: device set-up failed");
+        yz_cz_destroy(z);
+        return nullptr;
+    }
+    ok = hipMemcpy(z->S, s, n2 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(z->hN, h, n2 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(z->cN, conc, n3 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(z->hO, z->hN, n2 * sizeof(double), hipMemcpyDeviceToDevice) == hipSuccess &&
+         hipMemcpy(z->fH, z->hN, n2 * sizeof(double), hipMemcpyDeviceToDevice) == hipSuccess &&
+         hipMemcpy(z->tH, z->hN, n2 * sizeof(double), hipMemcpyDeviceToDevice) == hipSuccess &&
+         hipMemcpy(z->cO, z->cN, n3 * sizeof(double), hipMemcpyDeviceToDevice) == hipSuccess &&
+         hipMemcpy(z->fC, z->cN, n3 * sizeof(double), hipMemcpyDeviceToDevice) == hipSuccess &&
+         hipMemcpy(z->tC, z->cN, n3 * sizeof(double), hipMemcpyDeviceToDevice) == hipSuccess;
+    if (!ok) { yz::set_error("ConcZ: upload failed"); yz_cz_destroy(z); return nullptr; }
+    Args &a = z->a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = g; a.cs = *c; a.dt = dt; a.theta = 0.5;
+    {
+        double p[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
+        switch (c->conc_scheme) {
+        case YZ_CONV_UPWIND1: p[1] = 1; p[2] = 1; q[1] = 1; q[2] = -1; break;
+        case YZ_CONV_CENTRAL2: p[1] = 1; p[2] = 1; break;
+        case YZ_CONV_CENTRAL4: p[0] = -1; p[1] = 9; p[2] = 9; p[3] = -1; break;
+        case YZ_CONV_UPWIND2: p[0] = -1; p[1] = 3; p[2] = 3; p[3] = -1; q[0] = -1; q[1] = 3; q[2] = -3; q[3] = 1; break;
+        case YZ_CONV_QUICK: p[0] = -1; p[1] = 9; p[2] = 9; p[3] = -1; q[0] = -1; q[1] = 3; q[2] = -3; q[3] = 1; break;
+        case YZ_CONV_KK: p[0] = -1; p[1] = 7; p[2] = 7; p[3] = -1; q[0] = -3; q[1] = 9; q[2] = -9; q[3] = 3; break;
+        default: p[0] = -1; p[1] = 7; p[2] = 7; p[3] = -1; q[0] = -1; q[1] = 3; q[2] = -3; q[3] = 1; break;
+        }
+        double den = 0;
+        for (int j = 0; j < 4; ++j) den += p[j];
+        for (int j = 0; j < 4; ++j) { a.w1[j] = p[j] / den; a.w2[j] = q[j] / den; }
+    }
+    a.S = z->S; a.hO = z->hO; a.hN = z->hN; a.fH = z->fH; a.tH = z->tH; a.fC = z->fC; a.tC = z->tC;
+    a.Dx = z->f2[0]; a.Dy = z->f2[1]; a.sgx = z->f2[2]; a.sgy = z->f2[3]; a.Mx = z->f2[4]; a.My = z->f2[5];
+    a.qCapX = z->f2[6]; a.qExpX = z->f2[7]; a.qCapY = z->f2[8]; a.qExpY = z->f2[9]; a.qX = z->f2[10];
+    a.qY = z->f2[11]; a.colSrc = z->f2[12]; a.dtdh = z->f2[13];
+    a.Qx = z->Qx; a.Qy = z->Qy; a.W = z->W; a.zr = z->zr;
+    a.hP = z->Sh.A; a.hb = hb; a.hx = hx;
+    a.cA = z->cA; a.react = z->react; a.cP = z->cP; a.cb = z->cb; a.flags = z->flags;
+    return z;
+}
+
+int yz_cz_advance(yz_cz *z, int *lin_iter, int *passes, double *resid) {
+    if (!z) { yz::set_error("ConcZ: null context"); return -1; }
+    int l = 0, p = 0;
+    double r = 0;
+    const int rc = cz_advance(z, l, p, r);
+    if (lin_iter) *lin_iter = l;
+    if (passes) *passes = p;
+    if (resid) *resid = r;
+    return rc;
+}
+
+void yz_cz_get_h(yz_cz *z, double *h) {
+    if (!z || !h) return;
+    (void)hipStreamSynchronize(z->st);
+    (void)hipMemcpy(h, z->hN, z->g.n2 * sizeof(double), hipMemcpyDeviceToHost);
+}
+void yz_cz_get_c(yz_cz *z, double *c) {
+    if (!z || !c) return;
+    (void)hipStreamSynchronize(z->st);
+    (void)hipMemcpy(c, z->cN, z->g.n3 * sizeof(double), hipMemcpyDeviceToHost);
+}
+void yz_cz_volume_fluxes(yz_cz *z, double *qx, double *qy, double *w) {
+    if (!z) return;
+    (void)hipStreamSynchronize(z->st);
+    const size_t b = z->g.n3 * sizeof(double);
+    if (qx) (void)hipMemcpy(qx, z->Qx, b, hipMemcpyDeviceToHost);
+    if (qy) (void)hipMemcpy(qy, z->Qy, b, hipMemcpyDeviceToHost);
+    if (w) (void)hipMemcpy(w, z->W, b, hipMemcpyDeviceToHost);
+}
+void yz_cz_stats(yz_cz *z, double *out8) {
+    if (!z || !out8) return;
+    for (int k = 0; k < 8; ++k) out8[k] = z->stats[k];
+}
+void yz_cz_destroy(yz_cz *z) {
+    if (!z) return;
+    if (z->st) (void)hipStreamSynchronize(z->st);
+    solver_free(z->Sh);
+    double *ptrs[] = {z->S, z->hO, z->hN, z->fH, z->tH, z->mid, z->hO_keep, z->hN_keep, z->fC, z->tC, z->cO,
+                      z->cN, z->c0, z->cm, z->cPrev, z->keepO, z->keepN, z->prevKeep, z->r, z->rp, z->cn, z->d,
+                      z->ct, z->Qx, z->Qy, z->W, z->zr, z->cA, z->react, z->cP, z->cb, z->part, z->dsc,
+                      z->a.hb, z->a.hx};
+    for (double *p : ptrs) if (p) (void)hipFree(p);
+    for (double *p : z->f2) if (p) (void)hipFree(p);
+    for (double *p : z->kv) if (p) (void)hipFree(p);
+    for (double *p : z->lu) if (p) (void)hipFree(p);
+    for (double *p : z->colv) if (p) (void)hipFree(p);
+    if (z->flags) (void)hipFree(z->flags);
+    if (z->emax) (void)hipFree(z->emax);
+    if (z->hsc) (void)hipHostFree(z->hsc);
+    if (z->hflags) (void)hipHostFree(z->hflags);
+    if (z->st) (void)hipStreamDestroy(z->st);
+    delete z;
+}
+
+}  // extern "C"

@@ -35,6 +35,7 @@ namespace yz {
 
 static thread_local std::string g_err;
 static void set_err(const std::string &s) { g_err = s; }
+void set_error(const std::string &s) { g_err = s; }
 
 #define YZ_TRY(x)                                                                           \
     do {                                                                                    \

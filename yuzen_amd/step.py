@@ -272,3 +272,57 @@ class HipBatch:
         out = np.zeros(4)
         self.lib.yz_batch_stats(self.p, _pd(out))
         return out
+
+
+class HipCZ:
+    """One ConcZ film on the device (yz_cz_*): a 2-D film and its
+    concentration through the thickness, stepped as CZ_HeightEquation_2D::
+    TimeAdvance steps it."""
+
+    def __init__(self, case, h, s, c, dt):
+        self.lib = capi.load()
+        self.case = case.copy()
+        self.n2 = (case.nx + 4) * (case.ny + 4)
+        self.n3 = self.n2 * (case.nz + 4)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        assert h.size == self.n2 and s.size == self.n2 and c.size == self.n3
+        self.p = self.lib.yz_cz_create(C.byref(self.case), _pd(h), _pd(s), _pd(c), dt)
+        if not self.p:
+            raise RuntimeError("yz_cz_create: " + capi.last_error())
+
+    def close(self):
+        if getattr(self, "p", None):
+            self.lib.yz_cz_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
+
+    def advance(self):
+        lin, ps, r = C.c_int(), C.c_int(), C.c_double()
+        rc = self.lib.yz_cz_advance(self.p, C.byref(lin), C.byref(ps), C.byref(r))
+        if rc < 0:
+            raise RuntimeError("yz_cz_advance: " + capi.last_error())
+        return rc, lin.value, ps.value, r.value
+
+    def h(self):
+        out = np.empty(self.n2)
+        self.lib.yz_cz_get_h(self.p, _pd(out))
+        return out
+
+    def c(self):
+        out = np.empty(self.n3)
+        self.lib.yz_cz_get_c(self.p, _pd(out))
+        return out
+
+    def volume_fluxes(self):
+        qx, qy, w = np.empty(self.n3), np.empty(self.n3), np.empty(self.n3)
+        self.lib.yz_cz_volume_fluxes(self.p, _pd(qx), _pd(qy), _pd(w))
+        return qx, qy, w
+
+    def stats(self):
+        out = np.empty(8)
+        self.lib.yz_cz_stats(self.p, _pd(out))
+        return out
