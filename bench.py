@@ -25,7 +25,9 @@ synchronisations.
     the solves started from zero.)
 The line reports both, their ratio and how far apart their films end, and
 (unless --no-sweep) the batched parameter sweep: 256 films advanced
-together, in cell-steps/s, beside the same films advanced one at a time.
+together, in cell-steps/s, beside the same films advanced one at a time;
+and (unless --no-concz) the ConcZ film of config 5, 256^2 x 32, in FP64
+steps/s beside the CPU oracle's rate.
 
 Roofline: the dominant kernel of the learned run, the fine-level stencil
 product of the fused Krylov iteration (k_f_av: v = A ph with (rh, v)),
@@ -186,6 +188,52 @@ def sweep_leg(n_films=256, steps=3, warmup=1, alone=16):
     }
 
 
+def concz_leg(steps=10, warmup=2, cpu=True):
+    """Config 5, the ConcZ film (``yuzen_amd.cases.concz_2d``: 256^2 x 32,
+    capillary and Marangoni flow with the coupling passes, evaporation,
+    fixed dt 0.01): FP64 steps/s on the device, and the CPU oracle's rate
+    on one step of the same film at 64^2 x 32 (cells per second)."""
+    from yuzen_amd import cases
+    from yuzen_amd.step import HipCZ
+    import torch
+    c, h, s, cc, dt, _ = cases.concz_2d()
+    g = HipCZ(c, h, s, cc, dt)
+    for _ in range(warmup):
+        if g.advance()[0] != 0:
+            raise RuntimeError("ConcZ warm-up step failed")
+    s0 = g.stats()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        if g.advance()[0] != 0:
+            raise RuntimeError("ConcZ step failed")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    d = (g.stats() - s0) / steps
+    g.close()
+    cells = c.nx * c.ny * c.nz
+    out = {
+        "workload": f"ConcZ {c.nx}x{c.ny}x{c.nz}, dt {dt}, capillary + Marangoni + evaporation",
+        "dtype": "f64", "steps": steps, "warmup": warmup,
+        "steps_per_s": round(steps / el, 3), "ms_per_step": round(1000 * el / steps, 3),
+        "cell_steps_per_s": round(cells * steps / el, 1),
+        "film_solves_per_step": round(float(d[0]), 2), "film_iters_per_step": round(float(d[1]), 1),
+        "conc_solves_per_step": round(float(d[2]), 2), "conc_iters_per_step": round(float(d[3]), 1),
+        "coupling_passes_per_step": round(float(d[4]), 2),
+    }
+    if cpu:
+        from oracle import binding as ob
+        n = 64
+        co, ho, so, cco, dto, _ = cases.concz_2d(n=n, L=10.0 * n / c.nx)
+        o = ob.OracleCZ(co, ho, so, cco, dto)
+        t = time.perf_counter()
+        rc = o.advance()[0]
+        el = time.perf_counter() - t
+        out["cpu_oracle"] = {"cell_steps_per_s": round(n * n * co.nz / el, 1), "cores": 1, "kind": "port",
+                             "sample": f"one step of the same film at {n}x{n}x{co.nz} (same dx, dt), rc {rc}"}
+    return out
+
+
 def run(c, h, s, cc, steps, warmup):
     from yuzen_amd.step import HipStep
     g = HipStep(c, h, s, cc, DT)
@@ -225,11 +273,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched sweep")
+    ap.add_argument("--no-concz", action="store_true", help="skip the ConcZ film (config 5)")
+    ap.add_argument("--only-concz", action="store_true",
+                    help="run the ConcZ film alone and print its object (not the bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the step is single-GPU and no multi-GPU figure is produced", file=sys.stderr)
         sys.exit(2)
+    if args.only_concz:
+        print(json.dumps(concz_leg(cpu=not args.no_cpu)), flush=True)
+        return
 
     from yuzen_amd import capi, cases
 
@@ -349,6 +403,11 @@ def main():
             out["batched_sweep"] = sweep_leg()
         except Exception as e:  # reported, not fatal to the headline line
             out["batched_sweep"] = {"error": str(e)[:200]}
+    if not args.no_concz:
+        try:
+            out["concz"] = concz_leg(cpu=not args.no_cpu)
+        except Exception as e:  # reported, never hides the headline
+            out["concz"] = {"error": str(e)[:200]}
     if not args.no_cpu:
         pp = (P["passes"] if P is not None else L["passes"]) / args.steps
         out["cpu_baseline"] = cpu_baseline(args.n, pp)

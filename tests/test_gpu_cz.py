@@ -22,7 +22,7 @@ import pytest
 
 from oracle import binding as ob
 from tests.cz_cases import _allphys, _case, _grid, _real2, _real3, _solute
-from yuzen_amd import capi
+from yuzen_amd import capi, cases
 from yuzen_amd.step import HipCZ
 
 pytestmark = pytest.mark.gpu
@@ -117,13 +117,9 @@ def test_unsettled_passes_fail_on_both_and_keep_the_start():
 
 
 def _big(evap=1):
-    n, nz, L = 256, 32, 10.0
-    c = _case(n=n, nz=nz, L=L, marangoni=1, evaporation=evap, Ma=1.0, As=0.005, visc_exp=5.0, Df=1.0, Ev=0.05)
-    x = _grid(n, L)
-    X, Y = np.meshgrid(x, x)
-    h = (1.0 + 0.05 * np.cos(2 * np.pi * X / L) * np.cos(2 * np.pi * Y / L)).ravel()
-    cc = np.full((nz + 4) * (n + 4) ** 2, 0.3)
-    return c, h, np.zeros_like(h), cc
+    c, h, s, cc, _, _ = cases.concz_2d()
+    c.evaporation = evap
+    return c, h, s, cc
 
 
 def test_config5_size_solute_volume_and_uniformity():

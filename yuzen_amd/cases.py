@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 from .capi import (YzCase, case_defaults, BC_SYMMETRY, BC_PERIODIC,
-                   PC_AMG, PC_LEARNED)
+                   PC_AMG, PC_LEARNED, cz_case_defaults)
 
 _M64 = (1 << 64) - 1
 
@@ -202,3 +202,34 @@ def sweep_film(k, seed=4242, side=None, dx=0.01, dt=2e-6, nk_eps=1e-8):
 def sweep(n_films=256, seed=4242, **kw):
     """The batched parameter sweep: n_films films of sweep_film."""
     return [sweep_film(k, seed=seed, **kw) for k in range(n_films)]
+
+
+def concz_2d(n=256, nz=32, L=10.0, dt=0.01, steps=500, c0=0.3):
+    """Config 5, the ConcZ film: n x n cells x nz levels through the film,
+    periodic, h = 1 + 0.05 cos(2 pi x / L) cos(2 pi y / L) on a flat
+    substrate, c0 = 0.3 uniform; capillary and Marangoni flow
+    (sigma = 1 + Ma (c_s - c0), Ma = 1, on the surface concentration),
+    mu = exp(5 (c - c0)), Df = 1 with a uniform diffusivity, Raoult
+    evaporation with Ev = 0.05; As = Ev / (Pe Df) = 0.005 for Pe = 10
+    (Pe = E0 h0 / D0, Ev = E0 T0 / h0, Df = D0 T0 / L0^2:
+    NonDimensionalNumbers.cpp:119-123); fixed dt.
+
+    Returns ``(case, h, s, c, dt, steps)``: h and s padded
+    (n+4)^2, c over every cell of the 3-D grid with its ghosts."""
+    c = cz_case_defaults()
+    c.nx = c.ny = n
+    c.nz = nz
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, L, 0.0, L
+    for k in range(4):
+        c.bc[k] = BC_PERIODIC
+    c.capillary, c.marangoni, c.evaporation = 1, 1, 1
+    c.As, c.Ma, c.sig_c0 = 0.005, 1.0, c0
+    c.visc_c0, c.visc_exp = c0, 5.0
+    c.Df, c.diff_c0, c.diff_exp = 1.0, c0, 0.0
+    c.Ev = 0.05
+    dx = L / n
+    x = -2 * dx + (np.arange(n + 4) + 0.5) * dx
+    X, Y = np.meshgrid(x, x)
+    h = (1.0 + 0.05 * np.cos(2 * np.pi * X / L) * np.cos(2 * np.pi * Y / L)).ravel()
+    cc = np.full((nz + 4) * (n + 4) ** 2, c0)
+    return c, h, np.zeros_like(h), cc, dt, steps
