@@ -72,6 +72,7 @@ struct Geo3 {
     double dx, dy, dz, xlo, ylo;
     int wall[4];
     int perx, pery;
+    int off[21];      // the slots' offsets among the real cells (no edge within reach)
 };
 
 __host__ __device__ inline int I2(const Geo3 &g, int i, int j) { return j * g.Nx + i; }
@@ -844,6 +845,19 @@ __device__ inline int col3(const Geo3 &g, int i, int j, int k, int s) {
     const int kk = fold_in(k + c_coz[s], g.nz, 0);
     return (kk * g.ny + jj) * g.nx + ii;
 }
+// the 21-slot product of row q: plain offsets where no slot reaches an edge
+__device__ inline double prod21(const Geo3 &g, const double *P, const double *x, int q) {
+    const int n = g.nint;
+    const int i = q % g.nx, j = (q / g.nx) % g.ny, k = q / (g.nx * g.ny);
+    double s = 0.0;
+    if (i >= 2 && i < g.nx - 2 && j >= 2 && j < g.ny - 2 && k >= 2 && k < g.nz - 2) {
+#pragma unroll
+        for (int t = 0; t < 21; ++t) s += P[(size_t)t * n + q] * x[q + g.off[t]];
+    } else {
+        for (int t = 0; t < 21; ++t) s += P[(size_t)t * n + q] * x[col3(g, i, j, k, t)];
+    }
+    return s;
+}
 
 // block partial sums of up to 3 products, reduced in a fixed order
 template <int K>
@@ -864,9 +878,7 @@ __global__ void k3_spmv(Geo3 g, const double *P, const double *x, double *y, con
     double acc[1] = {0.0};
     const int n = g.nint;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const int i = q % g.nx, j = (q / g.nx) % g.ny, k = q / (g.nx * g.ny);
-        double s = 0.0;
-        for (int t = 0; t < 21; ++t) s += P[(size_t)t * n + q] * x[col3(g, i, j, k, t)];
+        const double s = prod21(g, P, x, q);
         y[q] = s;
         if (w) acc[0] += w[q] * s;
     }
@@ -988,9 +1000,7 @@ __global__ void k3_t(Geo3 g, const double *P, const double *sh, double *t, const
     double acc[2] = {0.0, 0.0};
     const int n = g.nint;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const int i = q % g.nx, j = (q / g.nx) % g.ny, k = q / (g.nx * g.ny);
-        double sum = 0.0;
-        for (int m = 0; m < 21; ++m) sum += P[(size_t)m * n + q] * sh[col3(g, i, j, k, m)];
+        const double sum = prod21(g, P, sh, q);
         t[q] = sum;
         acc[0] += sum * sum;
         acc[1] += sum * s[q];
@@ -1254,13 +1264,11 @@ __global__ void k_def_scale(Geo3 g, const double *H, double f, double *c) {
     if (H[p] > 0.0 && col_mean(g, c, p) > 0.0)
         for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] *= (1.0 - f);
 }
-// a sum over the columns in the reference's loop order (x outer, y inner)
-__global__ void k_colsum(Geo3 g, const double *v, double *out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    double s = 0.0;
-    for (int i = 0; i < g.nx; ++i)
-        for (int j = 0; j < g.ny; ++j) s += v[j * g.nx + i];
-    *out = s;
+// partial sums of a per-column array in a fixed order (reduced by k_final)
+__global__ void k_colsum(int n, const double *v, double *part) {
+    double acc[1] = {0.0};
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) acc[0] += v[q];
+    block_partials<1>(acc, part);
 }
 
 }  // namespace
@@ -1331,12 +1339,10 @@ int zero_flags(yz_cz *z) {
     return 0;
 }
 double colsum(yz_cz *z, const double *v, int *err) {
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, z->st, z->g, v, z->dsc);
-    if (hipMemcpyAsync(z->hsc, z->dsc, sizeof(double), hipMemcpyDeviceToHost, z->st) != hipSuccess || sync(z)) {
-        *err = 1;
-        return 0;
-    }
-    return z->hsc[0];
+    hipLaunchKernelGGL(k_colsum, dim3(RB), dim3(TBK), 0, z->st, z->g.nx * z->g.ny, v, z->part);
+    double out = 0.0;
+    if (read_sums(z, 1, &out)) *err = 1;
+    return out;
 }
 
 int apply_rows(yz_cz *z, double *x) {
@@ -1746,6 +1752,12 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
     g.dz = ((1.0 + 2 * dz0) - (0.0 - 2 * dz0)) / (double)g.Nz;
     for (int k = 0; k < 4; ++k) g.wall[k] = c->bc[k] != YZ_BC_PERIODIC;
     g.perx = !g.wall[0]; g.pery = !g.wall[2];
+    {
+        const int cox[21] = {0, 0, -1, 0, 1, -2, -1, 1, 2, -1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int coy[21] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, -1, 1, -1, 1};
+        const int coz[21] = {0, -2, -1, -1, -1, 0, 0, 0, 0, 1, 1, 1, 2, 0, 0, 0, 0, -1, -1, 1, 1};
+        for (int t = 0; t < 21; ++t) g.off[t] = (coz[t] * g.ny + coy[t]) * g.nx + cox[t];
+    }
     bool ok = hipStreamCreateWithFlags(&z->st, hipStreamNonBlocking) == hipSuccess;
     const size_t n2 = g.n2, n3 = g.n3, ni = g.nint;
     double **a2[] = {&z->S, &z->hO, &z->hN, &z->fH, &z->tH, &z->mid, &z->hO_keep, &z->hN_keep};
