@@ -175,6 +175,18 @@ int yz_step_get_h(yz_step *st, double *h);
 int yz_step_set_h(yz_step *st, const double *h);
 /* clock: time reached, step now in force, step before it */
 int yz_step_clock(yz_step *st, double *time, double *dt_now, double *dt_old);
+/* The restart state (field*.state, yuzen/src/IO/RestartState.h:24-49):
+ * yz_step_save_state writes what the next step depends on in the
+ * reference's layout and keys (WellMixed_ModelBase::WriteState,
+ * WellMixedModelBase.h:896-927: time, step, dt, adjustCount.height,
+ * height.old, height.correction, solver.height, solver.conc, field.h/s/c),
+ * plus keys the reference's reader ignores (conc.start; learn.* for the
+ * learned preconditioner). yz_step_load_state puts a context created for the
+ * same case in that state (BuildRestart, :1196-1221), so the next steps are
+ * those of the run that wrote it, to the bit. 0 or 2 (yz_last_error). */
+int yz_step_save_state(yz_step *st, const char *path);
+int yz_step_load_state(yz_step *st, const char *path);
+
 /* per-phase timing of the last yz_step_advance, milliseconds (HIP events):
  * [0] assembly, [1] residual, [2] AMG setup, [3] Krylov, [4] update,
  * [5] whole step */

@@ -18,7 +18,8 @@
 //   * Mobility::Face/FaceDK and derivatives over a grid of heights and slip
 //     lengths; the Koren limiter's face value, weights and derivative;
 //     NonDimensionalNumbers::Dp over contact angles; FaceValue and
-//     FaceWeights of the unlimited face schemes.
+//     FaceWeights of the unlimited face schemes;
+//   * a restart state file written by the reference's C_RestartState.
 // Output: binary files (little-endian) plus a JSON manifest, into the
 // directory given as argv[1].
 #include <cmath>
@@ -36,6 +37,7 @@
 #include "WellMixedTraits.h"
 #include "FiniteDifferences.h"
 #include "Mobility.h"
+#include "RestartState.h"
 
 using namespace yuzen;
 using namespace yuzen::grid;
@@ -419,6 +421,40 @@ void dump_dp(const std::string &dir) {
 
 }  // namespace
 
+// a restart state file written by the reference's own C_RestartState
+// (RestartState.cpp:43-81), with the keys and order of
+// WellMixed_ModelBase::WriteState (WellMixedModelBase.h:896-927) that a
+// run's next step depends on, for a 1-D film of 8 cells (12 rows with the
+// ghosts; the correction's ghosts are copies of the cells they mirror)
+void dump_state(const std::string &dir) {
+    const int n = 12;
+    std::vector<double> h(n), s(n), c(n), old(n), corr(n);
+    for (int i = 0; i < n; ++i) {
+        h[i] = 1.0 + 0.1 * std::sin(0.7 * i + 0.3);
+        s[i] = 0.01 * i;
+        c[i] = 0.5 + 0.01 * std::cos((double)i);
+        old[i] = h[i] - 1e-3 * std::cos((double)i);
+    }
+    for (int i = 2; i < n - 2; ++i) corr[i] = 1e-6 * std::sin((double)i);
+    corr[0] = corr[3]; corr[1] = corr[2]; corr[n - 2] = corr[n - 3]; corr[n - 1] = corr[n - 4];
+    C_RestartState st;
+    st.SetText("model", "Well-Mixed-1D");
+    st.Set("time", 0.0375);
+    st.Set("step", 41.0);
+    std::vector<double> dt(3);
+    dt[0] = 1.0e-3; dt[1] = 5.0e-4; dt[2] = 5.0e-4;
+    st.Set("dt", dt);
+    st.Set("adjustCount.height", 7.0);
+    st.SetVector("height.old", old);
+    st.SetVector("height.correction", corr);
+    st.SetVector("solver.height", std::vector<double>());
+    st.SetVector("solver.conc", std::vector<double>());
+    st.SetVector("field.h", h);
+    st.SetVector("field.s", s);
+    st.SetVector("field.c", c);
+    st.Write(dir + "/ref_state_wm1d.state");
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_golden OUTDIR\n"); return 1; }
     const std::string dir = argv[1];
@@ -437,6 +473,7 @@ int main(int argc, char **argv) {
     dump_limiter(dir);
     dump_dp(dir);
     dump_schemes(dir);
+    dump_state(dir);
     std::printf("golden vectors written to %s\n", dir.c_str());
     return 0;
 }

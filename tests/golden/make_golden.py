@@ -11,6 +11,8 @@ drives them) into a scratch directory and packs what it wrote:
   ref_scalars.npz mobility table, limiter table, Dp table, the unlimited
                   face schemes' FaceValue and FaceWeights
   manifest.json   the dimensionless parameters of every case
+  ref_state_wm1d.state  a restart state file (field*.state) written by the
+                  reference's C_RestartState for a 1-D film of 8 cells
 
     python tests/golden/make_golden.py
 """
@@ -68,6 +70,10 @@ def main():
                             schemes=read_table(os.path.join(tmp, "schemes.bin")))
         with open(os.path.join(HERE, "manifest.json"), "w") as f:
             json.dump(man, f, indent=1)
+        with open(os.path.join(tmp, "ref_state_wm1d.state"), "rb") as f:
+            state = f.read()
+        with open(os.path.join(HERE, "ref_state_wm1d.state"), "wb") as f:
+            f.write(state)
     print("wrote", os.path.join(HERE, "ref_terms.npz"))
 
 

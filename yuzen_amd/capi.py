@@ -182,6 +182,8 @@ def load():
     lib.yz_step_get_c.argtypes = [C.c_void_p, pd]
     lib.yz_step_face_fluxes.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yz_step_conc_stats.argtypes = [C.c_void_p, pd]
+    lib.yz_step_save_state.argtypes = [C.c_void_p, C.c_char_p]
+    lib.yz_step_load_state.argtypes = [C.c_void_p, C.c_char_p]
     lib.yz_linsolve_create.restype = C.c_void_p
     lib.yz_linsolve_destroy.argtypes = [C.c_void_p]
     lib.yz_linsolve.argtypes = [C.c_void_p, i, i, pi_, pi_, pd, pd, pd,

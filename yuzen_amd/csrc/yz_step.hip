@@ -30,6 +30,7 @@
 #include "yz_rows.h"
 #include "yz_solver.h"
 #include "yz_mlp.h"
+#include "yz_state.h"
 
 namespace yz {
 
@@ -175,6 +176,7 @@ struct yz_step {
     Mlp mlp;
     long long fallbacks = 0;
     double time = 0, dt_now = 0, dt_old = 0;
+    long long steps = 0;
     int tsc_count = 0;
     bool extrap_blocked = false, first_step = true;
     double extrap_move = 0, ew_prev = 0, ew_eta = 0;
@@ -1017,6 +1019,7 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
     S->dt_old = S->dt_now;
     S->dt_now = dt_request;
     S->first_step = false;
+    ++S->steps;
     if (nk_iter) *nk_iter = it;
     if (lin_iter) *lin_iter = (int)lin_sum;
     if (resid) *resid = r;
@@ -1073,6 +1076,138 @@ int yz_step_get_h(yz_step *S, double *h) {
     const int np = S->g.nxp * S->g.nyp;
     YZ_TRY(hipMemcpyAsync(h, S->hnew, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
     YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+
+// the restart state (WellMixed_ModelBase::WriteState / BuildRestart,
+// WellMixedModelBase.h:896-927, 1196-1221): the clock (time, step, dt now /
+// before / of the last step), the step control's count, the film of the
+// step before (height.old), the last Newton correction (height.correction,
+// which the next step's first solve starts from), the backends' memory
+// (none here: empty), the fields h, s, c with their ghosts; and what the
+// device step carries besides: the concentration equation's start
+// (conc.start, conc_model 1) and the learned preconditioner's weights,
+// Adam moments and training counters (learn.*)
+static std::vector<double> pad_interior(const Geo &g, const std::vector<double> &in) {
+    const int np = g.nxp * g.nyp;
+    std::vector<double> out((size_t)np, 0.0);
+    auto fold = [](int i, int n, int lo, int hi) {
+        const int m = n - 4;
+        if (i < 2) return lo == YZ_BC_PERIODIC ? i + m : 3 - i;
+        if (i > n - 3) return hi == YZ_BC_PERIODIC ? i - m : 2 * (n - 3) + 1 - i;
+        return i;
+    };
+    for (int iy = 0; iy < g.nyp; ++iy)
+        for (int ix = 0; ix < g.nxp; ++ix) {
+            const int fx = fold(ix, g.nxp, g.bc[0], g.bc[1]);
+            const int fy = g.dim == 2 ? fold(iy, g.nyp, g.bc[2], g.bc[3]) : 0;
+            const int q = (g.dim == 2 ? (fy - 2) * g.nx : 0) + (fx - 2);
+            out[(size_t)iy * g.nxp + ix] = in[(size_t)q];
+        }
+    return out;
+}
+
+int yz_step_save_state(yz_step *S, const char *path) {
+    if (!S || !path) { set_err("null argument"); return 2; }
+    const Geo &g = S->g;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    std::vector<double> hnew(np), hold(np), sv(np), cv(np), corr((size_t)g.N), c0;
+    YZ_TRY(hipMemcpyAsync(hnew.data(), S->hnew, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(hold.data(), S->hold, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(sv.data(), S->s, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(cv.data(), S->conc ? S->cfield : S->c, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(corr.data(), S->corr, g.N * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    if (S->conc) {
+        c0.resize(np);
+        YZ_TRY(hipMemcpyAsync(c0.data(), S->c0, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    }
+    const bool learned = S->cs.precond == YZ_PC_LEARNED || S->cs.precond == YZ_PC_LEARNED_AMG;
+    std::vector<float> lp, lm, lv;
+    if (learned) {
+        lp.resize(MLP_NP); lm.resize(MLP_NP); lv.resize(MLP_NP);
+        YZ_TRY(hipMemcpyAsync(lp.data(), S->mlp.params, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, S->st));
+        YZ_TRY(hipMemcpyAsync(lm.data(), S->mlp.adam_m, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, S->st));
+        YZ_TRY(hipMemcpyAsync(lv.data(), S->mlp.adam_v, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, S->st));
+    }
+    YZ_TRY(hipStreamSynchronize(S->st));
+    StateFile st;
+    st.set_text("model", g.dim == 2 ? "Well-Mixed-2D" : "Well-Mixed-1D");
+    st.set("time", S->time);
+    st.set("step", (double)S->steps);
+    st.set("dt", std::vector<double>{S->dt_now, S->dt_old, S->dt_old});
+    st.set("adjustCount.height", (double)S->tsc_count);
+    st.set_vector("height.old", hold);
+    st.set_vector("height.correction", pad_interior(g, corr));
+    st.set_vector("solver.height", std::vector<double>());
+    st.set_vector("solver.conc", std::vector<double>());
+    st.set_vector("field.h", hnew);
+    st.set_vector("field.s", sv);
+    st.set_vector("field.c", cv);
+    if (S->conc) st.set_vector("conc.start", c0);
+    if (learned) {
+        st.set("learn.state", std::vector<double>{(double)S->mlp.trained_passes, (double)S->mlp.adam_t});
+        st.set_vector("learn.params", std::vector<double>(lp.begin(), lp.end()));
+        st.set_vector("learn.adam_m", std::vector<double>(lm.begin(), lm.end()));
+        st.set_vector("learn.adam_v", std::vector<double>(lv.begin(), lv.end()));
+    }
+    if (!st.write(path)) { set_err(std::string("could not write the state file ") + path); return 2; }
+    return 0;
+}
+
+int yz_step_load_state(yz_step *S, const char *path) {
+    if (!S || !path) { set_err("null argument"); return 2; }
+    const Geo &g = S->g;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    StateFile st;
+    if (!st.read(path)) { set_err(std::string("not a readable state file: ") + path); return 2; }
+    const std::string model = g.dim == 2 ? "Well-Mixed-2D" : "Well-Mixed-1D";
+    if (st.text("model") != model) { set_err("the state file was written by " + st.text("model") + ", not " + model); return 2; }
+    const char *need[] = {"field.h", "field.s", "field.c", "height.old"};
+    for (const char *k : need)
+        if (st.vec(k).size() != np) { set_err(std::string("state vector ") + k + " missing or of another grid"); return 2; }
+    const double dt_now = st.get("dt", 0), dt_old = st.get("dt", 1);
+    if (!(dt_now > 0) || !(dt_old >= 0) || !std::isfinite(st.get("time"))) { set_err("state file: bad clock"); return 2; }
+    const bool learned = S->cs.precond == YZ_PC_LEARNED || S->cs.precond == YZ_PC_LEARNED_AMG;
+    YZ_TRY(hipMemcpyAsync(S->hnew, st.vec("field.h").data(), np * sizeof(double), hipMemcpyHostToDevice, S->st));
+    YZ_TRY(hipMemcpyAsync(S->h, st.vec("field.h").data(), np * sizeof(double), hipMemcpyHostToDevice, S->st));
+    YZ_TRY(hipMemcpyAsync(S->hold, st.vec("height.old").data(), np * sizeof(double), hipMemcpyHostToDevice, S->st));
+    YZ_TRY(hipMemcpyAsync(S->s, st.vec("field.s").data(), np * sizeof(double), hipMemcpyHostToDevice, S->st));
+    if (S->conc) {
+        YZ_TRY(hipMemcpyAsync(S->cfield, st.vec("field.c").data(), np * sizeof(double), hipMemcpyHostToDevice, S->st));
+        YZ_TRY(hipMemcpyAsync(S->c, st.vec("field.c").data(), np * sizeof(double), hipMemcpyHostToDevice, S->st));
+        const std::vector<double> &c0 = st.vec("conc.start").size() == np ? st.vec("conc.start") : st.vec("field.c");
+        YZ_TRY(hipMemcpyAsync(S->c0, c0.data(), np * sizeof(double), hipMemcpyHostToDevice, S->st));
+    } else {
+        YZ_TRY(hipMemcpyAsync(S->c, st.vec("field.c").data(), np * sizeof(double), hipMemcpyHostToDevice, S->st));
+    }
+    std::vector<double> corr((size_t)g.N, 0.0);
+    if (st.vec("height.correction").size() == np) {
+        const std::vector<double> &v = st.vec("height.correction");
+        for (int iy = 0; iy < g.ny; ++iy)
+            for (int ix = 0; ix < g.nx; ++ix)
+                corr[(size_t)iy * g.nx + ix] = v[(size_t)(g.dim == 2 ? iy + 2 : 0) * g.nxp + ix + 2];
+    }
+    YZ_TRY(hipMemcpyAsync(S->corr, corr.data(), g.N * sizeof(double), hipMemcpyHostToDevice, S->st));
+    if (learned && st.vec("learn.params").size() == (size_t)MLP_NP) {
+        std::vector<float> p(MLP_NP), m(MLP_NP, 0.0f), v(MLP_NP, 0.0f);
+        for (int i = 0; i < MLP_NP; ++i) p[i] = (float)st.vec("learn.params")[i];
+        if (st.vec("learn.adam_m").size() == (size_t)MLP_NP)
+            for (int i = 0; i < MLP_NP; ++i) m[i] = (float)st.vec("learn.adam_m")[i];
+        if (st.vec("learn.adam_v").size() == (size_t)MLP_NP)
+            for (int i = 0; i < MLP_NP; ++i) v[i] = (float)st.vec("learn.adam_v")[i];
+        YZ_TRY(hipMemcpyAsync(S->mlp.params, p.data(), MLP_NP * sizeof(float), hipMemcpyHostToDevice, S->st));
+        YZ_TRY(hipMemcpyAsync(S->mlp.adam_m, m.data(), MLP_NP * sizeof(float), hipMemcpyHostToDevice, S->st));
+        YZ_TRY(hipMemcpyAsync(S->mlp.adam_v, v.data(), MLP_NP * sizeof(float), hipMemcpyHostToDevice, S->st));
+        S->mlp.trained_passes = (int)st.get("learn.state", 0);
+        S->mlp.adam_t = (int)st.get("learn.state", 1);
+    }
+    YZ_TRY(hipStreamSynchronize(S->st));
+    S->time = st.get("time");
+    S->steps = (long long)st.get("step");
+    S->dt_now = dt_now;
+    S->dt_old = dt_old;
+    S->tsc_count = (int)st.get("adjustCount.height");
+    S->first_step = S->steps == 0 && dt_old == 0;
     return 0;
 }
 

@@ -10,6 +10,7 @@ Nothing here computes: every call goes to the HIP library, and a missing
 library raises.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -52,6 +53,14 @@ class HipStep:
 
     def __del__(self):
         self.close()
+
+    def save_state(self, path):
+        """The restart state file (yz_step_save_state)."""
+        _state_call(self.lib.yz_step_save_state, self.p, path, "yz_step_save_state")
+
+    def load_state(self, path):
+        """Continue from a restart state file (yz_step_load_state)."""
+        _state_call(self.lib.yz_step_load_state, self.p, path, "yz_step_load_state")
 
     def advance(self):
         nk, lin, r = C.c_int(), C.c_int(), C.c_double()
@@ -168,6 +177,12 @@ class HipStep:
         out = np.zeros(5)
         self._check(self.lib.yz_learn_stats(self.p, _pd(out)), "yz_learn_stats")
         return out
+
+
+def _state_call(fn, p, path, what):
+    rc = fn(p, os.fsencode(path))
+    if rc != 0:
+        raise RuntimeError(f"{what}: " + capi.last_error())
 
 
 class LinSolver:
