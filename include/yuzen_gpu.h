@@ -395,6 +395,13 @@ void yz_cz_volume_fluxes(yz_cz *st, double *qx, double *qy, double *w);
  * again, [5] solute moved by RedistributeExcess, [6] moved and [7] made up
  * by RedistributeDeficit (c times cells, summed) */
 void yz_cz_stats(yz_cz *st, double *out8);
+/* the restart state in the reference's field*.state layout and keys
+ * (ConcZ_Model_2D::WriteState, ConcZ_Model_2D.cpp:572-600: the clock,
+ * height.old, conc.previous and conc.previousDt, solver.height,
+ * solver.conc, field.h/s/c); a context created for the same film continues
+ * from it to the bit. 0 or 2 (yz_last_error). */
+int yz_cz_save_state(yz_cz *st, const char *path);
+int yz_cz_load_state(yz_cz *st, const char *path);
 void yz_cz_destroy(yz_cz *st);
 /* sizeof(yz_cz_case), for bindings to check their mirror */
 int yz_sizeof_cz_case(void);

@@ -9,8 +9,10 @@
   loaded into a fresh context and continued for k steps end with the same
   film (and concentration) to the bit: the plain FP64 2-D all-physics
   film, the same with the well-mixed concentration, a film under step
-  control (the step and its count come back), and the learned FP32 path
-  (the network and its training state come back).
+  control (the step and its count come back), the learned FP32 path
+  (the network and its training state come back), and the ConcZ film with
+  its coupling passes (the previous concentration the corrector
+  extrapolates from comes back).
 """
 import os
 
@@ -18,7 +20,8 @@ import numpy as np
 import pytest
 
 from yuzen_amd import capi, cases
-from yuzen_amd.step import HipStep
+from yuzen_amd.step import HipCZ, HipStep
+from tests.cz_cases import _allphys
 
 pytestmark = pytest.mark.gpu
 
@@ -89,3 +92,25 @@ def test_continuation_learned(tmp_path):
     # training spans the save: it runs on the first 6 Newton passes
     c.precond, c.krylov_fp32, c.learn_passes, c.learn_iters = capi.PC_LEARNED, 1, 6, 30
     _continuation(c, h, s, cc, dt, 1, "learned FP32 48^2", tmp_path)
+
+
+def test_continuation_concz(tmp_path):
+    c, h, s, cc = _allphys(per=False)
+    dt = 0.02
+    a = HipCZ(c, h, s, cc, dt)
+    for _ in range(4):
+        assert a.advance()[0] == 0
+    ha, ca = a.h(), a.c()
+    a.close()
+    b = HipCZ(c, h, s, cc, dt)
+    for _ in range(2):
+        assert b.advance()[0] == 0
+    path = str(tmp_path / "cz.state")
+    b.save_state(path)
+    b.close()
+    d = HipCZ(c, np.ones_like(h), s, np.zeros_like(cc), dt)
+    d.load_state(path)
+    for _ in range(2):
+        assert d.advance()[0] == 0
+    assert np.array_equal(d.h(), ha) and np.array_equal(d.c(), ca)
+    d.close()

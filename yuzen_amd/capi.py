@@ -214,6 +214,8 @@ def load():
     lib.yz_cz_volume_fluxes.argtypes = [C.c_void_p, pd, pd, pd]
     lib.yz_cz_stats.argtypes = [C.c_void_p, pd]
     lib.yz_cz_destroy.argtypes = [C.c_void_p]
+    lib.yz_cz_save_state.argtypes = [C.c_void_p, C.c_char_p]
+    lib.yz_cz_load_state.argtypes = [C.c_void_p, C.c_char_p]
     lib.yz_sizeof_cz_case.restype = C.c_int
     lib.yz_sizeof_case.restype = C.c_int
     lib.yz_sizeof_lin_settings.restype = C.c_int

@@ -45,6 +45,7 @@
 
 #include "../../include/yuzen_gpu.h"
 #include "yz_solver.h"
+#include "yz_state.h"
 
 namespace yz {
 void set_error(const std::string &s);
@@ -1300,6 +1301,8 @@ struct yz_cz {
     unsigned long long *emax = nullptr;
     double dtPrev = 1.0;
     bool prevSet = false;
+    double time = 0;
+    long long steps = 0;
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
@@ -1691,6 +1694,8 @@ int cz_advance(yz_cz *z, int &lin, int &passes_out, double &resid_out) {
     apply_rows(z, z->cN);
     if (cp_d2d(z, z->fC, z->cN, n3)) return -1;
     if (cp_d2d(z, z->tH, z->fH, n2) || cp_d2d(z, z->tC, z->fC, n3)) return -1;
+    z->time += z->dt;
+    ++z->steps;
     return sync(z);
 }
 
@@ -1863,6 +1868,79 @@ void yz_cz_stats(yz_cz *z, double *out8) {
     if (!z || !out8) return;
     for (int k = 0; k < 8; ++k) out8[k] = z->stats[k];
 }
+// the restart state (ConcZ_Model_2D::WriteState / BuildRestart,
+// ConcZ_Model_2D.cpp:572-600, 664-690): the clock, the step control's count
+// (0: the step is fixed here), the film at the start of the last step, the
+// concentration at the start of the last step and that step's dt (the
+// corrector extrapolates c to the next midpoint from them), the backends'
+// memory (none), the fields with their ghosts
+int yz_cz_save_state(yz_cz *z, const char *path) {
+    if (!z || !path) { yz::set_error("ConcZ: null argument"); return 2; }
+    const size_t n2 = z->g.n2, n3 = z->g.n3;
+    std::vector<double> h(n2), hold(n2), sv(n2), c(n3), cp;
+    if (hipStreamSynchronize(z->st) != hipSuccess ||
+        hipMemcpy(h.data(), z->hN, n2 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hold.data(), z->hO, n2 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(sv.data(), z->S, n2 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(c.data(), z->cN, n3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        yz::set_error("ConcZ: download failed");
+        return 2;
+    }
+    if (z->prevSet) {
+        cp.resize(n3);
+        if (hipMemcpy(cp.data(), z->cPrev, n3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+            yz::set_error("ConcZ: download failed");
+            return 2;
+        }
+    }
+    StateFile st;
+    st.set_text("model", "ConcZ-3D");
+    st.set("time", z->time);
+    st.set("step", (double)z->steps);
+    st.set("dt", std::vector<double>{z->dt, z->dt, z->dt});
+    st.set("adjustCount.height", 0.0);
+    st.set_vector("height.old", hold);
+    if (z->prevSet) {
+        st.set("conc.previousDt", z->dtPrev);
+        st.set_vector("conc.previous", cp);
+    }
+    st.set_vector("solver.height", std::vector<double>());
+    st.set_vector("solver.conc", std::vector<double>());
+    st.set_vector("field.h", h);
+    st.set_vector("field.s", sv);
+    st.set_vector("field.c", c);
+    if (!st.write(path)) { yz::set_error(std::string("ConcZ: could not write ") + path); return 2; }
+    return 0;
+}
+
+int yz_cz_load_state(yz_cz *z, const char *path) {
+    if (!z || !path) { yz::set_error("ConcZ: null argument"); return 2; }
+    const size_t n2 = z->g.n2, n3 = z->g.n3;
+    StateFile st;
+    if (!st.read(path)) { yz::set_error(std::string("ConcZ: not a readable state file: ") + path); return 2; }
+    if (st.text("model") != "ConcZ-3D") { yz::set_error("ConcZ: the state file was written by " + st.text("model")); return 2; }
+    if (st.vec("field.h").size() != n2 || st.vec("field.s").size() != n2 || st.vec("height.old").size() != n2 ||
+        st.vec("field.c").size() != n3) {
+        yz::set_error("ConcZ: state vectors missing or of another grid");
+        return 2;
+    }
+    const bool prev = st.scalars.count("conc.previousDt") && st.vec("conc.previous").size() == n3;
+    bool ok = hipStreamSynchronize(z->st) == hipSuccess;
+    for (double *d : {z->hN, z->fH, z->tH})
+        ok = ok && hipMemcpy(d, st.vec("field.h").data(), n2 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(z->hO, st.vec("height.old").data(), n2 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(z->S, st.vec("field.s").data(), n2 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    for (double *d : {z->cN, z->cO, z->fC, z->tC})
+        ok = ok && hipMemcpy(d, st.vec("field.c").data(), n3 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    if (prev) ok = ok && hipMemcpy(z->cPrev, st.vec("conc.previous").data(), n3 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) { yz::set_error("ConcZ: upload failed"); return 2; }
+    z->prevSet = prev;
+    z->dtPrev = prev ? st.get("conc.previousDt") : 1.0;
+    z->time = st.get("time");
+    z->steps = (long long)st.get("step");
+    return 0;
+}
+
 void yz_cz_destroy(yz_cz *z) {
     if (!z) return;
     if (z->st) (void)hipStreamSynchronize(z->st);

@@ -332,6 +332,14 @@ class HipCZ:
         self.lib.yz_cz_get_c(self.p, _pd(out))
         return out
 
+    def save_state(self, path):
+        """The restart state file (yz_cz_save_state)."""
+        _state_call(self.lib.yz_cz_save_state, self.p, path, "yz_cz_save_state")
+
+    def load_state(self, path):
+        """Continue from a restart state file (yz_cz_load_state)."""
+        _state_call(self.lib.yz_cz_load_state, self.p, path, "yz_cz_load_state")
+
     def volume_fluxes(self):
         qx, qy, w = np.empty(self.n3), np.empty(self.n3), np.empty(self.n3)
         self.lib.yz_cz_volume_fluxes(self.p, _pd(qx), _pd(qy), _pd(w))
