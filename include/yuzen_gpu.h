@@ -54,6 +54,9 @@ enum { YZ_PC_JACOBI = 0, YZ_PC_AMG = 1, YZ_PC_LEARNED = 2, YZ_PC_LEARNED_AMG = 3
 /* Everything one well-mixed film-height case needs, already made
  * dimensionless (reference NonDimensionalNumbers::Build,
  * yuzen/src/model/NonDimensionalNumbers.cpp:43-141). */
+/* evaporation laws (Physics.Evaporation.ConcFunc) */
+enum { YZ_EVAP_RAOULT = 0, YZ_EVAP_POWERN = 1 };
+
 typedef struct yz_case {
     int dim;                    /* 1: "xh", 2: "xyh" */
     int nx, ny;                 /* interior cells (ny = 1 in 1-D) */
@@ -131,6 +134,24 @@ typedef struct yz_case {
      * 28 % fewer iterations than starting from a correction up to 10^4
      * times larger than the one sought) */
     int zero_guess;
+    /* B of the film's residual (HeightEquationBase.h:753-787): the
+     * evaporation, a Crank-Nicolson mean of EvapAt at the start of the step
+     * and at the Newton guess (:830-846: Ev E(h, c), at most half the film
+     * and the solvent h (1 - c) per step; E from Evaporation.cpp:62-103,
+     * uniform in space), less the nozzle's supply at the middle of the step
+     * (SourceBase.cpp, WM2D/Source.cpp, WM1D/Source_1D.cpp). Both with the
+     * concentration frozen (conc_model 0); the batch takes neither. */
+    int evaporation;                /* Physics.Evaporation.Enable */
+    double Ev, evap_coef;           /* E = Ev evap_coef f(c) g(h) */
+    int evap_func;                  /* YZ_EVAP_RAOULT (default) or YZ_EVAP_POWERN */
+    double evap_power, evap_concr;  /* PowerN: max(1 - c - concr, 0)^power */
+    double molar_resin, molar_solvent;  /* Raoult: Fluid.MolarMass (2200, 106) */
+    double precursor;               /* PrecursorRatio: the ramp (h - h*) / (19 h*) in [0, 1]; 0 off */
+    int source;                     /* Physics.Source.Enable */
+    double src_radius, src_rate, src_tau;   /* Radius, InletRate, TimeConstant (dimensionless) */
+    double src_arm_length, src_arm_x, src_arm_y;   /* ArmLength, ArmCenter */
+    double src_omega, src_angle0, src_ramp, src_plateau;   /* RotationRate, StartAngle, RampTime, PlateauTime */
+    int src_subregions;             /* SubRegions */
 } yz_case;
 
 /* Face interpolation of the concentration's advection (reference
@@ -346,8 +367,6 @@ int yz_memory_set(yz_linctx *ctx, const double *buf, int len);
 /* (Grid.Dimension "xyh"), the step fixed (no step control), the        */
 /* concentration scheme unlimited; the create call refuses the rest.    */
 /* ------------------------------------------------------------------ */
-enum { YZ_EVAP_RAOULT = 0, YZ_EVAP_POWERN = 1 };
-
 typedef struct yz_cz_case {
     int nx, ny, nz;                 /* real cells: Grid.Nx, Ny, Nz */
     double xmin, xmax, ymin, ymax;  /* nondimensional */

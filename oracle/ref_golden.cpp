@@ -19,7 +19,9 @@
 //     lengths; the Koren limiter's face value, weights and derivative;
 //     NonDimensionalNumbers::Dp over contact angles; FaceValue and
 //     FaceWeights of the unlimited face schemes;
-//   * a restart state file written by the reference's C_RestartState.
+//   * a restart state file written by the reference's C_RestartState;
+//   * the evaporation rate of C_Evaporation and the nozzle's supply of
+//     Source_WM2D / Source_WM1D.
 // Output: binary files (little-endian) plus a JSON manifest, into the
 // directory given as argv[1].
 #include <cmath>
@@ -38,6 +40,10 @@
 #include "FiniteDifferences.h"
 #include "Mobility.h"
 #include "RestartState.h"
+#include "Evaporation.h"
+#include "TimeSteps.h"
+#include "WM2D/Source.h"
+#include "WM1D/Source_1D.h"
 
 using namespace yuzen;
 using namespace yuzen::grid;
@@ -455,6 +461,114 @@ void dump_state(const std::string &dir) {
     st.Write(dir + "/ref_state_wm1d.state");
 }
 
+// the evaporation rate of the reference's C_Evaporation (Evaporation.cpp
+// :62-143) at one cell of a 2-D field, over laws, precursor ramps, h and c:
+// rows [law (0 Raoult, 1 PowerN), PowerN, ConcR, PrecursorRatio, Coef, h,
+// c, E]; and the nozzle's supply of the reference's Source_WM2D and
+// Source_WM1D (SourceBase.cpp, WM2D/Source.cpp, WM1D/Source_1D.cpp) at every
+// row of a grid at several times: rows [dim, t, row, S], with the scales and
+// inputs in a header row
+void dump_evap_source(const std::string &dir) {
+    Params p;
+    std::vector<double> ev;
+    struct Law { const char *func; double n, concr, hstar, coef; };
+    const Law laws[] = {{"Raoult", 0.5, 0.0, 0.0, 1.0}, {"PowerN", 0.5, 0.0, 0.0, 0.7},
+                        {"PowerN", 0.0, 0.1, 0.0, 1.3}, {"PowerN", 1.5, 0.05, 0.0, 1.0},
+                        {"Raoult", 0.5, 0.0, 0.05, 1.0}};
+    const double hs[] = {0.01, 0.05, 0.07, 0.5, 1.0, 2.5};
+    const double cs[] = {-0.1, 0.0, 0.3, 0.5, 0.9, 1.0, 1.2};
+    for (const Law &L : laws) {
+        Input inp = MakeInput(tLaplace, p);
+        inp.Physics.Evaporation.Enable = true;
+        inp.Physics.Evaporation.Coef = L.coef;
+        inp.Physics.Evaporation.ConcFunc = L.func;
+        inp.Physics.Evaporation.PowerN = L.n;
+        inp.Physics.Evaporation.ConcR = L.concr;
+        inp.Physics.Evaporation.SpatialFunc = "uniform";
+        inp.Initial.SurfaceShape.PrecursorRatio = L.hstar;
+        inp.Grid = MakeGrid(true, 6, 5);
+        C_Evaporation e;
+        e.Build(inp, 1.0);
+        Field_WM2D field;
+        field.Build(inp.Grid, Flat(), 1.0);
+        C_Grid2D *G = field.GetGridPtr();
+        const Zface z = G->At<Zface>(3, 3);
+        for (double h : hs)
+            for (double c : cs) {
+                field.HH(z) = h;
+                field.CC(z) = c;
+                ev.push_back(std::string(L.func) == "PowerN" ? 1 : 0);
+                ev.push_back(L.n); ev.push_back(L.concr); ev.push_back(L.hstar); ev.push_back(L.coef);
+                ev.push_back(h); ev.push_back(c); ev.push_back(e(field, z));
+            }
+    }
+    {
+        std::ofstream f(dir + "/evap.bin", std::ios::binary);
+        const int hdr[2] = {(int)ev.size() / 8, 8};
+        f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+        write_vec(f, ev);
+    }
+    // the source: a rotating arm with ramps, 3x3 sub-regions (2-D) / 5 (1-D)
+    Input inp = MakeInput(tLaplace, p);
+    inp.Physics.Source.Enable = true;
+    inp.Physics.Source.Radius = 0.13;
+    inp.Physics.Source.InletRate = 0.8;
+    inp.Physics.Source.TimeConstant = 0.05;
+    inp.Physics.Source.ArmLength = 0.25;
+    inp.Physics.Source.ArmCenter.assign(2, 0.0);
+    inp.Physics.Source.ArmCenter[0] = 0.02;
+    inp.Physics.Source.ArmCenter[1] = -0.03;
+    inp.Physics.Source.RotationRate = 3.0;
+    inp.Physics.Source.StartAngle = 0.4;
+    inp.Physics.Source.RampTime = 0.2;
+    inp.Physics.Source.PlateauTime = 0.5;
+    std::vector<double> out;
+    const double times[] = {0.0, 0.01, 0.15, 0.45, 0.8, 1.0};
+    for (int twoD = 1; twoD >= 0; --twoD) {
+        inp.Grid = MakeGrid(twoD == 1, 24, 20);
+        inp.Physics.Source.SubRegions = twoD ? 3 : 5;
+        NonDimensionalNumbers nd;
+        nd.Build(inp, inp.Initial.SurfaceShape.TotalHeight);
+        C_TimeSteps ts;
+        ts.Build(1e-3, 0.0, 1e9, false);
+        out.push_back(-1); out.push_back(nd.ExtentScale()); out.push_back(nd.HeightScale()); out.push_back(nd.TimeScale());
+        if (twoD) {
+            Field_WM2D field;
+            field.Build(inp.Grid, Flat(), nd.ExtentScale());
+            C_Grid2D *G = field.GetGridPtr();
+            Source_WM2D src;
+            src.Build(inp.Physics, G, &nd, &ts);
+            for (double t : times) {
+                src.SetEvalTime(t);
+                src.updateFraction();
+                for (int iy = 0; iy < G->ny(); ++iy)
+                    for (int ix = 0; ix < G->nx(); ++ix) {
+                        out.push_back(2); out.push_back(t); out.push_back(iy * G->nx() + ix);
+                        out.push_back(src(G->At<Zface>(ix, iy)));
+                    }
+            }
+        } else {
+            Field_WM1D field;
+            field.Build(inp.Grid, Flat(), nd.ExtentScale());
+            C_Grid1D *G = field.GetGridPtr();
+            Source_WM1D src;
+            src.Build(inp.Physics, G, &nd, &ts);
+            for (double t : times) {
+                src.SetEvalTime(t);
+                src.updateFraction();
+                for (int ix = 0; ix < G->nx(); ++ix) {
+                    out.push_back(1); out.push_back(t); out.push_back(ix);
+                    out.push_back(src(G->At<Xedge>(ix)));
+                }
+            }
+        }
+    }
+    std::ofstream f(dir + "/source.bin", std::ios::binary);
+    const int hdr[2] = {(int)out.size() / 4, 4};
+    f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    write_vec(f, out);
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_golden OUTDIR\n"); return 1; }
     const std::string dir = argv[1];
@@ -474,6 +588,7 @@ int main(int argc, char **argv) {
     dump_dp(dir);
     dump_schemes(dir);
     dump_state(dir);
+    dump_evap_source(dir);
     std::printf("golden vectors written to %s\n", dir.c_str());
     return 0;
 }

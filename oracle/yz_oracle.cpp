@@ -68,7 +68,10 @@
  *
  * Only the limited Koren/van Leer height schemes (the default,
  * Numerics.Convection.Height = "Koren") and the Symmetry/Periodic edges are
- * restated; no evaporation, no source. The concentration is frozen
+ * restated. Evaporation and the nozzle source enter B with the concentration
+ * frozen (EvapAt HeightEquationBase.h:830-846, SetMatrixA's B :753-787,
+ * Evaporation.cpp:62-103, SourceBase.cpp:25-112, WM2D/Source.cpp,
+ * WM1D/Source_1D.cpp). The concentration is frozen
  * (Physics.ConcField.Model = "None") with conc_model 0 and solved with the
  * film (Model = "WM", unlimited face schemes, UTOPIA the default) with
  * conc_model 1.
@@ -1348,6 +1351,131 @@ struct Stepper {
     double coupled = 0;
     double cstat[5] = {0, 0, 0, 0, 0};
 
+    /* ---- B: evaporation and the nozzle's supply ---- */
+    vector<double> evap_old, src_cell, Bv;
+    /* Evaluate (Evaporation.cpp:62-103), uniform in space */
+    double evap_eval(double hh, double cc) const {
+        const yz_case &c = eq.cs;
+        double f_conc;
+        if (c.evap_func == YZ_EVAP_POWERN) {
+            double v = 1.0 - cc - c.evap_concr;
+            v = std::max(v, 0.0);
+            f_conc = std::pow(v, c.evap_power);
+        } else {
+            const double x = (cc < 0.0) ? 0.0 : ((cc > 1.0) ? 1.0 : cc);
+            const double chi = (x / c.molar_resin) / (x / c.molar_resin + (1.0 - x) / c.molar_solvent);
+            f_conc = 1.0 - chi;
+        }
+        double f_mod = 1.0;
+        if (c.precursor != 0.0) {
+            const double v = (hh - c.precursor) / ((20.0 - 1.0) * c.precursor);
+            f_mod = std::max(0.0, std::min(v, 1.0));
+        }
+        return c.evap_coef * f_conc * 1.0 * f_mod;
+    }
+    /* EvapAt (HeightEquationBase.h:830-846) of row i of the field (H, C) */
+    double evap_at(const vector<double> &H, const vector<double> &C, int i, double dt) const {
+        double evp = eq.cs.evaporation ? eq.cs.Ev * evap_eval(H[i], C[i]) : eq.cs.Ev * 0.0;
+        if (evp <= 0) return evp;
+        const double hh = H[i];
+        if (evp * dt > 0.5 * hh) evp = (hh > 0) ? 0.5 * hh / dt : 0.0;
+        const double cc = (C[i] < 0.0) ? 0.0 : ((C[i] > 1.0) ? 1.0 : C[i]);
+        const double solvent = (hh > 0.0) ? hh * (1.0 - cc) : 0.0;
+        if (evp * dt > solvent) evp = solvent / dt;
+        return evp;
+    }
+    /* NozzlePos, FlowRateNow (SourceBase.cpp:54-112) at time tt (the run
+     * starts at 0) */
+    void nozzle(double tt, double &xn, double &yn) const {
+        const yz_case &c = eq.cs;
+        const double Ts = c.src_ramp, Tm = c.src_plateau;
+        const double t = std::max(0.0, tt - 0.0);
+        double swept;
+        if (t <= Ts) swept = (Ts > 0) ? 0.5 * t * t / Ts : 0.0;
+        else if (t <= Ts + Tm) swept = 0.5 * Ts + (t - Ts);
+        else if (t <= 2.0 * Ts + Tm) {
+            const double tau = t - Ts - Tm;
+            swept = 0.5 * Ts + Tm + tau - 0.5 * tau * tau / Ts;
+        } else swept = Ts + Tm;
+        const double theta = c.src_angle0 + c.src_omega * swept;
+        xn = c.src_arm_x + c.src_arm_length * std::cos(theta);
+        yn = c.src_arm_y + c.src_arm_length * std::sin(theta);
+    }
+    double flow_now(double tt) const {
+        const double tau = eq.cs.src_tau;
+        double coeff = 1.0;
+        if (!(std::fabs(tau) <= std::numeric_limits<double>::epsilon())) coeff = 1.0 - std::exp(-tt / tau);
+        return eq.cs.src_rate * coeff;
+    }
+    /* the supply of every row at time tt: updateFraction + operator()
+     * (WM2D/Source.cpp:29-126, WM1D/Source_1D.cpp:26-111) */
+    void source_at(double tt) {
+        const yz_case &c = eq.cs;
+        const Grid &g = eq.g;
+        src_cell.assign(g.n, 0.0);
+        if (!c.source) return;
+        double xn, yn;
+        nozzle(tt, xn, yn);
+        const double phi = flow_now(tt);
+        const int ns = c.src_subregions;
+        for (int iy = 0; iy < g.nyp; ++iy)
+            for (int ix = 0; ix < g.nxp; ++ix) {
+                const double xc = g.X(ix);
+                double frac = 0.0;
+                if (g.dim == 2) {
+                    const double yc = g.Y(iy);
+                    const double rr = std::sqrt(std::pow(xc - xn, 2) + std::pow(yc - yn, 2));
+                    if (rr < c.src_radius + std::sqrt(g.dx * g.dx + g.dy * g.dy)) {
+                        const double dxs = g.dx / (double)ns, dys = g.dy / (double)ns;
+                        const double xL = xc - 0.5 * g.dx, yB = yc - 0.5 * g.dy;
+                        double alpha = 0.0;
+                        for (int a = 0; a < ns; a++) {
+                            const double xx = xL + ((double)a + 0.5) * dxs;
+                            for (int b = 0; b < ns; b++) {
+                                const double yy = yB + ((double)b + 0.5) * dys;
+                                const double r2 = std::sqrt(std::pow(xx - xn, 2) + std::pow(yy - yn, 2));
+                                alpha += (r2 <= c.src_radius ? 1.0 : 0.0);
+                            }
+                        }
+                        frac = alpha / (double)(ns * ns);
+                    }
+                } else {
+                    const double rr = std::fabs(xc - xn);
+                    if (rr < c.src_radius + 2.0 * g.dx) {
+                        const double dxs = g.dx / (double)ns, xL = xc - 0.5 * g.dx;
+                        double alpha = 0.0;
+                        for (int a = 0; a < ns; a++) {
+                            const double xx = xL + ((double)a + 0.5) * dxs;
+                            alpha += (std::fabs(xx - xn) <= c.src_radius ? 1.0 : 0.0);
+                        }
+                        frac = alpha / (double)ns;
+                    }
+                }
+                src_cell[g.row(ix, iy)] = phi * frac;
+            }
+    }
+    /* BeforeNKLoop's part of B (:1330-1341): the evaporation at the start of
+     * the step, the supply at its middle */
+    void b_before() {
+        const int n = eq.g.n;
+        evap_old.assign(n, 0.0);
+        for (int i = 0; i < n; ++i)
+            if (!eq.A.bc[i]) evap_old[i] = evap_at(phi_old, eq.c, i, dt_now);
+        source_at(time + 0.5 * dt_now);
+    }
+    /* B of the pass (SetMatrixA :765-787) at the guess */
+    void b_pass() {
+        const int n = eq.g.n;
+        const double th = eq.cs.theta;
+        Bv.assign(n, 0.0);
+        if (!eq.cs.evaporation && !eq.cs.source) return;
+        for (int i = 0; i < n; ++i) {
+            if (eq.A.bc[i]) continue;
+            const double evp = (1.0 - th) * evap_old[i] + th * evap_at(phi_guess, eq.c, i, dt_now);
+            Bv[i] = evp - src_cell[i];
+        }
+    }
+
     void init(const yz_case &c, const double *h, const double *s, const double *conc, double dt0) {
         eq.build(c);
         const int n = eq.g.n;
@@ -1379,7 +1507,7 @@ struct Stepper {
         const double th = eq.cs.theta, dt = dt_now;
         for (int i = 0; i < eq.g.n; ++i) {
             if (eq.A.bc[i]) { Rhs[i] = 0; continue; }
-            Rhs[i] = phi_old[i] - phi_guess[i] - (dt * th * Ax_guess[i]) - (dt * (1.0 - th) * Ax_old[i]) - dt * 0.0;
+            Rhs[i] = phi_old[i] - phi_guess[i] - (dt * th * Ax_guess[i]) - (dt * (1.0 - th) * Ax_old[i]) - dt * Bv[i];
         }
     }
     /* SatisfyBCRows for single-row sets */
@@ -1828,6 +1956,7 @@ struct Stepper {
         if (nk_pass % eq.cs.jac_interval == 0) { eq.set_matrix_J(); make_IA(); }
         ++nk_pass;
         eq.apply_A(phi_guess, Ax_guess);
+        b_pass();
         make_rhs();
         double eps = eq.cs.lin_eps;
         if (eq.cs.ew) {
@@ -1881,6 +2010,7 @@ struct Stepper {
         eq.h = phi_old;
         eq.set_matrix_A();
         eq.apply_A(phi_old, Ax_old);
+        b_before();
 
         double resid_eq_old = 1e40, resid_first = 0, resid_best = 1e40;
         int stall = 0;
@@ -2019,6 +2149,11 @@ void yzo_case_defaults(yz_case *c) {
     c->Df = 1.0; c->diff_c0 = 0.5; c->diff_exp = 0.0;
     c->conc_lin_maxit = 5000; c->conc_lin_eps = 1e-13; c->conc_row_scaling = 1;
     c->zero_guess = -1;
+    c->evaporation = 0; c->Ev = 1; c->evap_coef = 1; c->evap_func = YZ_EVAP_RAOULT; c->evap_power = 0.5;
+    c->evap_concr = 0; c->molar_resin = 2200; c->molar_solvent = 106; c->precursor = 0;
+    c->source = 0; c->src_radius = 1; c->src_rate = 0; c->src_tau = 0;
+    c->src_arm_length = 1; c->src_arm_x = 0; c->src_arm_y = 0;
+    c->src_omega = 0; c->src_angle0 = 0; c->src_ramp = 0; c->src_plateau = 1e10; c->src_subregions = 1;
 }
 
 void *yzo_create(const yz_case *c, const double *h, const double *s, const double *conc, double dt0) {

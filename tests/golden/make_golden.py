@@ -9,7 +9,8 @@ drives them) into a scratch directory and packs what it wrote:
                   h, s, c on every row, A(h)h, the FaceFlux of every face
                   of every real cell and the SideRate
   ref_scalars.npz mobility table, limiter table, Dp table, the unlimited
-                  face schemes' FaceValue and FaceWeights
+                  face schemes' FaceValue and FaceWeights, the evaporation
+                  rate of C_Evaporation, the nozzle supply of Source_WM2D/1D
   manifest.json   the dimensionless parameters of every case
   ref_state_wm1d.state  a restart state file (field*.state) written by the
                   reference's C_RestartState for a 1-D film of 8 cells
@@ -67,7 +68,9 @@ def main():
                             mobility=read_table(os.path.join(tmp, "mobility.bin")),
                             limiter=read_table(os.path.join(tmp, "limiter.bin")),
                             dp=read_table(os.path.join(tmp, "dp.bin")),
-                            schemes=read_table(os.path.join(tmp, "schemes.bin")))
+                            schemes=read_table(os.path.join(tmp, "schemes.bin")),
+                            evap=read_table(os.path.join(tmp, "evap.bin")),
+                            source=read_table(os.path.join(tmp, "source.bin")))
         with open(os.path.join(HERE, "manifest.json"), "w") as f:
             json.dump(man, f, indent=1)
         with open(os.path.join(tmp, "ref_state_wm1d.state"), "rb") as f:

@@ -55,6 +55,18 @@ class YzCase(C.Structure):
         ("conc_lin_maxit", C.c_int), ("conc_lin_eps", C.c_double),
         ("conc_row_scaling", C.c_int),
         ("zero_guess", C.c_int),
+        ("evaporation", C.c_int),
+        ("Ev", C.c_double), ("evap_coef", C.c_double),
+        ("evap_func", C.c_int),
+        ("evap_power", C.c_double), ("evap_concr", C.c_double),
+        ("molar_resin", C.c_double), ("molar_solvent", C.c_double),
+        ("precursor", C.c_double),
+        ("source", C.c_int),
+        ("src_radius", C.c_double), ("src_rate", C.c_double), ("src_tau", C.c_double),
+        ("src_arm_length", C.c_double), ("src_arm_x", C.c_double), ("src_arm_y", C.c_double),
+        ("src_omega", C.c_double), ("src_angle0", C.c_double), ("src_ramp", C.c_double),
+        ("src_plateau", C.c_double),
+        ("src_subregions", C.c_int),
     ]
 
     def copy(self):
@@ -97,6 +109,11 @@ def case_defaults():
     c.Df, c.diff_c0, c.diff_exp = 1.0, 0.5, 0.0
     c.conc_lin_maxit, c.conc_lin_eps, c.conc_row_scaling = 5000, 1e-13, 1
     c.zero_guess = -1
+    c.evaporation, c.Ev, c.evap_coef, c.evap_func, c.evap_power, c.evap_concr = 0, 1.0, 1.0, EVAP_RAOULT, 0.5, 0.0
+    c.molar_resin, c.molar_solvent, c.precursor = 2200.0, 106.0, 0.0
+    c.source, c.src_radius, c.src_rate, c.src_tau = 0, 1.0, 0.0, 0.0
+    c.src_arm_length, c.src_arm_x, c.src_arm_y = 1.0, 0.0, 0.0
+    c.src_omega, c.src_angle0, c.src_ramp, c.src_plateau, c.src_subregions = 0.0, 0.0, 0.0, 1e10, 1
     return c
 
 

@@ -771,6 +771,11 @@ void yz_case_defaults(yz_case *c) {
     c->Df = 1.0; c->diff_c0 = 0.5; c->diff_exp = 0.0;
     c->conc_lin_maxit = 5000; c->conc_lin_eps = 1e-13; c->conc_row_scaling = 1;
     c->zero_guess = -1;
+    c->evaporation = 0; c->Ev = 1; c->evap_coef = 1; c->evap_func = YZ_EVAP_RAOULT; c->evap_power = 0.5;
+    c->evap_concr = 0; c->molar_resin = 2200; c->molar_solvent = 106; c->precursor = 0;
+    c->source = 0; c->src_radius = 1; c->src_rate = 0; c->src_tau = 0;
+    c->src_arm_length = 1; c->src_arm_x = 0; c->src_arm_y = 0;
+    c->src_omega = 0; c->src_angle0 = 0; c->src_ramp = 0; c->src_plateau = 1e10; c->src_subregions = 1;
 }
 
 const char *yz_last_error(void) { return g_err.c_str(); }
