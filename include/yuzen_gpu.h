@@ -228,7 +228,9 @@ void yz_step_destroy(yz_step *st);
  *   (1-D: M m o p P), unscaled;
  *   ax[n]: A(h) h, face by face for the capillary and vertical-gravity
  *   terms (HeightEquationBase::ApplyA), 0 on ghost rows;
- *   rhs[n]: the Newton right-hand side (NewtonKantorovich::MakeRHS).
+ *   rhs[n]: the Newton right-hand side (NewtonKantorovich::MakeRHS),
+ *   with - dt B when the case has evaporation or a source: EvapAt of
+ *   h_old and of h, the supply at the context's time + dt / 2.
  * Any output pointer may be NULL. */
 int yz_step_assemble(yz_step *st, const double *h_old, const double *h,
                      double dt, double *ia, double *ax, double *rhs);

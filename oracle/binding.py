@@ -43,6 +43,8 @@ def load():
     lib.yzo_linsolve.argtypes = [C.c_int, C.c_int, pi_, pi_, pd, pd, pd, C.c_int,
                                  C.c_int, C.c_int, C.c_double, pi_, pd]
     lib.yzo_mobility.argtypes = [C.c_double, C.c_double, C.c_double, pd]
+    lib.yzo_evap.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, pd]
+    lib.yzo_source.argtypes = [C.c_void_p, C.c_double, pd]
     lib.yzo_limiter.argtypes = [pd, C.c_double, pd]
     lib.yzo_get_c.argtypes = [C.c_void_p, pd]
     lib.yzo_face_fluxes.argtypes = [C.c_void_p, pd, pd, pd, pd]
@@ -173,6 +175,18 @@ class OracleStep:
         h = np.ascontiguousarray(h, dtype=np.float64)
         self.lib.yzo_assemble(self.p, _pd(h_old), _pd(h), dt, _pd(A), _pd(J), _pd(IA), _pd(ax), _pd(rhs))
         return A, J, IA, ax, rhs
+
+    def evap(self, h, c, dt):
+        """(Ev E(h, c) of the law alone, EvapAt's capped rate at step dt)"""
+        out = np.empty(2)
+        self.lib.yzo_evap(self.p, h, c, dt, _pd(out))
+        return out[0], out[1]
+
+    def source(self, t):
+        """the nozzle's supply of every cell of the padded grid at time t"""
+        out = np.empty(self.n)
+        self.lib.yzo_source(self.p, t, _pd(out))
+        return out
 
 
 def linsolve(rowptr, col, val, rhs, x0, precond=1, scaling=1, maxit=5000, eps=1e-13):

@@ -89,7 +89,12 @@
  *     reference itself;
  *   BiCGSTAB / SA-AMG / Jacobi: the 1-D Poisson problem of the reference's
  *     backend test and the residual definition; AMG hierarchy and
- *     iteration counts: parity unpinned (only the solution to tolerance).
+ *     iteration counts: parity unpinned (only the solution to tolerance);
+ *   B (evap_eval, source_at): against the reference's own C_Evaporation
+ *     and Source_WM2D / Source_WM1D tables, to the bit; evap_at's caps, the
+ *     theta split of B and its place in Rhs: against closed forms of flat
+ *     films and the volume the supply adds
+ *     (tests/test_oracle_evap_source.py).
  */
 #include "../include/yuzen_gpu.h"
 
@@ -2251,6 +2256,8 @@ void yzo_assemble(void *p, const double *h_old, const double *h, double dt,
     st->phi_old = ho;
     st->eq.h = ho; st->eq.set_matrix_A(); st->eq.apply_A(ho, st->Ax_old);
     st->phi_guess = hg;
+    st->b_before();
+    st->b_pass();
     st->eq.h = hg; st->eq.set_matrix_A(); st->eq.set_matrix_J(); st->make_IA();
     st->eq.apply_A(hg, st->Ax_guess);
     st->make_rhs();
@@ -2261,6 +2268,21 @@ void yzo_assemble(void *p, const double *h_old, const double *h, double dt,
     if (ax) std::copy(st->Ax_guess.begin(), st->Ax_guess.end(), ax);
     if (rhs) std::copy(st->Rhs.begin(), st->Rhs.end(), rhs);
     st->dt_now = save_dt;
+}
+/* B's pieces (for the tests against the reference's tables): out2 =
+ * Ev E(h, c) of the law alone (Evaporation.cpp:62-103 times Ev), EvapAt's
+ * capped rate at step dt (HeightEquationBase.h:830-846) */
+void yzo_evap(void *p, double h, double c, double dt, double *out2) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    const std::vector<double> H(1, h), Cv(1, c);
+    out2[0] = st->eq.cs.evaporation ? st->eq.cs.Ev * st->evap_eval(h, c) : 0.0;
+    out2[1] = st->evap_at(H, Cv, 0, dt);
+}
+/* the nozzle's supply of every cell of the padded grid at time t */
+void yzo_source(void *p, double t, double *out) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    st->source_at(t);
+    std::copy(st->src_cell.begin(), st->src_cell.end(), out);
 }
 /* Mobility (for the tests against the reference's tables): out =
  * Face(h1,h2), Face_d1(h1,h2), Face_d1(h2,h1), D(h1), dD(h1), DperH(h1) */

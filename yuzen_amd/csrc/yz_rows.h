@@ -10,7 +10,8 @@
 //     (HeightEquationBase.h:1061-1091) and of IA = I + theta dt J
 //     (NewtonKantorovich.cpp:488-510);
 //   * the Newton right-hand side Rhs = h_old - h - dt (theta A(h)h +
-//     (1-theta) A(h_old)h_old) (NewtonKantorovich.cpp:512-523);
+//     (1-theta) A(h_old)h_old) - dt B (NewtonKantorovich.cpp:512-523), B
+//     the evaporation and the nozzle's supply when the case has them;
 // and writes the row-scaled IA planes and right-hand side
 // (LinearSolverBackend.cpp:227-252, ScaleRows: divided by the diagonal).
 // A and J are never stored: the pass reads h, s, c once and writes the
@@ -19,6 +20,43 @@
 #include "yz_physics.h"
 
 namespace yz {
+
+// evaporation in B: the rate law (Evaporation.cpp:62-103, uniform in
+// space) and its caps (EvapAt, HeightEquationBase.h:830-846)
+struct BTerm {
+    int evap, efunc;
+    double Ev, coef, power, concr, mres, msol, prec;
+};
+
+// Ev E(h, c), at most half the film and at most its solvent per step
+__device__ inline double evap_rate(const BTerm &b, double hh, double cc, double dt) {
+    double evp = b.Ev * 0.0;
+    if (b.evap) {
+        double f_conc;
+        if (b.efunc == YZ_EVAP_POWERN) {
+            const double v = fmax(1.0 - cc - b.concr, 0.0);
+            f_conc = pow(v, b.power);
+        } else {
+            // Raoult: 1 - the resin's mole fraction of the clamped c
+            const double x = cc < 0.0 ? 0.0 : (cc > 1.0 ? 1.0 : cc);
+            const double chi = (x / b.mres) / (x / b.mres + (1.0 - x) / b.msol);
+            f_conc = 1.0 - chi;
+        }
+        double f_mod = 1.0;
+        if (b.prec != 0.0) {
+            // the precursor ramp: 0 at h*, 1 from 20 h*
+            const double v = (hh - b.prec) / ((20.0 - 1.0) * b.prec);
+            f_mod = fmax(0.0, fmin(v, 1.0));
+        }
+        evp = b.Ev * (b.coef * f_conc * 1.0 * f_mod);
+    }
+    if (evp <= 0) return evp;
+    if (evp * dt > 0.5 * hh) evp = hh > 0 ? 0.5 * hh / dt : 0.0;
+    const double x = cc < 0.0 ? 0.0 : (cc > 1.0 ? 1.0 : cc);
+    const double solvent = hh > 0.0 ? hh * (1.0 - x) : 0.0;
+    if (evp * dt > solvent) evp = solvent / dt;
+    return evp;
+}
 
 struct RowArgs {
     Geo g;
@@ -48,6 +86,12 @@ struct RowArgs {
     float *mscale;             // optional: 1 / (1 + sum_k!=c |IA_k|) of the scaled row
     unsigned long long *rmax;  // mode 1: max |Rhs| (bits of a non-negative double)
     int *nonunit;              // FP32 planes: set when a row's stored diagonal is not exactly 1
+    // optional, mode 1: B = (1 - theta) evold + theta EvapAt(h, c) - src
+    // enters Rhs as - dt B (SetMatrixA, HeightEquationBase.h:765-787); J
+    // carries no derivative of it
+    BTerm bt;
+    const double *evold;       // N: EvapAt(h_old, c) of the step's start
+    const double *src;         // N: the nozzle's supply at the middle of the step
 };
 
 __device__ inline double visc_of(const RowArgs &a, double cc) { return exp(a.visc_exp * (cc - a.visc_c0)); }
@@ -400,7 +444,11 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
         if (MODE == 0) {
             A.ax[i] = ax;
         } else {
-            const double R = A.hold[p] - H[C0] - A.dt * A.theta * ax - A.dt * (1.0 - A.theta) * A.axold[i];
+            double R = A.hold[p] - H[C0] - A.dt * A.theta * ax - A.dt * (1.0 - A.theta) * A.axold[i];
+            if (A.evold) {
+                const double evp = (1.0 - A.theta) * A.evold[i] + A.theta * evap_rate(A.bt, H[C0], Cc[C0], A.dt);
+                R = R - A.dt * (evp - A.src[i]);
+            }
             rabs = fabs(R);
             if (A.rhs_raw) A.rhs_raw[i] = R;
             if (A.ax_raw) A.ax_raw[i] = ax;

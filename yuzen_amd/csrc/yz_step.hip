@@ -146,6 +146,54 @@ __global__ void k_f2d(int n, const float *a, double *b) {
     if (i < n) b[i] = (double)a[i];
 }
 
+// the nozzle's place and supply at one time, from the host
+struct Nozzle {
+    int on, ns;
+    double xn, yn, flow, radius;
+};
+
+// BeforeNKLoop's part of B (HeightEquationBase.h:1330-1341): EvapAt of the
+// step's start h^n, and the nozzle's supply at the middle of the step,
+// flow x the covered fraction of the cell, found by sampling ns (x ns)
+// sub-cells whose centres lie within the radius (WM2D/Source.cpp:29-126;
+// WM1D/Source_1D.cpp:26-111); cells farther than the radius plus a cell's
+// diagonal (1-D: plus two cells) are not sampled
+__global__ void k_bstart(Geo g, BTerm bt, Nozzle nz, const double *hold, const double *c, double dt,
+                         double *evold, double *src) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.N) return;
+    const int ix = i % g.nx + 2, iy = g.dim == 2 ? i / g.nx + 2 : 0;
+    const int p = iy * g.nxp + ix;
+    evold[i] = evap_rate(bt, hold[p], c[p], dt);
+    double frac = 0.0;
+    if (nz.on) {
+        const double xc = g.xlo + ix * g.dx + g.dx * 0.5;
+        if (g.dim == 2) {
+            const double yc = g.ylo + iy * g.dy + g.dy * 0.5;
+            const double ex = xc - nz.xn, ey = yc - nz.yn;
+            if (sqrt(ex * ex + ey * ey) < nz.radius + sqrt(g.dx * g.dx + g.dy * g.dy)) {
+                const double sx = g.dx / (double)nz.ns, sy = g.dy / (double)nz.ns;
+                const double x0 = xc - 0.5 * g.dx, y0 = yc - 0.5 * g.dy;
+                double hit = 0.0;
+                for (int a = 0; a < nz.ns; ++a) {
+                    const double u = x0 + ((double)a + 0.5) * sx - nz.xn;
+                    for (int b = 0; b < nz.ns; ++b) {
+                        const double v = y0 + ((double)b + 0.5) * sy - nz.yn;
+                        hit += sqrt(u * u + v * v) <= nz.radius ? 1.0 : 0.0;
+                    }
+                }
+                frac = hit / (double)(nz.ns * nz.ns);
+            }
+        } else if (fabs(xc - nz.xn) < nz.radius + 2.0 * g.dx) {
+            const double sx = g.dx / (double)nz.ns, x0 = xc - 0.5 * g.dx;
+            double hit = 0.0;
+            for (int a = 0; a < nz.ns; ++a) hit += fabs(x0 + ((double)a + 0.5) * sx - nz.xn) <= nz.radius ? 1.0 : 0.0;
+            frac = hit / (double)nz.ns;
+        }
+    }
+    src[i] = nz.flow * frac;
+}
+
 static double bits_to_double(unsigned long long b) {
     double d;
     std::memcpy(&d, &b, sizeof(d));
@@ -164,6 +212,7 @@ struct yz_step {
     bool own_stream = true;
     double *h = nullptr, *hold = nullptr, *hold2 = nullptr, *hnew = nullptr, *s = nullptr, *c = nullptr;
     double *axold = nullptr, *b = nullptr, *corr = nullptr, *scratch = nullptr;
+    double *evold = nullptr, *srcv = nullptr;   // B's parts fixed over a step (N each)
     unsigned long long *dmax = nullptr;   // [0] rhs, [1] corr, [2] move, [3] trunc
     int *dflag = nullptr;
     unsigned long long *hmax = nullptr;   // pinned mirror
@@ -205,6 +254,9 @@ struct yz_step {
     double cstat[5] = {0, 0, 0, 0, 0};
 };
 
+// evaporation or a source: B enters the right-hand side
+static bool has_b(const yz_case &c) { return c.evaporation || c.source; }
+
 static int check_case(const yz_case &c) {
     if (c.dim != 1 && c.dim != 2) { set_err("dim must be 1 or 2"); return 1; }
     if (c.nx < 3 || (c.dim == 2 && c.ny < 3)) { set_err("grid too small"); return 1; }
@@ -230,6 +282,15 @@ static int check_case(const yz_case &c) {
             set_err("conc_scheme: only the unlimited face schemes are implemented for the concentration"); return 1;
         }
         if (c.conc_lin_maxit < 1 || !(c.conc_lin_eps > 0)) { set_err("conc_lin_maxit >= 1 and conc_lin_eps > 0"); return 1; }
+    }
+    if (has_b(c)) {
+        if (c.conc_model != 0) {
+            set_err("evaporation and the source are built with the concentration frozen (conc_model 0)"); return 1;
+        }
+        if (c.evaporation && c.evap_func != YZ_EVAP_RAOULT && c.evap_func != YZ_EVAP_POWERN) {
+            set_err("evap_func must be YZ_EVAP_RAOULT or YZ_EVAP_POWERN"); return 1;
+        }
+        if (c.source && c.src_subregions < 1) { set_err("src_subregions must be >= 1"); return 1; }
     }
     if (c.learn_passes > 0 && (c.learn_iters < 0 || c.learn_probes < 1 || !(c.learn_lr >= 0))) {
         set_err("learn_iters >= 0, learn_probes >= 1 and learn_lr >= 0 are needed to train"); return 1;
@@ -285,6 +346,43 @@ static StencilOp fine_op(const Geo &g) {
     op.perx = g.bc[0] == YZ_BC_PERIODIC; op.pery = g.dim == 2 && g.bc[2] == YZ_BC_PERIODIC;
     for (int k = 0; k < g.ns; ++k) { op.ox[k] = slot_ox(g.dim, k); op.oy[k] = slot_oy(g.dim, k); }
     return op;
+}
+
+// the arm's angle angle0 + omega x (the time the arm has swept at full
+// speed: a linear ramp up over Ts, the plateau Tm, a ramp down), the nozzle
+// at the arm's end, and the flow rate (1 - exp(-t / tau)) of the full rate
+// (SourceBase.cpp:54-112); the run starts at time 0
+static Nozzle nozzle_at(const yz_case &c, double tt) {
+    Nozzle nz;
+    nz.on = c.source; nz.ns = c.src_subregions; nz.radius = c.src_radius;
+    nz.xn = nz.yn = nz.flow = 0;
+    if (!c.source) return nz;
+    const double Ts = c.src_ramp, Tm = c.src_plateau;
+    const double t = std::max(0.0, tt - 0.0);
+    double swept;
+    if (t <= Ts) swept = Ts > 0 ? 0.5 * t * t / Ts : 0.0;
+    else if (t <= Ts + Tm) swept = 0.5 * Ts + (t - Ts);
+    else if (t <= 2.0 * Ts + Tm) {
+        const double u = t - Ts - Tm;
+        swept = 0.5 * Ts + Tm + u - 0.5 * u * u / Ts;
+    } else swept = Ts + Tm;
+    const double ang = c.src_angle0 + c.src_omega * swept;
+    nz.xn = c.src_arm_x + c.src_arm_length * std::cos(ang);
+    nz.yn = c.src_arm_y + c.src_arm_length * std::sin(ang);
+    double rise = 1.0;
+    if (!(std::fabs(c.src_tau) <= std::numeric_limits<double>::epsilon())) rise = 1.0 - std::exp(-tt / c.src_tau);
+    nz.flow = c.src_rate * rise;
+    return nz;
+}
+
+// B's parts fixed over the step (evaporation at h^n, the supply at the
+// middle of the step t + dt/2), for the step's dt: into the row arguments
+static int b_before(yz_step *S, const double *hold, double dt, double t) {
+    if (!S->evold) return 0;
+    hipLaunchKernelGGL(k_bstart, dim3(nb(S->g.N)), dim3(TB), 0, S->st, S->g, S->ra.bt,
+                       nozzle_at(S->cs, t + 0.5 * dt), hold, S->c, dt, S->evold, S->srcv);
+    YZ_TRY(hipGetLastError());
+    return 0;
 }
 
 static void rows_launch(yz_step *S, int mode, const double *h) {
@@ -678,6 +776,7 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
     if (!guessed) YZ_TRY(hipMemcpyAsync(S->h, S->hold, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     // BeforeNKLoop + the explicit half A(h^n) h^n
     if (S->conc && conc_before(S)) return 2;
+    if (b_before(S, S->hold, S->dt_now, S->time)) return 2;
     {
         RowArgs a = S->ra;
         a.ax = S->axold;
@@ -807,6 +906,9 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
     for (double **p : pads) if (hipMalloc((void **)p, np * sizeof(double)) != hipSuccess) return fail("hipMalloc");
     double **vecs[] = {&S->axold, &S->b, &S->corr, &S->scratch};
     for (double **p : vecs) if (hipMalloc((void **)p, g.N * sizeof(double)) != hipSuccess) return fail("hipMalloc");
+    if (has_b(*c))
+        for (double **p : {&S->evold, &S->srcv})
+            if (hipMalloc((void **)p, g.N * sizeof(double)) != hipSuccess) return fail("hipMalloc");
     if (hipMalloc((void **)&S->dmax, 4 * sizeof(unsigned long long)) != hipSuccess) return fail("hipMalloc");
     if (hipMalloc((void **)&S->dflag, sizeof(int)) != hipSuccess) return fail("hipMalloc");
     if (hipHostMalloc((void **)&S->hmax, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) return fail("hipHostMalloc");
@@ -914,6 +1016,10 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
     a.s = S->s; a.c = S->c;
     a.ia = S->S.A; a.b = S->b; a.rmax = S->dmax;
     a.ax = S->axold;
+    a.bt.evap = c->evaporation; a.bt.efunc = c->evap_func; a.bt.Ev = c->Ev; a.bt.coef = c->evap_coef;
+    a.bt.power = c->evap_power; a.bt.concr = c->evap_concr; a.bt.mres = c->molar_resin;
+    a.bt.msol = c->molar_solvent; a.bt.prec = c->precursor;
+    a.evold = S->evold; a.src = S->srcv;
     S->dt_now = dt0; S->dt_old = 0;
     return S;
 }
@@ -937,7 +1043,8 @@ void yz_step_destroy(yz_step *S) {
     for (float *p : {S->b32, S->corr32, S->feat, S->mscale, S->M32})
         if (p) (void)hipFree(p);
     if (S->Yh) (void)hipFree(S->Yh);
-    for (double *p : {S->h, S->hold, S->hold2, S->hnew, S->s, S->c, S->axold, S->b, S->corr, S->scratch})
+    for (double *p : {S->h, S->hold, S->hold2, S->hnew, S->s, S->c, S->axold, S->b, S->corr, S->scratch,
+                      S->evold, S->srcv})
         if (p) (void)hipFree(p);
     if (S->dmax) (void)hipFree(S->dmax);
     if (S->dflag) (void)hipFree(S->dflag);
@@ -1266,6 +1373,7 @@ int yz_step_assemble(yz_step *S, const double *h_old, const double *h, double dt
     YZ_TRY(hipMemcpyAsync(S->h, h, np * sizeof(double), hipMemcpyHostToDevice, S->st));
     hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, S->hold);
     hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, S->h);
+    if (b_before(S, S->hold, dt, S->time)) return 2;
     RowArgs keep = S->ra;
     S->ra.ax = S->axold;
     rows_launch(S, 0, S->hold);
@@ -1346,6 +1454,7 @@ int yz_learn_sample(yz_step *S, float *ia, float *feat, float *mscale) {
     // half also at h (into scratch), so the state of the step is untouched
     // but for the pass-local iterate h
     YZ_TRY(hipMemcpyAsync(S->h, S->hnew, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    if (b_before(S, S->h, S->dt_now, S->time)) return 2;
     RowArgs keep = S->ra;
     S->ra.ax = S->scratch;
     rows_launch(S, 0, S->h);
@@ -1491,6 +1600,7 @@ static int bf_loop_begin(BatchFilm &F) {
         if (!S->hflag[0] && move > 0) { S->extrap_move = move; F.guessed = true; }
     }
     if (!F.guessed) YZ_TRY(hipMemcpyAsync(S->h, S->hold, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    if (b_before(S, S->hold, S->dt_now, S->time)) return 2;
     RowArgs keep = S->ra;
     S->ra.ax = S->axold;
     rows_launch(S, 0, S->hold);
