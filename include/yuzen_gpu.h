@@ -221,6 +221,17 @@ int yz_step_timing(yz_step *st, double *ms6);
 int yz_step_kernel_time(yz_step *st, int enable, long long *count, double *total_ms);
 void yz_step_destroy(yz_step *st);
 
+/* The film's face velocities at the context's film and concentration
+ * (WellMixed_HeightEquationBase::CalVelocity, HeightEquationBase.h:
+ * 1583-1601, which the reference runs at the end of each step for output):
+ * the plain sum of the terms' velocities (LubFields.cpp:467-495) with the
+ * equation's boundary rows (SetVelocityBC, WM2D/HeightEquation_2D.cpp:
+ * 32-151, WM1D/HeightEquation_1D.cpp:28-86). Padded arrays of n doubles:
+ * ux[row] on the Xm face of that row's cell (the reference's Yedge / Node of
+ * the same index), uy[row] on its Ym face (Xedge); uy may be NULL in 1-D.
+ * Faces neither the terms nor the boundary rows write are 0. */
+int yz_step_velocity(yz_step *st, double *ux, double *uy);
+
 /* Assembly-level access (parity tests): at film h (n doubles, padded),
  * with the step's dt and the explicit half Ax_old taken at h_old:
  *   ia[13*N] (2-D) / ia[5*N] (1-D): rows of I + theta dt J over the N

@@ -44,6 +44,7 @@ def load():
                                  C.c_int, C.c_int, C.c_double, pi_, pd]
     lib.yzo_mobility.argtypes = [C.c_double, C.c_double, C.c_double, pd]
     lib.yzo_evap.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, pd]
+    lib.yzo_velocity.argtypes = [C.c_void_p, pd, pd, C.c_int, pd, pd]
     lib.yzo_source.argtypes = [C.c_void_p, C.c_double, pd]
     lib.yzo_limiter.argtypes = [pd, C.c_double, pd]
     lib.yzo_get_c.argtypes = [C.c_void_p, pd]
@@ -140,6 +141,16 @@ class OracleStep:
         F, r = np.zeros(2 * self.n), np.zeros(self.n)
         self.lib.yzo_face_fluxes(self.p, _pd(h), _pd(c), _pd(F), _pd(r))
         return F.reshape(2, self.n), r
+
+    def velocity(self, h, c, bc=True):
+        """the face velocities of the field (h, c): (vx, vy), padded, vx on
+        the Xm face and vy on the Ym face of each cell; bc False leaves out
+        the boundary rows"""
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        vx, vy = np.zeros(self.n), np.zeros(self.n)
+        self.lib.yzo_velocity(self.p, _pd(h), _pd(c), 1 if bc else 0, _pd(vx), _pd(vy))
+        return vx, vy
 
     def conc_system(self):
         """the last concentration system: (rowptr, col, val, rhs), unscaled"""

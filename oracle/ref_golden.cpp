@@ -14,7 +14,8 @@
 //     storage order, and A(h)h (the conservative form for the capillary and
 //     vertical-gravity terms, AddConservative; the product otherwise);
 //   * the term's FaceFlux through every face of every real cell and its
-//     SideRate (what FaceVolumeFluxes sums for the concentration);
+//     SideRate (what FaceVolumeFluxes sums for the concentration), and its
+//     face velocities (CalVelocity);
 //   * Mobility::Face/FaceDK and derivatives over a grid of heights and slip
 //     lengths; the Koren limiter's face value, weights and derivative;
 //     NonDimensionalNumbers::Dp over contact angles; FaceValue and
@@ -31,6 +32,7 @@
 #include <sstream>
 #include <string>
 #include <vector>
+#include <type_traits>
 
 #include "NonDimensionalNumbers.h"
 #include "LubFields.h"
@@ -236,6 +238,22 @@ void dump_term(TermT &term, FieldT &field, const Shapes &sh, const std::string &
         for (int d = 0; d < ndir; ++d) ffv[(size_t)i * ndir + d] = term.FaceFlux(s, d, field, svec);
         side[i] = term.SideRate(s, field);
     }
+    // the term's face velocities (its CalVelocity, summed into the mixed
+    // field by Cal_MixedVelocity; the other terms' fields stay zero), before
+    // the equation's SetVelocityBC: x on the Xm face of every cell, y on its
+    // Ym face, in the cell's row
+    std::vector<double> vx(n, 0.0), vy(n, 0.0);
+    term.CalVelocity(field);
+    field.UU().Cal_MixedVelocity();
+    for (int i = 0; i < n; ++i) {
+        const int ix = i % nxp, iy = i / nxp;
+        if constexpr (std::is_same<FieldT, Field_WM2D>::value) {
+            vx[i] = field.UU().UU().x(G->template At<Yedge>(ix, iy));
+            vy[i] = field.UU().UU().y(G->template At<Xedge>(ix, iy));
+        } else {
+            vx[i] = field.UU().UU().x(G->template At<Node>(ix));
+        }
+    }
     std::ofstream f(path, std::ios::binary);
     const int hdr[4] = {n, (int)col.size(), (int)colj.size(), nxp};
     f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
@@ -243,6 +261,7 @@ void dump_term(TermT &term, FieldT &field, const Shapes &sh, const std::string &
     write_ivec(f, rpj); write_ivec(f, colj); write_vec(f, jv);
     write_vec(f, hv); write_vec(f, sv); write_vec(f, cv); write_vec(f, axv);
     write_vec(f, ffv); write_vec(f, side);
+    write_vec(f, vx); write_vec(f, vy);
 }
 
 template <class K>

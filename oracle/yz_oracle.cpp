@@ -897,6 +897,177 @@ struct Equation {
         if (mob.slip) return bf_mag(1, 0) * (mob.D(hoo) / visc(coo));
         return bf_mag(1, 0) * (hoo * hoo / visc(coo)) * hoo;
     }
+    /* ---- the face velocities (CalVelocity, HeightEquationBase.h:1583-1601):
+     * each term's velocity on the Xm face (x) and the Ym face (y) of every
+     * cell 2 <= ix <= nxp-2 (2 <= iy <= nyp-2; InsideForVelocity,
+     * GridTraits.h:239-244, 515-522), at the working field ---- */
+    bool vel_cell(int i) const {
+        const int ix = i % g.nxp, iy = i / g.nxp;
+        if (ix < 2 || g.nxp - 2 < ix) return false;
+        return cs.dim == 1 || !(iy < 2 || g.nyp - 2 < iy);
+    }
+    /* AxisMask: no y velocity on a grid 2 real cells deep (GridTraits.h:557-558) */
+    double axis_mask(int a, double q) const { return (a == 1 && g.nyp == 6) ? 0.0 : q; }
+    /* FaceHeight: the mean of the cells on either side of the face */
+    double face_height(int i, int a) const { return 0.5 * (h[nb(i, st.adj[2 * a])] + h[i]); }
+    /* LaplacePressure_2D.cpp:137-211, LaplacePressure_1D.cpp:84-136 */
+    double vel_capillary(int i, int a) const {
+        const double Mag = 1.0 / 3.0;
+        if (cs.dim == 1) {
+            const double dx = g.dx, rdx3 = 1.0 / (dx * dx * dx);
+            const double DL = cap_D(i, 0), hL = 0.5 * (h[nb(i, 1)] + h[i]);
+            const double hp = h[nb(i, 3)], ho = h[i], hm = h[nb(i, 1)], hM = h[nb(i, 0)];
+            const double sp = s[nb(i, 3)], so = s[i], sm = s[nb(i, 1)], sM = s[nb(i, 0)];
+            double Qhx = ((hp - 2.0 * ho + hm) - (ho - 2.0 * hm + hM)) * rdx3;
+            double Qsx = ((sp - 2.0 * so + sm) - (so - 2.0 * sm + sM)) * rdx3;
+            Qhx *= Mag * DL;
+            Qsx *= Mag * DL;
+            return (Qhx + Qsx) / hL;
+        }
+        enum { oM, mm, om, pm, Mo, mo, oo, po, Po, mp, op, pp, oP };
+        const double dx = g.dx, dy = g.dy;
+        const double rdx3 = 1.0 / (dx * dx * dx), rdy3 = 1.0 / (dy * dy * dy);
+        const double ratio2x = (dx * dx) / (dy * dy), ratio2y = (dy * dy) / (dx * dx);
+        double H[13], S[13];
+        for (int k = 0; k < 13; ++k) {
+            if (k == Po || k == oP) { H[k] = S[k] = 0; continue; }
+            H[k] = h[nb(i, k)]; S[k] = s[nb(i, k)];
+        }
+        auto qx = [&](const double *v) {
+            return (((v[po] - 2.0 * v[oo] + v[mo]) + (v[op] - 2.0 * v[oo] + v[om]) * ratio2x) -
+                    ((v[oo] - 2.0 * v[mo] + v[Mo]) + (v[mp] - 2.0 * v[mo] + v[mm]) * ratio2x)) * rdx3;
+        };
+        auto qy = [&](const double *v) {
+            return (((v[po] - 2.0 * v[oo] + v[mo]) * ratio2y + (v[op] - 2.0 * v[oo] + v[om])) -
+                    ((v[pm] - 2.0 * v[om] + v[mm]) * ratio2y + (v[oo] - 2.0 * v[om] + v[oM]))) * rdy3;
+        };
+        if (a == 0) {
+            const double DL = cap_D(i, 0), hL = 0.5 * (h[nb(i, mo)] + h[i]);
+            double Qhx = qx(H), Qsx = qx(S);
+            Qhx *= Mag * DL;
+            Qsx *= Mag * DL;
+            return (Qhx + Qsx) / hL;
+        }
+        const double DB = cap_D(i, 2), hB = 0.5 * (h[nb(i, om)] + h[i]);
+        double Qhy = qy(H), Qsy = qy(S);
+        Qhy *= Mag * DB;
+        Qsy *= Mag * DB;
+        return (Qhy + Qsy) / hB;
+    }
+    /* CV(a) = -C(2a) spacing over the face's two cells (VerticalGravityKernel.h
+     * :143-161): the difference across the Xm (Ym) face */
+    double cv_diff(int i, int a, double vadj, double vcen) const {
+        const double hsp = spacing(a), h2 = hsp * hsp;
+        const double cadj = -(1.0 / h2) * hsp, ccen = -(-1.0 / h2) * hsp;
+        double q = 0;
+        q += cadj * vadj;
+        q += ccen * vcen;
+        (void)i;
+        return q;
+    }
+    /* VerticalGravityKernel.h:435-470 */
+    double vel_vgrav(int i, int a) const {
+        const int adj = nb(i, st.adj[2 * a]);
+        const double Gf = vg_G(i, 2 * a);
+        double Q = cv_diff(i, a, h[adj] + s[adj], h[i] + s[i]);
+        Q *= (-1.0 * cs.Bo / 3.0) * Gf;
+        Q = axis_mask(a, Q);
+        return Q / face_height(i, a);
+    }
+    /* DisjoiningPressureKernel.h:442-475 */
+    double vel_disjoining(int i, int a) const {
+        const int adj = nb(i, st.adj[2 * a]);
+        const double Gf = vg_G(i, 2 * a);
+        double Q = cv_diff(i, a, Pi(h[adj]), Pi(h[i]));
+        Q *= (cs.Dp / 3.0) * Gf;
+        Q = axis_mask(a, Q);
+        return Q / face_height(i, a);
+    }
+    /* SolutoCapillarityKernel.h:387-405 with its NonlinearTerm :182-210
+     * (limited face height) */
+    double vel_marangoni(int i, int a) const {
+        const int d = 2 * a;
+        double S = 0;
+        if (!wall_face(i, d)) {
+            double dc, cc, f[4];
+            face_conc(i, d, dc, cc);
+            face_heights(i, d, f);
+            const double hh = lim.value(f, ma_up(dc));
+            const double mu = visc(cc);
+            S = mob.slip ? dc / mu * mob.Shear(hh) : dc / mu * hh;
+        }
+        return axis_mask(a, ma_coef() * S);
+    }
+    /* BodyForceKernel.h:694-720 (limited): -w Mag D, times VelocitySign -1
+     * (ParallelGravityKernel.h:112, CentrifugalKernel.h:144) */
+    double vel_bodyforce(int mode, int i, int a) const {
+        double hf, D, wt[4];
+        bf_face(mode, i, 2 * a, hf, D, wt, nullptr);
+        double Q = -bf_weight(mode, i, 2 * a) * bf_mag(mode, a) * D;
+        Q = axis_mask(a, Q);
+        return -1.0 * Q;
+    }
+    /* SetVelocityBC (WM2D/HeightEquation_2D.cpp:32-151, WM1D/
+     * HeightEquation_1D.cpp:28-86), Symmetry and Periodic edges */
+    void velocity_bc(vector<double> &vx, vector<double> &vy) const {
+        const int Nx = g.nxp, Ny = g.nyp;
+        auto per = [&](int k) { return g.bc[k] == YZ_BC_PERIODIC; };
+        if (cs.dim == 1) {
+            vx[0] = 0;
+            if (per(0)) vx[1] = vx[Nx - 3]; else { vx[1] = 0; vx[2] = 0; }
+            if (per(1)) vx[Nx - 1] = vx[3]; else { vx[Nx - 2] = 0.0; vx[Nx - 1] = 0.0; }
+            return;
+        }
+        auto X = [&](int ix, int iy) -> double & { return vx[g.row(ix, iy)]; };
+        auto Y = [&](int ix, int iy) -> double & { return vy[g.row(ix, iy)]; };
+        for (int jy = 1; jy < Ny - 1; ++jy) {
+            X(0, jy) = 0;
+            if (per(0)) { Y(1, jy) = Y(Nx - 3, jy); X(1, jy) = X(Nx - 3, jy); }
+            else { Y(1, jy) = Y(2, jy); X(1, jy) = 0; X(2, jy) = 0; }
+            if (per(1)) { Y(Nx - 2, jy) = Y(2, jy); X(Nx - 1, jy) = X(3, jy); }
+            else { Y(Nx - 2, jy) = Y(Nx - 3, jy); X(Nx - 2, jy) = 0.0; X(Nx - 1, jy) = 0.0; }
+        }
+        for (int ix = 1; ix < Nx - 1; ++ix) {
+            Y(ix, 0) = 0;
+            if (per(2)) { X(ix, 1) = X(ix, Ny - 3); Y(ix, 1) = Y(ix, Ny - 3); }
+            else { X(ix, 1) = X(ix, 2); Y(ix, 1) = 0; Y(ix, 2) = 0; }
+            if (per(3)) { X(ix, Ny - 2) = X(ix, 2); Y(ix, Ny - 1) = Y(ix, 3); }
+            else { X(ix, Ny - 2) = X(ix, Ny - 3); Y(ix, Ny - 2) = 0; Y(ix, Ny - 1) = 0; }
+        }
+    }
+    /* CalVelocity: every term's field with its boundary rows, their plain sum
+     * in Cal_MixedVelocity's order (LubFields.cpp:467-495: capillary,
+     * Marangoni, vertical gravity, in-plane gravity, centrifugal,
+     * disjoining), the boundary rows once more. bc = 0 leaves out the
+     * boundary rows (what the terms' CalVelocity alone write). */
+    void velocity(vector<double> &vx, vector<double> &vy, bool bc) const {
+        const int na = st.naxis;
+        const bool on[6] = {cs.capillary != 0, cs.marangoni != 0, cs.vertical_gravity != 0,
+                            cs.inplane_gravity != 0, cs.centrifugal != 0, cs.disjoining != 0};
+        vector<double> t[6][2];
+        for (int k = 0; k < 6; ++k)
+            for (int a = 0; a < 2; ++a) t[k][a].assign(g.n, 0.0);
+        for (int i = 0; i < g.n; ++i) {
+            if (!vel_cell(i)) continue;
+            for (int a = 0; a < na; ++a) {
+                if (on[0]) t[0][a][i] = vel_capillary(i, a);
+                if (on[1]) t[1][a][i] = vel_marangoni(i, a);
+                if (on[2]) t[2][a][i] = vel_vgrav(i, a);
+                if (on[3]) t[3][a][i] = vel_bodyforce(0, i, a);
+                if (on[4]) t[4][a][i] = vel_bodyforce(1, i, a);
+                if (on[5]) t[5][a][i] = vel_disjoining(i, a);
+            }
+        }
+        if (bc)
+            for (int k = 0; k < 6; ++k) velocity_bc(t[k][0], t[k][1]);
+        vx.assign(g.n, 0.0); vy.assign(g.n, 0.0);
+        for (int i = 0; i < g.n; ++i) {
+            vx[i] = t[0][0][i] + t[1][0][i] + t[2][0][i] + t[3][0][i] + t[4][0][i] + t[5][0][i];
+            vy[i] = t[0][1][i] + t[1][1][i] + t[2][1][i] + t[3][1][i] + t[4][1][i] + t[5][1][i];
+        }
+        if (bc) velocity_bc(vx, vy);
+    }
+
     /* FaceVolumeFluxes (HeightEquationBase.h:852-894) of the working field */
     void face_volume_fluxes(vector<double> F[2], vector<double> &r) const {
         for (int a = 0; a < st.naxis; ++a) F[a].assign(g.n, 0.0);
@@ -2268,6 +2439,21 @@ void yzo_assemble(void *p, const double *h_old, const double *h, double dt,
     if (ax) std::copy(st->Ax_guess.begin(), st->Ax_guess.end(), ax);
     if (rhs) std::copy(st->Rhs.begin(), st->Rhs.end(), rhs);
     st->dt_now = save_dt;
+}
+/* the face velocities of the field (h, c) (padded): vx on the Xm face and
+ * vy on the Ym face of every cell; bc = 0 without the boundary rows */
+void yzo_velocity(void *p, const double *h, const double *c, int bc, double *vx, double *vy) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    auto &eq = st->eq;
+    const int n = eq.g.n;
+    const std::vector<double> hs = eq.h, cs = eq.c;
+    eq.h.assign(h, h + n); eq.c.assign(c, c + n);
+    eq.fill_ghosts(eq.h); eq.fill_ghosts(eq.c);
+    std::vector<double> ux, uy;
+    eq.velocity(ux, uy, bc != 0);
+    std::copy(ux.begin(), ux.end(), vx);
+    if (vy) std::copy(uy.begin(), uy.end(), vy);
+    eq.h = hs; eq.c = cs;
 }
 /* B's pieces (for the tests against the reference's tables): out2 =
  * Ev E(h, c) of the law alone (Evaporation.cpp:62-103 times Ev), EvapAt's

@@ -43,7 +43,10 @@ def read_case(path):
         ndir = 2 if nxp == n else 4
         ff = np.fromfile(f, dtype=np.float64, count=n * ndir).reshape(n, ndir)
         side = np.fromfile(f, dtype=np.float64, count=n)
-    return dict(rp=rp, col=col, av=av, rpj=rpj, colj=colj, jv=jv, h=h, s=s, c=c, ax=ax, ff=ff, side=side)
+        vx = np.fromfile(f, dtype=np.float64, count=n)
+        vy = np.fromfile(f, dtype=np.float64, count=n)
+    return dict(rp=rp, col=col, av=av, rpj=rpj, colj=colj, jv=jv, h=h, s=s, c=c, ax=ax, ff=ff, side=side,
+                vx=vx, vy=vy)
 
 
 def read_table(path):

@@ -1,0 +1,62 @@
+"""The film's face velocities on the device (yz_step_velocity, CalVelocity
+with the equation's boundary rows) through the C-ABI, against the oracle
+(whose terms match the reference's CalVelocity to the bit and whose
+boundary rows are pinned by tests/test_oracle_velocity.py): every face of
+the padded grid within 1e-12 of the field's largest velocity, 1-D and 2-D,
+Symmetry and Periodic, with and without slip, every term on and each term
+alone; and after steps of the all-physics film, at the film the step
+reached."""
+import numpy as np
+import pytest
+
+from oracle import binding as ob
+from yuzen_amd import cases
+from yuzen_amd.step import HipStep
+
+from .helpers import TERMS, smooth_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(c, h, s, cc, label):
+    print(label, flush=True)
+    o = ob.OracleStep(c, h, s, cc, 1e-3)
+    g = HipStep(c, h, s, cc, 1e-3)
+    ux, uy = g.velocity()
+    g.close()
+    vx, vy = o.velocity(h, cc)
+    comps = ((ux, vx), (uy, vy)) if c.dim == 2 else ((ux, vx),)
+    for got, want in comps:
+        scale = np.abs(want).max()
+        assert scale > 0
+        err = np.abs(got - want).max() / scale
+        print(f"  max |dv| / max |v| = {err:.2e}")
+        assert err <= 1e-12
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("slip", [0.0, 0.05])
+def test_velocity_matches_oracle(dim, periodic, slip):
+    c, h, s, cc = smooth_case(dim, TERMS, slip=slip, periodic=periodic)
+    _compare(c, h, s, cc, f"velocity dim={dim} periodic={periodic} slip={slip}")
+
+
+@pytest.mark.parametrize("term", TERMS)
+def test_velocity_each_term_matches_oracle(term):
+    c, h, s, cc = smooth_case(2, [term], slip=0.05)
+    _compare(c, h, s, cc, f"velocity of {term}")
+
+
+def test_velocity_after_steps():
+    c, h, s, cc, dt, _ = cases.allphys_2d(n=48, L=2.0, dt=2e-5)
+    g = HipStep(c, h, s, cc, dt)
+    for _ in range(3):
+        assert g.advance()[0] == 0
+    hg = g.h()
+    ux, uy = g.velocity()
+    g.close()
+    o = ob.OracleStep(c, h, s, cc, dt)
+    vx, vy = o.velocity(hg, cc)
+    for got, want in ((ux, vx), (uy, vy)):
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()

@@ -72,6 +72,21 @@ def test_face_fluxes_match_reference(m):
     assert np.abs(r - g["side"]).max() <= 1e-12 * max(np.abs(g["side"]).max(), 1e-300)
 
 
+@pytest.mark.parametrize("m", MAN["cases"], ids=[m["name"] for m in MAN["cases"]])
+def test_face_velocities_match_reference(m):
+    """One term's face velocities (its CalVelocity, summed by
+    Cal_MixedVelocity) against the reference's, on every face the term
+    writes (the equation's boundary rows are not part of the term)."""
+    g = {k: TERMS[f"{m['name']}__{k}"] for k in ("h", "s", "c", "vx", "vy")}
+    c = case_from(m)
+    o = ob.OracleStep(c, g["h"], g["s"], g["c"], 1e-3)
+    vx, vy = o.velocity(g["h"], g["c"], bc=False)
+    for got, want in ((vx, g["vx"]), (vy, g["vy"])):
+        scale = max(np.abs(want).max(), 1e-300)
+        assert np.abs(got - want).max() <= 1e-12 * scale, (m["name"], np.abs(got - want).max() / scale)
+    assert np.count_nonzero(g["vx"]) > 0
+
+
 def test_conc_schemes_match_reference():
     for row in SCAL["schemes"]:
         k, sign, f, v, w = int(row[0]), row[1], row[2:6], row[6], row[7:11]

@@ -190,6 +190,7 @@ def load():
     lib.yz_step_create.argtypes = [P(YzCase), pd, pd, pd, d]
     lib.yz_step_advance.argtypes = [C.c_void_p, pi_, pi_, pd]
     lib.yz_step_get_h.argtypes = [C.c_void_p, pd]
+    lib.yz_step_velocity.argtypes = [C.c_void_p, pd, pd]
     lib.yz_step_set_h.argtypes = [C.c_void_p, pd]
     lib.yz_step_clock.argtypes = [C.c_void_p, pd, pd, pd]
     lib.yz_step_timing.argtypes = [C.c_void_p, pd]

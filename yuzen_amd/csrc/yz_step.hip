@@ -27,6 +27,7 @@
 
 #include "../../include/yuzen_gpu.h"
 #include "yz_conc.h"
+#include "yz_velocity.h"
 #include "yz_rows.h"
 #include "yz_solver.h"
 #include "yz_mlp.h"
@@ -1181,6 +1182,44 @@ int yz_step_conc_stats(yz_step *S, double *out5) {
     if (!S || !out5) { set_err("null argument"); return 2; }
     for (int k = 0; k < 5; ++k) out5[k] = S->cstat[k];
     return 0;
+}
+
+// CalVelocity (HeightEquationBase.h:1583-1601) of the context's film and
+// concentration: the face velocities with their boundary rows, padded
+int yz_step_velocity(yz_step *S, double *ux, double *uy) {
+    if (!S || !ux || (S->g.dim == 2 && !uy)) { set_err("null argument"); return 2; }
+    const Geo &g = S->g;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    double *d = nullptr;
+    YZ_TRY(hipMalloc((void **)&d, 2 * np * sizeof(double)));
+    double *vx = d, *vy = d + np;
+    RowArgs a = S->ra;
+    a.h = S->hnew;
+    int rc = 0;
+    do {
+        if (hipMemsetAsync(d, 0, 2 * np * sizeof(double), S->st) != hipSuccess) { rc = 2; break; }
+        const int cells = (g.nxp - 3) * (g.dim == 2 ? g.nyp - 3 : 1);
+        // the boundary rows twice: once for the terms' fields, once for the
+        // mixed field (HeightEquationBase.h:1591-1600)
+        if (g.dim == 2) {
+            hipLaunchKernelGGL(k_velocity<2>, dim3(nb(cells)), dim3(TB), 0, S->st, a, vx, vy);
+            for (int pass = 0; pass < 2; ++pass) {
+                hipLaunchKernelGGL(k_velocity_bc_x, dim3(nb(g.nyp)), dim3(TB), 0, S->st, g, vx, vy);
+                hipLaunchKernelGGL(k_velocity_bc_y, dim3(nb(g.nxp)), dim3(TB), 0, S->st, g, vx, vy);
+            }
+        } else {
+            hipLaunchKernelGGL(k_velocity<1>, dim3(nb(cells)), dim3(TB), 0, S->st, a, vx, vy);
+            for (int pass = 0; pass < 2; ++pass)
+                hipLaunchKernelGGL(k_velocity_bc_x, dim3(1), dim3(TB), 0, S->st, g, vx, vy);
+        }
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(ux, vx, np * sizeof(double), hipMemcpyDeviceToHost, S->st) != hipSuccess ||
+            (g.dim == 2 && hipMemcpyAsync(uy, vy, np * sizeof(double), hipMemcpyDeviceToHost, S->st) != hipSuccess) ||
+            hipStreamSynchronize(S->st) != hipSuccess) { rc = 2; break; }
+    } while (0);
+    (void)hipFree(d);
+    if (rc) set_err("device error in yz_step_velocity");
+    return rc;
 }
 
 int yz_step_get_h(yz_step *S, double *h) {

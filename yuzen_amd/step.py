@@ -75,6 +75,14 @@ class HipStep:
             raise RuntimeError(capi.last_error())
         return out
 
+    def velocity(self):
+        """the face velocities of the current film (CalVelocity): (ux, uy),
+        padded, ux on the Xm face and uy on the Ym face of each cell"""
+        ux, uy = np.empty(self.n), np.zeros(self.n)
+        if self.lib.yz_step_velocity(self.p, _pd(ux), _pd(uy)):
+            raise RuntimeError(capi.last_error())
+        return ux, uy
+
     def set_h(self, h):
         h = np.ascontiguousarray(h, dtype=np.float64)
         if self.lib.yz_step_set_h(self.p, _pd(h)):
