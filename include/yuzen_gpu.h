@@ -232,6 +232,29 @@ void yz_step_destroy(yz_step *st);
  * Faces neither the terms nor the boundary rows write are 0. */
 int yz_step_velocity(yz_step *st, double *ux, double *uy);
 
+/* The ASCII field file (.dat) of a well-mixed film, in the reference's
+ * layout (OutputNeutral::Write of Field_WM1D_Out / Field_WM2D_Out,
+ * Output.cpp:830-920, LubFields.cpp:204-290, 322-372, 602-900): a header
+ * line, then one line per point (the wall faces and the real cell centres,
+ * nx+2 (x ny+2) points, x fastest) with x (y), h+s, the velocity, c, h, s
+ * at 17 digits after the point. The fields are padded arrays of n doubles
+ * in the case's layout, the face velocities as yz_step_velocity gives them.
+ * dimensioned != 0 writes mm, um and m/s with the scales L0 [m], H0 [m],
+ * U0 [m/s]. The file is written to path.tmp and renamed. 0 / 2. Host only. */
+int yz_write_dat(const char *path, const yz_case *c, const double *h, const double *s, const double *conc,
+                 const double *ux, const double *uy, double L0, double H0, double U0, int dimensioned);
+/* Reads such a file as an initial condition (C_Restart::ReadAscii_WM1D /
+ * ReadAscii_WM2D, Restart.cpp:170-311): point k goes to the padded cell one
+ * in from the grid's outer ghost layer, h and s times H_coef, c as read;
+ * the other cells of h, s, conc are left as given (a step context made from
+ * them forms its ghosts itself). A row must hold the header's count of
+ * finite numbers (a seventh / ninth column when the header names PHI), and
+ * the file as many points as the grid. 0 / 2. Host only. */
+int yz_read_dat(const char *path, const yz_case *c, double H_coef, double *h, double *s, double *conc);
+/* yz_write_dat of the context's film, substrate, concentration and face
+ * velocities (the reference's field output at the end of a step). */
+int yz_step_write_dat(yz_step *st, const char *path, double L0, double H0, double U0, int dimensioned);
+
 /* Assembly-level access (parity tests): at film h (n doubles, padded),
  * with the step's dt and the explicit half Ax_old taken at h_old:
  *   ia[13*N] (2-D) / ia[5*N] (1-D): rows of I + theta dt J over the N

@@ -46,6 +46,8 @@
 #include "TimeSteps.h"
 #include "WM2D/Source.h"
 #include "WM1D/Source_1D.h"
+#include "Output.h"
+#include "Restart.h"
 
 using namespace yuzen;
 using namespace yuzen::grid;
@@ -588,6 +590,102 @@ void dump_evap_source(const std::string &dir) {
     write_vec(f, out);
 }
 
+// The ASCII field file (.dat) that OutputNeutral writes (Output.cpp:830-920)
+// from Field_WM1D_Out / Field_WM2D_Out (LubFields.cpp:204-290, 322-372,
+// 602-900): on a smooth film with a velocity field of known values, without
+// and with dimensions; then what ReadAscii_WM1D / ReadAscii_WM2D
+// (Restart.cpp:236-311) make of the first file with H_coef 1.7. Per
+// dimension: neutral_<d>d_in.bin (padded h, s, c, face x and y
+// velocities), neutral_<d>d_nd00000.dat, neutral_<d>d_dim00000.dat,
+// neutral_<d>d_read.bin (padded h, s, c after the read).
+template <class FieldT, class OutT>
+void neutral_case(const std::string &dir, bool twoD) {
+    Params p;
+    Input inp = MakeInput(tLaplace, p);
+    const int nx = twoD ? 18 : 28, ny = 14;
+    inp.Grid = MakeGrid(twoD, nx, ny);
+    NonDimensionalNumbers nd;
+    nd.Build(inp, inp.Initial.SurfaceShape.TotalHeight);
+    C_TimeSteps ts;
+    ts.Build(1e-3, 0.0, 1e9, false);
+    FieldT field;
+    field.Build(inp.Grid, Flat(), nd.ExtentScale());
+    auto *G = field.GetGridPtr();
+    const int nxp = nx + 4, nyp = twoD ? ny + 4 : 1, n = nxp * nyp;
+    std::vector<double> hv(n), sv(n), cv(n), vx(n), vy(n, 0.0);
+    Shapes sh{false};
+    const double dx = G->dx();
+    const double dy = twoD ? (0.8 + 4.0 * 0.8 / ny) / nyp : 0.0;
+    for (int iy = 0; iy < nyp; ++iy)
+        for (int ix = 0; ix < nxp; ++ix) {
+            const int i = iy * nxp + ix;
+            const double x = -0.5 - 2.0 * (1.0 / nx) + (ix + 0.5) * dx;
+            const double y = twoD ? -0.4 - 2.0 * (0.8 / ny) + (iy + 0.5) * dy : 0.0;
+            hv[i] = sh.H(x, y); sv[i] = sh.S(x, y); cv[i] = sh.C(x, y);
+            vx[i] = 0.1 * std::sin(0.37 * ix + 0.11 * iy) + 0.05 * iy;
+            if (twoD) vy[i] = 0.07 * std::cos(0.23 * ix - 0.41 * iy) - 0.02 * ix;
+        }
+    for (int iy = 0; iy < nyp; ++iy)
+        for (int ix = 0; ix < nxp; ++ix) {
+            const int i = iy * nxp + ix;
+            if constexpr (std::is_same<FieldT, Field_WM2D>::value) {
+                const Zface z = G->template At<Zface>(ix, iy);
+                field.HH(z) = hv[i]; field.SS(z) = sv[i]; field.CC(z) = cv[i];
+                field.UU().UU().x(G->template At<Yedge>(ix, iy)) = vx[i];
+                field.UU().UU().y(G->template At<Xedge>(ix, iy)) = vy[i];
+            } else {
+                const Xedge z = G->template At<Xedge>(ix);
+                field.HH(z) = hv[i]; field.SS(z) = sv[i]; field.CC(z) = cv[i];
+                field.UU().UU().x(G->template At<Node>(ix)) = vx[i];
+            }
+        }
+    const std::string tag = std::string("neutral_") + (twoD ? "2d" : "1d");
+    {
+        std::ofstream f(dir + "/" + tag + "_in.bin", std::ios::binary);
+        const int hdr[2] = {n, nxp};
+        f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+        write_vec(f, hv); write_vec(f, sv); write_vec(f, cv); write_vec(f, vx); write_vec(f, vy);
+    }
+    for (int dimd = 0; dimd < 2; ++dimd) {
+        OutT out;
+        out.Build(field, 1.3e-3, 2.5e-6, 0.07, dimd == 1);
+        out.ConvertFromStg(field);
+        OutputNeutral w;
+        w.Build(ts, nd, dimd == 1, FilePath(dir), tag + (dimd ? "_dim" : "_nd"), 0);
+        w.Write(out);
+    }
+    FieldT back;
+    back.Build(inp.Grid, Flat(), nd.ExtentScale());
+    Input::I_Restart ri;
+    ri.Enable = true; ri.ASCII = true; ri.FileName = dir + "/" + tag + "_nd00000.dat";
+    ri.RestartTime = 0; ri.FileIndex = 0;
+    bool ok;
+    if constexpr (std::is_same<FieldT, Field_WM2D>::value) ok = C_Restart::ReadAscii_WM2D(ri, back, 1.7);
+    else ok = C_Restart::ReadAscii_WM1D(ri, back, 1.7);
+    std::vector<double> rh(n), rs(n), rc(n);
+    auto *B = back.GetGridPtr();
+    for (int iy = 0; iy < nyp; ++iy)
+        for (int ix = 0; ix < nxp; ++ix) {
+            const int i = iy * nxp + ix;
+            if constexpr (std::is_same<FieldT, Field_WM2D>::value) {
+                const Zface z = B->template At<Zface>(ix, iy);
+                rh[i] = back.HH(z); rs[i] = back.SS(z); rc[i] = back.CC(z);
+            } else {
+                const Xedge z = B->template At<Xedge>(ix);
+                rh[i] = back.HH(z); rs[i] = back.SS(z); rc[i] = back.CC(z);
+            }
+        }
+    std::ofstream f(dir + "/" + tag + "_read.bin", std::ios::binary);
+    const int hdr[2] = {n, ok ? 1 : 0};
+    f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    write_vec(f, rh); write_vec(f, rs); write_vec(f, rc);
+}
+
+void dump_neutral(const std::string &dir) {
+    neutral_case<Field_WM2D, Field_WM2D_Out>(dir, true);
+    neutral_case<Field_WM1D, Field_WM1D_Out>(dir, false);
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_golden OUTDIR\n"); return 1; }
     const std::string dir = argv[1];
@@ -608,6 +706,7 @@ int main(int argc, char **argv) {
     dump_schemes(dir);
     dump_state(dir);
     dump_evap_source(dir);
+    dump_neutral(dir);
     std::printf("golden vectors written to %s\n", dir.c_str());
     return 0;
 }

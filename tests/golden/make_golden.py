@@ -76,6 +76,26 @@ def main():
                             source=read_table(os.path.join(tmp, "source.bin")))
         with open(os.path.join(HERE, "manifest.json"), "w") as f:
             json.dump(man, f, indent=1)
+        # the ASCII field files OutputNeutral wrote, their inputs and what
+        # the reference's ASCII restart read back from them
+        neutral = {}
+        os.makedirs(os.path.join(HERE, "neutral"), exist_ok=True)
+        for d in ("1d", "2d"):
+            with open(os.path.join(tmp, f"neutral_{d}_in.bin"), "rb") as f:
+                n, nxp = np.fromfile(f, dtype=np.int32, count=2)
+                for k in ("h", "s", "c", "vx", "vy"):
+                    neutral[f"{d}__{k}"] = np.fromfile(f, dtype=np.float64, count=n)
+                neutral[f"{d}__nxp"] = np.array(nxp)
+            with open(os.path.join(tmp, f"neutral_{d}_read.bin"), "rb") as f:
+                n, ok = np.fromfile(f, dtype=np.int32, count=2)
+                assert ok == 1
+                for k in ("h", "s", "c"):
+                    neutral[f"{d}__read_{k}"] = np.fromfile(f, dtype=np.float64, count=n)
+            for kind in ("nd", "dim"):
+                src = os.path.join(tmp, f"neutral_{d}_{kind}00000.dat")
+                with open(src, "rb") as f, open(os.path.join(HERE, "neutral", f"{d}_{kind}.dat"), "wb") as g:
+                    g.write(f.read())
+        np.savez_compressed(os.path.join(HERE, "ref_neutral.npz"), **neutral)
         with open(os.path.join(tmp, "ref_state_wm1d.state"), "rb") as f:
             state = f.read()
         with open(os.path.join(HERE, "ref_state_wm1d.state"), "wb") as f:

@@ -1,0 +1,116 @@
+"""The ASCII field file (.dat) through the C-ABI's host functions, against
+the reference's own writer and reader (OutputNeutral::Write and
+C_Restart::ReadAscii_WM1D / WM2D, built from the reference's sources by
+oracle/build_ref.sh; the fixtures are tests/golden/neutral/*.dat and
+tests/golden/ref_neutral.npz, written by oracle/ref_golden.cpp):
+* from the same padded fields and face velocities, yz_write_dat writes the
+  reference's file byte for byte, 1-D and 2-D, without and with dimensions;
+* yz_read_dat puts every number of the reference's file where the
+  reference's reader puts it, with its H_coef;
+* a file read back gives the real cells to the bit; a file whose point
+  count or rows do not fit is refused.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from yuzen_amd import capi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+NEU = np.load(os.path.join(GOLD, "ref_neutral.npz"))
+
+
+def _case(d):
+    c = capi.case_defaults()
+    if d == "2d":
+        c.dim, c.nx, c.ny = 2, 18, 14
+        c.xmin, c.xmax, c.ymin, c.ymax = -0.5, 0.5, -0.4, 0.4
+    else:
+        c.dim, c.nx, c.ny = 1, 28, 1
+        c.xmin, c.xmax, c.ymin, c.ymax = -0.5, 0.5, -1.0, 1.0
+    return c
+
+
+def _fields(d):
+    return [NEU[f"{d}__{k}"] for k in ("h", "s", "c", "vx", "vy")]
+
+
+@pytest.mark.parametrize("d", ["1d", "2d"])
+@pytest.mark.parametrize("kind", ["nd", "dim"])
+def test_writer_matches_reference_byte_for_byte(d, kind, tmp_path):
+    h, s, c, vx, vy = _fields(d)
+    out = str(tmp_path / "f.dat")
+    capi.write_dat(out, _case(d), h, s, c, vx, vy, L0=1.3e-3, H0=2.5e-6, U0=0.07, dimensioned=(kind == "dim"))
+    want = open(os.path.join(GOLD, "neutral", f"{d}_{kind}.dat"), "rb").read()
+    got = open(out, "rb").read()
+    assert got == want
+    assert not os.path.exists(out + ".tmp")
+
+
+def _written_cells(d, n):
+    nxp = int(NEU[f"{d}__nxp"])
+    nyp = n // nxp
+    m = np.zeros((nyp, nxp), dtype=bool)
+    if d == "1d":
+        m[0, 1:nxp - 1] = True
+    else:
+        m[1:nyp - 1, 1:nxp - 1] = True
+    return m.ravel()
+
+
+@pytest.mark.parametrize("d", ["1d", "2d"])
+def test_reader_matches_reference(d):
+    n = NEU[f"{d}__h"].size
+    z = np.zeros(n)
+    h, s, c = capi.read_dat(os.path.join(GOLD, "neutral", f"{d}_nd.dat"), _case(d), z, z, z, H_coef=1.7)
+    m = _written_cells(d, n)
+    for got, k in ((h, "h"), (s, "s"), (c, "c")):
+        want = NEU[f"{d}__read_{k}"]
+        assert np.array_equal(got[m], want[m]), k
+        assert np.all(got[~m] == 0)
+
+
+@pytest.mark.parametrize("d", ["1d", "2d"])
+def test_round_trip_keeps_the_real_cells(d, tmp_path):
+    h, s, c, vx, vy = _fields(d)
+    out = str(tmp_path / "f.dat")
+    capi.write_dat(out, _case(d), h, s, c, vx, vy)
+    z = np.full(h.size, -7.0)
+    h2, s2, c2 = capi.read_dat(out, _case(d), z, z, z)
+    nxp = int(NEU[f"{d}__nxp"])
+    nyp = h.size // nxp
+    real = np.zeros((nyp, nxp), dtype=bool)
+    if d == "1d":
+        real[0, 2:nxp - 2] = True
+    else:
+        real[2:nyp - 2, 2:nxp - 2] = True
+    real = real.ravel()
+    for a, b in ((h, h2), (s, s2), (c, c2)):
+        assert np.array_equal(a[real], b[real])
+
+
+def test_bad_files_are_refused(tmp_path):
+    h, s, c, vx, vy = _fields("1d")
+    good = str(tmp_path / "g.dat")
+    capi.write_dat(good, _case("1d"), h, s, c, vx, vy)
+    lines = open(good).read().splitlines()
+    z = np.zeros(h.size)
+    short = tmp_path / "short.dat"
+    short.write_text("\n".join(lines[:-1]) + "\n")
+    with pytest.raises(RuntimeError):
+        capi.read_dat(str(short), _case("1d"), z, z, z)
+    bad = tmp_path / "bad.dat"
+    bad.write_text("\n".join(lines[:5] + [lines[5].replace("e", "x", 1)] + lines[6:]) + "\n")
+    with pytest.raises(RuntimeError):
+        capi.read_dat(str(bad), _case("1d"), z, z, z)
+    # a seventh column is read only under a header that names PHI
+    phi = tmp_path / "phi.dat"
+    phi.write_text("\n".join([lines[0] + " \t PHI[-]"] + [l + "\t1.5" for l in lines[1:]]) + "\n")
+    h2, _, _ = capi.read_dat(str(phi), _case("1d"), z, z, z)
+    assert np.array_equal(h2[2:-2], h[2:-2])
+    nophi = tmp_path / "nophi.dat"
+    nophi.write_text("\n".join([lines[0]] + [l + "\t1.5" for l in lines[1:]]) + "\n")
+    with pytest.raises(RuntimeError):
+        capi.read_dat(str(nophi), _case("1d"), z, z, z)

@@ -7,6 +7,8 @@ sizes against the compiled library.
 import ctypes as C
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libyuzen_hip.so")
 
@@ -191,6 +193,9 @@ def load():
     lib.yz_step_advance.argtypes = [C.c_void_p, pi_, pi_, pd]
     lib.yz_step_get_h.argtypes = [C.c_void_p, pd]
     lib.yz_step_velocity.argtypes = [C.c_void_p, pd, pd]
+    lib.yz_write_dat.argtypes = [C.c_char_p, C.POINTER(YzCase), pd, pd, pd, pd, pd, d, d, d, C.c_int]
+    lib.yz_read_dat.argtypes = [C.c_char_p, C.POINTER(YzCase), d, pd, pd, pd]
+    lib.yz_step_write_dat.argtypes = [C.c_void_p, C.c_char_p, d, d, d, C.c_int]
     lib.yz_step_set_h.argtypes = [C.c_void_p, pd]
     lib.yz_step_clock.argtypes = [C.c_void_p, pd, pd, pd]
     lib.yz_step_timing.argtypes = [C.c_void_p, pd]
@@ -239,6 +244,28 @@ def load():
     lib.yz_sizeof_lin_settings.restype = C.c_int
     _lib = lib
     return lib
+
+
+def write_dat(path, case, h, s, c, ux, uy=None, L0=1.0, H0=1.0, U0=1.0, dimensioned=False):
+    """the reference's ASCII field file of padded fields (host code)"""
+    lib = load()
+    keep = [np.ascontiguousarray(v, dtype=np.float64) for v in (h, s, c, ux, uy if uy is not None else ux)]
+    ptrs = [v.ctypes.data_as(C.POINTER(C.c_double)) for v in keep]
+    if lib.yz_write_dat(os.fsencode(path), C.byref(case), *ptrs, L0, H0, U0, 1 if dimensioned else 0):
+        raise RuntimeError(last_error())
+
+
+def read_dat(path, case, h, s, c, H_coef=1.0):
+    """an ASCII field file into copies of the padded h, s, c (host code)"""
+    lib = load()
+    h, s, c = (np.array(v, dtype=np.float64) for v in (h, s, c))
+    if lib.yz_read_dat(os.fsencode(path), C.byref(case), H_coef, _arr_ptr(h), _arr_ptr(s), _arr_ptr(c)):
+        raise RuntimeError(last_error())
+    return h, s, c
+
+
+def _arr_ptr(v):
+    return v.ctypes.data_as(C.POINTER(C.c_double))
 
 
 def last_error():

@@ -1222,6 +1222,18 @@ int yz_step_velocity(yz_step *S, double *ux, double *uy) {
     return rc;
 }
 
+int yz_step_write_dat(yz_step *S, const char *path, double L0, double H0, double U0, int dimensioned) {
+    if (!S || !path) { set_err("null argument"); return 2; }
+    const size_t np = (size_t)S->g.nxp * S->g.nyp;
+    std::vector<double> h(np), s(np), c(np), ux(np), uy(np, 0.0);
+    if (yz_step_velocity(S, ux.data(), uy.data())) return 2;
+    YZ_TRY(hipMemcpyAsync(h.data(), S->hnew, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(s.data(), S->s, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(c.data(), S->c, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return yz_write_dat(path, &S->cs, h.data(), s.data(), c.data(), ux.data(), uy.data(), L0, H0, U0, dimensioned);
+}
+
 int yz_step_get_h(yz_step *S, double *h) {
     if (!S || !h) { set_err("null argument"); return 2; }
     const int np = S->g.nxp * S->g.nyp;

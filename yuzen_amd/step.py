@@ -83,6 +83,11 @@ class HipStep:
             raise RuntimeError(capi.last_error())
         return ux, uy
 
+    def write_dat(self, path, L0=1.0, H0=1.0, U0=1.0, dimensioned=False):
+        """the reference's ASCII field file of the current film"""
+        if self.lib.yz_step_write_dat(self.p, os.fsencode(path), L0, H0, U0, 1 if dimensioned else 0):
+            raise RuntimeError(capi.last_error())
+
     def set_h(self, h):
         h = np.ascontiguousarray(h, dtype=np.float64)
         if self.lib.yz_step_set_h(self.p, _pd(h)):
