@@ -232,6 +232,17 @@ void yz_step_destroy(yz_step *st);
  * Faces neither the terms nor the boundary rows write are 0. */
 int yz_step_velocity(yz_step *st, double *ux, double *uy);
 
+/* The energy budget of the context's film (Output.Energy: EnergyBudget.h
+ * :52-166, FluxPower HeightEquationBase.h:275-291): F the free energy
+ * (sigma/2 |grad(h+s)|^2 + Bo/2 (h+s)^2 + Dp w(h), summed over the real cells
+ * and faces times the cell measure) and D the power of the film's face
+ * volume fluxes against the potential phi = sigma lap(h+s) + Dp Pi(h) -
+ * Bo (h+s), times the cell measure: with Crank-Nicolson,
+ * (F^{n+1} - F^n)/dt + (D^n + D^{n+1})/2 = O(dt^2) for the capillary,
+ * disjoining and vertical-gravity terms on a closed or periodic film.
+ * The reference's Energy.dat writes F, D and that balance. 0 / 2. */
+int yz_step_energy(yz_step *st, double *F, double *D);
+
 /* The ASCII field file (.dat) of a well-mixed film, in the reference's
  * layout (OutputNeutral::Write of Field_WM1D_Out / Field_WM2D_Out,
  * Output.cpp:830-920, LubFields.cpp:204-290, 322-372, 602-900): a header

@@ -45,6 +45,7 @@ def load():
     lib.yzo_mobility.argtypes = [C.c_double, C.c_double, C.c_double, pd]
     lib.yzo_evap.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, pd]
     lib.yzo_velocity.argtypes = [C.c_void_p, pd, pd, C.c_int, pd, pd]
+    lib.yzo_energy.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_source.argtypes = [C.c_void_p, C.c_double, pd]
     lib.yzo_limiter.argtypes = [pd, C.c_double, pd]
     lib.yzo_get_c.argtypes = [C.c_void_p, pd]
@@ -151,6 +152,15 @@ class OracleStep:
         vx, vy = np.zeros(self.n), np.zeros(self.n)
         self.lib.yzo_velocity(self.p, _pd(h), _pd(c), 1 if bc else 0, _pd(vx), _pd(vy))
         return vx, vy
+
+    def energy(self, h, c):
+        """(F, D, phi): the free energy, the power of the face fluxes against
+        the potential, and the potential of the field (h, c)"""
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        out, phi = np.zeros(2), np.zeros(self.n)
+        self.lib.yzo_energy(self.p, _pd(h), _pd(c), _pd(out), _pd(phi))
+        return out[0], out[1], phi
 
     def conc_system(self):
         """the last concentration system: (rowptr, col, val, rhs), unscaled"""

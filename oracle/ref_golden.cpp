@@ -14,8 +14,9 @@
 //     storage order, and A(h)h (the conservative form for the capillary and
 //     vertical-gravity terms, AddConservative; the product otherwise);
 //   * the term's FaceFlux through every face of every real cell and its
-//     SideRate (what FaceVolumeFluxes sums for the concentration), and its
-//     face velocities (CalVelocity);
+//     SideRate (what FaceVolumeFluxes sums for the concentration), its
+//     face velocities (CalVelocity), and the free energy and potential of
+//     EnergyBudget.h with the term's parameters;
 //   * Mobility::Face/FaceDK and derivatives over a grid of heights and slip
 //     lengths; the Koren limiter's face value, weights and derivative;
 //     NonDimensionalNumbers::Dp over contact angles; FaceValue and
@@ -48,6 +49,7 @@
 #include "WM1D/Source_1D.h"
 #include "Output.h"
 #include "Restart.h"
+#include "EnergyBudget.h"
 
 using namespace yuzen;
 using namespace yuzen::grid;
@@ -165,6 +167,11 @@ void write_ivec(std::ofstream &f, const std::vector<int> &v) {
     f.write(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(int));
 }
 
+// the energy parameters of the case being dumped (set by run_case): the
+// free energy and the potential of EnergyBudget.h on the case's field
+PotentialParams g_pp;
+bool g_periodic = false;
+
 template <class Tr, class FieldT, class TermT>
 void dump_term(TermT &term, FieldT &field, const Shapes &sh, const std::string &path, bool conservative) {
     typedef typename Tr::Grid Grid;
@@ -256,6 +263,19 @@ void dump_term(TermT &term, FieldT &field, const Shapes &sh, const std::string &
             vx[i] = field.UU().UU().x(G->template At<Node>(ix));
         }
     }
+    // FreeEnergy and Potential (EnergyBudget.h) with this term's parameters
+    std::vector<double> en(1 + n, 0.0);
+    en[0] = energy_budget::FreeEnergy(field, g_pp, g_periodic, g_periodic);
+    {
+        // phi is indexed by the grid's linear index: to rows
+        std::vector<double> phi;
+        energy_budget::Potential(field, g_pp, phi);
+        for (int ii = G->imin(); ii <= G->imax(); ++ii) {
+            Site st = G->template Of<Site>(ii);
+            const int row = Tr::Index(G, st);
+            if (row >= 0 && row < n && ii < (int)phi.size()) en[1 + row] = phi[(size_t)ii];
+        }
+    }
     std::ofstream f(path, std::ios::binary);
     const int hdr[4] = {n, (int)col.size(), (int)colj.size(), nxp};
     f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
@@ -264,6 +284,7 @@ void dump_term(TermT &term, FieldT &field, const Shapes &sh, const std::string &
     write_vec(f, hv); write_vec(f, sv); write_vec(f, cv); write_vec(f, axv);
     write_vec(f, ffv); write_vec(f, side);
     write_vec(f, vx); write_vec(f, vy);
+    write_vec(f, en);
 }
 
 template <class K>
@@ -287,6 +308,15 @@ std::string run_case(const std::string &dir, bool twoD, Term t, double slip, boo
     name << (twoD ? "2d" : "1d") << "_" << kName[t] << "_slip" << (slip > 0 ? 1 : 0) << (periodic ? "_per" : "_sym");
     const std::string path = dir + "/" + name.str() + ".bin";
     const bool cons = (t == tLaplace || t == tVertical);
+    g_pp = PotentialParams();
+    g_pp.capillary = (t == tLaplace);
+    g_pp.dp = (t == tDisjoining) ? nd.Dp() : 0.0;
+    g_pp.hstar = nd.HStar();
+    g_pp.n = inp.Physics.DisjoiningPressure.ExponentN;
+    g_pp.m = inp.Physics.DisjoiningPressure.ExponentM;
+    g_pp.bo = (t == tVertical) ? nd.Bo() : 0.0;
+    g_pp.nd = &nd;
+    g_periodic = periodic;
     const C_Interpolation_base4::t_Scheme koren = C_Interpolation_base4::Koren;
     if (twoD) {
         Field_WM2D field;

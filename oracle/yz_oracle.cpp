@@ -1068,6 +1068,84 @@ struct Equation {
         if (bc) velocity_bc(vx, vy);
     }
 
+    /* ---- the energy budget (EnergyBudget.h:52-166; the parameters of
+     * WellMixedModelBase.h:869-880): the free energy F of the working field,
+     * the potential phi, and the power of the face fluxes against phi
+     * (FluxPower, HeightEquationBase.h:275-291) ---- */
+    double en_W(double hh) const {   /* w(H) = -int_{H*}^{H} Pi */
+        const double dp = cs.disjoining ? cs.Dp : 0.0;
+        if (dp == 0 || !(hh > 0)) return 0;
+        const double hs = cs.hstar, n = cs.dis_n, m = cs.dis_m;
+        const double in = std::pow(hs, n) * (std::pow(hh, 1.0 - n) - std::pow(hs, 1.0 - n)) / (1.0 - n);
+        const double im = std::pow(hs, m) * (std::pow(hh, 1.0 - m) - std::pow(hs, 1.0 - m)) / (1.0 - m);
+        return -dp * (in - im);
+    }
+    double en_cell(double hh, double f) const {
+        const double bo = cs.vertical_gravity ? cs.Bo : 0.0;
+        return 0.5 * bo * f * f + en_W(hh);
+    }
+    double en_face(double fi, double fj, double ci, double cj, double dn) const {
+        if (!cs.capillary) return 0;
+        const double gr = (fj - fi) / dn;
+        return 0.25 * (sigma(ci) + sigma(cj)) * gr * gr;
+    }
+    double free_energy() const {
+        auto v = [&](int i) { return h[i] + s[i]; };
+        const bool px = g.bc[0] == YZ_BC_PERIODIC, py = cs.dim == 2 && g.bc[2] == YZ_BC_PERIODIC;
+        double F = 0;
+        for (int i = 0; i < g.n; ++i) {
+            if (ghost_row(i)) continue;
+            F += en_cell(h[i], v(i));
+            const int xp = nb(i, st.adj[1]), xm = nb(i, st.adj[0]);
+            F += en_face(v(i), v(xp), c[i], c[xp], g.dx);
+            if (cs.dim == 2) {
+                const int yp = nb(i, st.adj[3]);
+                F += en_face(v(i), v(yp), c[i], c[yp], g.dy);
+            }
+            if (!px && ghost_row(xm)) F += en_face(v(xm), v(i), c[xm], c[i], g.dx);
+            if (cs.dim == 2) {
+                const int ym = nb(i, st.adj[2]);
+                if (!py && ghost_row(ym)) F += en_face(v(ym), v(i), c[ym], c[i], g.dy);
+            }
+        }
+        return cs.dim == 2 ? F * g.dx * g.dy : F * g.dx;
+    }
+    /* PotentialAt (LubFields.cpp:300-311) at every real cell, 0 elsewhere */
+    void potential(vector<double> &phi) const {
+        auto v = [&](int i) { return h[i] + s[i]; };
+        phi.assign(g.n, 0.0);
+        const double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy;
+        const double dp = cs.disjoining ? cs.Dp : 0.0, bo = cs.vertical_gravity ? cs.Bo : 0.0;
+        for (int i = 0; i < g.n; ++i) {
+            if (ghost_row(i)) continue;
+            const double vo = v(i);
+            double lap = (v(nb(i, st.adj[1])) - 2.0 * vo + v(nb(i, st.adj[0]))) / dx2;
+            if (cs.dim == 2) lap = lap + (v(nb(i, st.adj[3])) - 2.0 * vo + v(nb(i, st.adj[2]))) / dy2;
+            double p = 0;
+            if (cs.capillary) p += sigma(c[i]) * lap;
+            if (dp != 0 && h[i] > 0) {
+                const double r = cs.hstar / h[i];
+                p += dp * (std::pow(r, cs.dis_n) - std::pow(r, cs.dis_m));
+            }
+            if (bo != 0) p -= bo * vo;
+            phi[i] = p;
+        }
+    }
+    /* FluxPower times the cell measure: the dissipation D */
+    double flux_power() const {
+        vector<double> phi, F[2], r;
+        potential(phi);
+        face_volume_fluxes(F, r);
+        double p = 0;
+        for (int i = 0; i < g.n; ++i) {
+            if (ghost_row(i)) continue;
+            double div = 0;
+            for (int a = 0; a < st.naxis; ++a) div += F[a][i] - F[a][nb(i, st.adj[2 * a])];
+            p += phi[i] * (-div - r[i]);
+        }
+        return p * (cs.dim == 2 ? g.dx * g.dy : g.dx);
+    }
+
     /* FaceVolumeFluxes (HeightEquationBase.h:852-894) of the working field */
     void face_volume_fluxes(vector<double> F[2], vector<double> &r) const {
         for (int a = 0; a < st.naxis; ++a) F[a].assign(g.n, 0.0);
@@ -2453,6 +2531,24 @@ void yzo_velocity(void *p, const double *h, const double *c, int bc, double *vx,
     eq.velocity(ux, uy, bc != 0);
     std::copy(ux.begin(), ux.end(), vx);
     if (vy) std::copy(uy.begin(), uy.end(), vy);
+    eq.h = hs; eq.c = cs;
+}
+/* the energy budget of the field (h, c): out3 = F, D, and phi into phi
+ * (padded, may be NULL) */
+void yzo_energy(void *p, const double *h, const double *c, double *out2, double *phi) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    auto &eq = st->eq;
+    const int n = eq.g.n;
+    const std::vector<double> hs = eq.h, cs = eq.c;
+    eq.h.assign(h, h + n); eq.c.assign(c, c + n);
+    eq.fill_ghosts(eq.h); eq.fill_ghosts(eq.c);
+    out2[0] = eq.free_energy();
+    out2[1] = eq.flux_power();
+    if (phi) {
+        std::vector<double> ph;
+        eq.potential(ph);
+        std::copy(ph.begin(), ph.end(), phi);
+    }
     eq.h = hs; eq.c = cs;
 }
 /* B's pieces (for the tests against the reference's tables): out2 =

@@ -45,8 +45,9 @@ def read_case(path):
         side = np.fromfile(f, dtype=np.float64, count=n)
         vx = np.fromfile(f, dtype=np.float64, count=n)
         vy = np.fromfile(f, dtype=np.float64, count=n)
+        en = np.fromfile(f, dtype=np.float64, count=n + 1)
     return dict(rp=rp, col=col, av=av, rpj=rpj, colj=colj, jv=jv, h=h, s=s, c=c, ax=ax, ff=ff, side=side,
-                vx=vx, vy=vy)
+                vx=vx, vy=vy, energy=en[:1], phi=en[1:])
 
 
 def read_table(path):

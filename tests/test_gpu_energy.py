@@ -1,0 +1,55 @@
+"""The energy budget on the device (yz_step_energy) through the C-ABI,
+against the oracle (whose F and phi match the reference's EnergyBudget.h
+to the bit and whose D satisfies the Crank-Nicolson energy identity:
+tests/test_oracle_energy.py): F and D within 1e-12 of the oracle's at the
+same film, 1-D and 2-D, Symmetry and Periodic, every term on (the power of
+the fluxes that are not gradients included); and along device steps of a
+closed film the identity holds to O(dt^2)."""
+import numpy as np
+import pytest
+
+from oracle import binding as ob
+from yuzen_amd.step import HipStep
+
+from .helpers import TERMS, smooth_case
+from .test_oracle_energy import _cosine_film
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("slip", [0.0, 0.05])
+def test_energy_matches_oracle(dim, periodic, slip):
+    c, h, s, cc = smooth_case(dim, TERMS, slip=slip, periodic=periodic)
+    print(f"energy dim={dim} periodic={periodic} slip={slip}", flush=True)
+    g = HipStep(c, h, s, cc, 1e-3)
+    F, D = g.energy()
+    hg = g.h()
+    g.close()
+    Fo, Do, _ = ob.OracleStep(c, h, s, cc, 1e-3).energy(hg, cc)
+    print(f"  F {F:.15e} / {Fo:.15e}, D {D:.15e} / {Do:.15e}")
+    assert abs(F - Fo) <= 1e-12 * abs(Fo)
+    assert abs(D - Do) <= 1e-12 * abs(Do)
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_energy_identity_along_device_steps(dim):
+    res = []
+    for dt in (2e-5, 1e-5):
+        c, h, s = _cosine_film(dim)
+        c.ew, c.nk_eps = 0, 1e-13
+        cc = np.full(h.size, 0.5)
+        g = HipStep(c, h, s, cc, dt)
+        F0, D0 = g.energy()
+        worst = 0.0
+        for _ in range(3):
+            assert g.advance()[0] == 0
+            F1, D1 = g.energy()
+            assert D1 > 0 and F1 < F0
+            worst = max(worst, abs((F1 - F0) / dt + 0.5 * (D0 + D1)) / D1)
+            F0, D0 = F1, D1
+        g.close()
+        res.append(worst)
+    print(dim, res)
+    assert res[1] < 1e-2 and res[0] / res[1] > 3.0

@@ -28,6 +28,7 @@
 #include "../../include/yuzen_gpu.h"
 #include "yz_conc.h"
 #include "yz_velocity.h"
+#include "yz_energy.h"
 #include "yz_rows.h"
 #include "yz_solver.h"
 #include "yz_mlp.h"
@@ -1220,6 +1221,38 @@ int yz_step_velocity(yz_step *S, double *ux, double *uy) {
     (void)hipFree(d);
     if (rc) set_err("device error in yz_step_velocity");
     return rc;
+}
+
+// the energy budget of the context's film (Output.Energy, EnergyBudget.h):
+// the free energy F and the power D of the face fluxes against the potential
+int yz_step_energy(yz_step *S, double *F, double *D) {
+    if (!S || !F || !D) { set_err("null argument"); return 2; }
+    const Geo &g = S->g;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    const int nblk = (g.N + EB - 1) / EB;
+    double *d = nullptr;
+    YZ_TRY(hipMalloc((void **)&d, (3 * np + 2 * (size_t)nblk + 2) * sizeof(double)));
+    double *Fx = d, *Fy = d + np, *r = d + 2 * np, *part = d + 3 * np, *out = part + 2 * nblk;
+    RowArgs a = S->ra;
+    a.h = S->hnew;
+    double host[2] = {0, 0};
+    int rc = 0;
+    do {
+        if (hipMemsetAsync(d, 0, 3 * np * sizeof(double), S->st) != hipSuccess) { rc = 2; break; }
+        launch_face_fluxes(a, Fx, Fy, r, S->st);
+        const double cm = g.dim == 2 ? g.dx * g.dy : g.dx;
+        if (g.dim == 2) hipLaunchKernelGGL(k_energy_cells<2>, dim3(nblk), dim3(EB), 0, S->st, a, Fx, Fy, r, part);
+        else hipLaunchKernelGGL(k_energy_cells<1>, dim3(nblk), dim3(EB), 0, S->st, a, Fx, Fy, r, part);
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(EB), 0, S->st, nblk, part, cm, out);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(host, out, 2 * sizeof(double), hipMemcpyDeviceToHost, S->st) != hipSuccess ||
+            hipStreamSynchronize(S->st) != hipSuccess) { rc = 2; break; }
+    } while (0);
+    (void)hipFree(d);
+    if (rc) { set_err("device error in yz_step_energy"); return rc; }
+    *F = host[0];
+    *D = host[1];
+    return 0;
 }
 
 int yz_step_write_dat(yz_step *S, const char *path, double L0, double H0, double U0, int dimensioned) {

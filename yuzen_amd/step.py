@@ -83,6 +83,14 @@ class HipStep:
             raise RuntimeError(capi.last_error())
         return ux, uy
 
+    def energy(self):
+        """(F, D): the free energy of the current film and the power of its
+        face fluxes against the potential"""
+        F, D = C.c_double(), C.c_double()
+        if self.lib.yz_step_energy(self.p, C.byref(F), C.byref(D)):
+            raise RuntimeError(capi.last_error())
+        return F.value, D.value
+
     def write_dat(self, path, L0=1.0, H0=1.0, U0=1.0, dimensioned=False):
         """the reference's ASCII field file of the current film"""
         if self.lib.yz_step_write_dat(self.p, os.fsencode(path), L0, H0, U0, 1 if dimensioned else 0):
