@@ -243,6 +243,17 @@ int yz_step_velocity(yz_step *st, double *ux, double *uy);
  * The reference's Energy.dat writes F, D and that balance. 0 / 2. */
 int yz_step_energy(yz_step *st, double *F, double *D);
 
+/* The volume balance of the context's film (History.dat's columns,
+ * WellMixedModelBase.h:1339-1360): out4[0] the film's volume and [1] its
+ * solute (sums of h and h c over the real cells times the cell measure),
+ * [2] the volume the last step evaporated (B's Crank-Nicolson evaporation of
+ * its last Newton pass, as capped, times the cell measure and that step's
+ * dt: EvaporationRate, HeightEquationBase.h:359-369) and [3] the volume the
+ * nozzle supplied in it (SuppliedVolume, :293-303); 0 before the first step.
+ * On a closed or periodic film without the 1-D sideways outflow, the volume
+ * changes over a step by [3] - [2], to the linear solve's tolerance. 0 / 2. */
+int yz_step_balances(yz_step *st, double *out4);
+
 /* The ASCII field file (.dat) of a well-mixed film, in the reference's
  * layout (OutputNeutral::Write of Field_WM1D_Out / Field_WM2D_Out,
  * Output.cpp:830-920, LubFields.cpp:204-290, 322-372, 602-900): a header

@@ -53,3 +53,32 @@ def test_energy_identity_along_device_steps(dim):
         res.append(worst)
     print(dim, res)
     assert res[1] < 1e-2 and res[0] / res[1] > 3.0
+
+
+@pytest.mark.parametrize("coef", [50.0, 1e5])
+def test_volume_balance_closes(coef):
+    """On a periodic film with evaporation (capped where coef = 1e5) and a
+    nozzle, every step changes the volume by what the nozzle supplied less
+    what evaporated, as yz_step_balances reports them, to the linear
+    solve's tolerance; the volume and solute it reports are the film's."""
+    from .test_gpu_evap_source import DT, _with_b
+    c, h, s, cc = smooth_case(2, [t for t in TERMS if t != "centrifugal"], slip=0.05, periodic=True)
+    _with_b(c, coef=coef)
+    g = HipStep(c, h, s, cc, DT)
+    v0 = g.balances()
+    assert v0[2] == 0 and v0[3] == 0
+    cm = (c.xmax - c.xmin) / c.nx * (c.ymax - c.ymin) / c.ny
+    nxp, nyp = c.nx + 4, c.ny + 4
+    for k in range(4):
+        assert g.advance()[0] == 0
+        v1 = g.balances()
+        hg = g.h().reshape(nyp, nxp)[2:-2, 2:-2]
+        cg = cc.reshape(nyp, nxp)[2:-2, 2:-2]
+        gap = (v1[0] - v0[0]) - (v1[3] - v1[2])
+        print(f"step {k}: dV {v1[0] - v0[0]:.6e} supplied {v1[3]:.6e} evaporated {v1[2]:.6e} gap {gap:.2e}")
+        assert v1[2] > 0 and v1[3] > 0
+        assert abs(gap) <= 1e-10 * v1[0]
+        assert abs(v1[0] - hg.sum() * cm) <= 1e-12 * v1[0]
+        assert abs(v1[1] - (hg * cg).sum() * cm) <= 1e-12 * v1[1]
+        v0 = v1
+    g.close()

@@ -197,6 +197,7 @@ def load():
     lib.yz_read_dat.argtypes = [C.c_char_p, C.POINTER(YzCase), d, pd, pd, pd]
     lib.yz_step_write_dat.argtypes = [C.c_void_p, C.c_char_p, d, d, d, C.c_int]
     lib.yz_step_energy.argtypes = [C.c_void_p, pd, pd]
+    lib.yz_step_balances.argtypes = [C.c_void_p, pd]
     lib.yz_step_set_h.argtypes = [C.c_void_p, pd]
     lib.yz_step_clock.argtypes = [C.c_void_p, pd, pd, pd]
     lib.yz_step_timing.argtypes = [C.c_void_p, pd]

@@ -108,4 +108,53 @@ __global__ void __launch_bounds__(EB) k_energy_final(int nblocks, const double *
     }
 }
 
+// the step's volume balance: per real cell h, h c, and (when the case has
+// B) the evaporation and the supply of the last pass, summed per block in a
+// fixed tree
+__global__ void __launch_bounds__(EB) k_balance_cells(Geo g, const double *h, const double *c, const double *evp,
+                                                      const double *src, double *part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < g.N) {
+        const int p = (g.dim == 2 ? i / g.nx + 2 : 0) * g.nxp + i % g.nx + 2;
+        v[0] = h[p];
+        v[1] = h[p] * c[p];
+        if (evp) v[2] = evp[i];
+        if (src) v[3] = src[i];
+    }
+    __shared__ double sv[4][EB];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sv[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = EB / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sv[k][threadIdx.x] += sv[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[4 * blockIdx.x + k] = sv[k][0];
+}
+
+__global__ void __launch_bounds__(EB) k_balance_final(int nblocks, const double *part, double *out) {
+    __shared__ double sv[4][EB];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += EB)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] += part[4 * b + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sv[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = EB / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sv[k][threadIdx.x] += sv[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = sv[k][0];
+}
+
 }  // namespace yz

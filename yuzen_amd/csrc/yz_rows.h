@@ -92,6 +92,7 @@ struct RowArgs {
     BTerm bt;
     const double *evold;       // N: EvapAt(h_old, c) of the step's start
     const double *src;         // N: the nozzle's supply at the middle of the step
+    double *evp_out;           // optional, N: the evaporation part of B of the pass
 };
 
 __device__ inline double visc_of(const RowArgs &a, double cc) { return exp(a.visc_exp * (cc - a.visc_c0)); }
@@ -448,6 +449,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
             if (A.evold) {
                 const double evp = (1.0 - A.theta) * A.evold[i] + A.theta * evap_rate(A.bt, H[C0], Cc[C0], A.dt);
                 R = R - A.dt * (evp - A.src[i]);
+                if (A.evp_out) A.evp_out[i] = evp;
             }
             rabs = fabs(R);
             if (A.rhs_raw) A.rhs_raw[i] = R;

@@ -215,6 +215,7 @@ struct yz_step {
     double *h = nullptr, *hold = nullptr, *hold2 = nullptr, *hnew = nullptr, *s = nullptr, *c = nullptr;
     double *axold = nullptr, *b = nullptr, *corr = nullptr, *scratch = nullptr;
     double *evold = nullptr, *srcv = nullptr;   // B's parts fixed over a step (N each)
+    double *evpv = nullptr;                     // B's evaporation of the last pass (N)
     unsigned long long *dmax = nullptr;   // [0] rhs, [1] corr, [2] move, [3] trunc
     int *dflag = nullptr;
     unsigned long long *hmax = nullptr;   // pinned mirror
@@ -909,8 +910,10 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
     double **vecs[] = {&S->axold, &S->b, &S->corr, &S->scratch};
     for (double **p : vecs) if (hipMalloc((void **)p, g.N * sizeof(double)) != hipSuccess) return fail("hipMalloc");
     if (has_b(*c))
-        for (double **p : {&S->evold, &S->srcv})
+        for (double **p : {&S->evold, &S->srcv, &S->evpv}) {
             if (hipMalloc((void **)p, g.N * sizeof(double)) != hipSuccess) return fail("hipMalloc");
+            if (hipMemset(*p, 0, g.N * sizeof(double)) != hipSuccess) return fail("memset");
+        }
     if (hipMalloc((void **)&S->dmax, 4 * sizeof(unsigned long long)) != hipSuccess) return fail("hipMalloc");
     if (hipMalloc((void **)&S->dflag, sizeof(int)) != hipSuccess) return fail("hipMalloc");
     if (hipHostMalloc((void **)&S->hmax, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) return fail("hipHostMalloc");
@@ -1021,7 +1024,7 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
     a.bt.evap = c->evaporation; a.bt.efunc = c->evap_func; a.bt.Ev = c->Ev; a.bt.coef = c->evap_coef;
     a.bt.power = c->evap_power; a.bt.concr = c->evap_concr; a.bt.mres = c->molar_resin;
     a.bt.msol = c->molar_solvent; a.bt.prec = c->precursor;
-    a.evold = S->evold; a.src = S->srcv;
+    a.evold = S->evold; a.src = S->srcv; a.evp_out = S->evpv;
     S->dt_now = dt0; S->dt_old = 0;
     return S;
 }
@@ -1046,7 +1049,7 @@ void yz_step_destroy(yz_step *S) {
         if (p) (void)hipFree(p);
     if (S->Yh) (void)hipFree(S->Yh);
     for (double *p : {S->h, S->hold, S->hold2, S->hnew, S->s, S->c, S->axold, S->b, S->corr, S->scratch,
-                      S->evold, S->srcv})
+                      S->evold, S->srcv, S->evpv})
         if (p) (void)hipFree(p);
     if (S->dmax) (void)hipFree(S->dmax);
     if (S->dflag) (void)hipFree(S->dflag);
@@ -1252,6 +1255,36 @@ int yz_step_energy(yz_step *S, double *F, double *D) {
     if (rc) { set_err("device error in yz_step_energy"); return rc; }
     *F = host[0];
     *D = host[1];
+    return 0;
+}
+
+// the volume balance of the context's film (History.dat's volume columns,
+// WellMixedModelBase.h:1339-1360): the film's volume and solute, and what
+// the last step evaporated (EvaporationRate, HeightEquationBase.h:359-369,
+// times its dt) and what the nozzle supplied (SuppliedVolume :293-303)
+int yz_step_balances(yz_step *S, double *out4) {
+    if (!S || !out4) { set_err("null argument"); return 2; }
+    const Geo &g = S->g;
+    const int nblk = (g.N + EB - 1) / EB;
+    double *d = nullptr;
+    YZ_TRY(hipMalloc((void **)&d, (4 * (size_t)nblk + 4) * sizeof(double)));
+    double host[4] = {0, 0, 0, 0};
+    int rc = 0;
+    const bool ran = S->steps > 0;
+    hipLaunchKernelGGL(k_balance_cells, dim3(nblk), dim3(EB), 0, S->st, g, S->hnew, S->c,
+                       ran ? S->evpv : nullptr, (ran && S->cs.source) ? S->srcv : nullptr, d);
+    hipLaunchKernelGGL(k_balance_final, dim3(1), dim3(EB), 0, S->st, nblk, d, d + 4 * nblk);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(host, d + 4 * nblk, 4 * sizeof(double), hipMemcpyDeviceToHost, S->st) != hipSuccess ||
+        hipStreamSynchronize(S->st) != hipSuccess) rc = 2;
+    (void)hipFree(d);
+    if (rc) { set_err("device error in yz_step_balances"); return rc; }
+    const double cm = g.dim == 2 ? g.dx * g.dy : g.dx;
+    const double dt = ran ? S->dt_old : 0.0;
+    out4[0] = host[0] * cm;
+    out4[1] = host[1] * cm;
+    out4[2] = S->cs.evaporation ? host[2] * cm * dt : 0.0;
+    out4[3] = host[3] * cm * dt;
     return 0;
 }
 

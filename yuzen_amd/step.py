@@ -91,6 +91,15 @@ class HipStep:
             raise RuntimeError(capi.last_error())
         return F.value, D.value
 
+    def balances(self):
+        """(volume, solute, evaporated, supplied): the film's volume and
+        solute, and the volume the last step evaporated and the nozzle
+        supplied"""
+        out = np.zeros(4)
+        if self.lib.yz_step_balances(self.p, _pd(out)):
+            raise RuntimeError(capi.last_error())
+        return tuple(out)
+
     def write_dat(self, path, L0=1.0, H0=1.0, U0=1.0, dimensioned=False):
         """the reference's ASCII field file of the current film"""
         if self.lib.yz_step_write_dat(self.p, os.fsencode(path), L0, H0, U0, 1 if dimensioned else 0):
