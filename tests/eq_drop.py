@@ -1,0 +1,51 @@
+"""The reference's equilibrium-droplet verification (cases/eq, its
+docs/validation.md:316-327): the analytic equilibrium droplet (theta_e = 10
+deg, H* = 0.02, Hmax = 1, on +-300 um) read from the reference's own ASCII
+initial conditions at Nx = 1000, 2000, 4000; the capillary and disjoining
+velocities cancel up to the truncation error, and the reference's solver
+reports the largest dimensionless Ux of its field00000.dat as 18.94, 5.074
+and 1.294."""
+import gzip
+import math
+import os
+
+import numpy as np
+
+from yuzen_amd import capi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PUBLISHED = {1000: 18.94, 2000: 5.074, 4000: 1.294}
+
+# eq_n*.toml: sigma 30e-3 N/m, rho 1000 kg/m^3, h0 10 um, the capillary
+# length as L0 (NonDimensionalNumbers.cpp:45-52, 82-94)
+L0 = math.sqrt(0.03 / (1000.0 * 9.80665))
+H0 = 10e-6
+AS = (H0 / L0) ** -2
+HSTAR = 0.02
+DP = AS * (3.0 - 1.0) * (2.0 - 1.0) / (2.0 * HSTAR * (3.0 - 2.0)) * math.radians(10.0) ** 2
+
+
+def case(n):
+    c = capi.case_defaults()
+    c.dim, c.nx, c.ny = 1, n, 1
+    c.xmin, c.xmax, c.ymin, c.ymax = -0.3e-3 / L0, 0.3e-3 / L0, -1.0, 1.0
+    for k in range(4):
+        c.bc[k] = capi.BC_SYMMETRY
+    c.capillary, c.disjoining = 1, 1
+    c.vertical_gravity = c.inplane_gravity = c.centrifugal = c.marangoni = 0
+    c.Dp, c.hstar, c.dis_n, c.dis_m = DP, HSTAR, 3.0, 2.0
+    c.visc_c0, c.visc_exp, c.sig_c0, c.slip = 0.5, 10.0, 0.5, 0.0
+    return c
+
+
+def read(n, tmp):
+    """the droplet through yz_read_dat: HH in um, so H_coef = 1e-6 / h0"""
+    path = os.path.join(str(tmp), f"eq_n{n}.dat")
+    with gzip.open(os.path.join(HERE, "golden", "eq", f"eq_n{n}.dat.gz"), "rb") as f, open(path, "wb") as g:
+        g.write(f.read())
+    z = np.zeros(n + 4)
+    return capi.read_dat(path, case(n), z, z, z, H_coef=1e-6 / H0)
+
+
+def max_ux(path):
+    return np.abs(np.loadtxt(path, skiprows=1)[:, 2]).max()

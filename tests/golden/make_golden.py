@@ -14,9 +14,15 @@ drives them) into a scratch directory and packs what it wrote:
   manifest.json   the dimensionless parameters of every case
   ref_state_wm1d.state  a restart state file (field*.state) written by the
                   reference's C_RestartState for a 1-D film of 8 cells
+  ref_neutral.npz, neutral/*.dat  ASCII field files written by the
+                  reference's OutputNeutral, their inputs, and what its ASCII
+                  restart read back from them
+  eq/eq_n*.dat.gz the reference's analytic equilibrium droplets
+                  (cases/eq/eq_n1000, 2000, 4000.dat), copied as data
 
     python tests/golden/make_golden.py
 """
+import gzip
 import json
 import os
 import subprocess
@@ -28,6 +34,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 EXE = os.path.join(ROOT, "oracle", "_ref", "ref_golden")
+REF = os.environ.get("YUZEN_REFERENCE", "/root/reference")
 
 
 def read_case(path):
@@ -97,6 +104,14 @@ def main():
                 with open(src, "rb") as f, open(os.path.join(HERE, "neutral", f"{d}_{kind}.dat"), "wb") as g:
                     g.write(f.read())
         np.savez_compressed(os.path.join(HERE, "ref_neutral.npz"), **neutral)
+        # the reference's analytic equilibrium droplets (cases/eq, ASCII
+        # initial conditions at dx = 0.6, 0.3, 0.15 um), as data, gzipped
+        os.makedirs(os.path.join(HERE, "eq"), exist_ok=True)
+        for n in (1000, 2000, 4000):
+            with open(os.path.join(REF, "cases", "eq", f"eq_n{n}.dat"), "rb") as f:
+                raw = f.read()
+            with gzip.GzipFile(os.path.join(HERE, "eq", f"eq_n{n}.dat.gz"), "wb", mtime=0) as g:
+                g.write(raw)
         with open(os.path.join(tmp, "ref_state_wm1d.state"), "rb") as f:
             state = f.read()
         with open(os.path.join(HERE, "ref_state_wm1d.state"), "wb") as f:

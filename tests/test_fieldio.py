@@ -10,6 +10,7 @@ tests/golden/ref_neutral.npz, written by oracle/ref_golden.cpp):
 * a file read back gives the real cells to the bit; a file whose point
   count or rows do not fit is refused.
 """
+import math
 import os
 
 import numpy as np
@@ -114,3 +115,26 @@ def test_bad_files_are_refused(tmp_path):
     nophi.write_text("\n".join([lines[0]] + [l + "\t1.5" for l in lines[1:]]) + "\n")
     with pytest.raises(RuntimeError):
         capi.read_dat(str(nophi), _case("1d"), z, z, z)
+
+
+def test_equilibrium_droplet_residual_velocity(tmp_path):
+    """The reference's equilibrium droplets read with yz_read_dat, their
+    velocity (the oracle's, which is the reference's CalVelocity to the
+    bit) written with yz_write_dat: the largest Ux is the one the
+    reference's solver publishes, to its 4 digits, at 2nd order in dx."""
+    from oracle import binding as ob
+    from .eq_drop import PUBLISHED, case, max_ux, read
+    got = {}
+    for n, want in PUBLISHED.items():
+        h, s, c = read(n, tmp_path)
+        assert abs(h.max() - 1.0) < 1e-5
+        o = ob.OracleStep(case(n), h, s, c, 1e-6)
+        hg = o.h()
+        vx, vy = o.velocity(hg, c)
+        out = str(tmp_path / f"out{n}.dat")
+        capi.write_dat(out, case(n), hg, s, c, vx, vy)
+        got[n] = max_ux(out)
+        print(n, got[n], want)
+        assert abs(got[n] - want) <= 5e-4 * want
+    assert 1.85 < math.log2(got[1000] / got[2000]) < 2.05
+    assert 1.9 < math.log2(got[2000] / got[4000]) < 2.05

@@ -97,3 +97,28 @@ def test_step_field_file_and_restart(dim, tmp_path):
     assert np.array_equal(a.h(), b.h())
     for x in (g, a, b):
         x.close()
+
+
+def test_equilibrium_droplets_on_the_device(tmp_path):
+    """The reference's equilibrium droplets (cases/eq) read with
+    yz_read_dat into step contexts; the field file the context writes
+    (yz_step_write_dat: the device's face velocities at the points) has the
+    largest Ux the reference's solver publishes for its field00000.dat
+    (18.94, 5.074, 1.294), and the oracle's to 1e-10 (the capillary and
+    disjoining velocities, each some 2e3 at the contact line, cancel to
+    this residual, so the rounding of the terms shows at 1e-12 of it)."""
+    from .eq_drop import PUBLISHED, case, max_ux, read
+    for n, want in PUBLISHED.items():
+        print(f"equilibrium droplet Nx={n}", flush=True)
+        h, s, c = read(n, tmp_path)
+        g = HipStep(case(n), h, s, c, 1e-6)
+        out = str(tmp_path / f"dev{n}.dat")
+        g.write_dat(out)
+        hg = g.h()
+        g.close()
+        got = max_ux(out)
+        vx, _ = ob.OracleStep(case(n), h, s, c, 1e-6).velocity(hg, c)
+        ref = np.abs(0.5 * (vx[3:-2] + vx[2:-3])).max()
+        print(f"  max |Ux| {got:.6f} (published {want}, oracle {ref:.6f})")
+        assert abs(got - want) <= 5e-4 * want
+        assert abs(got - ref) <= 1e-10 * ref
