@@ -15,11 +15,15 @@ from yuzen_amd import capi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PUBLISHED = {1000: 18.94, 2000: 5.074, 4000: 1.294}
+# the same droplet at Nx = 2000 advanced for 0.01 s (eq_run.toml; its
+# docs/validation.md:353-362): t [s] -> (max |dh| [um], largest Ux)
+EVOLUTION = {2.5e-3: (4.23e-5, 0.709), 5.0e-3: (5.36e-5, 0.418), 7.5e-3: (6.02e-5, 0.296), 1.0e-2: (6.48e-5, 0.229)}
 
 # eq_n*.toml: sigma 30e-3 N/m, rho 1000 kg/m^3, h0 10 um, the capillary
 # length as L0 (NonDimensionalNumbers.cpp:45-52, 82-94)
 L0 = math.sqrt(0.03 / (1000.0 * 9.80665))
 H0 = 10e-6
+T0 = 0.1 * L0 ** 4 / (0.03 * H0 ** 3)    # mu0 L0^4 / (sigma0 h0^3)
 AS = (H0 / L0) ** -2
 HSTAR = 0.02
 DP = AS * (3.0 - 1.0) * (2.0 - 1.0) / (2.0 * HSTAR * (3.0 - 2.0)) * math.radians(10.0) ** 2
@@ -36,6 +40,18 @@ def case(n):
     c.Dp, c.hstar, c.dis_n, c.dis_m = DP, HSTAR, 3.0, 2.0
     c.visc_c0, c.visc_exp, c.sig_c0, c.slip = 0.5, 10.0, 0.5, 0.0
     return c
+
+
+def run_case(n):
+    """eq_run.toml's numerics: step control (Criterion 1e-2, KeepSteps 50,
+    ChangeFactor 0.75, MinDt 1e-15 s) from InitialDt 1e-6 s, Newton to
+    1e-12, linear solves to 1e-13"""
+    c = case(n)
+    c.step_control = 1
+    c.tsc_change, c.tsc_criterion, c.tsc_keep = 0.75, 1e-2, 50
+    c.dt_min, c.dt_max = 1e-15 / T0, 1e10
+    c.nk_max, c.nk_eps, c.lin_eps = 50, 1e-12, 1e-13
+    return c, 1e-6 / T0
 
 
 def read(n, tmp):
