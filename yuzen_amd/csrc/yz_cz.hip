@@ -56,7 +56,7 @@ using namespace yz;
 namespace {
 
 constexpr int TBK = 256;
-constexpr int RB = 512;          // reduction blocks
+constexpr int RB = 4096;         // reduction blocks
 inline int nblk(long n) { return (int)std::min<long>((n + TBK - 1) / TBK, 65535L * 16); }
 
 #define CZ_CHECK(x)                                                            \

@@ -711,16 +711,30 @@ __global__ void __launch_bounds__(BLK) k_post(StencilOp op, const T *__restrict_
     y[i] = x[i] + m[i] * (b[i] - ax);
 }
 
-// x = A^-1 b with the inverse stored transposed (coalesced over i)
+// x = A^-1 b with the inverse stored transposed (coalesced over i): 64 rows
+// per block, the block's DMV_W waves each take every DMV_W-th column, and
+// their partial sums are added in wave order (a fixed order, so the result
+// does not depend on scheduling). The coarsest level has a few hundred rows:
+// one thread per row walking all columns left most of the chip idle.
+static constexpr int DMV_W = 16;
 template <typename T>
-__global__ void k_dense_mv(int n, const double *__restrict__ AinvT, const T *__restrict__ b,
-                           T *__restrict__ x) {
-    const int i = blockIdx.x * BLK + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(64 * DMV_W) k_dense_mv(int n, const double *__restrict__ AinvT,
+                                                         const T *__restrict__ b, T *__restrict__ x) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
     double s = 0;
-    for (int j = 0; j < n; ++j) s += AinvT[(size_t)j * n + i] * b[j];
-    x[i] = s;
+    if (i < n)
+        for (int j = wave; j < n; j += DMV_W) s += AinvT[(size_t)j * n + i] * b[j];
+    __shared__ double part[DMV_W][64];
+    part[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && i < n) {
+        double t = 0;
+        for (int w = 0; w < DMV_W; ++w) t += part[w][lane];
+        x[i] = t;
+    }
 }
+static inline int dmv_blocks(int n) { return (n + 63) / 64; }
 
 // residual r = b - A x, and the explicit-stencil smoothing step
 // out = xbase + M t (the learned approximate inverse as the finest smoother)
@@ -921,7 +935,7 @@ static void vcycle(SolverT<T> &S, const T *in, T *out, const T *smoother) {
     hipStream_t st = S.stream;
     const int nl = (int)g.lv.size();
     if (nl == 1) {
-        hipLaunchKernelGGL((k_dense_mv<T>), dim3(nblk(g.nco)), dim3(BLK), 0, st, g.nco, g.coarse_inv, in, out);
+        hipLaunchKernelGGL((k_dense_mv<T>), dim3(dmv_blocks(g.nco)), dim3(64 * DMV_W), 0, st, g.nco, g.coarse_inv, in, out);
         return;
     }
     // downward
@@ -951,7 +965,7 @@ static void vcycle(SolverT<T> &S, const T *in, T *out, const T *smoother) {
         b = C.b;
     }
     AmgLevel<T> &Lc = g.lv[nl - 1];
-    hipLaunchKernelGGL((k_dense_mv<T>), dim3(nblk(g.nco)), dim3(BLK), 0, st, g.nco, g.coarse_inv, Lc.b, Lc.x);
+    hipLaunchKernelGGL((k_dense_mv<T>), dim3(dmv_blocks(g.nco)), dim3(64 * DMV_W), 0, st, g.nco, g.coarse_inv, Lc.b, Lc.x);
     // upward
     for (int l = nl - 2; l >= 0; --l) {
         AmgLevel<T> &L = g.lv[l];
