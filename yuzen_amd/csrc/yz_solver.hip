@@ -614,7 +614,7 @@ template <int NS, typename T>
 __global__ void __launch_bounds__(BLK) k_pre(StencilOp op, const T *__restrict__ val,
                                              const T *__restrict__ m, const T *__restrict__ b,
                                              T *__restrict__ x, T *__restrict__ r) {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= op.N) return;
     const int ix = i % op.nx, iy = i / op.nx;
     double s = 0;
@@ -631,16 +631,19 @@ __global__ void __launch_bounds__(BLK) k_pre(StencilOp op, const T *__restrict__
 // 16 x 16 coarse cells (256 in 1-D) stages the fine residual it needs
 // (a 56 x 56 tile; 776 cells in 1-D) in LDS, coalesced; each thread then
 // sums its 81 (9) weights, read coalesced, against the tile
-template <int DIM, typename T>
+// TC x TC coarse cells per block in 2-D: 16 x 16 on the large levels, 8 x 8
+// (one wave) on the small ones, so that a coarse level of a few thousand
+// cells still spreads over a hundred CUs
+template <int DIM, typename T, int TC = 16>
 __global__ void __launch_bounds__(256) k_restrict(StencilOp fop, const T *__restrict__ R,
                                                   const T *__restrict__ r, StencilOp cop,
                                                   T *__restrict__ bc) {
-    constexpr int TX = DIM == 2 ? 16 : 256, TY = DIM == 2 ? 16 : 1;
+    constexpr int TX = DIM == 2 ? TC : 256, TY = DIM == 2 ? TC : 1;
     constexpr int FX = 3 * TX + 8, FY = DIM == 2 ? 3 * TY + 8 : 1;
     __shared__ double tile[FY * FX];
     const int X0 = blockIdx.x * TX, Y0 = DIM == 2 ? blockIdx.y * TY : 0;
     const int fx0 = 3 * X0 - 3, fy0 = DIM == 2 ? 3 * Y0 - 3 : 0;
-    for (int t = threadIdx.x; t < FX * FY; t += 256) {
+    for (int t = threadIdx.x; t < FX * FY; t += TX * TY) {
         const int lx = t % FX, ly = t / FX;
         int gx = fx0 + lx, gy = fy0 + ly;
         bool ok = true;
@@ -675,7 +678,7 @@ __global__ void __launch_bounds__(256) k_restrict(StencilOp fop, const T *__rest
 template <typename T>
 __global__ void k_prolong(StencilOp fop, const T *__restrict__ P, StencilOp cop,
                           const T *__restrict__ xc, T *__restrict__ x) {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= fop.N) return;
     const bool two = fop.dim == 2;
     const int ix = i % fop.nx, iy = i / fop.nx;
@@ -704,7 +707,7 @@ template <int NS, typename T>
 __global__ void __launch_bounds__(BLK) k_post(StencilOp op, const T *__restrict__ val,
                                               const T *__restrict__ m, const T *__restrict__ b,
                                               const T *__restrict__ x, T *__restrict__ y) {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= op.N) return;
     const int ix = i % op.nx, iy = i / op.nx;
     const double ax = row_dot<NS>(op, val, x, i, ix, iy);
@@ -735,6 +738,11 @@ __global__ void __launch_bounds__(64 * DMV_W) k_dense_mv(int n, const double *__
     }
 }
 static inline int dmv_blocks(int n) { return (n + 63) / 64; }
+// the block size of a level's smoothing and prolongation: one wave on a
+// level of fewer than 128 x 128 cells, so that its few thousand cells
+// spread over more CUs than 256-thread blocks would reach
+static inline int lvl_blk(int n) { return n < 128 * 128 ? 64 : BLK; }
+static inline int lvl_grid(int n) { return (n + lvl_blk(n) - 1) / lvl_blk(n); }
 
 // residual r = b - A x, and the explicit-stencil smoothing step
 // out = xbase + M t (the learned approximate inverse as the finest smoother)
@@ -952,11 +960,14 @@ static void vcycle(SolverT<T> &S, const T *in, T *out, const T *smoother) {
         } else {
             dispatch_ns(L.op.ns, [&](auto c) {
                 constexpr int NS = decltype(c)::value;
-                hipLaunchKernelGGL((k_pre<NS, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv, b, L.x,
+                hipLaunchKernelGGL((k_pre<NS, T>), dim3(lvl_grid(L.op.N)), dim3(lvl_blk(L.op.N)), 0, st, L.op, L.val, L.minv, b, L.x,
                                    L.r);
             });
         }
-        if (L.op.dim == 2)
+        if (L.op.dim == 2 && C.op.N < 128 * 128)
+            hipLaunchKernelGGL((k_restrict<2, T, 8>), dim3((C.op.nx + 7) / 8, (C.op.ny + 7) / 8), dim3(64), 0, st,
+                               L.op, L.R, L.r, C.op, C.b);
+        else if (L.op.dim == 2)
             hipLaunchKernelGGL((k_restrict<2, T>), dim3((C.op.nx + 15) / 16, (C.op.ny + 15) / 16), dim3(256), 0, st,
                                L.op, L.R, L.r, C.op, C.b);
         else
@@ -971,7 +982,7 @@ static void vcycle(SolverT<T> &S, const T *in, T *out, const T *smoother) {
         AmgLevel<T> &L = g.lv[l];
         AmgLevel<T> &C = g.lv[l + 1];
         const T *bl = (l == 0) ? in : L.b;
-        hipLaunchKernelGGL((k_prolong<T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.P, C.op, C.x, L.x);
+        hipLaunchKernelGGL((k_prolong<T>), dim3(lvl_grid(L.op.N)), dim3(lvl_blk(L.op.N)), 0, st, L.op, L.P, C.op, C.x, L.x);
         T *dst = (l == 0) ? out : L.r;   // r is free here
         if (l == 0 && smoother) {
             dispatch_ns(L.op.ns, [&](auto c) {
@@ -983,7 +994,7 @@ static void vcycle(SolverT<T> &S, const T *in, T *out, const T *smoother) {
         } else {
             dispatch_ns(L.op.ns, [&](auto c) {
                 constexpr int NS = decltype(c)::value;
-                hipLaunchKernelGGL((k_post<NS, T>), dim3(nblk(L.op.N)), dim3(BLK), 0, st, L.op, L.val, L.minv, bl,
+                hipLaunchKernelGGL((k_post<NS, T>), dim3(lvl_grid(L.op.N)), dim3(lvl_blk(L.op.N)), 0, st, L.op, L.val, L.minv, bl,
                                    L.x, dst);
             });
         }
