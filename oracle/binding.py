@@ -26,7 +26,14 @@ def load():
         return _lib
     if not os.path.exists(LIB):
         build()
-    lib = C.CDLL(LIB)
+    _lib = load_from(LIB)
+    return _lib
+
+
+def load_from(path):
+    """the oracle's C API from a library at `path` (the tests load copies
+    built with a planted mistake this way)"""
+    lib = C.CDLL(path)
     P = C.POINTER
     pd, pi_ = P(C.c_double), P(C.c_int)
     lib.yzo_create.restype = C.c_void_p
@@ -66,7 +73,11 @@ def load():
     lib.yzo_cz_velocity.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_cz_redistribute.restype = C.c_double
     lib.yzo_cz_redistribute.argtypes = [C.c_void_p, pd, pd, pd, pd]
-    _lib = lib
+    lib.yzo_case_defaults.argtypes = [P(YzCase)]
+    lib.yzo_script_create.restype = C.c_void_p
+    lib.yzo_script_create.argtypes = [P(YzCase), C.c_int, pd, C.c_double, pd, C.c_int, pd, C.c_int,
+                                      pd, C.c_int, pi_, C.c_int]
+    lib.yzo_script_info.argtypes = [C.c_void_p, pi_, P(C.c_longlong), pd, C.c_int]
     return lib
 
 
@@ -283,3 +294,62 @@ class OracleCZ:
         mv, ad = C.c_double(), C.c_double()
         ex = self.lib.yzo_cz_redistribute(self.p, _pd(h), _pd(c), C.byref(mv), C.byref(ad))
         return c, ex, mv.value, ad.value
+
+
+def case_defaults(lib=None):
+    """the oracle's default case (its Newton and step-control numerics are
+    the reference's defaults)"""
+    c = YzCase()
+    (lib if lib is not None else load()).yzo_case_defaults(C.byref(c))
+    return c
+
+
+class ScriptedNewton:
+    """The oracle's Newton loop and step control over n rows, with each
+    pass's correction, coupled-side correction, right-hand-side norm and
+    solve return code taken from scripts instead of an assembled system
+    (the reference's ScriptedNK, yuzen/tests/test_newton_convergence.cpp:
+    38-84). Entry k of a script is used by the k-th solve of the run
+    (counted over every attempt and step); the last entry repeats."""
+
+    def __init__(self, case, phi0, dt0, dH=(), dC=(), rn=(), rc=(), lib=None):
+        self.lib = lib if lib is not None else load()
+        self.case = case.copy()
+        phi0 = np.ascontiguousarray(phi0, dtype=np.float64)
+        self.n = phi0.size
+        self._keep = [np.ascontiguousarray(v, dtype=np.float64) for v in (dH, dC, rn)]
+        rc = np.ascontiguousarray(rc, dtype=np.int32)
+        self._keep.append(rc)
+        a, b, r = self._keep[:3]
+        self.p = self.lib.yzo_script_create(C.byref(self.case), self.n, _pd(phi0), dt0,
+                                            _pd(a), a.size, _pd(b), b.size, _pd(r), r.size,
+                                            _pi(rc), rc.size)
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.lib.yzo_destroy(self.p)
+            self.p = None
+
+    def advance(self):
+        """one time step: (rc, Newton passes of the accepted attempt)"""
+        nk, lin, r = C.c_int(), C.c_int(), C.c_double()
+        rc = self.lib.yzo_advance(self.p, C.byref(nk), C.byref(lin), C.byref(r))
+        return rc, nk.value
+
+    def phi(self):
+        out = np.empty(self.n)
+        self.lib.yzo_get_h(self.p, _pd(out))
+        return out
+
+    def clock(self):
+        """(time, dt of the next step, dt of the last step)"""
+        t, a, b = C.c_double(), C.c_double(), C.c_double()
+        self.lib.yzo_clock(self.p, C.byref(t), C.byref(a), C.byref(b))
+        return t.value, a.value, b.value
+
+    def info(self):
+        """(solves so far, stagnation cuts, the tolerance of every solve)"""
+        calls, cuts = C.c_int(), C.c_longlong()
+        eta = np.zeros(4096)
+        m = self.lib.yzo_script_info(self.p, C.byref(calls), C.byref(cuts), _pd(eta), eta.size)
+        return calls.value, cuts.value, eta[:min(m, eta.size)].copy()

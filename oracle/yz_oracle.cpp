@@ -1592,6 +1592,29 @@ struct Stepper {
     int nk_pass = 0;
     long long lin_total = 0;
     bool first_step = true;
+    long long stag_cuts = 0;           /* _stagnationCuts (NewtonKantorovich.cpp:357) */
+
+    /* Scripted Newton control (test hook). In place of the assembly and
+     * the linear solve, each pass takes its correction (every row alike),
+     * the coupled side's correction, the norm of its right-hand side (for
+     * the Eisenstat-Walker tolerance) and the solve's return code from
+     * scripts, the way the reference's own test drives its Newton loop
+     * (yuzen/tests/test_newton_convergence.cpp:38-84: ScriptedNK, Pick
+     * by the count of solves so far, the last entry repeated). Everything
+     * else -- the exits, the damping, the extrapolation and its retry, the
+     * step control -- runs the code below unchanged. */
+    struct Script {
+        bool on = false;
+        vector<double> dH, dC, rn;
+        vector<int> rc;
+        int call = 0;
+        vector<double> eta;            /* the tolerance each solve was given */
+        template <class T> T pick(const vector<T> &v) const {
+            if (v.empty()) return T(0);
+            const size_t k = (size_t)call;
+            return k < v.size() ? v[k] : v.back();
+        }
+    } script;
 
     /* the well-mixed concentration (conc_model 1) */
     bool conc_on = false;
@@ -1740,7 +1763,21 @@ struct Stepper {
         if (conc_on) conc_init();
         phi_new = eq.h; phi_old = eq.h; phi_old2 = eq.h;
         Ax_old.assign(n, 0); Ax_guess.assign(n, 0); Rhs.assign(n, 0); corr.assign(n, 0);
-        dt_now = dt0; dt_old = 0;
+        /* C_TimeSteps::Build (TimeSteps.cpp:38-61) starts DT().old at the
+         * initial step, so the first step's truncation criterion is taken,
+         * with phi^{n-1} = phi^n */
+        dt_now = dt0; dt_old = dt0;
+    }
+    /* a scripted stepper over n rows, none of them a boundary row */
+    void init_script(const yz_case &c, int n, const double *phi0, double dt0) {
+        eq.cs = c;
+        eq.g.n = n;
+        eq.A.n = n;
+        eq.A.bc.assign(n, 0);
+        phi_new.assign(phi0, phi0 + n); phi_old = phi_new; phi_old2 = phi_new;
+        Ax_old.assign(n, 0); Ax_guess.assign(n, 0); Rhs.assign(n, 0); corr.assign(n, 0);
+        script.on = true;
+        dt_now = dt0; dt_old = dt0;
     }
     static double absmax(const vector<double> &v) {
         double m = 0;
@@ -2205,6 +2242,7 @@ struct Stepper {
 
     /* Solve (NewtonKantorovich.cpp:74-154) */
     int solve(int &iter, double &resid) {
+        if (script.on) return solve_scripted(iter, resid);
         eq.h = phi_guess;
         eq.set_matrix_A();
         if (nk_pass % eq.cs.jac_interval == 0) { eq.set_matrix_J(); make_IA(); }
@@ -2213,21 +2251,7 @@ struct Stepper {
         b_pass();
         make_rhs();
         double eps = eq.cs.lin_eps;
-        if (eq.cs.ew) {
-            const double kEtaMax = 1.0e-4;
-            const double rn = absmax(Rhs);
-            double eta = kEtaMax;
-            if (nk_pass > 1 && ew_prev > 0) {
-                const double ratio = rn / ew_prev;
-                eta = 0.9 * ratio * ratio;
-                const double safe = 0.9 * ew_eta * ew_eta;
-                if (safe > 0.1 && safe > eta) eta = safe;
-            }
-            if (eta > kEtaMax) eta = kEtaMax;
-            if (eta < eq.cs.lin_eps) eta = eq.cs.lin_eps;
-            ew_eta = eta; ew_prev = rn;
-            eps = eta;
-        }
+        if (eq.cs.ew) eps = ew_tolerance(absmax(Rhs));
         /* the previous pass's correction as the guess (the reference's), or
          * zero (zero_guess 1; the oracle is the FP64 path, so -1 is the
          * reference's) */
@@ -2236,6 +2260,38 @@ struct Stepper {
                                 eq.cs.row_scaling, eq.cs.lin_maxit, eps, iter, resid);
         if (ch == 0) satisfy_bc(corr);
         if (ch != 0) std::fill(corr.begin(), corr.end(), 0.0);
+        return ch;
+    }
+    /* Eisenstat-Walker choice 2 (NewtonKantorovich.cpp:108-126) on the
+     * norm rn of this pass's right-hand side; pass counts from 1 */
+    double ew_tolerance(double rn) {
+        const double kEtaMax = 1.0e-4;
+        double eta = kEtaMax;
+        if (nk_pass > 1 && ew_prev > 0) {
+            const double ratio = rn / ew_prev;
+            eta = 0.9 * ratio * ratio;
+            const double safe = 0.9 * ew_eta * ew_eta;
+            if (safe > 0.1 && safe > eta) eta = safe;
+        }
+        if (eta > kEtaMax) eta = kEtaMax;
+        if (eta < eq.cs.lin_eps) eta = eq.cs.lin_eps;
+        ew_eta = eta; ew_prev = rn;
+        return eta;
+    }
+    /* Solve of the scripted stepper: the tolerance is formed as in solve(),
+     * the correction is the script's; a failed solve leaves no correction
+     * (NewtonKantorovich.cpp:137-152) */
+    int solve_scripted(int &iter, double &resid) {
+        ++nk_pass;
+        double eps = eq.cs.lin_eps;
+        if (eq.cs.ew) eps = ew_tolerance(script.pick(script.rn));
+        script.eta.push_back(eps);
+        const double d = script.pick(script.dH);
+        const int ch = script.pick(script.rc);
+        std::fill(corr.begin(), corr.end(), ch == 0 ? d : 0.0);
+        iter = 1;
+        resid = 0.0;
+        ++script.call;
         return ch;
     }
     /* NKLoop (:157-422) */
@@ -2259,12 +2315,15 @@ struct Stepper {
                 extrap_move = move;
             }
         }
-        /* BeforeNKLoop: the working field from the state of the step */
-        if (conc_on) conc_before(); else eq.c = c_field;
-        eq.h = phi_old;
-        eq.set_matrix_A();
-        eq.apply_A(phi_old, Ax_old);
-        b_before();
+        coupled = 0;
+        if (!script.on) {
+            /* BeforeNKLoop: the working field from the state of the step */
+            if (conc_on) conc_before(); else eq.c = c_field;
+            eq.h = phi_old;
+            eq.set_matrix_A();
+            eq.apply_A(phi_old, Ax_old);
+            b_before();
+        }
 
         double resid_eq_old = 1e40, resid_first = 0, resid_best = 1e40;
         int stall = 0;
@@ -2273,12 +2332,13 @@ struct Stepper {
         int outer;
         resid_eq = 0;
         for (outer = 0; outer < eq.cs.nk_max; ++outer) {
-            if (conc_on && !conc_every()) { done = false; break; }
+            if (script.on) coupled = script.pick(script.dC);
+            else if (conc_on && !conc_every()) { done = false; break; }
             int it = 0; double lr = 0;
             const int ch = solve(it, lr);
             lin_step += it;
             resid_eq = absmax(corr);
-            if (conc_on && coupled > resid_eq) resid_eq = coupled;
+            if (coupled > resid_eq) resid_eq = coupled;
             if (std::getenv("YZO_DEBUG"))
                 std::fprintf(stderr, "[oracle] pass %d lin %d resid_lin %.3e corr %.3e rc %d\n",
                              outer, it, lr, resid_eq, ch);
@@ -2290,6 +2350,7 @@ struct Stepper {
                 const double floorPhi = 1000.0 * std::numeric_limits<double>::epsilon() * absmax(phi_guess);
                 done = (resid_eq < eq.cs.nk_eps) || (resid_first > 0 && resid_eq < resid_first * 1.0e-4)
                     || (resid_eq <= floorPhi);
+                if (!done) stag_cuts++;
                 break;
             }
             const double alpha = (resid_eq > resid_eq_old) ? 0.5 : 1.0;
@@ -2304,11 +2365,13 @@ struct Stepper {
         lin_total = lin_step;
         return done;
     }
+    /* Math::IsZero (MyMath_I.h:20-28): at or below the smallest normal */
+    static bool is_zero(double v) { return std::fabs(v) <= std::numeric_limits<double>::min(); }
     /* LocalTruncationCriterion (ImplicitSolver.cpp:271-307) */
     double truncation_criterion() const {
         double th = eq.cs.tsc_criterion;
-        if (th == 0) th = 1.0e-2;
-        if (dt_old == 0) return 1.0;
+        if (is_zero(th)) th = 1.0e-2;
+        if (is_zero(dt_old)) return 1.0;
         const double wfloor = eq.cs.tsc_floor * absmax(phi_old);
         double ek = 0;
         for (size_t i = 0; i < phi_guess.size(); ++i) {
@@ -2316,7 +2379,7 @@ struct Stepper {
             double w = std::fabs(ho);
             if (w < wfloor) w = wfloor;
             const double den = (dt_old + dt_now) * w;
-            const double e = (den == 0) ? 0.0 : 2.0 * (dt_now / dt_old) * (dt_old * hp + dt_now * hm - (dt_old + dt_now) * ho) / den;
+            const double e = is_zero(den) ? 0.0 : 2.0 * (dt_now / dt_old) * (dt_old * hp + dt_now * hm - (dt_old + dt_now) * ho) / den;
             ek = std::max(ek, std::fabs(e));
         }
         return 1.0 - ek / th;
@@ -2330,9 +2393,11 @@ struct Stepper {
         double dt_request = dt_now;
         long long lin_sum = 0;
         do {
+            const long long cuts0 = stag_cuts;
             ok = nk_loop(iter, resid);
             lin_sum += lin_total;
             if (!ok && extrap_move > 0) {
+                stag_cuts = cuts0;
                 extrap_blocked = true;
                 ok = nk_loop(iter, resid);
                 lin_sum += lin_total;
@@ -2360,7 +2425,7 @@ struct Stepper {
         } while (!step_done);
         phi_new = phi_guess;
         if (conc_on) conc_after();
-        eq.h = phi_new;
+        if (!script.on) eq.h = phi_new;
         time += dt_now;
         dt_old = dt_now;
         dt_now = dt_request;
@@ -2414,6 +2479,30 @@ void *yzo_create(const yz_case *c, const double *h, const double *s, const doubl
     auto *st = new yzo::Stepper();
     st->init(*c, h, s, conc, dt0);
     return st;
+}
+/* a stepper of the scripted Newton control (test hook, see Stepper::Script):
+ * n rows starting at phi0; the case gives the Newton and step-control
+ * numerics */
+void *yzo_script_create(const yz_case *c, int n, const double *phi0, double dt0,
+                        const double *dH, int ndH, const double *dC, int ndC,
+                        const double *rn, int nrn, const int *rc, int nrc) {
+    auto *st = new yzo::Stepper();
+    st->init_script(*c, n, phi0, dt0);
+    st->script.dH.assign(dH, dH + ndH);
+    st->script.dC.assign(dC, dC + ndC);
+    st->script.rn.assign(rn, rn + nrn);
+    st->script.rc.assign(rc, rc + nrc);
+    return st;
+}
+/* solves so far, stagnation cuts, and the tolerances the solves were given
+ * (up to `cap` of them); returns how many there were */
+int yzo_script_info(void *p, int *calls, long long *cuts, double *eta, int cap) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    if (calls) *calls = st->script.call;
+    if (cuts) *cuts = st->stag_cuts;
+    const int m = (int)st->script.eta.size();
+    for (int i = 0; i < m && i < cap; ++i) eta[i] = st->script.eta[i];
+    return m;
 }
 void yzo_destroy(void *p) { delete static_cast<yzo::Stepper *>(p); }
 int yzo_n(void *p) { return static_cast<yzo::Stepper *>(p)->eq.g.n; }

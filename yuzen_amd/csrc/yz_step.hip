@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -130,7 +131,8 @@ __global__ void k_truncation(Geo g, const double *hm, const double *ho, const do
         double w = fabs(ho[p]);
         if (w < wfloor) w = wfloor;
         const double den = (dt0 + dt) * w;
-        e = (den == 0) ? 0.0 : fabs(2.0 * (dt / dt0) * (dt0 * hp[p] + dt * hm[p] - (dt0 + dt) * ho[p]) / den);
+        // Math::IsZero (MyMath_I.h:20-28): at or below the smallest normal
+        e = (fabs(den) <= DBL_MIN) ? 0.0 : fabs(2.0 * (dt / dt0) * (dt0 * hp[p] + dt * hm[p] - (dt0 + dt) * ho[p]) / den);
     }
     __shared__ double sh[TB / 64];
     for (int off = 32; off > 0; off >>= 1) e = fmax(e, __shfl_down(e, off, 64));
@@ -1025,7 +1027,9 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
     a.bt.power = c->evap_power; a.bt.concr = c->evap_concr; a.bt.mres = c->molar_resin;
     a.bt.msol = c->molar_solvent; a.bt.prec = c->precursor;
     a.evold = S->evold; a.src = S->srcv; a.evp_out = S->evpv;
-    S->dt_now = dt0; S->dt_old = 0;
+    // C_TimeSteps::Build (TimeSteps.cpp:38-61) starts DT().old at the
+    // initial step: the first step's truncation criterion is taken too
+    S->dt_now = dt0; S->dt_old = dt0;
     return S;
 }
 
@@ -1089,10 +1093,10 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
         if (c.step_control) {
             double crit = -1.0;
             if (ok) {
-                if (S->dt_old == 0) crit = 1.0;
+                if (fabs(S->dt_old) <= DBL_MIN) crit = 1.0;
                 else {
                     double th = c.tsc_criterion;
-                    if (th == 0) th = 1.0e-2;
+                    if (fabs(th) <= DBL_MIN) th = 1.0e-2;
                     YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
                     hipLaunchKernelGGL(k_absmax, dim3(std::min(nb(np), 1024)), dim3(TB), 0, S->st, np, S->hold, S->dmax + 2);
                     if (sync_read(S)) return 2;
