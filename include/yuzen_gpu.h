@@ -196,6 +196,15 @@ int yz_step_get_h(yz_step *st, double *h);
 int yz_step_set_h(yz_step *st, const double *h);
 /* clock: time reached, step now in force, step before it */
 int yz_step_clock(yz_step *st, double *time, double *dt_now, double *dt_old);
+/* the Newton passes of the last step's accepted attempt (what the
+ * reference prints under YUZEN_DEBUG_RESIDUAL, NewtonKantorovich.cpp:
+ * 262-270): per pass the correction the exit test took (the larger of the
+ * film's and the coupled side's), the linear tolerance the pass's solve was
+ * given (Eisenstat-Walker, :108-126) and the relative residual it reached;
+ * up to cap of them; any array may be NULL. stag_cuts: the attempts ended by
+ * a stagnation far from convergence so far (_stagnationCuts, :357).
+ * Returns the number of passes, -1 on a null context. */
+int yz_step_pass_log(yz_step *st, double *corr, double *eta, double *lin_resid, int cap, long long *stag_cuts);
 /* The restart state (field*.state, yuzen/src/IO/RestartState.h:24-49):
  * yz_step_save_state writes what the next step depends on in the
  * reference's layout and keys (WellMixed_ModelBase::WriteState,

@@ -74,6 +74,7 @@ def load_from(path):
     lib.yzo_cz_redistribute.restype = C.c_double
     lib.yzo_cz_redistribute.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_case_defaults.argtypes = [P(YzCase)]
+    lib.yzo_pass_log.argtypes = [C.c_void_p, pd, pd, pd, C.c_int]
     lib.yzo_script_create.restype = C.c_void_p
     lib.yzo_script_create.argtypes = [P(YzCase), C.c_int, pd, C.c_double, pd, C.c_int, pd, C.c_int,
                                       pd, C.c_int, pi_, C.c_int]
@@ -191,6 +192,13 @@ class OracleStep:
         t, a, b = C.c_double(), C.c_double(), C.c_double()
         self.lib.yzo_clock(self.p, C.byref(t), C.byref(a), C.byref(b))
         return t.value, a.value, b.value
+
+    def pass_log(self):
+        """the passes of the last step's accepted attempt: (correction,
+        linear tolerance, linear relative residual) arrays"""
+        a, b, r = np.zeros(256), np.zeros(256), np.zeros(256)
+        m = self.lib.yzo_pass_log(self.p, _pd(a), _pd(b), _pd(r), 256)
+        return a[:m].copy(), b[:m].copy(), r[:m].copy()
 
     def pattern(self):
         rp = np.empty(self.n + 1, dtype=np.int32)

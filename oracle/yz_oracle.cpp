@@ -1593,6 +1593,9 @@ struct Stepper {
     long long lin_total = 0;
     bool first_step = true;
     long long stag_cuts = 0;           /* _stagnationCuts (NewtonKantorovich.cpp:357) */
+    /* the passes of the last attempt: correction, tolerance, residual */
+    vector<double> plog_corr, plog_eta, plog_lres;
+    double last_eps = 0;
 
     /* Scripted Newton control (test hook). In place of the assembly and
      * the linear solve, each pass takes its correction (every row alike),
@@ -2252,6 +2255,7 @@ struct Stepper {
         make_rhs();
         double eps = eq.cs.lin_eps;
         if (eq.cs.ew) eps = ew_tolerance(absmax(Rhs));
+        last_eps = eps;
         /* the previous pass's correction as the guess (the reference's), or
          * zero (zero_guess 1; the oracle is the FP64 path, so -1 is the
          * reference's) */
@@ -2286,6 +2290,7 @@ struct Stepper {
         double eps = eq.cs.lin_eps;
         if (eq.cs.ew) eps = ew_tolerance(script.pick(script.rn));
         script.eta.push_back(eps);
+        last_eps = eps;
         const double d = script.pick(script.dH);
         const int ch = script.pick(script.rc);
         std::fill(corr.begin(), corr.end(), ch == 0 ? d : 0.0);
@@ -2298,6 +2303,7 @@ struct Stepper {
     bool nk_loop(int &iter, double &resid_eq) {
         nk_pass = 0;
         int lin_step = 0;
+        plog_corr.clear(); plog_eta.clear(); plog_lres.clear();
         phi_guess = phi_old;
         extrap_move = 0;
         if (eq.cs.extrap && !extrap_blocked && !first_step) {
@@ -2339,6 +2345,7 @@ struct Stepper {
             lin_step += it;
             resid_eq = absmax(corr);
             if (coupled > resid_eq) resid_eq = coupled;
+            plog_corr.push_back(resid_eq); plog_eta.push_back(last_eps); plog_lres.push_back(lr);
             if (std::getenv("YZO_DEBUG"))
                 std::fprintf(stderr, "[oracle] pass %d lin %d resid_lin %.3e corr %.3e rc %d\n",
                              outer, it, lr, resid_eq, ch);
@@ -2502,6 +2509,17 @@ int yzo_script_info(void *p, int *calls, long long *cuts, double *eta, int cap) 
     if (cuts) *cuts = st->stag_cuts;
     const int m = (int)st->script.eta.size();
     for (int i = 0; i < m && i < cap; ++i) eta[i] = st->script.eta[i];
+    return m;
+}
+/* the passes of the last step's accepted attempt (yz_step_pass_log) */
+int yzo_pass_log(void *p, double *corr, double *eta, double *lres, int cap) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    const int m = (int)st->plog_corr.size();
+    for (int i = 0; i < m && i < cap; ++i) {
+        if (corr) corr[i] = st->plog_corr[i];
+        if (eta) eta[i] = st->plog_eta[i];
+        if (lres) lres[i] = st->plog_lres[i];
+    }
     return m;
 }
 void yzo_destroy(void *p) { delete static_cast<yzo::Stepper *>(p); }

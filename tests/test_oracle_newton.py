@@ -463,3 +463,22 @@ def test_planted_mistake_fails(planted, plant):
     for name in PLANTS[plant][2]:
         with pytest.raises(AssertionError):
             by_name[name](lib)
+
+
+def test_borderline_exit_check():
+    """The check the GPU parity tests apply to a step whose Newton counts
+    differ: it accepts corrections that straddle the exit threshold next to
+    it and refuses a pass that differs by more, a wide miss, or counts two
+    apart"""
+    from .helpers import borderline_exit
+    c = ob.case_defaults()
+    c.nk_eps, c.nk_eps_rel = 1e-10, 1e-6
+    first = 4.55e-3                      # threshold 4.55e-9
+    ok = borderline_exit(c, [first, 2e-5, 3.4e-9], [first, 2.1e-5, 7.6e-9, 1e-14])
+    assert "pass 3" in ok
+    with pytest.raises(AssertionError):   # the shorter run's exit far below the threshold
+        borderline_exit(c, [first, 2e-5, 3e-12], [first, 2e-5, 7.6e-9, 1e-14])
+    with pytest.raises(AssertionError):   # the trajectories part before the exit
+        borderline_exit(c, [first, 2e-5, 3.4e-9], [first, 9e-5, 7.6e-9, 1e-14])
+    with pytest.raises(AssertionError):   # two passes apart
+        borderline_exit(c, [first, 3.4e-9], [first, 2e-5, 7.6e-9, 1e-14])

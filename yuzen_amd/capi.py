@@ -200,6 +200,7 @@ def load():
     lib.yz_step_balances.argtypes = [C.c_void_p, pd]
     lib.yz_step_set_h.argtypes = [C.c_void_p, pd]
     lib.yz_step_clock.argtypes = [C.c_void_p, pd, pd, pd]
+    lib.yz_step_pass_log.argtypes = [C.c_void_p, pd, pd, pd, C.c_int, C.POINTER(C.c_longlong)]
     lib.yz_step_timing.argtypes = [C.c_void_p, pd]
     lib.yz_step_destroy.argtypes = [C.c_void_p]
     lib.yz_step_kernel_time.argtypes = [C.c_void_p, i, P(C.c_longlong), pd]

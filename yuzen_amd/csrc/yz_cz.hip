@@ -91,17 +91,23 @@ __host__ __device__ inline int fold_pad(int i, int np, int per) {
     return i;
 }
 
-// slots of a film row as the reference stores them (the diagonal, then
-// the insertion order of SetMatrixA), and where each lands among the
-// solver's 13 slots (oM mm om pm Mo mo oo po Po mp op pp oP)
-__constant__ int c_hox[13] = {0, -2, -1, 1, 2, 0, 0, 0, 0, -1, -1, 1, 1};
-__constant__ int c_hoy[13] = {0, 0, 0, 0, 0, -2, -1, 1, 2, -1, 1, -1, 1};
-__constant__ int c_hslot[13] = {6, 4, 5, 7, 8, 0, 2, 10, 12, 1, 9, 3, 11};
-// slots of a concentration row as stored (the diagonal, then put()'s order,
-// CZ_ConcEquation_2D.cpp:535-556)
-__constant__ int c_cox[21] = {0, 0, -1, 0, 1, -2, -1, 1, 2, -1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-__constant__ int c_coy[21] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, -1, 1, -1, 1};
-__constant__ int c_coz[21] = {0, -2, -1, -1, -1, 0, 0, 0, 0, 1, 1, 1, 2, 0, 0, 0, 0, -1, -1, 1, 1};
+// the 21 slots of a concentration row: the zeta column (the band the
+// column solves factor), the x line, the y line, then the x-zeta and
+// y-zeta corners; slot 2 is the diagonal
+#define YZ_S21X {0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, -1, 1, -1, 1, 0, 0, 0, 0}
+#define YZ_S21Y {0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, -1, 1, -1, 1}
+#define YZ_S21Z {-2, -1, 0, 1, 2, 0, 0, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1, -1, -1, 1, 1}
+__constant__ int c_s21x[21] = YZ_S21X;
+__constant__ int c_s21y[21] = YZ_S21Y;
+__constant__ int c_s21z[21] = YZ_S21Z;
+constexpr int kDiag21 = 2;
+__host__ __device__ constexpr int slot21(int ox, int oy, int oz) {
+    return (ox == 0 && oy == 0) ? 2 + oz
+         : (oy == 0 && oz == 0) ? (ox < 0 ? 7 + ox : 6 + ox)
+         : (ox == 0 && oz == 0) ? (oy < 0 ? 11 + oy : 10 + oy)
+         : (oy == 0) ? 13 + (ox > 0) + 2 * (oz > 0)
+         : 17 + (oy > 0) + 2 * (oz > 0);
+}
 
 struct Args {
     Geo3 g;
@@ -258,10 +264,34 @@ __global__ void k_faces(Args a) {
     }
 }
 
-// SetMatrixA of the film for one real cell, then the row of
-// I + theta dt A with its right-hand side h_old - (1 - theta) dt A h_old
-// - dt B (MakeMatrix_IA / MakeRHS, SemiImplicit.cpp:158-190), scaled by
-// its diagonal (ScaleRows), into the solver's planes
+// The film's row as the divergence of its four face fluxes
+// (CZ_HeightEquation_2D.cpp:750-1081 gives the same operator entry by
+// entry). The capillary flux through a face is
+//   kappa_f (d^3/dn^3 + d/dn d^2/dt^2)(h + s),   kappa_f = D_f sigma_f
+// (k_faces), with n the face normal and t the tangent: a fixed set of taps
+// around the face, below. A row of A is then (F_high - F_low) / ds over
+// both axes, so each face's taps enter the rows on either side of it with
+// opposite signs and the operator is conservative by construction. The
+// explicit fluxes of the stage (substrate relief, gravity, spin, Marangoni)
+// are k_faces' qExp, differenced the same way. The row of
+// I + theta dt A, its right-hand side h_old - (1 - theta) dt A h_old - dt B
+// (SemiImplicit.cpp:158-190) and the scaling by the diagonal follow.
+struct FaceTap {
+    int n, t;      // cell offset along the normal (from the face's high cell) and the tangent
+    double w;      // weight
+    bool mixed;    // the d/dn d^2/dt^2 part (else the normal third difference)
+};
+constexpr FaceTap kFaceTaps[10] = {
+    {+1, 0, 1.0, false}, {0, 0, -3.0, false}, {-1, 0, 3.0, false}, {-2, 0, -1.0, false},
+    {0, +1, 1.0, true},  {0, 0, -2.0, true},  {0, -1, 1.0, true},
+    {-1, +1, -1.0, true}, {-1, 0, 2.0, true}, {-1, -1, -1.0, true}};
+// the solver's slot of a cell offset (oM mm om pm Mo mo oo po Po mp op pp oP)
+__host__ __device__ constexpr int slot13(int ox, int oy) {
+    return oy == -2 ? 0 : oy == 2 ? 12 : oy == -1 ? 2 + ox : oy == 1 ? 10 + ox : 6 + ox;
+}
+constexpr int kSlotX[13] = {0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0};
+constexpr int kSlotY[13] = {-2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2};
+
 __global__ void k_hrows(Args a) {
     const Geo3 &g = a.g;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,124 +300,53 @@ __global__ void k_hrows(Args a) {
     const int ix = q % g.nx + 2, jy = q / g.nx + 2;
     const int p = I2(g, ix, jy);
     const yz_cz_case &cs = a.cs;
-    const double rdx = 1.0 / g.dx, rdx2 = rdx * rdx, rdx4 = rdx2 * rdx2;
-    const double rdy = 1.0 / g.dy, rdy2 = rdy * rdy, rdy4 = rdy2 * rdy2;
-    const double rdx2dy2 = rdx2 * rdy2;
-    auto SS = [&](int i, int j) { return a.S[I2(g, i, j)]; };
-    auto HH = [&](int i, int j) { return a.fH[I2(g, i, j)]; };
+    const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
+    // per axis: the normal part's scale 1/ds^4 and the mixed part's 1/(dx^2 dy^2)
+    const double rnx = (rdx * rdx) * (rdx * rdx), rny = (rdy * rdy) * (rdy * rdy);
+    const double rmix = (rdx * rdx) * (rdy * rdy);
     double A[13];
-    const double DL = a.Dx[p], DR = a.Dx[p + 1];
-    const double sigL = a.sgx[p], sigR = a.sgx[p + 1];
-    const double aL = cs.capillary ? DL * sigL * rdx4 : 0.0;
-    const double aR = cs.capillary ? DR * sigR * rdx4 : 0.0;
-    A[1] = +aL;
-    A[2] = -3.0 * aL - aR;
-    A[0] = +3.0 * aL + 3.0 * aR;
-    A[3] = -aL - 3.0 * aR;
-    A[4] = +aR;
-    const double DB = a.Dy[p], DT = a.Dy[p + g.Nx];
-    const double sigB = a.sgy[p], sigT = a.sgy[p + g.Nx];
-    const double bB = cs.capillary ? DB * sigB * rdy4 : 0.0;
-    const double bT = cs.capillary ? DT * sigT * rdy4 : 0.0;
-    A[5] = +bB;
-    A[6] = -3.0 * bB - bT;
-    A[0] += +3.0 * bB + 3.0 * bT;
-    A[7] = -bB - 3.0 * bT;
-    A[8] = +bT;
-    const double cL = cs.capillary ? DL * sigL * rdx2dy2 : 0.0;
-    const double cR = cs.capillary ? DR * sigR * rdx2dy2 : 0.0;
-    const double cB = cs.capillary ? DB * sigB * rdx2dy2 : 0.0;
-    const double cT = cs.capillary ? DT * sigT * rdx2dy2 : 0.0;
-    A[9] = +cL;
-    A[2] += -2.0 * cL;
-    A[10] = +cL;
-    A[11] = +cR;
-    A[3] += -2.0 * cR;
-    A[12] = +cR;
-    A[6] += -cL - cR;
-    A[0] += +2.0 * cL + 2.0 * cR;
-    A[7] += -cL - cR;
-    A[9] += +cB;
-    A[6] += -2.0 * cB;
-    A[11] += +cB;
-    A[10] += +cT;
-    A[7] += -2.0 * cT;
-    A[12] += +cT;
-    A[2] += -cB - cT;
-    A[0] += +2.0 * cB + 2.0 * cT;
-    A[3] += -cB - cT;
-
-    const double sPo = SS(ix + 2, jy), spo = SS(ix + 1, jy), smo = SS(ix - 1, jy), sMo = SS(ix - 2, jy);
-    const double soo = SS(ix, jy);
-    double dhdt = 0.0;
+#pragma unroll
+    for (int k = 0; k < 13; ++k) A[k] = 0.0;
     if (cs.capillary) {
-        const double d3sL = (spo - 3.0 * soo + 3.0 * smo - sMo) * rdx * rdx2;
-        const double d3sR = (sPo - 3.0 * spo + 3.0 * soo - smo) * rdx * rdx2;
-        dhdt -= (DR * sigR * d3sR - DL * sigL * d3sL) * rdx;
+        // kappa of the low and high face of each axis
+        const double kx[2] = {a.Dx[p] * a.sgx[p], a.Dx[p + 1] * a.sgx[p + 1]};
+        const double ky[2] = {a.Dy[p] * a.sgy[p], a.Dy[p + g.Nx] * a.sgy[p + g.Nx]};
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const double sg = side ? 1.0 : -1.0;
+#pragma unroll
+            for (int t = 0; t < 10; ++t) {
+                const FaceTap &f = kFaceTaps[t];
+                const double wx = sg * f.w * kx[side] * (f.mixed ? rmix : rnx);
+                const double wy = sg * f.w * ky[side] * (f.mixed ? rmix : rny);
+                A[slot13(side + f.n, f.t)] += wx;
+                A[slot13(f.t, side + f.n)] += wy;
+            }
+        }
     }
-    if (cs.vertical_gravity) {
-        const double hsP = HH(ix + 1, jy) + spo, hsO = HH(ix, jy) + soo, hsM = HH(ix - 1, jy) + smo;
-        const double gR = DR * cs.Bo * (hsP - hsO) * rdx;
-        const double gL = DL * cs.Bo * (hsO - hsM) * rdx;
-        dhdt += (gR - gL) * rdx;
-    }
-    if (cs.inplane_gravity) {
-        const double fR = DR * cs.Pgx, fL = DL * cs.Pgx;
-        dhdt -= (fR - fL) * rdx;
-    }
-    if (cs.centrifugal) {
-        const double xR = (g.xlo + (ix + 1) * g.dx) - cs.rot_x, xL = (g.xlo + ix * g.dx) - cs.rot_x;
-        dhdt -= (DR * cs.Sp * xR - DL * cs.Sp * xL) * rdx;
-    }
-    if (cs.marangoni) dhdt -= (a.Mx[p + 1] - a.Mx[p]) * rdx;
-    if (cs.capillary) {
-        const double soP = SS(ix, jy + 2), sop = SS(ix, jy + 1), som = SS(ix, jy - 1), soM = SS(ix, jy - 2);
-        const double d3sB = (sop - 3.0 * soo + 3.0 * som - soM) * rdy * rdy2;
-        const double d3sT = (soP - 3.0 * sop + 3.0 * soo - som) * rdy * rdy2;
-        dhdt -= (DT * sigT * d3sT - DB * sigB * d3sB) * rdy;
-        const double smm = SS(ix - 1, jy - 1), smp = SS(ix - 1, jy + 1);
-        const double spm = SS(ix + 1, jy - 1), spp = SS(ix + 1, jy + 1);
-        const double xL_ = ((som - 2.0 * soo + sop) - (smm - 2.0 * smo + smp)) * rdx * rdy2;
-        const double xR_ = ((spm - 2.0 * spo + spp) - (som - 2.0 * soo + sop)) * rdx * rdy2;
-        dhdt -= (DR * sigR * xR_ - DL * sigL * xL_) * rdx;
-        const double yB_ = ((smo - 2.0 * soo + spo) - (smm - 2.0 * som + spm)) * rdy * rdx2;
-        const double yT_ = ((smp - 2.0 * sop + spp) - (smo - 2.0 * soo + spo)) * rdy * rdx2;
-        dhdt -= (DT * sigT * yT_ - DB * sigB * yB_) * rdy;
-    }
-    if (cs.vertical_gravity) {
-        const double sop = SS(ix, jy + 1), som = SS(ix, jy - 1);
-        const double hsT = HH(ix, jy + 1) + sop, hsO = HH(ix, jy) + soo, hsB = HH(ix, jy - 1) + som;
-        const double gT = DT * cs.Bo * (hsT - hsO) * rdy;
-        const double gB = DB * cs.Bo * (hsO - hsB) * rdy;
-        dhdt += (gT - gB) * rdy;
-    }
-    if (cs.inplane_gravity) dhdt -= (DT * cs.Pgy - DB * cs.Pgy) * rdy;
-    if (cs.centrifugal) {
-        const double yT = (g.ylo + (jy + 1) * g.dy) - cs.rot_y, yB = (g.ylo + jy * g.dy) - cs.rot_y;
-        dhdt -= (DT * cs.Sp * yT - DB * cs.Sp * yB) * rdy;
-    }
-    if (cs.marangoni) dhdt -= (a.My[p + g.Nx] - a.My[p]) * rdy;
+    // the explicit fluxes' divergence, evaporation with its solvent limits
     double Evap = cs.Ev * evap_rate(a, a.fH, a.fC, ix, jy);
     if (Evap > 0.0) {
         Evap = solvent_limit(Evap, a.fH[p], column_mean(g, a.fC, ix, jy), a.dt);
         Evap = solvent_limit(Evap, a.tH[p], column_mean(g, a.tC, ix, jy), a.dt);
     }
-    dhdt -= Evap;
-    a.colSrc[p] = 0.0 - Evap - 0.0;
-    const double B = -dhdt;
-    // A h_old in storage order (CRS_Matrix::MatVec)
+    const double divq = (a.qExpX[p + 1] - a.qExpX[p]) * rdx + (a.qExpY[p + g.Nx] - a.qExpY[p]) * rdy;
+    a.colSrc[p] = -Evap;
+    const double B = divq + Evap;      // dh/dt = -A h - B
     double Ax = 0.0;
-    Ax += A[0] * a.hO[p];
-    for (int k = 1; k < 13; ++k) Ax += A[k] * a.hO[I2(g, ix + c_hox[k], jy + c_hoy[k])];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) Ax += A[k] * a.hO[I2(g, ix + kSlotX[k], jy + kSlotY[k])];
     const double th = a.theta, dt = a.dt;
     const double rhs = a.hO[p] - dt * (1.0 - th) * Ax - dt * B;
-    const double d = 1.0 + th * dt * A[0];
-    const double inv = (fabs(d) < 1.0e-30) ? 1.0 : 1.0 / d;
+    const double d = 1.0 + th * dt * A[6];
+    const bool scale = !(fabs(d) < 1.0e-30);
+    const double inv = scale ? 1.0 / d : 1.0;
+#pragma unroll
     for (int k = 0; k < 13; ++k) {
-        const double v = (k == 0) ? d : th * dt * A[k];
-        a.hP[(size_t)c_hslot[k] * N + q] = (fabs(d) < 1.0e-30) ? v : v * inv;
+        const double v = (k == 6) ? d : th * dt * A[k];
+        a.hP[(size_t)k * N + q] = scale ? v * inv : v;
     }
-    a.hb[q] = (fabs(d) < 1.0e-30) ? rhs : rhs * inv;
+    a.hb[q] = scale ? rhs * inv : rhs;
     a.hx[q] = a.hN[p];
 }
 
@@ -666,165 +625,152 @@ __global__ void k_cghost_x(Geo3 g, double *x) {
 }
 
 // ---------------------------------------------------------------------
-// SetMatrixA of the concentration for one real cell (:290-570), its row
-// of I + theta dt A and right-hand side -dt (A c_old + B) (SolveStage,
-// with A c_old in the form of ExplicitPart), scaled by the diagonal
+// The concentration row of one real cell as a balance of face fluxes.
+// Through each face runs a flux P_f per unit cell volume, along the axis:
+//   advection   Q_f c_f / ds, c_f the scheme's face value over the four
+//               cells along the axis (weights c1 + sign(Q_f) c2);
+//   diffusion   -G_f, the diffusive flux in the stretched coordinates:
+//     lateral face  G = (h_f D_f / ds) [ (c_hi - c_lo) / ds
+//                        - (zeta h_s + s_s)_f / h_f (c_{k+1} - c_{k-1},
+//                          summed over both cells) / (4 dzeta) ],
+//     zeta face     G = (D_f / dzeta) [ (As + |grad(zeta h + s)|^2) / h
+//                        (c_hi - c_lo) / dzeta - (zeta h_x + s_x)
+//                        (c_{i+1} - c_{i-1}, summed over both levels)
+//                        / (4 dx) - (the same along y) ],
+//     none through a wall;
+// the cell's inflow is P_low - P_high on every axis. The substrate carries
+// nothing; the free surface carries the evaporation flux W_T c of the cell
+// itself, a reaction term. A row of A is minus the inflow over h; its
+// diagonal is minus the sum of its off-diagonal entries plus the reaction
+// term, so that a uniform c is kept exactly. This is the operator of
+// CZ_ConcEquation_2D::SetMatrixA (CZ_ConcEquation_2D.cpp:301-573). The
+// stage's right-hand side is -dt (A c_old) in the balance form
+// react c_r + sum_k A_k (c_k - c_r) (ExplicitPart, :895-913).
 // ---------------------------------------------------------------------
+// fourth-order first derivatives: at the face between f0 and f1 (from
+// f_-1 .. f2), and at the centre of a cell (from its +-1, +-2 neighbours)
+__device__ inline double dface4(double fm1, double f0, double f1, double f2, double ds) {
+    return (fm1 - 27.0 * f0 + 27.0 * f1 - f2) / (24.0 * ds);
+}
+__device__ inline double dcentre4(double fm2, double fm1, double f1, double f2, double ds) {
+    return (fm2 - 8.0 * fm1 + 8.0 * f1 - f2) / (12.0 * ds);
+}
+// react c_r + sum over the off-diagonal slots of A_k (c_k - c_r)
+__device__ inline double balance_form(const Geo3 &g, const double (&A)[21], double react, const double *c, int ix,
+                                      int jy, int kz) {
+    const double cr = c[I3(g, ix, jy, kz)];
+    double s = react * cr;
+#pragma unroll
+    for (int k = 0; k < 21; ++k)
+        if (k != kDiag21) s += A[k] * (c[I3(g, ix + c_s21x[k], jy + c_s21y[k], kz + c_s21z[k])] - cr);
+    return s;
+}
+
 __global__ void k_crows(Args a, const double *cold) {
     const Geo3 &g = a.g;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.nint) return;
     const int ix = q % g.nx + 2, jy = (q / g.nx) % g.ny + 2, kz = q / (g.nx * g.ny) + 2;
     const yz_cz_case &cs = a.cs;
-    const double As = cs.As, Df = cs.Df, dx = g.dx, dy = g.dy, dz = g.dz;
-    auto C = [&](int i, int j, int k) { return a.fC[I3(g, i, j, k)]; };
-    auto Hh = [&](int i, int j) { return a.fH[I2(g, i, j)]; };
-    auto Ss = [&](int i, int j) { return a.S[I2(g, i, j)]; };
-    const bool wallXmin = (ix == 2) && g.wall[0];
-    const bool wallXmax = (ix == g.Nx - 3) && g.wall[1];
-    const bool wallYmin = (jy == 2) && g.wall[2];
-    const bool wallYmax = (jy == g.Ny - 3) && g.wall[3];
-    const double hoo = Hh(ix, jy);
-    const double hpo = Hh(ix + 1, jy), hPo = Hh(ix + 2, jy), hmo = Hh(ix - 1, jy), hMo = Hh(ix - 2, jy);
-    const double spo = Ss(ix + 1, jy), sPo = Ss(ix + 2, jy), smo = Ss(ix - 1, jy), sMo = Ss(ix - 2, jy);
-    const double soo = Ss(ix, jy);
-    const double hL = 0.5 * (hoo + hmo), hR = 0.5 * (hoo + hpo);
-    const double hx0 = (hMo - 8.0 * hmo + 8.0 * hpo - hPo) / (12.0 * dx);
-    const double sx0 = (sMo - 8.0 * smo + 8.0 * spo - sPo) / (12.0 * dx);
-    const double hxL = (hMo - 27.0 * hmo + 27.0 * hoo - hpo) / (24.0 * dx);
-    const double sxL = (sMo - 27.0 * smo + 27.0 * soo - spo) / (24.0 * dx);
-    const double hxR = (hmo - 27.0 * hoo + 27.0 * hpo - hPo) / (24.0 * dx);
-    const double sxR = (smo - 27.0 * soo + 27.0 * spo - sPo) / (24.0 * dx);
-    const double hoP = Hh(ix, jy + 2), hop = Hh(ix, jy + 1), hom = Hh(ix, jy - 1), hoM = Hh(ix, jy - 2);
-    const double soP = Ss(ix, jy + 2), sop = Ss(ix, jy + 1), som = Ss(ix, jy - 1), soM = Ss(ix, jy - 2);
-    const double hS = 0.5 * (hoo + hom), hN = 0.5 * (hoo + hop);
-    const double hy0 = (hoM - 8.0 * hom + 8.0 * hop - hoP) / (12.0 * dy);
-    const double sy0 = (soM - 8.0 * som + 8.0 * sop - soP) / (12.0 * dy);
-    const double hyS = (hoM - 27.0 * hom + 27.0 * hoo - hop) / (24.0 * dy);
-    const double syS = (soM - 27.0 * som + 27.0 * soo - sop) / (24.0 * dy);
-    const double hyN = (hom - 27.0 * hoo + 27.0 * hop - hoP) / (24.0 * dy);
-    const double syN = (som - 27.0 * soo + 27.0 * sop - soP) / (24.0 * dy);
-    const double zeta = dz * (-1.5 + (double)kz);
-    const bool bottom = (kz == 2), top = (kz == g.Nz - 3);
-    double xz[5][5], yz[5][3];
-    for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) xz[i][j] = 0.0;
-    for (int i = 0; i < 5; ++i) for (int j = 0; j < 3; ++j) yz[i][j] = 0.0;
-    double rct = 0.0;
-#define XZ(di, dk, v) (xz[(di) + 2][(dk) + 2] += (v))
-#define YZ(dj, dk, v) do { if ((dj) == 0) xz[2][(dk) + 2] += (v); else yz[(dj) + 2][(dk) + 1] += (v); } while (0)
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    auto cc = [&](int i, int j, int k) { return a.fC[I3(g, i, j, k)]; };
+    auto hh = [&](int i, int j) { return a.fH[I2(g, i, j)]; };
+    auto ss = [&](int i, int j) { return a.S[I2(g, i, j)]; };
     const int r3 = I3(g, ix, jy, kz);
-    {
-        const double QL = a.Qx[r3], QR = a.Qx[r3 + 1];
-        const double sL = sgn(QL), sR = sgn(QR);
-        for (int t = 0; t < 4; t++) {
-            XZ(t - 2, 0, +QL * (a.w1[t] + sL * a.w2[t]) / dx);
-            XZ(t - 1, 0, -QR * (a.w1[t] + sR * a.w2[t]) / dx);
+    const double zeta = dz * ((double)kz - 1.5);
+    const bool bottom = kz == 2, top = kz == g.Nz - 3;
+    double v[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) v[k] = 0.0;
+    auto add = [&](int ox, int oy, int oz, double w) { v[slot21(ox, oy, oz)] += w; };
+
+    // lateral faces; side 0 is the low face (inflow +P), side 1 the high (-P)
+#pragma unroll
+    for (int axis = 0; axis < 2; ++axis) {
+        const int ex = axis == 0, ey = axis == 1;
+        const double ds = axis == 0 ? dx : dy;
+        const double *Qf = axis == 0 ? a.Qx : a.Qy;
+        const int fstride = axis == 0 ? 1 : g.Nx;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const double sg = side ? -1.0 : 1.0;
+            const int hi = side, lo = side - 1;       // the face's cells, offsets along the axis
+            const double Q = Qf[r3 + side * fstride];
+            const double sq = sgn(Q);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) add((lo - 1 + t) * ex, (lo - 1 + t) * ey, 0, sg * Q * (a.w1[t] + sq * a.w2[t]) / ds);
+            if (wall_face(a, ix + hi * ex, jy + hi * ey, axis)) continue;
+            const int li = ix + lo * ex, lj = jy + lo * ey, hi_i = ix + hi * ex, hi_j = jy + hi * ey;
+            const double hf = 0.5 * (hh(li, lj) + hh(hi_i, hi_j));
+            const double Df = cs.Df * diffu(cs, 0.5 * (cc(li, lj, kz) + cc(hi_i, hi_j, kz)));
+            const double hs = dface4(hh(li - ex, lj - ey), hh(li, lj), hh(hi_i, hi_j), hh(hi_i + ex, hi_j + ey), ds);
+            const double sd = dface4(ss(li - ex, lj - ey), ss(li, lj), ss(hi_i, hi_j), ss(hi_i + ex, hi_j + ey), ds);
+            const double ka = hf * Df / ds;
+            const double km = ka * ((hs * zeta + sd) / hf) / (4.0 * dz);
+            add(hi * ex, hi * ey, 0, -sg * ka / ds);
+            add(lo * ex, lo * ey, 0, +sg * ka / ds);
+            add(lo * ex, lo * ey, +1, +sg * km);
+            add(hi * ex, hi * ey, +1, +sg * km);
+            add(lo * ex, lo * ey, -1, -sg * km);
+            add(hi * ex, hi * ey, -1, -sg * km);
         }
     }
+    // zeta faces
+    double rct = 0.0;
     {
-        const double QS = a.Qy[r3], QN = a.Qy[r3 + g.Nx];
-        const double sS = sgn(QS), sN = sgn(QN);
-        for (int t = 0; t < 4; t++) {
-            YZ(t - 2, 0, +QS * (a.w1[t] + sS * a.w2[t]) / dy);
-            YZ(t - 1, 0, -QN * (a.w1[t] + sN * a.w2[t]) / dy);
+        const double hc = hh(ix, jy);
+        const double hx0 = dcentre4(hh(ix - 2, jy), hh(ix - 1, jy), hh(ix + 1, jy), hh(ix + 2, jy), dx);
+        const double sx0 = dcentre4(ss(ix - 2, jy), ss(ix - 1, jy), ss(ix + 1, jy), ss(ix + 2, jy), dx);
+        const double hy0 = dcentre4(hh(ix, jy - 2), hh(ix, jy - 1), hh(ix, jy + 1), hh(ix, jy + 2), dy);
+        const double sy0 = dcentre4(ss(ix, jy - 2), ss(ix, jy - 1), ss(ix, jy + 1), ss(ix, jy + 2), dy);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const double sg = side ? -1.0 : 1.0;
+            const int hi = side, lo = side - 1;
+            if (side == 0 && bottom) continue;
+            const double W = a.W[r3 + side * g.n2];
+            if (side == 1 && top) { rct += W / dz; continue; }
+            const double sw = sgn(W);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) add(0, 0, lo - 1 + t, sg * W * (a.w1[t] + sw * a.w2[t]) / dz);
+            const double zf = zeta + (side ? 0.5 : -0.5) * dz;
+            const double Dz = cs.Df * diffu(cs, 0.5 * (cc(ix, jy, kz + lo) + cc(ix, jy, kz + hi)));
+            const double gx = hx0 * zf + sx0, gy = hy0 * zf + sy0;
+            const double K = Dz * ((cs.As + gx * gx + gy * gy) / hc) / (dz * dz);
+            const double kxw = Dz * gx / (4.0 * dx) / dz, kyw = Dz * gy / (4.0 * dy) / dz;
+            add(0, 0, hi, -sg * K);
+            add(0, 0, lo, +sg * K);
+            add(+1, 0, lo, +sg * kxw); add(+1, 0, hi, +sg * kxw);
+            add(-1, 0, lo, -sg * kxw); add(-1, 0, hi, -sg * kxw);
+            add(0, +1, lo, +sg * kyw); add(0, +1, hi, +sg * kyw);
+            add(0, -1, lo, -sg * kyw); add(0, -1, hi, -sg * kyw);
         }
     }
-    {
-        const double WB = a.W[r3], WT = a.W[r3 + g.n2];
-        if (!bottom) {
-            const double sB = sgn(WB);
-            for (int t = 0; t < 4; t++) XZ(0, t - 2, +WB * (a.w1[t] + sB * a.w2[t]) / dz);
-        }
-        if (!top) {
-            const double sT = sgn(WT);
-            for (int t = 0; t < 4; t++) XZ(0, t - 1, -WT * (a.w1[t] + sT * a.w2[t]) / dz);
-        } else {
-            rct += WT / dz;
-        }
-    }
-    const double Coo = C(ix, jy, kz);
-    {
-        const double DL = Df * diffu(cs, 0.5 * (C(ix - 1, jy, kz) + Coo));
-        const double DR = Df * diffu(cs, 0.5 * (Coo + C(ix + 1, jy, kz)));
-        const double AL = (hxL * zeta + sxL) / hL;
-        const double AR = (hxR * zeta + sxR) / hR;
-        const double aL = wallXmin ? 0.0 : hL * DL / dx, aR = wallXmax ? 0.0 : hR * DR / dx;
-        XZ(-1, 0, +aL / dx);
-        XZ(0, 0, -aL / dx);
-        XZ(-1, 1, +aL * AL / (4.0 * dz)); XZ(0, 1, +aL * AL / (4.0 * dz));
-        XZ(-1, -1, -aL * AL / (4.0 * dz)); XZ(0, -1, -aL * AL / (4.0 * dz));
-        XZ(1, 0, +aR / dx);
-        XZ(0, 0, -aR / dx);
-        XZ(0, 1, -aR * AR / (4.0 * dz)); XZ(1, 1, -aR * AR / (4.0 * dz));
-        XZ(0, -1, +aR * AR / (4.0 * dz)); XZ(1, -1, +aR * AR / (4.0 * dz));
-    }
-    {
-        const double DS = Df * diffu(cs, 0.5 * (C(ix, jy - 1, kz) + Coo));
-        const double DN = Df * diffu(cs, 0.5 * (Coo + C(ix, jy + 1, kz)));
-        const double AS = (hyS * zeta + syS) / hS;
-        const double AN = (hyN * zeta + syN) / hN;
-        const double aS = wallYmin ? 0.0 : hS * DS / dy, aN = wallYmax ? 0.0 : hN * DN / dy;
-        YZ(-1, 0, +aS / dy);
-        YZ(0, 0, -aS / dy);
-        YZ(-1, 1, +aS * AS / (4.0 * dz)); YZ(0, 1, +aS * AS / (4.0 * dz));
-        YZ(-1, -1, -aS * AS / (4.0 * dz)); YZ(0, -1, -aS * AS / (4.0 * dz));
-        YZ(1, 0, +aN / dy);
-        YZ(0, 0, -aN / dy);
-        YZ(0, 1, -aN * AN / (4.0 * dz)); YZ(1, 1, -aN * AN / (4.0 * dz));
-        YZ(0, -1, +aN * AN / (4.0 * dz)); YZ(1, -1, +aN * AN / (4.0 * dz));
-    }
-    for (int side = 0; side < 2; side++) {
-        if (side == 0 && bottom) continue;
-        if (side == 1 && top) continue;
-        const int k0 = (side == 0) ? kz - 1 : kz;
-        const double zf = zeta + (side == 0 ? -0.5 : 0.5) * dz;
-        const double Dz = Df * diffu(cs, 0.5 * (C(ix, jy, k0) + C(ix, jy, k0 + 1)));
-        const double hA = hx0 * zf + sx0;
-        const double hAy = hy0 * zf + sy0;
-        const double gsg = (side == 1) ? +1.0 : -1.0;
-        const double kzz = gsg * Dz * (As / hoo + (hA * hA + hAy * hAy) / hoo) / (dz * dz);
-        XZ(0, k0 + 1 - kz, +kzz);
-        XZ(0, k0 - kz, -kzz);
-        const double kx = -gsg * Dz * hA / (4.0 * dx) / dz;
-        XZ(1, k0 - kz, +kx); XZ(1, k0 + 1 - kz, +kx);
-        XZ(-1, k0 - kz, -kx); XZ(-1, k0 + 1 - kz, -kx);
-        const double ky = -gsg * Dz * hAy / (4.0 * dy) / dz;
-        YZ(1, k0 - kz, +ky); YZ(1, k0 + 1 - kz, +ky);
-        YZ(-1, k0 - kz, -ky); YZ(-1, k0 + 1 - kz, -ky);
-    }
-#undef XZ
-#undef YZ
-    const double rh = 1.0 / hoo;
+    const double rh = 1.0 / hh(ix, jy);
     double A[21];
     double offsum = 0.0;
-    {
-        const double v[20] = {xz[2][0], xz[1][1], xz[2][1], xz[3][1], xz[0][2], xz[1][2], xz[3][2], xz[4][2],
-                              xz[1][3], xz[2][3], xz[3][3], xz[2][4], yz[0][1], yz[1][1], yz[3][1], yz[4][1],
-                              yz[1][0], yz[3][0], yz[1][2], yz[3][2]};
-        for (int k = 0; k < 20; ++k) {
-            const double e = -v[k] * rh;
-            A[k + 1] = e;
-            offsum += e;
-        }
+#pragma unroll
+    for (int k = 0; k < 21; ++k) {
+        if (k == kDiag21) continue;
+        A[k] = -v[k] * rh;
+        offsum += A[k];
     }
     const double react = -rct * rh;
-    A[0] = -offsum + react;
+    A[kDiag21] = -offsum + react;
     const int n = g.nint;
+#pragma unroll
     for (int k = 0; k < 21; ++k) a.cA[(size_t)k * n + q] = A[k];
     a.react[q] = react;
-    // ExplicitPart of c_old (the ghosts set from this stage's rows)
-    const double xr = cold[r3];
-    double s = react * xr;
-    for (int k = 1; k < 21; ++k) s += A[k] * (cold[I3(g, ix + c_cox[k], jy + c_coy[k], kz + c_coz[k])] - xr);
-    const double rhs = -a.dt * (s + 0.0);
+    const double rhs = -a.dt * balance_form(g, A, react, cold, ix, jy, kz);
     if (rhs != 0.0) atomicOr(&a.flags[0], 1);
     const double th = a.theta, dt = a.dt;
-    const double d = 1.0 + th * dt * A[0];
+    const double d = 1.0 + th * dt * A[kDiag21];
     const bool sc = !(fabs(d) < 1.0e-30);
     const double inv = sc ? 1.0 / d : 1.0;
+#pragma unroll
     for (int k = 0; k < 21; ++k) {
-        const double v = (k == 0) ? d : th * dt * A[k];
-        a.cP[(size_t)k * n + q] = sc ? v * inv : v;
+        const double val = (k == kDiag21) ? d : th * dt * A[k];
+        a.cP[(size_t)k * n + q] = sc ? val * inv : val;
     }
     a.cb[q] = sc ? rhs * inv : rhs;
 }
@@ -841,9 +787,9 @@ __device__ inline int fold_in(int j, int n, int per) {
     return 2 * n - 1 - j;
 }
 __device__ inline int col3(const Geo3 &g, int i, int j, int k, int s) {
-    const int ii = fold_in(i + c_cox[s], g.nx, g.perx);
-    const int jj = fold_in(j + c_coy[s], g.ny, g.pery);
-    const int kk = fold_in(k + c_coz[s], g.nz, 0);
+    const int ii = fold_in(i + c_s21x[s], g.nx, g.perx);
+    const int jj = fold_in(j + c_s21y[s], g.ny, g.pery);
+    const int kk = fold_in(k + c_s21z[s], g.nz, 0);
     return (kk * g.ny + jj) * g.nx + ii;
 }
 // the 21-slot product of row q: plain offsets where no slot reaches an edge
@@ -896,13 +842,11 @@ __global__ void k3_lu(Geo3 g, const double *P, double *L1, double *L2, double *U
     auto band = [&](int l, double r[5]) {
         for (int o = 0; o < 5; ++o) r[o] = 0.0;
         const int q = l * nc + c;
-        const int sl[5] = {1, 3, 0, 10, 12};
-        const int dzs[5] = {-2, -1, 0, 1, 2};
-        for (int m = 0; m < 5; ++m) {
-            int t = l + dzs[m];
+        for (int m = 0; m < 5; ++m) {      // slots 0..4: levels l-2 .. l+2
+            int t = l + m - 2;
             if (t < 0) t = -1 - t;
             if (t >= nz) t = 2 * nz - 1 - t;
-            r[t - l + 2] += P[(size_t)sl[m] * n + q];
+            r[t - l + 2] += P[(size_t)m * n + q];
         }
     };
     double r0[5], r1[5], r2[5];
@@ -1054,10 +998,10 @@ __global__ void k_csolute(Args a, const double *ct, const double *d, double *par
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const int ix = q % g.nx + 2, jy = (q / g.nx) % g.ny + 2, kz = q / (g.nx * g.ny) + 2;
         const int r3 = I3(g, ix, jy, kz);
-        const double xr = ct[r3];
-        double s = a.react[q] * xr;
-        for (int k = 1; k < 21; ++k)
-            s += a.cA[(size_t)k * n + q] * (ct[I3(g, ix + c_cox[k], jy + c_coy[k], kz + c_coz[k])] - xr);
+        double A[21];
+#pragma unroll
+        for (int k = 0; k < 21; ++k) A[k] = a.cA[(size_t)k * n + q];
+        const double s = balance_form(g, A, a.react[q], ct, ix, jy, kz);
         const double h = a.fH[I2(g, ix, jy)];
         acc[0] += h * (-a.dt * (s + 0.0) - d[r3]);
         acc[1] += h * (1.0 + a.theta * a.dt * a.react[q]);
@@ -1758,9 +1702,7 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
     for (int k = 0; k < 4; ++k) g.wall[k] = c->bc[k] != YZ_BC_PERIODIC;
     g.perx = !g.wall[0]; g.pery = !g.wall[2];
     {
-        const int cox[21] = {0, 0, -1, 0, 1, -2, -1, 1, 2, -1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        const int coy[21] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, -1, 1, -1, 1};
-        const int coz[21] = {0, -2, -1, -1, -1, 0, 0, 0, 0, 1, 1, 1, 2, 0, 0, 0, 0, -1, -1, 1, 1};
+        const int cox[21] = YZ_S21X, coy[21] = YZ_S21Y, coz[21] = YZ_S21Z;
         for (int t = 0; t < 21; ++t) g.off[t] = (coz[t] * g.ny + coy[t]) * g.nx + cox[t];
     }
     bool ok = hipStreamCreateWithFlags(&z->st, hipStreamNonBlocking) == hipSuccess;
@@ -1924,7 +1866,7 @@ int yz_cz_load_state(yz_cz *z, const char *path) {
         yz::set_error("ConcZ: state vectors missing or of another grid");
         return 2;
     }
-    const bool prev = st.scalars.count("conc.previousDt") && st.vec("conc.previous").size() == n3;
+    const bool prev = st.has("conc.previousDt") && st.vec("conc.previous").size() == n3;
     bool ok = hipStreamSynchronize(z->st) == hipSuccess;
     for (double *d : {z->hN, z->fH, z->tH})
         ok = ok && hipMemcpy(d, st.vec("field.h").data(), n2 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;

@@ -1,141 +1,192 @@
-// yz_state.h -- the restart state file of a run (field*.state), in the
-// reference's layout (yuzen/src/IO/RestartState.h:24-49, RestartState.cpp
-// :43-146): a magic line, text lines "key v1 v2 ..." of scalars (17
-// significant digits, which a double survives exactly; keys in sorted
-// order), "vector name n" lines announcing the arrays in the order they were
-// set, "text key word" lines, a line "binary", then the arrays as raw
-// little-endian doubles. Unknown keys are ignored by a reader, so a file can
-// carry more than the reference's keys.
+// yz_state.h -- the restart state file of a run (field*.state).
 //
-// Host code only.
+// Format (what the reference's reader and writer agree on,
+// yuzen/src/IO/RestartState.h:24-49; the byte layout below is what its
+// writer produces for the same contents):
+//
+//   header   ASCII lines ending in '\n':
+//     1. the magic line "# Yuzen restart state 1"
+//     2. one line per scalar key, keys in ascending byte order:
+//          key SP v1 SP v2 ...   each value as printf "%.17g"
+//        (17 significant digits: a double reads back to the same bits)
+//     3. one line per array, in the order the arrays were first stored:
+//          "vector" SP name SP count
+//     4. one line per text key, keys in ascending byte order:
+//          "text" SP key SP word
+//     5. the line "binary"
+//   payload  the arrays' values as raw little-endian IEEE doubles, array
+//            after array in the order of their "vector" lines, nothing
+//            between or after them.
+//
+// A reader skips blank lines, takes every other header line by its first
+// word, keeps keys it does not know (the caller ignores them), and refuses a
+// file without the magic line or the "binary" line, with a scalar value that
+// is not a whole number token, or whose payload is shorter than its
+// "vector" lines announce.
+//
+// Host code only. The file is assembled in memory and written in one piece
+// to path + ".tmp", which is then renamed over path.
 #pragma once
-#include <cerrno>
 #include <cstdio>
 #include <cstdlib>
-#include <fstream>
+#include <cstring>
 #include <map>
-#include <sstream>
 #include <string>
-#include <utility>
 #include <vector>
 
 namespace yz {
 
-struct StateFile {
-    std::map<std::string, std::vector<double>> scalars, vectors;
-    std::vector<std::string> order;
-    std::map<std::string, std::string> texts;
-
-    void set(const std::string &k, double v) { scalars[k] = std::vector<double>(1, v); }
-    void set(const std::string &k, const std::vector<double> &v) { scalars[k] = v; }
+class StateFile {
+public:
+    void set(const std::string &k, double v) { scal_[k] = std::vector<double>(1, v); }
+    void set(const std::string &k, const std::vector<double> &v) { scal_[k] = v; }
     void set_vector(const std::string &k, const std::vector<double> &v) {
-        if (!vectors.count(k)) order.push_back(k);
-        vectors[k] = v;
+        for (auto &a : arrays_)
+            if (a.name == k) { a.values = v; return; }
+        arrays_.push_back(Array{k, v});
     }
-    void set_text(const std::string &k, const std::string &v) { texts[k] = v; }
+    void set_text(const std::string &k, const std::string &v) { text_[k] = v; }
+
     double get(const std::string &k, size_t i = 0, double dflt = 0) const {
-        auto it = scalars.find(k);
-        if (it == scalars.end() || it->second.size() <= i) return dflt;
-        return it->second[i];
+        const auto it = scal_.find(k);
+        return (it != scal_.end() && i < it->second.size()) ? it->second[i] : dflt;
     }
-    bool has_vector(const std::string &k) const { return vectors.count(k) != 0; }
+    bool has(const std::string &k) const { return scal_.count(k) != 0; }
+    bool has_vector(const std::string &k) const { return find(k) != nullptr; }
     const std::vector<double> &vec(const std::string &k) const {
-        static const std::vector<double> empty;
-        auto it = vectors.find(k);
-        return it == vectors.end() ? empty : it->second;
+        static const std::vector<double> none;
+        const Array *a = find(k);
+        return a ? a->values : none;
     }
     std::string text(const std::string &k) const {
-        auto it = texts.find(k);
-        return it == texts.end() ? std::string() : it->second;
+        const auto it = text_.find(k);
+        return it == text_.end() ? std::string() : it->second;
     }
 
-    static const char *magic() { return "# Yuzen restart state"; }
-
-    // to path + ".tmp", then renamed over path
     bool write(const std::string &path) const {
-        const std::string tmp = path + ".tmp";
-        {
-            std::ofstream f(tmp.c_str(), std::ios::binary);
-            if (!f.good()) return false;
-            f << magic() << " 1\n";
-            char buf[64];
-            for (const auto &kv : scalars) {
-                f << kv.first;
-                for (double v : kv.second) {
-                    std::snprintf(buf, sizeof(buf), " %.17g", v);
-                    f << buf;
-                }
-                f << "\n";
+        std::string out = kMagic;
+        out += " 1\n";
+        char num[40];
+        for (const auto &kv : scal_) {
+            out += kv.first;
+            for (double v : kv.second) {
+                std::snprintf(num, sizeof(num), " %.17g", v);
+                out += num;
             }
-            for (const auto &k : order) f << "vector " << k << " " << vectors.find(k)->second.size() << "\n";
-            for (const auto &kv : texts) f << "text " << kv.first << " " << kv.second << "\n";
-            f << "binary\n";
-            for (const auto &k : order) {
-                const std::vector<double> &v = vectors.find(k)->second;
-                if (!v.empty()) f.write((const char *)v.data(), (std::streamsize)(v.size() * sizeof(double)));
-            }
-            f.close();
-            if (f.fail()) { std::remove(tmp.c_str()); return false; }
+            out += '\n';
         }
-        if (std::rename(tmp.c_str(), path.c_str()) != 0) { std::remove(tmp.c_str()); return false; }
+        for (const Array &a : arrays_) {
+            std::snprintf(num, sizeof(num), " %zu\n", a.values.size());
+            out += "vector " + a.name + num;
+        }
+        for (const auto &kv : text_) out += "text " + kv.first + " " + kv.second + "\n";
+        out += "binary\n";
+        for (const Array &a : arrays_)
+            out.append(reinterpret_cast<const char *>(a.values.data()), a.values.size() * sizeof(double));
+
+        const std::string tmp = path + ".tmp";
+        FILE *f = std::fopen(tmp.c_str(), "wb");
+        if (!f) return false;
+        const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+        if (std::fclose(f) != 0 || !ok || std::rename(tmp.c_str(), path.c_str()) != 0) {
+            std::remove(tmp.c_str());
+            return false;
+        }
         return true;
     }
 
     // false when the file is missing, not a state file, or cut short
     bool read(const std::string &path) {
-        scalars.clear(); vectors.clear(); order.clear(); texts.clear();
-        std::ifstream f(path.c_str(), std::ios::binary);
-        if (!f.good()) return false;
+        scal_.clear(); arrays_.clear(); text_.clear();
+        std::vector<char> buf;
+        if (!slurp(path, buf)) return false;
+        const char *p = buf.data(), *end = p + buf.size();
         std::string line;
-        const std::string m = magic();
-        if (!std::getline(f, line) || line.compare(0, m.size(), m) != 0) return false;
-        std::vector<std::pair<std::string, size_t>> arrays;
-        bool binary = false;
-        while (std::getline(f, line)) {
-            if (line == "binary") { binary = true; break; }
-            std::istringstream is(line);
-            std::string key;
-            if (!(is >> key)) continue;
-            if (key == "text") {
-                std::string name, val;
-                is >> name >> val;
-                texts[name] = val;
-                continue;
+        if (!next_line(p, end, line) || line.compare(0, std::strlen(kMagic), kMagic) != 0) return false;
+        std::vector<std::pair<std::string, size_t>> announced;
+        bool payload = false;
+        while (next_line(p, end, line)) {
+            if (line == "binary") { payload = true; break; }
+            std::vector<std::string> w = words(line);
+            if (w.empty()) continue;
+            if (w[0] == "text") {
+                text_[w.size() > 1 ? w[1] : std::string()] = w.size() > 2 ? w[2] : std::string();
+            } else if (w[0] == "vector") {
+                if (w.size() < 3) return false;
+                char *e = nullptr;
+                const unsigned long long n = std::strtoull(w[2].c_str(), &e, 10);
+                if (e == w[2].c_str() || *e != '\0') return false;
+                announced.emplace_back(w[1], (size_t)n);
+            } else {
+                std::vector<double> v;
+                for (size_t i = 1; i < w.size(); ++i) {
+                    char *e = nullptr;
+                    v.push_back(std::strtod(w[i].c_str(), &e));
+                    if (e == w[i].c_str() || *e != '\0') return false;
+                }
+                scal_[w[0]] = v;
             }
-            if (key == "vector") {
-                std::string name;
-                size_t n = 0;
-                if (!(is >> name >> n)) return false;
-                arrays.push_back(std::make_pair(name, n));
-                continue;
-            }
-            std::vector<double> v;
-            std::string tok;
-            while (is >> tok) {
-                char *end = nullptr;
-                v.push_back(std::strtod(tok.c_str(), &end));
-                if (end == tok.c_str() || *end != '\0') return false;
-            }
-            scalars[key] = v;
         }
-        if (!binary) return false;
-        const std::streampos here = f.tellg();
-        f.seekg(0, std::ios::end);
-        const std::streampos end = f.tellg();
-        f.seekg(here);
-        if (here < 0 || end < here) return false;
-        size_t left = (size_t)(end - here);
-        for (const auto &a : arrays) {
-            if (a.second > left / sizeof(double)) return false;
-            left -= a.second * sizeof(double);
+        if (!payload) return false;
+        size_t avail = (size_t)(end - p) / sizeof(double);
+        for (const auto &a : announced) {
+            if (a.second > avail) return false;
+            avail -= a.second;
         }
-        for (const auto &a : arrays) {
+        for (const auto &a : announced) {
             std::vector<double> v(a.second);
-            if (!v.empty()) f.read((char *)v.data(), (std::streamsize)(v.size() * sizeof(double)));
-            if (!f) return false;
+            if (a.second) std::memcpy(v.data(), p, a.second * sizeof(double));
+            p += a.second * sizeof(double);
             set_vector(a.first, v);
         }
         return true;
+    }
+
+private:
+    static constexpr const char *kMagic = "# Yuzen restart state";   // then " <version>"
+    struct Array {
+        std::string name;
+        std::vector<double> values;
+    };
+    std::map<std::string, std::vector<double>> scal_;   // ascending key order
+    std::vector<Array> arrays_;                           // order first stored
+    std::map<std::string, std::string> text_;
+
+    const Array *find(const std::string &k) const {
+        for (const Array &a : arrays_)
+            if (a.name == k) return &a;
+        return nullptr;
+    }
+    static bool slurp(const std::string &path, std::vector<char> &buf) {
+        FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f) return false;
+        char chunk[1 << 16];
+        size_t got;
+        while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+        const bool ok = !std::ferror(f);
+        std::fclose(f);
+        return ok;
+    }
+    // the next '\n'-terminated line (without it); false at the end
+    static bool next_line(const char *&p, const char *end, std::string &line) {
+        if (p >= end) return false;
+        const char *q = static_cast<const char *>(std::memchr(p, '\n', (size_t)(end - p)));
+        const char *stop = q ? q : end;
+        line.assign(p, stop);
+        p = q ? q + 1 : end;
+        return true;
+    }
+    static std::vector<std::string> words(const std::string &s) {
+        std::vector<std::string> w;
+        size_t i = 0;
+        while (i < s.size()) {
+            while (i < s.size() && (s[i] == ' ' || s[i] == '\t' || s[i] == '\r')) ++i;
+            size_t j = i;
+            while (j < s.size() && s[j] != ' ' && s[j] != '\t' && s[j] != '\r') ++j;
+            if (j > i) w.emplace_back(s, i, j - i);
+            i = j;
+        }
+        return w;
     }
 };
 

@@ -237,6 +237,13 @@ struct yz_step {
     double nk_first = 0, nk_prev = 0;
     int nk_pass = 0;
     long long lin_step = 0;
+    // the Newton passes of the last attempt: correction (the larger of the
+    // film's and the coupled side's), the solve's tolerance and the
+    // relative residual it reached; the stagnation cuts of the run
+    // (_stagnationCuts, NewtonKantorovich.cpp:357)
+    std::vector<double> plog_corr, plog_eta, plog_lres;
+    double last_eps = 0;
+    long long stag_cuts = 0;
     hipEvent_t ev[12];
     double tms[6];
     // the well-mixed concentration (conc_model 1): the field's c, the
@@ -531,6 +538,7 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     if (f32) S->S32.unit_diag = S->hflag[0] == 0;
     S->tms[0] += ev_ms(S->ev[0], S->ev[1]);
     const double eps = ew_eps(S);
+    S->last_eps = eps;
     YZ_TRY(hipEventRecord(S->ev[2], S->st));
     int ch = -1;
     if (f32) {
@@ -765,6 +773,7 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
     S->nk_pass = 0;
     S->lin_step = 0;
     S->extrap_move = 0;
+    S->plog_corr.clear(); S->plog_eta.clear(); S->plog_lres.clear();
     bool guessed = false;
     if (c.extrap && !S->extrap_blocked && !S->first_step) {
         const double r = (S->dt_old > 0) ? S->dt_now / S->dt_old : 0.0;
@@ -813,6 +822,7 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
         if (sync_read(S)) return 2;
         resid_eq = bits_to_double(S->hmax[1]);
         if (S->conc && S->coupled > resid_eq) resid_eq = S->coupled;
+        S->plog_corr.push_back(resid_eq); S->plog_eta.push_back(S->last_eps); S->plog_lres.push_back(lr);
         if (std::getenv("YZ_DEBUG"))
             std::fprintf(stderr, "[hip] pass %d lin %d resid_lin %.3e corr %.3e rc %d\n", outer, it, lr, resid_eq, ch);
         if (ch != 0) { done = false; break; }
@@ -827,6 +837,7 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
             if (sync_read(S)) return 2;
             const double floorPhi = 1000.0 * std::numeric_limits<double>::epsilon() * bits_to_double(S->hmax[3]);
             done = (resid_eq < c.nk_eps) || (resid_first > 0 && resid_eq < resid_first * 1.0e-4) || (resid_eq <= floorPhi);
+            if (!done) S->stag_cuts++;
             break;
         }
         const double alpha = (resid_eq > resid_eq_old) ? 0.5 : 1.0;
@@ -1081,9 +1092,11 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
     int it = 0;
     double r = 0;
     do {
+        const long long cuts0 = S->stag_cuts;
         if (nk_loop(S, &it, &r, &ok)) return 2;
         lin_sum += S->lin_step;
         if (!ok && S->extrap_move > 0) {
+            S->stag_cuts = cuts0;
             S->extrap_blocked = true;
             const int e = nk_loop(S, &it, &r, &ok);
             S->extrap_blocked = false;
@@ -1452,6 +1465,18 @@ int yz_step_set_h(yz_step *S, const double *h) {
     hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, S->hnew);
     YZ_TRY(hipStreamSynchronize(S->st));
     return 0;
+}
+
+int yz_step_pass_log(yz_step *S, double *corr, double *eta, double *lin_resid, int cap, long long *stag_cuts) {
+    if (!S) { set_err("null step"); return -1; }
+    const int m = (int)S->plog_corr.size();
+    for (int i = 0; i < m && i < cap; ++i) {
+        if (corr) corr[i] = S->plog_corr[i];
+        if (eta) eta[i] = S->plog_eta[i];
+        if (lin_resid) lin_resid[i] = S->plog_lres[i];
+    }
+    if (stag_cuts) *stag_cuts = S->stag_cuts;
+    return m;
 }
 
 int yz_step_clock(yz_step *S, double *t, double *dt_now, double *dt_old) {

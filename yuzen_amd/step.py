@@ -138,6 +138,18 @@ class HipStep:
         self.lib.yz_step_clock(self.p, C.byref(t), C.byref(a), C.byref(b))
         return t.value, a.value, b.value
 
+    def pass_log(self):
+        """the passes of the last step's accepted attempt: (correction,
+        linear tolerance, linear relative residual) arrays"""
+        a, b, r = np.zeros(256), np.zeros(256), np.zeros(256)
+        m = self.lib.yz_step_pass_log(self.p, _pd(a), _pd(b), _pd(r), 256, None)
+        return a[:m].copy(), b[:m].copy(), r[:m].copy()
+
+    def stagnation_cuts(self):
+        n = C.c_longlong()
+        self.lib.yz_step_pass_log(self.p, None, None, None, 0, C.byref(n))
+        return n.value
+
     def timing(self):
         out = np.zeros(6)
         self.lib.yz_step_timing(self.p, _pd(out))
