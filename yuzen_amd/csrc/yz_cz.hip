@@ -129,6 +129,14 @@ __device__ inline double visc(const yz_cz_case &c, double x) { return 1.0 * exp(
 __device__ inline double diffu(const yz_cz_case &c, double x) { return 1.0 * exp(c.diff_exp * (x - c.diff_c0)); }
 __device__ inline double sigm(const yz_cz_case &c, double x) { return 1.0 + c.Ma * (x - c.sig_c0); }
 __device__ inline double sgn(double v) { return v >= 0.0 ? 1.0 : -1.0; }
+// fourth-order first derivatives: at the face between f0 and f1 (from
+// f_-1 .. f2), and at the centre of a cell (from its +-1, +-2 neighbours)
+__device__ inline double dface4(double fm1, double f0, double f1, double f2, double ds) {
+    return (fm1 - 27.0 * f0 + 27.0 * f1 - f2) / (24.0 * ds);
+}
+__device__ inline double dcentre4(double fm2, double fm1, double f1, double f2, double ds) {
+    return (fm2 - 8.0 * fm1 + 8.0 * f1 - f2) / (12.0 * ds);
+}
 
 // Evaluate (Evaporation.cpp:62-103, 148-220), uniform in space
 __device__ inline double evap_eval(const yz_cz_case &c, double hh, double cc) {
@@ -416,102 +424,169 @@ __global__ void k_hmid(Args a, double *mid, int with_rate) {
 }
 
 // ---------------------------------------------------------------------
-// the velocity through the film at one face (CalVelocity, Simpson's rule
-// up the column) and the layer volume fluxes of that face
-// (BuildVolumeFluxes: the layer means, and the difference from the
-// film's flux spread evenly through the depth). One thread per face.
+// The velocity through the film at the faces and the layer volume fluxes
+// (CalVelocity, CZ_HeightEquation_2D.cpp:1381-1530; BuildVolumeFluxes,
+// CZ_ConcEquation_2D.cpp:144-263). In the lubrication limit the velocity
+// along a face normal at height z above the substrate is
+//   u(z) = G int_s^z (s + h - z') / mu(z') dz' + tau int_s^z dz' / mu(z'),
+// G the driving gradient of the face (capillary pressure, gravity along
+// the slope, in-plane gravity, spin) and tau the Marangoni shear at the
+// surface; the integrals run by Simpson's rule between the levels' centres
+// (the first from the substrate, half a level long), mu at the centres and
+// at the interfaces from the level values of c averaged over the face's two
+// cells (at the substrate, over the first level and its ghost). A layer's
+// volume flux is the face film height times the mean of u over the layer
+// (the quadratic through the neighbouring centres; u = 0 at the
+// substrate), shifted evenly through the depth so that the layers add up to
+// the film's own flux through the face.
+//
+// One block works on a tile of TW consecutive faces of one row and one
+// direction; the layout of its LDS is [node][face]:
+//   phase 1  all threads: 1/mu at the 2 nz + 3 nodes of every face of the
+//            tile (the exponentials spread over the whole block);
+//   phase 2  one thread per face: the running Simpson sums up the column,
+//            u at the nz + 1 centres (the last one the ghost above the
+//            surface);
+//   phase 3  all threads: the layer means, into the LDS the nodes left;
+//   phase 4  one thread per face: the layers' total, in level order;
+//   phase 5  all threads: Q of every layer with the shift, coalesced along
+//            the row.
 // ---------------------------------------------------------------------
-__device__ void face_column(const Args &a, int ix, int jy, int dir) {
+constexpr int kVelThreads = 256;
+// the widest tile (64, 32, 16 or 8 faces) whose LDS fits in 64 KiB
+inline int vel_tile(int nz) {
+    for (int tw = 64; tw > 8; tw >>= 1)
+        if ((size_t)(3 * nz + 4) * tw * sizeof(double) <= 65536) return tw;
+    return 8;
+}
+inline size_t vel_lds(int nz, int tw) { return (size_t)(3 * nz + 4) * tw * sizeof(double) + 8 * tw * sizeof(double); }
+
+__global__ void __launch_bounds__(kVelThreads) k_face_velocity(Args a, int TW) {
+    extern __shared__ double lds[];
     const Geo3 &g = a.g;
     const yz_cz_case &cs = a.cs;
-    const int ax = dir == 0, ay = dir == 1;
-    const int bx = ay, by = ax;
-    auto P2 = [&](int k) { return I2(g, ix + k * ax, jy + k * ay); };
-    auto C = [&](int k, int kz) { return a.fC[kz * g.n2 + P2(k)]; };
-    const double ds = dir == 0 ? g.dx : g.dy;
-    const double dn = dir == 0 ? g.dy : g.dx;
-    const double ds3 = pow(ds, 3.0);
-    const double hpo = a.fH[P2(1)], hoo = a.fH[P2(0)], hmo = a.fH[P2(-1)], hMo = a.fH[P2(-2)];
-    const double spo = a.S[P2(1)], soo = a.S[P2(0)], smo = a.S[P2(-1)], sMo = a.S[P2(-2)];
-    const double hL = 0.5 * (hmo + hoo), sL = 0.5 * (smo + soo);
-    const double xL = dir == 0 ? (g.xlo + ix * g.dx) - cs.rot_x : (g.ylo + jy * g.dy) - cs.rot_y;
-    const bool w = wall_face(a, ix, jy, dir);
-    double *Q = dir == 0 ? a.Qx : a.Qy;
-    const double qfilm = dir == 0 ? a.qX[P2(0)] : a.qY[P2(0)];
-    const double hf = 0.5 * (a.fH[P2(-1)] + a.fH[P2(0)]);
-    double Cooo = C(0, g.Nz - 2), Cmoo = C(-1, g.Nz - 2), Coom = C(0, g.Nz - 3), Cmom = C(-1, g.Nz - 3);
-    const double dcdx_suf = 0.5 * ((-C(1, g.Nz - 2) + 27.0 * Cooo - 27.0 * Cmoo + C(-2, g.Nz - 2))
-                                   + (-C(1, g.Nz - 3) + 27.0 * Coom - 27.0 * Cmom + C(-2, g.Nz - 3))) / (24.0 * ds);
-    const double dx3hs = ((hpo + spo) - 3.0 * (hoo + soo) + 3.0 * (hmo + smo) - (hMo + sMo)) / ds3;
-    const double dxhs = (-(hpo + spo) + 27.0 * (hoo + soo) - 27.0 * (hmo + smo) + (hMo + sMo)) / (24.0 * ds);
-    const double sig = sigm(cs, 0.25 * (Cooo + Coom + Cmoo + Cmom));
-    Cooo = C(0, 2); Cmoo = C(-1, 2); Coom = C(0, 1); Cmom = C(-1, 1);
-    const double muB = visc(cs, 0.25 * (Cooo + Coom + Cmoo + Cmom));
-    double fB1 = 1.0 / muB, fB2 = sL / muB;
-    const double dzz = g.dz * hL;
-    double sum1 = 0, sum2 = 0;
-    auto HS = [&](int s, int t) {
-        const int p = I2(g, ix + s * ax + t * bx, jy + s * ay + t * by);
-        return a.fH[p] + a.S[p];
-    };
-    const double dn2 = dn * dn;
-    const double dxyy = ((HS(0, 1) - 2.0 * (hoo + soo) + HS(0, -1)) - (HS(-1, 1) - 2.0 * (hmo + smo) + HS(-1, -1))) / (ds * dn2);
-    const double kLap = cs.capillary ? sig * (dx3hs + dxyy) : 0.0;
-    const double kMa = cs.marangoni ? cs.As * cs.Ma * dcdx_suf : 0.0;
-    const double kBo = cs.vertical_gravity ? -cs.Bo * dxhs : 0.0;
-    const double kPg = cs.inplane_gravity ? (dir == 0 ? cs.Pgx : cs.Pgy) : 0.0;
-    const double kSp = cs.centrifugal ? cs.Sp * xL : 0.0;
-    const double hLsL = hL + sL;
-    // the profile at levels 2 .. Nz-2; the layer mean of level k needs
-    // u[k-1], u[k], u[k+1]
-    double um1 = 0.0, u0 = 0.0;
-    double sum = 0.0;
-    const int base = P2(0);
-    for (int kz = 2; kz <= g.Nz - 2; kz++) {
-        const double zT = sL + dzz * (-1.5 + (double)kz);
-        const double zM = (kz == 2) ? sL + 0.25 * dzz : sL + dzz * (-2.0 + (double)kz);
-        if (kz > 2) {
-            Coom = Cooo; Cmom = Cmoo;
-            Cooo = C(0, kz); Cmoo = C(-1, kz);
+    const int nz = g.nz, nodes = 2 * nz + 3;
+    const int dir = blockIdx.z, ax = dir == 0, ay = dir == 1;
+    double *node = lds;                               // [nodes][TW]: 1/mu; later the layer fluxes
+    double *uc = lds + (size_t)nodes * TW;            // [nz + 1][TW]: u at the centres
+    double *fp = uc + (size_t)(nz + 1) * TW;          // [8][TW]: per-face scalars
+    double *f_drive = fp, *f_shear = fp + TW, *f_top = fp + 2 * TW, *f_sub = fp + 3 * TW;
+    double *f_dzz = fp + 4 * TW, *f_hf = fp + 5 * TW, *f_shift = fp + 6 * TW, *f_ok = fp + 7 * TW;
+    const int jy = 2 + blockIdx.y;
+    const int ix0 = 2 + blockIdx.x * TW;
+    // the faces of this direction: x faces ix in [2, Nx-2] on rows [2, Ny-3],
+    // y faces ix in [2, Nx-3] on rows [2, Ny-2]
+    const int ixmax = dir == 0 ? g.Nx - 2 : g.Nx - 3, jymax = dir == 0 ? g.Ny - 3 : g.Ny - 2;
+    if (jy > jymax) return;
+    const int t = threadIdx.x;
+    auto cell = [&](int f, int k) { return I2(g, ix0 + f + k * ax, jy + k * ay); };   // k along the normal
+    auto cval = [&](int f, int k, int kz) { return a.fC[kz * g.n2 + cell(f, k)]; };
+
+    // phase 0: the face's own scalars
+    if (t < TW) {
+        const int ix = ix0 + t;
+        const bool ok = ix <= ixmax;
+        f_ok[t] = ok ? 1.0 : 0.0;
+        if (ok) {
+            const double ds = dir == 0 ? g.dx : g.dy, dn = dir == 0 ? g.dy : g.dx;
+            const int tx = ay, ty = ax;                   // the tangent
+            auto hs = [&](int k, int m) {
+                const int p = I2(g, ix + k * ax + m * tx, jy + k * ay + m * ty);
+                return a.fH[p] + a.S[p];
+            };
+            const double hlo = a.fH[cell(t, -1)], hhi = a.fH[cell(t, 0)];
+            const double slo = a.S[cell(t, -1)], shi = a.S[cell(t, 0)];
+            const double hface = 0.5 * (hlo + hhi), sface = 0.5 * (slo + shi);
+            const int top = g.Nz - 2, below = g.Nz - 3;
+            double drive = 0.0, shear = 0.0;
+            if (cs.capillary) {
+                const double sig = sigm(cs, 0.25 * (cval(t, 0, top) + cval(t, 0, below) + cval(t, -1, top) + cval(t, -1, below)));
+                const double d3n = (hs(1, 0) - 3.0 * hs(0, 0) + 3.0 * hs(-1, 0) - hs(-2, 0)) / (ds * ds * ds);
+                const double dnt2 = ((hs(0, 1) - 2.0 * hs(0, 0) + hs(0, -1)) - (hs(-1, 1) - 2.0 * hs(-1, 0) + hs(-1, -1))) /
+                                    (ds * dn * dn);
+                drive += sig * (d3n + dnt2);
+            }
+            if (cs.vertical_gravity) drive -= cs.Bo * dface4(hs(-2, 0), hs(-1, 0), hs(0, 0), hs(1, 0), ds);
+            if (cs.inplane_gravity) drive += dir == 0 ? cs.Pgx : cs.Pgy;
+            if (cs.centrifugal)
+                drive += cs.Sp * (dir == 0 ? (g.xlo + ix * g.dx) - cs.rot_x : (g.ylo + jy * g.dy) - cs.rot_y);
+            if (cs.marangoni) {
+                const double gt = dface4(cval(t, -2, top), cval(t, -1, top), cval(t, 0, top), cval(t, 1, top), ds);
+                const double gb = dface4(cval(t, -2, below), cval(t, -1, below), cval(t, 0, below), cval(t, 1, below), ds);
+                shear = cs.As * cs.Ma * 0.5 * (gt + gb);
+            }
+            const bool wall = wall_face(a, ix, jy, dir);
+            f_drive[t] = wall ? 0.0 : drive;
+            f_shear[t] = wall ? 0.0 : shear;
+            f_top[t] = hface + sface;
+            f_sub[t] = sface;
+            f_dzz[t] = g.dz * hface;
+            f_hf[t] = hface;
+            f_shift[t] = dir == 0 ? a.qX[cell(t, 0)] : a.qY[cell(t, 0)];
         }
-        const double muM = (kz == 2) ? visc(cs, 0.375 * (Cooo + Cmoo) + 0.125 * (Coom + Cmom))
-                                     : visc(cs, 0.25 * (Cooo + Coom + Cmoo + Cmom));
-        const double muT = visc(cs, 0.50 * (Cooo + Cmoo));
-        const double fT1 = 1.0 / muT, fT2 = zT / muT, fM1 = 1.0 / muM, fM2 = zM / muM;
-        const double dzt = (kz == 2) ? 0.5 * dzz : dzz;
-        sum1 += (fB1 + fT1 + 4.0 * fM1) * dzt / 6.0;
-        sum2 += (fB2 + fT2 + 4.0 * fM2) * dzt / 6.0;
-        fB1 = fT1; fB2 = fT2;
-        const double PP = hLsL * sum1 - sum2;
-        double uu = 0;
-        if (cs.capillary) uu += kLap * PP;
-        if (cs.marangoni) uu += kMa * sum1;
-        if (cs.vertical_gravity) uu += kBo * PP;
-        if (cs.inplane_gravity) uu += kPg * PP;
-        if (cs.centrifugal) uu += kSp * PP;
-        if (w) uu = 0.0;
-        if (kz >= 3) {
-            const int k = kz - 1;
-            double um;
-            if (k == 2) um = u0 + (uu - 3.0 * u0) / 18.0;
-            else um = u0 + (uu - 2.0 * u0 + um1) / 24.0;
-            const double qv = hf * um;
-            Q[k * g.n2 + base] = qv;
-            sum += qv * g.dz;
-        }
-        um1 = u0;
-        u0 = uu;
     }
-    const double d = qfilm - sum;
-    for (int kz = 2; kz < g.Nz - 2; kz++) Q[kz * g.n2 + base] += d;
-}
-__global__ void k_faces3(Args a) {
-    const Geo3 &g = a.g;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.n2) return;
-    const int ix = p % g.Nx, jy = p / g.Nx;
-    if (ix >= 2 && ix <= g.Nx - 2 && jy >= 2 && jy < g.Ny - 2) face_column(a, ix, jy, 0);
-    if (ix >= 2 && ix < g.Nx - 2 && jy >= 2 && jy <= g.Ny - 2) face_column(a, ix, jy, 1);
+    // phase 1: 1/mu at the nodes: 0 the substrate, 2j + 1 the lower end's
+    // midpoint of interval j, 2j + 2 the centre of level j + 2
+    for (int e = t; e < nodes * TW; e += kVelThreads) {
+        const int m = e / TW, f = e % TW;
+        if (ix0 + f > ixmax) continue;
+        double c;
+        if (m == 0) {
+            c = 0.25 * (cval(f, 0, 2) + cval(f, 0, 1) + cval(f, -1, 2) + cval(f, -1, 1));
+        } else if (m == 1) {
+            c = 0.375 * (cval(f, 0, 2) + cval(f, -1, 2)) + 0.125 * (cval(f, 0, 1) + cval(f, -1, 1));
+        } else if (m % 2 == 0) {
+            const int kz = m / 2 + 1;
+            c = 0.5 * (cval(f, 0, kz) + cval(f, -1, kz));
+        } else {
+            const int kz = (m + 1) / 2 + 1;
+            c = 0.25 * (cval(f, 0, kz) + cval(f, 0, kz - 1) + cval(f, -1, kz) + cval(f, -1, kz - 1));
+        }
+        node[e] = 1.0 / visc(cs, c);
+    }
+    __syncthreads();
+    // phase 2: u at the centres, from the running Simpson sums
+    if (t < TW && f_ok[t] != 0.0) {
+        const double dzz = f_dzz[t], zs = f_sub[t], ztop = f_top[t], G = f_drive[t], tau = f_shear[t];
+        double i1 = 0.0, iz = 0.0;                 // int dz/mu, int z dz/mu
+        double zlo = zs, wlo = node[t];
+        for (int j = 0; j <= nz; ++j) {
+            const double zmid = (j == 0) ? zs + 0.25 * dzz : zs + dzz * (double)j;
+            const double zc = zs + dzz * ((double)j + 0.5);
+            const double len = (j == 0) ? 0.5 * dzz : dzz;
+            const double wm = node[(2 * j + 1) * TW + t], wc = node[(2 * j + 2) * TW + t];
+            i1 += (wlo + 4.0 * wm + wc) * (len / 6.0);
+            iz += (zlo * wlo + 4.0 * zmid * wm + zc * wc) * (len / 6.0);
+            uc[j * TW + t] = G * (ztop * i1 - iz) + tau * i1;
+            zlo = zc;
+            wlo = wc;
+        }
+    }
+    __syncthreads();
+    // phase 3: layer fluxes h_f <u>_layer into the node storage
+    double *lay = node;                                // [nz][TW]
+    for (int e = t; e < nz * TW; e += kVelThreads) {
+        const int j = e / TW, f = e % TW;
+        if (f_ok[f] == 0.0) continue;
+        const double u0 = uc[j * TW + f], u1 = uc[(j + 1) * TW + f];
+        const double mean = (j == 0) ? u0 + (u1 - 3.0 * u0) / 18.0 : u0 + (u1 - 2.0 * u0 + uc[(j - 1) * TW + f]) / 24.0;
+        lay[e] = f_hf[f] * mean;
+    }
+    __syncthreads();
+    // phase 4: what the layers carry against the film's own flux
+    if (t < TW && f_ok[t] != 0.0) {
+        double tot = 0.0;
+        for (int j = 0; j < nz; ++j) tot += lay[j * TW + t] * g.dz;
+        f_shift[t] -= tot;
+    }
+    __syncthreads();
+    // phase 5: the layers' fluxes, shifted evenly
+    double *Q = dir == 0 ? a.Qx : a.Qy;
+    for (int e = t; e < nz * TW; e += kVelThreads) {
+        const int j = e / TW, f = e % TW;
+        if (f_ok[f] == 0.0) continue;
+        Q[(j + 2) * g.n2 + cell(f, 0)] = lay[e] + f_shift[f];
+    }
 }
 // the zeta volume fluxes of a column from the substrate up (the discrete
 // geometric conservation law), the free surface from the film's sources
@@ -647,14 +722,6 @@ __global__ void k_cghost_x(Geo3 g, double *x) {
 // stage's right-hand side is -dt (A c_old) in the balance form
 // react c_r + sum_k A_k (c_k - c_r) (ExplicitPart, :895-913).
 // ---------------------------------------------------------------------
-// fourth-order first derivatives: at the face between f0 and f1 (from
-// f_-1 .. f2), and at the centre of a cell (from its +-1, +-2 neighbours)
-__device__ inline double dface4(double fm1, double f0, double f1, double f2, double ds) {
-    return (fm1 - 27.0 * f0 + 27.0 * f1 - f2) / (24.0 * ds);
-}
-__device__ inline double dcentre4(double fm2, double fm1, double f1, double f2, double ds) {
-    return (fm2 - 8.0 * fm1 + 8.0 * f1 - f2) / (12.0 * ds);
-}
 // react c_r + sum over the off-diagonal slots of A_k (c_k - c_r)
 __device__ inline double balance_form(const Geo3 &g, const double (&A)[21], double react, const double *c, int ix,
                                       int jy, int kz) {
@@ -1052,162 +1119,246 @@ __global__ void k_extrap(int n3, double rr, const double *c, const double *prev,
 }
 
 // ---------------------------------------------------------------------
-// RedistributeExcess (:403-441), one thread per real column; the solute
-// moved in a per-column array
+// RedistributeExcess and RedistributeDeficit (CZ_ConcEquation_2D.cpp
+// :388-570), one wave per real column (kColPerBlock columns to a block);
+// a column's levels are taken 64 at a time, one to a lane.
+//
+// Excess: what a level holds above 1 runs down the column, level to level;
+// with out_k the excess leaving level k downward,
+//   out_k = max(c_k - 1 + out_{k+1}, 0),
+// a recurrence of maps x -> max(a + x, b), which compose associatively as
+// (a, b) o (a', b') = (a + a', max(a + b', b)): a scan over the lanes.
+// What reaches the bottom level above 1 fills the room (1 - c) of the
+// levels from the bottom up (a prefix sum of the room), and what no level
+// has room for is spread evenly through the column. The solute moved is
+// every excess that left a level plus what filled room.
+//
+// Deficit: within a column, the solute below 0 is taken from the levels
+// above 0 in proportion to what they hold, or, when the column holds too
+// little, the column is made uniform at its (negative) mean; between
+// columns, a column whose mean is below 0 takes from the room (h times the
+// mean) of its four neighbours, which give up the fractions taken, for up
+// to four passes; what is left is taken from every column in proportion.
 // ---------------------------------------------------------------------
-__global__ void k_excess(Geo3 g, double *c, double *moved) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.nx * g.ny) return;
-    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
-    const int kb = 2, kt = g.Nz - 3;
-    double mv = 0.0;
-    bool any = false;
-    for (int kz = kb; kz <= kt; kz++) if (c[kz * g.n2 + p] > 1.0) { any = true; break; }
-    if (any) {
-        for (int kz = kt; kz > kb; kz--) {
-            const double ck = c[kz * g.n2 + p];
-            if (ck > 1.0) {
-                const double e = ck - 1.0;
-                c[kz * g.n2 + p] = 1.0;
-                c[(kz - 1) * g.n2 + p] += e;
-                mv += e;
-            }
-        }
-        const double cb = c[kb * g.n2 + p];
-        if (cb > 1.0) {
-            double e = cb - 1.0;
-            c[kb * g.n2 + p] = 1.0;
-            for (int kz = kb + 1; kz <= kt && e > 0.0; kz++) {
-                const double ck = c[kz * g.n2 + p];
-                const double take = fmin(e, 1.0 - ck);
-                if (take > 0.0) { c[kz * g.n2 + p] = ck + take; e -= take; mv += take; }
-            }
-            if (e > 0.0)
-                for (int kz = kb; kz <= kt; kz++) c[kz * g.n2 + p] += e / (double)(kt - kb + 1);
-        }
-    }
-    moved[q] = mv;
+constexpr int kColPerBlock = 4;
+inline int col_blocks(int ncol) { return (ncol + kColPerBlock - 1) / kColPerBlock; }
+__device__ inline int col_index() { return blockIdx.x * kColPerBlock + (int)(threadIdx.x >> 6); }
+__device__ inline int lane64() { return (int)(threadIdx.x & 63); }
+// the sum over the 64 lanes, the same in every lane (butterfly, fixed order)
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
-// RedistributeDeficit (:449-570): whether any real cell is below 0
+// inclusive prefix sum over the lanes
+__device__ inline double wave_scan(double v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+// the sum of a column's real levels, in chunk order
+__device__ inline double col_total(const Geo3 &g, const double *c, int p, int lane) {
+    double s = 0.0;
+    for (int k0 = 0; k0 < g.nz; k0 += 64) {
+        const int k = k0 + lane;
+        s += wave_sum(k < g.nz ? c[(k + 2) * g.n2 + p] : 0.0);
+    }
+    return s;
+}
+__device__ inline int col_pad(const Geo3 &g, int q) { return I2(g, q % g.nx + 2, q / g.nx + 2); }
+
+__global__ void __launch_bounds__(64 * kColPerBlock) k_excess(Geo3 g, double *c, double *moved) {
+    const int q = col_index();
+    if (q >= g.nx * g.ny) return;
+    const int lane = lane64(), p = col_pad(g, q), nz = g.nz;
+    const double NEG = -1.0e300;
+    // every lane keeps to its own levels: i = 0 the top (level nz + 1), the
+    // chunks of 64 from the top down
+    auto row = [&](int i) { return (nz + 1 - i) * g.n2 + p; };
+    // down the column
+    double carry = 0.0, mv = 0.0, ybot = 0.0;   // what enters the chunk from above; moved; the bottom's value
+    for (int i0 = 0; i0 < nz; i0 += 64) {
+        const int i = i0 + lane;
+        const bool lv = i < nz, bottom = i == nz - 1;
+        const double ck = lv ? c[row(i)] : 0.0;
+        double A = (lv && !bottom) ? ck - 1.0 : 0.0, B = (lv && !bottom) ? 0.0 : NEG;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double a2 = __shfl_up(A, o, 64), b2 = __shfl_up(B, o, 64);
+            if (lane >= o) { B = fmax(A + b2, B); A = A + a2; }
+        }
+        const double out = fmax(A + carry, B);                 // leaves this level downward
+        double above = __shfl_up(out, 1, 64);                   // entered it from above
+        if (lane == 0) above = carry;
+        if (lv && !bottom) {
+            c[row(i)] = (out > 0.0) ? 1.0 : ck + above;
+            mv += out;
+        }
+        if (bottom) ybot = ck + above;
+        carry = __shfl(out, 63, 64);
+    }
+    mv = wave_sum(mv);
+    const int lb = (nz - 1) & 63;
+    const double e = __shfl(ybot, lb, 64) - 1.0;
+    if (!(e > 0.0)) {
+        if (lane == lb) c[row(nz - 1)] = ybot;
+        if (lane == 0) moved[q] = mv;
+        return;
+    }
+    if (lane == lb) c[row(nz - 1)] = 1.0;
+    // the bottom's excess fills the room of the levels above it, bottom up:
+    // chunks from the bottom one, lanes from the highest
+    double below = 0.0, filled = 0.0;           // room in the chunks below; solute taken
+    for (int i0 = ((nz - 1) / 64) * 64; i0 >= 0; i0 -= 64) {
+        const int i = i0 + lane;
+        const bool lv = i < nz - 1;
+        const double ck = lv ? c[row(i)] : 1.0;
+        const double room = fmax(1.0 - ck, 0.0);
+        double upto = room;                     // room of this level and every level below it in the chunk
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double u = __shfl_down(upto, o, 64);
+            if (lane + o < 64) upto += u;
+        }
+        upto += below;
+        const double take = fmin(fmax(e - (upto - room), 0.0), room);
+        if (lv && take > 0.0) c[row(i)] = ck + take;
+        filled += take;
+        below = __shfl(upto, 0, 64);
+    }
+    filled = wave_sum(filled);
+    mv += filled;
+    const double left = e - filled;
+    if (left > 0.0)
+        for (int i = lane; i < nz; i += 64) c[row(i)] += left / (double)nz;
+    if (lane == 0) moved[q] = mv;
+}
+// whether any real cell is below 0
 __global__ void k_below(Geo3 g, const double *c, int *flags) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.nint) return;
     const int ix = t % g.nx + 2, jy = (t / g.nx) % g.ny + 2, kz = t / (g.nx * g.ny) + 2;
     if (c[I3(g, ix, jy, kz)] < 0.0) atomicOr(&flags[1], 1);
 }
-// within each column: its cells below 0 from its cells above
-__global__ void k_deficit_col(Geo3 g, const double *H, double *c, double *moved, int *flags) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// within each column: the solute below 0 from its levels above 0
+__global__ void __launch_bounds__(64 * kColPerBlock) k_deficit_col(Geo3 g, const double *H, double *c, double *moved,
+                                                                   int *flags) {
+    const int q = col_index();
     if (q >= g.nx * g.ny) return;
-    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
-    const int kb = 2, kt = g.Nz - 3;
-    const double nLay = (double)(kt - kb + 1);
-    double def = 0.0, pos = 0.0;
-    for (int kz = kb; kz <= kt; kz++) {
-        const double ck = c[kz * g.n2 + p];
-        if (ck < 0.0) def -= ck; else pos += ck;
+    const int lane = lane64(), p = col_pad(g, q), nz = g.nz;
+    double lack = 0.0, hold = 0.0;
+    for (int k0 = 0; k0 < nz; k0 += 64) {
+        const int k = k0 + lane;
+        const double ck = k < nz ? c[(k + 2) * g.n2 + p] : 0.0;
+        lack += wave_sum(ck < 0.0 ? -ck : 0.0);
+        hold += wave_sum(ck < 0.0 ? 0.0 : ck);
     }
-    moved[q] = 0.0;
-    if (def == 0.0) return;
-    moved[q] = def * H[p] / nLay;
-    if (pos >= def) {
-        const double f = def / pos;
-        for (int kz = kb; kz <= kt; kz++) {
-            const double ck = c[kz * g.n2 + p];
-            c[kz * g.n2 + p] = (ck < 0.0) ? 0.0 : ck - f * ck;
+    if (lane == 0) moved[q] = lack * H[p] / (double)nz;
+    if (lack == 0.0) return;
+    if (hold >= lack) {
+        const double f = lack / hold;
+        for (int k = lane; k < nz; k += 64) {
+            const int r = (k + 2) * g.n2 + p;
+            const double ck = c[r];
+            c[r] = (ck < 0.0) ? 0.0 : ck - f * ck;
         }
     } else {
-        const double m = (pos - def) / nLay;
-        for (int kz = kb; kz <= kt; kz++) c[kz * g.n2 + p] = m;
-        atomicOr(&flags[2], 1);
+        const double m = (hold - lack) / (double)nz;
+        for (int k = lane; k < nz; k += 64) c[(k + 2) * g.n2 + p] = m;
+        if (lane == 0) atomicOr(&flags[2], 1);
     }
 }
-__device__ inline double col_mean(const Geo3 &g, const double *c, int p) {
-    double s = 0.0;
-    for (int kz = 2; kz <= g.Nz - 3; kz++) s += c[kz * g.n2 + p];
-    return s / (double)(g.Nz - 4);
+__device__ inline void col_fill(const Geo3 &g, double *c, int p, int lane, double v) {
+    for (int k = lane; k < g.nz; k += 64) c[(k + 2) * g.n2 + p] = v;
 }
-// a pass between columns: room of every column
-__global__ void k_def_room(Geo3 g, const double *H, const double *c, double *room, double *fcol) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.nx * g.ny) return;
-    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
-    const double m = col_mean(g, c, p), h = H[p];
-    room[q] = (m > 0.0 && h > 0.0) ? h * m : 0.0;
-    fcol[q] = 0.0;
+__device__ inline void col_scale(const Geo3 &g, double *c, int p, int lane, double f) {
+    for (int k = lane; k < g.nz; k += 64) c[(k + 2) * g.n2 + p] *= f;
 }
-// the columns below 0: what they take from their neighbours (f), their
-// new mean; added: what a column without film makes up
-__global__ void k_def_take(Geo3 g, const double *H, double *c, const double *room, double *fcol, double *added,
-                           int *flags) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// a pass between columns: the room (h times the mean) of every column
+__global__ void __launch_bounds__(64 * kColPerBlock) k_def_room(Geo3 g, const double *H, const double *c, double *room,
+                                                                double *fcol) {
+    const int q = col_index();
     if (q >= g.nx * g.ny) return;
+    const int lane = lane64(), p = col_pad(g, q);
+    const double m = col_total(g, c, p, lane) / (double)g.nz, h = H[p];
+    if (lane == 0) {
+        room[q] = (m > 0.0 && h > 0.0) ? h * m : 0.0;
+        fcol[q] = 0.0;
+    }
+}
+// the columns below 0: the fraction f of their neighbours' room they take
+// (all of it when that is not enough) and their new mean; added: what a
+// column without film makes up
+__global__ void __launch_bounds__(64 * kColPerBlock) k_def_take(Geo3 g, const double *H, double *c, const double *room,
+                                                                double *fcol, double *added, int *flags) {
+    const int q = col_index();
+    if (q >= g.nx * g.ny) return;
+    const int lane = lane64(), p = col_pad(g, q);
     const int i = q % g.nx, j = q / g.nx;
-    const int p = I2(g, i + 2, j + 2);
-    added[q] = 0.0;
-    const double m = col_mean(g, c, p), h = H[p];
+    const double m = col_total(g, c, p, lane) / (double)g.nz, h = H[p];
+    if (lane == 0) added[q] = 0.0;
     if (!(m < 0.0)) return;
     if (!(h > 0.0)) {
-        added[q] = -(h * m);
-        for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] *= 0.0;
+        if (lane == 0) added[q] = -(h * m);
+        col_scale(g, c, p, lane, 0.0);
         return;
     }
-    const double e = -h * m;
-    const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
-    double r = 0.0;
-    for (int t = 0; t < 4; t++) {
-        const int ii = i + di[t], jj = j + dj[t];
-        if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny) r += room[jj * g.nx + ii];
+    const double need = -h * m;
+    double avail = 0.0;
+    if (i > 0) avail += room[q - 1];
+    if (i < g.nx - 1) avail += room[q + 1];
+    if (j > 0) avail += room[q - g.nx];
+    if (j < g.ny - 1) avail += room[q + g.nx];
+    if (!(avail > 0.0)) {
+        if (lane == 0) atomicOr(&flags[3], 1);
+        return;
     }
-    if (!(r > 0.0)) { atomicOr(&flags[3], 1); return; }
-    const double f = (e < r) ? e / r : 1.0;
-    fcol[q] = f;
-    const double mNew = (f < 1.0) ? 0.0 : -(e - r) / h;
-    for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] = mNew;
-    if (f >= 1.0) atomicOr(&flags[3], 1);
+    const double f = (need < avail) ? need / avail : 1.0;
+    if (lane == 0) {
+        fcol[q] = f;
+        if (f >= 1.0) atomicOr(&flags[3], 1);
+    }
+    col_fill(g, c, p, lane, (f < 1.0) ? 0.0 : -(need - avail) / h);
 }
-// the neighbours give up what was taken, in the order the reference's
-// column loop adds it (x outer, y inner: west, south, north, east)
-__global__ void k_def_give(Geo3 g, const double *fcol, double *c, int *flags) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// the neighbours give up what was taken from them; the fractions are added
+// west, south, north, east
+__global__ void __launch_bounds__(64 * kColPerBlock) k_def_give(Geo3 g, const double *fcol, double *c, int *flags) {
+    const int q = col_index();
     if (q >= g.nx * g.ny) return;
+    const int lane = lane64(), p = col_pad(g, q);
     const int i = q % g.nx, j = q / g.nx;
-    const int p = I2(g, i + 2, j + 2);
-    double t = 0.0;
+    double tk = 0.0;
     bool got = false;
-    const int di[4] = {-1, 0, 0, 1}, dj[4] = {0, -1, 1, 0};
-    for (int u = 0; u < 4; u++) {
-        const int ii = i + di[u], jj = j + dj[u];
-        if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny) {
-            const double f = fcol[jj * g.nx + ii];
-            if (f > 0.0) { t += f; got = true; }
-        }
-    }
-    if (got && t > 0.0) {
-        for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] *= (1.0 - t);
-        if (col_mean(g, c, p) < 0.0) atomicOr(&flags[3], 1);
-    }
+    const int nb[4] = {i > 0 ? q - 1 : -1, j > 0 ? q - g.nx : -1, j < g.ny - 1 ? q + g.nx : -1, i < g.nx - 1 ? q + 1 : -1};
+    for (int u = 0; u < 4; ++u)
+        if (nb[u] >= 0 && fcol[nb[u]] > 0.0) { tk += fcol[nb[u]]; got = true; }
+    if (!(got && tk > 0.0)) return;
+    col_scale(g, c, p, lane, 1.0 - tk);
+    if (col_total(g, c, p, lane) / (double)g.nz < 0.0 && lane == 0) atomicOr(&flags[3], 1);
 }
-// what is left: lack and total per column
-__global__ void k_def_left(Geo3 g, const double *H, double *c, double *lack, double *total) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// what is still lacking, and the solute every other column holds
+__global__ void __launch_bounds__(64 * kColPerBlock) k_def_left(Geo3 g, const double *H, double *c, double *lack,
+                                                                double *total) {
+    const int q = col_index();
     if (q >= g.nx * g.ny) return;
-    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
-    const double m = col_mean(g, c, p), h = H[p];
-    lack[q] = 0.0; total[q] = 0.0;
+    const int lane = lane64(), p = col_pad(g, q);
+    const double m = col_total(g, c, p, lane) / (double)g.nz, h = H[p];
+    if (lane == 0) { lack[q] = 0.0; total[q] = 0.0; }
     if (m < 0.0) {
-        lack[q] = -(h * m);
-        for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] *= 0.0;
-    } else if (h > 0.0) {
+        if (lane == 0) lack[q] = -(h * m);
+        col_scale(g, c, p, lane, 0.0);
+    } else if (h > 0.0 && lane == 0) {
         total[q] = h * m;
     }
 }
-__global__ void k_def_scale(Geo3 g, const double *H, double f, double *c) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64 * kColPerBlock) k_def_scale(Geo3 g, const double *H, double f, double *c) {
+    const int q = col_index();
     if (q >= g.nx * g.ny) return;
-    const int p = I2(g, q % g.nx + 2, q / g.nx + 2);
-    if (H[p] > 0.0 && col_mean(g, c, p) > 0.0)
-        for (int kz = 2; kz <= g.Nz - 3; kz++) c[kz * g.n2 + p] *= (1.0 - f);
+    const int lane = lane64(), p = col_pad(g, q);
+    if (H[p] > 0.0 && col_total(g, c, p, lane) / (double)g.nz > 0.0) col_scale(g, c, p, lane, 1.0 - f);
 }
 // partial sums of a per-column array in a fixed order (reduced by k_final)
 __global__ void k_colsum(int n, const double *v, double *part) {
@@ -1445,7 +1596,9 @@ int c_stage(yz_cz *z, int &iter, double &resid) {
 // stages change, so both stages share them
 void volume_fluxes(yz_cz *z) {
     const Geo3 &g = z->g;
-    hipLaunchKernelGGL(k_faces3, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, z->a);
+    const int tw = vel_tile(g.nz);
+    hipLaunchKernelGGL(k_face_velocity, dim3((g.nx + tw) / tw, g.ny + 1, 2), dim3(kVelThreads), vel_lds(g.nz, tw),
+                       z->st, z->a, tw);
     hipLaunchKernelGGL(k_w, dim3(nblk(g.nx * g.ny)), dim3(TBK), 0, z->st, z->a);
 }
 
@@ -1477,32 +1630,32 @@ int redistribute_deficit(yz_cz *z, double &moved, double &added) {
     hipLaunchKernelGGL(k_below, dim3(nblk(g.nint)), dim3(TBK), 0, z->st, g, (const double *)z->cN, z->flags);
     if (read_flags(z)) return -1;
     if (!z->hflags[1]) return 0;
-    hipLaunchKernelGGL(k_deficit_col, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH, z->cN,
+    hipLaunchKernelGGL(k_deficit_col, dim3(col_blocks(nc)), dim3(64 * kColPerBlock), 0, z->st, g, (const double *)z->fH, z->cN,
                        z->colv[0], z->flags);
     moved = colsum(z, z->colv[0], &err);
     if (err || read_flags(z)) return -1;
     if (!z->hflags[2]) return 0;
     bool left = true;
     for (int pass = 0; pass < 4 && left; ++pass) {
-        hipLaunchKernelGGL(k_def_room, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH,
+        hipLaunchKernelGGL(k_def_room, dim3(col_blocks(nc)), dim3(64 * kColPerBlock), 0, z->st, g, (const double *)z->fH,
                            (const double *)z->cN, z->colv[0], z->colv[1]);
         if (zero_flags(z)) return -1;
-        hipLaunchKernelGGL(k_def_take, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH, z->cN,
+        hipLaunchKernelGGL(k_def_take, dim3(col_blocks(nc)), dim3(64 * kColPerBlock), 0, z->st, g, (const double *)z->fH, z->cN,
                            (const double *)z->colv[0], z->colv[1], z->colv[2], z->flags);
         added += colsum(z, z->colv[2], &err);
-        hipLaunchKernelGGL(k_def_give, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->colv[1], z->cN,
+        hipLaunchKernelGGL(k_def_give, dim3(col_blocks(nc)), dim3(64 * kColPerBlock), 0, z->st, g, (const double *)z->colv[1], z->cN,
                            z->flags);
         if (err || read_flags(z)) return -1;
         left = z->hflags[3] != 0;
     }
     if (!left) return 0;
-    hipLaunchKernelGGL(k_def_left, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH, z->cN, z->colv[0],
+    hipLaunchKernelGGL(k_def_left, dim3(col_blocks(nc)), dim3(64 * kColPerBlock), 0, z->st, g, (const double *)z->fH, z->cN, z->colv[0],
                        z->colv[1]);
     const double lack = colsum(z, z->colv[0], &err);
     const double total = colsum(z, z->colv[1], &err);
     if (err) return -1;
     const double f = (total > 0.0) ? std::min(1.0, lack / total) : 0.0;
-    if (f > 0.0) hipLaunchKernelGGL(k_def_scale, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, (const double *)z->fH, f, z->cN);
+    if (f > 0.0) hipLaunchKernelGGL(k_def_scale, dim3(col_blocks(nc)), dim3(64 * kColPerBlock), 0, z->st, g, (const double *)z->fH, f, z->cN);
     if (total < lack) added += lack - total;
     return 0;
 }
@@ -1624,7 +1777,7 @@ int cz_advance(yz_cz *z, int &lin, int &passes_out, double &resid_out) {
     passes_out = passes;
     // RedistributeExcess, RedistributeDeficit
     const int nc = g.nx * g.ny;
-    hipLaunchKernelGGL(k_excess, dim3(nblk(nc)), dim3(TBK), 0, z->st, g, z->cN, z->colv[3]);
+    hipLaunchKernelGGL(k_excess, dim3(col_blocks(nc)), dim3(64 * kColPerBlock), 0, z->st, g, z->cN, z->colv[3]);
     int err = 0;
     z->stats[5] += colsum(z, z->colv[3], &err);
     if (err) return -1;
