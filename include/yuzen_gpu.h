@@ -387,6 +387,20 @@ int yz_batch_get_h(yz_batch *b, int film, double *h);
 int yz_batch_stats(yz_batch *b, double *out4);
 void yz_batch_destroy(yz_batch *b);
 
+/* ---- tile tuner of the fused FP32 iteration (no reference counterpart) --
+ * The learned path's Krylov kernels stage 64 x R tiles (R, the tile rows,
+ * 8 or 16). yz_tune_tiles times `iters` iterations of a synthetic system of
+ * nx x ny cells for each R, three solves each, and makes the fastest the
+ * process's choice for that size; rows gets it and us2 the microseconds
+ * per iteration of R = 16 and R = 8. yz_tile_rows returns the choice in
+ * force (16 when none was made), yz_set_tile_rows pins one (returns 2 for
+ * another R). A film's result depends on its tile rows (the order of the
+ * dot products' partial sums), so every context and batch of the process
+ * uses the same R for films of one size. Return codes: 0, 2 device error. */
+int yz_tune_tiles(int nx, int ny, int iters, int *rows, double *us2);
+int yz_tile_rows(int nx, int ny);
+int yz_set_tile_rows(int nx, int ny, int rows);
+
 /* ---- linear-solve seam -------------------------------------------------
  * Solve A x = b, A in CRS (rowptr[n+1], col[nnz], val[nnz]; any column
  * order), with BiCGSTAB under settings s. When grid_nx > 0 the rows are

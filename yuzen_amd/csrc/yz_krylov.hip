@@ -42,7 +42,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
+#include <map>
+#include <utility>
 #include <vector>
 
 #include "../../include/yuzen_gpu.h"
@@ -62,9 +66,12 @@ namespace yz {
 
 namespace {
 
-constexpr int TX = 64, TY = 16, TB = 256;  // 64 x 16 tiles, 256 threads, four rows per thread
-constexpr int RPT = TY / (TB / TX);         // rows per thread
-constexpr int LW = TX + 4, LH = TY + 4;     // LDS tile with the 2-cell halo
+// tiles of TX x TY cells (TY, the tile rows, a template parameter: 8 or
+// 16, chosen per film size by the tile tuner), 256 threads, TY / 4 rows
+// per thread; the LDS tile adds the 2-cell halo
+constexpr int TX = 64, TB = 256;
+constexpr int LW = TX + 4;
+constexpr int kTileRows[2] = {16, 8};        // the tuner's candidates; the first is the default
 // blocks of every kernel (grid-stride over the tiles), so partials per
 // reduction: measured at 1024^2 against 256, 512, 1536, 2048 and 4096
 constexpr int G = 1024;
@@ -164,10 +171,10 @@ __device__ inline void commit_sc(const double (&c)[S_N], double *__restrict__ ou
 // stage f(j) over tile (x0, y0) and its 2-cell halo into L (folded at the
 // edges like the stencil's columns): coalesced row loads instead of 13
 // scattered loads per cell and vector
-template <class F>
-__device__ inline void stage(const StencilOp &op, int x0, int y0, float (&L)[LH][LW], F f) {
+template <int TY, class F>
+__device__ inline void stage(const StencilOp &op, int x0, int y0, float (&L)[TY + 4][LW], F f) {
     const bool inside = x0 >= 2 && x0 + TX + 2 <= op.nx && y0 >= 2 && y0 + TY + 2 <= op.ny;
-    for (int q = threadIdx.x; q < LH * LW; q += TB) {
+    for (int q = threadIdx.x; q < (TY + 4) * LW; q += TB) {
         const int ly = q / LW, lx = q - ly * LW;
         int gx = x0 + lx - 2, gy = y0 + ly - 2;
         if (!inside) { gx = fold_axis(gx, op.nx, op.perx); gy = fold_axis(gy, op.ny, op.pery); }
@@ -177,8 +184,8 @@ __device__ inline void stage(const StencilOp &op, int x0, int y0, float (&L)[LH]
 
 // the 13-point product of planes W (row i) with the staged tile at local
 // cell (lx, ly) (tile coordinates, without the halo)
-template <class T, class Cv>
-__device__ inline float apply13(const T *__restrict__ W, int N, int i, const float (&L)[LH][LW], int lx, int ly, Cv cv) {
+template <int TY, class T, class Cv>
+__device__ inline float apply13(const T *__restrict__ W, int N, int i, const float (&L)[TY + 4][LW], int lx, int ly, Cv cv) {
     float a = 0;
 #pragma unroll
     for (int k = 0; k < 13; ++k) a = fmaf(cv(W[k * N + i]), L[ly + 2 + OY[k]][lx + 2 + OX[k]], a);
@@ -187,7 +194,8 @@ __device__ inline float apply13(const T *__restrict__ W, int N, int i, const flo
 // the same for A with a unit diagonal: slot 6 (the centre) contributes its
 // staged value without a load; fmaf(1, x, a) and a + x round the same, so
 // the result is the same to the bit
-__device__ inline float apply13u(const float *__restrict__ W, int N, int i, const float (&L)[LH][LW], int lx, int ly) {
+template <int TY>
+__device__ inline float apply13u(const float *__restrict__ W, int N, int i, const float (&L)[TY + 4][LW], int lx, int ly) {
     float a = 0;
 #pragma unroll
     for (int k = 0; k < 13; ++k) {
@@ -217,6 +225,7 @@ struct Film {
     Vecs V;
     Ctl c;
     int g;
+    int ty;           // tile rows (the kernels' TY)
     double *gather;   // where the state read copies the scalars (S_N)
 };
 // ONE: a single film, its descriptor passed by value (in the kernel
@@ -231,21 +240,21 @@ struct Film {
     const Vecs &V = f.V
 
 // the cells of tile (x0, y0) this thread owns: column tx, rows ty, ty + 4,
-// ty + 8, ty + 12
-template <class F>
+// ty + 8, ...
+template <int TY, class F>
 __device__ inline void for_own(const StencilOp &op, int x0, int y0, F f) {
     const int lx = threadIdx.x & (TX - 1);
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
+    for (int r = 0; r < TY / (TB / TX); ++r) {
         const int ly = threadIdx.x / TX + r * (TB / TX);
         if (x0 + lx < op.nx && y0 + ly < op.ny) f(lx, ly, (y0 + ly) * op.nx + x0 + lx);
     }
 }
 
 // K0 (q = 0): r = b - A x, rh = r; partials (b, b), (r, r)
-template <bool ONE>
+template <bool ONE, int TY>
 __global__ void __launch_bounds__(TB) k_f_init(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk) {
-    __shared__ float L[LH][LW];
+    __shared__ float L[TY + 4][LW];
     FILM_PROLOGUE;
     const float *__restrict__ A = f.A, *__restrict__ b = f.b;
     const Seq sq = seq(c, 0);
@@ -254,11 +263,11 @@ __global__ void __launch_bounds__(TB) k_f_init(const Film one, const Film *__res
     for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
-        stage(op, x0, y0, L, [&](int j) { return x[j]; });
+        stage<TY>(op, x0, y0, L, [&](int j) { return x[j]; });
         __syncthreads();
-        for_own(op, x0, y0, [&](int lx, int ly, int i) {
+        for_own<TY>(op, x0, y0, [&](int lx, int ly, int i) {
             const float bi = b[i];
-            const float ri = bi - (f.unitd ? apply13u(A, op.N, i, L, lx, ly) : apply13(A, op.N, i, L, lx, ly, idf));
+            const float ri = bi - (f.unitd ? apply13u<TY>(A, op.N, i, L, lx, ly) : apply13<TY>(A, op.N, i, L, lx, ly, idf));
             V.r[i] = ri; V.rh[i] = ri;
             acc[0] += (double)bi * bi;
             acc[1] += (double)ri * ri;
@@ -272,10 +281,10 @@ __global__ void __launch_bounds__(TB) k_f_init(const Film one, const Film *__res
 // update of iteration k - 1 (or, after the half-step exit, x += alpha ph
 // and the end); r = s - omega t; p = r + beta (p_old - omega v); ph = M p;
 // partial (r, r).
-template <bool ONE>
+template <bool ONE, int TY>
 __global__ void __launch_bounds__(TB) k_f_pm(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk,
                                              int q, int first, int par) {
-    __shared__ float L[LH][LW];
+    __shared__ float L[TY + 4][LW];
     FILM_PROLOGUE;
     const __half *__restrict__ Y = f.Y;
     const float *__restrict__ ms = f.ms;
@@ -315,7 +324,7 @@ __global__ void __launch_bounds__(TB) k_f_pm(const Film one, const Film *__restr
         const float a = (float)alpha_prev;
         for (int t = lb; t < c.ntx * c.nty; t += f.g) {
             const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
-            for_own(op, x0, y0, [&](int, int, int i) { V.x[i] += a * V.ph[i]; });
+            for_own<TY>(op, x0, y0, [&](int, int, int i) { V.x[i] += a * V.ph[i]; });
         }
         return;
     }
@@ -328,10 +337,10 @@ __global__ void __launch_bounds__(TB) k_f_pm(const Film one, const Film *__restr
     for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
-        if (first) stage(op, x0, y0, L, [&](int j) { return rv[j]; });
-        else stage(op, x0, y0, L, [&](int j) { return (sv[j] - omega * tv[j]) + beta * pold[j] - bw * vv[j]; });
+        if (first) stage<TY>(op, x0, y0, L, [&](int j) { return rv[j]; });
+        else stage<TY>(op, x0, y0, L, [&](int j) { return (sv[j] - omega * tv[j]) + beta * pold[j] - bw * vv[j]; });
         __syncthreads();
-        for_own(op, x0, y0, [&](int lx, int ly, int i) {
+        for_own<TY>(op, x0, y0, [&](int lx, int ly, int i) {
             const float pi = L[ly + 2][lx + 2];
             const float ri = first ? pi : sv[i] - omega * tv[i];
             if (!first) {
@@ -339,7 +348,7 @@ __global__ void __launch_bounds__(TB) k_f_pm(const Film one, const Film *__restr
                 V.r[i] = ri;
             }
             pnew[i] = pi;
-            V.ph[i] = ms[i] * apply13(Y, op.N, i, L, lx, ly, h2f);
+            V.ph[i] = ms[i] * apply13<TY>(Y, op.N, i, L, lx, ly, h2f);
             acc[0] += (double)ri * ri;
         });
     }
@@ -347,10 +356,10 @@ __global__ void __launch_bounds__(TB) k_f_pm(const Film one, const Film *__restr
 }
 
 // K2 (q = 4k + 2): for k >= 1 the exits on ||r_k||; v = A ph; partial (rh, v)
-template <bool ONE>
+template <bool ONE, int TY>
 __global__ void __launch_bounds__(TB) k_f_av(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk,
                                              int q, int work) {
-    __shared__ float L[LH][LW];
+    __shared__ float L[TY + 4][LW];
     FILM_PROLOGUE;
     const float *__restrict__ A = f.A;
     const Seq sq = seq(c, q);
@@ -385,10 +394,10 @@ __global__ void __launch_bounds__(TB) k_f_av(const Film one, const Film *__restr
     for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
-        stage(op, x0, y0, L, [&](int j) { return ph[j]; });
+        stage<TY>(op, x0, y0, L, [&](int j) { return ph[j]; });
         __syncthreads();
-        for_own(op, x0, y0, [&](int lx, int ly, int i) {
-            const float a = f.unitd ? apply13u(A, op.N, i, L, lx, ly) : apply13(A, op.N, i, L, lx, ly, idf);
+        for_own<TY>(op, x0, y0, [&](int lx, int ly, int i) {
+            const float a = f.unitd ? apply13u<TY>(A, op.N, i, L, lx, ly) : apply13<TY>(A, op.N, i, L, lx, ly, idf);
             V.v[i] = a;
             acc[0] += (double)V.rh[i] * a;
         });
@@ -398,10 +407,10 @@ __global__ void __launch_bounds__(TB) k_f_av(const Film one, const Film *__restr
 
 // K3 (q = 4k + 3): alpha = rho / (rh, v); s = r - alpha v; sh = M s;
 // partials (s, s), (s, rh)
-template <bool ONE>
+template <bool ONE, int TY>
 __global__ void __launch_bounds__(TB) k_f_sm(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk,
                                              int q) {
-    __shared__ float L[LH][LW];
+    __shared__ float L[TY + 4][LW];
     FILM_PROLOGUE;
     const __half *__restrict__ Y = f.Y;
     const float *__restrict__ ms = f.ms;
@@ -422,12 +431,12 @@ __global__ void __launch_bounds__(TB) k_f_sm(const Film one, const Film *__restr
     for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
-        stage(op, x0, y0, L, [&](int j) { return rv[j] - alpha * vv[j]; });
+        stage<TY>(op, x0, y0, L, [&](int j) { return rv[j] - alpha * vv[j]; });
         __syncthreads();
-        for_own(op, x0, y0, [&](int lx, int ly, int i) {
+        for_own<TY>(op, x0, y0, [&](int lx, int ly, int i) {
             const float si = L[ly + 2][lx + 2];
             V.s[i] = si;
-            V.sh[i] = ms[i] * apply13(Y, op.N, i, L, lx, ly, h2f);
+            V.sh[i] = ms[i] * apply13<TY>(Y, op.N, i, L, lx, ly, h2f);
             acc[0] += (double)si * si;
             acc[1] += (double)si * V.rh[i];
         });
@@ -437,10 +446,10 @@ __global__ void __launch_bounds__(TB) k_f_sm(const Film one, const Film *__restr
 
 // K4 (q = 4k + 4): the half-step exit on ||s||; t = A sh; partials (t, t),
 // (t, s), (t, rh)
-template <bool ONE>
+template <bool ONE, int TY>
 __global__ void __launch_bounds__(TB) k_f_at(const Film one, const Film *__restrict__ films, const int2 *__restrict__ blk,
                                              int q) {
-    __shared__ float L[LH][LW];
+    __shared__ float L[TY + 4][LW];
     FILM_PROLOGUE;
     const float *__restrict__ A = f.A;
     const Seq sq = seq(c, q);
@@ -461,10 +470,10 @@ __global__ void __launch_bounds__(TB) k_f_at(const Film one, const Film *__restr
     for (int t = lb; t < c.ntx * c.nty; t += f.g) {
         const int x0 = (t % c.ntx) * TX, y0 = (t / c.ntx) * TY;
         __syncthreads();
-        stage(op, x0, y0, L, [&](int j) { return sh[j]; });
+        stage<TY>(op, x0, y0, L, [&](int j) { return sh[j]; });
         __syncthreads();
-        for_own(op, x0, y0, [&](int lx, int ly, int i) {
-            const float a = f.unitd ? apply13u(A, op.N, i, L, lx, ly) : apply13(A, op.N, i, L, lx, ly, idf);
+        for_own<TY>(op, x0, y0, [&](int lx, int ly, int i) {
+            const float a = f.unitd ? apply13u<TY>(A, op.N, i, L, lx, ly) : apply13<TY>(A, op.N, i, L, lx, ly, idf);
             V.t[i] = a;
             acc[0] += (double)a * a;
             acc[1] += (double)a * V.s[i];
@@ -515,17 +524,36 @@ void fused_table_free(FusedTable &T) {
     T = FusedTable();
 }
 
+// the tile rows of a film of nx x ny cells: the tuner's choice for that
+// size (fused_tile_set), 16 when it has none
+static std::map<std::pair<int, int>, int> &tile_table() {
+    static std::map<std::pair<int, int>, int> t;
+    return t;
+}
+int fused_tile_rows(int nx, int ny) {
+    const auto &t = tile_table();
+    const auto it = t.find(std::make_pair(nx, ny));
+    return it == t.end() ? kTileRows[0] : it->second;
+}
+int fused_tile_set(int nx, int ny, int rows) {
+    if (rows != kTileRows[0] && rows != kTileRows[1]) return -1;
+    tile_table()[std::make_pair(nx, ny)] = rows;
+    return 0;
+}
+
 int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, const float *const *b,
                            float *const *x, int maxit, const double *tol, int *iters, double *resid, int *rc,
                            hipStream_t st) {
     if (nf <= 0) return 0;
     int total = 0;
+    std::vector<int> ty(nf);
     for (int f = 0; f < nf; ++f) {
         const StencilOp &op = S[f]->op0;
         bool ok = op.dim == 2 && op.ns == 13 && S[f]->Mh && S[f]->Mscale;
         for (int k = 0; ok && k < 13; ++k) ok = op.ox[k] == OX[k] && op.oy[k] == OY[k];
         if (!ok) return -1;
-        const int nt = ((op.nx + TX - 1) / TX) * ((op.ny + TY - 1) / TY);
+        ty[f] = fused_tile_rows(op.nx, op.ny);
+        const int nt = ((op.nx + TX - 1) / TX) * ((op.ny + ty[f] - 1) / ty[f]);
         total += std::min(nt, G);
         Krylov<float> &kr = S[f]->kr;
         if (!kr.ftiles) {
@@ -547,50 +575,74 @@ int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, cons
         d.V.v = kr.v; d.V.s = kr.s; d.V.sh = kr.sh; d.V.t = kr.t;
         d.c.sc = kr.sc + 32;   // two copies of S_N scalars (the solver's array holds 64)
         d.c.part = kr.fpart;
-        d.c.ntx = (s.op0.nx + TX - 1) / TX; d.c.nty = (s.op0.ny + TY - 1) / TY;
+        d.ty = ty[f];
+        d.c.ntx = (s.op0.nx + TX - 1) / TX; d.c.nty = (s.op0.ny + d.ty - 1) / d.ty;
         d.c.maxit = maxit; d.c.window = s.stall_window; d.c.factor = s.stall_factor; d.c.tol = tol[f];
         d.g = std::min(d.c.ntx * d.c.nty, G);
         d.gather = T.d_gather + (size_t)f * S_N;
     }
     KR_CHECK(hipMemcpyAsync(T.d_films, T.h_films, (size_t)nf * sizeof(Film), hipMemcpyHostToDevice, st));
-    // block tables: [0] every block of the given films, [1] one block per film
-    auto fill = [&](int k, const int *which, int n, bool all) -> int {
-        int2 *B = (int2 *)T.h_blk[k];
-        int m = 0;
-        for (int u = 0; u < n; ++u) {
-            const int f = which ? which[u] : u;
-            const int g = all ? F[f].g : 1;
-            for (int l = 0; l < g; ++l) B[m++] = make_int2(f, l);
-        }
-        return m;
-    };
+    SolverT<float> &S0 = *S[0];
+    // a graph replays a fixed grid: with it, films that have finished keep
+    // their blocks (which return at once) instead of dropping out
+    const bool graph = S0.use_graph != 0;
+    // the films by tile rows (one launch per kernel and group); block
+    // tables: [0] every block of the group's films, [1] one block per film,
+    // each group's run contiguous
+    struct Group { int rows, b0, nb, c0, nc; };
+    Group grp[2];
     std::vector<int> active(nf);
-    for (int f = 0; f < nf; ++f) active[f] = f;
-    int nblk = fill(0, nullptr, nf, true);
-    int nchk = fill(1, nullptr, nf, false);
-    KR_CHECK(hipMemcpyAsync(T.d_blk[0], T.h_blk[0], (size_t)nblk * sizeof(int2), hipMemcpyHostToDevice, st));
-    KR_CHECK(hipMemcpyAsync(T.d_blk[1], T.h_blk[1], (size_t)nchk * sizeof(int2), hipMemcpyHostToDevice, st));
+    auto fill = [&](bool only_running, const double *h) {
+        int2 *B = (int2 *)T.h_blk[0], *C = (int2 *)T.h_blk[1];
+        int m = 0, mc = 0;
+        for (int u = 0; u < 2; ++u) {
+            Group &G_ = grp[u];
+            G_.rows = kTileRows[u]; G_.b0 = m; G_.c0 = mc;
+            for (int f = 0; f < nf; ++f) {
+                if (F[f].ty != G_.rows) continue;
+                if (only_running && (int)h[(size_t)f * S_N + S_STATE] != ST_RUN) continue;
+                for (int l = 0; l < F[f].g; ++l) B[m++] = make_int2(f, l);
+                C[mc++] = make_int2(f, 0);
+            }
+            G_.nb = m - G_.b0; G_.nc = mc - G_.c0;
+        }
+    };
+    fill(false, nullptr);
+    const int ntot = grp[0].nb + grp[1].nb, nctot = grp[0].nc + grp[1].nc;
+    KR_CHECK(hipMemcpyAsync(T.d_blk[0], T.h_blk[0], (size_t)ntot * sizeof(int2), hipMemcpyHostToDevice, st));
+    KR_CHECK(hipMemcpyAsync(T.d_blk[1], T.h_blk[1], (size_t)nctot * sizeof(int2), hipMemcpyHostToDevice, st));
     const Film *dF = (const Film *)T.d_films;
     const int2 *dB = (const int2 *)T.d_blk[0], *dC = (const int2 *)T.d_blk[1];
     const dim3 blk(TB);
     const bool one = nf == 1;
     const Film F0 = F[0];
-#define FLAUNCH(K, nb, B, ...)                                                              \
-    do {                                                                                    \
-        if (one) hipLaunchKernelGGL(K<true>, dim3(nb), blk, 0, st, F0, dF, B, ##__VA_ARGS__);  \
-        else hipLaunchKernelGGL(K<false>, dim3(nb), blk, 0, st, F0, dF, B, ##__VA_ARGS__);     \
+    // K over the blocks of every group (w: 0 the work table, 1 the one-block table)
+#define FLAUNCH(K, w, ...)                                                                              \
+    do {                                                                                                \
+        for (int u_ = 0; u_ < 2; ++u_) {                                                                \
+            const Group &g_ = grp[u_];                                                                  \
+            const int nb_ = (w) ? g_.nc : g_.nb;                                                        \
+            if (!nb_) continue;                                                                         \
+            const int2 *tb_ = (w) ? dC + g_.c0 : dB + g_.b0;                                            \
+            if (g_.rows == 16) {                                                                        \
+                if (one) hipLaunchKernelGGL((K<true, 16>), dim3(nb_), blk, 0, st, F0, dF, tb_, ##__VA_ARGS__); \
+                else hipLaunchKernelGGL((K<false, 16>), dim3(nb_), blk, 0, st, F0, dF, tb_, ##__VA_ARGS__);    \
+            } else {                                                                                    \
+                if (one) hipLaunchKernelGGL((K<true, 8>), dim3(nb_), blk, 0, st, F0, dF, tb_, ##__VA_ARGS__);  \
+                else hipLaunchKernelGGL((K<false, 8>), dim3(nb_), blk, 0, st, F0, dF, tb_, ##__VA_ARGS__);     \
+            }                                                                                           \
+        }                                                                                               \
     } while (0)
     const double *h = T.h_gather;
     // the scalars as K2 would see them next (one block per film, no work;
     // the K2 that follows derives the same values from the same inputs)
     auto check = [&]() -> int {
-        FLAUNCH(k_f_av, nchk, dC, 2, 0);
+        FLAUNCH(k_f_av, 1, 2, 0);
         KR_CHECK(hipGetLastError());
         KR_CHECK(hipMemcpyAsync(T.h_gather, T.d_gather, (size_t)nf * S_N * sizeof(double), hipMemcpyDeviceToHost, st));
         KR_CHECK(hipStreamSynchronize(st));
         return 0;
     };
-    SolverT<float> &S0 = *S[0];
     const bool timing = nf == 1 && S0.ktime;
     if (timing && !S0.kev[0]) { (void)hipEventCreate(&S0.kev[0]); (void)hipEventCreate(&S0.kev[1]); }
     // two iterations, k even then odd: every kernel argument is fixed by
@@ -599,29 +651,40 @@ int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, cons
     auto pair = [&](bool timed) {
         for (int u = 0; u < 2; ++u) {
             if (timed && u == 0) (void)hipEventRecord(S0.kev[0], st);
-            FLAUNCH(k_f_av, nblk, dB, 2, 1);
+            FLAUNCH(k_f_av, 0, 2, 1);
             if (timed && u == 0) (void)hipEventRecord(S0.kev[1], st);
-            FLAUNCH(k_f_sm, nblk, dB, 3);
-            FLAUNCH(k_f_at, nblk, dB, 4);
-            FLAUNCH(k_f_pm, nblk, dB, 5, 0, u == 0 ? 1 : 0);
+            FLAUNCH(k_f_sm, 0, 3);
+            FLAUNCH(k_f_at, 0, 4);
+            FLAUNCH(k_f_pm, 0, 5, 0, u == 0 ? 1 : 0);
         }
     };
-    // a graph only for one film: its tables never change between solves
-    const bool graph = nf == 1 && S0.use_graph;
-    if (graph && (!T.gexec || T.g_x != x[0] || T.g_maxit != maxit || T.g_window != S0.stall_window ||
-                  T.g_factor != S0.stall_factor || T.g_nblk != nblk)) {
-        if (T.gexec) { (void)hipGraphExecDestroy(T.gexec); T.gexec = nullptr; }
-        hipGraph_t gr = nullptr;
-        KR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        pair(false);
-        KR_CHECK(hipStreamEndCapture(st, &gr));
-        KR_CHECK(hipGraphInstantiate(&T.gexec, gr, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(gr);
-        T.g_x = x[0]; T.g_maxit = maxit; T.g_window = S0.stall_window; T.g_factor = S0.stall_factor;
-        T.g_nblk = nblk;
+    if (graph) {
+        // the graph's arguments: the grids, the tables, and (one film, whose
+        // descriptor goes by value) the film itself
+        std::vector<long long> key = {grp[0].nb, grp[1].nb, (long long)(intptr_t)T.d_blk[0],
+                                      (long long)(intptr_t)T.d_films, nf};
+        if (one) {
+            key.push_back((long long)(intptr_t)x[0]);
+            key.push_back(maxit);
+            key.push_back(S0.stall_window);
+            long long fb;
+            std::memcpy(&fb, &S0.stall_factor, sizeof(fb));
+            key.push_back(fb);
+            key.push_back((long long)(intptr_t)b[0]);
+        }
+        if (!T.gexec || key != T.g_key) {
+            if (T.gexec) { (void)hipGraphExecDestroy(T.gexec); T.gexec = nullptr; }
+            hipGraph_t gr = nullptr;
+            KR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            pair(false);
+            KR_CHECK(hipStreamEndCapture(st, &gr));
+            KR_CHECK(hipGraphInstantiate(&T.gexec, gr, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(gr);
+            T.g_key = key;
+        }
     }
-    FLAUNCH(k_f_init, nblk, dB);
-    FLAUNCH(k_f_pm, nblk, dB, 1, 1, 0);
+    FLAUNCH(k_f_init, 0);
+    FLAUNCH(k_f_pm, 0, 1, 1, 0);
     if (check()) return -1;
     auto running = [&](int f) { return (int)h[(size_t)f * S_N + S_STATE] == ST_RUN; };
     for (int f = 0; f < nf; ++f)
@@ -634,10 +697,13 @@ int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, cons
         int na = 0;
         for (int f = 0; f < nf; ++f) if (running(f)) active[na++] = f;
         if (!na) break;
-        if (na != last_na) {
+        if (!graph && na != last_na) {
             // only the films still running get blocks
-            nblk = fill(0, active.data(), na, true);
-            KR_CHECK(hipMemcpyAsync(T.d_blk[0], T.h_blk[0], (size_t)nblk * sizeof(int2), hipMemcpyHostToDevice, st));
+            fill(true, h);
+            KR_CHECK(hipMemcpyAsync(T.d_blk[0], T.h_blk[0], (size_t)(grp[0].nb + grp[1].nb) * sizeof(int2),
+                                    hipMemcpyHostToDevice, st));
+            KR_CHECK(hipMemcpyAsync(T.d_blk[1], T.h_blk[1], (size_t)(grp[0].nc + grp[1].nc) * sizeof(int2),
+                                    hipMemcpyHostToDevice, st));
             last_na = na;
         }
         for (int u = 0; u < CHUNK / 2; ++u) {
@@ -685,4 +751,86 @@ int bicgstab_stencil(SolverT<float> &S, const float *b, float *x, int maxit, dou
     return rc;
 }
 
+// ---- the tile tuner ------------------------------------------------------
+// The fused iteration's tile rows for a film of nx x ny cells, chosen by
+// measurement: the four kernels run `iters` iterations of a synthetic system
+// of that size (a unit diagonal and small couplings, the identity as M:
+// the same kernels, loads and launch shapes as a film's own system) for
+// each candidate, timed with HIP events on a stream of its own; the fastest
+// becomes the size's entry in the table that every context and batch of
+// the process reads (fused_tile_rows). A film's arithmetic depends on its
+// tile rows (the blocks' partial sums), so films of one size in one
+// process always solve with the same tiles.
+namespace {
+__global__ void k_tune_fill(int N, float *A, __half *Y, float *ms, float *b, float *x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int k = 0; k < 13; ++k) {
+        A[(size_t)k * N + i] = (k == 6) ? 1.0f : -0.04f;
+        Y[(size_t)k * N + i] = __float2half(k == 6 ? 1.0f : 0.0f);
+    }
+    ms[i] = 1.0f;
+    b[i] = 1.0f + 0.25f * (float)((i * 7919) % 13) / 13.0f;
+    x[i] = 0.0f;
+}
+}  // namespace
+
+int tune_tiles(int nx, int ny, int iters, int *rows_out, double *us_out) {
+    if (nx < 1 || ny < 1 || iters < 2) return -1;
+    hipStream_t st = nullptr;
+    KR_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    StencilOp op{};
+    op.dim = 2; op.nx = nx; op.ny = ny; op.N = nx * ny; op.ns = 13; op.perx = op.pery = 0;
+    for (int k = 0; k < 13; ++k) { op.ox[k] = OX[k]; op.oy[k] = OY[k]; }
+    SolverT<float> S;
+    __half *Y = nullptr;
+    float *ms = nullptr, *b = nullptr, *x = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = -1;
+    double best = 0;
+    int best_rows = kTileRows[0];
+    if (solver_init(S, op, st) == 0 && hipMalloc(&Y, (size_t)13 * op.N * sizeof(__half)) == hipSuccess &&
+        hipMalloc(&ms, (size_t)op.N * sizeof(float)) == hipSuccess &&
+        hipMalloc(&b, (size_t)op.N * sizeof(float)) == hipSuccess &&
+        hipMalloc(&x, (size_t)op.N * sizeof(float)) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess) {
+        hipLaunchKernelGGL(k_tune_fill, dim3((op.N + 255) / 256), dim3(256), 0, st, op.N, S.A, Y, ms, b, x);
+        S.Mh = Y; S.Mscale = ms; S.unit_diag = 1; S.stall_window = 0;
+        rc = 0;
+        for (int u = 0; u < 2 && rc == 0; ++u) {
+            fused_tile_set(nx, ny, kTileRows[u]);
+            int it = 0;
+            double r = 0;
+            // one untimed solve, then three timed (no tolerance: every one runs `iters`)
+            for (int rep = 0; rep < 4 && rc == 0; ++rep) {
+                if (rep == 1) (void)hipEventRecord(e0, st);
+                if (bicgstab_stencil(S, b, x, iters, 0.0, &it, &r) == 2) rc = -1;
+            }
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = -1;
+            float ms_ = 0;
+            (void)hipEventElapsedTime(&ms_, e0, e1);
+            const double us = 1e3 * ms_ / (3.0 * iters);
+            if (us_out) us_out[u] = us;
+            if (u == 0 || us < best) { best = us; best_rows = kTileRows[u]; }
+        }
+    }
+    fused_tile_set(nx, ny, best_rows);
+    if (rows_out) *rows_out = best_rows;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(Y); (void)hipFree(ms); (void)hipFree(b); (void)hipFree(x);
+    solver_free(S);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
 }  // namespace yz
+
+extern "C" {
+int yz_tune_tiles(int nx, int ny, int iters, int *rows, double *us2) {
+    return yz::tune_tiles(nx, ny, iters, rows, us2) == 0 ? 0 : 2;
+}
+int yz_tile_rows(int nx, int ny) { return yz::fused_tile_rows(nx, ny); }
+int yz_set_tile_rows(int nx, int ny, int rows) { return yz::fused_tile_set(nx, ny, rows) == 0 ? 0 : 2; }
+}

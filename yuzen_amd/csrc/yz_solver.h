@@ -46,11 +46,13 @@ struct FusedTable {
     double *d_gather = nullptr, *h_gather = nullptr;
     int cap_films = 0, cap_blk = 0;
     hipGraphExec_t gexec = nullptr;
-    const float *g_x = nullptr;
-    int g_maxit = -1, g_window = -1, g_nblk = -1;
-    double g_factor = 0;
+    std::vector<long long> g_key;   // what the captured graph was made for
 };
 void fused_table_free(FusedTable &T);
+// the fused iteration's tile rows for a film of nx x ny cells (the tile
+// tuner's choice, 16 when it has none), and setting that choice (8 or 16)
+int fused_tile_rows(int nx, int ny);
+int fused_tile_set(int nx, int ny, int rows);
 
 template <typename T>
 struct Krylov {

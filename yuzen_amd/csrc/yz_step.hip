@@ -1941,6 +1941,7 @@ int yz_batch_advance(yz_batch *B, int *status, int *nk_iter, int *lin_iter) {
             S->S32.Mh = S->Yh;
             S->S32.Mscale = S->mscale;
             S->S32.stall_window = S->cs.stall_window;
+            S->S32.use_graph = S->cs.graphs;
             sv[u] = &S->S32; bv[u] = S->b32; xv[u] = S->corr32;
             tol[u] = fp32_tol(S->cs, F.eps, F.resid_first, F.outer > 0 ? F.resid_eq : 0.0);
             if (zero_guess(S->cs)) YZ_TRY(hipMemsetAsync(S->corr32, 0, S->g.N * sizeof(float), B->st));
