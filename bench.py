@@ -29,10 +29,15 @@ together, in cell-steps/s, beside the same films advanced one at a time;
 and (unless --no-concz) the ConcZ film of config 5, 256^2 x 32, in FP64
 steps/s beside the CPU oracle's rate.
 
-Roofline: the dominant kernel of the learned run, the fine-level stencil
-product of the fused Krylov iteration (k_f_av: v = A ph with (rh, v)),
-timed with HIP events on the step's own stream over the timed region. CPU baseline: the
-FP64 CPU oracle on one host core over a bounded sample of the same workload.
+Before each learned run the tile tuner picks the fused iteration's tile
+rows for the film's size (yz_tune_tiles; the choice is in the line).
+
+Roofline: the kernel of the learned run with the largest share of its
+time, the first kernel of the fused Krylov iteration (K1, k_f_pm: the x and
+r update, p, ph = M p and (r, r)), its launches timed over the timed region
+by the start and stop stamps of the dispatch itself (hipExtLaunchKernel) on
+the step's own stream. CPU baseline: the FP64 CPU oracle over a bounded
+sample of the same workload.
 """
 import argparse
 import json
@@ -50,7 +55,11 @@ if ROOT not in sys.path:
 METRIC = "Newton steps/s, 1024² 2D film: learned-precond FP32 vs plain FP64 AMG @1e-8"
 HBM_PEAK_GBS = 8000.0
 DT = 2e-6
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "f_av_fp32_allphys1024_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "f_pm_fp32_allphys1024_traffic.json")
+# algorithmic bytes per cell of one K1 launch: s, t, p_old, v read once for
+# the staged p (16), x, ph, sh (12), M's 13 FP16 rows and FP32 row scale
+# (30); x, r, p, ph written (16)
+K1_BYTES_PER_CELL = 16 + 12 + 30 + 16
 
 
 def cpu_model():
@@ -141,8 +150,15 @@ def sweep_leg(n_films=256, steps=3, warmup=1, alone=16):
     g.learn_train(300, 2e-3, 4)
     params = g.learn_params()
     g.close()
+    from yuzen_amd import capi
     films = cases.sweep(n_films)
     cells = sum(f[0].nx * f[0].ny for f in films)
+    # the tile rows of each side, tuned on as many films of it as the sweep has
+    tiles = {}
+    for n in cases.SWEEP_SIDES:
+        k = sum(1 for f in films if f[0].nx == n)
+        rows, us = capi.tune_tiles(n, n, nfilms=max(k, 1), iters=20)
+        tiles[str(n)] = {"films": k, "rows": rows, "us_per_iter_16": round(us[0], 2), "us_per_iter_8": round(us[1], 2)}
     B = HipBatch(films)
     B.set_params(params)
     for _ in range(warmup):
@@ -159,6 +175,23 @@ def sweep_leg(n_films=256, steps=3, warmup=1, alone=16):
         passes += st[1]
         lins += st[2]
     fallbacks = int(B.stats()[3])
+    B.close()
+    # the same batch with its iteration replayed from a captured HIP graph
+    gfilms = []
+    for c, h, s, cc, dt in films:
+        c = c.copy()
+        c.graphs = 1
+        gfilms.append((c, h, s, cc, dt))
+    B = HipBatch(gfilms)
+    B.set_params(params)
+    for _ in range(warmup):
+        B.advance()
+    elg = 0.0
+    for _ in range(steps):
+        t = time.perf_counter()
+        if B.advance()[0] != 0:
+            raise RuntimeError("sweep step failed (graph)")
+        elg += time.perf_counter() - t
     B.close()
     sub = films[:alone]
     sub_cells = sum(f[0].nx * f[0].ny for f in sub)
@@ -179,6 +212,8 @@ def sweep_leg(n_films=256, steps=3, warmup=1, alone=16):
         "films": n_films, "cells": cells, "steps": steps, "warmup": warmup,
         "batch_steps_per_s": round(steps / el, 4),
         "cell_steps_per_s": round(cells * steps / el, 1),
+        "batch_steps_per_s_graph": round(steps / elg, 4),
+        "tile_rows_by_side": tiles,
         "newton_passes_per_film_step": round(passes / steps / n_films, 3),
         "krylov_iters_per_pass": round(lins / max(passes, 1), 2),
         "fp64_fallbacks": fallbacks,
@@ -271,6 +306,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-rows", type=int, default=0,
+                    help="pin the learned run's tile rows (8 or 16) instead of tuning them")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched sweep")
     ap.add_argument("--no-concz", action="store_true", help="skip the ConcZ film (config 5)")
@@ -288,6 +325,11 @@ def main():
     from yuzen_amd import capi, cases
 
     c, h, s, cc, dt, _ = cases.allphys_2d(n=args.n, dt=DT)
+    if args.tile_rows:
+        capi.set_tile_rows(args.n, args.n, args.tile_rows)
+        tile_rows, tile_us = args.tile_rows, [0.0, 0.0]
+    else:
+        tile_rows, tile_us = capi.tune_tiles(args.n, args.n, iters=20)
     cl = c.copy()
     cl.precond, cl.krylov_fp32 = capi.PC_LEARNED, 1
     cl.learn_passes, cl.learn_iters, cl.learn_probes, cl.learn_lr = 3, 150, 4, 2e-3
@@ -305,10 +347,7 @@ def main():
         P0 = run(cp0, h, s, cc, args.steps, args.warmup)
 
     N = c.nx * c.ny
-    # algorithmic bytes of one launch of the fine-level product v = A p,
-    # (rh, v): the 12 off-diagonal FP32 planes (the row-scaled diagonal is
-    # exactly 1 and is not read), p once, rh once, v written
-    bytes_per_launch = N * (12 * 4 + 4 + 4 + 4)
+    bytes_per_launch = N * K1_BYTES_PER_CELL
     avg_s = (L["kms"] / 1e3) / max(L["kcount"], 1)
     achieved = bytes_per_launch / avg_s / 1e9 if L["kcount"] else 0.0
     traffic, traffic_src = None, None
@@ -325,7 +364,9 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "yz::k_f_av (fused FP32 iteration: v = A ph on 64x16 LDS-staged tiles + (rh, v))",
+        "kernel": f"yz::k_f_pm (K1 of the fused FP32 iteration: x, r update, p, ph = M p on 64x{tile_rows} "
+                  "LDS-staged tiles, (r, r))",
+        "timing": "dispatch start/stop stamps (hipExtLaunchKernel) of one K1 per chunk of iterations",
         "launches": L["kcount"],
         "avg_us": round(avg_s * 1e6, 2),
         "bytes_per_launch": bytes_per_launch,
@@ -342,6 +383,7 @@ def main():
                 zip(["assembly", "residual", "setup", "krylov", "update", "step"], R["phases"])}
 
     v_l = args.steps / L["el"]
+    train_s = float(L["learn"][2]) / 1e3
     out = {
         "metric": METRIC,
         "value": round(v_l, 4),
@@ -365,6 +407,8 @@ def main():
             "initial_guess": "learned: zero per Newton pass; plain: the previous pass's correction (the reference's)",
             "domain": "[0,10]^2, Symmetry walls",
         },
+        "steps_per_s_incl_training": round(args.steps / (L["el"] + train_s), 4),
+        "tile_rows": {"chosen": tile_rows, "us_per_iter_16": round(tile_us[0], 2), "us_per_iter_8": round(tile_us[1], 2)},
         "newton_passes_per_step": round(L["passes"] / args.steps, 3),
         "krylov_iters_per_pass": round(L["lins"] / max(L["passes"], 1), 2),
         "phases_ms_per_step": phases(L),

@@ -221,12 +221,15 @@ int yz_step_load_state(yz_step *st, const char *path);
  * [0] assembly, [1] residual, [2] AMG setup, [3] Krylov, [4] update,
  * [5] whole step */
 int yz_step_timing(yz_step *st, double *ms6);
-/* launch timing of the fine-level stencil product of the Krylov iteration
- * (the SpMV v = A p with its fused dot product) of the case's own solve
- * (the FP32 inner solve when the case runs it), HIP events on the step's
- * stream: returns the launches timed and their total milliseconds since
- * the last reset; enable 1 resets and starts, 0 resets and stops, -1 only
- * reads */
+/* launch timing of one kernel of the Krylov iteration of the case's own
+ * solve, HIP events on the step's stream: with the fused FP32 iteration of
+ * the learned preconditioner, its first kernel (K1: the x and r update,
+ * p, ph = M p and (r, r)) once per chunk of iterations, with the start and
+ * stop stamps of the dispatch itself (hipExtLaunchKernel); otherwise the
+ * fine-level stencil product (the SpMV v = A p with its fused dot product)
+ * of every iteration between two event markers. Returns the launches timed
+ * and their total milliseconds since the last reset; enable 1 resets and
+ * starts, 0 resets and stops, -1 only reads */
 int yz_step_kernel_time(yz_step *st, int enable, long long *count, double *total_ms);
 void yz_step_destroy(yz_step *st);
 
@@ -389,15 +392,17 @@ void yz_batch_destroy(yz_batch *b);
 
 /* ---- tile tuner of the fused FP32 iteration (no reference counterpart) --
  * The learned path's Krylov kernels stage 64 x R tiles (R, the tile rows,
- * 8 or 16). yz_tune_tiles times `iters` iterations of a synthetic system of
- * nx x ny cells for each R, three solves each, and makes the fastest the
- * process's choice for that size; rows gets it and us2 the microseconds
- * per iteration of R = 16 and R = 8. yz_tile_rows returns the choice in
- * force (16 when none was made), yz_set_tile_rows pins one (returns 2 for
- * another R). A film's result depends on its tile rows (the order of the
- * dot products' partial sums), so every context and batch of the process
- * uses the same R for films of one size. Return codes: 0, 2 device error. */
-int yz_tune_tiles(int nx, int ny, int iters, int *rows, double *us2);
+ * 8 or 16). yz_tune_tiles solves `nfilms` synthetic systems of nx x ny
+ * cells together (one batch, as a sweep solves its films of that size) for
+ * `iters` iterations with each R, three times, and makes 8 the process's
+ * choice for that size when it is faster by more than 3 % (16 otherwise);
+ * rows gets the choice and us2 the microseconds per iteration with R = 16
+ * and R = 8. yz_tile_rows returns the choice in force (16 when none was
+ * made), yz_set_tile_rows pins one (returns 2 for another R). A film's
+ * result depends on its tile rows (the order of the dot products' partial
+ * sums), so every context and batch of the process uses the same R for
+ * films of one size. Return codes: 0, 2 device error or bad argument. */
+int yz_tune_tiles(int nx, int ny, int nfilms, int iters, int *rows, double *us2);
 int yz_tile_rows(int nx, int ny);
 int yz_set_tile_rows(int nx, int ny, int rows);
 

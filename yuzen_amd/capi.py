@@ -230,7 +230,7 @@ def load():
     lib.yz_batch_advance.argtypes = [C.c_void_p, pi_, pi_, pi_]
     lib.yz_batch_get_h.argtypes = [C.c_void_p, i, pd]
     lib.yz_batch_stats.argtypes = [C.c_void_p, pd]
-    lib.yz_tune_tiles.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), pd]
+    lib.yz_tune_tiles.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), pd]
     lib.yz_tile_rows.argtypes = [C.c_int, C.c_int]
     lib.yz_set_tile_rows.argtypes = [C.c_int, C.c_int, C.c_int]
     lib.yz_batch_destroy.argtypes = [C.c_void_p]
@@ -280,13 +280,13 @@ def last_error():
     return e.decode() if e else ""
 
 
-def tune_tiles(nx, ny, iters=40):
-    """the tile tuner for films of nx x ny cells: (the tile rows chosen,
-    [us per iteration with 16 rows, with 8])"""
+def tune_tiles(nx, ny, nfilms=1, iters=40):
+    """the tile tuner for films of nx x ny cells, solved nfilms at a time:
+    (the tile rows chosen, [us per iteration with 16 rows, with 8])"""
     lib = load()
     rows = C.c_int()
     us = (C.c_double * 2)()
-    if lib.yz_tune_tiles(nx, ny, iters, C.byref(rows), us):
+    if lib.yz_tune_tiles(nx, ny, nfilms, iters, C.byref(rows), us):
         raise RuntimeError("yz_tune_tiles: " + last_error())
     return rows.value, [us[0], us[1]]
 
