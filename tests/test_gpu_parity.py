@@ -144,3 +144,72 @@ def test_linsolve_seam_takes_the_concentration_system(dim):
     assert rc == 0 and rc0 == 0
     assert np.abs(x - x0).max() <= 1e-10 * max(np.abs(x0).max(), 1e-300)
     assert np.abs(x0).max() > 0
+
+
+def test_linsolve_seam_return_codes():
+    """The backend contract's codes through the seam: 1 when the iteration
+    cap is reached first, 2 for a matrix that is not a structured film or
+    concentration system (a row that is neither a copy row nor within the
+    stencil), and the matrix and right-hand side left as given."""
+    c, h, s, cc = smooth_case(2, TERMS, slip=0.05)
+    o = ob.OracleStep(c, h, s, cc, 1e-3)
+    o.set_h(h)
+    hf = o.h()
+    A, J, IA, ax, rhs = o.assemble(hf * 1.01, hf, 5e-3)
+    rp, col, bc = o.pattern()
+    IA0, rhs0 = IA.copy(), rhs.copy()
+    ls = LinSolver()
+    rc, x, it, r = ls.solve(rp, col, IA, rhs, np.zeros(o.n), epsilon=1e-14, max_iter=2)
+    print(f"cap: rc={rc} it={it} resid={r:.2e}")
+    assert rc == 1 and it == 2 and r > 1e-14
+    assert np.array_equal(IA, IA0) and np.array_equal(rhs, rhs0)
+    bad = IA.copy()
+    g = int(np.flatnonzero(bc)[0])                       # a ghost row: not a copy row any more
+    bad[rp[g]:rp[g + 1]] = 1.0
+    rc, x, it, r = ls.solve(rp, col, bad, rhs, np.zeros(o.n), epsilon=1e-12)
+    assert rc == 2
+    far = col.copy()
+    i = (c.nx + 4) * 6 + 6                              # an interior row reaching past the stencil
+    far[rp[i] + 1] = i + 3
+    rc, x, it, r = ls.solve(rp, far, IA, rhs, np.zeros(o.n), epsilon=1e-12, grid=(2, c.nx, c.ny, [0, 0, 0, 0]))
+    assert rc == 2
+
+
+def test_linsolve_context_reused_across_shapes():
+    """One context solving systems of the same size but another grid (the
+    transposed shape) or other edges (periodic, then Symmetry) gives what a
+    fresh context gives: its multigrid is rebuilt for the new operator."""
+    ls = LinSolver()
+    for nx, ny, per in ((18, 14, False), (14, 18, False), (14, 18, True), (14, 18, False)):
+        c, h, s, cc = smooth_case(2, TERMS, periodic=per, nx=nx, ny=ny)
+        o = ob.OracleStep(c, h, s, cc, 1e-3)
+        o.set_h(h)
+        hf = o.h()
+        _, _, IA, _, rhs = o.assemble(hf * 1.01, hf, 5e-3)
+        rp, col, _ = o.pattern()
+        rc, x, it, _ = ls.solve(rp, col, IA, rhs, np.zeros(o.n), epsilon=1e-12)
+        rc1, x1, it1, _ = LinSolver().solve(rp, col, IA, rhs, np.zeros(o.n), epsilon=1e-12)
+        print(f"{nx}x{ny} periodic={per}: reused it={it}, fresh it={it1}")
+        assert rc == 0 and rc1 == 0 and it == it1
+        assert np.array_equal(x, x1)
+
+
+def test_linsolve_seam_time_at_1024():
+    """The seam on the 1024^2 all-physics film's IA (1 056 784 rows): the
+    solution to the tolerance, and the time of a call (the CRS copied to the
+    device, folded into planes there, the multigrid set up, the solve, x
+    back) printed beside the solve alone."""
+    import time
+    c, h, s, cc, dt, _ = cases.allphys_2d(n=1024, dt=2e-6)
+    o = ob.OracleStep(c, h, s, cc, dt)
+    o.set_h(h)
+    hf = o.h()
+    _, _, IA, _, rhs = o.assemble(hf, hf, dt)
+    rp, col, _ = o.pattern()
+    ls = LinSolver()
+    ls.solve(rp, col, IA, rhs, np.zeros(o.n), epsilon=1e-8)        # first call: allocations
+    t = time.perf_counter()
+    rc, x, it, r = ls.solve(rp, col, IA, rhs, np.zeros(o.n), epsilon=1e-8)
+    el = time.perf_counter() - t
+    print(f"seam at 1024^2: rc={rc} it={it} resid={r:.2e}, {1e3 * el:.1f} ms per call")
+    assert rc == 0 and r <= 1e-8

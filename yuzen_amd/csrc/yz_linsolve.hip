@@ -3,8 +3,8 @@
 // (reference yuzen/src/numerics/LinearSolverBackend.h:56-103; the AMGCL
 // backend's Solve, LinearSolverBackend.cpp:539-728): x is the initial guess
 // and the result, the matrix and right-hand side are not modified, the
-// codes are 0 converged / 1 iteration cap / 2 breakdown, and nothing
-// throws across the boundary.
+// codes are 0 converged / 1 iteration cap / 2 breakdown or a matrix the
+// seam does not take, and nothing throws across the boundary.
 //
 // The CRS is a film-height or well-mixed concentration matrix of a
 // structured grid in the reference's row layout: two ghost layers per side
@@ -12,17 +12,21 @@
 // Symmetry and Periodic edges), interior rows within the 13-point stencil.
 // The grid comes from the settings or is read off the matrix itself (the
 // column offsets of an interior row of the 13-point film-height pattern or
-// of the 9-point wide concentration pattern, ConcEquationBase.h:242-258). The ghost rows are
-// eliminated, the interior rows become stencil planes, and the system is
-// solved on the device; the ghost entries of x are formed again from their
-// rows afterwards, as CRS_Matrix::SatisfyBCRows makes them.
+// of the 9-point wide concentration pattern, ConcEquationBase.h:242-258).
+//
+// The CRS goes to the device as it is; one thread per ghost row checks that
+// it is a copy row and records its source and offset, one thread per
+// interior row folds its entries into the stencil planes (ghost columns
+// onto the cells they copy, their offsets into the right-hand side) and
+// scales it; the system is solved on the device, and the ghost entries of
+// x are formed again from their rows (as CRS_Matrix::SatisfyBCRows makes
+// them) before x comes back.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
-#include <vector>
 
 #include "../../include/yuzen_gpu.h"
 #include "yz_physics.h"
@@ -30,15 +34,249 @@
 
 using namespace yz;
 
-namespace {
-thread_local std::string g_lin_err;
+namespace yz {
+void set_error(const std::string &s);
 }
+
+namespace {
+
+constexpr int LB = 256;
+inline int lblk(long n) { return (int)((n + LB - 1) / LB); }
+
+// the grid of a system and how its ghost rows copy
+struct LinGrid {
+    int n, nnz, dim, nx, ny, nxp, nyp;
+    int bc[4];
+};
+
+__device__ inline bool lin_inside(const LinGrid &G, int i) {
+    const int ix = i % G.nxp, iy = i / G.nxp;
+    return ix >= 2 && ix <= G.nxp - 3 && (G.dim == 1 || (iy >= 2 && iy <= G.nyp - 3));
+}
+
+// ghost rows: d x_g + o x_src = r_g with o = -d, so x_g = x_src + r_g / d
+__global__ void k_lin_ghosts(LinGrid G, const int *rowptr, const int *col, const double *val, const double *rhs,
+                             int *gsrc, double *gofs, int *bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.n) return;
+    if (lin_inside(G, i)) { gsrc[i] = -1; gofs[i] = 0.0; return; }
+    double d = 0, o = 0;
+    int src = -1, m = 0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        const double v = val[k];
+        if (v == 0) continue;
+        if (col[k] == i) d += v;
+        else { src = col[k]; o += v; ++m; }
+    }
+    if (m != 1 || d == 0 || o != -d || src < 0 || src >= G.n || !lin_inside(G, src)) {
+        atomicOr(bad, 1);
+        gsrc[i] = -1;
+        gofs[i] = 0.0;
+        return;
+    }
+    gsrc[i] = src;
+    gofs[i] = rhs[i] / d;
+}
+
+// an interior row into the planes: its entries by slot (ghost columns fold
+// onto the cell they copy, their offsets into b), scaled by its diagonal
+__global__ void k_lin_rows(LinGrid G, StencilOp op, int scaling, const int *rowptr, const int *col,
+                           const double *val, const double *rhs, const double *xin, const int *gsrc,
+                           const double *gofs, double *planes, double *b, double *x, int *bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    const int r = (G.dim == 2 ? iy + 2 : 0) * G.nxp + ix + 2;
+    double acc[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) acc[q] = 0.0;
+    double bi = rhs[r];
+    for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+        const int cc = col[k];
+        const double v = val[k];
+        const int dxo = cc % G.nxp - (ix + 2), dyo = cc / G.nxp - (G.dim == 2 ? iy + 2 : 0);
+        int slot = -1;
+        for (int q = 0; q < op.ns; ++q)
+            if (op.ox[q] == dxo && op.oy[q] == dyo) slot = q;
+        if (slot < 0 || cc < 0 || cc >= G.n) {
+            if (v != 0) atomicOr(bad, 2);
+            continue;
+        }
+        acc[slot] += v;
+        if (gsrc[cc] >= 0) bi -= v * gofs[cc];
+    }
+    const int c0 = G.dim == 2 ? 6 : 2;
+    const double d = acc[c0];
+    const bool sc = scaling && !(fabs(d) < 1.0e-30);
+    const double inv = sc ? 1.0 / d : 1.0;
+    for (int q = 0; q < op.ns; ++q) planes[(size_t)q * op.N + i] = sc ? acc[q] * inv : acc[q];
+    b[i] = sc ? bi * inv : bi;
+    x[i] = xin[r];
+}
+
+// the solution back into the padded rows: interior cells, then ghosts
+__global__ void k_lin_out_interior(LinGrid G, StencilOp op, const double *x, double *xout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= op.N) return;
+    const int ix = i % op.nx, iy = i / op.nx;
+    xout[(G.dim == 2 ? iy + 2 : 0) * G.nxp + ix + 2] = x[i];
+}
+__global__ void k_lin_out_ghosts(LinGrid G, const int *gsrc, const double *gofs, double *xout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.n) return;
+    if (gsrc[i] >= 0) xout[i] = xout[gsrc[i]] + gofs[i];
+}
+
+// the device arrays of one solve besides the solver's own
+struct LinBufs {
+    int *rowptr = nullptr, *col = nullptr, *gsrc = nullptr, *flag = nullptr;
+    double *val = nullptr, *rhs = nullptr, *xin = nullptr, *gofs = nullptr, *xout = nullptr;
+    double *b = nullptr, *x = nullptr;
+};
+
+// the grid of a system: given by the settings, or read off an interior
+// row's pattern (the largest forward offset of the 13-point film-height or
+// 9-point concentration row is two padded rows, of the 5-point 1-D row 2)
+int lin_grid(int n, int nnz, const int *rowptr, const int *col, const yz_lin_settings *s, LinGrid &G) {
+    G.n = n; G.nnz = nnz;
+    G.dim = s->grid_dim; G.nx = s->grid_nx; G.ny = s->grid_ny;
+    for (int k = 0; k < 4; ++k) G.bc[k] = s->bc[k];
+    if (G.nx <= 0) {
+        int maxoff = 0, kind = 0;
+        for (int i = 0; i < n && kind == 0; ++i) {
+            const int m = rowptr[i + 1] - rowptr[i];
+            if (m != 13 && m != 9 && m != 5) continue;
+            int mo = 0;
+            for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) mo = std::max(mo, col[k] - i);
+            if ((m == 13 || m == 9) && mo >= 4 && mo % 2 == 0) { maxoff = mo; kind = 13; }
+            else if (m == 5 && mo == 2) { maxoff = mo; kind = 5; }
+        }
+        if (kind == 13) {
+            const int nxp = maxoff / 2;
+            if (n % nxp != 0) return 2;
+            G.dim = 2; G.nx = nxp - 4; G.ny = n / nxp - 4;
+        } else if (kind == 5) {
+            G.dim = 1; G.nx = n - 4; G.ny = 1;
+        } else {
+            return 2;
+        }
+        G.bc[0] = G.bc[1] = G.bc[2] = G.bc[3] = -1;
+    }
+    G.nxp = G.nx + 4;
+    G.nyp = G.dim == 2 ? G.ny + 4 : 1;
+    if (G.nx < 3 || (G.dim == 2 && G.ny < 3) || G.nxp * G.nyp != n) return 2;
+    return 0;
+}
+
+StencilOp lin_op(const LinGrid &G) {
+    StencilOp op;
+    std::memset(&op, 0, sizeof(op));
+    op.dim = G.dim; op.nx = G.nx; op.ny = G.dim == 2 ? G.ny : 1; op.N = op.nx * op.ny;
+    op.ns = G.dim == 2 ? 13 : 5;
+    op.perx = G.bc[0] == YZ_BC_PERIODIC; op.pery = G.dim == 2 && G.bc[2] == YZ_BC_PERIODIC;
+    for (int k = 0; k < op.ns; ++k) { op.ox[k] = slot_ox(G.dim, k); op.oy[k] = slot_oy(G.dim, k); }
+    return op;
+}
+
+#define LIN_TRY(x)                                                        \
+    do {                                                                  \
+        if ((x) != hipSuccess) { yz::set_error("HIP error in the linear-solve seam"); return 2; } \
+    } while (0)
+
+// the solve on device buffers B (sized for the system) and solver S
+int lin_solve(Solver &S, LinBufs &B, LinGrid G, const int *rowptr, const int *col, const double *val,
+              const double *rhs, double *x_inout, const yz_lin_settings *s, int *iter, double *resid,
+              hipStream_t st) {
+    const int n = G.n, nnz = G.nnz;
+    LIN_TRY(hipMemcpyAsync(B.rowptr, rowptr, (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+    LIN_TRY(hipMemcpyAsync(B.col, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice, st));
+    LIN_TRY(hipMemcpyAsync(B.val, val, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice, st));
+    LIN_TRY(hipMemcpyAsync(B.rhs, rhs, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    LIN_TRY(hipMemcpyAsync(B.xin, x_inout, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    int zero2[2] = {0, 0};
+    LIN_TRY(hipMemcpyAsync(B.flag, zero2, sizeof(zero2), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_lin_ghosts, dim3(lblk(n)), dim3(LB), 0, st, G, B.rowptr, B.col, B.val, B.rhs, B.gsrc,
+                       B.gofs, B.flag);
+    // the edges' kinds from where the first ghosts copy from, when not given
+    if (G.bc[0] < 0) {
+        int src[2] = {-1, -1};
+        const int r0 = (G.dim == 2 ? 2 : 0) * G.nxp;      // first ghost of the first real row
+        LIN_TRY(hipMemcpyAsync(&src[0], B.gsrc + r0, sizeof(int), hipMemcpyDeviceToHost, st));
+        if (G.dim == 2) LIN_TRY(hipMemcpyAsync(&src[1], B.gsrc + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+        LIN_TRY(hipStreamSynchronize(st));
+        G.bc[0] = G.bc[1] = (src[0] >= 0 && src[0] % G.nxp == G.nxp - 4) ? YZ_BC_PERIODIC : YZ_BC_SYMMETRY;
+        if (G.dim == 2) G.bc[2] = G.bc[3] = (src[1] >= 0 && src[1] / G.nxp == G.nyp - 4) ? YZ_BC_PERIODIC : YZ_BC_SYMMETRY;
+        else G.bc[2] = G.bc[3] = YZ_BC_SYMMETRY;
+    }
+    const StencilOp op = lin_op(G);
+    if (solver_init(S, op, st)) { yz::set_error("solver set-up failed"); return 2; }
+    hipLaunchKernelGGL(k_lin_rows, dim3(lblk(op.N)), dim3(LB), 0, st, G, op, s->scaling, B.rowptr, B.col, B.val,
+                       B.rhs, B.xin, B.gsrc, B.gofs, S.A, B.b, B.x, B.flag);
+    int bad = 0;
+    LIN_TRY(hipMemcpyAsync(&bad, B.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    LIN_TRY(hipStreamSynchronize(st));
+    if (bad) { yz::set_error("the matrix is not a structured film or concentration system"); return 2; }
+    const int pc = s->precond == YZ_PC_AMG ? YZ_PC_AMG : YZ_PC_JACOBI;
+    int it = 0, tried = 0;
+    double rr = 0;
+    int rc = -1;
+    if (s->conc && pc == YZ_PC_AMG) {
+        // a concentration system: damped Jacobi first, at most 300
+        // iterations; from the guess again with the multigrid if it fails
+        if (jacobi_setup(S) != 0) return 2;
+        rc = bicgstab(S, B.b, B.x, YZ_PC_JACOBI, std::min(s->max_iter, 300), s->epsilon, &it, &rr);
+        if (rc != 0) {
+            tried = it;
+            hipLaunchKernelGGL(k_lin_rows, dim3(lblk(op.N)), dim3(LB), 0, st, G, op, s->scaling, B.rowptr, B.col,
+                               B.val, B.rhs, B.xin, B.gsrc, B.gofs, S.A, B.b, B.x, B.flag);
+        }
+    }
+    if (rc != 0) {
+        if ((pc == YZ_PC_AMG ? amg_setup(S) : jacobi_setup(S)) != 0) return 2;
+        rc = bicgstab(S, B.b, B.x, pc, s->max_iter, s->epsilon, &it, &rr);
+        it += tried;
+    }
+    LIN_TRY(hipMemcpyAsync(B.xout, B.xin, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_lin_out_interior, dim3(lblk(op.N)), dim3(LB), 0, st, G, op, B.x, B.xout);
+    hipLaunchKernelGGL(k_lin_out_ghosts, dim3(lblk(n)), dim3(LB), 0, st, G, B.gsrc, B.gofs, B.xout);
+    LIN_TRY(hipMemcpyAsync(x_inout, B.xout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    LIN_TRY(hipStreamSynchronize(st));
+    if (iter) *iter = it;
+    if (resid) *resid = rr;
+    return rc;
+}
+
+// the arrays of a system of n rows and nnz entries, in a row, 256-aligned:
+// from the context's allocation or from an arena
+template <class Carve>
+bool lin_bufs(LinBufs &B, int n, int nnz, int N, Carve carve) {
+    B.rowptr = (int *)carve((size_t)(n + 1) * sizeof(int));
+    B.col = (int *)carve((size_t)nnz * sizeof(int));
+    B.gsrc = (int *)carve((size_t)n * sizeof(int));
+    B.flag = (int *)carve(2 * sizeof(int));
+    B.val = (double *)carve((size_t)nnz * sizeof(double));
+    B.rhs = (double *)carve((size_t)n * sizeof(double));
+    B.xin = (double *)carve((size_t)n * sizeof(double));
+    B.gofs = (double *)carve((size_t)n * sizeof(double));
+    B.xout = (double *)carve((size_t)n * sizeof(double));
+    B.b = (double *)carve((size_t)N * sizeof(double));
+    B.x = (double *)carve((size_t)N * sizeof(double));
+    return B.rowptr && B.col && B.gsrc && B.flag && B.val && B.rhs && B.xin && B.gofs && B.xout && B.b && B.x;
+}
+size_t lin_bufs_bytes(int n, int nnz, int N) {
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return up((size_t)(n + 1) * 4) + up((size_t)nnz * 4) + up((size_t)n * 4) + up(8) + up((size_t)nnz * 8) +
+           5 * up((size_t)n * 8) + 2 * up((size_t)N * 8);
+}
+
+}  // namespace
 
 struct yz_linctx {
     Solver S;
     hipStream_t st = nullptr;
-    double *b = nullptr, *x = nullptr;
-    int N = 0;
+    LinBufs B;
+    char *mem = nullptr;
+    size_t mem_bytes = 0;
 };
 
 extern "C" {
@@ -52,8 +290,7 @@ yz_linctx *yz_linsolve_create(void) {
 void yz_linsolve_destroy(yz_linctx *c) {
     if (!c) return;
     solver_free(c->S);
-    if (c->b) (void)hipFree(c->b);
-    if (c->x) (void)hipFree(c->x);
+    if (c->mem) (void)hipFree(c->mem);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -72,140 +309,26 @@ int yz_linsolve(yz_linctx *ctx, int n, int nnz, const int *rowptr, const int *co
     if (iter) *iter = 0;
     if (resid) *resid = 0;
     if (!ctx || !rowptr || !col || !val || !rhs || !x_inout || !s || n <= 0 || nnz < n) return 2;
-    // ---- the grid: given, or read off the pattern ----
-    int dim = s->grid_dim, nx = s->grid_nx, ny = s->grid_ny;
-    int bc[4] = {s->bc[0], s->bc[1], s->bc[2], s->bc[3]};
-    if (nx <= 0) {
-        int maxoff = 0, cnt = 0;
-        for (int i = 0; i < n && cnt == 0; ++i) {
-            const int m = rowptr[i + 1] - rowptr[i];
-            if (m != 13 && m != 9 && m != 5) continue;
-            int mo = 0;
-            for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) mo = std::max(mo, col[k] - i);
-            if ((m == 13 || m == 9) && mo >= 4 && mo % 2 == 0) { maxoff = mo; cnt = 13; }
-            else if (m == 5 && mo == 2) { maxoff = mo; cnt = 5; }
-        }
-        if (cnt == 13) {
-            const int nxp = maxoff / 2;
-            if (n % nxp != 0) return 2;
-            dim = 2; nx = nxp - 4; ny = n / nxp - 4;
-        } else if (cnt == 5) {
-            dim = 1; nx = n - 4; ny = 1;
-        } else return 2;
-        bc[0] = bc[1] = bc[2] = bc[3] = -1;
+    LinGrid G;
+    if (lin_grid(n, nnz, rowptr, col, s, G)) { yz::set_error("no structured grid in the matrix"); return 2; }
+    const int N = G.nx * (G.dim == 2 ? G.ny : 1);
+    const size_t need = lin_bufs_bytes(n, nnz, N);
+    if (need > ctx->mem_bytes) {
+        if (ctx->mem) (void)hipFree(ctx->mem);
+        ctx->mem = nullptr;
+        ctx->mem_bytes = 0;
+        if (hipMalloc((void **)&ctx->mem, need) != hipSuccess) { yz::set_error("device allocation failed"); return 2; }
+        ctx->mem_bytes = need;
     }
-    const int nxp = nx + 4, nyp = dim == 2 ? ny + 4 : 1;
-    if (nx < 3 || (dim == 2 && ny < 3) || nxp * nyp != n) return 2;
-    auto inside = [&](int ix, int iy) {
-        return ix >= 2 && ix <= nxp - 3 && (dim == 1 || (iy >= 2 && iy <= nyp - 3));
+    size_t used = 0;
+    char *base = ctx->mem;
+    auto carve = [&](size_t b) -> void * {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + b;
+        return base + off;
     };
-    // ghost rows: x_g = x_src + r_g / d (diag d, the other entry -d)
-    std::vector<int> gsrc(n, -1);
-    std::vector<double> gofs(n, 0.0);
-    for (int i = 0; i < n; ++i) {
-        const int ix = i % nxp, iy = i / nxp;
-        if (inside(ix, iy)) continue;
-        double d = 0, o = 0;
-        int src = -1, m = 0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-            if (val[k] == 0) continue;
-            if (col[k] == i) d += val[k];
-            else { src = col[k]; o += val[k]; ++m; }
-        }
-        if (m != 1 || d == 0 || o != -d) return 2;
-        const int sx = src % nxp, sy = src / nxp;
-        if (!inside(sx, sy)) return 2;
-        gsrc[i] = src;
-        gofs[i] = rhs[i] / d;
-    }
-    // boundary kinds from the ghost sources, when not given
-    if (bc[0] < 0) {
-        const int iy0 = dim == 2 ? 2 : 0;
-        bc[0] = (gsrc[iy0 * nxp + 0] % nxp == nxp - 4) ? YZ_BC_PERIODIC : YZ_BC_SYMMETRY;
-        bc[1] = bc[0];
-        if (dim == 2) bc[2] = bc[3] = ((gsrc[0 * nxp + 2] / nxp) == nyp - 4) ? YZ_BC_PERIODIC : YZ_BC_SYMMETRY;
-        else bc[2] = bc[3] = YZ_BC_SYMMETRY;
-    }
-    // ---- interior rows -> stencil planes ----
-    StencilOp op;
-    std::memset(&op, 0, sizeof(op));
-    op.dim = dim; op.nx = nx; op.ny = dim == 2 ? ny : 1; op.N = op.nx * op.ny;
-    op.ns = dim == 2 ? 13 : 5;
-    op.perx = bc[0] == YZ_BC_PERIODIC; op.pery = dim == 2 && bc[2] == YZ_BC_PERIODIC;
-    for (int k = 0; k < op.ns; ++k) { op.ox[k] = slot_ox(dim, k); op.oy[k] = slot_oy(dim, k); }
-    const int N = op.N;
-    std::vector<double> planes((size_t)op.ns * N, 0.0), b(N), x(N);
-    for (int iy = 0; iy < op.ny; ++iy)
-        for (int ix = 0; ix < nx; ++ix) {
-            const int i = iy * nx + ix;
-            const int r = (dim == 2 ? iy + 2 : 0) * nxp + ix + 2;
-            double bi = rhs[r];
-            for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
-                const int cc = col[k];
-                const int dxo = cc % nxp - (ix + 2), dyo = cc / nxp - (dim == 2 ? iy + 2 : 0);
-                int slot = -1;
-                for (int q = 0; q < op.ns; ++q)
-                    if (op.ox[q] == dxo && op.oy[q] == dyo) { slot = q; break; }
-                if (slot < 0) { if (val[k] != 0) return 2; continue; }
-                planes[(size_t)slot * N + i] += val[k];
-                if (gsrc[cc] >= 0) bi -= val[k] * gofs[cc];
-            }
-            b[i] = bi;
-            x[i] = x_inout[r];
-        }
-    // row scaling by the stored diagonal
-    if (s->scaling) {
-        const int c0 = dim == 2 ? 6 : 2;
-        for (int i = 0; i < N; ++i) {
-            const double d = planes[(size_t)c0 * N + i];
-            if (std::fabs(d) < 1.0e-30) continue;
-            const double inv = 1.0 / d;
-            for (int q = 0; q < op.ns; ++q) planes[(size_t)q * N + i] *= inv;
-            b[i] *= inv;
-        }
-    }
-    Solver &S = ctx->S;
-    if (solver_init(S, op, ctx->st)) return 2;
-    if (ctx->N != N) {
-        if (ctx->b) (void)hipFree(ctx->b);
-        if (ctx->x) (void)hipFree(ctx->x);
-        ctx->b = ctx->x = nullptr;
-        if (hipMalloc((void **)&ctx->b, N * sizeof(double)) != hipSuccess) return 2;
-        if (hipMalloc((void **)&ctx->x, N * sizeof(double)) != hipSuccess) return 2;
-        ctx->N = N;
-    }
-    if (hipMemcpyAsync(S.A, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice, ctx->st) != hipSuccess) return 2;
-    if (hipMemcpyAsync(ctx->b, b.data(), N * sizeof(double), hipMemcpyHostToDevice, ctx->st) != hipSuccess) return 2;
-    if (hipMemcpyAsync(ctx->x, x.data(), N * sizeof(double), hipMemcpyHostToDevice, ctx->st) != hipSuccess) return 2;
-    const int pc = s->precond == YZ_PC_AMG ? YZ_PC_AMG : YZ_PC_JACOBI;
-    int it = 0, tried = 0;
-    double rr = 0;
-    int rc = -1;
-    if (s->conc && pc == YZ_PC_AMG) {
-        // a concentration system: damped Jacobi first, at most 300
-        // iterations; from the guess again with the multigrid if it fails
-        if (jacobi_setup(S) != 0) return 2;
-        rc = bicgstab(S, ctx->b, ctx->x, YZ_PC_JACOBI, std::min(s->max_iter, 300), s->epsilon, &it, &rr);
-        if (rc != 0) {
-            tried = it;
-            if (hipMemcpyAsync(ctx->x, x.data(), N * sizeof(double), hipMemcpyHostToDevice, ctx->st) != hipSuccess) return 2;
-        }
-    }
-    if (rc != 0) {
-        if ((pc == YZ_PC_AMG ? amg_setup(S) : jacobi_setup(S)) != 0) return 2;
-        rc = bicgstab(S, ctx->b, ctx->x, pc, s->max_iter, s->epsilon, &it, &rr);
-        it += tried;
-    }
-    if (hipMemcpyAsync(x.data(), ctx->x, N * sizeof(double), hipMemcpyDeviceToHost, ctx->st) != hipSuccess) return 2;
-    if (hipStreamSynchronize(ctx->st) != hipSuccess) return 2;
-    for (int iy = 0; iy < op.ny; ++iy)
-        for (int ix = 0; ix < nx; ++ix)
-            x_inout[(dim == 2 ? iy + 2 : 0) * nxp + ix + 2] = x[iy * nx + ix];
-    for (int i = 0; i < n; ++i)
-        if (gsrc[i] >= 0) x_inout[i] = x_inout[gsrc[i]] + gofs[i];
-    if (iter) *iter = it;
-    if (resid) *resid = rr;
-    return rc;
+    lin_bufs(ctx->B, n, nnz, N, carve);
+    return lin_solve(ctx->S, ctx->B, G, rowptr, col, val, rhs, x_inout, s, iter, resid, ctx->st);
 }
 
 }  // extern "C"

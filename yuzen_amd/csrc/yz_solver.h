@@ -99,7 +99,23 @@ struct SolverT {
     // every row's diagonal of A is stored as exactly 1 (row-scaled FP32
     // planes): the fused iteration does not read that plane
     int unit_diag = 0;
+    // the device arrays were carved from a caller's arena (solver_free
+    // then frees nothing)
+    int in_arena = 0;
 };
+
+// device memory given by a caller: while one is in use (solver_use_arena),
+// solver_init carves the solver's arrays from it, 256-byte aligned, and
+// calls no allocator; null ends it
+struct DevArena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    bool overflow = false;   // a request did not fit
+};
+void solver_use_arena(DevArena *a);
+// the next block of n bytes of the arena in use, or null when none is in
+// use or it is full
+void *arena_carve(size_t n);
 
 typedef SolverT<double> Solver;
 

@@ -766,11 +766,40 @@ __global__ void __launch_bounds__(BLK) k_spmv_add(StencilOp op, const T *__restr
 // ---------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------
+static DevArena *g_arena = nullptr;
+void solver_use_arena(DevArena *a) { g_arena = a; }
+void *arena_carve(size_t n) {
+    if (!g_arena) return nullptr;
+    const size_t off = (g_arena->used + 255) & ~(size_t)255;
+    if (off > g_arena->cap || n > g_arena->cap - off) { g_arena->overflow = true; return nullptr; }
+    g_arena->used = off + n;
+    return g_arena->base + off;
+}
+// the host copy of the scalars for a solver in an arena (no pinned
+// allocation; the device-to-host copies of read_scalars then go through
+// the runtime's staging)
+static double g_arena_host_sc[64];
+
 template <typename T>
 static int dalloc(T **p, size_t n) {
     if (*p) return 0;
+    if (g_arena) {
+        *p = (T *)arena_carve(n * sizeof(T));
+        if (!*p) {
+            return -1;
+        }
+        return 0;
+    }
     YZ_CHECK(hipMalloc((void **)p, n * sizeof(T)));
     return 0;
+}
+static bool same_op(const StencilOp &a, const StencilOp &b) {
+    if (a.dim != b.dim || a.nx != b.nx || a.ny != b.ny || a.N != b.N || a.ns != b.ns || a.perx != b.perx ||
+        a.pery != b.pery)
+        return false;
+    for (int k = 0; k < a.ns; ++k)
+        if (a.ox[k] != b.ox[k] || a.oy[k] != b.oy[k]) return false;
+    return true;
 }
 
 static StencilOp box_op(int dim, int nx, int ny, int perx, int pery) {
@@ -798,10 +827,13 @@ void launch_spmv(const StencilOp &op, const T *val, const T *x, T *y, hipStream_
 
 template <typename T>
 int solver_init(SolverT<T> &S, const StencilOp &op0, hipStream_t stream) {
-    if (S.kr.N == op0.N && S.op0.ns == op0.ns) { S.op0 = op0; S.stream = stream; return 0; }
+    // the hierarchy's shapes hang on the operator's grid, edges and slots:
+    // reused only for the same ones (a solver in an arena is made afresh)
+    if (S.kr.N == op0.N && !S.in_arena && !g_arena && same_op(S.op0, op0)) { S.stream = stream; return 0; }
     solver_free(S);
     S.op0 = op0;
     S.stream = stream;
+    S.in_arena = g_arena != nullptr;
     const int N = op0.N;
     if (dalloc(&S.A, (size_t)op0.ns * N)) return -1;
     Krylov<T> &k = S.kr;
@@ -811,7 +843,8 @@ int solver_init(SolverT<T> &S, const StencilOp &op0, hipStream_t stream) {
     if (dalloc(&k.part, 4 * RED_BLOCKS)) return -1;
     if (dalloc(&k.sc, 64)) return -1;
     YZ_CHECK(hipMemset(k.sc, 0, 64 * sizeof(double)));
-    YZ_CHECK(hipHostMalloc((void **)&k.host_sc, 64 * sizeof(double), hipHostMallocDefault));
+    if (S.in_arena) k.host_sc = g_arena_host_sc;
+    else YZ_CHECK(hipHostMalloc((void **)&k.host_sc, 64 * sizeof(double), hipHostMallocDefault));
     // multigrid hierarchy shapes (fixed for the grid)
     Amg<T> &g = S.amg;
     g.lv.clear();
@@ -856,7 +889,8 @@ int solver_init(SolverT<T> &S, const StencilOp &op0, hipStream_t stream) {
 
 template <typename T>
 void solver_free(SolverT<T> &S) {
-    auto f = [](auto *&p) { if (p) { (void)hipFree((void *)p); p = nullptr; } };
+    const bool own = !S.in_arena;
+    auto f = [own](auto *&p) { if (p && own) (void)hipFree((void *)p); p = nullptr; };
     for (size_t l = 0; l < S.amg.lv.size(); ++l) {
         AmgLevel<T> &L = S.amg.lv[l];
         if (l > 0) f(L.val);
@@ -871,8 +905,10 @@ void solver_free(SolverT<T> &S) {
     k.ftiles = 0;
     fused_table_free(k.table);
     for (auto &e : S.kev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
-    if (k.host_sc) { (void)hipHostFree(k.host_sc); k.host_sc = nullptr; }
+    if (k.host_sc && own) (void)hipHostFree(k.host_sc);
+    k.host_sc = nullptr;
     k.N = 0;
+    S.in_arena = 0;
 }
 
 template <typename T>
