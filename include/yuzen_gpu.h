@@ -435,6 +435,20 @@ int yz_linsolve(yz_linctx *ctx, int n, int nnz, const int *rowptr,
                 const int *col, const double *val, const double *rhs,
                 double *x_inout, const yz_lin_settings *s, int *iter,
                 double *resid);
+/* The same solve with no allocation. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Every device array
+ * of the solve -- the CRS copy, the planes, the Krylov vectors, the
+ * multigrid hierarchy -- is carved afresh on each call from a file-scope
+ * scratch array of 96 MiB that every GPU the library runs on holds until
+ * the process ends; the call makes no hipMalloc or hipFree and works on the
+ * current device's default stream. A system larger than the scratch
+ * (about 110 000 interior cells) returns 2 with yz_last_error() saying so.
+ * Calls that overlap on one GPU race on the scratch. Same arguments, codes
+ * and contract as yz_linsolve, without a context. */
+int yz_linsolve_noalloc(int n, int nnz, const int *rowptr, const int *col, const double *val,
+                        const double *rhs, double *x_inout, const yz_lin_settings *s, int *iter,
+                        double *resid);
 /* solve-to-solve state for restart (ILinearSolverBackend::Memory /
  * SetMemory): the device backend carries none, so Memory is empty */
 int yz_memory_get(yz_linctx *ctx, double *buf, int cap);

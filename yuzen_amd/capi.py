@@ -214,6 +214,7 @@ def load():
     lib.yz_linsolve_destroy.argtypes = [C.c_void_p]
     lib.yz_linsolve.argtypes = [C.c_void_p, i, i, pi_, pi_, pd, pd, pd,
                                 P(YzLinSettings), pi_, pd]
+    lib.yz_linsolve_noalloc.argtypes = [i, i, pi_, pi_, pd, pd, pd, P(YzLinSettings), pi_, pd]
     lib.yz_memory_get.argtypes = [C.c_void_p, pd, i]
     lib.yz_memory_set.argtypes = [C.c_void_p, pd, i]
     pf = P(C.c_float)

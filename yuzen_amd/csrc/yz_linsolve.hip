@@ -21,6 +21,10 @@
 // scales it; the system is solved on the device, and the ghost entries of
 // x are formed again from their rows (as CRS_Matrix::SatisfyBCRows makes
 // them) before x comes back.
+//
+// yz_linsolve_noalloc is the same solve on device memory the library holds
+// from its load: a file-scope array (kScratchBytes) that every call carves
+// afresh, so the call allocates nothing.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -269,6 +273,16 @@ size_t lin_bufs_bytes(int n, int nnz, int N) {
            5 * up((size_t)n * 8) + 2 * up((size_t)N * 8);
 }
 
+// ---- the scratch of the no-allocation entry point ---------------------
+// Every GPU the library runs on holds this array from the module's load
+// until the process ends. Host code must not use its name as an address:
+// yz_linsolve_noalloc takes the device address of this GPU's copy with
+// hipGetSymbolAddress, once per device, and uses that.
+constexpr size_t kScratchBytes = (size_t)96 << 20;
+__device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+char *g_scratch[kMaxDevices] = {};
+
 }  // namespace
 
 struct yz_linctx {
@@ -329,6 +343,47 @@ int yz_linsolve(yz_linctx *ctx, int n, int nnz, const int *rowptr, const int *co
     };
     lin_bufs(ctx->B, n, nnz, N, carve);
     return lin_solve(ctx->S, ctx->B, G, rowptr, col, val, rhs, x_inout, s, iter, resid, ctx->st);
+}
+
+int yz_linsolve_noalloc(int n, int nnz, const int *rowptr, const int *col, const double *val, const double *rhs,
+                        double *x_inout, const yz_lin_settings *s, int *iter, double *resid) {
+    if (iter) *iter = 0;
+    if (resid) *resid = 0;
+    if (!rowptr || !col || !val || !rhs || !x_inout || !s || n <= 0 || nnz < n) return 2;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) { yz::set_error("no device"); return 2; }
+    if (!g_scratch[dev]) {
+        // this device's copy of the scratch array, with the device current
+        char *p = nullptr;
+        if (hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            yz::set_error("the scratch array's device address is not available");
+            return 2;
+        }
+        g_scratch[dev] = p;
+    }
+    LinGrid G;
+    if (lin_grid(n, nnz, rowptr, col, s, G)) { yz::set_error("no structured grid in the matrix"); return 2; }
+    const int N = G.nx * (G.dim == 2 ? G.ny : 1);
+    DevArena arena;
+    arena.base = g_scratch[dev];
+    arena.cap = kScratchBytes;
+    solver_use_arena(&arena);
+    LinBufs B;
+    auto too_big = [&]() {
+        yz::set_error("the system exceeds the no-allocation scratch (" + std::to_string(kScratchBytes >> 20) +
+                      " MiB, about 110 000 interior cells)");
+        return 2;
+    };
+    lin_bufs(B, n, nnz, N, [](size_t b) { return arena_carve(b); });
+    if (arena.overflow) { solver_use_arena(nullptr); return too_big(); }
+    // a solver made afresh in the arena on every call (the default stream)
+    static Solver S;
+    solver_free(S);
+    int rc = lin_solve(S, B, G, rowptr, col, val, rhs, x_inout, s, iter, resid, nullptr);
+    solver_use_arena(nullptr);
+    solver_free(S);
+    if (arena.overflow) rc = too_big();
+    return rc;
 }
 
 }  // extern "C"

@@ -261,6 +261,25 @@ class LinSolver:
         return rc, x, it.value, r.value
 
 
+def linsolve_noalloc(rowptr, col, val, rhs, x0, precond=capi.PC_AMG, scaling=1, max_iter=5000, epsilon=1e-13,
+                     conc=0):
+    """yz_linsolve_noalloc: the seam's solve in the library's scratch (no
+    allocation); (rc, x, iterations, relative residual)"""
+    lib = capi.load()
+    s = capi.YzLinSettings()
+    s.precond, s.scaling, s.max_iter, s.epsilon = precond, scaling, max_iter, epsilon
+    s.conc = conc
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    it, r = C.c_int(), C.c_double()
+    rc = lib.yz_linsolve_noalloc(len(rowptr) - 1, len(col), _pi(rowptr), _pi(col), _pd(val), _pd(rhs), _pd(x),
+                                 C.byref(s), C.byref(it), C.byref(r))
+    return rc, x, it.value, r.value
+
+
 class HipBatch:
     """Independent 2-D films advanced together (the batched sweep): one
     fixed implicit step per ``advance()`` for every film, their Krylov
