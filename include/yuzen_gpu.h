@@ -217,6 +217,38 @@ int yz_step_pass_log(yz_step *st, double *corr, double *eta, double *lin_resid, 
 int yz_step_save_state(yz_step *st, const char *path);
 int yz_step_load_state(yz_step *st, const char *path);
 
+/* ---- the run's termination tests (WellMixed_ModelBase::CheckTermination,
+ * yuzen/src/model/well_mixed/WellMixedModelBase.h:1567-1679, after every
+ * accepted step; FieldInfo over the real cells, IO/FieldInfo.cpp:52-136;
+ * ChangeRateOfVariable, numerics/ImplicitSolver.cpp:225-240). */
+typedef struct yz_term {
+    double end_time;      /* Numerics.EndTime (nondimensional, the run starts at 0); <= 0: none */
+    long long max_steps;  /* Numerics.MaxSteps; 0: none */
+    double steady_eps;    /* Numerics.SteadyStateCriterion; <= 0 switches the test off */
+    int steady_steps;     /* Numerics.SteadyStateSteps */
+    int dry;              /* Physics.Evaporation.Enable: the dry-out test is taken */
+    int end_conc_type;    /* EndConcType: 0 mean, 1 max, 2 min */
+    double end_conc_value;/* EndConcValue */
+} yz_term;
+/* why the run stops after a step, in the reference's order of the tests */
+enum { YZ_RUN_GOING = 0, YZ_RUN_NAN = 1, YZ_RUN_CONC_RANGE = 2, YZ_RUN_NEGATIVE_FILM = 3,
+       YZ_RUN_NEGATIVE_VOLUME = 4, YZ_RUN_END = 5, YZ_RUN_DRY = 6, YZ_RUN_STEADY = 7 };
+/* The tests evaluated on the device at the end of every accepted step
+ * from the call on (t NULL: off), their outcome carried back by the step's
+ * own final synchronisation. The concentration's slack test ([0, 1] by
+ * 0.1, with the lowest c of the step's start) applies when the
+ * concentration is solved (conc_model 1). */
+int yz_step_set_term(yz_step *st, const yz_term *t);
+/* the outcome after the last step: status (YZ_RUN_*) and info8 =
+ * {fluid volume, mean c, min h, max c, min c (with the step's start when
+ * solved), change rate x span, steady count, steps} (either may be NULL;
+ * no synchronisation) */
+int yz_step_term_status(yz_step *st, int *status, double *info8);
+/* advance up to nsteps steps, stopping after the first step whose tests
+ * say so (yz_step_set_term) or that fails; *done gets the steps taken,
+ * *status the tests' outcome. Returns yz_step_advance's code. */
+int yz_step_run(yz_step *st, int nsteps, int *done, int *status);
+
 /* per-phase timing of the last yz_step_advance, milliseconds (HIP events):
  * [0] assembly, [1] residual, [2] AMG setup, [3] Krylov, [4] update,
  * [5] whole step */

@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from yuzen_amd.capi import YzCase, YzCzCase
+from yuzen_amd.capi import YzCase, YzCzCase, YzTerm
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -75,6 +75,7 @@ def load_from(path):
     lib.yzo_cz_redistribute.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_case_defaults.argtypes = [P(YzCase)]
     lib.yzo_pass_log.argtypes = [C.c_void_p, pd, pd, pd, C.c_int]
+    lib.yzo_check.argtypes = [C.c_void_p, P(YzTerm), pd]
     lib.yzo_script_create.restype = C.c_void_p
     lib.yzo_script_create.argtypes = [P(YzCase), C.c_int, pd, C.c_double, pd, C.c_int, pd, C.c_int,
                                       pd, C.c_int, pi_, C.c_int]
@@ -192,6 +193,12 @@ class OracleStep:
         t, a, b = C.c_double(), C.c_double(), C.c_double()
         self.lib.yzo_clock(self.p, C.byref(t), C.byref(a), C.byref(b))
         return t.value, a.value, b.value
+
+    def check(self, t):
+        """the run's termination tests after the last step: (status, info8)"""
+        info = np.zeros(8)
+        st = self.lib.yzo_check(self.p, C.byref(t), _pd(info))
+        return st, info
 
     def pass_log(self):
         """the passes of the last step's accepted attempt: (correction,

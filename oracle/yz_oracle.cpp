@@ -1593,6 +1593,10 @@ struct Stepper {
     long long lin_total = 0;
     bool first_step = true;
     long long stag_cuts = 0;           /* _stagnationCuts (NewtonKantorovich.cpp:357) */
+    /* the run's termination tests (CheckTermination) */
+    long long steps_done = 0;
+    int steady_count = 0;
+    double min_start_c = 0;            /* _minStart (HeightEquationBase.h:1363-1370) */
     /* the passes of the last attempt: correction, tolerance, residual */
     vector<double> plog_corr, plog_eta, plog_lres;
     double last_eps = 0;
@@ -2325,6 +2329,14 @@ struct Stepper {
         if (!script.on) {
             /* BeforeNKLoop: the working field from the state of the step */
             if (conc_on) conc_before(); else eq.c = c_field;
+            if (conc_on) {
+                /* the lowest c of a real cell at the start of the step */
+                min_start_c = 0;
+                for (int i = 0; i < eq.g.n; ++i) {
+                    const double cv = c_field[i];
+                    if (!eq.A.bc[i] && (cv < min_start_c || !(cv == cv))) min_start_c = cv;
+                }
+            }
             eq.h = phi_old;
             eq.set_matrix_A();
             eq.apply_A(phi_old, Ax_old);
@@ -2437,8 +2449,65 @@ struct Stepper {
         dt_old = dt_now;
         dt_now = dt_request;
         first_step = false;
+        ++steps_done;
         lin = (int)lin_sum;
         return 0;
+    }
+    /* ChangeRateOfVariable (ImplicitSolver.cpp:225-240): the relative rate
+     * of change per unit time of the step just taken, over every row */
+    double change_rate() const {
+        if (steps_done < 5) return 1;
+        const double dt = dt_old;                  /* LastStepDT */
+        if (is_zero(dt)) return 1;
+        double dmax = 0;
+        for (size_t i = 0; i < phi_new.size(); ++i) dmax = std::max(dmax, std::fabs(phi_new[i] - phi_old[i]));
+        const double scale = absmax(phi_new);
+        if (is_zero(scale)) return 0;
+        return dmax / (scale * dt);
+    }
+    /* CheckTermination (WellMixedModelBase.h:1567-1679) after a step, with
+     * FieldInfo over the real cells (FieldInfo.cpp:52-136); info8 as
+     * yz_step_term_status */
+    int check_termination(const yz_term &t, double *info) {
+        const Grid &g = eq.g;
+        double chr = 0;
+        if (t.steady_eps > 0) {
+            double tspan = t.end_time - 0.0;
+            if (!(tspan > 0)) tspan = 1.0;
+            chr = change_rate() * tspan;
+            if (chr < t.steady_eps) steady_count++;
+            else steady_count = 0;
+        }
+        const double dA = g.dim == 2 ? g.dx * g.dy : g.dx;
+        double vol = 0, solute = 0, minH = 1e40, maxC = -1e40, minC = 1e40;
+        for (int i = 0; i < g.n; ++i) {
+            if (eq.A.bc[i]) continue;
+            const double hh = phi_new[i], cc = c_field[i];
+            vol += hh * dA;
+            solute += cc * hh * dA;
+            minH = std::min(minH, hh);
+            maxC = std::max(maxC, cc);
+            minC = std::min(minC, cc);
+        }
+        const double ave = solute / vol;
+        if (conc_on) minC = std::min(minC, min_start_c);
+        if (info) {
+            info[0] = vol; info[1] = ave; info[2] = minH; info[3] = maxC; info[4] = minC;
+            info[5] = chr; info[6] = steady_count; info[7] = (double)steps_done;
+        }
+        if (!std::isfinite(vol) || !std::isfinite(maxC) || !std::isfinite(minC) || !std::isfinite(minH)) return YZ_RUN_NAN;
+        if (conc_on && (maxC > 1.0 + 0.1 || minC < -0.1)) return YZ_RUN_CONC_RANGE;
+        if (minH < -1.0e-3) return YZ_RUN_NEGATIVE_FILM;
+        if (vol < 0.0) return YZ_RUN_NEGATIVE_VOLUME;
+        const bool running = (t.end_time <= 0 || (0.0 <= time && time <= t.end_time)) &&
+                             (t.max_steps <= 0 || steps_done <= t.max_steps - 1);
+        if (!running) return YZ_RUN_END;
+        if (t.dry) {
+            const double v = t.end_conc_type == 0 ? ave : (t.end_conc_type == 1 ? maxC : minC);
+            if (v > t.end_conc_value) return YZ_RUN_DRY;
+        }
+        if (t.steady_eps > 0 && steady_count > t.steady_steps) return YZ_RUN_STEADY;
+        return YZ_RUN_GOING;
     }
 };
 
@@ -2510,6 +2579,10 @@ int yzo_script_info(void *p, int *calls, long long *cuts, double *eta, int cap) 
     const int m = (int)st->script.eta.size();
     for (int i = 0; i < m && i < cap; ++i) eta[i] = st->script.eta[i];
     return m;
+}
+/* the termination tests after the last step (yz_step_term_status) */
+int yzo_check(void *p, const yz_term *t, double *info8) {
+    return static_cast<yzo::Stepper *>(p)->check_termination(*t, info8);
 }
 /* the passes of the last step's accepted attempt (yz_step_pass_log) */
 int yzo_pass_log(void *p, double *corr, double *eta, double *lres, int cap) {

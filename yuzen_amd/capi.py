@@ -77,6 +77,27 @@ class YzCase(C.Structure):
         return c
 
 
+class YzTerm(C.Structure):
+    _fields_ = [
+        ("end_time", C.c_double), ("max_steps", C.c_longlong),
+        ("steady_eps", C.c_double), ("steady_steps", C.c_int),
+        ("dry", C.c_int), ("end_conc_type", C.c_int), ("end_conc_value", C.c_double),
+    ]
+
+
+RUN_GOING, RUN_NAN, RUN_CONC_RANGE, RUN_NEGATIVE_FILM, RUN_NEGATIVE_VOLUME, RUN_END, RUN_DRY, RUN_STEADY = range(8)
+
+
+def term(end_time=0.0, max_steps=0, steady_eps=1e-3, steady_steps=10, dry=0, end_conc_type=0, end_conc_value=0.95):
+    """the run's termination tests (the reference's defaults:
+    SteadyStateCriterion 1e-3, SteadyStateSteps 10, EndConcType mean,
+    EndConcValue 0.95)"""
+    t = YzTerm()
+    t.end_time, t.max_steps, t.steady_eps, t.steady_steps = end_time, max_steps, steady_eps, steady_steps
+    t.dry, t.end_conc_type, t.end_conc_value = dry, end_conc_type, end_conc_value
+    return t
+
+
 class YzLinSettings(C.Structure):
     _fields_ = [
         ("precond", C.c_int), ("scaling", C.c_int), ("max_iter", C.c_int),
@@ -200,6 +221,9 @@ def load():
     lib.yz_step_balances.argtypes = [C.c_void_p, pd]
     lib.yz_step_set_h.argtypes = [C.c_void_p, pd]
     lib.yz_step_clock.argtypes = [C.c_void_p, pd, pd, pd]
+    lib.yz_step_set_term.argtypes = [C.c_void_p, C.POINTER(YzTerm)]
+    lib.yz_step_term_status.argtypes = [C.c_void_p, C.POINTER(C.c_int), pd]
+    lib.yz_step_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.yz_step_pass_log.argtypes = [C.c_void_p, pd, pd, pd, C.c_int, C.POINTER(C.c_longlong)]
     lib.yz_step_timing.argtypes = [C.c_void_p, pd]
     lib.yz_step_destroy.argtypes = [C.c_void_p]

@@ -123,10 +123,13 @@ __global__ void k_pack(Geo g, const double *pad, double *out) {
     out[i] = pad[(g.dim == 2 ? iy + 2 : 0) * g.nxp + ix + 2];
 }
 
+// the weight floor is ScaleFloor x max |phi_old|, the maximum's bits as
+// k_absmax left them: no read-back between the two kernels
 __global__ void k_truncation(Geo g, const double *hm, const double *ho, const double *hp, double dt0, double dt,
-                             double wfloor, unsigned long long *out) {
+                             double floor_factor, const unsigned long long *hmaxbits, unsigned long long *out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0;
+    const double wfloor = floor_factor * __longlong_as_double((long long)*hmaxbits);
     if (p < g.nxp * g.nyp) {
         double w = fabs(ho[p]);
         if (w < wfloor) w = wfloor;
@@ -244,6 +247,14 @@ struct yz_step {
     std::vector<double> plog_corr, plog_eta, plog_lres;
     double last_eps = 0;
     long long stag_cuts = 0;
+    // the run's termination tests, reduced on the device after every
+    // accepted step and read back with the step's own synchronisation
+    int term_on = 0;
+    yz_term term{};
+    double *dterm = nullptr;    // partials of k_term_cells, then the 8 reduced values
+    double *hterm = nullptr;    // their host copy (pinned)
+    int term_status = 0, steady_count = 0;
+    double term_info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[12];
     double tms[6];
     // the well-mixed concentration (conc_model 1): the field's c, the
@@ -668,10 +679,168 @@ static int conc_solve(yz_step *S, bool commit, int *ch) {
     return 0;
 }
 
+// ---- the run's termination tests (CheckTermination,
+// WellMixedModelBase.h:1567-1679) -------------------------------------------
+// k_term_cells: per block, over the real cells: the sums of h and c h (the
+// fluid volume and the solute, times the cell area), min h, max c, min c,
+// whether a c is not a number; over every row: max |h - h_old| and max |h|
+// (ChangeRateOfVariable). k_term_final sums the blocks in their order.
+// FieldInfo's loop carries a NaN of c only from its last cell (Math::Min/Max
+// of a NaN and what follows keeps what follows); here any NaN stops the run.
+constexpr int TERM_B = 256, TERM_G = 512, TERM_Q = 8;
+constexpr int TERM_OUT = TERM_Q * TERM_G, TERM_MINSTART = TERM_OUT + TERM_Q;
+__global__ void __launch_bounds__(TERM_B) k_term_cells(Geo g, const double *h, const double *hold, const double *c,
+                                                       double *part) {
+    double v[TERM_Q] = {0.0, 0.0, 1e300, -1e300, 1e300, 0.0, 0.0, 0.0};
+    const int np = g.nxp * g.nyp;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const double hh = h[p];
+        v[5] = fmax(v[5], fabs(hh - hold[p]));
+        v[6] = fmax(v[6], fabs(hh));
+        const int ix = p % g.nxp, iy = p / g.nxp;
+        const bool real = ix >= 2 && ix < g.nxp - 2 && (g.dim == 1 || (iy >= 2 && iy < g.nyp - 2));
+        if (!real) continue;
+        const double cc = c[p];
+        v[0] += hh;
+        v[1] += cc * hh;
+        v[2] = fmin(v[2], hh);
+        v[3] = fmax(v[3], cc);
+        v[4] = fmin(v[4], cc);
+        if (!(cc == cc) || !(hh == hh)) v[7] = 1.0;
+    }
+    __shared__ double sv[TERM_Q][TERM_B];
+#pragma unroll
+    for (int k = 0; k < TERM_Q; ++k) sv[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = TERM_B / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sv[0][threadIdx.x] += sv[0][threadIdx.x + off];
+            sv[1][threadIdx.x] += sv[1][threadIdx.x + off];
+            sv[2][threadIdx.x] = fmin(sv[2][threadIdx.x], sv[2][threadIdx.x + off]);
+            sv[3][threadIdx.x] = fmax(sv[3][threadIdx.x], sv[3][threadIdx.x + off]);
+            sv[4][threadIdx.x] = fmin(sv[4][threadIdx.x], sv[4][threadIdx.x + off]);
+            sv[5][threadIdx.x] = fmax(sv[5][threadIdx.x], sv[5][threadIdx.x + off]);
+            sv[6][threadIdx.x] = fmax(sv[6][threadIdx.x], sv[6][threadIdx.x + off]);
+            sv[7][threadIdx.x] = fmax(sv[7][threadIdx.x], sv[7][threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < TERM_Q; ++k) part[k * TERM_G + blockIdx.x] = sv[k][0];
+}
+__global__ void __launch_bounds__(TERM_B) k_term_final(int nblocks, const double *part, double *out) {
+    __shared__ double sv[TERM_Q][TERM_B];
+    double v[TERM_Q] = {0.0, 0.0, 1e300, -1e300, 1e300, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += TERM_B) {
+        v[0] += part[0 * TERM_G + b];
+        v[1] += part[1 * TERM_G + b];
+        v[2] = fmin(v[2], part[2 * TERM_G + b]);
+        v[3] = fmax(v[3], part[3 * TERM_G + b]);
+        v[4] = fmin(v[4], part[4 * TERM_G + b]);
+        v[5] = fmax(v[5], part[5 * TERM_G + b]);
+        v[6] = fmax(v[6], part[6 * TERM_G + b]);
+        v[7] = fmax(v[7], part[7 * TERM_G + b]);
+    }
+#pragma unroll
+    for (int k = 0; k < TERM_Q; ++k) sv[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = TERM_B / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sv[0][threadIdx.x] += sv[0][threadIdx.x + off];
+            sv[1][threadIdx.x] += sv[1][threadIdx.x + off];
+            sv[2][threadIdx.x] = fmin(sv[2][threadIdx.x], sv[2][threadIdx.x + off]);
+            sv[3][threadIdx.x] = fmax(sv[3][threadIdx.x], sv[3][threadIdx.x + off]);
+            sv[4][threadIdx.x] = fmin(sv[4][threadIdx.x], sv[4][threadIdx.x + off]);
+            sv[5][threadIdx.x] = fmax(sv[5][threadIdx.x], sv[5][threadIdx.x + off]);
+            sv[6][threadIdx.x] = fmax(sv[6][threadIdx.x], sv[6][threadIdx.x + off]);
+            sv[7][threadIdx.x] = fmax(sv[7][threadIdx.x], sv[7][threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < TERM_Q; ++k) out[k] = sv[k][0];
+}
+// the lowest c of a real cell, from 0 (MinStartConc: a NaN stays)
+__global__ void __launch_bounds__(1024) k_min_start(Geo g, const double *c, double *out) {
+    __shared__ double sm[1024];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < g.N; i += blockDim.x) {
+        const int p = (g.dim == 2 ? i / g.nx + 2 : 0) * g.nxp + i % g.nx + 2;
+        const double v = c[p];
+        if (!(v == v) || !(m == m)) m = (m == m) ? v : m;
+        else if (v < m) m = v;
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            const double a = sm[threadIdx.x], b = sm[threadIdx.x + off];
+            sm[threadIdx.x] = (!(a == a)) ? a : ((!(b == b)) ? b : fmin(a, b));
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sm[0];
+}
+static int term_launch(yz_step *S) {
+    const Geo &g = S->g;
+    const int np = g.nxp * g.nyp;
+    const int nbk = std::min(TERM_G, (np + TERM_B - 1) / TERM_B);
+    hipLaunchKernelGGL(k_term_cells, dim3(nbk), dim3(TERM_B), 0, S->st, g, S->hnew, S->hold,
+                       S->conc ? S->cfield : S->c, S->dterm);
+    hipLaunchKernelGGL(k_term_final, dim3(1), dim3(TERM_B), 0, S->st, nbk, S->dterm, S->dterm + TERM_OUT);
+    YZ_TRY(hipGetLastError());
+    YZ_TRY(hipMemcpyAsync(S->hterm, S->dterm + TERM_OUT, (TERM_Q + 1) * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    return 0;
+}
+// the tests, in the reference's order, on the reduced values of the step
+// just taken (the clock and the step count already advanced)
+static void term_decide(yz_step *S) {
+    const yz_term &t = S->term;
+    const Geo &g = S->g;
+    const double *r = S->hterm;
+    const double dA = g.dim == 2 ? g.dx * g.dy : g.dx;
+    double chr = 0;
+    if (t.steady_eps > 0) {
+        double tspan = t.end_time - 0.0;
+        if (!(tspan > 0)) tspan = 1.0;
+        double rate;
+        if (S->steps < 5 || fabs(S->dt_old) <= DBL_MIN) rate = 1;
+        else if (fabs(r[6]) <= DBL_MIN) rate = 0;
+        else rate = r[5] / (r[6] * S->dt_old);
+        chr = rate * tspan;
+        if (chr < t.steady_eps) S->steady_count++;
+        else S->steady_count = 0;
+    }
+    const double vol = r[0] * dA, ave = (r[1] * dA) / vol, minH = r[2], maxC = r[3];
+    double minC = r[4];
+    if (S->conc) minC = std::min(minC, r[TERM_Q]);
+    if (r[7] != 0.0 || (S->conc && !(r[TERM_Q] == r[TERM_Q]))) minC = std::nan("");
+    double *info = S->term_info;
+    info[0] = vol; info[1] = ave; info[2] = minH; info[3] = maxC; info[4] = minC;
+    info[5] = chr; info[6] = S->steady_count; info[7] = (double)S->steps;
+    int st = YZ_RUN_GOING;
+    const bool running = (t.end_time <= 0 || (0.0 <= S->time && S->time <= t.end_time)) &&
+                         (t.max_steps <= 0 || S->steps <= t.max_steps - 1);
+    if (!std::isfinite(vol) || !std::isfinite(maxC) || !std::isfinite(minC) || !std::isfinite(minH)) st = YZ_RUN_NAN;
+    else if (S->conc && (maxC > 1.0 + 0.1 || minC < -0.1)) st = YZ_RUN_CONC_RANGE;
+    else if (minH < -1.0e-3) st = YZ_RUN_NEGATIVE_FILM;
+    else if (vol < 0.0) st = YZ_RUN_NEGATIVE_VOLUME;
+    else if (!running) st = YZ_RUN_END;
+    else if (t.dry && (t.end_conc_type == 0 ? ave : (t.end_conc_type == 1 ? maxC : minC)) > t.end_conc_value)
+        st = YZ_RUN_DRY;
+    else if (t.steady_eps > 0 && S->steady_count > t.steady_steps) st = YZ_RUN_STEADY;
+    S->term_status = st;
+}
+
 // BeforeNKLoop's concentration part (HeightEquationBase.h:1314-1388)
 static int conc_before(yz_step *S) {
     const Geo &g = S->g;
     const size_t np = (size_t)g.nxp * g.nyp;
+    // the lowest c of a real cell at the step's start (MinStartConc), for the
+    // termination tests
+    if (S->term_on) hipLaunchKernelGGL(k_min_start, dim3(1), dim3(1024), 0, S->st, g, S->cfield, S->dterm + TERM_MINSTART);
     YZ_TRY(hipMemcpyAsync(S->c, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     YZ_TRY(hipMemcpyAsync(S->cprev, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     conc_fluxes(S, S->hold, S->Fc0, S->rc0);
@@ -1070,6 +1239,8 @@ void yz_step_destroy(yz_step *S) {
     if (S->dflag) (void)hipFree(S->dflag);
     if (S->hmax) (void)hipHostFree(S->hmax);
     if (S->hflag) (void)hipHostFree(S->hflag);
+    if (S->dterm) (void)hipFree(S->dterm);
+    if (S->hterm) (void)hipHostFree(S->hterm);
     for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
     if (S->st && S->own_stream) (void)hipStreamDestroy(S->st);
     delete S;
@@ -1112,10 +1283,9 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
                     if (fabs(th) <= DBL_MIN) th = 1.0e-2;
                     YZ_TRY(hipMemsetAsync(S->dmax, 0, 4 * sizeof(unsigned long long), S->st));
                     hipLaunchKernelGGL(k_absmax, dim3(std::min(nb(np), 1024)), dim3(TB), 0, S->st, np, S->hold, S->dmax + 2);
-                    if (sync_read(S)) return 2;
-                    const double wfloor = c.tsc_floor * bits_to_double(S->hmax[2]);
                     hipLaunchKernelGGL(k_truncation, dim3(nb(np)), dim3(TB), 0, S->st, g, S->hold2, S->hold, S->h,
-                                       S->dt_old, S->dt_now, wfloor, S->dmax + 3);
+                                       S->dt_old, S->dt_now, c.tsc_floor, (const unsigned long long *)(S->dmax + 2),
+                                       S->dmax + 3);
                     if (sync_read(S)) return 2;
                     crit = 1.0 - bits_to_double(S->hmax[3]) / th;
                 }
@@ -1146,6 +1316,7 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
     } while (!step_done);
     if (S->conc && conc_after(S)) return 2;
     YZ_TRY(hipMemcpyAsync(S->hnew, S->h, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    if (S->term_on && term_launch(S)) return 2;
     YZ_TRY(hipEventRecord(S->ev[11], S->st));
     YZ_TRY(hipEventSynchronize(S->ev[11]));
     S->tms[5] = ev_ms(S->ev[10], S->ev[11]);
@@ -1154,6 +1325,7 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
     S->dt_now = dt_request;
     S->first_step = false;
     ++S->steps;
+    if (S->term_on) term_decide(S);
     if (nk_iter) *nk_iter = it;
     if (lin_iter) *lin_iter = (int)lin_sum;
     if (resid) *resid = r;
@@ -1465,6 +1637,42 @@ int yz_step_set_h(yz_step *S, const double *h) {
     hipLaunchKernelGGL(k_ghosts, dim3(nb(np)), dim3(TB), 0, S->st, g, S->hnew);
     YZ_TRY(hipStreamSynchronize(S->st));
     return 0;
+}
+
+int yz_step_set_term(yz_step *S, const yz_term *t) {
+    if (!S) { set_err("null step"); return 2; }
+    if (!t) { S->term_on = 0; return 0; }
+    if (!S->dterm) {
+        YZ_TRY(hipMalloc((void **)&S->dterm, (TERM_MINSTART + 1) * sizeof(double)));
+        YZ_TRY(hipMemset(S->dterm, 0, (TERM_MINSTART + 1) * sizeof(double)));
+        YZ_TRY(hipHostMalloc((void **)&S->hterm, (TERM_Q + 1) * sizeof(double), hipHostMallocDefault));
+    }
+    S->term = *t;
+    S->term_on = 1;
+    S->term_status = 0;
+    return 0;
+}
+
+int yz_step_term_status(yz_step *S, int *status, double *info8) {
+    if (!S) { set_err("null step"); return 2; }
+    if (status) *status = S->term_status;
+    if (info8) std::memcpy(info8, S->term_info, sizeof(S->term_info));
+    return 0;
+}
+
+int yz_step_run(yz_step *S, int nsteps, int *done, int *status) {
+    if (!S) { set_err("null step"); return 2; }
+    int k = 0, rc = 0;
+    for (; k < nsteps; ++k) {
+        int nk = 0, lin = 0;
+        double r = 0;
+        rc = yz_step_advance(S, &nk, &lin, &r);
+        if (rc != 0) break;
+        if (S->term_on && S->term_status != YZ_RUN_GOING) { ++k; break; }
+    }
+    if (done) *done = k;
+    if (status) *status = S->term_status;
+    return rc;
 }
 
 int yz_step_pass_log(yz_step *S, double *corr, double *eta, double *lin_resid, int cap, long long *stag_cuts) {

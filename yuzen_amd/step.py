@@ -145,6 +145,28 @@ class HipStep:
         m = self.lib.yz_step_pass_log(self.p, _pd(a), _pd(b), _pd(r), 256, None)
         return a[:m].copy(), b[:m].copy(), r[:m].copy()
 
+    def set_term(self, t):
+        """the run's termination tests after every step (capi.term(...)), or None"""
+        self._term = t
+        if self.lib.yz_step_set_term(self.p, C.byref(t) if t is not None else None):
+            raise RuntimeError("yz_step_set_term: " + capi.last_error())
+
+    def term_status(self):
+        """(status, info8) after the last step"""
+        st = C.c_int()
+        info = np.zeros(8)
+        self.lib.yz_step_term_status(self.p, C.byref(st), _pd(info))
+        return st.value, info
+
+    def run(self, nsteps):
+        """up to nsteps steps, stopping where a termination test says so:
+        (rc, steps done, status)"""
+        done, st = C.c_int(), C.c_int()
+        rc = self.lib.yz_step_run(self.p, nsteps, C.byref(done), C.byref(st))
+        if rc == 2:
+            raise RuntimeError("yz_step_run: " + capi.last_error())
+        return rc, done.value, st.value
+
     def stagnation_cuts(self):
         n = C.c_longlong()
         self.lib.yz_step_pass_log(self.p, None, None, None, 0, C.byref(n))
