@@ -140,7 +140,7 @@ typedef struct yz_case {
      * and the solvent h (1 - c) per step; E from Evaporation.cpp:62-103,
      * uniform in space), less the nozzle's supply at the middle of the step
      * (SourceBase.cpp, WM2D/Source.cpp, WM1D/Source_1D.cpp). Both with the
-     * concentration frozen (conc_model 0); the batch takes neither. */
+     * concentration frozen (conc_model 0); the batch takes both, per film. */
     int evaporation;                /* Physics.Evaporation.Enable */
     double Ev, evap_coef;           /* E = Ev evap_coef f(c) g(h) */
     int evap_func;                  /* YZ_EVAP_RAOULT (default) or YZ_EVAP_POWERN */
@@ -293,7 +293,9 @@ int yz_step_energy(yz_step *st, double *F, double *D);
  * [2] the volume the last step evaporated (B's Crank-Nicolson evaporation of
  * its last Newton pass, as capped, times the cell measure and that step's
  * dt: EvaporationRate, HeightEquationBase.h:359-369) and [3] the volume the
- * nozzle supplied in it (SuppliedVolume, :293-303); 0 before the first step.
+ * nozzle supplied in it (SuppliedVolume, :293-303); 0 before the context's
+ * first step and after yz_step_load_state until it steps again (the state
+ * file holds the film, not the last step's evaporation or supply).
  * On a closed or periodic film without the 1-D sideways outflow, the volume
  * changes over a step by [3] - [2], to the linear solve's tolerance. 0 / 2. */
 int yz_step_balances(yz_step *st, double *out4);
@@ -442,8 +444,13 @@ int yz_set_tile_rows(int nx, int ny, int rows);
  * Solve A x = b, A in CRS (rowptr[n+1], col[nnz], val[nnz]; any column
  * order), with BiCGSTAB under settings s. When grid_nx > 0 the rows are
  * the (grid_nx+4)(grid_ny+4) rows of a film-height matrix on that grid
- * (ghost rows included) and the structured AMG is available; otherwise
- * precond AMG falls back to Jacobi on the device. */
+ * (ghost rows included); with grid_nx 0 the grid is read off an interior
+ * row's pattern (13-point film-height or 9-point concentration rows in 2-D,
+ * 5-point in 1-D). The rows are folded into stencil planes on the device,
+ * so AMG and Jacobi both run on the structured grid. A matrix with no such
+ * grid, or whose rows do not fit it, returns 2 with yz_last_error() set:
+ * the seam serves the film-height and well-mixed concentration equations,
+ * not a general CRS. */
 typedef struct yz_lin_settings {
     int precond;       /* YZ_PC_JACOBI or YZ_PC_AMG */
     int scaling;       /* row scaling by the diagonal */

@@ -82,3 +82,29 @@ def test_volume_balance_closes(coef):
         assert abs(v1[1] - (hg * cg).sum() * cm) <= 1e-12 * v1[1]
         v0 = v1
     g.close()
+
+
+def test_balances_after_load_state(tmp_path):
+    """A context loaded from a state file has taken no step of its own: it
+    reports no evaporated or supplied volume until it steps, and then the
+    balance closes over that step as it does on the context that wrote the
+    file."""
+    from .test_gpu_evap_source import DT, _with_b
+    c, h, s, cc = smooth_case(2, [t for t in TERMS if t != "centrifugal"], slip=0.05, periodic=True)
+    _with_b(c, coef=50.0)
+    g = HipStep(c, h, s, cc, DT)
+    for _ in range(2):
+        assert g.advance()[0] == 0
+    assert min(g.balances()[2:]) > 0
+    path = str(tmp_path / "b.state")
+    g.save_state(path)
+    g.close()
+    r = HipStep(c, h, s, cc, DT)
+    r.load_state(path)
+    v0 = r.balances()
+    assert v0[2] == 0 and v0[3] == 0
+    assert r.advance()[0] == 0
+    v1 = r.balances()
+    gap = (v1[0] - v0[0]) - (v1[3] - v1[2])
+    assert v1[2] > 0 and v1[3] > 0 and abs(gap) <= 1e-10 * v1[0]
+    r.close()

@@ -236,6 +236,9 @@ struct yz_step {
     long long steps = 0;
     int tsc_count = 0;
     bool extrap_blocked = false, first_step = true;
+    // a step ran since create or load: the last step's evaporated and
+    // supplied volumes (evpv, srcv) are this context's own
+    bool stepped = false;
     double extrap_move = 0, ew_prev = 0, ew_eta = 0;
     double nk_first = 0, nk_prev = 0;
     int nk_pass = 0;
@@ -1324,6 +1327,7 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
     S->dt_old = S->dt_now;
     S->dt_now = dt_request;
     S->first_step = false;
+    S->stepped = true;
     ++S->steps;
     if (S->term_on) term_decide(S);
     if (nk_iter) *nk_iter = it;
@@ -1459,7 +1463,7 @@ int yz_step_balances(yz_step *S, double *out4) {
     YZ_TRY(hipMalloc((void **)&d, (4 * (size_t)nblk + 4) * sizeof(double)));
     double host[4] = {0, 0, 0, 0};
     int rc = 0;
-    const bool ran = S->steps > 0;
+    const bool ran = S->stepped;
     hipLaunchKernelGGL(k_balance_cells, dim3(nblk), dim3(EB), 0, S->st, g, S->hnew, S->c,
                        ran ? S->evpv : nullptr, (ran && S->cs.source) ? S->srcv : nullptr, d);
     hipLaunchKernelGGL(k_balance_final, dim3(1), dim3(EB), 0, S->st, nblk, d, d + 4 * nblk);
@@ -1626,6 +1630,7 @@ int yz_step_load_state(yz_step *S, const char *path) {
     S->dt_old = dt_old;
     S->tsc_count = (int)st.get("adjustCount.height");
     S->first_step = S->steps == 0 && dt_old == 0;
+    S->stepped = false;   // the file holds no last-step evaporation or supply
     return 0;
 }
 
