@@ -30,6 +30,12 @@ def load():
     return _lib
 
 
+def threads(n=0):
+    """the OpenMP threads of the oracle's row loops: n > 0 sets them;
+    returns the number in force"""
+    return load().yzo_threads(int(n))
+
+
 def load_from(path):
     """the oracle's C API from a library at `path` (the tests load copies
     built with a planted mistake this way)"""
@@ -40,6 +46,7 @@ def load_from(path):
     lib.yzo_create.argtypes = [P(YzCase), pd, pd, pd, C.c_double]
     lib.yzo_destroy.argtypes = [C.c_void_p]
     lib.yzo_n.argtypes = [C.c_void_p]
+    lib.yzo_threads.argtypes = [C.c_int]
     lib.yzo_nnz.argtypes = [C.c_void_p]
     lib.yzo_advance.argtypes = [C.c_void_p, pi_, pi_, pd]
     lib.yzo_get_h.argtypes = [C.c_void_p, pd]

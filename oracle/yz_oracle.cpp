@@ -106,6 +106,18 @@
 #include <limits>
 #include <string>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Row loops of the step's hot path (the term assembly, ApplyA, IA, the
+ * right-hand side, the matrix-vector products, the multigrid cycle's and
+ * BiCGSTAB's vector updates) run over OpenMP threads when the oracle is
+ * built with -fopenmp. Every such loop writes only its own row and computes
+ * it in the same order as the serial loop, so the results are the same to
+ * the bit for any thread count; the sums over rows (dot products, norms,
+ * the energy and balance reductions) stay serial, in row order. */
+#define YZO_PAR_MIN 4096
 
 namespace yzo {
 
@@ -224,6 +236,7 @@ struct Csr {
     }
     void matvec(const vector<double> &x, vector<double> &y) const {
         y.assign(n, 0.0);
+#pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) {
             double s = 0;
             for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[col[k]];
@@ -483,6 +496,7 @@ struct Equation {
     /* ---- SetMatrixA of each term ---- */
     void capillary_A(Csr &M) {
         const double Coef = 1.0 / 3.0;
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             double D[4], Tf[4];
@@ -527,6 +541,7 @@ struct Equation {
     }
     void vgrav_A(Csr &M) {
         const double Coef = -1.0 * cs.Bo / 3.0;
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             double G[4], Tface[4], Tsum[4];
@@ -562,6 +577,7 @@ struct Equation {
     double PiPerH(double hh) const { if (hh <= 0) return 0; return Pi(hh) / hh; }
     void disjoining_A(Csr &M) {
         const double Coef = cs.Dp / 3.0;
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             double G[4];
@@ -628,6 +644,7 @@ struct Equation {
     }
     void bodyforce_A(int mode, Csr &M) {
         const bool side = (mode == 1 && cs.dim == 1);
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             double W[4][13] = {{0}}, D[4], w[4];
@@ -664,6 +681,7 @@ struct Equation {
     }
     void marangoni_A(Csr &M) {
         const double Coef = ma_coef();
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             double Val[13] = {0};
@@ -691,6 +709,7 @@ struct Equation {
     /* ---- SetJacobiMatrix of each term ---- */
     void capillary_J(Csr &M) {
         const double Coef = 1.0 / 3.0;
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             const double Hoo = h[i];
@@ -713,6 +732,7 @@ struct Equation {
     }
     void vgrav_J(Csr &M) {
         const double Coef = -1.0 * cs.Bo / 3.0;
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             double a[4], b[4], gA[4], gC[4], kF[4];
@@ -732,6 +752,7 @@ struct Equation {
     }
     void disjoining_J(Csr &M) {
         const double Coef = cs.Dp / 3.0;
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             double Ad[4], G[4], gA[4], gC[4];
@@ -763,6 +784,7 @@ struct Equation {
     }
     void bodyforce_J(int mode, Csr &M) {
         const bool side = (mode == 1 && cs.dim == 1);
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             double X[4][13] = {{0}};
@@ -792,6 +814,7 @@ struct Equation {
     }
     void marangoni_J(Csr &M) {
         const double Coef = ma_coef();
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (M.bc[i]) continue;
             double a[4], gg[4][13] = {{0}};
@@ -1173,6 +1196,7 @@ struct Equation {
         if (cs.marangoni) marangoni_A(Arest);
         if (cs.centrifugal) bodyforce_A(1, Arest);
         if (cs.disjoining) disjoining_A(Arest);
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (A.bc[i]) continue;
             for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) A.val[k] += Arest.val[k];
@@ -1180,6 +1204,7 @@ struct Equation {
     }
     /* SetMatrixJacobi (:1061-1091) */
     void set_matrix_J() {
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (A.bc[i]) continue;
             for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) J.val[k] = A.val[k];
@@ -1198,6 +1223,7 @@ struct Equation {
         const bool any = cs.capillary || cs.vertical_gravity;
         if (!any) { A.matvec(x, y); return; }
         Arest.matvec(x, y);
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
             if (!A.bc[i]) continue;
             double v = 0;
@@ -1206,9 +1232,10 @@ struct Equation {
         }
         if (cs.capillary) {
             const double Coef = 1.0 / 3.0;
-            double vh[13], vs[13];
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
             for (int i = 0; i < g.n; ++i) {
                 if (A.bc[i]) continue;
+                double vh[13], vs[13];
                 for (int k = 0; k < st.ns; ++k) { vh[k] = x[nb(i, k)]; vs[k] = s[nb(i, k)]; }
                 double r = 0;
                 for (int d = 0; d < st.ndir; ++d) {
@@ -1220,6 +1247,7 @@ struct Equation {
         }
         if (cs.vertical_gravity) {
             const double Coef = -1.0 * cs.Bo / 3.0;
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
             for (int i = 0; i < g.n; ++i) {
                 if (A.bc[i]) continue;
                 double r = 0;
@@ -1250,6 +1278,7 @@ struct SpM {  /* plain CSR, sorted columns not required */
     vector<int> ptr, col;
     vector<double> val;
     void mv(const double *x, double *y) const {
+#pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) {
             double s = 0;
             for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[col[k]];
@@ -1283,24 +1312,50 @@ static SpM transpose(const SpM &A, int ncols) {
     return T;
 }
 /* C = A * B (A: n x m, B: m x p) */
+/* C = A B (B with p columns); each row's products are summed in A's then
+ * B's storage order and its columns sorted. Over OpenMP threads the rows
+ * are cut into contiguous chunks, each built apart and joined in row order:
+ * the same matrix as one thread builds. */
 static SpM spgemm(const SpM &A, const SpM &B, int p) {
     SpM C; C.n = A.n; C.ptr.assign(A.n + 1, 0);
-    vector<int> marker(p, -1);
-    vector<double> acc(p, 0.0);
-    vector<int> cols;
-    for (int i = 0; i < A.n; ++i) {
-        cols.clear();
-        for (int ka = A.ptr[i]; ka < A.ptr[i + 1]; ++ka) {
-            const int j = A.col[ka]; const double va = A.val[ka];
-            for (int kb = B.ptr[j]; kb < B.ptr[j + 1]; ++kb) {
-                const int c = B.col[kb];
-                if (marker[c] != i) { marker[c] = i; acc[c] = 0; cols.push_back(c); }
-                acc[c] += va * B.val[kb];
+    int nch = 1;
+#ifdef _OPENMP
+    if (A.n > YZO_PAR_MIN) nch = omp_get_max_threads();
+#endif
+    vector<vector<int>> ccol(nch);
+    vector<vector<double>> cval(nch);
+#pragma omp parallel if (nch > 1)
+    {
+        int t = 0, T = 1;
+#ifdef _OPENMP
+        t = omp_get_thread_num(); T = omp_get_num_threads();
+#endif
+        vector<int> marker(p, -1);
+        vector<double> acc(p, 0.0);
+        vector<int> cols;
+        for (int ch = t; ch < nch; ch += T) {
+            const int lo = (int)((long long)A.n * ch / nch), hi = (int)((long long)A.n * (ch + 1) / nch);
+            for (int i = lo; i < hi; ++i) {
+                cols.clear();
+                for (int ka = A.ptr[i]; ka < A.ptr[i + 1]; ++ka) {
+                    const int j = A.col[ka]; const double va = A.val[ka];
+                    for (int kb = B.ptr[j]; kb < B.ptr[j + 1]; ++kb) {
+                        const int c = B.col[kb];
+                        if (marker[c] != i) { marker[c] = i; acc[c] = 0; cols.push_back(c); }
+                        acc[c] += va * B.val[kb];
+                    }
+                }
+                std::sort(cols.begin(), cols.end());
+                for (int c : cols) { ccol[ch].push_back(c); cval[ch].push_back(acc[c]); }
+                C.ptr[i + 1] = (int)cols.size();
             }
         }
-        std::sort(cols.begin(), cols.end());
-        for (int c : cols) { C.col.push_back(c); C.val.push_back(acc[c]); }
-        C.ptr[i + 1] = (int)C.col.size();
+    }
+    for (int i = 0; i < A.n; ++i) C.ptr[i + 1] += C.ptr[i];
+    C.col.reserve(C.ptr[A.n]); C.val.reserve(C.ptr[A.n]);
+    for (int ch = 0; ch < nch; ++ch) {
+        C.col.insert(C.col.end(), ccol[ch].begin(), ccol[ch].end());
+        C.val.insert(C.val.end(), cval[ch].begin(), cval[ch].end());
     }
     return C;
 }
@@ -1470,16 +1525,20 @@ struct Amg {
         if (l + 1 == lv.size()) { coarse_solve(L.b, L.x); return; }
         const int n = L.A.n;
         /* pre-smoothing from x = 0: x = M b */
+        #pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) L.x[i] = L.spai[i] * L.b[i];
         L.A.mv(L.x.data(), L.r.data());
+        #pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) L.r[i] = L.b[i] - L.r[i];
         AmgLevel &Nx = lv[l + 1];
         L.R.mv(L.r.data(), Nx.b.data());
         cycle(l + 1);
         vector<double> corr(n);
         L.P.mv(Nx.x.data(), corr.data());
+        #pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) L.x[i] += corr[i];
         L.A.mv(L.x.data(), L.r.data());
+        #pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) L.x[i] += L.spai[i] * (L.b[i] - L.r[i]);
     }
     void apply(const vector<double> &rhs, vector<double> &x) {
@@ -1512,6 +1571,7 @@ static int bicgstab(const SpM &A, const vector<double> &b, vector<double> &x, Pc
         if (it == 0) p = r;
         else {
             const double beta = (rho1 * alpha) / (rho2 * omega);
+            #pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
             for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
         }
         M.apply(p, ph);
@@ -1519,6 +1579,7 @@ static int bicgstab(const SpM &A, const vector<double> &b, vector<double> &x, Pc
         const double rv = dot(rh, v);
         if (rv == 0) { iters = it; resid = res / nb; return 2; }
         alpha = rho1 / rv;
+        #pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
         res = std::sqrt(dot(s, s));
         if (res < eps) {
@@ -1531,7 +1592,9 @@ static int bicgstab(const SpM &A, const vector<double> &b, vector<double> &x, Pc
         const double tt = dot(t, t);
         omega = (tt != 0) ? dot(t, s) / tt : 0.0;
         if (omega == 0) { iters = it; resid = res / nb; return 2; }
+        #pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) x[i] += alpha * ph[i] + omega * sh[i];
+        #pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) r[i] = s[i] - omega * t[i];
         res = std::sqrt(dot(r, r));
         rho2 = rho1;
@@ -1753,6 +1816,7 @@ struct Stepper {
         const double th = eq.cs.theta;
         Bv.assign(n, 0.0);
         if (!eq.cs.evaporation && !eq.cs.source) return;
+#pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) {
             if (eq.A.bc[i]) continue;
             const double evp = (1.0 - th) * evap_old[i] + th * evap_at(phi_guess, eq.c, i, dt_now);
@@ -1794,6 +1858,7 @@ struct Stepper {
     void make_IA() {
         const double th = eq.cs.theta, dt = dt_now;
         Csr &J = eq.J, &IA = eq.IA;
+#pragma omp parallel for schedule(static) if (J.n > YZO_PAR_MIN)
         for (int i = 0; i < J.n; ++i) {
             const int p0 = J.ptr[i], p1 = J.ptr[i + 1];
             if (J.bc[i]) { for (int k = p0; k < p1; ++k) IA.val[k] = J.val[k]; continue; }
@@ -1803,6 +1868,7 @@ struct Stepper {
     }
     void make_rhs() {
         const double th = eq.cs.theta, dt = dt_now;
+#pragma omp parallel for schedule(static) if (eq.g.n > YZO_PAR_MIN)
         for (int i = 0; i < eq.g.n; ++i) {
             if (eq.A.bc[i]) { Rhs[i] = 0; continue; }
             Rhs[i] = phi_old[i] - phi_guess[i] - (dt * th * Ax_guess[i]) - (dt * (1.0 - th) * Ax_old[i]) - dt * Bv[i];
@@ -2519,6 +2585,18 @@ static thread_local std::string g_err;
 /* C API of the oracle (ctypes)                                         */
 /* ------------------------------------------------------------------ */
 extern "C" {
+
+/* the threads of the row loops: n > 0 sets them; returns the number in
+ * force (1 when built without OpenMP) */
+int yzo_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 void yzo_case_defaults(yz_case *c) {
     std::memset(c, 0, sizeof(*c));

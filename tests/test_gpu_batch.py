@@ -132,8 +132,8 @@ def test_batch_refuses_what_it_cannot_take():
         HipBatch([(c, h, s, cc, dt)])
 
 
-def test_config4_sweep_of_256_films():
-    """Config 4 at its own size: the 256 films of cases.sweep (sides 64 to
+def test_sweep_of_256_films():
+    """The parameter sweep at its own size: the 256 films of cases.sweep (sides 64 to
     512, 19.8 M cells) advanced one step together. Every film reaches, to
     the bit and with the same Newton and Krylov counts, what it reaches in
     a step context of its own with the same network; and one film of each

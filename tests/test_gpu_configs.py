@@ -1,8 +1,8 @@
-"""Parity at the benchmark configurations' own sizes, through the C-ABI.
+"""Parity at the benchmark workloads' own sizes, through the C-ABI.
 
-* Config 1 (1-D capillary leveling, 512 cells): all 200 steps against the
+* The 1-D capillary leveling film (512 cells): all 200 steps against the
   CPU oracle, max |dh| / max h <= 1e-10 after every step.
-* Config 2 (2-D spinodal dewetting, 224^2 periodic, adaptive dt): all 2000
+* The 2-D spinodal dewetting film (224^2 periodic, adaptive dt): all 2000
   steps against the oracle's own run of them
   (tests/golden/config2_spinodal224.npz, written by
   tests/golden/make_config2.py): the same step size after every step, the
@@ -18,7 +18,7 @@ pass k-1 left (relative residual up to 1e-4); the device's structured AMG
 and the oracle's AMG meet that tolerance with different error vectors, so
 a correction that lands next to the threshold may land on either side. The
 film bar holds at every step regardless.
-* Config 3 (2-D all-physics, 1024^2): one whole step of the plain FP64
+* The 2-D all-physics film (1024^2): one whole step of the plain FP64
   path against the oracle (the same bar, equal Newton counts), and of the
   learned FP32 path (max |dh| / max h <= 1e-8, the bar set for the learned
   path at this Newton tolerance); then a size-independent property over
@@ -70,12 +70,12 @@ def _pair(c, h, s, cc, dt, steps, label, every=1):
     g.close()
 
 
-def test_config1_all_200_steps():
+def test_leveling512_all_200_steps():
     c, h, s, cc, dt, steps = cases.leveling_1d(nx=512, steps=200)
-    _pair(c, h, s, cc, dt, steps, "config 1")
+    _pair(c, h, s, cc, dt, steps, "leveling 512")
 
 
-def test_config2_all_2000_steps():
+def test_spinodal224_all_2000_steps():
     gold = np.load(GOLDEN2)
     c, h, s, cc, dt, steps = cases.spinodal_2d(n=224)
     assert steps == len(gold["nk"])
@@ -96,13 +96,13 @@ def test_config2_all_2000_steps():
             hg = _interior(c, g.h())
             rel = np.abs(hg - gold[key]).max() / np.abs(gold[key]).max()
             worst = max(worst, rel)
-            print(f"config 2 step {k + 1}: t {clk[0]:.6e} dt {clk[1]:.3e} max|dh|/max h {rel:.2e}", flush=True)
+            print(f"spinodal 224 step {k + 1}: t {clk[0]:.6e} dt {clk[1]:.3e} max|dh|/max h {rel:.2e}", flush=True)
             assert rel <= 1e-10, f"step {k + 1}: {rel:.2e}"
-    print(f"config 2: {steps} steps, worst {worst:.2e}, borderline exits (step, oracle, device, why): {borderline}")
+    print(f"spinodal 224: {steps} steps, worst {worst:.2e}, borderline exits (step, oracle, device, why): {borderline}")
     g.close()
 
 
-def test_config3_full_size():
+def test_allphys1024_full_size():
     c, h, s, cc, dt, _ = cases.allphys_2d(n=1024, dt=2e-6)
     o = ob.OracleStep(c, h, s, cc, dt)
     ro = o.advance()
@@ -116,7 +116,7 @@ def test_config3_full_size():
         rg = g.advance()
         assert rg[0] == 0
         rel = np.abs(ho - g.h()).max() / np.abs(ho).max()
-        print(f"config 3 {name}: oracle nk={ro[1]} lin={ro[2]}, hip nk={rg[1]} lin={rg[2]}, rel {rel:.2e}")
+        print(f"all-physics 1024 {name}: oracle nk={ro[1]} lin={ro[2]}, hip nk={rg[1]} lin={rg[2]}, rel {rel:.2e}")
         assert rel <= bar
         if pc == capi.PC_AMG:
             assert rg[1] == ro[1]

@@ -122,7 +122,7 @@ def _big(evap=1):
     return c, h, s, cc
 
 
-def test_config5_size_solute_volume_and_uniformity():
+def test_concz256_size_solute_volume_and_uniformity():
     dt = 0.01
     c, h, s, cc = _big()
     g = HipCZ(c, h, s, cc, dt)

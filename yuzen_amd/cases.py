@@ -205,7 +205,7 @@ def sweep(n_films=256, seed=4242, **kw):
 
 
 def concz_2d(n=256, nz=32, L=10.0, dt=0.01, steps=500, c0=0.3):
-    """Config 5, the ConcZ film: n x n cells x nz levels through the film,
+    """The ConcZ film: n x n cells x nz levels through the film,
     periodic, h = 1 + 0.05 cos(2 pi x / L) cos(2 pi y / L) on a flat
     substrate, c0 = 0.3 uniform; capillary and Marangoni flow
     (sigma = 1 + Ma (c_s - c0), Ma = 1, on the surface concentration),
