@@ -256,8 +256,9 @@ int yz_step_timing(yz_step *st, double *ms6);
 /* launch timing of one kernel of the Krylov iteration of the case's own
  * solve, HIP events on the step's stream: with the fused FP32 iteration of
  * the learned preconditioner, its first kernel (K1: the x and r update,
- * p, ph = M p and (r, r)) once per chunk of iterations, with the start and
- * stop stamps of the dispatch itself (hipExtLaunchKernel); otherwise the
+ * p, ph = M p and (r, r)) once per chunk of eight iterations (the third
+ * iteration's, amid back-to-back launches), with the start and stop stamps
+ * of the dispatch itself (hipExtLaunchKernel); otherwise the
  * fine-level stencil product (the SpMV v = A p with its fused dot product)
  * of every iteration between two event markers. Returns the launches timed
  * and their total milliseconds since the last reset; enable 1 resets and
