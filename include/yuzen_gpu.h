@@ -440,6 +440,12 @@ void yz_batch_destroy(yz_batch *b);
 int yz_tune_tiles(int nx, int ny, int nfilms, int iters, int *rows, double *us2);
 int yz_tile_rows(int nx, int ny);
 int yz_set_tile_rows(int nx, int ny, int rows);
+/* The row kernel of 2-D films (process-wide): 1, the default, stages each
+ * tile of 64 x 4 cells with its halo in LDS and forms every face's
+ * quantities once for both of its cells; 0 forms them per cell in
+ * registers. Both give the same rows to the bit. Returns the previous
+ * setting; values other than 0 and 1 only read it. */
+int yz_set_row_kernel(int tiled);
 
 /* ---- linear-solve seam -------------------------------------------------
  * Solve A x = b, A in CRS (rowptr[n+1], col[nnz], val[nnz]; any column

@@ -91,6 +91,8 @@ def test_spinodal224_all_2000_steps():
         if rg[1] != gold["nk"][k]:
             lo = gold["pass_corr"][k, :gold["nk"][k]]
             borderline.append((k, int(gold["nk"][k]), rg[1], borderline_exit(c, lo, g.pass_log()[0])))
+        if (k + 1) % 100 == 0:
+            print(f"spinodal 224: {k + 1} steps, t {clk[0]:.6e} dt {clk[1]:.3e}", flush=True)
         key = f"h{k + 1}"
         if key in gold.files:
             hg = _interior(c, g.h())

@@ -258,6 +258,7 @@ def load():
     lib.yz_tune_tiles.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), pd]
     lib.yz_tile_rows.argtypes = [C.c_int, C.c_int]
     lib.yz_set_tile_rows.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.yz_set_row_kernel.argtypes = [C.c_int]
     lib.yz_batch_destroy.argtypes = [C.c_void_p]
     lib.yz_cz_case_defaults.argtypes = [P(YzCzCase)]
     lib.yz_cz_create.restype = C.c_void_p
@@ -318,6 +319,12 @@ def tune_tiles(nx, ny, nfilms=1, iters=40):
 
 def tile_rows(nx, ny):
     return load().yz_tile_rows(nx, ny)
+
+
+def set_row_kernel(tiled):
+    """the row kernel of 2-D films, process-wide: 1 LDS-tiled (the
+    default), 0 per cell; returns the previous setting"""
+    return load().yz_set_row_kernel(int(tiled))
 
 
 def set_tile_rows(nx, ny, rows):

@@ -121,8 +121,13 @@ struct Mob {
     }
     // FaceDK(h1, h2) with both derivatives, dF/dh1 and dF/dh2, from one
     // evaluation of g at each height (the same expressions as FaceDK,
-    // FaceDK_d1(h1, h2) and FaceDK_d1(h2, h1))
-    __device__ void FaceDK3(double h1, double h2, double &F, double &F1, double &F2) const {
+    // FaceDK_d1(h1, h2) and FaceDK_d1(h2, h1)). CACHED: g(h1), g(h2) come
+    // from the caller (G1, G2, each Gint of its height) instead of being
+    // formed here; the values are the same. Swapping h1 and h2 gives the
+    // same F and swaps F1 and F2, to the bit (hm and dl * dl do not change,
+    // dl and the differences of g change sign exactly).
+    template <bool CACHED>
+    __device__ void FaceDK3T(double h1, double h2, double G1, double G2, double &F, double &F1, double &F2) const {
         if (!(h1 > 0) || !(h2 > 0)) { F = F1 = F2 = 0; return; }
         const double hm = 0.5 * (h1 + h2), dl = h2 - h1;
         if (fabs(dl) < 1e-3 * hm) {
@@ -137,12 +142,15 @@ struct Mob {
             F2 = 0.5 * Fhm - Fdl2;
             return;
         }
-        const double g1 = Gint(h1), g2 = Gint(h2);
+        const double g1 = CACHED ? G1 : Gint(h1), g2 = CACHED ? G2 : Gint(h2);
         const double dg = g2 - g1;
         F = dl / dg;
         F1 = (-dg + dl / D(h1)) / (dg * dg);
         const double dg2 = g1 - g2, dl2 = h1 - h2;
         F2 = (-dg2 + dl2 / D(h2)) / (dg2 * dg2);
+    }
+    __device__ void FaceDK3(double h1, double h2, double &F, double &F1, double &F2) const {
+        FaceDK3T<false>(h1, h2, 0.0, 0.0, F, F1, F2);
     }
     // Mobility::Face / Face_d1 (0 when a side is dry)
     __device__ double Face(double h1, double h2) const {

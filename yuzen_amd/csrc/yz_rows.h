@@ -132,26 +132,85 @@ __device__ inline double face_grad(const RowArgs &A, const double *v, int d) {
 
 __device__ inline bool is_wall(int t) { return t == YZ_BC_SYMMETRY || t == YZ_BC_PINNING; }
 
-// mode 0: A(h)h only (the explicit half); mode 1: the whole pass
+// What the row of a cell takes from one of its faces, oriented for that
+// cell (c: the cell's own film, a: the neighbour's across the face)
+struct FaceQ {
+    double mu;               // viscosity at the face's mean concentration
+    double D, gC, gA;        // capillary: M sigma / mu and its derivatives in h_c, h_a (mode 1)
+    double G, T, gCf, gAf;   // potential terms: M / mu, M / (h_c + h_a) / mu, dM/dh_c / mu, dM/dh_a / mu
+};
+
+// The quantities of a face from the films hc, ha and concentrations cc, ca
+// of its two cells (Fm, Fm1, Fm2: the slip mobility and its derivatives in
+// hc and ha, slip only). Every expression is symmetric in the two cells:
+// with hc and ha (cc and ca, Fm1 and Fm2) exchanged the values are the same
+// to the bit and gC, gA (gCf, gAf) trade places, so a face can be formed
+// once for both of its cells.
+template <int MODE>
+__device__ inline void face_q(const RowArgs &A, double hc, double ha, double cc, double ca, double Fm, double Fm1,
+                              double Fm2, FaceQ &q) {
+    const Mob &mob = A.mob;
+    q.D = q.gC = q.gA = q.G = q.T = q.gCf = q.gAf = 0;
+    const double cf = 0.5 * (cc + ca);
+    q.mu = visc_of(A, cf);
+    if (A.cap) {
+        const double Hoo = hc, Haj = ha;
+        const double so = sig_of(A, cf) / q.mu;
+        const double val = mob.slip ? Fm : 2.0 * (Hoo * Hoo) * (Haj * Haj) / (Hoo + Haj);
+        q.D = val * so;
+        if (MODE == 1) {
+            const double den = (Hoo + Haj) * (Hoo + Haj);
+            const double vc = mob.slip ? Fm1 : 2.0 * (Hoo * Haj * Haj) * (Hoo + Haj + Haj) / den;
+            const double va = mob.slip ? Fm2 : 2.0 * (Haj * Hoo * Hoo) * (Haj + Hoo + Hoo) / den;
+            q.gC = vc * so;
+            q.gA = va * so;
+        }
+    }
+    if (A.vg || A.dj) {
+        const double ho = hc, mu = q.mu;
+        const double gf = mob.slip ? Fm : ((ho > 0 && ha > 0) ? 2.0 * (ho * ho) * (ha * ha) / (ho + ha) : 0.0);
+        q.G = gf / mu;
+        q.T = (gf == 0) ? 0.0 : gf / (ho + ha) / mu;
+        if (MODE == 1) {
+            if (mob.slip) { q.gCf = Fm1 / mu; q.gAf = Fm2 / mu; }
+            else if (ho > 0 && ha > 0) {
+                const double den = (ho + ha) * (ho + ha);
+                q.gCf = 2.0 * (ho * ha * ha) * (ho + ha + ha) / den / mu;
+                q.gAf = 2.0 * (ha * ho * ho) * (ha + ho + ho) / den / mu;
+            }
+        }
+    }
+}
+
+// the disjoining pressure at a point: Pi = (h*/h)^n - (h*/h)^m, dPi/dh and
+// Pi / h (0 where the film is not positive)
+__device__ inline void pi_point(const RowArgs &A, double hh, double &pis, double &dpis, double &pih) {
+    pis = dpis = pih = 0;
+    if (hh > 0) {
+        const double ih = 1.0 / hh;
+        const double r = A.hstar * ih;
+        const double rn = pow_k(r, A.pn, A.pn_i), rm = pow_k(r, A.pm, A.pm_i);
+        pis = rn - rm;
+        dpis = -(A.pn * rn - A.pm * rm) * ih;
+        pih = pis * ih;
+    }
+}
+
+// The row of cell i (padded position p, padded coordinates ixp, iyp) from
+// its stencil H, S, Cc, its faces fq and the disjoining values of its
+// centre and neighbours: A(h)h (mode 0), or the residual, IA and what goes
+// with them (mode 1; rabs: |Rhs|)
 template <int DIM, int MODE>
-__global__ void __launch_bounds__(256) k_rows(RowArgs A) {
+__device__ __forceinline__ void row_body(const RowArgs &A, int i, int ixp, int iyp, int p, const double (&H)[DIM == 2 ? 13 : 5],
+                                         const double (&S)[DIM == 2 ? 13 : 5], const double (&Cc)[DIM == 2 ? 13 : 5],
+                                         const FaceQ (&fq)[DIM == 2 ? 4 : 2], const double (&PiS)[DIM == 2 ? 13 : 5],
+                                         const double (&dPiS)[DIM == 2 ? 13 : 5], const double (&PiH)[DIM == 2 ? 13 : 5],
+                                         double &rabs) {
     constexpr int NS = DIM == 2 ? 13 : 5;
     constexpr int ND = DIM == 2 ? 4 : 2;
     constexpr int C0 = DIM == 2 ? 6 : 2;
     const Geo &g = A.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = i < g.N;
-    double rabs = 0;
-    if (active) {
-        const int ix = i % g.nx, iy = i / g.nx;
-        const int ixp = ix + 2, iyp = DIM == 2 ? iy + 2 : 0;
-        const int p = iyp * g.nxp + ixp;
-        double H[NS], S[NS], Cc[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int q = p + slot_ox(DIM, k) + slot_oy(DIM, k) * g.nxp;
-            H[k] = A.h[q]; S[k] = A.s[q]; Cc[k] = A.c[q];
-        }
+    {
         double Ar[NS], Jc[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k) { Ar[k] = 0; Jc[k] = 0; }
@@ -165,41 +224,13 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
             return iyp == g.nyp - 3 && is_wall(g.bc[3]);
         };
 
-        // per face: viscosity and surface tension at the mean concentration
-        // of the two cells (every term takes them there), and the slip
-        // mobility with both derivatives, evaluated once and shared by the
-        // capillary, vertical-gravity and disjoining terms
-        double muF[ND], sgF[ND], Fm[ND], Fm1[ND], Fm2[ND];
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            const int a = adj_slot(DIM, d);
-            const double cf = 0.5 * (Cc[C0] + Cc[a]);
-            muF[d] = visc_of(A, cf);
-            sgF[d] = sig_of(A, cf);
-            if (mob.slip) mob.FaceDK3(H[C0], H[a], Fm[d], Fm1[d], Fm2[d]);
-            else Fm[d] = Fm1[d] = Fm2[d] = 0;
-        }
-
         // ---- capillary pressure (conservative; coefficient 1/3) ----
         if (A.cap) {
             const double Coef = 1.0 / 3.0;
             const double Hoo = H[C0];
             double D[ND], gA[ND], gC[ND];
 #pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                const int a = adj_slot(DIM, d);
-                const double Haj = H[a];
-                const double so = sgF[d] / muF[d];
-                const double val = mob.slip ? Fm[d] : 2.0 * (Hoo * Hoo) * (Haj * Haj) / (Hoo + Haj);
-                D[d] = val * so;
-                if (MODE == 1) {
-                    const double den = (Hoo + Haj) * (Hoo + Haj);
-                    const double vc = mob.slip ? Fm1[d] : 2.0 * (Hoo * Haj * Haj) * (Hoo + Haj + Haj) / den;
-                    const double va = mob.slip ? Fm2[d] : 2.0 * (Haj * Hoo * Hoo) * (Haj + Hoo + Hoo) / den;
-                    gC[d] = vc * so;
-                    gA[d] = va * so;
-                }
-            }
+            for (int d = 0; d < ND; ++d) { D[d] = fq[d].D; gC[d] = fq[d].gC; gA[d] = fq[d].gA; }
             // residual, face by face
             double r = 0;
 #pragma unroll
@@ -246,25 +277,10 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
 
         // face mobility of the potential terms (vertical gravity, disjoining)
         double Gf[ND], Tk[ND], gAf[ND], gCf[ND], inv2[ND];
-        if (A.vg || A.dj) {
 #pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                const int a = adj_slot(DIM, d);
-                const double ho = H[C0], ha = H[a];
-                const double mu = muF[d];
-                const double gf = mob.slip ? Fm[d] : ((ho > 0 && ha > 0) ? 2.0 * (ho * ho) * (ha * ha) / (ho + ha) : 0.0);
-                Gf[d] = gf / mu;
-                Tk[d] = (gf == 0) ? 0.0 : gf / (ho + ha) / mu;
-                inv2[d] = A.inv_h2[d / 2];
-                if (MODE == 1) {
-                    if (mob.slip) { gCf[d] = Fm1[d] / mu; gAf[d] = Fm2[d] / mu; }
-                    else if (ho > 0 && ha > 0) {
-                        const double den = (ho + ha) * (ho + ha);
-                        gCf[d] = 2.0 * (ho * ha * ha) * (ho + ha + ha) / den / mu;
-                        gAf[d] = 2.0 * (ha * ho * ho) * (ha + ho + ho) / den / mu;
-                    } else { gCf[d] = gAf[d] = 0; }
-                }
-            }
+        for (int d = 0; d < ND; ++d) {
+            Gf[d] = fq[d].G; Tk[d] = fq[d].T; gCf[d] = fq[d].gCf; gAf[d] = fq[d].gAf;
+            inv2[d] = A.inv_h2[d / 2];
         }
 
         // ---- vertical gravity (conservative; -Bo/3) ----
@@ -330,7 +346,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
                 const double hf = limited_face(f, dv, wt, gd);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) { wt[q] *= ihs; gd[q] *= ihs; }
-                double D = (mob.slip ? mob.DperH(hf) : hf * hf) / muF[j];
+                double D = (mob.slip ? mob.DperH(hf) : hf * hf) / fq[j].mu;
                 if (wall(j)) D = 0;
                 const double sg = (j % 2 == 0) ? -1.0 : 1.0;
 #pragma unroll
@@ -364,7 +380,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
                 for (int q = 0; q < 4; ++q) f[q] = H[face_slot(DIM, d, q)];
                 const double hf = limited_face(f, up, wt, gd);
                 const bool wl = wall(d);
-                const double mu = muF[d];
+                const double mu = fq[d].mu;
                 const double Sf = wl ? 0.0 : (mob.slip ? dc / mu * (hf + 2.0 * mob.lambda) : dc / mu * hf);
                 const double sg = (d % 2 == 0) ? -1.0 : 1.0;
                 const double Sfh = Sf * ihh;
@@ -389,23 +405,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
         // ---- disjoining pressure (Dp/3), Pi = (h*/h)^n - (h*/h)^m ----
         if (A.dj) {
             const double Coef = A.Dp / 3.0;
-            // (h*/h)^n and (h*/h)^m once per point; Pi and dPi from them
-            double PiS[NS], dPiS[NS], PiH[NS];
-#pragma unroll
-            for (int k = 0; k < NS; ++k) { PiS[k] = 0; dPiS[k] = 0; PiH[k] = 0; }
-#pragma unroll
-            for (int d = -1; d < ND; ++d) {
-                const int k = d < 0 ? C0 : adj_slot(DIM, d);
-                const double hh = H[k];
-                if (hh > 0) {
-                    const double ih = 1.0 / hh;
-                    const double r = A.hstar * ih;
-                    const double rn = pow_k(r, A.pn, A.pn_i), rm = pow_k(r, A.pm, A.pm_i);
-                    PiS[k] = rn - rm;
-                    dPiS[k] = -(A.pn * rn - A.pm * rm) * ih;
-                    PiH[k] = PiS[k] * ih;
-                }
-            }
+            // (h*/h)^n and (h*/h)^m per point (pi_point), from the caller
             auto PiPerH = [&](int k) { return PiH[k]; };
             double cen = 0;
 #pragma unroll
@@ -503,17 +503,170 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs A) {
             }
         }
     }
-    if (MODE == 1) {
-        __shared__ double sh[4];
-        for (int off = 32; off > 0; off >>= 1) rabs = fmax(rabs, __shfl_down(rabs, off, 64));
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = rabs;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double m = 0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, sh[w]);
-            atomicMax(A.rmax, (unsigned long long)__double_as_longlong(m));
-        }
+}
+
+// the largest |Rhs| of the block into A.rmax (mode 1)
+__device__ inline void rows_rmax(const RowArgs &A, double rabs) {
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) rabs = fmax(rabs, __shfl_down(rabs, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = rabs;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, sh[w]);
+        atomicMax(A.rmax, (unsigned long long)__double_as_longlong(m));
     }
+}
+
+// One thread per interior cell, its stencil and faces formed in registers
+// (1-D films; 2-D films where the tiled kernel below is switched off).
+// mode 0: A(h)h only (the explicit half); mode 1: the whole pass
+template <int DIM, int MODE>
+__global__ void __launch_bounds__(256) k_rows(RowArgs A) {
+    constexpr int NS = DIM == 2 ? 13 : 5;
+    constexpr int ND = DIM == 2 ? 4 : 2;
+    constexpr int C0 = DIM == 2 ? 6 : 2;
+    const Geo &g = A.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rabs = 0;
+    if (i < g.N) {
+        const int ix = i % g.nx, iy = i / g.nx;
+        const int ixp = ix + 2, iyp = DIM == 2 ? iy + 2 : 0;
+        const int p = iyp * g.nxp + ixp;
+        double H[NS], S[NS], Cc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int q = p + slot_ox(DIM, k) + slot_oy(DIM, k) * g.nxp;
+            H[k] = A.h[q]; S[k] = A.s[q]; Cc[k] = A.c[q];
+        }
+        FaceQ fq[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const int a = adj_slot(DIM, d);
+            double Fm = 0, Fm1 = 0, Fm2 = 0;
+            if (A.mob.slip) A.mob.FaceDK3(H[C0], H[a], Fm, Fm1, Fm2);
+            face_q<MODE>(A, H[C0], H[a], Cc[C0], Cc[a], Fm, Fm1, Fm2, fq[d]);
+        }
+        double PiS[NS], dPiS[NS], PiH[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) { PiS[k] = 0; dPiS[k] = 0; PiH[k] = 0; }
+        if (A.dj) {
+#pragma unroll
+            for (int d = -1; d < ND; ++d) {
+                const int k = d < 0 ? C0 : adj_slot(DIM, d);
+                pi_point(A, H[k], PiS[k], dPiS[k], PiH[k]);
+            }
+        }
+        row_body<DIM, MODE>(A, i, ixp, iyp, p, H, S, Cc, fq, PiS, dPiS, PiH, rabs);
+    }
+    if (MODE == 1) rows_rmax(A, rabs);
+}
+
+// The same rows of a 2-D film from tiles of 64 x 4 cells (256 threads).
+// The tile's film, substrate and concentration with their 2-cell halo are
+// staged in LDS once; g(h) of the slip mobility and the disjoining powers
+// are formed once per point; every face's quantities (face_q) once per face
+// for both of its cells. A cell's row then reads its stencil and its four
+// faces from LDS: the same values as k_rows forms for itself, so the same
+// rows to the bit, with about half the transcendental work per cell.
+constexpr int RTX = 64, RTY = 4;
+constexpr int RLX = RTX + 4, RLY = RTY + 4;                        // with the halo
+constexpr int RNXF = (RTX + 1) * RTY, RNYF = RTX * (RTY + 1);      // x-faces, y-faces of a tile
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_rows_tiled(RowArgs A) {
+    constexpr int NS = 13, ND = 4, C0 = 6;
+    __shared__ double sH[RLY][RLX], sS[RLY][RLX], sC[RLY][RLX];
+    __shared__ double sG[RLY - 2][RLX - 2];              // g(h), points of the 1-cell halo
+    __shared__ double sP[3][RLY - 2][RLX - 2];           // Pi, dPi/dh, Pi/h there
+    __shared__ double sF[8][RNXF + RNYF];                // face_q of every face, for its lower cell
+    const Geo &g = A.g;
+    const int bx = blockIdx.x * RTX, by = blockIdx.y * RTY;   // the tile's first interior cell
+    const int tid = threadIdx.x;
+    // the tile and its halo (LDS point (lx, ly) is padded point (bx + lx, by + ly))
+    for (int q = tid; q < RLY * RLX; q += 256) {
+        const int ly = q / RLX, lx = q - ly * RLX;
+        const int px = bx + lx, py = by + ly;
+        double h = 0, s = 0, c = 0;
+        if (px < g.nxp && py < g.nyp) {
+            const int j = py * g.nxp + px;
+            h = A.h[j]; s = A.s[j]; c = A.c[j];
+        }
+        sH[ly][lx] = h; sS[ly][lx] = s; sC[ly][lx] = c;
+    }
+    __syncthreads();
+    const bool slip = A.mob.slip;
+    if (slip || A.dj) {
+        for (int q = tid; q < (RLY - 2) * (RLX - 2); q += 256) {
+            const int ly = q / (RLX - 2), lx = q - ly * (RLX - 2);
+            const double h = sH[ly + 1][lx + 1];
+            if (slip) sG[ly][lx] = h > 0 ? A.mob.Gint(h) : 0.0;
+            if (A.dj) pi_point(A, h, sP[0][ly][lx], sP[1][ly][lx], sP[2][ly][lx]);
+        }
+        __syncthreads();
+    }
+    // faces: x-face (fx, fy) lies between tile cells (fx - 1, fy) and
+    // (fx, fy), fx = 0..RTX; y-face (fx, fy) between (fx, fy - 1) and (fx, fy)
+    for (int f = tid; f < RNXF + RNYF; f += 256) {
+        int llx, lly, ulx, uly;   // LDS points of the lower and upper cell
+        if (f < RNXF) {
+            const int fy = f / (RTX + 1), fx = f - fy * (RTX + 1);
+            llx = fx + 1; lly = fy + 2; ulx = fx + 2; uly = fy + 2;
+        } else {
+            const int e = f - RNXF, fy = e / RTX, fx = e - fy * RTX;
+            llx = fx + 2; lly = fy + 1; ulx = fx + 2; uly = fy + 2;
+        }
+        const double hl = sH[lly][llx], hu = sH[uly][ulx];
+        double Fm = 0, Fm1 = 0, Fm2 = 0;
+        if (slip) A.mob.FaceDK3T<true>(hl, hu, sG[lly - 1][llx - 1], sG[uly - 1][ulx - 1], Fm, Fm1, Fm2);
+        FaceQ q;
+        face_q<MODE>(A, hl, hu, sC[lly][llx], sC[uly][ulx], Fm, Fm1, Fm2, q);
+        sF[0][f] = q.mu; sF[1][f] = q.D; sF[2][f] = q.gC; sF[3][f] = q.gA;
+        sF[4][f] = q.G; sF[5][f] = q.T; sF[6][f] = q.gCf; sF[7][f] = q.gAf;
+    }
+    __syncthreads();
+    const int tx = tid & (RTX - 1), ty = tid / RTX;
+    const int ix = bx + tx, iy = by + ty;
+    double rabs = 0;
+    if (ix < g.nx && iy < g.ny) {
+        const int i = iy * g.nx + ix;
+        const int ixp = ix + 2, iyp = iy + 2;
+        const int p = iyp * g.nxp + ixp;
+        double H[NS], S[NS], Cc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int ly = ty + 2 + slot_oy(2, k), lx = tx + 2 + slot_ox(2, k);
+            H[k] = sH[ly][lx]; S[k] = sS[ly][lx]; Cc[k] = sC[ly][lx];
+        }
+        // faces 0 (x-) and 2 (y-): the cell is their upper cell, so the
+        // derivatives trade places; faces 1 (x+) and 3 (y+): the lower
+        FaceQ fq[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const int f = d == 0 ? ty * (RTX + 1) + tx : d == 1 ? ty * (RTX + 1) + tx + 1
+                        : d == 2 ? RNXF + ty * RTX + tx : RNXF + (ty + 1) * RTX + tx;
+            const bool upper = (d % 2) == 0;
+            fq[d].mu = sF[0][f]; fq[d].D = sF[1][f];
+            fq[d].gC = upper ? sF[3][f] : sF[2][f];
+            fq[d].gA = upper ? sF[2][f] : sF[3][f];
+            fq[d].G = sF[4][f]; fq[d].T = sF[5][f];
+            fq[d].gCf = upper ? sF[7][f] : sF[6][f];
+            fq[d].gAf = upper ? sF[6][f] : sF[7][f];
+        }
+        double PiS[NS], dPiS[NS], PiH[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) { PiS[k] = 0; dPiS[k] = 0; PiH[k] = 0; }
+        if (A.dj) {
+#pragma unroll
+            for (int d = -1; d < ND; ++d) {
+                const int k = d < 0 ? C0 : adj_slot(2, d);
+                const int ly = ty + 1 + slot_oy(2, k), lx = tx + 1 + slot_ox(2, k);
+                PiS[k] = sP[0][ly][lx]; dPiS[k] = sP[1][ly][lx]; PiH[k] = sP[2][ly][lx];
+            }
+        }
+        row_body<2, MODE>(A, i, ixp, iyp, p, H, S, Cc, fq, PiS, dPiS, PiH, rabs);
+    }
+    if (MODE == 1) rows_rmax(A, rabs);
 }
 
 }  // namespace yz

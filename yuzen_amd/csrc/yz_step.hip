@@ -411,11 +411,19 @@ static int b_before(yz_step *S, const double *hold, double dt, double t) {
     return 0;
 }
 
+// the row kernel of 2-D films: the LDS-tiled one (1, the default) or the
+// one thread per cell with its faces in registers (0); the same rows
+static int g_rows_tiled = 1;
+
 static void rows_launch(yz_step *S, int mode, const double *h) {
     RowArgs a = S->ra;
     a.h = h;
     const dim3 grid(nb(S->g.N)), blk(TB);
-    if (S->g.dim == 2) {
+    if (S->g.dim == 2 && g_rows_tiled) {
+        const dim3 tgrid((S->g.nx + RTX - 1) / RTX, (S->g.ny + RTY - 1) / RTY);
+        if (mode == 0) hipLaunchKernelGGL((k_rows_tiled<0>), tgrid, dim3(RTX * RTY), 0, S->st, a);
+        else hipLaunchKernelGGL((k_rows_tiled<1>), tgrid, dim3(RTX * RTY), 0, S->st, a);
+    } else if (S->g.dim == 2) {
         if (mode == 0) hipLaunchKernelGGL((k_rows<2, 0>), grid, blk, 0, S->st, a);
         else hipLaunchKernelGGL((k_rows<2, 1>), grid, blk, 0, S->st, a);
     } else {
@@ -1455,6 +1463,12 @@ int yz_step_energy(yz_step *S, double *F, double *D) {
 // WellMixedModelBase.h:1339-1360): the film's volume and solute, and what
 // the last step evaporated (EvaporationRate, HeightEquationBase.h:359-369,
 // times its dt) and what the nozzle supplied (SuppliedVolume :293-303)
+int yz_set_row_kernel(int tiled) {
+    const int was = g_rows_tiled;
+    if (tiled == 0 || tiled == 1) g_rows_tiled = tiled;
+    return was;
+}
+
 int yz_step_balances(yz_step *S, double *out4) {
     if (!S || !out4) { set_err("null argument"); return 2; }
     const Geo &g = S->g;
