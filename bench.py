@@ -34,10 +34,13 @@ rows for the film's size (yz_tune_tiles; the choice is in the line).
 
 Roofline: the kernel of the learned run with the largest share of its
 time, the first kernel of the fused Krylov iteration (K1, k_f_pm: the x and
-r update, p, ph = M p and (r, r)), its launches timed over the timed region
-by the start and stop stamps of the dispatch itself (hipExtLaunchKernel) on
-the step's own stream. CPU baseline: the FP64 CPU oracle over a bounded
-sample of the same workload.
+r update, p, ph = M p and (r, r)), every launch of the timed region that
+did its work timed on the device clock as the kernels record it (from the
+end of the K4 before it to its own end: its duration on the busy stream,
+dispatch included); the traffic from the rocprofv3
+FETCH_SIZE and WRITE_SIZE counters kept under profiles/. CPU baseline: the
+FP64 CPU oracle over a bounded sample of the same workload (its whole first
+step, OpenMP row loops).
 """
 import argparse
 import json
@@ -376,7 +379,9 @@ def main():
         "traffic": traffic,
         "kernel": f"yz::k_f_pm (K1 of the fused FP32 iteration: x, r update, p, ph = M p on 64x{tile_rows} "
                   "LDS-staged tiles, (r, r))",
-        "timing": "dispatch start/stop stamps (hipExtLaunchKernel) of one K1 per chunk of iterations",
+        "timing": "every K1 launch of the timed region that did its work, from the last block exit of the K4 "
+                  "before it to its own last block exit, on the device clock (wall_clock64) as the kernels "
+                  "record them",
         "launches": L["kcount"],
         "avg_us": round(avg_s * 1e6, 2),
         "bytes_per_launch": bytes_per_launch,

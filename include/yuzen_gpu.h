@@ -254,15 +254,18 @@ int yz_step_run(yz_step *st, int nsteps, int *done, int *status);
  * [5] whole step */
 int yz_step_timing(yz_step *st, double *ms6);
 /* launch timing of one kernel of the Krylov iteration of the case's own
- * solve, HIP events on the step's stream: with the fused FP32 iteration of
- * the learned preconditioner, its first kernel (K1: the x and r update,
- * p, ph = M p and (r, r)) once per chunk of eight iterations (the third
- * iteration's, amid back-to-back launches), with the start and stop stamps
- * of the dispatch itself (hipExtLaunchKernel); otherwise the
- * fine-level stencil product (the SpMV v = A p with its fused dot product)
- * of every iteration between two event markers. Returns the launches timed
- * and their total milliseconds since the last reset; enable 1 resets and
- * starts, 0 resets and stops, -1 only reads */
+ * solve: with the fused FP32 iteration of the learned preconditioner, its
+ * first kernel (K1: the x and r update, p, ph = M p and (r, r)), every
+ * launch that did its work (iterations 1 on), each timed on the device's
+ * own constant-rate clock (wall_clock64) from the last block exit of the
+ * K4 before it to its own last block exit (its duration on the busy
+ * stream, dispatch included), the exits written by the kernels into slots
+ * per iteration and block and reduced after the solve; otherwise the
+ * fine-level stencil product (the SpMV
+ * v = A p with its fused dot product) of every iteration between two HIP
+ * event markers on the step's stream. Returns the launches timed and their
+ * total milliseconds since the last reset; enable 1 resets and starts, 0
+ * resets and stops, -1 only reads */
 int yz_step_kernel_time(yz_step *st, int enable, long long *count, double *total_ms);
 void yz_step_destroy(yz_step *st);
 

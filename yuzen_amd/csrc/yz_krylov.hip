@@ -77,6 +77,7 @@ constexpr int kTileRows[2] = {16, 8};        // the tuner's candidates; the firs
 // reduction: measured at 1024^2 against 256, 512, 1536, 2048 and 4096
 constexpr int G = 1024;
 constexpr int CHUNK = 8;   // iterations enqueued between two reads of the state (even)
+constexpr int kTsCap = 512;   // K1 timing slots (iterations)
 
 // slots of the 2-D 13-point stencil: oM mm om pm Mo mo oo po Po mp op pp oP
 constexpr int OX[13] = {0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0};
@@ -96,6 +97,11 @@ struct Ctl {
     int ntx, nty;    // tiles
     int maxit, window;
     double factor, tol;
+    // launch timing of K1 (null: off): per iteration k and block, on the
+    // device clock (wall_clock64), the block's exit from K1 at
+    // ts[k G + block] and from the K4 before it at ts[(tcap + k) G + block]
+    unsigned long long *ts;
+    int tcap;        // iterations with a slot (later ones share the last)
 };
 
 struct Seq {
@@ -308,6 +314,7 @@ __global__ void __launch_bounds__(TB) k_f_pm(const Film one, const Film *__restr
     const Seq sq = seq(c, q);
     double s[S_N];
     load_sc(sq.in, s);
+    const int it_in = (int)s[S_IT];
     const bool half = !first && s[S_STATE] == ST_HALF;
     const double alpha_prev = s[S_ALPHA];
     if (first) {
@@ -370,6 +377,47 @@ __global__ void __launch_bounds__(TB) k_f_pm(const Film one, const Film *__restr
         });
     }
     store_partials<1>(acc, sq.pout, t0_);
+    if (c.ts && !first) {
+        // this block's exit (a plain store to its own slot)
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const size_t k = it_in < c.tcap ? it_in : c.tcap - 1;
+            c.ts[k * G + lb] = wall_clock64();
+        }
+    }
+}
+
+// clears the timing slots (0: the block did not run that launch)
+__global__ void k_ts_reset(unsigned long long *ts, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ts[i] = 0;
+}
+// K1's period in each iteration k, one block per iteration: from the end
+// of the K4 before it (its last block's exit) to K1's own end (its last
+// block's exit), which on the busy stream is the launch's duration with its
+// dispatch; 0 when a block ran neither
+__global__ void k_ts_span(const unsigned long long *ts, int g, int tcap, unsigned long long *span) {
+    __shared__ unsigned long long e1[256], e4[256], lo[256];
+    const size_t k = blockIdx.x;
+    unsigned long long a = 0, b = 0, z = ~0ull;
+    for (int j = threadIdx.x; j < g; j += blockDim.x) {
+        const unsigned long long x1 = ts[k * G + j], x4 = ts[(tcap + k) * G + j];
+        a = x1 > a ? x1 : a;
+        b = x4 > b ? x4 : b;
+        const unsigned long long m = x1 < x4 ? x1 : x4;
+        z = m < z ? m : z;
+    }
+    e1[threadIdx.x] = a; e4[threadIdx.x] = b; lo[threadIdx.x] = z;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            e1[threadIdx.x] = e1[threadIdx.x + w] > e1[threadIdx.x] ? e1[threadIdx.x + w] : e1[threadIdx.x];
+            e4[threadIdx.x] = e4[threadIdx.x + w] > e4[threadIdx.x] ? e4[threadIdx.x + w] : e4[threadIdx.x];
+            lo[threadIdx.x] = lo[threadIdx.x + w] < lo[threadIdx.x] ? lo[threadIdx.x + w] : lo[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) span[k] = (lo[0] != 0 && e1[0] > e4[0]) ? e1[0] - e4[0] : 0;
 }
 
 // K2 (q = 4k + 2): for k >= 1 the exits on ||r_k||; v = A ph; partial (rh, v)
@@ -498,6 +546,15 @@ __global__ void __launch_bounds__(TB) k_f_at(const Film one, const Film *__restr
         });
     }
     store_partials<3>(acc, sq.pout, t0_);
+    if (c.ts) {
+        // this block's exit, where K1's period in this iteration begins
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int it = (int)s[S_IT];
+            const size_t k = it < c.tcap ? it : c.tcap - 1;
+            c.ts[(c.tcap + k) * G + lb] = wall_clock64();
+        }
+    }
 }
 
 }  // namespace
@@ -597,6 +654,19 @@ int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, cons
         d.c.maxit = maxit; d.c.window = s.stall_window; d.c.factor = s.stall_factor; d.c.tol = tol[f];
         d.g = std::min(d.c.ntx * d.c.nty, G);
         d.gather = T.d_gather + (size_t)f * S_N;
+        d.c.ts = nullptr;
+        d.c.tcap = kTsCap;
+        if (nf == 1 && s.ktime) {
+            if (!s.kts) {
+                // per iteration and block an entry and an exit, then the spans
+                KR_CHECK(hipMalloc((void **)&s.kts, (2 * (size_t)kTsCap * G + kTsCap) * sizeof(unsigned long long)));
+                int dev = 0, khz = 0;
+                KR_CHECK(hipGetDevice(&dev));
+                KR_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+                s.kclock_khz = khz;
+            }
+            d.c.ts = s.kts;
+        }
     }
     KR_CHECK(hipMemcpyAsync(T.d_films, T.h_films, (size_t)nf * sizeof(Film), hipMemcpyHostToDevice, st));
     SolverT<float> &S0 = *S[0];
@@ -660,30 +730,17 @@ int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, cons
         KR_CHECK(hipStreamSynchronize(st));
         return 0;
     };
-    const bool timing = nf == 1 && S0.ktime;
-    if (timing && !S0.kev[0]) { (void)hipEventCreate(&S0.kev[0]); (void)hipEventCreate(&S0.kev[1]); }
+    // K1 launch timing (one film): the device clock spans of its launches
+    const bool timing = nf == 1 && S0.ktime && F0.c.ts;
     // two iterations, k even then odd: every kernel argument is fixed by
     // its position (sequence parities 2, 3, 4, 5; p buffers by k's parity),
     // so the pair can be captured once as a HIP graph and replayed
-    // (timed: K1 of the pair's first iteration carries the dispatch's own
-    // start and stop timestamps, hipExtLaunchKernel, with no marker packets
-    // around it)
-    auto pair = [&](bool timed) {
+    auto pair = [&]() {
         for (int u = 0; u < 2; ++u) {
             FLAUNCH(k_f_av, 0, 2, 1);
             FLAUNCH(k_f_sm, 0, 3);
             FLAUNCH(k_f_at, 0, 4);
-            if (timed && u == 0) {
-                const Group &g0 = grp[0].nb ? grp[0] : grp[1];
-                if (g0.rows == 16)
-                    hipExtLaunchKernelGGL(k_f_pm<true, 16>, dim3(g0.nb), blk, 0, st, S0.kev[0], S0.kev[1], 0, F0, dF,
-                                          (const int2 *)(dB + g0.b0), 5, 0, 1);
-                else
-                    hipExtLaunchKernelGGL(k_f_pm<true, 8>, dim3(g0.nb), blk, 0, st, S0.kev[0], S0.kev[1], 0, F0, dF,
-                                          (const int2 *)(dB + g0.b0), 5, 0, 1);
-            } else {
-                FLAUNCH(k_f_pm, 0, 5, 0, u == 0 ? 1 : 0);
-            }
+            FLAUNCH(k_f_pm, 0, 5, 0, u == 0 ? 1 : 0);
         }
     };
     if (graph) {
@@ -699,17 +756,22 @@ int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, cons
             std::memcpy(&fb, &S0.stall_factor, sizeof(fb));
             key.push_back(fb);
             key.push_back((long long)(intptr_t)b[0]);
+            key.push_back((long long)(intptr_t)F0.c.ts);
         }
         if (!T.gexec || key != T.g_key) {
             if (T.gexec) { (void)hipGraphExecDestroy(T.gexec); T.gexec = nullptr; }
             hipGraph_t gr = nullptr;
             KR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            pair(false);
+            pair();
             KR_CHECK(hipStreamEndCapture(st, &gr));
             KR_CHECK(hipGraphInstantiate(&T.gexec, gr, nullptr, nullptr, 0));
             (void)hipGraphDestroy(gr);
             T.g_key = key;
         }
+    }
+    if (timing) {
+        const size_t n = 2 * (size_t)F0.c.tcap * G;
+        hipLaunchKernelGGL(k_ts_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, F0.c.ts, n);
     }
     FLAUNCH(k_f_init, 0);
     FLAUNCH(k_f_pm, 0, 1, 1, 0);
@@ -735,20 +797,25 @@ int bicgstab_stencil_batch(FusedTable &T, int nf, SolverT<float> *const *S, cons
             last_na = na;
         }
         for (int u = 0; u < CHUNK / 2; ++u) {
-            // the timed K1 is the chunk's third iteration's: it runs amid
-            // back-to-back iterations, not right after the host's read of
-            // the state (when the device has been idle for the round trip)
-            const bool timed = timing && u == 1;
-            if (graph && !timed) KR_CHECK(hipGraphLaunch(T.gexec, st));
-            else pair(timed);
+            if (graph) KR_CHECK(hipGraphLaunch(T.gexec, st));
+            else pair();
         }
         KR_CHECK(hipGetLastError());
         if (check()) return -1;
-        // counted only when the solve ran through the timed pair, so that
-        // the launch did its work (a K1 past the exit returns at once)
-        if (timing && CHUNK >= 4 && (int)h[S_IT] >= k + 4) {
-            float ms = 0;
-            if (hipEventElapsedTime(&ms, S0.kev[0], S0.kev[1]) == hipSuccess) { S0.kms += ms; S0.kcount++; }
+    }
+    if (timing && S0.kclock_khz > 0) {
+        // every K1 of iterations 1 .. it - 1 that did its work (the last
+        // slot, which iterations past it share, is left out)
+        const int n = std::min((int)h[S_IT] + 1, kTsCap - 1);
+        if (n > 0) {
+            unsigned long long *span = F0.c.ts + 2 * (size_t)kTsCap * G;   // after both exit tables
+            hipLaunchKernelGGL(k_ts_span, dim3(n), dim3(256), 0, st, (const unsigned long long *)F0.c.ts, F0.g,
+                               F0.c.tcap, span);
+            std::vector<unsigned long long> tv((size_t)n);
+            KR_CHECK(hipMemcpyAsync(tv.data(), span, tv.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            KR_CHECK(hipStreamSynchronize(st));
+            for (int k = 0; k < n; ++k)
+                if (tv[k]) { S0.kms += (double)tv[k] / S0.kclock_khz; ++S0.kcount; }
         }
     }
     for (int f = 0; f < nf; ++f) {

@@ -90,6 +90,10 @@ struct SolverT {
     hipEvent_t kev[2] = {nullptr, nullptr};
     long long kcount = 0;
     double kms = 0;
+    // the fused FP32 iteration's K1 timing slots (device) and the device
+    // clock's rate (kHz)
+    unsigned long long *kts = nullptr;
+    double kclock_khz = 0;
     // the stall test of the learned preconditioner: give up when the
     // residual has not fallen by stall_factor over stall_window iterations
     int stall_window = 0;

@@ -905,6 +905,7 @@ void solver_free(SolverT<T> &S) {
     k.ftiles = 0;
     fused_table_free(k.table);
     for (auto &e : S.kev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    if (S.kts) { (void)hipFree(S.kts); S.kts = nullptr; }
     if (k.host_sc && own) (void)hipHostFree(k.host_sc);
     k.host_sc = nullptr;
     k.N = 0;
