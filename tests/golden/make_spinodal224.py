@@ -1,4 +1,4 @@
-"""Writes tests/golden/config2_spinodal224.npz: the oracle's run of the 2-D
+"""Writes tests/golden/spinodal224_2000.npz: the oracle's run of the 2-D
 spinodal dewetting film at its full size (224^2 periodic cells, side 0.64,
 capillary + disjoining, adaptive dt from 1e-5, 2000 steps;
 yuzen_amd.cases.spinodal_2d).
@@ -10,9 +10,10 @@ tolerance of each pass); the interior film after steps 100, 500, 1000 and
 2000. The GPU test advances the device step over the same 2000 steps and
 checks it against these.
 
-    python tests/golden/make_config2.py [--steps 2000]
+    python tests/golden/make_spinodal224.py [--steps 2000]
 
-The oracle is single-threaded; the run takes a while on one core.
+The oracle runs its row loops over OpenMP threads (OMP_NUM_THREADS); the
+result does not depend on their number. The run takes about an hour.
 """
 import argparse
 import os
@@ -25,7 +26,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-OUT = os.path.join(HERE, "config2_spinodal224.npz")
+OUT = os.path.join(HERE, "spinodal224_2000.npz")
 KEEP = (100, 500, 1000, 2000)
 MAXP = 32
 
