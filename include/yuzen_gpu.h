@@ -449,6 +449,16 @@ int yz_set_tile_rows(int nx, int ny, int rows);
  * registers. Both give the same rows to the bit. Returns the previous
  * setting; values other than 0 and 1 only read it. */
 int yz_set_row_kernel(int tiled);
+/* Fault injection for tests of the failure paths, the counterpart of the
+ * reference's YUZEN_FAULT_AMGCL=k:m (LinearSolverBackend.cpp:628-645): the
+ * film's Newton-pass linear solves number k .. k+m-1 from this call on
+ * (process-wide, every context) break down, returning 2 before they
+ * iterate, so the step takes its failure path (retry from h^n after an
+ * extrapolated guess, a cut step under step control, yz_step_advance
+ * returning 1 otherwise). k = 0 turns it off. Without a call, the
+ * environment variable YZ_FAULT_SOLVE=k:m is read at the first solve.
+ * Returns 0. */
+int yz_set_fault(long k, long m);
 
 /* ---- linear-solve seam -------------------------------------------------
  * Solve A x = b, A in CRS (rowptr[n+1], col[nnz], val[nnz]; any column

@@ -30,6 +30,12 @@ def load():
     return _lib
 
 
+def set_fault(k, m=1):
+    """the oracle's fault injection: the film's Newton-pass solves k ..
+    k + m - 1 from now break down before they iterate (k = 0: off)"""
+    load().yzo_set_fault(int(k), int(m))
+
+
 def threads(n=0):
     """the OpenMP threads of the oracle's row loops: n > 0 sets them;
     returns the number in force"""
@@ -47,6 +53,7 @@ def load_from(path):
     lib.yzo_destroy.argtypes = [C.c_void_p]
     lib.yzo_n.argtypes = [C.c_void_p]
     lib.yzo_threads.argtypes = [C.c_int]
+    lib.yzo_set_fault.argtypes = [C.c_long, C.c_long]
     lib.yzo_nnz.argtypes = [C.c_void_p]
     lib.yzo_advance.argtypes = [C.c_void_p, pi_, pi_, pd]
     lib.yzo_get_h.argtypes = [C.c_void_p, pd]

@@ -1619,6 +1619,15 @@ struct Jacobi {
     }
 };
 
+/* Fault injection for the tests of the failure paths, as the reference's
+ * YUZEN_FAULT_AMGCL=k:m (LinearSolverBackend.cpp:628-645): the film's
+ * Newton-pass solves number k .. k+m-1, counted from the setting, break down
+ * (return 2) before they iterate. Process-wide; k = 0 is off. */
+static long g_fault_at = 0, g_fault_count = 0, g_fault_seen = 0;
+static bool fault_now() {
+    return g_fault_at > 0 && ++g_fault_seen >= g_fault_at && g_fault_seen < g_fault_at + g_fault_count;
+}
+
 /* one solve through the backend contract (LinearSolverBackend.cpp:539-728) */
 static int linsolve(const Csr &M, const vector<double> &rhs, vector<double> &x,
                     int precond, int scaling, int maxit, double eps, int &iter, double &resid) {
@@ -2330,6 +2339,12 @@ struct Stepper {
          * zero (zero_guess 1; the oracle is the FP64 path, so -1 is the
          * reference's) */
         if (eq.cs.zero_guess == 1) std::fill(corr.begin(), corr.end(), 0.0);
+        if (fault_now()) {
+            /* a breakdown made to happen before the solve iterates */
+            iter = 0; resid = 0;
+            std::fill(corr.begin(), corr.end(), 0.0);
+            return 2;
+        }
         const int ch = linsolve(eq.IA, Rhs, corr, eq.cs.precond == YZ_PC_JACOBI ? YZ_PC_JACOBI : YZ_PC_AMG,
                                 eq.cs.row_scaling, eq.cs.lin_maxit, eps, iter, resid);
         if (ch == 0) satisfy_bc(corr);
@@ -2585,6 +2600,12 @@ static thread_local std::string g_err;
 /* C API of the oracle (ctypes)                                         */
 /* ------------------------------------------------------------------ */
 extern "C" {
+
+/* the fault injection of the film's Newton-pass solves (fault_now):
+ * solves k .. k + m - 1 from now break down; k = 0 turns it off */
+void yzo_set_fault(long k, long m) {
+    yzo::g_fault_at = k; yzo::g_fault_count = m > 0 ? m : 1; yzo::g_fault_seen = 0;
+}
 
 /* the threads of the row loops: n > 0 sets them; returns the number in
  * force (1 when built without OpenMP) */

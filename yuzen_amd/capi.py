@@ -259,6 +259,7 @@ def load():
     lib.yz_tile_rows.argtypes = [C.c_int, C.c_int]
     lib.yz_set_tile_rows.argtypes = [C.c_int, C.c_int, C.c_int]
     lib.yz_set_row_kernel.argtypes = [C.c_int]
+    lib.yz_set_fault.argtypes = [C.c_long, C.c_long]
     lib.yz_batch_destroy.argtypes = [C.c_void_p]
     lib.yz_cz_case_defaults.argtypes = [P(YzCzCase)]
     lib.yz_cz_create.restype = C.c_void_p
@@ -319,6 +320,12 @@ def tune_tiles(nx, ny, nfilms=1, iters=40):
 
 def tile_rows(nx, ny):
     return load().yz_tile_rows(nx, ny)
+
+
+def set_fault(k, m=1):
+    """fault injection: the film's Newton-pass solves k .. k + m - 1 from now
+    break down before they iterate (k = 0: off)"""
+    load().yz_set_fault(int(k), int(m))
 
 
 def set_row_kernel(tiled):

@@ -535,6 +535,26 @@ static int fp64_fallback(yz_step *S, double eps, int *iter, double *resid, int *
 // floored at kEpsFP32; the Newton residual, the assembly and the update stay
 // FP64. When the FP32 solve fails or its stall test trips, the pass is
 // assembled again in FP64 and solved with the FP64 AMG-BiCGSTAB.
+// Fault injection for the tests of the failure paths, as the reference's
+// YUZEN_FAULT_AMGCL=k:m (LinearSolverBackend.cpp:628-645): the film's
+// Newton-pass solves number k .. k+m-1, counted from the setting
+// (yz_set_fault, or YZ_FAULT_SOLVE=k:m read at the first solve), break down
+// (return 2) before they iterate. Process-wide; k = 0 is off.
+static long g_fault_at = 0, g_fault_count = 0, g_fault_seen = 0;
+static bool g_fault_read = false;
+static bool fault_now() {
+    if (!g_fault_read) {
+        g_fault_read = true;
+        if (const char *e = std::getenv("YZ_FAULT_SOLVE")) {
+            char *end = nullptr;
+            g_fault_at = std::strtol(e, &end, 10);
+            g_fault_count = (end && *end == ':') ? std::strtol(end + 1, nullptr, 10) : 1;
+            g_fault_seen = 0;
+        }
+    }
+    return g_fault_at > 0 && ++g_fault_seen >= g_fault_at && g_fault_seen < g_fault_at + g_fault_count;
+}
+
 static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     *dev_err = 0;
     const yz_case &c = S->cs;
@@ -561,6 +581,12 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     S->tms[0] += ev_ms(S->ev[0], S->ev[1]);
     const double eps = ew_eps(S);
     S->last_eps = eps;
+    if (fault_now()) {
+        // a breakdown made to happen before the solve iterates
+        *iter = 0; *resid = 0;
+        YZ_TRY(hipMemsetAsync(S->corr, 0, S->g.N * sizeof(double), S->st));
+        return 2;
+    }
     YZ_TRY(hipEventRecord(S->ev[2], S->st));
     int ch = -1;
     if (f32) {
@@ -1312,7 +1338,12 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
             } else {
                 S->dt_now *= c.tsc_change;
                 dt_request = S->dt_now;
-                if (S->dt_now < c.dt_min) { set_err("step fell below MinDt"); return 1; }
+                if (S->dt_now < c.dt_min) {
+                    set_err("step fell below MinDt");
+                    if (nk_iter) *nk_iter = it;
+                    if (lin_iter) *lin_iter = (int)lin_sum;
+                    return 1;
+                }
             }
         } else {
             if (!ok) {
@@ -1463,6 +1494,12 @@ int yz_step_energy(yz_step *S, double *F, double *D) {
 // WellMixedModelBase.h:1339-1360): the film's volume and solute, and what
 // the last step evaporated (EvaporationRate, HeightEquationBase.h:359-369,
 // times its dt) and what the nozzle supplied (SuppliedVolume :293-303)
+int yz_set_fault(long k, long m) {
+    g_fault_read = true;
+    g_fault_at = k; g_fault_count = m > 0 ? m : 1; g_fault_seen = 0;
+    return 0;
+}
+
 int yz_set_row_kernel(int tiled) {
     const int was = g_rows_tiled;
     if (tiled == 0 || tiled == 1) g_rows_tiled = tiled;
