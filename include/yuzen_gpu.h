@@ -152,11 +152,17 @@ typedef struct yz_case {
     double src_arm_length, src_arm_x, src_arm_y;   /* ArmLength, ArmCenter */
     double src_omega, src_angle0, src_ramp, src_plateau;   /* RotationRate, StartAngle, RampTime, PlateauTime */
     int src_subregions;             /* SubRegions */
+    /* Numerics.Convection.Height (HeightEquationBase.h:609-616): the limited
+     * face interpolation of the upwinded terms (in-plane gravity, the
+     * centrifugal force, Marangoni), YZ_HSCHEME_KOREN (the default) or
+     * YZ_HSCHEME_VANLEER (FiniteDifferences.cpp:97-143) */
+    int height_scheme;
 } yz_case;
 
 /* Face interpolation of the concentration's advection (reference
  * FiniteDifferences.cpp:22-93, keys :214-228); the unlimited schemes.
  * UTOPIA is the reference's default (Input.cpp:2302-2303). */
+enum { YZ_HSCHEME_KOREN = 0, YZ_HSCHEME_VANLEER = 1 };
 enum { YZ_CONV_UPWIND1 = 0, YZ_CONV_CENTRAL2 = 1, YZ_CONV_CENTRAL4 = 2, YZ_CONV_UPWIND2 = 3,
        YZ_CONV_UTOPIA = 4, YZ_CONV_QUICK = 5, YZ_CONV_KK = 6 };
 

@@ -68,7 +68,7 @@ def load_from(path):
     lib.yzo_velocity.argtypes = [C.c_void_p, pd, pd, C.c_int, pd, pd]
     lib.yzo_energy.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_source.argtypes = [C.c_void_p, C.c_double, pd]
-    lib.yzo_limiter.argtypes = [pd, C.c_double, pd]
+    lib.yzo_limiter.argtypes = [pd, C.c_double, C.c_int, pd]
     lib.yzo_get_c.argtypes = [C.c_void_p, pd]
     lib.yzo_face_fluxes.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_conc_stats.argtypes = [C.c_void_p, pd]
@@ -103,10 +103,12 @@ def mobility(lam, h1, h2):
     return out
 
 
-def limiter(f, sign):
+def limiter(f, sign, vanleer=0):
+    """the limited face of four heights: [value, downwind weights w0..w3,
+    value again, derivative g0..g3]; Koren, or van Leer (vanleer 1)"""
     f = np.ascontiguousarray(f, dtype=np.float64)
     out = np.zeros(10)
-    load().yzo_limiter(_pd(f), sign, _pd(out))
+    load().yzo_limiter(_pd(f), sign, int(vanleer), _pd(out))
     return out
 
 

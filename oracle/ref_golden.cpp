@@ -18,7 +18,8 @@
 //     face velocities (CalVelocity), and the free energy and potential of
 //     EnergyBudget.h with the term's parameters;
 //   * Mobility::Face/FaceDK and derivatives over a grid of heights and slip
-//     lengths; the Koren limiter's face value, weights and derivative;
+//     lengths; the Koren and van Leer limiters' face value, weights and
+//     derivative; the upwinded terms with either limiter;
 //     NonDimensionalNumbers::Dp over contact angles; FaceValue and
 //     FaceWeights of the unlimited face schemes;
 //   * a restart state file written by the reference's C_RestartState;
@@ -293,7 +294,7 @@ void build_term(K &k, Input &inp, typename K::Grid *G, NonDimensionalNumbers *nd
 }
 
 std::string run_case(const std::string &dir, bool twoD, Term t, double slip, bool periodic, std::ostream &man,
-                     bool &first) {
+                     bool &first, bool vanleer = false) {
     Params p;
     p.slip = slip;
     if (t == tDisjoining) p.hstar = 0.1;
@@ -305,7 +306,8 @@ std::string run_case(const std::string &dir, bool twoD, Term t, double slip, boo
     Shapes sh{periodic};
     const auto bk = periodic ? BoundaryCondition::Periodic : BoundaryCondition::Symmetry;
     std::ostringstream name;
-    name << (twoD ? "2d" : "1d") << "_" << kName[t] << "_slip" << (slip > 0 ? 1 : 0) << (periodic ? "_per" : "_sym");
+    name << (twoD ? "2d" : "1d") << "_" << kName[t] << "_slip" << (slip > 0 ? 1 : 0) << (periodic ? "_per" : "_sym")
+         << (vanleer ? "_vl" : "");
     const std::string path = dir + "/" + name.str() + ".bin";
     const bool cons = (t == tLaplace || t == tVertical);
     g_pp = PotentialParams();
@@ -317,7 +319,8 @@ std::string run_case(const std::string &dir, bool twoD, Term t, double slip, boo
     g_pp.bo = (t == tVertical) ? nd.Bo() : 0.0;
     g_pp.nd = &nd;
     g_periodic = periodic;
-    const C_Interpolation_base4::t_Scheme koren = C_Interpolation_base4::Koren;
+    // the upwinded terms' face interpolation (Numerics.Convection.Height)
+    const C_Interpolation_base4::t_Scheme koren = vanleer ? C_Interpolation_base4::VanLeer : C_Interpolation_base4::Koren;
     if (twoD) {
         Field_WM2D field;
         field.Build(g, Flat(), nd.ExtentScale());
@@ -351,10 +354,10 @@ std::string run_case(const std::string &dir, bool twoD, Term t, double slip, boo
                   "\"xmin\": -0.5, \"xmax\": 0.5, \"ymin\": %.17g, \"ymax\": %.17g, "
                   "\"As\": %.17g, \"Ma\": %.17g, \"Bo\": %.17g, \"Pgx\": %.17g, \"Pgy\": %.17g, \"Sp\": %.17g, "
                   "\"Dp\": %.17g, \"hstar\": %.17g, \"slip\": %.17g, \"rot_x\": %.17g, \"rot_y\": %.17g, "
-                  "\"visc_c0\": 0.5, \"visc_exp\": %.17g, \"sig_c0\": 0.5}",
+                  "\"visc_c0\": 0.5, \"visc_exp\": %.17g, \"sig_c0\": 0.5, \"height_scheme\": %d}",
                   first ? "" : ",", name.str().c_str(), twoD ? 2 : 1, nx, twoD ? ny : 1, kName[t], periodic ? 1 : 0,
                   twoD ? -0.4 : -1.0, twoD ? 0.4 : 1.0, nd.As(), nd.Ma(), nd.Bo(), nd.Pgx(), nd.Pgy(), nd.Sp(),
-                  nd.Dp(), nd.HStar(), slip, p.rotx, p.roty, p.visc_exp);
+                  nd.Dp(), nd.HStar(), slip, p.rotx, p.roty, p.visc_exp, vanleer ? 1 : 0);
     man << buf;
     first = false;
     return path;
@@ -384,11 +387,12 @@ void dump_mobility(const std::string &dir) {
     write_vec(f, out);
 }
 
-void dump_limiter(const std::string &dir) {
-    // Koren's limited face value, downwind weights and derivative
-    // (FiniteDifferences.cpp) over four-point stencils through every branch
+void dump_limiter(const std::string &dir, bool vanleer) {
+    // Koren's (or van Leer's) limited face value, downwind weights and
+    // derivative (FiniteDifferences.cpp) over four-point stencils through
+    // every branch
     C_Interpolation_base4 k;
-    k.Build(C_Interpolation_base4::Koren);
+    k.Build(vanleer ? C_Interpolation_base4::VanLeer : C_Interpolation_base4::Koren);
     std::vector<double> out;
     unsigned long long z = 12345;
     auto rnd = [&]() {
@@ -411,7 +415,7 @@ void dump_limiter(const std::string &dir) {
         out.push_back(vd);
         for (int j = 0; j < 4; ++j) out.push_back(g[j]);
     }
-    std::ofstream fo(dir + "/limiter.bin", std::ios::binary);
+    std::ofstream fo(dir + (vanleer ? "/limiter_vl.bin" : "/limiter.bin"), std::ios::binary);
     const int hdr[2] = {(int)out.size() / 15, 15};
     fo.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
     write_vec(fo, out);
@@ -729,9 +733,15 @@ int main(int argc, char **argv) {
                 run_case(dir, d2 == 1, t, slip, false, man, first);
     for (int d2 = 0; d2 < 2; ++d2)
         for (Term t : all) run_case(dir, d2 == 1, t, 0.05, true, man, first);
+    // the upwinded terms with van Leer's limiter
+    for (int d2 = 0; d2 < 2; ++d2)
+        for (Term t : {tParallel, tCentrifugal, tSoluto})
+            for (double slip : {0.0, 0.05})
+                run_case(dir, d2 == 1, t, slip, false, man, first, true);
     man << "\n]}\n";
     dump_mobility(dir);
-    dump_limiter(dir);
+    dump_limiter(dir, false);
+    dump_limiter(dir, true);
     dump_dp(dir);
     dump_schemes(dir);
     dump_state(dir);

@@ -414,7 +414,7 @@ struct Equation {
         g.build(cs);
         st.build(cs.dim);
         mob.set(cs.slip);
-        lim.vanleer = 0;
+        lim.vanleer = cs.height_scheme == YZ_HSCHEME_VANLEER ? 1 : 0;
         build_pattern(A); Arest = A; J = A; IA = A;
         /* capillary C(d): LaplacePressure_2D.cpp:39-94 / _1D.cpp:35-77 */
         for (int d = 0; d < st.ndir; ++d) Ccap[d].assign(st.ns, 0.0);
@@ -2854,8 +2854,9 @@ void yzo_mobility(double lambda, double h1, double h2, double *out) {
     out[3] = m.D(h1); out[4] = m.dD(h1); out[5] = m.DperH(h1);
 }
 /* Koren limiter: out = value, weights[4], derivative value, g[4] */
-void yzo_limiter(const double *f, double sign, double *out) {
+void yzo_limiter(const double *f, double sign, int vanleer, double *out) {
     yzo::Limiter k;
+    k.vanleer = vanleer;
     out[0] = k.value(f, sign);
     k.weights(f, sign, out + 1);
     out[5] = k.derivative(f, sign, out + 6);

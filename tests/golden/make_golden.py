@@ -1,7 +1,7 @@
 """Writes the golden fixtures of tests/golden from the reference's own code.
 
 Runs oracle/_ref/ref_golden (built by oracle/build_ref.sh from the reference
-sources: its term kernels, Mobility, the Koren limiter and
+sources: its term kernels, Mobility, the Koren and van Leer limiters and
 NonDimensionalNumbers, driven the way the reference's test_jacobian.cpp
 drives them) into a scratch directory and packs what it wrote:
 
@@ -78,6 +78,7 @@ def main():
         np.savez_compressed(os.path.join(HERE, "ref_scalars.npz"),
                             mobility=read_table(os.path.join(tmp, "mobility.bin")),
                             limiter=read_table(os.path.join(tmp, "limiter.bin")),
+                            limiter_vl=read_table(os.path.join(tmp, "limiter_vl.bin")),
                             dp=read_table(os.path.join(tmp, "dp.bin")),
                             schemes=read_table(os.path.join(tmp, "schemes.bin")),
                             evap=read_table(os.path.join(tmp, "evap.bin")),

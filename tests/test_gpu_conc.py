@@ -45,6 +45,7 @@ def _case_from(m):
     c.hstar, c.dis_n, c.dis_m = m["hstar"], 3.0, 2.0
     c.slip, c.rot_x, c.rot_y = m["slip"], m["rot_x"], m["rot_y"]
     c.visc_c0, c.visc_exp, c.sig_c0 = m["visc_c0"], m["visc_exp"], m["sig_c0"]
+    c.height_scheme = m.get("height_scheme", capi.HSCHEME_KOREN)
     return c
 
 

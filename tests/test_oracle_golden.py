@@ -1,7 +1,8 @@
 """The CPU oracle against golden vectors written by the reference's own code.
 
 tests/golden holds what the reference's term kernels (SetMatrixA,
-SetJacobiMatrix, AddConservative), Mobility, the Koren limiter and
+SetJacobiMatrix, AddConservative; the upwinded terms with the Koren and
+the van Leer limiter), Mobility, both limiters and
 NonDimensionalNumbers compute on the shapes of its test_jacobian.cpp
 harness (tests/golden/make_golden.py, built by oracle/build_ref.sh from the
 reference sources). The oracle must reproduce them to rounding: 1e-12 of
@@ -35,6 +36,7 @@ def case_from(m):
     c.hstar, c.dis_n, c.dis_m = m["hstar"], 3.0, 2.0
     c.slip, c.rot_x, c.rot_y = m["slip"], m["rot_x"], m["rot_y"]
     c.visc_c0, c.visc_exp, c.sig_c0 = m["visc_c0"], m["visc_exp"], m["sig_c0"]
+    c.height_scheme = m.get("height_scheme", capi.HSCHEME_KOREN)
     return c
 
 
@@ -127,11 +129,12 @@ def test_mobility_matches_reference():
     assert worst <= 1e-13
 
 
-def test_limiter_matches_reference():
-    t = SCAL["limiter"]
+@pytest.mark.parametrize("vanleer", [0, 1], ids=["koren", "vanleer"])
+def test_limiter_matches_reference(vanleer):
+    t = SCAL["limiter_vl" if vanleer else "limiter"]
     for row in t:
         f, sign = row[0:4], row[4]
-        mine = ob.limiter(f, sign)
+        mine = ob.limiter(f, sign, vanleer)
         ref = np.concatenate([[row[5]], row[6:10], [row[10]], row[11:15]])
         assert np.allclose(mine, ref, rtol=0, atol=1e-15), (row, mine)
 

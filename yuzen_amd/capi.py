@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libyuzen_hip.so")
 
 BC_SYMMETRY, BC_PERIODIC, BC_PINNING, BC_FRESHINLET, BC_OUTLET = range(5)
 PC_JACOBI, PC_AMG, PC_LEARNED, PC_LEARNED_AMG = range(4)
+HSCHEME_KOREN, HSCHEME_VANLEER = 0, 1
 (CONV_UPWIND1, CONV_CENTRAL2, CONV_CENTRAL4, CONV_UPWIND2, CONV_UTOPIA, CONV_QUICK,
  CONV_KK) = range(7)
 LEARN_NPARAM, LEARN_NFEAT, LEARN_NOUT = 6288, 8, 13
@@ -69,6 +70,7 @@ class YzCase(C.Structure):
         ("src_omega", C.c_double), ("src_angle0", C.c_double), ("src_ramp", C.c_double),
         ("src_plateau", C.c_double),
         ("src_subregions", C.c_int),
+        ("height_scheme", C.c_int),
     ]
 
     def copy(self):

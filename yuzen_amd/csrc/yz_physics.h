@@ -179,14 +179,29 @@ __device__ inline double koren(double a, double b, double *pa, double *pb) {
     *pa = qa; *pb = qb;
     return p;
 }
+// van Leer: 2ab / (a + b) where a and b share a strict sign, written with
+// the ratios a / (a + b) and b / (a + b), both in (0, 1), so that it
+// neither underflows nor overflows for tiny or huge differences
+// (FiniteDifferences.cpp:112-124); 0 otherwise. pa, pb: its derivatives.
+__device__ inline double vanleer(double a, double b, double *pa, double *pb) {
+    double p = 0, qa = 0, qb = 0;
+    if ((a > 0 && b > 0) || (a < 0 && b < 0)) {
+        const double sum = a + b, ra = a / sum, rb = b / sum;
+        p = 2.0 * a * rb;
+        qa = 2.0 * rb * rb;
+        qb = 2.0 * ra * ra;
+    }
+    *pa = qa; *pb = qb;
+    return p;
+}
 // The limited face value hf, its downwind-form weights w (sum w f = hf)
-// and its derivative g = d hf / d f.
-__device__ inline double limited_face(const double f[4], double sign, double w[4], double g[4]) {
+// and its derivative g = d hf / d f; Koren, or van Leer (vl).
+__device__ inline double limited_face(const double f[4], double sign, double w[4], double g[4], int vl = 0) {
     int uu, u, d;
     if (sign >= 0) { uu = 0; u = 1; d = 2; } else { uu = 3; u = 2; d = 1; }
     const double a = f[u] - f[uu], b = f[d] - f[u];
     double pa, pb;
-    const double p = koren(a, b, &pa, &pb);
+    const double p = vl ? vanleer(a, b, &pa, &pb) : koren(a, b, &pa, &pb);
 #pragma unroll
     for (int j = 0; j < 4; ++j) { w[j] = 0; g[j] = 0; }
     const double psi = (b != 0) ? p / b : 0.0;

@@ -303,6 +303,9 @@ static int check_case(const yz_case &c) {
     if (c.stall_window < 0) { set_err("stall_window must be >= 0"); return 1; }
     if (c.zero_guess < -1 || c.zero_guess > 1) { set_err("zero_guess must be -1, 0 or 1"); return 1; }
     if (c.conc_model != 0 && c.conc_model != 1) { set_err("conc_model must be 0 (frozen) or 1 (well-mixed)"); return 1; }
+    if (c.height_scheme != YZ_HSCHEME_KOREN && c.height_scheme != YZ_HSCHEME_VANLEER) {
+        set_err("height_scheme: the limited schemes Koren and van Leer are implemented for the film height"); return 1;
+    }
     if (c.conc_model == 1) {
         if (c.conc_scheme < YZ_CONV_UPWIND1 || c.conc_scheme > YZ_CONV_KK) {
             set_err("conc_scheme: only the unlimited face schemes are implemented for the concentration"); return 1;
@@ -1237,6 +1240,7 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
     build_ccap(g, a.Ccap);
     a.theta = c->theta;
     a.row_scaling = c->row_scaling;
+    a.vanleer = c->height_scheme == YZ_HSCHEME_VANLEER ? 1 : 0;
     a.s = S->s; a.c = S->c;
     a.ia = S->S.A; a.b = S->b; a.rmax = S->dmax;
     a.ax = S->axold;

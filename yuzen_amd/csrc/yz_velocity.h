@@ -105,7 +105,7 @@ __device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) 
             double f[4], wt[4], gd[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) f[k] = q.H[face_slot(DIM, d, k)];
-            const double hf = limited_face(f, sgn(A.AsMa2) * sgn(dc), wt, gd);
+            const double hf = limited_face(f, sgn(A.AsMa2) * sgn(dc), wt, gd, A.vanleer);
             const double mu = visc_of(A, cc);
             S = mob.slip ? dc / mu * (hf + 2.0 * mob.lambda) : dc / mu * hf;
         }
@@ -132,7 +132,7 @@ __device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) 
         double f[4], wt[4], gd[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) f[k] = q.H[face_slot(DIM, d, k)];
-        const double hf = limited_face(f, dv, wt, gd);
+        const double hf = limited_face(f, dv, wt, gd, A.vanleer);
         double D = (mob.slip ? mob.DperH(hf) : hf * hf) / visc_of(A, cface);
         if (vel_wall<DIM>(g, q, a)) D = 0;
         const double Q = -w * Mag * D;
