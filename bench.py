@@ -237,7 +237,7 @@ def concz_leg(steps=10, warmup=2, cpu=True):
     capillary and Marangoni flow with the coupling passes, evaporation,
     fixed dt 0.01): FP64 steps/s on the device, and the CPU oracle's rate
     on one step of the same film at 64^2 x 32 (cells per second)."""
-    from yuzen_amd import cases
+    from yuzen_amd import capi, cases
     from yuzen_amd.step import HipCZ
     import torch
     c, h, s, cc, dt, _ = cases.concz_2d()
@@ -264,6 +264,32 @@ def concz_leg(steps=10, warmup=2, cpu=True):
         "film_solves_per_step": round(float(d[0]), 2), "film_iters_per_step": round(float(d[1]), 1),
         "conc_solves_per_step": round(float(d[2]), 2), "conc_iters_per_step": round(float(d[3]), 1),
         "coupling_passes_per_step": round(float(d[4]), 2),
+    }
+    # the same film with the learned preconditioner in its film solves
+    # (FP32 inside FP64 refinement, FP64 AMG finishing a stalled solve)
+    cl = c.copy()
+    cl.film_precond = capi.PC_LEARNED
+    g = HipCZ(cl, h, s, cc, dt)
+    for _ in range(warmup):
+        if g.advance()[0] != 0:
+            raise RuntimeError("ConcZ warm-up step failed (learned film solve)")
+    s0, l0 = g.stats(), g.learn_stats()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        if g.advance()[0] != 0:
+            raise RuntimeError("ConcZ step failed (learned film solve)")
+    torch.cuda.synchronize()
+    el2 = time.perf_counter() - t
+    d2, ld = (g.stats() - s0) / steps, (g.learn_stats() - l0) / steps
+    g.close()
+    out["learned_film_solve"] = {
+        "steps_per_s": round(steps / el2, 3), "ms_per_step": round(1000 * el2 / steps, 3),
+        "film_iters_per_step": round(float(d2[1]), 1), "refinement_rounds_per_step": round(float(ld[0]), 2),
+        "fp32_iters_per_step": round(float(ld[1]), 1), "amg_finished_per_step": round(float(ld[2]), 2),
+        "note": "FP32 BiCGSTAB with the learned approximate inverse (trained in place on the first 3 film "
+                "systems) inside FP64 refinement to h_eps; FP64 AMG finishes a solve whose round fails to halve "
+                "the residual",
     }
     if cpu:
         from oracle import binding as ob

@@ -549,6 +549,17 @@ typedef struct yz_cz_case {
     int conc_scheme;                /* YZ_CONV_*: Numerics.Convection.Concentration (UTOPIA) */
     int lin_maxit;                  /* LinearSolver.MaxIterations of both (5000) */
     double h_eps, c_eps;            /* LinearSolver.Epsilon of the film and the concentration (1e-13) */
+    /* The film's linear solve. YZ_PC_AMG (the default): FP64 BiCGSTAB with
+     * the structured SA-AMG, as the reference's backend solves it.
+     * YZ_PC_LEARNED (no reference counterpart): FP32 BiCGSTAB right-
+     * preconditioned by the learned 13-point approximate inverse (trained
+     * in place on the first learn_passes film systems, learn_iters Adam
+     * steps each, then frozen), inside FP64 iterative refinement to the same
+     * h_eps on the FP64 residual; when a round fails to halve the residual,
+     * FP64 AMG finishes the solve from the refined film. */
+    int film_precond;
+    int learn_passes, learn_iters, learn_probes;
+    double learn_lr;
 } yz_cz_case;
 
 void yz_cz_case_defaults(yz_cz_case *c);
@@ -579,6 +590,10 @@ void yz_cz_volume_fluxes(yz_cz *st, double *qx, double *qy, double *w);
  * again, [5] solute moved by RedistributeExcess, [6] moved and [7] made up
  * by RedistributeDeficit (c times cells, summed) */
 void yz_cz_stats(yz_cz *st, double *out8);
+/* the learned film solve since create: [0] refinement rounds, [1] FP32
+ * iterations, [2] solves that FP64 AMG finished, [3] milliseconds of
+ * training */
+void yz_cz_learn_stats(yz_cz *st, double *out4);
 /* the restart state in the reference's field*.state layout and keys
  * (ConcZ_Model_2D::WriteState, ConcZ_Model_2D.cpp:572-600: the clock,
  * height.old, conc.previous and conc.previousDt, solver.height,

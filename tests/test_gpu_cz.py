@@ -150,3 +150,27 @@ def test_concz256_size_solute_volume_and_uniformity():
         assert g.advance()[0] == 0
     assert np.abs(_real3(c, g.c()) - 0.37).max() <= 1e-14
     g.close()
+
+
+def test_learned_film_solve_matches_oracle():
+    """The film's solve with the learned preconditioner (film_precond
+    YZ_PC_LEARNED: FP32 BiCGSTAB with the network's approximate inverse
+    inside FP64 iterative refinement to the same h_eps, FP64 AMG finishing
+    a solve whose round fails to halve the residual) reaches the film the
+    oracle's FP64 AMG reaches, with the same coupling passes."""
+    c, h, s, cc = _allphys(per=True, n=40, nz=8)
+    c.film_precond = capi.PC_LEARNED
+    c.learn_passes, c.learn_iters = 2, 60
+    _, g = _pair(c, h, s, cc, 0.02, 4, "learned film solve 40^2 x 8", bar=1e-9)
+    ls = g.learn_stats()
+    print(f"learned film solve: rounds {ls[0]:.0f}, FP32 iterations {ls[1]:.0f}, AMG finished {ls[2]:.0f}, "
+          f"training {ls[3]:.1f} ms")
+    assert ls[0] > 0 and ls[1] > 0
+    g.close()
+
+
+def test_film_precond_refused():
+    c, h, s, cc = _allphys(per=True)
+    c.film_precond = capi.PC_JACOBI
+    with pytest.raises(Exception):
+        HipCZ(c, h, s, cc, 0.02)

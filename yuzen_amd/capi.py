@@ -167,6 +167,9 @@ class YzCzCase(C.Structure):
         ("conc_scheme", C.c_int),
         ("lin_maxit", C.c_int),
         ("h_eps", C.c_double), ("c_eps", C.c_double),
+        ("film_precond", C.c_int),
+        ("learn_passes", C.c_int), ("learn_iters", C.c_int), ("learn_probes", C.c_int),
+        ("learn_lr", C.c_double),
     ]
 
     def copy(self):
@@ -189,6 +192,8 @@ def cz_case_defaults():
     c.molar_resin, c.molar_solvent = 2200.0, 106.0
     c.conc_scheme = CONV_UTOPIA
     c.lin_maxit, c.h_eps, c.c_eps = 5000, 1e-13, 1e-13
+    c.film_precond = PC_AMG
+    c.learn_passes, c.learn_iters, c.learn_probes, c.learn_lr = 3, 150, 4, 2e-3
     return c
 
 
@@ -271,6 +276,7 @@ def load():
     lib.yz_cz_get_c.argtypes = [C.c_void_p, pd]
     lib.yz_cz_volume_fluxes.argtypes = [C.c_void_p, pd, pd, pd]
     lib.yz_cz_stats.argtypes = [C.c_void_p, pd]
+    lib.yz_cz_learn_stats.argtypes = [C.c_void_p, pd]
     lib.yz_cz_destroy.argtypes = [C.c_void_p]
     lib.yz_cz_save_state.argtypes = [C.c_void_p, C.c_char_p]
     lib.yz_cz_load_state.argtypes = [C.c_void_p, C.c_char_p]

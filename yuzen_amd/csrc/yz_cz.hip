@@ -39,11 +39,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/yuzen_gpu.h"
+#include "yz_mlp.h"
 #include "yz_solver.h"
 #include "yz_state.h"
 
@@ -1367,6 +1369,54 @@ __global__ void k_colsum(int n, const double *v, double *part) {
     block_partials<1>(acc, part);
 }
 
+
+// ---- the learned film solve ----------------------------------------------
+
+// The film's FP32 system and the learned preconditioner's inputs from the
+// FP64 row-scaled planes P (13 x N) and the film h at the solve's start:
+// the FP32 planes (a diagonal the scaling made 1 within an ulp stored as
+// exactly 1, else *nonunit set), and per row the features the network takes
+// (the stiffness and shape of the scaled row, the film) and the row scale
+// 1 / (1 + sum of |off-diagonal|), as the well-mixed film's row kernel forms
+// them
+__global__ void k_cz_feat(int N, const double *__restrict__ P, const double *__restrict__ h, float *__restrict__ A32,
+                          float *__restrict__ feat, float *__restrict__ mscale, int *__restrict__ nonunit) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double IA[13];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) IA[k] = P[(size_t)k * N + i];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) A32[(size_t)k * N + i] = (float)IA[k];
+    if (fabs(IA[6] - 1.0) <= 1.0e-12) A32[(size_t)6 * N + i] = 1.0f;
+    else atomicOr(nonunit, 1);
+    double off = 0;
+#pragma unroll
+    for (int k = 0; k < 13; ++k) if (k != 6) off += fabs(IA[k]);
+    // the far points of the four faces: Mo, Po, oM, oP
+    const double far = IA[4] + IA[8] + IA[0] + IA[12];
+    const double so = off > 1e-30 ? off : 1e-30;
+    feat[0 * (size_t)N + i] = (float)(0.25 * log10(so));
+    feat[1 * (size_t)N + i] = (float)(IA[5] / so);
+    feat[2 * (size_t)N + i] = (float)(IA[7] / so);
+    feat[3 * (size_t)N + i] = (float)(IA[2] / so);
+    feat[4 * (size_t)N + i] = (float)(IA[10] / so);
+    feat[5 * (size_t)N + i] = (float)(far / so);
+    feat[6 * (size_t)N + i] = (float)h[i];
+    feat[7 * (size_t)N + i] = 0.0f;
+    mscale[i] = (float)(1.0 / (1.0 + off));
+}
+// b32 = r / s (the residual at a scale FP32 holds well)
+__global__ void k_cz_to32(int N, const double *__restrict__ r, double inv_s, float *__restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) b[i] = (float)(r[i] * inv_s);
+}
+// x += s d
+__global__ void k_cz_refine(int N, double *__restrict__ x, double s, const float *__restrict__ d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) x[i] += s * (double)d[i];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------
@@ -1399,6 +1449,15 @@ struct yz_cz {
     double time = 0;
     long long steps = 0;
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // the learned film solve (film_precond YZ_PC_LEARNED): the FP32 system,
+    // the network's inputs and rows, the FP64 residual and the guess kept
+    SolverT<float> S32;
+    float *b32 = nullptr, *x32 = nullptr, *feat = nullptr, *mscale = nullptr;
+    __half *Yh = nullptr;
+    double *r64 = nullptr, *xkeep = nullptr;
+    int *nonunit = nullptr, *hnonunit = nullptr;
+    Mlp mlp;
+    double lstats[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -1452,6 +1511,97 @@ int apply_rows(yz_cz *z, double *x) {
     return 0;
 }
 
+// The film's solve with the learned preconditioner: FP32 BiCGSTAB with the
+// network's approximate inverse on the scaled FP64 residual, the correction
+// added in FP64, round after round until the FP64 residual meets h_eps;
+// FP64 AMG finishes from the refined film when a round fails to halve the
+// residual. Returns the solve's code (-1 on a device error).
+constexpr int kRefineRounds = 10;
+constexpr double kInnerTol = 1e-5;
+constexpr int kInnerMaxit = 600;
+int h_solve_learned(yz_cz *z, int *iters, double *resid) {
+    Args &a = z->a;
+    const int N = z->Sh.op0.N;
+    const yz_cz_case &c = z->cs;
+    const dim3 gb(nblk(N)), tb(TBK);
+    CZ_CHECK(hipMemsetAsync(z->nonunit, 0, sizeof(int), z->st));
+    hipLaunchKernelGGL(k_cz_feat, gb, tb, 0, z->st, N, (const double *)a.hP, (const double *)a.hx, z->S32.A, z->feat,
+                       z->mscale, z->nonunit);
+    CZ_CHECK(hipMemcpyAsync(z->hnonunit, z->nonunit, sizeof(int), hipMemcpyDeviceToHost, z->st));
+    if (sync(z)) return -1;
+    z->S32.unit_diag = z->hnonunit[0] == 0;
+    if (z->mlp.trained_passes < c.learn_passes && c.learn_iters > 0) {
+        hipEvent_t e0, e1;
+        CZ_CHECK(hipEventCreate(&e0));
+        CZ_CHECK(hipEventCreate(&e1));
+        CZ_CHECK(hipEventRecord(e0, z->st));
+        double l0 = 0, l1 = 0;
+        const int rc = mlp_train(z->mlp, z->S32, z->feat, z->mscale, c.learn_iters, c.learn_lr, c.learn_probes,
+                                 0x2345u + 104729u * (unsigned)z->mlp.trained_passes, &l0, &l1, z->st);
+        CZ_CHECK(hipEventRecord(e1, z->st));
+        CZ_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) { yz::set_error("ConcZ: training of the learned preconditioner failed"); return -1; }
+        z->lstats[3] += ms;
+        ++z->mlp.trained_passes;
+    }
+    if (mlp_infer_half(z->mlp, N, z->feat, z->Yh, z->st)) { yz::set_error("ConcZ: inference failed"); return -1; }
+    z->S32.Mh = z->Yh;
+    z->S32.Mscale = z->mscale;
+    double bb = 0, rr = 0;
+    if (residual_norms(z->Sh, (const double *)a.hx, (const double *)a.hb, z->r64, &bb, &rr)) return -1;
+    const double nb = std::sqrt(bb), eps = c.h_eps * nb;
+    double res = std::sqrt(rr);
+    if (nb == 0) {
+        CZ_CHECK(hipMemsetAsync(a.hx, 0, N * sizeof(double), z->st));
+        *resid = 0;
+        return 0;
+    }
+    for (int round = 0; round < kRefineRounds && res > eps; ++round) {
+        const double sc = res / std::sqrt((double)N);
+        hipLaunchKernelGGL(k_cz_to32, gb, tb, 0, z->st, N, (const double *)z->r64, 1.0 / sc, z->b32);
+        CZ_CHECK(hipMemsetAsync(z->x32, 0, N * sizeof(float), z->st));
+        int it = 0;
+        double r32 = 0;
+        const int ch = bicgstab_stencil(z->S32, (const float *)z->b32, z->x32, kInnerMaxit, kInnerTol, &it, &r32);
+        if (ch < 0) return -1;
+        *iters += it;
+        z->lstats[1] += it;
+        z->lstats[0] += 1;
+        if (std::getenv("YZ_DEBUG"))
+            std::fprintf(stderr, "[cz] learned round %d: |r| %.3e of %.3e, FP32 iterations %d (rc %d, %.2e)\n", round,
+                         res, nb, it, ch, r32);
+        if (ch == 2) break;                 // a breakdown: the correction is not kept
+        CZ_CHECK(hipMemcpyAsync(z->xkeep, a.hx, N * sizeof(double), hipMemcpyDeviceToDevice, z->st));
+        hipLaunchKernelGGL(k_cz_refine, gb, tb, 0, z->st, N, a.hx, sc, (const float *)z->x32);
+        if (residual_norms(z->Sh, (const double *)a.hx, (const double *)a.hb, z->r64, &bb, &rr)) return -1;
+        const double next = std::sqrt(rr);
+        if (!(next <= 0.5 * res)) {
+            // the round did not halve the residual: its correction is taken
+            // back, and the residual of the film before it restored
+            CZ_CHECK(hipMemcpyAsync(a.hx, z->xkeep, N * sizeof(double), hipMemcpyDeviceToDevice, z->st));
+            if (residual_norms(z->Sh, (const double *)a.hx, (const double *)a.hb, z->r64, &bb, &rr)) return -1;
+            res = std::sqrt(rr);
+            break;
+        }
+        res = next;
+    }
+    if (res <= eps) {
+        *resid = res / nb;
+        return 0;
+    }
+    // FP64 AMG from the refined film
+    z->lstats[2] += 1;
+    if (amg_setup(z->Sh)) { yz::set_error("ConcZ: film preconditioner set-up failed"); return -1; }
+    int it = 0;
+    const int ch = bicgstab(z->Sh, (const double *)a.hb, a.hx, YZ_PC_AMG, c.lin_maxit, c.h_eps, &it, resid);
+    *iters += it;
+    return ch;
+}
+
 // SolveStage of the film: assemble at the field, solve into hN
 int h_stage(yz_cz *z, int &iter, double &resid) {
     const Geo3 &g = z->g;
@@ -1459,10 +1609,16 @@ int h_stage(yz_cz *z, int &iter, double &resid) {
     hipLaunchKernelGGL(k_faces, dim3(nblk(g.n2)), dim3(TBK), 0, z->st, a);
     hipLaunchKernelGGL(k_hrows, dim3(nblk(g.nx * g.ny)), dim3(TBK), 0, z->st, a);
     if (sync(z)) return -1;
-    if (amg_setup(z->Sh)) { yz::set_error("ConcZ: film preconditioner set-up failed"); return -1; }
     int it = 0;
     double rs = 0;
-    const int ch = bicgstab(z->Sh, (const double *)a.hb, a.hx, YZ_PC_AMG, z->cs.lin_maxit, z->cs.h_eps, &it, &rs);
+    int ch;
+    if (z->cs.film_precond == YZ_PC_LEARNED) {
+        ch = h_solve_learned(z, &it, &rs);
+        if (ch < 0) return -1;
+    } else {
+        if (amg_setup(z->Sh)) { yz::set_error("ConcZ: film preconditioner set-up failed"); return -1; }
+        ch = bicgstab(z->Sh, (const double *)a.hb, a.hx, YZ_PC_AMG, z->cs.lin_maxit, z->cs.h_eps, &it, &rs);
+    }
     iter += it;
     resid = rs;
     z->stats[0] += 1;
@@ -1810,6 +1966,8 @@ void yz_cz_case_defaults(yz_cz_case *c) {
     c->molar_resin = 2200; c->molar_solvent = 106;
     c->conc_scheme = YZ_CONV_UTOPIA;
     c->lin_maxit = 5000; c->h_eps = 1e-13; c->c_eps = 1e-13;
+    c->film_precond = YZ_PC_AMG;
+    c->learn_passes = 3; c->learn_iters = 150; c->learn_probes = 4; c->learn_lr = 2e-3;
 }
 
 int yz_sizeof_cz_case(void) { return (int)sizeof(yz_cz_case); }
@@ -1828,6 +1986,10 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
     if ((c->bc[0] == YZ_BC_PERIODIC) != (c->bc[1] == YZ_BC_PERIODIC) ||
         (c->bc[2] == YZ_BC_PERIODIC) != (c->bc[3] == YZ_BC_PERIODIC)) {
         yz::set_error("ConcZ: a periodic edge needs its opposite edge periodic");
+        return nullptr;
+    }
+    if (c->film_precond != YZ_PC_AMG && c->film_precond != YZ_PC_LEARNED) {
+        yz::set_error("ConcZ: film_precond is YZ_PC_AMG or YZ_PC_LEARNED");
         return nullptr;
     }
     if (c->conc_scheme < YZ_CONV_UPWIND1 || c->conc_scheme > YZ_CONV_KK) {
@@ -1887,6 +2049,19 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
     double *hb = nullptr, *hx = nullptr;
     ok = ok && solver_init(z->Sh, op, z->st) == 0;
     ok = ok && !dal(&hb, (size_t)op.N) && !dal(&hx, (size_t)op.N);
+    if (ok && c->film_precond == YZ_PC_LEARNED) {
+        ok = solver_init(z->S32, op, z->st) == 0 && !dal(&z->b32, (size_t)op.N) && !dal(&z->x32, (size_t)op.N) &&
+             !dal(&z->feat, (size_t)8 * op.N) && !dal(&z->mscale, (size_t)op.N) &&
+             !dal(&z->Yh, (size_t)13 * op.N) && !dal(&z->r64, (size_t)op.N) && !dal(&z->xkeep, (size_t)op.N) &&
+             !dal(&z->nonunit, (size_t)1) &&
+             hipHostMalloc((void **)&z->hnonunit, sizeof(int), hipHostMallocDefault) == hipSuccess &&
+             mlp_init(z->mlp, op.N, z->st) == 0;
+        z->S32.use_graph = 1;
+        // a diverging FP32 solve stops once its residual has not halved over
+        // 40 iterations
+        z->S32.stall_window = 40;
+        z->S32.stall_factor = 0.5;
+    }
     if (!ok) {
         if (yz_last_error()[0] == 0) yz::set_error("ConcZ: device set-up failed");
         yz_cz_destroy(z);
@@ -1962,6 +2137,10 @@ void yz_cz_volume_fluxes(yz_cz *z, double *qx, double *qy, double *w) {
 void yz_cz_stats(yz_cz *z, double *out8) {
     if (!z || !out8) return;
     for (int k = 0; k < 8; ++k) out8[k] = z->stats[k];
+}
+void yz_cz_learn_stats(yz_cz *z, double *out4) {
+    if (!z || !out4) return;
+    for (int k = 0; k < 4; ++k) out4[k] = z->lstats[k];
 }
 // the restart state (ConcZ_Model_2D::WriteState / BuildRestart,
 // ConcZ_Model_2D.cpp:572-600, 664-690): the clock, the step control's count
@@ -2040,6 +2219,13 @@ void yz_cz_destroy(yz_cz *z) {
     if (!z) return;
     if (z->st) (void)hipStreamSynchronize(z->st);
     solver_free(z->Sh);
+    solver_free(z->S32);
+    mlp_free(z->mlp);
+    for (float *p : {z->b32, z->x32, z->feat, z->mscale}) if (p) (void)hipFree(p);
+    if (z->Yh) (void)hipFree(z->Yh);
+    for (double *p : {z->r64, z->xkeep}) if (p) (void)hipFree(p);
+    if (z->nonunit) (void)hipFree(z->nonunit);
+    if (z->hnonunit) (void)hipHostFree(z->hnonunit);
     double *ptrs[] = {z->S, z->hO, z->hN, z->fH, z->tH, z->mid, z->hO_keep, z->hN_keep, z->fC, z->tC, z->cO,
                       z->cN, z->c0, z->cm, z->cPrev, z->keepO, z->keepN, z->prevKeep, z->r, z->rp, z->cn, z->d,
                       z->ct, z->Qx, z->Qy, z->W, z->zr, z->cA, z->react, z->cP, z->cb, z->part, z->dsc,

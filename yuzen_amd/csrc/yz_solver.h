@@ -136,6 +136,10 @@ template <typename T> int jacobi_setup(SolverT<T> &S);
 template <typename T>
 int bicgstab(SolverT<T> &S, const T *b, T *x, int precond, int maxit, double tol, int *iters,
              double *resid);
+// r = b - A x on the fine planes S.A (the solver's own row products), with
+// (b, b) and (r, r) in fixed-order sums; synchronises the solver's stream
+template <typename T>
+int residual_norms(SolverT<T> &S, const T *x, const T *b, T *r, double *bb, double *rr);
 
 // BiCGSTAB right-preconditioned by the explicit 13-point stencil S.M, with
 // the iteration fused into five kernels per step and no host round trip

@@ -1149,6 +1149,22 @@ template int amg_setup<double>(SolverT<double> &);
 template int amg_setup<float>(SolverT<float> &);
 template int jacobi_setup<double>(SolverT<double> &);
 template int jacobi_setup<float>(SolverT<float> &);
+template <typename T>
+int residual_norms(SolverT<T> &S, const T *x, const T *b, T *r, double *bb, double *rr) {
+    const StencilOp &op = S.op0;
+    Krylov<T> &k = S.kr;
+    dispatch_ns(op.ns, [&](auto c) {
+        constexpr int NS = decltype(c)::value;
+        hipLaunchKernelGGL((k_init_residual<NS, T>), dim3(RED_BLOCKS), dim3(BLK), 0, S.stream, op, S.A, x, b, r, k.rh,
+                           k.part);
+    });
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLK), 0, S.stream, k.part, k.sc, (int)FIN_INIT);
+    if (read_scalars(S)) return 2;
+    *bb = k.host_sc[SC_BB];
+    *rr = k.host_sc[SC_RR];
+    return 0;
+}
+template int residual_norms<double>(SolverT<double> &, const double *, const double *, double *, double *, double *);
 template int bicgstab<double>(SolverT<double> &, const double *, double *, int, int, double, int *, double *);
 template int bicgstab<float>(SolverT<float> &, const float *, float *, int, int, double, int *, double *);
 template void launch_spmv<double>(const StencilOp &, const double *, const double *, double *, hipStream_t);

@@ -432,3 +432,10 @@ class HipCZ:
         out = np.empty(8)
         self.lib.yz_cz_stats(self.p, _pd(out))
         return out
+
+    def learn_stats(self):
+        """the learned film solve: refinement rounds, FP32 iterations,
+        solves that FP64 AMG finished, training milliseconds"""
+        out = np.empty(4)
+        self.lib.yz_cz_learn_stats(self.p, _pd(out))
+        return out
