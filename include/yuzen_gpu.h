@@ -286,6 +286,15 @@ void yz_step_destroy(yz_step *st);
  * Faces neither the terms nor the boundary rows write are 0. */
 int yz_step_velocity(yz_step *st, double *ux, double *uy);
 
+/* The velocity fields the .vts writes (Field_WM1D_Out / Field_WM2D_Out,
+ * LubFields.cpp:370-383, 832-846, as CalVelocity leaves them,
+ * HeightEquationBase.h:1584-1601): six padded fields of n doubles each in
+ * ux6 (and uy6; NULL in 1-D), in this order: the mixed velocity as
+ * yz_step_velocity gives it, then the capillary, Marangoni, vertical-gravity,
+ * in-plane gravity and centrifugal terms' own fields, each with the boundary
+ * rows once; a term the case leaves out is 0. 0 / 2. */
+int yz_step_velocity_terms(yz_step *st, double *ux6, double *uy6);
+
 /* The energy budget of the context's film (Output.Energy: EnergyBudget.h
  * :52-166, FluxPower HeightEquationBase.h:275-291): F the free energy
  * (sigma/2 |grad(h+s)|^2 + Bo/2 (h+s)^2 + Dp w(h), summed over the real cells
@@ -332,6 +341,25 @@ int yz_read_dat(const char *path, const yz_case *c, double H_coef, double *h, do
 /* yz_write_dat of the context's film, substrate, concentration and face
  * velocities (the reference's field output at the end of a step). */
 int yz_step_write_dat(yz_step *st, const char *path, double L0, double H0, double U0, int dimensioned);
+
+/* The VTK structured-grid file (.vts) of a well-mixed film, in the
+ * reference's layout (OutputVTK::Write of Field_WM1D_Out / Field_WM2D_Out,
+ * Output.cpp:375-637, LubFields.cpp:370-421, 832-880): the points of the
+ * .dat (nx+2 (x ny+2)) on two layers, the substrate and the free surface;
+ * the header names Conc, Height (h+s), Substrate, Velocity and the terms'
+ * V_Ca, V_Ma, V_Bo, V_Pg, V_Sp, and the coordinates (z = s + (h+s - s) k),
+ * with the time and Model="WM2D" in a YuzenData tag; the data follow raw,
+ * each block a 4-byte length and doubles. ux6 / uy6 are the six velocity
+ * fields of yz_step_velocity_terms (uy6 NULL in 1-D). dimensioned != 0
+ * writes the time times T0 [s], mm, um and m/s with L0, H0, U0. The driving
+ * potential's block (Output.FieldData.WritePotential) is not written.
+ * The file is written to path.tmp and renamed. 0 / 2. Host only. */
+int yz_write_vts(const char *path, const yz_case *c, double time, const double *h, const double *s,
+                 const double *conc, const double *ux6, const double *uy6, double L0, double H0, double U0,
+                 double T0, int dimensioned);
+/* yz_write_vts of the context's film, substrate, concentration, velocity
+ * fields and time. */
+int yz_step_write_vts(yz_step *st, const char *path, double L0, double H0, double U0, double T0, int dimensioned);
 
 /* Assembly-level access (parity tests): at film h (n doubles, padded),
  * with the step's dt and the explicit half Ax_old taken at h_old:

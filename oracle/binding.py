@@ -66,6 +66,7 @@ def load_from(path):
     lib.yzo_mobility.argtypes = [C.c_double, C.c_double, C.c_double, pd]
     lib.yzo_evap.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, pd]
     lib.yzo_velocity.argtypes = [C.c_void_p, pd, pd, C.c_int, pd, pd]
+    lib.yzo_velocity_terms.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_energy.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_source.argtypes = [C.c_void_p, C.c_double, pd]
     lib.yzo_limiter.argtypes = [pd, C.c_double, C.c_int, pd]
@@ -181,6 +182,16 @@ class OracleStep:
         vx, vy = np.zeros(self.n), np.zeros(self.n)
         self.lib.yzo_velocity(self.p, _pd(h), _pd(c), 1 if bc else 0, _pd(vx), _pd(vy))
         return vx, vy
+
+    def velocity_terms(self, h, c):
+        """the .vts's velocity fields of (h, c): (vx, vy), each (6, n), the
+        mixed field, then the capillary, Marangoni, vertical-gravity,
+        in-plane gravity and centrifugal terms' own, with boundary rows"""
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        vx, vy = np.zeros(6 * self.n), np.zeros(6 * self.n)
+        self.lib.yzo_velocity_terms(self.p, _pd(h), _pd(c), _pd(vx), _pd(vy))
+        return vx.reshape(6, self.n), vy.reshape(6, self.n)
 
     def energy(self, h, c):
         """(F, D, phi): the free energy, the power of the face fluxes against

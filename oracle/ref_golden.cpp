@@ -628,9 +628,11 @@ void dump_evap_source(const std::string &dir) {
 // from Field_WM1D_Out / Field_WM2D_Out (LubFields.cpp:204-290, 322-372,
 // 602-900): on a smooth film with a velocity field of known values, without
 // and with dimensions; then what ReadAscii_WM1D / ReadAscii_WM2D
-// (Restart.cpp:236-311) make of the first file with H_coef 1.7. Per
-// dimension: neutral_<d>d_in.bin (padded h, s, c, face x and y
-// velocities), neutral_<d>d_nd00000.dat, neutral_<d>d_dim00000.dat,
+// (Restart.cpp:236-311) make of the first file with H_coef 1.7; and the
+// VTK files OutputVTK writes (Output.cpp:375-637) of the same field with the
+// five terms' own velocity fields set. Per dimension: neutral_<d>d_in.bin
+// (padded h, s, c, face x and y velocities, the terms' x and y velocities,
+// the time and the time scale), neutral_<d>d_{nd,dim}00000.{dat,vts},
 // neutral_<d>d_read.bin (padded h, s, c after the read).
 template <class FieldT, class OutT>
 void neutral_case(const std::string &dir, bool twoD) {
@@ -641,12 +643,16 @@ void neutral_case(const std::string &dir, bool twoD) {
     NonDimensionalNumbers nd;
     nd.Build(inp, inp.Initial.SurfaceShape.TotalHeight);
     C_TimeSteps ts;
-    ts.Build(1e-3, 0.0, 1e9, false);
+    ts.Build(1.23456789e-3, 0.0, 1e9, false);
+    ++ts;   // the .vts carries the clock (rounded to 7 digits): one step on
     FieldT field;
     field.Build(inp.Grid, Flat(), nd.ExtentScale());
     auto *G = field.GetGridPtr();
     const int nxp = nx + 4, nyp = twoD ? ny + 4 : 1, n = nxp * nyp;
     std::vector<double> hv(n), sv(n), cv(n), vx(n), vy(n, 0.0);
+    // the five terms' own velocity fields the .vts writes beside the mixed
+    // one: capillary, Marangoni, vertical gravity, in-plane gravity, spin
+    std::vector<std::vector<double>> tx(5, std::vector<double>(n, 0.0)), ty(5, std::vector<double>(n, 0.0));
     Shapes sh{false};
     const double dx = G->dx();
     const double dy = twoD ? (0.8 + 4.0 * 0.8 / ny) / nyp : 0.0;
@@ -658,6 +664,10 @@ void neutral_case(const std::string &dir, bool twoD) {
             hv[i] = sh.H(x, y); sv[i] = sh.S(x, y); cv[i] = sh.C(x, y);
             vx[i] = 0.1 * std::sin(0.37 * ix + 0.11 * iy) + 0.05 * iy;
             if (twoD) vy[i] = 0.07 * std::cos(0.23 * ix - 0.41 * iy) - 0.02 * ix;
+            for (int k = 0; k < 5; ++k) {
+                tx[k][i] = 0.01 * (k + 1) * std::sin(0.29 * ix + 0.17 * (k + 1) * iy + k);
+                if (twoD) ty[k][i] = 0.02 * (k + 1) * std::cos(0.31 * ix - 0.13 * iy + 0.5 * k);
+            }
         }
     for (int iy = 0; iy < nyp; ++iy)
         for (int ix = 0; ix < nxp; ++ix) {
@@ -667,10 +677,21 @@ void neutral_case(const std::string &dir, bool twoD) {
                 field.HH(z) = hv[i]; field.SS(z) = sv[i]; field.CC(z) = cv[i];
                 field.UU().UU().x(G->template At<Yedge>(ix, iy)) = vx[i];
                 field.UU().UU().y(G->template At<Xedge>(ix, iy)) = vy[i];
+                auto *U = &field.UU();
+                std::remove_reference_t<decltype(U->U_LaplacePressure())> *terms[5] = {&U->U_LaplacePressure(), &U->U_SolutoCapillarity(), &U->U_VerticalGravity(),
+                                  &U->U_HorizontalGravity(), &U->U_Centrifugal()};
+                for (int k = 0; k < 5; ++k) {
+                    terms[k]->x(G->template At<Yedge>(ix, iy)) = tx[k][i];
+                    terms[k]->y(G->template At<Xedge>(ix, iy)) = ty[k][i];
+                }
             } else {
                 const Xedge z = G->template At<Xedge>(ix);
                 field.HH(z) = hv[i]; field.SS(z) = sv[i]; field.CC(z) = cv[i];
                 field.UU().UU().x(G->template At<Node>(ix)) = vx[i];
+                auto *U = &field.UU();
+                std::remove_reference_t<decltype(U->U_LaplacePressure())> *terms[5] = {&U->U_LaplacePressure(), &U->U_SolutoCapillarity(), &U->U_VerticalGravity(),
+                                  &U->U_HorizontalGravity(), &U->U_Centrifugal()};
+                for (int k = 0; k < 5; ++k) terms[k]->x(G->template At<Node>(ix)) = tx[k][i];
             }
         }
     const std::string tag = std::string("neutral_") + (twoD ? "2d" : "1d");
@@ -679,6 +700,9 @@ void neutral_case(const std::string &dir, bool twoD) {
         const int hdr[2] = {n, nxp};
         f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
         write_vec(f, hv); write_vec(f, sv); write_vec(f, cv); write_vec(f, vx); write_vec(f, vy);
+        for (int k = 0; k < 5; ++k) { write_vec(f, tx[k]); write_vec(f, ty[k]); }
+        const double clock[2] = {ts.NowTime(), nd.TimeScale()};
+        f.write(reinterpret_cast<const char *>(clock), sizeof(clock));
     }
     for (int dimd = 0; dimd < 2; ++dimd) {
         OutT out;
@@ -687,6 +711,9 @@ void neutral_case(const std::string &dir, bool twoD) {
         OutputNeutral w;
         w.Build(ts, nd, dimd == 1, FilePath(dir), tag + (dimd ? "_dim" : "_nd"), 0);
         w.Write(out);
+        OutputVTK v;
+        v.Build(ts, nd, dimd == 1, FilePath(dir), tag + (dimd ? "_dim" : "_nd"), 0);
+        v.Write(out);
     }
     FieldT back;
     back.Build(inp.Grid, Flat(), nd.ExtentScale());

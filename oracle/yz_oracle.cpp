@@ -1064,10 +1064,14 @@ struct Equation {
      * disjoining), the boundary rows once more. bc = 0 leaves out the
      * boundary rows (what the terms' CalVelocity alone write). */
     void velocity(vector<double> &vx, vector<double> &vy, bool bc) const {
+        vector<double> t[6][2];
+        velocity(vx, vy, bc, t);
+    }
+    /* the same, with each term's field (its boundary rows once) in t */
+    void velocity(vector<double> &vx, vector<double> &vy, bool bc, vector<double> (&t)[6][2]) const {
         const int na = st.naxis;
         const bool on[6] = {cs.capillary != 0, cs.marangoni != 0, cs.vertical_gravity != 0,
                             cs.inplane_gravity != 0, cs.centrifugal != 0, cs.disjoining != 0};
-        vector<double> t[6][2];
         for (int k = 0; k < 6; ++k)
             for (int a = 0; a < 2; ++a) t[k][a].assign(g.n, 0.0);
         for (int i = 0; i < g.n; ++i) {
@@ -2810,6 +2814,27 @@ void yzo_velocity(void *p, const double *h, const double *c, int bc, double *vx,
     eq.velocity(ux, uy, bc != 0);
     std::copy(ux.begin(), ux.end(), vx);
     if (vy) std::copy(uy.begin(), uy.end(), vy);
+    eq.h = hs; eq.c = cs;
+}
+/* the velocity fields of the .vts (Field_WM1D_Out / Field_WM2D_Out,
+ * LubFields.cpp:370-383, 832-846): vx6, vy6 six padded fields each, the
+ * mixed one, then the capillary, Marangoni, vertical-gravity, in-plane
+ * gravity and centrifugal terms' own, each with its boundary rows */
+void yzo_velocity_terms(void *p, const double *h, const double *c, double *vx6, double *vy6) {
+    auto *st = static_cast<yzo::Stepper *>(p);
+    auto &eq = st->eq;
+    const int n = eq.g.n;
+    const std::vector<double> hs = eq.h, cs = eq.c;
+    eq.h.assign(h, h + n); eq.c.assign(c, c + n);
+    eq.fill_ghosts(eq.h); eq.fill_ghosts(eq.c);
+    std::vector<double> ux, uy, t[6][2];
+    eq.velocity(ux, uy, true, t);
+    std::copy(ux.begin(), ux.end(), vx6);
+    if (vy6) std::copy(uy.begin(), uy.end(), vy6);
+    for (int k = 0; k < 5; ++k) {
+        std::copy(t[k][0].begin(), t[k][0].end(), vx6 + (size_t)(k + 1) * n);
+        if (vy6) std::copy(t[k][1].begin(), t[k][1].end(), vy6 + (size_t)(k + 1) * n);
+    }
     eq.h = hs; eq.c = cs;
 }
 /* the energy budget of the field (h, c): out3 = F, D, and phi into phi

@@ -14,9 +14,11 @@ drives them) into a scratch directory and packs what it wrote:
   manifest.json   the dimensionless parameters of every case
   ref_state_wm1d.state  a restart state file (field*.state) written by the
                   reference's C_RestartState for a 1-D film of 8 cells
-  ref_neutral.npz, neutral/*.dat  ASCII field files written by the
-                  reference's OutputNeutral, their inputs, and what its ASCII
-                  restart read back from them
+  ref_neutral.npz, neutral/*.dat, neutral/*.vts  ASCII field files
+                  written by the reference's OutputNeutral and VTK files
+                  written by its OutputVTK, their inputs (the terms' own
+                  velocity fields and the clock among them), and what its
+                  ASCII restart read back from the ASCII files
   eq/eq_n*.dat.gz the reference's analytic equilibrium droplets
                   (cases/eq/eq_n1000, 2000, 4000.dat), copied as data
 
@@ -94,6 +96,14 @@ def main():
                 n, nxp = np.fromfile(f, dtype=np.int32, count=2)
                 for k in ("h", "s", "c", "vx", "vy"):
                     neutral[f"{d}__{k}"] = np.fromfile(f, dtype=np.float64, count=n)
+                # the terms' own velocity fields (capillary, Marangoni,
+                # vertical gravity, in-plane gravity, centrifugal) and the
+                # clock the .vts carries (the time, the time scale)
+                for t in range(5):
+                    neutral[f"{d}__tx{t}"] = np.fromfile(f, dtype=np.float64, count=n)
+                    neutral[f"{d}__ty{t}"] = np.fromfile(f, dtype=np.float64, count=n)
+                neutral[f"{d}__clock"] = np.fromfile(f, dtype=np.float64, count=2)
+                assert neutral[f"{d}__clock"].size == 2 and not f.read()
                 neutral[f"{d}__nxp"] = np.array(nxp)
             with open(os.path.join(tmp, f"neutral_{d}_read.bin"), "rb") as f:
                 n, ok = np.fromfile(f, dtype=np.int32, count=2)
@@ -101,9 +111,10 @@ def main():
                 for k in ("h", "s", "c"):
                     neutral[f"{d}__read_{k}"] = np.fromfile(f, dtype=np.float64, count=n)
             for kind in ("nd", "dim"):
-                src = os.path.join(tmp, f"neutral_{d}_{kind}00000.dat")
-                with open(src, "rb") as f, open(os.path.join(HERE, "neutral", f"{d}_{kind}.dat"), "wb") as g:
-                    g.write(f.read())
+                for ext in ("dat", "vts"):
+                    src = os.path.join(tmp, f"neutral_{d}_{kind}00000.{ext}")
+                    with open(src, "rb") as f, open(os.path.join(HERE, "neutral", f"{d}_{kind}.{ext}"), "wb") as g:
+                        g.write(f.read())
         np.savez_compressed(os.path.join(HERE, "ref_neutral.npz"), **neutral)
         # the reference's analytic equilibrium droplets (cases/eq, ASCII
         # initial conditions at dx = 0.6, 0.3, 0.15 um), as data, gzipped

@@ -120,3 +120,29 @@ def test_mixed_is_the_sum_of_the_terms(periodic):
         assert np.abs(tx).max() > 0
         sx, sy = sx + tx, sy + ty
     assert np.array_equal(vx, sx) and np.array_equal(vy, sy)
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_terms_fields_in_their_slots(dim):
+    """the .vts's velocity fields (velocity_terms): the mixed field as
+    velocity() gives it, then each of the capillary, Marangoni,
+    vertical-gravity, in-plane gravity and centrifugal terms' own field, the
+    field of a case with that term alone (Symmetry walls, where one pass of
+    the boundary rows is two)"""
+    c, h, s, cc = smooth_case(dim, TERMS, slip=0.05, periodic=False)
+    o = ob.OracleStep(c, h, s, cc, 1e-3)
+    vx, vy = o.velocity(h, cc)
+    tx, ty = o.velocity_terms(h, cc)
+    assert np.array_equal(tx[0], vx) and np.array_equal(ty[0], vy)
+    order = ["capillary", "marangoni", "vertical_gravity", "inplane_gravity", "centrifugal", "disjoining"]
+    for k, t in enumerate(order[:5]):
+        ct, _, _, _ = smooth_case(dim, TERMS, slip=0.05, periodic=False)
+        for u in order:
+            setattr(ct, u, 1 if u in (t, "marangoni") else 0)
+        if t != "marangoni":
+            ct.As = 0.0
+        ax, ay = ob.OracleStep(ct, h, s, cc, 1e-3).velocity(h, cc)
+        assert np.abs(ax).max() > 0, t
+        assert np.array_equal(tx[k + 1], ax), t
+        if dim == 2:
+            assert np.array_equal(ty[k + 1], ay), t

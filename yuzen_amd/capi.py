@@ -224,6 +224,9 @@ def load():
     lib.yz_write_dat.argtypes = [C.c_char_p, C.POINTER(YzCase), pd, pd, pd, pd, pd, d, d, d, C.c_int]
     lib.yz_read_dat.argtypes = [C.c_char_p, C.POINTER(YzCase), d, pd, pd, pd]
     lib.yz_step_write_dat.argtypes = [C.c_void_p, C.c_char_p, d, d, d, C.c_int]
+    lib.yz_step_velocity_terms.argtypes = [C.c_void_p, pd, pd]
+    lib.yz_write_vts.argtypes = [C.c_char_p, C.POINTER(YzCase), d, pd, pd, pd, pd, pd, d, d, d, d, C.c_int]
+    lib.yz_step_write_vts.argtypes = [C.c_void_p, C.c_char_p, d, d, d, d, C.c_int]
     lib.yz_step_energy.argtypes = [C.c_void_p, pd, pd]
     lib.yz_step_balances.argtypes = [C.c_void_p, pd]
     lib.yz_step_set_h.argtypes = [C.c_void_p, pd]
@@ -293,6 +296,23 @@ def write_dat(path, case, h, s, c, ux, uy=None, L0=1.0, H0=1.0, U0=1.0, dimensio
     keep = [np.ascontiguousarray(v, dtype=np.float64) for v in (h, s, c, ux, uy if uy is not None else ux)]
     ptrs = [v.ctypes.data_as(C.POINTER(C.c_double)) for v in keep]
     if lib.yz_write_dat(os.fsencode(path), C.byref(case), *ptrs, L0, H0, U0, 1 if dimensioned else 0):
+        raise RuntimeError(last_error())
+
+
+def write_vts(path, case, time, h, s, c, ux6, uy6=None, L0=1.0, H0=1.0, U0=1.0, T0=1.0, dimensioned=False):
+    """the reference's VTK structured-grid file of padded fields (host
+    code): ux6 / uy6 the six velocity fields (mixed, then the capillary,
+    Marangoni, vertical-gravity, in-plane gravity and centrifugal terms'),
+    (6, n) each; uy6 None in 1-D"""
+    lib = load()
+    keep = [np.ascontiguousarray(v, dtype=np.float64) for v in (h, s, c)]
+    keep += [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (ux6, uy6 if uy6 is not None else ux6)]
+    if case.dim == 2 and uy6 is None:
+        raise ValueError("write_vts: a 2-D film needs uy6")
+    ptrs = [v.ctypes.data_as(C.POINTER(C.c_double)) for v in keep]
+    if case.dim == 1:
+        ptrs[4] = None
+    if lib.yz_write_vts(os.fsencode(path), C.byref(case), time, *ptrs, L0, H0, U0, T0, 1 if dimensioned else 0):
         raise RuntimeError(last_error())
 
 

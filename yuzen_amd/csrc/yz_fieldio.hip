@@ -1,8 +1,10 @@
-// yz_fieldio.hip -- the ASCII field file (.dat) of a well-mixed film: the
-// reference's output (OutputNeutral::Write, Output.cpp:830-920, of
-// Field_WM1D_Out / Field_WM2D_Out, LubFields.cpp:204-290, 322-372, 602-900)
-// and its ASCII restart, which reads such a file as an initial condition
-// (C_Restart::ReadAscii_WM1D / ReadAscii_WM2D, Restart.cpp:170-311).
+// yz_fieldio.hip -- the field files of a well-mixed film: the ASCII file
+// (.dat) of the reference's output (OutputNeutral::Write, Output.cpp:830-920,
+// of Field_WM1D_Out / Field_WM2D_Out, LubFields.cpp:204-290, 322-372,
+// 602-900), its ASCII restart, which reads such a file as an initial
+// condition (C_Restart::ReadAscii_WM1D / ReadAscii_WM2D, Restart.cpp:
+// 170-311), and the VTK structured grid (.vts, OutputVTK::Write,
+// Output.cpp:375-637).
 //
 // The file holds nxp-2 (x nyp-2) points: the wall faces and the real cell
 // centres. A point on the edge takes the mean of the ghost and the first real
@@ -270,6 +272,113 @@ int yz_read_dat(const char *path, const yz_case *c, double H_coef, double *h, do
             p = iy * L.nxp + ix;
             h[p] = v[6] * H_coef; s[p] = v[7] * H_coef; cc[p] = v[5];
         }
+    }
+    return 0;
+}
+
+// The VTK structured-grid file (.vts) of a well-mixed film
+// (OutputVTK::Write, Output.cpp:375-637, of Field_WM1D_Out / Field_WM2D_Out
+// as ConvertFromStg fills them, LubFields.cpp:370-421, 832-880): the film
+// drawn as two layers of the same points, the substrate surface (k = 0) and
+// the free surface (k = 1). An XML header names nine point arrays and the
+// coordinates, each an offset into the appended raw data that follows "_":
+// every block is a 4-byte length, then little-endian doubles, x fastest,
+// then y, then the layer. Scalars repeat the layer-0 value on layer 1;
+// velocities are (ux, uy, 0) at both layers; a point's z is s + (h+s - s) k.
+int yz_write_vts(const char *path, const yz_case *c, double time, const double *h, const double *s, const double *cc,
+                 const double *ux6, const double *uy6, double L0, double H0, double U0, double T0, int dimensioned) {
+    Layout L;
+    if (!path || !layout_of(c, L) || !h || !s || !cc || !ux6 || (L.dim == 2 && !uy6)) {
+        yz::set_error("yz_write_vts: bad argument");
+        return 2;
+    }
+    const size_t np = (size_t)L.nxp * L.nyp, m = (size_t)L.nox * L.noy;
+    std::vector<double> hh = points_of(L, h), ss = points_of(L, s), co = points_of(L, cc), hs(m);
+    for (size_t k = 0; k < m; ++k) hs[k] = hh[k] + ss[k];
+    // the mixed velocity, then the capillary, Marangoni, vertical-gravity,
+    // in-plane gravity and centrifugal terms' own fields
+    std::vector<double> vx[6], vy[6];
+    for (int f = 0; f < 6; ++f)
+        velocity_points(L, ux6 + f * np, L.dim == 2 ? uy6 + f * np : nullptr, vx[f], vy[f]);
+    std::vector<double> xx = coords(L.nox, L.xmin, L.xmax, L.dx);
+    std::vector<double> yy = L.dim == 2 ? coords(L.noy, L.ymin, L.ymax, L.dy) : std::vector<double>(1, 0.0);
+    if (dimensioned) {
+        time *= T0;
+        for (double &x : xx) x *= L0 * 1.0e3;
+        if (L.dim == 2)
+            for (double &y : yy) y *= L0 * 1.0e3;
+        for (size_t k = 0; k < m; ++k) {
+            ss[k] *= H0 * 1e6;
+            hs[k] *= H0 * 1e6;
+        }
+        for (int f = 0; f < 6; ++f)
+            for (size_t k = 0; k < m; ++k) {
+                vx[f][k] *= U0;
+                vy[f][k] *= U0;
+            }
+    }
+    const unsigned nz = 2, size_s = (unsigned)(m * nz * 8), size_v = 3 * size_s, step_s = size_s + 4,
+                   step_v = size_v + 4;
+    char line[512];
+    std::string out;
+    auto put = [&](const char *fmt, auto... a) {
+        std::snprintf(line, sizeof line, fmt, a...);
+        out += line;
+    };
+    out += "<VTKFile type=\"StructuredGrid\" GhostLevel=\"0\" byte_order=\"LittleEndian\">\n";
+    put("<YuzenData Time=\"%e\" Model=\"WM2D\" Nx=\"%d\" Ny=\"%d\" Nz=\"%d\" />\n", time, L.nox, L.noy, (int)nz);
+    put("<StructuredGrid WholeExtent=\"1 %d 1 %d 1 %d\">\n", L.nox, L.noy, (int)nz);
+    put("<Piece Extent=\"1 %d 1 %d 1 %d\">\n", L.nox, L.noy, (int)nz);
+    out += "<PointData>\n";
+    unsigned off = 0;
+    for (const char *name : {"Conc", "Height", "Substrate"}) {
+        put("<DataArray Name=\"%s\" type=\"Float64\" format=\"appended\" offset=\"%u\"/>\n", name, off);
+        off += step_s;
+    }
+    for (const char *name : {"Velocity", "V_Ca", "V_Ma", "V_Bo", "V_Pg", "V_Sp"}) {
+        put("<DataArray Name=\"%s\" type=\"Float64\"  NumberOfComponents=\"3\" format=\"appended\" offset=\"%u\"/>\n",
+            name, off);
+        off += step_v;
+    }
+    out += "</PointData>\n<Points>\n";
+    put("<DataArray Name=\"Coordinate\" type=\"Float64\" NumberOfComponents=\"3\" format=\"appended\" offset=\"%u\"/>\n",
+        off);
+    out += "</Points>\n</Piece>\n</StructuredGrid>\n<AppendedData encoding=\"raw\">\n   _";
+    auto raw = [&](const void *p, size_t bytes) { out.append(static_cast<const char *>(p), bytes); };
+    for (const std::vector<double> *v : {&co, &hs, &ss}) {
+        raw(&size_s, 4);
+        for (unsigned k = 0; k < nz; ++k) raw(v->data(), m * 8);
+    }
+    for (int f = 0; f < 6; ++f) {
+        raw(&size_v, 4);
+        for (unsigned kz = 0; kz < nz; ++kz)
+            for (size_t k = 0; k < m; ++k) {
+                const double u[3] = {vx[f][k], L.dim == 2 ? vy[f][k] : 0.0, 0.0};
+                raw(u, sizeof u);
+            }
+    }
+    raw(&size_v, 4);
+    for (unsigned kz = 0; kz < nz; ++kz) {
+        const double zeta = (double)kz;
+        for (int j = 0; j < L.noy; ++j)
+            for (int i = 0; i < L.nox; ++i) {
+                const size_t k = (size_t)j * L.nox + i;
+                const double p3[3] = {xx[i], yy[j], ss[k] + (hs[k] - ss[k]) * zeta};
+                raw(p3, sizeof p3);
+            }
+    }
+    out += "\n</AppendedData>\n</VTKFile>\n";
+    const std::string fpath = path, tmp = fpath + ".tmp";
+    FILE *fo = std::fopen(tmp.c_str(), "wb");
+    if (!fo) {
+        yz::set_error("yz_write_vts: cannot open the file");
+        return 2;
+    }
+    const bool bad = std::fwrite(out.data(), 1, out.size(), fo) != out.size();
+    if (std::fclose(fo) != 0 || bad || std::rename(tmp.c_str(), fpath.c_str()) != 0) {
+        std::remove(tmp.c_str());
+        yz::set_error("yz_write_vts: write failed");
+        return 2;
     }
     return 0;
 }

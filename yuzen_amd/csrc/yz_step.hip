@@ -1462,6 +1462,44 @@ int yz_step_velocity(yz_step *S, double *ux, double *uy) {
     return rc;
 }
 
+// the .vts's velocity fields of the context's film: the mixed one as
+// yz_step_velocity gives it, then the capillary, Marangoni, vertical-gravity,
+// in-plane gravity and centrifugal terms' own, each with its boundary rows
+// once (CalVelocity, HeightEquationBase.h:1589-1597)
+int yz_step_velocity_terms(yz_step *S, double *ux6, double *uy6) {
+    if (!S || !ux6 || (S->g.dim == 2 && !uy6)) { set_err("null argument"); return 2; }
+    const Geo &g = S->g;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    double *d = nullptr;
+    YZ_TRY(hipMalloc((void **)&d, 12 * np * sizeof(double)));
+    double *vx = d, *vy = d + 6 * np;
+    RowArgs a = S->ra;
+    a.h = S->hnew;
+    int rc = 0;
+    do {
+        if (hipMemsetAsync(d, 0, 12 * np * sizeof(double), S->st) != hipSuccess) { rc = 2; break; }
+        const int cells = (g.nxp - 3) * (g.dim == 2 ? g.nyp - 3 : 1);
+        if (g.dim == 2) hipLaunchKernelGGL(k_velocity_terms<2>, dim3(nb(cells)), dim3(TB), 0, S->st, a, vx, vy, np);
+        else hipLaunchKernelGGL(k_velocity_terms<1>, dim3(nb(cells)), dim3(TB), 0, S->st, a, vx, vy, np);
+        for (int f = 0; f < 6; ++f)
+            for (int pass = 0; pass < (f == 0 ? 2 : 1); ++pass) {
+                if (g.dim == 2) {
+                    hipLaunchKernelGGL(k_velocity_bc_x, dim3(nb(g.nyp)), dim3(TB), 0, S->st, g, vx + f * np, vy + f * np);
+                    hipLaunchKernelGGL(k_velocity_bc_y, dim3(nb(g.nxp)), dim3(TB), 0, S->st, g, vx + f * np, vy + f * np);
+                } else {
+                    hipLaunchKernelGGL(k_velocity_bc_x, dim3(1), dim3(TB), 0, S->st, g, vx + f * np, vy + f * np);
+                }
+            }
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(ux6, vx, 6 * np * sizeof(double), hipMemcpyDeviceToHost, S->st) != hipSuccess ||
+            (g.dim == 2 && hipMemcpyAsync(uy6, vy, 6 * np * sizeof(double), hipMemcpyDeviceToHost, S->st) != hipSuccess) ||
+            hipStreamSynchronize(S->st) != hipSuccess) { rc = 2; break; }
+    } while (0);
+    (void)hipFree(d);
+    if (rc) set_err("device error in yz_step_velocity_terms");
+    return rc;
+}
+
 // the energy budget of the context's film (Output.Energy, EnergyBudget.h):
 // the free energy F and the power D of the face fluxes against the potential
 int yz_step_energy(yz_step *S, double *F, double *D) {
@@ -1546,6 +1584,19 @@ int yz_step_write_dat(yz_step *S, const char *path, double L0, double H0, double
     YZ_TRY(hipMemcpyAsync(c.data(), S->c, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
     YZ_TRY(hipStreamSynchronize(S->st));
     return yz_write_dat(path, &S->cs, h.data(), s.data(), c.data(), ux.data(), uy.data(), L0, H0, U0, dimensioned);
+}
+
+int yz_step_write_vts(yz_step *S, const char *path, double L0, double H0, double U0, double T0, int dimensioned) {
+    if (!S || !path) { set_err("null argument"); return 2; }
+    const size_t np = (size_t)S->g.nxp * S->g.nyp;
+    std::vector<double> h(np), s(np), c(np), ux(6 * np), uy(6 * np, 0.0);
+    if (yz_step_velocity_terms(S, ux.data(), uy.data())) return 2;
+    YZ_TRY(hipMemcpyAsync(h.data(), S->hnew, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(s.data(), S->s, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipMemcpyAsync(c.data(), S->c, np * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return yz_write_vts(path, &S->cs, S->time, h.data(), s.data(), c.data(), ux.data(), uy.data(), L0, H0, U0, T0,
+                        dimensioned);
 }
 
 int yz_step_get_h(yz_step *S, double *h) {

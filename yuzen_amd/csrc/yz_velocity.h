@@ -50,8 +50,11 @@ __device__ inline bool vel_wall(const Geo &g, const VelCell<DIM> &q, int a) {
 // AxisMask: no y velocity on a grid two real cells deep
 __device__ inline double vel_mask(const Geo &g, int a, double v) { return (a == 1 && g.nyp == 6) ? 0.0 : v; }
 
+// each term's velocity on the face (a = 0: Xm, 1: Ym) into t: capillary,
+// Marangoni, vertical gravity, in-plane gravity, centrifugal, disjoining;
+// a term the case leaves out is 0
 template <int DIM>
-__device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) {
+__device__ void face_terms(const RowArgs &A, const VelCell<DIM> &q, int a, double t[6]) {
     constexpr int C0 = VelCell<DIM>::C0;
     const Geo &g = A.g;
     const Mob &mob = A.mob;
@@ -63,7 +66,8 @@ __device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) 
     // CV(a): the difference across the face, -C(2a) times the spacing
     const double h2 = hsp * hsp;
     const double cadj = -(1.0 / h2) * hsp, ccen = -(-1.0 / h2) * hsp;
-    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) t[k] = 0.0;
     if (A.cap) {
         const double Hoo = q.H[C0], Haj = q.H[as];
         const double val = mob.slip ? mob.FaceDK(Hoo, Haj) : 2.0 * (Hoo * Hoo) * (Haj * Haj) / (Hoo + Haj);
@@ -94,7 +98,7 @@ __device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) 
         }
         Qh *= Mag * Df;
         Qs *= Mag * Df;
-        v += (Qh + Qs) / hface;
+        t[0] = (Qh + Qs) / hface;
     }
     if (A.ma) {
         double S = 0;
@@ -109,7 +113,7 @@ __device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) 
             const double mu = visc_of(A, cc);
             S = mob.slip ? dc / mu * (hf + 2.0 * mob.lambda) : dc / mu * hf;
         }
-        v += vel_mask(g, a, A.AsMa2 * S);
+        t[1] = vel_mask(g, a, A.AsMa2 * S);
     }
     if (A.vg) {
         const double Gf = mob.Face(q.H[C0], q.H[as]) / visc_of(A, cface);
@@ -117,7 +121,7 @@ __device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) 
         Q += cadj * (q.H[as] + q.S[as]);
         Q += ccen * (q.H[C0] + q.S[C0]);
         Q *= (-1.0 * A.Bo / 3.0) * Gf;
-        v += vel_mask(g, a, Q) / hface;
+        t[2] = vel_mask(g, a, Q) / hface;
     }
 #pragma unroll
     for (int mode = 0; mode < 2; ++mode) {
@@ -136,7 +140,7 @@ __device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) 
         double D = (mob.slip ? mob.DperH(hf) : hf * hf) / visc_of(A, cface);
         if (vel_wall<DIM>(g, q, a)) D = 0;
         const double Q = -w * Mag * D;
-        v += -1.0 * vel_mask(g, a, Q);
+        t[3 + mode] = -1.0 * vel_mask(g, a, Q);
     }
     if (A.dj) {
         const double Gf = mob.Face(q.H[C0], q.H[as]) / visc_of(A, cface);
@@ -149,8 +153,20 @@ __device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) 
         Q += cadj * Pi(q.H[as]);
         Q += ccen * Pi(q.H[C0]);
         Q *= (A.Dp / 3.0) * Gf;
-        v += vel_mask(g, a, Q) / hface;
+        t[5] = vel_mask(g, a, Q) / hface;
     }
+}
+
+// the mixed velocity: the plain sum of the terms the case has, in order
+template <int DIM>
+__device__ double face_velocity(const RowArgs &A, const VelCell<DIM> &q, int a) {
+    double t[6];
+    face_terms<DIM>(A, q, a, t);
+    const bool on[6] = {A.cap != 0, A.ma != 0, A.vg != 0, A.pg != 0, A.cf != 0, A.dj != 0};
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        if (on[k]) v += t[k];
     return v;
 }
 
@@ -176,6 +192,43 @@ __global__ void k_velocity(RowArgs A, double *vx, double *vy) {
     }
     vx[p] = face_velocity<DIM>(A, q, 0);
     if (DIM == 2) vy[p] = face_velocity<DIM>(A, q, 1);
+}
+
+// the mixed field and the first five terms' own fields (the .vts's velocity
+// arrays, LubFields.cpp:378-383, 840-845): field f of six at vx + f np,
+// vy + f np; the mixed one as k_velocity writes it. A term's own value is
+// stored as the term formed it (a term the case leaves out is 0)
+template <int DIM>
+__global__ void k_velocity_terms(RowArgs A, double *vx, double *vy, size_t np) {
+    constexpr int NS = VelCell<DIM>::NS;
+    const Geo &g = A.g;
+    const int nxv = g.nxp - 3, nyv = DIM == 2 ? g.nyp - 3 : 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nxv * nyv) return;
+    VelCell<DIM> q;
+    q.ixp = t % nxv + 2;
+    q.iyp = DIM == 2 ? t / nxv + 2 : 0;
+    const int p = q.iyp * g.nxp + q.ixp;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const bool far = DIM == 2 ? (k == S_Po || k == S_oP) : (k == 4);
+        if (far) { q.H[k] = q.S[k] = q.Cc[k] = 0.0; continue; }
+        const int r = p + slot_ox(DIM, k) + slot_oy(DIM, k) * g.nxp;
+        q.H[k] = A.h[r]; q.S[k] = A.s[r]; q.Cc[k] = A.c[r];
+    }
+    const bool on[6] = {A.cap != 0, A.ma != 0, A.vg != 0, A.pg != 0, A.cf != 0, A.dj != 0};
+    for (int a = 0; a < DIM; ++a) {
+        double tv[6];
+        face_terms<DIM>(A, q, a, tv);
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (on[k]) v += tv[k];
+        double *out = a == 0 ? vx : vy;
+        out[p] = v;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) out[(k + 1) * np + p] = tv[k];
+    }
 }
 
 // the side columns of the boundary rows, one thread per row 1 <= jy <= nyp-2

@@ -83,6 +83,15 @@ class HipStep:
             raise RuntimeError(capi.last_error())
         return ux, uy
 
+    def velocity_terms(self):
+        """the .vts's velocity fields of the current film: (ux6, uy6), each
+        (6, n): the mixed field, then the capillary, Marangoni,
+        vertical-gravity, in-plane gravity and centrifugal terms' own"""
+        ux, uy = np.empty(6 * self.n), np.zeros(6 * self.n)
+        if self.lib.yz_step_velocity_terms(self.p, _pd(ux), _pd(uy)):
+            raise RuntimeError(capi.last_error())
+        return ux.reshape(6, self.n), uy.reshape(6, self.n)
+
     def energy(self):
         """(F, D): the free energy of the current film and the power of its
         face fluxes against the potential"""
@@ -103,6 +112,11 @@ class HipStep:
     def write_dat(self, path, L0=1.0, H0=1.0, U0=1.0, dimensioned=False):
         """the reference's ASCII field file of the current film"""
         if self.lib.yz_step_write_dat(self.p, os.fsencode(path), L0, H0, U0, 1 if dimensioned else 0):
+            raise RuntimeError(capi.last_error())
+
+    def write_vts(self, path, L0=1.0, H0=1.0, U0=1.0, T0=1.0, dimensioned=False):
+        """the reference's VTK structured-grid file of the current film"""
+        if self.lib.yz_step_write_vts(self.p, os.fsencode(path), L0, H0, U0, T0, 1 if dimensioned else 0):
             raise RuntimeError(capi.last_error())
 
     def set_h(self, h):
