@@ -152,17 +152,24 @@ typedef struct yz_case {
     double src_arm_length, src_arm_x, src_arm_y;   /* ArmLength, ArmCenter */
     double src_omega, src_angle0, src_ramp, src_plateau;   /* RotationRate, StartAngle, RampTime, PlateauTime */
     int src_subregions;             /* SubRegions */
-    /* Numerics.Convection.Height (HeightEquationBase.h:609-616): the limited
-     * face interpolation of the upwinded terms (in-plane gravity, the
-     * centrifugal force, Marangoni), YZ_HSCHEME_KOREN (the default) or
-     * YZ_HSCHEME_VANLEER (FiniteDifferences.cpp:97-143) */
+    /* Numerics.Convection.Height (HeightEquationBase.h:609-616): the face
+     * interpolation of the upwinded terms (in-plane gravity, the
+     * centrifugal force, Marangoni): the limited YZ_HSCHEME_KOREN (the
+     * default) or YZ_HSCHEME_VANLEER (FiniteDifferences.cpp:97-143), or one
+     * of the fixed-coefficient schemes (:22-73), which each term assembles
+     * by its own code (BodyForceKernel.h:263-545, SolutoCapillarityKernel.h:
+     * 104-405) */
     int height_scheme;
 } yz_case;
 
+/* The film height's face interpolation (the keys of FiniteDifferences.cpp
+ * :214-228): the two limited schemes, then the fixed-coefficient ones */
+enum { YZ_HSCHEME_KOREN = 0, YZ_HSCHEME_VANLEER = 1, YZ_HSCHEME_UPWIND1 = 2, YZ_HSCHEME_CENTRAL2 = 3,
+       YZ_HSCHEME_CENTRAL4 = 4, YZ_HSCHEME_UPWIND2 = 5, YZ_HSCHEME_UTOPIA = 6, YZ_HSCHEME_QUICK = 7,
+       YZ_HSCHEME_KK = 8 };
 /* Face interpolation of the concentration's advection (reference
  * FiniteDifferences.cpp:22-93, keys :214-228); the unlimited schemes.
  * UTOPIA is the reference's default (Input.cpp:2302-2303). */
-enum { YZ_HSCHEME_KOREN = 0, YZ_HSCHEME_VANLEER = 1 };
 enum { YZ_CONV_UPWIND1 = 0, YZ_CONV_CENTRAL2 = 1, YZ_CONV_CENTRAL4 = 2, YZ_CONV_UPWIND2 = 3,
        YZ_CONV_UTOPIA = 4, YZ_CONV_QUICK = 5, YZ_CONV_KK = 6 };
 

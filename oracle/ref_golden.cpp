@@ -293,8 +293,15 @@ void build_term(K &k, Input &inp, typename K::Grid *G, NonDimensionalNumbers *nd
     k.Build(inp.Physics, G, nd, bc);
 }
 
+// the height schemes by their number in the case (YZ_HSCHEME_*): Koren,
+// van Leer, then the fixed-coefficient ones
+const C_Interpolation_base4::t_Scheme kHeightScheme[] = {
+    C_Interpolation_base4::Koren,       C_Interpolation_base4::VanLeer,    C_Interpolation_base4::Upwind_1st,
+    C_Interpolation_base4::Central_2nd, C_Interpolation_base4::Central_4th, C_Interpolation_base4::Upwind_2nd,
+    C_Interpolation_base4::Upwind_3rd,  C_Interpolation_base4::QUICK,      C_Interpolation_base4::KK};
+
 std::string run_case(const std::string &dir, bool twoD, Term t, double slip, bool periodic, std::ostream &man,
-                     bool &first, bool vanleer = false) {
+                     bool &first, int scheme = 0) {
     Params p;
     p.slip = slip;
     if (t == tDisjoining) p.hstar = 0.1;
@@ -307,7 +314,7 @@ std::string run_case(const std::string &dir, bool twoD, Term t, double slip, boo
     const auto bk = periodic ? BoundaryCondition::Periodic : BoundaryCondition::Symmetry;
     std::ostringstream name;
     name << (twoD ? "2d" : "1d") << "_" << kName[t] << "_slip" << (slip > 0 ? 1 : 0) << (periodic ? "_per" : "_sym")
-         << (vanleer ? "_vl" : "");
+         << (scheme == 1 ? "_vl" : "") << (scheme > 1 ? "_hs" + std::to_string(scheme) : "");
     const std::string path = dir + "/" + name.str() + ".bin";
     const bool cons = (t == tLaplace || t == tVertical);
     g_pp = PotentialParams();
@@ -320,7 +327,7 @@ std::string run_case(const std::string &dir, bool twoD, Term t, double slip, boo
     g_pp.nd = &nd;
     g_periodic = periodic;
     // the upwinded terms' face interpolation (Numerics.Convection.Height)
-    const C_Interpolation_base4::t_Scheme koren = vanleer ? C_Interpolation_base4::VanLeer : C_Interpolation_base4::Koren;
+    const C_Interpolation_base4::t_Scheme koren = kHeightScheme[scheme];
     if (twoD) {
         Field_WM2D field;
         field.Build(g, Flat(), nd.ExtentScale());
@@ -357,7 +364,7 @@ std::string run_case(const std::string &dir, bool twoD, Term t, double slip, boo
                   "\"visc_c0\": 0.5, \"visc_exp\": %.17g, \"sig_c0\": 0.5, \"height_scheme\": %d}",
                   first ? "" : ",", name.str().c_str(), twoD ? 2 : 1, nx, twoD ? ny : 1, kName[t], periodic ? 1 : 0,
                   twoD ? -0.4 : -1.0, twoD ? 0.4 : 1.0, nd.As(), nd.Ma(), nd.Bo(), nd.Pgx(), nd.Pgy(), nd.Sp(),
-                  nd.Dp(), nd.HStar(), slip, p.rotx, p.roty, p.visc_exp, vanleer ? 1 : 0);
+                  nd.Dp(), nd.HStar(), slip, p.rotx, p.roty, p.visc_exp, scheme);
     man << buf;
     first = false;
     return path;
@@ -764,7 +771,16 @@ int main(int argc, char **argv) {
     for (int d2 = 0; d2 < 2; ++d2)
         for (Term t : {tParallel, tCentrifugal, tSoluto})
             for (double slip : {0.0, 0.05})
-                run_case(dir, d2 == 1, t, slip, false, man, first, true);
+                run_case(dir, d2 == 1, t, slip, false, man, first, 1);
+    // UTOPIA, with and without slip, 1-D and 2-D; the other fixed schemes
+    // in 2-D
+    for (int d2 = 0; d2 < 2; ++d2)
+        for (Term t : {tParallel, tCentrifugal, tSoluto})
+            for (double slip : {0.0, 0.05})
+                run_case(dir, d2 == 1, t, slip, false, man, first, 6);
+    for (int sch : {2, 3, 4, 5, 7, 8})
+        for (Term t : {tParallel, tCentrifugal, tSoluto})
+            run_case(dir, true, t, sch == 5 ? 0.05 : 0.0, false, man, first, sch);
     man << "\n]}\n";
     dump_mobility(dir);
     dump_limiter(dir, false);

@@ -362,6 +362,38 @@ struct Limiter {
 
 static inline double Sign(double v) { return v >= 0.0 ? 1.0 : -1.0; }
 
+/* The fixed coefficients of the unlimited four-point schemes
+ * (C_Interpolation_base4::SetCoef, FiniteDifferences.cpp:22-73): c1 and c2
+ * of LL L R RR over the sum of c1, for a YZ_CONV_* scheme (UTOPIA, the
+ * reference's Upwind_3rd, for any other value) */
+static void unlimited_coef(int conv, double c1[4], double c2[4]) {
+    double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+    switch (conv) {
+    case YZ_CONV_UPWIND1: a[1] = 1; a[2] = 1; b[1] = 1; b[2] = -1; break;
+    case YZ_CONV_CENTRAL2: a[1] = 1; a[2] = 1; break;
+    case YZ_CONV_CENTRAL4: a[0] = -1; a[1] = 9; a[2] = 9; a[3] = -1; break;
+    case YZ_CONV_UPWIND2: a[0] = -1; a[1] = 3; a[2] = 3; a[3] = -1; b[0] = -1; b[1] = 3; b[2] = -3; b[3] = 1; break;
+    case YZ_CONV_QUICK: a[0] = -1; a[1] = 9; a[2] = 9; a[3] = -1; b[0] = -1; b[1] = 3; b[2] = -3; b[3] = 1; break;
+    case YZ_CONV_KK: a[0] = -1; a[1] = 7; a[2] = 7; a[3] = -1; b[0] = -3; b[1] = 9; b[2] = -9; b[3] = 3; break;
+    default: a[0] = -1; a[1] = 7; a[2] = 7; a[3] = -1; b[0] = -1; b[1] = 3; b[2] = -3; b[3] = 1; break;
+    }
+    double den = 0;
+    for (int j = 0; j < 4; ++j) den += a[j];
+    for (int j = 0; j < 4; ++j) { c1[j] = a[j] / den; c2[j] = b[j] / den; }
+}
+/* the YZ_CONV_* scheme of an unlimited YZ_HSCHEME_* height scheme */
+static int height_conv(int hs) {
+    switch (hs) {
+    case YZ_HSCHEME_UPWIND1: return YZ_CONV_UPWIND1;
+    case YZ_HSCHEME_CENTRAL2: return YZ_CONV_CENTRAL2;
+    case YZ_HSCHEME_CENTRAL4: return YZ_CONV_CENTRAL4;
+    case YZ_HSCHEME_UPWIND2: return YZ_CONV_UPWIND2;
+    case YZ_HSCHEME_QUICK: return YZ_CONV_QUICK;
+    case YZ_HSCHEME_KK: return YZ_CONV_KK;
+    default: return YZ_CONV_UTOPIA;
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* The film-height equation                                             */
 /* ------------------------------------------------------------------ */
@@ -371,6 +403,10 @@ struct Equation {
     Stencil st;
     Mobility mob;
     Limiter lim;
+    /* Numerics.Convection.Height: limited (Koren, van Leer: lim) or one of
+     * the fixed-coefficient schemes (hc1, hc2 of LL L R RR) */
+    bool hlim = true;
+    double hc1[4] = {0, 0, 0, 0}, hc2[4] = {0, 0, 0, 0};
     vector<double> h, s, c;          /* the working field (guess) */
     vector<double> Ccap[4];          /* capillary C(d)[slot] */
     vector<double> dFace[4];         /* capillary D of the last SetMatrixA */
@@ -415,6 +451,8 @@ struct Equation {
         st.build(cs.dim);
         mob.set(cs.slip);
         lim.vanleer = cs.height_scheme == YZ_HSCHEME_VANLEER ? 1 : 0;
+        hlim = cs.height_scheme == YZ_HSCHEME_KOREN || cs.height_scheme == YZ_HSCHEME_VANLEER;
+        if (!hlim) unlimited_coef(height_conv(cs.height_scheme), hc1, hc2);
         build_pattern(A); Arest = A; J = A; IA = A;
         /* capillary C(d): LaplacePressure_2D.cpp:39-94 / _1D.cpp:35-77 */
         for (int d = 0; d < st.ndir; ++d) Ccap[d].assign(st.ns, 0.0);
@@ -642,7 +680,124 @@ struct Equation {
         D = (mob.slip ? mob.Upwind(hf) : hf * hf) / visc(cc);
         if (wall_face(i, j)) D = 0;
     }
+    /* ---- the fixed-coefficient schemes of the body forces
+     * (BodyForceKernel.h:263-545): the face value sum_j (c1_j + Sign c2_j)
+     * f_j upwinded by Drive for h_f^2/mu, and the same coefficients per
+     * spacing, upwinded by DriveFace, as A's weights ---- */
+    /* Drive (ParallelGravityKernel.h:83-88, CentrifugalKernel.h:88-93): the
+     * centrifugal arm from the mean of the two cells' coordinates
+     * (CellAvgCoord, GridTraits.h:219-220, 488-498) */
+    double bf_drive(int mode, int i, int d) const {
+        if (mode == 0) return d / 2 == 0 ? cs.Pgx : cs.Pgy;
+        if (cs.dim == 1) return bf_drive_face(mode, i, d);
+        const int ix = i % g.nxp, iy = i / g.nxp;
+        switch (d) {
+        case 0: return 0.5 * (g.X(ix - 1) + g.X(ix)) - cs.rot_x;
+        case 1: return 0.5 * (g.X(ix + 1) + g.X(ix)) - cs.rot_x;
+        case 2: return 0.5 * (g.Y(iy - 1) + g.Y(iy)) - cs.rot_y;
+        default: return 0.5 * (g.Y(iy + 1) + g.Y(iy)) - cs.rot_y;
+        }
+    }
+    /* the fixed face value of face d by the sign of pp (FaceStencil order) */
+    double bf_face_value(int i, int d, double pp) const {
+        double hh = 0;
+        for (int j = 0; j < 4; ++j) hh += h[nb(i, st.face[d][j])] * (hc1[j] + Sign(pp) * hc2[j]);
+        return hh;
+    }
+    /* NonlinearTerm (:263-302): h_f^2 / mu(c_f), 0 on a wall face */
+    double bf_nonlinear(int mode, int i, int d) const {
+        const int a = nb(i, st.adj[d]);
+        const double cc = 0.5 * (c[i] + c[a]);
+        if (wall_face(i, d)) return 0;
+        const double hh = bf_face_value(i, d, bf_drive(mode, i, d));
+        const double val = mob.slip ? mob.Upwind(hh) : hh * hh;
+        return val / visc(cc);
+    }
+    /* NonlinearJacobi (:305-336): its derivative by the face's point q */
+    double bf_nonlinear_d(int mode, int i, int d, int q) const {
+        const int a = nb(i, st.adj[d]);
+        const double cc = 0.5 * (c[i] + c[a]);
+        if (wall_face(i, d)) return 0;
+        const double pp = bf_drive(mode, i, d);
+        const double hh = bf_face_value(i, d, pp);
+        const double dhdhk = hc1[q] + Sign(pp) * hc2[q];
+        const double val = mob.slip ? mob.dUpwind(hh) * dhdhk : 2.0 * hh * dhdhk;
+        return val / visc(cc);
+    }
+    /* C(j)[idx] + Sign(dv) C2(j)[idx] of the face's point q: the
+     * coefficients per spacing (SetUpwindCoef(true), UpwindKernelBase.h
+     * :122-145) */
+    double bf_coef(int j, int q, double dv) const {
+        const double hs = spacing(j / 2);
+        return hc1[q] / hs + Sign(dv) * (hc2[q] / hs);
+    }
+    /* SetMatrixA (:344-399) */
+    void bodyforce_A_fixed(int mode, Csr &M) {
+        const bool side = (mode == 1 && cs.dim == 1);
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double dv[4], w[4], D[4], Cf[4][13] = {{0}};
+            for (int j = 0; j < st.ndir; ++j) {
+                dv[j] = bf_drive_face(mode, i, j);
+                w[j] = bf_weight(mode, i, j);
+                D[j] = bf_nonlinear(mode, i, j);
+                for (int q = 0; q < 4; ++q) Cf[j][st.face[j][q]] = bf_coef(j, q, dv[j]);
+            }
+            for (int a = 0; a < st.naxis; ++a) {
+                const int dm = 2 * a, dp = 2 * a + 1;
+                const double Coef = bf_mag(mode, a);
+                for (int q = 0; q < 5; ++q) {
+                    const int idx = st.axis[a][q];
+                    const double v = -w[dm] * Cf[dm][idx] * D[dm] + w[dp] * Cf[dp][idx] * D[dp];
+                    M.at(i, idx) += Coef * v;
+                }
+            }
+            if (side) {
+                const double hoo = h[i];
+                M.at(i, st.center) += mob.slip ? bf_mag(mode, 0) * (mob.DperH(hoo) / visc(c[i]))
+                                               : bf_mag(mode, 0) * (hoo * hoo / visc(c[i]));
+            }
+        }
+    }
+    /* SetJacobiMatrix (:405-492) */
+    void bodyforce_J_fixed(int mode, Csr &M) {
+        const bool side = (mode == 1 && cs.dim == 1);
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double acc[4], gg[4][13] = {{0}};
+            for (int j = 0; j < st.ndir; ++j) {
+                const double dv = bf_drive_face(mode, i, j), w = bf_weight(mode, i, j);
+                const double sg = (j % 2 == 0) ? -1.0 : +1.0;
+                acc[j] = 0;
+                for (int q = 0; q < 4; ++q) acc[j] += sg * w * bf_coef(j, q, dv) * h[nb(i, st.face[j][q])];
+                for (int q = 0; q < 4; ++q) gg[j][st.face[j][q]] = bf_nonlinear_d(mode, i, j, q);
+            }
+            for (int a = 0; a < st.naxis; ++a) {
+                const int dm = 2 * a, dp = 2 * a + 1;
+                const double Coef = bf_mag(mode, a);
+                for (int q = 0; q < 5; ++q) {
+                    const int idx = st.axis[a][q];
+                    const int first = st.dir_of_adj(idx);
+                    const bool inM = st.in_face(dm, idx), inP = st.in_face(dp, idx);
+                    double v = 0;
+                    if (first == dm && inM) v += gg[dm][idx] * acc[dm];
+                    if (first == dp && inP) v += gg[dp][idx] * acc[dp];
+                    if (first != dm && inM) v += gg[dm][idx] * acc[dm];
+                    if (first != dp && inP) v += gg[dp][idx] * acc[dp];
+                    M.at(i, idx) += Coef * v;
+                }
+            }
+            if (side) {
+                const double hoo = h[i];
+                M.at(i, st.center) += mob.slip ? bf_mag(mode, 0) * ((mob.dD(hoo) - mob.DperH(hoo)) / visc(c[i]))
+                                               : bf_mag(mode, 0) * (2.0 * hoo * hoo / visc(c[i]));
+            }
+        }
+    }
     void bodyforce_A(int mode, Csr &M) {
+        if (!hlim) { bodyforce_A_fixed(mode, M); return; }
         const bool side = (mode == 1 && cs.dim == 1);
 #pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
@@ -679,7 +834,97 @@ struct Equation {
         dc = (c[hi] - c[lo]) / hh;
         cc = (c[hi] + c[lo]) / 2.0;
     }
+    /* ---- the fixed-coefficient schemes of the Marangoni term
+     * (SolutoCapillarityKernel.h:182-405): S = dc/mu h_f (h_f + 2 Lambda
+     * with slip), h_f = sum (C + Up C2) h over the face's points, the
+     * coefficients not divided by the spacing (SetUpwindCoef(false)) ---- */
+    double ma_coef_q(int q, double up) const { return hc1[q] + up * hc2[q]; }
+    double ma_face_value(int i, int d, double up) const {
+        double hh = 0;
+        for (int q = 0; q < 4; ++q) hh += ma_coef_q(q, up) * h[nb(i, st.face[d][q])];
+        return hh;
+    }
+    /* NonlinearTerm (:182-211) */
+    double ma_nonlinear(int i, int d) const {
+        if (wall_face(i, d)) return 0;
+        double dc, cc;
+        face_conc(i, d, dc, cc);
+        const double hh = ma_face_value(i, d, ma_up(dc));
+        const double Visco = visc(cc);
+        if (mob.slip) return dc / Visco * mob.Shear(hh);
+        return dc / Visco * hh;
+    }
+    /* SetMatrixA (:234-320) */
+    void marangoni_A_fixed(Csr &M) {
+        const double Coef = ma_coef();
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double S[4], up[4], Val[13] = {0};
+            for (int d = 0; d < st.ndir; ++d) {
+                double dc, cc;
+                face_conc(i, d, dc, cc);
+                up[d] = ma_up(dc);
+                S[d] = ma_nonlinear(i, d);
+            }
+            /* stencil order, then the directions in stencil order */
+            for (int k = 0; k < st.ns; ++k)
+                for (int kk = 0; kk < st.ndir; ++kk) {
+                    const int d = st.sdir(kk);
+                    double cq = 0;
+                    for (int q = 0; q < 4; ++q) if (st.face[d][q] == k) cq = ma_coef_q(q, up[d]);
+                    const double hh = spacing(d / 2);
+                    const double sg = (d % 2 == 0) ? -1.0 : +1.0;
+                    Val[k] += sg * cq * S[d] / hh;
+                }
+            for (int k = 0; k < st.ns; ++k) {
+                bool wide = (k == st.center);
+                for (int d = 0; d < st.ndir; ++d) wide = wide || st.in_face(d, k);
+                if (wide) M.at(i, k) += Coef * Val[k];
+            }
+        }
+    }
+    /* SetJacobiMatrix (:322-385) with ScatterJacobiWide (UpwindKernelBase.h
+     * :160-214) */
+    void marangoni_J_fixed(Csr &M) {
+        const double Coef = ma_coef();
+#pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
+        for (int i = 0; i < g.n; ++i) {
+            if (M.bc[i]) continue;
+            double a[4], gg[4][13] = {{0}};
+            for (int d = 0; d < st.ndir; ++d) {
+                double dc, cc;
+                face_conc(i, d, dc, cc);
+                const double up = ma_up(dc);
+                const double hh = spacing(d / 2);
+                const double sg = (d % 2 == 0) ? -1.0 : +1.0;
+                a[d] = 0;
+                for (int q = 0; q < 4; ++q) a[d] += sg * ma_coef_q(q, up) * h[nb(i, st.face[d][q])] / hh;
+                const bool none = wall_face(i, d);
+                const double Visco = none ? 1.0 : visc(cc);
+                for (int q = 0; q < 4; ++q) gg[d][st.face[d][q]] = none ? 0.0 : dc / Visco * ma_coef_q(q, up);
+            }
+            for (int k = 0; k < st.ns; ++k) {
+                bool wide = (k == st.center);
+                for (int d = 0; d < st.ndir; ++d) wide = wide || st.in_face(d, k);
+                if (!wide) continue;
+                double v = 0;
+                if (k == st.center) {
+                    for (int j = 0; j < st.ndir; ++j) { const int d = st.sdir(j); v += gg[d][k] * a[d]; }
+                } else {
+                    const int first = st.dir_of_adj(k);
+                    if (first >= 0 && st.in_face(first, k)) v += gg[first][k] * a[first];
+                    for (int d = 0; d < st.ndir; ++d) {
+                        if (d == first || !st.in_face(d, k)) continue;
+                        v += gg[d][k] * a[d];
+                    }
+                }
+                M.at(i, k) += Coef * v;
+            }
+        }
+    }
     void marangoni_A(Csr &M) {
+        if (!hlim) { marangoni_A_fixed(M); return; }
         const double Coef = ma_coef();
 #pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
@@ -783,6 +1028,7 @@ struct Equation {
         }
     }
     void bodyforce_J(int mode, Csr &M) {
+        if (!hlim) { bodyforce_J_fixed(mode, M); return; }
         const bool side = (mode == 1 && cs.dim == 1);
 #pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
@@ -813,6 +1059,7 @@ struct Equation {
         }
     }
     void marangoni_J(Csr &M) {
+        if (!hlim) { marangoni_J_fixed(M); return; }
         const double Coef = ma_coef();
 #pragma omp parallel for schedule(static) if (g.n > YZO_PAR_MIN)
         for (int i = 0; i < g.n; ++i) {
@@ -876,17 +1123,30 @@ struct Equation {
         gf /= mu;
         return (-1.0 * cs.Bo / 3.0) * ((gf * (hh - hl) + (s[hi] - s[lo]) * (tk * (hl + hh))) / (dn * dn));
     }
-    double flux_bodyforce(int mode, int i, int j) const {   /* BodyForceKernel.h:118-151 (limited) */
+    double flux_bodyforce(int mode, int i, int j) const {   /* BodyForceKernel.h:118-151 */
+        if (!hlim) {
+            if (wall_face(i, j)) return 0;
+            const double D = bf_nonlinear(mode, i, j);
+            const double dv = bf_drive_face(mode, i, j);
+            double hs = 0;
+            for (int q = 0; q < 4; ++q) hs += bf_coef(j, q, dv) * h[nb(i, st.face[j][q])];
+            return bf_mag(mode, j / 2) * (bf_weight(mode, i, j) * hs * D);
+        }
         double hf, D, wt[4];
         bf_face(mode, i, j, hf, D, wt, nullptr);
         double hs = 0;
         for (int q = 0; q < 4; ++q) hs += wt[q] * h[nb(i, st.face[j][q])];
         return bf_mag(mode, j / 2) * (bf_weight(mode, i, j) * hs * D);
     }
-    double flux_marangoni(int i, int d) const {   /* SolutoCapillarityKernel.h:104-138 (limited) */
+    double flux_marangoni(int i, int d) const {   /* SolutoCapillarityKernel.h:104-138 */
         double dc, cc, f[4], wt[4];
         face_conc(i, d, dc, cc);
         const double hh = spacing(d / 2);
+        if (!hlim) {
+            const double hs = ma_face_value(i, d, ma_up(dc));
+            const double S = wall_face(i, d) ? 0.0 : dc / visc(cc) * (mob.slip ? mob.Shear(hs) : hs);
+            return ma_coef() * (S * hs / hh);
+        }
         face_heights(i, d, f);
         const double up = ma_up(dc);
         const double hf = lim.value(f, up);
@@ -1010,6 +1270,7 @@ struct Equation {
      * (limited face height) */
     double vel_marangoni(int i, int a) const {
         const int d = 2 * a;
+        if (!hlim) return axis_mask(a, ma_coef() * ma_nonlinear(i, d));   /* CalVelocity :387-405 */
         double S = 0;
         if (!wall_face(i, d)) {
             double dc, cc, f[4];
@@ -1024,6 +1285,26 @@ struct Equation {
     /* BodyForceKernel.h:694-720 (limited): -w Mag D, times VelocitySign -1
      * (ParallelGravityKernel.h:112, CentrifugalKernel.h:144) */
     double vel_bodyforce(int mode, int i, int a) const {
+        if (!hlim) {
+            /* CalVelocity (:494-545): the face value from CV = -C(2a) h and
+             * CV2 = -C2(2a) h, and from C, C2 times the spacing */
+            const int dm = 2 * a;
+            const double D = bf_nonlinear(mode, i, dm);
+            const double dv = bf_drive_face(mode, i, dm), wv = bf_weight(mode, i, dm);
+            const double hsp = spacing(a);
+            double Q = 0, hh = 0;
+            for (int q = 0; q < 4; ++q) {
+                const double v = h[nb(i, st.face[dm][q])];
+                const double cv = -(hc1[q] / hsp) * hsp, cv2 = -(hc2[q] / hsp) * hsp;
+                Q += wv * (cv + Sign(dv) * cv2) * v;
+                hh += bf_coef(dm, q, dv) * v;
+            }
+            hh *= hsp;
+            Q *= bf_mag(mode, a) * D;
+            Q = axis_mask(a, Q);
+            if (wall_face(i, dm)) { Q = 0; hh = 1; }
+            return -1.0 * Q / hh;
+        }
         double hf, D, wt[4];
         bf_face(mode, i, 2 * a, hf, D, wt, nullptr);
         double Q = -bf_weight(mode, i, 2 * a) * bf_mag(mode, a) * D;
@@ -1901,19 +2182,7 @@ struct Stepper {
     /* the scheme's coefficients (FiniteDifferences.cpp:22-93): c1 and c2
      * of LL L R RR, over the sum of c1 */
     void conc_scheme() {
-        double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
-        switch (eq.cs.conc_scheme) {
-        case YZ_CONV_UPWIND1: a[1] = 1; a[2] = 1; b[1] = 1; b[2] = -1; break;
-        case YZ_CONV_CENTRAL2: a[1] = 1; a[2] = 1; break;
-        case YZ_CONV_CENTRAL4: a[0] = -1; a[1] = 9; a[2] = 9; a[3] = -1; break;
-        case YZ_CONV_UPWIND2: a[0] = -1; a[1] = 3; a[2] = 3; a[3] = -1; b[0] = -1; b[1] = 3; b[2] = -3; b[3] = 1; break;
-        case YZ_CONV_QUICK: a[0] = -1; a[1] = 9; a[2] = 9; a[3] = -1; b[0] = -1; b[1] = 3; b[2] = -3; b[3] = 1; break;
-        case YZ_CONV_KK: a[0] = -1; a[1] = 7; a[2] = 7; a[3] = -1; b[0] = -3; b[1] = 9; b[2] = -9; b[3] = 3; break;
-        default: a[0] = -1; a[1] = 7; a[2] = 7; a[3] = -1; b[0] = -1; b[1] = 3; b[2] = -3; b[3] = 1; break;
-        }
-        double den = 0;
-        for (int j = 0; j < 4; ++j) den += a[j];
-        for (int j = 0; j < 4; ++j) { cw1[j] = a[j] / den; cw2[j] = b[j] / den; }
+        unlimited_coef(eq.cs.conc_scheme, cw1, cw2);
     }
     /* FaceWeights / FaceValue of an unlimited scheme (FiniteDifferences.cpp:152-170) */
     void conc_weights(double sign, double w[4]) const {

@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(_HERE, "libyuzen_hip.so")
 BC_SYMMETRY, BC_PERIODIC, BC_PINNING, BC_FRESHINLET, BC_OUTLET = range(5)
 PC_JACOBI, PC_AMG, PC_LEARNED, PC_LEARNED_AMG = range(4)
 HSCHEME_KOREN, HSCHEME_VANLEER = 0, 1
+HSCHEME_UPWIND1, HSCHEME_CENTRAL2, HSCHEME_CENTRAL4, HSCHEME_UPWIND2 = 2, 3, 4, 5
+HSCHEME_UTOPIA, HSCHEME_QUICK, HSCHEME_KK = 6, 7, 8
 (CONV_UPWIND1, CONV_CENTRAL2, CONV_CENTRAL4, CONV_UPWIND2, CONV_UTOPIA, CONV_QUICK,
  CONV_KK) = range(7)
 LEARN_NPARAM, LEARN_NFEAT, LEARN_NOUT = 6288, 8, 13

@@ -94,7 +94,7 @@ __device__ double face_flux(const RowArgs &A, const Cell<DIM> &q, int d) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) f[k] = q.H[face_slot(DIM, d, k)];
             const double up = sgn(A.AsMa2) * sgn(dc);
-            const double hf = limited_face(f, up, wt, gd, A.vanleer);
+            const double hf = face_interp(f, up, wt, gd, A.hs);
             const double Sf = wall_of<DIM>(A, q, d) ? 0.0 : dc / visc_of(A, cc) * (mob.slip ? hf + 2.0 * mob.lambda : hf);
             double hs = 0;
 #pragma unroll
@@ -114,7 +114,7 @@ __device__ double face_flux(const RowArgs &A, const Cell<DIM> &q, int d) {
         double f[4], wt[4], gd[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) f[k] = q.H[face_slot(DIM, d, k)];
-        const double hf = limited_face(f, dv, wt, gd, A.vanleer);
+        const double hf = face_interp(f, dv, wt, gd, A.hs);
 #pragma unroll
         for (int k = 0; k < 4; ++k) wt[k] /= hsp;
         const double cc = 0.5 * (q.Cc[C0] + q.Cc[as]);

@@ -46,6 +46,7 @@
 
 #include "../../include/yuzen_gpu.h"
 #include "yz_mlp.h"
+#include "yz_schemes.h"
 #include "yz_solver.h"
 #include "yz_state.h"
 
@@ -2081,19 +2082,7 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
     std::memset(&a, 0, sizeof(a));
     a.g = g; a.cs = *c; a.dt = dt; a.theta = 0.5;
     {
-        double p[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
-        switch (c->conc_scheme) {
-        case YZ_CONV_UPWIND1: p[1] = 1; p[2] = 1; q[1] = 1; q[2] = -1; break;
-        case YZ_CONV_CENTRAL2: p[1] = 1; p[2] = 1; break;
-        case YZ_CONV_CENTRAL4: p[0] = -1; p[1] = 9; p[2] = 9; p[3] = -1; break;
-        case YZ_CONV_UPWIND2: p[0] = -1; p[1] = 3; p[2] = 3; p[3] = -1; q[0] = -1; q[1] = 3; q[2] = -3; q[3] = 1; break;
-        case YZ_CONV_QUICK: p[0] = -1; p[1] = 9; p[2] = 9; p[3] = -1; q[0] = -1; q[1] = 3; q[2] = -3; q[3] = 1; break;
-        case YZ_CONV_KK: p[0] = -1; p[1] = 7; p[2] = 7; p[3] = -1; q[0] = -3; q[1] = 9; q[2] = -9; q[3] = 3; break;
-        default: p[0] = -1; p[1] = 7; p[2] = 7; p[3] = -1; q[0] = -1; q[1] = 3; q[2] = -3; q[3] = 1; break;
-        }
-        double den = 0;
-        for (int j = 0; j < 4; ++j) den += p[j];
-        for (int j = 0; j < 4; ++j) { a.w1[j] = p[j] / den; a.w2[j] = q[j] / den; }
+        scheme_coefs(c->conc_scheme, a.w1, a.w2);
     }
     a.S = z->S; a.hO = z->hO; a.hN = z->hN; a.fH = z->fH; a.tH = z->tH; a.fC = z->fC; a.tC = z->tC;
     a.Dx = z->f2[0]; a.Dy = z->f2[1]; a.sgx = z->f2[2]; a.sgy = z->f2[3]; a.Mx = z->f2[4]; a.My = z->f2[5];

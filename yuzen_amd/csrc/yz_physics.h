@@ -14,9 +14,10 @@
 //                      WM2D/LaplacePressure_2D.cpp:39-128, WM1D/...:35-77
 //   vertical gravity   VerticalGravityKernel.h:150-433
 //   disjoining         DisjoiningPressureKernel.h:150-435
-//   body forces        BodyForceKernel.h (LimitedFace, *_Limited),
-//                      ParallelGravityKernel.h, CentrifugalKernel.h
-//   Marangoni          SolutoCapillarityKernel.h (limited forms)
+//   body forces        BodyForceKernel.h (LimitedFace, *_Limited; the
+//                      fixed-coefficient forms :263-545), ParallelGravity-
+//                      Kernel.h, CentrifugalKernel.h
+//   Marangoni          SolutoCapillarityKernel.h (limited and fixed forms)
 //   mobility           ../Mobility.h:49-191
 //   limiter            ../../numerics/FiniteDifferences.cpp:94-228
 #pragma once
@@ -24,6 +25,7 @@
 #include <cmath>
 
 #include "../../include/yuzen_gpu.h"
+#include "yz_schemes.h"
 
 namespace yz {
 
@@ -211,6 +213,53 @@ __device__ inline double limited_face(const double f[4], double sign, double w[4
     g[u] = 1.0 + 0.5 * (pa - pb);
     g[d] = 0.5 * pb;
     return f[u] + 0.5 * p;
+}
+
+// The face scheme of the upwinded terms' film height (Numerics.Convection.
+// Height): kind 0 Koren, 1 van Leer (limited), 2 a fixed-coefficient scheme
+// whose coefficients of LL L R RR are c1 + c2 with the flow in +, c1 - c2
+// against it (FiniteDifferences.cpp:152-170).
+struct HScheme {
+    int kind;
+    double c1[4], c2[4];
+};
+// The face value of f by the scheme, with A's weights w (sum w f = the face
+// value) and its derivative g by f: for a fixed scheme both are the
+// coefficients c1 + Sign(sign) c2.
+__device__ inline double face_interp(const double f[4], double sign, double w[4], double g[4], const HScheme &S) {
+    if (S.kind < 2) return limited_face(f, sign, w, g, S.kind);
+    const double sg = sgn(sign);
+    double v = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        w[j] = S.c1[j] + sg * S.c2[j];
+        g[j] = w[j];
+        v += f[j] * w[j];
+    }
+    return v;
+}
+
+// The height scheme of the case (YZ_HSCHEME_*) as the device applies it.
+// Host code.
+inline HScheme height_scheme_of(int hs) {
+    HScheme S{};
+    if (hs == YZ_HSCHEME_KOREN || hs == YZ_HSCHEME_VANLEER) {
+        S.kind = hs == YZ_HSCHEME_VANLEER ? 1 : 0;
+        return S;
+    }
+    S.kind = 2;
+    int conv = YZ_CONV_UTOPIA;
+    switch (hs) {
+    case YZ_HSCHEME_UPWIND1: conv = YZ_CONV_UPWIND1; break;
+    case YZ_HSCHEME_CENTRAL2: conv = YZ_CONV_CENTRAL2; break;
+    case YZ_HSCHEME_CENTRAL4: conv = YZ_CONV_CENTRAL4; break;
+    case YZ_HSCHEME_UPWIND2: conv = YZ_CONV_UPWIND2; break;
+    case YZ_HSCHEME_QUICK: conv = YZ_CONV_QUICK; break;
+    case YZ_HSCHEME_KK: conv = YZ_CONV_KK; break;
+    default: break;
+    }
+    scheme_coefs(conv, S.c1, S.c2);
+    return S;
 }
 
 }  // namespace yz

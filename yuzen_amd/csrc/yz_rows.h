@@ -71,7 +71,7 @@ struct RowArgs {
     double Ccap[4][13];
     double theta, dt;
     int row_scaling;
-    int vanleer;               // the height scheme is van Leer (else Koren)
+    HScheme hs;                // the upwinded terms' face scheme of the film height
     const double *h, *s, *c;   // padded
     const double *hold;        // padded, h at the start of the step
     const double *axold;       // N: A(h_old) h_old
@@ -344,7 +344,7 @@ __device__ __forceinline__ void row_body(const RowArgs &A, int i, int ixp, int i
                 double f[4], wt[4], gd[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) f[q] = H[face_slot(DIM, j, q)];
-                const double hf = limited_face(f, dv, wt, gd, A.vanleer);
+                const double hf = face_interp(f, dv, wt, gd, A.hs);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) { wt[q] *= ihs; gd[q] *= ihs; }
                 double D = (mob.slip ? mob.DperH(hf) : hf * hf) / fq[j].mu;
@@ -379,7 +379,7 @@ __device__ __forceinline__ void row_body(const RowArgs &A, int i, int ixp, int i
                 double f[4], wt[4], gd[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) f[q] = H[face_slot(DIM, d, q)];
-                const double hf = limited_face(f, up, wt, gd, A.vanleer);
+                const double hf = face_interp(f, up, wt, gd, A.hs);
                 const bool wl = wall(d);
                 const double mu = fq[d].mu;
                 const double Sf = wl ? 0.0 : (mob.slip ? dc / mu * (hf + 2.0 * mob.lambda) : dc / mu * hf);

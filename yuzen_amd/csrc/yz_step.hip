@@ -303,8 +303,8 @@ static int check_case(const yz_case &c) {
     if (c.stall_window < 0) { set_err("stall_window must be >= 0"); return 1; }
     if (c.zero_guess < -1 || c.zero_guess > 1) { set_err("zero_guess must be -1, 0 or 1"); return 1; }
     if (c.conc_model != 0 && c.conc_model != 1) { set_err("conc_model must be 0 (frozen) or 1 (well-mixed)"); return 1; }
-    if (c.height_scheme != YZ_HSCHEME_KOREN && c.height_scheme != YZ_HSCHEME_VANLEER) {
-        set_err("height_scheme: the limited schemes Koren and van Leer are implemented for the film height"); return 1;
+    if (c.height_scheme < YZ_HSCHEME_KOREN || c.height_scheme > YZ_HSCHEME_KK) {
+        set_err("height_scheme: not one of the YZ_HSCHEME_* schemes"); return 1;
     }
     if (c.conc_model == 1) {
         if (c.conc_scheme < YZ_CONV_UPWIND1 || c.conc_scheme > YZ_CONV_KK) {
@@ -1197,20 +1197,10 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
         k.conv = c->conc_convection; k.diff = c->conc_diffusion;
         k.side = (c->dim == 1 && c->centrifugal) ? 1 : 0;
         // the scheme's c1 and c2 over the sum of c1 (FiniteDifferences.cpp:22-93)
-        double c1[4] = {0, 0, 0, 0}, c2[4] = {0, 0, 0, 0};
-        switch (c->conc_scheme) {
-        case YZ_CONV_UPWIND1: c1[1] = 1; c1[2] = 1; c2[1] = 1; c2[2] = -1; break;
-        case YZ_CONV_CENTRAL2: c1[1] = 1; c1[2] = 1; break;
-        case YZ_CONV_CENTRAL4: c1[0] = -1; c1[1] = 9; c1[2] = 9; c1[3] = -1; break;
-        case YZ_CONV_UPWIND2: c1[0] = -1; c1[1] = 3; c1[2] = 3; c1[3] = -1; c2[0] = -1; c2[1] = 3; c2[2] = -3; c2[3] = 1; break;
-        case YZ_CONV_QUICK: c1[0] = -1; c1[1] = 9; c1[2] = 9; c1[3] = -1; c2[0] = -1; c2[1] = 3; c2[2] = -3; c2[3] = 1; break;
-        case YZ_CONV_KK: c1[0] = -1; c1[1] = 7; c1[2] = 7; c1[3] = -1; c2[0] = -3; c2[1] = 9; c2[2] = -9; c2[3] = 3; break;
-        default: c1[0] = -1; c1[1] = 7; c1[2] = 7; c1[3] = -1; c2[0] = -1; c2[1] = 3; c2[2] = -3; c2[3] = 1; break;
-        }
-        double den = 0;
-        for (int j = 0; j < 4; ++j) den += c1[j];
+        double c1[4], c2[4];
+        scheme_coefs(c->conc_scheme, c1, c2);
         for (int j = 0; j < 4; ++j) {
-            const double a1 = c1[j] / den, a2 = c2[j] / den;
+            const double a1 = c1[j], a2 = c2[j];
             k.wpos[j] = a1 + 1.0 * a2;
             k.wneg[j] = a1 + (-1.0) * a2;
         }
@@ -1240,7 +1230,7 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
     build_ccap(g, a.Ccap);
     a.theta = c->theta;
     a.row_scaling = c->row_scaling;
-    a.vanleer = c->height_scheme == YZ_HSCHEME_VANLEER ? 1 : 0;
+    a.hs = height_scheme_of(c->height_scheme);
     a.s = S->s; a.c = S->c;
     a.ia = S->S.A; a.b = S->b; a.rmax = S->dmax;
     a.ax = S->axold;

@@ -109,7 +109,7 @@ __device__ void face_terms(const RowArgs &A, const VelCell<DIM> &q, int a, doubl
             double f[4], wt[4], gd[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) f[k] = q.H[face_slot(DIM, d, k)];
-            const double hf = limited_face(f, sgn(A.AsMa2) * sgn(dc), wt, gd, A.vanleer);
+            const double hf = face_interp(f, sgn(A.AsMa2) * sgn(dc), wt, gd, A.hs);
             const double mu = visc_of(A, cc);
             S = mob.slip ? dc / mu * (hf + 2.0 * mob.lambda) : dc / mu * hf;
         }
@@ -136,7 +136,7 @@ __device__ void face_terms(const RowArgs &A, const VelCell<DIM> &q, int a, doubl
         double f[4], wt[4], gd[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) f[k] = q.H[face_slot(DIM, d, k)];
-        const double hf = limited_face(f, dv, wt, gd, A.vanleer);
+        const double hf = face_interp(f, dv, wt, gd, A.hs);
         double D = (mob.slip ? mob.DperH(hf) : hf * hf) / visc_of(A, cface);
         if (vel_wall<DIM>(g, q, a)) D = 0;
         const double Q = -w * Mag * D;
