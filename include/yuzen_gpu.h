@@ -313,6 +313,47 @@ int yz_step_velocity_terms(yz_step *st, double *ux6, double *uy6);
  * The reference's Energy.dat writes F, D and that balance. 0 / 2. */
 int yz_step_energy(yz_step *st, double *F, double *D);
 
+/* FieldInfo of the context's film (FieldInfo.cpp:52-136; what History.dat
+ * and the termination tests read): out9 = the volume and the solute (the
+ * sums over the real cells of h and c h times the cell measure), the mean c
+ * (solute over volume), the largest and smallest h+s, h and c of a real
+ * cell. 0 / 2. */
+int yz_step_field_info(yz_step *st, double *out9);
+
+/* The scales of the run's time series (Output.Dimensioned and the scales of
+ * the non-dimensional numbers, WellMixedModelBase.h:681-700, 796-806). */
+typedef struct yz_series_scales {
+    int dimensioned;        /* 0: every column dimensionless */
+    double L0, H0, T0;      /* the length [m], height [m] and time [s] scales */
+    double sigma0;          /* Fluid.SurfaceTension.BaseValue [N/m] (Energy.dat) */
+    int energy;             /* Output.Energy.Enable: Energy.dat as well */
+} yz_series_scales;
+
+/* Opens the run's time series in the directory dir: History.dat and
+ * CalcInfo.dat, and Energy.dat when sc->energy, each truncated and given
+ * the reference's header line (OutputTimeline::Build, Output.cpp:1053-1094,
+ * with the labels of WellMixedModelBase.h:520-568). From then on every step
+ * of the context adds to the run's evaporated volume (B's evaporation of the
+ * step), the nozzle's supplied volume and solute, the volume and solute the
+ * 1-D centrifugal term flung sideways (its theta-weighted rate,
+ * HeightEquationBase.h:379-383, 1457-1459) and, with Energy.dat, the energy
+ * dissipated (WellMixedModelBase.h:1404-1411); the volume and solute at the
+ * opening are those VolErr and ResinVolErr are taken against. Opening again
+ * starts the sums anew. 0 / 2. Device reductions and a host read per step
+ * while open. */
+int yz_step_series_open(yz_step *st, const char *dir, const yz_series_scales *sc);
+/* Appends one row to each open file at the context's time, as the
+ * reference's FreqOutput writes them (WellMixedModelBase.h:768-832):
+ * History.dat the volume, solute, mean c, evaporated, outflowed volume and
+ * solute, VolErr, ResinVolErr, max/min h+s, max/min h, max/min c, supplied
+ * volume and solute (6 significant digits); CalcInfo.dat the last step's
+ * Newton passes and last correction, the last linear solve's iterations and
+ * relative residual, the next step's dt and ChangeRateOfVariable
+ * (ImplicitSolver.cpp:218-234); Energy.dat the free energy, the
+ * dissipation, the energy dissipated since the opening and the last step's
+ * balance (17 digits). 0 / 2. */
+int yz_step_series_write(yz_step *st);
+
 /* The volume balance of the context's film (History.dat's columns,
  * WellMixedModelBase.h:1339-1360): out4[0] the film's volume and [1] its
  * solute (sums of h and h c over the real cells times the cell measure),

@@ -38,6 +38,7 @@
 
 #include "NonDimensionalNumbers.h"
 #include "LubFields.h"
+#include "FieldInfo.h"
 #include "BoundaryCondition.h"
 #include "CRS_Matrix.h"
 #include "WellMixedTraits.h"
@@ -640,7 +641,8 @@ void dump_evap_source(const std::string &dir) {
 // five terms' own velocity fields set. Per dimension: neutral_<d>d_in.bin
 // (padded h, s, c, face x and y velocities, the terms' x and y velocities,
 // the time and the time scale), neutral_<d>d_{nd,dim}00000.{dat,vts},
-// neutral_<d>d_read.bin (padded h, s, c after the read).
+// neutral_<d>d_read.bin (padded h, s, c after the read); the _in.bin ends
+// with FieldInfo of the field.
 template <class FieldT, class OutT>
 void neutral_case(const std::string &dir, bool twoD) {
     Params p;
@@ -710,6 +712,13 @@ void neutral_case(const std::string &dir, bool twoD) {
         for (int k = 0; k < 5; ++k) { write_vec(f, tx[k]); write_vec(f, ty[k]); }
         const double clock[2] = {ts.NowTime(), nd.TimeScale()};
         f.write(reinterpret_cast<const char *>(clock), sizeof(clock));
+        // FieldInfo of the field (FieldInfo.cpp:52-136), what History.dat
+        // prints of it
+        FieldInfo fi;
+        fi.Visit(field);
+        const double info[9] = {fi.FluidVol(), fi.ResinVol(), fi.AveConc(), fi.MaxHS(), fi.MinHS(),
+                                fi.MaxH(),     fi.MinH(),     fi.MaxC(),    fi.MinC()};
+        f.write(reinterpret_cast<const char *>(info), sizeof(info));
     }
     for (int dimd = 0; dimd < 2; ++dimd) {
         OutT out;

@@ -103,7 +103,10 @@ def main():
                     neutral[f"{d}__tx{t}"] = np.fromfile(f, dtype=np.float64, count=n)
                     neutral[f"{d}__ty{t}"] = np.fromfile(f, dtype=np.float64, count=n)
                 neutral[f"{d}__clock"] = np.fromfile(f, dtype=np.float64, count=2)
-                assert neutral[f"{d}__clock"].size == 2 and not f.read()
+                # FieldInfo: volume, solute, mean c, max/min h+s, max/min
+                # h, max/min c
+                neutral[f"{d}__info"] = np.fromfile(f, dtype=np.float64, count=9)
+                assert neutral[f"{d}__info"].size == 9 and not f.read()
                 neutral[f"{d}__nxp"] = np.array(nxp)
             with open(os.path.join(tmp, f"neutral_{d}_read.bin"), "rb") as f:
                 n, ok = np.fromfile(f, dtype=np.int32, count=2)

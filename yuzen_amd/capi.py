@@ -23,6 +23,11 @@ LEARN_NPARAM, LEARN_NFEAT, LEARN_NOUT = 6288, 8, 13
 EVAP_RAOULT, EVAP_POWERN = 0, 1
 
 
+class YzSeriesScales(C.Structure):
+    _fields_ = [("dimensioned", C.c_int), ("L0", C.c_double), ("H0", C.c_double), ("T0", C.c_double),
+                ("sigma0", C.c_double), ("energy", C.c_int)]
+
+
 class YzCase(C.Structure):
     _fields_ = [
         ("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
@@ -227,6 +232,9 @@ def load():
     lib.yz_read_dat.argtypes = [C.c_char_p, C.POINTER(YzCase), d, pd, pd, pd]
     lib.yz_step_write_dat.argtypes = [C.c_void_p, C.c_char_p, d, d, d, C.c_int]
     lib.yz_step_velocity_terms.argtypes = [C.c_void_p, pd, pd]
+    lib.yz_step_field_info.argtypes = [C.c_void_p, pd]
+    lib.yz_step_series_open.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(YzSeriesScales)]
+    lib.yz_step_series_write.argtypes = [C.c_void_p]
     lib.yz_write_vts.argtypes = [C.c_char_p, C.POINTER(YzCase), d, pd, pd, pd, pd, pd, d, d, d, d, C.c_int]
     lib.yz_step_write_vts.argtypes = [C.c_void_p, C.c_char_p, d, d, d, d, C.c_int]
     lib.yz_step_energy.argtypes = [C.c_void_p, pd, pd]

@@ -157,4 +157,121 @@ __global__ void __launch_bounds__(EB) k_balance_final(int nblocks, const double 
         for (int k = 0; k < 4; ++k) out[k] = sv[k][0];
 }
 
+// ---- the run's time series (History.dat, CalcInfo.dat) ----------------
+// k_series_cells: per block, over the real cells, FieldInfo's quantities
+// (FieldInfo.cpp:52-136): [0] sum h dA, [1] sum c h dA, [2] max h, [3] min h,
+// [4] max h+s, [5] min h+s, [6] max c, [7] min c; over every row the pieces
+// of ChangeRateOfVariable (ImplicitSolver.cpp:225-240): [8] max |h - h_old|,
+// [9] max |h|; and the step's balances over the real cells: [10] B's
+// evaporation of the last pass, [11] the nozzle's supply, [12] the supply
+// times c (SuppliedResin with the concentration frozen,
+// HeightEquationBase.h:325-351), [13] the 1-D centrifugal term's sideways
+// rate at h and [14] times c, [15] the same at h_old and [16] times the
+// step's starting c (SideOutflowRate, BodyForceKernel.h:85-109).
+// k_series_final reduces the blocks in their order.
+constexpr int SER_Q = 17, SER_B = 256, SER_G = 512;
+__device__ inline void series_init(double v[SER_Q]) {
+#pragma unroll
+    for (int k = 0; k < SER_Q; ++k) v[k] = 0.0;
+    v[2] = v[4] = v[6] = -1e40;
+    v[3] = v[5] = v[7] = 1e40;
+}
+__device__ inline void series_merge(double *a, const double *b) {
+    a[0] += b[0]; a[1] += b[1];
+    a[2] = fmax(a[2], b[2]); a[3] = fmin(a[3], b[3]);
+    a[4] = fmax(a[4], b[4]); a[5] = fmin(a[5], b[5]);
+    a[6] = fmax(a[6], b[6]); a[7] = fmin(a[7], b[7]);
+    a[8] = fmax(a[8], b[8]); a[9] = fmax(a[9], b[9]);
+#pragma unroll
+    for (int k = 10; k < SER_Q; ++k) a[k] += b[k];
+}
+__global__ void __launch_bounds__(SER_B) k_series_cells(RowArgs A, const double *hold, const double *cstart,
+                                                        const double *evp, const double *src, int side,
+                                                        double *part) {
+    const Geo &g = A.g;
+    double v[SER_Q];
+    series_init(v);
+    const int np = g.nxp * g.nyp;
+    const double Mg = A.Sp / 3.0;
+    auto rate = [&](double hh, double cc) {
+        const double mu = visc_of(A, cc);
+        return A.mob.slip ? Mg * (A.mob.D(hh) / mu) : Mg * (hh * hh / mu) * hh;
+    };
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const double hh = A.h[p];
+        v[8] = fmax(v[8], fabs(hh - hold[p]));
+        v[9] = fmax(v[9], fabs(hh));
+        const int ix = p % g.nxp, iy = p / g.nxp;
+        const bool real = ix >= 2 && ix < g.nxp - 2 && (g.dim == 1 || (iy >= 2 && iy < g.nyp - 2));
+        if (!real) continue;
+        const int i = (g.dim == 2 ? (iy - 2) * g.nx : 0) + ix - 2;
+        const double cc = A.c[p], hs = hh + A.s[p];
+        // each cell's volume and solute times the cell measure, in
+        // FieldInfo's products (h dx dy, c h dx dy)
+        if (g.dim == 2) {
+            v[0] += hh * g.dx * g.dy;
+            v[1] += cc * hh * g.dx * g.dy;
+        } else {
+            v[0] += hh * g.dx;
+            v[1] += cc * hh * g.dx;
+        }
+        v[2] = fmax(v[2], hh); v[3] = fmin(v[3], hh);
+        v[4] = fmax(v[4], hs); v[5] = fmin(v[5], hs);
+        v[6] = fmax(v[6], cc); v[7] = fmin(v[7], cc);
+        if (evp) v[10] += evp[i];
+        if (src) { v[11] += src[i]; v[12] += src[i] * cc; }
+        if (side) {
+            const double r1 = rate(hh, cc), c0 = cstart[p], r0 = rate(hold[p], c0);
+            v[13] += r1; v[14] += r1 * cc;
+            v[15] += r0; v[16] += r0 * c0;
+        }
+    }
+    __shared__ double sv[SER_Q][SER_B];
+#pragma unroll
+    for (int k = 0; k < SER_Q; ++k) sv[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = SER_B / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            double a[SER_Q], b[SER_Q];
+#pragma unroll
+            for (int k = 0; k < SER_Q; ++k) { a[k] = sv[k][threadIdx.x]; b[k] = sv[k][threadIdx.x + off]; }
+            series_merge(a, b);
+#pragma unroll
+            for (int k = 0; k < SER_Q; ++k) sv[k][threadIdx.x] = a[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < SER_Q; ++k) part[k * SER_G + blockIdx.x] = sv[k][0];
+}
+__global__ void __launch_bounds__(SER_B) k_series_final(int nblocks, const double *part, double *out) {
+    double v[SER_Q];
+    series_init(v);
+    for (int b = threadIdx.x; b < nblocks; b += SER_B) {
+        double w[SER_Q];
+#pragma unroll
+        for (int k = 0; k < SER_Q; ++k) w[k] = part[k * SER_G + b];
+        series_merge(v, w);
+    }
+    __shared__ double sv[SER_Q][SER_B];
+#pragma unroll
+    for (int k = 0; k < SER_Q; ++k) sv[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = SER_B / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            double a[SER_Q], b[SER_Q];
+#pragma unroll
+            for (int k = 0; k < SER_Q; ++k) { a[k] = sv[k][threadIdx.x]; b[k] = sv[k][threadIdx.x + off]; }
+            series_merge(a, b);
+#pragma unroll
+            for (int k = 0; k < SER_Q; ++k) sv[k][threadIdx.x] = a[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < SER_Q; ++k) out[k] = sv[k][0];
+}
+
 }  // namespace yz

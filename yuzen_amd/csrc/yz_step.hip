@@ -258,6 +258,19 @@ struct yz_step {
     double *hterm = nullptr;    // their host copy (pinned)
     int term_status = 0, steady_count = 0;
     double term_info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // the run's time series (yz_step_series_open): where the files go, their
+    // scales, and what every step since the opening accumulates (the
+    // evaporated, supplied and sideways-outflowed volumes and solute, the
+    // energy dissipated), the volume and solute at the opening
+    bool ser_on = false;
+    std::string ser_dir;
+    yz_series_scales ser_sc{};
+    double *dser = nullptr;
+    double ser_evap = 0, ser_sup = 0, ser_supr = 0, ser_out = 0, ser_outr = 0, ser_vol0 = 0, ser_resin0 = 0;
+    double en_F = 0, en_D = 0, en_diss = 0, en_bal = 0;
+    bool en_step = false;
+    int last_lin_it = 0, last_nk = 0;
+    double last_lin_res = 0, last_nk_res = 0;
     hipEvent_t ev[12];
     double tms[6];
     // the well-mixed concentration (conc_model 1): the field's c, the
@@ -1032,6 +1045,7 @@ static int nk_loop(yz_step *S, int *iter_out, double *resid_out, bool *done_out)
         resid_eq = bits_to_double(S->hmax[1]);
         if (S->conc && S->coupled > resid_eq) resid_eq = S->coupled;
         S->plog_corr.push_back(resid_eq); S->plog_eta.push_back(S->last_eps); S->plog_lres.push_back(lr);
+        S->last_lin_it = it; S->last_lin_res = lr;
         if (std::getenv("YZ_DEBUG"))
             std::fprintf(stderr, "[hip] pass %d lin %d resid_lin %.3e corr %.3e rc %d\n", outer, it, lr, resid_eq, ch);
         if (ch != 0) { done = false; break; }
@@ -1248,6 +1262,7 @@ extern "C" {
 
 void yz_step_destroy(yz_step *S) {
     if (!S) return;
+    if (S->dser) (void)hipFree(S->dser);
     solver_free(S->S);
     solver_free(S->S32);
     solver_free(S->Sc);
@@ -1276,6 +1291,8 @@ void yz_step_destroy(yz_step *S) {
     if (S->st && S->own_stream) (void)hipStreamDestroy(S->st);
     delete S;
 }
+
+static int series_step(yz_step *S);
 
 // TimeAdvance (NewtonKantorovich.cpp:424-486) + TimeStepAdjustment
 // (ImplicitSolver.cpp:144-190)
@@ -1362,7 +1379,9 @@ int yz_step_advance(yz_step *S, int *nk_iter, int *lin_iter, double *resid) {
     S->first_step = false;
     S->stepped = true;
     ++S->steps;
+    S->last_nk = it; S->last_nk_res = r;
     if (S->term_on) term_decide(S);
+    if (S->ser_on && series_step(S)) return 2;
     if (nk_iter) *nk_iter = it;
     if (lin_iter) *lin_iter = (int)lin_sum;
     if (resid) *resid = r;
@@ -1519,6 +1538,177 @@ int yz_step_energy(yz_step *S, double *F, double *D) {
     if (rc) { set_err("device error in yz_step_energy"); return rc; }
     *F = host[0];
     *D = host[1];
+    return 0;
+}
+
+// ---- the run's time series (History.dat, CalcInfo.dat, Energy.dat) ------
+// FieldInfo, the change rate and the step's balances in one reduction
+// (k_series_cells); v[SER_Q] as k_series_cells lays them out
+static int series_reduce(yz_step *S, double *v) {
+    const Geo &g = S->g;
+    if (!S->dser && hipMalloc((void **)&S->dser, (SER_Q * (size_t)SER_G + SER_Q) * sizeof(double)) != hipSuccess) {
+        set_err("hipMalloc"); return 2;
+    }
+    const int np = g.nxp * g.nyp;
+    const int nbk = std::min(SER_G, (np + SER_B - 1) / SER_B);
+    RowArgs a = S->ra;
+    a.h = S->hnew;
+    a.c = S->conc ? S->cfield : S->c;
+    const bool ran = S->stepped;
+    const int side = (ran && g.dim == 1 && S->cs.centrifugal) ? 1 : 0;
+    hipLaunchKernelGGL(k_series_cells, dim3(nbk), dim3(SER_B), 0, S->st, a, S->hold, S->conc ? S->c0 : S->c,
+                       (ran && S->cs.evaporation) ? S->evpv : nullptr, (ran && S->cs.source) ? S->srcv : nullptr,
+                       side, S->dser);
+    hipLaunchKernelGGL(k_series_final, dim3(1), dim3(SER_B), 0, S->st, nbk, S->dser, S->dser + SER_Q * SER_G);
+    YZ_TRY(hipGetLastError());
+    YZ_TRY(hipMemcpyAsync(v, S->dser + SER_Q * SER_G, SER_Q * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    YZ_TRY(hipStreamSynchronize(S->st));
+    return 0;
+}
+static double cell_measure(const Geo &g) { return g.dim == 2 ? g.dx * g.dy : g.dx; }
+// what one step adds to the run's sums (WellMixedModelBase.h:1339-1411),
+// with the dt the step was solved with
+static int series_step(yz_step *S) {
+    double v[SER_Q];
+    if (series_reduce(S, v)) return 2;
+    const double cm = cell_measure(S->g), dt = S->dt_old, th = S->cs.theta;
+    if (S->cs.evaporation) S->ser_evap += v[10] * cm * dt;
+    if (S->cs.source) {
+        S->ser_sup += v[11] * cm * dt;
+        S->ser_supr += v[12] * cm * dt;
+    }
+    if (S->g.dim == 1 && S->cs.centrifugal) {
+        S->ser_out += (th * v[13] + (1.0 - th) * v[15]) * cm * dt;
+        S->ser_outr += (th * v[14] + (1.0 - th) * v[16]) * cm * dt;
+    }
+    if (S->ser_sc.energy) {
+        const double F0 = S->en_F, D0 = S->en_D;
+        if (yz_step_energy(S, &S->en_F, &S->en_D)) return 2;
+        S->en_diss += 0.5 * (D0 + S->en_D) * dt;
+        S->en_bal = (S->en_F - F0) / dt + 0.5 * (D0 + S->en_D);
+        S->en_step = true;
+    }
+    return 0;
+}
+// the header line of a time series (OutputTimeline::Build)
+static bool series_header(const std::string &path, bool dim, const std::vector<std::string> &labels) {
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::string line = dim ? "Time [s]" : "Time ";
+    for (const std::string &l : labels) line += "\t" + l;
+    line += "\n";
+    const bool ok = std::fputs(line.c_str(), f) >= 0;
+    return std::fclose(f) == 0 && ok;
+}
+static bool series_append(const std::string &path, const std::string &row) {
+    FILE *f = std::fopen(path.c_str(), "a");
+    if (!f) return false;
+    const bool ok = std::fputs((row + "\n").c_str(), f) >= 0;
+    return std::fclose(f) == 0 && ok;
+}
+static void put_e(std::string &row, const char *fmt, double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, fmt, v);
+    row += b;
+}
+
+int yz_step_field_info(yz_step *S, double *out9) {
+    if (!S || !out9) { set_err("null argument"); return 2; }
+    double v[SER_Q];
+    if (series_reduce(S, v)) return 2;
+    out9[0] = v[0]; out9[1] = v[1]; out9[2] = v[1] / v[0];
+    out9[3] = v[4]; out9[4] = v[5];   // h + s
+    out9[5] = v[2]; out9[6] = v[3];   // h
+    out9[7] = v[6]; out9[8] = v[7];   // c
+    return 0;
+}
+
+int yz_step_series_open(yz_step *S, const char *dir, const yz_series_scales *sc) {
+    if (!S || !dir || !sc) { set_err("null argument"); return 2; }
+    const bool dim = sc->dimensioned != 0, one = S->g.dim == 1;
+    const std::string uV = dim ? (one ? " [m^3/m]" : " [m^3]") : " [-]", uH = dim ? " [m]" : " [-]";
+    const std::string d = dir;
+    const std::vector<std::string> hist = {
+        "Volume" + uV, "ResinVol" + uV, "AveC [-]", "EvaporatedVol" + uV, "OutflowedVol" + uV,
+        "OutflowedResin" + uV, "VolErr [-]", "ResinVolErr [-]", "maxHS" + uH, "minHS" + uH, "maxH" + uH,
+        "minH" + uH, "maxC [-]", "minC [-]", "SuppliedVol" + uV, "SuppliedResin" + uV};
+    const std::vector<std::string> calc = {"NK_iter", "NK_resid", "BCG_iter", "BCG_resid", "deltaT", "ChangeRate"};
+    if (!series_header(d + "/History.dat", dim, hist) || !series_header(d + "/CalcInfo.dat", dim, calc)) {
+        set_err("yz_step_series_open: cannot write the files"); return 2;
+    }
+    if (sc->energy) {
+        const std::string uE = dim ? (one ? " [J/m]" : " [J]") : " [-]", uP = dim ? (one ? " [W/m]" : " [W]") : " [-]";
+        if (!series_header(d + "/Energy.dat", dim, {"FreeEnergy" + uE, "Dissipation" + uP, "Dissipated" + uE,
+                                                    "Balance" + uP})) {
+            set_err("yz_step_series_open: cannot write the files"); return 2;
+        }
+    }
+    double info[9];
+    if (yz_step_field_info(S, info)) return 2;
+    S->ser_vol0 = info[0]; S->ser_resin0 = info[1];
+    S->ser_evap = S->ser_sup = S->ser_supr = S->ser_out = S->ser_outr = 0;
+    S->en_diss = S->en_bal = 0; S->en_step = false;
+    if (sc->energy && yz_step_energy(S, &S->en_F, &S->en_D)) return 2;
+    S->ser_dir = d;
+    S->ser_sc = *sc;
+    S->ser_on = true;
+    return 0;
+}
+
+int yz_step_series_write(yz_step *S) {
+    if (!S) { set_err("null step"); return 2; }
+    if (!S->ser_on) { set_err("yz_step_series_write: no series open"); return 2; }
+    double v[SER_Q];
+    if (series_reduce(S, v)) return 2;
+    const yz_series_scales &sc = S->ser_sc;
+    const bool dim = sc.dimensioned != 0;
+    const double ts = dim ? sc.T0 : 1.0;
+    double vs = 1.0, hs = 1.0;
+    if (dim) {
+        vs = sc.L0 * sc.H0;
+        if (S->g.dim == 2) vs *= sc.L0;
+        hs = sc.H0;
+    }
+    const double vf = v[0], vr = v[1];
+    const double volErr = (vf + S->ser_out + S->ser_evap - S->ser_sup) / S->ser_vol0 - 1.0;
+    const double resinErr = (vr + S->ser_outr - S->ser_supr) / S->ser_resin0 - 1.0;
+    std::string row;
+    put_e(row, "%e", S->time * ts);
+    const double cols[16] = {vf * vs, vr * vs, vr / vf, S->ser_evap * vs, S->ser_out * vs, S->ser_outr * vs,
+                             volErr, resinErr, v[4] * hs, v[5] * hs, v[2] * hs, v[3] * hs, v[6], v[7],
+                             S->ser_sup * vs, S->ser_supr * vs};
+    for (double x : cols) put_e(row, "\t%e", x);
+    bool ok = series_append(S->ser_dir + "/History.dat", row);
+    // ChangeRateOfVariable of the last step
+    double rate;
+    if (S->steps < 5 || fabs(S->dt_old) <= DBL_MIN) rate = 1;
+    else if (fabs(v[9]) <= DBL_MIN) rate = 0;
+    else rate = v[8] / (v[9] * S->dt_old);
+    row.clear();
+    put_e(row, "%e", S->time * ts);
+    row += "\t" + std::to_string(S->last_nk);
+    put_e(row, "\t%e", S->last_nk_res);
+    row += "\t" + std::to_string(S->last_lin_it);
+    put_e(row, "\t%e", S->last_lin_res);
+    put_e(row, "\t%e", S->dt_now * ts);
+    put_e(row, "\t%e", rate / ts);
+    ok = ok && series_append(S->ser_dir + "/CalcInfo.dat", row);
+    if (sc.energy) {
+        double eS = 1, pS = 1;
+        if (dim) {
+            eS = sc.sigma0 * sc.H0 * sc.H0;
+            if (S->g.dim == 1) eS /= sc.L0;
+            pS = eS / sc.T0;
+        }
+        row.clear();
+        put_e(row, "%.17e", S->time * ts);
+        put_e(row, "\t%.17e", S->en_F * eS);
+        put_e(row, "\t%.17e", S->en_D * pS);
+        put_e(row, "\t%.17e", S->en_diss * eS);
+        put_e(row, "\t%.17e", (S->en_step ? S->en_bal : 0.0) * pS);
+        ok = ok && series_append(S->ser_dir + "/Energy.dat", row);
+    }
+    if (!ok) { set_err("yz_step_series_write: cannot append to the files"); return 2; }
     return 0;
 }
 

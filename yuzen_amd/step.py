@@ -92,6 +92,26 @@ class HipStep:
             raise RuntimeError(capi.last_error())
         return ux.reshape(6, self.n), uy.reshape(6, self.n)
 
+    def field_info(self):
+        """FieldInfo of the current film: (volume, solute, mean c, max h+s,
+        min h+s, max h, min h, max c, min c)"""
+        out = np.zeros(9)
+        if self.lib.yz_step_field_info(self.p, _pd(out)):
+            raise RuntimeError(capi.last_error())
+        return tuple(out)
+
+    def series_open(self, directory, dimensioned=False, L0=1.0, H0=1.0, T0=1.0, sigma0=1.0, energy=False):
+        """open the run's History.dat, CalcInfo.dat (and Energy.dat) in
+        directory; every step from now on adds to the run's sums"""
+        sc = capi.YzSeriesScales(1 if dimensioned else 0, L0, H0, T0, sigma0, 1 if energy else 0)
+        if self.lib.yz_step_series_open(self.p, os.fsencode(directory), C.byref(sc)):
+            raise RuntimeError(capi.last_error())
+
+    def series_write(self):
+        """append one row to each open time series"""
+        if self.lib.yz_step_series_write(self.p):
+            raise RuntimeError(capi.last_error())
+
     def energy(self):
         """(F, D): the free energy of the current film and the power of its
         face fluxes against the potential"""
