@@ -17,8 +17,9 @@ in HBM: W untimed steps, then exactly K steps timed between two device
 synchronisations.
   * learned (the value): FP32 BiCGSTAB preconditioned by the learned
     13-point approximate inverse; the network is trained in place on the
-    first Newton passes' own matrices (inside the warm-up), then frozen;
-    every pass's inexact solve starts from zero.
+    first Newton pass's own matrix (100 Adam steps on 2 probe vectors,
+    inside the warm-up), then frozen; every pass's inexact solve starts from
+    zero.
   * plain: FP64 BiCGSTAB preconditioned by the structured SA-AMG, each
     pass's solve started from the previous pass's correction, as the
     reference does. (Reported beside it, for information: the same with
@@ -367,7 +368,7 @@ def main():
         tile_rows, tile_us = capi.tune_tiles(args.n, args.n, iters=20)
     cl = c.copy()
     cl.precond, cl.krylov_fp32 = capi.PC_LEARNED, 1
-    cl.learn_passes, cl.learn_iters, cl.learn_probes, cl.learn_lr = 3, 150, 4, 2e-3
+    cl.learn_passes, cl.learn_iters, cl.learn_probes, cl.learn_lr = 1, 100, 2, 4e-3
     progress(f"tile rows {tile_rows}")
     L = run(cl, h, s, cc, args.steps, args.warmup)
     progress(f"learned run: {args.steps / L['el']:.2f} steps/s")

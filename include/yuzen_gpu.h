@@ -178,7 +178,8 @@ enum { YZ_CONV_UPWIND1 = 0, YZ_CONV_CENTRAL2 = 1, YZ_CONV_CENTRAL4 = 2, YZ_CONV_
  * NewtonEpsilon 1e-10, NewtonEpsilonRel 1e-6, JacobianUpdateInterval 1,
  * MaxIterations 5000, Epsilon 1e-13, RowScaling on, AMG, EW and
  * extrapolation on, step control off; learned preconditioner trained on
- * 2 passes of 100 Adam steps, rate 2e-3, 4 probes; stall window 40;
+ * the first pass, 100 Adam steps of rate 4e-3 on 2 probes each; stall
+ * window 40;
  * graphs off; concentration frozen (conc_model 0), and for conc_model 1:
  * convection and diffusion on, UTOPIA, Df 1, D(c) exponent 0 about 0.5,
  * the concentration solve's MaxIterations 5000, Epsilon 1e-13, RowScaling

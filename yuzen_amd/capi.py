@@ -134,7 +134,7 @@ def case_defaults():
     c.lin_maxit, c.lin_eps, c.row_scaling, c.precond = 5000, 1e-13, 1, PC_AMG
     c.tsc_change, c.tsc_criterion, c.tsc_floor, c.tsc_keep = 0.5, 0.01, 0.0, 50
     c.dt_min, c.dt_max = 1e-15, 1e30
-    c.learn_passes, c.learn_iters, c.learn_probes, c.learn_lr = 2, 100, 4, 2e-3
+    c.learn_passes, c.learn_iters, c.learn_probes, c.learn_lr = 1, 100, 2, 4e-3
     c.stall_window = 40
     c.graphs = 0
     c.conc_model, c.conc_convection, c.conc_diffusion, c.conc_scheme = 0, 1, 1, CONV_UTOPIA

@@ -1100,7 +1100,7 @@ void yz_case_defaults(yz_case *c) {
     c->lin_maxit = 5000; c->lin_eps = 1e-13; c->row_scaling = 1; c->precond = YZ_PC_AMG;
     c->tsc_change = 0.5; c->tsc_criterion = 0.01; c->tsc_floor = 0; c->tsc_keep = 50;
     c->dt_min = 1e-15; c->dt_max = 1e30;
-    c->learn_passes = 2; c->learn_iters = 100; c->learn_probes = 4; c->learn_lr = 2e-3;
+    c->learn_passes = 1; c->learn_iters = 100; c->learn_probes = 2; c->learn_lr = 4e-3;
     c->stall_window = 40;
     c->graphs = 0;
     c->conc_model = 0;
