@@ -139,8 +139,12 @@ typedef struct yz_case {
      * and at the Newton guess (:830-846: Ev E(h, c), at most half the film
      * and the solvent h (1 - c) per step; E from Evaporation.cpp:62-103,
      * uniform in space), less the nozzle's supply at the middle of the step
-     * (SourceBase.cpp, WM2D/Source.cpp, WM1D/Source_1D.cpp). Both with the
-     * concentration frozen (conc_model 0); the batch takes both, per film. */
+     * (SourceBase.cpp, WM2D/Source.cpp, WM1D/Source_1D.cpp). With the
+     * concentration solved (conc_model 1) the concentration equation takes
+     * them too: solvent-only evaporation and the supply at the local or the
+     * nozzle's concentration (ConcEquationBase.h:310-318, StepSources,
+     * HeightEquationBase.h:936-960). The batch takes both, per film, with c
+     * frozen. */
     int evaporation;                /* Physics.Evaporation.Enable */
     double Ev, evap_coef;           /* E = Ev evap_coef f(c) g(h) */
     int evap_func;                  /* YZ_EVAP_RAOULT (default) or YZ_EVAP_POWERN */
@@ -160,6 +164,12 @@ typedef struct yz_case {
      * by its own code (BodyForceKernel.h:263-545, SolutoCapillarityKernel.h:
      * 104-405) */
     int height_scheme;
+    /* Physics.Source.ConcType (SourceBase.cpp:37): 0 "local", the supplied
+     * liquid arrives at the concentration it lands on; 1 "supplied", at
+     * src_conc (Physics.Source.Conc). Enters the well-mixed concentration
+     * (conc_model 1) only. */
+    int src_conc_supplied;
+    double src_conc;
 } yz_case;
 
 /* The film height's face interpolation (the keys of FiniteDifferences.cpp

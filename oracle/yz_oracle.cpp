@@ -1994,6 +1994,10 @@ struct Stepper {
 
     /* ---- B: evaporation and the nozzle's supply ---- */
     vector<double> evap_old, src_cell, Bv;
+    /* B's evaporation of the last pass (_evapB), and the evaporation and
+     * supply the concentration equation takes (StepSources,
+     * HeightEquationBase.h:936-960: _evpStep, _srcStep) */
+    vector<double> evap_B, evp_step, src_step;
     /* Evaluate (Evaporation.cpp:62-103), uniform in space */
     double evap_eval(double hh, double cc) const {
         const yz_case &c = eq.cs;
@@ -2109,12 +2113,29 @@ struct Stepper {
         const int n = eq.g.n;
         const double th = eq.cs.theta;
         Bv.assign(n, 0.0);
+        evap_B.assign(n, 0.0);
         if (!eq.cs.evaporation && !eq.cs.source) return;
 #pragma omp parallel for schedule(static) if (n > YZO_PAR_MIN)
         for (int i = 0; i < n; ++i) {
             if (eq.A.bc[i]) continue;
             const double evp = (1.0 - th) * evap_old[i] + th * evap_at(phi_guess, eq.c, i, dt_now);
+            evap_B[i] = evp;
             Bv[i] = evp - src_cell[i];
+        }
+    }
+    /* StepSources (HeightEquationBase.h:936-960): the evaporation at the
+     * film of the pass, Crank-Nicolson with the step's start, and the
+     * supply; both 0 on the boundary rows and without the terms */
+    void step_sources() {
+        const int n = eq.g.n;
+        const double th = eq.cs.theta;
+        evp_step.assign(n, 0.0);
+        src_step.assign(n, 0.0);
+        if (!eq.cs.evaporation && !eq.cs.source) return;
+        for (int i = 0; i < n; ++i) {
+            if (eq.A.bc[i]) continue;
+            evp_step[i] = (1.0 - th) * evap_old[i] + th * evap_at(phi_guess, eq.c, i, dt_now);
+            src_step[i] = src_cell[i];
         }
     }
 
@@ -2295,6 +2316,8 @@ struct Stepper {
             for (int a = 0; a < eq.st.naxis; ++a) div += G2[a][i] - G2[a][eq.nb(i, eq.st.adj[2 * a])];
             const double cc0 = c0[i];
             double q = h0 * cc0 - dt * div;
+            if (!src_step.empty())
+                q += dt * src_step[i] * (eq.cs.src_conc_supplied ? eq.cs.src_conc : th * c1[i] + (1.0 - th) * cc0);
             if (eq.cs.centrifugal && eq.cs.dim == 1) q -= dt * (th * r1[i] * c1[i] + (1.0 - th) * r0[i] * cc0);
             cx[i] = q / h1;
         }
@@ -2319,8 +2342,14 @@ struct Stepper {
             for (int k = 0; k < eq.st.ns; ++k) M[k] = 0;
             const double h1 = phi_guess[i], h0 = phi_old[i];
             const double rr1 = (eq.cs.centrifugal && eq.cs.dim == 1) ? r1[i] : 0.0;
-            double diag = h1 + th * dt * (rr1 - 0.0);
-            double R = dt * 0.0 * c0[i];
+            /* the supply: at the local c, theta d of it is the new level's;
+             * at the nozzle's c, all of it is known (ConcEquationBase.h
+             * :310-318) */
+            const bool supplied = eq.cs.source && eq.cs.src_conc_supplied;
+            const double S = src_step.empty() ? 0.0 : src_step[i];
+            double diag = h1 + th * dt * (rr1 - (supplied ? 0.0 : S));
+            double R = dt * (evp_step.empty() ? 0.0 : evp_step[i]) * c0[i];
+            if (supplied) R += dt * S * (eq.cs.src_conc - c0[i]);
             for (int d = 0; d < eq.st.ndir; ++d) {
                 const int a = d / 2;
                 const bool high = (d % 2 == 1);
@@ -2431,6 +2460,7 @@ struct Stepper {
         eq.h = phi_guess;
         if (film_not_positive(phi_guess)) { coupled = 0; return true; }
         eq.face_volume_fluxes(F1, r1);
+        step_sources();
         int it = 0; double r = 0;
         if (conc_solve(false, it, r) != 0) return false;
         eq.c = cx;
@@ -2460,8 +2490,9 @@ struct Stepper {
                 div1 += F1[a][i] - F1[a][il];
                 div0 += F0[a][i] - F0[a][il];
             }
+            const double es = evp_step.empty() ? 0.0 : evp_step[i], ss = src_step.empty() ? 0.0 : src_step[i];
             const double r = (phi_old[i] - phi_guess[i]) / dt
-                           - (th * (div1 + r1[i]) + (1.0 - th) * (div0 + r0[i]) + (0.0 - 0.0));
+                           - (th * (div1 + r1[i]) + (1.0 - th) * (div0 + r0[i]) + (es - ss));
             rho[i] = r;
             sum += r;
             ++cnt;
@@ -2578,6 +2609,10 @@ struct Stepper {
     void conc_after() {
         eq.h = phi_guess;
         eq.face_volume_fluxes(F1, r1);
+        step_sources();
+        /* the evaporation as B took it in the last pass, the one the film's
+         * step holds (HeightEquationBase.h:1497-1500) */
+        if (eq.cs.evaporation && evap_B.size() == (size_t)eq.g.n) evp_step = evap_B;
         consistent_fluxes(dt_now);
         const bool noFilm = film_not_positive(phi_guess);
         const vector<double> lastPass = cx;

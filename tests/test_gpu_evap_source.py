@@ -12,8 +12,10 @@ closed forms: tests/test_oracle_evap_source.py).
   preconditioner (1e-9, as its other tests).
 * The batched sweep with B: films advanced together reach, to the bit,
   what each reaches alone.
-* A case the device step does not take (B with the well-mixed
-  concentration) is refused at creation.
+* B with the well-mixed concentration solved (evaporation and the nozzle
+  at the local or the nozzle's concentration): film and c against the
+  oracle step by step with equal Newton counts; a uniform c stays uniform
+  under a nozzle supplying at the local c.
 """
 import numpy as np
 import pytest
@@ -173,8 +175,48 @@ def test_batch_with_b_matches_films_alone():
     b.close()
 
 
-def test_b_with_well_mixed_concentration_is_refused():
+def _conc_pair(c, h, s, cc, dt, steps, label):
+    """film and concentration of the device against the oracle's, step by
+    step: the film to 1e-10, c to 1e-10 of its scale, equal Newton counts"""
+    o = ob.OracleStep(c, h, s, cc, dt)
+    g = HipStep(c, h, s, cc, dt)
+    try:
+        for k in range(steps):
+            ro, rg = o.advance(), g.advance()
+            ho, hg, co, cg = o.h(), g.h(), o.c(), g.c()
+            rh = np.abs(ho - hg).max() / np.abs(ho).max()
+            rc = np.abs(interior(c, co) - interior(c, cg)).max() / np.abs(interior(c, co)).max()
+            print(f"{label} step {k}: oracle nk={ro[1]} hip nk={rg[1]} dh {rh:.2e} dc {rc:.2e}", flush=True)
+            assert ro[0] == 0 and rg[0] == 0
+            assert ro[1] == rg[1]
+            assert rh <= 1e-10 and rc <= 1e-10
+    finally:
+        g.close()
+    return o
+
+
+@pytest.mark.parametrize("supplied", [False, True])
+@pytest.mark.parametrize("dim", [1, 2])
+def test_b_with_well_mixed_concentration_matches_oracle(dim, supplied):
+    """evaporation and the nozzle with the concentration solved in every
+    pass (ConcEquationBase.h:310-318, StepSources, ConsistentFluxes and
+    Reconstruct with the sources); the oracle's balance and closed forms are
+    pinned in tests/test_oracle_conc.py"""
+    c, h, s, cc = _film(dim)
+    c.conc_model, c.Df, c.diff_exp = 1, 0.05, 0.7
+    c.src_conc_supplied, c.src_conc = (1, 0.9) if supplied else (0, 0.0)
+    _conc_pair(c, h, s, cc, DT, 4, f"conc + B dim={dim} supplied={supplied}")
+
+
+def test_uniform_concentration_stays_uniform_under_a_local_nozzle():
     c, h, s, cc = _film(2)
-    c.conc_model = 1
-    with pytest.raises(RuntimeError):
-        HipStep(c, h, s, cc, DT)
+    c.evaporation = 0
+    c.conc_model, c.Df, c.diff_exp = 1, 0.05, 0.7
+    cu = np.full_like(cc, 0.37)
+    g = HipStep(c, h, s, cu, DT)
+    try:
+        for _ in range(3):
+            assert g.advance()[0] == 0
+            assert np.abs(interior(c, g.c()) - 0.37).max() <= 1e-14
+    finally:
+        g.close()

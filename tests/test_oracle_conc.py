@@ -162,3 +162,102 @@ def test_excess_above_one_is_moved_not_cut():
         sol = (_interior(c, o.h()) * ci).sum()
         assert abs(sol - sol0) <= 1e-13 * sol0
     assert moved > 0.0
+
+
+# ---- evaporation and the nozzle with the concentration solved
+# (ConcEquationBase.h:36-66, 261-418; StepSources, HeightEquationBase.h:936-960;
+# ConsistentFluxes :985-1058 with the sources; Reconstruct :460-518)
+
+def _with_b(c, evap=True, source=True, supplied=False):
+    if evap:
+        c.evaporation, c.Ev, c.evap_coef = 1, 1.0, 30.0
+        c.evap_func, c.precursor = capi.EVAP_RAOULT, 0.0
+        c.molar_resin, c.molar_solvent = 2200.0, 106.0
+    if source:
+        c.source, c.src_radius, c.src_rate, c.src_tau = 1, 0.25, 4000.0, 1e-3
+        c.src_arm_length, c.src_arm_x, c.src_arm_y = 0.0, 0.1, -0.05
+        c.src_omega, c.src_angle0, c.src_ramp, c.src_plateau = 0.0, 0.0, 0.0, 10.0
+        c.src_subregions = 4
+        c.src_conc_supplied, c.src_conc = (1, 0.9) if supplied else (0, 0.0)
+    return c
+
+
+def test_evaporation_of_a_flat_film_concentrates_it():
+    """a flat film at rest with a uniform c and a uniform rate: the film
+    loses dt E h per step (E the Raoult rate at c, capped as EvapAt caps it)
+    and, the solute staying, c1 = c0 h0 / h1 exactly: the concentration
+    equation's d h1 = dt E c0 with h1 = h0 - dt E"""
+    for dim in (1, 2):
+        c = _with_b(_flat_case(dim, 12, True, 0.0), source=False)
+        n = cases.padded_shape(c)
+        h = np.full(n[0] * n[1], 0.8)
+        cc = np.full_like(h, 0.3)
+        o = ob.OracleStep(c, h, np.zeros_like(h), cc, 1e-3)
+        h0, c0 = 0.8, 0.3
+        for _ in range(3):
+            assert o.advance()[0] == 0
+            hi, ci = _interior(c, o.h()), _interior(c, o.c())
+            assert np.ptp(hi) <= 1e-15 and np.ptp(ci) <= 1e-15
+            h1, c1 = hi.flat[0], ci.flat[0]
+            assert h1 < h0
+            assert abs(c1 - c0 * h0 / h1) <= 1e-14
+            h0, c0 = h1, c1
+
+
+@pytest.mark.parametrize("supplied", [False, True])
+def test_nozzle_adds_solute_at_its_concentration(supplied):
+    """the solute of a closed film with the nozzle: each step adds dt sum S
+    times the concentration the liquid arrives at, the nozzle's (ConcType
+    supplied) or the Crank-Nicolson mean of the cell's c over the step
+    (local), to rounding (Reconstruct); evaporation takes solvent only"""
+    c, h, s, cc, dt = _allphys()
+    c = _with_b(c, evap=True, source=True, supplied=supplied)
+    o = ob.OracleStep(c, h, s, cc, dt)
+    for k in range(3):
+        hb, cb = _interior(c, o.h()), _interior(c, o.c())
+        sol0 = (hb * cb).sum()
+        t0, dtk = o.clock()[0], o.clock()[1]
+        assert o.advance()[0] == 0
+        ha, ca = _interior(c, o.h()), _interior(c, o.c())
+        src = _interior(c, o.source(t0 + 0.5 * dtk))
+        arrive = 0.9 if supplied else 0.5 * ca + 0.5 * cb
+        added = dtk * (src * arrive).sum()
+        sol1 = (ha * ca).sum()
+        # a film below c = 1 everywhere: no excess moved
+        assert ca.max() < 1.0
+        print(f"step {k}: solute {sol0:.6e} -> {sol1:.6e}, added {added:.3e}")
+        assert added > 1e-6 * sol0
+        assert abs((sol1 - sol0) - added) <= 1e-13 * sol0
+
+
+def test_volume_and_concentration_move_together_with_b():
+    """evaporation and the nozzle with the concentration solved: the film's
+    volume changes by the supply less the evaporation (as with c frozen),
+    the concentration rises where the film evaporates, and a film with the
+    terms off is unchanged by the new code (the same steps as before)"""
+    c, h, s, cc, dt = _allphys(periodic=True)
+    cb = _with_b(c.copy(), evap=True, source=False)
+    o = ob.OracleStep(cb, h, s, cc, dt)
+    v0 = _interior(cb, o.h()).sum()
+    for _ in range(3):
+        assert o.advance()[0] == 0
+    v1 = _interior(cb, o.h()).sum()
+    assert v1 < v0
+    assert _interior(cb, o.c()).mean() > _interior(cb, cc).mean()
+
+
+def test_uniform_concentration_stays_uniform_under_a_local_nozzle():
+    """liquid supplied at the local concentration onto a uniform one keeps
+    it uniform: only when the fluxes the solute rides on take the supply
+    into account (ConsistentFluxes' sources) and the solve and Reconstruct
+    add it at the local c"""
+    c, h, s, cc, dt = _allphys()
+    c = _with_b(c, evap=False, source=True, supplied=False)
+    cu = np.full_like(cc, 0.37)
+    o = ob.OracleStep(c, h, s, cu, dt)
+    v0 = _interior(c, o.h()).sum()
+    for _ in range(3):
+        assert o.advance()[0] == 0
+        ci = _interior(c, o.c())
+        assert np.abs(ci - 0.37).max() <= 1e-14
+    assert _interior(c, o.h()).sum() > v0

@@ -78,6 +78,7 @@ class YzCase(C.Structure):
         ("src_plateau", C.c_double),
         ("src_subregions", C.c_int),
         ("height_scheme", C.c_int),
+        ("src_conc_supplied", C.c_int), ("src_conc", C.c_double),
     ]
 
     def copy(self):

@@ -35,6 +35,12 @@ struct ConcArgs {
     double *cx;              // the concentration of the solve, padded
     double *G2[2];           // solute fluxes, padded
     double *A, *b;           // the row-scaled system: ns planes x N, N
+    // the step's sources (StepSources, HeightEquationBase.h:936-960), N each
+    // or null: the evaporation rate and the nozzle's supply rate; supplied:
+    // the supply arrives at src_conc, else at the local c
+    const double *evp, *src;
+    int supplied;
+    double src_conc;
 };
 
 // FaceVolumeFluxes of the film a.h with concentration a.c: F[a][p] the flux
@@ -59,6 +65,9 @@ void launch_film_check(const Geo &g, const double *h, int *flag, hipStream_t st)
 void launch_conc_commit(const Geo &g, const double *cx, double *c0, double *c, double *cfield, hipStream_t st);
 // zero out d when it holds a value that is not finite (a failed solve)
 void launch_zero_if_nonfinite(int n, double *d, int *flag, hipStream_t st);
+// StepSources' evaporation at the film of the pass: evp = (1 - theta) evold
+// + theta EvapAt(h, c) over the real cells (N)
+void launch_step_evp(const RowArgs &a, const double *h, const double *c, double *evp, hipStream_t st);
 
 // ConsistentFluxes: F1 gains the flux whose divergence is what the step's
 // divergence misses of the film's step. scratch: at least N + 2 ny + 1024 + 8

@@ -187,8 +187,12 @@ __global__ void __launch_bounds__(CB) k_conc_rows(ConcArgs a) {
     const double h1 = a.h[p], h0 = a.hold[p];
     const double c0p = a.c0[p];
     const double rr1 = a.side ? a.r1[p] : 0.0;
-    double diag = h1 + th * dt * (rr1 - 0.0);
-    double R = dt * 0.0 * c0p;
+    // the supply: at the local c, theta d of it is the new level's; at the
+    // nozzle's c, all of it is known (ConcEquationBase.h:310-318)
+    const double S = a.src ? a.src[i] : 0.0;
+    double diag = h1 + th * dt * (rr1 - (a.supplied ? 0.0 : S));
+    double R = dt * (a.evp ? a.evp[i] : 0.0) * c0p;
+    if (a.supplied) R += dt * S * (a.src_conc - c0p);
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         const int ax = d / 2;
@@ -310,6 +314,7 @@ __global__ void __launch_bounds__(CB) k_reconstruct(ConcArgs a) {
     if (DIM == 2) div += a.G2[1][p] - a.G2[1][p - g.nxp];
     const double cc0 = a.c0[p];
     double q = h0 * cc0 - a.dt * div;
+    if (a.src) q += a.dt * a.src[i] * (a.supplied ? a.src_conc : th * a.cx[p] + (1.0 - th) * cc0);
     if (a.side) q -= a.dt * (th * a.r1[p] * a.cx[p] + (1.0 - th) * a.r0[p] * cc0);
     a.cx[p] = q / h1;
 }
@@ -405,7 +410,8 @@ __global__ void k_cf_rho(ConcArgs a, const double *Fx, const double *Fy, double 
         div1 += Fy[p] - Fy[p - g.nxp];
         div0 += a.F0[1][p] - a.F0[1][p - g.nxp];
     }
-    rho[i] = (a.hold[p] - a.h[p]) / a.dt - (th * (div1 + a.r1[p]) + (1.0 - th) * (div0 + a.r0[p]) + (0.0 - 0.0));
+    const double es = a.evp ? a.evp[i] : 0.0, ss = a.src ? a.src[i] : 0.0;
+    rho[i] = (a.hold[p] - a.h[p]) / a.dt - (th * (div1 + a.r1[p]) + (1.0 - th) * (div0 + a.r0[p]) + (es - ss));
 }
 // rowsum[jy] = sum over the row of (rho - mean), one block per row
 __global__ void __launch_bounds__(CB) k_cf_rowsum(Geo g, const double *rho, const double *mean_sum,
@@ -585,6 +591,17 @@ __global__ void k_ex_caps(Geo g, double *c, int under) {
 }
 
 }  // namespace
+
+__global__ void k_step_evp(RowArgs A, const double *h, const double *c, double *evp) {
+    const Geo &g = A.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.N) return;
+    const int p = pad_index(g, i);
+    evp[i] = (1.0 - A.theta) * A.evold[i] + A.theta * evap_rate(A.bt, h[p], c[p], A.dt);
+}
+void launch_step_evp(const RowArgs &a, const double *h, const double *c, double *evp, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_evp, dim3((a.g.N + CB - 1) / CB), dim3(CB), 0, st, a, h, c, evp);
+}
 
 void launch_face_fluxes(const RowArgs &a, double *Fx, double *Fy, double *r, hipStream_t st) {
     if (a.g.dim == 2) hipLaunchKernelGGL(k_face_flux<2>, dim3(cblocks(a.g.N)), dim3(CB), 0, st, a, Fx, Fy, r);

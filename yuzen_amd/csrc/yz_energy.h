@@ -164,8 +164,9 @@ __global__ void __launch_bounds__(EB) k_balance_final(int nblocks, const double 
 // of ChangeRateOfVariable (ImplicitSolver.cpp:225-240): [8] max |h - h_old|,
 // [9] max |h|; and the step's balances over the real cells: [10] B's
 // evaporation of the last pass, [11] the nozzle's supply, [12] the supply
-// times c (SuppliedResin with the concentration frozen,
-// HeightEquationBase.h:325-351), [13] the 1-D centrifugal term's sideways
+// times the concentration it arrives at (SuppliedResin,
+// HeightEquationBase.h:325-351: the nozzle's, or the mean of the step's
+// start and end c), [13] the 1-D centrifugal term's sideways
 // rate at h and [14] times c, [15] the same at h_old and [16] times the
 // step's starting c (SideOutflowRate, BodyForceKernel.h:85-109).
 // k_series_final reduces the blocks in their order.
@@ -186,8 +187,8 @@ __device__ inline void series_merge(double *a, const double *b) {
     for (int k = 10; k < SER_Q; ++k) a[k] += b[k];
 }
 __global__ void __launch_bounds__(SER_B) k_series_cells(RowArgs A, const double *hold, const double *cstart,
-                                                        const double *evp, const double *src, int side,
-                                                        double *part) {
+                                                        const double *evp, const double *src, int supplied,
+                                                        double src_conc, int side, double *part) {
     const Geo &g = A.g;
     double v[SER_Q];
     series_init(v);
@@ -219,7 +220,10 @@ __global__ void __launch_bounds__(SER_B) k_series_cells(RowArgs A, const double 
         v[4] = fmax(v[4], hs); v[5] = fmin(v[5], hs);
         v[6] = fmax(v[6], cc); v[7] = fmin(v[7], cc);
         if (evp) v[10] += evp[i];
-        if (src) { v[11] += src[i]; v[12] += src[i] * cc; }
+        if (src) {
+            v[11] += src[i];
+            v[12] += src[i] * (supplied ? src_conc : 0.5 * (cstart[p] + cc));
+        }
         if (side) {
             const double r1 = rate(hh, cc), c0 = cstart[p], r0 = rate(hold[p], c0);
             v[13] += r1; v[14] += r1 * cc;

@@ -284,6 +284,9 @@ struct yz_step {
     double *rc0 = nullptr, *rc1 = nullptr;
     double *ex[4] = {nullptr, nullptr, nullptr, nullptr};   // room, gain, f, e
     double *dconc = nullptr, *dsave = nullptr, *bconc = nullptr, *cscratch = nullptr, *csums = nullptr;
+    // the step's starting c (padded) and the evaporation StepSources gives
+    // the concentration at the film of a pass (N)
+    double *cstart = nullptr, *evps = nullptr;
     int *cflags = nullptr, *hcflags = nullptr;
     unsigned long long *cdmax = nullptr, *hcdmax = nullptr;
     double *hcsums = nullptr;
@@ -326,9 +329,6 @@ static int check_case(const yz_case &c) {
         if (c.conc_lin_maxit < 1 || !(c.conc_lin_eps > 0)) { set_err("conc_lin_maxit >= 1 and conc_lin_eps > 0"); return 1; }
     }
     if (has_b(c)) {
-        if (c.conc_model != 0) {
-            set_err("evaporation and the source are built with the concentration frozen (conc_model 0)"); return 1;
-        }
         if (c.evaporation && c.evap_func != YZ_EVAP_RAOULT && c.evap_func != YZ_EVAP_POWERN) {
             set_err("evap_func must be YZ_EVAP_RAOULT or YZ_EVAP_POWERN"); return 1;
         }
@@ -896,6 +896,7 @@ static int conc_before(yz_step *S) {
     if (S->term_on) hipLaunchKernelGGL(k_min_start, dim3(1), dim3(1024), 0, S->st, g, S->cfield, S->dterm + TERM_MINSTART);
     YZ_TRY(hipMemcpyAsync(S->c, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     YZ_TRY(hipMemcpyAsync(S->cprev, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    YZ_TRY(hipMemcpyAsync(S->cstart, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     conc_fluxes(S, S->hold, S->Fc0, S->rc0);
     YZ_TRY(hipMemsetAsync(S->dconc, 0, g.N * sizeof(double), S->st));
     YZ_TRY(hipGetLastError());
@@ -912,6 +913,16 @@ static int conc_every(yz_step *S, bool *ok) {
     if (film_not_positive(S, &no_film)) return 2;
     if (no_film) { S->coupled = 0; return 0; }
     conc_fluxes(S, S->h, S->Fc1, S->rc1);
+    // StepSources (HeightEquationBase.h:936-960): the evaporation at this
+    // pass's film with the pass's c
+    if (S->cs.evaporation) {
+        RowArgs ra = S->ra;
+        ra.dt = S->dt_now;
+        launch_step_evp(ra, S->h, S->c, S->evps, S->st);
+        S->ca.evp = S->evps;
+    } else {
+        S->ca.evp = nullptr;
+    }
     int ch = 0;
     if (conc_solve(S, false, &ch)) return 2;
     if (ch != 0) { *ok = false; return 0; }
@@ -967,6 +978,9 @@ static int conc_after(yz_step *S) {
     const Geo &g = S->g;
     const size_t np = (size_t)g.nxp * g.nyp;
     conc_fluxes(S, S->h, S->Fc1, S->rc1);
+    // the evaporation as B took it in the last pass, the one the film's step
+    // holds (HeightEquationBase.h:1497-1500)
+    S->ca.evp = S->cs.evaporation ? S->evpv : nullptr;
     ConcArgs a = S->ca;
     a.dt = S->dt_now;
     launch_consistent_fluxes(a, S->Fc1[0], S->Fc1[1], S->cscratch, S->st);
@@ -1184,13 +1198,14 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
         S->conc = true;
         double **cp[] = {&S->cfield, &S->c0, &S->cx, &S->cprev, &S->clast, &S->Fc0[0], &S->Fc0[1], &S->Fc1[0],
                          &S->Fc1[1], &S->G2[0], &S->G2[1], &S->rc0, &S->rc1, &S->ex[0], &S->ex[1], &S->ex[2],
-                         &S->ex[3]};
+                         &S->ex[3], &S->cstart};
         for (double **p : cp) {
             if (hipMalloc((void **)p, np * sizeof(double)) != hipSuccess) return fail("hipMalloc");
             if (hipMemset(*p, 0, np * sizeof(double)) != hipSuccess) return fail("memset");
         }
-        for (double **p : {&S->dconc, &S->dsave, &S->bconc})
+        for (double **p : {&S->dconc, &S->dsave, &S->bconc, &S->evps})
             if (hipMalloc((void **)p, g.N * sizeof(double)) != hipSuccess) return fail("hipMalloc");
+        if (hipMemset(S->evps, 0, g.N * sizeof(double)) != hipSuccess) return fail("memset");
         if (hipMemset(S->dconc, 0, g.N * sizeof(double)) != hipSuccess) return fail("memset");
         if (hipMalloc((void **)&S->cscratch, ((size_t)g.N + 2 * g.ny + 256 + 8) * sizeof(double)) != hipSuccess ||
             hipMalloc((void **)&S->csums, (256 + 8) * sizeof(double)) != hipSuccess ||
@@ -1225,6 +1240,12 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
         k.r0 = S->rc0; k.r1 = S->rc1;
         k.cx = S->cx;
         k.A = S->Sc.A; k.b = S->bconc;
+        // the step's sources (ConcEquationBase.h:310-318; HeightEquationBase.h
+        // :1379-1385): the supply at the local c or at the nozzle's
+        k.src = c->source ? S->srcv : nullptr;
+        k.supplied = (c->source && c->src_conc_supplied) ? 1 : 0;
+        k.src_conc = c->src_conc;
+        k.evp = nullptr;
     }
     // row kernel arguments
     RowArgs &a = S->ra;
@@ -1268,7 +1289,7 @@ void yz_step_destroy(yz_step *S) {
     solver_free(S->Sc);
     for (double *p : {S->cfield, S->c0, S->cx, S->cprev, S->clast, S->Fc0[0], S->Fc0[1], S->Fc1[0], S->Fc1[1],
                       S->G2[0], S->G2[1], S->rc0, S->rc1, S->ex[0], S->ex[1], S->ex[2], S->ex[3], S->dconc,
-                      S->dsave, S->bconc, S->cscratch, S->csums})
+                      S->dsave, S->bconc, S->cscratch, S->csums, S->cstart, S->evps})
         if (p) (void)hipFree(p);
     if (S->cflags) (void)hipFree(S->cflags);
     if (S->cdmax) (void)hipFree(S->cdmax);
@@ -1556,9 +1577,9 @@ static int series_reduce(yz_step *S, double *v) {
     a.c = S->conc ? S->cfield : S->c;
     const bool ran = S->stepped;
     const int side = (ran && g.dim == 1 && S->cs.centrifugal) ? 1 : 0;
-    hipLaunchKernelGGL(k_series_cells, dim3(nbk), dim3(SER_B), 0, S->st, a, S->hold, S->conc ? S->c0 : S->c,
+    hipLaunchKernelGGL(k_series_cells, dim3(nbk), dim3(SER_B), 0, S->st, a, S->hold, S->conc ? S->cstart : S->c,
                        (ran && S->cs.evaporation) ? S->evpv : nullptr, (ran && S->cs.source) ? S->srcv : nullptr,
-                       side, S->dser);
+                       (S->conc && S->cs.src_conc_supplied) ? 1 : 0, S->cs.src_conc, side, S->dser);
     hipLaunchKernelGGL(k_series_final, dim3(1), dim3(SER_B), 0, S->st, nbk, S->dser, S->dser + SER_Q * SER_G);
     YZ_TRY(hipGetLastError());
     YZ_TRY(hipMemcpyAsync(v, S->dser + SER_Q * SER_G, SER_Q * sizeof(double), hipMemcpyDeviceToHost, S->st));
