@@ -51,6 +51,7 @@ def load_from(path):
     lib.yzo_create.restype = C.c_void_p
     lib.yzo_create.argtypes = [P(YzCase), pd, pd, pd, C.c_double]
     lib.yzo_destroy.argtypes = [C.c_void_p]
+    lib.yzo_set_lin_mode.argtypes = [C.c_void_p, C.c_int]
     lib.yzo_n.argtypes = [C.c_void_p]
     lib.yzo_threads.argtypes = [C.c_int]
     lib.yzo_set_fault.argtypes = [C.c_long, C.c_long]
@@ -132,7 +133,12 @@ def _pi(a):
 class OracleStep:
     """The oracle's time stepper over one case."""
 
-    def __init__(self, case, h, s, c, dt0):
+    # the linear solves: the reference's algebraic multigrid, or the
+    # device's structured one (the same error vectors as the device's solves
+    # up to rounding, so Newton counts compare pass for pass)
+    LIN_ALGEBRAIC, LIN_STRUCTURED = 0, 1
+
+    def __init__(self, case, h, s, c, dt0, lin_mode=0):
         self.lib = load()
         self.case = case.copy()
         h = np.ascontiguousarray(h, dtype=np.float64)
@@ -141,6 +147,7 @@ class OracleStep:
         self.p = self.lib.yzo_create(C.byref(self.case), _pd(h), _pd(s), _pd(c), dt0)
         self.n = self.lib.yzo_n(self.p)
         self.nnz = self.lib.yzo_nnz(self.p)
+        self.lib.yzo_set_lin_mode(self.p, int(lin_mode))
 
     def __del__(self):
         if getattr(self, "p", None):

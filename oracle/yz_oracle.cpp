@@ -1834,8 +1834,8 @@ struct Amg {
 };
 
 /* BiCGSTAB with right preconditioning; returns 0 / 1 / 2 as the backend */
-template <class Pc>
-static int bicgstab(const SpM &A, const vector<double> &b, vector<double> &x, Pc &M,
+template <class Mat, class Pc>
+static int bicgstab(const Mat &A, const vector<double> &b, vector<double> &x, Pc &M,
                     int maxit, double tol, int &iters, double &resid) {
     const int n = A.n;
     vector<double> r(n), rh, p(n, 0), v(n, 0), ph(n), s(n), sh(n), t(n);
@@ -1935,6 +1935,92 @@ static int linsolve(const Csr &M, const vector<double> &rhs, vector<double> &x,
     return bicgstab(A, b, x, jac, maxit, eps, iter, resid);
 }
 
+}  // namespace yzo
+#include "yz_struct_mg.h"
+namespace yzo {
+
+/* The same solve through the device's structured multigrid (yz_struct_mg.h):
+ * the padded system folded onto its interior cells (ghost rows
+ * d x_g - d x_src = r_g give x_g = x_src + r_g / d; an interior row's entries
+ * go to the slot of their offset, those of ghost columns moving r_g / d into
+ * the right-hand side), scaled by the diagonal, solved, and the ghost entries
+ * of x formed again from their rows. Returns 2 for a matrix without that
+ * shape. */
+static int linsolve_structured(const Grid &g, const Stencil &st, const Csr &M, const vector<double> &rhs,
+                               vector<double> &x, int precond, int scaling, int maxit, double eps, int &iter,
+                               double &resid) {
+    smg::Op op;
+    op.dim = g.dim; op.nx = g.nx; op.ny = g.dim == 2 ? g.ny : 1; op.N = op.nx * op.ny; op.ns = st.ns;
+    op.perx = g.bc[0] == YZ_BC_PERIODIC; op.pery = g.dim == 2 && g.bc[2] == YZ_BC_PERIODIC;
+    for (int k = 0; k < st.ns; ++k) { op.ox[k] = st.ox[k]; op.oy[k] = st.oy[k]; }
+    const int n = M.n, N = op.N;
+    vector<int> gsrc(n, -1);
+    vector<double> gofs(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const int ix = i % g.nxp, iy = i / g.nxp;
+        if (g.inside(ix, g.dim == 2 ? iy : 2)) continue;
+        double d = 0, o = 0;
+        int src = -1, m = 0;
+        for (int k = M.ptr[i]; k < M.ptr[i + 1]; ++k) {
+            const double v = M.val[k];
+            if (v == 0) continue;
+            if (M.col[k] == i) d += v;
+            else { src = M.col[k]; o += v; ++m; }
+        }
+        if (m != 1 || d == 0 || o != -d) return 2;
+        gsrc[i] = src;
+        gofs[i] = rhs[i] / d;
+    }
+    vector<double> planes((size_t)op.ns * N, 0.0), b(N), xi(N);
+    const int c0 = st.center;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad) if (N > YZO_PAR_MIN)
+    for (int i = 0; i < N; ++i) {
+        const int ix = i % op.nx, iy = i / op.nx;
+        const int r = (g.dim == 2 ? iy + 2 : 0) * g.nxp + ix + 2;
+        double acc[25] = {0};
+        double bi = rhs[r];
+        for (int k = M.ptr[r]; k < M.ptr[r + 1]; ++k) {
+            const int cc = M.col[k];
+            const double v = M.val[k];
+            const int dxo = cc % g.nxp - (ix + 2), dyo = cc / g.nxp - (g.dim == 2 ? iy + 2 : 0);
+            int slot = -1;
+            for (int q = 0; q < op.ns; ++q) if (op.ox[q] == dxo && op.oy[q] == dyo) slot = q;
+            if (slot < 0) { if (v != 0) bad = 1; continue; }
+            acc[slot] += v;
+            if (gsrc[cc] >= 0) bi -= v * gofs[cc];
+        }
+        const double d = acc[c0];
+        const bool sc = scaling && !(std::fabs(d) < 1.0e-30);
+        const double inv = sc ? 1.0 / d : 1.0;
+        for (int q = 0; q < op.ns; ++q) planes[(size_t)q * N + i] = sc ? acc[q] * inv : acc[q];
+        b[i] = sc ? bi * inv : bi;
+        xi[i] = x[r];
+    }
+    if (bad) return 2;
+    smg::Mat A;
+    A.op = op; A.val = &planes; A.n = N;
+    int ch;
+    if (precond == YZ_PC_AMG) {
+        smg::Hierarchy hier;
+        hier.build(op, planes);
+        smg::Pc pc;
+        pc.h = &hier;
+        ch = bicgstab(A, b, xi, pc, maxit, eps, iter, resid);
+    } else {
+        smg::JacobiPc jac;
+        jac.build(op, planes);
+        ch = bicgstab(A, b, xi, jac, maxit, eps, iter, resid);
+    }
+    for (int i = 0; i < N; ++i) {
+        const int ix = i % op.nx, iy = i / op.nx;
+        x[(g.dim == 2 ? iy + 2 : 0) * g.nxp + ix + 2] = xi[i];
+    }
+    for (int i = 0; i < n; ++i)
+        if (gsrc[i] >= 0) x[i] = x[gsrc[i]] + gofs[i];
+    return ch;
+}
+
 /* ------------------------------------------------------------------ */
 /* Newton-Kantorovich step (NewtonKantorovich.cpp, ImplicitSolver.cpp)  */
 /* ------------------------------------------------------------------ */
@@ -1957,6 +2043,14 @@ struct Stepper {
     /* the passes of the last attempt: correction, tolerance, residual */
     vector<double> plog_corr, plog_eta, plog_lres;
     double last_eps = 0;
+    /* the linear solves: 0 the reference's algebraic multigrid (Amg),
+     * 1 the device's structured one (linsolve_structured) */
+    int lin_mode = 0;
+    int solve_sys(const Csr &M, const vector<double> &rhs, vector<double> &x, int precond, int scaling,
+                  int maxit, double eps, int &it, double &r) const {
+        if (lin_mode == 1) return linsolve_structured(eq.g, eq.st, M, rhs, x, precond, scaling, maxit, eps, it, r);
+        return linsolve(M, rhs, x, precond, scaling, maxit, eps, it, r);
+    }
 
     /* Scripted Newton control (test hook). In place of the assembly and
      * the linear solve, each pass takes its correction (every row alike),
@@ -2397,13 +2491,13 @@ struct Stepper {
         const int maxit = eq.cs.conc_lin_maxit;
         int it = 0;
         double r = 0;
-        int ch = linsolve(CM, rhsD, dvec, YZ_PC_JACOBI, eq.cs.conc_row_scaling, std::min(maxit, 300),
+        int ch = solve_sys(CM, rhsD, dvec, YZ_PC_JACOBI, eq.cs.conc_row_scaling, std::min(maxit, 300),
                           eq.cs.conc_lin_eps, it, r);
         cstat[0] += 1;
         if (ch != 0) {
             const int it_try = it;
             dvec = x0;
-            ch = linsolve(CM, rhsD, dvec, YZ_PC_AMG, eq.cs.conc_row_scaling, maxit, eq.cs.conc_lin_eps, it, r);
+            ch = solve_sys(CM, rhsD, dvec, YZ_PC_AMG, eq.cs.conc_row_scaling, maxit, eq.cs.conc_lin_eps, it, r);
             it += it_try;
             cstat[2] += 1;
         }
@@ -2653,7 +2747,7 @@ struct Stepper {
             std::fill(corr.begin(), corr.end(), 0.0);
             return 2;
         }
-        const int ch = linsolve(eq.IA, Rhs, corr, eq.cs.precond == YZ_PC_JACOBI ? YZ_PC_JACOBI : YZ_PC_AMG,
+        const int ch = solve_sys(eq.IA, Rhs, corr, eq.cs.precond == YZ_PC_JACOBI ? YZ_PC_JACOBI : YZ_PC_AMG,
                                 eq.cs.row_scaling, eq.cs.lin_maxit, eps, iter, resid);
         if (ch == 0) satisfy_bc(corr);
         if (ch != 0) std::fill(corr.begin(), corr.end(), 0.0);
@@ -3003,6 +3097,9 @@ int yzo_pass_log(void *p, double *corr, double *eta, double *lres, int cap) {
     return m;
 }
 void yzo_destroy(void *p) { delete static_cast<yzo::Stepper *>(p); }
+/* the stepper's linear solves: 0 the reference's algebraic multigrid,
+ * 1 the device's structured multigrid */
+void yzo_set_lin_mode(void *p, int mode) { static_cast<yzo::Stepper *>(p)->lin_mode = mode; }
 int yzo_n(void *p) { return static_cast<yzo::Stepper *>(p)->eq.g.n; }
 int yzo_advance(void *p, int *nk, int *lin, double *resid) {
     auto *st = static_cast<yzo::Stepper *>(p);

@@ -3,6 +3,11 @@ spinodal dewetting film at its full size (224^2 periodic cells, side 0.64,
 capillary + disjoining, adaptive dt from 1e-5, 2000 steps;
 yuzen_amd.cases.spinodal_2d).
 
+The oracle's linear solves run the device's structured multigrid
+(OracleStep lin_mode 1), so that the device's Newton counts can be asserted
+equal at every step: two multigrids leave different errors at the
+Eisenstat-Walker tolerance, and a pass's correction is that error.
+
 Stored per step: the return code, the Newton passes, the linear
 iterations, the clock and the step in force after it, and the accepted
 attempt's pass log (the correction the exit test took and the linear
@@ -35,11 +40,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--out", default=OUT)
+    ap.add_argument("--lin-mode", type=int, default=1, help="0 the reference's algebraic multigrid, 1 the device's")
     args = ap.parse_args()
     from yuzen_amd import cases
     from oracle import binding as ob
     c, h, s, cc, dt, _ = cases.spinodal_2d(n=224)
-    o = ob.OracleStep(c, h, s, cc, dt)
+    o = ob.OracleStep(c, h, s, cc, dt, lin_mode=args.lin_mode)
     n = args.steps
     rc = np.zeros(n, np.int32)
     nk = np.zeros(n, np.int32)
@@ -65,7 +71,7 @@ def main():
             print(f"step {k}: nk {nk[k]} lin {lin[k]} t {clock[k, 0]:.4e} dt {clock[k, 1]:.3e} "
                   f"({time.time() - t0:.0f} s)", flush=True)
     np.savez_compressed(args.out, rc=rc, nk=nk, lin=lin, clock=clock, pass_corr=pcorr, pass_eta=peta,
-                        **films)
+                        lin_mode=np.int32(args.lin_mode), **films)
     print(f"wrote {args.out}")
 
 

@@ -78,38 +78,3 @@ def interior(case, v):
     a = np.asarray(v).reshape((ny + 4) if dim == 2 else 1, nxp)
     return a[2:-2, 2:-2].ravel() if dim == 2 else a[0, 2:-2].copy()
 
-
-def borderline_exit(case, log_a, log_b, band=10.0):
-    """Why two runs of one step took Newton counts one apart, checked pass
-    by pass on their pass logs (corrections the exit test took). Returns a
-    description; raises AssertionError unless the difference is a
-    borderline exit:
-      * the counts differ by one;
-      * up to the shorter run's last pass the corrections agree to a
-        factor 2 (the same Newton trajectory, up to the linear solves'
-        error);
-      * at that pass the shorter run's correction fell below its exit
-        threshold (NewtonEpsilon, or NewtonEpsilonRel x its first
-        correction) and the longer run's did not, both within `band` of
-        the threshold: the two corrections straddle it, where the error an
-        Eisenstat-Walker-truncated solve leaves (relative residual up to
-        1e-4) decides the side."""
-    a, b = np.asarray(log_a), np.asarray(log_b)
-    assert abs(len(a) - len(b)) == 1, f"counts {len(a)} / {len(b)}"
-    short, long_ = (a, b) if len(a) < len(b) else (b, a)
-    k = len(short) - 1
-    for j in range(k):
-        r = short[j] / long_[j]
-        assert 0.5 <= r <= 2.0, f"pass {j}: corrections {short[j]:.3e} / {long_[j]:.3e}"
-
-    def thr(log):
-        t = case.nk_eps
-        if case.nk_eps_rel > 0:
-            t = max(t, case.nk_eps_rel * log[0])
-        return t
-    ts, tl = thr(short), thr(long_)
-    assert short[k] < ts, f"pass {k}: the shorter run's correction {short[k]:.3e} is not below {ts:.3e}"
-    assert long_[k] >= tl, f"pass {k}: the longer run's correction {long_[k]:.3e} is below {tl:.3e}"
-    assert short[k] >= ts / band and long_[k] <= tl * band, \
-        f"pass {k}: corrections {short[k]:.3e} / {long_[k]:.3e} not within {band} of {ts:.3e} / {tl:.3e}"
-    return f"pass {k + 1}: {short[k]:.3e} < {ts:.3e} <= {long_[k]:.3e} (threshold {tl:.3e})"

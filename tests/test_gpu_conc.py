@@ -83,7 +83,7 @@ def test_face_fluxes_match_reference_and_oracle():
 
 
 def _pair(c, h, s, cc, dt, steps, label, bar=1e-10, counts=True):
-    o = ob.OracleStep(c, h, s, cc, dt)
+    o = ob.OracleStep(c, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
     g = HipStep(c, h, s, cc, dt)
     worst_h = worst_c = 0.0
     moved = 0.0

@@ -83,7 +83,7 @@ def test_assembly_with_b_matches_oracle(dim):
 
 
 def _run_pair(c, h, s, cc, dt, steps, label, tol=1e-10, counts=True):
-    o = ob.OracleStep(c, h, s, cc, dt)
+    o = ob.OracleStep(c, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
     g = HipStep(c, h, s, cc, dt)
     for k in range(steps):
         print(f"{label} step {k}", flush=True)

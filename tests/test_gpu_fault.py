@@ -29,7 +29,7 @@ pytestmark = pytest.mark.gpu
 def _both(c, h, s, cc, dt, plan):
     """advance the oracle and the device through plan: a list of (fault k,
     m) before each step (None: no fault); the results per step"""
-    o = ob.OracleStep(c, h, s, cc, dt)
+    o = ob.OracleStep(c, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
     g = HipStep(c, h, s, cc, dt)
     out = []
     try:
@@ -89,7 +89,7 @@ def test_breakdown_of_the_learned_fp32_solve():
         print(f"learned, fault at the first solve: {r[:3]} {g.clock()}", flush=True)
         assert r[0] == 1 and r[1] == 1 and g.clock()[0] == 0.0
         assert g.advance()[0] == 0
-        o = ob.OracleStep(c, h, s, cc, dt)
+        o = ob.OracleStep(c, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
         assert o.advance()[0] == 0
         ho = o.h()
         assert np.abs(g.h() - ho).max() <= 1e-8 * np.abs(ho).max()

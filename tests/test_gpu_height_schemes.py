@@ -93,7 +93,7 @@ def test_scheme_steps(scheme, learned):
     if learned:
         c.precond, c.krylov_fp32 = capi.PC_LEARNED, 1
         c.learn_passes, c.learn_iters, c.learn_probes, c.learn_lr = 2, 40, 4, 2e-3
-    o = ob.OracleStep(c, h, s, cc, dt)
+    o = ob.OracleStep(c, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
     g = HipStep(c, h, s, cc, dt)
     for k in range(4):
         ro, rg = o.advance(), g.advance()

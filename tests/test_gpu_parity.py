@@ -62,7 +62,7 @@ def test_assembly_periodic(dim):
 
 
 def _run_pair(c, h, s, cc, dt, steps, label):
-    o = ob.OracleStep(c, h, s, cc, dt)
+    o = ob.OracleStep(c, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
     g = HipStep(c, h, s, cc, dt)
     worst = 0.0
     for k in range(steps):

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _pair_steps(c, h, s, cc, dt, t, steps, label):
-    o = ob.OracleStep(c, h, s, cc, dt)
+    o = ob.OracleStep(c, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
     g = HipStep(c, h, s, cc, dt)
     g.set_term(t)
     got = []
@@ -76,7 +76,7 @@ def test_negative_film():
 def test_run_under_step_control():
     c, h, s, cc, dt, _ = cases.spinodal_2d(n=32)
     steps = 12
-    o = ob.OracleStep(c, h, s, cc, dt)
+    o = ob.OracleStep(c, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
     for _ in range(steps):
         assert o.advance()[0] == 0
     g = HipStep(c, h, s, cc, dt)

@@ -714,6 +714,191 @@ __global__ void __launch_bounds__(BLK) k_post(StencilOp op, const T *__restrict_
     y[i] = x[i] + m[i] * (b[i] - ax);
 }
 
+// The coarse levels of a 2-D hierarchy in two kernels per visit instead of
+// four. A level of a few thousand cells costs each kernel its launch ramp
+// and a memory round trip or two whatever its size, so the down-sweep's
+// smoothing, residual and restriction run as one kernel per tile of coarse
+// cells and the up-sweep's prolongation and smoothing as one per tile of
+// fine cells, each recomputing the few halo values it needs from its
+// neighbours' inputs. Every value is formed by the same arithmetic, in the
+// same order, as the separate kernels (k_pre, k_restrict, k_prolong,
+// k_post), so the cycle's output is the same to the bit.
+//
+// Down-sweep of a TC x TC tile of coarse cells: x = M b on the fine window
+// of the tile's restriction plus the stencil's 2-cell reach (staged in LDS
+// at unfolded positions, each holding the folded cell's value, so the
+// stencil reads them without folding), the residual r = b - A x on the
+// restriction's window (3 TC + 6 fine cells per axis; zero off a
+// non-periodic grid), b_c = R r, and x written for the tile's own fine cells
+// (the up-sweep smooths from it). On these levels a kernel's time is its
+// chain of memory round trips, so each thread issues all the loads of a
+// phase (K1 window values, G2 residual rows at a time) before it uses any.
+template <int NS, typename T, int TC, int NT>
+__global__ void __launch_bounds__(NT) k_down2(StencilOp op, const T *__restrict__ val, const T *__restrict__ m,
+                                              const T *__restrict__ b, T *__restrict__ x,
+                                              const T *__restrict__ R, StencilOp cop, T *__restrict__ bc) {
+    constexpr int FW = 3 * TC + 6, XW = FW + 4;
+    constexpr int K1 = (XW * XW + NT - 1) / NT, K2 = (FW * FW + NT - 1) / NT, G2 = 2;
+    // values as the separate kernels store them: x and r in T
+    __shared__ T xs[XW * XW];
+    __shared__ double rs[FW * FW];
+    const int X0 = blockIdx.x * TC, Y0 = blockIdx.y * TC;
+    const int fx0 = 3 * X0 - 3, fy0 = 3 * Y0 - 3;
+    {
+        T mv[K1], bv[K1];
+#pragma unroll
+        for (int q = 0; q < K1; ++q) {
+            const int t = threadIdx.x + q * NT;
+            mv[q] = 0; bv[q] = 0;
+            if (t < XW * XW) {
+                const int gx = fold_axis(fx0 - 2 + t % XW, op.nx, op.perx);
+                const int gy = fold_axis(fy0 - 2 + t / XW, op.ny, op.pery);
+                const int c = gy * op.nx + gx;
+                mv[q] = m[c]; bv[q] = b[c];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < K1; ++q) {
+            const int t = threadIdx.x + q * NT;
+            if (t < XW * XW) xs[t] = mv[q] * bv[q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q0 = 0; q0 < K2; q0 += G2) {
+        T av[G2][NS], bi[G2];
+        int ii[G2];
+#pragma unroll
+        for (int g = 0; g < G2; ++g) {
+            const int t = threadIdx.x + (q0 + g) * NT;
+            ii[g] = -1;
+            if (q0 + g < K2 && t < FW * FW) {
+                int gx = fx0 + t % FW, gy = fy0 + t / FW;
+                bool ok = true;
+                if (op.perx) gx = ((gx % op.nx) + op.nx) % op.nx;
+                else ok = ok && gx >= 0 && gx < op.nx;
+                if (op.pery) gy = ((gy % op.ny) + op.ny) % op.ny;
+                else ok = ok && gy >= 0 && gy < op.ny;
+                if (ok) {
+                    const int i = gy * op.nx + gx;
+                    ii[g] = i;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) av[g][k] = val[(size_t)k * op.N + i];
+                    bi[g] = b[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G2; ++g) {
+            const int t = threadIdx.x + (q0 + g) * NT;
+            if (q0 + g < K2 && t < FW * FW) {
+                const int lx = t % FW, ly = t / FW;
+                double rv = 0.0;
+                if (ii[g] >= 0) {
+                    double s = 0;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) s += av[g][k] * xs[(ly + 2 + op.oy[k]) * XW + lx + 2 + op.ox[k]];
+                    rv = (T)(bi[g] - s);
+                }
+                rs[t] = rv;
+            }
+        }
+    }
+    for (int t = threadIdx.x; t < 9 * TC * TC; t += NT) {
+        const int ux = t % (3 * TC), uy = t / (3 * TC);
+        const int gx = 3 * X0 + ux, gy = 3 * Y0 + uy;
+        if (gx < op.nx && gy < op.ny) x[gy * op.nx + gx] = xs[(uy + 5) * XW + ux + 5];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < TC * TC; t += NT) {
+        const int tx = t % TC, ty = t / TC;
+        const int Ix = X0 + tx, Iy = Y0 + ty;
+        if (Ix >= cop.nx || Iy >= cop.ny) continue;
+        const int I = Iy * cop.nx + Ix;
+        double s = 0;
+#pragma unroll
+        for (int v = 0; v < 9; ++v)
+#pragma unroll
+            for (int u = 0; u < 9; ++u) s += R[(size_t)(v * 9 + u) * cop.N + I] * rs[(3 * ty + v) * FW + 3 * tx + u];
+        bc[I] = s;
+    }
+}
+
+// Up-sweep of a TF x TF tile of fine cells: x' = x + P x_c on the tile and
+// its 2-cell halo (staged in LDS as above, every load of a thread's window
+// values issued before their sums), then y = x' + M (b - A x') for the
+// tile's cells.
+template <int NS, typename T, int TF, int NT>
+__global__ void __launch_bounds__(NT) k_up2(StencilOp op, const T *__restrict__ val, const T *__restrict__ m,
+                                            const T *__restrict__ b, const T *__restrict__ x,
+                                            const T *__restrict__ P, StencilOp cop, const T *__restrict__ xc,
+                                            T *__restrict__ y) {
+    constexpr int W = TF + 4;
+    constexpr int K1 = (W * W + NT - 1) / NT;
+    __shared__ T xs[W * W];   // x' as k_prolong stores it, in T
+    const int X0 = blockIdx.x * TF, Y0 = blockIdx.y * TF;
+    {
+        T pv[K1][9], cv[K1][9], xv[K1];
+#pragma unroll
+        for (int q = 0; q < K1; ++q) {
+            const int t = threadIdx.x + q * NT;
+            xv[q] = 0;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) { pv[q][e] = 0; cv[q][e] = 0; }
+            if (t < W * W) {
+                const int gx = fold_axis(X0 - 2 + t % W, op.nx, op.perx);
+                const int gy = fold_axis(Y0 - 2 + t / W, op.ny, op.pery);
+                const int i = gy * op.nx + gx;
+                const int Ix = gx / 3, Iy = gy / 3;
+                xv[q] = x[i];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    int Jx = Ix + e % 3 - 1, Jy = Iy + e / 3 - 1;
+                    bool ok = true;
+                    if (cop.perx) Jx = (Jx + cop.nx) % cop.nx;
+                    else ok = ok && Jx >= 0 && Jx < cop.nx;
+                    if (cop.pery) Jy = (Jy + cop.ny) % cop.ny;
+                    else ok = ok && Jy >= 0 && Jy < cop.ny;
+                    if (ok) {
+                        pv[q][e] = P[(size_t)e * op.N + i];
+                        cv[q][e] = xc[Jy * cop.nx + Jx];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < K1; ++q) {
+            const int t = threadIdx.x + q * NT;
+            if (t < W * W) {
+                // the coarse neighbours off a non-periodic grid hold zeros,
+                // and P x_c sums the others in k_prolong's order
+                double s = 0;
+                const int gx = fold_axis(X0 - 2 + t % W, op.nx, op.perx);
+                const int gy = fold_axis(Y0 - 2 + t / W, op.ny, op.pery);
+                const int Ix = gx / 3, Iy = gy / 3;
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    const int Jx = Ix + e % 3 - 1, Jy = Iy + e / 3 - 1;
+                    const bool in = (cop.perx || (Jx >= 0 && Jx < cop.nx)) && (cop.pery || (Jy >= 0 && Jy < cop.ny));
+                    if (in) s += pv[q][e] * cv[q][e];
+                }
+                xs[t] = (T)(xv[q] + s);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x >= TF * TF) return;
+    const int tx = threadIdx.x % TF, ty = threadIdx.x / TF;
+    const int ix = X0 + tx, iy = Y0 + ty;
+    if (ix >= op.nx || iy >= op.ny) return;
+    const int i = iy * op.nx + ix;
+    double ax = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) ax += val[(size_t)k * op.N + i] * xs[(ty + 2 + op.oy[k]) * W + tx + 2 + op.ox[k]];
+    const double xi = xs[(ty + 2) * W + tx + 2];
+    y[i] = xi + m[i] * (b[i] - ax);
+}
+
 // x = A^-1 b with the inverse stored transposed (coalesced over i): 64 rows
 // per block, the block's DMV_W waves each take every DMV_W-th column, and
 // their partial sums are added in wave order (a fixed order, so the result
@@ -972,6 +1157,15 @@ int jacobi_setup(SolverT<T> &S) {
     return 0;
 }
 
+// whether level l of the cycle runs as the two fused kernels (k_down2,
+// k_up2): the coarse levels of a 2-D hierarchy. The finest level keeps the
+// separate kernels: its halo recomputation would re-read about a quarter
+// more of the largest operator for one launch saved.
+template <typename T>
+static inline bool fused_level(const AmgLevel<T> &L, int l) {
+    return l >= 1 && L.op.dim == 2;
+}
+
 // one V-cycle: out = M^-1 in; with smoother != null the finest level is
 // smoothed with that explicit stencil instead of SPAI0
 template <typename T>
@@ -988,6 +1182,19 @@ static void vcycle(SolverT<T> &S, const T *in, T *out, const T *smoother) {
     for (int l = 0; l + 1 < nl; ++l) {
         AmgLevel<T> &L = g.lv[l];
         AmgLevel<T> &C = g.lv[l + 1];
+        if (fused_level(L, l)) {
+            dispatch_ns(L.op.ns, [&](auto c) {
+                constexpr int NS = decltype(c)::value;
+                if (C.op.N < 128 * 128)
+                    hipLaunchKernelGGL((k_down2<NS, T, 8, 512>), dim3((C.op.nx + 7) / 8, (C.op.ny + 7) / 8), dim3(512),
+                                       0, st, L.op, L.val, L.minv, b, L.x, L.R, C.op, C.b);
+                else
+                    hipLaunchKernelGGL((k_down2<NS, T, 16, 512>), dim3((C.op.nx + 15) / 16, (C.op.ny + 15) / 16),
+                                       dim3(512), 0, st, L.op, L.val, L.minv, b, L.x, L.R, C.op, C.b);
+            });
+            b = C.b;
+            continue;
+        }
         if (l == 0 && smoother) {
             launch_spmv(L.op, smoother, b, L.x, st);
             dispatch_ns(L.op.ns, [&](auto c) {
@@ -1019,8 +1226,21 @@ static void vcycle(SolverT<T> &S, const T *in, T *out, const T *smoother) {
         AmgLevel<T> &L = g.lv[l];
         AmgLevel<T> &C = g.lv[l + 1];
         const T *bl = (l == 0) ? in : L.b;
-        hipLaunchKernelGGL((k_prolong<T>), dim3(lvl_grid(L.op.N)), dim3(lvl_blk(L.op.N)), 0, st, L.op, L.P, C.op, C.x, L.x);
         T *dst = (l == 0) ? out : L.r;   // r is free here
+        if (fused_level(L, l)) {
+            dispatch_ns(L.op.ns, [&](auto c) {
+                constexpr int NS = decltype(c)::value;
+                if (L.op.N < 128 * 128)
+                    hipLaunchKernelGGL((k_up2<NS, T, 8, 256>), dim3((L.op.nx + 7) / 8, (L.op.ny + 7) / 8), dim3(256),
+                                       0, st, L.op, L.val, L.minv, bl, L.x, L.P, C.op, C.x, dst);
+                else
+                    hipLaunchKernelGGL((k_up2<NS, T, 16, 256>), dim3((L.op.nx + 15) / 16, (L.op.ny + 15) / 16),
+                                       dim3(256), 0, st, L.op, L.val, L.minv, bl, L.x, L.P, C.op, C.x, dst);
+            });
+            if (l > 0) std::swap(L.x, L.r);
+            continue;
+        }
+        hipLaunchKernelGGL((k_prolong<T>), dim3(lvl_grid(L.op.N)), dim3(lvl_blk(L.op.N)), 0, st, L.op, L.P, C.op, C.x, L.x);
         if (l == 0 && smoother) {
             dispatch_ns(L.op.ns, [&](auto c) {
                 constexpr int NS = decltype(c)::value;
