@@ -178,10 +178,13 @@ enum { YZ_HSCHEME_KOREN = 0, YZ_HSCHEME_VANLEER = 1, YZ_HSCHEME_UPWIND1 = 2, YZ_
        YZ_HSCHEME_CENTRAL4 = 4, YZ_HSCHEME_UPWIND2 = 5, YZ_HSCHEME_UTOPIA = 6, YZ_HSCHEME_QUICK = 7,
        YZ_HSCHEME_KK = 8 };
 /* Face interpolation of the concentration's advection (reference
- * FiniteDifferences.cpp:22-93, keys :214-228); the unlimited schemes.
- * UTOPIA is the reference's default (Input.cpp:2302-2303). */
+ * FiniteDifferences.cpp:22-93, keys :214-228): the unlimited schemes, then
+ * the limited Koren and van Leer (their limiter frozen at the latest
+ * iterate, FaceWeights :166-190; the step completed by solves repeated until
+ * that settles, HeightEquationBase.h:1506-1525). UTOPIA is the reference's
+ * default (Input.cpp:2302-2303). */
 enum { YZ_CONV_UPWIND1 = 0, YZ_CONV_CENTRAL2 = 1, YZ_CONV_CENTRAL4 = 2, YZ_CONV_UPWIND2 = 3,
-       YZ_CONV_UTOPIA = 4, YZ_CONV_QUICK = 5, YZ_CONV_KK = 6 };
+       YZ_CONV_UTOPIA = 4, YZ_CONV_QUICK = 5, YZ_CONV_KK = 6, YZ_CONV_KOREN = 7, YZ_CONV_VANLEER = 8 };
 
 /* Fill c with the reference's defaults (Input.cpp getin defaults):
  * theta 0.5, NewtonMaxIterations 30, NewtonMinIterations 1,

@@ -74,7 +74,9 @@ def load_from(path):
     lib.yzo_get_c.argtypes = [C.c_void_p, pd]
     lib.yzo_face_fluxes.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_conc_stats.argtypes = [C.c_void_p, pd]
-    lib.yzo_conc_face.argtypes = [C.c_int, pd, C.c_double, pd]
+    lib.yzo_conc_face.argtypes = [C.c_int, pd, C.c_double, C.c_int, pd]
+    lib.yzo_conc_settle.argtypes = [C.c_void_p]
+    lib.yzo_conc_settle.restype = C.c_longlong
     lib.yzo_conc_nnz.argtypes = [C.c_void_p]
     lib.yzo_conc_system.argtypes = [C.c_void_p, pi_, pi_, pd, pd]
     lib.yzo_cz_case_defaults.argtypes = [P(YzCzCase)]
@@ -114,11 +116,12 @@ def limiter(f, sign, vanleer=0):
     return out
 
 
-def conc_face(scheme, f, sign):
-    """FaceValue and FaceWeights of an unlimited scheme: [value, w0..w3]"""
+def conc_face(scheme, f, sign, rowpos=-1):
+    """FaceValue and FaceWeights (rowpos: the row's cell among LL 0, L 1, R 2,
+    RR 3; a limited scheme's weights depend on it) of a concentration scheme"""
     f = np.ascontiguousarray(f, dtype=np.float64)
-    out = np.zeros(5)
-    load().yzo_conc_face(scheme, _pd(f), sign, _pd(out))
+    out = np.empty(5)
+    load().yzo_conc_face(scheme, _pd(f), sign, int(rowpos), _pd(out))
     return out
 
 
@@ -138,8 +141,8 @@ class OracleStep:
     # up to rounding, so Newton counts compare pass for pass)
     LIN_ALGEBRAIC, LIN_STRUCTURED = 0, 1
 
-    def __init__(self, case, h, s, c, dt0, lin_mode=0):
-        self.lib = load()
+    def __init__(self, case, h, s, c, dt0, lin_mode=0, lib=None):
+        self.lib = lib or load()
         self.case = case.copy()
         h = np.ascontiguousarray(h, dtype=np.float64)
         s = np.ascontiguousarray(s, dtype=np.float64)
@@ -217,6 +220,11 @@ class OracleStep:
         val, rhs = np.empty(nnz), np.empty(self.n)
         self.lib.yzo_conc_system(self.p, _pi(rp), _pi(col), _pd(val), _pd(rhs))
         return rp, col, val, rhs
+
+    def conc_settle(self):
+        """the solves that completed the steps of a limited concentration
+        scheme, over the run"""
+        return int(self.lib.yzo_conc_settle(self.p))
 
     def conc_stats(self):
         out = np.zeros(5)

@@ -21,7 +21,8 @@
 //     lengths; the Koren and van Leer limiters' face value, weights and
 //     derivative; the upwinded terms with either limiter;
 //     NonDimensionalNumbers::Dp over contact angles; FaceValue and
-//     FaceWeights of the unlimited face schemes;
+//     FaceWeights of the unlimited face schemes, and of the limited ones
+//     with the row's cell upwind and downwind;
 //   * a restart state file written by the reference's C_RestartState;
 //   * the evaporation rate of C_Evaporation and the nozzle's supply of
 //     Source_WM2D / Source_WM1D.
@@ -429,6 +430,43 @@ void dump_limiter(const std::string &dir, bool vanleer) {
     write_vec(fo, out);
 }
 
+void dump_limited_weights(const std::string &dir) {
+    // the limited schemes' FaceValue and FaceWeights with the row's cell at
+    // L (1) and at R (2) (FiniteDifferences.cpp:153-190: the upwind and the
+    // downwind form), over four-point stencils with zero differences and
+    // both flow signs
+    std::vector<double> out;
+    unsigned long long z = 4242;
+    auto rnd = [&]() {
+        z = z * 6364136223846793005ULL + 1442695040888963407ULL;
+        return ((z >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0;
+    };
+    for (int vl = 0; vl < 2; ++vl) {
+        C_Interpolation_base4 k;
+        k.Build(vl ? C_Interpolation_base4::VanLeer : C_Interpolation_base4::Koren);
+        for (int t = 0; t < 240; ++t) {
+            double f[4], w[4];
+            for (int j = 0; j < 4; ++j) f[j] = rnd();
+            if (t % 5 == 0) f[1] = f[0];            // a zero upwind difference
+            if (t % 7 == 0) f[2] = f[1];            // a zero downwind difference
+            if (t % 13 == 0) f[3] = f[2];
+            if (t % 11 == 0) f[1] = 0.5 * (f[0] + f[2]);
+            const double sign = (t % 2 == 0) ? 1.0 : -1.0;
+            const int rowpos = (t / 2) % 2 == 0 ? 1 : 2;
+            const double v = k.FaceValue(f, sign);
+            k.FaceWeights(f, sign, rowpos, w);
+            out.push_back(vl); out.push_back(sign); out.push_back(rowpos);
+            for (int j = 0; j < 4; ++j) out.push_back(f[j]);
+            out.push_back(v);
+            for (int j = 0; j < 4; ++j) out.push_back(w[j]);
+        }
+    }
+    std::ofstream fo(dir + "/limited_weights.bin", std::ios::binary);
+    const int hdr[2] = {(int)out.size() / 12, 12};
+    fo.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    write_vec(fo, out);
+}
+
 void dump_schemes(const std::string &dir) {
     // the unlimited face schemes of the concentration's advection
     // (FiniteDifferences.cpp:22-93, 152-170): FaceValue and FaceWeights
@@ -796,6 +834,7 @@ int main(int argc, char **argv) {
     dump_limiter(dir, true);
     dump_dp(dir);
     dump_schemes(dir);
+    dump_limited_weights(dir);
     dump_state(dir);
     dump_evap_source(dir);
     dump_neutral(dir);

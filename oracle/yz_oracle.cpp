@@ -339,14 +339,26 @@ struct Limiter {
         return f[u] + 0.5 * phi(f[u] - f[uu], f[d] - f[u]);
     }
     /* downwind form (rowPos = -1 in FaceWeights) */
-    void weights(const double f[4], double sign, double w[4]) const {
+    void weights(const double f[4], double sign, double w[4]) const { weights_at(f, sign, -1, w); }
+    /* FaceWeights (FiniteDifferences.cpp:166-190): the face value as weights
+     * of the four points, frozen at f; the row's own cell at position rowpos
+     * (LL 0, L 1, R 2, RR 3). Upwind cell of the row: f_U + chi/2 (f_U - f_UU),
+     * chi = phi / (f_U - f_UU); otherwise f_U + psi/2 (f_D - f_U),
+     * psi = phi / (f_D - f_U). Either keeps the frozen matrix an M-matrix. */
+    void weights_at(const double f[4], double sign, int rowpos, double w[4]) const {
         int uu, u, d; pos(sign, uu, u, d);
         const double a = f[u] - f[uu], b = f[d] - f[u];
         const double p = phi(a, b);
         for (int j = 0; j < 4; ++j) w[j] = 0;
-        const double psi = (b != 0) ? p / b : 0.0;
-        w[u] = 1.0 - 0.5 * psi;
-        w[d] = 0.5 * psi;
+        if (rowpos == u) {
+            const double chi = (a != 0) ? p / a : 0.0;
+            w[u] = 1.0 + 0.5 * chi;
+            w[uu] = -0.5 * chi;
+        } else {
+            const double psi = (b != 0) ? p / b : 0.0;
+            w[u] = 1.0 - 0.5 * psi;
+            w[d] = 0.5 * psi;
+        }
     }
     double derivative(const double f[4], double sign, double g[4]) const {
         int uu, u, d; pos(sign, uu, u, d);
@@ -2083,6 +2095,9 @@ struct Stepper {
     vector<double> F0[2], F1[2], G2[2], r0, r1;
     Csr CM;                            /* the concentration system (wide stencil, ghost rows) */
     double cw1[4] = {0, 0, 0, 0}, cw2[4] = {0, 0, 0, 0};   /* scheme c1, c2 */
+    bool conc_lim = false;             /* a limited scheme (Koren, van Leer) */
+    Limiter clim;
+    long long conc_settle = 0;         /* the completing solves of a limited scheme, over the run */
     double coupled = 0;
     double cstat[5] = {0, 0, 0, 0, 0};
 
@@ -2297,13 +2312,19 @@ struct Stepper {
     /* the scheme's coefficients (FiniteDifferences.cpp:22-93): c1 and c2
      * of LL L R RR, over the sum of c1 */
     void conc_scheme() {
-        unlimited_coef(eq.cs.conc_scheme, cw1, cw2);
+        conc_lim = eq.cs.conc_scheme == YZ_CONV_KOREN || eq.cs.conc_scheme == YZ_CONV_VANLEER;
+        clim.vanleer = eq.cs.conc_scheme == YZ_CONV_VANLEER;
+        if (!conc_lim) unlimited_coef(eq.cs.conc_scheme, cw1, cw2);
     }
-    /* FaceWeights / FaceValue of an unlimited scheme (FiniteDifferences.cpp:152-170) */
-    void conc_weights(double sign, double w[4]) const {
+    /* FaceWeights / FaceValue (FiniteDifferences.cpp:152-190): the fixed
+     * coefficients of an unlimited scheme, or a limited one's weights frozen
+     * at f with the row's cell at rowpos */
+    void conc_weights(const double f[4], double sign, int rowpos, double w[4]) const {
+        if (conc_lim) { clim.weights_at(f, sign, rowpos, w); return; }
         for (int j = 0; j < 4; ++j) w[j] = cw1[j] + Sign(sign) * cw2[j];
     }
     double conc_face(const double f[4], double sign) const {
+        if (conc_lim) return clim.value(f, sign);
         double v = 0;
         for (int j = 0; j < 4; ++j) v += (cw1[j] + Sign(sign) * cw2[j]) * f[j];
         return v;
@@ -2453,14 +2474,19 @@ struct Stepper {
                 const int lo = high ? i : ia;
                 const double f1v = F1[a][lo], f0v = F0[a][lo];
                 if (eq.cs.conc_convection) {
-                    double f0[4], w1[4], w0[4];
+                    /* a limited scheme's weights frozen at the latest iterate
+                     * (LimiterC, ConcEquationBase.h:157-161, 336-355) */
+                    double f0[4], fl[4], w1[4], w0[4];
                     int col[4];
                     for (int q = 0; q < 4; ++q) {
                         col[q] = eq.st.face[d][q];
                         f0[q] = c0[eq.nb(i, col[q])];
+                        fl[q] = c_iter[eq.nb(i, col[q])];
                     }
-                    conc_weights(f1v, w1);
-                    conc_weights(f0v, w0);
+                    /* the row's cell is L of its high face, R of its low one */
+                    const int rowpos = high ? 1 : 2;
+                    conc_weights(fl, f1v, rowpos, w1);
+                    conc_weights(f0, f0v, rowpos, w0);
                     double e1 = 0, e0 = 0;
                     for (int q = 0; q < 4; ++q) {
                         M[col[q]] += th * dt * sg * f1v * w1[q];
@@ -2711,9 +2737,19 @@ struct Stepper {
         const bool noFilm = film_not_positive(phi_guess);
         const vector<double> lastPass = cx;
         bool solved = true;
-        if (!noFilm) {
+        /* a limited scheme freezes its limiter at the last iterate: the
+         * completing solve is repeated, frozen at its own answer, until that
+         * settles (HeightEquationBase.h:1506-1525) */
+        const bool limited = eq.cs.conc_convection && conc_lim;
+        for (int k = 0; k < (limited ? 8 : 1) && !noFilm; ++k) {
             int it = 0; double r = 0;
-            if (conc_solve(true, it, r) != 0) solved = false;
+            if (conc_solve(true, it, r) != 0) { solved = false; break; }
+            if (!limited) break;
+            double dmax = 0;
+            for (size_t i = 0; i < cx.size(); ++i) dmax = std::max(dmax, std::fabs(cx[i] - c_iter[i]));
+            c_iter = cx;
+            ++conc_settle;
+            if (dmax <= 1e-12) break;
         }
         if (!solved) cx = lastPass;
         redistribute(cx);
@@ -3148,12 +3184,12 @@ void yzo_conc_system(void *p, int *rowptr, int *col, double *val, double *rhs) {
     std::copy(st->rhsD.begin(), st->rhsD.end(), rhs);
 }
 /* an unlimited face scheme (conc_scheme): out = FaceValue, FaceWeights[4] */
-void yzo_conc_face(int scheme, const double *f, double sign, double *out) {
+void yzo_conc_face(int scheme, const double *f, double sign, int rowpos, double *out) {
     yzo::Stepper st;
     st.eq.cs.conc_scheme = scheme;
     st.conc_scheme();
     out[0] = st.conc_face(f, sign);
-    st.conc_weights(sign, out + 1);
+    st.conc_weights(f, sign, rowpos, out + 1);
 }
 /* [0] concentration solves, [1] their iterations, [2] solves that needed
  * the multigrid, [3] solute moved above 1, [4] below 0 (last step) */
@@ -3161,6 +3197,9 @@ void yzo_conc_stats(void *p, double *out5) {
     auto *st = static_cast<yzo::Stepper *>(p);
     for (int k = 0; k < 5; ++k) out5[k] = st->cstat[k];
 }
+/* the solves that completed the steps of a limited concentration scheme,
+ * over the run (each step at least one, at most 8) */
+long long yzo_conc_settle(void *p) { return static_cast<yzo::Stepper *>(p)->conc_settle; }
 void yzo_clock(void *p, double *t, double *dt_now, double *dt_old) {
     auto *st = static_cast<yzo::Stepper *>(p);
     if (t) *t = st->time;

@@ -9,7 +9,8 @@ drives them) into a scratch directory and packs what it wrote:
                   h, s, c on every row, A(h)h, the FaceFlux of every face
                   of every real cell and the SideRate
   ref_scalars.npz mobility table, limiter table, Dp table, the unlimited
-                  face schemes' FaceValue and FaceWeights, the evaporation
+                  face schemes' FaceValue and FaceWeights, the limited
+                  ones' with the row's cell at L and at R, the evaporation
                   rate of C_Evaporation, the nozzle supply of Source_WM2D/1D
   manifest.json   the dimensionless parameters of every case
   ref_state_wm1d.state  a restart state file (field*.state) written by the
@@ -83,6 +84,7 @@ def main():
                             limiter_vl=read_table(os.path.join(tmp, "limiter_vl.bin")),
                             dp=read_table(os.path.join(tmp, "dp.bin")),
                             schemes=read_table(os.path.join(tmp, "schemes.bin")),
+                            limited_weights=read_table(os.path.join(tmp, "limited_weights.bin")),
                             evap=read_table(os.path.join(tmp, "evap.bin")),
                             source=read_table(os.path.join(tmp, "source.bin")))
         with open(os.path.join(HERE, "manifest.json"), "w") as f:

@@ -9,6 +9,10 @@ C-ABI, against the CPU oracle and the reference's own face fluxes.
   same Newton counts, for the all-physics film (Symmetry and Periodic),
   the 1-D leveling film with Marangoni flow, and a front near c = 1 that
   the UTOPIA face values overshoot (RedistributeExcess on both).
+* The limited schemes (Koren, van Leer): the same bars and counts, and
+  the same number of concentration solves (the step's completion repeated
+  until the frozen limiter settles), on a front carried over a periodic
+  line and on the all-physics film.
 * The learned FP32 inner solve with the concentration: film and
   concentration against the oracle to 1e-8.
 * At 1024^2 (the all-physics film with the concentration), properties
@@ -105,6 +109,10 @@ def _pair(c, h, s, cc, dt, steps, label, bar=1e-10, counts=True):
         # the excess moved is a sum of differences of near-equal values: it
         # carries the film's rounding, so it is held to the film's bar
         assert abs(so[3] - sg[3]) <= bar * max(so[3], 1e-300) + 1e-300
+        if counts:
+            # the same concentration solves, the completing re-solves of a
+            # limited scheme included
+            assert so[0] == sg[0], f"step {k}: concentration solves {so[0]:.0f} / {sg[0]:.0f}"
     st = g.conc_stats()
     print(f"{label}: {steps} steps, worst film {worst_h:.2e}, concentration {worst_c:.2e}; "
           f"device solves {st[0]:.0f}, iterations {st[1]:.0f}, multigrid {st[2]:.0f}, moved {moved:.3e}")
@@ -151,6 +159,28 @@ def test_overshoot_is_redistributed_as_the_oracle_does():
     conc = np.where((X > 0.3) & (X < 0.6), 0.999, 0.2).ravel()
     moved = _pair(c, h, np.zeros_like(h), conc, 4e-3, 6, "overshoot 1-D")
     assert moved > 0.0
+
+
+@pytest.mark.parametrize("scheme", [capi.CONV_KOREN, capi.CONV_VANLEER], ids=["koren", "vanleer"])
+def test_limited_schemes_match_oracle(scheme):
+    """Koren and van Leer: the weights frozen at the latest iterate, the
+    step completed by re-solves until that settles; a front carried over a
+    periodic line, and the all-physics film with Symmetry walls"""
+    c = capi.case_defaults()
+    c.dim, c.nx, c.ny = 1, 60, 1
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, 1.0, -1.0, 1.0
+    for k in range(4):
+        c.bc[k] = capi.BC_PERIODIC
+    c.capillary = 1
+    c.conc_model, c.conc_diffusion, c.conc_scheme = 1, 0, scheme
+    c.inplane_gravity, c.Pgx = 1, 0.6
+    X, _ = cases.cell_centres(c)
+    h = (np.ones_like(X) + 0.1 * np.sin(2 * np.pi * X)).ravel()
+    conc = np.where((X > 0.3) & (X < 0.6), 0.8, 0.2).ravel()
+    _pair(c, h, np.zeros_like(h), conc, 2e-3, 8, f"limited {scheme} front 1-D")
+    c, h, s, cc, dt, _ = cases.allphys_2d(n=40, L=1.5, dt=2e-5)
+    c.conc_model, c.Df, c.diff_exp, c.conc_scheme = 1, 0.05, 0.7, scheme
+    _pair(c, h, s, cc, dt, 4, f"limited {scheme} all-physics 40^2")
 
 
 def test_learned_fp32_with_concentration():

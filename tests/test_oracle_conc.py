@@ -261,3 +261,155 @@ def test_uniform_concentration_stays_uniform_under_a_local_nozzle():
         ci = _interior(c, o.c())
         assert np.abs(ci - 0.37).max() <= 1e-14
     assert _interior(c, o.h()).sum() > v0
+
+
+# ---- the limited schemes of the concentration's advection (Koren, van
+# Leer): the limiter frozen at the latest iterate (FaceWeights,
+# FiniteDifferences.cpp:166-190; LimiterC, ConcEquationBase.h:157-161), the
+# step completed by solves repeated until the frozen state is their own
+# answer (HeightEquationBase.h:1506-1525)
+
+def _koren_vl_phi(a, b, vanleer):
+    """the limiters from their published forms, r = a / b the ratio of the
+    upwind to the downwind difference: Koren max(0, min(2r, (2 + r)/3, 2))
+    (the kappa = 1/3 upwind face value where smooth), van Leer 2ab/(a + b)"""
+    out = np.zeros_like(a)
+    same = a * b > 0
+    if vanleer:
+        out[same] = 2 * a[same] * b[same] / (a[same] + b[same])
+    else:
+        r = a[same] / b[same]
+        out[same] = b[same] * np.maximum(0, np.minimum(np.minimum(2 * r, (2 + r) / 3), 2))
+    return out
+
+
+def _uniform_flow_line(scheme, conc_of_x, n=60):
+    c = _flat_case(1, n, True, 0.0)
+    c.conc_diffusion = 0
+    c.inplane_gravity, c.Pgx = 1, 0.6
+    c.conc_scheme = scheme
+    X, _ = cases.cell_centres(c)
+    h = np.ones_like(X)
+    return c, h.ravel(), conc_of_x(X).ravel()
+
+
+def limited_cn_residual(scheme, steps=3, lib=None, loose=False):
+    """On a flat film carried by in-plane gravity over a periodic line, the
+    step must solve c1 - c0 + dt/2 (div G(c1) + div G(c0)) = 0 with the
+    limited face value G = u (c_U + phi(c_U - c_UU, c_D - c_U) / 2) of the
+    answer itself; returns the largest residual over the steps, and the
+    completing solves the oracle counted. loose: a Newton tolerance that
+    lets the passes end before the concentration's iterate has settled, so
+    that only the completing solves can settle the frozen limiter"""
+    vl = scheme == capi.CONV_VANLEER
+    c, h, conc = _uniform_flow_line(
+        scheme, lambda X: 0.5 + 0.3 * np.sin(2 * np.pi * X) + 0.15 * np.cos(6 * np.pi * X))
+    if loose:
+        c.nk_eps, c.nk_eps_rel = 1e-3, 0.0
+    dt = 2e-3
+    o = ob.OracleStep(c, h, np.zeros(h.size), conc, dt, lib=lib)
+    F, _ = o.face_fluxes(h, conc)
+    u = F[0][2]
+    assert u > 0 and np.allclose(F[0][2:-3], u, rtol=0, atol=1e-12 * u)
+
+    def G(cc):
+        cm, cp = np.roll(cc, 1), np.roll(cc, -1)
+        return u * (cc + 0.5 * _koren_vl_phi(cc - cm, cp - cc, vl))
+
+    c0 = _interior(c, conc)
+    worst = 0.0
+    for _ in range(steps):
+        assert o.advance()[0] == 0
+        c1 = _interior(c, o.c())
+        assert np.abs(c1 - c0).max() > 1e-3
+        g1, g0 = G(c1), G(c0)
+        r = c1 - c0 + 0.5 * dt * ((g1 - np.roll(g1, 1)) + (g0 - np.roll(g0, 1)))
+        worst = max(worst, float(np.abs(r).max()))
+        c0 = c1
+    return worst, o.conc_settle(), o
+
+
+def offdiag_max(o, c):
+    """the largest off-diagonal entry of the real rows of the last
+    concentration system, relative to its diagonal"""
+    rp, col, val, _ = o.conc_system()
+    IX, IY = cases.interior_index(c)
+    nx, ny = c.nx, (c.ny if c.dim == 2 else 1)
+    real = ((IX >= 0) & (IX < nx) & (IY >= 0) & (IY < ny)).ravel()
+    worst = -np.inf
+    for i in np.flatnonzero(real):
+        worst = max(worst, float(val[rp[i] + 1:rp[i + 1]].max() / val[rp[i]]))
+    return worst
+
+
+@pytest.mark.parametrize("scheme", [capi.CONV_KOREN, capi.CONV_VANLEER], ids=["koren", "vanleer"])
+def test_limited_advection_is_the_limited_crank_nicolson_step(scheme):
+    for loose in (False, True):
+        worst, settle, o = limited_cn_residual(scheme, loose=loose)
+        off = offdiag_max(o, o.case)
+        print(f"loose Newton tolerance {loose}: limited Crank-Nicolson residual {worst:.2e}, "
+              f"completing solves {settle} over 3 steps, largest off-diagonal / diagonal {off:.2e}")
+        assert worst <= 1e-14
+        assert 3 < settle <= 24      # more than one per step: the frozen state moved
+        # the frozen weights keep the system an M-matrix in pure advection
+        assert off <= 0.0
+
+
+@pytest.mark.parametrize("scheme", [capi.CONV_KOREN, capi.CONV_VANLEER], ids=["koren", "vanleer"])
+def test_limited_advection_makes_no_new_extrema(scheme):
+    """a step of c carried over the line: the limited schemes keep c within
+    the step's two values (a total-variation-diminishing face value),
+    where UTOPIA overshoots"""
+    def run(sch):
+        c, h, conc = _uniform_flow_line(sch, lambda X: np.where((X > 0.3) & (X < 0.6), 0.8, 0.2))
+        o = ob.OracleStep(c, h, np.zeros(h.size), conc, 2e-3)
+        lo, hi = 1.0, 0.0
+        for _ in range(20):
+            assert o.advance()[0] == 0
+            ci = _interior(c, o.c())
+            lo, hi = min(lo, ci.min()), max(hi, ci.max())
+        return lo, hi
+    lo, hi = run(scheme)
+    assert lo >= 0.2 - 1e-14 and hi <= 0.8 + 1e-14, (lo, hi)
+    lo_u, hi_u = run(capi.CONV_UTOPIA)
+    assert lo_u < 0.2 - 1e-3 and hi_u > 0.8 + 1e-3
+
+
+@pytest.mark.parametrize("scheme", [capi.CONV_KOREN, capi.CONV_VANLEER], ids=["koren", "vanleer"])
+def test_uniform_concentration_stays_uniform_limited(scheme):
+    c, h, s, cc, dt = _allphys()
+    c.conc_scheme = scheme
+    o = ob.OracleStep(c, h, s, np.full_like(cc, 0.37), dt)
+    for _ in range(3):
+        assert o.advance()[0] == 0
+        ci = _interior(c, o.c())
+        assert np.abs(ci - 0.37).max() <= 1e-15
+
+
+LIMITED_PLANTS = {
+    # the step completed by one solve, its limiter left at the last pass's iterate
+    "no_settle": ("for (int k = 0; k < (limited ? 8 : 1) && !noFilm; ++k) {",
+                  "for (int k = 0; k < 1 && !noFilm; ++k) {"),
+    # the limiter frozen at the start of the step instead of the latest iterate
+    "limiter_at_c0": ("fl[q] = c_iter[eq.nb(i, col[q])];", "fl[q] = c0[eq.nb(i, col[q])];"),
+    # the row's cell taken as R of its high face (the other form of the weights)
+    "rowpos_swapped": ("const int rowpos = high ? 1 : 2;", "const int rowpos = high ? 2 : 1;"),
+}
+
+
+@pytest.fixture(scope="module")
+def limited_plants(tmp_path_factory):
+    from .test_oracle_newton import _build_plant
+    tmp = str(tmp_path_factory.mktemp("limited_plants"))
+    return {k: _build_plant(tmp, k, o, n) for k, (o, n) in LIMITED_PLANTS.items()}
+
+
+@pytest.mark.parametrize("plant", sorted(LIMITED_PLANTS))
+def test_limited_checks_fail_with_planted_mistakes(limited_plants, plant):
+    lib = ob.load_from(limited_plants[plant])
+    worst, off = 0.0, -np.inf
+    for loose in (False, True):
+        w, _, o = limited_cn_residual(capi.CONV_KOREN, lib=lib, loose=loose)
+        worst, off = max(worst, w), max(off, offdiag_max(o, o.case))
+    print(f"{plant}: limited Crank-Nicolson residual {worst:.2e}, largest off-diagonal / diagonal {off:.2e}")
+    assert worst > 1e-13 or off > 0.0

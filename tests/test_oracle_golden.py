@@ -96,6 +96,19 @@ def test_conc_schemes_match_reference():
         assert mine[0] == v and np.array_equal(mine[1:], w), (k, sign, mine, row)
 
 
+def test_limited_conc_weights_match_reference():
+    """Koren's and van Leer's FaceValue and FaceWeights with the row's cell
+    upwind (L for a positive flux) and downwind, through zero differences"""
+    t = SCAL["limited_weights"]
+    forms = set()
+    for row in t:
+        vl, sign, rowpos, f, v, w = int(row[0]), row[1], int(row[2]), row[3:7], row[7], row[8:12]
+        mine = ob.conc_face(capi.CONV_VANLEER if vl else capi.CONV_KOREN, f, sign, rowpos)
+        assert mine[0] == v and np.array_equal(mine[1:], w), (row, mine)
+        forms.add((vl, sign > 0, rowpos))
+    assert len(forms) == 8
+
+
 @pytest.mark.parametrize("m", MAN["cases"], ids=[m["name"] for m in MAN["cases"]])
 def test_terms_match_reference(m):
     g = {k: TERMS[f"{m['name']}__{k}"] for k in ("rp", "col", "av", "rpj", "colj", "jv", "h", "s", "c", "ax")}

@@ -18,7 +18,7 @@ HSCHEME_KOREN, HSCHEME_VANLEER = 0, 1
 HSCHEME_UPWIND1, HSCHEME_CENTRAL2, HSCHEME_CENTRAL4, HSCHEME_UPWIND2 = 2, 3, 4, 5
 HSCHEME_UTOPIA, HSCHEME_QUICK, HSCHEME_KK = 6, 7, 8
 (CONV_UPWIND1, CONV_CENTRAL2, CONV_CENTRAL4, CONV_UPWIND2, CONV_UTOPIA, CONV_QUICK,
- CONV_KK) = range(7)
+ CONV_KK, CONV_KOREN, CONV_VANLEER) = range(9)
 LEARN_NPARAM, LEARN_NFEAT, LEARN_NOUT = 6288, 8, 13
 EVAP_RAOULT, EVAP_POWERN = 0, 1
 

@@ -25,6 +25,11 @@ struct ConcArgs {
     int conv, diff;          // EnableConvection, EnableDiffusion
     int side;                // the 1-D centrifugal term's sideways outflow is on
     double wpos[4], wneg[4]; // face weights c1 + Sign(F) c2 for F >= 0 / F < 0
+    // a limited scheme (Koren, or van Leer: lim_vl): its weights frozen at
+    // climit, the latest iterate (padded), and its face values of the
+    // solution formed by the limiter itself
+    int lim, lim_vl;
+    const double *climit;
     double Df, diff_c0, diff_exp;
     int row_scaling;
     const double *h;         // the working film (the guess), padded
@@ -59,6 +64,10 @@ void launch_ghosts(const Geo &g, double *v, hipStream_t st);
 // cprev = c (dmax: bits of a non-negative double, atomicMax)
 void launch_conc_take(const Geo &g, const double *cx, double *c, double *cprev, unsigned long long *dmax,
                       hipStream_t st);
+// the frozen limiter state settling at the step's completion, over every
+// row: dmax = max |cx - climit| (bits of a non-negative double, atomicMax),
+// then climit = cx
+void launch_conc_settle(const Geo &g, const double *cx, double *climit, unsigned long long *dmax, hipStream_t st);
 // any real cell with !(h > 0): *flag |= 1
 void launch_film_check(const Geo &g, const double *h, int *flag, hipStream_t st);
 // the end of the step over every row: c0 = cx, c = cfield = min(cx, 1)

@@ -203,16 +203,30 @@ __global__ void __launch_bounds__(CB) k_conc_rows(ConcArgs a) {
         const int lo = high ? p : ia;
         const double f1v = a.F1[ax][lo], f0v = a.F0[ax][lo];
         if (a.conv) {
-            const double *w1 = f1v >= 0.0 ? a.wpos : a.wneg;
-            const double *w0 = f0v >= 0.0 ? a.wpos : a.wneg;
+            double w1[4], w0[4], f0[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) f0[q] = a.c0[slot_row<DIM>(g, p, face_slot(DIM, d, q))];
+            if (a.lim) {
+                // the row's cell is L of its high face, R of its low one
+                double fl[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) fl[q] = a.climit[slot_row<DIM>(g, p, face_slot(DIM, d, q))];
+                const int rowpos = high ? 1 : 2;
+                limited_weights(fl, f1v, rowpos, a.lim_vl, w1);
+                limited_weights(f0, f0v, rowpos, a.lim_vl, w0);
+            } else {
+                const double *u1 = f1v >= 0.0 ? a.wpos : a.wneg;
+                const double *u0 = f0v >= 0.0 ? a.wpos : a.wneg;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { w1[q] = u1[q]; w0[q] = u0[q]; }
+            }
             double e1 = 0, e0 = 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int col = face_slot(DIM, d, q);
-                const double f0 = a.c0[slot_row<DIM>(g, p, col)];
                 M[col] += th * dt * sg * f1v * w1[q];
-                e1 += w1[q] * (f0 - c0p);
-                e0 += w0[q] * (f0 - c0p);
+                e1 += w1[q] * (f0[q] - c0p);
+                e0 += w0[q] * (f0[q] - c0p);
             }
             R -= dt * sg * (th * f1v * e1 + (1.0 - th) * f0v * e0);
         } else {
@@ -263,14 +277,27 @@ __device__ double solute_flux(const ConcArgs &a, int p, int d) {
     const int ia = slot_row<DIM>(g, p, adj_slot(DIM, d));
     const int lo = high ? p : ia, hi = high ? ia : p;
     const double f1v = a.F1[ax][lo], f0v = a.F0[ax][lo];
-    const double *w1 = f1v >= 0.0 ? a.wpos : a.wneg;
-    const double *w0 = f0v >= 0.0 ? a.wpos : a.wneg;
     double v1 = 0, v0 = 0;
+    if (a.lim) {
+        // a limited scheme's face value evaluated at that very data
+        double f1[4], f0[4], w[4], gd[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int j = slot_row<DIM>(g, p, face_slot(DIM, d, q));
-        v1 += w1[q] * a.cx[j];
-        v0 += w0[q] * a.c0[j];
+        for (int q = 0; q < 4; ++q) {
+            const int j = slot_row<DIM>(g, p, face_slot(DIM, d, q));
+            f1[q] = a.cx[j];
+            f0[q] = a.c0[j];
+        }
+        v1 = limited_face(f1, f1v, w, gd, a.lim_vl);
+        v0 = limited_face(f0, f0v, w, gd, a.lim_vl);
+    } else {
+        const double *w1 = f1v >= 0.0 ? a.wpos : a.wneg;
+        const double *w0 = f0v >= 0.0 ? a.wpos : a.wneg;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = slot_row<DIM>(g, p, face_slot(DIM, d, q));
+            v1 += w1[q] * a.cx[j];
+            v0 += w0[q] * a.c0[j];
+        }
     }
     double G1 = f1v * v1;
     double G0 = f0v * v0;
@@ -328,6 +355,25 @@ __global__ void k_conc_take(Geo g, const double *cx, double *c, double *cprev, u
         c[p] = v;
         m = fabs(v - cprev[p]);
         cprev[p] = v;
+    }
+    __shared__ double sh[CB / 64];
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0;
+        for (int w = 0; w < CB / 64; ++w) mx = fmax(mx, sh[w]);
+        atomicMax(dmax, (unsigned long long)__double_as_longlong(mx));
+    }
+}
+
+__global__ void k_conc_settle(int n, const double *cx, double *climit, unsigned long long *dmax) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0;
+    if (p < n) {
+        const double v = cx[p];
+        m = fabs(v - climit[p]);
+        climit[p] = v;
     }
     __shared__ double sh[CB / 64];
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
@@ -630,6 +676,10 @@ void launch_reconstruct(const ConcArgs &a, hipStream_t st) {
 void launch_conc_take(const Geo &g, const double *cx, double *c, double *cprev, unsigned long long *dmax,
                       hipStream_t st) {
     hipLaunchKernelGGL(k_conc_take, dim3(cblocks(g.nxp * g.nyp)), dim3(CB), 0, st, g, cx, c, cprev, dmax);
+}
+void launch_conc_settle(const Geo &g, const double *cx, double *climit, unsigned long long *dmax, hipStream_t st) {
+    const int n = g.nxp * g.nyp;
+    hipLaunchKernelGGL(k_conc_settle, dim3(cblocks(n)), dim3(CB), 0, st, n, cx, climit, dmax);
 }
 void launch_film_check(const Geo &g, const double *h, int *flag, hipStream_t st) {
     hipLaunchKernelGGL(k_film_check, dim3(cblocks(g.N)), dim3(CB), 0, st, g, h, flag);

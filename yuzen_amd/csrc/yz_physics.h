@@ -215,6 +215,30 @@ __device__ inline double limited_face(const double f[4], double sign, double w[4
     return f[u] + 0.5 * p;
 }
 
+// FaceWeights of a limited scheme (FiniteDifferences.cpp:166-190): the face
+// value as weights of the four points frozen at f, with the row's own cell
+// at rowpos (LL 0, L 1, R 2, RR 3). The row's cell upwind: f_U + chi/2
+// (f_U - f_UU), chi = phi / (f_U - f_UU); otherwise f_U + psi/2 (f_D - f_U),
+// psi = phi / (f_D - f_U); either keeps the frozen system an M-matrix.
+__device__ inline void limited_weights(const double f[4], double sign, int rowpos, int vl, double w[4]) {
+    int uu, u, d;
+    if (sign >= 0) { uu = 0; u = 1; d = 2; } else { uu = 3; u = 2; d = 1; }
+    const double a = f[u] - f[uu], b = f[d] - f[u];
+    double pa, pb;
+    const double p = vl ? vanleer(a, b, &pa, &pb) : koren(a, b, &pa, &pb);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) w[j] = 0;
+    if (rowpos == u) {
+        const double chi = (a != 0) ? p / a : 0.0;
+        w[u] = 1.0 + 0.5 * chi;
+        w[uu] = -0.5 * chi;
+    } else {
+        const double psi = (b != 0) ? p / b : 0.0;
+        w[u] = 1.0 - 0.5 * psi;
+        w[d] = 0.5 * psi;
+    }
+}
+
 // The face scheme of the upwinded terms' film height (Numerics.Convection.
 // Height): kind 0 Koren, 1 van Leer (limited), 2 a fixed-coefficient scheme
 // whose coefficients of LL L R RR are c1 + c2 with the flow in +, c1 - c2

@@ -280,6 +280,9 @@ struct yz_step {
     // solve's guess) and its own FP64 solver
     bool conc = false;
     double *cfield = nullptr, *c0 = nullptr, *cx = nullptr, *cprev = nullptr, *clast = nullptr;
+    // what a limited scheme freezes its limiter at: the latest iterate as the
+    // solve gave it, before the cap (_cIterate, HeightEquationBase.h:423-429)
+    double *citer = nullptr;
     double *Fc0[2] = {nullptr, nullptr}, *Fc1[2] = {nullptr, nullptr}, *G2[2] = {nullptr, nullptr};
     double *rc0 = nullptr, *rc1 = nullptr;
     double *ex[4] = {nullptr, nullptr, nullptr, nullptr};   // room, gain, f, e
@@ -323,8 +326,8 @@ static int check_case(const yz_case &c) {
         set_err("height_scheme: not one of the YZ_HSCHEME_* schemes"); return 1;
     }
     if (c.conc_model == 1) {
-        if (c.conc_scheme < YZ_CONV_UPWIND1 || c.conc_scheme > YZ_CONV_KK) {
-            set_err("conc_scheme: only the unlimited face schemes are implemented for the concentration"); return 1;
+        if (c.conc_scheme < YZ_CONV_UPWIND1 || c.conc_scheme > YZ_CONV_VANLEER) {
+            set_err("conc_scheme: not one of the YZ_CONV_* schemes"); return 1;
         }
         if (c.conc_lin_maxit < 1 || !(c.conc_lin_eps > 0)) { set_err("conc_lin_maxit >= 1 and conc_lin_eps > 0"); return 1; }
     }
@@ -897,6 +900,7 @@ static int conc_before(yz_step *S) {
     YZ_TRY(hipMemcpyAsync(S->c, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     YZ_TRY(hipMemcpyAsync(S->cprev, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     YZ_TRY(hipMemcpyAsync(S->cstart, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
+    YZ_TRY(hipMemcpyAsync(S->citer, S->cfield, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     conc_fluxes(S, S->hold, S->Fc0, S->rc0);
     YZ_TRY(hipMemsetAsync(S->dconc, 0, g.N * sizeof(double), S->st));
     YZ_TRY(hipGetLastError());
@@ -926,6 +930,9 @@ static int conc_every(yz_step *S, bool *ok) {
     int ch = 0;
     if (conc_solve(S, false, &ch)) return 2;
     if (ch != 0) { *ok = false; return 0; }
+    // the limiter of the next pass from the iterate as solved
+    const size_t np = (size_t)S->g.nxp * S->g.nyp;
+    YZ_TRY(hipMemcpyAsync(S->citer, S->cx, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     YZ_TRY(hipMemsetAsync(S->cdmax, 0, sizeof(unsigned long long), S->st));
     launch_conc_take(S->g, S->cx, S->c, S->cprev, S->cdmax, S->st);
     YZ_TRY(hipMemcpyAsync(S->hcdmax, S->cdmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, S->st));
@@ -989,10 +996,20 @@ static int conc_after(yz_step *S) {
     if (film_not_positive(S, &no_film)) return 2;
     YZ_TRY(hipMemcpyAsync(S->clast, S->cx, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     bool solved = true;
-    if (!no_film) {
+    // a limited scheme freezes its limiter at the last iterate: the
+    // completing solve is repeated, frozen at its own answer, until that
+    // settles (HeightEquationBase.h:1506-1525)
+    const bool limited = S->cs.conc_convection && S->ca.lim;
+    for (int k = 0; k < (limited ? 8 : 1) && !no_film; ++k) {
         int ch = 0;
         if (conc_solve(S, true, &ch)) return 2;
-        if (ch != 0) solved = false;
+        if (ch != 0) { solved = false; break; }
+        if (!limited) break;
+        YZ_TRY(hipMemsetAsync(S->cdmax, 0, sizeof(unsigned long long), S->st));
+        launch_conc_settle(g, S->cx, S->citer, S->cdmax, S->st);
+        YZ_TRY(hipMemcpyAsync(S->hcdmax, S->cdmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, S->st));
+        YZ_TRY(hipStreamSynchronize(S->st));
+        if (bits_to_double(*S->hcdmax) <= 1e-12) break;
     }
     if (!solved) YZ_TRY(hipMemcpyAsync(S->cx, S->clast, np * sizeof(double), hipMemcpyDeviceToDevice, S->st));
     if (conc_redistribute(S)) return 2;
@@ -1198,7 +1215,7 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
         S->conc = true;
         double **cp[] = {&S->cfield, &S->c0, &S->cx, &S->cprev, &S->clast, &S->Fc0[0], &S->Fc0[1], &S->Fc1[0],
                          &S->Fc1[1], &S->G2[0], &S->G2[1], &S->rc0, &S->rc1, &S->ex[0], &S->ex[1], &S->ex[2],
-                         &S->ex[3], &S->cstart};
+                         &S->ex[3], &S->cstart, &S->citer};
         for (double **p : cp) {
             if (hipMalloc((void **)p, np * sizeof(double)) != hipSuccess) return fail("hipMalloc");
             if (hipMemset(*p, 0, np * sizeof(double)) != hipSuccess) return fail("memset");
@@ -1216,7 +1233,7 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
             hipHostMalloc((void **)&S->hcdmax, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void **)&S->hcsums, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
             return fail("hipHostMalloc");
-        for (double *p : {S->cfield, S->c0, S->cx, S->cprev})
+        for (double *p : {S->cfield, S->c0, S->cx, S->cprev, S->citer})
             if (hipMemcpy(p, S->c, np * sizeof(double), hipMemcpyDeviceToDevice) != hipSuccess) return fail("copy");
         if (solver_init(S->Sc, fine_op(g), S->st)) return fail("solver allocation");
         ConcArgs &k = S->ca;
@@ -1225,13 +1242,20 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
         k.theta_h = c->theta;
         k.conv = c->conc_convection; k.diff = c->conc_diffusion;
         k.side = (c->dim == 1 && c->centrifugal) ? 1 : 0;
-        // the scheme's c1 and c2 over the sum of c1 (FiniteDifferences.cpp:22-93)
-        double c1[4], c2[4];
-        scheme_coefs(c->conc_scheme, c1, c2);
-        for (int j = 0; j < 4; ++j) {
-            const double a1 = c1[j], a2 = c2[j];
-            k.wpos[j] = a1 + 1.0 * a2;
-            k.wneg[j] = a1 + (-1.0) * a2;
+        // an unlimited scheme's c1 and c2 over the sum of c1
+        // (FiniteDifferences.cpp:22-93); a limited one's limiter, frozen at
+        // the latest iterate
+        k.lim = c->conc_scheme == YZ_CONV_KOREN || c->conc_scheme == YZ_CONV_VANLEER;
+        k.lim_vl = c->conc_scheme == YZ_CONV_VANLEER;
+        k.climit = S->citer;
+        if (!k.lim) {
+            double c1[4], c2[4];
+            scheme_coefs(c->conc_scheme, c1, c2);
+            for (int j = 0; j < 4; ++j) {
+                const double a1 = c1[j], a2 = c2[j];
+                k.wpos[j] = a1 + 1.0 * a2;
+                k.wneg[j] = a1 + (-1.0) * a2;
+            }
         }
         k.Df = c->Df; k.diff_c0 = c->diff_c0; k.diff_exp = c->diff_exp;
         k.row_scaling = c->conc_row_scaling;
@@ -1287,7 +1311,7 @@ void yz_step_destroy(yz_step *S) {
     solver_free(S->S);
     solver_free(S->S32);
     solver_free(S->Sc);
-    for (double *p : {S->cfield, S->c0, S->cx, S->cprev, S->clast, S->Fc0[0], S->Fc0[1], S->Fc1[0], S->Fc1[1],
+    for (double *p : {S->cfield, S->c0, S->cx, S->cprev, S->clast, S->citer, S->Fc0[0], S->Fc0[1], S->Fc1[0], S->Fc1[1],
                       S->G2[0], S->G2[1], S->rc0, S->rc1, S->ex[0], S->ex[1], S->ex[2], S->ex[3], S->dconc,
                       S->dsave, S->bconc, S->cscratch, S->csums, S->cstart, S->evps})
         if (p) (void)hipFree(p);
