@@ -100,6 +100,28 @@ def test_step_allphys_2d_zero_guess():
     _run_pair(c, h, s, cc, dt, 4, "allphys_2d zero guess")
 
 
+@pytest.mark.parametrize("interval", [2, 3])
+def test_step_modified_newton(interval):
+    """JacobianUpdateInterval 2 and 3 (NewtonKantorovich.cpp:80-91): IA, its
+    row scale and its multigrid kept between the passes that form it; equal
+    Newton counts with the oracle run the same way; the learned FP32 path
+    keeps its approximate inverse too and holds the learned bar"""
+    c, h, s, cc, dt, _ = cases.allphys_2d(n=48, L=2.0, dt=2e-5)
+    c.jac_interval = interval
+    _run_pair(c, h, s, cc, dt, 4, f"allphys_2d jacobian interval {interval}")
+    cl = c.copy()
+    cl.precond, cl.krylov_fp32, cl.learn_passes, cl.learn_iters = capi.PC_LEARNED, 1, 1, 40
+    o = ob.OracleStep(c, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
+    g = HipStep(cl, h, s, cc, dt)
+    for k in range(3):
+        ro, rg = o.advance(), g.advance()
+        assert ro[0] == 0 and rg[0] == 0
+        rel = np.abs(o.h() - g.h()).max() / np.abs(o.h()).max()
+        print(f"learned FP32, interval {interval}, step {k}: oracle nk={ro[1]} hip nk={rg[1]} rel={rel:.2e}")
+        assert rel <= 1e-8
+    g.close()
+
+
 def test_step_spinodal_2d_small():
     c, h, s, cc, dt, _ = cases.spinodal_2d(n=32)
     _run_pair(c, h, s, cc, dt, 4, "spinodal_2d")

@@ -32,6 +32,7 @@ import numpy as np
 import pytest
 
 from oracle import binding as ob
+from yuzen_amd import cases
 
 TINY = 1e-20     # a correction counted as converged (the reference test's)
 BIG = 1.0        # a correction still moving
@@ -465,3 +466,28 @@ def test_planted_mistake_fails(planted, plant):
         with pytest.raises(AssertionError):
             by_name[name](lib)
 
+
+
+def test_modified_newton_jacobian_interval():
+    """JacobianUpdateInterval k (NewtonKantorovich.cpp:80-91): IA is formed
+    on the passes 0, k, 2k, ... of each attempt and kept in between. On the
+    nonlinear all-physics film the step converges to the same film (to the
+    Newton tolerance) in more passes, more the staler the Jacobian"""
+    c, h, s, cc, dt, _ = cases.allphys_2d(n=32, L=1.5, dt=2e-5)
+    runs = {}
+    for ji in (1, 2, 3):
+        cj = c.copy()
+        cj.jac_interval = ji
+        o = ob.OracleStep(cj, h, s, cc, dt, lin_mode=ob.OracleStep.LIN_STRUCTURED)
+        nks = []
+        for _ in range(4):
+            r = o.advance()
+            assert r[0] == 0
+            nks.append(r[1])
+        runs[ji] = (np.array(nks), o.h())
+    print({k: list(v[0]) for k, v in runs.items()})
+    ref = runs[1][1]
+    for ji in (2, 3):
+        assert np.abs(runs[ji][1] - ref).max() <= 1e-10 * np.abs(ref).max()
+        assert (runs[ji][0] >= runs[1][0]).all() and runs[ji][0].sum() > runs[1][0].sum()
+    assert runs[3][0].sum() > runs[2][0].sum()

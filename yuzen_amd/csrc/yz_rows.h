@@ -87,6 +87,11 @@ struct RowArgs {
     float *mscale;             // optional: 1 / (1 + sum_k!=c |IA_k|) of the scaled row
     unsigned long long *rmax;  // mode 1: max |Rhs| (bits of a non-negative double)
     int *nonunit;              // FP32 planes: set when a row's stored diagonal is not exactly 1
+    // the modified Newton method (JacobianUpdateInterval > 1): a pass that
+    // keeps the last IA (keep_ia) writes only the right-hand side, scaled by
+    // the row scale dinv that the last IA's pass stored (mode 1)
+    int keep_ia;
+    double *dinv;
     // optional, mode 1: B = (1 - theta) evold + theta EvapAt(h, c) - src
     // enters Rhs as - dt B (SetMatrixA, HeightEquationBase.h:765-787); J
     // carries no derivative of it
@@ -460,12 +465,19 @@ __device__ __forceinline__ void row_body(const RowArgs &A, int i, int ixp, int i
 #pragma unroll
             for (int k = 0; k < NS; ++k) IA[k] = td * (Jc[k] + Ar[k]);
             IA[C0] = 1.0 + IA[C0];
+            if (A.keep_ia) {
+                const double inv = A.dinv[i];
+                if (A.b32) A.b32[i] = (float)(A.row_scaling ? R * inv : R);
+                else A.b[i] = A.row_scaling ? R * inv : R;
+                return;
+            }
             if (A.ia_raw) {
 #pragma unroll
                 for (int k = 0; k < NS; ++k) A.ia_raw[(size_t)k * g.N + i] = IA[k];
             }
             const double dg = IA[C0];
             const double inv = (A.row_scaling && fabs(dg) >= 1.0e-30) ? 1.0 / dg : 1.0;
+            if (A.dinv) A.dinv[i] = inv;
             if (A.ia32) {
 #pragma unroll
                 for (int k = 0; k < NS; ++k) A.ia32[(size_t)k * g.N + i] = (float)(A.row_scaling ? IA[k] * inv : IA[k]);

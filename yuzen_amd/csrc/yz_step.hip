@@ -219,6 +219,8 @@ struct yz_step {
     bool own_stream = true;
     double *h = nullptr, *hold = nullptr, *hold2 = nullptr, *hnew = nullptr, *s = nullptr, *c = nullptr;
     double *axold = nullptr, *b = nullptr, *corr = nullptr, *scratch = nullptr;
+    // the row scale of the last pass that formed IA (JacobianUpdateInterval)
+    double *dinv = nullptr;
     double *evold = nullptr, *srcv = nullptr;   // B's parts fixed over a step (N each)
     double *evpv = nullptr;                     // B's evaporation of the last pass (N)
     unsigned long long *dmax = nullptr;   // [0] rhs, [1] corr, [2] move, [3] trunc
@@ -314,7 +316,7 @@ static int check_case(const yz_case &c) {
         (c.dim == 2 && (c.bc[2] == YZ_BC_PERIODIC) != (c.bc[3] == YZ_BC_PERIODIC))) {
         set_err("a Periodic edge needs a Periodic opposite edge"); return 1;
     }
-    if (c.jac_interval != 1) { set_err("JacobianUpdateInterval other than 1 is not implemented"); return 1; }
+    if (c.jac_interval < 1) { set_err("JacobianUpdateInterval must be at least 1"); return 1; }
     if (c.precond < YZ_PC_JACOBI || c.precond > YZ_PC_LEARNED_AMG) { set_err("unknown preconditioner"); return 1; }
     if (c.dim != 2 && (c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG)) {
         set_err("the learned preconditioner is built for the 2-D film"); return 1;
@@ -582,6 +584,13 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     S->ra.axold = S->axold;
     S->ra.hold = S->hold;
     RowArgs keep = S->ra;
+    // the modified Newton method (NewtonKantorovich.cpp:80-91): IA formed on
+    // the passes nk_pass % JacobianUpdateInterval == 0 of the attempt (the
+    // first always); the others keep it, with its row scale, its multigrid
+    // and its learned inverse, and form only the right-hand side
+    const bool jac = S->nk_pass % c.jac_interval == 0;
+    S->ra.keep_ia = jac ? 0 : 1;
+    S->ra.dinv = S->dinv;
     if (f32) {
         S->ra.ia32 = S->S32.A; S->ra.b32 = S->b32;
         if (c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG) { S->ra.feat = S->feat; S->ra.mscale = S->mscale; }
@@ -596,7 +605,7 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     YZ_TRY(hipEventRecord(S->ev[1], S->st));
     ++S->nk_pass;
     if (sync_read(S)) { *dev_err = 1; return 2; }
-    if (f32) S->S32.unit_diag = S->hflag[0] == 0;
+    if (f32 && jac) S->S32.unit_diag = S->hflag[0] == 0;
     S->tms[0] += ev_ms(S->ev[0], S->ev[1]);
     const double eps = ew_eps(S);
     S->last_eps = eps;
@@ -610,7 +619,7 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
     int ch = -1;
     if (f32) {
         int rc = 0;
-        if (c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG) {
+        if (jac && (c.precond == YZ_PC_LEARNED || c.precond == YZ_PC_LEARNED_AMG)) {
             // in-place training on the first passes' own matrices, then frozen
             if (S->mlp.trained_passes < c.learn_passes && c.learn_iters > 0) {
                 YZ_TRY(hipEventRecord(S->ev[8], S->st));
@@ -636,8 +645,8 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
         }
         S->S32.stall_window = c.stall_window;
         S->S32.use_graph = c.graphs;
-        if (!rc && (c.precond == YZ_PC_AMG || c.precond == YZ_PC_LEARNED_AMG)) rc = amg_setup(S->S32);
-        if (!rc && c.precond == YZ_PC_JACOBI) rc = jacobi_setup(S->S32);
+        if (jac && !rc && (c.precond == YZ_PC_AMG || c.precond == YZ_PC_LEARNED_AMG)) rc = amg_setup(S->S32);
+        if (jac && !rc && c.precond == YZ_PC_JACOBI) rc = jacobi_setup(S->S32);
         if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
         YZ_TRY(hipEventRecord(S->ev[3], S->st));
         const double eps32 = fp32_tol(c, eps, S->nk_first, S->nk_prev);
@@ -657,8 +666,8 @@ static int nk_solve(yz_step *S, int *iter, double *resid, int *dev_err) {
         ch = fp64_fallback(S, eps, iter, resid, dev_err);
         if (*dev_err) return 2;
     } else {
-        int rc;
-        if (c.precond == YZ_PC_AMG) rc = amg_setup(S->S); else rc = jacobi_setup(S->S);
+        int rc = 0;
+        if (jac) rc = c.precond == YZ_PC_AMG ? amg_setup(S->S) : jacobi_setup(S->S);
         if (rc) { set_err("preconditioner set-up failed"); *dev_err = 1; return 2; }
         YZ_TRY(hipEventRecord(S->ev[3], S->st));
         if (zero_guess(c)) YZ_TRY(hipMemsetAsync(S->corr, 0, S->g.N * sizeof(double), S->st));
@@ -1174,7 +1183,7 @@ static yz_step *step_new(const yz_case *c, const double *h, const double *s, con
     else if (hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking) != hipSuccess) return fail("stream");
     double **pads[] = {&S->h, &S->hold, &S->hold2, &S->hnew, &S->s, &S->c};
     for (double **p : pads) if (hipMalloc((void **)p, np * sizeof(double)) != hipSuccess) return fail("hipMalloc");
-    double **vecs[] = {&S->axold, &S->b, &S->corr, &S->scratch};
+    double **vecs[] = {&S->axold, &S->b, &S->corr, &S->scratch, &S->dinv};
     for (double **p : vecs) if (hipMalloc((void **)p, g.N * sizeof(double)) != hipSuccess) return fail("hipMalloc");
     if (has_b(*c))
         for (double **p : {&S->evold, &S->srcv, &S->evpv}) {
@@ -1323,7 +1332,7 @@ void yz_step_destroy(yz_step *S) {
     for (float *p : {S->b32, S->corr32, S->feat, S->mscale, S->M32})
         if (p) (void)hipFree(p);
     if (S->Yh) (void)hipFree(S->Yh);
-    for (double *p : {S->h, S->hold, S->hold2, S->hnew, S->s, S->c, S->axold, S->b, S->corr, S->scratch,
+    for (double *p : {S->h, S->hold, S->hold2, S->hnew, S->s, S->c, S->axold, S->b, S->corr, S->scratch, S->dinv,
                       S->evold, S->srcv, S->evpv})
         if (p) (void)hipFree(p);
     if (S->dmax) (void)hipFree(S->dmax);
@@ -2406,6 +2415,7 @@ yz_batch *yz_batch_create(int n, const yz_case *cases, const double *h, const do
         if (c.learn_passes != 0) { set_err("a batch uses one shared network: learn_passes 0 (yz_batch_set_params)"); return nullptr; }
         if (c.lin_maxit != cases[0].lin_maxit) { set_err("the films of a batch share lin_maxit"); return nullptr; }
         if (c.conc_model != 0) { set_err("a batch takes films with a frozen concentration (conc_model 0)"); return nullptr; }
+        if (c.jac_interval != 1) { set_err("a batch forms IA every pass (jac_interval 1)"); return nullptr; }
     }
     yz_batch *B = new yz_batch();
     auto fail = [&](const char *what) { if (what) set_err(what); yz_batch_destroy(B); return (yz_batch *)nullptr; };
