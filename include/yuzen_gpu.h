@@ -619,8 +619,9 @@ int yz_memory_set(yz_linctx *ctx, const double *buf, int len);
 /* update, the velocity through the film, the concentration step and    */
 /* the coupling passes, RedistributeExcess and RedistributeDeficit),    */
 /* with CZ_ConcEquation_2D.cpp for the concentration. The film is 2-D   */
-/* (Grid.Dimension "xyh"), the step fixed (no step control), the        */
-/* concentration scheme unlimited; the create call refuses the rest.    */
+/* (Grid.Dimension "xyh"), the step fixed (no step control); every      */
+/* concentration scheme, the limited ones with SettleLimiter's solves   */
+/* (CZ_ConcEquation_2D.cpp:945-973); the create call refuses the rest.  */
 /* ------------------------------------------------------------------ */
 typedef struct yz_cz_case {
     int nx, ny, nz;                 /* real cells: Grid.Nx, Ny, Nz */

@@ -88,6 +88,8 @@ def load_from(path):
     lib.yzo_cz_get_c.argtypes = [C.c_void_p, pd]
     lib.yzo_cz_volume_fluxes.argtypes = [C.c_void_p, pd, pd, pd]
     lib.yzo_cz_stats.argtypes = [C.c_void_p, pd]
+    lib.yzo_cz_settle.argtypes = [C.c_void_p]
+    lib.yzo_cz_settle.restype = C.c_longlong
     lib.yzo_cz_velocity.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_cz_redistribute.restype = C.c_double
     lib.yzo_cz_redistribute.argtypes = [C.c_void_p, pd, pd, pd, pd]
@@ -295,8 +297,8 @@ def linsolve(rowptr, col, val, rhs, x0, precond=1, scaling=1, maxit=5000, eps=1e
 class OracleCZ:
     """The oracle's ConcZ stepper over one film (yz_cz_oracle.cpp)."""
 
-    def __init__(self, case, h, s, c, dt):
-        self.lib = load()
+    def __init__(self, case, h, s, c, dt, lib=None):
+        self.lib = lib or load()
         self.case = case.copy()
         self.n2 = (case.nx + 4) * (case.ny + 4)
         self.n3 = self.n2 * (case.nz + 4)
@@ -335,6 +337,10 @@ class OracleCZ:
         out = np.empty(8)
         self.lib.yzo_cz_stats(self.p, _pd(out))
         return out
+
+    def settle(self):
+        """SettleLimiter's solves over the run (a limited scheme)"""
+        return int(self.lib.yzo_cz_settle(self.p))
 
     def velocity(self, h, c):
         h = np.ascontiguousarray(h, dtype=np.float64)

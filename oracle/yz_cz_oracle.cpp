@@ -46,10 +46,13 @@
  * AMGCL's (an un-vendored library), so the solution is pinned to that
  * tolerance and the iteration counts are not.
  *
+ * The limited concentration schemes (Koren, van Leer) freeze their
+ * limiter at the field the stage is solved at (AdvWeights :575-593) and
+ * settle it to the midpoint of the step (SettleLimiter :945-973).
+ *
  * Not restated (the step refuses them): the 1-D film (Ny = 1) and its
- * sideways outflow, the limited concentration schemes (SettleLimiter),
- * step control, the manufactured solution, the negative-power evaporation
- * profile.
+ * sideways outflow, step control, the manufactured solution, the
+ * negative-power evaporation profile.
  */
 
 namespace yzo {
@@ -61,6 +64,9 @@ struct CzStepper {
     double dt = 0;
     const double theta = 0.5;
     double c1w[4], c2w[4];
+    bool clim = false;           /* a limited scheme (Koren, van Leer) */
+    Limiter lim;
+    long long settle_passes = 0; /* SettleLimiter's solves over the run */
     bool wall[4];
 
     vector<double> S;                       /* substrate */
@@ -603,7 +609,11 @@ struct CzStepper {
     static constexpr int coy[21] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, -1, 1, -1, 1};
     static constexpr int coz[21] = {0, -2, -1, -1, -1, 0, 0, 0, 0, 1, 1, 1, 2, 0, 0, 0, 0, -1, -1, 1, 1};
 
-    void conc_weights(double F, double w[4]) const {
+    /* AdvWeights (:575-593): an unlimited scheme's weights, the same for
+     * both rows; a limited one's frozen at the field f, in the upwind form
+     * for the upwind cell and the downwind form for the other (rowpos) */
+    void conc_weights(const double f[4], double F, int rowpos, double w[4]) const {
+        if (clim) { lim.weights_at(f, F, rowpos, w); return; }
         for (int j = 0; j < 4; ++j) w[j] = c1w[j] + Sign(F) * c2w[j];
     }
 
@@ -653,8 +663,10 @@ struct CzStepper {
                     auto Y = [&](int dj, int dk, double v) { if (dj == 0) xz[2][dk + 2] += v; else yz[dj + 2][dk + 1] += v; };
                     {
                         const double QL = Qx[I3(ix, jy, kz)], QR = Qx[I3(ix + 1, jy, kz)];
+                        const double fL[4] = {C(ix - 2, jy, kz), C(ix - 1, jy, kz), C(ix, jy, kz), C(ix + 1, jy, kz)};
+                        const double fR[4] = {C(ix - 1, jy, kz), C(ix, jy, kz), C(ix + 1, jy, kz), C(ix + 2, jy, kz)};
                         double wL[4], wR[4];
-                        conc_weights(QL, wL); conc_weights(QR, wR);
+                        conc_weights(fL, QL, 2, wL); conc_weights(fR, QR, 1, wR);
                         for (int t = 0; t < 4; t++) {
                             X(t - 2, 0, +QL * wL[t] / dx);
                             X(t - 1, 0, -QR * wR[t] / dx);
@@ -662,8 +674,10 @@ struct CzStepper {
                     }
                     {
                         const double QS = Qy[I3(ix, jy, kz)], QN = Qy[I3(ix, jy + 1, kz)];
+                        const double fS[4] = {C(ix, jy - 2, kz), C(ix, jy - 1, kz), C(ix, jy, kz), C(ix, jy + 1, kz)};
+                        const double fN[4] = {C(ix, jy - 1, kz), C(ix, jy, kz), C(ix, jy + 1, kz), C(ix, jy + 2, kz)};
                         double wS[4], wN[4];
-                        conc_weights(QS, wS); conc_weights(QN, wN);
+                        conc_weights(fS, QS, 2, wS); conc_weights(fN, QN, 1, wN);
                         for (int t = 0; t < 4; t++) {
                             Y(t - 2, 0, +QS * wS[t] / dy);
                             Y(t - 1, 0, -QN * wN[t] / dy);
@@ -672,11 +686,13 @@ struct CzStepper {
                     {
                         const double WB = W[I3(ix, jy, kz)], WT = W[I3(ix, jy, kz + 1)];
                         if (!bottom) {
-                            double wB[4]; conc_weights(WB, wB);
+                            const double fB[4] = {C(ix, jy, kz - 2), C(ix, jy, kz - 1), C(ix, jy, kz), C(ix, jy, kz + 1)};
+                            double wB[4]; conc_weights(fB, WB, 2, wB);
                             for (int t = 0; t < 4; t++) X(0, t - 2, +WB * wB[t] / dz);
                         }
                         if (!top) {
-                            double wT[4]; conc_weights(WT, wT);
+                            const double fT[4] = {C(ix, jy, kz - 1), C(ix, jy, kz), C(ix, jy, kz + 1), C(ix, jy, kz + 2)};
+                            double wT[4]; conc_weights(fT, WT, 1, wT);
                             for (int t = 0; t < 4; t++) X(0, t - 1, -WT * wT[t] / dz);
                         } else {
                             rct += WT / dz;
@@ -910,10 +926,35 @@ struct CzStepper {
         apply_boundary_rows(c_new);
         return ch;
     }
-    /* SolveAtMidpoint (:935-944), unlimited schemes */
+    /* SettleLimiter (:945-973): a limited scheme's step solved again with
+     * the limiter frozen at the midpoint of what came out, until that
+     * settles to 1e-12 of the step's change (8 passes at most) */
+    int settle_limiter(int &iter, double &resid) {
+        if (!clim) return 0;
+        vector<double> mid(n3);
+        for (int pass = 0; pass < 8; pass++) {
+            for (int i = 0; i < n3; ++i) mid[i] = (c_old[i] + c_new[i]) * 0.5;
+            const vector<double> prev = c_new;
+            fC = mid;
+            int it = 0; double rs = 0;
+            const int ch = c_stage(it, rs);
+            iter += it; resid = rs;
+            ++settle_passes;
+            if (ch != 0) return ch;
+            double dmax = 0, cmax = 0;
+            for (int i = 0; i < n3; ++i) {
+                dmax = std::max(dmax, std::fabs(c_new[i] - prev[i]));
+                cmax = std::max(cmax, std::fabs(c_new[i] - c_old[i]));
+            }
+            if (!(dmax > 1e-12 * cmax)) break;
+        }
+        return 0;
+    }
+    /* SolveAtMidpoint (:935-944) */
     int c_solve_at_mid(const vector<double> &cmid, int &iter, double &resid) {
         fC = cmid;
-        const int ch = c_stage(iter, resid);
+        int ch = c_stage(iter, resid);
+        if (ch == 0) ch = settle_limiter(iter, resid);
         fC = c_old;
         return ch;
     }
@@ -925,6 +966,7 @@ struct CzStepper {
         for (int i = 0; i < n3; ++i) mid[i] = (c_old[i] + c_new[i]) * 0.5;
         fC = mid;
         ch = c_stage(iter, resid);
+        if (ch == 0) ch = settle_limiter(iter, resid);
         fC = c_old;
         return ch;
     }
@@ -1064,6 +1106,8 @@ struct CzStepper {
         const double dz0 = (1.0 - 0.0) / (double)c.nz;
         dz = ((1.0 + 2 * dz0) - (0.0 - 2 * dz0)) / (double)Nz;
         for (int k = 0; k < 4; ++k) wall[k] = c.bc[k] != YZ_BC_PERIODIC;
+        clim = c.conc_scheme == YZ_CONV_KOREN || c.conc_scheme == YZ_CONV_VANLEER;
+        lim.vanleer = c.conc_scheme == YZ_CONV_VANLEER;
         {
             double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
             switch (c.conc_scheme) {
@@ -1251,6 +1295,8 @@ void yzo_cz_volume_fluxes(void *p, double *qx, double *qy, double *w) {
     std::copy(st->Qy.begin(), st->Qy.end(), qy);
     std::copy(st->W.begin(), st->W.end(), w);
 }
+/* SettleLimiter's solves over the run (a limited scheme) */
+long long yzo_cz_settle(void *p) { return static_cast<yzo::CzStepper *>(p)->settle_passes; }
 void yzo_cz_stats(void *p, double *out8) {
     auto *st = static_cast<yzo::CzStepper *>(p);
     for (int k = 0; k < 8; ++k) out8[k] = st->st_[k];

@@ -26,6 +26,8 @@ index, or a transposed axis fails one of them:
 * RedistributeExcess and RedistributeDeficit on small columns with
   hand-computed answers.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -235,3 +237,111 @@ def test_redistribution_on_small_columns():
     sol = (2.0 * r.mean(axis=0)).sum()
     assert abs(sol - (2.0 * (8 * 0.5 - 0.2))) < 1e-13
     assert ad == 0.0
+
+
+# ---- the limited concentration schemes (Koren, van Leer): the limiter
+# frozen at the field each stage is solved at (AdvWeights,
+# CZ_ConcEquation_2D.cpp:575-593), settled to the midpoint of the step
+# (SettleLimiter :945-973)
+
+LIMITED = [capi.CONV_KOREN, capi.CONV_VANLEER]
+
+
+def _lateral_front(scheme, profile, lib=None, n=24, nz=8, Df=1e-3):
+    """a flat film carried along x by in-plane gravity (uniform volume
+    flux, no vertical flux), c varying along x only"""
+    L = 4.0
+    c = _case(n=n, nz=nz, L=L, per=True, inplane_gravity=1, Pgx=2.0, Df=Df)
+    c.conc_scheme = scheme
+    x = _grid(n, L)
+    X, _ = np.meshgrid(x, x)
+    h = np.ones_like(X)
+    cc = np.broadcast_to(profile(X)[None], (c.nz + 4, n + 4, n + 4)).copy()
+    return c, h.ravel(), cc.ravel(), ob.OracleCZ(c, h.ravel(), np.zeros(h.size), cc.ravel(), 0.05, lib=lib)
+
+
+def limited_midpoint_residual(scheme, lib=None):
+    """One step must be the midpoint rule of the limited scheme:
+    (c1 - c0)/dt + (G(c_m)_{i+1/2} - G(c_m)_{i-1/2})/dx = 0, c_m = (c0 + c1)/2,
+    G = Q (c_U + phi(c_U - c_UU, c_D - c_U)/2) with the limiter in its
+    published form; returns the residual over the change's scale"""
+    from tests.test_oracle_conc import _koren_vl_phi
+    L, dt = 4.0, 0.05
+    # diffusion negligible against the advection (the check leaves it out)
+    c, h, cc, o = _lateral_front(
+        scheme, lambda X: 0.4 + 0.2 * np.sin(2 * np.pi * X / L) + 0.1 * np.cos(6 * np.pi * X / L), lib,
+        n=16, nz=4, Df=1e-14)
+    assert o.advance()[0] == 0
+    n = c.nx
+    qx, qy, w = o.volume_fluxes()
+    Q = qx.reshape(c.nz + 4, n + 4, n + 4)[4, 5, 2:-2]
+    assert np.allclose(Q, Q[0], rtol=1e-14) and Q[0] > 0 and np.abs(w).max() == 0 and np.abs(qy).max() == 0
+    c0 = _real3(c, cc)[2, 3, :]
+    c1 = _real3(c, o.c())[2, 3, :]
+    m = 0.5 * (c0 + c1)
+    mm, mp = np.roll(m, 1), np.roll(m, -1)
+    G = Q[0] * (m + 0.5 * _koren_vl_phi(m - mm, mp - m, scheme == capi.CONV_VANLEER))
+    r = (c1 - c0) / dt + (G - np.roll(G, 1)) / (L / n)
+    return float(np.abs(r).max() / (np.abs(c1 - c0).max() / dt)), o.settle()
+
+
+@pytest.mark.parametrize("scheme", LIMITED, ids=["koren", "vanleer"])
+def test_limited_step_is_the_limited_midpoint_rule(scheme):
+    rel, settle = limited_midpoint_residual(scheme)
+    print(f"limited midpoint residual {rel:.2e} of the change, SettleLimiter solves {settle}")
+    assert rel <= 1e-10 and settle >= 1
+
+
+@pytest.mark.parametrize("scheme", LIMITED, ids=["koren", "vanleer"])
+def test_limited_front_makes_no_new_extrema(scheme):
+    def bounds(sch):
+        c, _, _, o = _lateral_front(sch, lambda X: np.where((X > 1.0) & (X < 2.5), 0.6, 0.2), n=16, nz=4)
+        lo, hi = 1.0, 0.0
+        for _ in range(6):
+            assert o.advance()[0] == 0
+            c3 = _real3(c, o.c())
+            lo, hi = min(lo, c3.min()), max(hi, c3.max())
+        return lo, hi
+    lo, hi = bounds(scheme)
+    assert lo >= 0.2 - 1e-13 and hi <= 0.6 + 1e-13, (lo, hi)
+    lo, hi = bounds(capi.CONV_UTOPIA)
+    assert lo < 0.2 - 1e-2 and hi > 0.6 + 1e-2
+
+
+@pytest.mark.parametrize("scheme", LIMITED, ids=["koren", "vanleer"])
+def test_limited_solute_kept_and_uniform_kept(scheme):
+    c, h, s, cc = _allphys(per=False)
+    c.conc_scheme = scheme
+    o = ob.OracleCZ(c, h, s, cc, 0.02)
+    sol0 = _solute(c, h, cc)
+    for _ in range(3):
+        assert o.advance()[0] == 0
+        assert abs(_solute(c, o.h(), o.c()) - sol0) <= 1e-13 * sol0
+    c.evaporation = 0
+    o = ob.OracleCZ(c, h, s, np.full_like(cc, 0.37), 0.02)
+    for _ in range(3):
+        assert o.advance()[0] == 0
+    assert np.abs(_real3(c, o.c()) - 0.37).max() < 1e-14
+
+
+def test_limited_midpoint_check_fails_without_settling(tmp_path):
+    """the same check on a copy of the oracle whose SettleLimiter does
+    nothing (the limiter left at the predictor's midpoint)"""
+    import shutil
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    d = tmp_path / "oracle"
+    d.mkdir()
+    for part in ("yz_oracle.cpp", "yz_struct_mg.h"):
+        shutil.copy(os.path.join(root, "oracle", part), d)
+    src = open(os.path.join(root, "oracle", "yz_cz_oracle.cpp")).read()
+    old = "        if (!clim) return 0;\n        vector<double> mid(n3);"
+    assert src.count(old) == 1
+    (d / "yz_cz_oracle.cpp").write_text(src.replace(old, "        return 0;\n        vector<double> mid(n3);"))
+    os.symlink(os.path.join(root, "include"), tmp_path / "include")
+    so = str(d / "liboracle.so")
+    subprocess.check_call(["g++", "-O0", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-w", "-o", so,
+                           str(d / "yz_oracle.cpp")])
+    rel, settle = limited_midpoint_residual(capi.CONV_KOREN, lib=ob.load_from(so))
+    print(f"without settling: midpoint residual {rel:.2e} of the change")
+    assert settle == 0 and rel > 1e-8

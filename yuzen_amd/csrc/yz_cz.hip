@@ -45,6 +45,7 @@
 #include <vector>
 
 #include "../../include/yuzen_gpu.h"
+#include "yz_limiter.h"
 #include "yz_mlp.h"
 #include "yz_schemes.h"
 #include "yz_solver.h"
@@ -117,6 +118,10 @@ struct Args {
     yz_cz_case cs;
     double dt, theta;
     double w1[4], w2[4];       // face weights of the scheme: c1, c2
+    // a limited scheme (Koren, or van Leer: lim_vl): the weights of a face
+    // frozen at the field fC, upwind form for the upwind cell, downwind form
+    // for the other (AdvWeights, CZ_ConcEquation_2D.cpp:575-593)
+    int lim, lim_vl;
     const double *S;
     double *hO, *hN, *fH, *tH, *fC, *tC;
     double *Dx, *Dy, *sgx, *sgy, *Mx, *My;
@@ -767,8 +772,19 @@ __global__ void k_crows(Args a, const double *cold) {
             const int hi = side, lo = side - 1;       // the face's cells, offsets along the axis
             const double Q = Qf[r3 + side * fstride];
             const double sq = sgn(Q);
+            double w[4];
+            if (a.lim) {
+                // the cell is R of its low face, L of its high one
+                double f[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) add((lo - 1 + t) * ex, (lo - 1 + t) * ey, 0, sg * Q * (a.w1[t] + sq * a.w2[t]) / ds);
+                for (int t = 0; t < 4; ++t) f[t] = cc(ix + (lo - 1 + t) * ex, jy + (lo - 1 + t) * ey, kz);
+                limited_weights(f, Q, side ? 1 : 2, a.lim_vl, w);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) w[t] = a.w1[t] + sq * a.w2[t];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) add((lo - 1 + t) * ex, (lo - 1 + t) * ey, 0, sg * Q * w[t] / ds);
             if (wall_face(a, ix + hi * ex, jy + hi * ey, axis)) continue;
             const int li = ix + lo * ex, lj = jy + lo * ey, hi_i = ix + hi * ex, hi_j = jy + hi * ey;
             const double hf = 0.5 * (hh(li, lj) + hh(hi_i, hi_j));
@@ -801,8 +817,18 @@ __global__ void k_crows(Args a, const double *cold) {
             const double W = a.W[r3 + side * g.n2];
             if (side == 1 && top) { rct += W / dz; continue; }
             const double sw = sgn(W);
+            double w[4];
+            if (a.lim) {
+                double f[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) add(0, 0, lo - 1 + t, sg * W * (a.w1[t] + sw * a.w2[t]) / dz);
+                for (int t = 0; t < 4; ++t) f[t] = cc(ix, jy, kz + lo - 1 + t);
+                limited_weights(f, W, side ? 1 : 2, a.lim_vl, w);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) w[t] = a.w1[t] + sw * a.w2[t];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) add(0, 0, lo - 1 + t, sg * W * w[t] / dz);
             const double zf = zeta + (side ? 0.5 : -0.5) * dz;
             const double Dz = cs.Df * diffu(cs, 0.5 * (cc(ix, jy, kz + lo) + cc(ix, jy, kz + hi)));
             const double gx = hx0 * zf + sx0, gy = hy0 * zf + sy0;
@@ -1098,6 +1124,23 @@ __global__ void k_resid(int n3, const double *c0, const double *c, const double 
     }
     for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o));
     if ((threadIdx.x & 63) == 0 && e > 0.0) atomicMax(emax, (unsigned long long)__double_as_longlong(e));
+}
+// SettleLimiter's test: emax[0] = max |c - prev|, emax[1] = max |c - c_old|
+// (bits of non-negative doubles, atomicMax); then prev = c
+__global__ void k_settle_max(int n3, const double *c, double *prev, const double *cold, unsigned long long *emax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0, f = 0.0;
+    if (i < n3) {
+        const double v = c[i];
+        e = fabs(v - prev[i]);
+        f = fabs(v - cold[i]);
+        prev[i] = v;
+    }
+    for (int o = 32; o > 0; o >>= 1) { e = fmax(e, __shfl_xor(e, o)); f = fmax(f, __shfl_xor(f, o)); }
+    if ((threadIdx.x & 63) == 0) {
+        if (e > 0.0) atomicMax(emax, (unsigned long long)__double_as_longlong(e));
+        if (f > 0.0) atomicMax(emax + 1, (unsigned long long)__double_as_longlong(f));
+    }
 }
 __global__ void k_aitken(int n3, const double *r, const double *rp, double *part) {
     double acc[2] = {0.0, 0.0};
@@ -1436,6 +1479,7 @@ struct yz_cz {
     double *fC = nullptr, *tC = nullptr, *cO = nullptr, *cN = nullptr, *c0 = nullptr, *cm = nullptr;
     double *cPrev = nullptr, *keepO = nullptr, *keepN = nullptr, *prevKeep = nullptr, *r = nullptr, *rp = nullptr;
     double *cn = nullptr, *d = nullptr, *ct = nullptr;
+    double *cSettle = nullptr;              // SettleLimiter's previous answer
     double *f2[16] = {nullptr};
     double *Qx = nullptr, *Qy = nullptr, *W = nullptr, *zr = nullptr;
     double *cA = nullptr, *react = nullptr, *cP = nullptr, *cb = nullptr;
@@ -1759,10 +1803,41 @@ void volume_fluxes(yz_cz *z) {
     hipLaunchKernelGGL(k_w, dim3(nblk(g.nx * g.ny)), dim3(TBK), 0, z->st, z->a);
 }
 
+// SettleLimiter (CZ_ConcEquation_2D.cpp:945-973): a limited scheme's step
+// solved again with its limiter frozen at the midpoint of what came out,
+// until that settles to 1e-12 of the step's change (8 passes at most)
+int c_settle(yz_cz *z, int &iter, double &resid) {
+    if (!z->a.lim) return 0;
+    const Geo3 &g = z->g;
+    for (int pass = 0; pass < 8; ++pass) {
+        hipLaunchKernelGGL(k_avg, dim3(nblk(g.n3)), dim3(TBK), 0, z->st, g.n3, (const double *)z->cO,
+                           (const double *)z->cN, z->fC);
+        if (cp_d2d(z, z->cSettle, z->cN, g.n3)) return -1;
+        int it = 0;
+        double rs = 0;
+        const int ch = c_stage(z, it, rs);
+        iter += it;
+        resid = rs;
+        if (ch != 0) return ch;
+        CZ_CHECK(hipMemsetAsync(z->emax, 0, 2 * sizeof(unsigned long long), z->st));
+        hipLaunchKernelGGL(k_settle_max, dim3(nblk(g.n3)), dim3(TBK), 0, z->st, g.n3, (const double *)z->cN,
+                           z->cSettle, (const double *)z->cO, z->emax);
+        unsigned long long eb[2];
+        CZ_CHECK(hipMemcpyAsync(eb, z->emax, sizeof(eb), hipMemcpyDeviceToHost, z->st));
+        CZ_CHECK(hipStreamSynchronize(z->st));
+        double dmax, cmax;
+        std::memcpy(&dmax, &eb[0], sizeof(dmax));
+        std::memcpy(&cmax, &eb[1], sizeof(cmax));
+        if (!(dmax > 1e-12 * cmax)) break;
+    }
+    return 0;
+}
+
 int c_solve_at_mid(yz_cz *z, const double *cmid, int &iter, double &resid) {
     const size_t n3 = z->g.n3;
     if (cp_d2d(z, z->fC, cmid, n3)) return -1;
-    const int ch = c_stage(z, iter, resid);
+    int ch = c_stage(z, iter, resid);
+    if (ch == 0) ch = c_settle(z, iter, resid);
     if (cp_d2d(z, z->fC, z->cO, n3)) return -1;
     return ch;
 }
@@ -1774,6 +1849,7 @@ int c_solve(yz_cz *z, int &iter, double &resid) {
     hipLaunchKernelGGL(k_avg, dim3(nblk(g.n3)), dim3(TBK), 0, z->st, g.n3, (const double *)z->cO,
                        (const double *)z->cN, z->fC);
     ch = c_stage(z, iter, resid);
+    if (ch == 0) ch = c_settle(z, iter, resid);
     if (cp_d2d(z, z->fC, z->cO, g.n3)) return -1;
     return ch;
 }
@@ -1993,8 +2069,8 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
         yz::set_error("ConcZ: film_precond is YZ_PC_AMG or YZ_PC_LEARNED");
         return nullptr;
     }
-    if (c->conc_scheme < YZ_CONV_UPWIND1 || c->conc_scheme > YZ_CONV_KK) {
-        yz::set_error("ConcZ: unknown concentration scheme (the limited schemes are not taken)");
+    if (c->conc_scheme < YZ_CONV_UPWIND1 || c->conc_scheme > YZ_CONV_VANLEER) {
+        yz::set_error("ConcZ: unknown concentration scheme");
         return nullptr;
     }
     if (!(dt > 0) || !(c->xmax > c->xmin) || !(c->ymax > c->ymin) || c->lin_maxit < 1) {
@@ -2027,14 +2103,14 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
     for (double **p : a2) ok = ok && !dal(p, n2);
     for (int k = 0; k < 16; ++k) ok = ok && !dal(&z->f2[k], n2);
     double **a3[] = {&z->fC, &z->tC, &z->cO, &z->cN, &z->c0, &z->cm, &z->cPrev, &z->keepO, &z->keepN,
-                     &z->prevKeep, &z->r, &z->rp, &z->cn, &z->d, &z->ct, &z->Qx, &z->Qy, &z->W};
+                     &z->prevKeep, &z->r, &z->rp, &z->cn, &z->d, &z->ct, &z->Qx, &z->Qy, &z->W, &z->cSettle};
     for (double **p : a3) ok = ok && !dal(p, n3);
     ok = ok && !dal(&z->zr, 4 * n2);
     ok = ok && !dal(&z->cA, 21 * ni) && !dal(&z->cP, 21 * ni) && !dal(&z->react, ni) && !dal(&z->cb, ni);
     for (int k = 0; k < 9; ++k) ok = ok && !dal(&z->kv[k], ni);
     for (int k = 0; k < 5; ++k) ok = ok && !dal(&z->lu[k], ni);
     for (int k = 0; k < 4; ++k) ok = ok && !dal(&z->colv[k], (size_t)g.nx * g.ny);
-    ok = ok && !dal(&z->part, 4 * RB) && !dal(&z->dsc, 8) && !dal(&z->flags, 4) && !dal(&z->emax, 1);
+    ok = ok && !dal(&z->part, 4 * RB) && !dal(&z->dsc, 8) && !dal(&z->flags, 4) && !dal(&z->emax, 2);
     ok = ok && hipHostMalloc((void **)&z->hsc, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&z->hflags, 4 * sizeof(int), hipHostMallocDefault) == hipSuccess;
     // the film's solver over the real cells
@@ -2081,9 +2157,9 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
     Args &a = z->a;
     std::memset(&a, 0, sizeof(a));
     a.g = g; a.cs = *c; a.dt = dt; a.theta = 0.5;
-    {
-        scheme_coefs(c->conc_scheme, a.w1, a.w2);
-    }
+    a.lim = c->conc_scheme == YZ_CONV_KOREN || c->conc_scheme == YZ_CONV_VANLEER;
+    a.lim_vl = c->conc_scheme == YZ_CONV_VANLEER;
+    if (!a.lim) scheme_coefs(c->conc_scheme, a.w1, a.w2);
     a.S = z->S; a.hO = z->hO; a.hN = z->hN; a.fH = z->fH; a.tH = z->tH; a.fC = z->fC; a.tC = z->tC;
     a.Dx = z->f2[0]; a.Dy = z->f2[1]; a.sgx = z->f2[2]; a.sgy = z->f2[3]; a.Mx = z->f2[4]; a.My = z->f2[5];
     a.qCapX = z->f2[6]; a.qExpX = z->f2[7]; a.qCapY = z->f2[8]; a.qExpY = z->f2[9]; a.qX = z->f2[10];
@@ -2217,7 +2293,7 @@ void yz_cz_destroy(yz_cz *z) {
     if (z->hnonunit) (void)hipHostFree(z->hnonunit);
     double *ptrs[] = {z->S, z->hO, z->hN, z->fH, z->tH, z->mid, z->hO_keep, z->hN_keep, z->fC, z->tC, z->cO,
                       z->cN, z->c0, z->cm, z->cPrev, z->keepO, z->keepN, z->prevKeep, z->r, z->rp, z->cn, z->d,
-                      z->ct, z->Qx, z->Qy, z->W, z->zr, z->cA, z->react, z->cP, z->cb, z->part, z->dsc,
+                      z->ct, z->Qx, z->Qy, z->W, z->zr, z->cA, z->react, z->cP, z->cb, z->part, z->dsc, z->cSettle,
                       z->a.hb, z->a.hx};
     for (double *p : ptrs) if (p) (void)hipFree(p);
     for (double *p : z->f2) if (p) (void)hipFree(p);
