@@ -651,6 +651,18 @@ typedef struct yz_cz_case {
     int film_precond;
     int learn_passes, learn_iters, learn_probes;
     double learn_lr;
+    /* Numerics.TimeStepControl (C_SemiImplicitEquation::TimeStepTrial,
+     * SemiImplicit.cpp:100-148; TimeStepAdjustment and
+     * LocalTruncationCriterion of the film, ImplicitSolver.cpp:144-190,
+     * 271-307): a step whose solve fails or whose film's truncation
+     * criterion is not positive is solved again at tsc_change times the
+     * step (the run fails below dt_min); after tsc_keep accepted steps with
+     * the criterion above 0.9 the next step is dt / tsc_change, at most
+     * dt_max. Off (0) by default: a failed step returns its code. */
+    int step_control;
+    double tsc_change, tsc_criterion, tsc_floor;
+    int tsc_keep;
+    double dt_min, dt_max;
 } yz_cz_case;
 
 void yz_cz_case_defaults(yz_cz_case *c);
@@ -670,6 +682,9 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s,
  * concentration), passes: coupling passes taken again, resid: the last
  * film solve's relative residual. */
 int yz_cz_advance(yz_cz *st, int *lin_iter, int *passes, double *resid);
+/* the step control's clock: [0] the step the next advance takes, [1] the
+ * last accepted step (DT().old), [2] the trials it rejected since create */
+void yz_cz_clock(yz_cz *st, double *out3);
 void yz_cz_get_h(yz_cz *st, double *h);
 void yz_cz_get_c(yz_cz *st, double *conc);
 /* the volume fluxes of the last concentration solve (BuildVolumeFluxes):

@@ -89,6 +89,7 @@ def load_from(path):
     lib.yzo_cz_volume_fluxes.argtypes = [C.c_void_p, pd, pd, pd]
     lib.yzo_cz_stats.argtypes = [C.c_void_p, pd]
     lib.yzo_cz_settle.argtypes = [C.c_void_p]
+    lib.yzo_cz_clock.argtypes = [C.c_void_p, pd]
     lib.yzo_cz_settle.restype = C.c_longlong
     lib.yzo_cz_velocity.argtypes = [C.c_void_p, pd, pd, pd, pd]
     lib.yzo_cz_redistribute.restype = C.c_double
@@ -336,6 +337,12 @@ class OracleCZ:
     def stats(self):
         out = np.empty(8)
         self.lib.yzo_cz_stats(self.p, _pd(out))
+        return out
+
+    def clock(self):
+        """(dt now, dt old, trials the step control rejected)"""
+        out = np.empty(3)
+        self.lib.yzo_cz_clock(self.p, _pd(out))
         return out
 
     def settle(self):

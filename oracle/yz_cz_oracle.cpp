@@ -1124,6 +1124,7 @@ struct CzStepper {
             for (int j = 0; j < 4; ++j) { c1w[j] = a[j] / den; c2w[j] = b[j] / den; }
         }
         dt = dt0;
+        dt_old = dt0;                  /* DT().old starts at the initial step */
         S.assign(s, s + n2);
         h_old.assign(h, h + n2); h_new = h_old;
         fH = h_old; tH = h_old;
@@ -1137,8 +1138,63 @@ struct CzStepper {
         return 0;
     }
 
-    /* ---- one step: TimeAdvance + Solve + AfterTimeStepTrial ---- */
+    /* ---- the step control (TimeStepTrial, SemiImplicit.cpp:100-148) ---- */
+    vector<double> h_old2;             /* _phi_old2: the start of the step before */
+    double dt_old = 0;                 /* DT().old */
+    int tsc_count = 0;
+    /* LocalTruncationCriterion (ImplicitSolver.cpp:271-307) of the film the
+     * step's solve gave, over every row */
+    double truncation_criterion() const {
+        double th = cs.tsc_criterion;
+        if (std::fabs(th) <= DBL_MIN) th = 1.0e-2;
+        if (std::fabs(dt_old) <= DBL_MIN) return 1.0;
+        double hmax = 0;
+        for (int i = 0; i < n2; ++i) hmax = std::max(hmax, std::fabs(h_old[i]));
+        const double wfloor = cs.tsc_floor * hmax;
+        double e = 0;
+        for (int i = 0; i < n2; ++i) {
+            double w = std::fabs(h_old[i]);
+            if (w < wfloor) w = wfloor;
+            const double den = (dt_old + dt) * w;
+            const double ei = (std::fabs(den) <= DBL_MIN)
+                ? 0.0 : 2.0 * (dt / dt_old) * (dt_old * h_new[i] + dt * h_old2[i] - (dt_old + dt) * h_old[i]) / den;
+            e = std::max(e, std::fabs(ei));
+        }
+        return 1.0 - e / th;
+    }
+    /* TimeAdvance with the step control: the trial repeated at a shorter
+     * step until one is accepted (TimeStepAdjustment, ImplicitSolver.cpp
+     * :144-190); 1 when the step falls below dt_min. Without it, one trial. */
     int advance(int &lin, int &passes_out, double &resid_out) {
+        if (!cs.step_control) return trial(lin, passes_out, resid_out, nullptr);
+        h_old2 = h_old;
+        int lin_sum = 0;
+        for (;;) {
+            double crit = -1.0;
+            const int rc = trial(lin, passes_out, resid_out, &crit);
+            lin_sum += lin;
+            if (rc == 0 && crit > 0.0) {
+                double dt_next = dt;
+                if (crit > 0.90 && dt < cs.dt_max && ++tsc_count >= cs.tsc_keep) {
+                    tsc_count = 0;
+                    dt_next = std::min(dt / cs.tsc_change, cs.dt_max);
+                }
+                dt_old = dt;
+                dt = dt_next;
+                lin = lin_sum;
+                return 0;
+            }
+            ++cuts;
+            dt *= cs.tsc_change;
+            if (dt < cs.dt_min) { lin = lin_sum; return 1; }
+        }
+    }
+    long long cuts = 0;                /* trials the step control rejected */
+
+    /* ---- one trial: TimeAdvance + Solve + AfterTimeStepTrial; with crit,
+     * the film's truncation criterion after its solve, and a film it
+     * rejects is not carried on (4, the step's start kept) ---- */
+    int trial(int &lin, int &passes_out, double &resid_out, double *crit) {
         lin = 0; passes_out = 0; resid_out = 0;
         const vector<double> h_old_keep = h_old, h_new_keep = h_new;
         h_old = h_new;
@@ -1162,6 +1218,10 @@ struct CzStepper {
         fH = h_old;
         if (extrap) fC = c_new;
         if (ch != 0) { h_old = h_old_keep; h_new = h_new_keep; fH = tH; fC = tC; return 1; }
+        if (crit) {
+            *crit = truncation_criterion();
+            if (!(*crit > 0.0)) { lin = iter; h_old = h_old_keep; h_new = h_new_keep; fH = tH; fC = tC; return 4; }
+        }
 
         /* AfterTimeStepTrial (:207-386) */
         conservative_update();
@@ -1266,6 +1326,8 @@ void yzo_cz_case_defaults(yz_cz_case *c) {
     c->molar_resin = 2200; c->molar_solvent = 106;
     c->conc_scheme = YZ_CONV_UTOPIA;
     c->lin_maxit = 5000; c->h_eps = 1e-13; c->c_eps = 1e-13;
+    c->step_control = 0; c->tsc_change = 0.5; c->tsc_criterion = 0.01; c->tsc_floor = 0; c->tsc_keep = 50;
+    c->dt_min = 1e-15; c->dt_max = 1e30;
 }
 void *yzo_cz_create(const yz_cz_case *c, const double *h, const double *s, const double *conc, double dt) {
     auto *st = new yzo::CzStepper();
@@ -1297,6 +1359,11 @@ void yzo_cz_volume_fluxes(void *p, double *qx, double *qy, double *w) {
 }
 /* SettleLimiter's solves over the run (a limited scheme) */
 long long yzo_cz_settle(void *p) { return static_cast<yzo::CzStepper *>(p)->settle_passes; }
+/* the clock of the step control: dt now and old, the rejected trials */
+void yzo_cz_clock(void *p, double *out3) {
+    auto *st = static_cast<yzo::CzStepper *>(p);
+    out3[0] = st->dt; out3[1] = st->dt_old; out3[2] = (double)st->cuts;
+}
 void yzo_cz_stats(void *p, double *out8) {
     auto *st = static_cast<yzo::CzStepper *>(p);
     for (int k = 0; k < 8; ++k) out8[k] = st->st_[k];

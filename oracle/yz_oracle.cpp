@@ -99,6 +99,7 @@
 #include "../include/yuzen_gpu.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>

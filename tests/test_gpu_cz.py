@@ -87,6 +87,17 @@ def test_limited_schemes_match_oracle(scheme):
     assert c3.min() >= 0.2 - 1e-13 and c3.max() <= 0.6 + 1e-13
 
 
+def test_step_control_matches_oracle():
+    """the step control: the same cuts and the same accepted steps as the
+    oracle, films and c within 1e-10 after every step"""
+    c, h, s, cc = _allphys(per=False)
+    c.step_control, c.tsc_keep, c.tsc_criterion = 1, 2, 2e-4
+    o, g = _pair(c, h, s, cc, 0.02, 6, "step control all-physics 12^2 x 8")
+    co, cg = o.clock(), g.clock()
+    print(f"oracle clock {co}, device clock {cg}")
+    assert co[0] == cg[0] and co[1] == cg[1] and co[2] == cg[2] and co[2] >= 9
+
+
 def test_without_marangoni_matches_oracle():
     c, h, s, cc = _allphys(per=False, n=14, nz=6)
     c.marangoni = 0

@@ -345,3 +345,60 @@ def test_limited_midpoint_check_fails_without_settling(tmp_path):
     rel, settle = limited_midpoint_residual(capi.CONV_KOREN, lib=ob.load_from(so))
     print(f"without settling: midpoint residual {rel:.2e} of the change")
     assert settle == 0 and rel > 1e-8
+
+
+# ---- the step control (TimeStepTrial, SemiImplicit.cpp:100-148;
+# TimeStepAdjustment and LocalTruncationCriterion, ImplicitSolver.cpp
+# :144-190, 271-307)
+
+def _criterion(c, hm, ho, hp, dt0, dt):
+    """LocalTruncationCriterion from its published form, over every row"""
+    th = c.tsc_criterion if c.tsc_criterion != 0 else 1e-2
+    w = np.maximum(np.abs(ho), c.tsc_floor * np.abs(ho).max())
+    e = np.abs(2.0 * (dt / dt0) * (dt0 * hp + dt * hm - (dt0 + dt) * ho) / ((dt0 + dt) * w))
+    return 1.0 - e.max() / th
+
+
+def test_step_control_accepts_by_the_truncation_criterion():
+    """a tight criterion: the steps are cut until accepted, every accepted
+    step has a positive criterion computed from the films themselves, a
+    step is never longer than the last accepted one unless tsc_keep steps
+    in a row had a criterion above 0.9, and then it is doubled"""
+    c, h, s, cc = _allphys(per=False)
+    c.step_control, c.tsc_keep, c.tsc_criterion = 1, 2, 2e-4
+    dt0 = 0.02
+    o = ob.OracleCZ(c, h, s, cc, dt0)
+    films, dts = [o.h()], []
+    prev_dt, cuts0 = dt0, 0
+    above = 0
+    for k in range(6):
+        assert o.advance()[0] == 0
+        nxt, old, cuts = o.clock()
+        dts.append(old)                   # the accepted step
+        films.append(o.h())
+        assert old <= prev_dt and np.isclose(np.log2(prev_dt / old), round(np.log2(prev_dt / old)))
+        if k >= 1:
+            crit = _criterion(c, films[-3], films[-2], films[-1], dts[-2], dts[-1])
+            assert crit > 0.0, (k, crit)
+            above = above + 1 if crit > 0.9 else 0
+        if nxt > old:
+            assert nxt == 2 * old
+        prev_dt = nxt
+    assert cuts >= 9                     # 0.02 cut at least nine times to 3.9e-5
+    assert any(b > a for a, b in zip(dts, dts[1:])) or nxt > dts[-1]
+
+
+def test_step_control_with_a_loose_criterion_is_the_fixed_step():
+    """a criterion that never rejects: the first steps are the fixed-step
+    run's to the bit, and the step doubles every tsc_keep steps"""
+    c, h, s, cc = _allphys(per=True)
+    dt0 = 0.01
+    f = ob.OracleCZ(c, h, s, cc, dt0)
+    cs = c.copy()
+    cs.step_control, cs.tsc_keep, cs.tsc_criterion = 1, 2, 1e6
+    o = ob.OracleCZ(cs, h, s, cc, dt0)
+    for k in range(2):
+        assert f.advance()[0] == 0 and o.advance()[0] == 0
+        assert np.array_equal(f.h(), o.h()) and np.array_equal(f.c(), o.c())
+    assert o.clock()[0] == 2 * dt0 and o.clock()[2] == 0
+    assert o.advance()[0] == 0 and o.clock()[1] == 2 * dt0

@@ -178,6 +178,10 @@ class YzCzCase(C.Structure):
         ("film_precond", C.c_int),
         ("learn_passes", C.c_int), ("learn_iters", C.c_int), ("learn_probes", C.c_int),
         ("learn_lr", C.c_double),
+        ("step_control", C.c_int),
+        ("tsc_change", C.c_double), ("tsc_criterion", C.c_double), ("tsc_floor", C.c_double),
+        ("tsc_keep", C.c_int),
+        ("dt_min", C.c_double), ("dt_max", C.c_double),
     ]
 
     def copy(self):
@@ -202,6 +206,8 @@ def cz_case_defaults():
     c.lin_maxit, c.h_eps, c.c_eps = 5000, 1e-13, 1e-13
     c.film_precond = PC_AMG
     c.learn_passes, c.learn_iters, c.learn_probes, c.learn_lr = 3, 150, 4, 2e-3
+    c.step_control, c.tsc_change, c.tsc_criterion, c.tsc_floor, c.tsc_keep = 0, 0.5, 0.01, 0.0, 50
+    c.dt_min, c.dt_max = 1e-15, 1e30
     return c
 
 
@@ -290,6 +296,7 @@ def load():
     lib.yz_cz_get_c.argtypes = [C.c_void_p, pd]
     lib.yz_cz_volume_fluxes.argtypes = [C.c_void_p, pd, pd, pd]
     lib.yz_cz_stats.argtypes = [C.c_void_p, pd]
+    lib.yz_cz_clock.argtypes = [C.c_void_p, pd]
     lib.yz_cz_learn_stats.argtypes = [C.c_void_p, pd]
     lib.yz_cz_destroy.argtypes = [C.c_void_p]
     lib.yz_cz_save_state.argtypes = [C.c_void_p, C.c_char_p]

@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1480,6 +1481,12 @@ struct yz_cz {
     double *cPrev = nullptr, *keepO = nullptr, *keepN = nullptr, *prevKeep = nullptr, *r = nullptr, *rp = nullptr;
     double *cn = nullptr, *d = nullptr, *ct = nullptr;
     double *cSettle = nullptr;              // SettleLimiter's previous answer
+    // the step control: the start of the step before (_phi_old2), DT().old,
+    // the count of accepted steps toward lengthening, the rejected trials
+    double *hO2 = nullptr;
+    double dt_old = 0;
+    int tsc_count = 0;
+    long long cuts = 0;
     double *f2[16] = {nullptr};
     double *Qx = nullptr, *Qy = nullptr, *W = nullptr, *zr = nullptr;
     double *cA = nullptr, *react = nullptr, *cP = nullptr, *cb = nullptr;
@@ -1894,7 +1901,53 @@ int redistribute_deficit(yz_cz *z, double &moved, double &added) {
 }
 
 // one step (TimeAdvance, Solve, AfterTimeStepTrial)
-int cz_advance(yz_cz *z, int &lin, int &passes_out, double &resid_out) {
+// LocalTruncationCriterion (ImplicitSolver.cpp:271-307) of the film the
+// trial's solve gave: max over every row of |2 (dt/dt0) (dt0 h+ + dt h- -
+// (dt0 + dt) h) / ((dt0 + dt) w)|, w = |h| floored at tsc_floor max |h|
+__global__ void k_cz_absmax(int n, const double *v, unsigned long long *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = i < n ? fabs(v[i]) : 0.0;
+    for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o));
+    if ((threadIdx.x & 63) == 0 && e > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(e));
+}
+__global__ void k_cz_trunc(int n, const double *hm, const double *ho, const double *hp, double dt0, double dt,
+                           double floor_factor, const unsigned long long *hmaxbits, unsigned long long *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (i < n) {
+        const double wfloor = floor_factor * __longlong_as_double((long long)*hmaxbits);
+        double w = fabs(ho[i]);
+        if (w < wfloor) w = wfloor;
+        const double den = (dt0 + dt) * w;
+        e = (fabs(den) <= DBL_MIN) ? 0.0 : fabs(2.0 * (dt / dt0) * (dt0 * hp[i] + dt * hm[i] - (dt0 + dt) * ho[i]) / den);
+    }
+    for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o));
+    if ((threadIdx.x & 63) == 0 && e > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(e));
+}
+int cz_criterion(yz_cz *z, double *crit) {
+    const yz_cz_case &c = z->cs;
+    double th = c.tsc_criterion;
+    if (fabs(th) <= DBL_MIN) th = 1.0e-2;
+    if (fabs(z->dt_old) <= DBL_MIN) { *crit = 1.0; return 0; }
+    const int n = (int)z->g.n2;
+    CZ_CHECK(hipMemsetAsync(z->emax, 0, 2 * sizeof(unsigned long long), z->st));
+    hipLaunchKernelGGL(k_cz_absmax, dim3(nblk(n)), dim3(TBK), 0, z->st, n, (const double *)z->hO, z->emax);
+    hipLaunchKernelGGL(k_cz_trunc, dim3(nblk(n)), dim3(TBK), 0, z->st, n, (const double *)z->hO2,
+                       (const double *)z->hO, (const double *)z->hN, z->dt_old, z->dt, c.tsc_floor,
+                       (const unsigned long long *)z->emax, z->emax + 1);
+    unsigned long long eb[2];
+    CZ_CHECK(hipMemcpyAsync(eb, z->emax, sizeof(eb), hipMemcpyDeviceToHost, z->st));
+    CZ_CHECK(hipStreamSynchronize(z->st));
+    double e;
+    std::memcpy(&e, &eb[1], sizeof(e));
+    *crit = 1.0 - e / th;
+    return 0;
+}
+
+// one trial of the step (TimeAdvance, Solve, AfterTimeStepTrial); with
+// crit, the film's truncation criterion after its solve, and a film it
+// rejects is not carried on (4, the step's start kept)
+int cz_advance(yz_cz *z, int &lin, int &passes_out, double &resid_out, double *crit) {
     const Geo3 &g = z->g;
     const size_t n2 = g.n2, n3 = g.n3;
     lin = 0; passes_out = 0; resid_out = 0;
@@ -1924,6 +1977,10 @@ int cz_advance(yz_cz *z, int &lin, int &passes_out, double &resid_out) {
     if (cp_d2d(z, z->fH, z->hO, n2)) return -1;
     if (extrap && cp_d2d(z, z->fC, z->cN, n3)) return -1;
     if (ch != 0) { if (restore()) return -1; return 1; }
+    if (crit) {
+        if (cz_criterion(z, crit)) return -1;
+        if (!(*crit > 0.0)) { lin = iter; if (restore()) return -1; return 4; }
+    }
 
     conservative_update(z);
     if (cp_d2d(z, z->fH, z->hN, n2)) return -1;
@@ -2045,6 +2102,8 @@ void yz_cz_case_defaults(yz_cz_case *c) {
     c->lin_maxit = 5000; c->h_eps = 1e-13; c->c_eps = 1e-13;
     c->film_precond = YZ_PC_AMG;
     c->learn_passes = 3; c->learn_iters = 150; c->learn_probes = 4; c->learn_lr = 2e-3;
+    c->step_control = 0; c->tsc_change = 0.5; c->tsc_criterion = 0.01; c->tsc_floor = 0; c->tsc_keep = 50;
+    c->dt_min = 1e-15; c->dt_max = 1e30;
 }
 
 int yz_sizeof_cz_case(void) { return (int)sizeof(yz_cz_case); }
@@ -2069,6 +2128,11 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
         yz::set_error("ConcZ: film_precond is YZ_PC_AMG or YZ_PC_LEARNED");
         return nullptr;
     }
+    if (c->step_control && (!(c->tsc_change > 0 && c->tsc_change < 1) || c->tsc_keep < 1 || !(c->dt_min > 0) ||
+                            !(c->dt_max >= dt))) {
+        yz::set_error("ConcZ: step control needs 0 < tsc_change < 1, tsc_keep >= 1, dt_min > 0, dt_max >= dt");
+        return nullptr;
+    }
     if (c->conc_scheme < YZ_CONV_UPWIND1 || c->conc_scheme > YZ_CONV_VANLEER) {
         yz::set_error("ConcZ: unknown concentration scheme");
         return nullptr;
@@ -2080,6 +2144,7 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
     yz_cz *z = new yz_cz();
     z->cs = *c;
     z->dt = dt;
+    z->dt_old = dt;   // DT().old starts at the initial step
     Geo3 &g = z->g;
     g.nx = c->nx; g.ny = c->ny; g.nz = c->nz;
     g.Nx = c->nx + 4; g.Ny = c->ny + 4; g.Nz = c->nz + 4;
@@ -2099,7 +2164,7 @@ yz_cz *yz_cz_create(const yz_cz_case *c, const double *h, const double *s, const
     }
     bool ok = hipStreamCreateWithFlags(&z->st, hipStreamNonBlocking) == hipSuccess;
     const size_t n2 = g.n2, n3 = g.n3, ni = g.nint;
-    double **a2[] = {&z->S, &z->hO, &z->hN, &z->fH, &z->tH, &z->mid, &z->hO_keep, &z->hN_keep};
+    double **a2[] = {&z->S, &z->hO, &z->hN, &z->fH, &z->tH, &z->mid, &z->hO_keep, &z->hN_keep, &z->hO2};
     for (double **p : a2) ok = ok && !dal(p, n2);
     for (int k = 0; k < 16; ++k) ok = ok && !dal(&z->f2[k], n2);
     double **a3[] = {&z->fC, &z->tC, &z->cO, &z->cN, &z->c0, &z->cm, &z->cPrev, &z->keepO, &z->keepN,
@@ -2174,11 +2239,49 @@ int yz_cz_advance(yz_cz *z, int *lin_iter, int *passes, double *resid) {
     if (!z) { yz::set_error("ConcZ: null context"); return -1; }
     int l = 0, p = 0;
     double r = 0;
-    const int rc = cz_advance(z, l, p, r);
+    int rc;
+    if (!z->cs.step_control) {
+        rc = cz_advance(z, l, p, r, nullptr);
+    } else {
+        // TimeStepTrial (SemiImplicit.cpp:100-148) with TimeStepAdjustment
+        // (ImplicitSolver.cpp:144-190): the trial repeated at a shorter step
+        // until one is accepted; a step accepted with the criterion above
+        // 0.9 counts toward lengthening the next one
+        const size_t n2 = z->g.n2;
+        if (cp_d2d(z, z->hO2, z->hO, n2)) return -1;
+        int lsum = 0;
+        for (;;) {
+            double crit = -1.0;
+            rc = cz_advance(z, l, p, r, &crit);
+            if (rc < 0) return -1;
+            lsum += l;
+            if (rc == 0 && crit > 0.0) {
+                double dt_next = z->dt;
+                if (crit > 0.90 && z->dt < z->cs.dt_max && ++z->tsc_count >= z->cs.tsc_keep) {
+                    z->tsc_count = 0;
+                    dt_next = std::min(z->dt / z->cs.tsc_change, z->cs.dt_max);
+                }
+                z->dt_old = z->dt;
+                z->dt = dt_next;
+                z->a.dt = dt_next;
+                break;
+            }
+            ++z->cuts;
+            z->dt *= z->cs.tsc_change;
+            z->a.dt = z->dt;
+            if (z->dt < z->cs.dt_min) { yz::set_error("ConcZ: the step fell below dt_min"); rc = 1; break; }
+        }
+        l = lsum;
+    }
     if (lin_iter) *lin_iter = l;
     if (passes) *passes = p;
     if (resid) *resid = r;
     return rc;
+}
+
+void yz_cz_clock(yz_cz *z, double *out3) {
+    if (!z || !out3) return;
+    out3[0] = z->dt; out3[1] = z->dt_old; out3[2] = (double)z->cuts;
 }
 
 void yz_cz_get_h(yz_cz *z, double *h) {
@@ -2291,7 +2394,7 @@ void yz_cz_destroy(yz_cz *z) {
     for (double *p : {z->r64, z->xkeep}) if (p) (void)hipFree(p);
     if (z->nonunit) (void)hipFree(z->nonunit);
     if (z->hnonunit) (void)hipHostFree(z->hnonunit);
-    double *ptrs[] = {z->S, z->hO, z->hN, z->fH, z->tH, z->mid, z->hO_keep, z->hN_keep, z->fC, z->tC, z->cO,
+    double *ptrs[] = {z->S, z->hO, z->hN, z->fH, z->tH, z->mid, z->hO_keep, z->hN_keep, z->hO2, z->fC, z->tC, z->cO,
                       z->cN, z->c0, z->cm, z->cPrev, z->keepO, z->keepN, z->prevKeep, z->r, z->rp, z->cn, z->d,
                       z->ct, z->Qx, z->Qy, z->W, z->zr, z->cA, z->react, z->cP, z->cb, z->part, z->dsc, z->cSettle,
                       z->a.hb, z->a.hx};

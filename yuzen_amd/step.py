@@ -467,6 +467,12 @@ class HipCZ:
         self.lib.yz_cz_stats(self.p, _pd(out))
         return out
 
+    def clock(self):
+        """(the next step, the last accepted step, trials rejected)"""
+        out = np.empty(3)
+        self.lib.yz_cz_clock(self.p, _pd(out))
+        return out
+
     def learn_stats(self):
         """the learned film solve: refinement rounds, FP32 iterations,
         solves that FP64 AMG finished, training milliseconds"""
