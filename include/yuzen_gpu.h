@@ -564,9 +564,11 @@ int yz_set_fault(long k, long m);
  * row's pattern (13-point film-height or 9-point concentration rows in 2-D,
  * 5-point in 1-D). The rows are folded into stencil planes on the device,
  * so AMG and Jacobi both run on the structured grid. A matrix with no such
- * grid, or whose rows do not fit it, returns 2 with yz_last_error() set:
- * the seam serves the film-height and well-mixed concentration equations,
- * not a general CRS. */
+ * grid, or whose rows do not fit it, is solved as a general CRS on the
+ * device: its rows scaled by their diagonals, a product giving each row a
+ * group of lanes of a wave, BiCGSTAB preconditioned by SPAI0 (for
+ * YZ_PC_AMG: there is no algebraic multigrid on the device) or damped
+ * Jacobi; the codes and the contract are the same. */
 typedef struct yz_lin_settings {
     int precond;       /* YZ_PC_JACOBI or YZ_PC_AMG */
     int scaling;       /* row scaling by the diagonal */

@@ -168,11 +168,28 @@ def test_linsolve_seam_takes_the_concentration_system(dim):
     assert np.abs(x0).max() > 0
 
 
+def _scaled_residual(rp, col, val, rhs, x):
+    """||b - A x|| / ||b|| of the system with its rows divided by their
+    diagonals (the scaling the seam's tolerance is on)"""
+    n = len(rp) - 1
+    r = np.empty(n)
+    bs = np.empty(n)
+    for i in range(n):
+        sl = slice(rp[i], rp[i + 1])
+        d = val[sl][col[sl] == i].sum()
+        d = d if abs(d) >= 1e-30 else 1.0
+        r[i] = (rhs[i] - val[sl] @ x[col[sl]]) / d
+        bs[i] = rhs[i] / d
+    return np.linalg.norm(r) / np.linalg.norm(bs)
+
+
 def test_linsolve_seam_return_codes():
     """The backend contract's codes through the seam: 1 when the iteration
-    cap is reached first, 2 for a matrix that is not a structured film or
-    concentration system (a row that is neither a copy row nor within the
-    stencil), and the matrix and right-hand side left as given."""
+    cap is reached first, 2 at a breakdown (a zero matrix: (rh, v) = 0),
+    and the matrix and right-hand side left as given. A matrix that is not
+    a structured film or concentration system (a ghost row that no longer
+    copies, an interior row reaching past the stencil) takes the general
+    CRS path and is solved to the tolerance."""
     c, h, s, cc = smooth_case(2, TERMS, slip=0.05)
     o = ob.OracleStep(c, h, s, cc, 1e-3)
     o.set_h(h)
@@ -187,14 +204,65 @@ def test_linsolve_seam_return_codes():
     assert np.array_equal(IA, IA0) and np.array_equal(rhs, rhs0)
     bad = IA.copy()
     g = int(np.flatnonzero(bc)[0])                       # a ghost row: not a copy row any more
-    bad[rp[g]:rp[g + 1]] = 1.0
-    rc, x, it, r = ls.solve(rp, col, bad, rhs, np.zeros(o.n), epsilon=1e-12)
+    bad[rp[g]:rp[g + 1]] = [2.0, 1.0]
+    rc, x, it, r = ls.solve(rp, col, bad, rhs, np.zeros(o.n), precond=capi.PC_JACOBI, epsilon=1e-10,
+                            max_iter=20000)
+    res = _scaled_residual(rp, col, bad, rhs, x)
+    print(f"no copy row (general path): rc={rc} it={it} resid={r:.2e} (checked {res:.2e})")
+    assert rc == 0 and res <= 1.01e-10
+    zero = np.zeros_like(IA)
+    rc, x, it, r = ls.solve(rp, col, zero, rhs, np.zeros(o.n), epsilon=1e-12)
+    print(f"zero matrix: rc={rc}")
     assert rc == 2
-    far = col.copy()
-    i = (c.nx + 4) * 6 + 6                              # an interior row reaching past the stencil
-    far[rp[i] + 1] = i + 3
-    rc, x, it, r = ls.solve(rp, far, IA, rhs, np.zeros(o.n), epsilon=1e-12, grid=(2, c.nx, c.ny, [0, 0, 0, 0]))
-    assert rc == 2
+
+
+@pytest.mark.parametrize("kind", ["conv_diff_3d", "random"])
+@pytest.mark.parametrize("precond", [capi.PC_AMG, capi.PC_JACOBI], ids=["spai0", "jacobi"])
+def test_linsolve_general_crs(kind, precond):
+    """The seam's general path (no structured grid): a 3-D 7-point
+    convection-diffusion system on 20^3 cells and a random sparse
+    diagonally dominant one (about 12 entries a row, columns in any order),
+    solved to the tolerance on the scaled system, the matrix left as given,
+    and the same bits on a second call"""
+    rng = np.random.default_rng(7)
+    if kind == "conv_diff_3d":
+        m = 20
+        n = m ** 3
+        rows = []
+        for k in range(n):
+            ix, iy, iz = k % m, (k // m) % m, k // (m * m)
+            ent = {k: 6.0 + 0.5 * rng.random()}
+            for d, (dx, dy, dz) in enumerate(((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))):
+                jx, jy, jz = ix + dx, iy + dy, iz + dz
+                if 0 <= jx < m and 0 <= jy < m and 0 <= jz < m:
+                    ent[jx + m * (jy + m * jz)] = -1.0 + (0.3 if d == 0 else 0.0)
+            rows.append(ent)
+    else:
+        n = 5000
+        rows = []
+        for k in range(n):
+            cols = rng.choice(n, 11, replace=False)
+            ent = {int(j): rng.uniform(-1, 1) for j in cols if j != k}
+            ent[k] = sum(abs(v) for v in ent.values()) + 0.5
+            rows.append(ent)
+    rp = np.zeros(n + 1, dtype=np.int32)
+    col, val = [], []
+    for k, ent in enumerate(rows):
+        items = list(ent.items())
+        rng.shuffle(items)
+        col += [j for j, _ in items]
+        val += [v for _, v in items]
+        rp[k + 1] = len(col)
+    col, val = np.array(col, dtype=np.int32), np.array(val)
+    rhs = rng.uniform(-1, 1, n)
+    val0 = val.copy()
+    ls = LinSolver()
+    rc, x, it, r = ls.solve(rp, col, val, rhs, np.zeros(n), precond=precond, epsilon=1e-12)
+    res = _scaled_residual(rp, col, val, rhs, x)
+    print(f"{kind}: n={n} nnz={len(col)} rc={rc} it={it} resid={r:.2e} (checked {res:.2e})")
+    assert rc == 0 and res <= 1.01e-12 and np.array_equal(val, val0)
+    rc2, x2, it2, _ = ls.solve(rp, col, val, rhs, np.zeros(n), precond=precond, epsilon=1e-12)
+    assert rc2 == 0 and it2 == it and np.array_equal(x, x2)
 
 
 def test_linsolve_context_reused_across_shapes():

@@ -182,6 +182,8 @@ StencilOp lin_op(const LinGrid &G) {
     return op;
 }
 
+constexpr int kNotGrid = 99;   // the rows do not fold onto a structured grid
+
 #define LIN_TRY(x)                                                        \
     do {                                                                  \
         if ((x) != hipSuccess) { yz::set_error("HIP error in the linear-solve seam"); return 2; } \
@@ -219,7 +221,7 @@ int lin_solve(Solver &S, LinBufs &B, LinGrid G, const int *rowptr, const int *co
     int bad = 0;
     LIN_TRY(hipMemcpyAsync(&bad, B.flag, sizeof(int), hipMemcpyDeviceToHost, st));
     LIN_TRY(hipStreamSynchronize(st));
-    if (bad) { yz::set_error("the matrix is not a structured film or concentration system"); return 2; }
+    if (bad) return kNotGrid;
     const int pc = s->precond == YZ_PC_AMG ? YZ_PC_AMG : YZ_PC_JACOBI;
     int it = 0, tried = 0;
     double rr = 0;
@@ -283,6 +285,173 @@ __device__ __align__(256) char scratch[kScratchBytes];
 constexpr int kMaxDevices = 64;
 char *g_scratch[kMaxDevices] = {};
 
+// ---------------------------------------------------------------------
+// A CRS with no structured grid: the system as it is on the device, its
+// rows scaled by their diagonals (ScaleRows), and BiCGSTAB with a
+// point-wise preconditioner: SPAI0 (m_i = a_ii / sum_j a_ij^2, AMGCL's
+// relaxation; the multigrid setting, for want of an algebraic multigrid on
+// the device) or damped Jacobi 0.72 / a_ii (the Jacobi and None settings,
+// LinearSolverBackend.cpp:662-679). A product y = A x gives each row a
+// group of G lanes of a wave (G from the mean row length, up to the whole
+// wave for long rows); the lanes stride over the row's entries and sum in a
+// fixed shuffle tree. Dot products: fixed block partials summed in order on
+// the host, so a solve gives the same bits every run.
+// ---------------------------------------------------------------------
+constexpr int CRB = 256;          // reduction blocks
+template <int G>
+__global__ void __launch_bounds__(256) k_crs_spmv(int n, const int *__restrict__ rp, const int *__restrict__ col,
+                                                  const double *__restrict__ val, const double *__restrict__ x,
+                                                  double *__restrict__ y) {
+    const int lane = threadIdx.x % G;
+    const int row = (blockIdx.x * 256 + threadIdx.x) / G;
+    double s = 0.0;
+    if (row < n)
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += G) s += val[k] * x[col[k]];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_down(s, o, G);
+    if (row < n && lane == 0) y[row] = s;
+}
+// the scaled copy of the system and the preconditioner's diagonal
+__global__ void k_crs_prep(int n, int scaling, int spai, const int *__restrict__ rp, const int *__restrict__ col,
+                           const double *__restrict__ val, const double *__restrict__ rhs, double *__restrict__ vs,
+                           double *__restrict__ b, double *__restrict__ m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) if (col[k] == i) d += val[k];
+    const bool sc = scaling && !(fabs(d) < 1.0e-30);
+    const double inv = sc ? 1.0 / d : 1.0;
+    double dd = 0.0, ss = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        const double v = sc ? val[k] * inv : val[k];
+        vs[k] = v;
+        ss += v * v;
+        if (col[k] == i) dd += v;
+    }
+    b[i] = sc ? rhs[i] * inv : rhs[i];
+    m[i] = spai ? (ss > 0 ? dd / ss : 0.0) : (dd != 0 ? 0.72 / dd : 0.0);
+}
+// partial sums of a.b and c.d over fixed blocks
+__global__ void __launch_bounds__(256) k_crs_dot2(int n, const double *a, const double *b, const double *c,
+                                                  const double *d, double *part) {
+    __shared__ double sh[2][256];
+    double u = 0, v = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += CRB * 256) {
+        u += a[i] * b[i];
+        if (c) v += c[i] * d[i];
+    }
+    sh[0][threadIdx.x] = u; sh[1][threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) { sh[0][threadIdx.x] += sh[0][threadIdx.x + w]; sh[1][threadIdx.x] += sh[1][threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[blockIdx.x] = sh[0][0]; part[CRB + blockIdx.x] = sh[1][0]; }
+}
+// y = a x + b y + c z (what BiCGSTAB's updates need), and y = m x
+__global__ void k_crs_lin(int n, double a, const double *x, double b, double *y, double c, const double *z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = a * x[i];
+    if (b != 0.0) v += b * y[i];
+    if (z) v += c * z[i];
+    y[i] = v;
+}
+__global__ void k_crs_diag(int n, const double *m, const double *x, double *y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = m[i] * x[i];
+}
+
+struct CrsSolve {
+    int n, G;
+    const int *rp, *col;
+    const double *val;
+    double *part, *hpart;
+    hipStream_t st;
+    void spmv(const double *x, double *y) const {
+        const int blocks = (int)(((long)n * G + 255) / 256);
+        switch (G) {
+        case 2: hipLaunchKernelGGL(k_crs_spmv<2>, dim3(blocks), dim3(256), 0, st, n, rp, col, val, x, y); break;
+        case 4: hipLaunchKernelGGL(k_crs_spmv<4>, dim3(blocks), dim3(256), 0, st, n, rp, col, val, x, y); break;
+        case 8: hipLaunchKernelGGL(k_crs_spmv<8>, dim3(blocks), dim3(256), 0, st, n, rp, col, val, x, y); break;
+        case 16: hipLaunchKernelGGL(k_crs_spmv<16>, dim3(blocks), dim3(256), 0, st, n, rp, col, val, x, y); break;
+        case 32: hipLaunchKernelGGL(k_crs_spmv<32>, dim3(blocks), dim3(256), 0, st, n, rp, col, val, x, y); break;
+        default: hipLaunchKernelGGL(k_crs_spmv<64>, dim3(blocks), dim3(256), 0, st, n, rp, col, val, x, y); break;
+        }
+    }
+    int dot2(const double *a, const double *b, const double *c, const double *d, double &u, double &v) const {
+        hipLaunchKernelGGL(k_crs_dot2, dim3(CRB), dim3(256), 0, st, n, a, b, c, d, part);
+        if (hipMemcpyAsync(hpart, part, 2 * CRB * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return -1;
+        u = 0; v = 0;
+        for (int k = 0; k < CRB; ++k) { u += hpart[k]; v += hpart[CRB + k]; }
+        return 0;
+    }
+    void lin(double a, const double *x, double b, double *y, double c = 0, const double *z = nullptr) const {
+        hipLaunchKernelGGL(k_crs_lin, dim3(lblk(n)), dim3(LB), 0, st, n, a, x, b, y, c, z);
+    }
+};
+
+// BiCGSTAB with right preconditioning by the diagonal m: the method and the
+// tests of the stencil solver (0 converged, 1 iteration cap, 2 breakdown)
+int crs_bicgstab(const CrsSolve &K, const double *m, const double *b, double *x, double **w, int maxit, double tol,
+                 int *iters, double *resid) {
+    const int n = K.n;
+    double *r = w[0], *rh = w[1], *p = w[2], *v = w[3], *sv = w[4], *t = w[5], *ph = w[6], *sh = w[7];
+    *iters = 0; *resid = 0;
+    K.spmv(x, r);
+    K.lin(1.0, b, -1.0, r);                     // r = b - A x
+    if (hipMemcpyAsync(rh, r, n * sizeof(double), hipMemcpyDeviceToDevice, K.st) != hipSuccess) return 2;
+    double bb, rr;
+    if (K.dot2(b, b, r, r, bb, rr)) return 2;
+    const double nb = std::sqrt(bb);
+    if (nb == 0) { (void)hipMemsetAsync(x, 0, n * sizeof(double), K.st); return 0; }
+    const double eps = tol * nb;
+    double res = std::sqrt(rr);
+    if (res <= eps) { *resid = res / nb; return 0; }
+    double rho1 = rr, rho2 = 1, alpha = 1, omega = 1, dum;
+    int it = 0;
+    for (; it < maxit; ++it) {
+        if (it > 0 && std::fabs(rho1) < 1e-300) { *iters = it; *resid = res / nb; return 2; }
+        if (it == 0) K.lin(1.0, r, 0.0, p);
+        else {
+            const double beta = (rho1 * alpha) / (rho2 * omega);
+            K.lin(1.0, r, beta, p, -beta * omega, v);   // p = r + beta (p - omega v)
+        }
+        hipLaunchKernelGGL(k_crs_diag, dim3(lblk(n)), dim3(LB), 0, K.st, n, m, p, ph);
+        K.spmv(ph, v);
+        double rv;
+        if (K.dot2(rh, v, nullptr, nullptr, rv, dum)) return 2;
+        if (rv == 0) { *iters = it; *resid = res / nb; return 2; }
+        alpha = rho1 / rv;
+        K.lin(1.0, r, 0.0, sv, -alpha, v);          // s = r - alpha v
+        double ss;
+        if (K.dot2(sv, sv, nullptr, nullptr, ss, dum)) return 2;
+        res = std::sqrt(ss);
+        if (res < eps) { K.lin(alpha, ph, 1.0, x); ++it; break; }
+        hipLaunchKernelGGL(k_crs_diag, dim3(lblk(n)), dim3(LB), 0, K.st, n, m, sv, sh);
+        K.spmv(sh, t);
+        double tt, ts;
+        if (K.dot2(t, t, t, sv, tt, ts)) return 2;
+        omega = tt != 0 ? ts / tt : 0.0;
+        if (omega == 0) { *iters = it; *resid = res / nb; return 2; }
+        K.lin(alpha, ph, 1.0, x, omega, sh);       // x += alpha ph + omega sh
+        K.lin(1.0, sv, 0.0, r, -omega, t);          // r = s - omega t
+        double rrh;
+        if (K.dot2(r, r, r, rh, rr, rrh)) return 2;
+        rho2 = rho1;
+        rho1 = rrh;
+        res = std::sqrt(rr);
+        if (res < eps) { ++it; break; }
+        if (!(res == res)) { *iters = it; *resid = res; return 2; }
+    }
+    *iters = it;
+    *resid = res / nb;
+    if (!(*resid == *resid)) return 2;
+    return *resid <= tol ? 0 : 1;
+}
+
 }  // namespace
 
 struct yz_linctx {
@@ -291,7 +460,64 @@ struct yz_linctx {
     LinBufs B;
     char *mem = nullptr;
     size_t mem_bytes = 0;
+    char *crs_mem = nullptr;       // the general CRS path's arrays
+    size_t crs_bytes = 0;
+    double *crs_host = nullptr;    // pinned block partials
 };
+
+namespace {
+// the general CRS path (no structured grid): the system, its scaled copy,
+// the preconditioner's diagonal and the Krylov vectors on the device
+int lin_solve_crs(yz_linctx *ctx, int n, int nnz, const int *rowptr, const int *col, const double *val,
+                  const double *rhs, double *x_inout, const yz_lin_settings *s, int *iter, double *resid) {
+    hipStream_t st = ctx->st;
+    const size_t need = (size_t)(n + 1) * 4 + (size_t)nnz * 4 + 2 * (size_t)nnz * 8 + 12 * (size_t)n * 8 +
+                        2 * CRB * 8 + 16 * 256;
+    if (need > ctx->crs_bytes) {
+        if (ctx->crs_mem) (void)hipFree(ctx->crs_mem);
+        ctx->crs_mem = nullptr;
+        ctx->crs_bytes = 0;
+        if (hipMalloc((void **)&ctx->crs_mem, need) != hipSuccess) { yz::set_error("device allocation failed"); return 2; }
+        ctx->crs_bytes = need;
+    }
+    if (!ctx->crs_host && hipHostMalloc((void **)&ctx->crs_host, 2 * CRB * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        yz::set_error("host allocation failed");
+        return 2;
+    }
+    size_t used = 0;
+    auto carve = [&](size_t b) -> void * {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + b;
+        return ctx->crs_mem + off;
+    };
+    int *drp = (int *)carve((size_t)(n + 1) * 4), *dcol = (int *)carve((size_t)nnz * 4);
+    double *dval = (double *)carve((size_t)nnz * 8), *vs = (double *)carve((size_t)nnz * 8);
+    double *drhs = (double *)carve((size_t)n * 8), *b = (double *)carve((size_t)n * 8);
+    double *x = (double *)carve((size_t)n * 8), *m = (double *)carve((size_t)n * 8);
+    double *w[8];
+    for (double *&v : w) v = (double *)carve((size_t)n * 8);
+    double *part = (double *)carve(2 * CRB * 8);
+    LIN_TRY(hipMemcpyAsync(drp, rowptr, (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+    LIN_TRY(hipMemcpyAsync(dcol, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice, st));
+    LIN_TRY(hipMemcpyAsync(dval, val, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice, st));
+    LIN_TRY(hipMemcpyAsync(drhs, rhs, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    LIN_TRY(hipMemcpyAsync(x, x_inout, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    const int spai = s->precond == YZ_PC_AMG ? 1 : 0;
+    hipLaunchKernelGGL(k_crs_prep, dim3(lblk(n)), dim3(LB), 0, st, n, s->scaling, spai, drp, dcol, dval, drhs, vs, b, m);
+    // lanes per row: the power of two nearest half the mean row length, 2 .. 64
+    int G = 2;
+    while (G < 64 && 2 * G < (nnz + n - 1) / n) G *= 2;
+    CrsSolve K{n, G, drp, dcol, vs, part, ctx->crs_host, st};
+    int it = 0;
+    double rr = 0;
+    const int rc = crs_bicgstab(K, m, b, x, w, s->max_iter, s->epsilon, &it, &rr);
+    LIN_TRY(hipMemcpyAsync(x_inout, x, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    LIN_TRY(hipStreamSynchronize(st));
+    if (iter) *iter = it;
+    if (resid) *resid = rr;
+    return rc;
+}
+}  // namespace
 
 extern "C" {
 
@@ -305,6 +531,8 @@ void yz_linsolve_destroy(yz_linctx *c) {
     if (!c) return;
     solver_free(c->S);
     if (c->mem) (void)hipFree(c->mem);
+    if (c->crs_mem) (void)hipFree(c->crs_mem);
+    if (c->crs_host) (void)hipHostFree(c->crs_host);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -324,7 +552,9 @@ int yz_linsolve(yz_linctx *ctx, int n, int nnz, const int *rowptr, const int *co
     if (resid) *resid = 0;
     if (!ctx || !rowptr || !col || !val || !rhs || !x_inout || !s || n <= 0 || nnz < n) return 2;
     LinGrid G;
-    if (lin_grid(n, nnz, rowptr, col, s, G)) { yz::set_error("no structured grid in the matrix"); return 2; }
+    // a matrix with no film-height or concentration grid takes the general path
+    if (lin_grid(n, nnz, rowptr, col, s, G))
+        return lin_solve_crs(ctx, n, nnz, rowptr, col, val, rhs, x_inout, s, iter, resid);
     const int N = G.nx * (G.dim == 2 ? G.ny : 1);
     const size_t need = lin_bufs_bytes(n, nnz, N);
     if (need > ctx->mem_bytes) {
@@ -342,7 +572,9 @@ int yz_linsolve(yz_linctx *ctx, int n, int nnz, const int *rowptr, const int *co
         return base + off;
     };
     lin_bufs(ctx->B, n, nnz, N, carve);
-    return lin_solve(ctx->S, ctx->B, G, rowptr, col, val, rhs, x_inout, s, iter, resid, ctx->st);
+    const int rc = lin_solve(ctx->S, ctx->B, G, rowptr, col, val, rhs, x_inout, s, iter, resid, ctx->st);
+    if (rc == kNotGrid) return lin_solve_crs(ctx, n, nnz, rowptr, col, val, rhs, x_inout, s, iter, resid);
+    return rc;
 }
 
 int yz_linsolve_noalloc(int n, int nnz, const int *rowptr, const int *col, const double *val, const double *rhs,
@@ -383,6 +615,11 @@ int yz_linsolve_noalloc(int n, int nnz, const int *rowptr, const int *col, const
     solver_use_arena(nullptr);
     solver_free(S);
     if (arena.overflow) rc = too_big();
+    if (rc == kNotGrid) {
+        yz::set_error("the matrix is not a structured film or concentration system (the general CRS path "
+                      "allocates: yz_linsolve)");
+        rc = 2;
+    }
     return rc;
 }
 
