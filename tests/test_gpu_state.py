@@ -94,13 +94,19 @@ def test_continuation_learned(tmp_path):
     _continuation(c, h, s, cc, dt, 1, "learned FP32 48^2", tmp_path)
 
 
-def test_continuation_concz(tmp_path):
+@pytest.mark.parametrize("control", [False, True], ids=["fixed", "step_control"])
+def test_continuation_concz(tmp_path, control):
+    """4 steps in one run against 2, saved, loaded into a fresh context and
+    2 more, to the bit; under the step control the clock (the next step, the
+    last accepted one) and the lengthening count travel in the file"""
     c, h, s, cc = _allphys(per=False)
+    if control:
+        c.step_control, c.tsc_keep, c.tsc_criterion = 1, 2, 2e-4
     dt = 0.02
     a = HipCZ(c, h, s, cc, dt)
     for _ in range(4):
         assert a.advance()[0] == 0
-    ha, ca = a.h(), a.c()
+    ha, ca, clk = a.h(), a.c(), a.clock()
     a.close()
     b = HipCZ(c, h, s, cc, dt)
     for _ in range(2):
@@ -113,4 +119,5 @@ def test_continuation_concz(tmp_path):
     for _ in range(2):
         assert d.advance()[0] == 0
     assert np.array_equal(d.h(), ha) and np.array_equal(d.c(), ca)
+    assert d.clock()[0] == clk[0] and d.clock()[1] == clk[1]
     d.close()

@@ -2339,8 +2339,10 @@ int yz_cz_save_state(yz_cz *z, const char *path) {
     st.set_text("model", "ConcZ-3D");
     st.set("time", z->time);
     st.set("step", (double)z->steps);
-    st.set("dt", std::vector<double>{z->dt, z->dt, z->dt});
-    st.set("adjustCount.height", 0.0);
+    // the clock (PutClock, RestartState.cpp:168-183): the step about to be
+    // taken, the last accepted one, the one the last step was solved with
+    st.set("dt", std::vector<double>{z->dt, z->dt_old, z->dt_old});
+    st.set("adjustCount.height", (double)z->tsc_count);
     st.set_vector("height.old", hold);
     if (z->prevSet) {
         st.set("conc.previousDt", z->dtPrev);
@@ -2380,6 +2382,14 @@ int yz_cz_load_state(yz_cz *z, const char *path) {
     z->dtPrev = prev ? st.get("conc.previousDt") : 1.0;
     z->time = st.get("time");
     z->steps = (long long)st.get("step");
+    // under the step control the clock's steps and the lengthening count
+    // continue from the file (TakeClock with restoreDt)
+    if (z->cs.step_control) {
+        z->dt = st.get("dt", 0, z->dt);
+        z->dt_old = st.get("dt", 1, z->dt);
+        z->a.dt = z->dt;
+        z->tsc_count = (int)st.get("adjustCount.height");
+    }
     return 0;
 }
 
